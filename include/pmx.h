@@ -1,0 +1,210 @@
+/*
+ * pmx.h -- C ABI of the MI355X-native Parmec DEM step engine (libpmx.so).
+ *
+ * The engine replaces the body of Parmec's dem() step loop for spheres,
+ * static/moving obstacle triangles, granular contacts and SPRING elements.
+ * All arrays are owned by the caller and laid out like Parmec's global SoA
+ * buffers (parmec.h:66-283): a k-component per-item array is one contiguous
+ * block of k*count doubles, component-major (component c of item i at
+ * c*count + i).  3x3 matrices are column-major, element (r,c) at 3c+r
+ * (macros.h:906-950).
+ *
+ * Errors: every entry point returns PMX_OK (0) or a negative PMX_E* code and
+ * leaves a message for pmx_last_error() (thread-local).  Contexts are not
+ * thread safe; one HIP stream per context.
+ */
+#ifndef PMX_H
+#define PMX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PMX_CUTOFF 64   /* radix-tree terminal node size, partition.h:29 */
+#define PMX_LSIZE 96    /* leaf capacity before re-partitioning, partition.h:31 */
+
+/* surface interaction parameters, constants.h:37 */
+enum { PMX_SPRING = 0, PMX_DAMPER, PMX_FRISTAT, PMX_FRIDYN, PMX_FRIROL, PMX_FRIDRIL, PMX_KSKN, PMX_NIPARAM };
+
+/* interaction kinds, constants.h:35 */
+enum { PMX_GRANULAR_FORCE = 0, PMX_USER_FORCE = 1 };
+
+/* spring types and flags, constants.h:61-63 */
+enum { PMX_SPRING_NONLINEAR_ELASTIC = 0, PMX_SPRING_GENERAL_NONLINEAR = 1 };
+enum { PMX_SPRING_YIELDED = 1, PMX_SPRING_UNLOADING = 2, PMX_SPRDIR_CONSTANT = 4,
+       PMX_SPRDIR_PLANAR = 8, PMX_SPRDIR_FOLLOWER = 16, PMX_SPRDIR_PROJECT = 32 };
+
+/* error codes */
+enum { PMX_OK = 0, PMX_EINVAL = -1, PMX_ENOMEM = -2, PMX_EDEVICE = -3,
+       PMX_ECAPACITY = -4, PMX_EUNSUPPORTED = -5, PMX_EREPART = -6 };
+
+/* arithmetic of the contact/force stage */
+enum { PMX_FP64 = 0,     /* plain FP64 drop-in */
+       PMX_MIXED = 1 };  /* FP32 contact geometry and forces, FP64 integration */
+
+/*
+ * State view: the SoA buffers of parmec.h (namespace parmec) that the step
+ * path reads and writes.  Counts follow parmec.h:36-283.
+ */
+typedef struct pmx_model
+{
+  /* particles, parmec.h:66-84 */
+  int parnum;
+  double *angular;   /* [6][parnum]: 0-2 body-frame Omega, 3-5 spatial omega */
+  double *linear;    /* [3][parnum] */
+  double *rotation;  /* [9][parnum] column-major */
+  double *position;  /* [6][parnum]: 0-2 current, 3-5 reference */
+  double *inertia;   /* [9][parnum] referential inertia */
+  double *inverse;   /* [9][parnum] inverse inertia (rewritten by every dem call) */
+  double *mass;      /* [parnum] */
+  double *invm;      /* [parnum] (rewritten by every dem call) */
+  double *force;     /* [3][parnum] (output) */
+  double *torque;    /* [3][parnum] (output) */
+  int *flags;        /* [parnum] */
+
+  /* ellipsoids (spheres: radii[1] < 0), parmec.h:86-96 */
+  int ellnum;
+  double *center;    /* [6][ellnum]: 0-2 current, 3-5 reference */
+  double *radii;     /* [3][ellnum] */
+  int *part;         /* [ellnum] ellipsoid -> particle */
+  int *ellcol;       /* [ellnum] colour */
+
+  /* triangles, parmec.h:98-102 */
+  int trinum;
+  double *tri;       /* [9][trinum]: vertex v, component c at (3v+c)*trinum + t */
+  int *tricol;       /* [trinum] */
+  int *triobs;       /* [trinum]: -1 static, <= -2 moving obstacle, >= 0 mesh particle */
+
+  /* moving obstacles, parmec.h:104-112 */
+  int obsnum;
+  int *trirng;       /* [2*obsnum] triangle ranges */
+  double *obspnt;    /* [3*obsnum] */
+  double *obslin;    /* [3*obsnum] (per step input) */
+  double *obsang;    /* [3*obsnum] (per step input) */
+
+  /* mesh nodes and faces (meshed particles), parmec.h:114-130 */
+  int nodnum;
+  double *nodes;     /* [6][nodnum]: 0-2 current, 3-5 reference */
+  int *nodpart;      /* [nodnum] */
+  int facnum;
+  int *facnod;       /* [3][facnum] */
+  int *factri;       /* [facnum] face -> triangle */
+
+  /* surface pairing, parmec.h:140-150 */
+  int pairnum;
+  int *pairs;        /* [2*pairnum] colour pairs (index 0 = default) */
+  int *ikind;        /* [pairnum] */
+  double *iparam;    /* [PMX_NIPARAM][pairnum] */
+
+  /* translational springs, parmec.h:152-200 (sorted by (part1, part2)) */
+  int sprnum;
+  int *sprtype;      /* [sprnum] */
+  int *sprpart;      /* [2][sprnum] */
+  double *sprpnt;    /* [2][6][sprnum]: point p (0-2 current, 3-5 reference) at (6p+c)*sprnum + s */
+  int *spridx;       /* [sprnum+1] into spring curve table */
+  double *spring;    /* [2][sprcrv]: stroke, force */
+  int sprcrv;        /* number of spring curve points */
+  int *dashidx;      /* [sprnum+1] into dashpot table */
+  double *dashpot;   /* [2][dashcrv] */
+  int dashcrv;
+  int *unidx;        /* [sprnum+1] into unload table */
+  double *unload;    /* [2][uncrv] */
+  int uncrv;
+  double *yield;     /* [2][sprnum] */
+  double *sprdir;    /* [6][sprnum] */
+  int *sprflg;       /* [sprnum] */
+  double *sprfric;   /* [sprnum] */
+  double *sprkskn;   /* [sprnum] */
+  double *sprsdsp;   /* [3][sprnum] */
+  double *stroke0;   /* [sprnum] */
+  double *stroke;    /* [3][sprnum] */
+  double *sprfrc;    /* [3][sprnum] */
+  int *unspring;     /* [sprnum] (-2: no unspring action) */
+  int *sproffset;    /* [sprnum] (-1: no load curve offset) */
+
+  /* time bookkeeping, parmec.h:44-49 */
+  double curtime;
+  double curstep;
+  int stepnum;
+} pmx_model;
+
+/*
+ * Per-step external inputs.  Parmec evaluates gravity and global damping
+ * callbacks on the host at every step (input.cpp:4424-4503); the engine takes
+ * them as tables evaluated at each step's curtime (pmx_step_times gives the
+ * times), or as constants when the table pointer is NULL.
+ */
+typedef struct pmx_forcing
+{
+  double gravity[3];         /* constant gravity when gtab == NULL */
+  const double *gtab;        /* [nsteps][3] gravity per step, or NULL */
+  double damping[6];         /* constant global damping when dtab == NULL */
+  const double *dtab;        /* [nsteps][6] damping per step, or NULL */
+  const double *obstab;      /* [nsteps][obsnum][6] obstacle (angular, linear) velocity at curtime+h, or NULL */
+  const double *obstab0;     /* [obsnum][6] obstacle velocity at t = h/2 (first call only), or NULL */
+} pmx_forcing;
+
+typedef struct pmx_options
+{
+  int precision;       /* PMX_FP64 or PMX_MIXED */
+  int device;          /* HIP device ordinal */
+  int row_capacity;    /* contacts per sphere row (0: default 32) */
+  int verbose;
+} pmx_options;
+
+/* one contact of the canonical pair set */
+typedef struct pmx_pair
+{
+  int master;        /* master ellipsoid */
+  int slave;         /* slave ellipsoid, or -(triangle+1) */
+  double state[3];   /* tangential spring displacement */
+} pmx_pair;
+
+typedef struct pmx_stats
+{
+  long long steps;         /* steps taken by the last pmx_dem call */
+  long long rebuilds;      /* partitioning tree (re)builds in the last call */
+  long long pairs;         /* live contact pairs after the last step */
+  int leaves;              /* leaves of the current partitioning tree */
+  int max_row;             /* largest contact row seen */
+  double seconds;          /* wall time of the last pmx_dem step loop */
+} pmx_stats;
+
+typedef struct pmx_ctx pmx_ctx;
+
+/* Number of steps dem(duration, step) takes from the given curtime
+ * (parmec.cpp:2418 accumulation `time < duration`), and the curtime value at
+ * the start of each step if times != NULL (capacity cap). */
+int pmx_step_times (double duration, double step, double curtime, double *times, int cap, int *nsteps);
+
+/* Create a context on a device (replaces parmec::init, parmec.cpp:2227). */
+int pmx_create (const pmx_options *opt, pmx_ctx **ctx);
+
+/* Copy a state view to the device; resets contact history when the
+ * particle/ellipsoid/triangle counts changed (parmec.cpp:288-1398 buffers). */
+int pmx_upload (pmx_ctx *ctx, const pmx_model *model);
+
+/* Run dem(duration, step) with output disabled (parmec.cpp:2315-2530).
+ * elapsed receives the wall seconds of the step loop (timer.h:36-49). */
+int pmx_dem (pmx_ctx *ctx, double duration, double step, const pmx_forcing *forcing, double *elapsed);
+
+/* Copy the device state back into the caller's view. */
+int pmx_download (pmx_ctx *ctx, pmx_model *model);
+
+/* Canonical contact pair set, sorted by (master, slave). */
+int pmx_contacts (pmx_ctx *ctx, pmx_pair *out, size_t cap, size_t *count);
+
+int pmx_get_stats (pmx_ctx *ctx, pmx_stats *stats);
+
+void pmx_destroy (pmx_ctx *ctx);
+
+const char *pmx_last_error (void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

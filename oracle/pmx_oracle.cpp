@@ -1,0 +1,1154 @@
+/*
+ * pmx_oracle.cpp -- CPU oracle of Parmec's DEM step path.
+ *
+ * TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg load this library as the checker; the product library
+ * (parmec_amd/csrc) never includes, links or calls it.
+ *
+ * Scalar restatement of the reference kernels, following their structure:
+ * k-d partitioning tree with leaf buffers (partition.ispc), per-particle
+ * master contact lists with find-or-insert and per-step slave copies
+ * (condet.ispc, forces.ispc), explicit integrator (dynamics.ispc), geometry
+ * update (shapes.ispc), first half step and inertia inversion (parmec.ispc)
+ * and the dem() stage order (parmec.cpp:2315-2530).  Floating point follows
+ * the reference macros' operation order (macros.h) and is compiled with
+ * -ffp-contract=off.  Canonical choices are listed in pmx_oracle.h.
+ *
+ * Built with OpenMP: the loops that are independent per item run in parallel;
+ * every result is independent of the thread count.
+ */
+#include "pmx_oracle.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail (int code, const char *msg)
+{
+  g_err = msg;
+  return code;
+}
+
+const double REAL_MAX_ = 1.7976931348623157E+308;
+
+/* ---------------------------------------------------------------- math */
+
+/* vectorisable exponential map, macros.h:1453-1534, including the reference's
+ * v02 = Omega2*Omega3 term (macros.h:1505) */
+void expmap (double O1, double O2, double O3, double L[9])
+{
+  double angsq, sx, cx, v0, v1, v2, v01, v02, v12, s0, s1, s2;
+  v0 = O1 * O1;
+  v1 = O2 * O2;
+  v2 = O3 * O3;
+  angsq = v0 + v1 + v2;
+  if (angsq < 3.0461741978671E-02)
+  {
+    sx = 1.0 + (-1.666666666666667E-1 + (8.333333333333333E-3 + (-1.984126984126984E-4 +
+         (2.755731922398589E-6 + (-2.505210838544172E-8 + 1.605904383682161E-10 * angsq
+         ) * angsq) * angsq) * angsq) * angsq) * angsq;
+    cx = 0.5 + (-4.166666666666667E-2 + (1.388888888888889E-3 + (-2.480158730158730E-5 +
+         (2.755731922398589E-7 + (-2.087675698786810E-9 + 1.147074559772972E-11 * angsq
+         ) * angsq) * angsq) * angsq) * angsq) * angsq;
+  }
+  else
+  {
+    double t = angsq;
+    angsq = std::sqrt (angsq);
+    double s = std::sin (angsq), c = std::cos (angsq);
+    sx = s / angsq;
+    cx = (1.0 - c) / t;
+  }
+  v01 = O1 * O2;
+  v02 = O2 * O3;
+  v12 = O2 * O3;
+  s0 = sx * O1;
+  s1 = sx * O2;
+  s2 = sx * O3;
+  L[0] = -cx * (v2 + v1);
+  L[1] = cx * v01;
+  L[2] = cx * v02;
+  L[3] = L[1];
+  L[4] = -cx * (v2 + v0);
+  L[5] = cx * v12;
+  L[6] = L[2];
+  L[7] = L[5];
+  L[8] = -cx * (v1 + v0);
+  L[0] += 1.0;
+  L[1] += s2;
+  L[2] -= s1;
+  L[3] -= s2;
+  L[4] += 1.0;
+  L[5] += s0;
+  L[6] += s1;
+  L[7] -= s0;
+  L[8] += 1.0;
+}
+
+/* column-major products, macros.h:906-1000 */
+inline void nvmul (const double A[9], const double B[3], double C[3])
+{
+  C[0] = A[0]*B[0] + A[3]*B[1] + A[6]*B[2];
+  C[1] = A[1]*B[0] + A[4]*B[1] + A[7]*B[2];
+  C[2] = A[2]*B[0] + A[5]*B[1] + A[8]*B[2];
+}
+inline void tvmul (const double A[9], const double B[3], double C[3])
+{
+  C[0] = A[0]*B[0] + A[1]*B[1] + A[2]*B[2];
+  C[1] = A[3]*B[0] + A[4]*B[1] + A[5]*B[2];
+  C[2] = A[6]*B[0] + A[7]*B[1] + A[8]*B[2];
+}
+inline void nnmul (const double A[9], const double B[9], double C[9])
+{
+  for (int c = 0; c < 3; c++)
+    for (int r = 0; r < 3; r++)
+      C[3*c+r] = A[r]*B[3*c] + A[3+r]*B[3*c+1] + A[6+r]*B[3*c+2];
+}
+
+/* Eberly closest point on a triangle, condet.ispc:294-400 (detection form) and
+ * 519-624 (update form; the two have the same operation sequence) */
+double triangle_sphere (double ax, double ay, double az, double bx, double by, double bz,
+                        double cx, double cy, double cz, double px, double py, double pz,
+                        double r, double point[3], double normal[3])
+{
+  double edge0[3] = {bx-ax, by-ay, bz-az};
+  double edge1[3] = {cx-ax, cy-ay, cz-az};
+  double v0[3] = {ax-px, ay-py, az-pz};
+  double a = edge0[0]*edge0[0] + edge0[1]*edge0[1] + edge0[2]*edge0[2];
+  double b = edge0[0]*edge1[0] + edge0[1]*edge1[1] + edge0[2]*edge1[2];
+  double c = edge1[0]*edge1[0] + edge1[1]*edge1[1] + edge1[2]*edge1[2];
+  double d = edge0[0]*v0[0] + edge0[1]*v0[1] + edge0[2]*v0[2];
+  double e = edge1[0]*v0[0] + edge1[1]*v0[1] + edge1[2]*v0[2];
+  double det = a*c - b*b;
+  double s = b*e - c*d;
+  double t = b*d - a*e;
+  auto clamp01 = [] (double x) { return x < 0.0 ? 0.0 : (x > 1.0 ? 1.0 : x); };
+
+  if (s + t < det)
+  {
+    if (s < 0.0)
+    {
+      if (t < 0.0)
+      {
+        if (d < 0.0) { s = clamp01 (-d/a); t = 0.0; }
+        else { s = 0.0; t = clamp01 (-e/c); }
+      }
+      else { s = 0.0; t = clamp01 (-e/c); }
+    }
+    else if (t < 0.0) { s = clamp01 (-d/a); t = 0.0; }
+    else
+    {
+      double invDet = 1.0 / det;
+      s *= invDet;
+      t *= invDet;
+    }
+  }
+  else
+  {
+    if (s < 0.0)
+    {
+      double tmp0 = b+d;
+      double tmp1 = c+e;
+      if (tmp1 > tmp0)
+      {
+        double numer = tmp1 - tmp0;
+        double denom = a-2*b+c;
+        s = clamp01 (numer/denom);
+        t = 1-s;
+      }
+      else { t = clamp01 (-e/c); s = 0.0; }
+    }
+    else if (t < 0.0)
+    {
+      if (a+d > b+e)
+      {
+        double numer = c+e-b-d;
+        double denom = a-2*b+c;
+        s = clamp01 (numer/denom);
+        t = 1-s;
+      }
+      else { s = clamp01 (-e/c); t = 0.0; }
+    }
+    else
+    {
+      double numer = c+e-b-d;
+      double denom = a-2*b+c;
+      s = clamp01 (numer/denom);
+      t = 1.0 - s;
+    }
+  }
+
+  point[0] = ax + s*edge0[0] + t*edge1[0];
+  point[1] = ay + s*edge0[1] + t*edge1[1];
+  point[2] = az + s*edge0[2] + t*edge1[2];
+  double q[3] = {px-point[0], py-point[1], pz-point[2]};
+  double len = std::sqrt (q[0]*q[0] + q[1]*q[1] + q[2]*q[2]);
+  double ilen = 1.0/len;
+  normal[0] = ilen*q[0];
+  normal[1] = ilen*q[1];
+  normal[2] = ilen*q[2];
+  return r - len;
+}
+
+/* ------------------------------------------------------- partitioning */
+
+/* 10-bit to 30-bit spread, partition.ispc:58-66 */
+inline unsigned expandbits (unsigned v)
+{
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+
+/* Morton code, partition.ispc:69-94.  A zero extent along an axis maps that
+ * axis to cell 0 (the reference divides by zero there). */
+unsigned morton (double x, double y, double z, const double e[6])
+{
+  double w[3] = {e[3]-e[0], e[4]-e[1], e[5]-e[2]};
+  double p[3] = {x, y, z};
+  unsigned q[3];
+  for (int k = 0; k < 3; k++)
+  {
+    double pk = w[k] > 0.0 ? (p[k]-e[k])/w[k] : 0.0;
+    double s = pk * 1024.0;
+    s = s > 0.0 ? s : 0.0;
+    s = s < 1023.0 ? s : 1023.0;
+    q[k] = expandbits ((unsigned)(int)s);
+  }
+  return q[0]*4 + q[1]*2 + q[2];
+}
+
+inline int lzc (unsigned x) { return x ? __builtin_clz (x) : 32; }
+
+/* generalised common prefix length, partition.ispc:272-281 */
+inline int delta (int i, unsigned codei, int j, int n, const unsigned *code)
+{
+  if (j < 0 || j > n-1) return -1;
+  unsigned codej = code[j];
+  if (codei == codej) return 32 + lzc ((unsigned)(i ^ j));
+  return lzc (codei ^ codej);
+}
+
+struct RNode { int first, size, split, dimension; double coord; };
+struct PNode { double coord; int dimension, left, right, leaf; };
+
+struct Tree
+{
+  std::vector<PNode> node;
+  std::vector<std::vector<int>> leaf;   /* stored ellipsoid indices (<= LSIZE) */
+  std::vector<int> leafsize;            /* drop counts (may exceed LSIZE) */
+};
+
+/* recursive conversion, partition.ispc:525-569 */
+void convert (const std::vector<RNode> &rt, int rnode, Tree &T, int pnode)
+{
+  T.node[pnode].coord = rt[rnode].coord;
+  T.node[pnode].dimension = rt[rnode].dimension;
+  auto mkleaf = [&] (int p) {
+    T.node[p].coord = 0.0; T.node[p].dimension = -1; T.node[p].left = T.node[p].right = -1;
+    T.node[p].leaf = (int) T.leaf.size ();
+    T.leaf.emplace_back ();
+  };
+  if (rt[rnode].dimension >= 0)
+  {
+    int l = (int) T.node.size (); T.node.push_back (PNode ());
+    int r = (int) T.node.size (); T.node.push_back (PNode ());
+    T.node[pnode].left = l; T.node[pnode].right = r; T.node[pnode].leaf = -1;
+    int j = rt[rnode].split;
+    if (rt[rnode].first != j) convert (rt, j, T, l); else mkleaf (l);
+    if (rt[rnode].first + rt[rnode].size - 1 != j + 1) convert (rt, j+1, T, r); else mkleaf (r);
+  }
+  else
+  {
+    T.node[pnode].left = T.node[pnode].right = -1;
+    T.node[pnode].leaf = (int) T.leaf.size ();
+    T.leaf.emplace_back ();
+  }
+}
+
+/* partitioning_create, partition.ispc:663-746, with a stable sort */
+void tree_create (int n, const double *cx, const double *cy, const double *cz, Tree &T)
+{
+  T.node.clear (); T.leaf.clear (); T.leafsize.clear ();
+  if (n == 0) return;
+  double e[6] = {REAL_MAX_, REAL_MAX_, REAL_MAX_, -REAL_MAX_, -REAL_MAX_, -REAL_MAX_};
+  for (int i = 0; i < n; i++)
+  {
+    if (cx[i] < e[0]) e[0] = cx[i];
+    if (cy[i] < e[1]) e[1] = cy[i];
+    if (cz[i] < e[2]) e[2] = cz[i];
+    if (cx[i] > e[3]) e[3] = cx[i];
+    if (cy[i] > e[4]) e[4] = cy[i];
+    if (cz[i] > e[5]) e[5] = cz[i];
+  }
+  std::vector<unsigned> code0 (n), code (n);
+  std::vector<int> order (n);
+  for (int i = 0; i < n; i++) { code0[i] = morton (cx[i], cy[i], cz[i], e); order[i] = i; }
+  std::stable_sort (order.begin (), order.end (), [&] (int a, int b) { return code0[a] < code0[b]; });
+  for (int i = 0; i < n; i++) code[i] = code0[order[i]];
+
+  const double *point[3] = {cx, cy, cz};
+  std::vector<RNode> rt (std::max (n-1, 1));
+  if (n == 1)
+  {
+    rt[0].first = 0; rt[0].size = 1; rt[0].dimension = -1; rt[0].split = 0; rt[0].coord = 0.0;
+  }
+  /* radix_tree_create, partition.ispc:311-394 */
+  for (int i = 0; i < n-1; i++)
+  {
+    unsigned codei = code[i];
+    int dd = delta (i, codei, i+1, n, code.data ()) - delta (i, codei, i-1, n, code.data ());
+    int d = (dd > 0) - (dd < 0);
+    int dmin = delta (i, codei, i-d, n, code.data ());
+    int lmax = 2;
+    while (delta (i, codei, i+lmax*d, n, code.data ()) > dmin) lmax *= 2;
+    int l = 0;
+    for (int t = lmax/2; t >= 1; t /= 2)
+      if (delta (i, codei, i+(l+t)*d, n, code.data ()) > dmin) l = l+t;
+    int j = i+l*d;
+    rt[i].first = d > 0 ? i : j;
+    rt[i].size = l+1;
+    if (rt[i].size <= PMX_CUTOFF)
+    {
+      rt[i].split = 0; rt[i].coord = 0.0; rt[i].dimension = -1;
+    }
+    else
+    {
+      int dnode = delta (i, codei, j, n, code.data ());
+      int s = 0;
+      for (int t = (l+1)/2;; t = (t+1)/2)
+      {
+        if (delta (i, codei, i+(s+t)*d, n, code.data ()) > dnode) s = s+t;
+        if (t == 1) break;
+      }
+      rt[i].split = i+s*d+std::min (d, 0);
+      int dimension = (dnode-2)%3;
+      double ret = REAL_MAX_;   /* mincoord, partition.ispc:297-308 */
+      for (int k = rt[i].split+1; k < rt[i].first+rt[i].size; k++)
+      {
+        double val = point[dimension][order[k]];
+        if (val < ret) ret = val;
+      }
+      rt[i].coord = ret;
+      rt[i].dimension = dimension;
+    }
+  }
+  T.node.push_back (PNode ());
+  convert (rt, 0, T, 0);
+  T.leafsize.assign (T.leaf.size (), 0);
+}
+
+/* partitioning_store / drop_ellipsoid, partition.ispc:600-660, 749-764;
+ * returns the overflow count */
+int tree_store (Tree &T, int n, const double *cx, const double *cy, const double *cz)
+{
+  if (n == 0) return 0;
+  for (auto &l : T.leaf) l.clear ();
+  std::fill (T.leafsize.begin (), T.leafsize.end (), 0);
+  const double *c[3] = {cx, cy, cz};
+  int repart = 0;
+  for (int i = 0; i < n; i++)
+  {
+    int node = 0;
+    while (T.node[node].dimension >= 0)
+    {
+      int d = T.node[node].dimension;
+      node = c[d][i] < T.node[node].coord ? T.node[node].left : T.node[node].right;
+    }
+    int L = T.node[node].leaf;
+    int j = T.leafsize[L]++;
+    if (j < PMX_LSIZE) T.leaf[L].push_back (i);
+    else repart++;
+  }
+  return repart;
+}
+
+/* leaves reached by an AABB, condet.ispc:98-113 and 412-426 */
+void tree_reach (const Tree &T, int node, const double lo[3], const double hi[3], std::vector<int> &out)
+{
+  int d = T.node[node].dimension;
+  if (d >= 0)
+  {
+    if (lo[d] <= T.node[node].coord) tree_reach (T, T.node[node].left, lo, hi, out);
+    if (hi[d] >= T.node[node].coord) tree_reach (T, T.node[node].right, lo, hi, out);
+  }
+  else out.push_back (T.node[node].leaf);
+}
+
+/* -------------------------------------------------------------- contacts */
+
+struct Contact
+{
+  int master, slave0, slave1, color0, color1;
+  double point[3], normal[3], depth, force[3], state[3];
+};
+
+struct Slave
+{
+  int master0, master1;   /* particle, ellipsoid */
+  int slave1;             /* the contact's slave ellipsoid or -(triangle+1) */
+  double point[3], force[3];
+};
+
+/* partner-ordered identity of a contact entry (see pmx_oracle.h) */
+inline unsigned long long side_key (int ell_or_negtri)
+{
+  if (ell_or_negtri >= 0) return (unsigned long long) ell_or_negtri;
+  return 0x80000000ull | (unsigned long long) (-ell_or_negtri - 1);
+}
+
+/* pairing, forces.ispc:98-122 */
+int pairing (int nummat, const int *pairs, int i, int j)
+{
+  int p[2] = {i, j}, start = 1, end = nummat;
+  if (i > j) { p[0] = j; p[1] = i; }
+  while (start < end)
+  {
+    int mid = ((end-start)>>1) + start;
+    if (p[0] > pairs[2*mid]) { if (start == mid) break; start = mid; }
+    else if (p[0] == pairs[2*mid])
+    {
+      if (p[1] > pairs[2*mid+1]) { if (start == mid) break; start = mid; }
+      else if (p[1] == pairs[2*mid+1]) return mid;
+      else end = mid;
+    }
+    else end = mid;
+  }
+  return 0;
+}
+
+} /* namespace */
+
+/* ------------------------------------------------------------- context */
+
+struct orc_ctx
+{
+  int nthreads = 1;
+  int parnum = 0, ellnum = 0, trinum = 0, nodnum = 0, facnum = 0, pairnum = 0;
+  std::vector<double> angular, linear, rotation, position, inertia, inverse, mass, invm, force, torque;
+  std::vector<int> flags;
+  std::vector<double> center, radii;
+  std::vector<int> part, ellcol;
+  std::vector<double> tri;
+  std::vector<int> tricol, triobs;
+  std::vector<double> nodes;
+  std::vector<int> nodpart, facnod, factri;
+  std::vector<int> pairs, ikind;
+  std::vector<double> iparam;
+  double curtime = 0.0, curstep = 0.0;
+  int stepnum = 0;
+  std::vector<std::vector<Contact>> master;
+  std::vector<std::vector<Slave>> slave;
+  pmx_stats stats {};
+
+  double *ang (int k) { return angular.data () + (size_t) k * parnum; }
+  double *lin (int k) { return linear.data () + (size_t) k * parnum; }
+  double *rot (int k) { return rotation.data () + (size_t) k * parnum; }
+  double *pos (int k) { return position.data () + (size_t) k * parnum; }
+  double *cen (int k) { return center.data () + (size_t) k * ellnum; }
+  double *rad (int k) { return radii.data () + (size_t) k * ellnum; }
+  double *trv (int v, int c) { return tri.data () + (size_t) (3*v+c) * trinum; }
+  double *nod (int k) { return nodes.data () + (size_t) k * nodnum; }
+};
+
+namespace {
+
+template <class T> void take (std::vector<T> &dst, const T *src, size_t n)
+{
+  dst.assign (src ? src : (const T*) nullptr, src ? src + n : (const T*) nullptr);
+  if (!src) dst.assign (n, T ());
+}
+
+/* euler_task, parmec.ispc:224-266 */
+void euler (orc_ctx &s, double step)
+{
+  #pragma omp parallel for num_threads(s.nthreads)
+  for (int i = 0; i < s.parnum; i++)
+  {
+    double O[3], L1[9], DL[9], L2[9], o[3];
+    O[0] = step*s.ang(0)[i];
+    O[1] = step*s.ang(1)[i];
+    O[2] = step*s.ang(2)[i];
+    expmap (O[0], O[1], O[2], DL);
+    for (int k = 0; k < 9; k++) L1[k] = s.rot(k)[i];
+    nnmul (L1, DL, L2);
+    for (int k = 0; k < 9; k++) s.rot(k)[i] = L2[k];
+    nvmul (L2, O, o);
+    s.ang(3)[i] = o[0]; s.ang(4)[i] = o[1]; s.ang(5)[i] = o[2];
+    s.pos(0)[i] += step * s.lin(0)[i];
+    s.pos(1)[i] += step * s.lin(1)[i];
+    s.pos(2)[i] += step * s.lin(2)[i];
+  }
+}
+
+/* invert, parmec.ispc:62-87 with INVERT, macros.h:1033-1051 */
+void invert_inertia (orc_ctx &s)
+{
+  for (int i = 0; i < s.parnum; i++)
+  {
+    double F[9], INV[9], det;
+    for (int k = 0; k < 9; k++) F[k] = s.inertia[(size_t) k*s.parnum + i];
+    det = F[0]*(F[4]*F[8]-F[5]*F[7]) + F[3]*(F[2]*F[7]-F[1]*F[8]) + (F[1]*F[5]-F[2]*F[4])*F[6];
+    if (det != 0.0)
+    {
+      det = 1.0 / det;
+      INV[0] = (F[4]*F[8]-F[5]*F[7])*det;
+      INV[1] = (F[2]*F[7]-F[1]*F[8])*det;
+      INV[2] = (F[1]*F[5]-F[2]*F[4])*det;
+      INV[3] = (F[5]*F[6]-F[3]*F[8])*det;
+      INV[4] = (F[0]*F[8]-F[2]*F[6])*det;
+      INV[5] = (F[2]*F[3]-F[0]*F[5])*det;
+      INV[6] = (F[3]*F[7]-F[4]*F[6])*det;
+      INV[7] = (F[1]*F[6]-F[0]*F[7])*det;
+      INV[8] = (F[0]*F[4]-F[1]*F[3])*det;
+      for (int k = 0; k < 9; k++) s.inverse[(size_t) k*s.parnum + i] = INV[k];
+    }
+    s.invm[i] = 1.0 / s.mass[i];
+  }
+}
+
+/* ellipsoids_task / nodes_task / faces_task, shapes.ispc:29-211 (spheres:
+ * centre only; the orientation update applies to ellipsoids, which this path
+ * does not support) */
+void shapes (orc_ctx &s)
+{
+  #pragma omp parallel for num_threads(s.nthreads)
+  for (int i = 0; i < s.ellnum; i++)
+  {
+    int j = s.part[i];
+    double L[9], X[3], x[3], C[3], c[3];
+    for (int k = 0; k < 9; k++) L[k] = s.rot(k)[j];
+    for (int k = 0; k < 3; k++) { X[k] = s.pos(3+k)[j]; x[k] = s.pos(k)[j]; C[k] = s.cen(3+k)[i]; }
+    C[0] -= X[0]; C[1] -= X[1]; C[2] -= X[2];
+    c[0] = x[0] + (L[0]*C[0] + L[3]*C[1] + L[6]*C[2]);
+    c[1] = x[1] + (L[1]*C[0] + L[4]*C[1] + L[7]*C[2]);
+    c[2] = x[2] + (L[2]*C[0] + L[5]*C[1] + L[8]*C[2]);
+    s.cen(0)[i] = c[0]; s.cen(1)[i] = c[1]; s.cen(2)[i] = c[2];
+  }
+  #pragma omp parallel for num_threads(s.nthreads)
+  for (int i = 0; i < s.nodnum; i++)
+  {
+    int j = s.nodpart[i];
+    double L[9], X[3], x[3], C[3], c[3];
+    for (int k = 0; k < 9; k++) L[k] = s.rot(k)[j];
+    for (int k = 0; k < 3; k++) { X[k] = s.pos(3+k)[j]; x[k] = s.pos(k)[j]; C[k] = s.nod(3+k)[i]; }
+    C[0] -= X[0]; C[1] -= X[1]; C[2] -= X[2];
+    c[0] = x[0] + (L[0]*C[0] + L[3]*C[1] + L[6]*C[2]);
+    c[1] = x[1] + (L[1]*C[0] + L[4]*C[1] + L[7]*C[2]);
+    c[2] = x[2] + (L[2]*C[0] + L[5]*C[1] + L[8]*C[2]);
+    s.nod(0)[i] = c[0]; s.nod(1)[i] = c[1]; s.nod(2)[i] = c[2];
+  }
+  for (int i = 0; i < s.facnum; i++)
+  {
+    int j = s.factri[i];
+    for (int v = 0; v < 3; v++)
+    {
+      int k = s.facnod[(size_t) v*s.facnum + i];
+      for (int c = 0; c < 3; c++) s.trv(v, c)[j] = s.nod(c)[k];
+    }
+  }
+}
+
+/* update_existing, condet.ispc:627-700 */
+int update_existing (orc_ctx &s)
+{
+  int bad = 0;
+  #pragma omp parallel for num_threads(s.nthreads) reduction(+:bad)
+  for (int l = 0; l < s.parnum; l++)
+  {
+    for (Contact &con : s.master[l])
+    {
+      int i = con.master, j = con.slave1;
+      if (!(s.rad(1)[i] < 0.0)) { bad++; continue; }
+      if (j < 0)
+      {
+        int u = -j-1;
+        con.depth = triangle_sphere (s.trv(0,0)[u], s.trv(0,1)[u], s.trv(0,2)[u],
+                                     s.trv(1,0)[u], s.trv(1,1)[u], s.trv(1,2)[u],
+                                     s.trv(2,0)[u], s.trv(2,1)[u], s.trv(2,2)[u],
+                                     s.cen(0)[i], s.cen(1)[i], s.cen(2)[i], s.rad(0)[i],
+                                     con.point, con.normal);
+      }
+      else if (s.rad(1)[j] < 0.0)
+      {
+        double p[3] = {s.cen(0)[i], s.cen(1)[i], s.cen(2)[i]};
+        double c[3] = {s.cen(0)[j], s.cen(1)[j], s.cen(2)[j]};
+        double q[3] = {p[0]-c[0], p[1]-c[1], p[2]-c[2]};
+        double len = std::sqrt (q[0]*q[0] + q[1]*q[1] + q[2]*q[2]);
+        double ilen = 1.0/len;
+        con.point[0] = 0.5*(p[0]+c[0]);
+        con.point[1] = 0.5*(p[1]+c[1]);
+        con.point[2] = 0.5*(p[2]+c[2]);
+        con.normal[0] = ilen*q[0];
+        con.normal[1] = ilen*q[1];
+        con.normal[2] = ilen*q[2];
+        con.depth = s.rad(0)[i]+s.rad(0)[j] - len;
+      }
+      else bad++;
+    }
+  }
+  return bad;
+}
+
+struct Cand { int j; double point[3], normal[3], depth; };
+
+/* test_ellipsoids / drop_ellipsoid, condet.ispc:98-290; the traversal of each
+ * sphere runs in parallel, insertions run sequentially afterwards */
+int test_ellipsoids (orc_ctx &s, const Tree &T)
+{
+  int n = s.ellnum, bad = 0;
+  std::vector<std::vector<Cand>> cand (n);
+  #pragma omp parallel for num_threads(s.nthreads) schedule(dynamic, 256) reduction(+:bad)
+  for (int i = 0; i < n; i++)
+  {
+    double p[3] = {s.cen(0)[i], s.cen(1)[i], s.cen(2)[i]};
+    double r[3] = {s.rad(0)[i], s.rad(1)[i], s.rad(2)[i]};
+    double rx = r[0] > r[1] ? r[0] : r[1];
+    rx = rx > r[2] ? rx : r[2];
+    double lo[3] = {p[0]-rx, p[1]-rx, p[2]-rx};
+    double hi[3] = {p[0]+rx, p[1]+rx, p[2]+rx};
+    if (!(r[1] < 0.0)) { bad++; continue; }
+    std::vector<int> leaves;
+    tree_reach (T, 0, lo, hi, leaves);
+    for (int L : leaves)
+      for (int j : T.leaf[L])
+      {
+        if (!(s.rad(1)[j] < 0.0)) { bad++; continue; }
+        double c[3] = {s.cen(0)[j], s.cen(1)[j], s.cen(2)[j]};
+        double q[3] = {p[0]-c[0], p[1]-c[1], p[2]-c[2]};
+        double len = std::sqrt (q[0]*q[0] + q[1]*q[1] + q[2]*q[2]);
+        double ilen = len > 0.0 ? 1.0/len : 1.0;
+        Cand k;
+        k.j = j;
+        k.point[0] = 0.5*(p[0]+c[0]);
+        k.point[1] = 0.5*(p[1]+c[1]);
+        k.point[2] = 0.5*(p[2]+c[2]);
+        k.normal[0] = ilen*q[0];
+        k.normal[1] = ilen*q[1];
+        k.normal[2] = ilen*q[2];
+        k.depth = rx + s.rad(0)[j] - len;
+        if (k.depth > 0.0 && s.part[i] != s.part[j]) cand[i].push_back (k);
+      }
+  }
+  for (int i = 0; i < n; i++)
+  {
+    int part = s.part[i];
+    for (const Cand &k : cand[i])
+    {
+      int lpart = s.part[k.j];
+      int index = std::min (part, lpart);
+      std::vector<Contact> &ml = s.master[index];
+      bool found = false;
+      for (const Contact &con : ml)
+      {
+        if (part == index) { if (con.master == i && con.slave0 == lpart && con.slave1 == k.j) { found = true; break; } }
+        else { if (con.master == k.j && con.slave0 == part && con.slave1 == i) { found = true; break; } }
+      }
+      if (found) continue;
+      Contact con;
+      double sg = part == index ? 1.0 : -1.0;
+      con.master = part == index ? i : k.j;
+      con.slave0 = part == index ? lpart : part;
+      con.slave1 = part == index ? k.j : i;
+      con.color0 = part == index ? s.ellcol[i] : s.ellcol[k.j];
+      con.color1 = part == index ? s.ellcol[k.j] : s.ellcol[i];
+      for (int d = 0; d < 3; d++)
+      {
+        con.point[d] = k.point[d];
+        con.normal[d] = sg > 0 ? k.normal[d] : -k.normal[d];
+        con.state[d] = 0.0;  /* canonical: zero tangential history at creation */
+        con.force[d] = 0.0;
+      }
+      con.depth = k.depth;
+      ml.push_back (con);
+    }
+  }
+  return bad;
+}
+
+/* test_triangles / drop_triangle, condet.ispc:412-516 */
+int test_triangles (orc_ctx &s, const Tree &T)
+{
+  int bad = 0;
+  struct TC { int t, j; double point[3], normal[3], depth; };
+  std::vector<std::vector<TC>> cand (s.trinum);
+  #pragma omp parallel for num_threads(s.nthreads) schedule(dynamic, 16) reduction(+:bad)
+  for (int t = 0; t < s.trinum; t++)
+  {
+    double ax = s.trv(0,0)[t], ay = s.trv(0,1)[t], az = s.trv(0,2)[t];
+    double bx = s.trv(1,0)[t], by = s.trv(1,1)[t], bz = s.trv(1,2)[t];
+    double cx = s.trv(2,0)[t], cy = s.trv(2,1)[t], cz = s.trv(2,2)[t];
+    auto mn = [] (double a, double b, double c) { double d = a < b ? a : b; return d < c ? d : c; };
+    auto mx = [] (double a, double b, double c) { double d = a > b ? a : b; return d > c ? d : c; };
+    double lo[3] = {mn (ax,bx,cx), mn (ay,by,cy), mn (az,bz,cz)};
+    double hi[3] = {mx (ax,bx,cx), mx (ay,by,cy), mx (az,bz,cz)};
+    std::vector<int> leaves;
+    tree_reach (T, 0, lo, hi, leaves);
+    for (int L : leaves)
+      for (int j : T.leaf[L])
+      {
+        if (!(s.rad(1)[j] < 0.0)) { bad++; continue; }
+        TC k; k.t = t; k.j = j;
+        k.depth = triangle_sphere (ax, ay, az, bx, by, bz, cx, cy, cz,
+                                   s.cen(0)[j], s.cen(1)[j], s.cen(2)[j], s.rad(0)[j], k.point, k.normal);
+        if (k.depth > 0.0) cand[t].push_back (k);
+      }
+  }
+  for (int t = 0; t < s.trinum; t++)
+    for (const TC &k : cand[t])
+    {
+      std::vector<Contact> &ml = s.master[s.part[k.j]];
+      bool found = false;
+      for (const Contact &con : ml)
+        if (con.master == k.j && con.slave1 == -(t+1)) { found = true; break; }
+      if (found) continue;
+      Contact con;
+      con.master = k.j;
+      con.slave0 = s.triobs[t];
+      con.slave1 = -(t+1);
+      con.color0 = s.ellcol[k.j];
+      con.color1 = s.tricol[t];
+      for (int d = 0; d < 3; d++) { con.point[d] = k.point[d]; con.normal[d] = k.normal[d]; con.state[d] = 0.0; con.force[d] = 0.0; }
+      con.depth = k.depth;
+      ml.push_back (con);
+    }
+  return bad;
+}
+
+/* granural_force, forces.ispc:125-192 */
+int granular_force (const double n[3], const double vij[3], double depth, int i, int j,
+                    const orc_ctx &s, int ij, double state[3], double step, double f[3])
+{
+  double ma;
+  if (j >= 0) ma = 1.0 / (s.invm[i] + s.invm[j]);
+  else ma = s.mass[i];
+  const double *ip = s.iparam.data ();
+  int np = s.pairnum;
+  double kn = ip[PMX_SPRING*np + ij];
+  double en = ip[PMX_DAMPER*np + ij] * 2.0 * std::sqrt (kn*ma);
+  double vn = vij[0]*n[0] + vij[1]*n[1] + vij[2]*n[2];
+  double fn = kn*depth + en*vn;
+  f[0] = fn*n[0];
+  f[1] = fn*n[1];
+  f[2] = fn*n[2];
+  double kt = kn * ip[PMX_KSKN*np + ij];
+  double et = en * ip[PMX_KSKN*np + ij];
+  double mus = ip[PMX_FRISTAT*np + ij];
+  double mud = ip[PMX_FRIDYN*np + ij];
+  double vt[3], ft[3];
+  vt[0] = vij[0] - vn*n[0];
+  vt[1] = vij[1] - vn*n[1];
+  vt[2] = vij[2] - vn*n[2];
+  state[0] += step*vt[0];
+  state[1] += step*vt[1];
+  state[2] += step*vt[2];
+  ft[0] = kt*state[0] + et*vt[0];
+  ft[1] = kt*state[1] + et*vt[1];
+  ft[2] = kt*state[2] + et*vt[2];
+  double len = std::sqrt (ft[0]*ft[0] + ft[1]*ft[1] + ft[2]*ft[2]);
+  double lim = mus*fn;
+  if (len > lim)
+  {
+    lim = mud*fn/len;
+    ft[0] *= lim;
+    ft[1] *= lim;
+    ft[2] *= lim;
+  }
+  f[0] += ft[0];
+  f[1] += ft[1];
+  f[2] += ft[2];
+  return depth < 0.0 ? 1 : 0;
+}
+
+/* forces: contacts_task (forces.ispc:195-407), then contacts_acc_task
+ * (forces.ispc:453-596) with the canonical summation order */
+int forces (orc_ctx &s, double step, const double g[3])
+{
+  int np = s.parnum, bad = 0;
+  #pragma omp parallel for num_threads(s.nthreads) schedule(dynamic, 256) reduction(+:bad)
+  for (int i = 0; i < np; i++)
+  {
+    double oi[3] = {s.ang(3)[i], s.ang(4)[i], s.ang(5)[i]};
+    double v[3] = {s.lin(0)[i], s.lin(1)[i], s.lin(2)[i]};
+    double x[3] = {s.pos(0)[i], s.pos(1)[i], s.pos(2)[i]};
+    std::vector<Contact> &ml = s.master[i];
+    std::vector<Contact> kept;
+    kept.reserve (ml.size ());
+    for (Contact &con : ml)
+    {
+      double z[3], vi[3], vj[3], oj[3], vij[3];
+      z[0] = con.point[0]-x[0];
+      z[1] = con.point[1]-x[1];
+      z[2] = con.point[2]-x[2];
+      vi[0] = oi[1]*z[2]-oi[2]*z[1] + v[0];
+      vi[1] = oi[2]*z[0]-oi[0]*z[2] + v[1];
+      vi[2] = oi[0]*z[1]-oi[1]*z[0] + v[2];
+      int j = con.slave0;
+      if (j >= 0)
+      {
+        z[0] = con.point[0]-s.pos(0)[j];
+        z[1] = con.point[1]-s.pos(1)[j];
+        z[2] = con.point[2]-s.pos(2)[j];
+        oj[0] = s.ang(3)[j]; oj[1] = s.ang(4)[j]; oj[2] = s.ang(5)[j];
+        vj[0] = oj[1]*z[2]-oj[2]*z[1] + s.lin(0)[j];
+        vj[1] = oj[2]*z[0]-oj[0]*z[2] + s.lin(1)[j];
+        vj[2] = oj[0]*z[1]-oj[1]*z[0] + s.lin(2)[j];
+      }
+      else if (-j-2 >= 0) { bad++; continue; }   /* moving obstacles are not on this path */
+      else { vj[0] = vj[1] = vj[2] = 0.0; }
+      vij[0] = vj[0] - vi[0];
+      vij[1] = vj[1] - vi[1];
+      vij[2] = vj[2] - vi[2];
+      int ij = pairing (s.pairnum, s.pairs.data (), con.color0, con.color1);
+      if (s.ikind[ij] != PMX_GRANULAR_FORCE) { bad++; continue; }
+      double f[3];
+      int gone = granular_force (con.normal, vij, con.depth, i, j, s, ij, con.state, step, f);
+      con.force[0] = f[0]; con.force[1] = f[1]; con.force[2] = f[2];
+      if (!gone) kept.push_back (con);   /* canonical: state moves with its pair */
+    }
+    ml.swap (kept);
+  }
+
+  /* symmetric copy into slave lists, forces.ispc:383-405 */
+  for (auto &sl : s.slave) sl.clear ();
+  for (int i = 0; i < np; i++)
+    for (const Contact &con : s.master[i])
+      if (con.slave0 >= 0)
+      {
+        Slave sv;
+        sv.master0 = i; sv.master1 = con.master; sv.slave1 = con.slave1;
+        for (int d = 0; d < 3; d++) { sv.point[d] = con.point[d]; sv.force[d] = -con.force[d]; }
+        s.slave[con.slave0].push_back (sv);
+      }
+
+  /* accumulation; each particle sums its contacts ordered by (partner identity,
+   * own identity), master- and slave-side entries interleaved */
+  #pragma omp parallel for num_threads(s.nthreads) schedule(dynamic, 256)
+  for (int i = 0; i < np; i++)
+  {
+    struct E { unsigned long long k0, k1; const double *p; double f[3]; };
+    std::vector<E> es;
+    for (const Contact &con : s.master[i])
+      es.push_back ({side_key (con.slave1), side_key (con.master), con.point, {con.force[0], con.force[1], con.force[2]}});
+    for (const Slave &sv : s.slave[i])
+      es.push_back ({side_key (sv.master1), side_key (sv.slave1), sv.point, {sv.force[0], sv.force[1], sv.force[2]}});
+    std::sort (es.begin (), es.end (), [] (const E &a, const E &b) { return a.k0 < b.k0 || (a.k0 == b.k0 && a.k1 < b.k1); });
+    double po[3] = {s.pos(0)[i], s.pos(1)[i], s.pos(2)[i]};
+    double fs[3] = {0.0, 0.0, 0.0}, ts[3] = {0.0, 0.0, 0.0};
+    for (const E &e : es)
+    {
+      double a[3] = {e.p[0]-po[0], e.p[1]-po[1], e.p[2]-po[2]};
+      fs[0] += e.f[0]; fs[1] += e.f[1]; fs[2] += e.f[2];
+      ts[0] += a[1]*e.f[2] - a[2]*e.f[1];
+      ts[1] += a[2]*e.f[0] - a[0]*e.f[2];
+      ts[2] += a[0]*e.f[1] - a[1]*e.f[0];
+    }
+    double ma = s.mass[i];
+    s.force[(size_t) 0*np + i] = fs[0] + ma * g[0];
+    s.force[(size_t) 1*np + i] = fs[1] + ma * g[1];
+    s.force[(size_t) 2*np + i] = fs[2] + ma * g[2];
+    s.torque[(size_t) 0*np + i] = ts[0];
+    s.torque[(size_t) 1*np + i] = ts[1];
+    s.torque[(size_t) 2*np + i] = ts[2];
+  }
+  return bad;
+}
+
+/* dynamics_task, dynamics.ispc:30-169 */
+void dynamics (orc_ctx &s, const double damping[6], double step0, double step1)
+{
+  int np = s.parnum;
+  double half = 0.5*step0;
+  double step = 0.5*(step0+step1);
+  #pragma omp parallel for num_threads(s.nthreads)
+  for (int i = 0; i < np; i++)
+  {
+    double O[3], o[3], v[3], L1[9], J[9], I[9], ma, im, f[3], t[3], T[3], DL[9], L2[9], A[3], B[3];
+    double *force[3] = {&s.force[i], &s.force[(size_t) np + i], &s.force[(size_t) 2*np + i]};
+    double *torque[3] = {&s.torque[i], &s.torque[(size_t) np + i], &s.torque[(size_t) 2*np + i]};
+    for (int k = 0; k < 3; k++) { O[k] = s.ang(k)[i]; v[k] = s.lin(k)[i]; }
+    for (int k = 0; k < 9; k++)
+    {
+      L1[k] = s.rot(k)[i];
+      J[k] = s.inertia[(size_t) k*np + i];
+      I[k] = s.inverse[(size_t) k*np + i];
+    }
+    im = s.invm[i];
+    if (damping[0] != 0.0 || damping[1] != 0.0 || damping[2] != 0.0)
+    {
+      ma = s.mass[i];
+      *force[0] -= ma * damping[0] * v[0];
+      *force[1] -= ma * damping[1] * v[1];
+      *force[2] -= ma * damping[2] * v[2];
+    }
+    if (damping[3] != 0.0 || damping[4] != 0.0 || damping[5] != 0.0)
+    {
+      o[0] = damping[3]*s.ang(3)[i];
+      o[1] = damping[4]*s.ang(4)[i];
+      o[2] = damping[5]*s.ang(5)[i];
+      tvmul (L1, o, t);
+      nvmul (J, t, T);
+      nvmul (L1, T, t);
+      *torque[0] -= t[0];
+      *torque[1] -= t[1];
+      *torque[2] -= t[2];
+    }
+    f[0] = *force[0]; f[1] = *force[1]; f[2] = *force[2];
+    t[0] = *torque[0]; t[1] = *torque[1]; t[2] = *torque[2];
+    tvmul (L1, t, T);
+    expmap (-half*O[0], -half*O[1], -half*O[2], DL);
+    nvmul (J, O, A);
+    nvmul (DL, A, B);
+    B[0] = B[0] + half*T[0];
+    B[1] = B[1] + half*T[1];
+    B[2] = B[2] + half*T[2];
+    nvmul (I, B, A);
+    nvmul (J, A, B);
+    T[0] -= A[1]*B[2] - A[2]*B[1];
+    T[1] -= A[2]*B[0] - A[0]*B[2];
+    T[2] -= A[0]*B[1] - A[1]*B[0];
+    T[0] *= step0; T[1] *= step0; T[2] *= step0;
+    O[0] = O[0] + (I[0]*T[0] + I[3]*T[1] + I[6]*T[2]);
+    O[1] = O[1] + (I[1]*T[0] + I[4]*T[1] + I[7]*T[2]);
+    O[2] = O[2] + (I[2]*T[0] + I[5]*T[1] + I[8]*T[2]);
+    im *= step0;
+    v[0] = v[0] + im*f[0];
+    v[1] = v[1] + im*f[1];
+    v[2] = v[2] + im*f[2];
+    expmap (step*O[0], step*O[1], step*O[2], DL);
+    nnmul (L1, DL, L2);
+    for (int k = 0; k < 9; k++) s.rot(k)[i] = L2[k];
+    nvmul (L2, O, o);
+    s.pos(0)[i] += step*v[0];
+    s.pos(1)[i] += step*v[1];
+    s.pos(2)[i] += step*v[2];
+    for (int k = 0; k < 3; k++) { s.ang(k)[i] = O[k]; s.ang(3+k)[i] = o[k]; s.lin(k)[i] = v[k]; }
+  }
+}
+
+} /* namespace */
+
+/* ------------------------------------------------------------------ API */
+
+extern "C" {
+
+const char *orc_last_error (void) { return g_err.c_str (); }
+
+int orc_create (int nthreads, orc_ctx **ctx)
+{
+  if (!ctx) return fail (PMX_EINVAL, "null ctx");
+  *ctx = new orc_ctx ();
+#ifdef _OPENMP
+  (*ctx)->nthreads = nthreads > 0 ? nthreads : omp_get_max_threads ();
+#else
+  (*ctx)->nthreads = 1;
+  (void) nthreads;
+#endif
+  return PMX_OK;
+}
+
+void orc_destroy (orc_ctx *ctx) { delete ctx; }
+
+int orc_upload (orc_ctx *s, const pmx_model *m)
+{
+  if (!s || !m) return fail (PMX_EINVAL, "null argument");
+  if (m->obsnum > 0) return fail (PMX_EUNSUPPORTED, "moving obstacles are not supported by the oracle");
+  if (m->sprnum > 0) return fail (PMX_EUNSUPPORTED, "springs are not supported by this oracle build");
+  bool same = s->parnum == m->parnum && s->ellnum == m->ellnum && s->trinum == m->trinum;
+  s->parnum = m->parnum; s->ellnum = m->ellnum; s->trinum = m->trinum;
+  s->nodnum = m->nodnum; s->facnum = m->facnum; s->pairnum = m->pairnum;
+  size_t np = m->parnum, ne = m->ellnum, nt = m->trinum;
+  take (s->angular, m->angular, 6*np);
+  take (s->linear, m->linear, 3*np);
+  take (s->rotation, m->rotation, 9*np);
+  take (s->position, m->position, 6*np);
+  take (s->inertia, m->inertia, 9*np);
+  take (s->inverse, m->inverse, 9*np);
+  take (s->mass, m->mass, np);
+  take (s->invm, m->invm, np);
+  take (s->force, m->force, 3*np);
+  take (s->torque, m->torque, 3*np);
+  take (s->flags, m->flags, np);
+  take (s->center, m->center, 6*ne);
+  take (s->radii, m->radii, 3*ne);
+  take (s->part, m->part, ne);
+  take (s->ellcol, m->ellcol, ne);
+  take (s->tri, m->tri, 9*nt);
+  take (s->tricol, m->tricol, nt);
+  take (s->triobs, m->triobs, nt);
+  take (s->nodes, m->nodes, 6*(size_t) m->nodnum);
+  take (s->nodpart, m->nodpart, (size_t) m->nodnum);
+  take (s->facnod, m->facnod, 3*(size_t) m->facnum);
+  take (s->factri, m->factri, (size_t) m->facnum);
+  take (s->pairs, m->pairs, 2*(size_t) m->pairnum);
+  take (s->ikind, m->ikind, (size_t) m->pairnum);
+  take (s->iparam, m->iparam, (size_t) PMX_NIPARAM * m->pairnum);
+  s->curtime = m->curtime; s->curstep = m->curstep; s->stepnum = m->stepnum;
+  if (!same)
+  {
+    s->master.assign (np, {});
+    s->slave.assign (np, {});
+  }
+  return PMX_OK;
+}
+
+int orc_download (orc_ctx *s, pmx_model *m)
+{
+  if (!s || !m) return fail (PMX_EINVAL, "null argument");
+  if (m->parnum != s->parnum || m->ellnum != s->ellnum || m->trinum != s->trinum)
+    return fail (PMX_EINVAL, "model counts differ from the uploaded state");
+  auto put = [] (double *dst, const std::vector<double> &src) { if (dst) std::memcpy (dst, src.data (), src.size ()*sizeof (double)); };
+  put (m->angular, s->angular); put (m->linear, s->linear); put (m->rotation, s->rotation);
+  put (m->position, s->position); put (m->inverse, s->inverse); put (m->invm, s->invm);
+  put (m->force, s->force); put (m->torque, s->torque); put (m->center, s->center);
+  put (m->tri, s->tri); put (m->nodes, s->nodes);
+  m->curtime = s->curtime; m->curstep = s->curstep; m->stepnum = s->stepnum;
+  return PMX_OK;
+}
+
+int orc_step_times (double duration, double step, double curtime, double *times, int cap, int *nsteps);
+
+int orc_dem (orc_ctx *s, double duration, double step, const pmx_forcing *fc, double *elapsed)
+{
+  if (!s) return fail (PMX_EINVAL, "null ctx");
+  if (!(duration > 0.0) || !(step > 0.0)) return fail (PMX_EINVAL, "duration and step must be positive");
+  auto t0 = std::chrono::steady_clock::now ();
+  double step0, step1;
+  if (s->curtime == 0.0)
+  {
+    step0 = step;
+    euler (*s, 0.5*step0);
+    shapes (*s);
+  }
+  else step0 = s->curstep;
+  step1 = step0;
+  invert_inertia (*s);
+  Tree T;
+  tree_create (s->ellnum, s->cen(0), s->cen(1), s->cen(2), T);
+  long long k = 0, rebuilds = 1;
+  for (double time = 0.0; time < duration; time += 0.5*(step0+step1), s->curtime += 0.5*(step0+step1), step0 = step1, k++)
+  {
+    if (tree_store (T, s->ellnum, s->cen(0), s->cen(1), s->cen(2)) > 0)
+    {
+      tree_create (s->ellnum, s->cen(0), s->cen(1), s->cen(2), T);
+      rebuilds++;
+      if (tree_store (T, s->ellnum, s->cen(0), s->cen(1), s->cen(2)) != 0) return fail (PMX_EREPART, "Repartitioning failed");
+    }
+    int bad = 0;
+    if (s->ellnum > 0)
+    {
+      bad += update_existing (*s);
+      bad += test_ellipsoids (*s, T);
+      bad += test_triangles (*s, T);
+    }
+    if (bad) return fail (PMX_EUNSUPPORTED, "ellipsoid (non-sphere) shapes are not supported");
+    double g[3], damp[6];
+    for (int d = 0; d < 3; d++) g[d] = fc && fc->gtab ? fc->gtab[3*k+d] : (fc ? fc->gravity[d] : 0.0);
+    for (int d = 0; d < 6; d++) damp[d] = fc && fc->dtab ? fc->dtab[6*k+d] : (fc ? fc->damping[d] : 0.0);
+    if (forces (*s, step0, g)) return fail (PMX_EUNSUPPORTED, "unsupported contact kind");
+    step1 = step0;
+    dynamics (*s, damp, step0, step1);
+    shapes (*s);
+  }
+  s->curstep = step1;
+  s->stepnum++;
+  size_t pairs = 0;
+  for (auto &ml : s->master) pairs += ml.size ();
+  s->stats.steps = k;
+  s->stats.rebuilds = rebuilds;
+  s->stats.pairs = (long long) pairs;
+  s->stats.leaves = (int) T.leaf.size ();
+  double dt = std::chrono::duration<double> (std::chrono::steady_clock::now () - t0).count ();
+  s->stats.seconds = dt;
+  if (elapsed) *elapsed = dt;
+  return PMX_OK;
+}
+
+int orc_step_times (double duration, double step, double curtime, double *times, int cap, int *nsteps)
+{
+  int k = 0;
+  for (double time = 0.0; time < duration; time += 0.5*(step+step), curtime += 0.5*(step+step), k++)
+    if (times && k < cap) times[k] = curtime;
+  if (nsteps) *nsteps = k;
+  return PMX_OK;
+}
+
+int orc_contacts (orc_ctx *s, pmx_pair *out, size_t cap, size_t *count)
+{
+  if (!s) return fail (PMX_EINVAL, "null ctx");
+  std::vector<pmx_pair> all;
+  for (auto &ml : s->master)
+    for (const Contact &c : ml)
+    {
+      pmx_pair p;
+      p.master = c.master;
+      p.slave = c.slave1;
+      for (int d = 0; d < 3; d++) p.state[d] = c.state[d];
+      all.push_back (p);
+    }
+  std::sort (all.begin (), all.end (), [] (const pmx_pair &a, const pmx_pair &b) {
+    return a.master < b.master || (a.master == b.master && side_key (a.slave) < side_key (b.slave)); });
+  if (count) *count = all.size ();
+  if (out) std::memcpy (out, all.data (), std::min (cap, all.size ()) * sizeof (pmx_pair));
+  return PMX_OK;
+}
+
+int orc_get_stats (orc_ctx *s, pmx_stats *st)
+{
+  if (!s || !st) return fail (PMX_EINVAL, "null argument");
+  *st = s->stats;
+  return PMX_OK;
+}
+
+void orc_expmap (const double omega[3], double lambda[9]) { expmap (omega[0], omega[1], omega[2], lambda); }
+
+double orc_triangle_sphere (const double a[3], const double b[3], const double c[3],
+                            const double p[3], double r, double point[3], double normal[3])
+{
+  return triangle_sphere (a[0], a[1], a[2], b[0], b[1], b[2], c[0], c[1], c[2], p[0], p[1], p[2], r, point, normal);
+}
+
+unsigned int orc_morton (const double p[3], const double extents[6]) { return morton (p[0], p[1], p[2], extents); }
+
+long long orc_tree_pairs (int n, const double *center, const double *radius, int *out, long long cap)
+{
+  Tree T;
+  const double *cx = center, *cy = center + n, *cz = center + 2*(size_t) n;
+  tree_create (n, cx, cy, cz, T);
+  if (tree_store (T, n, cx, cy, cz) > 0) return -1;
+  std::vector<std::pair<int,int>> pr;
+  for (int i = 0; i < n; i++)
+  {
+    double p[3] = {cx[i], cy[i], cz[i]}, rx = radius[i];
+    double lo[3] = {p[0]-rx, p[1]-rx, p[2]-rx}, hi[3] = {p[0]+rx, p[1]+rx, p[2]+rx};
+    std::vector<int> leaves;
+    tree_reach (T, 0, lo, hi, leaves);
+    for (int L : leaves)
+      for (int j : T.leaf[L])
+      {
+        double q[3] = {p[0]-cx[j], p[1]-cy[j], p[2]-cz[j]};
+        double len = std::sqrt (q[0]*q[0] + q[1]*q[1] + q[2]*q[2]);
+        if (rx + radius[j] - len > 0.0 && i != j) pr.push_back ({std::min (i, j), std::max (i, j)});
+      }
+  }
+  std::sort (pr.begin (), pr.end ());
+  pr.erase (std::unique (pr.begin (), pr.end ()), pr.end ());
+  long long m = (long long) pr.size ();
+  for (long long k = 0; k < m && k < cap; k++) { out[2*k] = pr[k].first; out[2*k+1] = pr[k].second; }
+  return m;
+}
+
+} /* extern "C" */

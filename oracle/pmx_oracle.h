@@ -1,0 +1,54 @@
+/*
+ * pmx_oracle.h -- CPU oracle of Parmec's DEM step path (TEST INFRASTRUCTURE).
+ *
+ * This is a scalar C++ restatement of the reference ISPC kernels, used only by
+ * tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg, as the
+ * checker.  It is never linked into or called by the product library.
+ *
+ * Canonical choices where the reference is nondeterministic (documented per
+ * function in pmx_oracle.cpp):
+ *   - Morton codes are sorted stably (partition.ispc:97-243 is unstable);
+ *   - tangential contact state starts at zero and moves with its pair
+ *     (condet.ispc:52-81 leaves it uninitialised, forces.ispc:329-364 does not
+ *     move it on compaction);
+ *   - per-particle force/torque sums run sequentially over the particle's
+ *     contacts ordered by the partner's identity (forces.ispc:492-577 sums in
+ *     lane/task-dependent order).
+ */
+#ifndef PMX_ORACLE_H
+#define PMX_ORACLE_H
+
+#include "../include/pmx.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_ctx orc_ctx;
+
+int orc_create (int nthreads, orc_ctx **ctx);
+int orc_upload (orc_ctx *ctx, const pmx_model *model);
+int orc_dem (orc_ctx *ctx, double duration, double step, const pmx_forcing *forcing, double *elapsed);
+int orc_download (orc_ctx *ctx, pmx_model *model);
+int orc_contacts (orc_ctx *ctx, pmx_pair *out, size_t cap, size_t *count);
+int orc_get_stats (orc_ctx *ctx, pmx_stats *stats);
+void orc_destroy (orc_ctx *ctx);
+const char *orc_last_error (void);
+
+/* pure-function hooks for known-answer tests */
+void orc_expmap (const double omega[3], double lambda[9]);
+double orc_triangle_sphere (const double a[3], const double b[3], const double c[3],
+                            const double p[3], double r, double point[3], double normal[3]);
+/* leaf-reachability pair set of a freshly built tree over the given centres
+ * (sphere-sphere pairs with positive depth, as (master, slave) ellipsoid
+ * indices assuming ellipsoid i belongs to particle i); returns count, writes
+ * at most cap pairs */
+long long orc_tree_pairs (int n, const double *center, const double *radius, int *out, long long cap);
+/* Morton code of one point given the extents (partition.ispc:69-94) */
+unsigned int orc_morton (const double p[3], const double extents[6]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

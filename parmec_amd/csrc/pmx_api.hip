@@ -1,0 +1,320 @@
+/*
+ * pmx_api.hip -- C ABI of libpmx.so and the host step loop.
+ *
+ * pmx_dem() follows dem() (parmec.cpp:2315-2530) for the supported subset:
+ * first call (curtime == 0): euler(h/2) and a full shapes update
+ * (parmec.cpp:2386-2400); every call: invert_inertia (2411) and a fresh
+ * partitioning tree (2415); per step: partitioning_store with rebuild on leaf
+ * overflow (2420-2427), condet (2429), gravity (2432), forces (2434), step1 =
+ * step0 (2450), dynamics (2461), shapes (2485-2496); loop control
+ * `time < duration` with time += (step0+step1)/2 (2418); after the loop
+ * curstep = step1 and stepnum++ (2521-2523).
+ */
+#include "pmx_internal.h"
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+
+static thread_local std::string g_last_error;
+
+void pmx_set_error (const std::string &msg) { g_last_error = msg; }
+
+namespace {
+
+int fail (int code, const std::string &msg) { pmx_set_error (msg); return code; }
+
+template <class T> int up (pmx_ctx &c, dvec<T> &d, const T *src, size_t n, const char *what)
+{
+  if (d.alloc (std::max (n, (size_t) 1)) != hipSuccess) return fail (PMX_ENOMEM, std::string ("out of device memory: ") + what);
+  if (n == 0) return PMX_OK;
+  if (!src) return fail (PMX_EINVAL, std::string ("missing array: ") + what);
+  PMX_HIP (hipMemcpyAsync (d.p, src, n * sizeof (T), hipMemcpyHostToDevice, c.stream));
+  return PMX_OK;
+}
+
+template <class T> int down (pmx_ctx &c, T *dst, const dvec<T> &d, size_t n)
+{
+  if (!dst || n == 0) return PMX_OK;
+  PMX_HIP (hipMemcpyAsync (dst, d.p, n * sizeof (T), hipMemcpyDeviceToHost, c.stream));
+  return PMX_OK;
+}
+
+int alloc_rows (pmx_ctx &c)
+{
+  size_t E = std::max (c.E, 1);
+  for (int b = 0; b < 2; b++)
+    if (c.rowkey[b].alloc ((size_t) c.cap * E) || c.rowst[b].alloc (3 * (size_t) c.cap * E) || c.rowcnt[b].alloc (E))
+      return fail (PMX_ENOMEM, "out of device memory (contact rows)");
+  if (c.hits.alloc ((size_t) c.hcap * E) || c.hitcnt.alloc (E) || c.xbuf.alloc ((size_t) c.xcap * E) ||
+      c.xcnt.alloc (E) || c.leafof.alloc (E) || c.slot.alloc (E) || c.lmem.alloc (E) || c.lpos.alloc (4 * E))
+    return fail (PMX_ENOMEM, "out of device memory (broad phase)");
+  PMX_HIP (hipMemsetAsync (c.rowcnt[0].p, 0, E * sizeof (int), c.stream));
+  PMX_HIP (hipMemsetAsync (c.rowcnt[1].p, 0, E * sizeof (int), c.stream));
+  c.cur = 0;
+  return PMX_OK;
+}
+
+} /* namespace */
+
+extern "C" {
+
+const char *pmx_last_error (void) { return g_last_error.c_str (); }
+
+int pmx_step_times (double duration, double step, double curtime, double *times, int cap, int *nsteps)
+{
+  if (!(step > 0.0)) return fail (PMX_EINVAL, "step must be positive");
+  int k = 0;
+  for (double time = 0.0; time < duration; time += 0.5*(step+step), curtime += 0.5*(step+step), k++)
+    if (times && k < cap) times[k] = curtime;
+  if (nsteps) *nsteps = k;
+  return PMX_OK;
+}
+
+int pmx_create (const pmx_options *opt, pmx_ctx **out)
+{
+  if (!out) return fail (PMX_EINVAL, "null ctx pointer");
+  *out = nullptr;
+  pmx_ctx *c = new pmx_ctx ();
+  if (opt) c->opt = *opt;
+  if (c->opt.precision != PMX_FP64)
+  {
+    delete c;
+    return fail (PMX_EUNSUPPORTED, "only PMX_FP64 is available in this build");
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount (&ndev) != hipSuccess || ndev == 0)
+  {
+    delete c;
+    return fail (PMX_EDEVICE, "no HIP device available");
+  }
+  c->device = c->opt.device;
+  if (hipSetDevice (c->device) != hipSuccess || hipStreamCreateWithFlags (&c->stream, hipStreamNonBlocking) != hipSuccess)
+  {
+    delete c;
+    return fail (PMX_EDEVICE, "cannot open HIP device/stream");
+  }
+  if (c->opt.row_capacity > 0) { c->cap = c->opt.row_capacity; c->hcap = c->opt.row_capacity; c->xcap = std::max (8, c->opt.row_capacity / 2); }
+  if (c->dflags.alloc (FLAG_COUNT) != hipSuccess) { delete c; return fail (PMX_ENOMEM, "out of device memory"); }
+  if (hipMemset (c->dflags.p, 0, FLAG_COUNT * sizeof (int)) != hipSuccess) { delete c; return fail (PMX_EDEVICE, "memset failed"); }
+  *out = c;
+  return PMX_OK;
+}
+
+void pmx_destroy (pmx_ctx *c)
+{
+  if (!c) return;
+  (void) hipSetDevice (c->device);
+  if (c->stream) (void) hipStreamSynchronize (c->stream);
+  dvec<double> *d[] = {&c->angular, &c->linear, &c->rotation, &c->position, &c->inertia, &c->inverse, &c->mass,
+                       &c->invm, &c->force, &c->torque, &c->center, &c->radii, &c->tri, &c->iparam, &c->gtab, &c->dtab,
+                       &c->lpos, &c->rowst[0], &c->rowst[1], &c->tree.ext, &c->tree.coord, &c->tree.cell};
+  for (auto *p : d) p->release ();
+  dvec<int> *ii[] = {&c->flags, &c->part, &c->ellcol, &c->tricol, &c->triobs, &c->pairs, &c->ikind, &c->leafof,
+                     &c->slot, &c->lcount, &c->loff, &c->lmem, &c->rowcnt[0], &c->rowcnt[1], &c->hitcnt, &c->xbuf,
+                     &c->xcnt, &c->dflags, &c->tree.order, &c->tree.order2, &c->tree.hist, &c->tree.scanbuf,
+                     &c->tree.first, &c->tree.size, &c->tree.split, &c->tree.dim, &c->tree.parent, &c->tree.left,
+                     &c->tree.right, &c->tree.nleafkids, &c->tree.leafpar, &c->tree.tricnt, &c->tree.trioff,
+                     &c->tree.trilist};
+  for (auto *p : ii) p->release ();
+  c->rowkey[0].release (); c->rowkey[1].release (); c->hits.release ();
+  c->tree.code.release (); c->tree.code2.release ();
+  if (c->stream) (void) hipStreamDestroy (c->stream);
+  delete c;
+}
+
+int pmx_upload (pmx_ctx *c, const pmx_model *m)
+{
+  if (!c || !m) return fail (PMX_EINVAL, "null argument");
+  if (m->parnum < 0 || m->ellnum < 0 || m->trinum < 0 || m->pairnum < 1)
+    return fail (PMX_EINVAL, "invalid counts (pairnum must include the default pairing)");
+  if (m->obsnum > 0) return fail (PMX_EUNSUPPORTED, "moving obstacles are not supported on the device path yet");
+  if (m->sprnum > 0) return fail (PMX_EUNSUPPORTED, "SPRING elements are not supported on the device path yet");
+  if (m->nodnum > 0 || m->facnum > 0) return fail (PMX_EUNSUPPORTED, "meshed particles are not supported on the device path yet");
+  for (int e = 0; e < m->ellnum; e++)
+  {
+    if (!(m->radii[(size_t) m->ellnum + e] < 0.0)) return fail (PMX_EUNSUPPORTED, "only spherical ellipsoids are supported");
+    if (m->part[e] < 0 || m->part[e] >= m->parnum) return fail (PMX_EINVAL, "ellipsoid particle index out of range");
+  }
+  for (int t = 0; t < m->trinum; t++)
+    if (m->triobs[t] != -1) return fail (PMX_EUNSUPPORTED, "only static obstacle triangles are supported");
+  (void) hipSetDevice (c->device);
+  bool same = c->uploaded && c->N == m->parnum && c->E == m->ellnum && c->T == m->trinum;
+  c->N = m->parnum; c->E = m->ellnum; c->T = m->trinum; c->P = m->pairnum;
+  size_t N = c->N, E = c->E, T = c->T, P = c->P;
+  int rc = 0;
+  if ((rc = up (*c, c->angular, m->angular, 6*N, "angular")) || (rc = up (*c, c->linear, m->linear, 3*N, "linear")) ||
+      (rc = up (*c, c->rotation, m->rotation, 9*N, "rotation")) || (rc = up (*c, c->position, m->position, 6*N, "position")) ||
+      (rc = up (*c, c->inertia, m->inertia, 9*N, "inertia")) || (rc = up (*c, c->inverse, m->inverse, 9*N, "inverse")) ||
+      (rc = up (*c, c->mass, m->mass, N, "mass")) || (rc = up (*c, c->invm, m->invm, N, "invm")) ||
+      (rc = up (*c, c->center, m->center, 6*E, "center")) || (rc = up (*c, c->radii, m->radii, 3*E, "radii")) ||
+      (rc = up (*c, c->part, m->part, E, "part")) || (rc = up (*c, c->ellcol, m->ellcol, E, "ellcol")) ||
+      (rc = up (*c, c->tri, m->tri, 9*T, "tri")) || (rc = up (*c, c->tricol, m->tricol, T, "tricol")) ||
+      (rc = up (*c, c->triobs, m->triobs, T, "triobs")) || (rc = up (*c, c->pairs, m->pairs, 2*P, "pairs")) ||
+      (rc = up (*c, c->ikind, m->ikind, P, "ikind")) || (rc = up (*c, c->iparam, m->iparam, (size_t) PMX_NIPARAM*P, "iparam")))
+    return rc;
+  if (c->force.alloc (3*std::max (N, (size_t) 1)) || c->torque.alloc (3*std::max (N, (size_t) 1)))
+    return fail (PMX_ENOMEM, "out of device memory");
+  if (m->force) PMX_HIP (hipMemcpyAsync (c->force.p, m->force, 3*N*sizeof (double), hipMemcpyHostToDevice, c->stream));
+  if (m->torque) PMX_HIP (hipMemcpyAsync (c->torque.p, m->torque, 3*N*sizeof (double), hipMemcpyHostToDevice, c->stream));
+  if (!same)
+  {
+    if ((rc = alloc_rows (*c))) return rc;
+  }
+  c->curtime = m->curtime; c->curstep = m->curstep; c->stepnum = m->stepnum;
+  PMX_HIP (hipStreamSynchronize (c->stream));
+  c->uploaded = true;
+  return PMX_OK;
+}
+
+int pmx_download (pmx_ctx *c, pmx_model *m)
+{
+  if (!c || !m) return fail (PMX_EINVAL, "null argument");
+  if (m->parnum != c->N || m->ellnum != c->E || m->trinum != c->T)
+    return fail (PMX_EINVAL, "model counts differ from the uploaded state");
+  (void) hipSetDevice (c->device);
+  size_t N = c->N, E = c->E, T = c->T;
+  int rc = 0;
+  if ((rc = down (*c, m->angular, c->angular, 6*N)) || (rc = down (*c, m->linear, c->linear, 3*N)) ||
+      (rc = down (*c, m->rotation, c->rotation, 9*N)) || (rc = down (*c, m->position, c->position, 6*N)) ||
+      (rc = down (*c, m->inverse, c->inverse, 9*N)) || (rc = down (*c, m->invm, c->invm, N)) ||
+      (rc = down (*c, m->force, c->force, 3*N)) || (rc = down (*c, m->torque, c->torque, 3*N)) ||
+      (rc = down (*c, m->center, c->center, 6*E)) || (rc = down (*c, m->tri, c->tri, 9*T)))
+    return rc;
+  PMX_HIP (hipStreamSynchronize (c->stream));
+  m->curtime = c->curtime; m->curstep = c->curstep; m->stepnum = c->stepnum;
+  return PMX_OK;
+}
+
+int pmx_dem (pmx_ctx *c, double duration, double step, const pmx_forcing *fc, double *elapsed)
+{
+  if (!c) return fail (PMX_EINVAL, "null ctx");
+  if (!c->uploaded) return fail (PMX_EINVAL, "pmx_upload must precede pmx_dem");
+  if (!(duration > 0.0) || !(step > 0.0)) return fail (PMX_EINVAL, "duration and step must be positive");
+  (void) hipSetDevice (c->device);
+  auto t0 = std::chrono::steady_clock::now ();
+  int rc = 0;
+  double step0, step1;
+  if (c->curtime == 0.0)
+  {
+    step0 = step;
+    if ((rc = step_euler (*c, 0.5*step0)) || (rc = step_shapes (*c))) return rc;
+  }
+  else step0 = c->curstep;
+  step1 = step0;
+  if ((rc = step_invert (*c))) return rc;
+
+  /* per-step tables of the host-evaluated inputs (input.cpp:4424-4503) */
+  int nsteps = 0;
+  pmx_step_times (duration, step0, c->curtime, nullptr, 0, &nsteps);
+  {
+    std::vector<double> g (3 * (size_t) std::max (nsteps, 1)), d (6 * (size_t) std::max (nsteps, 1));
+    for (int k = 0; k < nsteps; k++)
+    {
+      for (int q = 0; q < 3; q++) g[3*(size_t) k + q] = fc && fc->gtab ? fc->gtab[3*(size_t) k + q] : (fc ? fc->gravity[q] : 0.0);
+      for (int q = 0; q < 6; q++) d[6*(size_t) k + q] = fc && fc->dtab ? fc->dtab[6*(size_t) k + q] : (fc ? fc->damping[q] : 0.0);
+    }
+    if ((rc = up (*c, c->gtab, g.data (), g.size (), "gravity table")) || (rc = up (*c, c->dtab, d.data (), d.size (), "damping table")))
+      return rc;
+  }
+
+  long long rebuilds = 0;
+  if (c->E > 0)
+  {
+    if ((rc = tree_build (*c)) || (rc = tree_trilists (*c))) return rc;
+    rebuilds++;
+  }
+  PMX_HIP (hipMemsetAsync (c->dflags.p, 0, FLAG_COUNT * sizeof (int), c->stream));
+  c->stats.max_row = 0;
+  auto tl = std::chrono::steady_clock::now ();
+  long long k = 0;
+  for (double time = 0.0; time < duration; time += 0.5*(step0+step1), c->curtime += 0.5*(step0+step1), step0 = step1, k++)
+  {
+    if (c->E > 0)
+    {
+      int overflow = 0;
+      if ((rc = step_bin (*c, &overflow))) return rc;
+      if (overflow > 0)
+      {
+        if ((rc = tree_build (*c)) || (rc = tree_trilists (*c))) return rc;
+        rebuilds++;
+        if ((rc = step_bin (*c, &overflow))) return rc;
+        if (overflow > 0) return fail (PMX_EREPART, "Repartitioning failed");
+      }
+      if ((rc = step_detect (*c))) return rc;
+    }
+    const double *gk = c->gtab.p + 3*(size_t) std::min<long long> (k, std::max (nsteps-1, 0));
+    const double *dk = c->dtab.p + 6*(size_t) std::min<long long> (k, std::max (nsteps-1, 0));
+    if ((rc = step_rows (*c, gk, nullptr, step0))) return rc;
+    step1 = step0;
+    if ((rc = step_dynamics (*c, dk, nullptr, step0, step1))) return rc;
+    if ((rc = step_shapes (*c))) return rc;
+  }
+  /* surface errors of the last step */
+  if (c->E > 0)
+  {
+    int dummy = 0;
+    if ((rc = step_bin (*c, &dummy))) return rc;
+  }
+  PMX_HIP (hipStreamSynchronize (c->stream));
+  c->curstep = step1;
+  c->stepnum++;
+  double dt = std::chrono::duration<double> (std::chrono::steady_clock::now () - t0).count ();
+  c->stats.steps = k;
+  c->stats.rebuilds = rebuilds;
+  c->stats.leaves = c->tree.nleaves;
+  c->stats.seconds = std::chrono::duration<double> (std::chrono::steady_clock::now () - tl).count ();
+  {
+    std::vector<int> cnt (std::max (c->E, 1));
+    if (c->E > 0)
+    {
+      PMX_HIP (hipMemcpy (cnt.data (), c->rowcnt[c->cur].p, sizeof (int) * c->E, hipMemcpyDeviceToHost));
+    }
+    long long total = 0;
+    for (int e = 0; e < c->E; e++) total += cnt[e];
+    c->stats.pairs = total;  /* entries; sphere pairs are held twice */
+  }
+  if (elapsed) *elapsed = dt;
+  return PMX_OK;
+}
+
+int pmx_contacts (pmx_ctx *c, pmx_pair *out, size_t cap, size_t *count)
+{
+  if (!c) return fail (PMX_EINVAL, "null ctx");
+  (void) hipSetDevice (c->device);
+  size_t E = c->E;
+  std::vector<pmx_pair> all;
+  if (E > 0)
+  {
+    std::vector<uint32_t> key ((size_t) c->cap * E);
+    std::vector<double> st (3 * (size_t) c->cap * E);
+    std::vector<int> cnt (E), part (E);
+    PMX_HIP (hipMemcpy (key.data (), c->rowkey[c->cur].p, key.size () * sizeof (uint32_t), hipMemcpyDeviceToHost));
+    PMX_HIP (hipMemcpy (st.data (), c->rowst[c->cur].p, st.size () * sizeof (double), hipMemcpyDeviceToHost));
+    PMX_HIP (hipMemcpy (cnt.data (), c->rowcnt[c->cur].p, E * sizeof (int), hipMemcpyDeviceToHost));
+    PMX_HIP (hipMemcpy (part.data (), c->part.p, E * sizeof (int), hipMemcpyDeviceToHost));
+    for (size_t e = 0; e < E; e++)
+      for (int k = 0; k < cnt[e]; k++)
+      {
+        uint32_t kk = key[(size_t) k * E + e];
+        bool tri = (kk & PMX_TRI_FLAG) != 0;
+        if (!tri && !(part[e] < part[kk])) continue;
+        pmx_pair p;
+        p.master = (int) e;
+        p.slave = tri ? -(int) (kk & ~PMX_TRI_FLAG) - 1 : (int) kk;
+        for (int d = 0; d < 3; d++) p.state[d] = st[((size_t) d * c->cap + k) * E + e];
+        all.push_back (p);
+      }
+  }
+  if (count) *count = all.size ();
+  if (out) std::memcpy (out, all.data (), std::min (cap, all.size ()) * sizeof (pmx_pair));
+  return PMX_OK;
+}
+
+int pmx_get_stats (pmx_ctx *c, pmx_stats *s)
+{
+  if (!c || !s) return fail (PMX_EINVAL, "null argument");
+  *s = c->stats;
+  return PMX_OK;
+}
+
+} /* extern "C" */

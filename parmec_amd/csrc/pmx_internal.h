@@ -1,0 +1,128 @@
+/*
+ * pmx_internal.h -- device buffers and launcher declarations shared by the
+ * translation units of libpmx.so.  Not part of the public ABI.
+ */
+#ifndef PMX_INTERNAL_H
+#define PMX_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/pmx.h"
+
+#define PMX_TPB 256            /* threads per block of the elementwise kernels */
+#define PMX_TRI_FLAG 0x80000000u  /* row key flag of a triangle partner */
+
+/* device error/overflow flags, one int each */
+enum { FLAG_LEAF_OVERFLOW = 0, FLAG_ROW_OVERFLOW, FLAG_HIT_OVERFLOW, FLAG_EXTRA_OVERFLOW,
+       FLAG_TRILIST_OVERFLOW, FLAG_MAXROW, FLAG_BAD_KIND, FLAG_COUNT };
+
+void pmx_set_error (const std::string &msg);
+
+#define PMX_HIP(call) do { hipError_t e_ = (call); if (e_ != hipSuccess) { \
+  pmx_set_error (std::string ("HIP error: ") + hipGetErrorString (e_) + " at " + __FILE__ + ":" + std::to_string (__LINE__)); \
+  return PMX_EDEVICE; } } while (0)
+
+template <class T> struct dvec
+{
+  T *p = nullptr;
+  size_t n = 0;
+  hipError_t alloc (size_t count)
+  {
+    if (count <= n && p) return hipSuccess;
+    if (p) { (void) hipFree (p); p = nullptr; n = 0; }
+    if (count == 0) return hipSuccess;
+    hipError_t e = hipMalloc ((void**) &p, count * sizeof (T));
+    if (e == hipSuccess) n = count;
+    return e;
+  }
+  void release () { if (p) (void) hipFree (p); p = nullptr; n = 0; }
+};
+
+/* partitioning tree on the device (the frozen k-d tree of partition.ispc) */
+struct TreeDev
+{
+  int n = 0;                 /* points the tree was built over */
+  int nleaves = 0;
+  int root = -1;             /* node reference of the root */
+  /* scratch of the build */
+  dvec<double> ext;          /* [6] extents; partials follow */
+  dvec<uint32_t> code, code2;
+  dvec<int> order, order2;
+  dvec<int> hist;            /* radix histogram / scan scratch */
+  dvec<int> scanbuf;         /* block sums for scans */
+  /* internal nodes (n-1) */
+  dvec<int> first, size, split, dim, parent;
+  dvec<double> coord;
+  dvec<int> left, right;     /* child refs: >= 0 branch node, < 0 leaf -(L+1) */
+  dvec<int> nleafkids;       /* per internal node: leaf children (scan -> leaf ids) */
+  /* leaves */
+  dvec<int> leafpar;         /* [nleaves] parent branch node*2 + side (or -1 for a root leaf) */
+  dvec<double> cell;         /* [6][nleaves]: lower bounds 0-2, upper bounds 3-5 */
+  dvec<int> tricnt, trioff, trilist;  /* leaf -> triangles whose AABB reaches it */
+  int ntrilist = 0;
+};
+
+struct pmx_ctx
+{
+  pmx_options opt {};
+  hipStream_t stream = nullptr;
+  int device = 0;
+  int N = 0, E = 0, T = 0, P = 0, NOD = 0, FAC = 0;
+
+  /* particles */
+  dvec<double> angular, linear, rotation, position, inertia, inverse, mass, invm, force, torque;
+  dvec<int> flags;
+  /* ellipsoids */
+  dvec<double> center, radii;
+  dvec<int> part, ellcol;
+  /* triangles */
+  dvec<double> tri;
+  dvec<int> tricol, triobs;
+  /* pairing */
+  dvec<int> pairs, ikind;
+  dvec<double> iparam;
+  /* per-step tables */
+  dvec<double> gtab, dtab;
+
+  /* broad phase per step */
+  TreeDev tree;
+  dvec<int> leafof, slot, lcount, loff, lmem;
+  dvec<double> lpos;           /* [4][E] leaf-ordered x, y, z, r */
+  /* contact rows (ping-pong) */
+  int cap = 32, hcap = 32, xcap = 16;
+  int cur = 0;
+  dvec<uint32_t> rowkey[2];    /* [cap][E] */
+  dvec<double> rowst[2];       /* [3][cap][E] */
+  dvec<int> rowcnt[2];         /* [E] */
+  dvec<uint32_t> hits;         /* [hcap][E] own hits */
+  dvec<int> hitcnt;            /* [E] */
+  dvec<int> xbuf;              /* [xcap][E] reverse hits */
+  dvec<int> xcnt;              /* [E] */
+  dvec<int> dflags;            /* [FLAG_COUNT] */
+
+  double curtime = 0.0, curstep = 0.0;
+  int stepnum = 0;
+  bool uploaded = false;
+  pmx_stats stats {};
+};
+
+/* ---- tree (pmx_tree.hip) ---- */
+int tree_build (pmx_ctx &c);                 /* from current centres */
+int tree_trilists (pmx_ctx &c);              /* leaf -> triangle lists */
+
+/* ---- scans (pmx_tree.hip) ---- */
+int scan_exclusive (pmx_ctx &c, int *data, int n, int *total_host);
+
+/* ---- step kernels (pmx_step.hip) ---- */
+int step_bin (pmx_ctx &c, int *overflow);
+int step_detect (pmx_ctx &c);
+int step_rows (pmx_ctx &c, const double *g_dev, const double g[3], double h);
+int step_dynamics (pmx_ctx &c, const double *d_dev, const double d[6], double step0, double step1);
+int step_shapes (pmx_ctx &c);
+int step_euler (pmx_ctx &c, double h);
+int step_invert (pmx_ctx &c);
+
+#endif

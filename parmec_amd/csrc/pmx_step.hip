@@ -1,0 +1,642 @@
+/*
+ * pmx_step.hip -- per-step kernels of the FP64 DEM step (the body of
+ * dem()'s loop, parmec.cpp:2418-2517, for spheres and static triangles).
+ *
+ * Contact bookkeeping.  The reference keeps per-particle master lists that
+ * persist between steps and slave copies rebuilt every step (condet.h,
+ * forces.ispc:195-407).  Here every sphere owns a row: its contacts sorted by
+ * the partner's identity (partner ellipsoid, or 0x80000000|triangle), stored
+ * slot-major (slot k of sphere e at k*E+e) in ping-pong buffers.  Both
+ * spheres of a pair hold the entry; the tangential state lives in the master's
+ * row (master = sphere of the smaller particle index, as condet.ispc:187).
+ * Each step:
+ *   bin     : centre descends the frozen tree to its leaf (partition.ispc:600-646)
+ *   scatter : leaf-ordered copy of centres/radii
+ *   detect  : sphere AABB reaches leaves (condet.ispc:98-113); hits with positive
+ *             depth are new-pair candidates.  A hit whose partner's AABB does not
+ *             reach this sphere's leaf is also posted to the partner (the
+ *             reference finds such a pair from one side only).  Triangles come
+ *             from per-leaf lists of triangles whose AABB reaches the leaf
+ *             (condet.ispc:412-426).
+ *   rows    : merge old row with hits; recompute geometry (update_existing,
+ *             condet.ispc:627-700); drop pairs with depth < 0 (forces.ispc:329-364);
+ *             granular law per pair (forces.ispc:125-192, evaluated identically
+ *             by both spheres of a pair from the master's side); per-sphere sum
+ *             in row order plus m*g (forces.ispc:453-596).
+ *   dynamics, shapes (dynamics.ispc:30-169, shapes.ispc:29-98).
+ */
+#include "pmx_internal.h"
+#include "pmx_device.h"
+#include <algorithm>
+#include <cfloat>
+
+using namespace pmxd;
+
+namespace {
+
+inline dim3 grid (int n) { return dim3 ((n + PMX_TPB - 1) / PMX_TPB); }
+
+__global__ void k_fill_i (int *a, int n, int v)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+/* partitioning_store: descend to the leaf, take a slot (partition.ispc:600-646) */
+__global__ void k_bin (int E, const double *center, int root, const int *left, const int *right, const int *dim,
+                       const double *coord, int *leafof, int *slot, int *lcount, int *flags)
+{
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  double c[3] = {center[e], center[(size_t) E + e], center[2*(size_t) E + e]};
+  int ref = root;
+  while (ref >= 0)
+  {
+    int d = dim[ref];
+    ref = c[d] < coord[ref] ? left[ref] : right[ref];
+  }
+  int L = -ref-1;
+  int s = atomicAdd (&lcount[L], 1);
+  leafof[e] = L;
+  slot[e] = s;
+  if (s >= PMX_LSIZE) atomicAdd (&flags[FLAG_LEAF_OVERFLOW], 1);
+}
+
+__global__ void k_scatter (int E, const double *center, const double *radii, const int *leafof, const int *slot,
+                           const int *loff, int *lmem, double *lpos)
+{
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  int pos = loff[leafof[e]] + slot[e];
+  lmem[pos] = e;
+  lpos[pos] = center[e];
+  lpos[(size_t) E + pos] = center[(size_t) E + e];
+  lpos[2*(size_t) E + pos] = center[2*(size_t) E + e];
+  lpos[3*(size_t) E + pos] = radii[e];
+}
+
+/* detection (test_ellipsoids + test_triangles), one thread per sphere in
+ * leaf order */
+__global__ __launch_bounds__(PMX_TPB)
+void k_detect (int E, const double *center, const double *radii, const int *part, int root, const int *left,
+               const int *right, const int *dim, const double *coord, const double *cell, int nleaves,
+               const int *leafof, const int *lcount, const int *loff, const int *lmem, const double *lpos,
+               const double *tri, int T, const int *tricnt, const int *trioff, const int *trilist,
+               uint32_t *hits, int *hitcnt, int hcap, int *xbuf, int *xcnt, int xcap, int *flags)
+{
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= E) return;
+  int e = lmem[k];
+  double p[3] = {center[e], center[(size_t) E + e], center[2*(size_t) E + e]};
+  double r0 = radii[e], r1 = radii[(size_t) E + e], r2 = radii[2*(size_t) E + e];
+  double rx = r0 > r1 ? r0 : r1;
+  rx = rx > r2 ? rx : r2;
+  double lo[3] = {p[0]-rx, p[1]-rx, p[2]-rx};
+  double hi[3] = {p[0]+rx, p[1]+rx, p[2]+rx};
+  int pe = part[e];
+  int me = leafof[e];
+  double cl[3] = {cell[me], cell[(size_t) nleaves + me], cell[2*(size_t) nleaves + me]};
+  double cu[3] = {cell[3*(size_t) nleaves + me], cell[4*(size_t) nleaves + me], cell[5*(size_t) nleaves + me]};
+  int hc = 0;
+
+  int stack[64];
+  int sp = 0, ref = root;
+  for (;;)
+  {
+    if (ref >= 0)
+    {
+      int d = dim[ref];
+      double c = coord[ref];
+      bool gl = lo[d] <= c, gr = hi[d] >= c;
+      if (gl && gr)
+      {
+        if (sp < 64) stack[sp++] = right[ref]; else atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1);
+        ref = left[ref];
+        continue;
+      }
+      if (gl) { ref = left[ref]; continue; }
+      if (gr) { ref = right[ref]; continue; }
+    }
+    else
+    {
+      int L = -ref-1;
+      int a = loff[L], b = a + lcount[L];
+      for (int m = a; m < b; m++)
+      {
+        int j = lmem[m];
+        double cx = lpos[m], cy = lpos[(size_t) E + m], cz = lpos[2*(size_t) E + m], rj = lpos[3*(size_t) E + m];
+        double qx = p[0]-cx, qy = p[1]-cy, qz = p[2]-cz;
+        double len = sqrt (qx*qx + qy*qy + qz*qz);
+        double depth = rx + rj - len;
+        if (depth > 0.0 && pe != part[j])
+        {
+          if (hc < hcap) hits[(size_t) hc * E + e] = (uint32_t) j;
+          else atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1);
+          hc++;
+          /* does the partner's AABB reach this sphere's leaf? */
+          double ljx = cx-rj, ljy = cy-rj, ljz = cz-rj, hjx = cx+rj, hjy = cy+rj, hjz = cz+rj;
+          bool back = ljx <= cu[0] && ljy <= cu[1] && ljz <= cu[2] && hjx >= cl[0] && hjy >= cl[1] && hjz >= cl[2];
+          if (!back)
+          {
+            int s = atomicAdd (&xcnt[j], 1);
+            if (s < xcap) xbuf[(size_t) s * E + j] = e;
+            else atomicAdd (&flags[FLAG_EXTRA_OVERFLOW], 1);
+          }
+        }
+      }
+    }
+    if (sp == 0) break;
+    ref = stack[--sp];
+  }
+
+  if (T > 0)
+  {
+    int a = trioff[me], b = a + tricnt[me];
+    for (int m = a; m < b; m++)
+    {
+      int t = trilist[m];
+      double pt[3], nr[3];
+      double depth = triangle_sphere (tri[t], tri[(size_t) T + t], tri[2*(size_t) T + t],
+                                      tri[3*(size_t) T + t], tri[4*(size_t) T + t], tri[5*(size_t) T + t],
+                                      tri[6*(size_t) T + t], tri[7*(size_t) T + t], tri[8*(size_t) T + t],
+                                      p[0], p[1], p[2], r0, pt, nr);
+      if (depth > 0.0)
+      {
+        if (hc < hcap) hits[(size_t) hc * E + e] = PMX_TRI_FLAG | (uint32_t) t;
+        else atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1);
+        hc++;
+      }
+    }
+  }
+  hitcnt[e] = hc < hcap ? hc : hcap;
+}
+
+__global__ void k_force_init (int N, const double *mass, const double *gtab, int k, double *force, double *torque)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double ma = mass[i];
+  const double *g = gtab + 3*(size_t) k;
+  force[i] = 0.0 + ma * g[0];
+  force[(size_t) N + i] = 0.0 + ma * g[1];
+  force[2*(size_t) N + i] = 0.0 + ma * g[2];
+  torque[i] = 0.0;
+  torque[(size_t) N + i] = 0.0;
+  torque[2*(size_t) N + i] = 0.0;
+}
+
+#define ROWS_TPB 128
+
+/* row merge + contact forces + per-sphere accumulation */
+__global__ __launch_bounds__(ROWS_TPB)
+void k_rows (int E, int N, const double *center, const double *radii, const int *part, const int *ellcol,
+             const double *tri, int T, const int *tricol, const int *triobs,
+             const double *position, const double *linear, const double *angular, const double *mass,
+             const double *invm, int pairnum, const int *pairs, const int *ikind, const double *iparam,
+             const uint32_t *okey, const double *ost, const int *ocnt,
+             uint32_t *nkey, double *nst, int *ncnt, int cap,
+             const uint32_t *hits, const int *hitcnt, int hcap, const int *xbuf, const int *xcnt, int xcap,
+             const double *gtab, int kstep, double h, double *force, double *torque, int *flags)
+{
+  extern __shared__ uint32_t sh[];
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  int hx = hcap + xcap;
+  uint32_t *H = sh + (size_t) threadIdx.x * hx;
+  int nh = hitcnt[e];
+  for (int q = 0; q < nh; q++) H[q] = hits[(size_t) q * E + e];
+  int nx = xcnt[e];
+  nx = nx < xcap ? nx : xcap;
+  for (int q = 0; q < nx; q++) H[nh + q] = (uint32_t) xbuf[(size_t) q * E + e];
+  nh += nx;
+  for (int a = 1; a < nh; a++)   /* insertion sort of the hit keys */
+  {
+    uint32_t v = H[a];
+    int b = a - 1;
+    while (b >= 0 && H[b] > v) { H[b+1] = H[b]; b--; }
+    H[b+1] = v;
+  }
+
+  int pe = part[e];
+  double xe[3] = {position[pe], position[(size_t) N + pe], position[2*(size_t) N + pe]};
+  double pc[3] = {center[e], center[(size_t) E + e], center[2*(size_t) E + e]};
+  double re = radii[e];
+  int oc = ocnt[e];
+  int io = 0, ih = 0, cnt = 0;
+  double fs[3] = {0.0, 0.0, 0.0}, ts[3] = {0.0, 0.0, 0.0};
+
+  while (io < oc || ih < nh)
+  {
+    uint32_t ko = io < oc ? okey[(size_t) io * E + e] : 0xFFFFFFFFu;
+    uint32_t kh = ih < nh ? H[ih] : 0xFFFFFFFFu;
+    uint32_t key;
+    bool old;
+    int kold = -1;
+    if (io < oc && (ih >= nh || ko <= kh))
+    {
+      key = ko; old = true; kold = io; io++;
+      if (ih < nh && kh == ko) ih++;
+    }
+    else { key = kh; old = false; ih++; }
+
+    double point[3], normal[3], depth, st[3] = {0.0, 0.0, 0.0};
+    bool master;
+    int i, j, col0, col1;
+    if (key & PMX_TRI_FLAG)
+    {
+      int t = (int) (key & ~PMX_TRI_FLAG);
+      depth = triangle_sphere (tri[t], tri[(size_t) T + t], tri[2*(size_t) T + t],
+                               tri[3*(size_t) T + t], tri[4*(size_t) T + t], tri[5*(size_t) T + t],
+                               tri[6*(size_t) T + t], tri[7*(size_t) T + t], tri[8*(size_t) T + t],
+                               pc[0], pc[1], pc[2], re, point, normal);
+      if (old && depth < 0.0) continue;
+      master = true;
+      if (old)
+      {
+        st[0] = ost[(size_t) (0*cap + kold) * E + e];
+        st[1] = ost[(size_t) (1*cap + kold) * E + e];
+        st[2] = ost[(size_t) (2*cap + kold) * E + e];
+      }
+      i = pe;
+      j = triobs[t];
+      col0 = ellcol[e];
+      col1 = tricol[t];
+    }
+    else
+    {
+      int e2 = (int) key;
+      int p2 = part[e2];
+      master = pe < p2;
+      int em = master ? e : e2, es = master ? e2 : e;
+      double pm[3] = {center[em], center[(size_t) E + em], center[2*(size_t) E + em]};
+      double ps[3] = {center[es], center[(size_t) E + es], center[2*(size_t) E + es]};
+      double qx = pm[0]-ps[0], qy = pm[1]-ps[1], qz = pm[2]-ps[2];
+      double len = sqrt (qx*qx + qy*qy + qz*qz);
+      double ilen = old ? 1.0/len : (len > 0.0 ? 1.0/len : 1.0);
+      point[0] = 0.5*(pm[0]+ps[0]);
+      point[1] = 0.5*(pm[1]+ps[1]);
+      point[2] = 0.5*(pm[2]+ps[2]);
+      normal[0] = ilen*qx;
+      normal[1] = ilen*qy;
+      normal[2] = ilen*qz;
+      depth = radii[em] + radii[es] - len;
+      if (old && depth < 0.0) continue;
+      if (old)
+      {
+        if (master)
+        {
+          st[0] = ost[(size_t) (0*cap + kold) * E + e];
+          st[1] = ost[(size_t) (1*cap + kold) * E + e];
+          st[2] = ost[(size_t) (2*cap + kold) * E + e];
+        }
+        else
+        {
+          int lo = 0, hi = ocnt[e2] - 1, at = -1;
+          while (lo <= hi)
+          {
+            int mid = (lo + hi) >> 1;
+            uint32_t km = okey[(size_t) mid * E + e2];
+            if (km == (uint32_t) e) { at = mid; break; }
+            if (km < (uint32_t) e) lo = mid + 1; else hi = mid - 1;
+          }
+          if (at < 0) atomicAdd (&flags[FLAG_BAD_KIND], 1);
+          else
+          {
+            st[0] = ost[(size_t) (0*cap + at) * E + e2];
+            st[1] = ost[(size_t) (1*cap + at) * E + e2];
+            st[2] = ost[(size_t) (2*cap + at) * E + e2];
+          }
+        }
+      }
+      i = part[em];
+      j = part[es];
+      col0 = ellcol[em];
+      col1 = ellcol[es];
+    }
+
+    /* contacts_task velocities, forces.ispc:219-287 */
+    double xi[3] = {position[i], position[(size_t) N + i], position[2*(size_t) N + i]};
+    double oi[3] = {angular[3*(size_t) N + i], angular[4*(size_t) N + i], angular[5*(size_t) N + i]};
+    double vi[3], vj[3], z[3];
+    z[0] = point[0]-xi[0];
+    z[1] = point[1]-xi[1];
+    z[2] = point[2]-xi[2];
+    vi[0] = oi[1]*z[2]-oi[2]*z[1] + linear[i];
+    vi[1] = oi[2]*z[0]-oi[0]*z[2] + linear[(size_t) N + i];
+    vi[2] = oi[0]*z[1]-oi[1]*z[0] + linear[2*(size_t) N + i];
+    if (j >= 0)
+    {
+      z[0] = point[0]-position[j];
+      z[1] = point[1]-position[(size_t) N + j];
+      z[2] = point[2]-position[2*(size_t) N + j];
+      double oj[3] = {angular[3*(size_t) N + j], angular[4*(size_t) N + j], angular[5*(size_t) N + j]};
+      vj[0] = oj[1]*z[2]-oj[2]*z[1] + linear[j];
+      vj[1] = oj[2]*z[0]-oj[0]*z[2] + linear[(size_t) N + j];
+      vj[2] = oj[0]*z[1]-oj[1]*z[0] + linear[2*(size_t) N + j];
+    }
+    else { vj[0] = vj[1] = vj[2] = 0.0; }
+    double vij[3] = {vj[0]-vi[0], vj[1]-vi[1], vj[2]-vi[2]};
+    int ij = pairing (pairnum, pairs, col0, col1);
+    if (ikind[ij] != PMX_GRANULAR_FORCE) atomicAdd (&flags[FLAG_BAD_KIND], 1);
+    double ma = j >= 0 ? 1.0 / (invm[i] + invm[j]) : mass[i];
+    double f[3];
+    granular (normal, vij, depth, ma, iparam[PMX_SPRING*pairnum + ij], iparam[PMX_DAMPER*pairnum + ij],
+              iparam[PMX_KSKN*pairnum + ij], iparam[PMX_FRISTAT*pairnum + ij], iparam[PMX_FRIDYN*pairnum + ij],
+              st, h, f);
+
+    if (cnt < cap)
+    {
+      nkey[(size_t) cnt * E + e] = key;
+      if (master)
+      {
+        nst[(size_t) (0*cap + cnt) * E + e] = st[0];
+        nst[(size_t) (1*cap + cnt) * E + e] = st[1];
+        nst[(size_t) (2*cap + cnt) * E + e] = st[2];
+      }
+    }
+    else atomicAdd (&flags[FLAG_ROW_OVERFLOW], 1);
+    cnt++;
+
+    /* contacts_acc_task, forces.ispc:492-533: slaves receive -f */
+    if (!master) { f[0] = -f[0]; f[1] = -f[1]; f[2] = -f[2]; }
+    double a[3] = {point[0]-xe[0], point[1]-xe[1], point[2]-xe[2]};
+    fs[0] += f[0]; fs[1] += f[1]; fs[2] += f[2];
+    ts[0] += a[1]*f[2] - a[2]*f[1];
+    ts[1] += a[2]*f[0] - a[0]*f[2];
+    ts[2] += a[0]*f[1] - a[1]*f[0];
+  }
+  ncnt[e] = cnt < cap ? cnt : cap;
+  atomicMax (&flags[FLAG_MAXROW], cnt);
+  const double *g = gtab + 3*(size_t) kstep;
+  double ma = mass[pe];
+  force[pe] = fs[0] + ma * g[0];
+  force[(size_t) N + pe] = fs[1] + ma * g[1];
+  force[2*(size_t) N + pe] = fs[2] + ma * g[2];
+  torque[pe] = ts[0];
+  torque[(size_t) N + pe] = ts[1];
+  torque[2*(size_t) N + pe] = ts[2];
+}
+
+/* dynamics_task, dynamics.ispc:30-169 */
+__global__ void k_dynamics (int N, double *angular, double *linear, double *rotation, double *position,
+                            const double *inertia, const double *inverse, const double *mass, const double *invm,
+                            const double *dtab, int kstep, double *force, double *torque, double step0, double step1)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double *damping = dtab + 6*(size_t) kstep;
+  double half = 0.5*step0;
+  double step = 0.5*(step0+step1);
+  double O[3], o[3], v[3], L1[9], J[9], I[9], im, f[3], t[3], T[3], DL[9], L2[9], A[3], B[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) { O[k] = angular[(size_t) k*N + i]; v[k] = linear[(size_t) k*N + i]; }
+#pragma unroll
+  for (int k = 0; k < 9; k++)
+  {
+    L1[k] = rotation[(size_t) k*N + i];
+    J[k] = inertia[(size_t) k*N + i];
+    I[k] = inverse[(size_t) k*N + i];
+  }
+  im = invm[i];
+  f[0] = force[i]; f[1] = force[(size_t) N + i]; f[2] = force[2*(size_t) N + i];
+  t[0] = torque[i]; t[1] = torque[(size_t) N + i]; t[2] = torque[2*(size_t) N + i];
+  if (damping[0] != 0.0 || damping[1] != 0.0 || damping[2] != 0.0)
+  {
+    double ma = mass[i];
+    f[0] -= ma * damping[0] * v[0];
+    f[1] -= ma * damping[1] * v[1];
+    f[2] -= ma * damping[2] * v[2];
+    force[i] = f[0]; force[(size_t) N + i] = f[1]; force[2*(size_t) N + i] = f[2];
+  }
+  if (damping[3] != 0.0 || damping[4] != 0.0 || damping[5] != 0.0)
+  {
+    double w[3], tt[3], TT[3];
+    w[0] = damping[3]*angular[3*(size_t) N + i];
+    w[1] = damping[4]*angular[4*(size_t) N + i];
+    w[2] = damping[5]*angular[5*(size_t) N + i];
+    tvmul (L1, w, tt);
+    nvmul (J, tt, TT);
+    nvmul (L1, TT, tt);
+    t[0] -= tt[0]; t[1] -= tt[1]; t[2] -= tt[2];
+    torque[i] = t[0]; torque[(size_t) N + i] = t[1]; torque[2*(size_t) N + i] = t[2];
+  }
+  tvmul (L1, t, T);
+  expmap (-half*O[0], -half*O[1], -half*O[2], DL);
+  nvmul (J, O, A);
+  nvmul (DL, A, B);
+  B[0] = B[0] + half*T[0];
+  B[1] = B[1] + half*T[1];
+  B[2] = B[2] + half*T[2];
+  nvmul (I, B, A);
+  nvmul (J, A, B);
+  T[0] -= A[1]*B[2] - A[2]*B[1];
+  T[1] -= A[2]*B[0] - A[0]*B[2];
+  T[2] -= A[0]*B[1] - A[1]*B[0];
+  T[0] *= step0; T[1] *= step0; T[2] *= step0;
+  O[0] = O[0] + (I[0]*T[0] + I[3]*T[1] + I[6]*T[2]);
+  O[1] = O[1] + (I[1]*T[0] + I[4]*T[1] + I[7]*T[2]);
+  O[2] = O[2] + (I[2]*T[0] + I[5]*T[1] + I[8]*T[2]);
+  im *= step0;
+  v[0] = v[0] + im*f[0];
+  v[1] = v[1] + im*f[1];
+  v[2] = v[2] + im*f[2];
+  expmap (step*O[0], step*O[1], step*O[2], DL);
+  nnmul (L1, DL, L2);
+#pragma unroll
+  for (int k = 0; k < 9; k++) rotation[(size_t) k*N + i] = L2[k];
+  nvmul (L2, O, o);
+  position[i] += step*v[0];
+  position[(size_t) N + i] += step*v[1];
+  position[2*(size_t) N + i] += step*v[2];
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+  {
+    angular[(size_t) k*N + i] = O[k];
+    angular[(size_t) (3+k)*N + i] = o[k];
+    linear[(size_t) k*N + i] = v[k];
+  }
+}
+
+/* ellipsoids_task centre update, shapes.ispc:29-67 */
+__global__ void k_shapes (int E, int N, const int *part, double *center, const double *rotation, const double *position)
+{
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  int j = part[e];
+  double L[9], C[3], c[3];
+#pragma unroll
+  for (int k = 0; k < 9; k++) L[k] = rotation[(size_t) k*N + j];
+#pragma unroll
+  for (int k = 0; k < 3; k++) C[k] = center[(size_t) (3+k)*E + e] - position[(size_t) (3+k)*N + j];
+  c[0] = position[j] + (L[0]*C[0] + L[3]*C[1] + L[6]*C[2]);
+  c[1] = position[(size_t) N + j] + (L[1]*C[0] + L[4]*C[1] + L[7]*C[2]);
+  c[2] = position[2*(size_t) N + j] + (L[2]*C[0] + L[5]*C[1] + L[8]*C[2]);
+  center[e] = c[0];
+  center[(size_t) E + e] = c[1];
+  center[2*(size_t) E + e] = c[2];
+}
+
+/* euler_task, parmec.ispc:224-266 */
+__global__ void k_euler (int N, double *angular, const double *linear, double *rotation, double *position, double step)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double O[3], L1[9], DL[9], L2[9], o[3];
+  O[0] = step*angular[i];
+  O[1] = step*angular[(size_t) N + i];
+  O[2] = step*angular[2*(size_t) N + i];
+  expmap (O[0], O[1], O[2], DL);
+#pragma unroll
+  for (int k = 0; k < 9; k++) L1[k] = rotation[(size_t) k*N + i];
+  nnmul (L1, DL, L2);
+#pragma unroll
+  for (int k = 0; k < 9; k++) rotation[(size_t) k*N + i] = L2[k];
+  nvmul (L2, O, o);
+  angular[3*(size_t) N + i] = o[0];
+  angular[4*(size_t) N + i] = o[1];
+  angular[5*(size_t) N + i] = o[2];
+  position[i] += step * linear[i];
+  position[(size_t) N + i] += step * linear[(size_t) N + i];
+  position[2*(size_t) N + i] += step * linear[2*(size_t) N + i];
+}
+
+/* invert, parmec.ispc:62-87 (INVERT, macros.h:1033-1051) */
+__global__ void k_invert (int N, const double *inertia, double *inverse, const double *mass, double *invm)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double F[9], det;
+#pragma unroll
+  for (int k = 0; k < 9; k++) F[k] = inertia[(size_t) k*N + i];
+  det = F[0]*(F[4]*F[8]-F[5]*F[7]) + F[3]*(F[2]*F[7]-F[1]*F[8]) + (F[1]*F[5]-F[2]*F[4])*F[6];
+  if (det != 0.0)
+  {
+    det = 1.0 / det;
+    inverse[i] = (F[4]*F[8]-F[5]*F[7])*det;
+    inverse[(size_t) 1*N + i] = (F[2]*F[7]-F[1]*F[8])*det;
+    inverse[(size_t) 2*N + i] = (F[1]*F[5]-F[2]*F[4])*det;
+    inverse[(size_t) 3*N + i] = (F[5]*F[6]-F[3]*F[8])*det;
+    inverse[(size_t) 4*N + i] = (F[0]*F[8]-F[2]*F[6])*det;
+    inverse[(size_t) 5*N + i] = (F[2]*F[3]-F[0]*F[5])*det;
+    inverse[(size_t) 6*N + i] = (F[3]*F[7]-F[4]*F[6])*det;
+    inverse[(size_t) 7*N + i] = (F[1]*F[6]-F[0]*F[7])*det;
+    inverse[(size_t) 8*N + i] = (F[0]*F[4]-F[1]*F[3])*det;
+  }
+  invm[i] = 1.0 / mass[i];
+}
+
+} /* namespace */
+
+int step_bin (pmx_ctx &c, int *overflow)
+{
+  TreeDev &t = c.tree;
+  int L = std::max (t.nleaves, 1);
+  if (c.lcount.alloc (L) || c.loff.alloc (L)) { pmx_set_error ("out of device memory (bins)"); return PMX_ENOMEM; }
+  hipLaunchKernelGGL (k_fill_i, grid (L), dim3 (PMX_TPB), 0, c.stream, c.lcount.p, L, 0);
+  hipLaunchKernelGGL (k_fill_i, dim3 (1), dim3 (64), 0, c.stream, c.dflags.p + FLAG_LEAF_OVERFLOW, 1, 0);
+  hipLaunchKernelGGL (k_bin, grid (c.E), dim3 (PMX_TPB), 0, c.stream, c.E, (const double*) c.center.p, t.root,
+                      (const int*) t.left.p, (const int*) t.right.p, (const int*) t.dim.p, (const double*) t.coord.p,
+                      c.leafof.p, c.slot.p, c.lcount.p, c.dflags.p);
+  PMX_HIP (hipGetLastError ());
+  int fl[FLAG_COUNT];
+  PMX_HIP (hipMemcpyAsync (fl, c.dflags.p, sizeof (fl), hipMemcpyDeviceToHost, c.stream));
+  PMX_HIP (hipStreamSynchronize (c.stream));
+  *overflow = fl[FLAG_LEAF_OVERFLOW];
+  if (fl[FLAG_ROW_OVERFLOW] || fl[FLAG_HIT_OVERFLOW] || fl[FLAG_EXTRA_OVERFLOW] || fl[FLAG_TRILIST_OVERFLOW])
+  {
+    pmx_set_error ("contact capacity exceeded (row " + std::to_string (fl[FLAG_ROW_OVERFLOW]) + ", hits " +
+                   std::to_string (fl[FLAG_HIT_OVERFLOW]) + ", reverse " + std::to_string (fl[FLAG_EXTRA_OVERFLOW]) +
+                   "); raise pmx_options.row_capacity");
+    return PMX_ECAPACITY;
+  }
+  if (fl[FLAG_BAD_KIND])
+  {
+    pmx_set_error ("unsupported interaction kind or inconsistent contact rows");
+    return PMX_EUNSUPPORTED;
+  }
+  c.stats.max_row = std::max (c.stats.max_row, fl[FLAG_MAXROW]);
+  return PMX_OK;
+}
+
+int step_detect (pmx_ctx &c)
+{
+  TreeDev &t = c.tree;
+  int L = std::max (t.nleaves, 1);
+  PMX_HIP (hipMemcpyAsync (c.loff.p, c.lcount.p, sizeof (int) * L, hipMemcpyDeviceToDevice, c.stream));
+  int rc = scan_exclusive (c, c.loff.p, L, nullptr);
+  if (rc) return rc;
+  hipLaunchKernelGGL (k_scatter, grid (c.E), dim3 (PMX_TPB), 0, c.stream, c.E, (const double*) c.center.p,
+                      (const double*) c.radii.p, (const int*) c.leafof.p, (const int*) c.slot.p, (const int*) c.loff.p,
+                      c.lmem.p, c.lpos.p);
+  hipLaunchKernelGGL (k_fill_i, grid (c.E), dim3 (PMX_TPB), 0, c.stream, c.xcnt.p, c.E, 0);
+  hipLaunchKernelGGL (k_detect, grid (c.E), dim3 (PMX_TPB), 0, c.stream, c.E, (const double*) c.center.p,
+                      (const double*) c.radii.p, (const int*) c.part.p, t.root, (const int*) t.left.p,
+                      (const int*) t.right.p, (const int*) t.dim.p, (const double*) t.coord.p, (const double*) t.cell.p,
+                      t.nleaves, (const int*) c.leafof.p, (const int*) c.lcount.p, (const int*) c.loff.p,
+                      (const int*) c.lmem.p, (const double*) c.lpos.p, (const double*) c.tri.p, c.T,
+                      (const int*) t.tricnt.p, (const int*) t.trioff.p, (const int*) t.trilist.p,
+                      c.hits.p, c.hitcnt.p, c.hcap, c.xbuf.p, c.xcnt.p, c.xcap, c.dflags.p);
+  PMX_HIP (hipGetLastError ());
+  return PMX_OK;
+}
+
+int step_rows (pmx_ctx &c, const double *g_dev, const double *g, double h)
+{
+  (void) g;
+  int kstep = 0;
+  int o = c.cur, n = 1 - c.cur;
+  hipLaunchKernelGGL (k_force_init, grid (c.N), dim3 (PMX_TPB), 0, c.stream, c.N, (const double*) c.mass.p, g_dev,
+                      kstep, c.force.p, c.torque.p);
+  if (c.E > 0)
+  {
+    size_t shm = (size_t) ROWS_TPB * (c.hcap + c.xcap) * sizeof (uint32_t);
+    hipLaunchKernelGGL (k_rows, dim3 ((c.E + ROWS_TPB - 1) / ROWS_TPB), dim3 (ROWS_TPB), shm, c.stream, c.E, c.N,
+                        (const double*) c.center.p, (const double*) c.radii.p, (const int*) c.part.p,
+                        (const int*) c.ellcol.p, (const double*) c.tri.p, c.T, (const int*) c.tricol.p,
+                        (const int*) c.triobs.p, (const double*) c.position.p, (const double*) c.linear.p,
+                        (const double*) c.angular.p, (const double*) c.mass.p, (const double*) c.invm.p, c.P,
+                        (const int*) c.pairs.p, (const int*) c.ikind.p, (const double*) c.iparam.p,
+                        (const uint32_t*) c.rowkey[o].p, (const double*) c.rowst[o].p, (const int*) c.rowcnt[o].p,
+                        c.rowkey[n].p, c.rowst[n].p, c.rowcnt[n].p, c.cap, (const uint32_t*) c.hits.p,
+                        (const int*) c.hitcnt.p, c.hcap, (const int*) c.xbuf.p, (const int*) c.xcnt.p, c.xcap,
+                        g_dev, kstep, h, c.force.p, c.torque.p, c.dflags.p);
+  }
+  PMX_HIP (hipGetLastError ());
+  c.cur = n;
+  return PMX_OK;
+}
+
+int step_dynamics (pmx_ctx &c, const double *d_dev, const double *d, double step0, double step1)
+{
+  (void) d;
+  hipLaunchKernelGGL (k_dynamics, grid (c.N), dim3 (PMX_TPB), 0, c.stream, c.N, c.angular.p, c.linear.p, c.rotation.p,
+                      c.position.p, (const double*) c.inertia.p, (const double*) c.inverse.p, (const double*) c.mass.p,
+                      (const double*) c.invm.p, d_dev, 0, c.force.p, c.torque.p, step0, step1);
+  PMX_HIP (hipGetLastError ());
+  return PMX_OK;
+}
+
+int step_shapes (pmx_ctx &c)
+{
+  if (c.E == 0) return PMX_OK;
+  hipLaunchKernelGGL (k_shapes, grid (c.E), dim3 (PMX_TPB), 0, c.stream, c.E, c.N, (const int*) c.part.p, c.center.p,
+                      (const double*) c.rotation.p, (const double*) c.position.p);
+  PMX_HIP (hipGetLastError ());
+  return PMX_OK;
+}
+
+int step_euler (pmx_ctx &c, double h)
+{
+  hipLaunchKernelGGL (k_euler, grid (c.N), dim3 (PMX_TPB), 0, c.stream, c.N, c.angular.p, (const double*) c.linear.p,
+                      c.rotation.p, c.position.p, h);
+  PMX_HIP (hipGetLastError ());
+  return PMX_OK;
+}
+
+int step_invert (pmx_ctx &c)
+{
+  hipLaunchKernelGGL (k_invert, grid (c.N), dim3 (PMX_TPB), 0, c.stream, c.N, (const double*) c.inertia.p,
+                      c.inverse.p, (const double*) c.mass.p, c.invm.p);
+  PMX_HIP (hipGetLastError ());
+  return PMX_OK;
+}

@@ -1,0 +1,600 @@
+/*
+ * pmx_tree.hip -- device build of Parmec's frozen partitioning tree
+ * (partitioning_create, partition.ispc:663-746).
+ *
+ *   extents -> 30-bit Morton codes (partition.ispc:58-94)
+ *           -> stable LSD radix sort, 6-bit digits, per-wave ballot ranking
+ *              staged in LDS (replaces the task quick sort + tournament merge,
+ *              partition.ispc:97-243)
+ *           -> Karras internal nodes, CUTOFF terminals, split dimension
+ *              (dnode-2)%3 and split coordinate = min coordinate of the right
+ *              child's points (partition.ispc:268-394)
+ *           -> compact leaf numbering of the children of branch nodes
+ *              (partitioning_tree_create, partition.ispc:525-569)
+ *           -> per-leaf reach cell: an AABB [lo,hi] reaches a leaf through the
+ *              descent rule "lo[d] <= coord -> left, hi[d] >= coord -> right"
+ *              (condet.ispc:104-113) iff lo <= U and hi >= L component-wise,
+ *              U = min coord over ancestors left of the path, L = max over
+ *              ancestors right of it.
+ */
+#include "pmx_internal.h"
+#include <cfloat>
+#include <cmath>
+
+namespace {
+
+__device__ __forceinline__ unsigned expandbits (unsigned v)
+{
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+
+__device__ __forceinline__ int lzc (unsigned x) { return x ? __clz (x) : 32; }
+
+__device__ __forceinline__ int delta (int i, unsigned codei, int j, int n, const uint32_t *code)
+{
+  if (j < 0 || j > n-1) return -1;
+  unsigned codej = code[j];
+  if (codei == codej) return 32 + lzc ((unsigned) (i ^ j));
+  return lzc (codei ^ codej);
+}
+
+/* ---------------------------------------------------------------- scan */
+
+__global__ void k_scan_block (int *data, int n, int *bsums)
+{
+  __shared__ int wsum[PMX_TPB/64];
+  int base = blockIdx.x * (PMX_TPB*4) + threadIdx.x * 4;
+  int v[4], s = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) { v[k] = base+k < n ? data[base+k] : 0; s += v[k]; }
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int x = s;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1)
+  {
+    int y = __shfl_up (x, off, 64);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads ();
+  int wpre = 0;
+  for (int i = 0; i < w; i++) wpre += wsum[i];
+  int run = wpre + x - s;
+#pragma unroll
+  for (int k = 0; k < 4; k++) { if (base+k < n) data[base+k] = run; run += v[k]; }
+  if (bsums && threadIdx.x == PMX_TPB-1) bsums[blockIdx.x] = wpre + x;
+}
+
+__global__ void k_scan_add (int *data, int n, const int *boffs)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) data[i] += boffs[i / (PMX_TPB*4)];
+}
+
+int scan_rec (pmx_ctx &c, int *data, int n, int *scratch, size_t scratch_n)
+{
+  const int per = PMX_TPB*4;
+  int nb = (n + per - 1) / per;
+  if (nb <= 1)
+  {
+    hipLaunchKernelGGL (k_scan_block, dim3 (1), dim3 (PMX_TPB), 0, c.stream, data, n, (int*) nullptr);
+    PMX_HIP (hipGetLastError ());
+    return PMX_OK;
+  }
+  if ((size_t) nb > scratch_n) { pmx_set_error ("scan scratch too small"); return PMX_ENOMEM; }
+  hipLaunchKernelGGL (k_scan_block, dim3 (nb), dim3 (PMX_TPB), 0, c.stream, data, n, scratch);
+  PMX_HIP (hipGetLastError ());
+  int rc = scan_rec (c, scratch, nb, scratch + nb, scratch_n - nb);
+  if (rc) return rc;
+  hipLaunchKernelGGL (k_scan_add, dim3 ((n + PMX_TPB - 1)/PMX_TPB), dim3 (PMX_TPB), 0, c.stream, data, n, (const int*) scratch);
+  PMX_HIP (hipGetLastError ());
+  return PMX_OK;
+}
+
+/* ------------------------------------------------------------- extents */
+
+__device__ __forceinline__ void block_minmax6 (double e[6], double *sh /* [6][PMX_TPB/64] */)
+{
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1)
+  {
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+    {
+      double a = __shfl_xor (e[k], off, 64);
+      double b = __shfl_xor (e[3+k], off, 64);
+      e[k] = a < e[k] ? a : e[k];
+      e[3+k] = b > e[3+k] ? b : e[3+k];
+    }
+  }
+  if (lane == 0)
+    for (int k = 0; k < 6; k++) sh[k*(PMX_TPB/64) + w] = e[k];
+  __syncthreads ();
+  if (threadIdx.x == 0)
+  {
+    for (int ww = 1; ww < PMX_TPB/64; ww++)
+      for (int k = 0; k < 3; k++)
+      {
+        double a = sh[k*(PMX_TPB/64) + ww], b = sh[(3+k)*(PMX_TPB/64) + ww];
+        e[k] = a < e[k] ? a : e[k];
+        e[3+k] = b > e[3+k] ? b : e[3+k];
+      }
+  }
+}
+
+__global__ void k_extents_partial (int n, const double *cx, const double *cy, const double *cz, double *part)
+{
+  __shared__ double sh[6*(PMX_TPB/64)];
+  double e[6] = {DBL_MAX, DBL_MAX, DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+  {
+    double x = cx[i], y = cy[i], z = cz[i];
+    if (x < e[0]) e[0] = x;
+    if (y < e[1]) e[1] = y;
+    if (z < e[2]) e[2] = z;
+    if (x > e[3]) e[3] = x;
+    if (y > e[4]) e[4] = y;
+    if (z > e[5]) e[5] = z;
+  }
+  block_minmax6 (e, sh);
+  if (threadIdx.x == 0) for (int k = 0; k < 6; k++) part[6*blockIdx.x + k] = e[k];
+}
+
+__global__ void k_extents_final (int nb, double *ext /* ext[0..5] out, partials from ext+6 */)
+{
+  __shared__ double sh[6*(PMX_TPB/64)];
+  double e[6] = {DBL_MAX, DBL_MAX, DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX};
+  const double *part = ext + 6;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
+    for (int k = 0; k < 3; k++)
+    {
+      double a = part[6*b + k], c = part[6*b + 3 + k];
+      e[k] = a < e[k] ? a : e[k];
+      e[3+k] = c > e[3+k] ? c : e[3+k];
+    }
+  block_minmax6 (e, sh);
+  if (threadIdx.x == 0) for (int k = 0; k < 6; k++) ext[k] = e[k];
+}
+
+/* -------------------------------------------------------------- morton */
+
+__global__ void k_morton (int n, const double *cx, const double *cy, const double *cz, const double *ext,
+                          uint32_t *code, int *order)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double p[3] = {cx[i], cy[i], cz[i]};
+  unsigned q[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+  {
+    double w = ext[3+k] - ext[k];
+    double pk = w > 0.0 ? (p[k]-ext[k])/w : 0.0;   /* zero extent -> cell 0 */
+    double s = pk * 1024.0;
+    s = s > 0.0 ? s : 0.0;
+    s = s < 1023.0 ? s : 1023.0;
+    q[k] = expandbits ((unsigned) (int) s);
+  }
+  code[i] = q[0]*4 + q[1]*2 + q[2];
+  order[i] = i;
+}
+
+/* ---------------------------------------------------------- radix sort */
+
+#define RADIX_BITS 6
+#define RADIX 64
+
+__global__ void k_radix_hist (int n, const uint32_t *key, int shift, int *hist, int nb)
+{
+  __shared__ int h[RADIX];
+  if (threadIdx.x < RADIX) h[threadIdx.x] = 0;
+  __syncthreads ();
+  int i = blockIdx.x * PMX_TPB + threadIdx.x;
+  if (i < n) atomicAdd (&h[(key[i] >> shift) & (RADIX-1)], 1);
+  __syncthreads ();
+  if (threadIdx.x < RADIX) hist[threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+}
+
+/* stable scatter: rank inside the block = same-digit elements of lower
+ * thread index, from per-wave match masks (6 ballots) and per-wave digit
+ * counts staged in LDS */
+__global__ void k_radix_scatter (int n, const uint32_t *key, const int *val, int shift, const int *hist, int nb,
+                                 uint32_t *key2, int *val2)
+{
+  __shared__ int wcnt[PMX_TPB/64][RADIX];
+  int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  for (int k = tid; k < (PMX_TPB/64)*RADIX; k += PMX_TPB) (&wcnt[0][0])[k] = 0;
+  __syncthreads ();
+  int i = blockIdx.x * PMX_TPB + tid;
+  bool valid = i < n;
+  uint32_t k = valid ? key[i] : 0u;
+  int digit = (int) ((k >> shift) & (RADIX-1));
+  unsigned long long m = __ballot (valid);
+#pragma unroll
+  for (int b = 0; b < RADIX_BITS; b++)
+  {
+    int bit = (digit >> b) & 1;
+    unsigned long long bb = __ballot (bit);
+    m &= bit ? bb : ~bb;
+  }
+  unsigned long long lt = lane ? ((1ull << lane) - 1ull) : 0ull;
+  int rank = __popcll (m & lt);
+  if (valid && (m >> lane) == 1ull) wcnt[w][digit] = __popcll (m);
+  __syncthreads ();
+  if (!valid) return;
+  int pre = 0;
+  for (int ww = 0; ww < w; ww++) pre += wcnt[ww][digit];
+  int pos = hist[digit * nb + blockIdx.x] + pre + rank;
+  key2[pos] = k;
+  val2[pos] = val[i];
+}
+
+/* ----------------------------------------------------------- karras */
+
+__global__ void k_karras (int n, const uint32_t *code, int *first, int *size, int *split, int *dim)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n-1) return;
+  unsigned codei = code[i];
+  int dd = delta (i, codei, i+1, n, code) - delta (i, codei, i-1, n, code);
+  int d = (dd > 0) - (dd < 0);
+  int dmin = delta (i, codei, i-d, n, code);
+  int lmax = 2;
+  while (delta (i, codei, i+lmax*d, n, code) > dmin) lmax *= 2;
+  int l = 0;
+  for (int t = lmax/2; t >= 1; t /= 2)
+    if (delta (i, codei, i+(l+t)*d, n, code) > dmin) l = l+t;
+  int j = i+l*d;
+  first[i] = d > 0 ? i : j;
+  size[i] = l+1;
+  if (l+1 <= PMX_CUTOFF)
+  {
+    split[i] = 0;
+    dim[i] = -1;
+  }
+  else
+  {
+    int dnode = delta (i, codei, j, n, code);
+    int s = 0;
+    for (int t = (l+1)/2;; t = (t+1)/2)
+    {
+      if (delta (i, codei, i+(s+t)*d, n, code) > dnode) s = s+t;
+      if (t == 1) break;
+    }
+    split[i] = i+s*d+min (d, 0);
+    dim[i] = (dnode-2)%3;
+  }
+}
+
+/* leaf children per branch node; branch flags for the coordinate pass */
+__global__ void k_children (int n, const int *first, const int *size, const int *split, const int *dim,
+                            int *nleafkids, int *isbranch, int *parent)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n-1) return;
+  if (i == 0) parent[0] = -1;
+  if (dim[i] < 0) { nleafkids[i] = 0; isbranch[i] = 0; return; }
+  isbranch[i] = 1;
+  int j = split[i];
+  int k = 0;
+  if (first[i] == j || dim[j] < 0) k++; else parent[j] = 2*i;
+  if (first[i]+size[i]-1 == j+1 || dim[j+1] < 0) k++; else parent[j+1] = 2*i+1;
+  nleafkids[i] = k;
+}
+
+__global__ void k_assign (int n, const int *first, const int *size, const int *split, const int *dim,
+                          const int *leafbase, int *left, int *right, int *leafpar)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n-1 || dim[i] < 0) return;
+  int j = split[i];
+  int L = leafbase[i];
+  if (first[i] == j || dim[j] < 0) { left[i] = -(L+1); leafpar[L] = 2*i; L++; }
+  else left[i] = j;
+  if (first[i]+size[i]-1 == j+1 || dim[j+1] < 0) { right[i] = -(L+1); leafpar[L] = 2*i+1; }
+  else right[i] = j+1;
+}
+
+__global__ void k_branchlist (int n, const int *isbranch, const int *bpos, int *blist)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n-1) return;
+  if (isbranch[i]) blist[bpos[i]] = i;
+}
+
+/* split coordinate = min of the right child's points along the split
+ * dimension (mincoord, partition.ispc:297-308); one wave per branch node */
+__global__ void k_coord (const int *blist, const int *first, const int *size, const int *split, const int *dim,
+                         const int *order, const double *cx, const double *cy, const double *cz, double *coord)
+{
+  int i = blist[blockIdx.x];
+  int d = dim[i];
+  const double *pt = d == 0 ? cx : (d == 1 ? cy : cz);
+  int a = split[i]+1, b = first[i]+size[i];
+  double m = DBL_MAX;
+  for (int k = a + (int) threadIdx.x; k < b; k += blockDim.x)
+  {
+    double v = pt[order[k]];
+    m = v < m ? v : m;
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1)
+  {
+    double o = __shfl_xor (m, off, 64);
+    m = o < m ? o : m;
+  }
+  if (threadIdx.x == 0) coord[i] = m;
+}
+
+__global__ void k_cells (int nleaves, const int *leafpar, const int *parent, const int *dim, const double *coord,
+                         double *cell)
+{
+  int L = blockIdx.x * blockDim.x + threadIdx.x;
+  if (L >= nleaves) return;
+  double lo[3] = {-INFINITY, -INFINITY, -INFINITY}, hi[3] = {INFINITY, INFINITY, INFINITY};
+  int p = leafpar[L];
+  int guard = 0;
+  while (p >= 0 && guard < 4096)
+  {
+    int node = p >> 1, side = p & 1;
+    int d = dim[node];
+    double c = coord[node];
+    if (side == 0) hi[d] = c < hi[d] ? c : hi[d];
+    else lo[d] = c > lo[d] ? c : lo[d];
+    p = parent[node];
+    guard++;
+  }
+  for (int k = 0; k < 3; k++) { cell[k*nleaves + L] = lo[k]; cell[(3+k)*nleaves + L] = hi[k]; }
+}
+
+/* leaves reached by a box, in the reference's descent order */
+__device__ __forceinline__ int reach_leaves_count (int root, const int *left, const int *right, const int *dim,
+                                                    const double *coord, const double lo[3], const double hi[3],
+                                                    int *tricnt, int t, const int *trioff, int *trilist, int *fill,
+                                                    int *flag)
+{
+  int stack[64];
+  int sp = 0, ref = root, visited = 0;
+  for (;;)
+  {
+    if (ref >= 0)
+    {
+      int d = dim[ref];
+      double c = coord[ref];
+      bool gl = lo[d] <= c, gr = hi[d] >= c;
+      if (gl && gr)
+      {
+        if (sp < 64) stack[sp++] = right[ref]; else atomicAdd (flag, 1);
+        ref = left[ref];
+        continue;
+      }
+      if (gl) { ref = left[ref]; continue; }
+      if (gr) { ref = right[ref]; continue; }
+    }
+    else
+    {
+      int L = -ref-1;
+      if (trilist) { int s = atomicAdd (&fill[L], 1); trilist[trioff[L] + s] = t; }
+      else atomicAdd (&tricnt[L], 1);
+      visited++;
+    }
+    if (sp == 0) break;
+    ref = stack[--sp];
+  }
+  return visited;
+}
+
+__global__ void k_tri_reach (int T, const double *tri, int root, const int *left, const int *right, const int *dim,
+                             const double *coord, int *tricnt, const int *trioff, int *trilist, int *fill, int *flag)
+{
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  double v[3][3];
+  for (int a = 0; a < 3; a++) for (int b = 0; b < 3; b++) v[a][b] = tri[(size_t) (3*a+b)*T + t];
+  double lo[3], hi[3];
+  for (int b = 0; b < 3; b++)
+  {
+    double mn = v[0][b] < v[1][b] ? v[0][b] : v[1][b];
+    mn = mn < v[2][b] ? mn : v[2][b];
+    double mx = v[0][b] > v[1][b] ? v[0][b] : v[1][b];
+    mx = mx > v[2][b] ? mx : v[2][b];
+    lo[b] = mn; hi[b] = mx;
+  }
+  reach_leaves_count (root, left, right, dim, coord, lo, hi, tricnt, t, trioff, trilist, fill, flag);
+}
+
+__global__ void k_fill_int (int *a, int n, int v)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+inline dim3 grid (int n) { return dim3 ((n + PMX_TPB - 1) / PMX_TPB); }
+
+} /* namespace */
+
+int scan_exclusive (pmx_ctx &c, int *data, int n, int *total_host)
+{
+  if (n <= 0) { if (total_host) *total_host = 0; return PMX_OK; }
+  size_t need = (size_t) n / (PMX_TPB*4) + 64;
+  if (c.tree.scanbuf.alloc (need) != hipSuccess) { pmx_set_error ("out of device memory (scan)"); return PMX_ENOMEM; }
+  int last = 0;
+  if (total_host)
+    PMX_HIP (hipMemcpyAsync (&last, data + n - 1, sizeof (int), hipMemcpyDeviceToHost, c.stream));
+  int rc = scan_rec (c, data, n, c.tree.scanbuf.p, c.tree.scanbuf.n);
+  if (rc) return rc;
+  if (total_host)
+  {
+    int lastex = 0;
+    PMX_HIP (hipMemcpyAsync (&lastex, data + n - 1, sizeof (int), hipMemcpyDeviceToHost, c.stream));
+    PMX_HIP (hipStreamSynchronize (c.stream));
+    *total_host = lastex + last;
+  }
+  return PMX_OK;
+}
+
+int tree_build (pmx_ctx &c)
+{
+  TreeDev &t = c.tree;
+  int n = c.E;
+  t.n = n;
+  if (n == 0) { t.nleaves = 0; t.root = -1; return PMX_OK; }
+  const double *cx = c.center.p, *cy = c.center.p + n, *cz = c.center.p + 2*(size_t) n;
+  int nbext = std::min (1024, (n + PMX_TPB - 1) / PMX_TPB);
+  int nbsort = (n + PMX_TPB - 1) / PMX_TPB;
+  int ni = std::max (n-1, 1);
+  if (t.ext.alloc (6 + 6*(size_t) nbext) || t.code.alloc (n) || t.code2.alloc (n) || t.order.alloc (n) ||
+      t.order2.alloc (n) || t.hist.alloc ((size_t) RADIX * nbsort + 1) || t.first.alloc (ni) || t.size.alloc (ni) ||
+      t.split.alloc (ni) || t.dim.alloc (ni) || t.parent.alloc (ni) || t.coord.alloc (ni) || t.left.alloc (ni) ||
+      t.right.alloc (ni) || t.nleafkids.alloc (ni + 1))
+  { pmx_set_error ("out of device memory (tree)"); return PMX_ENOMEM; }
+
+  hipLaunchKernelGGL (k_extents_partial, dim3 (nbext), dim3 (PMX_TPB), 0, c.stream, n, cx, cy, cz, t.ext.p + 6);
+  hipLaunchKernelGGL (k_extents_final, dim3 (1), dim3 (PMX_TPB), 0, c.stream, nbext, t.ext.p);
+  hipLaunchKernelGGL (k_morton, grid (n), dim3 (PMX_TPB), 0, c.stream, n, cx, cy, cz, (const double*) t.ext.p,
+                      t.code.p, t.order.p);
+  PMX_HIP (hipGetLastError ());
+
+  uint32_t *k0 = t.code.p, *k1 = t.code2.p;
+  int *v0 = t.order.p, *v1 = t.order2.p;
+  for (int shift = 0; shift < 30; shift += RADIX_BITS)
+  {
+    hipLaunchKernelGGL (k_radix_hist, dim3 (nbsort), dim3 (PMX_TPB), 0, c.stream, n, (const uint32_t*) k0, shift,
+                        t.hist.p, nbsort);
+    PMX_HIP (hipGetLastError ());
+    int rc = scan_exclusive (c, t.hist.p, RADIX * nbsort, nullptr);
+    if (rc) return rc;
+    hipLaunchKernelGGL (k_radix_scatter, dim3 (nbsort), dim3 (PMX_TPB), 0, c.stream, n, (const uint32_t*) k0,
+                        (const int*) v0, shift, (const int*) t.hist.p, nbsort, k1, v1);
+    PMX_HIP (hipGetLastError ());
+    std::swap (k0, k1);
+    std::swap (v0, v1);
+  }
+  /* 5 passes: the sorted result sits in code2/order2 */
+  const uint32_t *code = k0;
+  const int *order = v0;
+
+  if (n == 1)
+  {
+    t.nleaves = 1;
+    t.root = -1;
+    if (t.leafpar.alloc (1) || t.cell.alloc (6)) { pmx_set_error ("out of device memory (tree)"); return PMX_ENOMEM; }
+    hipLaunchKernelGGL (k_fill_int, dim3 (1), dim3 (64), 0, c.stream, t.leafpar.p, 1, -1);
+    hipLaunchKernelGGL (k_cells, dim3 (1), dim3 (64), 0, c.stream, 1, (const int*) t.leafpar.p, (const int*) t.parent.p,
+                        (const int*) t.dim.p, (const double*) t.coord.p, t.cell.p);
+    PMX_HIP (hipGetLastError ());
+    return PMX_OK;
+  }
+
+  hipLaunchKernelGGL (k_karras, grid (n-1), dim3 (PMX_TPB), 0, c.stream, n, code, t.first.p, t.size.p, t.split.p, t.dim.p);
+  PMX_HIP (hipGetLastError ());
+
+  int rootsize = 0, rootdim = -1;
+  PMX_HIP (hipMemcpyAsync (&rootdim, t.dim.p, sizeof (int), hipMemcpyDeviceToHost, c.stream));
+  PMX_HIP (hipStreamSynchronize (c.stream));
+  (void) rootsize;
+  if (rootdim < 0)  /* whole set within CUTOFF: one leaf */
+  {
+    t.nleaves = 1;
+    t.root = -1;
+    if (t.leafpar.alloc (1) || t.cell.alloc (6)) { pmx_set_error ("out of device memory (tree)"); return PMX_ENOMEM; }
+    hipLaunchKernelGGL (k_fill_int, dim3 (1), dim3 (64), 0, c.stream, t.leafpar.p, 1, -1);
+    hipLaunchKernelGGL (k_cells, dim3 (1), dim3 (64), 0, c.stream, 1, (const int*) t.leafpar.p, (const int*) t.parent.p,
+                        (const int*) t.dim.p, (const double*) t.coord.p, t.cell.p);
+    PMX_HIP (hipGetLastError ());
+    return PMX_OK;
+  }
+
+  /* branch flags reuse code2 as int scratch only if sizes allow; use hist */
+  if (t.hist.alloc (std::max ((size_t) RADIX * nbsort + 1, (size_t) 2*ni + 2)) != hipSuccess)
+  { pmx_set_error ("out of device memory (tree)"); return PMX_ENOMEM; }
+  int *isbranch = t.hist.p;          /* [ni] -> scanned into branch positions */
+  int *blist = t.hist.p + ni + 1;    /* [nbranch] */
+  hipLaunchKernelGGL (k_children, grid (n-1), dim3 (PMX_TPB), 0, c.stream, n, (const int*) t.first.p,
+                      (const int*) t.size.p, (const int*) t.split.p, (const int*) t.dim.p, t.nleafkids.p, isbranch,
+                      t.parent.p);
+  PMX_HIP (hipGetLastError ());
+
+  /* branch list for the coordinate pass */
+  int nbranch = 0;
+  {
+    /* keep the flags: scan a copy inside blist area first */
+    PMX_HIP (hipMemcpyAsync (blist, isbranch, sizeof (int) * (n-1), hipMemcpyDeviceToDevice, c.stream));
+    int rc = scan_exclusive (c, blist, n-1, &nbranch);
+    if (rc) return rc;
+    /* isbranch keeps flags, blist holds positions; compact into isbranch's tail-free copy */
+    if (t.order2.alloc (n) != hipSuccess) { pmx_set_error ("out of device memory (tree)"); return PMX_ENOMEM; }
+  }
+  dvec<int> branchidx;
+  if (branchidx.alloc (std::max (nbranch, 1)) != hipSuccess) { pmx_set_error ("out of device memory (tree)"); return PMX_ENOMEM; }
+  hipLaunchKernelGGL (k_branchlist, grid (n-1), dim3 (PMX_TPB), 0, c.stream, n, (const int*) isbranch,
+                      (const int*) blist, branchidx.p);
+  PMX_HIP (hipGetLastError ());
+  if (nbranch > 0)
+  {
+    hipLaunchKernelGGL (k_coord, dim3 (nbranch), dim3 (64), 0, c.stream, (const int*) branchidx.p,
+                        (const int*) t.first.p, (const int*) t.size.p, (const int*) t.split.p, (const int*) t.dim.p,
+                        order, cx, cy, cz, t.coord.p);
+    PMX_HIP (hipGetLastError ());
+  }
+
+  /* leaf numbering */
+  int nleaves = 0;
+  int rc = scan_exclusive (c, t.nleafkids.p, n-1, &nleaves);
+  if (rc) { branchidx.release (); return rc; }
+  t.nleaves = nleaves;
+  if (t.leafpar.alloc (nleaves) || t.cell.alloc (6*(size_t) nleaves))
+  { branchidx.release (); pmx_set_error ("out of device memory (tree)"); return PMX_ENOMEM; }
+  hipLaunchKernelGGL (k_assign, grid (n-1), dim3 (PMX_TPB), 0, c.stream, n, (const int*) t.first.p, (const int*) t.size.p,
+                      (const int*) t.split.p, (const int*) t.dim.p, (const int*) t.nleafkids.p, t.left.p, t.right.p,
+                      t.leafpar.p);
+  hipLaunchKernelGGL (k_cells, grid (nleaves), dim3 (PMX_TPB), 0, c.stream, nleaves, (const int*) t.leafpar.p,
+                      (const int*) t.parent.p, (const int*) t.dim.p, (const double*) t.coord.p, t.cell.p);
+  PMX_HIP (hipGetLastError ());
+  t.root = 0;
+  PMX_HIP (hipStreamSynchronize (c.stream));
+  branchidx.release ();
+  return PMX_OK;
+}
+
+int tree_trilists (pmx_ctx &c)
+{
+  TreeDev &t = c.tree;
+  int L = std::max (t.nleaves, 1);
+  if (t.tricnt.alloc (L) || t.trioff.alloc (L + 1)) { pmx_set_error ("out of device memory (trilists)"); return PMX_ENOMEM; }
+  hipLaunchKernelGGL (k_fill_int, grid (L), dim3 (PMX_TPB), 0, c.stream, t.tricnt.p, L, 0);
+  PMX_HIP (hipGetLastError ());
+  t.ntrilist = 0;
+  if (c.T == 0 || c.E == 0 || t.nleaves == 0)
+  {
+    hipLaunchKernelGGL (k_fill_int, grid (L), dim3 (PMX_TPB), 0, c.stream, t.trioff.p, L, 0);
+    PMX_HIP (hipGetLastError ());
+    return PMX_OK;
+  }
+  int *flag = c.dflags.p + FLAG_TRILIST_OVERFLOW;
+  hipLaunchKernelGGL (k_tri_reach, grid (c.T), dim3 (PMX_TPB), 0, c.stream, c.T, (const double*) c.tri.p, t.root,
+                      (const int*) t.left.p, (const int*) t.right.p, (const int*) t.dim.p, (const double*) t.coord.p,
+                      t.tricnt.p, (const int*) nullptr, (int*) nullptr, (int*) nullptr, flag);
+  PMX_HIP (hipGetLastError ());
+  PMX_HIP (hipMemcpyAsync (t.trioff.p, t.tricnt.p, sizeof (int) * t.nleaves, hipMemcpyDeviceToDevice, c.stream));
+  int total = 0;
+  int rc = scan_exclusive (c, t.trioff.p, t.nleaves, &total);
+  if (rc) return rc;
+  t.ntrilist = total;
+  if (t.trilist.alloc (std::max (total, 1))) { pmx_set_error ("out of device memory (trilists)"); return PMX_ENOMEM; }
+  dvec<int> fill;
+  if (fill.alloc (L)) { pmx_set_error ("out of device memory (trilists)"); return PMX_ENOMEM; }
+  hipLaunchKernelGGL (k_fill_int, grid (L), dim3 (PMX_TPB), 0, c.stream, fill.p, L, 0);
+  hipLaunchKernelGGL (k_tri_reach, grid (c.T), dim3 (PMX_TPB), 0, c.stream, c.T, (const double*) c.tri.p, t.root,
+                      (const int*) t.left.p, (const int*) t.right.p, (const int*) t.dim.p, (const double*) t.coord.p,
+                      (int*) nullptr, (const int*) t.trioff.p, t.trilist.p, fill.p, flag);
+  PMX_HIP (hipGetLastError ());
+  PMX_HIP (hipStreamSynchronize (c.stream));
+  fill.release ();
+  return PMX_OK;
+}

@@ -1,0 +1,387 @@
+"""Parmec's Python input API (input.cpp:4093-4119) over the MI355X engine.
+
+A deck written for Parmec runs unchanged against this module for the
+supported subset:
+
+    from parmec_amd.parmec import *
+    mat = MATERIAL(2000.0, 1e6, 0.25)
+    SPHERE((0.0, 0.0, 0.01), 0.005, mat, 1)
+    OBSTACLE([(0,0,0, 1,0,0, 1,1,0)], 2)
+    GRANULAR(0, 0, 1e6, 0.1, 0.3)
+    GRAVITY(0.0, 0.0, -9.81)
+    DEM(0.02, 1e-5)
+
+Names, argument order, defaults and error types follow the reference:
+MATERIAL (input.cpp:728-746), SPHERE (749-837), OBSTACLE (1367-1437),
+GRANULAR (2384-2455), VELOCITY (2798-2837), GRAVITY (2840-2933),
+DAMPING (2936-3013), TSERIES (626-725), DEM (3958-4091), RESET (610-623).
+Gravity and damping callbacks and TSERIES are evaluated on the host at each
+step's curtime (input.cpp:4424-4503) and handed to the device as per-step
+tables.  The step loop itself is libpmx's pmx_dem (C++ host + HIP kernels).
+
+Not on this path (raise NotImplementedError): MESH, ANALYTICAL, SPRING and
+the other constraint/spring kinds, moving obstacles, PRESCRIBE, HISTORY,
+OUTPUT, CRITICAL and adaptive time stepping.
+"""
+import bisect
+import math
+
+import numpy as np
+
+from . import pmx
+
+__all__ = ["RESET", "TSERIES", "MATERIAL", "SPHERE", "OBSTACLE", "GRANULAR", "VELOCITY", "GRAVITY", "DAMPING",
+           "DEM", "MESH", "ANALYTICAL", "SPRING", "PRESCRIBE", "HISTORY", "OUTPUT", "CRITICAL", "state", "contacts",
+           "engine_options"]
+
+_DENSITY, _YOUNG, _POISSON = 0, 1, 2
+
+
+class _Series:
+    """Piecewise-linear time series (timeseries.cpp:241-263)."""
+
+    def __init__(self, times=None, values=None, constant=None):
+        self.t = list(times or [])
+        self.v = list(values or [])
+        self.constant = constant
+
+    def __call__(self, time):
+        if self.constant is not None:
+            return self.constant
+        t, v = self.t, self.v
+        if time < t[0]:
+            return v[0]
+        if time > t[-1]:
+            return v[-1]
+        i = min(max(bisect.bisect_right(t, time) - 1, 0), len(t) - 2)
+        dt = time - t[i]
+        return v[i] + (v[i + 1] - v[i]) * (dt / (t[i + 1] - t[i]))
+
+
+class _State:
+    def __init__(self):
+        self.reset()
+
+    def reset(self):
+        self.materials = []          # (density, young, poisson)
+        self.sph = []                # (cx, cy, cz, r, material, color)
+        self.vel = {}                # particle -> (lin or None, ang or None)
+        self.tris = []               # 9-tuples
+        self.tricol = []
+        self.pairs = [(0, 0, [0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.5])]   # pair_reset, parmec.cpp:375-388
+        self.tms = []
+        self.gravity = [0.0, 0.0, 0.0]
+        self.gravfunc = [None, None, None]
+        self.lindamp = None
+        self.angdamp = None
+        self.model = None            # SoA state once DEM has run
+        self.engine = None
+        self.dirty = True
+        self.options = {"row_capacity": 0, "device": 0}
+
+
+_S = _State()
+
+
+def engine_options(**kw):
+    """Device options of the engine (row_capacity, device)."""
+    _S.options.update(kw)
+    _S.engine = None
+
+
+def _is_number(x):
+    return isinstance(x, (int, float)) and not isinstance(x, bool)
+
+
+def _tuple(x, name, n):
+    if not isinstance(x, tuple) or len(x) != n:
+        raise TypeError(f"'{name}' must be a tuple of {n} elements")
+    return tuple(float(v) for v in x)
+
+
+def RESET():
+    """Reset the simulation (input.cpp:610-623)."""
+    if _S.engine is not None:
+        _S.engine.close()
+    _S.reset()
+
+
+def TSERIES(points):
+    """Create a time series; returns its number (input.cpp:626-725)."""
+    if _is_number(points):
+        ts = _Series(constant=float(points))
+    elif isinstance(points, str):
+        data = np.loadtxt(points).reshape(-1, 2)
+        ts = _Series(list(data[:, 0]), list(data[:, 1]))
+    elif isinstance(points, list):
+        if points and isinstance(points[0], (list, tuple)):
+            ts = _Series([float(p[0]) for p in points], [float(p[1]) for p in points])
+        else:
+            if len(points) < 4:
+                raise ValueError("Time series must have at least two points")
+            ts = _Series([float(x) for x in points[0::2]], [float(x) for x in points[1::2]])
+    else:
+        raise TypeError("Invalid points format")
+    _S.tms.append(ts)
+    return len(_S.tms) - 1
+
+
+def MATERIAL(density, young, poisson):
+    """Create a bulk material; returns its number (input.cpp:728-746)."""
+    for name, v in (("density", density), ("young", young), ("poisson", poisson)):
+        if not v > 0:
+            raise ValueError(f"'{name}' must be positive")
+    _S.materials.append((float(density), float(young), float(poisson)))
+    return len(_S.materials) - 1
+
+
+def SPHERE(center, radius, material, color):
+    """Create a spherical particle; returns the particle number (input.cpp:749-837)."""
+    c = _tuple(center, "center", 3)
+    if not radius > 0:
+        raise ValueError("'radius' must be positive")
+    if not (0 <= material < len(_S.materials)):
+        raise ValueError("'material' must be >= 0 and < number of materials")
+    if not color > 0:
+        raise ValueError("'color' must be positive")
+    _S.sph.append((c[0], c[1], c[2], float(radius), int(material), int(color)))
+    _S.dirty = True
+    return len(_S.sph) - 1
+
+
+def OBSTACLE(triangles, color, point=None, linear=None, angular=None):
+    """Create obstacle triangles (input.cpp:1367-1437)."""
+    if not isinstance(triangles, list) or any(not isinstance(t, tuple) or len(t) != 9 for t in triangles):
+        raise TypeError("'triangles' must be a list of 9-element tuples")
+    if isinstance(color, list):
+        if len(color) != len(triangles):
+            raise ValueError("'color' list must have one entry per triangle")
+        cols = [int(c) for c in color]
+    else:
+        cols = [int(color)] * len(triangles)
+    if any(c <= 0 for c in cols):
+        raise ValueError("'color' must be positive")
+    if point is not None and (linear is not None or angular is not None):
+        raise NotImplementedError("moving obstacles are not on the device path")
+    _S.tris.extend(tuple(float(x) for x in t) for t in triangles)
+    _S.tricol.extend(cols)
+    _S.dirty = True
+
+
+def GRANULAR(color1, color2, spring, damper=1.0, friction=0.0, rolling=0.0, drilling=0.0, kskn=0.5):
+    """Granular interaction parameters of a colour pair (input.cpp:2384-2455)."""
+    if not spring > 0:
+        raise ValueError("'spring' must be positive")
+    if damper < 0 or rolling < 0 or drilling < 0:
+        raise ValueError("'damper', 'rolling' and 'drilling' must be non-negative")
+    if not kskn > 0:
+        raise ValueError("'kskn' must be positive")
+    if isinstance(friction, tuple):
+        if len(friction) != 2:
+            raise TypeError("'friction' must be a number or a (static, dynamic) tuple")
+        fri = (float(friction[0]), float(friction[1]))
+    else:
+        fri = (float(friction), float(friction))
+    if fri[0] < 0 or fri[1] < 0:
+        raise ValueError("'friction' must be non-negative")
+    par = [float(spring), float(damper), fri[0], fri[1], float(rolling), float(drilling), float(kskn)]
+    if color1 == 0 and color2 == 0:
+        _S.pairs[0] = (0, 0, par)
+    elif color1 > 0 and color2 > 0:
+        _S.pairs.append((min(color1, color2), max(color1, color2), par))
+    else:
+        raise ValueError("Invalid color values")
+    _S.dirty = True
+
+
+def VELOCITY(parnum, linear=None, angular=None):
+    """Set initial velocities of a particle (input.cpp:2798-2837)."""
+    if not (0 <= parnum < len(_S.sph)):
+        raise ValueError("'parnum' must be >= 0 and < number of particles")
+    lin = _tuple(linear, "linear", 3) if linear is not None else None
+    ang = _tuple(angular, "angular", 3) if angular is not None else None
+    if _S.model is not None:
+        m = _S.model
+        if lin:
+            m.linear[:, parnum] = lin
+        if ang:
+            m.angular[3:, parnum] = ang
+            L = m.rotation[:, parnum]
+            o = ang
+            m.angular[0, parnum] = L[0] * o[0] + L[1] * o[1] + L[2] * o[2]   # TVMUL, macros.h:966-971
+            m.angular[1, parnum] = L[3] * o[0] + L[4] * o[1] + L[5] * o[2]
+            m.angular[2, parnum] = L[6] * o[0] + L[7] * o[1] + L[8] * o[2]
+        _S.dirty = True
+    else:
+        _S.vel[parnum] = (lin, ang)
+
+
+def _component(x, name):
+    if callable(x):
+        return ("func", x)
+    if isinstance(x, float):
+        return ("const", x)
+    if isinstance(x, int) and not isinstance(x, bool):
+        if not (0 <= x < len(_S.tms)):
+            raise ValueError(f"Invalid TSERIES number for {name}")
+        return ("tms", x)
+    raise ValueError(f"Invalid {name} component")
+
+
+def GRAVITY(gx, gy, gz):
+    """Set gravity; a float is constant, a callable is g(t), an int is a TSERIES
+    number (input.cpp:2840-2933)."""
+    comps = [_component(g, n) for g, n in ((gx, "gx"), (gy, "gy"), (gz, "gz"))]
+    for k, (kind, v) in enumerate(comps):
+        if kind == "const":
+            _S.gravity[k] = v
+            _S.gravfunc[k] = None
+        elif kind == "func":
+            _S.gravfunc[k] = v
+        else:
+            _S.gravfunc[k] = _S.tms[v]
+
+
+def DAMPING(linear, angular):
+    """Global damping: callables returning 3-tuples, or 3-tuples of TSERIES
+    numbers (input.cpp:2936-3013)."""
+    def one(x, name):
+        if callable(x):
+            return x
+        if isinstance(x, tuple):
+            if len(x) != 3:
+                raise ValueError(f"Invalid {name} damping tuple size")
+            ids = [int(v) for v in x]
+            if any(not (0 <= i < len(_S.tms)) for i in ids):
+                raise ValueError(f"Invalid TSERIES number in {name} damping tuple")
+            return lambda t, ids=ids: tuple(_S.tms[i](t) for i in ids)
+        raise ValueError(f"Invalid {name} damping argument type")
+    _S.lindamp = one(linear, "linear")
+    _S.angdamp = one(angular, "angular")
+
+
+def _not_here(name):
+    def f(*a, **k):
+        raise NotImplementedError(f"{name} is not on the MI355X step path")
+    f.__name__ = name
+    return f
+
+
+MESH = _not_here("MESH")
+ANALYTICAL = _not_here("ANALYTICAL")
+SPRING = _not_here("SPRING")
+PRESCRIBE = _not_here("PRESCRIBE")
+HISTORY = _not_here("HISTORY")
+OUTPUT = _not_here("OUTPUT")
+CRITICAL = _not_here("CRITICAL")
+
+
+def _build_model():
+    """SoA model of everything created so far (the buffers of parmec.h)."""
+    n, t = len(_S.sph), len(_S.tris)
+    old = _S.model
+    npairs = len(_S.pairs)
+    m = pmx.Model(parnum=n, ellnum=n, trinum=t, pairnum=npairs)
+    if n:
+        a = np.array([s[:4] for s in _S.sph], dtype=np.float64)
+        mat = np.array([s[4] for s in _S.sph])
+        col = np.array([s[5] for s in _S.sph], dtype=np.int32)
+        dens = np.array([_S.materials[k][_DENSITY] for k in mat])
+        r = a[:, 3]
+        m.position[:3] = a[:, :3].T
+        m.position[3:] = a[:, :3].T
+        m.center[:3] = a[:, :3].T
+        m.center[3:] = a[:, :3].T
+        m.radii[0] = r
+        m.radii[1:] = -1.0
+        m.part[:] = np.arange(n, dtype=np.int32)
+        m.ellcol[:] = col
+        volume = (4.0 / 3.0) * math.pi * r * r * r
+        m.mass[:] = volume * dens
+        m.invm[:] = 1.0 / m.mass
+        j = 0.4 * m.mass * r * r
+        for k in (0, 4, 8):
+            m.inertia[k] = j
+        if old is not None:   # particles created earlier keep their evolved state
+            k = old.parnum
+            for name in ("angular", "linear", "rotation", "position", "force", "torque", "center"):
+                getattr(m, name)[:, :k] = getattr(old, name)[:, :k]
+            m.curtime, m.curstep, m.stepnum = old.curtime, old.curstep, old.stepnum
+        for p, (lin, ang) in _S.vel.items():
+            if lin:
+                m.linear[:, p] = lin
+            if ang:
+                m.angular[3:, p] = ang
+                L = m.rotation[:, p]
+                o = ang
+                m.angular[0, p] = L[0] * o[0] + L[1] * o[1] + L[2] * o[2]
+                m.angular[1, p] = L[3] * o[0] + L[4] * o[1] + L[5] * o[2]
+                m.angular[2, p] = L[6] * o[0] + L[7] * o[1] + L[8] * o[2]
+        _S.vel.clear()
+    if t:
+        m.tri[:] = np.array(_S.tris, dtype=np.float64).T
+        m.tricol[:] = np.array(_S.tricol, dtype=np.int32)
+        m.triobs[:] = -1
+    # sort_materials, parmec.cpp:1636-1663: pairs ordered by colours
+    pairs = sorted(_S.pairs, key=lambda p: (p[0], p[1]))
+    for i, (c1, c2, par) in enumerate(pairs):
+        m.pairs[2 * i], m.pairs[2 * i + 1] = c1, c2
+        m.ikind[i] = pmx.GRANULAR_FORCE
+        m.iparam[:, i] = par
+    return m
+
+
+def DEM(duration, step, interval=None, prefix=None, adaptive=0.0):
+    """Run the DEM time stepping; returns the wall time in seconds
+    (input.cpp:3958-4091, parmec.cpp:2315-2530).  Output files are not written
+    (interval and prefix are accepted and ignored)."""
+    if not duration > 0:
+        raise ValueError("'duration' must be positive")
+    if not step > 0:
+        raise ValueError("'step' must be positive")
+    if prefix is not None and not isinstance(prefix, str):
+        raise TypeError("'prefix' must be a string")
+    if not (0.0 <= adaptive <= 1.0):
+        raise ValueError("'adaptive' must be >= 0 and <= 1")
+    if adaptive > 0.0:
+        raise NotImplementedError("adaptive time stepping is not on the MI355X step path")
+    if _S.engine is None:
+        _S.engine = pmx.Engine(device=_S.options["device"], row_capacity=_S.options["row_capacity"])
+        _S.dirty = True
+    if _S.dirty or _S.model is None:
+        _S.model = _build_model()
+        _S.engine.upload(_S.model)
+        _S.dirty = False
+    m = _S.model
+    h = step if m.curtime == 0.0 else m.curstep
+    times = pmx.step_times(duration, h, m.curtime)
+    gtab = None
+    if any(f is not None for f in _S.gravfunc):
+        gtab = np.empty((len(times), 3))
+        for k in range(3):
+            f = _S.gravfunc[k]
+            gtab[:, k] = [float(f(t)) for t in times] if f is not None else _S.gravity[k]
+    dtab = None
+    if _S.lindamp is not None or _S.angdamp is not None:
+        dtab = np.zeros((len(times), 6))
+        for i, t in enumerate(times):
+            if _S.lindamp is not None:
+                dtab[i, :3] = _S.lindamp(t)
+            if _S.angdamp is not None:
+                dtab[i, 3:] = _S.angdamp(t)
+    f = pmx.forcing(len(times), gravity=tuple(_S.gravity), gtab=gtab, dtab=dtab)
+    elapsed = _S.engine.dem(float(duration), float(step), f)
+    _S.engine.download(m)
+    return elapsed
+
+
+def state():
+    """The SoA model (numpy arrays shaped like parmec.h's buffers)."""
+    if _S.model is None:
+        _S.model = _build_model()
+    return _S.model
+
+
+def contacts():
+    """Current contact pairs (master ellipsoid, slave ellipsoid or -(triangle+1), state)."""
+    return _S.engine.contacts() if _S.engine is not None else None

@@ -1,0 +1,251 @@
+"""ctypes binding of libpmx.so (include/pmx.h) and the SoA model container.
+
+`Model` owns numpy arrays laid out like Parmec's global buffers
+(parmec.h:66-283): a k-component per-item array has shape (k, n), C order,
+so component c of item i sits at c*n + i.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libpmx.so")
+
+PMX_OK = 0
+PMX_FP64 = 0
+PMX_MIXED = 1
+NIPARAM = 7
+SPRING, DAMPER, FRISTAT, FRIDYN, FRIROL, FRIDRIL, KSKN = range(7)
+GRANULAR_FORCE = 0
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class pmx_model(C.Structure):
+    _fields_ = [
+        ("parnum", C.c_int), ("angular", _dp), ("linear", _dp), ("rotation", _dp), ("position", _dp),
+        ("inertia", _dp), ("inverse", _dp), ("mass", _dp), ("invm", _dp), ("force", _dp), ("torque", _dp),
+        ("flags", _ip),
+        ("ellnum", C.c_int), ("center", _dp), ("radii", _dp), ("part", _ip), ("ellcol", _ip),
+        ("trinum", C.c_int), ("tri", _dp), ("tricol", _ip), ("triobs", _ip),
+        ("obsnum", C.c_int), ("trirng", _ip), ("obspnt", _dp), ("obslin", _dp), ("obsang", _dp),
+        ("nodnum", C.c_int), ("nodes", _dp), ("nodpart", _ip), ("facnum", C.c_int), ("facnod", _ip), ("factri", _ip),
+        ("pairnum", C.c_int), ("pairs", _ip), ("ikind", _ip), ("iparam", _dp),
+        ("sprnum", C.c_int), ("sprtype", _ip), ("sprpart", _ip), ("sprpnt", _dp), ("spridx", _ip), ("spring", _dp),
+        ("sprcrv", C.c_int), ("dashidx", _ip), ("dashpot", _dp), ("dashcrv", C.c_int), ("unidx", _ip),
+        ("unload", _dp), ("uncrv", C.c_int), ("yield_", _dp), ("sprdir", _dp), ("sprflg", _ip), ("sprfric", _dp),
+        ("sprkskn", _dp), ("sprsdsp", _dp), ("stroke0", _dp), ("stroke", _dp), ("sprfrc", _dp), ("unspring", _ip),
+        ("sproffset", _ip),
+        ("curtime", C.c_double), ("curstep", C.c_double), ("stepnum", C.c_int),
+    ]
+
+
+class pmx_forcing(C.Structure):
+    _fields_ = [("gravity", C.c_double * 3), ("gtab", _dp), ("damping", C.c_double * 6), ("dtab", _dp),
+                ("obstab", _dp), ("obstab0", _dp)]
+
+
+class pmx_options(C.Structure):
+    _fields_ = [("precision", C.c_int), ("device", C.c_int), ("row_capacity", C.c_int), ("verbose", C.c_int)]
+
+
+class pmx_pair(C.Structure):
+    _fields_ = [("master", C.c_int), ("slave", C.c_int), ("state", C.c_double * 3)]
+
+
+class pmx_stats(C.Structure):
+    _fields_ = [("steps", C.c_longlong), ("rebuilds", C.c_longlong), ("pairs", C.c_longlong), ("leaves", C.c_int),
+                ("max_row", C.c_int), ("seconds", C.c_double)]
+
+
+EXPORTS = ["pmx_step_times", "pmx_create", "pmx_upload", "pmx_dem", "pmx_download", "pmx_contacts",
+           "pmx_get_stats", "pmx_destroy", "pmx_last_error"]
+
+
+class PmxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"pmx error {code}: {msg}")
+        self.code = code
+
+
+def _declare(lib, prefix):
+    f = lambda n: getattr(lib, prefix + n)
+    f("step_times").argtypes = [C.c_double, C.c_double, C.c_double, _dp, C.c_int, _ip]
+    f("last_error").restype = C.c_char_p
+    f("upload").argtypes = [C.c_void_p, C.POINTER(pmx_model)]
+    f("download").argtypes = [C.c_void_p, C.POINTER(pmx_model)]
+    f("dem").argtypes = [C.c_void_p, C.c_double, C.c_double, C.POINTER(pmx_forcing), _dp]
+    f("contacts").argtypes = [C.c_void_p, C.POINTER(pmx_pair), C.c_size_t, C.POINTER(C.c_size_t)]
+    f("get_stats").argtypes = [C.c_void_p, C.POINTER(pmx_stats)]
+    f("destroy").argtypes = [C.c_void_p]
+    f("destroy").restype = None
+    return lib
+
+
+_lib = None
+
+
+def load(path=LIBPATH):
+    """Load libpmx.so; raises when the HIP library has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise ImportError(f"{path} is missing: build it with `python parmec_amd/build.py`")
+        lib = C.CDLL(path)
+        lib.pmx_create.argtypes = [C.POINTER(pmx_options), C.POINTER(C.c_void_p)]
+        _lib = _declare(lib, "pmx_")
+    return _lib
+
+
+def step_times(duration, step, curtime=0.0, lib=None, prefix="pmx_"):
+    """Curtime at the start of each step of dem(duration, step) (parmec.cpp:2418)."""
+    lib = lib or load()
+    f = getattr(lib, prefix + "step_times")
+    n = C.c_int(0)
+    f(duration, step, curtime, None, 0, C.byref(n))
+    times = np.zeros(max(n.value, 1))
+    f(duration, step, curtime, times.ctypes.data_as(_dp), n.value, C.byref(n))
+    return times[: n.value]
+
+
+def _ptr(a, kind):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_dp if kind == "d" else _ip)
+
+
+class Model:
+    """SoA state of a Parmec model (the arrays the step path reads and writes)."""
+
+    def __init__(self, parnum=0, ellnum=0, trinum=0, pairnum=1):
+        self.parnum, self.ellnum, self.trinum, self.pairnum = parnum, ellnum, trinum, pairnum
+        N, E, T, P = parnum, ellnum, trinum, pairnum
+        z = lambda *s: np.zeros(s, dtype=np.float64)
+        zi = lambda *s: np.zeros(s, dtype=np.int32)
+        self.angular, self.linear, self.rotation, self.position = z(6, N), z(3, N), z(9, N), z(6, N)
+        self.rotation[[0, 4, 8]] = 1.0
+        self.inertia, self.inverse, self.mass, self.invm = z(9, N), z(9, N), z(N), z(N)
+        self.force, self.torque, self.flags = z(3, N), z(3, N), zi(N)
+        self.center, self.radii, self.part, self.ellcol = z(6, E), z(3, E), zi(E), zi(E)
+        self.tri, self.tricol, self.triobs = z(9, T), zi(T), np.full(T, -1, dtype=np.int32)
+        self.pairs, self.ikind, self.iparam = zi(2 * P), zi(P), z(NIPARAM, P)
+        self.iparam[DAMPER, 0] = 1.0
+        self.iparam[KSKN, 0] = 0.5
+        self.curtime, self.curstep, self.stepnum = 0.0, 0.0, 0
+
+    def copy(self):
+        m = Model.__new__(Model)
+        for k, v in self.__dict__.items():
+            setattr(m, k, v.copy() if isinstance(v, np.ndarray) else v)
+        return m
+
+    def check(self):
+        for name in ("angular", "linear", "rotation", "position", "inertia", "inverse", "mass", "invm", "force",
+                     "torque", "center", "radii", "tri", "iparam"):
+            a = getattr(self, name)
+            if a.dtype != np.float64 or not a.flags.c_contiguous:
+                setattr(self, name, np.ascontiguousarray(a, dtype=np.float64))
+        for name in ("flags", "part", "ellcol", "tricol", "triobs", "pairs", "ikind"):
+            a = getattr(self, name)
+            if a.dtype != np.int32 or not a.flags.c_contiguous:
+                setattr(self, name, np.ascontiguousarray(a, dtype=np.int32))
+
+    def struct(self):
+        self.check()
+        s = pmx_model()
+        s.parnum, s.ellnum, s.trinum, s.pairnum = self.parnum, self.ellnum, self.trinum, self.pairnum
+        for name in ("angular", "linear", "rotation", "position", "inertia", "inverse", "mass", "invm", "force",
+                     "torque", "center", "radii", "tri", "iparam"):
+            setattr(s, name, _ptr(getattr(self, name), "d"))
+        for name in ("flags", "part", "ellcol", "tricol", "triobs", "pairs", "ikind"):
+            setattr(s, name, _ptr(getattr(self, name), "i"))
+        s.curtime, s.curstep, s.stepnum = self.curtime, self.curstep, self.stepnum
+        self._keep = s
+        return s
+
+    def take_time(self, s):
+        self.curtime, self.curstep, self.stepnum = s.curtime, s.curstep, s.stepnum
+
+
+def forcing(nsteps=0, gravity=(0.0, 0.0, 0.0), gtab=None, damping=(0.0,) * 6, dtab=None):
+    f = pmx_forcing()
+    for k in range(3):
+        f.gravity[k] = gravity[k]
+    for k in range(6):
+        f.damping[k] = damping[k]
+    keep = []
+    if gtab is not None:
+        gtab = np.ascontiguousarray(gtab, dtype=np.float64).reshape(-1, 3)
+        assert gtab.shape[0] >= nsteps
+        f.gtab = gtab.ctypes.data_as(_dp)
+        keep.append(gtab)
+    if dtab is not None:
+        dtab = np.ascontiguousarray(dtab, dtype=np.float64).reshape(-1, 6)
+        assert dtab.shape[0] >= nsteps
+        f.dtab = dtab.ctypes.data_as(_dp)
+        keep.append(dtab)
+    f._keep = keep
+    return f
+
+
+class Engine:
+    """One device context of libpmx (pmx_create ... pmx_destroy)."""
+
+    prefix = "pmx_"
+
+    def __init__(self, precision=PMX_FP64, device=0, row_capacity=0, lib=None):
+        self.lib = lib or load()
+        opt = pmx_options(precision, device, row_capacity, 0)
+        h = C.c_void_p()
+        self._call("create", C.byref(opt), C.byref(h))
+        self.h = h
+
+    def _call(self, name, *args):
+        rc = getattr(self.lib, self.prefix + name)(*args)
+        if rc != PMX_OK:
+            raise PmxError(rc, getattr(self.lib, self.prefix + "last_error")().decode())
+        return rc
+
+    def upload(self, model):
+        self._call("upload", self.h, C.byref(model.struct()))
+
+    def download(self, model):
+        s = model.struct()
+        self._call("download", self.h, C.byref(s))
+        model.take_time(s)
+
+    def dem(self, duration, step, forcing_=None):
+        el = C.c_double(0.0)
+        self._call("dem", self.h, duration, step, C.byref(forcing_) if forcing_ is not None else None, C.byref(el))
+        return el.value
+
+    def contacts(self):
+        n = C.c_size_t(0)
+        self._call("contacts", self.h, None, 0, C.byref(n))
+        arr = (pmx_pair * max(n.value, 1))()
+        self._call("contacts", self.h, arr, n.value, C.byref(n))
+        out = np.zeros(n.value, dtype=[("master", np.int32), ("slave", np.int32), ("state", np.float64, 3)])
+        if n.value:
+            raw = np.ctypeslib.as_array(arr)[: n.value]
+            out["master"] = raw["master"]
+            out["slave"] = raw["slave"]
+            out["state"] = raw["state"]
+        return out
+
+    def stats(self):
+        s = pmx_stats()
+        self._call("get_stats", self.h, C.byref(s))
+        return {k: getattr(s, k) for k, _ in pmx_stats._fields_}
+
+    def close(self):
+        if getattr(self, "h", None):
+            getattr(self.lib, self.prefix + "destroy")(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
