@@ -173,6 +173,15 @@ typedef struct pmx_stats
   double seconds;          /* wall time of the last pmx_dem step loop */
 } pmx_stats;
 
+/* per-kernel timing collected with HIP events on the context's stream */
+typedef struct pmx_kernel_time
+{
+  char name[32];
+  long long launches;
+  double ms;               /* summed event time */
+  double bytes;            /* summed algorithmic HBM bytes of the launches */
+} pmx_kernel_time;
+
 typedef struct pmx_ctx pmx_ctx;
 
 /* Number of steps dem(duration, step) takes from the given curtime
@@ -198,6 +207,14 @@ int pmx_download (pmx_ctx *ctx, pmx_model *model);
 int pmx_contacts (pmx_ctx *ctx, pmx_pair *out, size_t cap, size_t *count);
 
 int pmx_get_stats (pmx_ctx *ctx, pmx_stats *stats);
+
+/* Enable (1) or disable (0) per-kernel event timing; enabling clears totals.
+ * Timing stands in for the reference's dem() wall clock (timer.h:36-49). */
+int pmx_set_profiling (pmx_ctx *ctx, int on);
+
+/* Per-kernel totals since profiling was enabled; count receives the number
+ * of entries (at most cap are written). */
+int pmx_get_profile (pmx_ctx *ctx, pmx_kernel_time *out, int cap, int *count);
 
 void pmx_destroy (pmx_ctx *ctx);
 

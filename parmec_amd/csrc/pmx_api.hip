@@ -56,6 +56,23 @@ int alloc_rows (pmx_ctx &c)
 
 } /* namespace */
 
+void prof_resolve (pmx_ctx &c)
+{
+  for (auto &p : c.prof.pend)
+  {
+    float ms = 0.0f;
+    if (hipEventElapsedTime (&ms, p.a, p.b) == hipSuccess)
+    {
+      c.prof.ms[p.kid] += ms;
+      c.prof.n[p.kid] += 1;
+      c.prof.bytes[p.kid] += p.bytes;
+    }
+    c.prof.pool.push_back (p.a);
+    c.prof.pool.push_back (p.b);
+  }
+  c.prof.pend.clear ();
+}
+
 extern "C" {
 
 const char *pmx_last_error (void) { return g_last_error.c_str (); }
@@ -220,6 +237,7 @@ int pmx_dem (pmx_ctx *c, double duration, double step, const pmx_forcing *fc, do
   long long rebuilds = 0;
   if (c->E > 0)
   {
+    ProfScope ps (*c, K_TREE, 0.0);
     if ((rc = tree_build (*c)) || (rc = tree_trilists (*c))) return rc;
     rebuilds++;
   }
@@ -235,7 +253,10 @@ int pmx_dem (pmx_ctx *c, double duration, double step, const pmx_forcing *fc, do
       if ((rc = step_bin (*c, &overflow))) return rc;
       if (overflow > 0)
       {
-        if ((rc = tree_build (*c)) || (rc = tree_trilists (*c))) return rc;
+        {
+          ProfScope ps (*c, K_TREE, 0.0);
+          if ((rc = tree_build (*c)) || (rc = tree_trilists (*c))) return rc;
+        }
         rebuilds++;
         if ((rc = step_bin (*c, &overflow))) return rc;
         if (overflow > 0) return fail (PMX_EREPART, "Repartitioning failed");
@@ -256,6 +277,7 @@ int pmx_dem (pmx_ctx *c, double duration, double step, const pmx_forcing *fc, do
     if ((rc = step_bin (*c, &dummy))) return rc;
   }
   PMX_HIP (hipStreamSynchronize (c->stream));
+  prof_resolve (*c);
   c->curstep = step1;
   c->stepnum++;
   double dt = std::chrono::duration<double> (std::chrono::steady_clock::now () - t0).count ();
@@ -307,6 +329,37 @@ int pmx_contacts (pmx_ctx *c, pmx_pair *out, size_t cap, size_t *count)
   }
   if (count) *count = all.size ();
   if (out) std::memcpy (out, all.data (), std::min (cap, all.size ()) * sizeof (pmx_pair));
+  return PMX_OK;
+}
+
+int pmx_set_profiling (pmx_ctx *c, int on)
+{
+  if (!c) return fail (PMX_EINVAL, "null ctx");
+  (void) hipSetDevice (c->device);
+  PMX_HIP (hipStreamSynchronize (c->stream));
+  prof_resolve (*c);
+  c->prof.on = on != 0;
+  for (int k = 0; k < K_COUNT; k++) { c->prof.ms[k] = 0.0; c->prof.bytes[k] = 0.0; c->prof.n[k] = 0; }
+  return PMX_OK;
+}
+
+int pmx_get_profile (pmx_ctx *c, pmx_kernel_time *out, int cap, int *count)
+{
+  static const char *names[K_COUNT] = {"bin", "scatter", "detect", "force_init", "rows", "dynamics", "shapes",
+                                       "tree_build", "tree_trilists"};
+  if (!c) return fail (PMX_EINVAL, "null ctx");
+  (void) hipSetDevice (c->device);
+  PMX_HIP (hipStreamSynchronize (c->stream));
+  prof_resolve (*c);
+  if (count) *count = K_COUNT;
+  for (int k = 0; k < K_COUNT && out && k < cap; k++)
+  {
+    std::memset (out[k].name, 0, sizeof (out[k].name));
+    std::strncpy (out[k].name, names[k], sizeof (out[k].name) - 1);
+    out[k].launches = c->prof.n[k];
+    out[k].ms = c->prof.ms[k];
+    out[k].bytes = c->prof.bytes[k];
+  }
   return PMX_OK;
 }
 

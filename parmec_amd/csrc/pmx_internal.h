@@ -65,6 +65,28 @@ struct TreeDev
   int ntrilist = 0;
 };
 
+/* kernels timed by the profiler */
+enum { K_BIN = 0, K_SCATTER, K_DETECT, K_FORCE_INIT, K_ROWS, K_DYNAMICS, K_SHAPES, K_TREE, K_TRILIST, K_COUNT };
+
+/* HIP-event timing of the launches on the context stream; events are pooled
+ * and resolved at the per-step synchronisation */
+struct Prof
+{
+  bool on = false;
+  struct Pend { int kid; hipEvent_t a, b; double bytes; };
+  std::vector<hipEvent_t> pool;
+  std::vector<Pend> pend;
+  double ms[K_COUNT] = {0}, bytes[K_COUNT] = {0};
+  long long n[K_COUNT] = {0};
+  hipEvent_t get ()
+  {
+    if (!pool.empty ()) { hipEvent_t e = pool.back (); pool.pop_back (); return e; }
+    hipEvent_t e = nullptr;
+    (void) hipEventCreate (&e);
+    return e;
+  }
+};
+
 struct pmx_ctx
 {
   pmx_options opt {};
@@ -107,7 +129,31 @@ struct pmx_ctx
   int stepnum = 0;
   bool uploaded = false;
   pmx_stats stats {};
+  Prof prof;
 };
+
+/* bracket a launch with events when profiling; bytes = algorithmic HBM bytes */
+struct ProfScope
+{
+  pmx_ctx &c;
+  int kid;
+  double bytes;
+  hipEvent_t a = nullptr;
+  ProfScope (pmx_ctx &c_, int kid_, double bytes_) : c (c_), kid (kid_), bytes (bytes_)
+  {
+    if (c.prof.on) { a = c.prof.get (); (void) hipEventRecord (a, c.stream); }
+  }
+  ~ProfScope ()
+  {
+    if (!c.prof.on || !a) return;
+    hipEvent_t b = c.prof.get ();
+    (void) hipEventRecord (b, c.stream);
+    c.prof.pend.push_back ({kid, a, b, bytes});
+  }
+};
+
+/* resolve pending event pairs (call after the stream is synchronised) */
+void prof_resolve (pmx_ctx &c);
 
 /* ---- tree (pmx_tree.hip) ---- */
 int tree_build (pmx_ctx &c);                 /* from current centres */

@@ -534,13 +534,17 @@ int step_bin (pmx_ctx &c, int *overflow)
   if (c.lcount.alloc (L) || c.loff.alloc (L)) { pmx_set_error ("out of device memory (bins)"); return PMX_ENOMEM; }
   hipLaunchKernelGGL (k_fill_i, grid (L), dim3 (PMX_TPB), 0, c.stream, c.lcount.p, L, 0);
   hipLaunchKernelGGL (k_fill_i, dim3 (1), dim3 (64), 0, c.stream, c.dflags.p + FLAG_LEAF_OVERFLOW, 1, 0);
+  {
+  ProfScope ps (c, K_BIN, 32.0 * c.E);
   hipLaunchKernelGGL (k_bin, grid (c.E), dim3 (PMX_TPB), 0, c.stream, c.E, (const double*) c.center.p, t.root,
                       (const int*) t.left.p, (const int*) t.right.p, (const int*) t.dim.p, (const double*) t.coord.p,
                       c.leafof.p, c.slot.p, c.lcount.p, c.dflags.p);
+  }
   PMX_HIP (hipGetLastError ());
   int fl[FLAG_COUNT];
   PMX_HIP (hipMemcpyAsync (fl, c.dflags.p, sizeof (fl), hipMemcpyDeviceToHost, c.stream));
   PMX_HIP (hipStreamSynchronize (c.stream));
+  prof_resolve (c);
   *overflow = fl[FLAG_LEAF_OVERFLOW];
   if (fl[FLAG_ROW_OVERFLOW] || fl[FLAG_HIT_OVERFLOW] || fl[FLAG_EXTRA_OVERFLOW] || fl[FLAG_TRILIST_OVERFLOW])
   {
@@ -565,10 +569,14 @@ int step_detect (pmx_ctx &c)
   PMX_HIP (hipMemcpyAsync (c.loff.p, c.lcount.p, sizeof (int) * L, hipMemcpyDeviceToDevice, c.stream));
   int rc = scan_exclusive (c, c.loff.p, L, nullptr);
   if (rc) return rc;
+  {
+  ProfScope ps (c, K_SCATTER, 76.0 * c.E);
   hipLaunchKernelGGL (k_scatter, grid (c.E), dim3 (PMX_TPB), 0, c.stream, c.E, (const double*) c.center.p,
                       (const double*) c.radii.p, (const int*) c.leafof.p, (const int*) c.slot.p, (const int*) c.loff.p,
                       c.lmem.p, c.lpos.p);
+  }
   hipLaunchKernelGGL (k_fill_i, grid (c.E), dim3 (PMX_TPB), 0, c.stream, c.xcnt.p, c.E, 0);
+  ProfScope ps (c, K_DETECT, 68.0 * c.E);
   hipLaunchKernelGGL (k_detect, grid (c.E), dim3 (PMX_TPB), 0, c.stream, c.E, (const double*) c.center.p,
                       (const double*) c.radii.p, (const int*) c.part.p, t.root, (const int*) t.left.p,
                       (const int*) t.right.p, (const int*) t.dim.p, (const double*) t.coord.p, (const double*) t.cell.p,
@@ -585,10 +593,14 @@ int step_rows (pmx_ctx &c, const double *g_dev, const double *g, double h)
   (void) g;
   int kstep = 0;
   int o = c.cur, n = 1 - c.cur;
+  {
+  ProfScope ps (c, K_FORCE_INIT, 56.0 * c.N);
   hipLaunchKernelGGL (k_force_init, grid (c.N), dim3 (PMX_TPB), 0, c.stream, c.N, (const double*) c.mass.p, g_dev,
                       kstep, c.force.p, c.torque.p);
+  }
   if (c.E > 0)
   {
+    ProfScope ps (c, K_ROWS, 212.0 * c.E);
     size_t shm = (size_t) ROWS_TPB * (c.hcap + c.xcap) * sizeof (uint32_t);
     hipLaunchKernelGGL (k_rows, dim3 ((c.E + ROWS_TPB - 1) / ROWS_TPB), dim3 (ROWS_TPB), shm, c.stream, c.E, c.N,
                         (const double*) c.center.p, (const double*) c.radii.p, (const int*) c.part.p,
@@ -609,6 +621,7 @@ int step_rows (pmx_ctx &c, const double *g_dev, const double *g, double h)
 int step_dynamics (pmx_ctx &c, const double *d_dev, const double *d, double step0, double step1)
 {
   (void) d;
+  ProfScope ps (c, K_DYNAMICS, 512.0 * c.N);
   hipLaunchKernelGGL (k_dynamics, grid (c.N), dim3 (PMX_TPB), 0, c.stream, c.N, c.angular.p, c.linear.p, c.rotation.p,
                       c.position.p, (const double*) c.inertia.p, (const double*) c.inverse.p, (const double*) c.mass.p,
                       (const double*) c.invm.p, d_dev, 0, c.force.p, c.torque.p, step0, step1);
@@ -619,6 +632,7 @@ int step_dynamics (pmx_ctx &c, const double *d_dev, const double *d, double step
 int step_shapes (pmx_ctx &c)
 {
   if (c.E == 0) return PMX_OK;
+  ProfScope ps (c, K_SHAPES, 168.0 * c.E);
   hipLaunchKernelGGL (k_shapes, grid (c.E), dim3 (PMX_TPB), 0, c.stream, c.E, c.N, (const int*) c.part.p, c.center.p,
                       (const double*) c.rotation.p, (const double*) c.position.p);
   PMX_HIP (hipGetLastError ());

@@ -375,6 +375,12 @@ def DEM(duration, step, interval=None, prefix=None, adaptive=0.0):
     return elapsed
 
 
+def _use_engine(engine):
+    """Run DEM on a caller-supplied engine object (same methods as pmx.Engine)."""
+    _S.engine = engine
+    _S.dirty = True
+
+
 def state():
     """The SoA model (numpy arrays shaped like parmec.h's buffers)."""
     if _S.model is None:

@@ -60,8 +60,12 @@ class pmx_stats(C.Structure):
                 ("max_row", C.c_int), ("seconds", C.c_double)]
 
 
+class pmx_kernel_time(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("launches", C.c_longlong), ("ms", C.c_double), ("bytes", C.c_double)]
+
+
 EXPORTS = ["pmx_step_times", "pmx_create", "pmx_upload", "pmx_dem", "pmx_download", "pmx_contacts",
-           "pmx_get_stats", "pmx_destroy", "pmx_last_error"]
+           "pmx_get_stats", "pmx_destroy", "pmx_last_error", "pmx_set_profiling", "pmx_get_profile"]
 
 
 class PmxError(RuntimeError):

@@ -1,0 +1,133 @@
+"""Host-side tests: the Parmec input API mirror, the model layout and the
+C ABI of libpmx.so (no device calls)."""
+import ctypes as C
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+from parmec_amd import parmec as P
+from parmec_amd import pmx, workloads
+from tests.conftest import ROOT, gpu_available
+
+
+@pytest.fixture(autouse=True)
+def fresh():
+    P._S.reset()
+    yield
+    P._S.reset()
+
+
+def _deck_box(n=200):
+    m = workloads.box_lattice(n=n, seed=4)
+    mat = P.MATERIAL(2000.0, 1e6, 0.25)
+    for i in range(n):
+        P.SPHERE(tuple(m.center[:3, i]), 0.005, mat, 1)
+    P.OBSTACLE([tuple(t) for t in m.tri.T], 2)
+    P.GRANULAR(0, 0, 1e6, 0.1, 0.3)
+    P.GRAVITY(0.0, 0.0, -9.81)
+    return m
+
+
+def test_deck_builds_the_same_soa_as_the_generator():
+    ref = _deck_box()
+    m = P._build_model()
+    for name in ("position", "center", "radii", "mass", "inertia", "tri", "tricol", "triobs", "part", "ellcol",
+                 "iparam", "pairs", "ikind", "rotation"):
+        np.testing.assert_array_equal(getattr(m, name), getattr(ref, name), err_msg=name)
+    assert m.mass[0] == (4.0 / 3.0) * math.pi * 0.005 * 0.005 * 0.005 * 2000.0
+
+
+def test_argument_errors_follow_the_reference():
+    mat = P.MATERIAL(1000.0, 1e6, 0.2)
+    with pytest.raises(ValueError):
+        P.MATERIAL(-1.0, 1e6, 0.2)
+    with pytest.raises(ValueError):
+        P.SPHERE((0.0, 0.0, 0.0), -0.1, mat, 1)
+    with pytest.raises(ValueError):
+        P.SPHERE((0.0, 0.0, 0.0), 0.1, mat + 1, 1)
+    with pytest.raises(ValueError):
+        P.SPHERE((0.0, 0.0, 0.0), 0.1, mat, 0)
+    with pytest.raises(TypeError):
+        P.SPHERE((0.0, 0.0), 0.1, mat, 1)
+    with pytest.raises(ValueError):
+        P.GRANULAR(0, 1, 1e6)
+    with pytest.raises(ValueError):
+        P.GRAVITY(0, 0, -10)          # ints are TSERIES numbers (input.cpp:2857-2867)
+    ts = P.TSERIES([0.0, 0.0, 1.0, -10.0])
+    P.GRAVITY(0.0, 0.0, ts)
+    with pytest.raises(ValueError):
+        P.DEM(-1.0, 1e-3)
+    with pytest.raises(NotImplementedError):
+        P.MESH([], [], mat, 1)
+
+
+def test_tseries_interpolation_and_clamping():
+    ts = P._S
+    k = P.TSERIES([[0.0, 1.0], [1.0, 3.0], [3.0, -1.0]])
+    f = ts.tms[k]
+    assert f(-1.0) == 1.0 and f(5.0) == -1.0
+    assert f(0.5) == 2.0 and f(2.0) == 1.0
+    assert ts.tms[P.TSERIES(4.5)](10.0) == 4.5
+
+
+def test_dem_on_injected_engine_matches_direct_engine():
+    from oracle import orc
+    _deck_box(100)
+    P.VELOCITY(3, (0.5, 0.0, 0.0), (0.0, 0.0, 2.0))
+    P._use_engine(orc.Oracle(1))
+    P.DEM(50 * 1e-5 - 0.5e-5, 1e-5)
+    a = P.state().copy()
+    ref = workloads.box_lattice(n=100, seed=4)
+    ref.linear[:, 3] = (0.5, 0.0, 0.0)
+    ref.angular[3:, 3] = (0.0, 0.0, 2.0)
+    ref.angular[:3, 3] = (0.0, 0.0, 2.0)
+    o = orc.Oracle(1)
+    o.upload(ref)
+    o.dem(50 * 1e-5 - 0.5e-5, 1e-5, pmx.forcing(gravity=(0.0, 0.0, -9.81)))
+    o.download(ref)
+    for name in ("position", "linear", "angular", "rotation"):
+        np.testing.assert_array_equal(getattr(a, name), getattr(ref, name), err_msg=name)
+    # a second DEM call continues from curtime (parmec.cpp:2386-2409)
+    P.DEM(10 * 1e-5 - 0.5e-5, 1e-5)
+    assert abs(P.state().curtime - 60e-5) < 1e-15
+
+
+def test_gravity_callback_is_tabulated_at_step_times():
+    from oracle import orc
+    mat = P.MATERIAL(1000.0, 1e6, 0.2)
+    P.SPHERE((0.0, 0.0, 0.0), 0.1, mat, 1)
+    seen = []
+    P.GRAVITY(0.0, 0.0, lambda t: (seen.append(t), -9.81 * (1.0 + t))[1])
+    P._use_engine(orc.Oracle(1))
+    P.DEM(10 * 1e-3 - 0.5e-3, 1e-3)
+    assert len(seen) == 10
+    np.testing.assert_allclose(seen, np.arange(10) * 1e-3, atol=1e-15)
+
+
+def _declared_symbols():
+    text = open(os.path.join(ROOT, "include", "pmx.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char \*)\s*\*?\s*(pmx_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(pmx.LIBPATH)
+    syms = _declared_symbols()
+    assert len(syms) >= 11
+    for s in syms:
+        assert hasattr(lib, s), s
+
+
+def test_step_times_reproduce_the_reference_accumulation():
+    # parmec.cpp:2418: time += 0.5*(h+h) while time < duration
+    t = pmx.step_times(0.01, 1e-4)
+    assert len(t) == 101
+    assert len(pmx.step_times((500 - 0.5) * 2e-6, 2e-6)) == 500
+
+
+@pytest.mark.skipif(gpu_available(), reason="checks the failure path on a host without a device")
+def test_engine_fails_loudly_without_a_device():
+    with pytest.raises(pmx.PmxError):
+        pmx.Engine()
