@@ -760,7 +760,10 @@ int granular_force (const double n[3], const double vij[3], double depth, int i,
   ft[2] = kt*state[2] + et*vt[2];
   double len = std::sqrt (ft[0]*ft[0] + ft[1]*ft[1] + ft[2]*ft[2]);
   double lim = mus*fn;
-  if (len > lim)
+  /* the reference divides by len below even when the tangential force is
+   * exactly zero (f_n < 0 on a damped rebound): 0*(-inf) = NaN there.  With
+   * len == 0 the tangential force is the zero vector, which is kept. */
+  if (len > lim && len > 0.0)
   {
     lim = mud*fn/len;
     ft[0] *= lim;

@@ -128,7 +128,7 @@ void pmx_destroy (pmx_ctx *c)
   for (auto *p : d) p->release ();
   dvec<int> *ii[] = {&c->flags, &c->part, &c->ellcol, &c->tricol, &c->triobs, &c->pairs, &c->ikind, &c->leafof,
                      &c->slot, &c->lcount, &c->loff, &c->lmem, &c->rowcnt[0], &c->rowcnt[1], &c->hitcnt, &c->xbuf,
-                     &c->xcnt, &c->dflags, &c->tree.order, &c->tree.order2, &c->tree.hist, &c->tree.scanbuf,
+                     &c->xcnt, &c->dflags, &c->parell, &c->nosph, &c->tree.order, &c->tree.order2, &c->tree.hist, &c->tree.scanbuf,
                      &c->tree.first, &c->tree.size, &c->tree.split, &c->tree.dim, &c->tree.parent, &c->tree.left,
                      &c->tree.right, &c->tree.nleafkids, &c->tree.leafpar, &c->tree.tricnt, &c->tree.trioff,
                      &c->tree.trilist};
@@ -154,6 +154,13 @@ int pmx_upload (pmx_ctx *c, const pmx_model *m)
   }
   for (int t = 0; t < m->trinum; t++)
     if (m->triobs[t] != -1) return fail (PMX_EUNSUPPORTED, "only static obstacle triangles are supported");
+  std::vector<int> parell (std::max (m->parnum, 1), -1), nosph;
+  for (int e = 0; e < m->ellnum; e++)
+  {
+    if (parell[m->part[e]] >= 0) return fail (PMX_EUNSUPPORTED, "a particle may own at most one sphere on this path");
+    parell[m->part[e]] = e;
+  }
+  for (int i = 0; i < m->parnum; i++) if (parell[i] < 0) nosph.push_back (i);
   (void) hipSetDevice (c->device);
   bool same = c->uploaded && c->N == m->parnum && c->E == m->ellnum && c->T == m->trinum;
   c->N = m->parnum; c->E = m->ellnum; c->T = m->trinum; c->P = m->pairnum;
@@ -171,6 +178,11 @@ int pmx_upload (pmx_ctx *c, const pmx_model *m)
     return rc;
   if (c->force.alloc (3*std::max (N, (size_t) 1)) || c->torque.alloc (3*std::max (N, (size_t) 1)))
     return fail (PMX_ENOMEM, "out of device memory");
+  c->nnosph = (int) nosph.size ();
+  if ((rc = up (*c, c->parell, parell.data (), N, "parell")) ||
+      (rc = up (*c, c->nosph, nosph.data (), nosph.size (), "nosph")))
+    return rc;
+  c->bin_hint = false;
   if (m->force) PMX_HIP (hipMemcpyAsync (c->force.p, m->force, 3*N*sizeof (double), hipMemcpyHostToDevice, c->stream));
   if (m->torque) PMX_HIP (hipMemcpyAsync (c->torque.p, m->torque, 3*N*sizeof (double), hipMemcpyHostToDevice, c->stream));
   if (!same)
@@ -268,7 +280,6 @@ int pmx_dem (pmx_ctx *c, double duration, double step, const pmx_forcing *fc, do
     if ((rc = step_rows (*c, gk, nullptr, step0))) return rc;
     step1 = step0;
     if ((rc = step_dynamics (*c, dk, nullptr, step0, step1))) return rc;
-    if ((rc = step_shapes (*c))) return rc;
   }
   /* surface errors of the last step */
   if (c->E > 0)

@@ -216,7 +216,9 @@ __device__ __forceinline__ void granular (const double n[3], const double vij[3]
   double ft2 = kt*state[2] + et*vt2;
   double len = sqrt (ft0*ft0 + ft1*ft1 + ft2*ft2);
   double lim = mus*fn;
-  if (len > lim)
+  /* len == 0: the tangential force is the zero vector and stays so (the
+   * reference would divide by zero here when f_n < 0) */
+  if (len > lim && len > 0.0)
   {
     lim = mud*fn/len;
     ft0 *= lim;

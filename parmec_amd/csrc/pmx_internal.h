@@ -124,6 +124,10 @@ struct pmx_ctx
   dvec<int> xbuf;              /* [xcap][E] reverse hits */
   dvec<int> xcnt;              /* [E] */
   dvec<int> dflags;            /* [FLAG_COUNT] */
+  dvec<int> parell;            /* [N] the particle's sphere, or -1 */
+  dvec<int> nosph;             /* particles without a sphere */
+  int nnosph = 0;
+  bool bin_hint = false;       /* leafof holds leaves of the current tree */
 
   double curtime = 0.0, curstep = 0.0;
   int stepnum = 0;

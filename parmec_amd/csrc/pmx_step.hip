@@ -44,18 +44,34 @@ __global__ void k_fill_i (int *a, int n, int v)
 
 /* partitioning_store: descend to the leaf, take a slot (partition.ispc:600-646) */
 __global__ void k_bin (int E, const double *center, int root, const int *left, const int *right, const int *dim,
-                       const double *coord, int *leafof, int *slot, int *lcount, int *flags)
+                       const double *coord, const double *cell, int nleaves, int hint, int *leafof, int *slot,
+                       int *lcount, int *flags)
 {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= E) return;
   double c[3] = {center[e], center[(size_t) E + e], center[2*(size_t) E + e]};
-  int ref = root;
-  while (ref >= 0)
+  int L = -1;
+  if (hint)
   {
-    int d = dim[ref];
-    ref = c[d] < coord[ref] ? left[ref] : right[ref];
+    /* the descent "c[d] < coord -> left" ends in leaf L iff lower <= c < upper
+     * along every axis, with lower/upper the leaf's reach cell */
+    int h = leafof[e];
+    bool in = true;
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+      in = in && cell[(size_t) k*nleaves + h] <= c[k] && c[k] < cell[(size_t) (3+k)*nleaves + h];
+    if (in) L = h;
   }
-  int L = -ref-1;
+  if (L < 0)
+  {
+    int ref = root;
+    while (ref >= 0)
+    {
+      int d = dim[ref];
+      ref = c[d] < coord[ref] ? left[ref] : right[ref];
+    }
+    L = -ref-1;
+  }
   int s = atomicAdd (&lcount[L], 1);
   leafof[e] = L;
   slot[e] = s;
@@ -69,10 +85,9 @@ __global__ void k_scatter (int E, const double *center, const double *radii, con
   if (e >= E) return;
   int pos = loff[leafof[e]] + slot[e];
   lmem[pos] = e;
-  lpos[pos] = center[e];
-  lpos[(size_t) E + pos] = center[(size_t) E + e];
-  lpos[2*(size_t) E + pos] = center[2*(size_t) E + e];
-  lpos[3*(size_t) E + pos] = radii[e];
+  double2 *rec = (double2*) (lpos + 4*(size_t) pos);
+  rec[0] = make_double2 (center[e], center[(size_t) E + e]);
+  rec[1] = make_double2 (center[2*(size_t) E + e], radii[e]);
 }
 
 /* detection (test_ellipsoids + test_triangles), one thread per sphere in
@@ -123,11 +138,18 @@ void k_detect (int E, const double *center, const double *radii, const int *part
       int a = loff[L], b = a + lcount[L];
       for (int m = a; m < b; m++)
       {
-        int j = lmem[m];
-        double cx = lpos[m], cy = lpos[(size_t) E + m], cz = lpos[2*(size_t) E + m], rj = lpos[3*(size_t) E + m];
+        const double2 *rec = (const double2*) (lpos + 4*(size_t) m);
+        double2 r01 = rec[0], r23 = rec[1];
+        double cx = r01.x, cy = r01.y, cz = r23.x, rj = r23.y;
         double qx = p[0]-cx, qy = p[1]-cy, qz = p[2]-cz;
-        double len = sqrt (qx*qx + qy*qy + qz*qz);
-        double depth = rx + rj - len;
+        double d2 = qx*qx + qy*qy + qz*qz;
+        double sr = rx + rj;
+        /* d2 >= sr^2 (1 + 1e-10) implies the rounded sqrt(d2) >= sr, hence
+         * depth = sr - len <= 0: such pairs are skipped without the sqrt */
+        if (d2 >= sr * sr * 1.0000000001) continue;
+        double len = sqrt (d2);
+        double depth = sr - len;
+        int j = lmem[m];
         if (depth > 0.0 && pe != part[j])
         {
           if (hc < hcap) hits[(size_t) hc * E + e] = (uint32_t) j;
@@ -171,10 +193,12 @@ void k_detect (int E, const double *center, const double *radii, const int *part
   hitcnt[e] = hc < hcap ? hc : hcap;
 }
 
-__global__ void k_force_init (int N, const double *mass, const double *gtab, int k, double *force, double *torque)
+__global__ void k_force_init (int N, const int *list, int nlist, const double *mass, const double *gtab, int k,
+                              double *force, double *torque)
 {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nlist) return;
+  int i = list[q];
   double ma = mass[i];
   const double *g = gtab + 3*(size_t) k;
   force[i] = 0.0 + ma * g[0];
@@ -200,7 +224,9 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
 {
   extern __shared__ uint32_t sh[];
   int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= E) return;
+  int cnt = 0;
+  if (e < E)
+  {
   int hx = hcap + xcap;
   uint32_t *H = sh + (size_t) threadIdx.x * hx;
   int nh = hitcnt[e];
@@ -222,7 +248,7 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
   double pc[3] = {center[e], center[(size_t) E + e], center[2*(size_t) E + e]};
   double re = radii[e];
   int oc = ocnt[e];
-  int io = 0, ih = 0, cnt = 0;
+  int io = 0, ih = 0;
   double fs[3] = {0.0, 0.0, 0.0}, ts[3] = {0.0, 0.0, 0.0};
 
   while (io < oc || ih < nh)
@@ -366,7 +392,6 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
     ts[2] += a[0]*f[1] - a[1]*f[0];
   }
   ncnt[e] = cnt < cap ? cnt : cap;
-  atomicMax (&flags[FLAG_MAXROW], cnt);
   const double *g = gtab + 3*(size_t) kstep;
   double ma = mass[pe];
   force[pe] = fs[0] + ma * g[0];
@@ -375,12 +400,22 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
   torque[pe] = ts[0];
   torque[(size_t) N + pe] = ts[1];
   torque[2*(size_t) N + pe] = ts[2];
+  }
+  int mx = cnt;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1)
+  {
+    int o = __shfl_xor (mx, off, 64);
+    mx = o > mx ? o : mx;
+  }
+  if ((threadIdx.x & 63) == 0 && mx > 0) atomicMax (&flags[FLAG_MAXROW], mx);
 }
 
 /* dynamics_task, dynamics.ispc:30-169 */
 __global__ void k_dynamics (int N, double *angular, double *linear, double *rotation, double *position,
                             const double *inertia, const double *inverse, const double *mass, const double *invm,
-                            const double *dtab, int kstep, double *force, double *torque, double step0, double step1)
+                            const double *dtab, int kstep, double *force, double *torque, double step0, double step1,
+                            const int *parell, int E, double *center)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
@@ -445,15 +480,30 @@ __global__ void k_dynamics (int N, double *angular, double *linear, double *rota
 #pragma unroll
   for (int k = 0; k < 9; k++) rotation[(size_t) k*N + i] = L2[k];
   nvmul (L2, O, o);
-  position[i] += step*v[0];
-  position[(size_t) N + i] += step*v[1];
-  position[2*(size_t) N + i] += step*v[2];
+  double xn[3];
+  xn[0] = position[i] + step*v[0];
+  xn[1] = position[(size_t) N + i] + step*v[1];
+  xn[2] = position[2*(size_t) N + i] + step*v[2];
+  position[i] = xn[0];
+  position[(size_t) N + i] = xn[1];
+  position[2*(size_t) N + i] = xn[2];
 #pragma unroll
   for (int k = 0; k < 3; k++)
   {
     angular[(size_t) k*N + i] = O[k];
     angular[(size_t) (3+k)*N + i] = o[k];
     linear[(size_t) k*N + i] = v[k];
+  }
+  /* ellipsoids_task for the particle's sphere, shapes.ispc:29-67 */
+  int e = parell ? parell[i] : -1;
+  if (e >= 0)
+  {
+    double C[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) C[k] = center[(size_t) (3+k)*E + e] - position[(size_t) (3+k)*N + i];
+    center[e] = xn[0] + (L2[0]*C[0] + L2[3]*C[1] + L2[6]*C[2]);
+    center[(size_t) E + e] = xn[1] + (L2[1]*C[0] + L2[4]*C[1] + L2[7]*C[2]);
+    center[2*(size_t) E + e] = xn[2] + (L2[2]*C[0] + L2[5]*C[1] + L2[8]*C[2]);
   }
 }
 
@@ -538,7 +588,8 @@ int step_bin (pmx_ctx &c, int *overflow)
   ProfScope ps (c, K_BIN, 32.0 * c.E);
   hipLaunchKernelGGL (k_bin, grid (c.E), dim3 (PMX_TPB), 0, c.stream, c.E, (const double*) c.center.p, t.root,
                       (const int*) t.left.p, (const int*) t.right.p, (const int*) t.dim.p, (const double*) t.coord.p,
-                      c.leafof.p, c.slot.p, c.lcount.p, c.dflags.p);
+                      (const double*) t.cell.p, t.nleaves, c.bin_hint ? 1 : 0, c.leafof.p, c.slot.p, c.lcount.p,
+                      c.dflags.p);
   }
   PMX_HIP (hipGetLastError ());
   int fl[FLAG_COUNT];
@@ -546,6 +597,7 @@ int step_bin (pmx_ctx &c, int *overflow)
   PMX_HIP (hipStreamSynchronize (c.stream));
   prof_resolve (c);
   *overflow = fl[FLAG_LEAF_OVERFLOW];
+  c.bin_hint = true;
   if (fl[FLAG_ROW_OVERFLOW] || fl[FLAG_HIT_OVERFLOW] || fl[FLAG_EXTRA_OVERFLOW] || fl[FLAG_TRILIST_OVERFLOW])
   {
     pmx_set_error ("contact capacity exceeded (row " + std::to_string (fl[FLAG_ROW_OVERFLOW]) + ", hits " +
@@ -593,10 +645,11 @@ int step_rows (pmx_ctx &c, const double *g_dev, const double *g, double h)
   (void) g;
   int kstep = 0;
   int o = c.cur, n = 1 - c.cur;
+  if (c.nnosph > 0)
   {
-  ProfScope ps (c, K_FORCE_INIT, 56.0 * c.N);
-  hipLaunchKernelGGL (k_force_init, grid (c.N), dim3 (PMX_TPB), 0, c.stream, c.N, (const double*) c.mass.p, g_dev,
-                      kstep, c.force.p, c.torque.p);
+    ProfScope ps (c, K_FORCE_INIT, 60.0 * c.nnosph);
+    hipLaunchKernelGGL (k_force_init, grid (c.nnosph), dim3 (PMX_TPB), 0, c.stream, c.N, (const int*) c.nosph.p,
+                        c.nnosph, (const double*) c.mass.p, g_dev, kstep, c.force.p, c.torque.p);
   }
   if (c.E > 0)
   {
@@ -621,10 +674,11 @@ int step_rows (pmx_ctx &c, const double *g_dev, const double *g, double h)
 int step_dynamics (pmx_ctx &c, const double *d_dev, const double *d, double step0, double step1)
 {
   (void) d;
-  ProfScope ps (c, K_DYNAMICS, 512.0 * c.N);
+  ProfScope ps (c, K_DYNAMICS, 512.0 * c.N + 56.0 * c.E);
   hipLaunchKernelGGL (k_dynamics, grid (c.N), dim3 (PMX_TPB), 0, c.stream, c.N, c.angular.p, c.linear.p, c.rotation.p,
                       c.position.p, (const double*) c.inertia.p, (const double*) c.inverse.p, (const double*) c.mass.p,
-                      (const double*) c.invm.p, d_dev, 0, c.force.p, c.torque.p, step0, step1);
+                      (const double*) c.invm.p, d_dev, 0, c.force.p, c.torque.p, step0, step1,
+                      (const int*) c.parell.p, c.E, c.center.p);
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
 }

@@ -443,6 +443,7 @@ int tree_build (pmx_ctx &c)
   TreeDev &t = c.tree;
   int n = c.E;
   t.n = n;
+  c.bin_hint = false;
   if (n == 0) { t.nleaves = 0; t.root = -1; return PMX_OK; }
   const double *cx = c.center.p, *cy = c.center.p + n, *cz = c.center.p + 2*(size_t) n;
   int nbext = std::min (1024, (n + PMX_TPB - 1) / PMX_TPB);

@@ -58,3 +58,27 @@ def test_hopper_small(engines):
     (mg, cg, sg), (mc, cc, sc) = _both(engines, m, 200 * 2e-6, 2e-6, f)
     assert len(cc) > 50
     compare(mg, cg, mc, cc)
+
+
+def test_box10k_full_run(engines):
+    """10,000 equal spheres (r = 5 mm, 2000 kg/m^3) on a 25x20x20 jittered
+    lattice (spacing 2.2r, jitter +-0.1r) in a 6-plane box, kn = 1e6,
+    damping ratio 0.1, mu = 0.3, gravity -9.81 z, dt = 1e-5, 2,000 steps."""
+    print("case box10k steps=2000", flush=True)
+    m = workloads.box_lattice(n=10000, seed=0)
+    f = pmx.forcing(gravity=(0.0, 0.0, -9.81))
+    (mg, cg, sg), (mc, cc, sc) = _both(engines, m, 2000 * 1e-5 - 0.5e-5, 1e-5, f)
+    assert sg["steps"] == sc["steps"] == 2000
+    assert len(cc) > 20
+    compare(mg, cg, mc, cc)
+
+
+def test_hopper_2m_short(engines):
+    """The 2,000,000-sphere hopper at full size, 6 steps (the oracle's cost
+    bounds the length)."""
+    print("case hopper n=2000000 steps=6", flush=True)
+    m = workloads.hopper(n=2000000, seed=1)
+    f = pmx.forcing(gravity=(0.0, 0.0, -9.81))
+    (mg, cg, sg), (mc, cc, sc) = _both(engines, m, 6 * 2e-6 - 1e-6, 2e-6, f)
+    assert sg["steps"] == 6
+    compare(mg, cg, mc, cc)

@@ -269,3 +269,44 @@ def test_sliding_sphere_rolls_at_five_sevenths():
     assert abs(mm.linear[0, 0] - 5.0 / 7.0 * v0) < 0.02 * v0
     # rolling without slip: omega_y * r = v
     assert abs(mm.angular[4, 0] * r - mm.linear[0, 0]) < 0.02 * v0
+
+
+@pytest.mark.parametrize("mu", [0.0, 0.3])
+def test_rebound_from_floor(mu):
+    """Normal impact on a floor triangle.  mu = 0: the rebound speed follows
+    e = exp(-zeta*pi/sqrt(1-zeta^2)) for the obstacle's effective mass m.
+    mu > 0: f_n turns negative before separation and the reference's friction
+    cap (forces.ispc:170-178) rescales whatever tangential force is left to
+    mu*f_n; the in-plane velocity stays exactly zero and nothing overflows."""
+    r, rho, kn, zeta = 0.01, 2000.0, 1e6, 0.3
+    # closest point (0.25, 0.25, 0) is exact in this triangle
+    m = _one_sphere(r=r, rho=rho, x=(0.25, 0.25, r * 1.001), v=(0.0, 0.0, -0.2))
+    m.trinum = 1
+    m.tri = np.array([[0.0], [0.0], [0.0], [1.0], [0.0], [0.0], [0.0], [1.0], [0.0]])
+    m.tricol = np.full(1, 2, dtype=np.int32)
+    m.triobs = np.full(1, -1, dtype=np.int32)
+    m.iparam[pmx.SPRING, 0], m.iparam[pmx.DAMPER, 0] = kn, zeta
+    m.iparam[pmx.FRISTAT, 0] = m.iparam[pmx.FRIDYN, 0] = mu
+    wn = math.sqrt(kn / m.mass[0])
+    h = 0.02 * 2.0 / wn
+    mm, o = _oracle_run(m, 4.0 * math.pi / wn, h)
+    assert np.isfinite(mm.position).all() and np.isfinite(mm.linear).all()
+    assert mm.linear[0, 0] == 0.0 and mm.linear[1, 0] == 0.0
+    if mu == 0.0:
+        e = math.exp(-zeta * math.pi / math.sqrt(1 - zeta * zeta))
+        assert abs(mm.linear[2, 0] / 0.2 - e) < 0.02 * e
+
+
+def test_box10k_full_run_stays_finite():
+    """The 10,000-sphere box run (2,000 steps): spheres rebound off the
+    axis-aligned walls with exactly zero tangential force, where the
+    reference's cap divides by |f_t| = 0; every state must stay finite."""
+    from oracle import orc
+    m = workloads.box_lattice(n=10000, seed=0)
+    o = orc.Oracle(0)
+    o.upload(m)
+    o.dem(2000 * 1e-5 - 0.5e-5, 1e-5, pmx.forcing(gravity=(0.0, 0.0, -9.81)))
+    o.download(m)
+    for name in ("position", "linear", "angular", "rotation", "force", "torque"):
+        assert np.isfinite(getattr(m, name)).all(), name
+    assert len(o.contacts()) > 20
