@@ -1154,4 +1154,41 @@ long long orc_tree_pairs (int n, const double *center, const double *radius, int
   return m;
 }
 
+/* broad-phase statistics of a fresh tree: [0] leaves, [1] mean leaves reached
+ * per sphere AABB, [2] mean leaf tests per sphere, [3] mean candidate leaves
+ * of a leaf's union box, [4] max candidates, [5] mean spheres per leaf */
+void orc_tree_stats (int n, const double *center, const double *radius, double *out)
+{
+  Tree T;
+  const double *cx = center, *cy = center + n, *cz = center + 2*(size_t) n;
+  tree_create (n, cx, cy, cz, T);
+  tree_store (T, n, cx, cy, cz);
+  double reach = 0, tests = 0;
+  size_t L = T.leaf.size ();
+  std::vector<double> blo (3*L, 1e300), bhi (3*L, -1e300);
+  std::vector<int> lof (n);
+  for (size_t l = 0; l < L; l++) for (int j : T.leaf[l]) lof[j] = (int) l;
+  for (int i = 0; i < n; i++)
+  {
+    double p[3] = {cx[i], cy[i], cz[i]}, rx = radius[i];
+    double lo[3] = {p[0]-rx, p[1]-rx, p[2]-rx}, hi[3] = {p[0]+rx, p[1]+rx, p[2]+rx};
+    std::vector<int> leaves;
+    tree_reach (T, 0, lo, hi, leaves);
+    reach += leaves.size ();
+    for (int l : leaves) tests += T.leaf[l].size ();
+    int l = lof[i];
+    for (int k = 0; k < 3; k++) { blo[3*l+k] = std::min (blo[3*l+k], lo[k]); bhi[3*l+k] = std::max (bhi[3*l+k], hi[k]); }
+  }
+  double cand = 0, cmax = 0;
+  for (size_t l = 0; l < L; l++)
+  {
+    if (T.leaf[l].empty ()) continue;
+    std::vector<int> leaves;
+    tree_reach (T, 0, &blo[3*l], &bhi[3*l], leaves);
+    cand += leaves.size ();
+    cmax = std::max (cmax, (double) leaves.size ());
+  }
+  out[0] = (double) L; out[1] = reach / n; out[2] = tests / n; out[3] = cand / L; out[4] = cmax; out[5] = (double) n / L;
+}
+
 } /* extern "C" */

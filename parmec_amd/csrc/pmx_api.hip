@@ -13,6 +13,7 @@
 #include "pmx_internal.h"
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 
 static thread_local std::string g_last_error;
@@ -131,9 +132,9 @@ void pmx_destroy (pmx_ctx *c)
                      &c->xcnt, &c->dflags, &c->parell, &c->nosph, &c->tree.order, &c->tree.order2, &c->tree.hist, &c->tree.scanbuf,
                      &c->tree.first, &c->tree.size, &c->tree.split, &c->tree.dim, &c->tree.parent, &c->tree.left,
                      &c->tree.right, &c->tree.nleafkids, &c->tree.leafpar, &c->tree.tricnt, &c->tree.trioff,
-                     &c->tree.trilist};
+                     &c->tree.trilist, &c->tree.nbrcnt, &c->tree.nbroff, &c->tree.nbrlist};
   for (auto *p : ii) p->release ();
-  c->rowkey[0].release (); c->rowkey[1].release (); c->hits.release ();
+  c->rowkey[0].release (); c->rowkey[1].release (); c->hits.release (); c->iso.release ();
   c->tree.code.release (); c->tree.code2.release ();
   if (c->stream) (void) hipStreamDestroy (c->stream);
   delete c;
@@ -155,12 +156,26 @@ int pmx_upload (pmx_ctx *c, const pmx_model *m)
   for (int t = 0; t < m->trinum; t++)
     if (m->triobs[t] != -1) return fail (PMX_EUNSUPPORTED, "only static obstacle triangles are supported");
   std::vector<int> parell (std::max (m->parnum, 1), -1), nosph;
+  double rmax = 0.0;
+  for (int e = 0; e < m->ellnum; e++)
+    for (int k = 0; k < 3; k++) rmax = std::max (rmax, m->radii[(size_t) k*m->ellnum + e]);
   for (int e = 0; e < m->ellnum; e++)
   {
     if (parell[m->part[e]] >= 0) return fail (PMX_EUNSUPPORTED, "a particle may own at most one sphere on this path");
     parell[m->part[e]] = e;
   }
   for (int i = 0; i < m->parnum; i++) if (parell[i] < 0) nosph.push_back (i);
+  std::vector<signed char> iso (std::max (m->parnum, 1), 0);
+  int niso = 0;
+  for (int i = 0; i < m->parnum; i++)
+  {
+    const double *J = m->inertia;
+    size_t N = m->parnum;
+    bool d = J[i] > 0.0 && J[i] == J[4*N + i] && J[i] == J[8*N + i];
+    for (int k : {1, 2, 3, 5, 6, 7}) d = d && J[k*N + i] == 0.0 && !std::signbit (J[k*N + i]);
+    iso[i] = d ? 1 : 0;
+    niso += d;
+  }
   (void) hipSetDevice (c->device);
   bool same = c->uploaded && c->N == m->parnum && c->E == m->ellnum && c->T == m->trinum;
   c->N = m->parnum; c->E = m->ellnum; c->T = m->trinum; c->P = m->pairnum;
@@ -179,10 +194,13 @@ int pmx_upload (pmx_ctx *c, const pmx_model *m)
   if (c->force.alloc (3*std::max (N, (size_t) 1)) || c->torque.alloc (3*std::max (N, (size_t) 1)))
     return fail (PMX_ENOMEM, "out of device memory");
   c->nnosph = (int) nosph.size ();
+  c->niso = niso;
+  if ((rc = up (*c, c->iso, iso.data (), N, "iso"))) return rc;
   if ((rc = up (*c, c->parell, parell.data (), N, "parell")) ||
       (rc = up (*c, c->nosph, nosph.data (), nosph.size (), "nosph")))
     return rc;
   c->bin_hint = false;
+  c->rmax = rmax;
   if (m->force) PMX_HIP (hipMemcpyAsync (c->force.p, m->force, 3*N*sizeof (double), hipMemcpyHostToDevice, c->stream));
   if (m->torque) PMX_HIP (hipMemcpyAsync (c->torque.p, m->torque, 3*N*sizeof (double), hipMemcpyHostToDevice, c->stream));
   if (!same)
@@ -250,7 +268,7 @@ int pmx_dem (pmx_ctx *c, double duration, double step, const pmx_forcing *fc, do
   if (c->E > 0)
   {
     ProfScope ps (*c, K_TREE, 0.0);
-    if ((rc = tree_build (*c)) || (rc = tree_trilists (*c))) return rc;
+    if ((rc = tree_build (*c)) || (rc = tree_trilists (*c)) || (rc = tree_nbrlists (*c))) return rc;
     rebuilds++;
   }
   PMX_HIP (hipMemsetAsync (c->dflags.p, 0, FLAG_COUNT * sizeof (int), c->stream));
@@ -267,7 +285,7 @@ int pmx_dem (pmx_ctx *c, double duration, double step, const pmx_forcing *fc, do
       {
         {
           ProfScope ps (*c, K_TREE, 0.0);
-          if ((rc = tree_build (*c)) || (rc = tree_trilists (*c))) return rc;
+          if ((rc = tree_build (*c)) || (rc = tree_trilists (*c)) || (rc = tree_nbrlists (*c))) return rc;
         }
         rebuilds++;
         if ((rc = step_bin (*c, &overflow))) return rc;

@@ -63,6 +63,8 @@ struct TreeDev
   dvec<double> cell;         /* [6][nleaves]: lower bounds 0-2, upper bounds 3-5 */
   dvec<int> tricnt, trioff, trilist;  /* leaf -> triangles whose AABB reaches it */
   int ntrilist = 0;
+  dvec<int> nbrcnt, nbroff, nbrlist;  /* leaf -> leaves overlapping its cell inflated by rmax */
+  int nnbr = 0;
 };
 
 /* kernels timed by the profiler */
@@ -126,8 +128,11 @@ struct pmx_ctx
   dvec<int> dflags;            /* [FLAG_COUNT] */
   dvec<int> parell;            /* [N] the particle's sphere, or -1 */
   dvec<int> nosph;             /* particles without a sphere */
+  dvec<signed char> iso;       /* [N] 1 when inertia is diag(j, j, j) with exact zeros */
+  int niso = 0;
   int nnosph = 0;
   bool bin_hint = false;       /* leafof holds leaves of the current tree */
+  double rmax = 0.0;           /* largest sphere AABB half-width */
 
   double curtime = 0.0, curstep = 0.0;
   int stepnum = 0;
@@ -162,6 +167,7 @@ void prof_resolve (pmx_ctx &c);
 /* ---- tree (pmx_tree.hip) ---- */
 int tree_build (pmx_ctx &c);                 /* from current centres */
 int tree_trilists (pmx_ctx &c);              /* leaf -> triangle lists */
+int tree_nbrlists (pmx_ctx &c);              /* leaf -> neighbour leaf lists */
 
 /* ---- scans (pmx_tree.hip) ---- */
 int scan_exclusive (pmx_ctx &c, int *data, int n, int *total_host);

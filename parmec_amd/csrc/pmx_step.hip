@@ -48,31 +48,50 @@ __global__ void k_bin (int E, const double *center, int root, const int *left, c
                        int *lcount, int *flags)
 {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= E) return;
-  double c[3] = {center[e], center[(size_t) E + e], center[2*(size_t) E + e]};
+  bool valid = e < E;
   int L = -1;
-  if (hint)
+  if (valid)
   {
-    /* the descent "c[d] < coord -> left" ends in leaf L iff lower <= c < upper
-     * along every axis, with lower/upper the leaf's reach cell */
-    int h = leafof[e];
-    bool in = true;
-#pragma unroll
-    for (int k = 0; k < 3; k++)
-      in = in && cell[(size_t) k*nleaves + h] <= c[k] && c[k] < cell[(size_t) (3+k)*nleaves + h];
-    if (in) L = h;
-  }
-  if (L < 0)
-  {
-    int ref = root;
-    while (ref >= 0)
+    double c[3] = {center[e], center[(size_t) E + e], center[2*(size_t) E + e]};
+    if (hint)
     {
-      int d = dim[ref];
-      ref = c[d] < coord[ref] ? left[ref] : right[ref];
+      /* the descent "c[d] < coord -> left" ends in leaf h iff lower <= c < upper
+       * along every axis, with lower/upper the leaf's reach cell */
+      int h = leafof[e];
+      bool in = true;
+#pragma unroll
+      for (int k = 0; k < 3; k++)
+        in = in && cell[(size_t) k*nleaves + h] <= c[k] && c[k] < cell[(size_t) (3+k)*nleaves + h];
+      if (in) L = h;
     }
-    L = -ref-1;
+    if (L < 0)
+    {
+      int ref = root;
+      while (ref >= 0)
+      {
+        int d = dim[ref];
+        ref = c[d] < coord[ref] ? left[ref] : right[ref];
+      }
+      L = -ref-1;
+    }
   }
-  int s = atomicAdd (&lcount[L], 1);
+  /* slot = leaf count before this sphere: one atomic per distinct leaf of the wave */
+  int lane = threadIdx.x & 63;
+  unsigned long long todo = __ballot (valid);
+  unsigned long long lt = lane ? ((1ull << lane) - 1ull) : 0ull;
+  int s = 0;
+  while (todo)
+  {
+    int leader = __ffsll ((long long) todo) - 1;
+    int Lw = __shfl (L, leader, 64);
+    unsigned long long grp = __ballot (valid && L == Lw) & todo;
+    int base = 0;
+    if (lane == leader) base = atomicAdd (&lcount[Lw], __popcll (grp));
+    base = __shfl (base, leader, 64);
+    if ((grp >> lane) & 1ull) s = base + __popcll (grp & lt);
+    todo &= ~grp;
+  }
+  if (!valid) return;
   leafof[e] = L;
   slot[e] = s;
   if (s >= PMX_LSIZE) atomicAdd (&flags[FLAG_LEAF_OVERFLOW], 1);
@@ -90,107 +109,186 @@ __global__ void k_scatter (int E, const double *center, const double *radii, con
   rec[1] = make_double2 (center[2*(size_t) E + e], radii[e]);
 }
 
-/* detection (test_ellipsoids + test_triangles), one thread per sphere in
- * leaf order */
-__global__ __launch_bounds__(PMX_TPB)
+#define DET_TPB 128     /* >= PMX_LSIZE: one sphere of the leaf per thread slot */
+#define DET_CMAX 256    /* candidate leaves of one block */
+
+/* Detection (test_ellipsoids + test_triangles, condet.ispc:98-516), one block
+ * per leaf Q.  A sphere reaches leaf L through the reference descent iff its
+ * AABB overlaps L's cell.  The block collects the leaves reached by the union
+ * of its spheres' AABBs once; for each candidate leaf it stages the leaf's
+ * records in LDS, lists the block's spheres whose own AABB reaches it, and
+ * spreads the (sphere, record) tests over all threads.  Hit slots per sphere
+ * come from LDS counters; the hit order is irrelevant (rows sort their keys). */
+__global__ __launch_bounds__(DET_TPB)
 void k_detect (int E, const double *center, const double *radii, const int *part, int root, const int *left,
                const int *right, const int *dim, const double *coord, const double *cell, int nleaves,
-               const int *leafof, const int *lcount, const int *loff, const int *lmem, const double *lpos,
+               const int *lcount, const int *loff, const int *lmem, const double *lpos,
                const double *tri, int T, const int *tricnt, const int *trioff, const int *trilist,
+               const int *nbrcnt, const int *nbroff, const int *nbrlist,
                uint32_t *hits, int *hitcnt, int hcap, int *xbuf, int *xcnt, int xcap, int *flags)
 {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= E) return;
-  int e = lmem[k];
-  double p[3] = {center[e], center[(size_t) E + e], center[2*(size_t) E + e]};
-  double r0 = radii[e], r1 = radii[(size_t) E + e], r2 = radii[2*(size_t) E + e];
-  double rx = r0 > r1 ? r0 : r1;
-  rx = rx > r2 ? rx : r2;
-  double lo[3] = {p[0]-rx, p[1]-rx, p[2]-rx};
-  double hi[3] = {p[0]+rx, p[1]+rx, p[2]+rx};
-  int pe = part[e];
-  int me = leafof[e];
-  double cl[3] = {cell[me], cell[(size_t) nleaves + me], cell[2*(size_t) nleaves + me]};
-  double cu[3] = {cell[3*(size_t) nleaves + me], cell[4*(size_t) nleaves + me], cell[5*(size_t) nleaves + me]};
-  int hc = 0;
-
-  int stack[64];
-  int sp = 0, ref = root;
-  for (;;)
+  __shared__ double srec[4*DET_TPB];   /* staged records of the candidate leaf */
+  __shared__ int smem[DET_TPB];
+  __shared__ double ssph[4*DET_TPB];   /* block spheres: x, y, z, rx */
+  __shared__ int sidx[DET_TPB], spart[DET_TPB], shc[DET_TPB], slist[DET_TPB];
+  __shared__ int nsl;
+  int Q = blockIdx.x;
+  int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  int qoff = loff[Q], qcnt = lcount[Q];
+  bool valid = t < qcnt;
+  double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+  double r0 = 0.0;
+  if (valid)
   {
-    if (ref >= 0)
+    int e = lmem[qoff + t];
+    double p0 = center[e], p1 = center[(size_t) E + e], p2 = center[2*(size_t) E + e];
+    r0 = radii[e];
+    double r1 = radii[(size_t) E + e], r2 = radii[2*(size_t) E + e];
+    double rx = r0 > r1 ? r0 : r1;
+    rx = rx > r2 ? rx : r2;
+    lo[0] = p0-rx; lo[1] = p1-rx; lo[2] = p2-rx;
+    hi[0] = p0+rx; hi[1] = p1+rx; hi[2] = p2+rx;
+    ssph[4*t] = p0; ssph[4*t+1] = p1; ssph[4*t+2] = p2; ssph[4*t+3] = rx;
+    sidx[t] = e;
+    spart[t] = part[e];
+  }
+  shc[t] = 0;
+  double cl[3] = {cell[Q], cell[(size_t) nleaves + Q], cell[2*(size_t) nleaves + Q]};
+  double cu[3] = {cell[3*(size_t) nleaves + Q], cell[4*(size_t) nleaves + Q], cell[5*(size_t) nleaves + Q]};
+
+  __syncthreads ();
+  int nc = nbrcnt[Q], c0 = nbroff[Q];
+  for (int ci = 0; ci < nc; ci++)
+  {
+    int L = nbrlist[c0 + ci];
+    /* spheres of this block whose own AABB reaches L (condet.ispc:104-113) */
+    bool reach = valid && lo[0] <= cell[3*(size_t) nleaves + L] && lo[1] <= cell[4*(size_t) nleaves + L] &&
+                 lo[2] <= cell[5*(size_t) nleaves + L] && hi[0] >= cell[L] && hi[1] >= cell[(size_t) nleaves + L] &&
+                 hi[2] >= cell[2*(size_t) nleaves + L];
+    int off = loff[L], cnt = lcount[L];
+    if (!__syncthreads_or (reach && cnt > 0)) continue;
+    if (t < cnt)
     {
-      int d = dim[ref];
-      double c = coord[ref];
-      bool gl = lo[d] <= c, gr = hi[d] >= c;
-      if (gl && gr)
-      {
-        if (sp < 64) stack[sp++] = right[ref]; else atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1);
-        ref = left[ref];
-        continue;
-      }
-      if (gl) { ref = left[ref]; continue; }
-      if (gr) { ref = right[ref]; continue; }
+      const double2 *rec = (const double2*) (lpos + 4*(size_t) (off + t));
+      double2 a = rec[0], c = rec[1];
+      srec[4*t] = a.x; srec[4*t+1] = a.y; srec[4*t+2] = c.x; srec[4*t+3] = c.y;
+      smem[t] = lmem[off + t];
     }
-    else
+    if (t == 0) nsl = 0;
+    __syncthreads ();
+    unsigned long long rm = __ballot (reach);
+    int wbase = 0;
+    if (lane == 0 && rm) wbase = atomicAdd (&nsl, __popcll (rm));
+    wbase = __shfl (wbase, 0, 64);
+    if (reach)
     {
-      int L = -ref-1;
-      int a = loff[L], b = a + lcount[L];
-      for (int m = a; m < b; m++)
+      unsigned long long lt = lane ? ((1ull << lane) - 1ull) : 0ull;
+      slist[wbase + __popcll (rm & lt)] = t;
+    }
+    __syncthreads ();
+    if (L == Q)
+    {
+      /* own leaf: every sphere reaches it and the test is symmetric
+       * (depth r_a + r_b - |c_a - c_b|), so each unordered pair is tested
+       * once: pairs (a, (a+k) mod n), k = 1..(n-1)/2, plus k = n/2, a < n/2 */
+      int n = cnt, half = (n - 1) / 2;
+      int work = n * half + ((n & 1) ? 0 : n / 2);
+      unsigned inv = ((1u << 20) + (unsigned) n - 1u) / (unsigned) n;
+      for (int q = t; q < work; q += DET_TPB)
       {
-        const double2 *rec = (const double2*) (lpos + 4*(size_t) m);
-        double2 r01 = rec[0], r23 = rec[1];
-        double cx = r01.x, cy = r01.y, cz = r23.x, rj = r23.y;
-        double qx = p[0]-cx, qy = p[1]-cy, qz = p[2]-cz;
+        int k = (int) (((unsigned) q * inv) >> 20), a = q - k * n;
+        int bb = a + k + 1;
+        bb = bb >= n ? bb - n : bb;
+        double qx = ssph[4*a]-ssph[4*bb], qy = ssph[4*a+1]-ssph[4*bb+1], qz = ssph[4*a+2]-ssph[4*bb+2];
         double d2 = qx*qx + qy*qy + qz*qz;
-        double sr = rx + rj;
-        /* d2 >= sr^2 (1 + 1e-10) implies the rounded sqrt(d2) >= sr, hence
-         * depth = sr - len <= 0: such pairs are skipped without the sqrt */
+        double sr = ssph[4*a+3] + srec[4*bb+3];
         if (d2 >= sr * sr * 1.0000000001) continue;
         double len = sqrt (d2);
         double depth = sr - len;
-        int j = lmem[m];
-        if (depth > 0.0 && pe != part[j])
+        if (depth > 0.0 && spart[a] != spart[bb])
         {
-          if (hc < hcap) hits[(size_t) hc * E + e] = (uint32_t) j;
-          else atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1);
-          hc++;
-          /* does the partner's AABB reach this sphere's leaf? */
-          double ljx = cx-rj, ljy = cy-rj, ljz = cz-rj, hjx = cx+rj, hjy = cy+rj, hjz = cz+rj;
-          bool back = ljx <= cu[0] && ljy <= cu[1] && ljz <= cu[2] && hjx >= cl[0] && hjy >= cl[1] && hjz >= cl[2];
-          if (!back)
-          {
-            int s = atomicAdd (&xcnt[j], 1);
-            if (s < xcap) xbuf[(size_t) s * E + j] = e;
-            else atomicAdd (&flags[FLAG_EXTRA_OVERFLOW], 1);
-          }
+          int ea = sidx[a], eb = sidx[bb];
+          int sa = atomicAdd (&shc[a], 1), sb = atomicAdd (&shc[bb], 1);
+          if (sa < hcap) hits[(size_t) sa * E + ea] = (uint32_t) eb; else atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1);
+          if (sb < hcap) hits[(size_t) sb * E + eb] = (uint32_t) ea; else atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1);
         }
       }
     }
-    if (sp == 0) break;
-    ref = stack[--sp];
+    else
+    {
+    int work = nsl * cnt;
+    /* a = q / cnt exactly for q * cnt < 2^20 (q < 96*96, cnt <= 96) */
+    unsigned inv = ((1u << 20) + (unsigned) cnt - 1u) / (unsigned) cnt;
+    for (int q = t; q < work; q += DET_TPB)
+    {
+      int a = (int) (((unsigned) q * inv) >> 20), m = q - a * cnt;
+      int ti = slist[a];
+      double px = ssph[4*ti], py = ssph[4*ti+1], pz = ssph[4*ti+2], rx = ssph[4*ti+3];
+      double cx = srec[4*m], cy = srec[4*m+1], cz = srec[4*m+2], rj = srec[4*m+3];
+      double qx = px-cx, qy = py-cy, qz = pz-cz;
+      double d2 = qx*qx + qy*qy + qz*qz;
+      double sr = rx + rj;
+      /* d2 >= sr^2 (1 + 1e-10) implies the rounded sqrt(d2) >= sr, hence
+       * depth = sr - len <= 0: such pairs are skipped without the sqrt */
+      if (d2 >= sr * sr * 1.0000000001) continue;
+      double len = sqrt (d2);
+      double depth = sr - len;
+      int j = smem[m];
+      if (depth > 0.0 && spart[ti] != part[j])
+      {
+        int e = sidx[ti];
+        int slot = atomicAdd (&shc[ti], 1);
+        if (slot < hcap) hits[(size_t) slot * E + e] = (uint32_t) j;
+        else atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1);
+        /* does the partner's AABB reach this sphere's leaf Q? */
+        double ljx = cx-rj, ljy = cy-rj, ljz = cz-rj, hjx = cx+rj, hjy = cy+rj, hjz = cz+rj;
+        bool back = ljx <= cu[0] && ljy <= cu[1] && ljz <= cu[2] && hjx >= cl[0] && hjy >= cl[1] && hjz >= cl[2];
+        if (!back)
+        {
+          int s = atomicAdd (&xcnt[j], 1);
+          if (s < xcap) xbuf[(size_t) s * E + j] = e;
+          else atomicAdd (&flags[FLAG_EXTRA_OVERFLOW], 1);
+        }
+      }
+    }
+    }
+    __syncthreads ();
   }
 
-  if (T > 0)
+  /* triangles whose AABB reaches Q (test_triangles, condet.ispc:412-516) */
+  int ntq = T > 0 ? tricnt[Q] : 0;
+  if (ntq > 0 && qcnt > 0)
   {
-    int a = trioff[me], b = a + tricnt[me];
-    for (int m = a; m < b; m++)
+    int a0 = trioff[Q];
+    int work = qcnt * ntq;
+    for (int q = t; q < work; q += DET_TPB)
     {
-      int t = trilist[m];
+      int ti = q / ntq, m = q - ti * ntq;
+      int tt = trilist[a0 + m];
+      double v[9];
+#pragma unroll
+      for (int k = 0; k < 9; k++) v[k] = tri[(size_t) k*T + tt];
+      double px = ssph[4*ti], py = ssph[4*ti+1], pz = ssph[4*ti+2], rr = radii[sidx[ti]];
+      /* the returned point lies on the triangle, so its distance is at least
+       * the distance to the triangle's box; skip when that exceeds r by a
+       * margin far above the rounding of these few operations */
+      double gx = fmax (fmax (fmin (fmin (v[0], v[3]), v[6]) - px, px - fmax (fmax (v[0], v[3]), v[6])), 0.0);
+      double gy = fmax (fmax (fmin (fmin (v[1], v[4]), v[7]) - py, py - fmax (fmax (v[1], v[4]), v[7])), 0.0);
+      double gz = fmax (fmax (fmin (fmin (v[2], v[5]), v[8]) - pz, pz - fmax (fmax (v[2], v[5]), v[8])), 0.0);
+      if (gx*gx + gy*gy + gz*gz > rr*rr*1.000001) continue;
       double pt[3], nr[3];
-      double depth = triangle_sphere (tri[t], tri[(size_t) T + t], tri[2*(size_t) T + t],
-                                      tri[3*(size_t) T + t], tri[4*(size_t) T + t], tri[5*(size_t) T + t],
-                                      tri[6*(size_t) T + t], tri[7*(size_t) T + t], tri[8*(size_t) T + t],
-                                      p[0], p[1], p[2], r0, pt, nr);
+      double depth = triangle_sphere (v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8], px, py, pz, rr, pt, nr);
       if (depth > 0.0)
       {
-        if (hc < hcap) hits[(size_t) hc * E + e] = PMX_TRI_FLAG | (uint32_t) t;
+        int slot = atomicAdd (&shc[ti], 1);
+        if (slot < hcap) hits[(size_t) slot * E + sidx[ti]] = PMX_TRI_FLAG | (uint32_t) tt;
         else atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1);
-        hc++;
       }
     }
   }
-  hitcnt[e] = hc < hcap ? hc : hcap;
+  __syncthreads ();
+  if (valid) hitcnt[sidx[t]] = shc[t] < hcap ? shc[t] : hcap;
+  (void) r0;
 }
 
 __global__ void k_force_init (int N, const int *list, int nlist, const double *mass, const double *gtab, int k,
@@ -415,7 +513,7 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
 __global__ void k_dynamics (int N, double *angular, double *linear, double *rotation, double *position,
                             const double *inertia, const double *inverse, const double *mass, const double *invm,
                             const double *dtab, int kstep, double *force, double *torque, double step0, double step1,
-                            const int *parell, int E, double *center)
+                            const int *parell, int E, double *center, const signed char *iso)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
@@ -426,11 +524,26 @@ __global__ void k_dynamics (int N, double *angular, double *linear, double *rota
 #pragma unroll
   for (int k = 0; k < 3; k++) { O[k] = angular[(size_t) k*N + i]; v[k] = linear[(size_t) k*N + i]; }
 #pragma unroll
-  for (int k = 0; k < 9; k++)
+  for (int k = 0; k < 9; k++) L1[k] = rotation[(size_t) k*N + i];
+  if (iso[i])
   {
-    L1[k] = rotation[(size_t) k*N + i];
-    J[k] = inertia[(size_t) k*N + i];
-    I[k] = inverse[(size_t) k*N + i];
+    /* inertia diag(j, j, j) with exact zeros elsewhere, hence inverse
+     * diag(v, v, v) (INVERT, macros.h:1033-1051): the same values as the
+     * full loads, from two of them */
+    double j = inertia[i], v = inverse[i];
+#pragma unroll
+    for (int k = 0; k < 9; k++) { J[k] = 0.0; I[k] = 0.0; }
+    J[0] = J[4] = J[8] = j;
+    I[0] = I[4] = I[8] = v;
+  }
+  else
+  {
+#pragma unroll
+    for (int k = 0; k < 9; k++)
+    {
+      J[k] = inertia[(size_t) k*N + i];
+      I[k] = inverse[(size_t) k*N + i];
+    }
   }
   im = invm[i];
   f[0] = force[i]; f[1] = force[(size_t) N + i]; f[2] = force[2*(size_t) N + i];
@@ -629,12 +742,13 @@ int step_detect (pmx_ctx &c)
   }
   hipLaunchKernelGGL (k_fill_i, grid (c.E), dim3 (PMX_TPB), 0, c.stream, c.xcnt.p, c.E, 0);
   ProfScope ps (c, K_DETECT, 68.0 * c.E);
-  hipLaunchKernelGGL (k_detect, grid (c.E), dim3 (PMX_TPB), 0, c.stream, c.E, (const double*) c.center.p,
+  hipLaunchKernelGGL (k_detect, dim3 (t.nleaves), dim3 (DET_TPB), 0, c.stream, c.E, (const double*) c.center.p,
                       (const double*) c.radii.p, (const int*) c.part.p, t.root, (const int*) t.left.p,
                       (const int*) t.right.p, (const int*) t.dim.p, (const double*) t.coord.p, (const double*) t.cell.p,
-                      t.nleaves, (const int*) c.leafof.p, (const int*) c.lcount.p, (const int*) c.loff.p,
+                      t.nleaves, (const int*) c.lcount.p, (const int*) c.loff.p,
                       (const int*) c.lmem.p, (const double*) c.lpos.p, (const double*) c.tri.p, c.T,
                       (const int*) t.tricnt.p, (const int*) t.trioff.p, (const int*) t.trilist.p,
+                      (const int*) t.nbrcnt.p, (const int*) t.nbroff.p, (const int*) t.nbrlist.p,
                       c.hits.p, c.hitcnt.p, c.hcap, c.xbuf.p, c.xcnt.p, c.xcap, c.dflags.p);
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
@@ -674,11 +788,11 @@ int step_rows (pmx_ctx &c, const double *g_dev, const double *g, double h)
 int step_dynamics (pmx_ctx &c, const double *d_dev, const double *d, double step0, double step1)
 {
   (void) d;
-  ProfScope ps (c, K_DYNAMICS, 512.0 * c.N + 56.0 * c.E);
+  ProfScope ps (c, K_DYNAMICS, 384.0 * c.N + 128.0 * (c.N - c.niso) + 56.0 * c.E);
   hipLaunchKernelGGL (k_dynamics, grid (c.N), dim3 (PMX_TPB), 0, c.stream, c.N, c.angular.p, c.linear.p, c.rotation.p,
                       c.position.p, (const double*) c.inertia.p, (const double*) c.inverse.p, (const double*) c.mass.p,
                       (const double*) c.invm.p, d_dev, 0, c.force.p, c.torque.p, step0, step1,
-                      (const int*) c.parell.p, c.E, c.center.p);
+                      (const int*) c.parell.p, c.E, c.center.p, (const signed char*) c.iso.p);
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
 }

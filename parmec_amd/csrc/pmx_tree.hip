@@ -408,6 +408,49 @@ __global__ void k_tri_reach (int T, const double *tri, int root, const int *left
   reach_leaves_count (root, left, right, dim, coord, lo, hi, tricnt, t, trioff, trilist, fill, flag);
 }
 
+/* leaves whose closed cell overlaps leaf Q's cell inflated by rmax: every leaf
+ * a sphere binned in Q can reach (its centre lies in Q's cell and its AABB
+ * half-width is at most rmax; rounding is monotone) */
+__global__ void k_leaf_nbr (int nleaves, const double *cell, double rmax, int root, const int *left, const int *right,
+                            const int *dim, const double *coord, int *cnt, const int *off, int *list, int *flag)
+{
+  int Q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (Q >= nleaves) return;
+  double lo[3], hi[3];
+  for (int k = 0; k < 3; k++)
+  {
+    lo[k] = cell[(size_t) k*nleaves + Q] - rmax;
+    hi[k] = cell[(size_t) (3+k)*nleaves + Q] + rmax;
+  }
+  int stack[64];
+  int sp = 0, ref = root, n = 0;
+  for (;;)
+  {
+    if (ref >= 0)
+    {
+      int d = dim[ref];
+      double c = coord[ref];
+      bool gl = lo[d] <= c, gr = hi[d] >= c;
+      if (gl && gr)
+      {
+        if (sp < 64) stack[sp++] = right[ref]; else atomicAdd (flag, 1);
+        ref = left[ref];
+        continue;
+      }
+      if (gl) { ref = left[ref]; continue; }
+      if (gr) { ref = right[ref]; continue; }
+    }
+    else
+    {
+      if (list) list[off[Q] + n] = -ref-1;
+      n++;
+    }
+    if (sp == 0) break;
+    ref = stack[--sp];
+  }
+  if (!list) cnt[Q] = n;
+}
+
 __global__ void k_fill_int (int *a, int n, int v)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -560,6 +603,31 @@ int tree_build (pmx_ctx &c)
   t.root = 0;
   PMX_HIP (hipStreamSynchronize (c.stream));
   branchidx.release ();
+  return PMX_OK;
+}
+
+int tree_nbrlists (pmx_ctx &c)
+{
+  TreeDev &t = c.tree;
+  int L = std::max (t.nleaves, 1);
+  if (t.nbrcnt.alloc (L) || t.nbroff.alloc (L + 1)) { pmx_set_error ("out of device memory (neighbour lists)"); return PMX_ENOMEM; }
+  t.nnbr = 0;
+  if (c.E == 0 || t.nleaves == 0) return PMX_OK;
+  int *flag = c.dflags.p + FLAG_TRILIST_OVERFLOW;
+  hipLaunchKernelGGL (k_leaf_nbr, grid (t.nleaves), dim3 (PMX_TPB), 0, c.stream, t.nleaves, (const double*) t.cell.p,
+                      c.rmax, t.root, (const int*) t.left.p, (const int*) t.right.p, (const int*) t.dim.p,
+                      (const double*) t.coord.p, t.nbrcnt.p, (const int*) nullptr, (int*) nullptr, flag);
+  PMX_HIP (hipGetLastError ());
+  PMX_HIP (hipMemcpyAsync (t.nbroff.p, t.nbrcnt.p, sizeof (int) * t.nleaves, hipMemcpyDeviceToDevice, c.stream));
+  int total = 0;
+  int rc = scan_exclusive (c, t.nbroff.p, t.nleaves, &total);
+  if (rc) return rc;
+  t.nnbr = total;
+  if (t.nbrlist.alloc (std::max (total, 1))) { pmx_set_error ("out of device memory (neighbour lists)"); return PMX_ENOMEM; }
+  hipLaunchKernelGGL (k_leaf_nbr, grid (t.nleaves), dim3 (PMX_TPB), 0, c.stream, t.nleaves, (const double*) t.cell.p,
+                      c.rmax, t.root, (const int*) t.left.p, (const int*) t.right.p, (const int*) t.dim.p,
+                      (const double*) t.coord.p, t.nbrcnt.p, (const int*) t.nbroff.p, t.nbrlist.p, flag);
+  PMX_HIP (hipGetLastError ());
   return PMX_OK;
 }
 
