@@ -448,6 +448,11 @@ struct orc_ctx
   std::vector<int> nodpart, facnod, factri;
   std::vector<int> pairs, ikind;
   std::vector<double> iparam;
+  /* springs (sorted by part1, part2) */
+  int sprnum = 0;
+  std::vector<int> sprtype, sprpart, spridx, dashidx, unidx, sprflg, unspring, sproffset;
+  std::vector<double> sprpnt, spring, dashpot, unload, yield, sprdir, sprfric, sprkskn, sprsdsp, stroke0, stroke, sprfrc;
+  int sprcrv = 0, dashcrv = 0, uncrv = 0;
   double curtime = 0.0, curstep = 0.0;
   int stepnum = 0;
   std::vector<std::vector<Contact>> master;
@@ -942,6 +947,383 @@ void dynamics (orc_ctx &s, const double damping[6], double step0, double step1)
   }
 }
 
+
+/* ------------------------------------------------------------- springs */
+
+/* look-up table linear spline with last-interval extrapolation,
+ * forces.ispc:410-426 */
+double look_up (const double *x, const double *y, int lo, int hi, double arg, double *slope)
+{
+  int h0 = hi;
+  while (hi > lo+1)
+  {
+    int mid = lo + (hi-lo)/2;
+    if (arg > x[mid]) lo = mid;
+    else hi = mid;
+  }
+  if (lo+1 == h0) lo--;
+  *slope = (y[lo+1]-y[lo])/(x[lo+1]-x[lo]);
+  return y[lo] + (*slope)*(arg-x[lo]);
+}
+
+/* slope of the spline searched along x (i == 0) or y (i == 1),
+ * forces.ispc:429-450 */
+double look_up_slope (int i, const double *x, const double *y, int lo, int hi, double arg, double *at)
+{
+  const double *tab[2] = {x, y};
+  int h0 = hi;
+  while (hi > lo+1)
+  {
+    int mid = lo + (hi-lo)/2;
+    if (arg > tab[i][mid]) lo = mid;
+    else hi = mid;
+  }
+  if (lo+1 == h0) lo--;
+  if (at)
+  {
+    int j = (i == 0 ? 1 : 0);
+    at[0] = tab[j][lo] + (tab[j][lo+1]-tab[j][lo])*(arg-tab[i][lo])/(tab[i][lo+1]-tab[i][lo]);
+  }
+  return (tab[1][lo+1]-tab[1][lo])/(tab[0][lo+1]-tab[0][lo]);
+}
+
+inline void vecskew (const double v[3], double A[9])
+{
+  A[0] = 0; A[1] = v[2]; A[2] = -v[1];
+  A[3] = -v[2]; A[4] = 0; A[5] = v[0];
+  A[6] = v[1]; A[7] = -v[0]; A[8] = 0;
+}
+
+inline void tnmul (const double A[9], const double B[9], double C[9])
+{
+  for (int c = 0; c < 3; c++)
+    for (int r = 0; r < 3; r++)
+      C[3*c+r] = A[3*r]*B[3*c] + A[3*r+1]*B[3*c+1] + A[3*r+2]*B[3*c+2];
+}
+
+/* smallest eigenvalue of a symmetric 3x3 matrix, macros.h:1538-1593 */
+double min_eigenvalue (const double A[9])
+{
+  double p1 = A[3]*A[3] + A[6]*A[6] + A[7]*A[7];
+  if (p1 == 0.0)
+  {
+    double r = A[0] > A[4] ? A[0] : A[4];
+    return r > A[8] ? r : A[8];   /* the reference returns the maximum here */
+  }
+  double q = (A[0]+A[4]+A[8])/3.0;
+  double p2 = (A[0]-q)*(A[0]-q) + (A[4]-q)*(A[4]-q) + (A[8]-q)*(A[8]-q) + 2*p1;
+  double p = std::sqrt (p2 / 6.0);
+  double invp = 1.0 / p;
+  double B[9] = {invp*(A[0]-q), invp*A[1], invp*A[2], invp*A[3], invp*(A[4]-q), invp*A[5],
+                 invp*A[6], invp*A[7], invp*(A[8]-q)};
+  double det = B[0]*(B[4]*B[8]-B[5]*B[7]) + B[3]*(B[2]*B[7]-B[1]*B[8]) + B[6]*(B[1]*B[5]-B[2]*B[4]);
+  double r = 0.5 * det;
+  double phi;
+  if (r <= -1.0) phi = 3.14159265358979323846 / 3.0;
+  else if (r >= 1.0) phi = 0.0;
+  else phi = std::acos (r) / 3.0;
+  return q + 2.0 * p * std::cos (phi + (2*3.14159265358979323846/3));
+}
+
+/* equivalent point mass along a direction, parmec.ispc:423-482 (point1 and
+ * point2 are passed as the springs task passes them: reference point minus
+ * the particle's reference position) */
+double eqm (const orc_ctx &s, int part1, const double point1[3], int part2, const double point2[3], const double dir[3])
+{
+  size_t N = s.parnum;
+  double Jiv[9], im, A[3], Ask[9], tmp[9], Miv[9];
+  for (int k = 0; k < 9; k++) Jiv[k] = s.inverse[k*N + part1];
+  im = s.invm[part1];
+  A[0] = s.position[3*N + part1] - point1[0];
+  A[1] = s.position[4*N + part1] - point1[1];
+  A[2] = s.position[5*N + part1] - point1[2];
+  vecskew (A, Ask);
+  tnmul (Ask, Jiv, tmp);
+  nnmul (tmp, Ask, Miv);
+  Miv[0] += im; Miv[4] += im; Miv[8] += im;
+  if (part2 >= 0)
+  {
+    for (int k = 0; k < 9; k++) Jiv[k] = s.inverse[k*N + part2];
+    im = s.invm[part2];
+    A[0] = s.position[3*N + part2] - point2[0];
+    A[1] = s.position[4*N + part2] - point2[1];
+    A[2] = s.position[5*N + part2] - point2[2];
+    vecskew (A, Ask);
+    tnmul (Ask, Jiv, tmp);
+    nnmul (tmp, Ask, Jiv);
+    for (int k = 0; k < 9; k++) Miv[k] = Miv[k] + Jiv[k];
+    Miv[0] += im; Miv[4] += im; Miv[8] += im;
+  }
+  if (dir[0]*dir[0] + dir[1]*dir[1] + dir[2]*dir[2] > 0.0)
+  {
+    nvmul (Miv, dir, tmp);
+    return 1.0/(tmp[0]*dir[0] + tmp[1]*dir[1] + tmp[2]*dir[2]);
+  }
+  return 1.0/min_eigenvalue (Miv);
+}
+
+/* springs_task, forces.ispc:641-1412, run as one task: per spring force and
+ * torque, accumulated locally per part1 run and per part2 run and flushed to
+ * the particles where the run ends (forces.ispc:1203-1411) */
+int springs (orc_ctx &s, double step)
+{
+  int S = s.sprnum;
+  size_t N = s.parnum, SS = S;
+  auto P = [&] (int p, int c) { return s.sprpnt.data () + (size_t) (6*p+c) * SS; };
+  double *force = s.force.data (), *torque = s.torque.data ();
+  double fj[3] = {0, 0, 0}, tj[3] = {0, 0, 0}, fk[3] = {0, 0, 0}, tk[3] = {0, 0, 0};
+  int j0 = -1, k0 = -1;
+  double oj[3], vj[3], xj[3], Xj[3], Lj[9], ok[3], vk[3], xk[3], Xk[3], Lk[9];
+  const double *sx = s.spring.data (), *sy = s.spring.data () + s.sprcrv;
+  const double *dx = s.dashpot.data (), *dy = s.dashpot.data () + s.dashcrv;
+  const double *ux = s.unload.data (), *uy = s.unload.data () + s.uncrv;
+  for (int i = 0; i < S; i++)
+  {
+    double len = 0.0, inv, velocity, spring_force = 0.0, dashpot_force = 0.0, total_force;
+    double Yj[3], Yk[3] = {0, 0, 0}, p[3], q[3], z[3] = {0, 0, 0}, w[3], frc[3], trq[3], u[3], a[3], b[3];
+    double stiffness = 0.0, damping = 0.0;
+    int j = s.sprpart[i], k = s.sprpart[SS + i];
+    if (j0 != j)
+    {
+      for (int c = 0; c < 3; c++)
+      {
+        oj[c] = s.angular[(3+c)*N + j]; vj[c] = s.linear[c*N + j];
+        xj[c] = s.position[c*N + j]; Xj[c] = s.position[(3+c)*N + j];
+      }
+      for (int c = 0; c < 9; c++) Lj[c] = s.rotation[c*N + j];
+    }
+    for (int c = 0; c < 3; c++) Yj[c] = P(0, 3+c)[i] - Xj[c];
+    p[0] = xj[0] + (Lj[0]*Yj[0] + Lj[3]*Yj[1] + Lj[6]*Yj[2]);
+    p[1] = xj[1] + (Lj[1]*Yj[0] + Lj[4]*Yj[1] + Lj[7]*Yj[2]);
+    p[2] = xj[2] + (Lj[2]*Yj[0] + Lj[5]*Yj[1] + Lj[8]*Yj[2]);
+    for (int c = 0; c < 3; c++) P(0, c)[i] = p[c];
+    for (int c = 0; c < 3; c++) u[c] = vj[c] * -1.0;
+    for (int c = 0; c < 3; c++) a[c] = p[c] - xj[c];
+    u[0] -= oj[1]*a[2] - oj[2]*a[1];
+    u[1] -= oj[2]*a[0] - oj[0]*a[2];
+    u[2] -= oj[0]*a[1] - oj[1]*a[0];
+    int flg = s.sprflg[i];
+    if (k >= 0)
+    {
+      if (k0 != k)
+      {
+        for (int c = 0; c < 3; c++)
+        {
+          ok[c] = s.angular[(3+c)*N + k]; vk[c] = s.linear[c*N + k];
+          xk[c] = s.position[c*N + k]; Xk[c] = s.position[(3+c)*N + k];
+        }
+        for (int c = 0; c < 9; c++) Lk[c] = s.rotation[c*N + k];
+      }
+      if (flg & PMX_SPRDIR_PROJECT)
+      {
+        for (int c = 0; c < 3; c++) w[c] = s.sprdir[(3+c)*SS + i];
+        nvmul (Lk, w, z);
+        for (int c = 0; c < 3; c++) s.sprdir[c*SS + i] = z[c];
+      }
+      for (int c = 0; c < 3; c++) Yk[c] = P(1, 3+c)[i] - Xk[c];
+      q[0] = xk[0] + (Lk[0]*Yk[0] + Lk[3]*Yk[1] + Lk[6]*Yk[2]);
+      q[1] = xk[1] + (Lk[1]*Yk[0] + Lk[4]*Yk[1] + Lk[7]*Yk[2]);
+      q[2] = xk[2] + (Lk[2]*Yk[0] + Lk[5]*Yk[1] + Lk[8]*Yk[2]);
+      for (int c = 0; c < 3; c++) P(1, c)[i] = q[c];
+      for (int c = 0; c < 3; c++) u[c] += vk[c];
+      for (int c = 0; c < 3; c++) b[c] = q[c] - xk[c];
+      u[0] += ok[1]*b[2] - ok[2]*b[1];
+      u[1] += ok[2]*b[0] - ok[0]*b[2];
+      u[2] += ok[0]*b[1] - ok[1]*b[0];
+    }
+    else
+    {
+      for (int c = 0; c < 3; c++) q[c] = P(1, c)[i];
+      if (flg & PMX_SPRDIR_PROJECT) for (int c = 0; c < 3; c++) z[c] = s.sprdir[c*SS + i];
+    }
+    switch (flg & (PMX_SPRDIR_FOLLOWER|PMX_SPRDIR_CONSTANT|PMX_SPRDIR_PLANAR|PMX_SPRDIR_PROJECT))
+    {
+    case PMX_SPRDIR_FOLLOWER:
+      for (int c = 0; c < 3; c++) w[c] = q[c] - p[c];
+      len = std::sqrt (w[0]*w[0] + w[1]*w[1] + w[2]*w[2]);
+      inv = len > 0.0 ? 1.0/len : 0.0;
+      for (int c = 0; c < 3; c++) { z[c] = inv*w[c]; s.sprdir[c*SS + i] = z[c]; }
+      break;
+    case PMX_SPRDIR_CONSTANT:
+      for (int c = 0; c < 3; c++) { w[c] = q[c] - p[c]; z[c] = s.sprdir[c*SS + i]; }
+      len = w[0]*z[0] + w[1]*z[1] + w[2]*z[2];
+      break;
+    case PMX_SPRDIR_PLANAR:
+      for (int c = 0; c < 3; c++) w[c] = q[c] - p[c];
+      inv = w[0]*s.sprdir[3*SS + i] + w[1]*s.sprdir[4*SS + i] + w[2]*s.sprdir[5*SS + i];
+      for (int c = 0; c < 3; c++) w[c] -= inv*s.sprdir[(3+c)*SS + i];
+      len = std::sqrt (w[0]*w[0] + w[1]*w[1] + w[2]*w[2]);
+      inv = len > 0.0 ? 1.0/len : 0.0;
+      for (int c = 0; c < 3; c++) { z[c] = inv*w[c]; s.sprdir[c*SS + i] = z[c]; }
+      break;
+    case PMX_SPRDIR_PROJECT:
+      for (int c = 0; c < 3; c++) w[c] = p[c] - q[c];
+      len = w[0]*z[0] + w[1]*z[1] + w[2]*z[2];
+      break;
+    }
+    /* the reference evaluates the relative normal velocity only on the
+     * force-law branch; evaluated here for every spring (its friction uses it) */
+    velocity = z[0]*u[0] + z[1]*u[1] + z[2]*u[2];
+    double offset = 0.0;
+    int unspr = s.unspring[i];
+    double *stroke = s.stroke.data (), *yld = s.yield.data ();
+    if (unspr <= -2)
+    {
+      if (s.sprtype[i] == PMX_SPRING_NONLINEAR_ELASTIC)
+      {
+        stroke[i] = len - s.stroke0[i];
+        spring_force = look_up (sx, sy, s.spridx[i], s.spridx[i+1], stroke[i] - offset, &stiffness);
+      }
+      else
+      {
+        if ((s.sprflg[i] & PMX_SPRING_YIELDED) == 0)
+        {
+          stroke[i] = len - s.stroke0[i];
+          spring_force = look_up (sx, sy, s.spridx[i], s.spridx[i+1], stroke[i] - offset, &stiffness);
+          if (spring_force < yld[i] || spring_force > yld[SS + i])
+          {
+            s.sprflg[i] |= PMX_SPRING_YIELDED;
+            if (spring_force < 0) { stroke[SS + i] = stroke[i]; stroke[2*SS + i] = 0.0; yld[SS + i] = 0.0; }
+            else { stroke[SS + i] = 0.0; stroke[2*SS + i] = stroke[i]; yld[i] = 0.0; }
+          }
+        }
+        else
+        {
+          double s0 = stroke[i];
+          double s1 = len - s.stroke0[i];
+          double ds = s1 - s0;
+          double f0 = s.sprfrc[SS + i];
+          bool loading = false;
+          stroke[i] = s1;
+          if (f0 * ds > 0)
+          {
+            if (s.sprflg[i] & PMX_SPRING_UNLOADING)
+            {
+              double dfds = look_up_slope (1, ux, uy, s.unidx[i], s.unidx[i+1], f0, nullptr);
+              spring_force = f0 + dfds*ds;
+              stiffness = dfds;   /* the reference leaves it unset on this branch */
+              if (spring_force < yld[i] || spring_force > yld[SS + i])
+              {
+                s.sprflg[i] &= ~PMX_SPRING_UNLOADING;
+                loading = true;
+              }
+            }
+            else loading = true;
+          }
+          else
+          {
+            double at;
+            double dfds = look_up_slope (1, ux, uy, s.unidx[i], s.unidx[i+1], f0, &at);
+            stiffness = dfds;
+            spring_force = f0 + dfds*ds;
+            if (spring_force * f0 < 0.0)
+              spring_force = look_up (ux, uy, s.unidx[i], s.unidx[i+1], ds+at, &stiffness);
+            if (spring_force < yld[i] || spring_force > yld[SS + i])
+            {
+              s.sprflg[i] &= ~PMX_SPRING_UNLOADING;
+              loading = true;
+            }
+            else s.sprflg[i] |= PMX_SPRING_UNLOADING;
+          }
+          if (loading)
+          {
+            if (ds < 0) stroke[SS + i] += ds;
+            else stroke[2*SS + i] += ds;
+            spring_force = look_up (sx, sy, s.spridx[i], s.spridx[i+1], ds < 0 ? stroke[SS + i] : stroke[2*SS + i], &stiffness);
+            if (ds < 0) yld[i] = spring_force;
+            else yld[SS + i] = spring_force;
+          }
+        }
+      }
+      if (s.dashidx[i]+1 == s.dashidx[i+1])
+      {
+        double mass = eqm (s, j, Yj, k, Yk, z);
+        double ratio = dx[s.dashidx[i]];
+        damping = ratio * 2.0 * std::sqrt (std::fabs (stiffness) * mass);
+        dashpot_force = velocity * damping;
+      }
+      else dashpot_force = look_up (dx, dy, s.dashidx[i], s.dashidx[i+1], velocity, &damping);
+      total_force = spring_force + (spring_force == 0.0 ? 0.0 : dashpot_force);
+    }
+    else if (unspr == -1)
+    {
+      stroke[i] = len - s.stroke0[i];
+      stiffness = 0.;
+      damping = 0.;
+      spring_force = dashpot_force = total_force = 0.;
+    }
+    else return 1;   /* load-curve unloading (UNSPRING) is not on this path */
+
+    for (int c = 0; c < 3; c++) frc[c] = z[c] * total_force;
+    s.sprfrc[i] = total_force;
+    s.sprfrc[SS + i] = spring_force;
+    double fric = s.sprfric[i];
+    if (fric > 0.0)
+    {
+      double us[3] = {u[0] - velocity*z[0], u[1] - velocity*z[1], u[2] - velocity*z[2]};
+      double flim = fric*std::fabs (total_force);
+      double fs[3] = {0.0, 0.0, 0.0};
+      double kskn = s.sprkskn[i];
+      double *sd = s.sprsdsp.data ();
+      if (kskn == 0.0)
+      {
+        double ulen = std::sqrt (us[0]*us[0] + us[1]*us[1] + us[2]*us[2]);
+        if (ulen > 0.0)
+        {
+          double coef = flim/ulen;
+          for (int c = 0; c < 3; c++) fs[c] = coef*us[c];
+        }
+      }
+      else
+      {
+        if (spring_force == 0.0) { sd[i] = 0.0; sd[SS + i] = 0.0; sd[2*SS + i] = 0.0; }
+        double ds[3] = {sd[i] + us[0]*step, sd[SS + i] + us[1]*step, sd[2*SS + i] + us[2]*step};
+        double c0 = kskn * std::fabs (stiffness);
+        double c1 = spring_force == 0.0 ? 0.0 : kskn * std::fabs (damping);
+        for (int c = 0; c < 3; c++) fs[c] = c0 * ds[c] + c1 * us[c];
+        double flen = std::sqrt (fs[0]*fs[0] + fs[1]*fs[1] + fs[2]*fs[2]);
+        if (flen > flim)
+        {
+          double coef = flim/flen;
+          for (int c = 0; c < 3; c++) fs[c] *= coef;
+          double dl0 = std::sqrt (ds[0]*ds[0] + ds[1]*ds[1] + ds[2]*ds[2]);
+          coef = flim/(c0*dl0);
+          for (int c = 0; c < 3; c++) sd[c*SS + i] = coef*ds[c];
+        }
+        else for (int c = 0; c < 3; c++) sd[c*SS + i] = ds[c];
+      }
+      s.sprfrc[2*SS + i] = std::sqrt (fs[0]*fs[0] + fs[1]*fs[1] + fs[2]*fs[2]);
+      for (int c = 0; c < 3; c++) frc[c] += fs[c];
+    }
+    trq[0] = a[1]*frc[2] - a[2]*frc[1];
+    trq[1] = a[2]*frc[0] - a[0]*frc[2];
+    trq[2] = a[0]*frc[1] - a[1]*frc[0];
+    if (j0 != j && j0 >= 0)
+    {
+      for (int c = 0; c < 3; c++) { force[c*N + j0] += fj[c]; torque[c*N + j0] += tj[c]; fj[c] = frc[c]; tj[c] = trq[c]; }
+    }
+    else
+      for (int c = 0; c < 3; c++) { fj[c] += frc[c]; tj[c] += trq[c]; }
+    if (k >= 0)
+    {
+      trq[0] = b[1]*frc[2] - b[2]*frc[1];
+      trq[1] = b[2]*frc[0] - b[0]*frc[2];
+      trq[2] = b[0]*frc[1] - b[1]*frc[0];
+      if (k0 != k && k0 >= 0)
+      {
+        for (int c = 0; c < 3; c++) { force[c*N + k0] -= fk[c]; torque[c*N + k0] -= tk[c]; fk[c] = frc[c]; tk[c] = trq[c]; }
+      }
+      else
+        for (int c = 0; c < 3; c++) { fk[c] += frc[c]; tk[c] += trq[c]; }
+      k0 = k;
+    }
+    j0 = j;
+  }
+  if (j0 >= 0) for (int c = 0; c < 3; c++) { force[c*N + j0] += fj[c]; torque[c*N + j0] += tj[c]; }
+  if (k0 >= 0) for (int c = 0; c < 3; c++) { force[c*N + k0] -= fk[c]; torque[c*N + k0] -= tk[c]; }
+  return 0;
+}
+
 } /* namespace */
 
 /* ------------------------------------------------------------------ API */
@@ -969,7 +1351,14 @@ int orc_upload (orc_ctx *s, const pmx_model *m)
 {
   if (!s || !m) return fail (PMX_EINVAL, "null argument");
   if (m->obsnum > 0) return fail (PMX_EUNSUPPORTED, "moving obstacles are not supported by the oracle");
-  if (m->sprnum > 0) return fail (PMX_EUNSUPPORTED, "springs are not supported by this oracle build");
+  for (int i = 0; i < m->sprnum; i++)
+  {
+    if (m->sproffset && m->sproffset[i] != -1) return fail (PMX_EUNSUPPORTED, "spring stroke offsets are not supported");
+    if (m->unspring && m->unspring[i] >= 0) return fail (PMX_EUNSUPPORTED, "UNSPRING load curves are not supported");
+    if (i && (m->sprpart[i-1] > m->sprpart[i] ||
+              (m->sprpart[i-1] == m->sprpart[i] && m->sprpart[m->sprnum + i-1] > m->sprpart[m->sprnum + i])))
+      return fail (PMX_EINVAL, "springs must be sorted by (part1, part2)");
+  }
   bool same = s->parnum == m->parnum && s->ellnum == m->ellnum && s->trinum == m->trinum;
   s->parnum = m->parnum; s->ellnum = m->ellnum; s->trinum = m->trinum;
   s->nodnum = m->nodnum; s->facnum = m->facnum; s->pairnum = m->pairnum;
@@ -999,6 +1388,29 @@ int orc_upload (orc_ctx *s, const pmx_model *m)
   take (s->pairs, m->pairs, 2*(size_t) m->pairnum);
   take (s->ikind, m->ikind, (size_t) m->pairnum);
   take (s->iparam, m->iparam, (size_t) PMX_NIPARAM * m->pairnum);
+  size_t S = m->sprnum;
+  s->sprnum = m->sprnum;
+  s->sprcrv = m->sprcrv; s->dashcrv = m->dashcrv; s->uncrv = m->uncrv;
+  take (s->sprtype, m->sprtype, S);
+  take (s->sprpart, m->sprpart, 2*S);
+  take (s->sprpnt, m->sprpnt, 12*S);
+  take (s->spridx, m->spridx, S ? S+1 : 0);
+  take (s->spring, m->spring, 2*(size_t) m->sprcrv);
+  take (s->dashidx, m->dashidx, S ? S+1 : 0);
+  take (s->dashpot, m->dashpot, 2*(size_t) m->dashcrv);
+  take (s->unidx, m->unidx, S ? S+1 : 0);
+  take (s->unload, m->unload, 2*(size_t) m->uncrv);
+  take (s->yield, m->yield, 2*S);
+  take (s->sprdir, m->sprdir, 6*S);
+  take (s->sprflg, m->sprflg, S);
+  take (s->sprfric, m->sprfric, S);
+  take (s->sprkskn, m->sprkskn, S);
+  take (s->sprsdsp, m->sprsdsp, 3*S);
+  take (s->stroke0, m->stroke0, S);
+  take (s->stroke, m->stroke, 3*S);
+  take (s->sprfrc, m->sprfrc, 3*S);
+  take (s->unspring, m->unspring, S);
+  take (s->sproffset, m->sproffset, S);
   s->curtime = m->curtime; s->curstep = m->curstep; s->stepnum = m->stepnum;
   if (!same)
   {
@@ -1018,6 +1430,9 @@ int orc_download (orc_ctx *s, pmx_model *m)
   put (m->position, s->position); put (m->inverse, s->inverse); put (m->invm, s->invm);
   put (m->force, s->force); put (m->torque, s->torque); put (m->center, s->center);
   put (m->tri, s->tri); put (m->nodes, s->nodes);
+  put (m->sprpnt, s->sprpnt); put (m->sprdir, s->sprdir); put (m->sprsdsp, s->sprsdsp); put (m->stroke, s->stroke);
+  put (m->sprfrc, s->sprfrc); put (m->yield, s->yield);
+  if (m->sprflg) std::memcpy (m->sprflg, s->sprflg.data (), s->sprflg.size () * sizeof (int));
   m->curtime = s->curtime; m->curstep = s->curstep; m->stepnum = s->stepnum;
   return PMX_OK;
 }
@@ -1062,6 +1477,7 @@ int orc_dem (orc_ctx *s, double duration, double step, const pmx_forcing *fc, do
     for (int d = 0; d < 3; d++) g[d] = fc && fc->gtab ? fc->gtab[3*k+d] : (fc ? fc->gravity[d] : 0.0);
     for (int d = 0; d < 6; d++) damp[d] = fc && fc->dtab ? fc->dtab[6*k+d] : (fc ? fc->damping[d] : 0.0);
     if (forces (*s, step0, g)) return fail (PMX_EUNSUPPORTED, "unsupported contact kind");
+    if (s->sprnum && springs (*s, step0)) return fail (PMX_EUNSUPPORTED, "unsupported spring kind");
     step1 = step0;
     dynamics (*s, damp, step0, step1);
     shapes (*s);

@@ -146,15 +146,27 @@ int pmx_upload (pmx_ctx *c, const pmx_model *m)
   if (m->parnum < 0 || m->ellnum < 0 || m->trinum < 0 || m->pairnum < 1)
     return fail (PMX_EINVAL, "invalid counts (pairnum must include the default pairing)");
   if (m->obsnum > 0) return fail (PMX_EUNSUPPORTED, "moving obstacles are not supported on the device path yet");
-  if (m->sprnum > 0) return fail (PMX_EUNSUPPORTED, "SPRING elements are not supported on the device path yet");
-  if (m->nodnum > 0 || m->facnum > 0) return fail (PMX_EUNSUPPORTED, "meshed particles are not supported on the device path yet");
+  for (int i = 0; i < m->sprnum; i++)
+  {
+    if (m->sproffset && m->sproffset[i] != -1) return fail (PMX_EUNSUPPORTED, "spring stroke offsets are not supported");
+    if (m->unspring[i] >= 0) return fail (PMX_EUNSUPPORTED, "UNSPRING load curves are not supported");
+    if (m->sprpart[i] < 0 || m->sprpart[i] >= m->parnum || m->sprpart[m->sprnum + i] < -1 ||
+        m->sprpart[m->sprnum + i] >= m->parnum) return fail (PMX_EINVAL, "spring particle index out of range");
+    if (i && (m->sprpart[i-1] > m->sprpart[i] ||
+              (m->sprpart[i-1] == m->sprpart[i] && m->sprpart[m->sprnum + i-1] > m->sprpart[m->sprnum + i])))
+      return fail (PMX_EINVAL, "springs must be sorted by (part1, part2)");
+  }
   for (int e = 0; e < m->ellnum; e++)
   {
     if (!(m->radii[(size_t) m->ellnum + e] < 0.0)) return fail (PMX_EUNSUPPORTED, "only spherical ellipsoids are supported");
     if (m->part[e] < 0 || m->part[e] >= m->parnum) return fail (PMX_EINVAL, "ellipsoid particle index out of range");
   }
   for (int t = 0; t < m->trinum; t++)
-    if (m->triobs[t] != -1) return fail (PMX_EUNSUPPORTED, "only static obstacle triangles are supported");
+  {
+    if (m->triobs[t] <= -2) return fail (PMX_EUNSUPPORTED, "moving obstacle triangles are not supported");
+    if (m->triobs[t] >= 0 && m->ellnum > 0)
+      return fail (PMX_EUNSUPPORTED, "contact between spheres and meshed particles is not supported yet");
+  }
   std::vector<int> parell (std::max (m->parnum, 1), -1), nosph;
   double rmax = 0.0;
   for (int e = 0; e < m->ellnum; e++)
@@ -179,6 +191,7 @@ int pmx_upload (pmx_ctx *c, const pmx_model *m)
   (void) hipSetDevice (c->device);
   bool same = c->uploaded && c->N == m->parnum && c->E == m->ellnum && c->T == m->trinum;
   c->N = m->parnum; c->E = m->ellnum; c->T = m->trinum; c->P = m->pairnum;
+  c->NOD = m->nodnum; c->FAC = m->facnum;
   size_t N = c->N, E = c->E, T = c->T, P = c->P;
   int rc = 0;
   if ((rc = up (*c, c->angular, m->angular, 6*N, "angular")) || (rc = up (*c, c->linear, m->linear, 3*N, "linear")) ||
@@ -207,6 +220,12 @@ int pmx_upload (pmx_ctx *c, const pmx_model *m)
   {
     if ((rc = alloc_rows (*c))) return rc;
   }
+  if ((rc = up (*c, c->nodes, m->nodes, 6*(size_t) c->NOD, "nodes")) ||
+      (rc = up (*c, c->nodpart, m->nodpart, (size_t) c->NOD, "nodpart")) ||
+      (rc = up (*c, c->facnod, m->facnod, 3*(size_t) c->FAC, "facnod")) ||
+      (rc = up (*c, c->factri, m->factri, (size_t) c->FAC, "factri")) ||
+      (rc = springs_setup (*c, m)))
+    return rc;
   c->curtime = m->curtime; c->curstep = m->curstep; c->stepnum = m->stepnum;
   PMX_HIP (hipStreamSynchronize (c->stream));
   c->uploaded = true;
@@ -225,7 +244,8 @@ int pmx_download (pmx_ctx *c, pmx_model *m)
       (rc = down (*c, m->rotation, c->rotation, 9*N)) || (rc = down (*c, m->position, c->position, 6*N)) ||
       (rc = down (*c, m->inverse, c->inverse, 9*N)) || (rc = down (*c, m->invm, c->invm, N)) ||
       (rc = down (*c, m->force, c->force, 3*N)) || (rc = down (*c, m->torque, c->torque, 3*N)) ||
-      (rc = down (*c, m->center, c->center, 6*E)) || (rc = down (*c, m->tri, c->tri, 9*T)))
+      (rc = down (*c, m->center, c->center, 6*E)) || (rc = down (*c, m->tri, c->tri, 9*T)) ||
+      (rc = down (*c, m->nodes, c->nodes, 6*(size_t) c->NOD)) || (rc = springs_download (*c, m)))
     return rc;
   PMX_HIP (hipStreamSynchronize (c->stream));
   m->curtime = c->curtime; m->curstep = c->curstep; m->stepnum = c->stepnum;
@@ -244,7 +264,7 @@ int pmx_dem (pmx_ctx *c, double duration, double step, const pmx_forcing *fc, do
   if (c->curtime == 0.0)
   {
     step0 = step;
-    if ((rc = step_euler (*c, 0.5*step0)) || (rc = step_shapes (*c))) return rc;
+    if ((rc = step_euler (*c, 0.5*step0)) || (rc = step_shapes (*c)) || (rc = step_mesh_shapes (*c))) return rc;
   }
   else step0 = c->curstep;
   step1 = step0;
@@ -296,8 +316,10 @@ int pmx_dem (pmx_ctx *c, double duration, double step, const pmx_forcing *fc, do
     const double *gk = c->gtab.p + 3*(size_t) std::min<long long> (k, std::max (nsteps-1, 0));
     const double *dk = c->dtab.p + 6*(size_t) std::min<long long> (k, std::max (nsteps-1, 0));
     if ((rc = step_rows (*c, gk, nullptr, step0))) return rc;
+    if ((rc = step_springs (*c, step0))) return rc;
     step1 = step0;
     if ((rc = step_dynamics (*c, dk, nullptr, step0, step1))) return rc;
+    if ((rc = step_mesh_shapes (*c))) return rc;
   }
   /* surface errors of the last step */
   if (c->E > 0)
@@ -375,7 +397,7 @@ int pmx_set_profiling (pmx_ctx *c, int on)
 int pmx_get_profile (pmx_ctx *c, pmx_kernel_time *out, int cap, int *count)
 {
   static const char *names[K_COUNT] = {"bin", "scatter", "detect", "force_init", "rows", "dynamics", "shapes",
-                                       "tree_build", "tree_trilists"};
+                                       "tree_build", "tree_trilists", "springs", "spring_acc"};
   if (!c) return fail (PMX_EINVAL, "null ctx");
   (void) hipSetDevice (c->device);
   PMX_HIP (hipStreamSynchronize (c->stream));

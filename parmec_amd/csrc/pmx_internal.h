@@ -68,7 +68,8 @@ struct TreeDev
 };
 
 /* kernels timed by the profiler */
-enum { K_BIN = 0, K_SCATTER, K_DETECT, K_FORCE_INIT, K_ROWS, K_DYNAMICS, K_SHAPES, K_TREE, K_TRILIST, K_COUNT };
+enum { K_BIN = 0, K_SCATTER, K_DETECT, K_FORCE_INIT, K_ROWS, K_DYNAMICS, K_SHAPES, K_TREE, K_TRILIST, K_SPRINGS,
+       K_SPRING_ACC, K_COUNT };
 
 /* HIP-event timing of the launches on the context stream; events are pooled
  * and resolved at the per-step synchronisation */
@@ -110,6 +111,15 @@ struct pmx_ctx
   dvec<double> iparam;
   /* per-step tables */
   dvec<double> gtab, dtab;
+  /* meshed particles: nodes and faces */
+  dvec<double> nodes;
+  dvec<int> nodpart, facnod, factri;
+  /* springs */
+  int S = 0, sprcrv = 0, dashcrv = 0, uncrv = 0;
+  dvec<int> sp_type, sp_part, sp_spridx, sp_dashidx, sp_unidx, sp_unspring, sp_flg;
+  dvec<double> sp_pnt, sp_dir, sp_yield, sp_sdsp, sp_stroke, sp_frc, sp_stroke0, sp_fric, sp_kskn;
+  dvec<double> sp_curve, sp_dash, sp_unload, sp_out;
+  dvec<int> sp_evoff, sp_evrun, sp_runoff, sp_runspr;
 
   /* broad phase per step */
   TreeDev tree;
@@ -180,5 +190,11 @@ int step_dynamics (pmx_ctx &c, const double *d_dev, const double d[6], double st
 int step_shapes (pmx_ctx &c);
 int step_euler (pmx_ctx &c, double h);
 int step_invert (pmx_ctx &c);
+
+/* ---- springs and meshes (pmx_spring.hip) ---- */
+int springs_setup (pmx_ctx &c, const pmx_model *m);
+int springs_download (pmx_ctx &c, pmx_model *m);
+int step_springs (pmx_ctx &c, double h);
+int step_mesh_shapes (pmx_ctx &c);
 
 #endif

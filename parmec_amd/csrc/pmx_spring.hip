@@ -1,0 +1,578 @@
+/*
+ * pmx_spring.hip -- SPRING elements (springs_task, forces.ispc:641-1412) and
+ * the geometry update of meshed particles (nodes_task/faces_task,
+ * shapes.ispc:101-211).
+ *
+ * k_springs: one thread per spring computes the current points, direction,
+ *   stroke, curve force (elastic or yield/unload), dashpot, friction, and the
+ *   spring's force and the torques it applies to both particles; spring state
+ *   (points, direction, stroke, yield limits, flags, friction history) is
+ *   updated in place.
+ * k_spring_acc: one thread per particle adds the spring contributions in the
+ *   order the reference's single-task loop flushes them: a local sum per run
+ *   of consecutive springs sharing part1 (added) or part2 (subtracted),
+ *   flushed where the run ends (forces.ispc:1203-1411).  The run table is
+ *   built on the host at upload; springs are sorted by (part1, part2) as
+ *   sort_springs leaves them (parmec.cpp:1683-1966).
+ */
+#include "pmx_internal.h"
+#include "pmx_device.h"
+#include <algorithm>
+
+using namespace pmxd;
+
+namespace {
+
+inline dim3 grid (int n) { return dim3 ((n + PMX_TPB - 1) / PMX_TPB); }
+
+/* forces.ispc:410-426 */
+__device__ __forceinline__ double look_up (const double *x, const double *y, int lo, int hi, double arg, double *slope)
+{
+  int h0 = hi;
+  while (hi > lo+1)
+  {
+    int mid = lo + (hi-lo)/2;
+    if (arg > x[mid]) lo = mid;
+    else hi = mid;
+  }
+  if (lo+1 == h0) lo--;
+  *slope = (y[lo+1]-y[lo])/(x[lo+1]-x[lo]);
+  return y[lo] + (*slope)*(arg-x[lo]);
+}
+
+/* forces.ispc:429-450, searching along y (the unload curves) */
+__device__ __forceinline__ double look_up_slope_y (const double *x, const double *y, int lo, int hi, double arg, double *at)
+{
+  int h0 = hi;
+  while (hi > lo+1)
+  {
+    int mid = lo + (hi-lo)/2;
+    if (arg > y[mid]) lo = mid;
+    else hi = mid;
+  }
+  if (lo+1 == h0) lo--;
+  if (at) at[0] = x[lo] + (x[lo+1]-x[lo])*(arg-y[lo])/(y[lo+1]-y[lo]);
+  return (y[lo+1]-y[lo])/(x[lo+1]-x[lo]);
+}
+
+__device__ __forceinline__ void skew (const double v[3], double A[9])
+{
+  A[0] = 0; A[1] = v[2]; A[2] = -v[1];
+  A[3] = -v[2]; A[4] = 0; A[5] = v[0];
+  A[6] = v[1]; A[7] = -v[0]; A[8] = 0;
+}
+
+__device__ __forceinline__ void tnmul (const double A[9], const double B[9], double C[9])
+{
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+      C[3*c+r] = A[3*r]*B[3*c] + A[3*r+1]*B[3*c+1] + A[3*r+2]*B[3*c+2];
+}
+
+/* extremum_eigenvalue (macros.h:1538-1593) as called by eqm with minimum = 1 */
+__device__ double min_eigenvalue (const double A[9])
+{
+  double p1 = A[3]*A[3] + A[6]*A[6] + A[7]*A[7];
+  if (p1 == 0.0)
+  {
+    double r = A[0] > A[4] ? A[0] : A[4];
+    return r > A[8] ? r : A[8];
+  }
+  double q = (A[0]+A[4]+A[8])/3.0;
+  double p2 = (A[0]-q)*(A[0]-q) + (A[4]-q)*(A[4]-q) + (A[8]-q)*(A[8]-q) + 2*p1;
+  double p = sqrt (p2 / 6.0);
+  double invp = 1.0 / p;
+  double B[9] = {invp*(A[0]-q), invp*A[1], invp*A[2], invp*A[3], invp*(A[4]-q), invp*A[5],
+                 invp*A[6], invp*A[7], invp*(A[8]-q)};
+  double det = B[0]*(B[4]*B[8]-B[5]*B[7]) + B[3]*(B[2]*B[7]-B[1]*B[8]) + B[6]*(B[1]*B[5]-B[2]*B[4]);
+  double r = 0.5 * det;
+  double phi;
+  if (r <= -1.0) phi = 3.14159265358979323846 / 3.0;
+  else if (r >= 1.0) phi = 0.0;
+  else phi = acos (r) / 3.0;
+  return q + 2.0 * p * cos (phi + (2*3.14159265358979323846/3));
+}
+
+/* eqm, parmec.ispc:423-482 */
+__device__ double eqm (int N, const double *inverse, const double *invm, const double *position, int part1,
+                       const double point1[3], int part2, const double point2[3], const double dir[3])
+{
+  double Jiv[9], im, A[3], Ask[9], tmp[9], Miv[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) Jiv[k] = inverse[(size_t) k*N + part1];
+  im = invm[part1];
+#pragma unroll
+  for (int k = 0; k < 3; k++) A[k] = position[(size_t) (3+k)*N + part1] - point1[k];
+  skew (A, Ask);
+  tnmul (Ask, Jiv, tmp);
+  nnmul (tmp, Ask, Miv);
+  Miv[0] += im; Miv[4] += im; Miv[8] += im;
+  if (part2 >= 0)
+  {
+#pragma unroll
+    for (int k = 0; k < 9; k++) Jiv[k] = inverse[(size_t) k*N + part2];
+    im = invm[part2];
+#pragma unroll
+    for (int k = 0; k < 3; k++) A[k] = position[(size_t) (3+k)*N + part2] - point2[k];
+    skew (A, Ask);
+    tnmul (Ask, Jiv, tmp);
+    nnmul (tmp, Ask, Jiv);
+#pragma unroll
+    for (int k = 0; k < 9; k++) Miv[k] = Miv[k] + Jiv[k];
+    Miv[0] += im; Miv[4] += im; Miv[8] += im;
+  }
+  if (dir[0]*dir[0] + dir[1]*dir[1] + dir[2]*dir[2] > 0.0)
+  {
+    nvmul (Miv, dir, tmp);
+    return 1.0/(tmp[0]*dir[0] + tmp[1]*dir[1] + tmp[2]*dir[2]);
+  }
+  return 1.0/min_eigenvalue (Miv);
+}
+
+struct SprArgs
+{
+  int S, N;
+  const int *type, *part, *spridx, *dashidx, *unidx, *unspring;
+  int *flg;
+  double *pnt, *dir, *yield, *sdsp, *stroke, *frc;
+  const double *stroke0, *fric, *kskn;
+  const double *sx, *sy, *dx, *dy, *ux, *uy;
+  const double *angular, *linear, *rotation, *position, *inverse, *invm;
+  double *out;   /* [9][S]: force, torque on part1, torque on part2 */
+  int *flags;
+};
+
+__global__ void k_springs (SprArgs A, double step)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.S) return;
+  size_t S = A.S, N = A.N;
+  auto P = [&] (int p, int c) -> double & { return A.pnt[(size_t) (6*p+c) * S + i]; };
+  int j = A.part[i], k = A.part[S + i];
+  double oj[3], vj[3], xj[3], Xj[3], Lj[9];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+  {
+    oj[c] = A.angular[(3+c)*N + j]; vj[c] = A.linear[c*N + j];
+    xj[c] = A.position[c*N + j]; Xj[c] = A.position[(3+c)*N + j];
+  }
+#pragma unroll
+  for (int c = 0; c < 9; c++) Lj[c] = A.rotation[c*N + j];
+  double len = 0.0, inv, velocity, spring_force = 0.0, dashpot_force = 0.0, total_force;
+  double Yj[3], Yk[3] = {0.0, 0.0, 0.0}, p[3], q[3], z[3] = {0.0, 0.0, 0.0}, w[3], frc[3], u[3], a[3], b[3] = {0.0, 0.0, 0.0};
+  double stiffness = 0.0, damping = 0.0;
+#pragma unroll
+  for (int c = 0; c < 3; c++) Yj[c] = P (0, 3+c) - Xj[c];
+  p[0] = xj[0] + (Lj[0]*Yj[0] + Lj[3]*Yj[1] + Lj[6]*Yj[2]);
+  p[1] = xj[1] + (Lj[1]*Yj[0] + Lj[4]*Yj[1] + Lj[7]*Yj[2]);
+  p[2] = xj[2] + (Lj[2]*Yj[0] + Lj[5]*Yj[1] + Lj[8]*Yj[2]);
+#pragma unroll
+  for (int c = 0; c < 3; c++) { P (0, c) = p[c]; u[c] = vj[c] * -1.0; a[c] = p[c] - xj[c]; }
+  u[0] -= oj[1]*a[2] - oj[2]*a[1];
+  u[1] -= oj[2]*a[0] - oj[0]*a[2];
+  u[2] -= oj[0]*a[1] - oj[1]*a[0];
+  int flg = A.flg[i];
+  if (k >= 0)
+  {
+    double ok[3], vk[3], xk[3], Xk[3], Lk[9];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+    {
+      ok[c] = A.angular[(3+c)*N + k]; vk[c] = A.linear[c*N + k];
+      xk[c] = A.position[c*N + k]; Xk[c] = A.position[(3+c)*N + k];
+    }
+#pragma unroll
+    for (int c = 0; c < 9; c++) Lk[c] = A.rotation[c*N + k];
+    if (flg & PMX_SPRDIR_PROJECT)
+    {
+#pragma unroll
+      for (int c = 0; c < 3; c++) w[c] = A.dir[(3+c)*S + i];
+      nvmul (Lk, w, z);
+#pragma unroll
+      for (int c = 0; c < 3; c++) A.dir[c*S + i] = z[c];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) Yk[c] = P (1, 3+c) - Xk[c];
+    q[0] = xk[0] + (Lk[0]*Yk[0] + Lk[3]*Yk[1] + Lk[6]*Yk[2]);
+    q[1] = xk[1] + (Lk[1]*Yk[0] + Lk[4]*Yk[1] + Lk[7]*Yk[2]);
+    q[2] = xk[2] + (Lk[2]*Yk[0] + Lk[5]*Yk[1] + Lk[8]*Yk[2]);
+#pragma unroll
+    for (int c = 0; c < 3; c++) { P (1, c) = q[c]; u[c] += vk[c]; b[c] = q[c] - xk[c]; }
+    u[0] += ok[1]*b[2] - ok[2]*b[1];
+    u[1] += ok[2]*b[0] - ok[0]*b[2];
+    u[2] += ok[0]*b[1] - ok[1]*b[0];
+  }
+  else
+  {
+#pragma unroll
+    for (int c = 0; c < 3; c++) q[c] = P (1, c);
+    if (flg & PMX_SPRDIR_PROJECT)
+    {
+#pragma unroll
+      for (int c = 0; c < 3; c++) z[c] = A.dir[c*S + i];
+    }
+  }
+  switch (flg & (PMX_SPRDIR_FOLLOWER|PMX_SPRDIR_CONSTANT|PMX_SPRDIR_PLANAR|PMX_SPRDIR_PROJECT))
+  {
+  case PMX_SPRDIR_FOLLOWER:
+    for (int c = 0; c < 3; c++) w[c] = q[c] - p[c];
+    len = sqrt (w[0]*w[0] + w[1]*w[1] + w[2]*w[2]);
+    inv = len > 0.0 ? 1.0/len : 0.0;
+    for (int c = 0; c < 3; c++) { z[c] = inv*w[c]; A.dir[c*S + i] = z[c]; }
+    break;
+  case PMX_SPRDIR_CONSTANT:
+    for (int c = 0; c < 3; c++) { w[c] = q[c] - p[c]; z[c] = A.dir[c*S + i]; }
+    len = w[0]*z[0] + w[1]*z[1] + w[2]*z[2];
+    break;
+  case PMX_SPRDIR_PLANAR:
+    for (int c = 0; c < 3; c++) w[c] = q[c] - p[c];
+    inv = w[0]*A.dir[3*S + i] + w[1]*A.dir[4*S + i] + w[2]*A.dir[5*S + i];
+    for (int c = 0; c < 3; c++) w[c] -= inv*A.dir[(3+c)*S + i];
+    len = sqrt (w[0]*w[0] + w[1]*w[1] + w[2]*w[2]);
+    inv = len > 0.0 ? 1.0/len : 0.0;
+    for (int c = 0; c < 3; c++) { z[c] = inv*w[c]; A.dir[c*S + i] = z[c]; }
+    break;
+  case PMX_SPRDIR_PROJECT:
+    for (int c = 0; c < 3; c++) w[c] = p[c] - q[c];
+    len = w[0]*z[0] + w[1]*z[1] + w[2]*z[2];
+    break;
+  }
+  velocity = z[0]*u[0] + z[1]*u[1] + z[2]*u[2];
+  int unspr = A.unspring[i];
+  double *stroke = A.stroke, *yld = A.yield;
+  if (unspr <= -2)
+  {
+    if (A.type[i] == PMX_SPRING_NONLINEAR_ELASTIC)
+    {
+      stroke[i] = len - A.stroke0[i];
+      spring_force = look_up (A.sx, A.sy, A.spridx[i], A.spridx[i+1], stroke[i] - 0.0, &stiffness);
+    }
+    else if ((flg & PMX_SPRING_YIELDED) == 0)
+    {
+      stroke[i] = len - A.stroke0[i];
+      spring_force = look_up (A.sx, A.sy, A.spridx[i], A.spridx[i+1], stroke[i] - 0.0, &stiffness);
+      if (spring_force < yld[i] || spring_force > yld[S + i])
+      {
+        flg |= PMX_SPRING_YIELDED;
+        if (spring_force < 0) { stroke[S + i] = stroke[i]; stroke[2*S + i] = 0.0; yld[S + i] = 0.0; }
+        else { stroke[S + i] = 0.0; stroke[2*S + i] = stroke[i]; yld[i] = 0.0; }
+      }
+    }
+    else
+    {
+      double s0 = stroke[i];
+      double s1 = len - A.stroke0[i];
+      double ds = s1 - s0;
+      double f0 = A.frc[S + i];
+      bool loading = false;
+      stroke[i] = s1;
+      if (f0 * ds > 0)
+      {
+        if (flg & PMX_SPRING_UNLOADING)
+        {
+          double dfds = look_up_slope_y (A.ux, A.uy, A.unidx[i], A.unidx[i+1], f0, nullptr);
+          spring_force = f0 + dfds*ds;
+          stiffness = dfds;
+          if (spring_force < yld[i] || spring_force > yld[S + i]) { flg &= ~PMX_SPRING_UNLOADING; loading = true; }
+        }
+        else loading = true;
+      }
+      else
+      {
+        double at;
+        double dfds = look_up_slope_y (A.ux, A.uy, A.unidx[i], A.unidx[i+1], f0, &at);
+        stiffness = dfds;
+        spring_force = f0 + dfds*ds;
+        if (spring_force * f0 < 0.0)
+          spring_force = look_up (A.ux, A.uy, A.unidx[i], A.unidx[i+1], ds+at, &stiffness);
+        if (spring_force < yld[i] || spring_force > yld[S + i]) { flg &= ~PMX_SPRING_UNLOADING; loading = true; }
+        else flg |= PMX_SPRING_UNLOADING;
+      }
+      if (loading)
+      {
+        if (ds < 0) stroke[S + i] += ds;
+        else stroke[2*S + i] += ds;
+        spring_force = look_up (A.sx, A.sy, A.spridx[i], A.spridx[i+1], ds < 0 ? stroke[S + i] : stroke[2*S + i], &stiffness);
+        if (ds < 0) yld[i] = spring_force;
+        else yld[S + i] = spring_force;
+      }
+    }
+    if (A.dashidx[i]+1 == A.dashidx[i+1])
+    {
+      double mass = eqm (A.N, A.inverse, A.invm, A.position, j, Yj, k, Yk, z);
+      double ratio = A.dx[A.dashidx[i]];
+      damping = ratio * 2.0 * sqrt (fabs (stiffness) * mass);
+      dashpot_force = velocity * damping;
+    }
+    else dashpot_force = look_up (A.dx, A.dy, A.dashidx[i], A.dashidx[i+1], velocity, &damping);
+    total_force = spring_force + (spring_force == 0.0 ? 0.0 : dashpot_force);
+  }
+  else
+  {
+    /* unspr == -1: inactive spring (load-curve unloading is rejected at upload) */
+    stroke[i] = len - A.stroke0[i];
+    stiffness = 0.;
+    damping = 0.;
+    spring_force = dashpot_force = total_force = 0.;
+  }
+  A.flg[i] = flg;
+#pragma unroll
+  for (int c = 0; c < 3; c++) frc[c] = z[c] * total_force;
+  A.frc[i] = total_force;
+  A.frc[S + i] = spring_force;
+  double fric = A.fric[i];
+  if (fric > 0.0)
+  {
+    double us[3] = {u[0] - velocity*z[0], u[1] - velocity*z[1], u[2] - velocity*z[2]};
+    double flim = fric*fabs (total_force);
+    double fs[3] = {0.0, 0.0, 0.0};
+    double kskn = A.kskn[i];
+    double *sd = A.sdsp;
+    if (kskn == 0.0)
+    {
+      double ulen = sqrt (us[0]*us[0] + us[1]*us[1] + us[2]*us[2]);
+      if (ulen > 0.0)
+      {
+        double coef = flim/ulen;
+        for (int c = 0; c < 3; c++) fs[c] = coef*us[c];
+      }
+    }
+    else
+    {
+      if (spring_force == 0.0) { sd[i] = 0.0; sd[S + i] = 0.0; sd[2*S + i] = 0.0; }
+      double ds[3] = {sd[i] + us[0]*step, sd[S + i] + us[1]*step, sd[2*S + i] + us[2]*step};
+      double c0 = kskn * fabs (stiffness);
+      double c1 = spring_force == 0.0 ? 0.0 : kskn * fabs (damping);
+      for (int c = 0; c < 3; c++) fs[c] = c0 * ds[c] + c1 * us[c];
+      double flen = sqrt (fs[0]*fs[0] + fs[1]*fs[1] + fs[2]*fs[2]);
+      if (flen > flim)
+      {
+        double coef = flim/flen;
+        for (int c = 0; c < 3; c++) fs[c] *= coef;
+        double dl0 = sqrt (ds[0]*ds[0] + ds[1]*ds[1] + ds[2]*ds[2]);
+        coef = flim/(c0*dl0);
+        for (int c = 0; c < 3; c++) sd[c*S + i] = coef*ds[c];
+      }
+      else for (int c = 0; c < 3; c++) sd[c*S + i] = ds[c];
+    }
+    A.frc[2*S + i] = sqrt (fs[0]*fs[0] + fs[1]*fs[1] + fs[2]*fs[2]);
+    for (int c = 0; c < 3; c++) frc[c] += fs[c];
+  }
+  double *o = A.out;
+  o[i] = frc[0]; o[S + i] = frc[1]; o[2*S + i] = frc[2];
+  o[3*S + i] = a[1]*frc[2] - a[2]*frc[1];
+  o[4*S + i] = a[2]*frc[0] - a[0]*frc[2];
+  o[5*S + i] = a[0]*frc[1] - a[1]*frc[0];
+  o[6*S + i] = b[1]*frc[2] - b[2]*frc[1];
+  o[7*S + i] = b[2]*frc[0] - b[0]*frc[2];
+  o[8*S + i] = b[0]*frc[1] - b[1]*frc[0];
+}
+
+/* per particle: its run flushes in the reference's order */
+__global__ void k_spring_acc (int N, int S, const int *evoff, const int *evrun, const int *runoff, const int *runspr,
+                              const double *out, double *force, double *torque)
+{
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= N) return;
+  int e0 = evoff[b], e1 = evoff[b+1];
+  if (e0 == e1) return;
+  double f[3] = {force[b], force[(size_t) N + b], force[2*(size_t) N + b]};
+  double t[3] = {torque[b], torque[(size_t) N + b], torque[2*(size_t) N + b]};
+  for (int ev = e0; ev < e1; ev++)
+  {
+    int code = evrun[ev];
+    int run = code >> 2;          /* run number */
+    bool kside = code & 1;        /* part2 run: subtracted, torque from column 6-8 */
+    bool zero = code & 2;         /* first run of its side: starts from 0.0 */
+    int r0 = runoff[run], r1 = runoff[run+1];
+    int tc = kside ? 6 : 3;
+    double af[3], at[3];
+    int s = runspr[r0];
+    if (zero)
+    {
+      for (int c = 0; c < 3; c++) { af[c] = 0.0 + out[(size_t) c*S + s]; at[c] = 0.0 + out[(size_t) (tc+c)*S + s]; }
+    }
+    else
+    {
+      for (int c = 0; c < 3; c++) { af[c] = out[(size_t) c*S + s]; at[c] = out[(size_t) (tc+c)*S + s]; }
+    }
+    for (int r = r0 + 1; r < r1; r++)
+    {
+      s = runspr[r];
+      for (int c = 0; c < 3; c++) { af[c] += out[(size_t) c*S + s]; at[c] += out[(size_t) (tc+c)*S + s]; }
+    }
+    if (kside) for (int c = 0; c < 3; c++) { f[c] -= af[c]; t[c] -= at[c]; }
+    else for (int c = 0; c < 3; c++) { f[c] += af[c]; t[c] += at[c]; }
+  }
+  force[b] = f[0]; force[(size_t) N + b] = f[1]; force[2*(size_t) N + b] = f[2];
+  torque[b] = t[0]; torque[(size_t) N + b] = t[1]; torque[2*(size_t) N + b] = t[2];
+}
+
+/* nodes_task, shapes.ispc:101-186 */
+__global__ void k_nodes (int NOD, int N, const int *nodpart, double *nodes, const double *rotation, const double *position)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= NOD) return;
+  int j = nodpart[i];
+  double L[9], C[3];
+#pragma unroll
+  for (int k = 0; k < 9; k++) L[k] = rotation[(size_t) k*N + j];
+#pragma unroll
+  for (int k = 0; k < 3; k++) C[k] = nodes[(size_t) (3+k)*NOD + i] - position[(size_t) (3+k)*N + j];
+  nodes[i] = position[j] + (L[0]*C[0] + L[3]*C[1] + L[6]*C[2]);
+  nodes[(size_t) NOD + i] = position[(size_t) N + j] + (L[1]*C[0] + L[4]*C[1] + L[7]*C[2]);
+  nodes[2*(size_t) NOD + i] = position[2*(size_t) N + j] + (L[2]*C[0] + L[5]*C[1] + L[8]*C[2]);
+}
+
+/* faces_task, shapes.ispc:189-211 */
+__global__ void k_faces (int FAC, int NOD, int T, const int *facnod, const int *factri, const double *nodes, double *tri)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= FAC) return;
+  int t = factri[i];
+#pragma unroll
+  for (int v = 0; v < 3; v++)
+  {
+    int k = facnod[(size_t) v*FAC + i];
+#pragma unroll
+    for (int c = 0; c < 3; c++) tri[(size_t) (3*v+c)*T + t] = nodes[(size_t) c*NOD + k];
+  }
+}
+
+} /* namespace */
+
+int springs_setup (pmx_ctx &c, const pmx_model *m)
+{
+  int S = m->sprnum;
+  c.S = S;
+  if (S == 0) return PMX_OK;
+  /* run table of the single-task flush order (forces.ispc:1203-1411) */
+  struct Ev { int pos, sub, body, code; };
+  std::vector<Ev> ev;
+  std::vector<int> runoff (1, 0), runspr;
+  int nrun = 0;
+  const int *p1 = m->sprpart, *p2 = m->sprpart + S;
+  bool firstj = true, firstk = true;
+  for (int i = 0; i < S; )
+  {
+    int e = i;
+    while (e < S && p1[e] == p1[i]) e++;
+    for (int q = i; q < e; q++) runspr.push_back (q);
+    runoff.push_back ((int) runspr.size ());
+    ev.push_back ({e, 0, p1[i], (nrun << 2) | (firstj ? 2 : 0)});
+    firstj = false;
+    nrun++;
+    i = e;
+  }
+  {
+    int cur = -1, start = (int) runspr.size ();
+    for (int i = 0; i <= S; i++)
+    {
+      if (i < S && p2[i] < 0) continue;
+      int k = i < S ? p2[i] : -2;
+      if (cur >= 0 && k != cur)
+      {
+        runoff.push_back ((int) runspr.size ());
+        ev.push_back ({i, 1, cur, (nrun << 2) | 1 | (firstk ? 2 : 0)});
+        firstk = false;
+        nrun++;
+        start = (int) runspr.size ();
+      }
+      (void) start;
+      if (i < S) { runspr.push_back (i); cur = k; }
+    }
+  }
+  std::stable_sort (ev.begin (), ev.end (), [] (const Ev &a, const Ev &b) {
+    return a.body < b.body || (a.body == b.body && (a.pos < b.pos || (a.pos == b.pos && a.sub < b.sub))); });
+  std::vector<int> evoff (c.N + 1, 0), evrun (ev.size ());
+  for (auto &x : ev) evoff[x.body + 1]++;
+  for (int b = 0; b < c.N; b++) evoff[b+1] += evoff[b];
+  for (size_t q = 0; q < ev.size (); q++) evrun[q] = ev[q].code;
+  auto upi = [&] (dvec<int> &d, const int *src, size_t n) -> int {
+    if (d.alloc (std::max (n, (size_t) 1)) != hipSuccess) { pmx_set_error ("out of device memory (springs)"); return PMX_ENOMEM; }
+    if (n) PMX_HIP (hipMemcpyAsync (d.p, src, n * sizeof (int), hipMemcpyHostToDevice, c.stream));
+    return PMX_OK;
+  };
+  auto upd = [&] (dvec<double> &d, const double *src, size_t n) -> int {
+    if (d.alloc (std::max (n, (size_t) 1)) != hipSuccess) { pmx_set_error ("out of device memory (springs)"); return PMX_ENOMEM; }
+    if (n) PMX_HIP (hipMemcpyAsync (d.p, src, n * sizeof (double), hipMemcpyHostToDevice, c.stream));
+    return PMX_OK;
+  };
+  int rc;
+  size_t SS = S;
+  if ((rc = upi (c.sp_evoff, evoff.data (), evoff.size ())) || (rc = upi (c.sp_evrun, evrun.data (), evrun.size ())) ||
+      (rc = upi (c.sp_runoff, runoff.data (), runoff.size ())) || (rc = upi (c.sp_runspr, runspr.data (), runspr.size ())) ||
+      (rc = upi (c.sp_type, m->sprtype, SS)) || (rc = upi (c.sp_part, m->sprpart, 2*SS)) ||
+      (rc = upi (c.sp_spridx, m->spridx, SS+1)) || (rc = upi (c.sp_dashidx, m->dashidx, SS+1)) ||
+      (rc = upi (c.sp_unidx, m->unidx, SS+1)) || (rc = upi (c.sp_unspring, m->unspring, SS)) ||
+      (rc = upi (c.sp_flg, m->sprflg, SS)) || (rc = upd (c.sp_pnt, m->sprpnt, 12*SS)) ||
+      (rc = upd (c.sp_dir, m->sprdir, 6*SS)) || (rc = upd (c.sp_yield, m->yield, 2*SS)) ||
+      (rc = upd (c.sp_sdsp, m->sprsdsp, 3*SS)) || (rc = upd (c.sp_stroke, m->stroke, 3*SS)) ||
+      (rc = upd (c.sp_frc, m->sprfrc, 3*SS)) || (rc = upd (c.sp_stroke0, m->stroke0, SS)) ||
+      (rc = upd (c.sp_fric, m->sprfric, SS)) || (rc = upd (c.sp_kskn, m->sprkskn, SS)) ||
+      (rc = upd (c.sp_curve, m->spring, 2*(size_t) m->sprcrv)) || (rc = upd (c.sp_dash, m->dashpot, 2*(size_t) m->dashcrv)) ||
+      (rc = upd (c.sp_unload, m->unload, 2*(size_t) m->uncrv)))
+    return rc;
+  c.sprcrv = m->sprcrv; c.dashcrv = m->dashcrv; c.uncrv = m->uncrv;
+  if (c.sp_out.alloc (9*SS) != hipSuccess) { pmx_set_error ("out of device memory (springs)"); return PMX_ENOMEM; }
+  return PMX_OK;
+}
+
+int springs_download (pmx_ctx &c, pmx_model *m)
+{
+  size_t S = c.S;
+  if (S == 0) return PMX_OK;
+  auto dn = [&] (double *dst, const dvec<double> &d, size_t n) -> int {
+    if (dst && n) PMX_HIP (hipMemcpyAsync (dst, d.p, n * sizeof (double), hipMemcpyDeviceToHost, c.stream));
+    return PMX_OK;
+  };
+  int rc;
+  if ((rc = dn (m->sprpnt, c.sp_pnt, 12*S)) || (rc = dn (m->sprdir, c.sp_dir, 6*S)) || (rc = dn (m->yield, c.sp_yield, 2*S)) ||
+      (rc = dn (m->sprsdsp, c.sp_sdsp, 3*S)) || (rc = dn (m->stroke, c.sp_stroke, 3*S)) || (rc = dn (m->sprfrc, c.sp_frc, 3*S)))
+    return rc;
+  if (m->sprflg) PMX_HIP (hipMemcpyAsync (m->sprflg, c.sp_flg.p, S * sizeof (int), hipMemcpyDeviceToHost, c.stream));
+  return PMX_OK;
+}
+
+int step_springs (pmx_ctx &c, double h)
+{
+  if (c.S == 0) return PMX_OK;
+  SprArgs A;
+  A.S = c.S; A.N = c.N;
+  A.type = c.sp_type.p; A.part = c.sp_part.p; A.spridx = c.sp_spridx.p; A.dashidx = c.sp_dashidx.p;
+  A.unidx = c.sp_unidx.p; A.unspring = c.sp_unspring.p; A.flg = c.sp_flg.p;
+  A.pnt = c.sp_pnt.p; A.dir = c.sp_dir.p; A.yield = c.sp_yield.p; A.sdsp = c.sp_sdsp.p; A.stroke = c.sp_stroke.p;
+  A.frc = c.sp_frc.p; A.stroke0 = c.sp_stroke0.p; A.fric = c.sp_fric.p; A.kskn = c.sp_kskn.p;
+  A.sx = c.sp_curve.p; A.sy = c.sp_curve.p + c.sprcrv;
+  A.dx = c.sp_dash.p; A.dy = c.sp_dash.p + c.dashcrv;
+  A.ux = c.sp_unload.p; A.uy = c.sp_unload.p + c.uncrv;
+  A.angular = c.angular.p; A.linear = c.linear.p; A.rotation = c.rotation.p; A.position = c.position.p;
+  A.inverse = c.inverse.p; A.invm = c.invm.p; A.out = c.sp_out.p; A.flags = c.dflags.p;
+  {
+    ProfScope ps (c, K_SPRINGS, 420.0 * c.S);
+    hipLaunchKernelGGL (k_springs, grid (c.S), dim3 (PMX_TPB), 0, c.stream, A, h);
+  }
+  PMX_HIP (hipGetLastError ());
+  {
+    ProfScope ps (c, K_SPRING_ACC, 96.0 * c.N + 2 * 72.0 * c.S);
+    hipLaunchKernelGGL (k_spring_acc, grid (c.N), dim3 (PMX_TPB), 0, c.stream, c.N, c.S, (const int*) c.sp_evoff.p,
+                        (const int*) c.sp_evrun.p, (const int*) c.sp_runoff.p, (const int*) c.sp_runspr.p,
+                        (const double*) c.sp_out.p, c.force.p, c.torque.p);
+  }
+  PMX_HIP (hipGetLastError ());
+  return PMX_OK;
+}
+
+int step_mesh_shapes (pmx_ctx &c)
+{
+  if (c.NOD == 0) return PMX_OK;
+  hipLaunchKernelGGL (k_nodes, grid (c.NOD), dim3 (PMX_TPB), 0, c.stream, c.NOD, c.N, (const int*) c.nodpart.p,
+                      c.nodes.p, (const double*) c.rotation.p, (const double*) c.position.p);
+  if (c.FAC > 0)
+    hipLaunchKernelGGL (k_faces, grid (c.FAC), dim3 (PMX_TPB), 0, c.stream, c.FAC, c.NOD, c.T, (const int*) c.facnod.p,
+                        (const int*) c.factri.p, (const double*) c.nodes.p, c.tri.p);
+  PMX_HIP (hipGetLastError ());
+  return PMX_OK;
+}

@@ -19,8 +19,9 @@ Gravity and damping callbacks and TSERIES are evaluated on the host at each
 step's curtime (input.cpp:4424-4503) and handed to the device as per-step
 tables.  The step loop itself is libpmx's pmx_dem (C++ host + HIP kernels).
 
-Not on this path (raise NotImplementedError): MESH, ANALYTICAL, SPRING and
-the other constraint/spring kinds, moving obstacles, PRESCRIBE, HISTORY,
+SPRING (1440-1788) is supported without stroke offsets (load-curve
+TSERIES).  Not on this path (raise NotImplementedError): MESH, ANALYTICAL,
+torsion springs, unsprings, joints, moving obstacles, PRESCRIBE, HISTORY,
 OUTPUT, CRITICAL and adaptive time stepping.
 """
 import bisect
@@ -30,7 +31,7 @@ import numpy as np
 
 from . import pmx
 
-__all__ = ["RESET", "TSERIES", "MATERIAL", "SPHERE", "OBSTACLE", "GRANULAR", "VELOCITY", "GRAVITY", "DAMPING",
+__all__ = ["RESET", "TSERIES", "MATERIAL", "SPHERE", "OBSTACLE", "GRANULAR", "SPRING", "VELOCITY", "GRAVITY", "DAMPING",
            "DEM", "MESH", "ANALYTICAL", "SPRING", "PRESCRIBE", "HISTORY", "OUTPUT", "CRITICAL", "state", "contacts",
            "engine_options"]
 
@@ -78,6 +79,7 @@ class _State:
         self.engine = None
         self.dirty = True
         self.options = {"row_capacity": 0, "device": 0}
+        self.springs = []            # SPRING records in creation order
 
 
 _S = _State()
@@ -260,6 +262,158 @@ def DAMPING(linear, angular):
     _S.angdamp = one(angular, "angular")
 
 
+_REAL_MAX = 1.7976931348623157e308
+
+
+def _norm3(v):
+    n = math.sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2])
+    return n
+
+
+def SPRING(part1, point1, part2, geom2, spring, dashpot=None, direction=None, planar=None, unload=None, ylim=None,
+           inactive=False, offset=-1, friction=0.0, kskn=0.0):
+    """Translational spring between particles (input.cpp:1440-1788); returns
+    the spring number.  geom2 is a point (part2 >= 0, or -1 for a fixed
+    point) or [point, normal] for a point-to-plane spring."""
+    if not (isinstance(part1, int) and part1 >= 0):
+        raise ValueError("'part1' must be non-negative")
+    p1 = _tuple(point1, "point1", 3)
+    if offset < -1 or offset >= len(_S.tms):
+        raise ValueError("offset TSERIES index is out of range")
+    if offset != -1:
+        raise NotImplementedError("spring stroke offsets are not on the MI355X step path")
+    if part2 < -1:
+        raise ValueError("Particle two index is out of range [-1, 0, ...]")
+    if not isinstance(spring, list) or len(spring) < 4 or len(spring) % 2:
+        raise ValueError("Invalid spring lookup table list length")
+    if isinstance(dashpot, list) and (len(dashpot) < 4 or len(dashpot) % 2):
+        raise ValueError("Invalid dashpot lookup table list length")
+    if unload is not None and (len(unload) < 4 or len(unload) % 2):
+        raise ValueError("Invalid unload lookup table list length")
+    if friction < 0 or kskn < 0:
+        raise ValueError("'friction' and 'kskn' must be non-negative")
+    r = {"part": (part1, part2), "pnt": [list(p1), None], "dir": [0.0] * 6, "flg": 0, "fric": float(friction),
+         "kskn": float(kskn), "unspring": -1 if inactive else -3, "yield": (0.0, 0.0), "unload": None}
+    if isinstance(geom2, tuple):
+        r["pnt"][1] = list(_tuple(geom2, "geom2", 3))
+        project = False
+    elif isinstance(geom2, list) and len(geom2) == 2:
+        q = list(_tuple(geom2[0], "geom2", 3))
+        d = list(_tuple(geom2[1], "geom2", 3))
+        ln = _norm3(d)
+        d = [d[0] / ln, d[1] / ln, d[2] / ln]            # NORMALIZE, macros.h:427-433
+        r["dir"] = d + d
+        w = [p1[0] - q[0], p1[1] - q[1], p1[2] - q[2]]
+        ln = d[0] * w[0] + d[1] * w[1] + d[2] * w[2]
+        r["pnt"][1] = [p1[0] - ln * d[0], p1[1] - ln * d[1], p1[2] - ln * d[2]]   # projection onto the plane
+        project = True
+    else:
+        raise TypeError("'geom2' must be a tuple or a [point, normal] list")
+    xs = [float(v) for v in spring[0::2]]
+    if any(xs[k] <= xs[k - 1] for k in range(1, len(xs))):
+        raise ValueError("Spring stroke values must increase")
+    r["spring"] = (xs, [float(v) for v in spring[1::2]])
+    if isinstance(dashpot, list):
+        vs = [float(v) for v in dashpot[0::2]]
+        if any(vs[k] <= vs[k - 1] for k in range(1, len(vs))):
+            raise ValueError("Dashpot velocity values must increase")
+        r["dashpot"] = (vs, [float(v) for v in dashpot[1::2]])
+    elif dashpot is not None:
+        ratio = float(dashpot)
+        if ratio < 0.0:
+            raise ValueError("Critical damping ratio not in [0.0, +Inf) interval")
+        r["dashpot"] = ([ratio], [ratio])
+    else:
+        r["dashpot"] = ([-_REAL_MAX, _REAL_MAX], [0.0, 0.0])
+    if unload is not None:
+        us, uf = [float(v) for v in unload[0::2]], [float(v) for v in unload[1::2]]
+        for k in range(1, len(us)):
+            if (uf[k] - uf[k - 1]) / (us[k] - us[k - 1]) <= 0.0:
+                raise ValueError("Unloading curve must be monotonically increasing")
+        r["unload"] = (us, uf)
+    p2 = r["pnt"][1]
+    if project:
+        r["flg"] |= pmx.SPRDIR_PROJECT
+        r["stroke0"] = 0.0
+    elif direction is not None:
+        d = list(_tuple(direction, "direction", 3))
+        ln = _norm3(d)
+        if ln == 0.0:
+            raise ValueError("Invalid zero direction")
+        inv = 1.0 / ln
+        d = [d[0] * inv, d[1] * inv, d[2] * inv]
+        r["dir"] = d + d
+        if planar is None or planar == "OFF":
+            r["flg"] |= pmx.SPRDIR_CONSTANT
+        elif planar == "ON":
+            r["flg"] |= pmx.SPRDIR_PLANAR
+        else:
+            raise ValueError("Invalid planar switch")
+        dif = [p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]]
+        dot = dif[0] * d[0] + dif[1] * d[1] + dif[2] * d[2]
+        if r["flg"] & pmx.SPRDIR_CONSTANT:
+            r["stroke0"] = dot
+        else:
+            dif = [dif[0] - dot * d[0], dif[1] - dot * d[1], dif[2] - dot * d[2]]
+            r["stroke0"] = _norm3(dif)
+    else:
+        r["flg"] |= pmx.SPRDIR_FOLLOWER
+        dif = [p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]]
+        r["stroke0"] = _norm3(dif)
+    if ylim is not None:
+        y = _tuple(ylim, "ylim", 2)
+        if y[0] > 0.0:
+            raise ValueError("Compressive yield limit must be <= 0.0")
+        if y[1] < 0.0:
+            raise ValueError("Tensile yield limit must be >= 0.0")
+        r["yield"] = y
+    _S.springs.append(r)
+    _S.dirty = True
+    return len(_S.springs) - 1
+
+
+def _spring_arrays(m):
+    """Spring records -> SoA arrays, sorted by (part1, part2) as sort_springs
+    leaves them (parmec.cpp:1683-1966; ties keep creation order)."""
+    recs = _S.springs
+    order = sorted(range(len(recs)), key=lambda i: (recs[i]["part"][0], recs[i]["part"][1], i))
+    S = len(recs)
+    ncrv = sum(len(recs[i]["spring"][0]) for i in order)
+    ndash = sum(len(recs[i]["dashpot"][0]) for i in order)
+    nun = sum(len(recs[i]["unload"][0]) for i in order if recs[i]["unload"])
+    m.set_springs(S, ncrv, ndash, nun)
+    cs = ds = us = 0
+    for k, i in enumerate(order):
+        r = recs[i]
+        m.sprpart[:, k] = r["part"]
+        for p in range(2):
+            m.sprpnt[p, :3, k] = r["pnt"][p]
+            m.sprpnt[p, 3:, k] = r["pnt"][p]
+        m.sprdir[:, k] = r["dir"]
+        m.sprflg[k] = r["flg"]
+        m.sprfric[k], m.sprkskn[k] = r["fric"], r["kskn"]
+        m.unspring[k] = r["unspring"]
+        m.yield_[:, k] = r["yield"]
+        m.stroke0[k] = r["stroke0"]
+        x, y = r["spring"]
+        m.spridx[k] = cs
+        m.spring[0, cs:cs + len(x)], m.spring[1, cs:cs + len(x)] = x, y
+        cs += len(x)
+        x, y = r["dashpot"]
+        m.dashidx[k] = ds
+        m.dashpot[0, ds:ds + len(x)], m.dashpot[1, ds:ds + len(x)] = x, y
+        ds += len(x)
+        m.unidx[k] = us
+        if r["unload"]:
+            x, y = r["unload"]
+            m.unload[0, us:us + len(x)], m.unload[1, us:us + len(x)] = x, y
+            us += len(x)
+            m.sprtype[k] = pmx.SPRING_GENERAL_NONLINEAR
+        else:
+            m.sprtype[k] = pmx.SPRING_NONLINEAR_ELASTIC
+    m.spridx[S], m.dashidx[S], m.unidx[S] = cs, ds, us
+
+
 def _not_here(name):
     def f(*a, **k):
         raise NotImplementedError(f"{name} is not on the MI355X step path")
@@ -269,7 +423,6 @@ def _not_here(name):
 
 MESH = _not_here("MESH")
 ANALYTICAL = _not_here("ANALYTICAL")
-SPRING = _not_here("SPRING")
 PRESCRIBE = _not_here("PRESCRIBE")
 HISTORY = _not_here("HISTORY")
 OUTPUT = _not_here("OUTPUT")
@@ -322,6 +475,8 @@ def _build_model():
         m.tri[:] = np.array(_S.tris, dtype=np.float64).T
         m.tricol[:] = np.array(_S.tricol, dtype=np.int32)
         m.triobs[:] = -1
+    if _S.springs:
+        _spring_arrays(m)
     # sort_materials, parmec.cpp:1636-1663: pairs ordered by colours
     pairs = sorted(_S.pairs, key=lambda p: (p[0], p[1]))
     for i, (c1, c2, par) in enumerate(pairs):

@@ -18,6 +18,8 @@ PMX_MIXED = 1
 NIPARAM = 7
 SPRING, DAMPER, FRISTAT, FRIDYN, FRIROL, FRIDRIL, KSKN = range(7)
 GRANULAR_FORCE = 0
+SPRING_NONLINEAR_ELASTIC, SPRING_GENERAL_NONLINEAR = 0, 1
+SPRING_YIELDED, SPRING_UNLOADING, SPRDIR_CONSTANT, SPRDIR_PLANAR, SPRDIR_FOLLOWER, SPRDIR_PROJECT = 1, 2, 4, 8, 16, 32
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -138,6 +140,42 @@ class Model:
         self.iparam[DAMPER, 0] = 1.0
         self.iparam[KSKN, 0] = 0.5
         self.curtime, self.curstep, self.stepnum = 0.0, 0.0, 0
+        self.set_mesh(0, 0)
+        self.set_springs(0, 0, 0, 0)
+
+    def set_mesh(self, nodnum, facnum):
+        """Mesh nodes [6][nodnum] (current, reference), node -> particle, and
+        faces [3][facnum] of node numbers with face -> triangle."""
+        self.nodnum, self.facnum = nodnum, facnum
+        self.nodes = np.zeros((6, nodnum))
+        self.nodpart = np.zeros(nodnum, dtype=np.int32)
+        self.facnod = np.zeros((3, facnum), dtype=np.int32)
+        self.factri = np.zeros(facnum, dtype=np.int32)
+
+    def set_springs(self, sprnum, sprcrv, dashcrv, uncrv):
+        """SPRING arrays of parmec.h:152-200; curve tables are [2][n] (x, y)."""
+        S = sprnum
+        self.sprnum, self.sprcrv, self.dashcrv, self.uncrv = S, sprcrv, dashcrv, uncrv
+        self.sprtype = np.zeros(S, dtype=np.int32)
+        self.sprpart = np.zeros((2, S), dtype=np.int32)
+        self.sprpnt = np.zeros((2, 6, S))
+        self.spridx = np.zeros(S + 1, dtype=np.int32)
+        self.spring = np.zeros((2, sprcrv))
+        self.dashidx = np.zeros(S + 1, dtype=np.int32)
+        self.dashpot = np.zeros((2, dashcrv))
+        self.unidx = np.zeros(S + 1, dtype=np.int32)
+        self.unload = np.zeros((2, uncrv))
+        self.yield_ = np.zeros((2, S))
+        self.sprdir = np.zeros((6, S))
+        self.sprflg = np.zeros(S, dtype=np.int32)
+        self.sprfric = np.zeros(S)
+        self.sprkskn = np.zeros(S)
+        self.sprsdsp = np.zeros((3, S))
+        self.stroke0 = np.zeros(S)
+        self.stroke = np.zeros((3, S))
+        self.sprfrc = np.zeros((3, S))
+        self.unspring = np.full(S, -3, dtype=np.int32)
+        self.sproffset = np.full(S, -1, dtype=np.int32)
 
     def copy(self):
         m = Model.__new__(Model)
@@ -145,13 +183,18 @@ class Model:
             setattr(m, k, v.copy() if isinstance(v, np.ndarray) else v)
         return m
 
+    _DBL = ("angular", "linear", "rotation", "position", "inertia", "inverse", "mass", "invm", "force", "torque",
+            "center", "radii", "tri", "iparam", "nodes", "sprpnt", "spring", "dashpot", "unload", "yield_", "sprdir",
+            "sprfric", "sprkskn", "sprsdsp", "stroke0", "stroke", "sprfrc")
+    _INT = ("flags", "part", "ellcol", "tricol", "triobs", "pairs", "ikind", "nodpart", "facnod", "factri", "sprtype",
+            "sprpart", "spridx", "dashidx", "unidx", "sprflg", "unspring", "sproffset")
+
     def check(self):
-        for name in ("angular", "linear", "rotation", "position", "inertia", "inverse", "mass", "invm", "force",
-                     "torque", "center", "radii", "tri", "iparam"):
+        for name in self._DBL:
             a = getattr(self, name)
             if a.dtype != np.float64 or not a.flags.c_contiguous:
                 setattr(self, name, np.ascontiguousarray(a, dtype=np.float64))
-        for name in ("flags", "part", "ellcol", "tricol", "triobs", "pairs", "ikind"):
+        for name in self._INT:
             a = getattr(self, name)
             if a.dtype != np.int32 or not a.flags.c_contiguous:
                 setattr(self, name, np.ascontiguousarray(a, dtype=np.int32))
@@ -160,10 +203,11 @@ class Model:
         self.check()
         s = pmx_model()
         s.parnum, s.ellnum, s.trinum, s.pairnum = self.parnum, self.ellnum, self.trinum, self.pairnum
-        for name in ("angular", "linear", "rotation", "position", "inertia", "inverse", "mass", "invm", "force",
-                     "torque", "center", "radii", "tri", "iparam"):
+        s.nodnum, s.facnum, s.sprnum = self.nodnum, self.facnum, self.sprnum
+        s.sprcrv, s.dashcrv, s.uncrv = self.sprcrv, self.dashcrv, self.uncrv
+        for name in self._DBL:
             setattr(s, name, _ptr(getattr(self, name), "d"))
-        for name in ("flags", "part", "ellcol", "tricol", "triobs", "pairs", "ikind"):
+        for name in self._INT:
             setattr(s, name, _ptr(getattr(self, name), "i"))
         s.curtime, s.curstep, s.stepnum = self.curtime, self.curstep, self.stepnum
         self._keep = s

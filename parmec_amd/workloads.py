@@ -149,3 +149,121 @@ def hopper(n=2000000, seed=1, density=2000.0, r_median=0.002, sigma=0.25, r_min=
     v[2] -= pour_speed
     m.linear[:] = v
     return m
+
+
+def _cuboid_faces():
+    """12 triangles (node numbers of a box's 8 corners, i*4 + j*2 + k for
+    offsets (i, j, k) in {0,1}^3), outward oriented."""
+    quads = [(0, 2, 6, 4), (1, 5, 7, 3), (0, 4, 5, 1), (2, 3, 7, 6), (0, 1, 3, 2), (4, 6, 7, 5)]
+    tris = []
+    for a, b, c, d in quads:
+        tris.append((a, b, c))
+        tris.append((a, c, d))
+    return tris
+
+
+def spring_lattice(nx=40, ny=40, nz=20, size=(0.2, 0.2, 0.4), gap=0.01, density=1800.0, seed=0,
+                   stiffness=(1e5, 1e7), curve_step=0.005, damping_ratio=0.05, anchors=True):
+    """Lattice of meshed cuboid bodies joined to their 6 neighbours by SPRING
+    elements with random 5-point piecewise-linear force-stroke curves.
+
+    Each body is a MESH of 8 nodes and 12 triangles (its mass properties are
+    those of a solid cuboid).  Springs run between facing face centres
+    (follower direction, initial stroke = gap), curves pass through (0, 0)
+    with strokes curve_step*(-2, -1, 0, 1, 2) and slopes uniform in
+    `stiffness` (seeded); dashpots use a critical damping ratio.  With
+    anchors, every bottom body also hangs on a spring to a fixed point
+    `gap` below its bottom face (part2 = -1), so a base excitation can be
+    applied in the base frame as a time-dependent gravity.
+    """
+    rng = np.random.default_rng(seed)
+    a, b, c = size
+    n = nx * ny * nz
+    ii, jj, kk = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    ii, jj, kk = ii.ravel(), jj.ravel(), kk.ravel()
+    idx = lambda i, j, k: (i * ny + j) * nz + k
+    centers = np.stack([ii * (a + gap) + 0.5 * a, jj * (b + gap) + 0.5 * b, kk * (c + gap) + 0.5 * c + gap], axis=1)
+    mass = density * a * b * c
+    m = pmx.Model(parnum=n, ellnum=0, trinum=12 * n, pairnum=1)
+    m.position[:3] = centers.T
+    m.position[3:] = centers.T
+    m.mass[:] = mass
+    m.invm[:] = 1.0 / mass
+    m.inertia[0] = mass * (b * b + c * c) / 12.0
+    m.inertia[4] = mass * (a * a + c * c) / 12.0
+    m.inertia[8] = mass * (a * a + b * b) / 12.0
+    for k in (0, 4, 8):
+        m.inverse[k] = 1.0 / m.inertia[k]
+    # mesh: 8 corner nodes, 12 faces per body
+    m.set_mesh(8 * n, 12 * n)
+    off = np.array([[(q >> 2) & 1, (q >> 1) & 1, q & 1] for q in range(8)], dtype=np.float64) - 0.5
+    corners = centers[:, None, :] + off[None, :, :] * np.array([a, b, c])
+    corners = corners.reshape(-1, 3)
+    m.nodes[:3] = corners.T
+    m.nodes[3:] = corners.T
+    m.nodpart[:] = np.repeat(np.arange(n, dtype=np.int32), 8)
+    faces = np.array(_cuboid_faces(), dtype=np.int32)
+    fac = (np.arange(n)[:, None, None] * 8 + faces[None, :, :]).reshape(-1, 3)
+    m.facnod[:] = fac.T
+    m.factri[:] = np.arange(12 * n, dtype=np.int32)
+    m.tri[:] = np.concatenate([corners[fac[:, 0]], corners[fac[:, 1]], corners[fac[:, 2]]], axis=1).T
+    m.tricol[:] = 1
+    m.triobs[:] = np.repeat(np.arange(n, dtype=np.int32), 12)
+    # springs: (part1, part2, point1, point2)
+    sp = []
+    for d, (di, dj, dk) in enumerate(((1, 0, 0), (0, 1, 0), (0, 0, 1))):
+        sel = (ii + di < nx) & (jj + dj < ny) & (kk + dk < nz)
+        p1 = idx(ii[sel], jj[sel], kk[sel])
+        p2 = idx(ii[sel] + di, jj[sel] + dj, kk[sel] + dk)
+        half = 0.5 * np.array([a, b, c])[d]
+        e = np.zeros(3)
+        e[d] = 1.0
+        x1 = centers[p1] + half * e
+        x2 = centers[p2] - half * e
+        sp.append((p1, p2, x1, x2))
+    if anchors:
+        bot = np.nonzero(kk == 0)[0]
+        x1 = centers[bot] - np.array([0.0, 0.0, 0.5 * c])
+        x2 = x1 - np.array([0.0, 0.0, gap])
+        sp.append((bot, np.full(len(bot), -1), x1, x2))
+    p1 = np.concatenate([s[0] for s in sp])
+    p2 = np.concatenate([s[1] for s in sp])
+    x1 = np.concatenate([s[2] for s in sp])
+    x2 = np.concatenate([s[3] for s in sp])
+    order = np.lexsort((p2, p1))      # sort_springs: by (part1, part2)
+    p1, p2, x1, x2 = p1[order], p2[order], x1[order], x2[order]
+    S = len(p1)
+    m.set_springs(S, 5 * S, S, 0)
+    m.sprtype[:] = pmx.SPRING_NONLINEAR_ELASTIC
+    m.sprpart[0] = p1
+    m.sprpart[1] = p2
+    m.sprpnt[0, :3] = x1.T
+    m.sprpnt[0, 3:] = x1.T
+    m.sprpnt[1, :3] = x2.T
+    m.sprpnt[1, 3:] = x2.T
+    slopes = rng.uniform(stiffness[0], stiffness[1], size=(S, 4))
+    h = curve_step
+    fx = np.array([-2 * h, -h, 0.0, h, 2 * h])
+    fy = np.stack([-(slopes[:, 1] + slopes[:, 0]) * h, -slopes[:, 1] * h, np.zeros(S), slopes[:, 2] * h,
+                   (slopes[:, 2] + slopes[:, 3]) * h], axis=1)
+    m.spring[0] = np.tile(fx, S)
+    m.spring[1] = fy.ravel()
+    m.spridx[:] = 5 * np.arange(S + 1, dtype=np.int32)
+    m.dashpot[0] = damping_ratio
+    m.dashpot[1] = damping_ratio
+    m.dashidx[:] = np.arange(S + 1, dtype=np.int32)
+    m.unidx[:] = 0
+    m.sprflg[:] = pmx.SPRDIR_FOLLOWER
+    dif = x2 - x1
+    m.stroke0[:] = np.sqrt((dif * dif).sum(axis=1))
+    return m
+
+
+def base_excitation(times, amplitude=9.81, frequency=5.0, g=9.81):
+    """Gravity table (per step) of a base shaken horizontally with
+    a(t) = amplitude*sin(2 pi f t), seen in the base frame."""
+    t = np.asarray(times)
+    gt = np.zeros((len(t), 3))
+    gt[:, 0] = -amplitude * np.sin(2.0 * math.pi * frequency * t)
+    gt[:, 2] = -g
+    return gt
