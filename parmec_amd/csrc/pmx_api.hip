@@ -112,7 +112,9 @@ int pmx_create (const pmx_options *opt, pmx_ctx **out)
     return fail (PMX_EDEVICE, "cannot open HIP device/stream");
   }
   if (c->opt.row_capacity > 0) { c->cap = c->opt.row_capacity; c->hcap = c->opt.row_capacity; c->xcap = std::max (8, c->opt.row_capacity / 2); }
-  if (c->dflags.alloc (FLAG_COUNT) != hipSuccess) { delete c; return fail (PMX_ENOMEM, "out of device memory"); }
+  if (c->dflags.alloc (FLAG_COUNT) != hipSuccess || c->ctl.alloc (CTL_COUNT) != hipSuccess)
+  { delete c; return fail (PMX_ENOMEM, "out of device memory"); }
+  if (hipMemset (c->ctl.p, 0, CTL_COUNT * sizeof (int)) != hipSuccess) { delete c; return fail (PMX_EDEVICE, "memset failed"); }
   if (hipMemset (c->dflags.p, 0, FLAG_COUNT * sizeof (int)) != hipSuccess) { delete c; return fail (PMX_EDEVICE, "memset failed"); }
   *out = c;
   return PMX_OK;
@@ -136,6 +138,8 @@ void pmx_destroy (pmx_ctx *c)
   for (auto *p : ii) p->release ();
   c->rowkey[0].release (); c->rowkey[1].release (); c->hits.release (); c->iso.release ();
   c->tree.code.release (); c->tree.code2.release ();
+  c->ctl.release ();
+  if (c->gexec) (void) hipGraphExecDestroy (c->gexec);
   if (c->stream) (void) hipStreamDestroy (c->stream);
   delete c;
 }
@@ -252,6 +256,55 @@ int pmx_download (pmx_ctx *c, pmx_model *m)
   return PMX_OK;
 }
 
+/* build the partitioning tree and its per-leaf lists from the current
+ * centres (partitioning_create, partition.ispc:663-746) */
+static int rebuild (pmx_ctx *c)
+{
+  int rc;
+  ProfScope ps (*c, K_TREE, 0.0);
+  if ((rc = tree_build (*c)) || (rc = tree_trilists (*c)) || (rc = tree_nbrlists (*c)) || (rc = step_prepare (*c)))
+    return rc;
+  int zero[2] = {0, 0};
+  PMX_HIP (hipMemcpyAsync (c->ctl.p + CTL_HINT, zero, sizeof (int), hipMemcpyHostToDevice, c->stream));
+  PMX_HIP (hipMemcpyAsync (c->ctl.p + CTL_ABORT, zero, sizeof (int), hipMemcpyHostToDevice, c->stream));
+  PMX_HIP (hipStreamSynchronize (c->stream));
+  c->graph_valid = false;
+  return PMX_OK;
+}
+
+/* read the control block and the error flags (synchronises the stream) */
+static int read_flags (pmx_ctx *c, int ctl[CTL_COUNT])
+{
+  int fl[FLAG_COUNT];
+  PMX_HIP (hipMemcpyAsync (ctl, c->ctl.p, sizeof (int) * CTL_COUNT, hipMemcpyDeviceToHost, c->stream));
+  PMX_HIP (hipMemcpyAsync (fl, c->dflags.p, sizeof (fl), hipMemcpyDeviceToHost, c->stream));
+  PMX_HIP (hipStreamSynchronize (c->stream));
+  prof_resolve (*c);
+  if (fl[FLAG_ROW_OVERFLOW] || fl[FLAG_HIT_OVERFLOW] || fl[FLAG_EXTRA_OVERFLOW] || fl[FLAG_TRILIST_OVERFLOW])
+    return fail (PMX_ECAPACITY, "contact capacity exceeded (row " + std::to_string (fl[FLAG_ROW_OVERFLOW]) + ", hits " +
+                 std::to_string (fl[FLAG_HIT_OVERFLOW]) + ", reverse " + std::to_string (fl[FLAG_EXTRA_OVERFLOW]) +
+                 "); raise pmx_options.row_capacity");
+  if (fl[FLAG_BAD_KIND]) return fail (PMX_EUNSUPPORTED, "unsupported interaction kind or inconsistent contact rows");
+  c->stats.max_row = std::max (c->stats.max_row, fl[FLAG_MAXROW]);
+  return PMX_OK;
+}
+
+static int capture (pmx_ctx *c, double h)
+{
+  if (c->gexec) { (void) hipGraphExecDestroy (c->gexec); c->gexec = nullptr; }
+  hipGraph_t g = nullptr;
+  PMX_HIP (hipStreamBeginCapture (c->stream, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue_step (*c, h);
+  hipError_t e = hipStreamEndCapture (c->stream, &g);
+  if (rc) { if (g) (void) hipGraphDestroy (g); return rc; }
+  PMX_HIP (e);
+  e = hipGraphInstantiate (&c->gexec, g, nullptr, nullptr, 0);
+  (void) hipGraphDestroy (g);
+  PMX_HIP (e);
+  c->graph_valid = true;
+  return PMX_OK;
+}
+
 int pmx_dem (pmx_ctx *c, double duration, double step, const pmx_forcing *fc, double *elapsed)
 {
   if (!c) return fail (PMX_EINVAL, "null ctx");
@@ -264,7 +317,7 @@ int pmx_dem (pmx_ctx *c, double duration, double step, const pmx_forcing *fc, do
   if (c->curtime == 0.0)
   {
     step0 = step;
-    if ((rc = step_euler (*c, 0.5*step0)) || (rc = step_shapes (*c)) || (rc = step_mesh_shapes (*c))) return rc;
+    if ((rc = step_euler (*c, 0.5*step0)) || (rc = step_shapes (*c)) || (rc = step_mesh_shapes (*c, false))) return rc;
   }
   else step0 = c->curstep;
   step1 = step0;
@@ -283,56 +336,57 @@ int pmx_dem (pmx_ctx *c, double duration, double step, const pmx_forcing *fc, do
     if ((rc = up (*c, c->gtab, g.data (), g.size (), "gravity table")) || (rc = up (*c, c->dtab, d.data (), d.size (), "damping table")))
       return rc;
   }
+  int ctl0[CTL_COUNT] = {0};
+  PMX_HIP (hipMemcpyAsync (c->ctl.p, ctl0, sizeof (ctl0), hipMemcpyHostToDevice, c->stream));
+  PMX_HIP (hipMemsetAsync (c->dflags.p, 0, FLAG_COUNT * sizeof (int), c->stream));
+  c->cur0 = c->cur;
+  c->graph_valid = false;
+  c->stats.max_row = 0;
 
   long long rebuilds = 0;
   if (c->E > 0)
   {
-    ProfScope ps (*c, K_TREE, 0.0);
-    if ((rc = tree_build (*c)) || (rc = tree_trilists (*c)) || (rc = tree_nbrlists (*c))) return rc;
+    if ((rc = rebuild (c))) return rc;
     rebuilds++;
   }
-  PMX_HIP (hipMemsetAsync (c->dflags.p, 0, FLAG_COUNT * sizeof (int), c->stream));
-  c->stats.max_row = 0;
   auto tl = std::chrono::steady_clock::now ();
-  long long k = 0;
-  for (double time = 0.0; time < duration; time += 0.5*(step0+step1), c->curtime += 0.5*(step0+step1), step0 = step1, k++)
+  /* with profiling on, every step runs eagerly and synchronises (events per
+   * launch); otherwise one step is captured as a graph and replayed in
+   * chunks, with the stream synchronised once per chunk */
+  const int chunk = c->prof.on ? 1 : 64;
+  int done = 0, failed_at = -1;
+  while (done < nsteps)
   {
-    if (c->E > 0)
+    int n = std::min (chunk, nsteps - done);
+    if (c->prof.on)
     {
-      int overflow = 0;
-      if ((rc = step_bin (*c, &overflow))) return rc;
-      if (overflow > 0)
-      {
-        {
-          ProfScope ps (*c, K_TREE, 0.0);
-          if ((rc = tree_build (*c)) || (rc = tree_trilists (*c)) || (rc = tree_nbrlists (*c))) return rc;
-        }
-        rebuilds++;
-        if ((rc = step_bin (*c, &overflow))) return rc;
-        if (overflow > 0) return fail (PMX_EREPART, "Repartitioning failed");
-      }
-      if ((rc = step_detect (*c))) return rc;
+      if ((rc = enqueue_step (*c, step0))) return rc;
     }
-    const double *gk = c->gtab.p + 3*(size_t) std::min<long long> (k, std::max (nsteps-1, 0));
-    const double *dk = c->dtab.p + 6*(size_t) std::min<long long> (k, std::max (nsteps-1, 0));
-    if ((rc = step_rows (*c, gk, nullptr, step0))) return rc;
-    if ((rc = step_springs (*c, step0))) return rc;
-    step1 = step0;
-    if ((rc = step_dynamics (*c, dk, nullptr, step0, step1))) return rc;
-    if ((rc = step_mesh_shapes (*c))) return rc;
+    else
+    {
+      if (!c->graph_valid && (rc = capture (c, step0))) return rc;
+      for (int q = 0; q < n; q++) PMX_HIP (hipGraphLaunch (c->gexec, c->stream));
+    }
+    int ctl[CTL_COUNT];
+    if ((rc = read_flags (c, ctl))) return rc;
+    done = ctl[CTL_KSTEP];
+    if (ctl[CTL_ABORT])
+    {
+      /* a leaf overflowed in step `done`: re-partition from the current
+       * centres and repeat that step (parmec.cpp:2420-2427) */
+      if (failed_at == done) return fail (PMX_EREPART, "Repartitioning failed");
+      failed_at = done;
+      if ((rc = rebuild (c))) return rc;
+      rebuilds++;
+    }
   }
-  /* surface errors of the last step */
-  if (c->E > 0)
-  {
-    int dummy = 0;
-    if ((rc = step_bin (*c, &dummy))) return rc;
-  }
-  PMX_HIP (hipStreamSynchronize (c->stream));
-  prof_resolve (*c);
+  /* time bookkeeping exactly as the reference loop accumulates it */
+  for (int k = 0; k < nsteps; k++) c->curtime += 0.5*(step0+step1);
+  c->cur = (c->cur0 + nsteps) & 1;
   c->curstep = step1;
   c->stepnum++;
   double dt = std::chrono::duration<double> (std::chrono::steady_clock::now () - t0).count ();
-  c->stats.steps = k;
+  c->stats.steps = nsteps;
   c->stats.rebuilds = rebuilds;
   c->stats.leaves = c->tree.nleaves;
   c->stats.seconds = std::chrono::duration<double> (std::chrono::steady_clock::now () - tl).count ();

@@ -15,6 +15,12 @@
 #define PMX_TPB 256            /* threads per block of the elementwise kernels */
 #define PMX_TRI_FLAG 0x80000000u  /* row key flag of a triangle partner */
 
+/* device control block of a dem call: abort flag (leaf overflow in the
+ * current step; every later kernel of the step is a no-op), completed steps
+ * (indexes the per-step tables and the contact-row ping-pong), bin hint
+ * validity (leafof holds leaves of the current tree) */
+enum { CTL_ABORT = 0, CTL_KSTEP, CTL_HINT, CTL_COUNT = 8 };
+
 /* device error/overflow flags, one int each */
 enum { FLAG_LEAF_OVERFLOW = 0, FLAG_ROW_OVERFLOW, FLAG_HIT_OVERFLOW, FLAG_EXTRA_OVERFLOW,
        FLAG_TRILIST_OVERFLOW, FLAG_MAXROW, FLAG_BAD_KIND, FLAG_COUNT };
@@ -136,6 +142,10 @@ struct pmx_ctx
   dvec<int> xbuf;              /* [xcap][E] reverse hits */
   dvec<int> xcnt;              /* [E] */
   dvec<int> dflags;            /* [FLAG_COUNT] */
+  dvec<int> ctl;               /* [CTL_COUNT] */
+  int cur0 = 0;                /* row buffer holding the rows at step 0 of the call */
+  hipGraphExec_t gexec = nullptr;
+  bool graph_valid = false;
   dvec<int> parell;            /* [N] the particle's sphere, or -1 */
   dvec<int> nosph;             /* particles without a sphere */
   dvec<signed char> iso;       /* [N] 1 when inertia is diag(j, j, j) with exact zeros */
@@ -180,13 +190,16 @@ int tree_trilists (pmx_ctx &c);              /* leaf -> triangle lists */
 int tree_nbrlists (pmx_ctx &c);              /* leaf -> neighbour leaf lists */
 
 /* ---- scans (pmx_tree.hip) ---- */
-int scan_exclusive (pmx_ctx &c, int *data, int n, int *total_host);
+int scan_exclusive (pmx_ctx &c, int *data, int n, int *total_host, const int *ctl = nullptr);
 
-/* ---- step kernels (pmx_step.hip) ---- */
-int step_bin (pmx_ctx &c, int *overflow);
+/* ---- step kernels (pmx_step.hip); step_* only enqueue work ---- */
+int step_prepare (pmx_ctx &c);               /* per-tree buffers, after a (re)build */
+int step_bin (pmx_ctx &c);
 int step_detect (pmx_ctx &c);
-int step_rows (pmx_ctx &c, const double *g_dev, const double g[3], double h);
-int step_dynamics (pmx_ctx &c, const double *d_dev, const double d[6], double step0, double step1);
+int step_rows (pmx_ctx &c, double h);
+int step_dynamics (pmx_ctx &c, double step0, double step1);
+int step_end (pmx_ctx &c);
+int enqueue_step (pmx_ctx &c, double h);     /* one whole time step */
 int step_shapes (pmx_ctx &c);
 int step_euler (pmx_ctx &c, double h);
 int step_invert (pmx_ctx &c);
@@ -195,6 +208,6 @@ int step_invert (pmx_ctx &c);
 int springs_setup (pmx_ctx &c, const pmx_model *m);
 int springs_download (pmx_ctx &c, pmx_model *m);
 int step_springs (pmx_ctx &c, double h);
-int step_mesh_shapes (pmx_ctx &c);
+int step_mesh_shapes (pmx_ctx &c, bool gated = true);
 
 #endif

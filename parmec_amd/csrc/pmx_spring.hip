@@ -142,10 +142,12 @@ struct SprArgs
   const double *angular, *linear, *rotation, *position, *inverse, *invm;
   double *out;   /* [9][S]: force, torque on part1, torque on part2 */
   int *flags;
+  const int *ctl;
 };
 
 __global__ void k_springs (SprArgs A, double step)
 {
+  if (A.ctl[CTL_ABORT]) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.S) return;
   size_t S = A.S, N = A.N;
@@ -372,8 +374,9 @@ __global__ void k_springs (SprArgs A, double step)
 
 /* per particle: its run flushes in the reference's order */
 __global__ void k_spring_acc (int N, int S, const int *evoff, const int *evrun, const int *runoff, const int *runspr,
-                              const double *out, double *force, double *torque)
+                              const double *out, double *force, double *torque, const int *ctl)
 {
+  if (ctl[CTL_ABORT]) return;
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= N) return;
   int e0 = evoff[b], e1 = evoff[b+1];
@@ -411,8 +414,10 @@ __global__ void k_spring_acc (int N, int S, const int *evoff, const int *evrun, 
 }
 
 /* nodes_task, shapes.ispc:101-186 */
-__global__ void k_nodes (int NOD, int N, const int *nodpart, double *nodes, const double *rotation, const double *position)
+__global__ void k_nodes (int NOD, int N, const int *nodpart, double *nodes, const double *rotation, const double *position,
+                         const int *ctl)
 {
+  if (ctl && ctl[CTL_ABORT]) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= NOD) return;
   int j = nodpart[i];
@@ -427,8 +432,10 @@ __global__ void k_nodes (int NOD, int N, const int *nodpart, double *nodes, cons
 }
 
 /* faces_task, shapes.ispc:189-211 */
-__global__ void k_faces (int FAC, int NOD, int T, const int *facnod, const int *factri, const double *nodes, double *tri)
+__global__ void k_faces (int FAC, int NOD, int T, const int *facnod, const int *factri, const double *nodes, double *tri,
+                         const int *ctl)
 {
+  if (ctl && ctl[CTL_ABORT]) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= FAC) return;
   int t = factri[i];
@@ -549,7 +556,7 @@ int step_springs (pmx_ctx &c, double h)
   A.dx = c.sp_dash.p; A.dy = c.sp_dash.p + c.dashcrv;
   A.ux = c.sp_unload.p; A.uy = c.sp_unload.p + c.uncrv;
   A.angular = c.angular.p; A.linear = c.linear.p; A.rotation = c.rotation.p; A.position = c.position.p;
-  A.inverse = c.inverse.p; A.invm = c.invm.p; A.out = c.sp_out.p; A.flags = c.dflags.p;
+  A.inverse = c.inverse.p; A.invm = c.invm.p; A.out = c.sp_out.p; A.flags = c.dflags.p; A.ctl = c.ctl.p;
   {
     ProfScope ps (c, K_SPRINGS, 420.0 * c.S);
     hipLaunchKernelGGL (k_springs, grid (c.S), dim3 (PMX_TPB), 0, c.stream, A, h);
@@ -559,20 +566,21 @@ int step_springs (pmx_ctx &c, double h)
     ProfScope ps (c, K_SPRING_ACC, 96.0 * c.N + 2 * 72.0 * c.S);
     hipLaunchKernelGGL (k_spring_acc, grid (c.N), dim3 (PMX_TPB), 0, c.stream, c.N, c.S, (const int*) c.sp_evoff.p,
                         (const int*) c.sp_evrun.p, (const int*) c.sp_runoff.p, (const int*) c.sp_runspr.p,
-                        (const double*) c.sp_out.p, c.force.p, c.torque.p);
+                        (const double*) c.sp_out.p, c.force.p, c.torque.p, (const int*) c.ctl.p);
   }
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
 }
 
-int step_mesh_shapes (pmx_ctx &c)
+int step_mesh_shapes (pmx_ctx &c, bool gated)
 {
   if (c.NOD == 0) return PMX_OK;
+  const int *ctl = gated ? (const int*) c.ctl.p : nullptr;
   hipLaunchKernelGGL (k_nodes, grid (c.NOD), dim3 (PMX_TPB), 0, c.stream, c.NOD, c.N, (const int*) c.nodpart.p,
-                      c.nodes.p, (const double*) c.rotation.p, (const double*) c.position.p);
+                      c.nodes.p, (const double*) c.rotation.p, (const double*) c.position.p, ctl);
   if (c.FAC > 0)
     hipLaunchKernelGGL (k_faces, grid (c.FAC), dim3 (PMX_TPB), 0, c.stream, c.FAC, c.NOD, c.T, (const int*) c.facnod.p,
-                        (const int*) c.factri.p, (const double*) c.nodes.p, c.tri.p);
+                        (const int*) c.factri.p, (const double*) c.nodes.p, c.tri.p, ctl);
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
 }

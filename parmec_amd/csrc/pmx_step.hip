@@ -36,17 +36,33 @@ namespace {
 
 inline dim3 grid (int n) { return dim3 ((n + PMX_TPB - 1) / PMX_TPB); }
 
-__global__ void k_fill_i (int *a, int n, int v)
+__global__ void k_fill_i (const int *ctl, int *a, int n, int v)
 {
+  if (ctl[CTL_ABORT]) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = v;
 }
 
+__global__ void k_copy_i (const int *ctl, int *dst, const int *src, int n)
+{
+  if (ctl[CTL_ABORT]) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+/* the step completed: advance the step counter */
+__global__ void k_step_end (int *ctl)
+{
+  if (threadIdx.x == 0 && !ctl[CTL_ABORT]) ctl[CTL_KSTEP] += 1;
+}
+
 /* partitioning_store: descend to the leaf, take a slot (partition.ispc:600-646) */
 __global__ void k_bin (int E, const double *center, int root, const int *left, const int *right, const int *dim,
-                       const double *coord, const double *cell, int nleaves, int hint, int *leafof, int *slot,
+                       const double *coord, const double *cell, int nleaves, int *ctl, int *leafof, int *slot,
                        int *lcount, int *flags)
 {
+  if (ctl[CTL_ABORT]) return;
+  int hint = ctl[CTL_HINT];
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   bool valid = e < E;
   int L = -1;
@@ -94,13 +110,19 @@ __global__ void k_bin (int E, const double *center, int root, const int *left, c
   if (!valid) return;
   leafof[e] = L;
   slot[e] = s;
-  if (s >= PMX_LSIZE) atomicAdd (&flags[FLAG_LEAF_OVERFLOW], 1);
+  if (s >= PMX_LSIZE)
+  {
+    atomicAdd (&flags[FLAG_LEAF_OVERFLOW], 1);
+    atomicExch (&ctl[CTL_ABORT], 1);   /* re-partition before this step goes on */
+  }
 }
 
-__global__ void k_scatter (int E, const double *center, const double *radii, const int *leafof, const int *slot,
+__global__ void k_scatter (int *ctl, int E, const double *center, const double *radii, const int *leafof, const int *slot,
                            const int *loff, int *lmem, double *lpos)
 {
+  if (ctl[CTL_ABORT]) return;
   int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e == 0) ctl[CTL_HINT] = 1;   /* the bin of this step completed: leafof is valid */
   if (e >= E) return;
   int pos = loff[leafof[e]] + slot[e];
   lmem[pos] = e;
@@ -125,8 +147,9 @@ void k_detect (int E, const double *center, const double *radii, const int *part
                const int *lcount, const int *loff, const int *lmem, const double *lpos,
                const double *tri, int T, const int *tricnt, const int *trioff, const int *trilist,
                const int *nbrcnt, const int *nbroff, const int *nbrlist,
-               uint32_t *hits, int *hitcnt, int hcap, int *xbuf, int *xcnt, int xcap, int *flags)
+               uint32_t *hits, int *hitcnt, int hcap, int *xbuf, int *xcnt, int xcap, int *flags, const int *ctl)
 {
+  if (ctl[CTL_ABORT]) return;
   __shared__ double srec[4*DET_TPB];   /* staged records of the candidate leaf */
   __shared__ int smem[DET_TPB];
   __shared__ double ssph[4*DET_TPB];   /* block spheres: x, y, z, rx */
@@ -291,9 +314,11 @@ void k_detect (int E, const double *center, const double *radii, const int *part
   (void) r0;
 }
 
-__global__ void k_force_init (int N, const int *list, int nlist, const double *mass, const double *gtab, int k,
+__global__ void k_force_init (const int *ctl, int N, const int *list, int nlist, const double *mass, const double *gtab,
                               double *force, double *torque)
 {
+  if (ctl[CTL_ABORT]) return;
+  int k = ctl[CTL_KSTEP];
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nlist) return;
   int i = list[q];
@@ -315,12 +340,20 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
              const double *tri, int T, const int *tricol, const int *triobs,
              const double *position, const double *linear, const double *angular, const double *mass,
              const double *invm, int pairnum, const int *pairs, const int *ikind, const double *iparam,
-             const uint32_t *okey, const double *ost, const int *ocnt,
-             uint32_t *nkey, double *nst, int *ncnt, int cap,
+             uint32_t *key0, double *st0, int *cnt0, uint32_t *key1, double *st1, int *cnt1, int cur0, int cap,
              const uint32_t *hits, const int *hitcnt, int hcap, const int *xbuf, const int *xcnt, int xcap,
-             const double *gtab, int kstep, double h, double *force, double *torque, int *flags)
+             const double *gtab, const int *ctl, double h, double *force, double *torque, int *flags)
 {
   extern __shared__ uint32_t sh[];
+  if (ctl[CTL_ABORT]) return;
+  int kstep = ctl[CTL_KSTEP];
+  bool odd = ((cur0 + kstep) & 1) != 0;   /* rows of the previous step */
+  const uint32_t *okey = odd ? key1 : key0;
+  const double *ost = odd ? st1 : st0;
+  const int *ocnt = odd ? cnt1 : cnt0;
+  uint32_t *nkey = odd ? key0 : key1;
+  double *nst = odd ? st0 : st1;
+  int *ncnt = odd ? cnt0 : cnt1;
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   int cnt = 0;
   if (e < E)
@@ -512,9 +545,11 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
 /* dynamics_task, dynamics.ispc:30-169 */
 __global__ void k_dynamics (int N, double *angular, double *linear, double *rotation, double *position,
                             const double *inertia, const double *inverse, const double *mass, const double *invm,
-                            const double *dtab, int kstep, double *force, double *torque, double step0, double step1,
+                            const double *dtab, const int *ctl, double *force, double *torque, double step0, double step1,
                             const int *parell, int E, double *center, const signed char *iso)
 {
+  if (ctl[CTL_ABORT]) return;
+  int kstep = ctl[CTL_KSTEP];
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const double *damping = dtab + 6*(size_t) kstep;
@@ -690,40 +725,29 @@ __global__ void k_invert (int N, const double *inertia, double *inverse, const d
 
 } /* namespace */
 
-int step_bin (pmx_ctx &c, int *overflow)
+int step_prepare (pmx_ctx &c)
+{
+  int L = std::max (c.tree.nleaves, 1);
+  if (c.lcount.alloc (L) || c.loff.alloc (L)) { pmx_set_error ("out of device memory (bins)"); return PMX_ENOMEM; }
+  /* size the scan scratch for the per-step leaf scan now: nothing may
+   * allocate while a step is being captured */
+  size_t need = (size_t) L / (PMX_TPB*4) + 64;
+  if (c.tree.scanbuf.alloc (need) != hipSuccess) { pmx_set_error ("out of device memory (scan)"); return PMX_ENOMEM; }
+  return PMX_OK;
+}
+
+int step_bin (pmx_ctx &c)
 {
   TreeDev &t = c.tree;
   int L = std::max (t.nleaves, 1);
-  if (c.lcount.alloc (L) || c.loff.alloc (L)) { pmx_set_error ("out of device memory (bins)"); return PMX_ENOMEM; }
-  hipLaunchKernelGGL (k_fill_i, grid (L), dim3 (PMX_TPB), 0, c.stream, c.lcount.p, L, 0);
-  hipLaunchKernelGGL (k_fill_i, dim3 (1), dim3 (64), 0, c.stream, c.dflags.p + FLAG_LEAF_OVERFLOW, 1, 0);
+  hipLaunchKernelGGL (k_fill_i, grid (L), dim3 (PMX_TPB), 0, c.stream, (const int*) c.ctl.p, c.lcount.p, L, 0);
   {
-  ProfScope ps (c, K_BIN, 32.0 * c.E);
-  hipLaunchKernelGGL (k_bin, grid (c.E), dim3 (PMX_TPB), 0, c.stream, c.E, (const double*) c.center.p, t.root,
-                      (const int*) t.left.p, (const int*) t.right.p, (const int*) t.dim.p, (const double*) t.coord.p,
-                      (const double*) t.cell.p, t.nleaves, c.bin_hint ? 1 : 0, c.leafof.p, c.slot.p, c.lcount.p,
-                      c.dflags.p);
+    ProfScope ps (c, K_BIN, 32.0 * c.E);
+    hipLaunchKernelGGL (k_bin, grid (c.E), dim3 (PMX_TPB), 0, c.stream, c.E, (const double*) c.center.p, t.root,
+                        (const int*) t.left.p, (const int*) t.right.p, (const int*) t.dim.p, (const double*) t.coord.p,
+                        (const double*) t.cell.p, t.nleaves, c.ctl.p, c.leafof.p, c.slot.p, c.lcount.p, c.dflags.p);
   }
   PMX_HIP (hipGetLastError ());
-  int fl[FLAG_COUNT];
-  PMX_HIP (hipMemcpyAsync (fl, c.dflags.p, sizeof (fl), hipMemcpyDeviceToHost, c.stream));
-  PMX_HIP (hipStreamSynchronize (c.stream));
-  prof_resolve (c);
-  *overflow = fl[FLAG_LEAF_OVERFLOW];
-  c.bin_hint = true;
-  if (fl[FLAG_ROW_OVERFLOW] || fl[FLAG_HIT_OVERFLOW] || fl[FLAG_EXTRA_OVERFLOW] || fl[FLAG_TRILIST_OVERFLOW])
-  {
-    pmx_set_error ("contact capacity exceeded (row " + std::to_string (fl[FLAG_ROW_OVERFLOW]) + ", hits " +
-                   std::to_string (fl[FLAG_HIT_OVERFLOW]) + ", reverse " + std::to_string (fl[FLAG_EXTRA_OVERFLOW]) +
-                   "); raise pmx_options.row_capacity");
-    return PMX_ECAPACITY;
-  }
-  if (fl[FLAG_BAD_KIND])
-  {
-    pmx_set_error ("unsupported interaction kind or inconsistent contact rows");
-    return PMX_EUNSUPPORTED;
-  }
-  c.stats.max_row = std::max (c.stats.max_row, fl[FLAG_MAXROW]);
   return PMX_OK;
 }
 
@@ -731,43 +755,44 @@ int step_detect (pmx_ctx &c)
 {
   TreeDev &t = c.tree;
   int L = std::max (t.nleaves, 1);
-  PMX_HIP (hipMemcpyAsync (c.loff.p, c.lcount.p, sizeof (int) * L, hipMemcpyDeviceToDevice, c.stream));
-  int rc = scan_exclusive (c, c.loff.p, L, nullptr);
+  hipLaunchKernelGGL (k_copy_i, grid (L), dim3 (PMX_TPB), 0, c.stream, (const int*) c.ctl.p, c.loff.p,
+                      (const int*) c.lcount.p, L);
+  int rc = scan_exclusive (c, c.loff.p, L, nullptr, c.ctl.p);
   if (rc) return rc;
   {
-  ProfScope ps (c, K_SCATTER, 76.0 * c.E);
-  hipLaunchKernelGGL (k_scatter, grid (c.E), dim3 (PMX_TPB), 0, c.stream, c.E, (const double*) c.center.p,
-                      (const double*) c.radii.p, (const int*) c.leafof.p, (const int*) c.slot.p, (const int*) c.loff.p,
-                      c.lmem.p, c.lpos.p);
+    ProfScope ps (c, K_SCATTER, 76.0 * c.E);
+    hipLaunchKernelGGL (k_scatter, grid (c.E), dim3 (PMX_TPB), 0, c.stream, c.ctl.p, c.E, (const double*) c.center.p,
+                        (const double*) c.radii.p, (const int*) c.leafof.p, (const int*) c.slot.p, (const int*) c.loff.p,
+                        c.lmem.p, c.lpos.p);
   }
-  hipLaunchKernelGGL (k_fill_i, grid (c.E), dim3 (PMX_TPB), 0, c.stream, c.xcnt.p, c.E, 0);
-  ProfScope ps (c, K_DETECT, 68.0 * c.E);
-  hipLaunchKernelGGL (k_detect, dim3 (t.nleaves), dim3 (DET_TPB), 0, c.stream, c.E, (const double*) c.center.p,
-                      (const double*) c.radii.p, (const int*) c.part.p, t.root, (const int*) t.left.p,
-                      (const int*) t.right.p, (const int*) t.dim.p, (const double*) t.coord.p, (const double*) t.cell.p,
-                      t.nleaves, (const int*) c.lcount.p, (const int*) c.loff.p,
-                      (const int*) c.lmem.p, (const double*) c.lpos.p, (const double*) c.tri.p, c.T,
-                      (const int*) t.tricnt.p, (const int*) t.trioff.p, (const int*) t.trilist.p,
-                      (const int*) t.nbrcnt.p, (const int*) t.nbroff.p, (const int*) t.nbrlist.p,
-                      c.hits.p, c.hitcnt.p, c.hcap, c.xbuf.p, c.xcnt.p, c.xcap, c.dflags.p);
+  hipLaunchKernelGGL (k_fill_i, grid (c.E), dim3 (PMX_TPB), 0, c.stream, (const int*) c.ctl.p, c.xcnt.p, c.E, 0);
+  {
+    ProfScope ps (c, K_DETECT, 68.0 * c.E);
+    hipLaunchKernelGGL (k_detect, dim3 (t.nleaves), dim3 (DET_TPB), 0, c.stream, c.E, (const double*) c.center.p,
+                        (const double*) c.radii.p, (const int*) c.part.p, t.root, (const int*) t.left.p,
+                        (const int*) t.right.p, (const int*) t.dim.p, (const double*) t.coord.p, (const double*) t.cell.p,
+                        t.nleaves, (const int*) c.lcount.p, (const int*) c.loff.p,
+                        (const int*) c.lmem.p, (const double*) c.lpos.p, (const double*) c.tri.p, c.T,
+                        (const int*) t.tricnt.p, (const int*) t.trioff.p, (const int*) t.trilist.p,
+                        (const int*) t.nbrcnt.p, (const int*) t.nbroff.p, (const int*) t.nbrlist.p,
+                        c.hits.p, c.hitcnt.p, c.hcap, c.xbuf.p, c.xcnt.p, c.xcap, c.dflags.p, (const int*) c.ctl.p);
+  }
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
 }
 
-int step_rows (pmx_ctx &c, const double *g_dev, const double *g, double h)
+int step_rows (pmx_ctx &c, double h)
 {
-  (void) g;
-  int kstep = 0;
-  int o = c.cur, n = 1 - c.cur;
   if (c.nnosph > 0)
   {
     ProfScope ps (c, K_FORCE_INIT, 60.0 * c.nnosph);
-    hipLaunchKernelGGL (k_force_init, grid (c.nnosph), dim3 (PMX_TPB), 0, c.stream, c.N, (const int*) c.nosph.p,
-                        c.nnosph, (const double*) c.mass.p, g_dev, kstep, c.force.p, c.torque.p);
+    hipLaunchKernelGGL (k_force_init, grid (c.nnosph), dim3 (PMX_TPB), 0, c.stream, (const int*) c.ctl.p, c.N,
+                        (const int*) c.nosph.p, c.nnosph, (const double*) c.mass.p, (const double*) c.gtab.p,
+                        c.force.p, c.torque.p);
   }
   if (c.E > 0)
   {
-    ProfScope ps (c, K_ROWS, 212.0 * c.E);
+    ProfScope ps (c, K_ROWS, 136.0 * c.E);
     size_t shm = (size_t) ROWS_TPB * (c.hcap + c.xcap) * sizeof (uint32_t);
     hipLaunchKernelGGL (k_rows, dim3 ((c.E + ROWS_TPB - 1) / ROWS_TPB), dim3 (ROWS_TPB), shm, c.stream, c.E, c.N,
                         (const double*) c.center.p, (const double*) c.radii.p, (const int*) c.part.p,
@@ -775,25 +800,40 @@ int step_rows (pmx_ctx &c, const double *g_dev, const double *g, double h)
                         (const int*) c.triobs.p, (const double*) c.position.p, (const double*) c.linear.p,
                         (const double*) c.angular.p, (const double*) c.mass.p, (const double*) c.invm.p, c.P,
                         (const int*) c.pairs.p, (const int*) c.ikind.p, (const double*) c.iparam.p,
-                        (const uint32_t*) c.rowkey[o].p, (const double*) c.rowst[o].p, (const int*) c.rowcnt[o].p,
-                        c.rowkey[n].p, c.rowst[n].p, c.rowcnt[n].p, c.cap, (const uint32_t*) c.hits.p,
+                        c.rowkey[0].p, c.rowst[0].p, c.rowcnt[0].p, c.rowkey[1].p, c.rowst[1].p, c.rowcnt[1].p,
+                        c.cur0, c.cap, (const uint32_t*) c.hits.p,
                         (const int*) c.hitcnt.p, c.hcap, (const int*) c.xbuf.p, (const int*) c.xcnt.p, c.xcap,
-                        g_dev, kstep, h, c.force.p, c.torque.p, c.dflags.p);
+                        (const double*) c.gtab.p, (const int*) c.ctl.p, h, c.force.p, c.torque.p, c.dflags.p);
   }
   PMX_HIP (hipGetLastError ());
-  c.cur = n;
   return PMX_OK;
 }
 
-int step_dynamics (pmx_ctx &c, const double *d_dev, const double *d, double step0, double step1)
+int step_dynamics (pmx_ctx &c, double step0, double step1)
 {
-  (void) d;
   ProfScope ps (c, K_DYNAMICS, 384.0 * c.N + 128.0 * (c.N - c.niso) + 56.0 * c.E);
   hipLaunchKernelGGL (k_dynamics, grid (c.N), dim3 (PMX_TPB), 0, c.stream, c.N, c.angular.p, c.linear.p, c.rotation.p,
                       c.position.p, (const double*) c.inertia.p, (const double*) c.inverse.p, (const double*) c.mass.p,
-                      (const double*) c.invm.p, d_dev, 0, c.force.p, c.torque.p, step0, step1,
-                      (const int*) c.parell.p, c.E, c.center.p, (const signed char*) c.iso.p);
+                      (const double*) c.invm.p, (const double*) c.dtab.p, (const int*) c.ctl.p, c.force.p, c.torque.p,
+                      step0, step1, (const int*) c.parell.p, c.E, c.center.p, (const signed char*) c.iso.p);
   PMX_HIP (hipGetLastError ());
+  return PMX_OK;
+}
+
+int step_end (pmx_ctx &c)
+{
+  hipLaunchKernelGGL (k_step_end, dim3 (1), dim3 (64), 0, c.stream, c.ctl.p);
+  PMX_HIP (hipGetLastError ());
+  return PMX_OK;
+}
+
+int enqueue_step (pmx_ctx &c, double h)
+{
+  int rc;
+  if (c.E > 0 && ((rc = step_bin (c)) || (rc = step_detect (c)))) return rc;
+  if ((rc = step_rows (c, h)) || (rc = step_springs (c, h)) || (rc = step_dynamics (c, h, h)) ||
+      (rc = step_mesh_shapes (c)) || (rc = step_end (c)))
+    return rc;
   return PMX_OK;
 }
 

@@ -44,8 +44,9 @@ __device__ __forceinline__ int delta (int i, unsigned codei, int j, int n, const
 
 /* ---------------------------------------------------------------- scan */
 
-__global__ void k_scan_block (int *data, int n, int *bsums)
+__global__ void k_scan_block (int *data, int n, int *bsums, const int *ctl)
 {
+  if (ctl && ctl[CTL_ABORT]) return;
   __shared__ int wsum[PMX_TPB/64];
   int base = blockIdx.x * (PMX_TPB*4) + threadIdx.x * 4;
   int v[4], s = 0;
@@ -69,28 +70,29 @@ __global__ void k_scan_block (int *data, int n, int *bsums)
   if (bsums && threadIdx.x == PMX_TPB-1) bsums[blockIdx.x] = wpre + x;
 }
 
-__global__ void k_scan_add (int *data, int n, const int *boffs)
+__global__ void k_scan_add (int *data, int n, const int *boffs, const int *ctl)
 {
+  if (ctl && ctl[CTL_ABORT]) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) data[i] += boffs[i / (PMX_TPB*4)];
 }
 
-int scan_rec (pmx_ctx &c, int *data, int n, int *scratch, size_t scratch_n)
+int scan_rec (pmx_ctx &c, int *data, int n, int *scratch, size_t scratch_n, const int *ctl)
 {
   const int per = PMX_TPB*4;
   int nb = (n + per - 1) / per;
   if (nb <= 1)
   {
-    hipLaunchKernelGGL (k_scan_block, dim3 (1), dim3 (PMX_TPB), 0, c.stream, data, n, (int*) nullptr);
+    hipLaunchKernelGGL (k_scan_block, dim3 (1), dim3 (PMX_TPB), 0, c.stream, data, n, (int*) nullptr, ctl);
     PMX_HIP (hipGetLastError ());
     return PMX_OK;
   }
   if ((size_t) nb > scratch_n) { pmx_set_error ("scan scratch too small"); return PMX_ENOMEM; }
-  hipLaunchKernelGGL (k_scan_block, dim3 (nb), dim3 (PMX_TPB), 0, c.stream, data, n, scratch);
+  hipLaunchKernelGGL (k_scan_block, dim3 (nb), dim3 (PMX_TPB), 0, c.stream, data, n, scratch, ctl);
   PMX_HIP (hipGetLastError ());
-  int rc = scan_rec (c, scratch, nb, scratch + nb, scratch_n - nb);
+  int rc = scan_rec (c, scratch, nb, scratch + nb, scratch_n - nb, ctl);
   if (rc) return rc;
-  hipLaunchKernelGGL (k_scan_add, dim3 ((n + PMX_TPB - 1)/PMX_TPB), dim3 (PMX_TPB), 0, c.stream, data, n, (const int*) scratch);
+  hipLaunchKernelGGL (k_scan_add, dim3 ((n + PMX_TPB - 1)/PMX_TPB), dim3 (PMX_TPB), 0, c.stream, data, n, (const int*) scratch, ctl);
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
 }
@@ -461,15 +463,16 @@ inline dim3 grid (int n) { return dim3 ((n + PMX_TPB - 1) / PMX_TPB); }
 
 } /* namespace */
 
-int scan_exclusive (pmx_ctx &c, int *data, int n, int *total_host)
+int scan_exclusive (pmx_ctx &c, int *data, int n, int *total_host, const int *ctl)
 {
   if (n <= 0) { if (total_host) *total_host = 0; return PMX_OK; }
   size_t need = (size_t) n / (PMX_TPB*4) + 64;
+  if (need > c.tree.scanbuf.n && ctl) { pmx_set_error ("scan scratch not prepared"); return PMX_ENOMEM; }
   if (c.tree.scanbuf.alloc (need) != hipSuccess) { pmx_set_error ("out of device memory (scan)"); return PMX_ENOMEM; }
   int last = 0;
   if (total_host)
     PMX_HIP (hipMemcpyAsync (&last, data + n - 1, sizeof (int), hipMemcpyDeviceToHost, c.stream));
-  int rc = scan_rec (c, data, n, c.tree.scanbuf.p, c.tree.scanbuf.n);
+  int rc = scan_rec (c, data, n, c.tree.scanbuf.p, c.tree.scanbuf.n, ctl);
   if (rc) return rc;
   if (total_host)
   {

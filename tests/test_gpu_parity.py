@@ -82,3 +82,36 @@ def test_hopper_2m_short(engines):
     (mg, cg, sg), (mc, cc, sc) = _both(engines, m, 6 * 2e-6 - 1e-6, 2e-6, f)
     assert sg["steps"] == 6
     compare(mg, cg, mc, cc)
+
+
+def test_eager_profiled_path_matches_graph_path(engines):
+    """With profiling on every step runs eagerly (events per launch); without,
+    one captured step is replayed as a graph.  Both must give the oracle's
+    state."""
+    from oracle import orc
+    m = workloads.box_lattice(n=4000, seed=3, spacing_factor=2.05, jitter=0.02)
+    rng = np.random.default_rng(7)
+    m.linear[:] = rng.uniform(-1.0, 1.0, size=m.linear.shape)
+    f = pmx.forcing(gravity=(0.0, 0.0, -9.81))
+    g = pmx.Engine()
+    g.lib.pmx_set_profiling(g.h, 1)
+    mg, cg, sg = run(g, m, 150 * 1e-5 - 0.5e-5, 1e-5, f, calls=2)
+    g.close()
+    c = orc.Oracle(0)
+    mc, cc, sc = run(c, m, 150 * 1e-5 - 0.5e-5, 1e-5, f, calls=2)
+    c.close()
+    compare(mg, cg, mc, cc)
+
+
+def test_leaf_overflow_repartition(engines):
+    """Spheres converge on the centre so inner leaves exceed 96 entries:
+    the tree is rebuilt from the current centres and the step repeated
+    (parmec.cpp:2420-2427), identically on both sides."""
+    print("case converging flow (re-partitioning)", flush=True)
+    m = workloads.box_lattice(n=4000, seed=3)
+    xc = m.center[:3].mean(axis=1, keepdims=True)
+    m.linear[:] = -75.0 * (m.center[:3] - xc)
+    (mg, cg, sg), (mc, cc, sc) = _both(engines, m, 200 * 1e-5 - 0.5e-5, 1e-5, pmx.forcing())
+    assert sc["rebuilds"] > 5
+    assert sg["rebuilds"] == sc["rebuilds"]
+    compare(mg, cg, mc, cc)
