@@ -203,6 +203,27 @@ int pmx_dem (pmx_ctx *ctx, double duration, double step, const pmx_forcing *forc
 /* Copy the device state back into the caller's view. */
 int pmx_download (pmx_ctx *ctx, pmx_model *model);
 
+/*
+ * Batches: count independent models stepped together (many small instances
+ * fill the GPU where one would not).  The models are concatenated on the
+ * device; each instance keeps its own partitioning tree, pairing table and
+ * forcing, and its result is bitwise the result of stepping it alone.  All
+ * instances must share curtime, curstep and stepnum.  No reference
+ * counterpart: Parmec runs one model per process (parmec.h:36-283 globals),
+ * and a sweep is a loop of separate runs (python/acc_sweep.py:23-131).
+ */
+int pmx_upload_batch (pmx_ctx *ctx, const pmx_model *models, int count);
+
+/* dem(duration, step) for every instance; forcing holds one entry per
+ * instance (or NULL: no gravity, no damping). */
+int pmx_dem_batch (pmx_ctx *ctx, double duration, double step, const pmx_forcing *forcing, double *elapsed);
+
+/* Copy every instance's state back into its own view. */
+int pmx_download_batch (pmx_ctx *ctx, pmx_model *models, int count);
+
+/* Contact pairs of one instance, numbered as in that instance's model. */
+int pmx_contacts_instance (pmx_ctx *ctx, int instance, pmx_pair *out, size_t cap, size_t *count);
+
 /* Canonical contact pair set, sorted by (master, slave). */
 int pmx_contacts (pmx_ctx *ctx, pmx_pair *out, size_t cap, size_t *count);
 

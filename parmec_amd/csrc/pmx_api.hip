@@ -57,6 +57,146 @@ int alloc_rows (pmx_ctx &c)
 
 } /* namespace */
 
+/* checks a single model (one instance) for what the device path supports */
+int validate_model (const pmx_model *m)
+{
+  if (m->parnum < 0 || m->ellnum < 0 || m->trinum < 0 || m->pairnum < 1)
+    return fail (PMX_EINVAL, "invalid counts (pairnum must include the default pairing)");
+  if (m->obsnum > 0) return fail (PMX_EUNSUPPORTED, "moving obstacles are not supported on the device path yet");
+  for (int i = 0; i < m->sprnum; i++)
+  {
+    if (m->sproffset && m->sproffset[i] != -1) return fail (PMX_EUNSUPPORTED, "spring stroke offsets are not supported");
+    if (m->unspring[i] >= 0) return fail (PMX_EUNSUPPORTED, "UNSPRING load curves are not supported");
+    if (m->sprpart[i] < 0 || m->sprpart[i] >= m->parnum || m->sprpart[m->sprnum + i] < -1 ||
+        m->sprpart[m->sprnum + i] >= m->parnum) return fail (PMX_EINVAL, "spring particle index out of range");
+    if (i && (m->sprpart[i-1] > m->sprpart[i] ||
+              (m->sprpart[i-1] == m->sprpart[i] && m->sprpart[m->sprnum + i-1] > m->sprpart[m->sprnum + i])))
+      return fail (PMX_EINVAL, "springs must be sorted by (part1, part2)");
+  }
+  for (int e = 0; e < m->ellnum; e++)
+  {
+    if (!(m->radii[(size_t) m->ellnum + e] < 0.0)) return fail (PMX_EUNSUPPORTED, "only spherical ellipsoids are supported");
+    if (m->part[e] < 0 || m->part[e] >= m->parnum) return fail (PMX_EINVAL, "ellipsoid particle index out of range");
+  }
+  for (int t = 0; t < m->trinum; t++)
+  {
+    if (m->triobs[t] <= -2) return fail (PMX_EUNSUPPORTED, "moving obstacle triangles are not supported");
+    if (m->triobs[t] >= m->parnum) return fail (PMX_EINVAL, "triangle particle index out of range");
+    if (m->triobs[t] >= 0 && m->ellnum > 0)
+      return fail (PMX_EUNSUPPORTED, "contact between spheres and meshed particles is not supported yet");
+  }
+  for (int q = 0; q < m->nodnum; q++)
+    if (m->nodpart[q] < 0 || m->nodpart[q] >= m->parnum) return fail (PMX_EINVAL, "node particle index out of range");
+  for (int f = 0; f < m->facnum; f++)
+  {
+    if (m->factri[f] < 0 || m->factri[f] >= m->trinum) return fail (PMX_EINVAL, "face triangle index out of range");
+    for (int k = 0; k < 3; k++)
+      if (m->facnod[(size_t) k*m->facnum + f] < 0 || m->facnod[(size_t) k*m->facnum + f] >= m->nodnum)
+        return fail (PMX_EINVAL, "face node index out of range");
+  }
+  std::vector<char> seen (std::max (m->parnum, 1), 0);
+  for (int e = 0; e < m->ellnum; e++)
+  {
+    if (seen[m->part[e]]) return fail (PMX_EUNSUPPORTED, "a particle may own at most one sphere on this path");
+    seen[m->part[e]] = 1;
+  }
+  return PMX_OK;
+}
+
+/* item counts of one model as offsets past its end */
+InstOff model_counts (const pmx_model &m)
+{
+  InstOff o;
+  o.par = m.parnum; o.ell = m.ellnum; o.tri = m.trinum; o.nod = m.nodnum; o.fac = m.facnum; o.spr = m.sprnum;
+  o.scrv = m.sprcrv; o.dcrv = m.dashcrv; o.ucrv = m.uncrv; o.pair = m.pairnum;
+  return o;
+}
+
+/* copy a (possibly concatenated) model to the device; off[i] are the
+ * instance offsets, off[I] the totals */
+int upload_core (pmx_ctx *c, const pmx_model *m, const std::vector<InstOff> &off)
+{
+  int I = (int) off.size () - 1;
+  std::vector<int> parell (std::max (m->parnum, 1), -1), nosph;
+  double rmax = 0.0;
+  for (int e = 0; e < m->ellnum; e++)
+    for (int k = 0; k < 3; k++) rmax = std::max (rmax, m->radii[(size_t) k*m->ellnum + e]);
+  for (int e = 0; e < m->ellnum; e++) parell[m->part[e]] = e;
+  for (int i = 0; i < m->parnum; i++) if (parell[i] < 0) nosph.push_back (i);
+  std::vector<signed char> iso (std::max (m->parnum, 1), 0);
+  int niso = 0;
+  for (int i = 0; i < m->parnum; i++)
+  {
+    const double *J = m->inertia;
+    size_t N = m->parnum;
+    bool d = J[i] > 0.0 && J[i] == J[4*N + i] && J[i] == J[8*N + i];
+    for (int k : {1, 2, 3, 5, 6, 7}) d = d && J[k*N + i] == 0.0 && !std::signbit (J[k*N + i]);
+    iso[i] = d ? 1 : 0;
+    niso += d;
+  }
+  std::vector<int> pinst (std::max (m->parnum, 1)), einst (std::max (m->ellnum, 1)), tinst (std::max (m->trinum, 1));
+  std::vector<int> eoff (I + 1), poff (I + 1);
+  for (int i = 0; i < I; i++)
+  {
+    for (int q = off[i].par; q < off[i+1].par; q++) pinst[q] = i;
+    for (int q = off[i].ell; q < off[i+1].ell; q++) einst[q] = i;
+    for (int q = off[i].tri; q < off[i+1].tri; q++) tinst[q] = i;
+  }
+  for (int i = 0; i <= I; i++) { eoff[i] = off[i].ell; poff[i] = off[i].pair; }
+  (void) hipSetDevice (c->device);
+  bool same = c->uploaded && c->N == m->parnum && c->E == m->ellnum && c->T == m->trinum && c->I == I;
+  for (int i = 0; same && i <= I; i++) same = c->off[i].ell == off[i].ell && c->off[i].tri == off[i].tri;
+  c->I = I;
+  c->off = off;
+  c->N = m->parnum; c->E = m->ellnum; c->T = m->trinum; c->P = m->pairnum;
+  c->NOD = m->nodnum; c->FAC = m->facnum;
+  size_t N = c->N, E = c->E, T = c->T, P = c->P;
+  int rc = 0;
+  if ((rc = up (*c, c->angular, m->angular, 6*N, "angular")) || (rc = up (*c, c->linear, m->linear, 3*N, "linear")) ||
+      (rc = up (*c, c->rotation, m->rotation, 9*N, "rotation")) || (rc = up (*c, c->position, m->position, 6*N, "position")) ||
+      (rc = up (*c, c->inertia, m->inertia, 9*N, "inertia")) || (rc = up (*c, c->inverse, m->inverse, 9*N, "inverse")) ||
+      (rc = up (*c, c->mass, m->mass, N, "mass")) || (rc = up (*c, c->invm, m->invm, N, "invm")) ||
+      (rc = up (*c, c->center, m->center, 6*E, "center")) || (rc = up (*c, c->radii, m->radii, 3*E, "radii")) ||
+      (rc = up (*c, c->part, m->part, E, "part")) || (rc = up (*c, c->ellcol, m->ellcol, E, "ellcol")) ||
+      (rc = up (*c, c->tri, m->tri, 9*T, "tri")) || (rc = up (*c, c->tricol, m->tricol, T, "tricol")) ||
+      (rc = up (*c, c->triobs, m->triobs, T, "triobs")) || (rc = up (*c, c->pairs, m->pairs, 2*P, "pairs")) ||
+      (rc = up (*c, c->ikind, m->ikind, P, "ikind")) || (rc = up (*c, c->iparam, m->iparam, (size_t) PMX_NIPARAM*P, "iparam")))
+    return rc;
+  if ((rc = up (*c, c->par_inst, pinst.data (), N, "instances")) || (rc = up (*c, c->ell_inst, einst.data (), E, "instances")) ||
+      (rc = up (*c, c->tri_inst, tinst.data (), T, "instances")) ||
+      (rc = up (*c, c->ell_off, eoff.data (), (size_t) I + 1, "instances")) ||
+      (rc = up (*c, c->pair_off, poff.data (), (size_t) I + 1, "instances")))
+    return rc;
+  if (c->force.alloc (3*std::max (N, (size_t) 1)) || c->torque.alloc (3*std::max (N, (size_t) 1)) ||
+      c->ovf.alloc (std::max (I, 1)))
+    return fail (PMX_ENOMEM, "out of device memory");
+  PMX_HIP (hipMemsetAsync (c->ovf.p, 0, sizeof (int) * std::max (I, 1), c->stream));
+  c->nnosph = (int) nosph.size ();
+  c->niso = niso;
+  if ((rc = up (*c, c->iso, iso.data (), N, "iso"))) return rc;
+  if ((rc = up (*c, c->parell, parell.data (), N, "parell")) ||
+      (rc = up (*c, c->nosph, nosph.data (), nosph.size (), "nosph")))
+    return rc;
+  c->bin_hint = false;
+  c->rmax = rmax;
+  if (m->force) PMX_HIP (hipMemcpyAsync (c->force.p, m->force, 3*N*sizeof (double), hipMemcpyHostToDevice, c->stream));
+  if (m->torque) PMX_HIP (hipMemcpyAsync (c->torque.p, m->torque, 3*N*sizeof (double), hipMemcpyHostToDevice, c->stream));
+  if (!same)
+  {
+    if ((rc = alloc_rows (*c))) return rc;
+  }
+  if ((rc = up (*c, c->nodes, m->nodes, 6*(size_t) c->NOD, "nodes")) ||
+      (rc = up (*c, c->nodpart, m->nodpart, (size_t) c->NOD, "nodpart")) ||
+      (rc = up (*c, c->facnod, m->facnod, 3*(size_t) c->FAC, "facnod")) ||
+      (rc = up (*c, c->factri, m->factri, (size_t) c->FAC, "factri")) ||
+      (rc = springs_setup (*c, m)))
+    return rc;
+  c->curtime = m->curtime; c->curstep = m->curstep; c->stepnum = m->stepnum;
+  PMX_HIP (hipStreamSynchronize (c->stream));
+  c->uploaded = true;
+  return PMX_OK;
+}
+
 void prof_resolve (pmx_ctx &c)
 {
   for (auto &p : c.prof.pend)
@@ -147,93 +287,11 @@ void pmx_destroy (pmx_ctx *c)
 int pmx_upload (pmx_ctx *c, const pmx_model *m)
 {
   if (!c || !m) return fail (PMX_EINVAL, "null argument");
-  if (m->parnum < 0 || m->ellnum < 0 || m->trinum < 0 || m->pairnum < 1)
-    return fail (PMX_EINVAL, "invalid counts (pairnum must include the default pairing)");
-  if (m->obsnum > 0) return fail (PMX_EUNSUPPORTED, "moving obstacles are not supported on the device path yet");
-  for (int i = 0; i < m->sprnum; i++)
-  {
-    if (m->sproffset && m->sproffset[i] != -1) return fail (PMX_EUNSUPPORTED, "spring stroke offsets are not supported");
-    if (m->unspring[i] >= 0) return fail (PMX_EUNSUPPORTED, "UNSPRING load curves are not supported");
-    if (m->sprpart[i] < 0 || m->sprpart[i] >= m->parnum || m->sprpart[m->sprnum + i] < -1 ||
-        m->sprpart[m->sprnum + i] >= m->parnum) return fail (PMX_EINVAL, "spring particle index out of range");
-    if (i && (m->sprpart[i-1] > m->sprpart[i] ||
-              (m->sprpart[i-1] == m->sprpart[i] && m->sprpart[m->sprnum + i-1] > m->sprpart[m->sprnum + i])))
-      return fail (PMX_EINVAL, "springs must be sorted by (part1, part2)");
-  }
-  for (int e = 0; e < m->ellnum; e++)
-  {
-    if (!(m->radii[(size_t) m->ellnum + e] < 0.0)) return fail (PMX_EUNSUPPORTED, "only spherical ellipsoids are supported");
-    if (m->part[e] < 0 || m->part[e] >= m->parnum) return fail (PMX_EINVAL, "ellipsoid particle index out of range");
-  }
-  for (int t = 0; t < m->trinum; t++)
-  {
-    if (m->triobs[t] <= -2) return fail (PMX_EUNSUPPORTED, "moving obstacle triangles are not supported");
-    if (m->triobs[t] >= 0 && m->ellnum > 0)
-      return fail (PMX_EUNSUPPORTED, "contact between spheres and meshed particles is not supported yet");
-  }
-  std::vector<int> parell (std::max (m->parnum, 1), -1), nosph;
-  double rmax = 0.0;
-  for (int e = 0; e < m->ellnum; e++)
-    for (int k = 0; k < 3; k++) rmax = std::max (rmax, m->radii[(size_t) k*m->ellnum + e]);
-  for (int e = 0; e < m->ellnum; e++)
-  {
-    if (parell[m->part[e]] >= 0) return fail (PMX_EUNSUPPORTED, "a particle may own at most one sphere on this path");
-    parell[m->part[e]] = e;
-  }
-  for (int i = 0; i < m->parnum; i++) if (parell[i] < 0) nosph.push_back (i);
-  std::vector<signed char> iso (std::max (m->parnum, 1), 0);
-  int niso = 0;
-  for (int i = 0; i < m->parnum; i++)
-  {
-    const double *J = m->inertia;
-    size_t N = m->parnum;
-    bool d = J[i] > 0.0 && J[i] == J[4*N + i] && J[i] == J[8*N + i];
-    for (int k : {1, 2, 3, 5, 6, 7}) d = d && J[k*N + i] == 0.0 && !std::signbit (J[k*N + i]);
-    iso[i] = d ? 1 : 0;
-    niso += d;
-  }
-  (void) hipSetDevice (c->device);
-  bool same = c->uploaded && c->N == m->parnum && c->E == m->ellnum && c->T == m->trinum;
-  c->N = m->parnum; c->E = m->ellnum; c->T = m->trinum; c->P = m->pairnum;
-  c->NOD = m->nodnum; c->FAC = m->facnum;
-  size_t N = c->N, E = c->E, T = c->T, P = c->P;
-  int rc = 0;
-  if ((rc = up (*c, c->angular, m->angular, 6*N, "angular")) || (rc = up (*c, c->linear, m->linear, 3*N, "linear")) ||
-      (rc = up (*c, c->rotation, m->rotation, 9*N, "rotation")) || (rc = up (*c, c->position, m->position, 6*N, "position")) ||
-      (rc = up (*c, c->inertia, m->inertia, 9*N, "inertia")) || (rc = up (*c, c->inverse, m->inverse, 9*N, "inverse")) ||
-      (rc = up (*c, c->mass, m->mass, N, "mass")) || (rc = up (*c, c->invm, m->invm, N, "invm")) ||
-      (rc = up (*c, c->center, m->center, 6*E, "center")) || (rc = up (*c, c->radii, m->radii, 3*E, "radii")) ||
-      (rc = up (*c, c->part, m->part, E, "part")) || (rc = up (*c, c->ellcol, m->ellcol, E, "ellcol")) ||
-      (rc = up (*c, c->tri, m->tri, 9*T, "tri")) || (rc = up (*c, c->tricol, m->tricol, T, "tricol")) ||
-      (rc = up (*c, c->triobs, m->triobs, T, "triobs")) || (rc = up (*c, c->pairs, m->pairs, 2*P, "pairs")) ||
-      (rc = up (*c, c->ikind, m->ikind, P, "ikind")) || (rc = up (*c, c->iparam, m->iparam, (size_t) PMX_NIPARAM*P, "iparam")))
-    return rc;
-  if (c->force.alloc (3*std::max (N, (size_t) 1)) || c->torque.alloc (3*std::max (N, (size_t) 1)))
-    return fail (PMX_ENOMEM, "out of device memory");
-  c->nnosph = (int) nosph.size ();
-  c->niso = niso;
-  if ((rc = up (*c, c->iso, iso.data (), N, "iso"))) return rc;
-  if ((rc = up (*c, c->parell, parell.data (), N, "parell")) ||
-      (rc = up (*c, c->nosph, nosph.data (), nosph.size (), "nosph")))
-    return rc;
-  c->bin_hint = false;
-  c->rmax = rmax;
-  if (m->force) PMX_HIP (hipMemcpyAsync (c->force.p, m->force, 3*N*sizeof (double), hipMemcpyHostToDevice, c->stream));
-  if (m->torque) PMX_HIP (hipMemcpyAsync (c->torque.p, m->torque, 3*N*sizeof (double), hipMemcpyHostToDevice, c->stream));
-  if (!same)
-  {
-    if ((rc = alloc_rows (*c))) return rc;
-  }
-  if ((rc = up (*c, c->nodes, m->nodes, 6*(size_t) c->NOD, "nodes")) ||
-      (rc = up (*c, c->nodpart, m->nodpart, (size_t) c->NOD, "nodpart")) ||
-      (rc = up (*c, c->facnod, m->facnod, 3*(size_t) c->FAC, "facnod")) ||
-      (rc = up (*c, c->factri, m->factri, (size_t) c->FAC, "factri")) ||
-      (rc = springs_setup (*c, m)))
-    return rc;
-  c->curtime = m->curtime; c->curstep = m->curstep; c->stepnum = m->stepnum;
-  PMX_HIP (hipStreamSynchronize (c->stream));
-  c->uploaded = true;
-  return PMX_OK;
+  int rc = validate_model (m);
+  if (rc) return rc;
+  std::vector<InstOff> off (2);
+  off[1] = model_counts (*m);
+  return upload_core (c, m, off);
 }
 
 int pmx_download (pmx_ctx *c, pmx_model *m)
@@ -258,12 +316,13 @@ int pmx_download (pmx_ctx *c, pmx_model *m)
 
 /* build the partitioning tree and its per-leaf lists from the current
  * centres (partitioning_create, partition.ispc:663-746) */
-static int rebuild (pmx_ctx *c)
+static int rebuild (pmx_ctx *c, bool all)
 {
   int rc;
   ProfScope ps (*c, K_TREE, 0.0);
-  if ((rc = tree_build (*c)) || (rc = tree_trilists (*c)) || (rc = tree_nbrlists (*c)) || (rc = step_prepare (*c)))
+  if ((rc = tree_build (*c, all)) || (rc = tree_trilists (*c)) || (rc = tree_nbrlists (*c)) || (rc = step_prepare (*c)))
     return rc;
+  PMX_HIP (hipMemsetAsync (c->ovf.p, 0, sizeof (int) * std::max (c->I, 1), c->stream));
   int zero[2] = {0, 0};
   PMX_HIP (hipMemcpyAsync (c->ctl.p + CTL_HINT, zero, sizeof (int), hipMemcpyHostToDevice, c->stream));
   PMX_HIP (hipMemcpyAsync (c->ctl.p + CTL_ABORT, zero, sizeof (int), hipMemcpyHostToDevice, c->stream));
@@ -305,7 +364,9 @@ static int capture (pmx_ctx *c, double h)
   return PMX_OK;
 }
 
-int pmx_dem (pmx_ctx *c, double duration, double step, const pmx_forcing *fc, double *elapsed)
+/* dem() over every instance of the context; fc holds nfc entries: one
+ * forcing for all instances, or one per instance */
+static int dem_core (pmx_ctx *c, double duration, double step, const pmx_forcing *fc, int nfc, double *elapsed)
 {
   if (!c) return fail (PMX_EINVAL, "null ctx");
   if (!c->uploaded) return fail (PMX_EINVAL, "pmx_upload must precede pmx_dem");
@@ -323,15 +384,21 @@ int pmx_dem (pmx_ctx *c, double duration, double step, const pmx_forcing *fc, do
   step1 = step0;
   if ((rc = step_invert (*c))) return rc;
 
-  /* per-step tables of the host-evaluated inputs (input.cpp:4424-4503) */
+  /* per-step tables of the host-evaluated inputs (input.cpp:4424-4503),
+   * laid out [step][instance][component] */
   int nsteps = 0;
   pmx_step_times (duration, step0, c->curtime, nullptr, 0, &nsteps);
   {
-    std::vector<double> g (3 * (size_t) std::max (nsteps, 1)), d (6 * (size_t) std::max (nsteps, 1));
-    for (int k = 0; k < nsteps; k++)
+    size_t I = c->I, ns = std::max (nsteps, 1);
+    std::vector<double> g (3 * ns * I, 0.0), d (6 * ns * I, 0.0);
+    for (size_t i = 0; i < I; i++)
     {
-      for (int q = 0; q < 3; q++) g[3*(size_t) k + q] = fc && fc->gtab ? fc->gtab[3*(size_t) k + q] : (fc ? fc->gravity[q] : 0.0);
-      for (int q = 0; q < 6; q++) d[6*(size_t) k + q] = fc && fc->dtab ? fc->dtab[6*(size_t) k + q] : (fc ? fc->damping[q] : 0.0);
+      const pmx_forcing *f = fc ? fc + (nfc > 1 ? i : 0) : nullptr;
+      for (int k = 0; k < nsteps; k++)
+      {
+        for (int q = 0; q < 3; q++) g[3*((size_t) k*I + i) + q] = f && f->gtab ? f->gtab[3*(size_t) k + q] : (f ? f->gravity[q] : 0.0);
+        for (int q = 0; q < 6; q++) d[6*((size_t) k*I + i) + q] = f && f->dtab ? f->dtab[6*(size_t) k + q] : (f ? f->damping[q] : 0.0);
+      }
     }
     if ((rc = up (*c, c->gtab, g.data (), g.size (), "gravity table")) || (rc = up (*c, c->dtab, d.data (), d.size (), "damping table")))
       return rc;
@@ -346,7 +413,7 @@ int pmx_dem (pmx_ctx *c, double duration, double step, const pmx_forcing *fc, do
   long long rebuilds = 0;
   if (c->E > 0)
   {
-    if ((rc = rebuild (c))) return rc;
+    if ((rc = rebuild (c, true))) return rc;
     rebuilds++;
   }
   auto tl = std::chrono::steady_clock::now ();
@@ -376,7 +443,7 @@ int pmx_dem (pmx_ctx *c, double duration, double step, const pmx_forcing *fc, do
        * centres and repeat that step (parmec.cpp:2420-2427) */
       if (failed_at == done) return fail (PMX_EREPART, "Repartitioning failed");
       failed_at = done;
-      if ((rc = rebuild (c))) return rc;
+      if ((rc = rebuild (c, false))) return rc;
       rebuilds++;
     }
   }
@@ -402,6 +469,17 @@ int pmx_dem (pmx_ctx *c, double duration, double step, const pmx_forcing *fc, do
   }
   if (elapsed) *elapsed = dt;
   return PMX_OK;
+}
+
+int pmx_dem (pmx_ctx *c, double duration, double step, const pmx_forcing *fc, double *elapsed)
+{
+  return dem_core (c, duration, step, fc, 1, elapsed);
+}
+
+int pmx_dem_batch (pmx_ctx *c, double duration, double step, const pmx_forcing *fc, double *elapsed)
+{
+  if (!c) return fail (PMX_EINVAL, "null ctx");
+  return dem_core (c, duration, step, fc, c->I, elapsed);
 }
 
 int pmx_contacts (pmx_ctx *c, pmx_pair *out, size_t cap, size_t *count)

@@ -1,0 +1,254 @@
+/*
+ * pmx_batch.hip -- many independent models stepped together.
+ *
+ * A batch is the concatenation of its instances' SoA arrays: every item
+ * family (particles, spheres, triangles, nodes, faces, springs, curve points,
+ * pairings) is laid out instance after instance, and every index an instance
+ * stores (ellipsoid -> particle, triangle -> particle, node -> particle,
+ * face -> node and triangle, spring -> particle and curve point) is shifted by
+ * the offset of the family it points into.  The step kernels then run once
+ * over the whole batch; what keeps the instances apart is
+ *   - a partitioning tree per instance (segmented build), so no contact is
+ *     ever formed across instances and each instance's tree is exactly the
+ *     tree it would have alone (rebuilt only when that instance overflows);
+ *   - per-instance pairing tables, gravity and damping tables;
+ *   - per-instance spring flush sequences.
+ * Contact keys of an instance are global ellipsoid/triangle numbers, shifted
+ * by the same offsets, so their order is the order the instance has alone and
+ * every per-body sum adds the same terms in the same order: each instance's
+ * result is bitwise the result of stepping it alone.
+ */
+#include "pmx_internal.h"
+#include <algorithm>
+#include <cstring>
+
+namespace {
+
+int fail (int code, const std::string &msg) { pmx_set_error (msg); return code; }
+
+/* host copy of a concatenated model: owns the arrays the view points to */
+struct Merged
+{
+  pmx_model m {};
+  std::vector<std::vector<double>> dv;
+  std::vector<std::vector<int>> iv;
+  double *D (size_t n) { dv.emplace_back (std::max (n, (size_t) 1), 0.0); return dv.back ().data (); }
+  int *Iv (size_t n) { iv.emplace_back (std::max (n, (size_t) 1), 0); return iv.back ().data (); }
+};
+
+/* component-major k-component arrays: dst[c*total + off_i + j] = src_i[c*n_i + j] (+ shift) */
+template <class T, class Get, class Cnt, class Off, class Map>
+void cat (T *dst, int k, int count, size_t total, Get get, Cnt cnt, Off off, Map map)
+{
+  for (int i = 0; i < count; i++)
+  {
+    const T *src = get (i);
+    size_t n = cnt (i), o = off (i);
+    if (!src || n == 0) continue;
+    for (int c = 0; c < k; c++)
+      for (size_t j = 0; j < n; j++) dst[c*total + o + j] = map (src[c*n + j], i);
+  }
+}
+
+template <class T, class Get, class Cnt, class Off>
+void split (const T *src, int k, int count, size_t total, Get get, Cnt cnt, Off off)
+{
+  for (int i = 0; i < count; i++)
+  {
+    T *dst = get (i);
+    size_t n = cnt (i), o = off (i);
+    if (!dst || n == 0) continue;
+    for (int c = 0; c < k; c++)
+      std::memcpy (dst + c*n, src + c*total + o, n * sizeof (T));
+  }
+}
+
+} /* namespace */
+
+extern "C" {
+
+int pmx_upload_batch (pmx_ctx *c, const pmx_model *ms, int count)
+{
+  if (!c || !ms || count < 1) return fail (PMX_EINVAL, "null argument or empty batch");
+  for (int i = 0; i < count; i++)
+  {
+    int rc = validate_model (&ms[i]);
+    if (rc) return fail (rc, "instance " + std::to_string (i) + ": " + pmx_last_error ());
+    if (ms[i].curtime != ms[0].curtime || ms[i].curstep != ms[0].curstep || ms[i].stepnum != ms[0].stepnum)
+      return fail (PMX_EINVAL, "instances of a batch must share curtime, curstep and stepnum");
+  }
+  std::vector<InstOff> off (count + 1);
+  for (int i = 0; i < count; i++)
+  {
+    InstOff n = model_counts (ms[i]), &a = off[i], &b = off[i+1];
+    b.par = a.par + n.par; b.ell = a.ell + n.ell; b.tri = a.tri + n.tri; b.nod = a.nod + n.nod;
+    b.fac = a.fac + n.fac; b.spr = a.spr + n.spr; b.scrv = a.scrv + n.scrv; b.dcrv = a.dcrv + n.dcrv;
+    b.ucrv = a.ucrv + n.ucrv; b.pair = a.pair + n.pair;
+    if ((long long) a.ell + n.ell >= (long long) PMX_TRI_FLAG || (long long) a.tri + n.tri >= (long long) PMX_TRI_FLAG)
+      return fail (PMX_EINVAL, "batch too large");
+  }
+  const InstOff &tot = off[count];
+  Merged g;
+  pmx_model &m = g.m;
+  auto id = [] (auto v, int) { return v; };
+  auto P = [&] (int i) { return (size_t) ms[i].parnum; };
+  auto Po = [&] (int i) { return (size_t) off[i].par; };
+  auto E = [&] (int i) { return (size_t) ms[i].ellnum; };
+  auto Eo = [&] (int i) { return (size_t) off[i].ell; };
+  auto T = [&] (int i) { return (size_t) ms[i].trinum; };
+  auto To = [&] (int i) { return (size_t) off[i].tri; };
+  auto Nd = [&] (int i) { return (size_t) ms[i].nodnum; };
+  auto Ndo = [&] (int i) { return (size_t) off[i].nod; };
+  auto F = [&] (int i) { return (size_t) ms[i].facnum; };
+  auto Fo = [&] (int i) { return (size_t) off[i].fac; };
+  auto S = [&] (int i) { return (size_t) ms[i].sprnum; };
+  auto So = [&] (int i) { return (size_t) off[i].spr; };
+  auto Q = [&] (int i) { return (size_t) ms[i].pairnum; };
+  auto Qo = [&] (int i) { return (size_t) off[i].pair; };
+  auto shp = [&] (int v, int i) { return v + off[i].par; };
+  auto shp1 = [&] (int v, int i) { return v >= 0 ? v + off[i].par : v; };
+  size_t NP = tot.par, NE = tot.ell, NT = tot.tri, NN = tot.nod, NF = tot.fac, NS = tot.spr, NQ = tot.pair;
+
+#define CATD(field, k, cnt, o, tot_) { m.field = g.D ((k) * (tot_)); \
+    cat<double> (m.field, k, count, tot_, [&] (int i) { return (const double*) ms[i].field; }, cnt, o, id); }
+#define CATI(field, k, cnt, o, tot_, map) { m.field = g.Iv ((k) * (tot_)); \
+    cat<int> (m.field, k, count, tot_, [&] (int i) { return (const int*) ms[i].field; }, cnt, o, map); }
+
+  m.parnum = (int) NP;
+  CATD (angular, 6, P, Po, NP); CATD (linear, 3, P, Po, NP); CATD (rotation, 9, P, Po, NP);
+  CATD (position, 6, P, Po, NP); CATD (inertia, 9, P, Po, NP); CATD (inverse, 9, P, Po, NP);
+  CATD (mass, 1, P, Po, NP); CATD (invm, 1, P, Po, NP);
+  CATI (flags, 1, P, Po, NP, id);
+  bool haveforce = true;
+  for (int i = 0; i < count; i++) haveforce = haveforce && (ms[i].parnum == 0 || (ms[i].force && ms[i].torque));
+  if (haveforce) { CATD (force, 3, P, Po, NP); CATD (torque, 3, P, Po, NP); }
+  m.ellnum = (int) NE;
+  CATD (center, 6, E, Eo, NE); CATD (radii, 3, E, Eo, NE);
+  CATI (part, 1, E, Eo, NE, shp); CATI (ellcol, 1, E, Eo, NE, id);
+  m.trinum = (int) NT;
+  CATD (tri, 9, T, To, NT); CATI (tricol, 1, T, To, NT, id); CATI (triobs, 1, T, To, NT, shp1);
+  m.nodnum = (int) NN;
+  CATD (nodes, 6, Nd, Ndo, NN); CATI (nodpart, 1, Nd, Ndo, NN, shp);
+  m.facnum = (int) NF;
+  CATI (facnod, 3, F, Fo, NF, [&] (int v, int i) { return v + off[i].nod; });
+  CATI (factri, 1, F, Fo, NF, [&] (int v, int i) { return v + off[i].tri; });
+  m.pairnum = (int) NQ;
+  /* colour pairs are (i, j) couples, row-major */
+  m.pairs = g.Iv (2*NQ);
+  for (int i = 0; i < count; i++)
+    if (ms[i].pairnum) std::memcpy (m.pairs + 2*off[i].pair, ms[i].pairs, 2*sizeof (int)*ms[i].pairnum);
+  CATI (ikind, 1, Q, Qo, NQ, id); CATD (iparam, PMX_NIPARAM, Q, Qo, NQ);
+  m.sprnum = (int) NS;
+  if (NS > 0)
+  {
+    CATI (sprtype, 1, S, So, NS, id); CATI (sprpart, 2, S, So, NS, shp1);
+    CATD (sprpnt, 12, S, So, NS); CATD (yield, 2, S, So, NS); CATD (sprdir, 6, S, So, NS);
+    CATI (sprflg, 1, S, So, NS, id); CATD (sprfric, 1, S, So, NS); CATD (sprkskn, 1, S, So, NS);
+    CATD (sprsdsp, 3, S, So, NS); CATD (stroke0, 1, S, So, NS); CATD (stroke, 3, S, So, NS);
+    CATD (sprfrc, 3, S, So, NS); CATI (unspring, 1, S, So, NS, id);
+    m.sproffset = nullptr;
+    /* curve index ranges [sprnum+1] and the curve tables [2][points] */
+    m.spridx = g.Iv (NS + 1); m.dashidx = g.Iv (NS + 1); m.unidx = g.Iv (NS + 1);
+    for (int i = 0; i < count; i++)
+      for (int s = 0; s < ms[i].sprnum; s++)
+      {
+        m.spridx[off[i].spr + s] = ms[i].spridx[s] + off[i].scrv;
+        m.dashidx[off[i].spr + s] = ms[i].dashidx[s] + off[i].dcrv;
+        m.unidx[off[i].spr + s] = ms[i].unidx[s] + off[i].ucrv;
+      }
+    m.spridx[NS] = tot.scrv; m.dashidx[NS] = tot.dcrv; m.unidx[NS] = tot.ucrv;
+    m.sprcrv = tot.scrv; m.dashcrv = tot.dcrv; m.uncrv = tot.ucrv;
+    auto SC = [&] (int i) { return (size_t) ms[i].sprcrv; };
+    auto SCo = [&] (int i) { return (size_t) off[i].scrv; };
+    auto DC = [&] (int i) { return (size_t) ms[i].dashcrv; };
+    auto DCo = [&] (int i) { return (size_t) off[i].dcrv; };
+    auto UC = [&] (int i) { return (size_t) ms[i].uncrv; };
+    auto UCo = [&] (int i) { return (size_t) off[i].ucrv; };
+    CATD (spring, 2, SC, SCo, (size_t) tot.scrv);
+    CATD (dashpot, 2, DC, DCo, (size_t) tot.dcrv);
+    CATD (unload, 2, UC, UCo, (size_t) tot.ucrv);
+  }
+#undef CATD
+#undef CATI
+  m.curtime = ms[0].curtime; m.curstep = ms[0].curstep; m.stepnum = ms[0].stepnum;
+  return upload_core (c, &m, off);
+}
+
+int pmx_download_batch (pmx_ctx *c, pmx_model *ms, int count)
+{
+  if (!c || !ms) return fail (PMX_EINVAL, "null argument");
+  if (count != c->I) return fail (PMX_EINVAL, "instance count differs from the uploaded batch");
+  const std::vector<InstOff> &off = c->off;
+  for (int i = 0; i < count; i++)
+  {
+    InstOff n = model_counts (ms[i]);
+    if (n.par != off[i+1].par - off[i].par || n.ell != off[i+1].ell - off[i].ell || n.tri != off[i+1].tri - off[i].tri ||
+        n.nod != off[i+1].nod - off[i].nod || n.spr != off[i+1].spr - off[i].spr)
+      return fail (PMX_EINVAL, "instance " + std::to_string (i) + ": counts differ from the uploaded state");
+  }
+  const InstOff &tot = off[count];
+  size_t NP = tot.par, NE = tot.ell, NT = tot.tri, NN = tot.nod, NS = tot.spr;
+  Merged g;
+  pmx_model &m = g.m;
+  m.parnum = (int) NP; m.ellnum = (int) NE; m.trinum = (int) NT; m.nodnum = (int) NN; m.sprnum = (int) NS;
+  m.angular = g.D (6*NP); m.linear = g.D (3*NP); m.rotation = g.D (9*NP); m.position = g.D (6*NP);
+  m.inverse = g.D (9*NP); m.invm = g.D (NP); m.force = g.D (3*NP); m.torque = g.D (3*NP);
+  m.center = g.D (6*NE); m.tri = g.D (9*NT); m.nodes = g.D (6*NN);
+  if (NS > 0)
+  {
+    m.sprpnt = g.D (12*NS); m.sprdir = g.D (6*NS); m.yield = g.D (2*NS); m.sprsdsp = g.D (3*NS);
+    m.stroke = g.D (3*NS); m.sprfrc = g.D (3*NS); m.sprflg = g.Iv (NS);
+  }
+  int rc = pmx_download (c, &m);
+  if (rc) return rc;
+  auto P = [&] (int i) { return (size_t) ms[i].parnum; };
+  auto Po = [&] (int i) { return (size_t) off[i].par; };
+  auto E = [&] (int i) { return (size_t) ms[i].ellnum; };
+  auto Eo = [&] (int i) { return (size_t) off[i].ell; };
+  auto T = [&] (int i) { return (size_t) ms[i].trinum; };
+  auto To = [&] (int i) { return (size_t) off[i].tri; };
+  auto Nd = [&] (int i) { return (size_t) ms[i].nodnum; };
+  auto Ndo = [&] (int i) { return (size_t) off[i].nod; };
+  auto S = [&] (int i) { return (size_t) ms[i].sprnum; };
+  auto So = [&] (int i) { return (size_t) off[i].spr; };
+#define SPD(field, k, cnt, o, tot_) split<double> (m.field, k, count, tot_, [&] (int i) { return ms[i].field; }, cnt, o)
+  SPD (angular, 6, P, Po, NP); SPD (linear, 3, P, Po, NP); SPD (rotation, 9, P, Po, NP); SPD (position, 6, P, Po, NP);
+  SPD (inverse, 9, P, Po, NP); SPD (invm, 1, P, Po, NP); SPD (force, 3, P, Po, NP); SPD (torque, 3, P, Po, NP);
+  SPD (center, 6, E, Eo, NE); SPD (tri, 9, T, To, NT); SPD (nodes, 6, Nd, Ndo, NN);
+  if (NS > 0)
+  {
+    SPD (sprpnt, 12, S, So, NS); SPD (sprdir, 6, S, So, NS); SPD (yield, 2, S, So, NS); SPD (sprsdsp, 3, S, So, NS);
+    SPD (stroke, 3, S, So, NS); SPD (sprfrc, 3, S, So, NS);
+    split<int> (m.sprflg, 1, count, NS, [&] (int i) { return ms[i].sprflg; }, S, So);
+  }
+#undef SPD
+  for (int i = 0; i < count; i++) { ms[i].curtime = m.curtime; ms[i].curstep = m.curstep; ms[i].stepnum = m.stepnum; }
+  return PMX_OK;
+}
+
+/* the contact pairs of one instance, in the instance's own numbering */
+int pmx_contacts_instance (pmx_ctx *c, int inst, pmx_pair *out, size_t cap, size_t *count)
+{
+  if (!c) return fail (PMX_EINVAL, "null ctx");
+  if (inst < 0 || inst >= c->I) return fail (PMX_EINVAL, "instance out of range");
+  size_t all = 0;
+  int rc = pmx_contacts (c, nullptr, 0, &all);
+  if (rc) return rc;
+  std::vector<pmx_pair> v (all);
+  if ((rc = pmx_contacts (c, v.data (), all, &all))) return rc;
+  int e0 = c->off[inst].ell, e1 = c->off[inst+1].ell, t0 = c->off[inst].tri;
+  size_t k = 0;
+  for (const pmx_pair &p : v)
+  {
+    if (p.master < e0 || p.master >= e1) continue;
+    pmx_pair q = p;
+    q.master -= e0;
+    q.slave = p.slave >= 0 ? p.slave - e0 : -(-p.slave - 1 - t0) - 1;
+    if (out && k < cap) out[k] = q;
+    k++;
+  }
+  if (count) *count = k;
+  return PMX_OK;
+}
+
+} /* extern "C" */

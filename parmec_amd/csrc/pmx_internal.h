@@ -6,6 +6,7 @@
 #define PMX_INTERNAL_H
 
 #include <hip/hip_runtime.h>
+#include <climits>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -14,6 +15,7 @@
 
 #define PMX_TPB 256            /* threads per block of the elementwise kernels */
 #define PMX_TRI_FLAG 0x80000000u  /* row key flag of a triangle partner */
+#define PMX_NO_ROOT INT_MIN       /* tree root of an instance without spheres */
 
 /* device control block of a dem call: abort flag (leaf overflow in the
  * current step; every later kernel of the step is a no-op), completed steps
@@ -52,10 +54,12 @@ struct TreeDev
 {
   int n = 0;                 /* points the tree was built over */
   int nleaves = 0;
-  int root = -1;             /* node reference of the root */
+  int root = -1;             /* node reference of the root of instance 0 */
+  dvec<int> roots;           /* [I] node reference of each instance's root */
+  dvec<int> leafinst;        /* [nleaves] instance of each leaf */
   /* scratch of the build */
-  dvec<double> ext;          /* [6] extents; partials follow */
-  dvec<uint32_t> code, code2;
+  dvec<double> ext;          /* [6][I] extents (single instance: partials follow) */
+  dvec<uint64_t> code, code2; /* (instance << 30) | Morton code */
   dvec<int> order, order2;
   dvec<int> hist;            /* radix histogram / scan scratch */
   dvec<int> scanbuf;         /* block sums for scans */
@@ -71,6 +75,12 @@ struct TreeDev
   int ntrilist = 0;
   dvec<int> nbrcnt, nbroff, nbrlist;  /* leaf -> leaves overlapping its cell inflated by rmax */
   int nnbr = 0;
+};
+
+/* offsets of one instance's items in the concatenated arrays of a batch */
+struct InstOff
+{
+  int par = 0, ell = 0, tri = 0, nod = 0, fac = 0, spr = 0, scrv = 0, dcrv = 0, ucrv = 0, pair = 0;
 };
 
 /* kernels timed by the profiler */
@@ -102,6 +112,16 @@ struct pmx_ctx
   hipStream_t stream = nullptr;
   int device = 0;
   int N = 0, E = 0, T = 0, P = 0, NOD = 0, FAC = 0;
+  /* instances stepped together (1 for a single model); each owns a
+   * contiguous range of particles, spheres and triangles and its own tree */
+  int I = 1;
+  std::vector<InstOff> off;    /* [I+1] prefix offsets of each item family */
+  dvec<int> par_inst, ell_inst, tri_inst, ell_off, pair_off;
+  dvec<double> bcen;           /* [3][E] centres the current tree was built from */
+  dvec<int> ovf;               /* [I] leaf overflow of the instance in the aborted step */
+  double curtime0 = 0.0;       /* per-instance time bookkeeping */
+  std::vector<double> icurtime, icurstep;
+  std::vector<int> istepnum;
 
   /* particles */
   dvec<double> angular, linear, rotation, position, inertia, inverse, mass, invm, force, torque;
@@ -184,8 +204,14 @@ struct ProfScope
 /* resolve pending event pairs (call after the stream is synchronised) */
 void prof_resolve (pmx_ctx &c);
 
+/* ---- upload (pmx_api.hip) ---- */
+int validate_model (const pmx_model *m);
+InstOff model_counts (const pmx_model &m);
+int upload_core (pmx_ctx *c, const pmx_model *m, const std::vector<InstOff> &off);
+
 /* ---- tree (pmx_tree.hip) ---- */
-int tree_build (pmx_ctx &c);                 /* from current centres */
+int tree_build (pmx_ctx &c, bool all);       /* all: snapshot every instance's centres; else only
+                                                the instances flagged in c.ovf */
 int tree_trilists (pmx_ctx &c);              /* leaf -> triangle lists */
 int tree_nbrlists (pmx_ctx &c);              /* leaf -> neighbour leaf lists */
 
@@ -205,7 +231,7 @@ int step_euler (pmx_ctx &c, double h);
 int step_invert (pmx_ctx &c);
 
 /* ---- springs and meshes (pmx_spring.hip) ---- */
-int springs_setup (pmx_ctx &c, const pmx_model *m);
+int springs_setup (pmx_ctx &c, const pmx_model *m);   /* per-instance ranges from c.off */
 int springs_download (pmx_ctx &c, pmx_model *m);
 int step_springs (pmx_ctx &c, double h);
 int step_mesh_shapes (pmx_ctx &c, bool gated = true);

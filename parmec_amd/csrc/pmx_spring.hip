@@ -455,40 +455,41 @@ int springs_setup (pmx_ctx &c, const pmx_model *m)
   int S = m->sprnum;
   c.S = S;
   if (S == 0) return PMX_OK;
-  /* run table of the single-task flush order (forces.ispc:1203-1411) */
+  /* run table of the single-task flush order (forces.ispc:1203-1411); in a
+   * batch every instance's springs form their own sequence */
   struct Ev { int pos, sub, body, code; };
   std::vector<Ev> ev;
   std::vector<int> runoff (1, 0), runspr;
   int nrun = 0;
   const int *p1 = m->sprpart, *p2 = m->sprpart + S;
-  bool firstj = true, firstk = true;
-  for (int i = 0; i < S; )
+  for (int inst = 0; inst < c.I; inst++)
   {
-    int e = i;
-    while (e < S && p1[e] == p1[i]) e++;
-    for (int q = i; q < e; q++) runspr.push_back (q);
-    runoff.push_back ((int) runspr.size ());
-    ev.push_back ({e, 0, p1[i], (nrun << 2) | (firstj ? 2 : 0)});
-    firstj = false;
-    nrun++;
-    i = e;
-  }
-  {
-    int cur = -1, start = (int) runspr.size ();
-    for (int i = 0; i <= S; i++)
+    int s0 = c.off[inst].spr, s1 = c.off[inst+1].spr;
+    bool firstj = true, firstk = true;
+    for (int i = s0; i < s1; )
     {
-      if (i < S && p2[i] < 0) continue;
-      int k = i < S ? p2[i] : -2;
+      int e = i;
+      while (e < s1 && p1[e] == p1[i]) e++;
+      for (int q = i; q < e; q++) runspr.push_back (q);
+      runoff.push_back ((int) runspr.size ());
+      ev.push_back ({e, 0, p1[i], (nrun << 2) | (firstj ? 2 : 0)});
+      firstj = false;
+      nrun++;
+      i = e;
+    }
+    int cur = -1;
+    for (int i = s0; i <= s1; i++)
+    {
+      if (i < s1 && p2[i] < 0) continue;
+      int k = i < s1 ? p2[i] : -2;
       if (cur >= 0 && k != cur)
       {
         runoff.push_back ((int) runspr.size ());
         ev.push_back ({i, 1, cur, (nrun << 2) | 1 | (firstk ? 2 : 0)});
         firstk = false;
         nrun++;
-        start = (int) runspr.size ();
       }
-      (void) start;
-      if (i < S) { runspr.push_back (i); cur = k; }
+      if (i < s1) { runspr.push_back (i); cur = k; }
     }
   }
   std::stable_sort (ev.begin (), ev.end (), [] (const Ev &a, const Ev &b) {

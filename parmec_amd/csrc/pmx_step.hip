@@ -57,9 +57,9 @@ __global__ void k_step_end (int *ctl)
 }
 
 /* partitioning_store: descend to the leaf, take a slot (partition.ispc:600-646) */
-__global__ void k_bin (int E, const double *center, int root, const int *left, const int *right, const int *dim,
-                       const double *coord, const double *cell, int nleaves, int *ctl, int *leafof, int *slot,
-                       int *lcount, int *flags)
+__global__ void k_bin (int E, const double *center, const int *roots, const int *ell_inst, const int *left,
+                       const int *right, const int *dim, const double *coord, const double *cell, int nleaves, int *ctl,
+                       int *leafof, int *slot, int *lcount, int *flags, int *ovf)
 {
   if (ctl[CTL_ABORT]) return;
   int hint = ctl[CTL_HINT];
@@ -82,7 +82,7 @@ __global__ void k_bin (int E, const double *center, int root, const int *left, c
     }
     if (L < 0)
     {
-      int ref = root;
+      int ref = roots[ell_inst[e]];
       while (ref >= 0)
       {
         int d = dim[ref];
@@ -113,7 +113,8 @@ __global__ void k_bin (int E, const double *center, int root, const int *left, c
   if (s >= PMX_LSIZE)
   {
     atomicAdd (&flags[FLAG_LEAF_OVERFLOW], 1);
-    atomicExch (&ctl[CTL_ABORT], 1);   /* re-partition before this step goes on */
+    atomicExch (&ovf[ell_inst[e]], 1);   /* the instance whose tree is rebuilt */
+    atomicExch (&ctl[CTL_ABORT], 1);     /* re-partition before this step goes on */
   }
 }
 
@@ -142,8 +143,7 @@ __global__ void k_scatter (int *ctl, int E, const double *center, const double *
  * spreads the (sphere, record) tests over all threads.  Hit slots per sphere
  * come from LDS counters; the hit order is irrelevant (rows sort their keys). */
 __global__ __launch_bounds__(DET_TPB)
-void k_detect (int E, const double *center, const double *radii, const int *part, int root, const int *left,
-               const int *right, const int *dim, const double *coord, const double *cell, int nleaves,
+void k_detect (int E, const double *center, const double *radii, const int *part, const double *cell, int nleaves,
                const int *lcount, const int *loff, const int *lmem, const double *lpos,
                const double *tri, int T, const int *tricnt, const int *trioff, const int *trilist,
                const int *nbrcnt, const int *nbroff, const int *nbrlist,
@@ -315,7 +315,7 @@ void k_detect (int E, const double *center, const double *radii, const int *part
 }
 
 __global__ void k_force_init (const int *ctl, int N, const int *list, int nlist, const double *mass, const double *gtab,
-                              double *force, double *torque)
+                              int ninst, const int *par_inst, double *force, double *torque)
 {
   if (ctl[CTL_ABORT]) return;
   int k = ctl[CTL_KSTEP];
@@ -323,7 +323,7 @@ __global__ void k_force_init (const int *ctl, int N, const int *list, int nlist,
   if (q >= nlist) return;
   int i = list[q];
   double ma = mass[i];
-  const double *g = gtab + 3*(size_t) k;
+  const double *g = gtab + 3*((size_t) k*ninst + (ninst > 1 ? par_inst[i] : 0));
   force[i] = 0.0 + ma * g[0];
   force[(size_t) N + i] = 0.0 + ma * g[1];
   force[2*(size_t) N + i] = 0.0 + ma * g[2];
@@ -340,6 +340,7 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
              const double *tri, int T, const int *tricol, const int *triobs,
              const double *position, const double *linear, const double *angular, const double *mass,
              const double *invm, int pairnum, const int *pairs, const int *ikind, const double *iparam,
+             int ninst, const int *ell_inst, const int *pair_off,
              uint32_t *key0, double *st0, int *cnt0, uint32_t *key1, double *st1, int *cnt1, int cur0, int cap,
              const uint32_t *hits, const int *hitcnt, int hcap, const int *xbuf, const int *xcnt, int xcap,
              const double *gtab, const int *ctl, double h, double *force, double *torque, int *flags)
@@ -379,6 +380,9 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
   double pc[3] = {center[e], center[(size_t) E + e], center[2*(size_t) E + e]};
   double re = radii[e];
   int oc = ocnt[e];
+  int inst = ninst > 1 ? ell_inst[e] : 0;
+  int po = 0, pn = pairnum;   /* the instance's pairing table */
+  if (ninst > 1) { po = pair_off[inst]; pn = pair_off[inst+1] - po; }
   int io = 0, ih = 0;
   double fs[3] = {0.0, 0.0, 0.0}, ts[3] = {0.0, 0.0, 0.0};
 
@@ -493,7 +497,7 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
     }
     else { vj[0] = vj[1] = vj[2] = 0.0; }
     double vij[3] = {vj[0]-vi[0], vj[1]-vi[1], vj[2]-vi[2]};
-    int ij = pairing (pairnum, pairs, col0, col1);
+    int ij = po + pairing (pn, pairs + 2*(size_t) po, col0, col1);
     if (ikind[ij] != PMX_GRANULAR_FORCE) atomicAdd (&flags[FLAG_BAD_KIND], 1);
     double ma = j >= 0 ? 1.0 / (invm[i] + invm[j]) : mass[i];
     double f[3];
@@ -523,7 +527,7 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
     ts[2] += a[0]*f[1] - a[1]*f[0];
   }
   ncnt[e] = cnt < cap ? cnt : cap;
-  const double *g = gtab + 3*(size_t) kstep;
+  const double *g = gtab + 3*((size_t) kstep*ninst + inst);
   double ma = mass[pe];
   force[pe] = fs[0] + ma * g[0];
   force[(size_t) N + pe] = fs[1] + ma * g[1];
@@ -546,13 +550,14 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
 __global__ void k_dynamics (int N, double *angular, double *linear, double *rotation, double *position,
                             const double *inertia, const double *inverse, const double *mass, const double *invm,
                             const double *dtab, const int *ctl, double *force, double *torque, double step0, double step1,
-                            const int *parell, int E, double *center, const signed char *iso)
+                            const int *parell, int E, double *center, const signed char *iso, int ninst,
+                            const int *par_inst)
 {
   if (ctl[CTL_ABORT]) return;
   int kstep = ctl[CTL_KSTEP];
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  const double *damping = dtab + 6*(size_t) kstep;
+  const double *damping = dtab + 6*((size_t) kstep*ninst + (ninst > 1 ? par_inst[i] : 0));
   double half = 0.5*step0;
   double step = 0.5*(step0+step1);
   double O[3], o[3], v[3], L1[9], J[9], I[9], im, f[3], t[3], T[3], DL[9], L2[9], A[3], B[3];
@@ -743,9 +748,10 @@ int step_bin (pmx_ctx &c)
   hipLaunchKernelGGL (k_fill_i, grid (L), dim3 (PMX_TPB), 0, c.stream, (const int*) c.ctl.p, c.lcount.p, L, 0);
   {
     ProfScope ps (c, K_BIN, 32.0 * c.E);
-    hipLaunchKernelGGL (k_bin, grid (c.E), dim3 (PMX_TPB), 0, c.stream, c.E, (const double*) c.center.p, t.root,
-                        (const int*) t.left.p, (const int*) t.right.p, (const int*) t.dim.p, (const double*) t.coord.p,
-                        (const double*) t.cell.p, t.nleaves, c.ctl.p, c.leafof.p, c.slot.p, c.lcount.p, c.dflags.p);
+    hipLaunchKernelGGL (k_bin, grid (c.E), dim3 (PMX_TPB), 0, c.stream, c.E, (const double*) c.center.p,
+                        (const int*) t.roots.p, (const int*) c.ell_inst.p, (const int*) t.left.p, (const int*) t.right.p, (const int*) t.dim.p, (const double*) t.coord.p,
+                        (const double*) t.cell.p, t.nleaves, c.ctl.p, c.leafof.p, c.slot.p, c.lcount.p, c.dflags.p,
+                        c.ovf.p);
   }
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
@@ -769,8 +775,7 @@ int step_detect (pmx_ctx &c)
   {
     ProfScope ps (c, K_DETECT, 68.0 * c.E);
     hipLaunchKernelGGL (k_detect, dim3 (t.nleaves), dim3 (DET_TPB), 0, c.stream, c.E, (const double*) c.center.p,
-                        (const double*) c.radii.p, (const int*) c.part.p, t.root, (const int*) t.left.p,
-                        (const int*) t.right.p, (const int*) t.dim.p, (const double*) t.coord.p, (const double*) t.cell.p,
+                        (const double*) c.radii.p, (const int*) c.part.p, (const double*) t.cell.p,
                         t.nleaves, (const int*) c.lcount.p, (const int*) c.loff.p,
                         (const int*) c.lmem.p, (const double*) c.lpos.p, (const double*) c.tri.p, c.T,
                         (const int*) t.tricnt.p, (const int*) t.trioff.p, (const int*) t.trilist.p,
@@ -788,7 +793,7 @@ int step_rows (pmx_ctx &c, double h)
     ProfScope ps (c, K_FORCE_INIT, 60.0 * c.nnosph);
     hipLaunchKernelGGL (k_force_init, grid (c.nnosph), dim3 (PMX_TPB), 0, c.stream, (const int*) c.ctl.p, c.N,
                         (const int*) c.nosph.p, c.nnosph, (const double*) c.mass.p, (const double*) c.gtab.p,
-                        c.force.p, c.torque.p);
+                        c.I, (const int*) c.par_inst.p, c.force.p, c.torque.p);
   }
   if (c.E > 0)
   {
@@ -800,7 +805,7 @@ int step_rows (pmx_ctx &c, double h)
                         (const int*) c.triobs.p, (const double*) c.position.p, (const double*) c.linear.p,
                         (const double*) c.angular.p, (const double*) c.mass.p, (const double*) c.invm.p, c.P,
                         (const int*) c.pairs.p, (const int*) c.ikind.p, (const double*) c.iparam.p,
-                        c.rowkey[0].p, c.rowst[0].p, c.rowcnt[0].p, c.rowkey[1].p, c.rowst[1].p, c.rowcnt[1].p,
+                        c.I, (const int*) c.ell_inst.p, (const int*) c.pair_off.p, c.rowkey[0].p, c.rowst[0].p, c.rowcnt[0].p, c.rowkey[1].p, c.rowst[1].p, c.rowcnt[1].p,
                         c.cur0, c.cap, (const uint32_t*) c.hits.p,
                         (const int*) c.hitcnt.p, c.hcap, (const int*) c.xbuf.p, (const int*) c.xcnt.p, c.xcap,
                         (const double*) c.gtab.p, (const int*) c.ctl.p, h, c.force.p, c.torque.p, c.dflags.p);
@@ -815,7 +820,8 @@ int step_dynamics (pmx_ctx &c, double step0, double step1)
   hipLaunchKernelGGL (k_dynamics, grid (c.N), dim3 (PMX_TPB), 0, c.stream, c.N, c.angular.p, c.linear.p, c.rotation.p,
                       c.position.p, (const double*) c.inertia.p, (const double*) c.inverse.p, (const double*) c.mass.p,
                       (const double*) c.invm.p, (const double*) c.dtab.p, (const int*) c.ctl.p, c.force.p, c.torque.p,
-                      step0, step1, (const int*) c.parell.p, c.E, c.center.p, (const signed char*) c.iso.p);
+                      step0, step1, (const int*) c.parell.p, c.E, c.center.p, (const signed char*) c.iso.p, c.I,
+                      (const int*) c.par_inst.p);
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
 }

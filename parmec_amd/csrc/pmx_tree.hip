@@ -34,11 +34,14 @@ __device__ __forceinline__ unsigned expandbits (unsigned v)
 
 __device__ __forceinline__ int lzc (unsigned x) { return x ? __clz (x) : 32; }
 
-__device__ __forceinline__ int delta (int i, unsigned codei, int j, int n, const uint32_t *code)
+/* generalised common prefix length within one instance's sorted range
+ * [lo, hi] (partition.ispc:272-281); equal codes tie-break on the positions
+ * relative to the range, as in a run of that instance alone */
+__device__ __forceinline__ int delta (int i, unsigned codei, int j, int lo, int hi, const uint64_t *key)
 {
-  if (j < 0 || j > n-1) return -1;
-  unsigned codej = code[j];
-  if (codei == codej) return 32 + lzc ((unsigned) (i ^ j));
+  if (j < lo || j > hi) return -1;
+  unsigned codej = (unsigned) (key[j] & 0x3FFFFFFFull);
+  if (codei == codej) return 32 + lzc ((unsigned) ((i - lo) ^ (j - lo)));
   return lzc (codei ^ codej);
 }
 
@@ -163,13 +166,52 @@ __global__ void k_extents_final (int nb, double *ext /* ext[0..5] out, partials 
   if (threadIdx.x == 0) for (int k = 0; k < 6; k++) ext[k] = e[k];
 }
 
-/* -------------------------------------------------------------- morton */
+/* per-instance extents, one block per instance */
+__global__ void k_extents_seg (const int *off, const double *cx, const double *cy, const double *cz, double *ext)
+{
+  __shared__ double sh[6*(PMX_TPB/64)];
+  int inst = blockIdx.x;
+  double e[6] = {DBL_MAX, DBL_MAX, DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX};
+  for (int i = off[inst] + threadIdx.x; i < off[inst+1]; i += blockDim.x)
+  {
+    double x = cx[i], y = cy[i], z = cz[i];
+    if (x < e[0]) e[0] = x;
+    if (y < e[1]) e[1] = y;
+    if (z < e[2]) e[2] = z;
+    if (x > e[3]) e[3] = x;
+    if (y > e[4]) e[4] = y;
+    if (z > e[5]) e[5] = z;
+  }
+  block_minmax6 (e, sh);
+  if (threadIdx.x == 0) for (int k = 0; k < 6; k++) ext[6*inst + k] = e[k];
+}
 
-__global__ void k_morton (int n, const double *cx, const double *cy, const double *cz, const double *ext,
-                          uint32_t *code, int *order)
+__global__ void k_set_i (int *a, int n, int v)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+/* copy the current centres of the selected instances (all when sel is null) */
+__global__ void k_snap (int n, const double *center, double *bcen, const int *inst, const int *sel)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (sel && !sel[inst[i]]) return;
+  bcen[i] = center[i];
+  bcen[(size_t) n + i] = center[(size_t) n + i];
+  bcen[2*(size_t) n + i] = center[2*(size_t) n + i];
+}
+
+/* -------------------------------------------------------------- morton */
+
+__global__ void k_morton (int n, const double *cx, const double *cy, const double *cz, const double *ext_all,
+                          const int *inst, uint64_t *code, int *order)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int ins = inst[i];
+  const double *ext = ext_all + 6*ins;
   double p[3] = {cx[i], cy[i], cz[i]};
   unsigned q[3];
 #pragma unroll
@@ -182,7 +224,7 @@ __global__ void k_morton (int n, const double *cx, const double *cy, const doubl
     s = s < 1023.0 ? s : 1023.0;
     q[k] = expandbits ((unsigned) (int) s);
   }
-  code[i] = q[0]*4 + q[1]*2 + q[2];
+  code[i] = ((uint64_t) ins << 30) | (uint64_t) (q[0]*4 + q[1]*2 + q[2]);
   order[i] = i;
 }
 
@@ -191,7 +233,7 @@ __global__ void k_morton (int n, const double *cx, const double *cy, const doubl
 #define RADIX_BITS 6
 #define RADIX 64
 
-__global__ void k_radix_hist (int n, const uint32_t *key, int shift, int *hist, int nb)
+__global__ void k_radix_hist (int n, const uint64_t *key, int shift, int *hist, int nb)
 {
   __shared__ int h[RADIX];
   if (threadIdx.x < RADIX) h[threadIdx.x] = 0;
@@ -205,8 +247,8 @@ __global__ void k_radix_hist (int n, const uint32_t *key, int shift, int *hist, 
 /* stable scatter: rank inside the block = same-digit elements of lower
  * thread index, from per-wave match masks (6 ballots) and per-wave digit
  * counts staged in LDS */
-__global__ void k_radix_scatter (int n, const uint32_t *key, const int *val, int shift, const int *hist, int nb,
-                                 uint32_t *key2, int *val2)
+__global__ void k_radix_scatter (int n, const uint64_t *key, const int *val, int shift, const int *hist, int nb,
+                                 uint64_t *key2, int *val2)
 {
   __shared__ int wcnt[PMX_TPB/64][RADIX];
   int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -214,7 +256,7 @@ __global__ void k_radix_scatter (int n, const uint32_t *key, const int *val, int
   __syncthreads ();
   int i = blockIdx.x * PMX_TPB + tid;
   bool valid = i < n;
-  uint32_t k = valid ? key[i] : 0u;
+  uint64_t k = valid ? key[i] : 0ull;
   int digit = (int) ((k >> shift) & (RADIX-1));
   unsigned long long m = __ballot (valid);
 #pragma unroll
@@ -238,19 +280,22 @@ __global__ void k_radix_scatter (int n, const uint32_t *key, const int *val, int
 
 /* ----------------------------------------------------------- karras */
 
-__global__ void k_karras (int n, const uint32_t *code, int *first, int *size, int *split, int *dim)
+__global__ void k_karras (int n, const uint64_t *key, const int *off, int *first, int *size, int *split, int *dim)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n-1) return;
-  unsigned codei = code[i];
-  int dd = delta (i, codei, i+1, n, code) - delta (i, codei, i-1, n, code);
+  if (i >= n) return;
+  int ins = (int) (key[i] >> 30);
+  int lo = off[ins], hi = off[ins+1] - 1;
+  if (i >= hi) { dim[i] = -1; size[i] = 0; first[i] = i; split[i] = 0; return; }   /* not an internal node */
+  unsigned codei = (unsigned) (key[i] & 0x3FFFFFFFull);
+  int dd = delta (i, codei, i+1, lo, hi, key) - delta (i, codei, i-1, lo, hi, key);
   int d = (dd > 0) - (dd < 0);
-  int dmin = delta (i, codei, i-d, n, code);
+  int dmin = delta (i, codei, i-d, lo, hi, key);
   int lmax = 2;
-  while (delta (i, codei, i+lmax*d, n, code) > dmin) lmax *= 2;
+  while (delta (i, codei, i+lmax*d, lo, hi, key) > dmin) lmax *= 2;
   int l = 0;
   for (int t = lmax/2; t >= 1; t /= 2)
-    if (delta (i, codei, i+(l+t)*d, n, code) > dmin) l = l+t;
+    if (delta (i, codei, i+(l+t)*d, lo, hi, key) > dmin) l = l+t;
   int j = i+l*d;
   first[i] = d > 0 ? i : j;
   size[i] = l+1;
@@ -261,11 +306,11 @@ __global__ void k_karras (int n, const uint32_t *code, int *first, int *size, in
   }
   else
   {
-    int dnode = delta (i, codei, j, n, code);
+    int dnode = delta (i, codei, j, lo, hi, key);
     int s = 0;
     for (int t = (l+1)/2;; t = (t+1)/2)
     {
-      if (delta (i, codei, i+(s+t)*d, n, code) > dnode) s = s+t;
+      if (delta (i, codei, i+(s+t)*d, lo, hi, key) > dnode) s = s+t;
       if (t == 1) break;
     }
     split[i] = i+s*d+min (d, 0);
@@ -273,39 +318,54 @@ __global__ void k_karras (int n, const uint32_t *code, int *first, int *size, in
   }
 }
 
-/* leaf children per branch node; branch flags for the coordinate pass */
-__global__ void k_children (int n, const int *first, const int *size, const int *split, const int *dim,
-                            int *nleafkids, int *isbranch, int *parent)
+/* leaf children per branch node (plus the root itself when an instance's
+ * whole range is one terminal node); branch flags for the coordinate pass */
+__global__ void k_children (int n, const uint64_t *key, const int *off, const int *first, const int *size,
+                            const int *split, const int *dim, int *nleafkids, int *isbranch, int *parent)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n-1) return;
-  if (i == 0) parent[0] = -1;
-  if (dim[i] < 0) { nleafkids[i] = 0; isbranch[i] = 0; return; }
+  if (i >= n) return;
+  int ins = (int) (key[i] >> 30);
+  int lo = off[ins];
+  int k = 0;
+  if (i == lo)
+  {
+    parent[i] = -1;
+    if (dim[i] < 0) k = 1;            /* root leaf */
+  }
+  if (dim[i] < 0) { nleafkids[i] = k; isbranch[i] = 0; return; }
   isbranch[i] = 1;
   int j = split[i];
-  int k = 0;
   if (first[i] == j || dim[j] < 0) k++; else parent[j] = 2*i;
   if (first[i]+size[i]-1 == j+1 || dim[j+1] < 0) k++; else parent[j+1] = 2*i+1;
   nleafkids[i] = k;
 }
 
-__global__ void k_assign (int n, const int *first, const int *size, const int *split, const int *dim,
-                          const int *leafbase, int *left, int *right, int *leafpar)
+__global__ void k_assign (int n, const uint64_t *key, const int *off, const int *first, const int *size,
+                          const int *split, const int *dim, const int *leafbase, int *left, int *right, int *leafpar,
+                          int *leafinst, int *roots)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n-1 || dim[i] < 0) return;
-  int j = split[i];
+  if (i >= n) return;
+  int ins = (int) (key[i] >> 30);
   int L = leafbase[i];
-  if (first[i] == j || dim[j] < 0) { left[i] = -(L+1); leafpar[L] = 2*i; L++; }
+  if (i == off[ins])
+  {
+    if (dim[i] < 0) { roots[ins] = -(L+1); leafpar[L] = -1; leafinst[L] = ins; L++; }
+    else roots[ins] = i;
+  }
+  if (dim[i] < 0) return;
+  int j = split[i];
+  if (first[i] == j || dim[j] < 0) { left[i] = -(L+1); leafpar[L] = 2*i; leafinst[L] = ins; L++; }
   else left[i] = j;
-  if (first[i]+size[i]-1 == j+1 || dim[j+1] < 0) { right[i] = -(L+1); leafpar[L] = 2*i+1; }
+  if (first[i]+size[i]-1 == j+1 || dim[j+1] < 0) { right[i] = -(L+1); leafpar[L] = 2*i+1; leafinst[L] = ins; }
   else right[i] = j+1;
 }
 
 __global__ void k_branchlist (int n, const int *isbranch, const int *bpos, int *blist)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n-1) return;
+  if (i >= n) return;
   if (isbranch[i]) blist[bpos[i]] = i;
 }
 
@@ -391,11 +451,14 @@ __device__ __forceinline__ int reach_leaves_count (int root, const int *left, co
   return visited;
 }
 
-__global__ void k_tri_reach (int T, const double *tri, int root, const int *left, const int *right, const int *dim,
-                             const double *coord, int *tricnt, const int *trioff, int *trilist, int *fill, int *flag)
+__global__ void k_tri_reach (int T, const double *tri, const int *roots, const int *tri_inst, const int *left,
+                             const int *right, const int *dim, const double *coord, int *tricnt, const int *trioff,
+                             int *trilist, int *fill, int *flag)
 {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= T) return;
+  int root = roots[tri_inst[t]];
+  if (root == PMX_NO_ROOT) return;     /* the triangle's instance has no spheres */
   double v[3][3];
   for (int a = 0; a < 3; a++) for (int b = 0; b < 3; b++) v[a][b] = tri[(size_t) (3*a+b)*T + t];
   double lo[3], hi[3];
@@ -413,11 +476,13 @@ __global__ void k_tri_reach (int T, const double *tri, int root, const int *left
 /* leaves whose closed cell overlaps leaf Q's cell inflated by rmax: every leaf
  * a sphere binned in Q can reach (its centre lies in Q's cell and its AABB
  * half-width is at most rmax; rounding is monotone) */
-__global__ void k_leaf_nbr (int nleaves, const double *cell, double rmax, int root, const int *left, const int *right,
-                            const int *dim, const double *coord, int *cnt, const int *off, int *list, int *flag)
+__global__ void k_leaf_nbr (int nleaves, const double *cell, double rmax, const int *roots, const int *leafinst,
+                            const int *left, const int *right, const int *dim, const double *coord, int *cnt,
+                            const int *off, int *list, int *flag)
 {
   int Q = blockIdx.x * blockDim.x + threadIdx.x;
   if (Q >= nleaves) return;
+  int root = roots[leafinst[Q]];
   double lo[3], hi[3];
   for (int k = 0; k < 3; k++)
   {
@@ -484,102 +549,82 @@ int scan_exclusive (pmx_ctx &c, int *data, int n, int *total_host, const int *ct
   return PMX_OK;
 }
 
-int tree_build (pmx_ctx &c)
+int tree_build (pmx_ctx &c, bool all)
 {
   TreeDev &t = c.tree;
-  int n = c.E;
+  int n = c.E, I = c.I;
   t.n = n;
   c.bin_hint = false;
   if (n == 0) { t.nleaves = 0; t.root = -1; return PMX_OK; }
-  const double *cx = c.center.p, *cy = c.center.p + n, *cz = c.center.p + 2*(size_t) n;
+  /* the tree of an instance is a function of the centres it was last built
+   * from, so a rebuild forced by one instance leaves the others unchanged */
+  if (c.bcen.alloc (3*(size_t) n) != hipSuccess) { pmx_set_error ("out of device memory (tree)"); return PMX_ENOMEM; }
+  hipLaunchKernelGGL (k_snap, grid (n), dim3 (PMX_TPB), 0, c.stream, n, (const double*) c.center.p, c.bcen.p,
+                      (const int*) c.ell_inst.p, all ? (const int*) nullptr : (const int*) c.ovf.p);
+  PMX_HIP (hipGetLastError ());
+  const double *cx = c.bcen.p, *cy = c.bcen.p + n, *cz = c.bcen.p + 2*(size_t) n;
   int nbext = std::min (1024, (n + PMX_TPB - 1) / PMX_TPB);
   int nbsort = (n + PMX_TPB - 1) / PMX_TPB;
-  int ni = std::max (n-1, 1);
-  if (t.ext.alloc (6 + 6*(size_t) nbext) || t.code.alloc (n) || t.code2.alloc (n) || t.order.alloc (n) ||
-      t.order2.alloc (n) || t.hist.alloc ((size_t) RADIX * nbsort + 1) || t.first.alloc (ni) || t.size.alloc (ni) ||
-      t.split.alloc (ni) || t.dim.alloc (ni) || t.parent.alloc (ni) || t.coord.alloc (ni) || t.left.alloc (ni) ||
-      t.right.alloc (ni) || t.nleafkids.alloc (ni + 1))
+  if (t.ext.alloc (std::max (6*(size_t) I, 6 + 6*(size_t) nbext)) || t.code.alloc (n) || t.code2.alloc (n) ||
+      t.order.alloc (n) || t.order2.alloc (n) || t.hist.alloc (std::max ((size_t) RADIX * nbsort + 1, 2*(size_t) n + 2)) ||
+      t.first.alloc (n) || t.size.alloc (n) || t.split.alloc (n) || t.dim.alloc (n) || t.parent.alloc (n) ||
+      t.coord.alloc (n) || t.left.alloc (n) || t.right.alloc (n) || t.nleafkids.alloc (n + 1) || t.roots.alloc (I))
   { pmx_set_error ("out of device memory (tree)"); return PMX_ENOMEM; }
 
-  hipLaunchKernelGGL (k_extents_partial, dim3 (nbext), dim3 (PMX_TPB), 0, c.stream, n, cx, cy, cz, t.ext.p + 6);
-  hipLaunchKernelGGL (k_extents_final, dim3 (1), dim3 (PMX_TPB), 0, c.stream, nbext, t.ext.p);
+  /* extents per instance (partition.ispc:29-54, 669-684) */
+  if (I == 1)
+  {
+    hipLaunchKernelGGL (k_extents_partial, dim3 (nbext), dim3 (PMX_TPB), 0, c.stream, n, cx, cy, cz, t.ext.p + 6);
+    hipLaunchKernelGGL (k_extents_final, dim3 (1), dim3 (PMX_TPB), 0, c.stream, nbext, t.ext.p);
+  }
+  else
+    hipLaunchKernelGGL (k_extents_seg, dim3 (I), dim3 (PMX_TPB), 0, c.stream, (const int*) c.ell_off.p, cx, cy, cz, t.ext.p);
   hipLaunchKernelGGL (k_morton, grid (n), dim3 (PMX_TPB), 0, c.stream, n, cx, cy, cz, (const double*) t.ext.p,
-                      t.code.p, t.order.p);
+                      (const int*) c.ell_inst.p, t.code.p, t.order.p);
   PMX_HIP (hipGetLastError ());
 
-  uint32_t *k0 = t.code.p, *k1 = t.code2.p;
+  /* stable LSD radix sort of (instance, code) */
+  int bits = 30;
+  while ((1ll << (bits - 30)) < I) bits++;
+  uint64_t *k0 = t.code.p, *k1 = t.code2.p;
   int *v0 = t.order.p, *v1 = t.order2.p;
-  for (int shift = 0; shift < 30; shift += RADIX_BITS)
+  for (int shift = 0; shift < bits; shift += RADIX_BITS)
   {
-    hipLaunchKernelGGL (k_radix_hist, dim3 (nbsort), dim3 (PMX_TPB), 0, c.stream, n, (const uint32_t*) k0, shift,
+    hipLaunchKernelGGL (k_radix_hist, dim3 (nbsort), dim3 (PMX_TPB), 0, c.stream, n, (const uint64_t*) k0, shift,
                         t.hist.p, nbsort);
     PMX_HIP (hipGetLastError ());
     int rc = scan_exclusive (c, t.hist.p, RADIX * nbsort, nullptr);
     if (rc) return rc;
-    hipLaunchKernelGGL (k_radix_scatter, dim3 (nbsort), dim3 (PMX_TPB), 0, c.stream, n, (const uint32_t*) k0,
+    hipLaunchKernelGGL (k_radix_scatter, dim3 (nbsort), dim3 (PMX_TPB), 0, c.stream, n, (const uint64_t*) k0,
                         (const int*) v0, shift, (const int*) t.hist.p, nbsort, k1, v1);
     PMX_HIP (hipGetLastError ());
     std::swap (k0, k1);
     std::swap (v0, v1);
   }
-  /* 5 passes: the sorted result sits in code2/order2 */
-  const uint32_t *code = k0;
+  const uint64_t *key = k0;
   const int *order = v0;
 
-  if (n == 1)
-  {
-    t.nleaves = 1;
-    t.root = -1;
-    if (t.leafpar.alloc (1) || t.cell.alloc (6)) { pmx_set_error ("out of device memory (tree)"); return PMX_ENOMEM; }
-    hipLaunchKernelGGL (k_fill_int, dim3 (1), dim3 (64), 0, c.stream, t.leafpar.p, 1, -1);
-    hipLaunchKernelGGL (k_cells, dim3 (1), dim3 (64), 0, c.stream, 1, (const int*) t.leafpar.p, (const int*) t.parent.p,
-                        (const int*) t.dim.p, (const double*) t.coord.p, t.cell.p);
-    PMX_HIP (hipGetLastError ());
-    return PMX_OK;
-  }
-
-  hipLaunchKernelGGL (k_karras, grid (n-1), dim3 (PMX_TPB), 0, c.stream, n, code, t.first.p, t.size.p, t.split.p, t.dim.p);
+  /* Karras nodes per instance range (partition.ispc:311-394); an instance
+   * without spheres keeps PMX_NO_ROOT */
+  hipLaunchKernelGGL (k_set_i, grid (I), dim3 (PMX_TPB), 0, c.stream, t.roots.p, I, PMX_NO_ROOT);
+  hipLaunchKernelGGL (k_karras, grid (n), dim3 (PMX_TPB), 0, c.stream, n, key, (const int*) c.ell_off.p, t.first.p,
+                      t.size.p, t.split.p, t.dim.p);
+  PMX_HIP (hipGetLastError ());
+  int *isbranch = t.hist.p;           /* [n] */
+  int *blist = t.hist.p + n + 1;      /* [n] scanned positions */
+  hipLaunchKernelGGL (k_children, grid (n), dim3 (PMX_TPB), 0, c.stream, n, key, (const int*) c.ell_off.p,
+                      (const int*) t.first.p, (const int*) t.size.p, (const int*) t.split.p, (const int*) t.dim.p,
+                      t.nleafkids.p, isbranch, t.parent.p);
   PMX_HIP (hipGetLastError ());
 
-  int rootsize = 0, rootdim = -1;
-  PMX_HIP (hipMemcpyAsync (&rootdim, t.dim.p, sizeof (int), hipMemcpyDeviceToHost, c.stream));
-  PMX_HIP (hipStreamSynchronize (c.stream));
-  (void) rootsize;
-  if (rootdim < 0)  /* whole set within CUTOFF: one leaf */
-  {
-    t.nleaves = 1;
-    t.root = -1;
-    if (t.leafpar.alloc (1) || t.cell.alloc (6)) { pmx_set_error ("out of device memory (tree)"); return PMX_ENOMEM; }
-    hipLaunchKernelGGL (k_fill_int, dim3 (1), dim3 (64), 0, c.stream, t.leafpar.p, 1, -1);
-    hipLaunchKernelGGL (k_cells, dim3 (1), dim3 (64), 0, c.stream, 1, (const int*) t.leafpar.p, (const int*) t.parent.p,
-                        (const int*) t.dim.p, (const double*) t.coord.p, t.cell.p);
-    PMX_HIP (hipGetLastError ());
-    return PMX_OK;
-  }
-
-  /* branch flags reuse code2 as int scratch only if sizes allow; use hist */
-  if (t.hist.alloc (std::max ((size_t) RADIX * nbsort + 1, (size_t) 2*ni + 2)) != hipSuccess)
-  { pmx_set_error ("out of device memory (tree)"); return PMX_ENOMEM; }
-  int *isbranch = t.hist.p;          /* [ni] -> scanned into branch positions */
-  int *blist = t.hist.p + ni + 1;    /* [nbranch] */
-  hipLaunchKernelGGL (k_children, grid (n-1), dim3 (PMX_TPB), 0, c.stream, n, (const int*) t.first.p,
-                      (const int*) t.size.p, (const int*) t.split.p, (const int*) t.dim.p, t.nleafkids.p, isbranch,
-                      t.parent.p);
-  PMX_HIP (hipGetLastError ());
-
-  /* branch list for the coordinate pass */
+  /* split coordinates of the branch nodes (mincoord, partition.ispc:297-308) */
   int nbranch = 0;
-  {
-    /* keep the flags: scan a copy inside blist area first */
-    PMX_HIP (hipMemcpyAsync (blist, isbranch, sizeof (int) * (n-1), hipMemcpyDeviceToDevice, c.stream));
-    int rc = scan_exclusive (c, blist, n-1, &nbranch);
-    if (rc) return rc;
-    /* isbranch keeps flags, blist holds positions; compact into isbranch's tail-free copy */
-    if (t.order2.alloc (n) != hipSuccess) { pmx_set_error ("out of device memory (tree)"); return PMX_ENOMEM; }
-  }
+  PMX_HIP (hipMemcpyAsync (blist, isbranch, sizeof (int) * n, hipMemcpyDeviceToDevice, c.stream));
+  int rc = scan_exclusive (c, blist, n, &nbranch);
+  if (rc) return rc;
   dvec<int> branchidx;
   if (branchidx.alloc (std::max (nbranch, 1)) != hipSuccess) { pmx_set_error ("out of device memory (tree)"); return PMX_ENOMEM; }
-  hipLaunchKernelGGL (k_branchlist, grid (n-1), dim3 (PMX_TPB), 0, c.stream, n, (const int*) isbranch,
+  hipLaunchKernelGGL (k_branchlist, grid (n), dim3 (PMX_TPB), 0, c.stream, n, (const int*) isbranch,
                       (const int*) blist, branchidx.p);
   PMX_HIP (hipGetLastError ());
   if (nbranch > 0)
@@ -590,20 +635,20 @@ int tree_build (pmx_ctx &c)
     PMX_HIP (hipGetLastError ());
   }
 
-  /* leaf numbering */
+  /* leaf numbering and reach cells */
   int nleaves = 0;
-  int rc = scan_exclusive (c, t.nleafkids.p, n-1, &nleaves);
+  rc = scan_exclusive (c, t.nleafkids.p, n, &nleaves);
   if (rc) { branchidx.release (); return rc; }
   t.nleaves = nleaves;
-  if (t.leafpar.alloc (nleaves) || t.cell.alloc (6*(size_t) nleaves))
+  if (t.leafpar.alloc (nleaves) || t.leafinst.alloc (nleaves) || t.cell.alloc (6*(size_t) nleaves))
   { branchidx.release (); pmx_set_error ("out of device memory (tree)"); return PMX_ENOMEM; }
-  hipLaunchKernelGGL (k_assign, grid (n-1), dim3 (PMX_TPB), 0, c.stream, n, (const int*) t.first.p, (const int*) t.size.p,
-                      (const int*) t.split.p, (const int*) t.dim.p, (const int*) t.nleafkids.p, t.left.p, t.right.p,
-                      t.leafpar.p);
+  hipLaunchKernelGGL (k_assign, grid (n), dim3 (PMX_TPB), 0, c.stream, n, key, (const int*) c.ell_off.p,
+                      (const int*) t.first.p, (const int*) t.size.p, (const int*) t.split.p, (const int*) t.dim.p,
+                      (const int*) t.nleafkids.p, t.left.p, t.right.p, t.leafpar.p, t.leafinst.p, t.roots.p);
   hipLaunchKernelGGL (k_cells, grid (nleaves), dim3 (PMX_TPB), 0, c.stream, nleaves, (const int*) t.leafpar.p,
                       (const int*) t.parent.p, (const int*) t.dim.p, (const double*) t.coord.p, t.cell.p);
   PMX_HIP (hipGetLastError ());
-  t.root = 0;
+  PMX_HIP (hipMemcpyAsync (&t.root, t.roots.p, sizeof (int), hipMemcpyDeviceToHost, c.stream));
   PMX_HIP (hipStreamSynchronize (c.stream));
   branchidx.release ();
   return PMX_OK;
@@ -618,8 +663,9 @@ int tree_nbrlists (pmx_ctx &c)
   if (c.E == 0 || t.nleaves == 0) return PMX_OK;
   int *flag = c.dflags.p + FLAG_TRILIST_OVERFLOW;
   hipLaunchKernelGGL (k_leaf_nbr, grid (t.nleaves), dim3 (PMX_TPB), 0, c.stream, t.nleaves, (const double*) t.cell.p,
-                      c.rmax, t.root, (const int*) t.left.p, (const int*) t.right.p, (const int*) t.dim.p,
-                      (const double*) t.coord.p, t.nbrcnt.p, (const int*) nullptr, (int*) nullptr, flag);
+                      c.rmax, (const int*) t.roots.p, (const int*) t.leafinst.p, (const int*) t.left.p,
+                      (const int*) t.right.p, (const int*) t.dim.p, (const double*) t.coord.p, t.nbrcnt.p,
+                      (const int*) nullptr, (int*) nullptr, flag);
   PMX_HIP (hipGetLastError ());
   PMX_HIP (hipMemcpyAsync (t.nbroff.p, t.nbrcnt.p, sizeof (int) * t.nleaves, hipMemcpyDeviceToDevice, c.stream));
   int total = 0;
@@ -628,8 +674,9 @@ int tree_nbrlists (pmx_ctx &c)
   t.nnbr = total;
   if (t.nbrlist.alloc (std::max (total, 1))) { pmx_set_error ("out of device memory (neighbour lists)"); return PMX_ENOMEM; }
   hipLaunchKernelGGL (k_leaf_nbr, grid (t.nleaves), dim3 (PMX_TPB), 0, c.stream, t.nleaves, (const double*) t.cell.p,
-                      c.rmax, t.root, (const int*) t.left.p, (const int*) t.right.p, (const int*) t.dim.p,
-                      (const double*) t.coord.p, t.nbrcnt.p, (const int*) t.nbroff.p, t.nbrlist.p, flag);
+                      c.rmax, (const int*) t.roots.p, (const int*) t.leafinst.p, (const int*) t.left.p,
+                      (const int*) t.right.p, (const int*) t.dim.p, (const double*) t.coord.p, t.nbrcnt.p,
+                      (const int*) t.nbroff.p, t.nbrlist.p, flag);
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
 }
@@ -649,9 +696,10 @@ int tree_trilists (pmx_ctx &c)
     return PMX_OK;
   }
   int *flag = c.dflags.p + FLAG_TRILIST_OVERFLOW;
-  hipLaunchKernelGGL (k_tri_reach, grid (c.T), dim3 (PMX_TPB), 0, c.stream, c.T, (const double*) c.tri.p, t.root,
-                      (const int*) t.left.p, (const int*) t.right.p, (const int*) t.dim.p, (const double*) t.coord.p,
-                      t.tricnt.p, (const int*) nullptr, (int*) nullptr, (int*) nullptr, flag);
+  hipLaunchKernelGGL (k_tri_reach, grid (c.T), dim3 (PMX_TPB), 0, c.stream, c.T, (const double*) c.tri.p,
+                      (const int*) t.roots.p, (const int*) c.tri_inst.p, (const int*) t.left.p, (const int*) t.right.p,
+                      (const int*) t.dim.p, (const double*) t.coord.p, t.tricnt.p, (const int*) nullptr, (int*) nullptr,
+                      (int*) nullptr, flag);
   PMX_HIP (hipGetLastError ());
   PMX_HIP (hipMemcpyAsync (t.trioff.p, t.tricnt.p, sizeof (int) * t.nleaves, hipMemcpyDeviceToDevice, c.stream));
   int total = 0;
@@ -662,9 +710,10 @@ int tree_trilists (pmx_ctx &c)
   dvec<int> fill;
   if (fill.alloc (L)) { pmx_set_error ("out of device memory (trilists)"); return PMX_ENOMEM; }
   hipLaunchKernelGGL (k_fill_int, grid (L), dim3 (PMX_TPB), 0, c.stream, fill.p, L, 0);
-  hipLaunchKernelGGL (k_tri_reach, grid (c.T), dim3 (PMX_TPB), 0, c.stream, c.T, (const double*) c.tri.p, t.root,
-                      (const int*) t.left.p, (const int*) t.right.p, (const int*) t.dim.p, (const double*) t.coord.p,
-                      (int*) nullptr, (const int*) t.trioff.p, t.trilist.p, fill.p, flag);
+  hipLaunchKernelGGL (k_tri_reach, grid (c.T), dim3 (PMX_TPB), 0, c.stream, c.T, (const double*) c.tri.p,
+                      (const int*) t.roots.p, (const int*) c.tri_inst.p, (const int*) t.left.p, (const int*) t.right.p,
+                      (const int*) t.dim.p, (const double*) t.coord.p, (int*) nullptr, (const int*) t.trioff.p,
+                      t.trilist.p, fill.p, flag);
   PMX_HIP (hipGetLastError ());
   PMX_HIP (hipStreamSynchronize (c.stream));
   fill.release ();

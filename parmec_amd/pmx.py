@@ -101,6 +101,11 @@ def load(path=LIBPATH):
             raise ImportError(f"{path} is missing: build it with `python parmec_amd/build.py`")
         lib = C.CDLL(path)
         lib.pmx_create.argtypes = [C.POINTER(pmx_options), C.POINTER(C.c_void_p)]
+        lib.pmx_upload_batch.argtypes = [C.c_void_p, C.POINTER(pmx_model), C.c_int]
+        lib.pmx_download_batch.argtypes = [C.c_void_p, C.POINTER(pmx_model), C.c_int]
+        lib.pmx_dem_batch.argtypes = [C.c_void_p, C.c_double, C.c_double, C.POINTER(pmx_forcing), _dp]
+        lib.pmx_contacts_instance.argtypes = [C.c_void_p, C.c_int, C.POINTER(pmx_pair), C.c_size_t,
+                                              C.POINTER(C.c_size_t)]
         _lib = _declare(lib, "pmx_")
     return _lib
 
@@ -269,11 +274,37 @@ class Engine:
         self._call("dem", self.h, duration, step, C.byref(forcing_) if forcing_ is not None else None, C.byref(el))
         return el.value
 
-    def contacts(self):
+    def upload_batch(self, models):
+        """pmx_upload_batch: the models are stepped together, each as if alone."""
+        arr = (pmx_model * len(models))(*[m.struct() for m in models])
+        self._call("upload_batch", self.h, arr, len(models))
+
+    def download_batch(self, models):
+        structs = [m.struct() for m in models]
+        arr = (pmx_model * len(models))(*structs)
+        self._call("download_batch", self.h, arr, len(models))
+        for m, s in zip(models, arr):
+            m.take_time(s)
+
+    def dem_batch(self, duration, step, forcings=None):
+        """pmx_dem_batch with one pmx_forcing per instance (or None)."""
+        el = C.c_double(0.0)
+        arr = None
+        if forcings is not None:
+            arr = (pmx_forcing * len(forcings))(*forcings)
+            arr._keep = [getattr(f, "_keep", None) for f in forcings]
+        self._call("dem_batch", self.h, duration, step, arr, C.byref(el))
+        return el.value
+
+    def contacts(self, instance=None):
         n = C.c_size_t(0)
-        self._call("contacts", self.h, None, 0, C.byref(n))
+        if instance is None:
+            fn, pre = "contacts", ()
+        else:
+            fn, pre = "contacts_instance", (instance,)
+        self._call(fn, self.h, *pre, None, 0, C.byref(n))
         arr = (pmx_pair * max(n.value, 1))()
-        self._call("contacts", self.h, arr, n.value, C.byref(n))
+        self._call(fn, self.h, *pre, arr, n.value, C.byref(n))
         out = np.zeros(n.value, dtype=[("master", np.int32), ("slave", np.int32), ("state", np.float64, 3)])
         if n.value:
             raw = np.ctypeslib.as_array(arr)[: n.value]

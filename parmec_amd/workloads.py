@@ -267,3 +267,28 @@ def base_excitation(times, amplitude=9.81, frequency=5.0, g=9.81):
     gt[:, 0] = -amplitude * np.sin(2.0 * math.pi * frequency * t)
     gt[:, 2] = -g
     return gt
+
+
+def sweep_instance(i, nmin=1000, nmax=64000, radius=0.005):
+    """Instance i of the acceleration sweep: a box_lattice set-up with a
+    seeded particle count in [nmin, nmax], base amplitude in [0.1, 2] g and
+    frequency in [1, 20] Hz (seed = instance number).  Returns
+    (model, amplitude m/s^2, frequency Hz)."""
+    rng = np.random.default_rng(1000003 + i)
+    n = int(rng.integers(nmin, nmax + 1))
+    amp = float(rng.uniform(0.1, 2.0)) * 9.81
+    freq = float(rng.uniform(1.0, 20.0))
+    return box_lattice(n=n, seed=i, radius=radius), amp, freq
+
+
+def sweep(instances=512, nmin=1000, nmax=64000):
+    """The batched sweep: independent instances of sweep_instance."""
+    return [sweep_instance(i, nmin, nmax) for i in range(instances)]
+
+
+def sweep_forcing(amp, freq, nsteps, step, curtime=0.0):
+    """Per-step gravity of a sweep instance: g(t) = (A sin 2 pi f t, 0, -9.81)
+    evaluated at each step's curtime, as the reference's GRAVITY callback
+    would be (input.cpp:4424-4503)."""
+    t = curtime + step * np.arange(nsteps)
+    return base_excitation(t, amplitude=-amp, frequency=freq)

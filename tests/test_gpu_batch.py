@@ -1,0 +1,106 @@
+"""Batches: every instance of a batch must give bitwise what it gives alone
+(state, contact pairs and tangential history), whatever else is in the batch.
+The single runs are themselves pinned to the oracle by the parity tests."""
+import numpy as np
+import pytest
+
+from parmec_amd import pmx, workloads
+from helpers import compare, spring_modes_model
+
+pytestmark = pytest.mark.gpu
+
+
+def _alone(m, duration, step, f):
+    eng = pmx.Engine()
+    mm = m.copy()
+    eng.upload(mm)
+    eng.dem(duration, step, f)
+    eng.download(mm)
+    out = mm, eng.contacts(), eng.stats()
+    eng.close()
+    return out
+
+
+def _batch(models, duration, step, forcings, calls=1):
+    eng = pmx.Engine()
+    ms = [m.copy() for m in models]
+    eng.upload_batch(ms)
+    for _ in range(calls):
+        eng.dem_batch(duration, step, forcings)
+    eng.download_batch(ms)
+    cs = [eng.contacts(i) for i in range(len(ms))]
+    st = eng.stats()
+    eng.close()
+    return ms, cs, st
+
+
+def _sweep_models(count, nmin, nmax, nsteps, step, stir=0.5):
+    """Sweep instances, stirred with seeded random velocities (m/s) so that
+    contacts form and break within a short run."""
+    models, forcings = [], []
+    for i in range(count):
+        m, amp, freq = workloads.sweep_instance(i, nmin, nmax)
+        m.linear[:] = np.random.default_rng(i).normal(scale=stir, size=m.linear.shape)
+        models.append(m)
+        forcings.append(pmx.forcing(nsteps, gtab=workloads.sweep_forcing(amp, freq, nsteps, step)))
+    return models, forcings
+
+
+def test_batch_equals_alone_sweep():
+    """Sweep instances of uneven size, each with its own gravity table."""
+    step, nsteps = 1e-5, 300
+    print(f"case sweep batch 8 instances x {nsteps} steps", flush=True)
+    models, forcings = _sweep_models(8, 300, 3000, nsteps, step)
+    duration = (nsteps - 0.5) * step
+    ms, cs, _ = _batch(models, duration, step, forcings)
+    for i, m in enumerate(models):
+        ma, ca, _ = _alone(m, duration, step, forcings[i])
+        assert len(ca) > 0
+        compare(ms[i], cs[i], ma, ca)
+
+
+def test_batch_mixed_kinds_and_pairings():
+    """Spheres with different pairing tables, a spring model and a repartition
+    in one instance, stepped in one batch over two dem calls."""
+    step = 1e-5
+    a = workloads.box_lattice(n=1500, seed=5)
+    b = workloads.box_lattice(n=900, seed=6, friction=0.0, damper=0.3)
+    c = workloads.box_lattice(n=4000, seed=3)
+    xc = c.center[:3].mean(axis=1, keepdims=True)
+    c.linear[:] = -75.0 * (c.center[:3] - xc)       # converging flow: leaf overflow
+    d = spring_modes_model(seed=2)
+    e = workloads.spring_lattice(nx=4, ny=3, nz=3, seed=1)
+    models = [a, b, c, d, e]
+    fs = [pmx.forcing(gravity=(0.0, 0.0, -9.81)), pmx.forcing(gravity=(1.0, 0.0, -9.81)), pmx.forcing(),
+          pmx.forcing(gravity=(0.0, 0.0, -9.81), damping=(0.1, 0.1, 0.1, 0.0, 0.0, 0.0)),
+          pmx.forcing(gravity=(0.0, 0.0, -9.81))]
+    duration = 100 * step - 0.5 * step
+    print("case mixed batch (spheres, pairings, springs, repartition), 2 calls", flush=True)
+    ms, cs, st = _batch(models, duration, step, fs, calls=2)
+    assert st["rebuilds"] > 2
+    for i, m in enumerate(models):
+        eng = pmx.Engine()
+        mm = m.copy()
+        eng.upload(mm)
+        for _ in range(2):
+            eng.dem(duration, step, fs[i])
+        eng.download(mm)
+        ca = eng.contacts()
+        eng.close()
+        compare(ms[i], cs[i], mm, ca)
+        if m.sprnum:
+            for name in ("stroke", "sprfrc", "sprpnt", "sprdir"):
+                assert np.array_equal(getattr(ms[i], name), getattr(mm, name)), name
+
+
+def test_batch_order_invariance():
+    """The same instances in another order give the same per-instance results."""
+    step, nsteps = 1e-5, 200
+    models, forcings = _sweep_models(5, 500, 2500, nsteps, step)
+    duration = (nsteps - 0.5) * step
+    print("case batch order invariance", flush=True)
+    ms1, cs1, _ = _batch(models, duration, step, forcings)
+    perm = [3, 0, 4, 2, 1]
+    ms2, cs2, _ = _batch([models[p] for p in perm], duration, step, [forcings[p] for p in perm])
+    for k, p in enumerate(perm):
+        compare(ms2[k], cs2[k], ms1[p], cs1[p])
