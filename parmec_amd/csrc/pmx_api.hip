@@ -43,8 +43,10 @@ template <class T> int down (pmx_ctx &c, T *dst, const dvec<T> &d, size_t n)
 int alloc_rows (pmx_ctx &c)
 {
   size_t E = std::max (c.E, 1);
+  bool mixed = c.opt.precision == PMX_MIXED;
   for (int b = 0; b < 2; b++)
-    if (c.rowkey[b].alloc ((size_t) c.cap * E) || c.rowst[b].alloc (3 * (size_t) c.cap * E) || c.rowcnt[b].alloc (E))
+    if (c.rowkey[b].alloc ((size_t) c.cap * E) || c.rowcnt[b].alloc (E) ||
+        (mixed ? c.rowst32[b].alloc (3 * (size_t) c.cap * E) : c.rowst[b].alloc (3 * (size_t) c.cap * E)))
       return fail (PMX_ENOMEM, "out of device memory (contact rows)");
   if (c.hits.alloc ((size_t) c.hcap * E) || c.hitcnt.alloc (E) || c.xbuf.alloc ((size_t) c.xcap * E) ||
       c.xcnt.alloc (E) || c.leafof.alloc (E) || c.slot.alloc (E) || c.lmem.alloc (E) || c.lpos.alloc (4 * E))
@@ -234,10 +236,10 @@ int pmx_create (const pmx_options *opt, pmx_ctx **out)
   *out = nullptr;
   pmx_ctx *c = new pmx_ctx ();
   if (opt) c->opt = *opt;
-  if (c->opt.precision != PMX_FP64)
+  if (c->opt.precision != PMX_FP64 && c->opt.precision != PMX_MIXED)
   {
     delete c;
-    return fail (PMX_EUNSUPPORTED, "only PMX_FP64 is available in this build");
+    return fail (PMX_EINVAL, "precision must be PMX_FP64 or PMX_MIXED");
   }
   int ndev = 0;
   if (hipGetDeviceCount (&ndev) != hipSuccess || ndev == 0)
@@ -494,7 +496,14 @@ int pmx_contacts (pmx_ctx *c, pmx_pair *out, size_t cap, size_t *count)
     std::vector<double> st (3 * (size_t) c->cap * E);
     std::vector<int> cnt (E), part (E);
     PMX_HIP (hipMemcpy (key.data (), c->rowkey[c->cur].p, key.size () * sizeof (uint32_t), hipMemcpyDeviceToHost));
-    PMX_HIP (hipMemcpy (st.data (), c->rowst[c->cur].p, st.size () * sizeof (double), hipMemcpyDeviceToHost));
+    if (c->opt.precision == PMX_MIXED)
+    {
+      std::vector<float> s32 (st.size ());
+      PMX_HIP (hipMemcpy (s32.data (), c->rowst32[c->cur].p, s32.size () * sizeof (float), hipMemcpyDeviceToHost));
+      for (size_t q = 0; q < st.size (); q++) st[q] = s32[q];
+    }
+    else
+      PMX_HIP (hipMemcpy (st.data (), c->rowst[c->cur].p, st.size () * sizeof (double), hipMemcpyDeviceToHost));
     PMX_HIP (hipMemcpy (cnt.data (), c->rowcnt[c->cur].p, E * sizeof (int), hipMemcpyDeviceToHost));
     PMX_HIP (hipMemcpy (part.data (), c->part.p, E * sizeof (int), hipMemcpyDeviceToHost));
     for (size_t e = 0; e < E; e++)

@@ -192,33 +192,34 @@ __device__ __forceinline__ int pairing (int nummat, const int *pairs, int i, int
   return 0;
 }
 
-/* granular spring-dashpot with Coulomb friction, forces.ispc:125-192 */
-__device__ __forceinline__ void granular (const double n[3], const double vij[3], double depth, double ma,
-                                          double kn, double damper, double kskn, double mus, double mud,
-                                          double state[3], double step, double f[3])
+/* granular spring-dashpot with Coulomb friction, forces.ispc:125-192; R is
+ * the arithmetic of the law (double: the FP64 path; float: mixed precision) */
+template <class R>
+__device__ __forceinline__ void granular (const R n[3], const R vij[3], R depth, R ma, R kn, R damper, R kskn, R mus,
+                                          R mud, R state[3], R step, R f[3])
 {
-  double en = damper * 2.0 * sqrt (kn*ma);
-  double vn = vij[0]*n[0] + vij[1]*n[1] + vij[2]*n[2];
-  double fn = kn*depth + en*vn;
+  R en = damper * R (2.0) * sqrt (kn*ma);
+  R vn = vij[0]*n[0] + vij[1]*n[1] + vij[2]*n[2];
+  R fn = kn*depth + en*vn;
   f[0] = fn*n[0];
   f[1] = fn*n[1];
   f[2] = fn*n[2];
-  double kt = kn * kskn;
-  double et = en * kskn;
-  double vt0 = vij[0] - vn*n[0];
-  double vt1 = vij[1] - vn*n[1];
-  double vt2 = vij[2] - vn*n[2];
+  R kt = kn * kskn;
+  R et = en * kskn;
+  R vt0 = vij[0] - vn*n[0];
+  R vt1 = vij[1] - vn*n[1];
+  R vt2 = vij[2] - vn*n[2];
   state[0] += step*vt0;
   state[1] += step*vt1;
   state[2] += step*vt2;
-  double ft0 = kt*state[0] + et*vt0;
-  double ft1 = kt*state[1] + et*vt1;
-  double ft2 = kt*state[2] + et*vt2;
-  double len = sqrt (ft0*ft0 + ft1*ft1 + ft2*ft2);
-  double lim = mus*fn;
+  R ft0 = kt*state[0] + et*vt0;
+  R ft1 = kt*state[1] + et*vt1;
+  R ft2 = kt*state[2] + et*vt2;
+  R len = sqrt (ft0*ft0 + ft1*ft1 + ft2*ft2);
+  R lim = mus*fn;
   /* len == 0: the tangential force is the zero vector and stays so (the
    * reference would divide by zero here when f_n < 0) */
-  if (len > lim && len > 0.0)
+  if (len > lim && len > R (0.0))
   {
     lim = mud*fn/len;
     ft0 *= lim;
@@ -228,6 +229,15 @@ __device__ __forceinline__ void granular (const double n[3], const double vij[3]
   f[0] += ft0;
   f[1] += ft1;
   f[2] += ft2;
+}
+
+/* compensated (Kahan) float accumulation */
+__device__ __forceinline__ void kahan_add (float &sum, float &comp, float x)
+{
+  float y = x - comp;
+  float t = sum + y;
+  comp = (t - sum) - y;
+  sum = t;
 }
 
 } /* namespace pmxd */

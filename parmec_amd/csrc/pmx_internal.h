@@ -33,10 +33,15 @@ void pmx_set_error (const std::string &msg);
   pmx_set_error (std::string ("HIP error: ") + hipGetErrorString (e_) + " at " + __FILE__ + ":" + std::to_string (__LINE__)); \
   return PMX_EDEVICE; } } while (0)
 
+/* device array owned by its holder (freed with it) */
 template <class T> struct dvec
 {
   T *p = nullptr;
   size_t n = 0;
+  dvec () = default;
+  dvec (const dvec &) = delete;
+  dvec &operator= (const dvec &) = delete;
+  ~dvec () { release (); }
   hipError_t alloc (size_t count)
   {
     if (count <= n && p) return hipSuccess;
@@ -68,6 +73,8 @@ struct TreeDev
   dvec<double> coord;
   dvec<int> left, right;     /* child refs: >= 0 branch node, < 0 leaf -(L+1) */
   dvec<int> nleafkids;       /* per internal node: leaf children (scan -> leaf ids) */
+  dvec<double> cmin;         /* [2][3][n] minima of each node's children (split coordinates) */
+  dvec<int> upcnt;           /* [n] arrivals of the bottom-up pass */
   /* leaves */
   dvec<int> leafpar;         /* [nleaves] parent branch node*2 + side (or -1 for a root leaf) */
   dvec<double> cell;         /* [6][nleaves]: lower bounds 0-2, upper bounds 3-5 */
@@ -155,7 +162,8 @@ struct pmx_ctx
   int cap = 32, hcap = 32, xcap = 16;
   int cur = 0;
   dvec<uint32_t> rowkey[2];    /* [cap][E] */
-  dvec<double> rowst[2];       /* [3][cap][E] */
+  dvec<double> rowst[2];       /* [3][cap][E] tangential state (FP64 path) */
+  dvec<float> rowst32[2];      /* [3][cap][E] tangential state (mixed precision) */
   dvec<int> rowcnt[2];         /* [E] */
   dvec<uint32_t> hits;         /* [hcap][E] own hits */
   dvec<int> hitcnt;            /* [E] */

@@ -334,14 +334,20 @@ __global__ void k_force_init (const int *ctl, int N, const int *list, int nlist,
 
 #define ROWS_TPB 128
 
-/* row merge + contact forces + per-sphere accumulation */
+/* row merge + contact forces + per-sphere accumulation.  R is the arithmetic
+ * of the contact geometry and force law: double (the FP64 path) or float
+ * (mixed precision: differences of FP64 positions are formed in FP64 and then
+ * narrowed, the depth that decides whether a contact persists stays FP64, the
+ * tangential state is stored in FP32 and the per-sphere sums are
+ * Kahan-compensated FP32) */
+template <class R>
 __global__ __launch_bounds__(ROWS_TPB)
 void k_rows (int E, int N, const double *center, const double *radii, const int *part, const int *ellcol,
              const double *tri, int T, const int *tricol, const int *triobs,
              const double *position, const double *linear, const double *angular, const double *mass,
              const double *invm, int pairnum, const int *pairs, const int *ikind, const double *iparam,
              int ninst, const int *ell_inst, const int *pair_off,
-             uint32_t *key0, double *st0, int *cnt0, uint32_t *key1, double *st1, int *cnt1, int cur0, int cap,
+             uint32_t *key0, R *st0, int *cnt0, uint32_t *key1, R *st1, int *cnt1, int cur0, int cap,
              const uint32_t *hits, const int *hitcnt, int hcap, const int *xbuf, const int *xcnt, int xcap,
              const double *gtab, const int *ctl, double h, double *force, double *torque, int *flags)
 {
@@ -349,11 +355,12 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
   if (ctl[CTL_ABORT]) return;
   int kstep = ctl[CTL_KSTEP];
   bool odd = ((cur0 + kstep) & 1) != 0;   /* rows of the previous step */
+  constexpr bool mixed = sizeof (R) == 4;
   const uint32_t *okey = odd ? key1 : key0;
-  const double *ost = odd ? st1 : st0;
+  const R *ost = odd ? st1 : st0;
   const int *ocnt = odd ? cnt1 : cnt0;
   uint32_t *nkey = odd ? key0 : key1;
-  double *nst = odd ? st0 : st1;
+  R *nst = odd ? st0 : st1;
   int *ncnt = odd ? cnt0 : cnt1;
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   int cnt = 0;
@@ -384,7 +391,8 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
   int po = 0, pn = pairnum;   /* the instance's pairing table */
   if (ninst > 1) { po = pair_off[inst]; pn = pair_off[inst+1] - po; }
   int io = 0, ih = 0;
-  double fs[3] = {0.0, 0.0, 0.0}, ts[3] = {0.0, 0.0, 0.0};
+  R fs[3] = {0.0, 0.0, 0.0}, ts[3] = {0.0, 0.0, 0.0};
+  R cf[3] = {0.0, 0.0, 0.0}, ct[3] = {0.0, 0.0, 0.0};   /* Kahan compensation (mixed) */
 
   while (io < oc || ih < nh)
   {
@@ -400,16 +408,19 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
     }
     else { key = kh; old = false; ih++; }
 
-    double point[3], normal[3], depth, st[3] = {0.0, 0.0, 0.0};
+    double point[3], depth;
+    R normal[3], st[3] = {0.0, 0.0, 0.0};
     bool master;
     int i, j, col0, col1;
     if (key & PMX_TRI_FLAG)
     {
       int t = (int) (key & ~PMX_TRI_FLAG);
+      double nd[3];   /* closest-point regions and depth in FP64: they decide the pair */
       depth = triangle_sphere (tri[t], tri[(size_t) T + t], tri[2*(size_t) T + t],
                                tri[3*(size_t) T + t], tri[4*(size_t) T + t], tri[5*(size_t) T + t],
                                tri[6*(size_t) T + t], tri[7*(size_t) T + t], tri[8*(size_t) T + t],
-                               pc[0], pc[1], pc[2], re, point, normal);
+                               pc[0], pc[1], pc[2], re, point, nd);
+      normal[0] = (R) nd[0]; normal[1] = (R) nd[1]; normal[2] = (R) nd[2];
       if (old && depth < 0.0) continue;
       master = true;
       if (old)
@@ -433,13 +444,24 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
       double ps[3] = {center[es], center[(size_t) E + es], center[2*(size_t) E + es]};
       double qx = pm[0]-ps[0], qy = pm[1]-ps[1], qz = pm[2]-ps[2];
       double len = sqrt (qx*qx + qy*qy + qz*qz);
-      double ilen = old ? 1.0/len : (len > 0.0 ? 1.0/len : 1.0);
       point[0] = 0.5*(pm[0]+ps[0]);
       point[1] = 0.5*(pm[1]+ps[1]);
       point[2] = 0.5*(pm[2]+ps[2]);
-      normal[0] = ilen*qx;
-      normal[1] = ilen*qy;
-      normal[2] = ilen*qz;
+      if (mixed)
+      {
+        R lf = (R) len;
+        R il = old ? R (1.0)/lf : (lf > R (0.0) ? R (1.0)/lf : R (1.0));
+        normal[0] = il*(R) qx;
+        normal[1] = il*(R) qy;
+        normal[2] = il*(R) qz;
+      }
+      else
+      {
+        double ilen = old ? 1.0/len : (len > 0.0 ? 1.0/len : 1.0);
+        normal[0] = (R) (ilen*qx);
+        normal[1] = (R) (ilen*qy);
+        normal[2] = (R) (ilen*qz);
+      }
       depth = radii[em] + radii[es] - len;
       if (old && depth < 0.0) continue;
       if (old)
@@ -477,33 +499,33 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
 
     /* contacts_task velocities, forces.ispc:219-287 */
     double xi[3] = {position[i], position[(size_t) N + i], position[2*(size_t) N + i]};
-    double oi[3] = {angular[3*(size_t) N + i], angular[4*(size_t) N + i], angular[5*(size_t) N + i]};
-    double vi[3], vj[3], z[3];
-    z[0] = point[0]-xi[0];
-    z[1] = point[1]-xi[1];
-    z[2] = point[2]-xi[2];
-    vi[0] = oi[1]*z[2]-oi[2]*z[1] + linear[i];
-    vi[1] = oi[2]*z[0]-oi[0]*z[2] + linear[(size_t) N + i];
-    vi[2] = oi[0]*z[1]-oi[1]*z[0] + linear[2*(size_t) N + i];
+    R oi[3] = {(R) angular[3*(size_t) N + i], (R) angular[4*(size_t) N + i], (R) angular[5*(size_t) N + i]};
+    R vi[3], vj[3], z[3];
+    z[0] = (R) (point[0]-xi[0]);
+    z[1] = (R) (point[1]-xi[1]);
+    z[2] = (R) (point[2]-xi[2]);
+    vi[0] = oi[1]*z[2]-oi[2]*z[1] + (R) linear[i];
+    vi[1] = oi[2]*z[0]-oi[0]*z[2] + (R) linear[(size_t) N + i];
+    vi[2] = oi[0]*z[1]-oi[1]*z[0] + (R) linear[2*(size_t) N + i];
     if (j >= 0)
     {
-      z[0] = point[0]-position[j];
-      z[1] = point[1]-position[(size_t) N + j];
-      z[2] = point[2]-position[2*(size_t) N + j];
-      double oj[3] = {angular[3*(size_t) N + j], angular[4*(size_t) N + j], angular[5*(size_t) N + j]};
-      vj[0] = oj[1]*z[2]-oj[2]*z[1] + linear[j];
-      vj[1] = oj[2]*z[0]-oj[0]*z[2] + linear[(size_t) N + j];
-      vj[2] = oj[0]*z[1]-oj[1]*z[0] + linear[2*(size_t) N + j];
+      z[0] = (R) (point[0]-position[j]);
+      z[1] = (R) (point[1]-position[(size_t) N + j]);
+      z[2] = (R) (point[2]-position[2*(size_t) N + j]);
+      R oj[3] = {(R) angular[3*(size_t) N + j], (R) angular[4*(size_t) N + j], (R) angular[5*(size_t) N + j]};
+      vj[0] = oj[1]*z[2]-oj[2]*z[1] + (R) linear[j];
+      vj[1] = oj[2]*z[0]-oj[0]*z[2] + (R) linear[(size_t) N + j];
+      vj[2] = oj[0]*z[1]-oj[1]*z[0] + (R) linear[2*(size_t) N + j];
     }
     else { vj[0] = vj[1] = vj[2] = 0.0; }
-    double vij[3] = {vj[0]-vi[0], vj[1]-vi[1], vj[2]-vi[2]};
+    R vij[3] = {vj[0]-vi[0], vj[1]-vi[1], vj[2]-vi[2]};
     int ij = po + pairing (pn, pairs + 2*(size_t) po, col0, col1);
     if (ikind[ij] != PMX_GRANULAR_FORCE) atomicAdd (&flags[FLAG_BAD_KIND], 1);
     double ma = j >= 0 ? 1.0 / (invm[i] + invm[j]) : mass[i];
-    double f[3];
-    granular (normal, vij, depth, ma, iparam[PMX_SPRING*pairnum + ij], iparam[PMX_DAMPER*pairnum + ij],
-              iparam[PMX_KSKN*pairnum + ij], iparam[PMX_FRISTAT*pairnum + ij], iparam[PMX_FRIDYN*pairnum + ij],
-              st, h, f);
+    R f[3];
+    granular<R> (normal, vij, (R) depth, (R) ma, (R) iparam[PMX_SPRING*pairnum + ij], (R) iparam[PMX_DAMPER*pairnum + ij],
+                 (R) iparam[PMX_KSKN*pairnum + ij], (R) iparam[PMX_FRISTAT*pairnum + ij],
+                 (R) iparam[PMX_FRIDYN*pairnum + ij], st, (R) h, f);
 
     if (cnt < cap)
     {
@@ -520,21 +542,32 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
 
     /* contacts_acc_task, forces.ispc:492-533: slaves receive -f */
     if (!master) { f[0] = -f[0]; f[1] = -f[1]; f[2] = -f[2]; }
-    double a[3] = {point[0]-xe[0], point[1]-xe[1], point[2]-xe[2]};
-    fs[0] += f[0]; fs[1] += f[1]; fs[2] += f[2];
-    ts[0] += a[1]*f[2] - a[2]*f[1];
-    ts[1] += a[2]*f[0] - a[0]*f[2];
-    ts[2] += a[0]*f[1] - a[1]*f[0];
+    R a[3] = {(R) (point[0]-xe[0]), (R) (point[1]-xe[1]), (R) (point[2]-xe[2])};
+    if constexpr (mixed)
+    {
+      kahan_add (fs[0], cf[0], f[0]); kahan_add (fs[1], cf[1], f[1]); kahan_add (fs[2], cf[2], f[2]);
+      kahan_add (ts[0], ct[0], a[1]*f[2] - a[2]*f[1]);
+      kahan_add (ts[1], ct[1], a[2]*f[0] - a[0]*f[2]);
+      kahan_add (ts[2], ct[2], a[0]*f[1] - a[1]*f[0]);
+    }
+    else
+    {
+      fs[0] += f[0]; fs[1] += f[1]; fs[2] += f[2];
+      ts[0] += a[1]*f[2] - a[2]*f[1];
+      ts[1] += a[2]*f[0] - a[0]*f[2];
+      ts[2] += a[0]*f[1] - a[1]*f[0];
+    }
   }
+  (void) cf; (void) ct;
   ncnt[e] = cnt < cap ? cnt : cap;
   const double *g = gtab + 3*((size_t) kstep*ninst + inst);
   double ma = mass[pe];
-  force[pe] = fs[0] + ma * g[0];
-  force[(size_t) N + pe] = fs[1] + ma * g[1];
-  force[2*(size_t) N + pe] = fs[2] + ma * g[2];
-  torque[pe] = ts[0];
-  torque[(size_t) N + pe] = ts[1];
-  torque[2*(size_t) N + pe] = ts[2];
+  force[pe] = (double) fs[0] + ma * g[0];
+  force[(size_t) N + pe] = (double) fs[1] + ma * g[1];
+  force[2*(size_t) N + pe] = (double) fs[2] + ma * g[2];
+  torque[pe] = (double) ts[0];
+  torque[(size_t) N + pe] = (double) ts[1];
+  torque[2*(size_t) N + pe] = (double) ts[2];
   }
   int mx = cnt;
 #pragma unroll
@@ -799,16 +832,28 @@ int step_rows (pmx_ctx &c, double h)
   {
     ProfScope ps (c, K_ROWS, 136.0 * c.E);
     size_t shm = (size_t) ROWS_TPB * (c.hcap + c.xcap) * sizeof (uint32_t);
-    hipLaunchKernelGGL (k_rows, dim3 ((c.E + ROWS_TPB - 1) / ROWS_TPB), dim3 (ROWS_TPB), shm, c.stream, c.E, c.N,
-                        (const double*) c.center.p, (const double*) c.radii.p, (const int*) c.part.p,
-                        (const int*) c.ellcol.p, (const double*) c.tri.p, c.T, (const int*) c.tricol.p,
-                        (const int*) c.triobs.p, (const double*) c.position.p, (const double*) c.linear.p,
-                        (const double*) c.angular.p, (const double*) c.mass.p, (const double*) c.invm.p, c.P,
-                        (const int*) c.pairs.p, (const int*) c.ikind.p, (const double*) c.iparam.p,
-                        c.I, (const int*) c.ell_inst.p, (const int*) c.pair_off.p, c.rowkey[0].p, c.rowst[0].p, c.rowcnt[0].p, c.rowkey[1].p, c.rowst[1].p, c.rowcnt[1].p,
-                        c.cur0, c.cap, (const uint32_t*) c.hits.p,
-                        (const int*) c.hitcnt.p, c.hcap, (const int*) c.xbuf.p, (const int*) c.xcnt.p, c.xcap,
-                        (const double*) c.gtab.p, (const int*) c.ctl.p, h, c.force.p, c.torque.p, c.dflags.p);
+    if (c.opt.precision == PMX_MIXED)
+      hipLaunchKernelGGL (k_rows<float>, dim3 ((c.E + ROWS_TPB - 1) / ROWS_TPB), dim3 (ROWS_TPB), shm, c.stream, c.E, c.N,
+                          (const double*) c.center.p, (const double*) c.radii.p, (const int*) c.part.p,
+                          (const int*) c.ellcol.p, (const double*) c.tri.p, c.T, (const int*) c.tricol.p,
+                          (const int*) c.triobs.p, (const double*) c.position.p, (const double*) c.linear.p,
+                          (const double*) c.angular.p, (const double*) c.mass.p, (const double*) c.invm.p, c.P,
+                          (const int*) c.pairs.p, (const int*) c.ikind.p, (const double*) c.iparam.p,
+                          c.I, (const int*) c.ell_inst.p, (const int*) c.pair_off.p, c.rowkey[0].p, c.rowst32[0].p, c.rowcnt[0].p, c.rowkey[1].p, c.rowst32[1].p, c.rowcnt[1].p,
+                          c.cur0, c.cap, (const uint32_t*) c.hits.p,
+                          (const int*) c.hitcnt.p, c.hcap, (const int*) c.xbuf.p, (const int*) c.xcnt.p, c.xcap,
+                          (const double*) c.gtab.p, (const int*) c.ctl.p, h, c.force.p, c.torque.p, c.dflags.p);
+    else
+      hipLaunchKernelGGL (k_rows<double>, dim3 ((c.E + ROWS_TPB - 1) / ROWS_TPB), dim3 (ROWS_TPB), shm, c.stream, c.E, c.N,
+                          (const double*) c.center.p, (const double*) c.radii.p, (const int*) c.part.p,
+                          (const int*) c.ellcol.p, (const double*) c.tri.p, c.T, (const int*) c.tricol.p,
+                          (const int*) c.triobs.p, (const double*) c.position.p, (const double*) c.linear.p,
+                          (const double*) c.angular.p, (const double*) c.mass.p, (const double*) c.invm.p, c.P,
+                          (const int*) c.pairs.p, (const int*) c.ikind.p, (const double*) c.iparam.p,
+                          c.I, (const int*) c.ell_inst.p, (const int*) c.pair_off.p, c.rowkey[0].p, c.rowst[0].p, c.rowcnt[0].p, c.rowkey[1].p, c.rowst[1].p, c.rowcnt[1].p,
+                          c.cur0, c.cap, (const uint32_t*) c.hits.p,
+                          (const int*) c.hitcnt.p, c.hcap, (const int*) c.xbuf.p, (const int*) c.xcnt.p, c.xcap,
+                          (const double*) c.gtab.p, (const int*) c.ctl.p, h, c.force.p, c.torque.p, c.dflags.p);
   }
   PMX_HIP (hipGetLastError ());
   return PMX_OK;

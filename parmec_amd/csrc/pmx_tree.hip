@@ -321,7 +321,7 @@ __global__ void k_karras (int n, const uint64_t *key, const int *off, int *first
 /* leaf children per branch node (plus the root itself when an instance's
  * whole range is one terminal node); branch flags for the coordinate pass */
 __global__ void k_children (int n, const uint64_t *key, const int *off, const int *first, const int *size,
-                            const int *split, const int *dim, int *nleafkids, int *isbranch, int *parent)
+                            const int *split, const int *dim, int *nleafkids, int *parent)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -333,8 +333,7 @@ __global__ void k_children (int n, const uint64_t *key, const int *off, const in
     parent[i] = -1;
     if (dim[i] < 0) k = 1;            /* root leaf */
   }
-  if (dim[i] < 0) { nleafkids[i] = k; isbranch[i] = 0; return; }
-  isbranch[i] = 1;
+  if (dim[i] < 0) { nleafkids[i] = k; return; }
   int j = split[i];
   if (first[i] == j || dim[j] < 0) k++; else parent[j] = 2*i;
   if (first[i]+size[i]-1 == j+1 || dim[j+1] < 0) k++; else parent[j+1] = 2*i+1;
@@ -362,35 +361,66 @@ __global__ void k_assign (int n, const uint64_t *key, const int *off, const int 
   else right[i] = j+1;
 }
 
-__global__ void k_branchlist (int n, const int *isbranch, const int *bpos, int *blist)
+/* split coordinate = min of the right child's points along the split
+ * dimension (mincoord, partition.ispc:297-308), for every branch node at once:
+ * the minima of each node's range (all three axes) are formed bottom-up.  A
+ * thread starts at each branch node with a terminal child, takes the minima
+ * of its terminal children directly (at most CUTOFF points each) and climbs;
+ * at every node the second of the two arrivals (acquire-release counter)
+ * combines the children and goes on, the first stops.  cm[side][d][node]
+ * holds the minimum of the node's left (side 0) or right (side 1) child. */
+__device__ __forceinline__ void range_min (int a, int b, const int *order, const double *cx, const double *cy,
+                                           const double *cz, double m[3])
 {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  if (isbranch[i]) blist[bpos[i]] = i;
+  m[0] = m[1] = m[2] = DBL_MAX;
+  for (int k = a; k < b; k++)
+  {
+    int q = order[k];
+    double x = cx[q], y = cy[q], z = cz[q];
+    m[0] = x < m[0] ? x : m[0];
+    m[1] = y < m[1] ? y : m[1];
+    m[2] = z < m[2] ? z : m[2];
+  }
 }
 
-/* split coordinate = min of the right child's points along the split
- * dimension (mincoord, partition.ispc:297-308); one wave per branch node */
-__global__ void k_coord (const int *blist, const int *first, const int *size, const int *split, const int *dim,
-                         const int *order, const double *cx, const double *cy, const double *cz, double *coord)
+__global__ void k_coord_up (int n, const int *first, const int *size, const int *split, const int *dim,
+                            const int *parent, const int *order, const double *cx, const double *cy, const double *cz,
+                            double *cm, int *cnt, double *coord)
 {
-  int i = blist[blockIdx.x];
-  int d = dim[i];
-  const double *pt = d == 0 ? cx : (d == 1 ? cy : cz);
-  int a = split[i]+1, b = first[i]+size[i];
-  double m = DBL_MAX;
-  for (int k = a + (int) threadIdx.x; k < b; k += blockDim.x)
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || dim[i] < 0) return;
+  int j = split[i];
+  bool lt = first[i] == j || dim[j] < 0;
+  bool rt = first[i] + size[i] - 1 == j + 1 || dim[j+1] < 0;
+  if (!lt && !rt) return;
+  double m[3];
+  if (lt)
   {
-    double v = pt[order[k]];
-    m = v < m ? v : m;
+    if (first[i] == j) range_min (j, j+1, order, cx, cy, cz, m);
+    else range_min (first[j], first[j] + size[j], order, cx, cy, cz, m);
+    for (int d = 0; d < 3; d++) cm[(size_t) d*n + i] = m[d];
   }
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1)
+  if (rt)
   {
-    double o = __shfl_xor (m, off, 64);
-    m = o < m ? o : m;
+    if (first[i] + size[i] - 1 == j + 1) range_min (j+1, j+2, order, cx, cy, cz, m);
+    else range_min (first[j+1], first[j+1] + size[j+1], order, cx, cy, cz, m);
+    for (int d = 0; d < 3; d++) cm[(size_t) (3+d)*n + i] = m[d];
   }
-  if (threadIdx.x == 0) coord[i] = m;
+  int node = i, add = (int) lt + (int) rt;
+  for (;;)
+  {
+    int old = __hip_atomic_fetch_add (&cnt[node], add, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (old + add < 2) return;        /* the other child is not done yet */
+    double l[3], r[3];
+    for (int d = 0; d < 3; d++) { l[d] = cm[(size_t) d*n + node]; r[d] = cm[(size_t) (3+d)*n + node]; }
+    coord[node] = r[dim[node]];
+    int p = parent[node];
+    if (p < 0) return;                /* an instance root */
+    int pn = p >> 1, side = p & 1;
+    for (int d = 0; d < 3; d++) cm[(size_t) (3*side+d)*n + pn] = l[d] < r[d] ? l[d] : r[d];
+    node = pn;
+    add = 1;
+  }
 }
 
 __global__ void k_cells (int nleaves, const int *leafpar, const int *parent, const int *dim, const double *coord,
@@ -568,7 +598,8 @@ int tree_build (pmx_ctx &c, bool all)
   if (t.ext.alloc (std::max (6*(size_t) I, 6 + 6*(size_t) nbext)) || t.code.alloc (n) || t.code2.alloc (n) ||
       t.order.alloc (n) || t.order2.alloc (n) || t.hist.alloc (std::max ((size_t) RADIX * nbsort + 1, 2*(size_t) n + 2)) ||
       t.first.alloc (n) || t.size.alloc (n) || t.split.alloc (n) || t.dim.alloc (n) || t.parent.alloc (n) ||
-      t.coord.alloc (n) || t.left.alloc (n) || t.right.alloc (n) || t.nleafkids.alloc (n + 1) || t.roots.alloc (I))
+      t.coord.alloc (n) || t.left.alloc (n) || t.right.alloc (n) || t.nleafkids.alloc (n + 1) || t.roots.alloc (I) ||
+      t.cmin.alloc (6*(size_t) n) || t.upcnt.alloc (n))
   { pmx_set_error ("out of device memory (tree)"); return PMX_ENOMEM; }
 
   /* extents per instance (partition.ispc:29-54, 669-684) */
@@ -610,38 +641,25 @@ int tree_build (pmx_ctx &c, bool all)
   hipLaunchKernelGGL (k_karras, grid (n), dim3 (PMX_TPB), 0, c.stream, n, key, (const int*) c.ell_off.p, t.first.p,
                       t.size.p, t.split.p, t.dim.p);
   PMX_HIP (hipGetLastError ());
-  int *isbranch = t.hist.p;           /* [n] */
-  int *blist = t.hist.p + n + 1;      /* [n] scanned positions */
   hipLaunchKernelGGL (k_children, grid (n), dim3 (PMX_TPB), 0, c.stream, n, key, (const int*) c.ell_off.p,
                       (const int*) t.first.p, (const int*) t.size.p, (const int*) t.split.p, (const int*) t.dim.p,
-                      t.nleafkids.p, isbranch, t.parent.p);
+                      t.nleafkids.p, t.parent.p);
   PMX_HIP (hipGetLastError ());
 
   /* split coordinates of the branch nodes (mincoord, partition.ispc:297-308) */
-  int nbranch = 0;
-  PMX_HIP (hipMemcpyAsync (blist, isbranch, sizeof (int) * n, hipMemcpyDeviceToDevice, c.stream));
-  int rc = scan_exclusive (c, blist, n, &nbranch);
-  if (rc) return rc;
-  dvec<int> branchidx;
-  if (branchidx.alloc (std::max (nbranch, 1)) != hipSuccess) { pmx_set_error ("out of device memory (tree)"); return PMX_ENOMEM; }
-  hipLaunchKernelGGL (k_branchlist, grid (n), dim3 (PMX_TPB), 0, c.stream, n, (const int*) isbranch,
-                      (const int*) blist, branchidx.p);
+  PMX_HIP (hipMemsetAsync (t.upcnt.p, 0, sizeof (int) * n, c.stream));
+  hipLaunchKernelGGL (k_coord_up, grid (n), dim3 (PMX_TPB), 0, c.stream, n, (const int*) t.first.p,
+                      (const int*) t.size.p, (const int*) t.split.p, (const int*) t.dim.p, (const int*) t.parent.p,
+                      order, cx, cy, cz, t.cmin.p, t.upcnt.p, t.coord.p);
   PMX_HIP (hipGetLastError ());
-  if (nbranch > 0)
-  {
-    hipLaunchKernelGGL (k_coord, dim3 (nbranch), dim3 (64), 0, c.stream, (const int*) branchidx.p,
-                        (const int*) t.first.p, (const int*) t.size.p, (const int*) t.split.p, (const int*) t.dim.p,
-                        order, cx, cy, cz, t.coord.p);
-    PMX_HIP (hipGetLastError ());
-  }
 
   /* leaf numbering and reach cells */
   int nleaves = 0;
-  rc = scan_exclusive (c, t.nleafkids.p, n, &nleaves);
-  if (rc) { branchidx.release (); return rc; }
+  int rc = scan_exclusive (c, t.nleafkids.p, n, &nleaves);
+  if (rc) return rc;
   t.nleaves = nleaves;
   if (t.leafpar.alloc (nleaves) || t.leafinst.alloc (nleaves) || t.cell.alloc (6*(size_t) nleaves))
-  { branchidx.release (); pmx_set_error ("out of device memory (tree)"); return PMX_ENOMEM; }
+  { pmx_set_error ("out of device memory (tree)"); return PMX_ENOMEM; }
   hipLaunchKernelGGL (k_assign, grid (n), dim3 (PMX_TPB), 0, c.stream, n, key, (const int*) c.ell_off.p,
                       (const int*) t.first.p, (const int*) t.size.p, (const int*) t.split.p, (const int*) t.dim.p,
                       (const int*) t.nleafkids.p, t.left.p, t.right.p, t.leafpar.p, t.leafinst.p, t.roots.p);
@@ -650,7 +668,6 @@ int tree_build (pmx_ctx &c, bool all)
   PMX_HIP (hipGetLastError ());
   PMX_HIP (hipMemcpyAsync (&t.root, t.roots.p, sizeof (int), hipMemcpyDeviceToHost, c.stream));
   PMX_HIP (hipStreamSynchronize (c.stream));
-  branchidx.release ();
   return PMX_OK;
 }
 
