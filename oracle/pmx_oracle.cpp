@@ -444,6 +444,9 @@ struct orc_ctx
   std::vector<int> part, ellcol;
   std::vector<double> tri;
   std::vector<int> tricol, triobs;
+  int obsnum = 0;
+  std::vector<int> trirng;                    /* [2*obsnum] */
+  std::vector<double> obspnt, obsang, obslin;  /* [3*obsnum] */
   std::vector<double> nodes;
   std::vector<int> nodpart, facnod, factri;
   std::vector<int> pairs, ikind;
@@ -523,6 +526,49 @@ void invert_inertia (orc_ctx &s)
     }
     s.invm[i] = 1.0 / s.mass[i];
   }
+}
+
+/* obstacles, obstacles.ispc:28-78: every moving obstacle advances its
+ * triangles by the rigid motion of the step, rotation expmap(h*omega) about
+ * its point x and translation h*v, then x += h*v.  Canonical: angular and
+ * linear velocities are read with stride 3, as they are stored (the
+ * reference strides them by 6, which reads another obstacle's values from
+ * index 1 on). */
+void obstacles (orc_ctx &s, double step)
+{
+  for (int i = 0; i < s.obsnum; i++)
+  {
+    double *x = &s.obspnt[3*i], *o = &s.obsang[3*i], *v = &s.obslin[3*i];
+    double y[3], DL[9];
+    y[0] = x[0] + step*v[0];
+    y[1] = x[1] + step*v[1];
+    y[2] = x[2] + step*v[2];
+    expmap (step*o[0], step*o[1], step*o[2], DL);
+    for (int j = s.trirng[2*i]; j < s.trirng[2*i+1]; j++)
+      for (int a = 0; a < 3; a++)
+      {
+        double z[3] = {s.trv (a, 0)[j] - x[0], s.trv (a, 1)[j] - x[1], s.trv (a, 2)[j] - x[2]};
+        s.trv (a, 0)[j] = DL[0]*z[0]+DL[3]*z[1]+DL[6]*z[2] + y[0];
+        s.trv (a, 1)[j] = DL[1]*z[0]+DL[4]*z[1]+DL[7]*z[2] + y[1];
+        s.trv (a, 2)[j] = DL[2]*z[0]+DL[5]*z[1]+DL[8]*z[2] + y[2];
+      }
+    x[0] = y[0];
+    x[1] = y[1];
+    x[2] = y[2];
+  }
+}
+
+/* obstaclev, input.cpp:4199-4244: obstacle velocities at a time, here read
+ * from a host-evaluated table row ([obsnum][6]: angular 0-2, linear 3-5;
+ * zero without a table) */
+void obstaclev (orc_ctx &s, const double *row)
+{
+  for (int i = 0; i < s.obsnum; i++)
+    for (int d = 0; d < 3; d++)
+    {
+      s.obsang[3*i+d] = row ? row[6*i+d] : 0.0;
+      s.obslin[3*i+d] = row ? row[6*i+3+d] : 0.0;
+    }
 }
 
 /* ellipsoids_task / nodes_task / faces_task, shapes.ispc:29-211 (spheres:
@@ -815,7 +861,17 @@ int forces (orc_ctx &s, double step, const double g[3])
         vj[1] = oj[2]*z[0]-oj[0]*z[2] + s.lin(1)[j];
         vj[2] = oj[0]*z[1]-oj[1]*z[0] + s.lin(2)[j];
       }
-      else if (-j-2 >= 0) { bad++; continue; }   /* moving obstacles are not on this path */
+      else if (-j-2 >= 0)   /* moving obstacle, forces.ispc:265-280 */
+      {
+        int l = -j-2;
+        z[0] = con.point[0]-s.obspnt[3*l];
+        z[1] = con.point[1]-s.obspnt[3*l+1];
+        z[2] = con.point[2]-s.obspnt[3*l+2];
+        oj[0] = s.obsang[3*l]; oj[1] = s.obsang[3*l+1]; oj[2] = s.obsang[3*l+2];
+        vj[0] = oj[1]*z[2]-oj[2]*z[1] + s.obslin[3*l];
+        vj[1] = oj[2]*z[0]-oj[0]*z[2] + s.obslin[3*l+1];
+        vj[2] = oj[0]*z[1]-oj[1]*z[0] + s.obslin[3*l+2];
+      }
       else { vj[0] = vj[1] = vj[2] = 0.0; }
       vij[0] = vj[0] - vi[0];
       vij[1] = vj[1] - vi[1];
@@ -1350,7 +1406,12 @@ void orc_destroy (orc_ctx *ctx) { delete ctx; }
 int orc_upload (orc_ctx *s, const pmx_model *m)
 {
   if (!s || !m) return fail (PMX_EINVAL, "null argument");
-  if (m->obsnum > 0) return fail (PMX_EUNSUPPORTED, "moving obstacles are not supported by the oracle");
+  if (m->obsnum < 0) return fail (PMX_EINVAL, "invalid obstacle count");
+  for (int t = 0; t < m->trinum; t++)
+    if (m->triobs[t] <= -2 && -m->triobs[t]-2 >= m->obsnum) return fail (PMX_EINVAL, "triangle obstacle index out of range");
+  for (int i = 0; i < m->obsnum; i++)
+    if (m->trirng[2*i] < 0 || m->trirng[2*i] > m->trirng[2*i+1] || m->trirng[2*i+1] > m->trinum)
+      return fail (PMX_EINVAL, "obstacle triangle range out of bounds");
   for (int i = 0; i < m->sprnum; i++)
   {
     if (m->sproffset && m->sproffset[i] != -1) return fail (PMX_EUNSUPPORTED, "spring stroke offsets are not supported");
@@ -1381,6 +1442,11 @@ int orc_upload (orc_ctx *s, const pmx_model *m)
   take (s->tri, m->tri, 9*nt);
   take (s->tricol, m->tricol, nt);
   take (s->triobs, m->triobs, nt);
+  s->obsnum = m->obsnum;
+  take (s->trirng, m->trirng, 2*(size_t) m->obsnum);
+  take (s->obspnt, m->obspnt, 3*(size_t) m->obsnum);
+  take (s->obsang, m->obsang, 3*(size_t) m->obsnum);
+  take (s->obslin, m->obslin, 3*(size_t) m->obsnum);
   take (s->nodes, m->nodes, 6*(size_t) m->nodnum);
   take (s->nodpart, m->nodpart, (size_t) m->nodnum);
   take (s->facnod, m->facnod, 3*(size_t) m->facnum);
@@ -1430,6 +1496,7 @@ int orc_download (orc_ctx *s, pmx_model *m)
   put (m->position, s->position); put (m->inverse, s->inverse); put (m->invm, s->invm);
   put (m->force, s->force); put (m->torque, s->torque); put (m->center, s->center);
   put (m->tri, s->tri); put (m->nodes, s->nodes);
+  if (m->obsnum == s->obsnum) { put (m->obspnt, s->obspnt); put (m->obsang, s->obsang); put (m->obslin, s->obslin); }
   put (m->sprpnt, s->sprpnt); put (m->sprdir, s->sprdir); put (m->sprsdsp, s->sprsdsp); put (m->stroke, s->stroke);
   put (m->sprfrc, s->sprfrc); put (m->yield, s->yield);
   if (m->sprflg) std::memcpy (m->sprflg, s->sprflg.data (), s->sprflg.size () * sizeof (int));
@@ -1450,6 +1517,8 @@ int orc_dem (orc_ctx *s, double duration, double step, const pmx_forcing *fc, do
     step0 = step;
     euler (*s, 0.5*step0);
     shapes (*s);
+    obstaclev (*s, fc ? fc->obstab0 : nullptr);   /* velocities at step0/2 */
+    obstacles (*s, step0);
   }
   else step0 = s->curstep;
   step1 = step0;
@@ -1481,6 +1550,8 @@ int orc_dem (orc_ctx *s, double duration, double step, const pmx_forcing *fc, do
     step1 = step0;
     dynamics (*s, damp, step0, step1);
     shapes (*s);
+    obstaclev (*s, fc && fc->obstab ? fc->obstab + 6*(size_t) k * s->obsnum : nullptr);   /* at curtime+step0 */
+    obstacles (*s, 0.5*(step0+step1));
   }
   s->curstep = step1;
   s->stepnum++;

@@ -64,7 +64,20 @@ int validate_model (const pmx_model *m)
 {
   if (m->parnum < 0 || m->ellnum < 0 || m->trinum < 0 || m->pairnum < 1)
     return fail (PMX_EINVAL, "invalid counts (pairnum must include the default pairing)");
-  if (m->obsnum > 0) return fail (PMX_EUNSUPPORTED, "moving obstacles are not supported on the device path yet");
+  if (m->obsnum < 0) return fail (PMX_EINVAL, "invalid obstacle count");
+  {
+    std::vector<char> moved (std::max (m->trinum, 1), 0);
+    for (int i = 0; i < m->obsnum; i++)
+    {
+      if (m->trirng[2*i] < 0 || m->trirng[2*i] > m->trirng[2*i+1] || m->trirng[2*i+1] > m->trinum)
+        return fail (PMX_EINVAL, "obstacle triangle range out of bounds");
+      for (int t = m->trirng[2*i]; t < m->trirng[2*i+1]; t++)
+      {
+        if (moved[t]) return fail (PMX_EINVAL, "obstacle triangle ranges overlap");
+        moved[t] = 1;
+      }
+    }
+  }
   for (int i = 0; i < m->sprnum; i++)
   {
     if (m->sproffset && m->sproffset[i] != -1) return fail (PMX_EUNSUPPORTED, "spring stroke offsets are not supported");
@@ -82,7 +95,7 @@ int validate_model (const pmx_model *m)
   }
   for (int t = 0; t < m->trinum; t++)
   {
-    if (m->triobs[t] <= -2) return fail (PMX_EUNSUPPORTED, "moving obstacle triangles are not supported");
+    if (m->triobs[t] <= -2 && -m->triobs[t]-2 >= m->obsnum) return fail (PMX_EINVAL, "triangle obstacle index out of range");
     if (m->triobs[t] >= m->parnum) return fail (PMX_EINVAL, "triangle particle index out of range");
     if (m->triobs[t] >= 0 && m->ellnum > 0)
       return fail (PMX_EUNSUPPORTED, "contact between spheres and meshed particles is not supported yet");
@@ -110,7 +123,7 @@ InstOff model_counts (const pmx_model &m)
 {
   InstOff o;
   o.par = m.parnum; o.ell = m.ellnum; o.tri = m.trinum; o.nod = m.nodnum; o.fac = m.facnum; o.spr = m.sprnum;
-  o.scrv = m.sprcrv; o.dcrv = m.dashcrv; o.ucrv = m.uncrv; o.pair = m.pairnum;
+  o.scrv = m.sprcrv; o.dcrv = m.dashcrv; o.ucrv = m.uncrv; o.pair = m.pairnum; o.obs = m.obsnum;
   return o;
 }
 
@@ -186,6 +199,20 @@ int upload_core (pmx_ctx *c, const pmx_model *m, const std::vector<InstOff> &off
   if (!same)
   {
     if ((rc = alloc_rows (*c))) return rc;
+  }
+  /* moving obstacles */
+  {
+    c->O = m->obsnum;
+    std::vector<int> mtri, mtobs, tmov (std::max (m->trinum, 1), 0);
+    for (int i = 0; i < m->obsnum; i++)
+      for (int t = m->trirng[2*i]; t < m->trirng[2*i+1]; t++) { mtri.push_back (t); mtobs.push_back (i); tmov[t] = 1; }
+    c->nmtri = (int) mtri.size ();
+    size_t O = c->O;
+    if ((rc = up (*c, c->obspnt, m->obspnt, 3*O, "obspnt")) || (rc = up (*c, c->obsang, m->obsang, 3*O, "obsang")) ||
+        (rc = up (*c, c->obslin, m->obslin, 3*O, "obslin")) || (rc = up (*c, c->mtri, mtri.data (), mtri.size (), "obstacles")) ||
+        (rc = up (*c, c->mtobs, mtobs.data (), mtobs.size (), "obstacles")) ||
+        (rc = up (*c, c->tmov, tmov.data (), T, "obstacles")))
+      return rc;
   }
   if ((rc = up (*c, c->nodes, m->nodes, 6*(size_t) c->NOD, "nodes")) ||
       (rc = up (*c, c->nodpart, m->nodpart, (size_t) c->NOD, "nodpart")) ||
@@ -311,6 +338,10 @@ int pmx_download (pmx_ctx *c, pmx_model *m)
       (rc = down (*c, m->center, c->center, 6*E)) || (rc = down (*c, m->tri, c->tri, 9*T)) ||
       (rc = down (*c, m->nodes, c->nodes, 6*(size_t) c->NOD)) || (rc = springs_download (*c, m)))
     return rc;
+  if (m->obsnum == c->O && c->O > 0 &&
+      ((rc = down (*c, m->obspnt, c->obspnt, 3*(size_t) c->O)) || (rc = down (*c, m->obsang, c->obsang, 3*(size_t) c->O)) ||
+       (rc = down (*c, m->obslin, c->obslin, 3*(size_t) c->O))))
+    return rc;
   PMX_HIP (hipStreamSynchronize (c->stream));
   m->curtime = c->curtime; m->curstep = c->curstep; m->stepnum = c->stepnum;
   return PMX_OK;
@@ -377,10 +408,27 @@ static int dem_core (pmx_ctx *c, double duration, double step, const pmx_forcing
   auto t0 = std::chrono::steady_clock::now ();
   int rc = 0;
   double step0, step1;
+  int ctl0[CTL_COUNT] = {0};
+  PMX_HIP (hipMemcpyAsync (c->ctl.p, ctl0, sizeof (ctl0), hipMemcpyHostToDevice, c->stream));
+  PMX_HIP (hipMemsetAsync (c->dflags.p, 0, FLAG_COUNT * sizeof (int), c->stream));
   if (c->curtime == 0.0)
   {
     step0 = step;
     if ((rc = step_euler (*c, 0.5*step0)) || (rc = step_shapes (*c)) || (rc = step_mesh_shapes (*c, false))) return rc;
+    if (c->O > 0)
+    {
+      /* obstaclev (step0/2) and obstacles (step0), parmec.cpp:2403-2405 */
+      std::vector<double> t0 (6 * (size_t) c->O, 0.0);
+      for (int i = 0; i < c->I; i++)
+      {
+        const pmx_forcing *f = fc ? fc + (nfc > 1 ? i : 0) : nullptr;
+        int o0 = c->off[i].obs, no = c->off[i+1].obs - o0;
+        if (f && f->obstab0) std::memcpy (&t0[6*(size_t) o0], f->obstab0, 6*sizeof (double)*no);
+      }
+      if ((rc = up (*c, c->obstab, t0.data (), t0.size (), "obstacle table")) ||
+          (rc = step_obstacles (*c, c->obstab.p, 0, step0)))
+        return rc;
+    }
   }
   else step0 = c->curstep;
   step1 = step0;
@@ -404,10 +452,22 @@ static int dem_core (pmx_ctx *c, double duration, double step, const pmx_forcing
     }
     if ((rc = up (*c, c->gtab, g.data (), g.size (), "gravity table")) || (rc = up (*c, c->dtab, d.data (), d.size (), "damping table")))
       return rc;
+    if (c->O > 0)
+    {
+      /* obstaclev at curtime+step0 of every step, [step][obstacle][6] */
+      size_t O = c->O;
+      std::vector<double> ot (6 * ns * O, 0.0);
+      for (int i = 0; i < c->I; i++)
+      {
+        const pmx_forcing *f = fc ? fc + (nfc > 1 ? i : 0) : nullptr;
+        int o0 = c->off[i].obs, no = c->off[i+1].obs - o0;
+        if (!f || !f->obstab || no == 0) continue;
+        for (int k = 0; k < nsteps; k++)
+          std::memcpy (&ot[6*((size_t) k*O + o0)], f->obstab + 6*(size_t) k*no, 6*sizeof (double)*no);
+      }
+      if ((rc = up (*c, c->obstab, ot.data (), ot.size (), "obstacle table"))) return rc;
+    }
   }
-  int ctl0[CTL_COUNT] = {0};
-  PMX_HIP (hipMemcpyAsync (c->ctl.p, ctl0, sizeof (ctl0), hipMemcpyHostToDevice, c->stream));
-  PMX_HIP (hipMemsetAsync (c->dflags.p, 0, FLAG_COUNT * sizeof (int), c->stream));
   c->cur0 = c->cur;
   c->graph_valid = false;
   c->stats.max_row = 0;

@@ -83,7 +83,7 @@ int pmx_upload_batch (pmx_ctx *c, const pmx_model *ms, int count)
     InstOff n = model_counts (ms[i]), &a = off[i], &b = off[i+1];
     b.par = a.par + n.par; b.ell = a.ell + n.ell; b.tri = a.tri + n.tri; b.nod = a.nod + n.nod;
     b.fac = a.fac + n.fac; b.spr = a.spr + n.spr; b.scrv = a.scrv + n.scrv; b.dcrv = a.dcrv + n.dcrv;
-    b.ucrv = a.ucrv + n.ucrv; b.pair = a.pair + n.pair;
+    b.ucrv = a.ucrv + n.ucrv; b.pair = a.pair + n.pair; b.obs = a.obs + n.obs;
     if ((long long) a.ell + n.ell >= (long long) PMX_TRI_FLAG || (long long) a.tri + n.tri >= (long long) PMX_TRI_FLAG)
       return fail (PMX_EINVAL, "batch too large");
   }
@@ -126,7 +126,25 @@ int pmx_upload_batch (pmx_ctx *c, const pmx_model *ms, int count)
   CATD (center, 6, E, Eo, NE); CATD (radii, 3, E, Eo, NE);
   CATI (part, 1, E, Eo, NE, shp); CATI (ellcol, 1, E, Eo, NE, id);
   m.trinum = (int) NT;
-  CATD (tri, 9, T, To, NT); CATI (tricol, 1, T, To, NT, id); CATI (triobs, 1, T, To, NT, shp1);
+  CATD (tri, 9, T, To, NT); CATI (tricol, 1, T, To, NT, id);
+  /* triobs: >= 0 a particle, -1 static, <= -2 moving obstacle -(l+2) */
+  CATI (triobs, 1, T, To, NT, [&] (int v, int i) { return v >= 0 ? v + off[i].par : (v <= -2 ? v - off[i].obs : v); });
+  m.obsnum = tot.obs;
+  m.trirng = g.Iv (2*(size_t) tot.obs);
+  m.obspnt = g.D (3*(size_t) tot.obs); m.obsang = g.D (3*(size_t) tot.obs); m.obslin = g.D (3*(size_t) tot.obs);
+  for (int i = 0; i < count; i++)
+    for (int o = 0; o < ms[i].obsnum; o++)
+    {
+      int q = off[i].obs + o;
+      m.trirng[2*q] = ms[i].trirng[2*o] + off[i].tri;
+      m.trirng[2*q+1] = ms[i].trirng[2*o+1] + off[i].tri;
+      for (int d = 0; d < 3; d++)
+      {
+        m.obspnt[3*q+d] = ms[i].obspnt[3*o+d];
+        m.obsang[3*q+d] = ms[i].obsang ? ms[i].obsang[3*o+d] : 0.0;
+        m.obslin[3*q+d] = ms[i].obslin ? ms[i].obslin[3*o+d] : 0.0;
+      }
+    }
   m.nodnum = (int) NN;
   CATD (nodes, 6, Nd, Ndo, NN); CATI (nodpart, 1, Nd, Ndo, NN, shp);
   m.facnum = (int) NF;
@@ -194,6 +212,9 @@ int pmx_download_batch (pmx_ctx *c, pmx_model *ms, int count)
   m.angular = g.D (6*NP); m.linear = g.D (3*NP); m.rotation = g.D (9*NP); m.position = g.D (6*NP);
   m.inverse = g.D (9*NP); m.invm = g.D (NP); m.force = g.D (3*NP); m.torque = g.D (3*NP);
   m.center = g.D (6*NE); m.tri = g.D (9*NT); m.nodes = g.D (6*NN);
+  size_t NO = tot.obs;
+  m.obsnum = (int) NO;
+  m.obspnt = g.D (3*NO); m.obsang = g.D (3*NO); m.obslin = g.D (3*NO);
   if (NS > 0)
   {
     m.sprpnt = g.D (12*NS); m.sprdir = g.D (6*NS); m.yield = g.D (2*NS); m.sprsdsp = g.D (3*NS);
@@ -222,7 +243,14 @@ int pmx_download_batch (pmx_ctx *c, pmx_model *ms, int count)
     split<int> (m.sprflg, 1, count, NS, [&] (int i) { return ms[i].sprflg; }, S, So);
   }
 #undef SPD
-  for (int i = 0; i < count; i++) { ms[i].curtime = m.curtime; ms[i].curstep = m.curstep; ms[i].stepnum = m.stepnum; }
+  for (int i = 0; i < count; i++)
+  {
+    ms[i].curtime = m.curtime; ms[i].curstep = m.curstep; ms[i].stepnum = m.stepnum;
+    size_t o0 = off[i].obs, no = ms[i].obsnum;
+    if (ms[i].obspnt) std::memcpy (ms[i].obspnt, m.obspnt + 3*o0, 3*no*sizeof (double));
+    if (ms[i].obsang) std::memcpy (ms[i].obsang, m.obsang + 3*o0, 3*no*sizeof (double));
+    if (ms[i].obslin) std::memcpy (ms[i].obslin, m.obslin + 3*o0, 3*no*sizeof (double));
+  }
   return PMX_OK;
 }
 

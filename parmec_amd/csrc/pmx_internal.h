@@ -87,7 +87,7 @@ struct TreeDev
 /* offsets of one instance's items in the concatenated arrays of a batch */
 struct InstOff
 {
-  int par = 0, ell = 0, tri = 0, nod = 0, fac = 0, spr = 0, scrv = 0, dcrv = 0, ucrv = 0, pair = 0;
+  int par = 0, ell = 0, tri = 0, nod = 0, fac = 0, spr = 0, scrv = 0, dcrv = 0, ucrv = 0, pair = 0, obs = 0;
 };
 
 /* kernels timed by the profiler */
@@ -139,6 +139,12 @@ struct pmx_ctx
   /* triangles */
   dvec<double> tri;
   dvec<int> tricol, triobs;
+  /* moving obstacles: point, angular and linear velocity [3*O]; the
+   * triangles they move (mtri) with their obstacle (mtobs); per-step reach
+   * lists leaf -> moving triangles (mtlist [nleaves][mcap]) */
+  int O = 0, nmtri = 0, mcap = 32;
+  dvec<double> obspnt, obsang, obslin, obstab;
+  dvec<int> mtri, mtobs, tmov, mtcnt, mtlist;
   /* pairing */
   dvec<int> pairs, ikind;
   dvec<double> iparam;
@@ -222,6 +228,7 @@ int tree_build (pmx_ctx &c, bool all);       /* all: snapshot every instance's c
                                                 the instances flagged in c.ovf */
 int tree_trilists (pmx_ctx &c);              /* leaf -> triangle lists */
 int tree_nbrlists (pmx_ctx &c);              /* leaf -> neighbour leaf lists */
+int step_mtri (pmx_ctx &c);                  /* leaf -> moving obstacle triangles, per step */
 
 /* ---- scans (pmx_tree.hip) ---- */
 int scan_exclusive (pmx_ctx &c, int *data, int n, int *total_host, const int *ctl = nullptr);
@@ -234,6 +241,7 @@ int step_rows (pmx_ctx &c, double h);
 int step_dynamics (pmx_ctx &c, double step0, double step1);
 int step_end (pmx_ctx &c);
 int enqueue_step (pmx_ctx &c, double h);     /* one whole time step */
+int step_obstacles (pmx_ctx &c, const double *tab, int row, double h);
 int step_shapes (pmx_ctx &c);
 int step_euler (pmx_ctx &c, double h);
 int step_invert (pmx_ctx &c);

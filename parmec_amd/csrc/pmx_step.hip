@@ -146,7 +146,7 @@ __global__ __launch_bounds__(DET_TPB)
 void k_detect (int E, const double *center, const double *radii, const int *part, const double *cell, int nleaves,
                const int *lcount, const int *loff, const int *lmem, const double *lpos,
                const double *tri, int T, const int *tricnt, const int *trioff, const int *trilist,
-               const int *nbrcnt, const int *nbroff, const int *nbrlist,
+               const int *mtcnt, const int *mtlist, int mcap, const int *nbrcnt, const int *nbroff, const int *nbrlist,
                uint32_t *hits, int *hitcnt, int hcap, int *xbuf, int *xcnt, int xcap, int *flags, const int *ctl)
 {
   if (ctl[CTL_ABORT]) return;
@@ -278,8 +278,12 @@ void k_detect (int E, const double *center, const double *radii, const int *part
     __syncthreads ();
   }
 
-  /* triangles whose AABB reaches Q (test_triangles, condet.ispc:412-516) */
-  int ntq = T > 0 ? tricnt[Q] : 0;
+  /* triangles whose AABB reaches Q (test_triangles, condet.ispc:412-516):
+   * static ones from the rebuild-time lists, moving ones from this step's */
+  int nsq = T > 0 ? tricnt[Q] : 0;
+  int nmq = mtcnt ? mtcnt[Q] : 0;
+  nmq = nmq < mcap ? nmq : mcap;
+  int ntq = nsq + nmq;
   if (ntq > 0 && qcnt > 0)
   {
     int a0 = trioff[Q];
@@ -287,7 +291,7 @@ void k_detect (int E, const double *center, const double *radii, const int *part
     for (int q = t; q < work; q += DET_TPB)
     {
       int ti = q / ntq, m = q - ti * ntq;
-      int tt = trilist[a0 + m];
+      int tt = m < nsq ? trilist[a0 + m] : mtlist[(size_t) Q*mcap + (m - nsq)];
       double v[9];
 #pragma unroll
       for (int k = 0; k < 9; k++) v[k] = tri[(size_t) k*T + tt];
@@ -349,7 +353,8 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
              int ninst, const int *ell_inst, const int *pair_off,
              uint32_t *key0, R *st0, int *cnt0, uint32_t *key1, R *st1, int *cnt1, int cur0, int cap,
              const uint32_t *hits, const int *hitcnt, int hcap, const int *xbuf, const int *xcnt, int xcap,
-             const double *gtab, const int *ctl, double h, double *force, double *torque, int *flags)
+             const double *gtab, const int *ctl, double h, double *force, double *torque, int *flags,
+             const double *obspnt, const double *obsang, const double *obslin)
 {
   extern __shared__ uint32_t sh[];
   if (ctl[CTL_ABORT]) return;
@@ -516,6 +521,17 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
       vj[0] = oj[1]*z[2]-oj[2]*z[1] + (R) linear[j];
       vj[1] = oj[2]*z[0]-oj[0]*z[2] + (R) linear[(size_t) N + j];
       vj[2] = oj[0]*z[1]-oj[1]*z[0] + (R) linear[2*(size_t) N + j];
+    }
+    else if (j <= -2)   /* moving obstacle, forces.ispc:265-280 */
+    {
+      int l = -j-2;
+      z[0] = (R) (point[0]-obspnt[3*l]);
+      z[1] = (R) (point[1]-obspnt[3*l+1]);
+      z[2] = (R) (point[2]-obspnt[3*l+2]);
+      R oj[3] = {(R) obsang[3*l], (R) obsang[3*l+1], (R) obsang[3*l+2]};
+      vj[0] = oj[1]*z[2]-oj[2]*z[1] + (R) obslin[3*l];
+      vj[1] = oj[2]*z[0]-oj[0]*z[2] + (R) obslin[3*l+1];
+      vj[2] = oj[0]*z[1]-oj[1]*z[0] + (R) obslin[3*l+2];
     }
     else { vj[0] = vj[1] = vj[2] = 0.0; }
     R vij[3] = {vj[0]-vi[0], vj[1]-vi[1], vj[2]-vi[2]};
@@ -694,6 +710,53 @@ __global__ void k_dynamics (int N, double *angular, double *linear, double *rota
 }
 
 /* ellipsoids_task centre update, shapes.ispc:29-67 */
+/* obstacles, obstacles.ispc:28-78, one thread per moving triangle: the
+ * rigid motion of its obstacle over the step, rotation expmap(h*omega) about
+ * the obstacle point x and translation h*v, with the velocities the
+ * obstaclev call of this step set (row `row` of tab [rows][O][6], or row
+ * ctl[KSTEP] when row < 0).  Canonical stride 3 for the velocities. */
+__global__ void k_obstacles (int nm, const int *mtri, const int *mtobs, int T, double *tri, const double *obspnt,
+                             const double *tab, int O, int row, double h, const int *ctl)
+{
+  if (ctl[CTL_ABORT]) return;
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nm) return;
+  int t = mtri[q], i = mtobs[q];
+  const double *r = tab + 6*((size_t) (row >= 0 ? row : ctl[CTL_KSTEP]) * O + i);
+  double x[3] = {obspnt[3*i], obspnt[3*i+1], obspnt[3*i+2]};
+  double y[3], DL[9];
+  y[0] = x[0] + h*r[3];
+  y[1] = x[1] + h*r[4];
+  y[2] = x[2] + h*r[5];
+  expmap (h*r[0], h*r[1], h*r[2], DL);
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+  {
+    double z[3] = {tri[(size_t) (3*a)*T + t] - x[0], tri[(size_t) (3*a+1)*T + t] - x[1], tri[(size_t) (3*a+2)*T + t] - x[2]};
+    tri[(size_t) (3*a)*T + t] = DL[0]*z[0]+DL[3]*z[1]+DL[6]*z[2] + y[0];
+    tri[(size_t) (3*a+1)*T + t] = DL[1]*z[0]+DL[4]*z[1]+DL[7]*z[2] + y[1];
+    tri[(size_t) (3*a+2)*T + t] = DL[2]*z[0]+DL[5]*z[1]+DL[8]*z[2] + y[2];
+  }
+}
+
+/* then per obstacle: x += h*v and the velocities of the step become the
+ * current ones (read by the next step's contact law) */
+__global__ void k_obspnt (int O, double *obspnt, double *obsang, double *obslin, const double *tab, int row, double h,
+                          const int *ctl)
+{
+  if (ctl[CTL_ABORT]) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= O) return;
+  const double *r = tab + 6*((size_t) (row >= 0 ? row : ctl[CTL_KSTEP]) * O + i);
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+  {
+    obspnt[3*i+d] = obspnt[3*i+d] + h*r[3+d];
+    obsang[3*i+d] = r[d];
+    obslin[3*i+d] = r[3+d];
+  }
+}
+
 __global__ void k_shapes (int E, int N, const int *part, double *center, const double *rotation, const double *position)
 {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -771,6 +834,8 @@ int step_prepare (pmx_ctx &c)
    * allocate while a step is being captured */
   size_t need = (size_t) L / (PMX_TPB*4) + 64;
   if (c.tree.scanbuf.alloc (need) != hipSuccess) { pmx_set_error ("out of device memory (scan)"); return PMX_ENOMEM; }
+  if (c.nmtri > 0 && (c.mtcnt.alloc (L) || c.mtlist.alloc ((size_t) L * c.mcap)))
+  { pmx_set_error ("out of device memory (obstacle lists)"); return PMX_ENOMEM; }
   return PMX_OK;
 }
 
@@ -812,6 +877,7 @@ int step_detect (pmx_ctx &c)
                         t.nleaves, (const int*) c.lcount.p, (const int*) c.loff.p,
                         (const int*) c.lmem.p, (const double*) c.lpos.p, (const double*) c.tri.p, c.T,
                         (const int*) t.tricnt.p, (const int*) t.trioff.p, (const int*) t.trilist.p,
+                        c.nmtri > 0 ? (const int*) c.mtcnt.p : (const int*) nullptr, (const int*) c.mtlist.p, c.mcap,
                         (const int*) t.nbrcnt.p, (const int*) t.nbroff.p, (const int*) t.nbrlist.p,
                         c.hits.p, c.hitcnt.p, c.hcap, c.xbuf.p, c.xcnt.p, c.xcap, c.dflags.p, (const int*) c.ctl.p);
   }
@@ -842,7 +908,8 @@ int step_rows (pmx_ctx &c, double h)
                           c.I, (const int*) c.ell_inst.p, (const int*) c.pair_off.p, c.rowkey[0].p, c.rowst32[0].p, c.rowcnt[0].p, c.rowkey[1].p, c.rowst32[1].p, c.rowcnt[1].p,
                           c.cur0, c.cap, (const uint32_t*) c.hits.p,
                           (const int*) c.hitcnt.p, c.hcap, (const int*) c.xbuf.p, (const int*) c.xcnt.p, c.xcap,
-                          (const double*) c.gtab.p, (const int*) c.ctl.p, h, c.force.p, c.torque.p, c.dflags.p);
+                          (const double*) c.gtab.p, (const int*) c.ctl.p, h, c.force.p, c.torque.p, c.dflags.p,
+                            (const double*) c.obspnt.p, (const double*) c.obsang.p, (const double*) c.obslin.p);
     else
       hipLaunchKernelGGL (k_rows<double>, dim3 ((c.E + ROWS_TPB - 1) / ROWS_TPB), dim3 (ROWS_TPB), shm, c.stream, c.E, c.N,
                           (const double*) c.center.p, (const double*) c.radii.p, (const int*) c.part.p,
@@ -853,7 +920,8 @@ int step_rows (pmx_ctx &c, double h)
                           c.I, (const int*) c.ell_inst.p, (const int*) c.pair_off.p, c.rowkey[0].p, c.rowst[0].p, c.rowcnt[0].p, c.rowkey[1].p, c.rowst[1].p, c.rowcnt[1].p,
                           c.cur0, c.cap, (const uint32_t*) c.hits.p,
                           (const int*) c.hitcnt.p, c.hcap, (const int*) c.xbuf.p, (const int*) c.xcnt.p, c.xcap,
-                          (const double*) c.gtab.p, (const int*) c.ctl.p, h, c.force.p, c.torque.p, c.dflags.p);
+                          (const double*) c.gtab.p, (const int*) c.ctl.p, h, c.force.p, c.torque.p, c.dflags.p,
+                          (const double*) c.obspnt.p, (const double*) c.obsang.p, (const double*) c.obslin.p);
   }
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
@@ -878,12 +946,27 @@ int step_end (pmx_ctx &c)
   return PMX_OK;
 }
 
+/* obstaclev + obstacles at the end of a step (parmec.cpp:2498-2500); tab
+ * holds the velocities per step (row < 0: the step's own row) */
+int step_obstacles (pmx_ctx &c, const double *tab, int row, double h)
+{
+  if (c.O == 0) return PMX_OK;
+  if (c.nmtri > 0)
+    hipLaunchKernelGGL (k_obstacles, grid (c.nmtri), dim3 (PMX_TPB), 0, c.stream, c.nmtri, (const int*) c.mtri.p,
+                        (const int*) c.mtobs.p, c.T, c.tri.p, (const double*) c.obspnt.p, tab, c.O, row, h,
+                        (const int*) c.ctl.p);
+  hipLaunchKernelGGL (k_obspnt, grid (c.O), dim3 (PMX_TPB), 0, c.stream, c.O, c.obspnt.p, c.obsang.p, c.obslin.p, tab,
+                      row, h, (const int*) c.ctl.p);
+  PMX_HIP (hipGetLastError ());
+  return PMX_OK;
+}
+
 int enqueue_step (pmx_ctx &c, double h)
 {
   int rc;
-  if (c.E > 0 && ((rc = step_bin (c)) || (rc = step_detect (c)))) return rc;
+  if (c.E > 0 && ((rc = step_bin (c)) || (rc = step_mtri (c)) || (rc = step_detect (c)))) return rc;
   if ((rc = step_rows (c, h)) || (rc = step_springs (c, h)) || (rc = step_dynamics (c, h, h)) ||
-      (rc = step_mesh_shapes (c)) || (rc = step_end (c)))
+      (rc = step_mesh_shapes (c)) || (rc = step_obstacles (c, c.obstab.p, -1, h)) || (rc = step_end (c)))
     return rc;
   return PMX_OK;
 }

@@ -192,6 +192,14 @@ __global__ void k_set_i (int *a, int n, int v)
   if (i < n) a[i] = v;
 }
 
+/* the same inside a step (a no-op once the step is aborted) */
+__global__ void k_set_ctl (const int *ctl, int *a, int n, int v)
+{
+  if (ctl[CTL_ABORT]) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
 /* copy the current centres of the selected instances (all when sel is null) */
 __global__ void k_snap (int n, const double *center, double *bcen, const int *inst, const int *sel)
 {
@@ -481,12 +489,13 @@ __device__ __forceinline__ int reach_leaves_count (int root, const int *left, co
   return visited;
 }
 
-__global__ void k_tri_reach (int T, const double *tri, const int *roots, const int *tri_inst, const int *left,
-                             const int *right, const int *dim, const double *coord, int *tricnt, const int *trioff,
-                             int *trilist, int *fill, int *flag)
+__global__ void k_tri_reach (int T, const double *tri, const int *tmov, const int *roots, const int *tri_inst,
+                             const int *left, const int *right, const int *dim, const double *coord, int *tricnt,
+                             const int *trioff, int *trilist, int *fill, int *flag)
 {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= T) return;
+  if (tmov[t]) return;                 /* moving obstacle triangles: reach lists per step */
   int root = roots[tri_inst[t]];
   if (root == PMX_NO_ROOT) return;     /* the triangle's instance has no spheres */
   double v[3][3];
@@ -501,6 +510,56 @@ __global__ void k_tri_reach (int T, const double *tri, const int *roots, const i
     lo[b] = mn; hi[b] = mx;
   }
   reach_leaves_count (root, left, right, dim, coord, lo, hi, tricnt, t, trioff, trilist, fill, flag);
+}
+
+/* leaves reached by each moving obstacle triangle's current box (the
+ * traversal of test_triangles, condet.ispc:412-426), appended to per-leaf
+ * lists of capacity mcap; run every step since the triangles move */
+__global__ void k_mtri_reach (int nm, const int *mtri, int T, const double *tri, const int *roots, const int *tri_inst,
+                              const int *left, const int *right, const int *dim, const double *coord, int *mtcnt,
+                              int *mtlist, int mcap, int *flag, const int *ctl)
+{
+  if (ctl[CTL_ABORT]) return;
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nm) return;
+  int t = mtri[q];
+  int root = roots[tri_inst[t]];
+  if (root == PMX_NO_ROOT) return;
+  double lo[3], hi[3];
+  for (int b = 0; b < 3; b++)
+  {
+    double x0 = tri[(size_t) b*T + t], x1 = tri[(size_t) (3+b)*T + t], x2 = tri[(size_t) (6+b)*T + t];
+    double mn = x0 < x1 ? x0 : x1, mx = x0 > x1 ? x0 : x1;
+    lo[b] = mn < x2 ? mn : x2;
+    hi[b] = mx > x2 ? mx : x2;
+  }
+  int stack[64];
+  int sp = 0, ref = root;
+  for (;;)
+  {
+    if (ref >= 0)
+    {
+      int d = dim[ref];
+      double cc = coord[ref];
+      bool gl = lo[d] <= cc, gr = hi[d] >= cc;
+      if (gl && gr)
+      {
+        if (sp < 64) stack[sp++] = right[ref];
+        else { atomicAdd (flag, 1); return; }
+        ref = left[ref];
+      }
+      else if (gl) ref = left[ref];
+      else if (gr) ref = right[ref];
+      else { if (sp == 0) break; ref = stack[--sp]; }
+      continue;
+    }
+    int L = -ref-1;
+    int s = atomicAdd (&mtcnt[L], 1);
+    if (s < mcap) mtlist[(size_t) L*mcap + s] = t;
+    else atomicAdd (flag, 1);
+    if (sp == 0) break;
+    ref = stack[--sp];
+  }
 }
 
 /* leaves whose closed cell overlaps leaf Q's cell inflated by rmax: every leaf
@@ -557,6 +616,21 @@ __global__ void k_fill_int (int *a, int n, int v)
 inline dim3 grid (int n) { return dim3 ((n + PMX_TPB - 1) / PMX_TPB); }
 
 } /* namespace */
+
+int step_mtri (pmx_ctx &c)
+{
+  if (c.nmtri == 0 || c.E == 0) return PMX_OK;
+  TreeDev &t = c.tree;
+  int L = std::max (t.nleaves, 1);
+  hipLaunchKernelGGL (k_set_ctl, grid (L), dim3 (PMX_TPB), 0, c.stream, (const int*) c.ctl.p, c.mtcnt.p, L, 0);
+  hipLaunchKernelGGL (k_mtri_reach, grid (c.nmtri), dim3 (PMX_TPB), 0, c.stream, c.nmtri, (const int*) c.mtri.p, c.T,
+                      (const double*) c.tri.p, (const int*) t.roots.p, (const int*) c.tri_inst.p, (const int*) t.left.p,
+                      (const int*) t.right.p, (const int*) t.dim.p, (const double*) t.coord.p, c.mtcnt.p, c.mtlist.p,
+                      c.mcap, c.dflags.p + FLAG_TRILIST_OVERFLOW, (const int*) c.ctl.p);
+  PMX_HIP (hipGetLastError ());
+  return PMX_OK;
+}
+
 
 int scan_exclusive (pmx_ctx &c, int *data, int n, int *total_host, const int *ctl)
 {
@@ -714,7 +788,7 @@ int tree_trilists (pmx_ctx &c)
   }
   int *flag = c.dflags.p + FLAG_TRILIST_OVERFLOW;
   hipLaunchKernelGGL (k_tri_reach, grid (c.T), dim3 (PMX_TPB), 0, c.stream, c.T, (const double*) c.tri.p,
-                      (const int*) t.roots.p, (const int*) c.tri_inst.p, (const int*) t.left.p, (const int*) t.right.p,
+                      (const int*) c.tmov.p, (const int*) t.roots.p, (const int*) c.tri_inst.p, (const int*) t.left.p, (const int*) t.right.p,
                       (const int*) t.dim.p, (const double*) t.coord.p, t.tricnt.p, (const int*) nullptr, (int*) nullptr,
                       (int*) nullptr, flag);
   PMX_HIP (hipGetLastError ());
@@ -728,7 +802,7 @@ int tree_trilists (pmx_ctx &c)
   if (fill.alloc (L)) { pmx_set_error ("out of device memory (trilists)"); return PMX_ENOMEM; }
   hipLaunchKernelGGL (k_fill_int, grid (L), dim3 (PMX_TPB), 0, c.stream, fill.p, L, 0);
   hipLaunchKernelGGL (k_tri_reach, grid (c.T), dim3 (PMX_TPB), 0, c.stream, c.T, (const double*) c.tri.p,
-                      (const int*) t.roots.p, (const int*) c.tri_inst.p, (const int*) t.left.p, (const int*) t.right.p,
+                      (const int*) c.tmov.p, (const int*) t.roots.p, (const int*) c.tri_inst.p, (const int*) t.left.p, (const int*) t.right.p,
                       (const int*) t.dim.p, (const double*) t.coord.p, (int*) nullptr, (const int*) t.trioff.p,
                       t.trilist.p, fill.p, flag);
   PMX_HIP (hipGetLastError ());

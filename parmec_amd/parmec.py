@@ -69,6 +69,8 @@ class _State:
         self.vel = {}                # particle -> (lin or None, ang or None)
         self.tris = []               # 9-tuples
         self.tricol = []
+        self.triobs = []
+        self.obstacles = []          # (first triangle, end, point, linear callable, angular callable)
         self.pairs = [(0, 0, [0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.5])]   # pair_reset, parmec.cpp:375-388
         self.tms = []
         self.gravity = [0.0, 0.0, 0.0]
@@ -163,10 +165,21 @@ def OBSTACLE(triangles, color, point=None, linear=None, angular=None):
         cols = [int(color)] * len(triangles)
     if any(c <= 0 for c in cols):
         raise ValueError("'color' must be positive")
+    obs = -1
     if point is not None and (linear is not None or angular is not None):
-        raise NotImplementedError("moving obstacles are not on the device path")
+        if not isinstance(point, tuple) or len(point) != 3:
+            raise TypeError("'point' must be a 3-element tuple")
+        for f, name in ((linear, "linear"), (angular, "angular")):
+            if f is not None and not callable(f):
+                raise TypeError(f"'{name}' must be callable")
+        # the obstacle's own index k (triobs = -k-2, the value the contact law
+        # decodes); the reference stores -(k+1)-2 here (input.cpp:1391, 1433),
+        # which makes the law read the next obstacle's velocity
+        obs = -len(_S.obstacles) - 2
+        _S.obstacles.append((len(_S.tris), len(_S.tris) + len(triangles), tuple(float(x) for x in point), linear, angular))
     _S.tris.extend(tuple(float(x) for x in t) for t in triangles)
     _S.tricol.extend(cols)
+    _S.triobs.extend([obs] * len(triangles))
     _S.dirty = True
 
 
@@ -474,7 +487,15 @@ def _build_model():
     if t:
         m.tri[:] = np.array(_S.tris, dtype=np.float64).T
         m.tricol[:] = np.array(_S.tricol, dtype=np.int32)
-        m.triobs[:] = -1
+        m.triobs[:] = np.array(_S.triobs, dtype=np.int32)
+    m.set_obstacles(len(_S.obstacles))
+    for k, (a, b, pnt, lin, ang) in enumerate(_S.obstacles):
+        m.trirng[k] = (a, b)
+        m.obspnt[k] = pnt
+    if old is not None:   # moved obstacle triangles and obstacles keep their state
+        m.tri[:, :old.trinum] = old.tri
+        k = old.obsnum
+        m.obspnt[:k], m.obslin[:k], m.obsang[:k] = old.obspnt, old.obslin, old.obsang
     if _S.springs:
         _spring_arrays(m)
     # sort_materials, parmec.cpp:1636-1663: pairs ordered by colours
@@ -524,7 +545,22 @@ def DEM(duration, step, interval=None, prefix=None, adaptive=0.0):
                 dtab[i, :3] = _S.lindamp(t)
             if _S.angdamp is not None:
                 dtab[i, 3:] = _S.angdamp(t)
-    f = pmx.forcing(len(times), gravity=tuple(_S.gravity), gtab=gtab, dtab=dtab)
+    obstab = obstab0 = None
+    if _S.obstacles:
+        # obstaclev (input.cpp:4199-4244) at curtime+h of each step, and at
+        # h/2 before the first step of the first call
+        def vel(t):
+            row = np.zeros((len(_S.obstacles), 6))
+            for k, (_, _, _, lin, ang) in enumerate(_S.obstacles):
+                if ang is not None:
+                    row[k, :3] = [float(x) for x in ang(t)]
+                if lin is not None:
+                    row[k, 3:] = [float(x) for x in lin(t)]
+            return row
+        obstab = np.array([vel(t + h) for t in times]).reshape(len(times), len(_S.obstacles), 6)
+        if m.curtime == 0.0:
+            obstab0 = vel(0.5 * h)
+    f = pmx.forcing(len(times), gravity=tuple(_S.gravity), gtab=gtab, dtab=dtab, obstab=obstab, obstab0=obstab0)
     elapsed = _S.engine.dem(float(duration), float(step), f)
     _S.engine.download(m)
     return elapsed

@@ -147,6 +147,16 @@ class Model:
         self.curtime, self.curstep, self.stepnum = 0.0, 0.0, 0
         self.set_mesh(0, 0)
         self.set_springs(0, 0, 0, 0)
+        self.set_obstacles(0)
+
+    def set_obstacles(self, obsnum):
+        """Moving obstacles (parmec.h:104-112): triangle ranges [obsnum][2],
+        point, linear and angular velocity [obsnum][3]."""
+        self.obsnum = obsnum
+        self.trirng = np.zeros((obsnum, 2), dtype=np.int32)
+        self.obspnt = np.zeros((obsnum, 3))
+        self.obslin = np.zeros((obsnum, 3))
+        self.obsang = np.zeros((obsnum, 3))
 
     def set_mesh(self, nodnum, facnum):
         """Mesh nodes [6][nodnum] (current, reference), node -> particle, and
@@ -190,9 +200,9 @@ class Model:
 
     _DBL = ("angular", "linear", "rotation", "position", "inertia", "inverse", "mass", "invm", "force", "torque",
             "center", "radii", "tri", "iparam", "nodes", "sprpnt", "spring", "dashpot", "unload", "yield_", "sprdir",
-            "sprfric", "sprkskn", "sprsdsp", "stroke0", "stroke", "sprfrc")
+            "sprfric", "sprkskn", "sprsdsp", "stroke0", "stroke", "sprfrc", "obspnt", "obslin", "obsang")
     _INT = ("flags", "part", "ellcol", "tricol", "triobs", "pairs", "ikind", "nodpart", "facnod", "factri", "sprtype",
-            "sprpart", "spridx", "dashidx", "unidx", "sprflg", "unspring", "sproffset")
+            "sprpart", "spridx", "dashidx", "unidx", "sprflg", "unspring", "sproffset", "trirng")
 
     def check(self):
         for name in self._DBL:
@@ -209,6 +219,7 @@ class Model:
         s = pmx_model()
         s.parnum, s.ellnum, s.trinum, s.pairnum = self.parnum, self.ellnum, self.trinum, self.pairnum
         s.nodnum, s.facnum, s.sprnum = self.nodnum, self.facnum, self.sprnum
+        s.obsnum = getattr(self, "obsnum", 0)
         s.sprcrv, s.dashcrv, s.uncrv = self.sprcrv, self.dashcrv, self.uncrv
         for name in self._DBL:
             setattr(s, name, _ptr(getattr(self, name), "d"))
@@ -222,7 +233,11 @@ class Model:
         self.curtime, self.curstep, self.stepnum = s.curtime, s.curstep, s.stepnum
 
 
-def forcing(nsteps=0, gravity=(0.0, 0.0, 0.0), gtab=None, damping=(0.0,) * 6, dtab=None):
+def forcing(nsteps=0, gravity=(0.0, 0.0, 0.0), gtab=None, damping=(0.0,) * 6, dtab=None, obstab=None, obstab0=None):
+    """Per-step inputs of dem(): gravity and damping constants or tables
+    ([nsteps][3], [nsteps][6]); moving obstacle velocities (angular 0-2,
+    linear 3-5) per step at curtime+step ([nsteps][obsnum][6]) and at step/2
+    for the first call ([obsnum][6])."""
     f = pmx_forcing()
     for k in range(3):
         f.gravity[k] = gravity[k]
@@ -239,6 +254,15 @@ def forcing(nsteps=0, gravity=(0.0, 0.0, 0.0), gtab=None, damping=(0.0,) * 6, dt
         assert dtab.shape[0] >= nsteps
         f.dtab = dtab.ctypes.data_as(_dp)
         keep.append(dtab)
+    if obstab is not None:
+        obstab = np.ascontiguousarray(obstab, dtype=np.float64)
+        assert obstab.shape[0] >= nsteps and obstab.shape[-1] == 6
+        f.obstab = obstab.ctypes.data_as(_dp)
+        keep.append(obstab)
+    if obstab0 is not None:
+        obstab0 = np.ascontiguousarray(obstab0, dtype=np.float64)
+        f.obstab0 = obstab0.ctypes.data_as(_dp)
+        keep.append(obstab0)
     f._keep = keep
     return f
 

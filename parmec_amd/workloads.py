@@ -292,3 +292,40 @@ def sweep_forcing(amp, freq, nsteps, step, curtime=0.0):
     would be (input.cpp:4424-4503)."""
     t = curtime + step * np.arange(nsteps)
     return base_excitation(t, amplitude=-amp, frequency=freq)
+
+
+def stirred_box(n=2000, seed=0, piston_speed=-0.2, stir_rate=20.0):
+    """box_lattice with two moving obstacles: a piston plate over the pack
+    descending at piston_speed (m/s) and a vertical blade through the pack
+    turning about the box's vertical axis at stir_rate (rad/s).  Returns the
+    model and a function giving the obstacle velocity table of dem()
+    (obstab [nsteps][2][6], obstab0 [2][6])."""
+    m = box_lattice(n=n, seed=seed)
+    lo = m.tri.reshape(3, 3, -1).min(axis=(0, 2))
+    hi = m.tri.reshape(3, 3, -1).max(axis=(0, 2))
+    top = float(m.center[2].max() + 0.006)
+    cx, cy = 0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1])
+    w = 0.45 * min(hi[0] - lo[0], hi[1] - lo[1])
+    piston = [(lo[0] + 1e-3, lo[1] + 1e-3, top, hi[0] - 1e-3, lo[1] + 1e-3, top, hi[0] - 1e-3, hi[1] - 1e-3, top),
+              (lo[0] + 1e-3, lo[1] + 1e-3, top, hi[0] - 1e-3, hi[1] - 1e-3, top, lo[0] + 1e-3, hi[1] - 1e-3, top)]
+    zb = float(m.center[2].min())
+    blade = [(cx - w, cy, zb, cx + w, cy, zb, cx + w, cy, top - 0.01),
+             (cx - w, cy, zb, cx + w, cy, top - 0.01, cx - w, cy, top - 0.01)]
+    T0 = m.trinum
+    extra = np.array(piston + blade, dtype=np.float64)
+    m.trinum = T0 + 4
+    m.tri = np.ascontiguousarray(np.concatenate([m.tri, extra.T], axis=1))
+    m.tricol = np.concatenate([m.tricol, np.full(4, 2, dtype=np.int32)])
+    m.triobs = np.concatenate([m.triobs, np.array([-2, -2, -3, -3], dtype=np.int32)])
+    m.set_obstacles(2)
+    m.trirng[0] = (T0, T0 + 2)
+    m.trirng[1] = (T0 + 2, T0 + 4)
+    m.obspnt[0] = (cx, cy, top)
+    m.obspnt[1] = (cx, cy, 0.5 * (zb + top))
+
+    def table(nsteps):
+        row = np.zeros((2, 6))
+        row[0, 5] = piston_speed
+        row[1, 2] = stir_rate
+        return np.tile(row, (nsteps, 1, 1)), row.copy()
+    return m, table
