@@ -1,15 +1,32 @@
 """Benchmark of the MI355X DEM step path.
 
-One step = one DEM time step (bin, detect, contact forces, integration,
-geometry update) over the whole model, with the model resident in HBM.
-Default workload: 2,000,000 polydisperse spheres (lognormal radius, median
-2 mm, sigma 0.25, clipped 1-4 mm) poured into a conical hopper of ~5,000
-static triangles, granular law k_n = 1e6 N/m, damping ratio 0.1, mu = 0.3,
-dt = 2e-6 s, FP64.
+Headline (value): the batched acceleration sweep in mixed precision.  512
+independent instances of the 10,000-sphere box set-up (sphere count uniform in
+[1k, 64k], base excitation g(t) = (A sin 2 pi f t, 0, -9.81) with A in
+[0.1, 2] g and f in [1, 20] Hz, seed = instance number), dt = 1e-5 s, stepped
+together in one batch (one fused launch per stage, the time step replayed as a
+HIP graph), FP32 contact law / FP64 integration.  The timed window is steps
+[warmup, warmup + steps) of the sweep's 10,000-step run, with the state and
+all inputs resident in HBM.
 
-Prints one JSON line (see the bench contract in the repository docs).
+Beside it, in the same JSON line:
+  plain_fp64  -- the drop-in FP64 path stepping sweep instances one at a time
+                 (a sample of instances over the same window), the
+                 comparison the batched mixed form is measured against;
+  hopper      -- 2,000,000 polydisperse spheres in a 5,000-triangle hopper,
+                 FP64 and mixed, dt = 2e-6 s;
+  roofline    -- the dominant kernel of the timed window: algorithmic bytes
+                 per launch over its mean duration from HIP events around
+                 every launch on the library's stream (the window runs its
+                 steps eagerly for this, about 1% slower than graph replay);
+  cpu_baseline -- the C++/OpenMP restatement of the reference step (the
+                 oracle) on a bounded sample of the same sweep state.
+
+One step = one DEM time step (bin, detect, contact forces, integration,
+geometry update) over every instance.
 """
 import argparse
+import ctypes as C
 import json
 import os
 import platform
@@ -21,6 +38,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-steps/s plain FP64 vs batched mixed; p50/p99 sweep-request latency"
 HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def host_info():
@@ -41,142 +62,224 @@ def exact_duration(k, dt):
     return (k - 0.5) * dt
 
 
-def cpu_baseline(model, dt, steps, threads):
-    """Time the CPU oracle (C++/OpenMP restatement of the reference step) on a
-    bounded sample of the same workload: one untimed step, then `steps` timed."""
-    from oracle import orc
-    from parmec_amd import pmx
-    o = orc.Oracle(threads)
-    m = model.copy()
-    o.upload(m)
+def kernel_table(pmx, eng):
+    arr = (pmx.pmx_kernel_time * 32)()
+    cnt = C.c_int(0)
+    eng.lib.pmx_get_profile.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int)]
+    eng.lib.pmx_get_profile(eng.h, C.cast(arr, C.c_void_p), 32, C.byref(cnt))
+    out = {}
+    for k in range(cnt.value):
+        e = arr[k]
+        if e.launches:
+            out[e.name.decode()] = {"launches": int(e.launches), "ms": float(e.ms), "bytes": float(e.bytes)}
+    return out
+
+
+def roofline(kernels):
+    """The step kernel with the largest total time (kernels with algorithmic
+    bytes; the tree rebuild is reported in kernels_ms_per_launch)."""
+    dom = max((k for k in kernels if kernels[k]["bytes"] > 0), key=lambda k: kernels[k]["ms"])
+    d = kernels[dom]
+    avg_s = d["ms"] / d["launches"] * 1e-3
+    per_launch = d["bytes"] / d["launches"]
+    achieved = per_launch / avg_s / 1e9
+    return {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "avg_launch_us": avg_s * 1e6,
+            "algorithmic_bytes_per_launch": per_launch, "sampled_launches": d["launches"]}
+
+
+def sweep_forcings(pmx, workloads, insts, nsteps, dt, t0):
+    return [pmx.forcing(nsteps, gtab=workloads.sweep_forcing(a, f, nsteps, dt, t0)) for (_, a, f) in insts]
+
+
+def plain_fp64_sample(pmx, workloads, insts, sample, warmup, steps, dt):
+    """The drop-in FP64 path, one instance at a time, over the same window."""
+    tot_ps, tot_s, ids = 0, 0.0, []
+    for i in sample:
+        m, a, f = insts[i]
+        m = m.copy()
+        e = pmx.Engine()
+        e.upload(m)
+        e.dem(exact_duration(warmup, dt), dt, pmx.forcing(warmup, gtab=workloads.sweep_forcing(a, f, warmup, dt)))
+        fs = pmx.forcing(steps, gtab=workloads.sweep_forcing(a, f, steps, dt, warmup * dt))
+        import torch
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e.dem(exact_duration(steps, dt), dt, fs)
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+        st = e.stats()
+        e.close()
+        tot_ps += m.parnum * st["steps"]
+        tot_s += sec
+        ids.append(i)
+    return tot_ps / tot_s, ids
+
+
+def hopper_leg(pmx, workloads, n, steps, dt):
+    import torch
+    model = workloads.hopper(n=n, seed=1)
+    out = {"spheres": model.parnum, "triangles": model.trinum, "dt": dt, "steps": steps}
     f = pmx.forcing(gravity=(0.0, 0.0, -9.81))
-    o.dem(exact_duration(1, dt), dt, f)
-    t0 = time.perf_counter()
-    o.dem(exact_duration(steps, dt), dt, f)
-    sec = time.perf_counter() - t0
-    st = o.stats()
-    o.close()
-    return model.parnum * st["steps"] / sec, st["steps"], sec
+    for prec, name in ((pmx.PMX_FP64, "fp64"), (pmx.PMX_MIXED, "mixed")):
+        e = pmx.Engine(precision=prec)
+        m = model.copy()
+        e.upload(m)
+        e.dem(exact_duration(20, dt), dt, f)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e.dem(exact_duration(steps, dt), dt, f)
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+        out[name] = model.parnum * steps / sec
+        e.close()
+        log(f"hopper {name}: {out[name]:.3e} particle-steps/s")
+    return out
+
+
+def cpu_leg(pmx, models, forcings_fn, ids, steps, threads):
+    """Oracle on a bounded sample: the sample instances' current states,
+    stepped on the host's cores."""
+    from oracle import orc
+    tot_ps, tot_s = 0, 0.0
+    for i in ids:
+        o = orc.Oracle(threads)
+        m = models[i].copy()
+        o.upload(m)
+        t0 = time.perf_counter()
+        o.dem(exact_duration(steps, 1e-5), 1e-5, forcings_fn(i, steps))
+        tot_s += time.perf_counter() - t0
+        tot_ps += m.parnum * o.stats()["steps"]
+        o.close()
+    return tot_ps / tot_s, tot_ps, tot_s
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--particles", type=int, default=2000000)
-    ap.add_argument("--triangles", type=int, default=5000)
-    ap.add_argument("--dt", type=float, default=2e-6)
-    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=1000, help="timed steps of the sweep window")
+    ap.add_argument("--warmup", type=int, default=3000, help="sweep steps before the window (contacts form)")
+    ap.add_argument("--instances", type=int, default=512)
+    ap.add_argument("--nmin", type=int, default=1000)
+    ap.add_argument("--nmax", type=int, default=64000)
+    ap.add_argument("--dt", type=float, default=1e-5)
+    ap.add_argument("--precision", choices=("mixed", "fp64"), default="mixed")
+    ap.add_argument("--plain-sample", type=int, default=8, help="instances timed on the plain FP64 path")
+    ap.add_argument("--hopper", type=int, default=2000000, help="hopper spheres (0: skip)")
+    ap.add_argument("--hopper-steps", type=int, default=200)
+    ap.add_argument("--cpu-instances", type=int, default=16)
+    ap.add_argument("--cpu-steps", type=int, default=200)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--profile-only", action="store_true", help="skip the CPU leg and JSON extras")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
-              "(multi-GPU sharding is design-only)", file=sys.stderr)
+              "(the multi-GPU sharding of the path is a design, not built)", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import torch
     from parmec_amd import pmx, workloads
 
+    dt, W, K = args.dt, args.warmup, args.steps
     t_setup = time.perf_counter()
-    model = workloads.hopper(n=args.particles, ntri=args.triangles, seed=1)
-    print(f"[bench] workload built: {model.parnum} spheres, {model.trinum} triangles "
-          f"({time.perf_counter() - t_setup:.1f} s)", file=sys.stderr, flush=True)
+    insts = [workloads.sweep_instance(i, args.nmin, args.nmax) for i in range(args.instances)]
+    models = [x[0] for x in insts]
+    ntot = sum(m.parnum for m in models)
+    log(f"sweep built: {len(models)} instances, {ntot} spheres ({time.perf_counter() - t_setup:.1f} s)")
 
-    eng = pmx.Engine()
-    m = model.copy()
-    eng.upload(m)
-    f = pmx.forcing(gravity=(0.0, 0.0, -9.81))
-    eng.dem(exact_duration(args.warmup, args.dt), args.dt, f)
-    print(f"[bench] warmup {args.warmup} steps done", file=sys.stderr, flush=True)
-
-    lib = eng.lib
-    lib.pmx_set_profiling(eng.h, 1)
+    prec = pmx.PMX_MIXED if args.precision == "mixed" else pmx.PMX_FP64
+    eng = pmx.Engine(precision=prec)
+    eng.upload_batch(models)
+    if W > 0:
+        done = 0
+        while done < W:
+            n = min(1000, W - done)
+            eng.dem_batch(exact_duration(n, dt), dt, sweep_forcings(pmx, workloads, insts, n, dt, done * dt))
+            done += n
+            log(f"warmup {done}/{W} steps")
+    fs = sweep_forcings(pmx, workloads, insts, K, dt, W * dt)
+    eng.lib.pmx_set_profiling(eng.h, 1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    eng.dem(exact_duration(args.steps, args.dt), args.dt, f)
+    eng.dem_batch(exact_duration(K, dt), dt, fs)
     torch.cuda.synchronize()
     sec = time.perf_counter() - t0
     st = eng.stats()
-    assert st["steps"] == args.steps, (st, args.steps)
-    print(f"[bench] timed {args.steps} steps in {sec:.3f} s", file=sys.stderr, flush=True)
+    assert st["steps"] == K, (st, K)
+    kernels = kernel_table(pmx, eng)
+    eng.lib.pmx_set_profiling(eng.h, 0)
+    value = ntot * K / sec
+    log(f"timed {K} steps in {sec:.3f} s: {value:.3e} particle-steps/s, pairs {st['pairs']}, rebuilds {st['rebuilds']}")
 
-    import ctypes as C
-    cnt = C.c_int(0)
-    arr = (pmx.pmx_kernel_time * 16)() if hasattr(pmx, "pmx_kernel_time") else None
-    lib.pmx_get_profile.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int)]
-    lib.pmx_get_profile(eng.h, C.cast(arr, C.c_void_p), 16, C.byref(cnt))
-    kernels = {}
-    for k in range(cnt.value):
-        e = arr[k]
-        if e.launches:
-            kernels[e.name.decode()] = {"launches": e.launches, "ms": e.ms, "bytes": e.bytes}
-    dom = max(kernels, key=lambda k: kernels[k]["ms"])
-    d = kernels[dom]
-    avg_s = d["ms"] / d["launches"] * 1e-3
-    bytes_per_launch = d["bytes"] / d["launches"]
-    achieved = bytes_per_launch / avg_s / 1e9
-    contacts = eng.contacts()
-    pairs_per_particle = 2.0 * len(contacts) / model.parnum
+    state = None
+    if not args.no_cpu:
+        state = [m.copy() for m in models]
+        eng.download_batch(state)
+    eng.close()
 
-    value = model.parnum * args.steps / sec
+    stride = max(1, args.instances // max(1, args.plain_sample))
+    sample = list(range(0, args.instances, stride))[: args.plain_sample]
+    plain, plain_ids = plain_fp64_sample(pmx, workloads, insts, sample, W, min(K, 1000), dt)
+    log(f"plain fp64 (one instance at a time): {plain:.3e} particle-steps/s")
+
+    hopper = hopper_leg(pmx, workloads, args.hopper, args.hopper_steps, 2e-6) if args.hopper > 0 else None
+
+    cpu = None
+    if not args.no_cpu:
+        model_name, ncpu = host_info()
+        threads = args.cpu_threads or min(16, ncpu or 1)
+        ids = sample[: max(1, args.cpu_instances)] if args.cpu_instances <= len(sample) else \
+            list(range(0, args.instances, max(1, args.instances // args.cpu_instances)))[: args.cpu_instances]
+        t_at = (W + K) * dt
+
+        def ffn(i, n):
+            _, a, f = insts[i]
+            return pmx.forcing(n, gtab=workloads.sweep_forcing(a, f, n, dt, t_at))
+        rate, ps, s = cpu_leg(pmx, state, ffn, ids, args.cpu_steps, threads)
+        cpu = {"value": rate, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+               "sample": f"{len(ids)} sweep instances ({ps // args.cpu_steps} spheres) at their state after step "
+                         f"{W + K}, {args.cpu_steps} steps each, C++/OpenMP restatement of the reference step "
+                         f"({s:.1f} s of CPU work) on {model_name}"}
+        log(f"cpu baseline: {rate:.3e} particle-steps/s on {threads} threads")
+
+    pairs_per_particle = 2.0 * st["pairs"] / 2.0 / ntot   # entries hold sphere pairs twice
     out = {
         "metric": METRIC,
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * sec / args.steps,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": 1e3 * sec / K,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32 contact law, f64 state" if prec == pmx.PMX_MIXED else "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"hopper: {model.parnum} polydisperse spheres (lognormal r, median 2 mm, sigma 0.25, "
-                        f"clipped 1-4 mm) poured into a 60-degree conical hopper of {model.trinum} static "
-                        f"triangles, granular kn=1e6 zeta=0.1 mu=0.3, dt={args.dt}, plain FP64",
-            "particles": model.parnum,
-            "triangles": model.trinum,
-            "dt": args.dt,
-            "precision": "fp64",
+            "workload": f"sweep: {len(models)} instances of the box set-up (r = 5 mm spheres on a jittered "
+                        f"lattice in a 12-triangle box, kn = 1e6, zeta = 0.1, mu = 0.3), {args.nmin}-{args.nmax} "
+                        f"spheres each (seeded), base excitation 0.1-2 g at 1-20 Hz, dt = {dt}, batched, "
+                        f"{args.precision}; timed steps {W}-{W + K} of the 10,000-step run",
+            "instances": len(models),
+            "spheres": ntot,
+            "dt": dt,
+            "precision": args.precision,
+            "window": [W, W + K],
             "contact_pairs_per_particle": pairs_per_particle,
             "tree_rebuilds": st["rebuilds"],
             "leaves": st["leaves"],
         },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": dom,
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "avg_launch_us": avg_s * 1e6,
-            "algorithmic_bytes_per_launch": bytes_per_launch,
-        },
-        "kernels_ms_per_step": {k: v["ms"] / args.steps for k, v in kernels.items()},
+        "plain_fp64": {"value": plain, "unit": "particle-steps/s", "instances_sampled": plain_ids,
+                       "how": "drop-in FP64 path, each sampled instance alone (upload, warmup, timed window)"},
+        "speedup_vs_plain_fp64": value / plain,
+        "hopper": hopper,
+        "roofline": roofline(kernels) if kernels else None,
+        "kernels_ms_per_launch": {k: v["ms"] / v["launches"] for k, v in kernels.items()},
+        "kernels_ms_per_step": {k: v["ms"] / K for k, v in kernels.items()},
+        "cpu_baseline": cpu,
     }
-    eng.close()
-
-    if not args.no_cpu and not args.profile_only:
-        cpu_model, ncpu = host_info()
-        threads = args.cpu_threads or min(ncpu or 1, 16)
-        print(f"[bench] CPU oracle sample: {args.cpu_steps} steps on {threads} threads", file=sys.stderr, flush=True)
-        v, steps, csec = cpu_baseline(model, args.dt, args.cpu_steps, threads)
-        out["cpu_baseline"] = {
-            "value": v,
-            "unit": "particle-steps/s",
-            "cores": threads,
-            "kind": "port",
-            "sample": f"{steps} steps of the same {model.parnum}-sphere hopper after one untimed step "
-                      f"({csec:.1f} s), C++/OpenMP restatement of the reference step on {cpu_model}",
-        }
     print(json.dumps(out), flush=True)
 
 

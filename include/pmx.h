@@ -224,13 +224,29 @@ int pmx_download_batch (pmx_ctx *ctx, pmx_model *models, int count);
 /* Contact pairs of one instance, numbered as in that instance's model. */
 int pmx_contacts_instance (pmx_ctx *ctx, int instance, pmx_pair *out, size_t cap, size_t *count);
 
+/* One instance's state into its own view. */
+int pmx_download_instance (pmx_ctx *ctx, int instance, pmx_model *model);
+
+/* dem() with one duration per instance: every instance takes its own step
+ * count (the reference's accumulation for its duration) and is frozen once
+ * done; done(user, i) is called from inside the call as soon as instance i
+ * has finished, when pmx_download_instance and pmx_contacts_instance already
+ * give its final state.  Afterwards the instances are at different times:
+ * further dem calls on the context are refused.  No reference counterpart
+ * (used by the request loop, pmx_server_*). */
+typedef void (*pmx_done_fn) (void *user, int instance);
+int pmx_dem_batch_durations (pmx_ctx *ctx, const double *durations, double step, const pmx_forcing *forcing,
+                             pmx_done_fn done, void *user, double *elapsed);
+
 /* Canonical contact pair set, sorted by (master, slave). */
 int pmx_contacts (pmx_ctx *ctx, pmx_pair *out, size_t cap, size_t *count);
 
 int pmx_get_stats (pmx_ctx *ctx, pmx_stats *stats);
 
-/* Enable (1) or disable (0) per-kernel event timing; enabling clears totals.
- * Timing stands in for the reference's dem() wall clock (timer.h:36-49). */
+/* Enable (1) or disable (0) per-kernel event timing; any call clears the
+ * totals.  While enabled every step runs eagerly (not as a graph replay) with
+ * a HIP event pair around each kernel on the context's stream.  Timing stands
+ * in for the reference's dem() wall clock (timer.h:36-49). */
 int pmx_set_profiling (pmx_ctx *ctx, int on);
 
 /* Per-kernel totals since profiling was enabled; count receives the number

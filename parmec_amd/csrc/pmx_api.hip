@@ -161,6 +161,9 @@ int upload_core (pmx_ctx *c, const pmx_model *m, const std::vector<InstOff> &off
   (void) hipSetDevice (c->device);
   bool same = c->uploaded && c->N == m->parnum && c->E == m->ellnum && c->T == m->trinum && c->I == I;
   for (int i = 0; same && i <= I; i++) same = c->off[i].ell == off[i].ell && c->off[i].tri == off[i].tri;
+  /* contact history is kept across uploads only while every instance's rows
+   * sit in the same row buffer */
+  for (int i = 0; same && i < I; i++) same = (int) c->icur.size () == I && c->icur[i] == c->cur;
   c->I = I;
   c->off = off;
   c->N = m->parnum; c->E = m->ellnum; c->T = m->trinum; c->P = m->pairnum;
@@ -221,6 +224,16 @@ int upload_core (pmx_ctx *c, const pmx_model *m, const std::vector<InstOff> &off
       (rc = springs_setup (*c, m)))
     return rc;
   c->curtime = m->curtime; c->curstep = m->curstep; c->stepnum = m->stepnum;
+  c->icurtime.assign (I, m->curtime);
+  c->icurstep.assign (I, m->curstep);
+  c->istepnum.assign (I, m->stepnum);
+  if (!same) c->icur.assign (I, c->cur);
+  c->staggered = false;
+  {
+    std::vector<int> oi (std::max (off[I].obs, 1), 0);
+    for (int i = 0; i < I; i++) for (int q = off[i].obs; q < off[i+1].obs; q++) oi[q] = i;
+    if ((rc = up (*c, c->obs_inst, oi.data (), (size_t) off[I].obs, "obstacles"))) return rc;
+  }
   PMX_HIP (hipStreamSynchronize (c->stream));
   c->uploaded = true;
   return PMX_OK;
@@ -309,6 +322,8 @@ void pmx_destroy (pmx_ctx *c)
   c->tree.code.release (); c->tree.code2.release ();
   c->ctl.release ();
   if (c->gexec) (void) hipGraphExecDestroy (c->gexec);
+  prof_resolve (*c);
+  for (hipEvent_t e : c->prof.pool) (void) hipEventDestroy (e);
   if (c->stream) (void) hipStreamDestroy (c->stream);
   delete c;
 }
@@ -397,21 +412,48 @@ static int capture (pmx_ctx *c, double h)
   return PMX_OK;
 }
 
-/* dem() over every instance of the context; fc holds nfc entries: one
- * forcing for all instances, or one per instance */
-static int dem_core (pmx_ctx *c, double duration, double step, const pmx_forcing *fc, int nfc, double *elapsed)
+} /* extern "C" */
+
+/* the instance has completed its steps of the call: its time bookkeeping
+ * (parmec.cpp:2418, 2521-2523) and the row buffer holding its rows */
+static void finish_instance (pmx_ctx *c, int i, int nst, double step0, double step1)
+{
+  for (int k = 0; k < nst; k++) c->icurtime[i] += 0.5*(step0+step1);
+  c->icurstep[i] = step1;
+  c->istepnum[i]++;
+  c->icur[i] = (c->cur0 + nst) & 1;
+}
+
+/* dem() over every instance of the context.  durations: one per instance
+ * (instances then run different step counts; each is frozen once its own
+ * steps are done), or null for `duration` everywhere.  fc holds nfc
+ * entries: one forcing for all instances, or one per instance.  done(user,
+ * i) is called from inside the call as soon as instance i has finished. */
+int dem_core (pmx_ctx *c, const double *durations, double duration, double step, const pmx_forcing *fc, int nfc,
+              void (*done_fn) (void*, int), void *user, double *elapsed)
 {
   if (!c) return fail (PMX_EINVAL, "null ctx");
   if (!c->uploaded) return fail (PMX_EINVAL, "pmx_upload must precede pmx_dem");
-  if (!(duration > 0.0) || !(step > 0.0)) return fail (PMX_EINVAL, "duration and step must be positive");
+  if (!(step > 0.0)) return fail (PMX_EINVAL, "step must be positive");
+  int I = c->I;
+  for (int i = 0; i < I; i++)
+  {
+    double d = durations ? durations[i] : duration;
+    if (!(d > 0.0)) return fail (PMX_EINVAL, "durations must be positive");
+    if (c->icurtime[i] != c->icurtime[0] || c->icurstep[i] != c->icurstep[0])
+      return fail (PMX_EINVAL, "instances are at different times after a call with different durations; "
+                               "download them and upload a new batch");
+  }
   (void) hipSetDevice (c->device);
   auto t0 = std::chrono::steady_clock::now ();
   int rc = 0;
   double step0, step1;
+  const double curtime = c->icurtime[0];
   int ctl0[CTL_COUNT] = {0};
   PMX_HIP (hipMemcpyAsync (c->ctl.p, ctl0, sizeof (ctl0), hipMemcpyHostToDevice, c->stream));
   PMX_HIP (hipMemsetAsync (c->dflags.p, 0, FLAG_COUNT * sizeof (int), c->stream));
-  if (c->curtime == 0.0)
+  c->staggered = false;
+  if (curtime == 0.0)
   {
     step0 = step;
     if ((rc = step_euler (*c, 0.5*step0)) || (rc = step_shapes (*c)) || (rc = step_mesh_shapes (*c, false))) return rc;
@@ -419,7 +461,7 @@ static int dem_core (pmx_ctx *c, double duration, double step, const pmx_forcing
     {
       /* obstaclev (step0/2) and obstacles (step0), parmec.cpp:2403-2405 */
       std::vector<double> t0 (6 * (size_t) c->O, 0.0);
-      for (int i = 0; i < c->I; i++)
+      for (int i = 0; i < I; i++)
       {
         const pmx_forcing *f = fc ? fc + (nfc > 1 ? i : 0) : nullptr;
         int o0 = c->off[i].obs, no = c->off[i+1].obs - o0;
@@ -430,21 +472,31 @@ static int dem_core (pmx_ctx *c, double duration, double step, const pmx_forcing
         return rc;
     }
   }
-  else step0 = c->curstep;
+  else step0 = c->icurstep[0];
   step1 = step0;
   if ((rc = step_invert (*c))) return rc;
 
-  /* per-step tables of the host-evaluated inputs (input.cpp:4424-4503),
-   * laid out [step][instance][component] */
+  /* steps of each instance (the reference's accumulation, Q13) */
+  std::vector<int> nst (I, 0);
   int nsteps = 0;
-  pmx_step_times (duration, step0, c->curtime, nullptr, 0, &nsteps);
+  for (int i = 0; i < I; i++)
   {
-    size_t I = c->I, ns = std::max (nsteps, 1);
+    pmx_step_times (durations ? durations[i] : duration, step0, curtime, nullptr, 0, &nst[i]);
+    nsteps = std::max (nsteps, nst[i]);
+  }
+  for (int i = 0; i < I; i++) c->staggered = c->staggered || nst[i] != nsteps;
+  if (c->staggered && (rc = up (*c, c->istop, nst.data (), I, "instance steps"))) return rc;
+
+  /* per-step tables of the host-evaluated inputs (input.cpp:4424-4503),
+   * laid out [step][instance][component]; rows past an instance's own steps
+   * are not read */
+  {
+    size_t ns = std::max (nsteps, 1);
     std::vector<double> g (3 * ns * I, 0.0), d (6 * ns * I, 0.0);
-    for (size_t i = 0; i < I; i++)
+    for (int i = 0; i < I; i++)
     {
       const pmx_forcing *f = fc ? fc + (nfc > 1 ? i : 0) : nullptr;
-      for (int k = 0; k < nsteps; k++)
+      for (int k = 0; k < nst[i]; k++)
       {
         for (int q = 0; q < 3; q++) g[3*((size_t) k*I + i) + q] = f && f->gtab ? f->gtab[3*(size_t) k + q] : (f ? f->gravity[q] : 0.0);
         for (int q = 0; q < 6; q++) d[6*((size_t) k*I + i) + q] = f && f->dtab ? f->dtab[6*(size_t) k + q] : (f ? f->damping[q] : 0.0);
@@ -457,12 +509,12 @@ static int dem_core (pmx_ctx *c, double duration, double step, const pmx_forcing
       /* obstaclev at curtime+step0 of every step, [step][obstacle][6] */
       size_t O = c->O;
       std::vector<double> ot (6 * ns * O, 0.0);
-      for (int i = 0; i < c->I; i++)
+      for (int i = 0; i < I; i++)
       {
         const pmx_forcing *f = fc ? fc + (nfc > 1 ? i : 0) : nullptr;
         int o0 = c->off[i].obs, no = c->off[i+1].obs - o0;
         if (!f || !f->obstab || no == 0) continue;
-        for (int k = 0; k < nsteps; k++)
+        for (int k = 0; k < nst[i]; k++)
           std::memcpy (&ot[6*((size_t) k*O + o0)], f->obstab + 6*(size_t) k*no, 6*sizeof (double)*no);
       }
       if ((rc = up (*c, c->obstab, ot.data (), ot.size (), "obstacle table"))) return rc;
@@ -479,14 +531,27 @@ static int dem_core (pmx_ctx *c, double duration, double step, const pmx_forcing
     rebuilds++;
   }
   auto tl = std::chrono::steady_clock::now ();
+  std::vector<char> reported (I, 0);
+  auto report = [&] (int upto) {
+    for (int i = 0; i < I; i++)
+      if (!reported[i] && nst[i] <= upto)
+      {
+        finish_instance (c, i, nst[i], step0, step1);
+        reported[i] = 1;
+        if (done_fn) done_fn (user, i);
+      }
+  };
   /* with profiling on, every step runs eagerly and synchronises (events per
    * launch); otherwise one step is captured as a graph and replayed in
-   * chunks, with the stream synchronised once per chunk */
+   * chunks, with the stream synchronised once per chunk (and where an
+   * instance finishes) */
   const int chunk = c->prof.on ? 1 : 64;
   int done = 0, failed_at = -1;
   while (done < nsteps)
   {
-    int n = std::min (chunk, nsteps - done);
+    int next = nsteps;
+    for (int i = 0; i < I; i++) if (!reported[i] && nst[i] > done) next = std::min (next, nst[i]);
+    int n = std::min (chunk, next - done);
     if (c->prof.on)
     {
       if ((rc = enqueue_step (*c, step0))) return rc;
@@ -501,83 +566,62 @@ static int dem_core (pmx_ctx *c, double duration, double step, const pmx_forcing
     done = ctl[CTL_KSTEP];
     if (ctl[CTL_ABORT])
     {
-      /* a leaf overflowed in step `done`: re-partition from the current
-       * centres and repeat that step (parmec.cpp:2420-2427) */
+      /* a leaf overflowed in step `done`: re-partition the overflowing
+       * instances from the current centres and repeat that step
+       * (parmec.cpp:2420-2427) */
       if (failed_at == done) return fail (PMX_EREPART, "Repartitioning failed");
       failed_at = done;
       if ((rc = rebuild (c, false))) return rc;
       rebuilds++;
     }
+    report (done);
   }
-  /* time bookkeeping exactly as the reference loop accumulates it */
-  for (int k = 0; k < nsteps; k++) c->curtime += 0.5*(step0+step1);
+  report (nsteps);
   c->cur = (c->cur0 + nsteps) & 1;
-  c->curstep = step1;
-  c->stepnum++;
+  c->curtime = c->icurtime[0]; c->curstep = c->icurstep[0]; c->stepnum = c->istepnum[0];
   double dt = std::chrono::duration<double> (std::chrono::steady_clock::now () - t0).count ();
   c->stats.steps = nsteps;
   c->stats.rebuilds = rebuilds;
   c->stats.leaves = c->tree.nleaves;
   c->stats.seconds = std::chrono::duration<double> (std::chrono::steady_clock::now () - tl).count ();
   {
-    std::vector<int> cnt (std::max (c->E, 1));
-    if (c->E > 0)
-    {
-      PMX_HIP (hipMemcpy (cnt.data (), c->rowcnt[c->cur].p, sizeof (int) * c->E, hipMemcpyDeviceToHost));
-    }
     long long total = 0;
-    for (int e = 0; e < c->E; e++) total += cnt[e];
+    for (int i = 0; i < I; i++)
+    {
+      int e0 = c->off[i].ell, ne = c->off[i+1].ell - e0;
+      if (ne == 0) continue;
+      std::vector<int> cnt (ne);
+      PMX_HIP (hipMemcpy (cnt.data (), c->rowcnt[c->icur[i]].p + e0, sizeof (int) * ne, hipMemcpyDeviceToHost));
+      for (int e = 0; e < ne; e++) total += cnt[e];
+    }
     c->stats.pairs = total;  /* entries; sphere pairs are held twice */
   }
   if (elapsed) *elapsed = dt;
   return PMX_OK;
 }
 
+extern "C" {
+
 int pmx_dem (pmx_ctx *c, double duration, double step, const pmx_forcing *fc, double *elapsed)
 {
-  return dem_core (c, duration, step, fc, 1, elapsed);
+  return dem_core (c, nullptr, duration, step, fc, 1, nullptr, nullptr, elapsed);
 }
 
 int pmx_dem_batch (pmx_ctx *c, double duration, double step, const pmx_forcing *fc, double *elapsed)
 {
   if (!c) return fail (PMX_EINVAL, "null ctx");
-  return dem_core (c, duration, step, fc, c->I, elapsed);
+  return dem_core (c, nullptr, duration, step, fc, c->I, nullptr, nullptr, elapsed);
 }
 
 int pmx_contacts (pmx_ctx *c, pmx_pair *out, size_t cap, size_t *count)
 {
   if (!c) return fail (PMX_EINVAL, "null ctx");
   (void) hipSetDevice (c->device);
-  size_t E = c->E;
   std::vector<pmx_pair> all;
-  if (E > 0)
+  for (int i = 0; i < c->I; i++)
   {
-    std::vector<uint32_t> key ((size_t) c->cap * E);
-    std::vector<double> st (3 * (size_t) c->cap * E);
-    std::vector<int> cnt (E), part (E);
-    PMX_HIP (hipMemcpy (key.data (), c->rowkey[c->cur].p, key.size () * sizeof (uint32_t), hipMemcpyDeviceToHost));
-    if (c->opt.precision == PMX_MIXED)
-    {
-      std::vector<float> s32 (st.size ());
-      PMX_HIP (hipMemcpy (s32.data (), c->rowst32[c->cur].p, s32.size () * sizeof (float), hipMemcpyDeviceToHost));
-      for (size_t q = 0; q < st.size (); q++) st[q] = s32[q];
-    }
-    else
-      PMX_HIP (hipMemcpy (st.data (), c->rowst[c->cur].p, st.size () * sizeof (double), hipMemcpyDeviceToHost));
-    PMX_HIP (hipMemcpy (cnt.data (), c->rowcnt[c->cur].p, E * sizeof (int), hipMemcpyDeviceToHost));
-    PMX_HIP (hipMemcpy (part.data (), c->part.p, E * sizeof (int), hipMemcpyDeviceToHost));
-    for (size_t e = 0; e < E; e++)
-      for (int k = 0; k < cnt[e]; k++)
-      {
-        uint32_t kk = key[(size_t) k * E + e];
-        bool tri = (kk & PMX_TRI_FLAG) != 0;
-        if (!tri && !(part[e] < part[kk])) continue;
-        pmx_pair p;
-        p.master = (int) e;
-        p.slave = tri ? -(int) (kk & ~PMX_TRI_FLAG) - 1 : (int) kk;
-        for (int d = 0; d < 3; d++) p.state[d] = st[((size_t) d * c->cap + k) * E + e];
-        all.push_back (p);
-      }
+    int rc = contacts_range (*c, i, false, all);
+    if (rc) return rc;
   }
   if (count) *count = all.size ();
   if (out) std::memcpy (out, all.data (), std::min (cap, all.size ()) * sizeof (pmx_pair));

@@ -26,16 +26,6 @@ namespace {
 
 int fail (int code, const std::string &msg) { pmx_set_error (msg); return code; }
 
-/* host copy of a concatenated model: owns the arrays the view points to */
-struct Merged
-{
-  pmx_model m {};
-  std::vector<std::vector<double>> dv;
-  std::vector<std::vector<int>> iv;
-  double *D (size_t n) { dv.emplace_back (std::max (n, (size_t) 1), 0.0); return dv.back ().data (); }
-  int *Iv (size_t n) { iv.emplace_back (std::max (n, (size_t) 1), 0); return iv.back ().data (); }
-};
-
 /* component-major k-component arrays: dst[c*total + off_i + j] = src_i[c*n_i + j] (+ shift) */
 template <class T, class Get, class Cnt, class Off, class Map>
 void cat (T *dst, int k, int count, size_t total, Get get, Cnt cnt, Off off, Map map)
@@ -65,11 +55,56 @@ void split (const T *src, int k, int count, size_t total, Get get, Cnt cnt, Off 
 
 } /* namespace */
 
-extern "C" {
-
-int pmx_upload_batch (pmx_ctx *c, const pmx_model *ms, int count)
+int contacts_range (pmx_ctx &c, int inst, bool local, std::vector<pmx_pair> &out)
 {
-  if (!c || !ms || count < 1) return fail (PMX_EINVAL, "null argument or empty batch");
+  int e0 = c.off[inst].ell, ne = c.off[inst+1].ell - e0, t0 = c.off[inst].tri;
+  if (ne == 0) return PMX_OK;
+  size_t E = c.E, cap = c.cap;
+  int par = c.icur[inst];
+  std::vector<uint32_t> key (cap * ne);
+  std::vector<double> st (3 * cap * ne);
+  std::vector<int> cnt (ne), part (ne);
+  PMX_HIP (hipMemcpy2D (key.data (), ne * sizeof (uint32_t), c.rowkey[par].p + e0, E * sizeof (uint32_t),
+                        ne * sizeof (uint32_t), cap, hipMemcpyDeviceToHost));
+  if (c.opt.precision == PMX_MIXED)
+  {
+    std::vector<float> s32 (st.size ());
+    PMX_HIP (hipMemcpy2D (s32.data (), ne * sizeof (float), c.rowst32[par].p + e0, E * sizeof (float),
+                          ne * sizeof (float), 3 * cap, hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < st.size (); q++) st[q] = s32[q];
+  }
+  else
+    PMX_HIP (hipMemcpy2D (st.data (), ne * sizeof (double), c.rowst[par].p + e0, E * sizeof (double),
+                          ne * sizeof (double), 3 * cap, hipMemcpyDeviceToHost));
+  PMX_HIP (hipMemcpy (cnt.data (), c.rowcnt[par].p + e0, ne * sizeof (int), hipMemcpyDeviceToHost));
+  PMX_HIP (hipMemcpy (part.data (), c.part.p + e0, ne * sizeof (int), hipMemcpyDeviceToHost));
+  std::vector<int> ppart;
+  for (int e = 0; e < ne; e++)
+    for (int k = 0; k < cnt[e]; k++)
+    {
+      uint32_t kk = key[(size_t) k * ne + e];
+      bool tri = (kk & PMX_TRI_FLAG) != 0;
+      if (!tri)
+      {
+        /* a sphere pair is listed once, by its master (smaller particle) */
+        int p2 = part[(int) kk - e0];
+        if (!(part[e] < p2)) continue;
+      }
+      pmx_pair p;
+      p.master = e + (local ? 0 : e0);
+      int t = (int) (kk & ~PMX_TRI_FLAG);
+      p.slave = tri ? -(t - (local ? t0 : 0)) - 1 : (int) kk - (local ? e0 : 0);
+      for (int d = 0; d < 3; d++) p.state[d] = st[((size_t) d * cap + k) * ne + e];
+      out.push_back (p);
+    }
+  return PMX_OK;
+}
+
+
+
+int merge_models (const pmx_model *ms, int count, Merged &g, std::vector<InstOff> &off)
+{
+  if (!ms || count < 1) return fail (PMX_EINVAL, "null argument or empty batch");
   for (int i = 0; i < count; i++)
   {
     int rc = validate_model (&ms[i]);
@@ -77,7 +112,7 @@ int pmx_upload_batch (pmx_ctx *c, const pmx_model *ms, int count)
     if (ms[i].curtime != ms[0].curtime || ms[i].curstep != ms[0].curstep || ms[i].stepnum != ms[0].stepnum)
       return fail (PMX_EINVAL, "instances of a batch must share curtime, curstep and stepnum");
   }
-  std::vector<InstOff> off (count + 1);
+  off.assign (count + 1, InstOff ());
   for (int i = 0; i < count; i++)
   {
     InstOff n = model_counts (ms[i]), &a = off[i], &b = off[i+1];
@@ -88,7 +123,6 @@ int pmx_upload_batch (pmx_ctx *c, const pmx_model *ms, int count)
       return fail (PMX_EINVAL, "batch too large");
   }
   const InstOff &tot = off[count];
-  Merged g;
   pmx_model &m = g.m;
   auto id = [] (auto v, int) { return v; };
   auto P = [&] (int i) { return (size_t) ms[i].parnum; };
@@ -189,7 +223,19 @@ int pmx_upload_batch (pmx_ctx *c, const pmx_model *ms, int count)
 #undef CATD
 #undef CATI
   m.curtime = ms[0].curtime; m.curstep = ms[0].curstep; m.stepnum = ms[0].stepnum;
-  return upload_core (c, &m, off);
+  return PMX_OK;
+}
+
+extern "C" {
+
+int pmx_upload_batch (pmx_ctx *c, const pmx_model *ms, int count)
+{
+  if (!c) return fail (PMX_EINVAL, "null ctx");
+  Merged g;
+  std::vector<InstOff> off;
+  int rc = merge_models (ms, count, g, off);
+  if (rc) return rc;
+  return upload_core (c, &g.m, off);
 }
 
 int pmx_download_batch (pmx_ctx *c, pmx_model *ms, int count)
@@ -245,7 +291,7 @@ int pmx_download_batch (pmx_ctx *c, pmx_model *ms, int count)
 #undef SPD
   for (int i = 0; i < count; i++)
   {
-    ms[i].curtime = m.curtime; ms[i].curstep = m.curstep; ms[i].stepnum = m.stepnum;
+    ms[i].curtime = c->icurtime[i]; ms[i].curstep = c->icurstep[i]; ms[i].stepnum = c->istepnum[i];
     size_t o0 = off[i].obs, no = ms[i].obsnum;
     if (ms[i].obspnt) std::memcpy (ms[i].obspnt, m.obspnt + 3*o0, 3*no*sizeof (double));
     if (ms[i].obsang) std::memcpy (ms[i].obsang, m.obsang + 3*o0, 3*no*sizeof (double));
@@ -259,24 +305,62 @@ int pmx_contacts_instance (pmx_ctx *c, int inst, pmx_pair *out, size_t cap, size
 {
   if (!c) return fail (PMX_EINVAL, "null ctx");
   if (inst < 0 || inst >= c->I) return fail (PMX_EINVAL, "instance out of range");
-  size_t all = 0;
-  int rc = pmx_contacts (c, nullptr, 0, &all);
+  (void) hipSetDevice (c->device);
+  std::vector<pmx_pair> v;
+  int rc = contacts_range (*c, inst, true, v);
   if (rc) return rc;
-  std::vector<pmx_pair> v (all);
-  if ((rc = pmx_contacts (c, v.data (), all, &all))) return rc;
-  int e0 = c->off[inst].ell, e1 = c->off[inst+1].ell, t0 = c->off[inst].tri;
-  size_t k = 0;
-  for (const pmx_pair &p : v)
-  {
-    if (p.master < e0 || p.master >= e1) continue;
-    pmx_pair q = p;
-    q.master -= e0;
-    q.slave = p.slave >= 0 ? p.slave - e0 : -(-p.slave - 1 - t0) - 1;
-    if (out && k < cap) out[k] = q;
-    k++;
-  }
-  if (count) *count = k;
+  if (count) *count = v.size ();
+  if (out) std::memcpy (out, v.data (), std::min (cap, v.size ()) * sizeof (pmx_pair));
   return PMX_OK;
+}
+
+/* one instance's state into its own view (its ranges of every array) */
+int pmx_download_instance (pmx_ctx *c, int inst, pmx_model *m)
+{
+  if (!c || !m) return fail (PMX_EINVAL, "null argument");
+  if (inst < 0 || inst >= c->I) return fail (PMX_EINVAL, "instance out of range");
+  const InstOff &a = c->off[inst], &b = c->off[inst+1];
+  InstOff n = model_counts (*m);
+  if (n.par != b.par - a.par || n.ell != b.ell - a.ell || n.tri != b.tri - a.tri || n.nod != b.nod - a.nod ||
+      n.spr != b.spr - a.spr || n.obs != b.obs - a.obs)
+    return fail (PMX_EINVAL, "model counts differ from the instance");
+  (void) hipSetDevice (c->device);
+  const InstOff &t = c->off[c->I];
+  /* k components of n items at offset o, out of total items per component */
+  auto dn = [&] (void *dst, const void *src, size_t elem, int k, size_t tot, size_t o, size_t cnt) -> int {
+    if (!dst || cnt == 0 || k == 0) return PMX_OK;
+    PMX_HIP (hipMemcpy2DAsync (dst, cnt * elem, (const char*) src + o * elem, tot * elem, cnt * elem, k,
+                               hipMemcpyDeviceToHost, c->stream));
+    return PMX_OK;
+  };
+  size_t np = n.par, ne = n.ell, nt = n.tri, nn = n.nod, ns = n.spr, no = n.obs;
+  int rc;
+  if ((rc = dn (m->angular, c->angular.p, 8, 6, t.par, a.par, np)) || (rc = dn (m->linear, c->linear.p, 8, 3, t.par, a.par, np)) ||
+      (rc = dn (m->rotation, c->rotation.p, 8, 9, t.par, a.par, np)) || (rc = dn (m->position, c->position.p, 8, 6, t.par, a.par, np)) ||
+      (rc = dn (m->inverse, c->inverse.p, 8, 9, t.par, a.par, np)) || (rc = dn (m->invm, c->invm.p, 8, 1, t.par, a.par, np)) ||
+      (rc = dn (m->force, c->force.p, 8, 3, t.par, a.par, np)) || (rc = dn (m->torque, c->torque.p, 8, 3, t.par, a.par, np)) ||
+      (rc = dn (m->center, c->center.p, 8, 6, t.ell, a.ell, ne)) || (rc = dn (m->tri, c->tri.p, 8, 9, t.tri, a.tri, nt)) ||
+      (rc = dn (m->nodes, c->nodes.p, 8, 6, t.nod, a.nod, nn)) ||
+      (rc = dn (m->obspnt, c->obspnt.p, 8, 1, 3*(size_t) t.obs, 3*(size_t) a.obs, 3*no)) ||
+      (rc = dn (m->obsang, c->obsang.p, 8, 1, 3*(size_t) t.obs, 3*(size_t) a.obs, 3*no)) ||
+      (rc = dn (m->obslin, c->obslin.p, 8, 1, 3*(size_t) t.obs, 3*(size_t) a.obs, 3*no)))
+    return rc;
+  if (ns > 0 &&
+      ((rc = dn (m->sprpnt, c->sp_pnt.p, 8, 12, t.spr, a.spr, ns)) || (rc = dn (m->sprdir, c->sp_dir.p, 8, 6, t.spr, a.spr, ns)) ||
+       (rc = dn (m->yield, c->sp_yield.p, 8, 2, t.spr, a.spr, ns)) || (rc = dn (m->sprsdsp, c->sp_sdsp.p, 8, 3, t.spr, a.spr, ns)) ||
+       (rc = dn (m->stroke, c->sp_stroke.p, 8, 3, t.spr, a.spr, ns)) || (rc = dn (m->sprfrc, c->sp_frc.p, 8, 3, t.spr, a.spr, ns)) ||
+       (rc = dn (m->sprflg, c->sp_flg.p, 4, 1, t.spr, a.spr, ns))))
+    return rc;
+  PMX_HIP (hipStreamSynchronize (c->stream));
+  m->curtime = c->icurtime[inst]; m->curstep = c->icurstep[inst]; m->stepnum = c->istepnum[inst];
+  return PMX_OK;
+}
+
+int pmx_dem_batch_durations (pmx_ctx *c, const double *durations, double step, const pmx_forcing *forcing,
+                             pmx_done_fn done, void *user, double *elapsed)
+{
+  if (!c || !durations) return fail (PMX_EINVAL, "null argument");
+  return dem_core (c, durations, 0.0, step, forcing, c->I, done, user, elapsed);
 }
 
 } /* extern "C" */

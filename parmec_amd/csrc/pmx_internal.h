@@ -6,6 +6,7 @@
 #define PMX_INTERNAL_H
 
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <climits>
 #include <cstdint>
 #include <string>
@@ -98,7 +99,7 @@ enum { K_BIN = 0, K_SCATTER, K_DETECT, K_FORCE_INIT, K_ROWS, K_DYNAMICS, K_SHAPE
  * and resolved at the per-step synchronisation */
 struct Prof
 {
-  bool on = false;
+  bool on = false;             /* every step runs eagerly with an event pair per kernel */
   struct Pend { int kid; hipEvent_t a, b; double bytes; };
   std::vector<hipEvent_t> pool;
   std::vector<Pend> pend;
@@ -126,7 +127,12 @@ struct pmx_ctx
   dvec<int> par_inst, ell_inst, tri_inst, ell_off, pair_off;
   dvec<double> bcen;           /* [3][E] centres the current tree was built from */
   dvec<int> ovf;               /* [I] leaf overflow of the instance in the aborted step */
-  double curtime0 = 0.0;       /* per-instance time bookkeeping */
+  /* per-instance step counts of a dem call whose instances run different
+   * durations: instance i is frozen (every kernel skips its items) once
+   * ctl[KSTEP] reaches istop[i]; null pointer when all run the same count */
+  dvec<int> istop, obs_inst;
+  bool staggered = false;      /* the last dem call left instances at different times */
+  std::vector<int> icur;       /* row buffer holding each instance's rows */
   std::vector<double> icurtime, icurstep;
   std::vector<int> istepnum;
 
@@ -195,6 +201,12 @@ struct pmx_ctx
   Prof prof;
 };
 
+/* the item's instance has completed its steps of the current dem call */
+__device__ __forceinline__ bool frozen (const int *stop, int inst, const int *ctl)
+{
+  return stop && ctl[CTL_KSTEP] >= stop[inst];
+}
+
 /* bracket a launch with events when profiling; bytes = algorithmic HBM bytes */
 struct ProfScope
 {
@@ -222,6 +234,25 @@ void prof_resolve (pmx_ctx &c);
 int validate_model (const pmx_model *m);
 InstOff model_counts (const pmx_model &m);
 int upload_core (pmx_ctx *c, const pmx_model *m, const std::vector<InstOff> &off);
+
+int dem_core (pmx_ctx *c, const double *durations, double duration, double step, const pmx_forcing *fc, int nfc,
+              void (*done_fn) (void*, int), void *user, double *elapsed);
+
+/* ---- instances (pmx_batch.hip) ---- */
+/* host copy of a (possibly concatenated) model: owns the arrays the view points to */
+struct Merged
+{
+  pmx_model m {};
+  std::vector<std::vector<double>> dv;
+  std::vector<std::vector<int>> iv;
+  double *D (size_t n) { dv.emplace_back (std::max (n, (size_t) 1), 0.0); return dv.back ().data (); }
+  int *Iv (size_t n) { iv.emplace_back (std::max (n, (size_t) 1), 0); return iv.back ().data (); }
+};
+/* concatenate models (validated one by one, index fields shifted) */
+int merge_models (const pmx_model *ms, int count, Merged &g, std::vector<InstOff> &off);
+/* append instance i's contact pairs (global numbering, or the instance's own
+ * when local) from the row buffer holding its rows */
+int contacts_range (pmx_ctx &c, int inst, bool local, std::vector<pmx_pair> &out);
 
 /* ---- tree (pmx_tree.hip) ---- */
 int tree_build (pmx_ctx &c, bool all);       /* all: snapshot every instance's centres; else only

@@ -143,13 +143,14 @@ struct SprArgs
   double *out;   /* [9][S]: force, torque on part1, torque on part2 */
   int *flags;
   const int *ctl;
+  const int *stop, *par_inst;   /* frozen instances (staggered dem calls) */
 };
 
 __global__ void k_springs (SprArgs A, double step)
 {
   if (A.ctl[CTL_ABORT]) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= A.S) return;
+  if (i >= A.S || frozen (A.stop, A.par_inst[A.part[i]], A.ctl)) return;
   size_t S = A.S, N = A.N;
   auto P = [&] (int p, int c) -> double & { return A.pnt[(size_t) (6*p+c) * S + i]; };
   int j = A.part[i], k = A.part[S + i];
@@ -374,11 +375,12 @@ __global__ void k_springs (SprArgs A, double step)
 
 /* per particle: its run flushes in the reference's order */
 __global__ void k_spring_acc (int N, int S, const int *evoff, const int *evrun, const int *runoff, const int *runspr,
-                              const double *out, double *force, double *torque, const int *ctl)
+                              const double *out, double *force, double *torque, const int *ctl, const int *stop,
+                              const int *par_inst)
 {
   if (ctl[CTL_ABORT]) return;
   int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= N) return;
+  if (b >= N || frozen (stop, par_inst[b], ctl)) return;
   int e0 = evoff[b], e1 = evoff[b+1];
   if (e0 == e1) return;
   double f[3] = {force[b], force[(size_t) N + b], force[2*(size_t) N + b]};
@@ -415,12 +417,13 @@ __global__ void k_spring_acc (int N, int S, const int *evoff, const int *evrun, 
 
 /* nodes_task, shapes.ispc:101-186 */
 __global__ void k_nodes (int NOD, int N, const int *nodpart, double *nodes, const double *rotation, const double *position,
-                         const int *ctl)
+                         const int *ctl, const int *stop, const int *par_inst)
 {
   if (ctl && ctl[CTL_ABORT]) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= NOD) return;
   int j = nodpart[i];
+  if (ctl && frozen (stop, par_inst[j], ctl)) return;
   double L[9], C[3];
 #pragma unroll
   for (int k = 0; k < 9; k++) L[k] = rotation[(size_t) k*N + j];
@@ -433,12 +436,13 @@ __global__ void k_nodes (int NOD, int N, const int *nodpart, double *nodes, cons
 
 /* faces_task, shapes.ispc:189-211 */
 __global__ void k_faces (int FAC, int NOD, int T, const int *facnod, const int *factri, const double *nodes, double *tri,
-                         const int *ctl)
+                         const int *ctl, const int *stop, const int *tri_inst)
 {
   if (ctl && ctl[CTL_ABORT]) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= FAC) return;
   int t = factri[i];
+  if (ctl && frozen (stop, tri_inst[t], ctl)) return;
 #pragma unroll
   for (int v = 0; v < 3; v++)
   {
@@ -558,6 +562,7 @@ int step_springs (pmx_ctx &c, double h)
   A.ux = c.sp_unload.p; A.uy = c.sp_unload.p + c.uncrv;
   A.angular = c.angular.p; A.linear = c.linear.p; A.rotation = c.rotation.p; A.position = c.position.p;
   A.inverse = c.inverse.p; A.invm = c.invm.p; A.out = c.sp_out.p; A.flags = c.dflags.p; A.ctl = c.ctl.p;
+  A.stop = c.staggered ? c.istop.p : nullptr; A.par_inst = c.par_inst.p;
   {
     ProfScope ps (c, K_SPRINGS, 420.0 * c.S);
     hipLaunchKernelGGL (k_springs, grid (c.S), dim3 (PMX_TPB), 0, c.stream, A, h);
@@ -567,7 +572,8 @@ int step_springs (pmx_ctx &c, double h)
     ProfScope ps (c, K_SPRING_ACC, 96.0 * c.N + 2 * 72.0 * c.S);
     hipLaunchKernelGGL (k_spring_acc, grid (c.N), dim3 (PMX_TPB), 0, c.stream, c.N, c.S, (const int*) c.sp_evoff.p,
                         (const int*) c.sp_evrun.p, (const int*) c.sp_runoff.p, (const int*) c.sp_runspr.p,
-                        (const double*) c.sp_out.p, c.force.p, c.torque.p, (const int*) c.ctl.p);
+                        (const double*) c.sp_out.p, c.force.p, c.torque.p, (const int*) c.ctl.p,
+                        c.staggered ? (const int*) c.istop.p : (const int*) nullptr, (const int*) c.par_inst.p);
   }
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
@@ -578,10 +584,12 @@ int step_mesh_shapes (pmx_ctx &c, bool gated)
   if (c.NOD == 0) return PMX_OK;
   const int *ctl = gated ? (const int*) c.ctl.p : nullptr;
   hipLaunchKernelGGL (k_nodes, grid (c.NOD), dim3 (PMX_TPB), 0, c.stream, c.NOD, c.N, (const int*) c.nodpart.p,
-                      c.nodes.p, (const double*) c.rotation.p, (const double*) c.position.p, ctl);
+                      c.nodes.p, (const double*) c.rotation.p, (const double*) c.position.p, ctl,
+                      c.staggered ? (const int*) c.istop.p : (const int*) nullptr, (const int*) c.par_inst.p);
   if (c.FAC > 0)
     hipLaunchKernelGGL (k_faces, grid (c.FAC), dim3 (PMX_TPB), 0, c.stream, c.FAC, c.NOD, c.T, (const int*) c.facnod.p,
-                        (const int*) c.factri.p, (const double*) c.nodes.p, c.tri.p, ctl);
+                        (const int*) c.factri.p, (const double*) c.nodes.p, c.tri.p, ctl,
+                        c.staggered ? (const int*) c.istop.p : (const int*) nullptr, (const int*) c.tri_inst.p);
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
 }

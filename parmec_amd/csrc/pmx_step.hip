@@ -59,12 +59,12 @@ __global__ void k_step_end (int *ctl)
 /* partitioning_store: descend to the leaf, take a slot (partition.ispc:600-646) */
 __global__ void k_bin (int E, const double *center, const int *roots, const int *ell_inst, const int *left,
                        const int *right, const int *dim, const double *coord, const double *cell, int nleaves, int *ctl,
-                       int *leafof, int *slot, int *lcount, int *flags, int *ovf)
+                       int *leafof, int *slot, int *lcount, int *flags, int *ovf, const int *stop)
 {
   if (ctl[CTL_ABORT]) return;
   int hint = ctl[CTL_HINT];
   int e = blockIdx.x * blockDim.x + threadIdx.x;
-  bool valid = e < E;
+  bool valid = e < E && !frozen (stop, ell_inst[e < E ? e : 0], ctl);
   int L = -1;
   if (valid)
   {
@@ -119,12 +119,12 @@ __global__ void k_bin (int E, const double *center, const int *roots, const int 
 }
 
 __global__ void k_scatter (int *ctl, int E, const double *center, const double *radii, const int *leafof, const int *slot,
-                           const int *loff, int *lmem, double *lpos)
+                           const int *loff, int *lmem, double *lpos, const int *stop, const int *ell_inst)
 {
   if (ctl[CTL_ABORT]) return;
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e == 0) ctl[CTL_HINT] = 1;   /* the bin of this step completed: leafof is valid */
-  if (e >= E) return;
+  if (e >= E || frozen (stop, ell_inst[e], ctl)) return;
   int pos = loff[leafof[e]] + slot[e];
   lmem[pos] = e;
   double2 *rec = (double2*) (lpos + 4*(size_t) pos);
@@ -147,9 +147,11 @@ void k_detect (int E, const double *center, const double *radii, const int *part
                const int *lcount, const int *loff, const int *lmem, const double *lpos,
                const double *tri, int T, const int *tricnt, const int *trioff, const int *trilist,
                const int *mtcnt, const int *mtlist, int mcap, const int *nbrcnt, const int *nbroff, const int *nbrlist,
-               uint32_t *hits, int *hitcnt, int hcap, int *xbuf, int *xcnt, int xcap, int *flags, const int *ctl)
+               uint32_t *hits, int *hitcnt, int hcap, int *xbuf, int *xcnt, int xcap, int *flags, const int *ctl,
+               const int *stop, const int *leafinst)
 {
   if (ctl[CTL_ABORT]) return;
+  if (frozen (stop, leafinst[blockIdx.x], ctl)) return;   /* uniform over the block */
   __shared__ double srec[4*DET_TPB];   /* staged records of the candidate leaf */
   __shared__ int smem[DET_TPB];
   __shared__ double ssph[4*DET_TPB];   /* block spheres: x, y, z, rx */
@@ -319,13 +321,14 @@ void k_detect (int E, const double *center, const double *radii, const int *part
 }
 
 __global__ void k_force_init (const int *ctl, int N, const int *list, int nlist, const double *mass, const double *gtab,
-                              int ninst, const int *par_inst, double *force, double *torque)
+                              int ninst, const int *par_inst, double *force, double *torque, const int *stop)
 {
   if (ctl[CTL_ABORT]) return;
   int k = ctl[CTL_KSTEP];
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nlist) return;
   int i = list[q];
+  if (frozen (stop, par_inst[i], ctl)) return;
   double ma = mass[i];
   const double *g = gtab + 3*((size_t) k*ninst + (ninst > 1 ? par_inst[i] : 0));
   force[i] = 0.0 + ma * g[0];
@@ -354,7 +357,7 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
              uint32_t *key0, R *st0, int *cnt0, uint32_t *key1, R *st1, int *cnt1, int cur0, int cap,
              const uint32_t *hits, const int *hitcnt, int hcap, const int *xbuf, const int *xcnt, int xcap,
              const double *gtab, const int *ctl, double h, double *force, double *torque, int *flags,
-             const double *obspnt, const double *obsang, const double *obslin)
+             const double *obspnt, const double *obsang, const double *obslin, const int *stop)
 {
   extern __shared__ uint32_t sh[];
   if (ctl[CTL_ABORT]) return;
@@ -369,7 +372,7 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
   int *ncnt = odd ? cnt0 : cnt1;
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   int cnt = 0;
-  if (e < E)
+  if (e < E && !frozen (stop, ell_inst[e], ctl))
   {
   int hx = hcap + xcap;
   uint32_t *H = sh + (size_t) threadIdx.x * hx;
@@ -600,12 +603,12 @@ __global__ void k_dynamics (int N, double *angular, double *linear, double *rota
                             const double *inertia, const double *inverse, const double *mass, const double *invm,
                             const double *dtab, const int *ctl, double *force, double *torque, double step0, double step1,
                             const int *parell, int E, double *center, const signed char *iso, int ninst,
-                            const int *par_inst)
+                            const int *par_inst, const int *stop)
 {
   if (ctl[CTL_ABORT]) return;
   int kstep = ctl[CTL_KSTEP];
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
+  if (i >= N || frozen (stop, par_inst[i], ctl)) return;
   const double *damping = dtab + 6*((size_t) kstep*ninst + (ninst > 1 ? par_inst[i] : 0));
   double half = 0.5*step0;
   double step = 0.5*(step0+step1);
@@ -716,12 +719,14 @@ __global__ void k_dynamics (int N, double *angular, double *linear, double *rota
  * obstaclev call of this step set (row `row` of tab [rows][O][6], or row
  * ctl[KSTEP] when row < 0).  Canonical stride 3 for the velocities. */
 __global__ void k_obstacles (int nm, const int *mtri, const int *mtobs, int T, double *tri, const double *obspnt,
-                             const double *tab, int O, int row, double h, const int *ctl)
+                             const double *tab, int O, int row, double h, const int *ctl, const int *stop,
+                             const int *tri_inst)
 {
   if (ctl[CTL_ABORT]) return;
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nm) return;
   int t = mtri[q], i = mtobs[q];
+  if (frozen (stop, tri_inst[t], ctl)) return;
   const double *r = tab + 6*((size_t) (row >= 0 ? row : ctl[CTL_KSTEP]) * O + i);
   double x[3] = {obspnt[3*i], obspnt[3*i+1], obspnt[3*i+2]};
   double y[3], DL[9];
@@ -742,11 +747,11 @@ __global__ void k_obstacles (int nm, const int *mtri, const int *mtobs, int T, d
 /* then per obstacle: x += h*v and the velocities of the step become the
  * current ones (read by the next step's contact law) */
 __global__ void k_obspnt (int O, double *obspnt, double *obsang, double *obslin, const double *tab, int row, double h,
-                          const int *ctl)
+                          const int *ctl, const int *stop, const int *obs_inst)
 {
   if (ctl[CTL_ABORT]) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= O) return;
+  if (i >= O || frozen (stop, obs_inst[i], ctl)) return;
   const double *r = tab + 6*((size_t) (row >= 0 ? row : ctl[CTL_KSTEP]) * O + i);
 #pragma unroll
   for (int d = 0; d < 3; d++)
@@ -849,7 +854,7 @@ int step_bin (pmx_ctx &c)
     hipLaunchKernelGGL (k_bin, grid (c.E), dim3 (PMX_TPB), 0, c.stream, c.E, (const double*) c.center.p,
                         (const int*) t.roots.p, (const int*) c.ell_inst.p, (const int*) t.left.p, (const int*) t.right.p, (const int*) t.dim.p, (const double*) t.coord.p,
                         (const double*) t.cell.p, t.nleaves, c.ctl.p, c.leafof.p, c.slot.p, c.lcount.p, c.dflags.p,
-                        c.ovf.p);
+                        c.ovf.p, c.staggered ? (const int*) c.istop.p : (const int*) nullptr);
   }
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
@@ -867,7 +872,8 @@ int step_detect (pmx_ctx &c)
     ProfScope ps (c, K_SCATTER, 76.0 * c.E);
     hipLaunchKernelGGL (k_scatter, grid (c.E), dim3 (PMX_TPB), 0, c.stream, c.ctl.p, c.E, (const double*) c.center.p,
                         (const double*) c.radii.p, (const int*) c.leafof.p, (const int*) c.slot.p, (const int*) c.loff.p,
-                        c.lmem.p, c.lpos.p);
+                        c.lmem.p, c.lpos.p, c.staggered ? (const int*) c.istop.p : (const int*) nullptr,
+                        (const int*) c.ell_inst.p);
   }
   hipLaunchKernelGGL (k_fill_i, grid (c.E), dim3 (PMX_TPB), 0, c.stream, (const int*) c.ctl.p, c.xcnt.p, c.E, 0);
   {
@@ -879,7 +885,8 @@ int step_detect (pmx_ctx &c)
                         (const int*) t.tricnt.p, (const int*) t.trioff.p, (const int*) t.trilist.p,
                         c.nmtri > 0 ? (const int*) c.mtcnt.p : (const int*) nullptr, (const int*) c.mtlist.p, c.mcap,
                         (const int*) t.nbrcnt.p, (const int*) t.nbroff.p, (const int*) t.nbrlist.p,
-                        c.hits.p, c.hitcnt.p, c.hcap, c.xbuf.p, c.xcnt.p, c.xcap, c.dflags.p, (const int*) c.ctl.p);
+                        c.hits.p, c.hitcnt.p, c.hcap, c.xbuf.p, c.xcnt.p, c.xcap, c.dflags.p, (const int*) c.ctl.p,
+                        c.staggered ? (const int*) c.istop.p : (const int*) nullptr, (const int*) t.leafinst.p);
   }
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
@@ -892,7 +899,8 @@ int step_rows (pmx_ctx &c, double h)
     ProfScope ps (c, K_FORCE_INIT, 60.0 * c.nnosph);
     hipLaunchKernelGGL (k_force_init, grid (c.nnosph), dim3 (PMX_TPB), 0, c.stream, (const int*) c.ctl.p, c.N,
                         (const int*) c.nosph.p, c.nnosph, (const double*) c.mass.p, (const double*) c.gtab.p,
-                        c.I, (const int*) c.par_inst.p, c.force.p, c.torque.p);
+                        c.I, (const int*) c.par_inst.p, c.force.p, c.torque.p,
+                        c.staggered ? (const int*) c.istop.p : (const int*) nullptr);
   }
   if (c.E > 0)
   {
@@ -909,7 +917,8 @@ int step_rows (pmx_ctx &c, double h)
                           c.cur0, c.cap, (const uint32_t*) c.hits.p,
                           (const int*) c.hitcnt.p, c.hcap, (const int*) c.xbuf.p, (const int*) c.xcnt.p, c.xcap,
                           (const double*) c.gtab.p, (const int*) c.ctl.p, h, c.force.p, c.torque.p, c.dflags.p,
-                            (const double*) c.obspnt.p, (const double*) c.obsang.p, (const double*) c.obslin.p);
+                            (const double*) c.obspnt.p, (const double*) c.obsang.p, (const double*) c.obslin.p,
+                          c.staggered ? (const int*) c.istop.p : (const int*) nullptr);
     else
       hipLaunchKernelGGL (k_rows<double>, dim3 ((c.E + ROWS_TPB - 1) / ROWS_TPB), dim3 (ROWS_TPB), shm, c.stream, c.E, c.N,
                           (const double*) c.center.p, (const double*) c.radii.p, (const int*) c.part.p,
@@ -921,7 +930,8 @@ int step_rows (pmx_ctx &c, double h)
                           c.cur0, c.cap, (const uint32_t*) c.hits.p,
                           (const int*) c.hitcnt.p, c.hcap, (const int*) c.xbuf.p, (const int*) c.xcnt.p, c.xcap,
                           (const double*) c.gtab.p, (const int*) c.ctl.p, h, c.force.p, c.torque.p, c.dflags.p,
-                          (const double*) c.obspnt.p, (const double*) c.obsang.p, (const double*) c.obslin.p);
+                          (const double*) c.obspnt.p, (const double*) c.obsang.p, (const double*) c.obslin.p,
+                          c.staggered ? (const int*) c.istop.p : (const int*) nullptr);
   }
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
@@ -934,7 +944,7 @@ int step_dynamics (pmx_ctx &c, double step0, double step1)
                       c.position.p, (const double*) c.inertia.p, (const double*) c.inverse.p, (const double*) c.mass.p,
                       (const double*) c.invm.p, (const double*) c.dtab.p, (const int*) c.ctl.p, c.force.p, c.torque.p,
                       step0, step1, (const int*) c.parell.p, c.E, c.center.p, (const signed char*) c.iso.p, c.I,
-                      (const int*) c.par_inst.p);
+                      (const int*) c.par_inst.p, c.staggered ? (const int*) c.istop.p : (const int*) nullptr);
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
 }
@@ -954,9 +964,11 @@ int step_obstacles (pmx_ctx &c, const double *tab, int row, double h)
   if (c.nmtri > 0)
     hipLaunchKernelGGL (k_obstacles, grid (c.nmtri), dim3 (PMX_TPB), 0, c.stream, c.nmtri, (const int*) c.mtri.p,
                         (const int*) c.mtobs.p, c.T, c.tri.p, (const double*) c.obspnt.p, tab, c.O, row, h,
-                        (const int*) c.ctl.p);
+                        (const int*) c.ctl.p, c.staggered ? (const int*) c.istop.p : (const int*) nullptr,
+                        (const int*) c.tri_inst.p);
   hipLaunchKernelGGL (k_obspnt, grid (c.O), dim3 (PMX_TPB), 0, c.stream, c.O, c.obspnt.p, c.obsang.p, c.obslin.p, tab,
-                      row, h, (const int*) c.ctl.p);
+                      row, h, (const int*) c.ctl.p, c.staggered ? (const int*) c.istop.p : (const int*) nullptr,
+                      (const int*) c.obs_inst.p);
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
 }

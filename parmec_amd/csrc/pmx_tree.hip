@@ -517,12 +517,13 @@ __global__ void k_tri_reach (int T, const double *tri, const int *tmov, const in
  * lists of capacity mcap; run every step since the triangles move */
 __global__ void k_mtri_reach (int nm, const int *mtri, int T, const double *tri, const int *roots, const int *tri_inst,
                               const int *left, const int *right, const int *dim, const double *coord, int *mtcnt,
-                              int *mtlist, int mcap, int *flag, const int *ctl)
+                              int *mtlist, int mcap, int *flag, const int *ctl, const int *stop)
 {
   if (ctl[CTL_ABORT]) return;
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nm) return;
   int t = mtri[q];
+  if (frozen (stop, tri_inst[t], ctl)) return;
   int root = roots[tri_inst[t]];
   if (root == PMX_NO_ROOT) return;
   double lo[3], hi[3];
@@ -626,7 +627,8 @@ int step_mtri (pmx_ctx &c)
   hipLaunchKernelGGL (k_mtri_reach, grid (c.nmtri), dim3 (PMX_TPB), 0, c.stream, c.nmtri, (const int*) c.mtri.p, c.T,
                       (const double*) c.tri.p, (const int*) t.roots.p, (const int*) c.tri_inst.p, (const int*) t.left.p,
                       (const int*) t.right.p, (const int*) t.dim.p, (const double*) t.coord.p, c.mtcnt.p, c.mtlist.p,
-                      c.mcap, c.dflags.p + FLAG_TRILIST_OVERFLOW, (const int*) c.ctl.p);
+                      c.mcap, c.dflags.p + FLAG_TRILIST_OVERFLOW, (const int*) c.ctl.p,
+                      c.staggered ? (const int*) c.istop.p : (const int*) nullptr);
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
 }
