@@ -74,8 +74,8 @@ struct TreeDev
   dvec<double> coord;
   dvec<int> left, right;     /* child refs: >= 0 branch node, < 0 leaf -(L+1) */
   dvec<int> nleafkids;       /* per internal node: leaf children (scan -> leaf ids) */
-  dvec<double> cmin;         /* [2][3][n] minima of each node's children (split coordinates) */
-  dvec<int> upcnt;           /* [n] arrivals of the bottom-up pass */
+  dvec<double> cmin;         /* [3][n] minima of a node's terminal right child */
+  dvec<unsigned long long> cmkey;  /* [3][n] ordered keys of the minima of each node's range */
   /* leaves */
   dvec<int> leafpar;         /* [nleaves] parent branch node*2 + side (or -1 for a root leaf) */
   dvec<double> cell;         /* [6][nleaves]: lower bounds 0-2, upper bounds 3-5 */

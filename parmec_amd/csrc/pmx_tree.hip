@@ -370,13 +370,25 @@ __global__ void k_assign (int n, const uint64_t *key, const int *off, const int 
 }
 
 /* split coordinate = min of the right child's points along the split
- * dimension (mincoord, partition.ispc:297-308), for every branch node at once:
- * the minima of each node's range (all three axes) are formed bottom-up.  A
- * thread starts at each branch node with a terminal child, takes the minima
- * of its terminal children directly (at most CUTOFF points each) and climbs;
- * at every node the second of the two arrivals (acquire-release counter)
- * combines the children and goes on, the first stops.  cm[side][d][node]
- * holds the minimum of the node's left (side 0) or right (side 1) child. */
+ * dimension (mincoord, partition.ispc:297-308), for every branch node at once.
+ * Each terminal child range (at most CUTOFF points, or one point) takes its
+ * minima directly and folds them into every ancestor with 64-bit atomic
+ * minima of order-preserving keys (exact; relaxed atomics suffice since the
+ * results are read by the next kernel); then a node's split coordinate is
+ * its right child's minimum along the split dimension.  cm[d][node] holds
+ * the key of the minimum of the node's whole range. */
+__device__ __forceinline__ unsigned long long dkey (double v)
+{
+  unsigned long long b = (unsigned long long) __double_as_longlong (v);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+__device__ __forceinline__ double dval (unsigned long long k)
+{
+  unsigned long long b = (k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
+  return __longlong_as_double ((long long) b);
+}
+
 __device__ __forceinline__ void range_min (int a, int b, const int *order, const double *cx, const double *cy,
                                            const double *cz, double m[3])
 {
@@ -391,44 +403,41 @@ __device__ __forceinline__ void range_min (int a, int b, const int *order, const
   }
 }
 
-__global__ void k_coord_up (int n, const int *first, const int *size, const int *split, const int *dim,
-                            const int *parent, const int *order, const double *cx, const double *cy, const double *cz,
-                            double *cm, int *cnt, double *coord)
+/* one thread per (branch node, side) whose child is a terminal range */
+__global__ void k_minfold (int n, const int *first, const int *size, const int *split, const int *dim,
+                           const int *parent, const int *order, const double *cx, const double *cy, const double *cz,
+                           unsigned long long *cm, double *rmin)
+{
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= 2*n) return;
+  int i = q >> 1, side = q & 1;
+  if (dim[i] < 0) return;
+  int j = split[i] + side;
+  int a, b;
+  if (side == 0 && first[i] == j) { a = j; b = j+1; }
+  else if (side == 1 && first[i] + size[i] - 1 == j) { a = j; b = j+1; }
+  else if (dim[j] < 0) { a = first[j]; b = first[j] + size[j]; }
+  else return;                       /* a branch child: folded by its own ranges */
+  double m[3];
+  range_min (a, b, order, cx, cy, cz, m);
+  if (side == 1) for (int d = 0; d < 3; d++) rmin[(size_t) d*n + i] = m[d];   /* terminal right child */
+  unsigned long long k[3] = {dkey (m[0]), dkey (m[1]), dkey (m[2])};
+  for (int node = i, guard = 0; node >= 0 && guard < 4096; guard++)
+  {
+    for (int d = 0; d < 3; d++) atomicMin (&cm[(size_t) d*n + node], k[d]);
+    int p = parent[node];
+    node = p < 0 ? -1 : (p >> 1);
+  }
+}
+
+__global__ void k_coord_set (int n, const int *first, const int *size, const int *split, const int *dim,
+                             const unsigned long long *cm, const double *rmin, double *coord)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || dim[i] < 0) return;
-  int j = split[i];
-  bool lt = first[i] == j || dim[j] < 0;
-  bool rt = first[i] + size[i] - 1 == j + 1 || dim[j+1] < 0;
-  if (!lt && !rt) return;
-  double m[3];
-  if (lt)
-  {
-    if (first[i] == j) range_min (j, j+1, order, cx, cy, cz, m);
-    else range_min (first[j], first[j] + size[j], order, cx, cy, cz, m);
-    for (int d = 0; d < 3; d++) cm[(size_t) d*n + i] = m[d];
-  }
-  if (rt)
-  {
-    if (first[i] + size[i] - 1 == j + 1) range_min (j+1, j+2, order, cx, cy, cz, m);
-    else range_min (first[j+1], first[j+1] + size[j+1], order, cx, cy, cz, m);
-    for (int d = 0; d < 3; d++) cm[(size_t) (3+d)*n + i] = m[d];
-  }
-  int node = i, add = (int) lt + (int) rt;
-  for (;;)
-  {
-    int old = __hip_atomic_fetch_add (&cnt[node], add, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (old + add < 2) return;        /* the other child is not done yet */
-    double l[3], r[3];
-    for (int d = 0; d < 3; d++) { l[d] = cm[(size_t) d*n + node]; r[d] = cm[(size_t) (3+d)*n + node]; }
-    coord[node] = r[dim[node]];
-    int p = parent[node];
-    if (p < 0) return;                /* an instance root */
-    int pn = p >> 1, side = p & 1;
-    for (int d = 0; d < 3; d++) cm[(size_t) (3*side+d)*n + pn] = l[d] < r[d] ? l[d] : r[d];
-    node = pn;
-    add = 1;
-  }
+  int d = dim[i], j = split[i] + 1;
+  bool term = first[i] + size[i] - 1 == j || dim[j] < 0;
+  coord[i] = term ? rmin[(size_t) d*n + i] : dval (cm[(size_t) d*n + j]);
 }
 
 __global__ void k_cells (int nleaves, const int *leafpar, const int *parent, const int *dim, const double *coord,
@@ -675,7 +684,7 @@ int tree_build (pmx_ctx &c, bool all)
       t.order.alloc (n) || t.order2.alloc (n) || t.hist.alloc (std::max ((size_t) RADIX * nbsort + 1, 2*(size_t) n + 2)) ||
       t.first.alloc (n) || t.size.alloc (n) || t.split.alloc (n) || t.dim.alloc (n) || t.parent.alloc (n) ||
       t.coord.alloc (n) || t.left.alloc (n) || t.right.alloc (n) || t.nleafkids.alloc (n + 1) || t.roots.alloc (I) ||
-      t.cmin.alloc (6*(size_t) n) || t.upcnt.alloc (n))
+      t.cmin.alloc (3*(size_t) n) || t.cmkey.alloc (3*(size_t) n))
   { pmx_set_error ("out of device memory (tree)"); return PMX_ENOMEM; }
 
   /* extents per instance (partition.ispc:29-54, 669-684) */
@@ -723,10 +732,13 @@ int tree_build (pmx_ctx &c, bool all)
   PMX_HIP (hipGetLastError ());
 
   /* split coordinates of the branch nodes (mincoord, partition.ispc:297-308) */
-  PMX_HIP (hipMemsetAsync (t.upcnt.p, 0, sizeof (int) * n, c.stream));
-  hipLaunchKernelGGL (k_coord_up, grid (n), dim3 (PMX_TPB), 0, c.stream, n, (const int*) t.first.p,
+  PMX_HIP (hipMemsetAsync (t.cmkey.p, 0xFF, sizeof (unsigned long long) * 3 * (size_t) n, c.stream));
+  hipLaunchKernelGGL (k_minfold, grid (2*n), dim3 (PMX_TPB), 0, c.stream, n, (const int*) t.first.p,
                       (const int*) t.size.p, (const int*) t.split.p, (const int*) t.dim.p, (const int*) t.parent.p,
-                      order, cx, cy, cz, t.cmin.p, t.upcnt.p, t.coord.p);
+                      order, cx, cy, cz, t.cmkey.p, t.cmin.p);
+  hipLaunchKernelGGL (k_coord_set, grid (n), dim3 (PMX_TPB), 0, c.stream, n, (const int*) t.first.p,
+                      (const int*) t.size.p, (const int*) t.split.p, (const int*) t.dim.p,
+                      (const unsigned long long*) t.cmkey.p, (const double*) t.cmin.p, t.coord.p);
   PMX_HIP (hipGetLastError ());
 
   /* leaf numbering and reach cells */
