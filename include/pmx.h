@@ -39,7 +39,7 @@ enum { PMX_SPRING_YIELDED = 1, PMX_SPRING_UNLOADING = 2, PMX_SPRDIR_CONSTANT = 4
 
 /* error codes */
 enum { PMX_OK = 0, PMX_EINVAL = -1, PMX_ENOMEM = -2, PMX_EDEVICE = -3,
-       PMX_ECAPACITY = -4, PMX_EUNSUPPORTED = -5, PMX_EREPART = -6 };
+       PMX_ECAPACITY = -4, PMX_EUNSUPPORTED = -5, PMX_EREPART = -6, PMX_EBUSY = -7 };
 
 /* arithmetic of the contact/force stage */
 enum { PMX_FP64 = 0,     /* plain FP64 drop-in */
@@ -256,6 +256,70 @@ int pmx_get_profile (pmx_ctx *ctx, pmx_kernel_time *out, int cap, int *count);
 void pmx_destroy (pmx_ctx *ctx);
 
 const char *pmx_last_error (void);
+
+/*
+ * Resident request loop (serving).  Requests are independent simulations:
+ * a model, one dem(duration, step) call and its forcing tables.  A scheduler
+ * thread admits queued requests in arrival order while they fit the HBM
+ * budget; requests queued within gather_s of each other (up to max_batch,
+ * same step and time state) form one batch context stepped by its own worker
+ * thread and HIP stream, so batches admitted at different times overlap on
+ * the device (at most max_active batches at once).  Each request is frozen
+ * and copied back as soon as its own steps are done; its result is bitwise
+ * that of pmx_upload + pmx_dem on it alone.  No reference counterpart: the
+ * resident form of a sweep's loop of separate runs (python/acc_sweep.py).
+ */
+enum { PMX_REQ_QUEUED = 0, PMX_REQ_RUNNING = 1, PMX_REQ_DONE = 2, PMX_REQ_FAILED = 3 };
+
+typedef struct pmx_server_options
+{
+  int precision;        /* PMX_FP64 or PMX_MIXED */
+  int device;
+  int row_capacity;     /* 0: default */
+  int max_batch;        /* requests per batch context (0: 256) */
+  int max_active;       /* batch contexts stepping at once (0: 16) */
+  double gather_s;      /* how long the oldest queued request waits for others to batch with */
+  size_t hbm_budget;    /* bytes of device memory the admitted requests may use (0: 200 GiB) */
+} pmx_server_options;
+
+typedef struct pmx_request_info
+{
+  long long id;
+  int status;           /* PMX_REQ_* */
+  int batch;            /* requests in the batch it ran in */
+  long long steps;
+  long long particles;
+  long long pairs;      /* contact pairs at the end */
+  size_t bytes;         /* device memory charged to the budget */
+  double submit_s, admit_s, done_s;   /* seconds since the server started */
+} pmx_request_info;
+
+typedef struct pmx_server_stats
+{
+  long long queued, running, done, failed, batches;
+  int active_batches;
+  size_t bytes_in_use;
+  double particle_steps;   /* of the finished requests */
+  double seconds;          /* since the server started */
+} pmx_server_stats;
+
+typedef struct pmx_server pmx_server;
+
+int pmx_server_create (const pmx_server_options *opt, pmx_server **server);
+
+/* Queue a request (the model and the forcing tables are copied). */
+int pmx_server_submit (pmx_server *server, const pmx_model *model, double duration, double step,
+                       const pmx_forcing *forcing, long long *id);
+
+/* Wait for a request (timeout < 0: no limit; PMX_EBUSY when it runs out),
+ * copy its final state into out (same counts as the submitted model; NULL
+ * arrays are skipped) and release it. */
+int pmx_server_wait (pmx_server *server, long long id, double timeout, pmx_model *out, pmx_request_info *info);
+
+int pmx_server_get_stats (pmx_server *server, pmx_server_stats *stats);
+
+/* Finish the admitted and queued requests, then stop. */
+void pmx_server_destroy (pmx_server *server);
 
 #ifdef __cplusplus
 }

@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpmx.so")
-SOURCES = ["pmx_api.hip", "pmx_batch.hip", "pmx_tree.hip", "pmx_step.hip", "pmx_spring.hip"]
+SOURCES = ["pmx_api.hip", "pmx_batch.hip", "pmx_serve.hip", "pmx_tree.hip", "pmx_step.hip", "pmx_spring.hip"]
 HEADERS = ["pmx_internal.h", "pmx_device.h", os.path.join("..", "..", "include", "pmx.h")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -50,7 +50,7 @@ def build(verbose=False, jobs=None):
     if failed:
         msg = "\n".join(f"--- {s}\n{t}" for s, t in failed)
         raise RuntimeError(f"hipcc failed:\n{msg}")
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB] + objs
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", LIB] + objs
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

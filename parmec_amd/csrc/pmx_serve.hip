@@ -1,0 +1,382 @@
+/*
+ * pmx_serve.hip -- resident request loop over the batched step path.
+ *
+ * Requests are independent simulations (a model, a duration, a step and
+ * the per-step forcing tables), the unit of a parameter sweep.  A scheduler
+ * thread admits them in arrival order while they fit the HBM budget:
+ * requests queued together (within the gather window, up to max_batch) are
+ * concatenated into one batch context, and each context is stepped by its
+ * own worker thread on its own HIP stream, so batches admitted at different
+ * times overlap on the device.  Inside a batch every request runs its own
+ * step count (pmx_dem_batch_durations): it is frozen when done and its state
+ * is copied back at once, so a short request does not wait for a long one.
+ * A batch's device memory is released when its last request has finished.
+ *
+ * Each request's result is bitwise the result of running it alone with
+ * pmx_upload + pmx_dem (one dem call of the request's duration): instances
+ * of a batch never interact and keep their own partitioning trees.
+ *
+ * No reference counterpart: Parmec runs one model per process; this is the
+ * resident form of the loop of separate DEM runs a sweep script performs
+ * (python/acc_sweep.py).
+ */
+#include "pmx_internal.h"
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+
+namespace {
+
+int fail (int code, const std::string &msg) { pmx_set_error (msg); return code; }
+
+double now_s (std::chrono::steady_clock::time_point t0)
+{
+  return std::chrono::duration<double> (std::chrono::steady_clock::now () - t0).count ();
+}
+
+struct Req
+{
+  long long id = 0;
+  Merged g;                       /* owned copy of the model; the result is written here */
+  double duration = 0.0, step = 0.0;
+  int nsteps = 0;
+  std::vector<double> gtab, dtab, obstab, obstab0;
+  pmx_forcing f {};
+  size_t bytes = 0;
+  double t_submit = 0.0, t_admit = -1.0, t_done = -1.0;
+  int status = PMX_REQ_QUEUED;
+  int batch = 0;                  /* requests in its batch */
+  long long pairs = 0;
+  std::string err;
+};
+
+struct Group
+{
+  std::vector<Req*> reqs;         /* a request is the waiter's once reported: never touched again */
+  std::vector<char> reported;
+  size_t bytes = 0;
+  std::thread th;
+  bool finished = false;
+};
+
+/* device bytes of one request in a context (rows, broad phase, state) */
+size_t estimate_bytes (const pmx_model &m, int precision, int cap)
+{
+  size_t E = m.ellnum, N = m.parnum, T = m.trinum, S = m.sprnum;
+  size_t st = precision == PMX_MIXED ? 4 : 8;
+  size_t per_sphere = 2 * (size_t) cap * (4 + 3 * st) + 4 * (size_t) cap + 2 * (size_t) cap   /* rows, hits, reverse hits */
+                      + 48 + 24 + 8 + 32 + 24 + 24 + 200;                                      /* state, records, tree */
+  return E * per_sphere + N * 420 + T * 120 + S * 460 + (size_t) m.nodnum * 56 + (size_t) m.facnum * 16 +
+         ((size_t) 1 << 20);
+}
+
+void copy_outputs (const pmx_model &s, pmx_model *d)
+{
+  auto cp = [] (double *dst, const double *src, size_t n) { if (dst && src && n) std::memcpy (dst, src, n * sizeof (double)); };
+  size_t N = s.parnum, E = s.ellnum, T = s.trinum, S = s.sprnum, NOD = s.nodnum, O = s.obsnum;
+  cp (d->angular, s.angular, 6*N); cp (d->linear, s.linear, 3*N); cp (d->rotation, s.rotation, 9*N);
+  cp (d->position, s.position, 6*N); cp (d->inverse, s.inverse, 9*N); cp (d->invm, s.invm, N);
+  cp (d->force, s.force, 3*N); cp (d->torque, s.torque, 3*N); cp (d->center, s.center, 6*E);
+  cp (d->tri, s.tri, 9*T); cp (d->nodes, s.nodes, 6*NOD);
+  cp (d->obspnt, s.obspnt, 3*O); cp (d->obsang, s.obsang, 3*O); cp (d->obslin, s.obslin, 3*O);
+  cp (d->sprpnt, s.sprpnt, 12*S); cp (d->sprdir, s.sprdir, 6*S); cp (d->yield, s.yield, 2*S);
+  cp (d->sprsdsp, s.sprsdsp, 3*S); cp (d->stroke, s.stroke, 3*S); cp (d->sprfrc, s.sprfrc, 3*S);
+  if (d->sprflg && s.sprflg && S) std::memcpy (d->sprflg, s.sprflg, S * sizeof (int));
+  d->curtime = s.curtime; d->curstep = s.curstep; d->stepnum = s.stepnum;
+}
+
+} /* namespace */
+
+struct pmx_server
+{
+  pmx_server_options opt {};
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<Req*> queue;
+  std::map<long long, Req*> reqs;
+  std::vector<Group*> groups;
+  size_t used = 0;
+  long long next_id = 1;
+  bool stop = false;
+  std::thread sched;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now ();
+  long long done = 0, failed = 0, batches = 0;
+  double particle_steps = 0.0;
+};
+
+namespace {
+
+struct DoneCtx { pmx_server *s; Group *g; pmx_ctx *c; };
+
+void on_done (void *user, int i)
+{
+  DoneCtx *d = (DoneCtx*) user;
+  Req *r = d->g->reqs[i];
+  int rc = pmx_download_instance (d->c, i, &r->g.m);
+  size_t npairs = 0;
+  if (!rc) rc = pmx_contacts_instance (d->c, i, nullptr, 0, &npairs);
+  std::string err = rc ? pmx_last_error () : "";
+  {
+    std::lock_guard<std::mutex> lk (d->s->mu);
+    r->t_done = now_s (d->s->t0);
+    r->pairs = (long long) npairs;
+    r->status = rc ? PMX_REQ_FAILED : PMX_REQ_DONE;
+    r->err = err;
+    if (rc) d->s->failed++;
+    else { d->s->done++; d->s->particle_steps += (double) r->g.m.parnum * r->nsteps; }
+    d->g->reported[i] = 1;
+  }
+  d->s->cv.notify_all ();
+}
+
+void worker (pmx_server *s, Group *g)
+{
+  pmx_options o {};
+  o.precision = s->opt.precision;
+  o.device = s->opt.device;
+  o.row_capacity = s->opt.row_capacity;
+  pmx_ctx *c = nullptr;
+  int n = (int) g->reqs.size ();
+  int rc = pmx_create (&o, &c);
+  if (!rc)
+  {
+    std::vector<pmx_model> ms (n);
+    std::vector<double> dur (n);
+    std::vector<pmx_forcing> fs (n);
+    for (int i = 0; i < n; i++) { ms[i] = g->reqs[i]->g.m; dur[i] = g->reqs[i]->duration; fs[i] = g->reqs[i]->f; }
+    rc = pmx_upload_batch (c, ms.data (), n);
+    if (!rc)
+    {
+      DoneCtx d {s, g, c};
+      double el = 0.0;
+      rc = pmx_dem_batch_durations (c, dur.data (), g->reqs[0]->step, fs.data (), on_done, &d, &el);
+    }
+  }
+  std::string err = rc ? pmx_last_error () : "";
+  if (c) pmx_destroy (c);
+  {
+    std::lock_guard<std::mutex> lk (s->mu);
+    for (size_t i = 0; i < g->reqs.size (); i++)
+      if (!g->reported[i])
+      {
+        Req *r = g->reqs[i];
+        r->status = PMX_REQ_FAILED;
+        r->err = err.empty () ? "batch ended before the request finished" : err;
+        r->t_done = now_s (s->t0);
+        s->failed++;
+        g->reported[i] = 1;
+      }
+    s->used -= g->bytes;
+    g->finished = true;
+  }
+  s->cv.notify_all ();
+}
+
+void scheduler (pmx_server *s)
+{
+  std::unique_lock<std::mutex> lk (s->mu);
+  for (;;)
+  {
+    /* the lock is released while a finished worker is joined; every state
+     * change made meanwhile (stop, freed memory, new requests) is read again
+     * below with the lock held up to the next wait, so no wake-up is lost */
+    for (size_t k = 0; k < s->groups.size (); )
+    {
+      Group *g = s->groups[k];
+      if (g->finished)
+      {
+        lk.unlock ();
+        g->th.join ();
+        lk.lock ();
+        delete g;
+        s->groups.erase (s->groups.begin () + k);
+      }
+      else k++;
+    }
+    if (s->stop) break;
+    int active = (int) s->groups.size ();
+    if (!s->queue.empty () && active < s->opt.max_active)
+    {
+      /* gather: let requests arriving within the window join one batch */
+      double wait = s->opt.gather_s - (now_s (s->t0) - s->queue.front ()->t_submit);
+      if (wait > 0.0 && (int) s->queue.size () < s->opt.max_batch)
+      {
+        s->cv.wait_for (lk, std::chrono::duration<double> (wait));
+        continue;
+      }
+      std::vector<Req*> take;
+      size_t bytes = 0;
+      while (!s->queue.empty () && (int) take.size () < s->opt.max_batch)
+      {
+        Req *r = s->queue.front ();
+        if (!take.empty () && (r->step != take[0]->step || r->g.m.curtime != take[0]->g.m.curtime ||
+                               r->g.m.curstep != take[0]->g.m.curstep || r->g.m.stepnum != take[0]->g.m.stepnum))
+          break;
+        if (s->used + bytes + r->bytes > s->opt.hbm_budget) break;   /* arrival order: wait for memory */
+        take.push_back (r);
+        bytes += r->bytes;
+        s->queue.pop_front ();
+      }
+      if (take.empty ())
+      {
+        if (active == 0)
+        {
+          /* nothing runs and the head still does not fit: it never will */
+          Req *r = s->queue.front ();
+          s->queue.pop_front ();
+          r->status = PMX_REQ_FAILED;
+          r->err = "request exceeds the HBM budget";
+          r->t_done = now_s (s->t0);
+          s->failed++;
+          s->cv.notify_all ();
+          continue;
+        }
+        s->cv.wait (lk);
+        continue;
+      }
+      Group *g = new Group;
+      g->reqs = take;
+      g->reported.assign (take.size (), 0);
+      g->bytes = bytes;
+      s->used += bytes;
+      double t = now_s (s->t0);
+      for (Req *r : take) { r->status = PMX_REQ_RUNNING; r->t_admit = t; r->batch = (int) take.size (); }
+      s->batches++;
+      s->groups.push_back (g);
+      g->th = std::thread (worker, s, g);
+      continue;
+    }
+    s->cv.wait (lk);
+  }
+}
+
+} /* namespace */
+
+extern "C" {
+
+int pmx_server_create (const pmx_server_options *opt, pmx_server **out)
+{
+  if (!out) return fail (PMX_EINVAL, "null server pointer");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount (&ndev) != hipSuccess || ndev == 0) return fail (PMX_EDEVICE, "no HIP device available");
+  pmx_server *s = new pmx_server ();
+  if (opt) s->opt = *opt;
+  if (s->opt.precision != PMX_FP64 && s->opt.precision != PMX_MIXED)
+  { delete s; return fail (PMX_EINVAL, "precision must be PMX_FP64 or PMX_MIXED"); }
+  if (s->opt.hbm_budget == 0) s->opt.hbm_budget = (size_t) 200 << 30;
+  if (s->opt.max_batch <= 0) s->opt.max_batch = 256;
+  if (s->opt.max_active <= 0) s->opt.max_active = 16;
+  if (s->opt.gather_s < 0.0) s->opt.gather_s = 0.0;
+  s->sched = std::thread (scheduler, s);
+  *out = s;
+  return PMX_OK;
+}
+
+int pmx_server_submit (pmx_server *s, const pmx_model *m, double duration, double step, const pmx_forcing *f,
+                       long long *id)
+{
+  if (!s || !m) return fail (PMX_EINVAL, "null argument");
+  if (!(duration > 0.0) || !(step > 0.0)) return fail (PMX_EINVAL, "duration and step must be positive");
+  Req *r = new Req ();
+  std::vector<InstOff> off;
+  int rc = merge_models (m, 1, r->g, off);
+  if (rc) { delete r; return rc; }
+  pmx_model &c = r->g.m;
+  if (!c.force) c.force = r->g.D (3*(size_t) c.parnum);
+  if (!c.torque) c.torque = r->g.D (3*(size_t) c.parnum);
+  r->duration = duration;
+  r->step = step;
+  double h = m->curtime == 0.0 ? step : m->curstep;
+  pmx_step_times (duration, h, m->curtime, nullptr, 0, &r->nsteps);
+  size_t ns = r->nsteps, O = m->obsnum;
+  for (int d = 0; d < 3; d++) r->f.gravity[d] = f ? f->gravity[d] : 0.0;
+  for (int d = 0; d < 6; d++) r->f.damping[d] = f ? f->damping[d] : 0.0;
+  if (f && f->gtab) { r->gtab.assign (f->gtab, f->gtab + 3*ns); r->f.gtab = r->gtab.data (); }
+  if (f && f->dtab) { r->dtab.assign (f->dtab, f->dtab + 6*ns); r->f.dtab = r->dtab.data (); }
+  if (f && f->obstab && O) { r->obstab.assign (f->obstab, f->obstab + 6*ns*O); r->f.obstab = r->obstab.data (); }
+  if (f && f->obstab0 && O) { r->obstab0.assign (f->obstab0, f->obstab0 + 6*O); r->f.obstab0 = r->obstab0.data (); }
+  r->bytes = estimate_bytes (c, s->opt.precision, s->opt.row_capacity > 0 ? s->opt.row_capacity : 32);
+  {
+    std::lock_guard<std::mutex> lk (s->mu);
+    r->id = s->next_id++;
+    r->t_submit = now_s (s->t0);
+    s->reqs[r->id] = r;
+    s->queue.push_back (r);
+    if (id) *id = r->id;
+  }
+  s->cv.notify_all ();
+  return PMX_OK;
+}
+
+int pmx_server_wait (pmx_server *s, long long id, double timeout, pmx_model *out, pmx_request_info *info)
+{
+  if (!s) return fail (PMX_EINVAL, "null server");
+  std::unique_lock<std::mutex> lk (s->mu);
+  auto it = s->reqs.find (id);
+  if (it == s->reqs.end ()) return fail (PMX_EINVAL, "unknown request id");
+  Req *r = it->second;
+  auto ready = [r] { return r->status == PMX_REQ_DONE || r->status == PMX_REQ_FAILED; };
+  if (timeout < 0.0) s->cv.wait (lk, ready);
+  else if (!s->cv.wait_for (lk, std::chrono::duration<double> (timeout), ready))
+    return fail (PMX_EBUSY, "request not finished");
+  if (info)
+  {
+    info->id = r->id; info->status = r->status; info->batch = r->batch; info->steps = r->nsteps;
+    info->particles = r->g.m.parnum; info->pairs = r->pairs; info->bytes = r->bytes;
+    info->submit_s = r->t_submit; info->admit_s = r->t_admit; info->done_s = r->t_done;
+  }
+  int rc = PMX_OK;
+  if (r->status == PMX_REQ_FAILED) rc = fail (PMX_EDEVICE, "request failed: " + r->err);
+  else if (out)
+  {
+    InstOff a = model_counts (*out), b = model_counts (r->g.m);
+    if (a.par != b.par || a.ell != b.ell || a.tri != b.tri || a.nod != b.nod || a.spr != b.spr || a.obs != b.obs)
+      rc = fail (PMX_EINVAL, "result view counts differ from the request's model");
+    else copy_outputs (r->g.m, out);
+  }
+  s->reqs.erase (it);
+  delete r;
+  return rc;
+}
+
+int pmx_server_get_stats (pmx_server *s, pmx_server_stats *st)
+{
+  if (!s || !st) return fail (PMX_EINVAL, "null argument");
+  std::lock_guard<std::mutex> lk (s->mu);
+  st->queued = (long long) s->queue.size ();
+  st->running = 0;
+  for (auto &kv : s->reqs) st->running += kv.second->status == PMX_REQ_RUNNING;
+  st->done = s->done;
+  st->failed = s->failed;
+  st->batches = s->batches;
+  st->active_batches = (int) s->groups.size ();
+  st->bytes_in_use = s->used;
+  st->particle_steps = s->particle_steps;
+  st->seconds = now_s (s->t0);
+  return PMX_OK;
+}
+
+void pmx_server_destroy (pmx_server *s)
+{
+  if (!s) return;
+  {
+    std::unique_lock<std::mutex> lk (s->mu);
+    s->cv.wait (lk, [s] {
+      bool busy = !s->queue.empty ();
+      for (Group *g : s->groups) busy = busy || !g->finished;
+      return !busy; });
+    s->stop = true;
+  }
+  s->cv.notify_all ();
+  s->sched.join ();
+  for (Group *g : s->groups) { if (g->th.joinable ()) g->th.join (); delete g; }
+  for (auto &kv : s->reqs) delete kv.second;
+  delete s;
+}
+
+} /* extern "C" */

@@ -70,6 +70,27 @@ EXPORTS = ["pmx_step_times", "pmx_create", "pmx_upload", "pmx_dem", "pmx_downloa
            "pmx_get_stats", "pmx_destroy", "pmx_last_error", "pmx_set_profiling", "pmx_get_profile"]
 
 
+class pmx_server_options(C.Structure):
+    _fields_ = [("precision", C.c_int), ("device", C.c_int), ("row_capacity", C.c_int), ("max_batch", C.c_int),
+                ("max_active", C.c_int), ("gather_s", C.c_double), ("hbm_budget", C.c_size_t)]
+
+
+class pmx_request_info(C.Structure):
+    _fields_ = [("id", C.c_longlong), ("status", C.c_int), ("batch", C.c_int), ("steps", C.c_longlong),
+                ("particles", C.c_longlong), ("pairs", C.c_longlong), ("bytes", C.c_size_t), ("submit_s", C.c_double),
+                ("admit_s", C.c_double), ("done_s", C.c_double)]
+
+
+class pmx_server_stats(C.Structure):
+    _fields_ = [("queued", C.c_longlong), ("running", C.c_longlong), ("done", C.c_longlong),
+                ("failed", C.c_longlong), ("batches", C.c_longlong), ("active_batches", C.c_int),
+                ("bytes_in_use", C.c_size_t), ("particle_steps", C.c_double), ("seconds", C.c_double)]
+
+
+PMX_EBUSY = -7
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
+
+
 class PmxError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"pmx error {code}: {msg}")
@@ -106,6 +127,15 @@ def load(path=LIBPATH):
         lib.pmx_dem_batch.argtypes = [C.c_void_p, C.c_double, C.c_double, C.POINTER(pmx_forcing), _dp]
         lib.pmx_contacts_instance.argtypes = [C.c_void_p, C.c_int, C.POINTER(pmx_pair), C.c_size_t,
                                               C.POINTER(C.c_size_t)]
+        lib.pmx_download_instance.argtypes = [C.c_void_p, C.c_int, C.POINTER(pmx_model)]
+        lib.pmx_server_create.argtypes = [C.POINTER(pmx_server_options), C.POINTER(C.c_void_p)]
+        lib.pmx_server_submit.argtypes = [C.c_void_p, C.POINTER(pmx_model), C.c_double, C.c_double,
+                                          C.POINTER(pmx_forcing), C.POINTER(C.c_longlong)]
+        lib.pmx_server_wait.argtypes = [C.c_void_p, C.c_longlong, C.c_double, C.POINTER(pmx_model),
+                                        C.POINTER(pmx_request_info)]
+        lib.pmx_server_get_stats.argtypes = [C.c_void_p, C.POINTER(pmx_server_stats)]
+        lib.pmx_server_destroy.argtypes = [C.c_void_p]
+        lib.pmx_server_destroy.restype = None
         _lib = _declare(lib, "pmx_")
     return _lib
 
@@ -345,6 +375,56 @@ class Engine:
     def close(self):
         if getattr(self, "h", None):
             getattr(self.lib, self.prefix + "destroy")(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """The resident request loop (pmx_server_*): submit models with their
+    duration, step and forcing; wait for each result."""
+
+    def __init__(self, precision=PMX_FP64, device=0, row_capacity=0, max_batch=0, max_active=0, gather_s=0.0,
+                 hbm_budget=0, lib=None):
+        self.lib = lib or load()
+        opt = pmx_server_options(precision, device, row_capacity, max_batch, max_active, gather_s, int(hbm_budget))
+        h = C.c_void_p()
+        self._check(self.lib.pmx_server_create(C.byref(opt), C.byref(h)))
+        self.h = h
+
+    def _check(self, rc):
+        if rc != PMX_OK:
+            raise PmxError(rc, self.lib.pmx_last_error().decode())
+
+    def submit(self, model, duration, step, forcing_=None):
+        i = C.c_longlong(0)
+        self._check(self.lib.pmx_server_submit(self.h, C.byref(model.struct()), duration, step,
+                                               C.byref(forcing_) if forcing_ is not None else None, C.byref(i)))
+        return i.value
+
+    def wait(self, rid, model=None, timeout=-1.0):
+        """Blocks until request rid is done; fills model (a Model with the
+        submitted counts) with its final state; returns the request info."""
+        info = pmx_request_info()
+        s = model.struct() if model is not None else None
+        self._check(self.lib.pmx_server_wait(self.h, rid, timeout, C.byref(s) if s is not None else None,
+                                             C.byref(info)))
+        if model is not None:
+            model.take_time(s)
+        return {k: getattr(info, k) for k, _ in pmx_request_info._fields_}
+
+    def stats(self):
+        st = pmx_server_stats()
+        self._check(self.lib.pmx_server_get_stats(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in pmx_server_stats._fields_}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.pmx_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -329,3 +329,32 @@ def stirred_box(n=2000, seed=0, piston_speed=-0.2, stir_rate=20.0):
         row[1, 2] = stir_rate
         return np.tile(row, (nsteps, 1, 1)), row.copy()
     return m, table
+
+
+def serve_request(i, seed=0, median=20000, sigma=0.8, nmin=2000, nmax=500000, smin=500, smax=20000,
+                  spring_share=0.3, step=1e-5):
+    """Request i of the serving stream: a box set-up of spheres (70 %) or a
+    spring lattice of meshed bodies (30 %), sized lognormal (median, sigma)
+    clipped to [nmin, nmax] spheres or bodies, running smin-smax steps under
+    the sweep's base excitation (amplitude 0.1-2 g, 1-20 Hz).  Returns
+    (model, duration, step, forcing)."""
+    rng = np.random.default_rng([seed, i])
+    n = int(np.clip(rng.lognormal(math.log(median), sigma), nmin, nmax))
+    nsteps = int(rng.integers(smin, smax + 1))
+    amp = float(rng.uniform(0.1, 2.0)) * 9.81
+    freq = float(rng.uniform(1.0, 20.0))
+    if rng.uniform() < spring_share:
+        a = max(2, int(round((n / 2.0) ** (1.0 / 3.0))))
+        m = spring_lattice(nx=a, ny=a, nz=max(1, n // (a * a)), seed=int(rng.integers(1 << 30)))
+    else:
+        m = box_lattice(n=n, seed=int(rng.integers(1 << 30)))
+    g = sweep_forcing(amp, freq, nsteps, step)
+    return m, (nsteps - 0.5) * step, step, pmx.forcing(nsteps, gtab=g)
+
+
+def serve_stream(count, rate=20.0, seed=0, **kw):
+    """Poisson arrivals (rate per second) of serve_request's requests:
+    [(arrival time s, model, duration, step, forcing)]."""
+    rng = np.random.default_rng(seed + 7919)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=count))
+    return [(float(t[i]),) + serve_request(i, seed, **kw) for i in range(count)]
