@@ -15,6 +15,12 @@ Beside it, in the same JSON line:
                  comparison the batched mixed form is measured against;
   hopper      -- 2,000,000 polydisperse spheres in a 5,000-triangle hopper,
                  FP64 and mixed, dt = 2e-6 s;
+  serving     -- a Poisson stream (20 requests/s) of independent simulations
+                 of uneven size (70 % sphere boxes, 30 % spring lattices,
+                 lognormal size, median 20k, 500-20,000 steps) submitted in
+                 real time to the resident request loop: p50/p99 latency and
+                 particle-steps/s, batched mixed beside the plain FP64 path
+                 serving one request at a time;
   roofline    -- the dominant kernel of the timed window: algorithmic bytes
                  per launch over its mean duration from HIP events around
                  every launch on the library's stream (the window runs its
@@ -137,6 +143,40 @@ def hopper_leg(pmx, workloads, n, steps, dt):
     return out
 
 
+def serve_leg(pmx, workloads, count, rate, seed):
+    """A Poisson stream of independent requests (boxes of spheres and spring
+    lattices of uneven size and length) submitted in real time to the
+    resident request loop: batched mixed precision, and the plain FP64 path
+    serving them one at a time.  Latency = result on the host - submission."""
+    import numpy as np
+    stream = workloads.serve_stream(count, rate=rate, seed=seed)
+    out = {"requests": count, "rate_per_s": rate, "seed": seed,
+           "spheres": int(sum(r[1].ellnum for r in stream)), "bodies": int(sum(r[1].parnum for r in stream)),
+           "particle_steps": float(sum(r[1].parnum * int(round(r[2] / r[3] + 0.5)) for r in stream))}
+    forms = (("batched_mixed", pmx.PMX_MIXED, dict(gather_s=0.05)),
+             ("plain_fp64", pmx.PMX_FP64, dict(max_batch=1, max_active=1)))
+    for name, prec, kw in forms:
+        srv = pmx.Server(precision=prec, **kw)
+        ids = []
+        t0 = time.perf_counter()
+        for (ta, m, d, h, f) in stream:
+            wait = ta - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+            ids.append(srv.submit(m, d, h, f))
+        infos = [srv.wait(i) for i in ids]
+        st = srv.stats()
+        srv.close()
+        lat = np.array([i["done_s"] - i["submit_s"] for i in infos])
+        ps = float(sum(i["particles"] * i["steps"] for i in infos))
+        span = max(i["done_s"] for i in infos) - min(i["submit_s"] for i in infos)
+        out[name] = {"p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+                     "value": ps / span, "unit": "particle-steps/s", "span_s": span, "batches": st["batches"]}
+        log(f"serve {name}: p50 {out[name]['p50_s']:.2f} s, p99 {out[name]['p99_s']:.2f} s, "
+            f"{out[name]['value']:.3e} particle-steps/s over {span:.1f} s, {st['batches']} batches")
+    return out
+
+
 def cpu_leg(pmx, models, forcings_fn, ids, steps, threads):
     """Oracle on a bounded sample: the sample instances' current states,
     stepped on the host's cores."""
@@ -167,6 +207,8 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=8, help="instances timed on the plain FP64 path")
     ap.add_argument("--hopper", type=int, default=2000000, help="hopper spheres (0: skip)")
     ap.add_argument("--hopper-steps", type=int, default=200)
+    ap.add_argument("--serve", type=int, default=24, help="requests of the serving stream (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--cpu-instances", type=int, default=16)
     ap.add_argument("--cpu-steps", type=int, default=200)
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -226,6 +268,7 @@ def main():
         log(f"plain fp64 (one instance at a time): {plain:.3e} particle-steps/s")
 
     hopper = hopper_leg(pmx, workloads, args.hopper, args.hopper_steps, 2e-6) if args.hopper > 0 else None
+    serving = serve_leg(pmx, workloads, args.serve, args.serve_rate, 0) if args.serve > 0 else None
 
     cpu = None
     if not args.no_cpu:
@@ -276,6 +319,7 @@ def main():
                        "how": "drop-in FP64 path, each sampled instance alone (upload, warmup, timed window)"},
         "speedup_vs_plain_fp64": value / plain if plain else None,
         "hopper": hopper,
+        "serving": serving,
         "roofline": roofline(kernels) if kernels else None,
         "kernels_ms_per_launch": {k: v["ms"] / v["launches"] for k, v in kernels.items()},
         "kernels_ms_per_step": {k: v["ms"] / K for k, v in kernels.items()},

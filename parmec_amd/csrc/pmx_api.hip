@@ -532,7 +532,15 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
   }
   auto tl = std::chrono::steady_clock::now ();
   std::vector<char> reported (I, 0);
-  auto report = [&] (int upto) {
+  auto report = [&] (int upto) -> int {
+    bool any = false;
+    for (int i = 0; i < I; i++) any = any || (!reported[i] && nst[i] <= upto);
+    if (!any) return PMX_OK;
+    /* mesh geometry of the current particle state (finished instances are
+     * frozen at their final state) */
+    int r = step_mesh_shapes (*c, false);
+    if (r) return r;
+    PMX_HIP (hipStreamSynchronize (c->stream));
     for (int i = 0; i < I; i++)
       if (!reported[i] && nst[i] <= upto)
       {
@@ -540,6 +548,7 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
         reported[i] = 1;
         if (done_fn) done_fn (user, i);
       }
+    return PMX_OK;
   };
   /* with profiling on, every step runs eagerly and synchronises (events per
    * launch); otherwise one step is captured as a graph and replayed in
@@ -574,9 +583,9 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
       if ((rc = rebuild (c, false))) return rc;
       rebuilds++;
     }
-    report (done);
+    if ((rc = report (done))) return rc;
   }
-  report (nsteps);
+  if ((rc = report (nsteps))) return rc;
   c->cur = (c->cur0 + nsteps) & 1;
   c->curtime = c->icurtime[0]; c->curstep = c->icurstep[0]; c->stepnum = c->istepnum[0];
   double dt = std::chrono::duration<double> (std::chrono::steady_clock::now () - t0).count ();

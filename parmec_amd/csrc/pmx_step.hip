@@ -977,8 +977,13 @@ int enqueue_step (pmx_ctx &c, double h)
 {
   int rc;
   if (c.E > 0 && ((rc = step_bin (c)) || (rc = step_mtri (c)) || (rc = step_detect (c)))) return rc;
+  /* mesh nodes and faces (shapes.ispc:101-211) are not updated per step:
+   * no contact on this path reads mesh triangles (spheres and meshed
+   * particles never share an instance) and springs carry their own points,
+   * so they are formed from the final particle state when an instance
+   * finishes, with the same arithmetic and the same result */
   if ((rc = step_rows (c, h)) || (rc = step_springs (c, h)) || (rc = step_dynamics (c, h, h)) ||
-      (rc = step_mesh_shapes (c)) || (rc = step_obstacles (c, c.obstab.p, -1, h)) || (rc = step_end (c)))
+      (rc = step_obstacles (c, c.obstab.p, -1, h)) || (rc = step_end (c)))
     return rc;
   return PMX_OK;
 }
