@@ -132,16 +132,31 @@ __global__ void k_scatter (int *ctl, int E, const double *center, const double *
   rec[1] = make_double2 (center[2*(size_t) E + e], radii[e]);
 }
 
-#define DET_TPB 128     /* >= PMX_LSIZE: one sphere of the leaf per thread slot */
-#define DET_CMAX 256    /* candidate leaves of one block */
+#define DET_TPB 256     /* >= PMX_LSIZE: one sphere of the leaf per thread slot; 4 waves */
+
+/* one sphere-sphere test of the reference leaf test (condet.ispc:128-144):
+ * depth = r_a + r_b - |c_a - c_b| > 0.  d2 >= sr^2 (1 + 1e-10) implies the
+ * rounded sqrt(d2) >= sr, hence depth <= 0: such pairs skip the sqrt */
+__device__ __forceinline__ bool overlap (double ax, double ay, double az, double ar, double bx, double by, double bz,
+                                         double br)
+{
+  double qx = ax-bx, qy = ay-by, qz = az-bz;
+  double d2 = qx*qx + qy*qy + qz*qz;
+  double sr = ar + br;
+  if (d2 >= sr * sr * 1.0000000001) return false;
+  double len = sqrt (d2);
+  return sr - len > 0.0;
+}
 
 /* Detection (test_ellipsoids + test_triangles, condet.ispc:98-516), one block
  * per leaf Q.  A sphere reaches leaf L through the reference descent iff its
- * AABB overlaps L's cell.  The block collects the leaves reached by the union
- * of its spheres' AABBs once; for each candidate leaf it stages the leaf's
- * records in LDS, lists the block's spheres whose own AABB reaches it, and
- * spreads the (sphere, record) tests over all threads.  Hit slots per sphere
- * come from LDS counters; the hit order is irrelevant (rows sort their keys). */
+ * AABB overlaps L's cell.  The block's spheres sit in LDS; each wave takes
+ * candidate leaves (Q's neighbour list) in turn: it forms the list of the
+ * block's spheres whose AABB reaches the candidate (two ballots), stages the
+ * candidate's records in its own LDS area and spreads the (sphere, record)
+ * tests over its 64 lanes.  Waves never wait for each other inside the
+ * candidate loop.  Hit slots per sphere come from LDS counters; the hit
+ * order is irrelevant (rows sort their keys). */
 __global__ __launch_bounds__(DET_TPB)
 void k_detect (int E, const double *center, const double *radii, const int *part, const double *cell, int nleaves,
                const int *lcount, const int *loff, const int *lmem, const double *lpos,
@@ -152,133 +167,124 @@ void k_detect (int E, const double *center, const double *radii, const int *part
 {
   if (ctl[CTL_ABORT]) return;
   if (frozen (stop, leafinst[blockIdx.x], ctl)) return;   /* uniform over the block */
-  __shared__ double srec[4*DET_TPB];   /* staged records of the candidate leaf */
-  __shared__ int smem[DET_TPB];
-  __shared__ double ssph[4*DET_TPB];   /* block spheres: x, y, z, rx */
-  __shared__ int sidx[DET_TPB], spart[DET_TPB], shc[DET_TPB], slist[DET_TPB];
-  __shared__ int nsl;
+  constexpr int NW = DET_TPB/64;
+  __shared__ double ssph[4*PMX_LSIZE];   /* block spheres: x, y, z, rx */
+  __shared__ int sidx[PMX_LSIZE], spart[PMX_LSIZE], shc[PMX_LSIZE];
+  __shared__ double wrec[NW][4*PMX_LSIZE];   /* per wave: the candidate's records */
+  __shared__ int wmem[NW][PMX_LSIZE], wlist[NW][PMX_LSIZE];
   int Q = blockIdx.x;
   int t = threadIdx.x, lane = t & 63, w = t >> 6;
   int qoff = loff[Q], qcnt = lcount[Q];
   bool valid = t < qcnt;
-  double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
-  double r0 = 0.0;
   if (valid)
   {
     int e = lmem[qoff + t];
-    double p0 = center[e], p1 = center[(size_t) E + e], p2 = center[2*(size_t) E + e];
-    r0 = radii[e];
-    double r1 = radii[(size_t) E + e], r2 = radii[2*(size_t) E + e];
+    double r0 = radii[e], r1 = radii[(size_t) E + e], r2 = radii[2*(size_t) E + e];
     double rx = r0 > r1 ? r0 : r1;
     rx = rx > r2 ? rx : r2;
-    lo[0] = p0-rx; lo[1] = p1-rx; lo[2] = p2-rx;
-    hi[0] = p0+rx; hi[1] = p1+rx; hi[2] = p2+rx;
-    ssph[4*t] = p0; ssph[4*t+1] = p1; ssph[4*t+2] = p2; ssph[4*t+3] = rx;
+    ssph[4*t] = center[e]; ssph[4*t+1] = center[(size_t) E + e]; ssph[4*t+2] = center[2*(size_t) E + e];
+    ssph[4*t+3] = rx;
     sidx[t] = e;
     spart[t] = part[e];
+    shc[t] = 0;
   }
-  shc[t] = 0;
   double cl[3] = {cell[Q], cell[(size_t) nleaves + Q], cell[2*(size_t) nleaves + Q]};
   double cu[3] = {cell[3*(size_t) nleaves + Q], cell[4*(size_t) nleaves + Q], cell[5*(size_t) nleaves + Q]};
-
   __syncthreads ();
+
+  /* this lane's two block spheres (lane, lane + 64) */
+  bool v0 = lane < qcnt, v1 = lane + 64 < qcnt;
+  double a0[4] = {0.0, 0.0, 0.0, 0.0}, a1[4] = {0.0, 0.0, 0.0, 0.0};
+  if (v0) for (int k = 0; k < 4; k++) a0[k] = ssph[4*lane+k];
+  if (v1) for (int k = 0; k < 4; k++) a1[k] = ssph[4*(lane+64)+k];
+  double *rec = wrec[w];
+  int *mem = wmem[w], *lst = wlist[w];
+  unsigned long long lt = lane ? ((1ull << lane) - 1ull) : 0ull;
   int nc = nbrcnt[Q], c0 = nbroff[Q];
-  for (int ci = 0; ci < nc; ci++)
+  for (int ci = w; ci < nc; ci += NW)
   {
     int L = nbrlist[c0 + ci];
-    /* spheres of this block whose own AABB reaches L (condet.ispc:104-113) */
-    bool reach = valid && lo[0] <= cell[3*(size_t) nleaves + L] && lo[1] <= cell[4*(size_t) nleaves + L] &&
-                 lo[2] <= cell[5*(size_t) nleaves + L] && hi[0] >= cell[L] && hi[1] >= cell[(size_t) nleaves + L] &&
-                 hi[2] >= cell[2*(size_t) nleaves + L];
-    int off = loff[L], cnt = lcount[L];
-    if (!__syncthreads_or (reach && cnt > 0)) continue;
-    if (t < cnt)
-    {
-      const double2 *rec = (const double2*) (lpos + 4*(size_t) (off + t));
-      double2 a = rec[0], c = rec[1];
-      srec[4*t] = a.x; srec[4*t+1] = a.y; srec[4*t+2] = c.x; srec[4*t+3] = c.y;
-      smem[t] = lmem[off + t];
-    }
-    if (t == 0) nsl = 0;
-    __syncthreads ();
-    unsigned long long rm = __ballot (reach);
-    int wbase = 0;
-    if (lane == 0 && rm) wbase = atomicAdd (&nsl, __popcll (rm));
-    wbase = __shfl (wbase, 0, 64);
-    if (reach)
-    {
-      unsigned long long lt = lane ? ((1ull << lane) - 1ull) : 0ull;
-      slist[wbase + __popcll (rm & lt)] = t;
-    }
-    __syncthreads ();
-    if (L == Q)
+    int cnt = lcount[L];
+    if (cnt == 0) continue;
+    int off = loff[L];
+    /* the block spheres whose own AABB reaches L (condet.ispc:104-113) */
+    double Ll[3] = {cell[L], cell[(size_t) nleaves + L], cell[2*(size_t) nleaves + L]};
+    double Lu[3] = {cell[3*(size_t) nleaves + L], cell[4*(size_t) nleaves + L], cell[5*(size_t) nleaves + L]};
+    bool r0 = v0 && a0[0]-a0[3] <= Lu[0] && a0[1]-a0[3] <= Lu[1] && a0[2]-a0[3] <= Lu[2] &&
+              a0[0]+a0[3] >= Ll[0] && a0[1]+a0[3] >= Ll[1] && a0[2]+a0[3] >= Ll[2];
+    bool r1 = v1 && a1[0]-a1[3] <= Lu[0] && a1[1]-a1[3] <= Lu[1] && a1[2]-a1[3] <= Lu[2] &&
+              a1[0]+a1[3] >= Ll[0] && a1[1]+a1[3] >= Ll[1] && a1[2]+a1[3] >= Ll[2];
+    unsigned long long m0 = __ballot (r0), m1 = __ballot (r1);
+    if (!(m0 | m1)) continue;
+    int n0 = __popcll (m0), nsl = n0 + __popcll (m1);
+    bool own = L == Q;
+    /* stage: the reaching spheres' slots and the candidate's records */
+    __builtin_amdgcn_wave_barrier ();
+    if (r0) lst[__popcll (m0 & lt)] = lane;
+    if (r1) lst[n0 + __popcll (m1 & lt)] = lane + 64;
+    if (!own)
+      for (int k = lane; k < cnt; k += 64)
+      {
+        const double2 *rp = (const double2*) (lpos + 4*(size_t) (off + k));
+        double2 x = rp[0], y = rp[1];
+        rec[4*k] = x.x; rec[4*k+1] = x.y; rec[4*k+2] = y.x; rec[4*k+3] = y.y;
+        mem[k] = lmem[off + k];
+      }
+    __builtin_amdgcn_wave_barrier ();
+    if (own)
     {
       /* own leaf: every sphere reaches it and the test is symmetric
        * (depth r_a + r_b - |c_a - c_b|), so each unordered pair is tested
        * once: pairs (a, (a+k) mod n), k = 1..(n-1)/2, plus k = n/2, a < n/2 */
-      int n = cnt, half = (n - 1) / 2;
-      int work = n * half + ((n & 1) ? 0 : n / 2);
+      int n = cnt, halfn = (n - 1) / 2;
+      int work = n * halfn + ((n & 1) ? 0 : n / 2);
       unsigned inv = ((1u << 20) + (unsigned) n - 1u) / (unsigned) n;
-      for (int q = t; q < work; q += DET_TPB)
+      for (int q = lane; q < work; q += 64)
       {
         int k = (int) (((unsigned) q * inv) >> 20), a = q - k * n;
         int bb = a + k + 1;
         bb = bb >= n ? bb - n : bb;
-        double qx = ssph[4*a]-ssph[4*bb], qy = ssph[4*a+1]-ssph[4*bb+1], qz = ssph[4*a+2]-ssph[4*bb+2];
-        double d2 = qx*qx + qy*qy + qz*qz;
-        double sr = ssph[4*a+3] + srec[4*bb+3];
-        if (d2 >= sr * sr * 1.0000000001) continue;
-        double len = sqrt (d2);
-        double depth = sr - len;
-        if (depth > 0.0 && spart[a] != spart[bb])
-        {
-          int ea = sidx[a], eb = sidx[bb];
-          int sa = atomicAdd (&shc[a], 1), sb = atomicAdd (&shc[bb], 1);
-          if (sa < hcap) hits[(size_t) sa * E + ea] = (uint32_t) eb; else atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1);
-          if (sb < hcap) hits[(size_t) sb * E + eb] = (uint32_t) ea; else atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1);
-        }
+        /* rx = max (r0, r1, r2) is r0 for a sphere (r1, r2 < 0) */
+        if (!overlap (ssph[4*a], ssph[4*a+1], ssph[4*a+2], ssph[4*a+3],
+                      ssph[4*bb], ssph[4*bb+1], ssph[4*bb+2], ssph[4*bb+3]))
+          continue;
+        if (spart[a] == spart[bb]) continue;
+        int ea = sidx[a], eb = sidx[bb];
+        int sa = atomicAdd (&shc[a], 1), sb = atomicAdd (&shc[bb], 1);
+        if (sa < hcap) hits[(size_t) sa * E + ea] = (uint32_t) eb; else atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1);
+        if (sb < hcap) hits[(size_t) sb * E + eb] = (uint32_t) ea; else atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1);
       }
     }
     else
     {
-    int work = nsl * cnt;
-    /* a = q / cnt exactly for q * cnt < 2^20 (q < 96*96, cnt <= 96) */
-    unsigned inv = ((1u << 20) + (unsigned) cnt - 1u) / (unsigned) cnt;
-    for (int q = t; q < work; q += DET_TPB)
-    {
-      int a = (int) (((unsigned) q * inv) >> 20), m = q - a * cnt;
-      int ti = slist[a];
-      double px = ssph[4*ti], py = ssph[4*ti+1], pz = ssph[4*ti+2], rx = ssph[4*ti+3];
-      double cx = srec[4*m], cy = srec[4*m+1], cz = srec[4*m+2], rj = srec[4*m+3];
-      double qx = px-cx, qy = py-cy, qz = pz-cz;
-      double d2 = qx*qx + qy*qy + qz*qz;
-      double sr = rx + rj;
-      /* d2 >= sr^2 (1 + 1e-10) implies the rounded sqrt(d2) >= sr, hence
-       * depth = sr - len <= 0: such pairs are skipped without the sqrt */
-      if (d2 >= sr * sr * 1.0000000001) continue;
-      double len = sqrt (d2);
-      double depth = sr - len;
-      int j = smem[m];
-      if (depth > 0.0 && spart[ti] != part[j])
+      int work = nsl * cnt;
+      /* a = q / cnt exactly for q * cnt < 2^20 (q < 96*96, cnt <= 96) */
+      unsigned inv = ((1u << 20) + (unsigned) cnt - 1u) / (unsigned) cnt;
+      for (int q = lane; q < work; q += 64)
       {
+        int a = (int) (((unsigned) q * inv) >> 20), m = q - a * cnt;
+        int ti = lst[a];
+        double cx = rec[4*m], cy = rec[4*m+1], cz = rec[4*m+2], rj = rec[4*m+3];
+        if (!overlap (ssph[4*ti], ssph[4*ti+1], ssph[4*ti+2], ssph[4*ti+3], cx, cy, cz, rj)) continue;
+        int j = mem[m];
+        if (spart[ti] == part[j]) continue;
         int e = sidx[ti];
         int slot = atomicAdd (&shc[ti], 1);
         if (slot < hcap) hits[(size_t) slot * E + e] = (uint32_t) j;
         else atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1);
         /* does the partner's AABB reach this sphere's leaf Q? */
-        double ljx = cx-rj, ljy = cy-rj, ljz = cz-rj, hjx = cx+rj, hjy = cy+rj, hjz = cz+rj;
-        bool back = ljx <= cu[0] && ljy <= cu[1] && ljz <= cu[2] && hjx >= cl[0] && hjy >= cl[1] && hjz >= cl[2];
+        bool back = cx-rj <= cu[0] && cy-rj <= cu[1] && cz-rj <= cu[2] &&
+                    cx+rj >= cl[0] && cy+rj >= cl[1] && cz+rj >= cl[2];
         if (!back)
         {
-          int s = atomicAdd (&xcnt[j], 1);
-          if (s < xcap) xbuf[(size_t) s * E + j] = e;
+          int sx = atomicAdd (&xcnt[j], 1);
+          if (sx < xcap) xbuf[(size_t) sx * E + j] = e;
           else atomicAdd (&flags[FLAG_EXTRA_OVERFLOW], 1);
         }
       }
     }
-    }
-    __syncthreads ();
   }
+  __syncthreads ();
 
   /* triangles whose AABB reaches Q (test_triangles, condet.ispc:412-516):
    * static ones from the rebuild-time lists, moving ones from this step's */
@@ -288,12 +294,12 @@ void k_detect (int E, const double *center, const double *radii, const int *part
   int ntq = nsq + nmq;
   if (ntq > 0 && qcnt > 0)
   {
-    int a0 = trioff[Q];
+    int toff = trioff[Q];
     int work = qcnt * ntq;
     for (int q = t; q < work; q += DET_TPB)
     {
       int ti = q / ntq, m = q - ti * ntq;
-      int tt = m < nsq ? trilist[a0 + m] : mtlist[(size_t) Q*mcap + (m - nsq)];
+      int tt = m < nsq ? trilist[toff + m] : mtlist[(size_t) Q*mcap + (m - nsq)];
       double v[9];
 #pragma unroll
       for (int k = 0; k < 9; k++) v[k] = tri[(size_t) k*T + tt];
@@ -317,7 +323,6 @@ void k_detect (int E, const double *center, const double *radii, const int *part
   }
   __syncthreads ();
   if (valid) hitcnt[sidx[t]] = shc[t] < hcap ? shc[t] : hcap;
-  (void) r0;
 }
 
 __global__ void k_force_init (const int *ctl, int N, const int *list, int nlist, const double *mass, const double *gtab,
