@@ -153,6 +153,8 @@ typedef struct pmx_options
   int device;          /* HIP device ordinal */
   int row_capacity;    /* contacts per sphere row (0: default 32) */
   int verbose;
+  int tune;            /* 0: pick kernel block shapes by timing them at the first dem call
+                          (models of 50,000 spheres or more); 1: fixed defaults */
 } pmx_options;
 
 /* one contact of the canonical pair set */
@@ -171,6 +173,8 @@ typedef struct pmx_stats
   int leaves;              /* leaves of the current partitioning tree */
   int max_row;             /* largest contact row seen */
   double seconds;          /* wall time of the last pmx_dem step loop */
+  int det_waves;           /* block shapes in use: waves per detection block, */
+  int rows_block;          /* threads per contact-row block */
 } pmx_stats;
 
 /* per-kernel timing collected with HIP events on the context's stream */

@@ -530,6 +530,18 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
     if ((rc = rebuild (c, true))) return rc;
     rebuilds++;
   }
+  /* block shapes for this workload, once per context (large models only:
+   * small ones are bound by launch latency, not by the shapes) */
+  if (c->opt.tune == 0 && !c->tuned && c->E >= 50000)
+  {
+    if ((rc = autotune (*c, step0))) return rc;
+    int z[CTL_COUNT] = {0};
+    PMX_HIP (hipMemcpyAsync (c->ctl.p, z, sizeof (z), hipMemcpyHostToDevice, c->stream));
+    PMX_HIP (hipMemsetAsync (c->dflags.p, 0, FLAG_COUNT * sizeof (int), c->stream));
+    PMX_HIP (hipStreamSynchronize (c->stream));
+  }
+  c->stats.det_waves = c->det_nw;
+  c->stats.rows_block = c->rows_tpb;
   auto tl = std::chrono::steady_clock::now ();
   std::vector<char> reported (I, 0);
   auto report = [&] (int upto) -> int {

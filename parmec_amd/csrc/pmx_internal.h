@@ -186,6 +186,9 @@ struct pmx_ctx
   int cur0 = 0;                /* row buffer holding the rows at step 0 of the call */
   hipGraphExec_t gexec = nullptr;
   bool graph_valid = false;
+  int det_nw = 4;              /* waves per k_detect block (autotuned) */
+  int rows_tpb = 128;          /* threads per k_rows block (autotuned) */
+  bool tuned = false;
   dvec<int> parell;            /* [N] the particle's sphere, or -1 */
   dvec<int> nosph;             /* particles without a sphere */
   dvec<signed char> iso;       /* [N] 1 when inertia is diag(j, j, j) with exact zeros */
@@ -268,6 +271,10 @@ int scan_exclusive (pmx_ctx &c, int *data, int n, int *total_host, const int *ct
 int step_prepare (pmx_ctx &c);               /* per-tree buffers, after a (re)build */
 int step_bin (pmx_ctx &c);
 int step_detect (pmx_ctx &c);
+int step_scatter (pmx_ctx &c);
+int launch_detect (pmx_ctx &c, int nw);
+int launch_rows (pmx_ctx &c, int tpb, double h);
+int autotune (pmx_ctx &c, double h);          /* block shapes of detect and rows */
 int step_rows (pmx_ctx &c, double h);
 int step_dynamics (pmx_ctx &c, double step0, double step1);
 int step_end (pmx_ctx &c);

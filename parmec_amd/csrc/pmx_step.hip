@@ -132,7 +132,8 @@ __global__ void k_scatter (int *ctl, int E, const double *center, const double *
   rec[1] = make_double2 (center[2*(size_t) E + e], radii[e]);
 }
 
-#define DET_TPB 256     /* >= PMX_LSIZE: one sphere of the leaf per thread slot; 4 waves */
+/* k_detect runs NW waves per block (NW * 64 >= PMX_LSIZE: one sphere of the
+ * leaf per thread slot); NW is picked per workload by the autotuner */
 
 /* one sphere-sphere test of the reference leaf test (condet.ispc:128-144):
  * depth = r_a + r_b - |c_a - c_b| > 0.  d2 >= sr^2 (1 + 1e-10) implies the
@@ -157,7 +158,8 @@ __device__ __forceinline__ bool overlap (double ax, double ay, double az, double
  * tests over its 64 lanes.  Waves never wait for each other inside the
  * candidate loop.  Hit slots per sphere come from LDS counters; the hit
  * order is irrelevant (rows sort their keys). */
-__global__ __launch_bounds__(DET_TPB)
+template <int NW>
+__global__ __launch_bounds__(NW*64)
 void k_detect (int E, const double *center, const double *radii, const int *part, const double *cell, int nleaves,
                const int *lcount, const int *loff, const int *lmem, const double *lpos,
                const double *tri, int T, const int *tricnt, const int *trioff, const int *trilist,
@@ -167,7 +169,6 @@ void k_detect (int E, const double *center, const double *radii, const int *part
 {
   if (ctl[CTL_ABORT]) return;
   if (frozen (stop, leafinst[blockIdx.x], ctl)) return;   /* uniform over the block */
-  constexpr int NW = DET_TPB/64;
   __shared__ double ssph[4*PMX_LSIZE];   /* block spheres: x, y, z, rx */
   __shared__ int sidx[PMX_LSIZE], spart[PMX_LSIZE], shc[PMX_LSIZE];
   __shared__ double wrec[NW][4*PMX_LSIZE];   /* per wave: the candidate's records */
@@ -296,7 +297,7 @@ void k_detect (int E, const double *center, const double *radii, const int *part
   {
     int toff = trioff[Q];
     int work = qcnt * ntq;
-    for (int q = t; q < work; q += DET_TPB)
+    for (int q = t; q < work; q += NW*64)
     {
       int ti = q / ntq, m = q - ti * ntq;
       int tt = m < nsq ? trilist[toff + m] : mtlist[(size_t) Q*mcap + (m - nsq)];
@@ -344,7 +345,6 @@ __global__ void k_force_init (const int *ctl, int N, const int *list, int nlist,
   torque[2*(size_t) N + i] = 0.0;
 }
 
-#define ROWS_TPB 128
 
 /* row merge + contact forces + per-sphere accumulation.  R is the arithmetic
  * of the contact geometry and force law: double (the FP64 path) or float
@@ -352,8 +352,8 @@ __global__ void k_force_init (const int *ctl, int N, const int *list, int nlist,
  * narrowed, the depth that decides whether a contact persists stays FP64, the
  * tangential state is stored in FP32 and the per-sphere sums are
  * Kahan-compensated FP32) */
-template <class R>
-__global__ __launch_bounds__(ROWS_TPB)
+template <class R, int TPB>
+__global__ __launch_bounds__(TPB)
 void k_rows (int E, int N, const double *center, const double *radii, const int *part, const int *ellcol,
              const double *tri, int T, const int *tricol, const int *triobs,
              const double *position, const double *linear, const double *angular, const double *mass,
@@ -865,7 +865,9 @@ int step_bin (pmx_ctx &c)
   return PMX_OK;
 }
 
-int step_detect (pmx_ctx &c)
+/* leaf offsets, leaf-ordered records and cleared reverse-hit counts: the
+ * inputs of k_detect */
+int step_scatter (pmx_ctx &c)
 {
   TreeDev &t = c.tree;
   int L = std::max (t.nleaves, 1);
@@ -881,9 +883,16 @@ int step_detect (pmx_ctx &c)
                         (const int*) c.ell_inst.p);
   }
   hipLaunchKernelGGL (k_fill_i, grid (c.E), dim3 (PMX_TPB), 0, c.stream, (const int*) c.ctl.p, c.xcnt.p, c.E, 0);
-  {
-    ProfScope ps (c, K_DETECT, 68.0 * c.E);
-    hipLaunchKernelGGL (k_detect, dim3 (t.nleaves), dim3 (DET_TPB), 0, c.stream, c.E, (const double*) c.center.p,
+  PMX_HIP (hipGetLastError ());
+  return PMX_OK;
+}
+
+int launch_detect (pmx_ctx &c, int nw)
+{
+  TreeDev &t = c.tree;
+  ProfScope ps (c, K_DETECT, 68.0 * c.E);
+  if (nw == 2)
+    hipLaunchKernelGGL (k_detect<2>, dim3 (t.nleaves), dim3 (128), 0, c.stream, c.E, (const double*) c.center.p,
                         (const double*) c.radii.p, (const int*) c.part.p, (const double*) t.cell.p,
                         t.nleaves, (const int*) c.lcount.p, (const int*) c.loff.p,
                         (const int*) c.lmem.p, (const double*) c.lpos.p, (const double*) c.tri.p, c.T,
@@ -892,7 +901,71 @@ int step_detect (pmx_ctx &c)
                         (const int*) t.nbrcnt.p, (const int*) t.nbroff.p, (const int*) t.nbrlist.p,
                         c.hits.p, c.hitcnt.p, c.hcap, c.xbuf.p, c.xcnt.p, c.xcap, c.dflags.p, (const int*) c.ctl.p,
                         c.staggered ? (const int*) c.istop.p : (const int*) nullptr, (const int*) t.leafinst.p);
-  }
+  else if (nw == 8)
+    hipLaunchKernelGGL (k_detect<8>, dim3 (t.nleaves), dim3 (512), 0, c.stream, c.E, (const double*) c.center.p,
+                        (const double*) c.radii.p, (const int*) c.part.p, (const double*) t.cell.p,
+                        t.nleaves, (const int*) c.lcount.p, (const int*) c.loff.p,
+                        (const int*) c.lmem.p, (const double*) c.lpos.p, (const double*) c.tri.p, c.T,
+                        (const int*) t.tricnt.p, (const int*) t.trioff.p, (const int*) t.trilist.p,
+                        c.nmtri > 0 ? (const int*) c.mtcnt.p : (const int*) nullptr, (const int*) c.mtlist.p, c.mcap,
+                        (const int*) t.nbrcnt.p, (const int*) t.nbroff.p, (const int*) t.nbrlist.p,
+                        c.hits.p, c.hitcnt.p, c.hcap, c.xbuf.p, c.xcnt.p, c.xcap, c.dflags.p, (const int*) c.ctl.p,
+                        c.staggered ? (const int*) c.istop.p : (const int*) nullptr, (const int*) t.leafinst.p);
+  else
+    hipLaunchKernelGGL (k_detect<4>, dim3 (t.nleaves), dim3 (256), 0, c.stream, c.E, (const double*) c.center.p,
+                        (const double*) c.radii.p, (const int*) c.part.p, (const double*) t.cell.p,
+                        t.nleaves, (const int*) c.lcount.p, (const int*) c.loff.p,
+                        (const int*) c.lmem.p, (const double*) c.lpos.p, (const double*) c.tri.p, c.T,
+                        (const int*) t.tricnt.p, (const int*) t.trioff.p, (const int*) t.trilist.p,
+                        c.nmtri > 0 ? (const int*) c.mtcnt.p : (const int*) nullptr, (const int*) c.mtlist.p, c.mcap,
+                        (const int*) t.nbrcnt.p, (const int*) t.nbroff.p, (const int*) t.nbrlist.p,
+                        c.hits.p, c.hitcnt.p, c.hcap, c.xbuf.p, c.xcnt.p, c.xcap, c.dflags.p, (const int*) c.ctl.p,
+                        c.staggered ? (const int*) c.istop.p : (const int*) nullptr, (const int*) t.leafinst.p);
+  PMX_HIP (hipGetLastError ());
+  return PMX_OK;
+}
+
+int step_detect (pmx_ctx &c)
+{
+  int rc = step_scatter (c);
+  if (rc) return rc;
+  return launch_detect (c, c.det_nw);
+}
+
+template <class R, int TPB>
+void rows_launch (pmx_ctx &c, R *st0, R *st1, double h)
+{
+  size_t shm = (size_t) TPB * (c.hcap + c.xcap) * sizeof (uint32_t);
+  hipLaunchKernelGGL ((k_rows<R, TPB>), dim3 ((c.E + TPB - 1) / TPB), dim3 (TPB), shm, c.stream, c.E, c.N,
+                      (const double*) c.center.p, (const double*) c.radii.p, (const int*) c.part.p,
+                      (const int*) c.ellcol.p, (const double*) c.tri.p, c.T, (const int*) c.tricol.p,
+                      (const int*) c.triobs.p, (const double*) c.position.p, (const double*) c.linear.p,
+                      (const double*) c.angular.p, (const double*) c.mass.p, (const double*) c.invm.p, c.P,
+                      (const int*) c.pairs.p, (const int*) c.ikind.p, (const double*) c.iparam.p,
+                      c.I, (const int*) c.ell_inst.p, (const int*) c.pair_off.p,
+                      c.rowkey[0].p, st0, c.rowcnt[0].p, c.rowkey[1].p, st1, c.rowcnt[1].p,
+                      c.cur0, c.cap, (const uint32_t*) c.hits.p, (const int*) c.hitcnt.p, c.hcap,
+                      (const int*) c.xbuf.p, (const int*) c.xcnt.p, c.xcap, (const double*) c.gtab.p,
+                      (const int*) c.ctl.p, h, c.force.p, c.torque.p, c.dflags.p,
+                      (const double*) c.obspnt.p, (const double*) c.obsang.p, (const double*) c.obslin.p,
+                      c.staggered ? (const int*) c.istop.p : (const int*) nullptr);
+}
+
+template <class R>
+void rows_launch_tpb (pmx_ctx &c, int tpb, R *st0, R *st1, double h)
+{
+  if (tpb == 64) rows_launch<R, 64> (c, st0, st1, h);
+  else if (tpb == 256) rows_launch<R, 256> (c, st0, st1, h);
+  else rows_launch<R, 128> (c, st0, st1, h);
+}
+
+/* k_rows with tpb threads per block (64, 128 or 256) */
+int launch_rows (pmx_ctx &c, int tpb, double h)
+{
+  if (c.E == 0) return PMX_OK;
+  ProfScope ps (c, K_ROWS, 136.0 * c.E);
+  if (c.opt.precision == PMX_MIXED) rows_launch_tpb<float> (c, tpb, c.rowst32[0].p, c.rowst32[1].p, h);
+  else rows_launch_tpb<double> (c, tpb, c.rowst[0].p, c.rowst[1].p, h);
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
 }
@@ -907,37 +980,8 @@ int step_rows (pmx_ctx &c, double h)
                         c.I, (const int*) c.par_inst.p, c.force.p, c.torque.p,
                         c.staggered ? (const int*) c.istop.p : (const int*) nullptr);
   }
-  if (c.E > 0)
-  {
-    ProfScope ps (c, K_ROWS, 136.0 * c.E);
-    size_t shm = (size_t) ROWS_TPB * (c.hcap + c.xcap) * sizeof (uint32_t);
-    if (c.opt.precision == PMX_MIXED)
-      hipLaunchKernelGGL (k_rows<float>, dim3 ((c.E + ROWS_TPB - 1) / ROWS_TPB), dim3 (ROWS_TPB), shm, c.stream, c.E, c.N,
-                          (const double*) c.center.p, (const double*) c.radii.p, (const int*) c.part.p,
-                          (const int*) c.ellcol.p, (const double*) c.tri.p, c.T, (const int*) c.tricol.p,
-                          (const int*) c.triobs.p, (const double*) c.position.p, (const double*) c.linear.p,
-                          (const double*) c.angular.p, (const double*) c.mass.p, (const double*) c.invm.p, c.P,
-                          (const int*) c.pairs.p, (const int*) c.ikind.p, (const double*) c.iparam.p,
-                          c.I, (const int*) c.ell_inst.p, (const int*) c.pair_off.p, c.rowkey[0].p, c.rowst32[0].p, c.rowcnt[0].p, c.rowkey[1].p, c.rowst32[1].p, c.rowcnt[1].p,
-                          c.cur0, c.cap, (const uint32_t*) c.hits.p,
-                          (const int*) c.hitcnt.p, c.hcap, (const int*) c.xbuf.p, (const int*) c.xcnt.p, c.xcap,
-                          (const double*) c.gtab.p, (const int*) c.ctl.p, h, c.force.p, c.torque.p, c.dflags.p,
-                            (const double*) c.obspnt.p, (const double*) c.obsang.p, (const double*) c.obslin.p,
-                          c.staggered ? (const int*) c.istop.p : (const int*) nullptr);
-    else
-      hipLaunchKernelGGL (k_rows<double>, dim3 ((c.E + ROWS_TPB - 1) / ROWS_TPB), dim3 (ROWS_TPB), shm, c.stream, c.E, c.N,
-                          (const double*) c.center.p, (const double*) c.radii.p, (const int*) c.part.p,
-                          (const int*) c.ellcol.p, (const double*) c.tri.p, c.T, (const int*) c.tricol.p,
-                          (const int*) c.triobs.p, (const double*) c.position.p, (const double*) c.linear.p,
-                          (const double*) c.angular.p, (const double*) c.mass.p, (const double*) c.invm.p, c.P,
-                          (const int*) c.pairs.p, (const int*) c.ikind.p, (const double*) c.iparam.p,
-                          c.I, (const int*) c.ell_inst.p, (const int*) c.pair_off.p, c.rowkey[0].p, c.rowst[0].p, c.rowcnt[0].p, c.rowkey[1].p, c.rowst[1].p, c.rowcnt[1].p,
-                          c.cur0, c.cap, (const uint32_t*) c.hits.p,
-                          (const int*) c.hitcnt.p, c.hcap, (const int*) c.xbuf.p, (const int*) c.xcnt.p, c.xcap,
-                          (const double*) c.gtab.p, (const int*) c.ctl.p, h, c.force.p, c.torque.p, c.dflags.p,
-                          (const double*) c.obspnt.p, (const double*) c.obsang.p, (const double*) c.obslin.p,
-                          c.staggered ? (const int*) c.istop.p : (const int*) nullptr);
-  }
+  int rc = launch_rows (c, c.rows_tpb, h);
+  if (rc) return rc;
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
 }
@@ -1016,5 +1060,54 @@ int step_invert (pmx_ctx &c)
   hipLaunchKernelGGL (k_invert, grid (c.N), dim3 (PMX_TPB), 0, c.stream, c.N, (const double*) c.inertia.p,
                       c.inverse.p, (const double*) c.mass.p, c.invm.p);
   PMX_HIP (hipGetLastError ());
+  return PMX_OK;
+}
+
+/* Autotuner: pick the waves per block of k_detect and the block size of
+ * k_rows for this workload by timing each variant on the current state
+ * (both kernels only read their inputs and rewrite their outputs, so runs
+ * repeat exactly; block shapes never change a result).  Called once per
+ * context, at a step boundary with the control block clear; the caller
+ * clears the control block and flags afterwards. */
+int autotune (pmx_ctx &c, double h)
+{
+  int rc;
+  if ((rc = step_bin (c)) || (rc = step_mtri (c)) || (rc = step_scatter (c))) return rc;
+  int ctl[CTL_COUNT];
+  PMX_HIP (hipMemcpyAsync (ctl, c.ctl.p, sizeof (ctl), hipMemcpyDeviceToHost, c.stream));
+  PMX_HIP (hipStreamSynchronize (c.stream));
+  if (ctl[CTL_ABORT]) return PMX_OK;   /* a leaf overflows: tune at a later call */
+  hipEvent_t e0, e1;
+  PMX_HIP (hipEventCreate (&e0));
+  PMX_HIP (hipEventCreate (&e1));
+  auto best_of = [&] (const int *opts, int nopt, auto run) -> int {
+    int best = opts[0];
+    float bt = 1e30f;
+    for (int k = 0; k < nopt; k++)
+      for (int rep = 0; rep < 3; rep++)
+      {
+        (void) hipEventRecord (e0, c.stream);
+        if (run (opts[k])) return -1;
+        (void) hipEventRecord (e1, c.stream);
+        float ms = 0.0f;
+        if (hipEventSynchronize (e1) != hipSuccess || hipEventElapsedTime (&ms, e0, e1) != hipSuccess) return -1;
+        if (ms < bt) { bt = ms; best = opts[k]; }
+      }
+    return best;
+  };
+  const int nws[3] = {2, 4, 8}, tpbs[3] = {64, 128, 256};
+  int nw = best_of (nws, 3, [&] (int v) {
+    hipLaunchKernelGGL (k_fill_i, grid (c.E), dim3 (PMX_TPB), 0, c.stream, (const int*) c.ctl.p, c.xcnt.p, c.E, 0);
+    return launch_detect (c, v); });
+  int tpb = nw > 0 ? best_of (tpbs, 3, [&] (int v) {
+    hipLaunchKernelGGL (k_fill_i, grid (c.E), dim3 (PMX_TPB), 0, c.stream, (const int*) c.ctl.p, c.xcnt.p, c.E, 0);
+    int r = launch_detect (c, nw);
+    return r ? r : launch_rows (c, v, h); }) : -1;
+  (void) hipEventDestroy (e0);
+  (void) hipEventDestroy (e1);
+  if (nw < 0 || tpb < 0) { pmx_set_error ("autotuning failed"); return PMX_EDEVICE; }
+  c.det_nw = nw;
+  c.rows_tpb = tpb;
+  c.tuned = true;
   return PMX_OK;
 }

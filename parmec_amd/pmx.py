@@ -50,7 +50,8 @@ class pmx_forcing(C.Structure):
 
 
 class pmx_options(C.Structure):
-    _fields_ = [("precision", C.c_int), ("device", C.c_int), ("row_capacity", C.c_int), ("verbose", C.c_int)]
+    _fields_ = [("precision", C.c_int), ("device", C.c_int), ("row_capacity", C.c_int), ("verbose", C.c_int),
+                ("tune", C.c_int)]
 
 
 class pmx_pair(C.Structure):
@@ -59,7 +60,7 @@ class pmx_pair(C.Structure):
 
 class pmx_stats(C.Structure):
     _fields_ = [("steps", C.c_longlong), ("rebuilds", C.c_longlong), ("pairs", C.c_longlong), ("leaves", C.c_int),
-                ("max_row", C.c_int), ("seconds", C.c_double)]
+                ("max_row", C.c_int), ("seconds", C.c_double), ("det_waves", C.c_int), ("rows_block", C.c_int)]
 
 
 class pmx_kernel_time(C.Structure):
@@ -302,9 +303,9 @@ class Engine:
 
     prefix = "pmx_"
 
-    def __init__(self, precision=PMX_FP64, device=0, row_capacity=0, lib=None):
+    def __init__(self, precision=PMX_FP64, device=0, row_capacity=0, lib=None, tune=0):
         self.lib = lib or load()
-        opt = pmx_options(precision, device, row_capacity, 0)
+        opt = pmx_options(precision, device, row_capacity, 0, tune)
         h = C.c_void_p()
         self._call("create", C.byref(opt), C.byref(h))
         self.h = h
