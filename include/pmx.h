@@ -125,6 +125,14 @@ typedef struct pmx_model
   int *unspring;     /* [sprnum] (-2: no unspring action) */
   int *sproffset;    /* [sprnum] (-1: no load curve offset) */
 
+  /* prescribed motion, parmec.h:244-251 (PRESCRIBE, input.cpp:2647-2795):
+   * per entry the particle and the kind of its linear and angular parts
+   * (-1 none, 0 velocity, 1 acceleration); values per step in
+   * pmx_forcing.prstab */
+  int prsnum;
+  int *prspart;      /* [prsnum] */
+  int *prskind;      /* [2][prsnum]: linear kind, angular kind */
+
   /* time bookkeeping, parmec.h:44-49 */
   double curtime;
   double curstep;
@@ -145,6 +153,8 @@ typedef struct pmx_forcing
   const double *dtab;        /* [nsteps][6] damping per step, or NULL */
   const double *obstab;      /* [nsteps][obsnum][6] obstacle (angular, linear) velocity at curtime+h, or NULL */
   const double *obstab0;     /* [obsnum][6] obstacle velocity at t = h/2 (first call only), or NULL */
+  const double *prstab;      /* [nsteps][prsnum][6] prescribed linear 0-2 and angular 3-5 values at each
+                                step's curtime (input.cpp:4247-4421), or NULL (zeros) */
 } pmx_forcing;
 
 typedef struct pmx_options

@@ -444,6 +444,8 @@ struct orc_ctx
   std::vector<int> part, ellcol;
   std::vector<double> tri;
   std::vector<int> tricol, triobs;
+  int prsnum = 0;
+  std::vector<int> prspart, prskind;          /* [prsnum], [2][prsnum] */
   int obsnum = 0;
   std::vector<int> trirng;                    /* [2*obsnum] */
   std::vector<double> obspnt, obsang, obslin;  /* [3*obsnum] */
@@ -555,6 +557,52 @@ void obstacles (orc_ctx &s, double step)
     x[0] = y[0];
     x[1] = y[1];
     x[2] = y[2];
+  }
+}
+
+/* prescribe_acceleration, input.cpp:4247-4341: before dynamics, a prescribed
+ * acceleration sets f = m a (angular: t = J alpha with the referential
+ * inertia), a prescribed velocity zeroes the force (torque).  row: the
+ * step's [prsnum][6] values (linear 0-2, angular 3-5), zeros when null */
+void prescribe_acceleration (orc_ctx &s, const double *row)
+{
+  size_t N = s.parnum;
+  for (int i = 0; i < s.prsnum; i++)
+  {
+    int j = s.prspart[i], lk = s.prskind[i], ak = s.prskind[s.prsnum + i];
+    double a[3], b[3];
+    for (int d = 0; d < 3; d++) { a[d] = row ? row[6*i+d] : 0.0; b[d] = row ? row[6*i+3+d] : 0.0; }
+    if (lk == 1) for (int d = 0; d < 3; d++) s.force[d*N + j] = s.mass[j] * a[d];
+    else if (lk == 0) for (int d = 0; d < 3; d++) s.force[d*N + j] = 0.0;
+    if (ak == 1)
+    {
+      double J[9], t[3];
+      for (int k = 0; k < 9; k++) J[k] = s.inertia[k*N + j];
+      nvmul (J, b, t);
+      for (int d = 0; d < 3; d++) s.torque[d*N + j] = t[d];
+    }
+    else if (ak == 0) for (int d = 0; d < 3; d++) s.torque[d*N + j] = 0.0;
+  }
+}
+
+/* prescribe_velocity, input.cpp:4344-4421: after dynamics, a prescribed
+ * velocity is set (angular: spatial omega, body Omega = L^T omega with the
+ * updated rotation) */
+void prescribe_velocity (orc_ctx &s, const double *row)
+{
+  size_t N = s.parnum;
+  for (int i = 0; i < s.prsnum; i++)
+  {
+    int j = s.prspart[i], lk = s.prskind[i], ak = s.prskind[s.prsnum + i];
+    if (lk == 0) for (int d = 0; d < 3; d++) s.linear[d*N + j] = row ? row[6*i+d] : 0.0;
+    if (ak == 0)
+    {
+      double o[3], L[9], O[3];
+      for (int d = 0; d < 3; d++) o[d] = row ? row[6*i+3+d] : 0.0;
+      for (int k = 0; k < 9; k++) L[k] = s.rotation[k*N + j];
+      tvmul (L, o, O);
+      for (int d = 0; d < 3; d++) { s.angular[d*N + j] = O[d]; s.angular[(3+d)*N + j] = o[d]; }
+    }
   }
 }
 
@@ -1407,6 +1455,8 @@ int orc_upload (orc_ctx *s, const pmx_model *m)
 {
   if (!s || !m) return fail (PMX_EINVAL, "null argument");
   if (m->obsnum < 0) return fail (PMX_EINVAL, "invalid obstacle count");
+  for (int i = 0; i < m->prsnum; i++)
+    if (m->prspart[i] < 0 || m->prspart[i] >= m->parnum) return fail (PMX_EINVAL, "prescribed particle out of range");
   for (int t = 0; t < m->trinum; t++)
     if (m->triobs[t] <= -2 && -m->triobs[t]-2 >= m->obsnum) return fail (PMX_EINVAL, "triangle obstacle index out of range");
   for (int i = 0; i < m->obsnum; i++)
@@ -1442,6 +1492,9 @@ int orc_upload (orc_ctx *s, const pmx_model *m)
   take (s->tri, m->tri, 9*nt);
   take (s->tricol, m->tricol, nt);
   take (s->triobs, m->triobs, nt);
+  s->prsnum = m->prsnum;
+  take (s->prspart, m->prspart, (size_t) m->prsnum);
+  take (s->prskind, m->prskind, 2*(size_t) m->prsnum);
   s->obsnum = m->obsnum;
   take (s->trirng, m->trirng, 2*(size_t) m->obsnum);
   take (s->obspnt, m->obspnt, 3*(size_t) m->obsnum);
@@ -1548,7 +1601,10 @@ int orc_dem (orc_ctx *s, double duration, double step, const pmx_forcing *fc, do
     if (forces (*s, step0, g)) return fail (PMX_EUNSUPPORTED, "unsupported contact kind");
     if (s->sprnum && springs (*s, step0)) return fail (PMX_EUNSUPPORTED, "unsupported spring kind");
     step1 = step0;
+    const double *prow = fc && fc->prstab ? fc->prstab + 6*(size_t) k * s->prsnum : nullptr;
+    prescribe_acceleration (*s, prow);
     dynamics (*s, damp, step0, step1);
+    prescribe_velocity (*s, prow);
     shapes (*s);
     obstaclev (*s, fc && fc->obstab ? fc->obstab + 6*(size_t) k * s->obsnum : nullptr);   /* at curtime+step0 */
     obstacles (*s, 0.5*(step0+step1));

@@ -15,6 +15,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
 
 static thread_local std::string g_last_error;
 
@@ -65,6 +66,14 @@ int validate_model (const pmx_model *m)
   if (m->parnum < 0 || m->ellnum < 0 || m->trinum < 0 || m->pairnum < 1)
     return fail (PMX_EINVAL, "invalid counts (pairnum must include the default pairing)");
   if (m->obsnum < 0) return fail (PMX_EINVAL, "invalid obstacle count");
+  if (m->prsnum < 0) return fail (PMX_EINVAL, "invalid prescribed-motion count");
+  for (int i = 0; i < m->prsnum; i++)
+  {
+    if (m->prspart[i] < 0 || m->prspart[i] >= m->parnum) return fail (PMX_EINVAL, "prescribed particle out of range");
+    for (int k = 0; k < 2; k++)
+      if (m->prskind[(size_t) k*m->prsnum + i] < -1 || m->prskind[(size_t) k*m->prsnum + i] > 1)
+        return fail (PMX_EINVAL, "prescribed motion kind must be -1, 0 or 1");
+  }
   {
     std::vector<char> moved (std::max (m->trinum, 1), 0);
     for (int i = 0; i < m->obsnum; i++)
@@ -123,7 +132,7 @@ InstOff model_counts (const pmx_model &m)
 {
   InstOff o;
   o.par = m.parnum; o.ell = m.ellnum; o.tri = m.trinum; o.nod = m.nodnum; o.fac = m.facnum; o.spr = m.sprnum;
-  o.scrv = m.sprcrv; o.dcrv = m.dashcrv; o.ucrv = m.uncrv; o.pair = m.pairnum; o.obs = m.obsnum;
+  o.scrv = m.sprcrv; o.dcrv = m.dashcrv; o.ucrv = m.uncrv; o.pair = m.pairnum; o.obs = m.obsnum; o.prs = m.prsnum;
   return o;
 }
 
@@ -202,6 +211,30 @@ int upload_core (pmx_ctx *c, const pmx_model *m, const std::vector<InstOff> &off
   if (!same)
   {
     if ((rc = alloc_rows (*c))) return rc;
+  }
+  /* prescribed motion: entries in order, folded to one per particle (the
+   * force comes from the particle's last linear/angular entry, the velocity
+   * from its last velocity entry, as the reference's sequential loops) */
+  {
+    c->PRS = m->prsnum;
+    std::map<int, int> slot;
+    std::vector<int> pp, lf, lv, af, av;
+    for (int i = 0; i < m->prsnum; i++)
+    {
+      int j = m->prspart[i], lk = m->prskind[i], ak = m->prskind[(size_t) m->prsnum + i];
+      auto it = slot.find (j);
+      int q;
+      if (it == slot.end ()) { q = (int) pp.size (); slot[j] = q; pp.push_back (j); lf.push_back (-1); lv.push_back (-1); af.push_back (-1); av.push_back (-1); }
+      else q = it->second;
+      if (lk >= 0) { lf[q] = 2*i + lk; if (lk == 0) lv[q] = i; }
+      if (ak >= 0) { af[q] = 2*i + ak; if (ak == 0) av[q] = i; }
+    }
+    c->nprs = (int) pp.size ();
+    size_t n = pp.size ();
+    if ((rc = up (*c, c->prs_part, pp.data (), n, "prescribed")) || (rc = up (*c, c->prs_lf, lf.data (), n, "prescribed")) ||
+        (rc = up (*c, c->prs_lv, lv.data (), n, "prescribed")) || (rc = up (*c, c->prs_af, af.data (), n, "prescribed")) ||
+        (rc = up (*c, c->prs_av, av.data (), n, "prescribed")))
+      return rc;
   }
   /* moving obstacles */
   {
@@ -504,6 +537,21 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
     }
     if ((rc = up (*c, c->gtab, g.data (), g.size (), "gravity table")) || (rc = up (*c, c->dtab, d.data (), d.size (), "damping table")))
       return rc;
+    if (c->PRS > 0)
+    {
+      /* prescribed values at each step's curtime, [step][entry][6] */
+      size_t P = c->PRS;
+      std::vector<double> pt (6 * ns * P, 0.0);
+      for (int i = 0; i < I; i++)
+      {
+        const pmx_forcing *f = fc ? fc + (nfc > 1 ? i : 0) : nullptr;
+        int p0 = c->off[i].prs, np = c->off[i+1].prs - p0;
+        if (!f || !f->prstab || np == 0) continue;
+        for (int k = 0; k < nst[i]; k++)
+          std::memcpy (&pt[6*((size_t) k*P + p0)], f->prstab + 6*(size_t) k*np, 6*sizeof (double)*np);
+      }
+      if ((rc = up (*c, c->prstab, pt.data (), pt.size (), "prescribed table"))) return rc;
+    }
     if (c->O > 0)
     {
       /* obstaclev at curtime+step0 of every step, [step][obstacle][6] */

@@ -118,7 +118,7 @@ int merge_models (const pmx_model *ms, int count, Merged &g, std::vector<InstOff
     InstOff n = model_counts (ms[i]), &a = off[i], &b = off[i+1];
     b.par = a.par + n.par; b.ell = a.ell + n.ell; b.tri = a.tri + n.tri; b.nod = a.nod + n.nod;
     b.fac = a.fac + n.fac; b.spr = a.spr + n.spr; b.scrv = a.scrv + n.scrv; b.dcrv = a.dcrv + n.dcrv;
-    b.ucrv = a.ucrv + n.ucrv; b.pair = a.pair + n.pair; b.obs = a.obs + n.obs;
+    b.ucrv = a.ucrv + n.ucrv; b.pair = a.pair + n.pair; b.obs = a.obs + n.obs; b.prs = a.prs + n.prs;
     if ((long long) a.ell + n.ell >= (long long) PMX_TRI_FLAG || (long long) a.tri + n.tri >= (long long) PMX_TRI_FLAG)
       return fail (PMX_EINVAL, "batch too large");
   }
@@ -190,6 +190,16 @@ int merge_models (const pmx_model *ms, int count, Merged &g, std::vector<InstOff
   for (int i = 0; i < count; i++)
     if (ms[i].pairnum) std::memcpy (m.pairs + 2*off[i].pair, ms[i].pairs, 2*sizeof (int)*ms[i].pairnum);
   CATI (ikind, 1, Q, Qo, NQ, id); CATD (iparam, PMX_NIPARAM, Q, Qo, NQ);
+  m.prsnum = tot.prs;
+  m.prspart = g.Iv ((size_t) tot.prs);
+  m.prskind = g.Iv (2 * (size_t) tot.prs);
+  for (int i = 0; i < count; i++)
+    for (int q = 0; q < ms[i].prsnum; q++)
+    {
+      m.prspart[off[i].prs + q] = ms[i].prspart[q] + off[i].par;
+      m.prskind[off[i].prs + q] = ms[i].prskind[q];
+      m.prskind[tot.prs + off[i].prs + q] = ms[i].prskind[ms[i].prsnum + q];
+    }
   m.sprnum = (int) NS;
   if (NS > 0)
   {

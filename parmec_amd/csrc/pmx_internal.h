@@ -88,7 +88,7 @@ struct TreeDev
 /* offsets of one instance's items in the concatenated arrays of a batch */
 struct InstOff
 {
-  int par = 0, ell = 0, tri = 0, nod = 0, fac = 0, spr = 0, scrv = 0, dcrv = 0, ucrv = 0, pair = 0, obs = 0;
+  int par = 0, ell = 0, tri = 0, nod = 0, fac = 0, spr = 0, scrv = 0, dcrv = 0, ucrv = 0, pair = 0, obs = 0, prs = 0;
 };
 
 /* kernels timed by the profiler */
@@ -151,6 +151,11 @@ struct pmx_ctx
   int O = 0, nmtri = 0, mcap = 32;
   dvec<double> obspnt, obsang, obslin, obstab;
   dvec<int> mtri, mtobs, tmov, mtcnt, mtlist;
+  /* prescribed motion: one device entry per particle with the table columns
+   * (of [step][PRS][6]) its force and velocity come from */
+  int PRS = 0, nprs = 0;
+  dvec<int> prs_part, prs_lf, prs_lv, prs_af, prs_av;
+  dvec<double> prstab;
   /* pairing */
   dvec<int> pairs, ikind;
   dvec<double> iparam;
@@ -280,6 +285,8 @@ int step_dynamics (pmx_ctx &c, double step0, double step1);
 int step_end (pmx_ctx &c);
 int enqueue_step (pmx_ctx &c, double h);     /* one whole time step */
 int step_obstacles (pmx_ctx &c, const double *tab, int row, double h);
+int step_prescribe_acc (pmx_ctx &c);           /* before dynamics */
+int step_prescribe_vel (pmx_ctx &c);           /* after dynamics */
 int step_shapes (pmx_ctx &c);
 int step_euler (pmx_ctx &c, double h);
 int step_invert (pmx_ctx &c);

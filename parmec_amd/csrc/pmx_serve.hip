@@ -44,7 +44,7 @@ struct Req
   Merged g;                       /* owned copy of the model; the result is written here */
   double duration = 0.0, step = 0.0;
   int nsteps = 0;
-  std::vector<double> gtab, dtab, obstab, obstab0;
+  std::vector<double> gtab, dtab, obstab, obstab0, prstab;
   pmx_forcing f {};
   size_t bytes = 0;
   double t_submit = 0.0, t_admit = -1.0, t_done = -1.0;
@@ -300,6 +300,8 @@ int pmx_server_submit (pmx_server *s, const pmx_model *m, double duration, doubl
   if (f && f->dtab) { r->dtab.assign (f->dtab, f->dtab + 6*ns); r->f.dtab = r->dtab.data (); }
   if (f && f->obstab && O) { r->obstab.assign (f->obstab, f->obstab + 6*ns*O); r->f.obstab = r->obstab.data (); }
   if (f && f->obstab0 && O) { r->obstab0.assign (f->obstab0, f->obstab0 + 6*O); r->f.obstab0 = r->obstab0.data (); }
+  size_t PR = m->prsnum;
+  if (f && f->prstab && PR) { r->prstab.assign (f->prstab, f->prstab + 6*ns*PR); r->f.prstab = r->prstab.data (); }
   r->bytes = estimate_bytes (c, s->opt.precision, s->opt.row_capacity > 0 ? s->opt.row_capacity : 32);
   {
     std::lock_guard<std::mutex> lk (s->mu);

@@ -718,6 +718,65 @@ __global__ void k_dynamics (int N, double *angular, double *linear, double *rota
 }
 
 /* ellipsoids_task centre update, shapes.ispc:29-67 */
+/* prescribe_acceleration, input.cpp:4247-4341, one thread per prescribed
+ * particle: lf = 2*column + kind of its last linear entry (kind 1:
+ * f = m a, kind 0: f = 0), af the same for the angular part (t = J alpha
+ * with the referential inertia), values from row ctl[KSTEP] of tab */
+__global__ void k_prs_acc (int n, int N, const int *part, const int *lf, const int *af, const double *tab, int P,
+                           const double *mass, const double *inertia, double *force, double *torque, const int *ctl,
+                           const int *stop, const int *par_inst)
+{
+  if (ctl[CTL_ABORT]) return;
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  int j = part[q];
+  if (frozen (stop, par_inst[j], ctl)) return;
+  const double *row = tab + 6*(size_t) ctl[CTL_KSTEP] * P;
+  int l = lf[q], a = af[q];
+  if (l >= 0)
+  {
+    const double *v = row + 6*(l >> 1);
+    double ma = mass[j];
+    for (int d = 0; d < 3; d++) force[(size_t) d*N + j] = (l & 1) ? ma * v[d] : 0.0;
+  }
+  if (a >= 0)
+  {
+    const double *v = row + 6*(a >> 1) + 3;
+    double J[9], t[3] = {0.0, 0.0, 0.0};
+    if (a & 1)
+    {
+      for (int k = 0; k < 9; k++) J[k] = inertia[(size_t) k*N + j];
+      nvmul (J, v, t);
+    }
+    for (int d = 0; d < 3; d++) torque[(size_t) d*N + j] = t[d];
+  }
+}
+
+/* prescribe_velocity, input.cpp:4344-4421: lv / av the column of the
+ * particle's last velocity entry; body Omega = L^T omega with the rotation
+ * dynamics just updated */
+__global__ void k_prs_vel (int n, int N, const int *part, const int *lv, const int *av, const double *tab, int P,
+                           const double *rotation, double *linear, double *angular, const int *ctl, const int *stop,
+                           const int *par_inst)
+{
+  if (ctl[CTL_ABORT]) return;
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  int j = part[q];
+  if (frozen (stop, par_inst[j], ctl)) return;
+  const double *row = tab + 6*(size_t) ctl[CTL_KSTEP] * P;
+  if (lv[q] >= 0)
+    for (int d = 0; d < 3; d++) linear[(size_t) d*N + j] = row[6*lv[q] + d];
+  if (av[q] >= 0)
+  {
+    const double *o = row + 6*av[q] + 3;
+    double L[9], O[3];
+    for (int k = 0; k < 9; k++) L[k] = rotation[(size_t) k*N + j];
+    tvmul (L, o, O);
+    for (int d = 0; d < 3; d++) { angular[(size_t) d*N + j] = O[d]; angular[(size_t) (3+d)*N + j] = o[d]; }
+  }
+}
+
 /* obstacles, obstacles.ispc:28-78, one thread per moving triangle: the
  * rigid motion of its obstacle over the step, rotation expmap(h*omega) about
  * the obstacle point x and translation h*v, with the velocities the
@@ -1022,6 +1081,28 @@ int step_obstacles (pmx_ctx &c, const double *tab, int row, double h)
   return PMX_OK;
 }
 
+int step_prescribe_acc (pmx_ctx &c)
+{
+  if (c.nprs == 0) return PMX_OK;
+  hipLaunchKernelGGL (k_prs_acc, grid (c.nprs), dim3 (PMX_TPB), 0, c.stream, c.nprs, c.N, (const int*) c.prs_part.p,
+                      (const int*) c.prs_lf.p, (const int*) c.prs_af.p, (const double*) c.prstab.p, c.PRS,
+                      (const double*) c.mass.p, (const double*) c.inertia.p, c.force.p, c.torque.p, (const int*) c.ctl.p,
+                      c.staggered ? (const int*) c.istop.p : (const int*) nullptr, (const int*) c.par_inst.p);
+  PMX_HIP (hipGetLastError ());
+  return PMX_OK;
+}
+
+int step_prescribe_vel (pmx_ctx &c)
+{
+  if (c.nprs == 0) return PMX_OK;
+  hipLaunchKernelGGL (k_prs_vel, grid (c.nprs), dim3 (PMX_TPB), 0, c.stream, c.nprs, c.N, (const int*) c.prs_part.p,
+                      (const int*) c.prs_lv.p, (const int*) c.prs_av.p, (const double*) c.prstab.p, c.PRS,
+                      (const double*) c.rotation.p, c.linear.p, c.angular.p, (const int*) c.ctl.p,
+                      c.staggered ? (const int*) c.istop.p : (const int*) nullptr, (const int*) c.par_inst.p);
+  PMX_HIP (hipGetLastError ());
+  return PMX_OK;
+}
+
 int enqueue_step (pmx_ctx &c, double h)
 {
   int rc;
@@ -1031,7 +1112,8 @@ int enqueue_step (pmx_ctx &c, double h)
    * particles never share an instance) and springs carry their own points,
    * so they are formed from the final particle state when an instance
    * finishes, with the same arithmetic and the same result */
-  if ((rc = step_rows (c, h)) || (rc = step_springs (c, h)) || (rc = step_dynamics (c, h, h)) ||
+  if ((rc = step_rows (c, h)) || (rc = step_springs (c, h)) || (rc = step_prescribe_acc (c)) ||
+      (rc = step_dynamics (c, h, h)) || (rc = step_prescribe_vel (c)) ||
       (rc = step_obstacles (c, c.obstab.p, -1, h)) || (rc = step_end (c)))
     return rc;
   return PMX_OK;

@@ -20,9 +20,10 @@ step's curtime (input.cpp:4424-4503) and handed to the device as per-step
 tables.  The step loop itself is libpmx's pmx_dem (C++ host + HIP kernels).
 
 SPRING (1440-1788) is supported without stroke offsets (load-curve
-TSERIES).  Not on this path (raise NotImplementedError): MESH, ANALYTICAL,
-torsion springs, unsprings, joints, moving obstacles, PRESCRIBE, HISTORY,
-OUTPUT, CRITICAL and adaptive time stepping.
+TSERIES); OBSTACLE with point/linear/angular (moving obstacles) and PRESCRIBE
+(2647-2795) tabulate their callables or TSERIES at the step times.  Not on
+this path (raise NotImplementedError): MESH, ANALYTICAL, torsion springs,
+unsprings, joints, HISTORY, OUTPUT, CRITICAL and adaptive time stepping.
 """
 import bisect
 import math
@@ -71,6 +72,7 @@ class _State:
         self.tricol = []
         self.triobs = []
         self.obstacles = []          # (first triangle, end, point, linear callable, angular callable)
+        self.prescribed = []         # (particle, linear, angular, linear kind, angular kind)
         self.pairs = [(0, 0, [0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.5])]   # pair_reset, parmec.cpp:375-388
         self.tms = []
         self.gravity = [0.0, 0.0, 0.0]
@@ -436,7 +438,6 @@ def _not_here(name):
 
 MESH = _not_here("MESH")
 ANALYTICAL = _not_here("ANALYTICAL")
-PRESCRIBE = _not_here("PRESCRIBE")
 HISTORY = _not_here("HISTORY")
 OUTPUT = _not_here("OUTPUT")
 CRITICAL = _not_here("CRITICAL")
@@ -488,6 +489,11 @@ def _build_model():
         m.tri[:] = np.array(_S.tris, dtype=np.float64).T
         m.tricol[:] = np.array(_S.tricol, dtype=np.int32)
         m.triobs[:] = np.array(_S.triobs, dtype=np.int32)
+    m.set_prescribed(len(_S.prescribed))
+    for k, (p, lin, ang, lk, ak) in enumerate(_S.prescribed):
+        m.prspart[k] = p
+        m.prskind[0, k] = lk if lin is not None else -1
+        m.prskind[1, k] = ak if ang is not None else -1
     m.set_obstacles(len(_S.obstacles))
     for k, (a, b, pnt, lin, ang) in enumerate(_S.obstacles):
         m.trirng[k] = (a, b)
@@ -505,6 +511,42 @@ def _build_model():
         m.ikind[i] = pmx.GRANULAR_FORCE
         m.iparam[:, i] = par
     return m
+
+
+def _history(x, name):
+    """A PRESCRIBE history: a callable of time or a tuple of 3 TSERIES numbers."""
+    if x is None:
+        return None
+    if callable(x):
+        return x
+    if isinstance(x, tuple):
+        if len(x) != 3:
+            raise ValueError(f"'{name}' tuple size != 3")
+        ids = [int(v) for v in x]
+        if any(i < 0 or i >= len(_S.tms) for i in ids):
+            raise ValueError(f"'{name}' time series number out of range")
+        series = [_S.tms[i] for i in ids]
+        return lambda t: tuple(s(t) for s in series)
+    raise ValueError(f"'{name}' is neither a tuple nor a callback")
+
+
+def PRESCRIBE(parnum, linear=None, angular=None, kind="vv"):
+    """Prescribe the linear and angular velocity ('v') or acceleration ('a')
+    of a particle (input.cpp:2647-2795); kind is 'vv', 'va', 'av' or 'aa'
+    (linear, angular).  A history is a callable of time returning a 3-tuple,
+    or a tuple of 3 TSERIES numbers."""
+    n = len(_S.sph)
+    if not isinstance(parnum, int) or parnum < 0 or parnum >= n:
+        raise ValueError("'parnum' must be >= 0 and < the number of particles")
+    if not isinstance(kind, str):
+        raise TypeError("'kind' must be a string")
+    if kind not in ("vv", "va", "av", "aa"):
+        raise ValueError("Invalid time history kind")
+    lin, ang = _history(linear, "linear"), _history(angular, "angular")
+    if lin is None and ang is None:
+        return
+    _S.prescribed.append((parnum, lin, ang, 0 if kind[0] == "v" else 1, 0 if kind[1] == "v" else 1))
+    _S.dirty = True
 
 
 def DEM(duration, step, interval=None, prefix=None, adaptive=0.0):
@@ -560,7 +602,18 @@ def DEM(duration, step, interval=None, prefix=None, adaptive=0.0):
         obstab = np.array([vel(t + h) for t in times]).reshape(len(times), len(_S.obstacles), 6)
         if m.curtime == 0.0:
             obstab0 = vel(0.5 * h)
-    f = pmx.forcing(len(times), gravity=tuple(_S.gravity), gtab=gtab, dtab=dtab, obstab=obstab, obstab0=obstab0)
+    prstab = None
+    if _S.prescribed:
+        # prescribe_acceleration / prescribe_velocity at each step's curtime
+        prstab = np.zeros((len(times), len(_S.prescribed), 6))
+        for k, (_, lin, ang, _, _) in enumerate(_S.prescribed):
+            for i, t in enumerate(times):
+                if lin is not None:
+                    prstab[i, k, :3] = [float(v) for v in lin(t)]
+                if ang is not None:
+                    prstab[i, k, 3:] = [float(v) for v in ang(t)]
+    f = pmx.forcing(len(times), gravity=tuple(_S.gravity), gtab=gtab, dtab=dtab, obstab=obstab, obstab0=obstab0,
+                    prstab=prstab)
     elapsed = _S.engine.dem(float(duration), float(step), f)
     _S.engine.download(m)
     return elapsed

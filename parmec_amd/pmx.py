@@ -40,13 +40,14 @@ class pmx_model(C.Structure):
         ("unload", _dp), ("uncrv", C.c_int), ("yield_", _dp), ("sprdir", _dp), ("sprflg", _ip), ("sprfric", _dp),
         ("sprkskn", _dp), ("sprsdsp", _dp), ("stroke0", _dp), ("stroke", _dp), ("sprfrc", _dp), ("unspring", _ip),
         ("sproffset", _ip),
+        ("prsnum", C.c_int), ("prspart", _ip), ("prskind", _ip),
         ("curtime", C.c_double), ("curstep", C.c_double), ("stepnum", C.c_int),
     ]
 
 
 class pmx_forcing(C.Structure):
     _fields_ = [("gravity", C.c_double * 3), ("gtab", _dp), ("damping", C.c_double * 6), ("dtab", _dp),
-                ("obstab", _dp), ("obstab0", _dp)]
+                ("obstab", _dp), ("obstab0", _dp), ("prstab", _dp)]
 
 
 class pmx_options(C.Structure):
@@ -179,6 +180,15 @@ class Model:
         self.set_mesh(0, 0)
         self.set_springs(0, 0, 0, 0)
         self.set_obstacles(0)
+        self.set_prescribed(0)
+
+    def set_prescribed(self, prsnum):
+        """Prescribed motion entries (parmec.h:244-251): particle, and the kind
+        of the linear and angular parts [2][prsnum] (-1 none, 0 velocity,
+        1 acceleration)."""
+        self.prsnum = prsnum
+        self.prspart = np.zeros(prsnum, dtype=np.int32)
+        self.prskind = np.full((2, prsnum), -1, dtype=np.int32)
 
     def set_obstacles(self, obsnum):
         """Moving obstacles (parmec.h:104-112): triangle ranges [obsnum][2],
@@ -233,7 +243,7 @@ class Model:
             "center", "radii", "tri", "iparam", "nodes", "sprpnt", "spring", "dashpot", "unload", "yield_", "sprdir",
             "sprfric", "sprkskn", "sprsdsp", "stroke0", "stroke", "sprfrc", "obspnt", "obslin", "obsang")
     _INT = ("flags", "part", "ellcol", "tricol", "triobs", "pairs", "ikind", "nodpart", "facnod", "factri", "sprtype",
-            "sprpart", "spridx", "dashidx", "unidx", "sprflg", "unspring", "sproffset", "trirng")
+            "sprpart", "spridx", "dashidx", "unidx", "sprflg", "unspring", "sproffset", "trirng", "prspart", "prskind")
 
     def check(self):
         for name in self._DBL:
@@ -251,6 +261,7 @@ class Model:
         s.parnum, s.ellnum, s.trinum, s.pairnum = self.parnum, self.ellnum, self.trinum, self.pairnum
         s.nodnum, s.facnum, s.sprnum = self.nodnum, self.facnum, self.sprnum
         s.obsnum = getattr(self, "obsnum", 0)
+        s.prsnum = getattr(self, "prsnum", 0)
         s.sprcrv, s.dashcrv, s.uncrv = self.sprcrv, self.dashcrv, self.uncrv
         for name in self._DBL:
             setattr(s, name, _ptr(getattr(self, name), "d"))
@@ -264,7 +275,8 @@ class Model:
         self.curtime, self.curstep, self.stepnum = s.curtime, s.curstep, s.stepnum
 
 
-def forcing(nsteps=0, gravity=(0.0, 0.0, 0.0), gtab=None, damping=(0.0,) * 6, dtab=None, obstab=None, obstab0=None):
+def forcing(nsteps=0, gravity=(0.0, 0.0, 0.0), gtab=None, damping=(0.0,) * 6, dtab=None, obstab=None, obstab0=None,
+            prstab=None):
     """Per-step inputs of dem(): gravity and damping constants or tables
     ([nsteps][3], [nsteps][6]); moving obstacle velocities (angular 0-2,
     linear 3-5) per step at curtime+step ([nsteps][obsnum][6]) and at step/2
@@ -294,6 +306,11 @@ def forcing(nsteps=0, gravity=(0.0, 0.0, 0.0), gtab=None, damping=(0.0,) * 6, dt
         obstab0 = np.ascontiguousarray(obstab0, dtype=np.float64)
         f.obstab0 = obstab0.ctypes.data_as(_dp)
         keep.append(obstab0)
+    if prstab is not None:
+        prstab = np.ascontiguousarray(prstab, dtype=np.float64)
+        assert prstab.shape[0] >= nsteps and prstab.shape[-1] == 6
+        f.prstab = prstab.ctypes.data_as(_dp)
+        keep.append(prstab)
     f._keep = keep
     return f
 

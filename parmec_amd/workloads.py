@@ -163,7 +163,7 @@ def _cuboid_faces():
 
 
 def spring_lattice(nx=40, ny=40, nz=20, size=(0.2, 0.2, 0.4), gap=0.01, density=1800.0, seed=0,
-                   stiffness=(1e5, 1e7), curve_step=0.005, damping_ratio=0.05, anchors=True):
+                   stiffness=(1e5, 1e7), curve_step=0.005, damping_ratio=0.05, anchors=True, prescribed_base=False):
     """Lattice of meshed cuboid bodies joined to their 6 neighbours by SPRING
     elements with random 5-point piecewise-linear force-stroke curves.
 
@@ -174,7 +174,10 @@ def spring_lattice(nx=40, ny=40, nz=20, size=(0.2, 0.2, 0.4), gap=0.01, density=
     `stiffness` (seeded); dashpots use a critical damping ratio.  With
     anchors, every bottom body also hangs on a spring to a fixed point
     `gap` below its bottom face (part2 = -1), so a base excitation can be
-    applied in the base frame as a time-dependent gravity.
+    applied in the base frame as a time-dependent gravity.  With
+    prescribed_base, the bottom layer instead gets PRESCRIBE entries of kind
+    'av' (linear acceleration from base_prescription, angular velocity zero),
+    the survey's drive of this configuration.
     """
     rng = np.random.default_rng(seed)
     a, b, c = size
@@ -256,7 +259,23 @@ def spring_lattice(nx=40, ny=40, nz=20, size=(0.2, 0.2, 0.4), gap=0.01, density=
     m.sprflg[:] = pmx.SPRDIR_FOLLOWER
     dif = x2 - x1
     m.stroke0[:] = np.sqrt((dif * dif).sum(axis=1))
+    if prescribed_base:
+        bot = np.nonzero(kk == 0)[0]
+        m.set_prescribed(len(bot))
+        m.prspart[:] = bot
+        m.prskind[0] = 1      # linear: acceleration
+        m.prskind[1] = 0      # angular: velocity
     return m
+
+
+def base_prescription(times, prsnum, amplitude=9.81, frequency=5.0):
+    """Per-step table [nsteps][prsnum][6] of a base layer driven with
+    horizontal acceleration a(t) = amplitude sin(2 pi f t) and zero angular
+    velocity (PRESCRIBE kind 'av', input.cpp:4247-4421)."""
+    t = np.asarray(times)
+    tab = np.zeros((len(t), prsnum, 6))
+    tab[:, :, 0] = (amplitude * np.sin(2.0 * math.pi * frequency * t))[:, None]
+    return tab
 
 
 def base_excitation(times, amplitude=9.81, frequency=5.0, g=9.81):

@@ -48,3 +48,31 @@ def test_spring_lattice(dims, steps):
     assert sg["steps"] == steps
     compare(mg, cg, mc, cc)
     _compare_springs(mg, mc)
+
+
+@pytest.mark.parametrize("dims,steps,calls", [((6, 5, 4), 400, 2), ((40, 40, 20), 150, 1)])
+def test_spring_lattice_prescribed_base(dims, steps, calls):
+    """The spring-lattice configuration driven as proposed: bottom layer under
+    PRESCRIBE kind 'av' (horizontal base acceleration 1 g at 5 Hz, zero
+    angular velocity), gravity on; over one or two dem calls."""
+    from oracle import orc
+    print(f"case prescribed-base lattice {dims} steps={steps} calls={calls}", flush=True)
+    m = workloads.spring_lattice(*dims, anchors=False, prescribed_base=True, seed=2)
+    out = []
+    for eng in (pmx.Engine(), orc.Oracle(0)):
+        mm = m.copy()
+        eng.upload(mm)
+        for call in range(calls):
+            times = pmx.step_times(steps * 1e-5 - 0.5e-5, 1e-5, mm.curtime)
+            f = pmx.forcing(len(times), gravity=(0.0, 0.0, -9.81),
+                            prstab=workloads.base_prescription(times, mm.prsnum))
+            eng.dem(steps * 1e-5 - 0.5e-5, 1e-5, f)
+            eng.download(mm)
+        out.append((mm, eng.contacts()))
+        eng.close()
+    (mg, cg), (mc, cc) = out
+    compare(mg, cg, mc, cc)
+    _compare_springs(mg, mc)
+    bot = mc.prspart
+    assert np.abs(mc.linear[0, bot]).max() > 0.0      # the base moved
+    np.testing.assert_array_equal(mc.angular[3:, bot], 0.0)
