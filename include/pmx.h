@@ -132,6 +132,8 @@ typedef struct pmx_model
   int prsnum;
   int *prspart;      /* [prsnum] */
   int *prskind;      /* [2][prsnum]: linear kind, angular kind */
+  int *prscol;       /* [prsnum] column of the entry's history in pmx_forcing.prstab (entries driven by one
+                        history share it), or NULL: column = entry index */
 
   /* time bookkeeping, parmec.h:44-49 */
   double curtime;
@@ -153,8 +155,9 @@ typedef struct pmx_forcing
   const double *dtab;        /* [nsteps][6] damping per step, or NULL */
   const double *obstab;      /* [nsteps][obsnum][6] obstacle (angular, linear) velocity at curtime+h, or NULL */
   const double *obstab0;     /* [obsnum][6] obstacle velocity at t = h/2 (first call only), or NULL */
-  const double *prstab;      /* [nsteps][prsnum][6] prescribed linear 0-2 and angular 3-5 values at each
-                                step's curtime (input.cpp:4247-4421), or NULL (zeros) */
+  const double *prstab;      /* [nsteps][prscols][6] prescribed linear 0-2 and angular 3-5 values of each
+                                history column at each step's curtime (input.cpp:4247-4421), or NULL (zeros) */
+  int prscols;               /* columns per step of prstab (0: the model's, max prscol + 1 or prsnum) */
 } pmx_forcing;
 
 typedef struct pmx_options

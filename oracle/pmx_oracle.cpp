@@ -445,7 +445,8 @@ struct orc_ctx
   std::vector<double> tri;
   std::vector<int> tricol, triobs;
   int prsnum = 0;
-  std::vector<int> prspart, prskind;          /* [prsnum], [2][prsnum] */
+  std::vector<int> prspart, prskind, prscol;  /* [prsnum], [2][prsnum], [prsnum] */
+  int prscols = 0;                            /* history columns */
   int obsnum = 0;
   std::vector<int> trirng;                    /* [2*obsnum] */
   std::vector<double> obspnt, obsang, obslin;  /* [3*obsnum] */
@@ -571,7 +572,8 @@ void prescribe_acceleration (orc_ctx &s, const double *row)
   {
     int j = s.prspart[i], lk = s.prskind[i], ak = s.prskind[s.prsnum + i];
     double a[3], b[3];
-    for (int d = 0; d < 3; d++) { a[d] = row ? row[6*i+d] : 0.0; b[d] = row ? row[6*i+3+d] : 0.0; }
+    int c = s.prscol[i];
+    for (int d = 0; d < 3; d++) { a[d] = row ? row[6*c+d] : 0.0; b[d] = row ? row[6*c+3+d] : 0.0; }
     if (lk == 1) for (int d = 0; d < 3; d++) s.force[d*N + j] = s.mass[j] * a[d];
     else if (lk == 0) for (int d = 0; d < 3; d++) s.force[d*N + j] = 0.0;
     if (ak == 1)
@@ -594,11 +596,12 @@ void prescribe_velocity (orc_ctx &s, const double *row)
   for (int i = 0; i < s.prsnum; i++)
   {
     int j = s.prspart[i], lk = s.prskind[i], ak = s.prskind[s.prsnum + i];
-    if (lk == 0) for (int d = 0; d < 3; d++) s.linear[d*N + j] = row ? row[6*i+d] : 0.0;
+    int c = s.prscol[i];
+    if (lk == 0) for (int d = 0; d < 3; d++) s.linear[d*N + j] = row ? row[6*c+d] : 0.0;
     if (ak == 0)
     {
       double o[3], L[9], O[3];
-      for (int d = 0; d < 3; d++) o[d] = row ? row[6*i+3+d] : 0.0;
+      for (int d = 0; d < 3; d++) o[d] = row ? row[6*c+3+d] : 0.0;
       for (int k = 0; k < 9; k++) L[k] = s.rotation[k*N + j];
       tvmul (L, o, O);
       for (int d = 0; d < 3; d++) { s.angular[d*N + j] = O[d]; s.angular[(3+d)*N + j] = o[d]; }
@@ -1495,6 +1498,13 @@ int orc_upload (orc_ctx *s, const pmx_model *m)
   s->prsnum = m->prsnum;
   take (s->prspart, m->prspart, (size_t) m->prsnum);
   take (s->prskind, m->prskind, 2*(size_t) m->prsnum);
+  s->prscol.resize (m->prsnum);
+  s->prscols = 0;
+  for (int i = 0; i < m->prsnum; i++)
+  {
+    s->prscol[i] = m->prscol ? m->prscol[i] : i;
+    s->prscols = std::max (s->prscols, s->prscol[i] + 1);
+  }
   s->obsnum = m->obsnum;
   take (s->trirng, m->trirng, 2*(size_t) m->obsnum);
   take (s->obspnt, m->obspnt, 3*(size_t) m->obsnum);
@@ -1601,7 +1611,7 @@ int orc_dem (orc_ctx *s, double duration, double step, const pmx_forcing *fc, do
     if (forces (*s, step0, g)) return fail (PMX_EUNSUPPORTED, "unsupported contact kind");
     if (s->sprnum && springs (*s, step0)) return fail (PMX_EUNSUPPORTED, "unsupported spring kind");
     step1 = step0;
-    const double *prow = fc && fc->prstab ? fc->prstab + 6*(size_t) k * s->prsnum : nullptr;
+    const double *prow = fc && fc->prstab ? fc->prstab + 6*(size_t) k * (fc->prscols > 0 ? fc->prscols : s->prscols) : nullptr;
     prescribe_acceleration (*s, prow);
     dynamics (*s, damp, step0, step1);
     prescribe_velocity (*s, prow);

@@ -73,6 +73,7 @@ int validate_model (const pmx_model *m)
     for (int k = 0; k < 2; k++)
       if (m->prskind[(size_t) k*m->prsnum + i] < -1 || m->prskind[(size_t) k*m->prsnum + i] > 1)
         return fail (PMX_EINVAL, "prescribed motion kind must be -1, 0 or 1");
+    if (m->prscol && m->prscol[i] < 0) return fail (PMX_EINVAL, "prescribed history column out of range");
   }
   {
     std::vector<char> moved (std::max (m->trinum, 1), 0);
@@ -133,6 +134,12 @@ InstOff model_counts (const pmx_model &m)
   InstOff o;
   o.par = m.parnum; o.ell = m.ellnum; o.tri = m.trinum; o.nod = m.nodnum; o.fac = m.facnum; o.spr = m.sprnum;
   o.scrv = m.sprcrv; o.dcrv = m.dashcrv; o.ucrv = m.uncrv; o.pair = m.pairnum; o.obs = m.obsnum; o.prs = m.prsnum;
+  o.pcol = m.prsnum;
+  if (m.prscol)
+  {
+    o.pcol = 0;
+    for (int i = 0; i < m.prsnum; i++) o.pcol = std::max (o.pcol, m.prscol[i] + 1);
+  }
   return o;
 }
 
@@ -217,17 +224,19 @@ int upload_core (pmx_ctx *c, const pmx_model *m, const std::vector<InstOff> &off
    * from its last velocity entry, as the reference's sequential loops) */
   {
     c->PRS = m->prsnum;
+    c->PRSC = off[I].pcol;
     std::map<int, int> slot;
     std::vector<int> pp, lf, lv, af, av;
     for (int i = 0; i < m->prsnum; i++)
     {
       int j = m->prspart[i], lk = m->prskind[i], ak = m->prskind[(size_t) m->prsnum + i];
+      int col = m->prscol ? m->prscol[i] : i;
       auto it = slot.find (j);
       int q;
       if (it == slot.end ()) { q = (int) pp.size (); slot[j] = q; pp.push_back (j); lf.push_back (-1); lv.push_back (-1); af.push_back (-1); av.push_back (-1); }
       else q = it->second;
-      if (lk >= 0) { lf[q] = 2*i + lk; if (lk == 0) lv[q] = i; }
-      if (ak >= 0) { af[q] = 2*i + ak; if (ak == 0) av[q] = i; }
+      if (lk >= 0) { lf[q] = 2*col + lk; if (lk == 0) lv[q] = col; }
+      if (ak >= 0) { af[q] = 2*col + ak; if (ak == 0) av[q] = col; }
     }
     c->nprs = (int) pp.size ();
     size_t n = pp.size ();
@@ -537,18 +546,21 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
     }
     if ((rc = up (*c, c->gtab, g.data (), g.size (), "gravity table")) || (rc = up (*c, c->dtab, d.data (), d.size (), "damping table")))
       return rc;
-    if (c->PRS > 0)
+    if (c->PRSC > 0)
     {
-      /* prescribed values at each step's curtime, [step][entry][6] */
-      size_t P = c->PRS;
+      /* prescribed values of each history column at each step's curtime,
+       * [step][column][6] */
+      size_t P = c->PRSC;
       std::vector<double> pt (6 * ns * P, 0.0);
       for (int i = 0; i < I; i++)
       {
         const pmx_forcing *f = fc ? fc + (nfc > 1 ? i : 0) : nullptr;
-        int p0 = c->off[i].prs, np = c->off[i+1].prs - p0;
+        int p0 = c->off[i].pcol, np = c->off[i+1].pcol - p0;
         if (!f || !f->prstab || np == 0) continue;
+        int stride = f->prscols > 0 ? f->prscols : np;
+        if (stride < np) return fail (PMX_EINVAL, "prescribed table has fewer columns than the model's histories");
         for (int k = 0; k < nst[i]; k++)
-          std::memcpy (&pt[6*((size_t) k*P + p0)], f->prstab + 6*(size_t) k*np, 6*sizeof (double)*np);
+          std::memcpy (&pt[6*((size_t) k*P + p0)], f->prstab + 6*(size_t) k*stride, 6*sizeof (double)*np);
       }
       if ((rc = up (*c, c->prstab, pt.data (), pt.size (), "prescribed table"))) return rc;
     }

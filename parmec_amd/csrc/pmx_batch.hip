@@ -119,6 +119,7 @@ int merge_models (const pmx_model *ms, int count, Merged &g, std::vector<InstOff
     b.par = a.par + n.par; b.ell = a.ell + n.ell; b.tri = a.tri + n.tri; b.nod = a.nod + n.nod;
     b.fac = a.fac + n.fac; b.spr = a.spr + n.spr; b.scrv = a.scrv + n.scrv; b.dcrv = a.dcrv + n.dcrv;
     b.ucrv = a.ucrv + n.ucrv; b.pair = a.pair + n.pair; b.obs = a.obs + n.obs; b.prs = a.prs + n.prs;
+    b.pcol = a.pcol + n.pcol;
     if ((long long) a.ell + n.ell >= (long long) PMX_TRI_FLAG || (long long) a.tri + n.tri >= (long long) PMX_TRI_FLAG)
       return fail (PMX_EINVAL, "batch too large");
   }
@@ -193,12 +194,14 @@ int merge_models (const pmx_model *ms, int count, Merged &g, std::vector<InstOff
   m.prsnum = tot.prs;
   m.prspart = g.Iv ((size_t) tot.prs);
   m.prskind = g.Iv (2 * (size_t) tot.prs);
+  m.prscol = g.Iv ((size_t) tot.prs);
   for (int i = 0; i < count; i++)
     for (int q = 0; q < ms[i].prsnum; q++)
     {
       m.prspart[off[i].prs + q] = ms[i].prspart[q] + off[i].par;
       m.prskind[off[i].prs + q] = ms[i].prskind[q];
       m.prskind[tot.prs + off[i].prs + q] = ms[i].prskind[ms[i].prsnum + q];
+      m.prscol[off[i].prs + q] = (ms[i].prscol ? ms[i].prscol[q] : q) + off[i].pcol;
     }
   m.sprnum = (int) NS;
   if (NS > 0)

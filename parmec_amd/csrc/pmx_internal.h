@@ -88,7 +88,7 @@ struct TreeDev
 /* offsets of one instance's items in the concatenated arrays of a batch */
 struct InstOff
 {
-  int par = 0, ell = 0, tri = 0, nod = 0, fac = 0, spr = 0, scrv = 0, dcrv = 0, ucrv = 0, pair = 0, obs = 0, prs = 0;
+  int par = 0, ell = 0, tri = 0, nod = 0, fac = 0, spr = 0, scrv = 0, dcrv = 0, ucrv = 0, pair = 0, obs = 0, prs = 0, pcol = 0;
 };
 
 /* kernels timed by the profiler */
@@ -153,7 +153,7 @@ struct pmx_ctx
   dvec<int> mtri, mtobs, tmov, mtcnt, mtlist;
   /* prescribed motion: one device entry per particle with the table columns
    * (of [step][PRS][6]) its force and velocity come from */
-  int PRS = 0, nprs = 0;
+  int PRS = 0, PRSC = 0, nprs = 0;   /* entries, history columns, device entries */
   dvec<int> prs_part, prs_lf, prs_lv, prs_af, prs_av;
   dvec<double> prstab;
   /* pairing */

@@ -300,8 +300,14 @@ int pmx_server_submit (pmx_server *s, const pmx_model *m, double duration, doubl
   if (f && f->dtab) { r->dtab.assign (f->dtab, f->dtab + 6*ns); r->f.dtab = r->dtab.data (); }
   if (f && f->obstab && O) { r->obstab.assign (f->obstab, f->obstab + 6*ns*O); r->f.obstab = r->obstab.data (); }
   if (f && f->obstab0 && O) { r->obstab0.assign (f->obstab0, f->obstab0 + 6*O); r->f.obstab0 = r->obstab0.data (); }
-  size_t PR = m->prsnum;
-  if (f && f->prstab && PR) { r->prstab.assign (f->prstab, f->prstab + 6*ns*PR); r->f.prstab = r->prstab.data (); }
+  size_t PR = off[1].pcol;
+  if (f && f->prstab && PR)
+  {
+    size_t stride = f->prscols > 0 ? (size_t) f->prscols : PR;
+    r->prstab.assign (f->prstab, f->prstab + 6*ns*stride);
+    r->f.prstab = r->prstab.data ();
+    r->f.prscols = (int) stride;
+  }
   r->bytes = estimate_bytes (c, s->opt.precision, s->opt.row_capacity > 0 ? s->opt.row_capacity : 32);
   {
     std::lock_guard<std::mutex> lk (s->mu);

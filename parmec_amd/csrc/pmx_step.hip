@@ -1085,7 +1085,7 @@ int step_prescribe_acc (pmx_ctx &c)
 {
   if (c.nprs == 0) return PMX_OK;
   hipLaunchKernelGGL (k_prs_acc, grid (c.nprs), dim3 (PMX_TPB), 0, c.stream, c.nprs, c.N, (const int*) c.prs_part.p,
-                      (const int*) c.prs_lf.p, (const int*) c.prs_af.p, (const double*) c.prstab.p, c.PRS,
+                      (const int*) c.prs_lf.p, (const int*) c.prs_af.p, (const double*) c.prstab.p, c.PRSC,
                       (const double*) c.mass.p, (const double*) c.inertia.p, c.force.p, c.torque.p, (const int*) c.ctl.p,
                       c.staggered ? (const int*) c.istop.p : (const int*) nullptr, (const int*) c.par_inst.p);
   PMX_HIP (hipGetLastError ());
@@ -1096,7 +1096,7 @@ int step_prescribe_vel (pmx_ctx &c)
 {
   if (c.nprs == 0) return PMX_OK;
   hipLaunchKernelGGL (k_prs_vel, grid (c.nprs), dim3 (PMX_TPB), 0, c.stream, c.nprs, c.N, (const int*) c.prs_part.p,
-                      (const int*) c.prs_lv.p, (const int*) c.prs_av.p, (const double*) c.prstab.p, c.PRS,
+                      (const int*) c.prs_lv.p, (const int*) c.prs_av.p, (const double*) c.prstab.p, c.PRSC,
                       (const double*) c.rotation.p, c.linear.p, c.angular.p, (const int*) c.ctl.p,
                       c.staggered ? (const int*) c.istop.p : (const int*) nullptr, (const int*) c.par_inst.p);
   PMX_HIP (hipGetLastError ());

@@ -490,10 +490,12 @@ def _build_model():
         m.tricol[:] = np.array(_S.tricol, dtype=np.int32)
         m.triobs[:] = np.array(_S.triobs, dtype=np.int32)
     m.set_prescribed(len(_S.prescribed))
+    cols = {}
     for k, (p, lin, ang, lk, ak) in enumerate(_S.prescribed):
         m.prspart[k] = p
         m.prskind[0, k] = lk if lin is not None else -1
         m.prskind[1, k] = ak if ang is not None else -1
+        m.prscol[k] = cols.setdefault((id(lin), id(ang)), len(cols))   # entries sharing histories share a column
     m.set_obstacles(len(_S.obstacles))
     for k, (a, b, pnt, lin, ang) in enumerate(_S.obstacles):
         m.trirng[k] = (a, b)
@@ -604,14 +606,21 @@ def DEM(duration, step, interval=None, prefix=None, adaptive=0.0):
             obstab0 = vel(0.5 * h)
     prstab = None
     if _S.prescribed:
-        # prescribe_acceleration / prescribe_velocity at each step's curtime
-        prstab = np.zeros((len(times), len(_S.prescribed), 6))
+        # prescribe_acceleration / prescribe_velocity at each step's curtime,
+        # one column per distinct pair of histories
+        ncol = int(m.prscol.max()) + 1
+        prstab = np.zeros((len(times), ncol, 6))
+        done = set()
         for k, (_, lin, ang, _, _) in enumerate(_S.prescribed):
+            c = int(m.prscol[k])
+            if c in done:
+                continue
+            done.add(c)
             for i, t in enumerate(times):
                 if lin is not None:
-                    prstab[i, k, :3] = [float(v) for v in lin(t)]
+                    prstab[i, c, :3] = [float(v) for v in lin(t)]
                 if ang is not None:
-                    prstab[i, k, 3:] = [float(v) for v in ang(t)]
+                    prstab[i, c, 3:] = [float(v) for v in ang(t)]
     f = pmx.forcing(len(times), gravity=tuple(_S.gravity), gtab=gtab, dtab=dtab, obstab=obstab, obstab0=obstab0,
                     prstab=prstab)
     elapsed = _S.engine.dem(float(duration), float(step), f)

@@ -40,14 +40,14 @@ class pmx_model(C.Structure):
         ("unload", _dp), ("uncrv", C.c_int), ("yield_", _dp), ("sprdir", _dp), ("sprflg", _ip), ("sprfric", _dp),
         ("sprkskn", _dp), ("sprsdsp", _dp), ("stroke0", _dp), ("stroke", _dp), ("sprfrc", _dp), ("unspring", _ip),
         ("sproffset", _ip),
-        ("prsnum", C.c_int), ("prspart", _ip), ("prskind", _ip),
+        ("prsnum", C.c_int), ("prspart", _ip), ("prskind", _ip), ("prscol", _ip),
         ("curtime", C.c_double), ("curstep", C.c_double), ("stepnum", C.c_int),
     ]
 
 
 class pmx_forcing(C.Structure):
     _fields_ = [("gravity", C.c_double * 3), ("gtab", _dp), ("damping", C.c_double * 6), ("dtab", _dp),
-                ("obstab", _dp), ("obstab0", _dp), ("prstab", _dp)]
+                ("obstab", _dp), ("obstab0", _dp), ("prstab", _dp), ("prscols", C.c_int)]
 
 
 class pmx_options(C.Structure):
@@ -189,6 +189,7 @@ class Model:
         self.prsnum = prsnum
         self.prspart = np.zeros(prsnum, dtype=np.int32)
         self.prskind = np.full((2, prsnum), -1, dtype=np.int32)
+        self.prscol = np.arange(prsnum, dtype=np.int32)
 
     def set_obstacles(self, obsnum):
         """Moving obstacles (parmec.h:104-112): triangle ranges [obsnum][2],
@@ -243,7 +244,7 @@ class Model:
             "center", "radii", "tri", "iparam", "nodes", "sprpnt", "spring", "dashpot", "unload", "yield_", "sprdir",
             "sprfric", "sprkskn", "sprsdsp", "stroke0", "stroke", "sprfrc", "obspnt", "obslin", "obsang")
     _INT = ("flags", "part", "ellcol", "tricol", "triobs", "pairs", "ikind", "nodpart", "facnod", "factri", "sprtype",
-            "sprpart", "spridx", "dashidx", "unidx", "sprflg", "unspring", "sproffset", "trirng", "prspart", "prskind")
+            "sprpart", "spridx", "dashidx", "unidx", "sprflg", "unspring", "sproffset", "trirng", "prspart", "prskind", "prscol")
 
     def check(self):
         for name in self._DBL:
@@ -308,8 +309,9 @@ def forcing(nsteps=0, gravity=(0.0, 0.0, 0.0), gtab=None, damping=(0.0,) * 6, dt
         keep.append(obstab0)
     if prstab is not None:
         prstab = np.ascontiguousarray(prstab, dtype=np.float64)
-        assert prstab.shape[0] >= nsteps and prstab.shape[-1] == 6
+        assert prstab.ndim == 3 and prstab.shape[0] >= nsteps and prstab.shape[-1] == 6
         f.prstab = prstab.ctypes.data_as(_dp)
+        f.prscols = prstab.shape[1]
         keep.append(prstab)
     f._keep = keep
     return f

@@ -265,16 +265,17 @@ def spring_lattice(nx=40, ny=40, nz=20, size=(0.2, 0.2, 0.4), gap=0.01, density=
         m.prspart[:] = bot
         m.prskind[0] = 1      # linear: acceleration
         m.prskind[1] = 0      # angular: velocity
+        m.prscol[:] = 0       # one history drives the whole base
     return m
 
 
-def base_prescription(times, prsnum, amplitude=9.81, frequency=5.0):
-    """Per-step table [nsteps][prsnum][6] of a base layer driven with
+def base_prescription(times, amplitude=9.81, frequency=5.0):
+    """Per-step table [nsteps][1][6] of one history driving a base layer:
     horizontal acceleration a(t) = amplitude sin(2 pi f t) and zero angular
     velocity (PRESCRIBE kind 'av', input.cpp:4247-4421)."""
     t = np.asarray(times)
-    tab = np.zeros((len(t), prsnum, 6))
-    tab[:, :, 0] = (amplitude * np.sin(2.0 * math.pi * frequency * t))[:, None]
+    tab = np.zeros((len(t), 1, 6))
+    tab[:, 0, 0] = amplitude * np.sin(2.0 * math.pi * frequency * t)
     return tab
 
 
@@ -364,11 +365,14 @@ def serve_request(i, seed=0, median=20000, sigma=0.8, nmin=2000, nmax=500000, sm
     freq = float(rng.uniform(1.0, 20.0))
     if rng.uniform() < spring_share:
         a = max(2, int(round((n / 2.0) ** (1.0 / 3.0))))
-        m = spring_lattice(nx=a, ny=a, nz=max(1, n // (a * a)), seed=int(rng.integers(1 << 30)))
+        m = spring_lattice(nx=a, ny=a, nz=max(1, n // (a * a)), seed=int(rng.integers(1 << 30)), anchors=False,
+                           prescribed_base=True)
+        times = step * np.arange(nsteps)
+        f = pmx.forcing(nsteps, gravity=(0.0, 0.0, -9.81), prstab=base_prescription(times, amp, freq))
     else:
         m = box_lattice(n=n, seed=int(rng.integers(1 << 30)))
-    g = sweep_forcing(amp, freq, nsteps, step)
-    return m, (nsteps - 0.5) * step, step, pmx.forcing(nsteps, gtab=g)
+        f = pmx.forcing(nsteps, gtab=sweep_forcing(amp, freq, nsteps, step))
+    return m, (nsteps - 0.5) * step, step, f
 
 
 def serve_stream(count, rate=20.0, seed=0, **kw):

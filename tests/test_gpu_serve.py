@@ -26,6 +26,9 @@ def _requests(step=1e-5):
     reqs.append((s, 220.5 * step, step, pmx.forcing(gravity=(0.0, 0.0, -9.81))))
     e = workloads.spring_lattice(nx=3, ny=3, nz=2, seed=4)
     reqs.append((e, 260.5 * step, step, pmx.forcing(gravity=(0.0, 0.0, -9.81))))
+    pl = workloads.spring_lattice(nx=4, ny=3, nz=3, seed=6, anchors=False, prescribed_base=True)
+    reqs.append((pl, 240.5 * step, step, pmx.forcing(241, gravity=(0.0, 0.0, -9.81),
+                                                      prstab=workloads.base_prescription(step * np.arange(241)))))
     b, table = workloads.stirred_box(n=1200, seed=5)
     tab, tab0 = table(300)
     reqs.append((b, 299.5 * step, step, pmx.forcing(300, gravity=(0.0, 0.0, -9.81), obstab=tab, obstab0=tab0)))

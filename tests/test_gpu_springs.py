@@ -65,7 +65,7 @@ def test_spring_lattice_prescribed_base(dims, steps, calls):
         for call in range(calls):
             times = pmx.step_times(steps * 1e-5 - 0.5e-5, 1e-5, mm.curtime)
             f = pmx.forcing(len(times), gravity=(0.0, 0.0, -9.81),
-                            prstab=workloads.base_prescription(times, mm.prsnum))
+                            prstab=workloads.base_prescription(times))
             eng.dem(steps * 1e-5 - 0.5e-5, 1e-5, f)
             eng.download(mm)
         out.append((mm, eng.contacts()))
