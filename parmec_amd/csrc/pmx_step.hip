@@ -604,6 +604,11 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
 }
 
 /* dynamics_task, dynamics.ispc:30-169 */
+/* ISO: the launch over particles whose inertia is diag(j, j, j) with exact
+ * zeros elsewhere (inverse diag(v, v, v), INVERT macros.h:1033-1051): J x
+ * and I x are the scalar products j x and v x, equal in value to the
+ * matrix products; the other launch takes the remaining particles */
+template <bool ISO>
 __global__ void k_dynamics (int N, double *angular, double *linear, double *rotation, double *position,
                             const double *inertia, const double *inverse, const double *mass, const double *invm,
                             const double *dtab, const int *ctl, double *force, double *torque, double step0, double step1,
@@ -613,25 +618,20 @@ __global__ void k_dynamics (int N, double *angular, double *linear, double *rota
   if (ctl[CTL_ABORT]) return;
   int kstep = ctl[CTL_KSTEP];
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N || frozen (stop, par_inst[i], ctl)) return;
+  if (i >= N || (iso[i] != 0) != ISO || frozen (stop, par_inst[i], ctl)) return;
   const double *damping = dtab + 6*((size_t) kstep*ninst + (ninst > 1 ? par_inst[i] : 0));
   double half = 0.5*step0;
   double step = 0.5*(step0+step1);
-  double O[3], o[3], v[3], L1[9], J[9], I[9], im, f[3], t[3], T[3], DL[9], L2[9], A[3], B[3];
+  double O[3], o[3], v[3], L1[9], im, f[3], t[3], T[3], DL[9], A[3], B[3];
+  double J[ISO ? 1 : 9], I[ISO ? 1 : 9];
 #pragma unroll
   for (int k = 0; k < 3; k++) { O[k] = angular[(size_t) k*N + i]; v[k] = linear[(size_t) k*N + i]; }
 #pragma unroll
   for (int k = 0; k < 9; k++) L1[k] = rotation[(size_t) k*N + i];
-  if (iso[i])
+  if constexpr (ISO)
   {
-    /* inertia diag(j, j, j) with exact zeros elsewhere, hence inverse
-     * diag(v, v, v) (INVERT, macros.h:1033-1051): the same values as the
-     * full loads, from two of them */
-    double j = inertia[i], v = inverse[i];
-#pragma unroll
-    for (int k = 0; k < 9; k++) { J[k] = 0.0; I[k] = 0.0; }
-    J[0] = J[4] = J[8] = j;
-    I[0] = I[4] = I[8] = v;
+    J[0] = inertia[i];
+    I[0] = inverse[i];
   }
   else
   {
@@ -642,6 +642,10 @@ __global__ void k_dynamics (int N, double *angular, double *linear, double *rota
       I[k] = inverse[(size_t) k*N + i];
     }
   }
+  auto Jx = [&] (const double x[3], double y[3]) {
+    if constexpr (ISO) { y[0] = J[0]*x[0]; y[1] = J[0]*x[1]; y[2] = J[0]*x[2]; } else nvmul (J, x, y); };
+  auto Ix = [&] (const double x[3], double y[3]) {
+    if constexpr (ISO) { y[0] = I[0]*x[0]; y[1] = I[0]*x[1]; y[2] = I[0]*x[2]; } else nvmul (I, x, y); };
   im = invm[i];
   f[0] = force[i]; f[1] = force[(size_t) N + i]; f[2] = force[2*(size_t) N + i];
   t[0] = torque[i]; t[1] = torque[(size_t) N + i]; t[2] = torque[2*(size_t) N + i];
@@ -660,40 +664,61 @@ __global__ void k_dynamics (int N, double *angular, double *linear, double *rota
     w[1] = damping[4]*angular[4*(size_t) N + i];
     w[2] = damping[5]*angular[5*(size_t) N + i];
     tvmul (L1, w, tt);
-    nvmul (J, tt, TT);
+    Jx (tt, TT);
     nvmul (L1, TT, tt);
     t[0] -= tt[0]; t[1] -= tt[1]; t[2] -= tt[2];
     torque[i] = t[0]; torque[(size_t) N + i] = t[1]; torque[2*(size_t) N + i] = t[2];
   }
   tvmul (L1, t, T);
   expmap (-half*O[0], -half*O[1], -half*O[2], DL);
-  nvmul (J, O, A);
+  Jx (O, A);
   nvmul (DL, A, B);
   B[0] = B[0] + half*T[0];
   B[1] = B[1] + half*T[1];
   B[2] = B[2] + half*T[2];
-  nvmul (I, B, A);
-  nvmul (J, A, B);
+  Ix (B, A);
+  Jx (A, B);
   T[0] -= A[1]*B[2] - A[2]*B[1];
   T[1] -= A[2]*B[0] - A[0]*B[2];
   T[2] -= A[0]*B[1] - A[1]*B[0];
   T[0] *= step0; T[1] *= step0; T[2] *= step0;
-  O[0] = O[0] + (I[0]*T[0] + I[3]*T[1] + I[6]*T[2]);
-  O[1] = O[1] + (I[1]*T[0] + I[4]*T[1] + I[7]*T[2]);
-  O[2] = O[2] + (I[2]*T[0] + I[5]*T[1] + I[8]*T[2]);
+  Ix (T, A);
+  O[0] = O[0] + A[0];
+  O[1] = O[1] + A[1];
+  O[2] = O[2] + A[2];
   im *= step0;
   v[0] = v[0] + im*f[0];
   v[1] = v[1] + im*f[1];
   v[2] = v[2] + im*f[2];
   expmap (step*O[0], step*O[1], step*O[2], DL);
-  nnmul (L1, DL, L2);
-#pragma unroll
-  for (int k = 0; k < 9; k++) rotation[(size_t) k*N + i] = L2[k];
-  nvmul (L2, O, o);
   double xn[3];
   xn[0] = position[i] + step*v[0];
   xn[1] = position[(size_t) N + i] + step*v[1];
   xn[2] = position[2*(size_t) N + i] + step*v[2];
+  /* the particle's sphere (ellipsoids_task, shapes.ispc:29-67) */
+  int e = parell ? parell[i] : -1;
+  double C[3] = {0.0, 0.0, 0.0};
+  if (e >= 0)
+  {
+#pragma unroll
+    for (int k = 0; k < 3; k++) C[k] = center[(size_t) (3+k)*E + e] - position[(size_t) (3+k)*N + i];
+  }
+  /* L2 = L1 DL (NNMUL) one column at a time: each column is stored at once
+   * and folded into omega = L2 Omega and L2 C with the same left-to-right
+   * sums as the whole-matrix products */
+  double cc[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+  {
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+    {
+      double l = L1[r]*DL[3*c] + L1[3+r]*DL[3*c+1] + L1[6+r]*DL[3*c+2];
+      rotation[(size_t) (3*c+r)*N + i] = l;
+      o[r] = c ? o[r] + l*O[c] : l*O[c];
+      cc[r] = c ? cc[r] + l*C[c] : l*C[c];
+    }
+  }
   position[i] = xn[0];
   position[(size_t) N + i] = xn[1];
   position[2*(size_t) N + i] = xn[2];
@@ -704,16 +729,11 @@ __global__ void k_dynamics (int N, double *angular, double *linear, double *rota
     angular[(size_t) (3+k)*N + i] = o[k];
     linear[(size_t) k*N + i] = v[k];
   }
-  /* ellipsoids_task for the particle's sphere, shapes.ispc:29-67 */
-  int e = parell ? parell[i] : -1;
   if (e >= 0)
   {
-    double C[3];
-#pragma unroll
-    for (int k = 0; k < 3; k++) C[k] = center[(size_t) (3+k)*E + e] - position[(size_t) (3+k)*N + i];
-    center[e] = xn[0] + (L2[0]*C[0] + L2[3]*C[1] + L2[6]*C[2]);
-    center[(size_t) E + e] = xn[1] + (L2[1]*C[0] + L2[4]*C[1] + L2[7]*C[2]);
-    center[2*(size_t) E + e] = xn[2] + (L2[2]*C[0] + L2[5]*C[1] + L2[8]*C[2]);
+    center[e] = xn[0] + cc[0];
+    center[(size_t) E + e] = xn[1] + cc[1];
+    center[2*(size_t) E + e] = xn[2] + cc[2];
   }
 }
 
@@ -1048,11 +1068,14 @@ int step_rows (pmx_ctx &c, double h)
 int step_dynamics (pmx_ctx &c, double step0, double step1)
 {
   ProfScope ps (c, K_DYNAMICS, 384.0 * c.N + 128.0 * (c.N - c.niso) + 56.0 * c.E);
-  hipLaunchKernelGGL (k_dynamics, grid (c.N), dim3 (PMX_TPB), 0, c.stream, c.N, c.angular.p, c.linear.p, c.rotation.p,
-                      c.position.p, (const double*) c.inertia.p, (const double*) c.inverse.p, (const double*) c.mass.p,
-                      (const double*) c.invm.p, (const double*) c.dtab.p, (const int*) c.ctl.p, c.force.p, c.torque.p,
-                      step0, step1, (const int*) c.parell.p, c.E, c.center.p, (const signed char*) c.iso.p, c.I,
-                      (const int*) c.par_inst.p, c.staggered ? (const int*) c.istop.p : (const int*) nullptr);
+#define DYN_ARGS c.N, c.angular.p, c.linear.p, c.rotation.p, \
+                 c.position.p, (const double*) c.inertia.p, (const double*) c.inverse.p, (const double*) c.mass.p, \
+                 (const double*) c.invm.p, (const double*) c.dtab.p, (const int*) c.ctl.p, c.force.p, c.torque.p, \
+                 step0, step1, (const int*) c.parell.p, c.E, c.center.p, (const signed char*) c.iso.p, c.I, \
+                 (const int*) c.par_inst.p, c.staggered ? (const int*) c.istop.p : (const int*) nullptr
+  if (c.niso > 0) hipLaunchKernelGGL (k_dynamics<true>, grid (c.N), dim3 (PMX_TPB), 0, c.stream, DYN_ARGS);
+  if (c.niso < c.N) hipLaunchKernelGGL (k_dynamics<false>, grid (c.N), dim3 (PMX_TPB), 0, c.stream, DYN_ARGS);
+#undef DYN_ARGS
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
 }
