@@ -346,6 +346,8 @@ __global__ void k_force_init (const int *ctl, int N, const int *list, int nlist,
 }
 
 
+#define ROWS_SORT 8   /* hit keys a thread sorts in LDS (more: sorted in the hit array) */
+
 /* row merge + contact forces + per-sphere accumulation.  R is the arithmetic
  * of the contact geometry and force law: double (the FP64 path) or float
  * (mixed precision: differences of FP64 positions are formed in FP64 and then
@@ -360,11 +362,11 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
              const double *invm, int pairnum, const int *pairs, const int *ikind, const double *iparam,
              int ninst, const int *ell_inst, const int *pair_off,
              uint32_t *key0, R *st0, int *cnt0, uint32_t *key1, R *st1, int *cnt1, int cur0, int cap,
-             const uint32_t *hits, const int *hitcnt, int hcap, const int *xbuf, const int *xcnt, int xcap,
+             uint32_t *hits, const int *hitcnt, int hcap, const int *xbuf, const int *xcnt, int xcap,
              const double *gtab, const int *ctl, double h, double *force, double *torque, int *flags,
              const double *obspnt, const double *obsang, const double *obslin, const int *stop)
 {
-  extern __shared__ uint32_t sh[];
+  __shared__ uint32_t sh[ROWS_SORT * TPB];
   if (ctl[CTL_ABORT]) return;
   int kstep = ctl[CTL_KSTEP];
   bool odd = ((cur0 + kstep) & 1) != 0;   /* rows of the previous step */
@@ -379,20 +381,25 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
   int cnt = 0;
   if (e < E && !frozen (stop, ell_inst[e], ctl))
   {
-  int hx = hcap + xcap;
-  uint32_t *H = sh + (size_t) threadIdx.x * hx;
+  /* the step's hit keys (own hits, then reverse hits), sorted: in a small
+   * LDS buffer, or in place in the sphere's column of the hit array when
+   * there are more (both orders are the same unique sorted set) */
   int nh = hitcnt[e];
-  for (int q = 0; q < nh; q++) H[q] = hits[(size_t) q * E + e];
   int nx = xcnt[e];
   nx = nx < xcap ? nx : xcap;
-  for (int q = 0; q < nx; q++) H[nh + q] = (uint32_t) xbuf[(size_t) q * E + e];
+  bool small = nh + nx <= ROWS_SORT;
+  uint32_t *H = small ? sh + threadIdx.x : hits + e;
+  size_t hs = small ? (size_t) TPB : (size_t) E;   /* stride of H */
+  if (small) for (int q = 0; q < nh; q++) H[q*hs] = hits[(size_t) q * E + e];
+  else if (nh + nx > hcap) { atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1); nx = hcap - nh; }
+  for (int q = 0; q < nx; q++) H[(nh + q)*hs] = (uint32_t) xbuf[(size_t) q * E + e];
   nh += nx;
   for (int a = 1; a < nh; a++)   /* insertion sort of the hit keys */
   {
-    uint32_t v = H[a];
+    uint32_t v = H[a*hs];
     int b = a - 1;
-    while (b >= 0 && H[b] > v) { H[b+1] = H[b]; b--; }
-    H[b+1] = v;
+    while (b >= 0 && H[b*hs] > v) { H[(b+1)*hs] = H[b*hs]; b--; }
+    H[(b+1)*hs] = v;
   }
 
   int pe = part[e];
@@ -410,7 +417,7 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
   while (io < oc || ih < nh)
   {
     uint32_t ko = io < oc ? okey[(size_t) io * E + e] : 0xFFFFFFFFu;
-    uint32_t kh = ih < nh ? H[ih] : 0xFFFFFFFFu;
+    uint32_t kh = ih < nh ? H[ih*hs] : 0xFFFFFFFFu;
     uint32_t key;
     bool old;
     int kold = -1;
@@ -1014,8 +1021,7 @@ int step_detect (pmx_ctx &c)
 template <class R, int TPB>
 void rows_launch (pmx_ctx &c, R *st0, R *st1, double h)
 {
-  size_t shm = (size_t) TPB * (c.hcap + c.xcap) * sizeof (uint32_t);
-  hipLaunchKernelGGL ((k_rows<R, TPB>), dim3 ((c.E + TPB - 1) / TPB), dim3 (TPB), shm, c.stream, c.E, c.N,
+  hipLaunchKernelGGL ((k_rows<R, TPB>), dim3 ((c.E + TPB - 1) / TPB), dim3 (TPB), 0, c.stream, c.E, c.N,
                       (const double*) c.center.p, (const double*) c.radii.p, (const int*) c.part.p,
                       (const int*) c.ellcol.p, (const double*) c.tri.p, c.T, (const int*) c.tricol.p,
                       (const int*) c.triobs.p, (const double*) c.position.p, (const double*) c.linear.p,
@@ -1023,7 +1029,7 @@ void rows_launch (pmx_ctx &c, R *st0, R *st1, double h)
                       (const int*) c.pairs.p, (const int*) c.ikind.p, (const double*) c.iparam.p,
                       c.I, (const int*) c.ell_inst.p, (const int*) c.pair_off.p,
                       c.rowkey[0].p, st0, c.rowcnt[0].p, c.rowkey[1].p, st1, c.rowcnt[1].p,
-                      c.cur0, c.cap, (const uint32_t*) c.hits.p, (const int*) c.hitcnt.p, c.hcap,
+                      c.cur0, c.cap, c.hits.p, (const int*) c.hitcnt.p, c.hcap,
                       (const int*) c.xbuf.p, (const int*) c.xcnt.p, c.xcap, (const double*) c.gtab.p,
                       (const int*) c.ctl.p, h, c.force.p, c.torque.p, c.dflags.p,
                       (const double*) c.obspnt.p, (const double*) c.obsang.p, (const double*) c.obslin.p,
