@@ -367,6 +367,8 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
              const double *obspnt, const double *obsang, const double *obslin, const int *stop)
 {
   __shared__ uint32_t sh[ROWS_SORT * TPB];
+  __shared__ int lst[TPB];         /* the block's busy spheres, in order */
+  __shared__ int wbase[TPB / 64];  /* busy spheres of the waves before */
   if (ctl[CTL_ABORT]) return;
   int kstep = ctl[CTL_KSTEP];
   bool odd = ((cur0 + kstep) & 1) != 0;   /* rows of the previous step */
@@ -377,10 +379,44 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
   uint32_t *nkey = odd ? key0 : key1;
   R *nst = odd ? st0 : st1;
   int *ncnt = odd ? cnt0 : cnt1;
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  int cnt = 0;
-  if (e < E && !frozen (stop, ell_inst[e], ctl))
+
+  /* A sphere with neither a contact in the previous step's row nor a hit in
+   * this step has an empty row and zero sums: the merge below would give
+   * force 0 + m g and torque 0, written here directly.  The block's other
+   * (busy) spheres are packed in order onto its first threads, so that the
+   * merge runs in as few waves as there are busy spheres / 64 */
   {
+    int e = blockIdx.x * TPB + threadIdx.x;
+    bool live = e < E && !frozen (stop, ell_inst[e], ctl);
+    bool busy = live && (hitcnt[e] | xcnt[e] | ocnt[e]) != 0;
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    unsigned long long m = __ballot (busy);
+    if (lane == 0) wbase[w] = __popcll (m);
+    __syncthreads ();
+    int base = 0;
+    for (int k = 0; k < w; k++) base += wbase[k];
+    if (busy) lst[base + __popcll (m & ((1ull << lane) - 1ull))] = e;
+    else if (live)
+    {
+      ncnt[e] = 0;
+      int pe = part[e];
+      const double *g = gtab + 3*((size_t) kstep*ninst + (ninst > 1 ? ell_inst[e] : 0));
+      double ma = mass[pe];
+      force[pe] = 0.0 + ma * g[0];
+      force[(size_t) N + pe] = 0.0 + ma * g[1];
+      force[2*(size_t) N + pe] = 0.0 + ma * g[2];
+      torque[pe] = 0.0;
+      torque[(size_t) N + pe] = 0.0;
+      torque[2*(size_t) N + pe] = 0.0;
+    }
+    __syncthreads ();
+  }
+  int nbusy = 0;
+  for (int k = 0; k < TPB / 64; k++) nbusy += wbase[k];
+  int cnt = 0;
+  if (threadIdx.x < nbusy)
+  {
+  int e = lst[threadIdx.x];
   /* the step's hit keys (own hits, then reverse hits), sorted: in a small
    * LDS buffer, or in place in the sphere's column of the hit array when
    * there are more (both orders are the same unique sorted set) */
