@@ -314,7 +314,8 @@ def main():
             "contact_pairs_per_particle": pairs_per_particle,
             "tree_rebuilds": st["rebuilds"],
             "leaves": st["leaves"],
-            "autotuned_block_shapes": {"detect_waves_per_block": st["det_waves"], "rows_threads_per_block": st["rows_block"]},
+            "block_shapes": {"detect_waves_per_block": st["det_waves"], "detect_autotuned": True,
+                             "rows_threads_per_block": st["rows_block"]},
         },
         "plain_fp64": {"value": plain, "unit": "particle-steps/s", "instances_sampled": plain_ids,
                        "how": "drop-in FP64 path, each sampled instance alone (upload, warmup, timed window)"},

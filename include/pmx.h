@@ -166,8 +166,11 @@ typedef struct pmx_options
   int device;          /* HIP device ordinal */
   int row_capacity;    /* contacts per sphere row (0: default 32) */
   int verbose;
-  int tune;            /* 0: pick kernel block shapes by timing them at the first dem call
-                          (models of 50,000 spheres or more); 1: fixed defaults */
+  int tune;            /* 0: pick the detection block shape by timing it (models of 50,000
+                          spheres or more: at the first dem call and again after 256
+                          steps); 1: the fixed det_waves below */
+  int det_waves;       /* tune = 1: waves per detection block, 2, 4 or 8 (0: 4) */
+  int rows_block;      /* threads per contact-row block, 64, 128, 256 or 512 (0: 512) */
 } pmx_options;
 
 /* one contact of the canonical pair set */

@@ -323,6 +323,13 @@ int pmx_create (const pmx_options *opt, pmx_ctx **out)
     delete c;
     return fail (PMX_EINVAL, "precision must be PMX_FP64 or PMX_MIXED");
   }
+  if (c->opt.det_waves != 0 && c->opt.det_waves != 2 && c->opt.det_waves != 4 && c->opt.det_waves != 8)
+  { delete c; return fail (PMX_EINVAL, "det_waves must be 2, 4 or 8"); }
+  if (c->opt.rows_block != 0 && c->opt.rows_block != 64 && c->opt.rows_block != 128 && c->opt.rows_block != 256 &&
+      c->opt.rows_block != 512)
+  { delete c; return fail (PMX_EINVAL, "rows_block must be 64, 128, 256 or 512"); }
+  if (c->opt.det_waves) c->det_nw = c->opt.det_waves;
+  if (c->opt.rows_block) c->rows_tpb = c->opt.rows_block;
   int ndev = 0;
   if (hipGetDeviceCount (&ndev) != hipSuccess || ndev == 0)
   {
@@ -338,8 +345,12 @@ int pmx_create (const pmx_options *opt, pmx_ctx **out)
   if (c->opt.row_capacity > 0) { c->cap = c->opt.row_capacity; c->hcap = c->opt.row_capacity; c->xcap = std::max (8, c->opt.row_capacity / 2); }
   if (c->dflags.alloc (FLAG_COUNT) != hipSuccess || c->ctl.alloc (CTL_COUNT) != hipSuccess)
   { delete c; return fail (PMX_ENOMEM, "out of device memory"); }
-  if (hipMemset (c->ctl.p, 0, CTL_COUNT * sizeof (int)) != hipSuccess) { delete c; return fail (PMX_EDEVICE, "memset failed"); }
-  if (hipMemset (c->dflags.p, 0, FLAG_COUNT * sizeof (int)) != hipSuccess) { delete c; return fail (PMX_EDEVICE, "memset failed"); }
+  /* on the context's stream, never the legacy one: contexts of other
+   * threads may be capturing a step */
+  if (hipMemsetAsync (c->ctl.p, 0, CTL_COUNT * sizeof (int), c->stream) != hipSuccess ||
+      hipMemsetAsync (c->dflags.p, 0, FLAG_COUNT * sizeof (int), c->stream) != hipSuccess ||
+      hipStreamSynchronize (c->stream) != hipSuccess)
+  { delete c; return fail (PMX_EDEVICE, "memset failed"); }
   *out = c;
   return PMX_OK;
 }
@@ -590,13 +601,27 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
     if ((rc = rebuild (c, true))) return rc;
     rebuilds++;
   }
-  /* block shapes for this workload, once per context (large models only:
-   * small ones are bound by launch latency, not by the shapes) */
-  if (c->opt.tune == 0 && !c->tuned && c->E >= 50000)
-  {
-    if ((rc = autotune (*c, step0))) return rc;
-    int z[CTL_COUNT] = {0};
+  /* block shapes for this workload (large models only: small ones are
+   * bound by launch latency, not by the shapes): a first pass on the state
+   * of the first call, and a second once the context has taken
+   * PMX_RETUNE_STEPS steps, when the contact rows have settled (the first
+   * steps have no contact of a previous step) */
+  auto tune = [&] (int kstep) -> int {
+    int r = autotune (*c, step0);
+    if (r) return r;
+    int z[CTL_COUNT] = {0};   /* hint off: the next bin descends from the roots */
+    z[CTL_KSTEP] = kstep;
     PMX_HIP (hipMemcpyAsync (c->ctl.p, z, sizeof (z), hipMemcpyHostToDevice, c->stream));
+    PMX_HIP (hipStreamSynchronize (c->stream));
+    c->graph_valid = false;
+    c->stats.det_waves = c->det_nw;
+    c->stats.rows_block = c->rows_tpb;
+    return PMX_OK;
+  };
+  bool tuning = c->opt.tune == 0 && c->E >= 50000;
+  if (tuning && c->tuned == 0)
+  {
+    if ((rc = tune (0))) return rc;
     PMX_HIP (hipMemsetAsync (c->dflags.p, 0, FLAG_COUNT * sizeof (int), c->stream));
     PMX_HIP (hipStreamSynchronize (c->stream));
   }
@@ -656,8 +681,12 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
       rebuilds++;
     }
     if ((rc = report (done))) return rc;
+    if (tuning && c->tuned == 1 && !ctl[CTL_ABORT] && done < nsteps && c->stepped + done >= PMX_RETUNE_STEPS &&
+        (rc = tune (done)))
+      return rc;
   }
   if ((rc = report (nsteps))) return rc;
+  c->stepped += nsteps;
   c->cur = (c->cur0 + nsteps) & 1;
   c->curtime = c->icurtime[0]; c->curstep = c->icurstep[0]; c->stepnum = c->istepnum[0];
   double dt = std::chrono::duration<double> (std::chrono::steady_clock::now () - t0).count ();
@@ -672,7 +701,9 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
       int e0 = c->off[i].ell, ne = c->off[i+1].ell - e0;
       if (ne == 0) continue;
       std::vector<int> cnt (ne);
-      PMX_HIP (hipMemcpy (cnt.data (), c->rowcnt[c->icur[i]].p + e0, sizeof (int) * ne, hipMemcpyDeviceToHost));
+      PMX_HIP (hipMemcpyAsync (cnt.data (), c->rowcnt[c->icur[i]].p + e0, sizeof (int) * ne, hipMemcpyDeviceToHost,
+                               c->stream));
+      PMX_HIP (hipStreamSynchronize (c->stream));
       for (int e = 0; e < ne; e++) total += cnt[e];
     }
     c->stats.pairs = total;  /* entries; sphere pairs are held twice */

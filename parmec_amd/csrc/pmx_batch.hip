@@ -64,20 +64,25 @@ int contacts_range (pmx_ctx &c, int inst, bool local, std::vector<pmx_pair> &out
   std::vector<uint32_t> key (cap * ne);
   std::vector<double> st (3 * cap * ne);
   std::vector<int> cnt (ne), part (ne);
-  PMX_HIP (hipMemcpy2D (key.data (), ne * sizeof (uint32_t), c.rowkey[par].p + e0, E * sizeof (uint32_t),
-                        ne * sizeof (uint32_t), cap, hipMemcpyDeviceToHost));
+  /* every copy on the context's stream: a copy on the legacy stream would
+   * have to wait for the streams of other contexts, which is refused while
+   * one of them captures a step */
+  PMX_HIP (hipMemcpy2DAsync (key.data (), ne * sizeof (uint32_t), c.rowkey[par].p + e0, E * sizeof (uint32_t),
+                             ne * sizeof (uint32_t), cap, hipMemcpyDeviceToHost, c.stream));
   if (c.opt.precision == PMX_MIXED)
   {
     std::vector<float> s32 (st.size ());
-    PMX_HIP (hipMemcpy2D (s32.data (), ne * sizeof (float), c.rowst32[par].p + e0, E * sizeof (float),
-                          ne * sizeof (float), 3 * cap, hipMemcpyDeviceToHost));
+    PMX_HIP (hipMemcpy2DAsync (s32.data (), ne * sizeof (float), c.rowst32[par].p + e0, E * sizeof (float),
+                               ne * sizeof (float), 3 * cap, hipMemcpyDeviceToHost, c.stream));
+    PMX_HIP (hipStreamSynchronize (c.stream));
     for (size_t q = 0; q < st.size (); q++) st[q] = s32[q];
   }
   else
-    PMX_HIP (hipMemcpy2D (st.data (), ne * sizeof (double), c.rowst[par].p + e0, E * sizeof (double),
-                          ne * sizeof (double), 3 * cap, hipMemcpyDeviceToHost));
-  PMX_HIP (hipMemcpy (cnt.data (), c.rowcnt[par].p + e0, ne * sizeof (int), hipMemcpyDeviceToHost));
-  PMX_HIP (hipMemcpy (part.data (), c.part.p + e0, ne * sizeof (int), hipMemcpyDeviceToHost));
+    PMX_HIP (hipMemcpy2DAsync (st.data (), ne * sizeof (double), c.rowst[par].p + e0, E * sizeof (double),
+                               ne * sizeof (double), 3 * cap, hipMemcpyDeviceToHost, c.stream));
+  PMX_HIP (hipMemcpyAsync (cnt.data (), c.rowcnt[par].p + e0, ne * sizeof (int), hipMemcpyDeviceToHost, c.stream));
+  PMX_HIP (hipMemcpyAsync (part.data (), c.part.p + e0, ne * sizeof (int), hipMemcpyDeviceToHost, c.stream));
+  PMX_HIP (hipStreamSynchronize (c.stream));
   std::vector<int> ppart;
   for (int e = 0; e < ne; e++)
     for (int k = 0; k < cnt[e]; k++)

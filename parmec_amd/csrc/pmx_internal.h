@@ -15,6 +15,7 @@
 #include "../../include/pmx.h"
 
 #define PMX_TPB 256            /* threads per block of the elementwise kernels */
+#define PMX_RETUNE_STEPS 256   /* steps of a context before the second autotuner pass */
 #define PMX_TRI_FLAG 0x80000000u  /* row key flag of a triangle partner */
 #define PMX_NO_ROOT INT_MIN       /* tree root of an instance without spheres */
 
@@ -192,8 +193,9 @@ struct pmx_ctx
   hipGraphExec_t gexec = nullptr;
   bool graph_valid = false;
   int det_nw = 4;              /* waves per k_detect block (autotuned) */
-  int rows_tpb = 128;          /* threads per k_rows block (autotuned) */
-  bool tuned = false;
+  int rows_tpb = 512;          /* threads per k_rows block */
+  int tuned = 0;                /* autotuner passes done: at the first call, then once settled */
+  long long stepped = 0;        /* steps taken by this context over all calls */
   dvec<int> parell;            /* [N] the particle's sphere, or -1 */
   dvec<int> nosph;             /* particles without a sphere */
   dvec<signed char> iso;       /* [N] 1 when inertia is diag(j, j, j) with exact zeros */

@@ -132,6 +132,13 @@ __global__ void k_scatter (int *ctl, int E, const double *center, const double *
   rec[1] = make_double2 (center[2*(size_t) E + e], radii[e]);
 }
 
+__device__ __forceinline__ unsigned long long readlane_u64 (unsigned long long x, int k)
+{
+  unsigned lo = (unsigned) __builtin_amdgcn_readlane ((int) (unsigned) x, k);
+  unsigned hi = (unsigned) __builtin_amdgcn_readlane ((int) (unsigned) (x >> 32), k);
+  return ((unsigned long long) hi << 32) | lo;
+}
+
 /* k_detect runs NW waves per block (NW * 64 >= PMX_LSIZE: one sphere of the
  * leaf per thread slot); NW is picked per workload by the autotuner */
 
@@ -193,94 +200,117 @@ void k_detect (int E, const double *center, const double *radii, const int *part
   double cu[3] = {cell[3*(size_t) nleaves + Q], cell[4*(size_t) nleaves + Q], cell[5*(size_t) nleaves + Q]};
   __syncthreads ();
 
-  /* this lane's two block spheres (lane, lane + 64) */
-  bool v0 = lane < qcnt, v1 = lane + 64 < qcnt;
-  double a0[4] = {0.0, 0.0, 0.0, 0.0}, a1[4] = {0.0, 0.0, 0.0, 0.0};
-  if (v0) for (int k = 0; k < 4; k++) a0[k] = ssph[4*lane+k];
-  if (v1) for (int k = 0; k < 4; k++) a1[k] = ssph[4*(lane+64)+k];
   double *rec = wrec[w];
   int *mem = wmem[w], *lst = wlist[w];
   unsigned long long lt = lane ? ((1ull << lane) - 1ull) : 0ull;
   int nc = nbrcnt[Q], c0 = nbroff[Q];
-  for (int ci = w; ci < nc; ci += NW)
+  int ncw = nc > w ? (nc - w + NW - 1) / NW : 0;   /* candidates of this wave: w, w + NW, ... */
+  for (int p0 = 0; p0 < ncw; p0 += 64)
   {
-    int L = nbrlist[c0 + ci];
-    int cnt = lcount[L];
-    if (cnt == 0) continue;
-    int off = loff[L];
-    /* the block spheres whose own AABB reaches L (condet.ispc:104-113) */
-    double Ll[3] = {cell[L], cell[(size_t) nleaves + L], cell[2*(size_t) nleaves + L]};
-    double Lu[3] = {cell[3*(size_t) nleaves + L], cell[4*(size_t) nleaves + L], cell[5*(size_t) nleaves + L]};
-    bool r0 = v0 && a0[0]-a0[3] <= Lu[0] && a0[1]-a0[3] <= Lu[1] && a0[2]-a0[3] <= Lu[2] &&
-              a0[0]+a0[3] >= Ll[0] && a0[1]+a0[3] >= Ll[1] && a0[2]+a0[3] >= Ll[2];
-    bool r1 = v1 && a1[0]-a1[3] <= Lu[0] && a1[1]-a1[3] <= Lu[1] && a1[2]-a1[3] <= Lu[2] &&
-              a1[0]+a1[3] >= Ll[0] && a1[1]+a1[3] >= Ll[1] && a1[2]+a1[3] >= Ll[2];
-    unsigned long long m0 = __ballot (r0), m1 = __ballot (r1);
-    if (!(m0 | m1)) continue;
-    int n0 = __popcll (m0), nsl = n0 + __popcll (m1);
-    bool own = L == Q;
-    /* stage: the reaching spheres' slots and the candidate's records */
-    __builtin_amdgcn_wave_barrier ();
-    if (r0) lst[__popcll (m0 & lt)] = lane;
-    if (r1) lst[n0 + __popcll (m1 & lt)] = lane + 64;
-    if (!own)
-      for (int k = lane; k < cnt; k += 64)
-      {
-        const double2 *rp = (const double2*) (lpos + 4*(size_t) (off + k));
-        double2 x = rp[0], y = rp[1];
-        rec[4*k] = x.x; rec[4*k+1] = x.y; rec[4*k+2] = y.x; rec[4*k+3] = y.y;
-        mem[k] = lmem[off + k];
-      }
-    __builtin_amdgcn_wave_barrier ();
-    if (own)
+    /* up to 64 candidates at once, one per lane: leaf, count, offset and
+     * which block spheres' own AABB reaches its cell (condet.ispc:104-113) */
+    int np = ncw - p0 < 64 ? ncw - p0 : 64;
+    int bL = 0, bcnt = 0, boff = 0;
+    unsigned long long bm0 = 0, bm1 = 0;
+    if (lane < np)
     {
-      /* own leaf: every sphere reaches it and the test is symmetric
-       * (depth r_a + r_b - |c_a - c_b|), so each unordered pair is tested
-       * once: pairs (a, (a+k) mod n), k = 1..(n-1)/2, plus k = n/2, a < n/2 */
-      int n = cnt, halfn = (n - 1) / 2;
-      int work = n * halfn + ((n & 1) ? 0 : n / 2);
-      unsigned inv = ((1u << 20) + (unsigned) n - 1u) / (unsigned) n;
-      for (int q = lane; q < work; q += 64)
+      bL = nbrlist[c0 + w + (p0 + lane) * NW];
+      bcnt = lcount[bL];
+      boff = loff[bL];
+      double Ll[3], Lu[3];
+#pragma unroll
+      for (int d = 0; d < 3; d++)
       {
-        int k = (int) (((unsigned) q * inv) >> 20), a = q - k * n;
-        int bb = a + k + 1;
-        bb = bb >= n ? bb - n : bb;
-        /* rx = max (r0, r1, r2) is r0 for a sphere (r1, r2 < 0) */
-        if (!overlap (ssph[4*a], ssph[4*a+1], ssph[4*a+2], ssph[4*a+3],
-                      ssph[4*bb], ssph[4*bb+1], ssph[4*bb+2], ssph[4*bb+3]))
-          continue;
-        if (spart[a] == spart[bb]) continue;
-        int ea = sidx[a], eb = sidx[bb];
-        int sa = atomicAdd (&shc[a], 1), sb = atomicAdd (&shc[bb], 1);
-        if (sa < hcap) hits[(size_t) sa * E + ea] = (uint32_t) eb; else atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1);
-        if (sb < hcap) hits[(size_t) sb * E + eb] = (uint32_t) ea; else atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1);
+        Ll[d] = cell[(size_t) d*nleaves + bL];
+        Lu[d] = cell[(size_t) (3+d)*nleaves + bL];
       }
-    }
-    else
-    {
-      int work = nsl * cnt;
-      /* a = q / cnt exactly for q * cnt < 2^20 (q < 96*96, cnt <= 96) */
-      unsigned inv = ((1u << 20) + (unsigned) cnt - 1u) / (unsigned) cnt;
-      for (int q = lane; q < work; q += 64)
-      {
-        int a = (int) (((unsigned) q * inv) >> 20), m = q - a * cnt;
-        int ti = lst[a];
-        double cx = rec[4*m], cy = rec[4*m+1], cz = rec[4*m+2], rj = rec[4*m+3];
-        if (!overlap (ssph[4*ti], ssph[4*ti+1], ssph[4*ti+2], ssph[4*ti+3], cx, cy, cz, rj)) continue;
-        int j = mem[m];
-        if (spart[ti] == part[j]) continue;
-        int e = sidx[ti];
-        int slot = atomicAdd (&shc[ti], 1);
-        if (slot < hcap) hits[(size_t) slot * E + e] = (uint32_t) j;
-        else atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1);
-        /* does the partner's AABB reach this sphere's leaf Q? */
-        bool back = cx-rj <= cu[0] && cy-rj <= cu[1] && cz-rj <= cu[2] &&
-                    cx+rj >= cl[0] && cy+rj >= cl[1] && cz+rj >= cl[2];
-        if (!back)
+      if (bcnt > 0)
+        for (int sI = 0; sI < qcnt; sI++)
         {
-          int sx = atomicAdd (&xcnt[j], 1);
-          if (sx < xcap) xbuf[(size_t) sx * E + j] = e;
-          else atomicAdd (&flags[FLAG_EXTRA_OVERFLOW], 1);
+          double x = ssph[4*sI], y = ssph[4*sI+1], z = ssph[4*sI+2], r = ssph[4*sI+3];
+          bool in = x-r <= Lu[0] && y-r <= Lu[1] && z-r <= Lu[2] && x+r >= Ll[0] && y+r >= Ll[1] && z+r >= Ll[2];
+          if (in)
+          {
+            if (sI < 64) bm0 |= 1ull << sI;
+            else bm1 |= 1ull << (sI - 64);
+          }
+        }
+    }
+    /* the candidates that some block sphere reaches, in turn */
+    unsigned long long todo = __ballot ((bm0 | bm1) != 0);
+    while (todo)
+    {
+      int k = __ffsll ((long long) todo) - 1;
+      todo &= todo - 1ull;
+      int L = __builtin_amdgcn_readlane (bL, k);
+      int cnt = __builtin_amdgcn_readlane (bcnt, k);
+      int off = __builtin_amdgcn_readlane (boff, k);
+      unsigned long long m0 = readlane_u64 (bm0, k), m1 = readlane_u64 (bm1, k);
+      int n0 = __popcll (m0), nsl = n0 + __popcll (m1);
+      bool own = L == Q;
+      /* stage: the reaching spheres' slots and the candidate's records */
+      __builtin_amdgcn_wave_barrier ();
+      if ((m0 >> lane) & 1ull) lst[__popcll (m0 & lt)] = lane;
+      if ((m1 >> lane) & 1ull) lst[n0 + __popcll (m1 & lt)] = lane + 64;
+      if (!own)
+        for (int kk = lane; kk < cnt; kk += 64)
+        {
+          const double2 *rp = (const double2*) (lpos + 4*(size_t) (off + kk));
+          double2 x = rp[0], y = rp[1];
+          rec[4*kk] = x.x; rec[4*kk+1] = x.y; rec[4*kk+2] = y.x; rec[4*kk+3] = y.y;
+          mem[kk] = lmem[off + kk];
+        }
+      __builtin_amdgcn_wave_barrier ();
+      if (own)
+      {
+        /* own leaf: every sphere reaches it and the test is symmetric
+         * (depth r_a + r_b - |c_a - c_b|), so each unordered pair is tested
+         * once: pairs (a, (a+k) mod n), k = 1..(n-1)/2, plus k = n/2, a < n/2 */
+        int n = cnt, halfn = (n - 1) / 2;
+        int work = n * halfn + ((n & 1) ? 0 : n / 2);
+        unsigned inv = ((1u << 20) + (unsigned) n - 1u) / (unsigned) n;
+        for (int q = lane; q < work; q += 64)
+        {
+          int kk = (int) (((unsigned) q * inv) >> 20), a = q - kk * n;
+          int bb = a + kk + 1;
+          bb = bb >= n ? bb - n : bb;
+          /* rx = max (r0, r1, r2) is r0 for a sphere (r1, r2 < 0) */
+          if (!overlap (ssph[4*a], ssph[4*a+1], ssph[4*a+2], ssph[4*a+3],
+                        ssph[4*bb], ssph[4*bb+1], ssph[4*bb+2], ssph[4*bb+3]))
+            continue;
+          if (spart[a] == spart[bb]) continue;
+          int ea = sidx[a], eb = sidx[bb];
+          int sa = atomicAdd (&shc[a], 1), sb = atomicAdd (&shc[bb], 1);
+          if (sa < hcap) hits[(size_t) sa * E + ea] = (uint32_t) eb; else atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1);
+          if (sb < hcap) hits[(size_t) sb * E + eb] = (uint32_t) ea; else atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1);
+        }
+      }
+      else
+      {
+        int work = nsl * cnt;
+        /* a = q / cnt exactly for q * cnt < 2^20 (q < 96*96, cnt <= 96) */
+        unsigned inv = ((1u << 20) + (unsigned) cnt - 1u) / (unsigned) cnt;
+        for (int q = lane; q < work; q += 64)
+        {
+          int a = (int) (((unsigned) q * inv) >> 20), m = q - a * cnt;
+          int ti = lst[a];
+          double cx = rec[4*m], cy = rec[4*m+1], cz = rec[4*m+2], rj = rec[4*m+3];
+          if (!overlap (ssph[4*ti], ssph[4*ti+1], ssph[4*ti+2], ssph[4*ti+3], cx, cy, cz, rj)) continue;
+          int j = mem[m];
+          if (spart[ti] == part[j]) continue;
+          int e = sidx[ti];
+          int slot = atomicAdd (&shc[ti], 1);
+          if (slot < hcap) hits[(size_t) slot * E + e] = (uint32_t) j;
+          else atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1);
+          /* does the partner's AABB reach this sphere's leaf Q? */
+          bool back = cx-rj <= cu[0] && cy-rj <= cu[1] && cz-rj <= cu[2] &&
+                      cx+rj >= cl[0] && cy+rj >= cl[1] && cz+rj >= cl[2];
+          if (!back)
+          {
+            int sx = atomicAdd (&xcnt[j], 1);
+            if (sx < xcap) xbuf[(size_t) sx * E + j] = e;
+            else atomicAdd (&flags[FLAG_EXTRA_OVERFLOW], 1);
+          }
         }
       }
     }
@@ -1077,10 +1107,11 @@ void rows_launch_tpb (pmx_ctx &c, int tpb, R *st0, R *st1, double h)
 {
   if (tpb == 64) rows_launch<R, 64> (c, st0, st1, h);
   else if (tpb == 256) rows_launch<R, 256> (c, st0, st1, h);
+  else if (tpb == 512) rows_launch<R, 512> (c, st0, st1, h);
   else rows_launch<R, 128> (c, st0, st1, h);
 }
 
-/* k_rows with tpb threads per block (64, 128 or 256) */
+/* k_rows with tpb threads per block (64, 128, 256 or 512) */
 int launch_rows (pmx_ctx &c, int tpb, double h)
 {
   if (c.E == 0) return PMX_OK;
@@ -1210,14 +1241,24 @@ int step_invert (pmx_ctx &c)
   return PMX_OK;
 }
 
-/* Autotuner: pick the waves per block of k_detect and the block size of
- * k_rows for this workload by timing each variant on the current state
- * (both kernels only read their inputs and rewrite their outputs, so runs
- * repeat exactly; block shapes never change a result).  Called once per
- * context, at a step boundary with the control block clear; the caller
- * clears the control block and flags afterwards. */
+/* Autotuner: pick the waves per block of k_detect for this workload by
+ * timing each variant on the current state (detection only reads its inputs
+ * and rewrites its outputs, so runs repeat exactly; block shapes never change
+ * a result).  The block size of k_rows is not tuned: its cost depends on the
+ * share of spheres in contact, which the early steps where tuning runs do
+ * not show, and its largest block (most spheres per compaction) is best or
+ * level on the workloads measured.  Called at a step boundary with the
+ * control block clear; the caller resets the control block afterwards (the
+ * next step redoes everything the timings wrote).  Counts a pass in c.tuned
+ * when it completes. */
 int autotune (pmx_ctx &c, double h)
 {
+  struct NoProf   /* the timings are not profiled steps */
+  {
+    Prof &p; bool was;
+    NoProf (Prof &p_) : p (p_), was (p_.on) { p.on = false; }
+    ~NoProf () { p.on = was; }
+  } np (c.prof);
   int rc;
   if ((rc = step_bin (c)) || (rc = step_mtri (c)) || (rc = step_scatter (c))) return rc;
   int ctl[CTL_COUNT];
@@ -1227,12 +1268,14 @@ int autotune (pmx_ctx &c, double h)
   hipEvent_t e0, e1;
   PMX_HIP (hipEventCreate (&e0));
   PMX_HIP (hipEventCreate (&e1));
-  auto best_of = [&] (const int *opts, int nopt, auto run) -> int {
+  /* prep runs before the timed region, run inside it */
+  auto best_of = [&] (const int *opts, int nopt, auto prep, auto run) -> int {
     int best = opts[0];
     float bt = 1e30f;
     for (int k = 0; k < nopt; k++)
       for (int rep = 0; rep < 3; rep++)
       {
+        if (prep ()) return -1;
         (void) hipEventRecord (e0, c.stream);
         if (run (opts[k])) return -1;
         (void) hipEventRecord (e1, c.stream);
@@ -1242,19 +1285,16 @@ int autotune (pmx_ctx &c, double h)
       }
     return best;
   };
-  const int nws[3] = {2, 4, 8}, tpbs[3] = {64, 128, 256};
-  int nw = best_of (nws, 3, [&] (int v) {
+  const int nws[3] = {2, 4, 8};
+  /* the reverse-hit counts start at zero for each detection */
+  auto clear = [&] () {
     hipLaunchKernelGGL (k_fill_i, grid (c.E), dim3 (PMX_TPB), 0, c.stream, (const int*) c.ctl.p, c.xcnt.p, c.E, 0);
-    return launch_detect (c, v); });
-  int tpb = nw > 0 ? best_of (tpbs, 3, [&] (int v) {
-    hipLaunchKernelGGL (k_fill_i, grid (c.E), dim3 (PMX_TPB), 0, c.stream, (const int*) c.ctl.p, c.xcnt.p, c.E, 0);
-    int r = launch_detect (c, nw);
-    return r ? r : launch_rows (c, v, h); }) : -1;
+    return hipGetLastError () == hipSuccess ? 0 : -1; };
+  int nw = best_of (nws, 3, clear, [&] (int v) { return launch_detect (c, v); });
   (void) hipEventDestroy (e0);
   (void) hipEventDestroy (e1);
-  if (nw < 0 || tpb < 0) { pmx_set_error ("autotuning failed"); return PMX_EDEVICE; }
+  if (nw < 0) { pmx_set_error ("autotuning failed"); return PMX_EDEVICE; }
   c.det_nw = nw;
-  c.rows_tpb = tpb;
-  c.tuned = true;
+  c.tuned++;
   return PMX_OK;
 }

@@ -52,7 +52,7 @@ class pmx_forcing(C.Structure):
 
 class pmx_options(C.Structure):
     _fields_ = [("precision", C.c_int), ("device", C.c_int), ("row_capacity", C.c_int), ("verbose", C.c_int),
-                ("tune", C.c_int)]
+                ("tune", C.c_int), ("det_waves", C.c_int), ("rows_block", C.c_int)]
 
 
 class pmx_pair(C.Structure):
@@ -322,9 +322,9 @@ class Engine:
 
     prefix = "pmx_"
 
-    def __init__(self, precision=PMX_FP64, device=0, row_capacity=0, lib=None, tune=0):
+    def __init__(self, precision=PMX_FP64, device=0, row_capacity=0, lib=None, tune=0, det_waves=0, rows_block=0):
         self.lib = lib or load()
-        opt = pmx_options(precision, device, row_capacity, 0, tune)
+        opt = pmx_options(precision, device, row_capacity, 0, tune, det_waves, rows_block)
         h = C.c_void_p()
         self._call("create", C.byref(opt), C.byref(h))
         self.h = h

@@ -131,3 +131,11 @@ def test_step_times_reproduce_the_reference_accumulation():
 def test_engine_fails_loudly_without_a_device():
     with pytest.raises(pmx.PmxError):
         pmx.Engine()
+
+
+@pytest.mark.parametrize("shape", [{"det_waves": 3}, {"rows_block": 96}, {"det_waves": 16, "rows_block": 128}])
+def test_engine_rejects_unknown_block_shapes(shape):
+    """Fixed block shapes are checked before any device is opened."""
+    with pytest.raises(pmx.PmxError) as err:
+        pmx.Engine(tune=1, **shape)
+    assert "must be" in str(err.value)
