@@ -213,6 +213,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=200)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run an elementwise kernel reading and writing MIB MiB 10 times "
+                         "(calibrates the HBM byte counters) and exit")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -221,6 +224,18 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_probe > 0:
+        # a kernel, not a DMA copy (which counters do not see): bitwise_not
+        # reads n bytes and writes n bytes per dispatch
+        n = args.copy_probe << 20
+        a = torch.ones(n, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(10):
+            torch.bitwise_not(a, out=b)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": n, "copy_probe_bytes_written": n, "dispatches": 10}))
+        return
+
     from parmec_amd import pmx, workloads
 
     dt, W, K = args.dt, args.warmup, args.steps

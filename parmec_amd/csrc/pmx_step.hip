@@ -91,22 +91,25 @@ __global__ void k_bin (int E, const double *center, const int *roots, const int 
       L = -ref-1;
     }
   }
-  /* slot = leaf count before this sphere: one atomic per distinct leaf of the wave */
+  /* slot = leaf count before this sphere: the wave's spheres grouped by
+   * leaf, then one atomic per group, all groups' atomics in one instruction */
   int lane = threadIdx.x & 63;
   unsigned long long todo = __ballot (valid);
   unsigned long long lt = lane ? ((1ull << lane) - 1ull) : 0ull;
-  int s = 0;
+  int leader = lane;
+  unsigned long long grp = 0;
   while (todo)
   {
-    int leader = __ffsll ((long long) todo) - 1;
-    int Lw = __shfl (L, leader, 64);
-    unsigned long long grp = __ballot (valid && L == Lw) & todo;
-    int base = 0;
-    if (lane == leader) base = atomicAdd (&lcount[Lw], __popcll (grp));
-    base = __shfl (base, leader, 64);
-    if ((grp >> lane) & 1ull) s = base + __popcll (grp & lt);
-    todo &= ~grp;
+    int ld = __ffsll ((long long) todo) - 1;
+    int Lw = __shfl (L, ld, 64);
+    unsigned long long g = __ballot (valid && L == Lw) & todo;
+    if ((g >> lane) & 1ull) { leader = ld; grp = g; }
+    todo &= ~g;
   }
+  int base = 0;
+  if (valid && leader == lane) base = atomicAdd (&lcount[L], __popcll (grp));
+  base = __shfl (base, leader, 64);
+  int s = base + __popcll (grp & lt);
   if (!valid) return;
   leafof[e] = L;
   slot[e] = s;
@@ -130,6 +133,16 @@ __global__ void k_scatter (int *ctl, int E, const double *center, const double *
   double2 *rec = (double2*) (lpos + 4*(size_t) pos);
   rec[0] = make_double2 (center[e], center[(size_t) E + e]);
   rec[1] = make_double2 (center[2*(size_t) E + e], radii[e]);
+}
+
+/* Workgroups are dealt to the 8 XCDs round robin (block b to XCD b % 8),
+ * each XCD with its own L2.  Block b takes item xcd_block (b, n) so that
+ * each XCD walks one contiguous stretch of the items: neighbouring leaves
+ * (near in Morton order) then meet in the same L2.  A bijection of [0, n). */
+__device__ __forceinline__ int xcd_block (int b, int n)
+{
+  int x = b & 7, k = b >> 3, q = n >> 3, r = n & 7;
+  return x * q + (x < r ? x : r) + k;
 }
 
 __device__ __forceinline__ unsigned long long readlane_u64 (unsigned long long x, int k)
@@ -175,12 +188,12 @@ void k_detect (int E, const double *center, const double *radii, const int *part
                const int *stop, const int *leafinst)
 {
   if (ctl[CTL_ABORT]) return;
-  if (frozen (stop, leafinst[blockIdx.x], ctl)) return;   /* uniform over the block */
+  int Q = xcd_block (blockIdx.x, gridDim.x);
+  if (frozen (stop, leafinst[Q], ctl)) return;   /* uniform over the block */
   __shared__ double ssph[4*PMX_LSIZE];   /* block spheres: x, y, z, rx */
   __shared__ int sidx[PMX_LSIZE], spart[PMX_LSIZE], shc[PMX_LSIZE];
   __shared__ double wrec[NW][4*PMX_LSIZE];   /* per wave: the candidate's records */
   __shared__ int wmem[NW][PMX_LSIZE], wlist[NW][PMX_LSIZE];
-  int Q = blockIdx.x;
   int t = threadIdx.x, lane = t & 63, w = t >> 6;
   int qoff = loff[Q], qcnt = lcount[Q];
   bool valid = t < qcnt;
