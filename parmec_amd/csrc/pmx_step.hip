@@ -213,6 +213,40 @@ void k_detect (int E, const double *center, const double *radii, const int *part
   double cu[3] = {cell[3*(size_t) nleaves + Q], cell[4*(size_t) nleaves + Q], cell[5*(size_t) nleaves + Q]};
   __syncthreads ();
 
+  /* the AABB of the block's spheres, widened by a relative margin: a pair
+   * that the FP64 test below accepts has |c_a - c_b| < (r_a + r_b)(1 + 2e-10)
+   * along every axis, so a candidate record whose AABB misses this box by
+   * more than 1e-9 (|c_b| + r_b) overlaps no block sphere (the margins stay
+   * far above the rounding of every comparison) */
+  double blo[3] = {0.0, 0.0, 0.0}, bhi[3] = {0.0, 0.0, 0.0};
+  if (qcnt > 0)
+  {
+#pragma unroll
+    for (int d = 0; d < 3; d++) { blo[d] = DBL_MAX; bhi[d] = -DBL_MAX; }
+    for (int k = lane; k < qcnt; k += 64)
+#pragma unroll
+      for (int d = 0; d < 3; d++)
+      {
+        blo[d] = fmin (blo[d], ssph[4*k+d] - ssph[4*k+3]);
+        bhi[d] = fmax (bhi[d], ssph[4*k+d] + ssph[4*k+3]);
+      }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1)
+#pragma unroll
+      for (int d = 0; d < 3; d++)
+      {
+        blo[d] = fmin (blo[d], __shfl_xor (blo[d], o, 64));
+        bhi[d] = fmax (bhi[d], __shfl_xor (bhi[d], o, 64));
+      }
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+    {
+      double tol = 1e-9 * (fabs (blo[d]) + fabs (bhi[d]) + (bhi[d] - blo[d]));
+      blo[d] -= tol;
+      bhi[d] += tol;
+    }
+  }
+
   double *rec = wrec[w];
   int *mem = wmem[w], *lst = wlist[w];
   unsigned long long lt = lane ? ((1ull << lane) - 1ull) : 0ull;
@@ -266,14 +300,42 @@ void k_detect (int E, const double *center, const double *radii, const int *part
       if ((m0 >> lane) & 1ull) lst[__popcll (m0 & lt)] = lane;
       if ((m1 >> lane) & 1ull) lst[n0 + __popcll (m1 & lt)] = lane + 64;
       if (!own)
-        for (int kk = lane; kk < cnt; kk += 64)
+      {
+        /* only the records whose AABB meets the block spheres' box, packed */
+        int kept = 0;
+        for (int b0 = 0; b0 < cnt; b0 += 64)
         {
-          const double2 *rp = (const double2*) (lpos + 4*(size_t) (off + kk));
-          double2 x = rp[0], y = rp[1];
-          rec[4*kk] = x.x; rec[4*kk+1] = x.y; rec[4*kk+2] = y.x; rec[4*kk+3] = y.y;
-          mem[kk] = lmem[off + kk];
+          int kk = b0 + lane;
+          bool ok = false;
+          double2 x = make_double2 (0.0, 0.0), y = x;
+          int mj = 0;
+          if (kk < cnt)
+          {
+            const double2 *rp = (const double2*) (lpos + 4*(size_t) (off + kk));
+            x = rp[0]; y = rp[1];
+            mj = lmem[off + kk];
+            double c[3] = {x.x, x.y, y.x}, r = y.y;
+            ok = true;
+#pragma unroll
+            for (int d = 0; d < 3; d++)
+            {
+              double tb = 1e-9 * (fabs (c[d]) + r);
+              ok = ok && (c[d] - r) - bhi[d] <= tb && blo[d] - (c[d] + r) <= tb;
+            }
+          }
+          unsigned long long mk = __ballot (ok);
+          if (ok)
+          {
+            int p = kept + __popcll (mk & lt);
+            rec[4*p] = x.x; rec[4*p+1] = x.y; rec[4*p+2] = y.x; rec[4*p+3] = y.y;
+            mem[p] = mj;
+          }
+          kept += __popcll (mk);
         }
+        cnt = kept;
+      }
       __builtin_amdgcn_wave_barrier ();
+      if (cnt == 0) continue;
       if (own)
       {
         /* own leaf: every sphere reaches it and the test is symmetric
