@@ -155,7 +155,7 @@ int upload_core (pmx_ctx *c, const pmx_model *m, const std::vector<InstOff> &off
   for (int e = 0; e < m->ellnum; e++) parell[m->part[e]] = e;
   for (int i = 0; i < m->parnum; i++) if (parell[i] < 0) nosph.push_back (i);
   std::vector<signed char> iso (std::max (m->parnum, 1), 0);
-  int niso = 0;
+  int niso = 0, nzc = 0;
   for (int i = 0; i < m->parnum; i++)
   {
     const double *J = m->inertia;
@@ -164,6 +164,16 @@ int upload_core (pmx_ctx *c, const pmx_model *m, const std::vector<InstOff> &off
     for (int k : {1, 2, 3, 5, 6, 7}) d = d && J[k*N + i] == 0.0 && !std::signbit (J[k*N + i]);
     iso[i] = d ? 1 : 0;
     niso += d;
+    /* bit 1: the sphere's body offset (reference centre - reference
+     * position, as the device forms it) is +0 in every component */
+    int e = parell[i];
+    bool zc = e >= 0;
+    for (int k = 0; zc && k < 3; k++)
+    {
+      double C = m->center[(size_t) (3+k)*m->ellnum + e] - m->position[(size_t) (3+k)*N + i];
+      zc = C == 0.0 && !std::signbit (C);
+    }
+    if (zc) { iso[i] |= 2; nzc++; }
   }
   std::vector<int> pinst (std::max (m->parnum, 1)), einst (std::max (m->ellnum, 1)), tinst (std::max (m->trinum, 1));
   std::vector<int> eoff (I + 1), poff (I + 1);
@@ -207,6 +217,7 @@ int upload_core (pmx_ctx *c, const pmx_model *m, const std::vector<InstOff> &off
   PMX_HIP (hipMemsetAsync (c->ovf.p, 0, sizeof (int) * std::max (I, 1), c->stream));
   c->nnosph = (int) nosph.size ();
   c->niso = niso;
+  c->nzc = nzc;
   if ((rc = up (*c, c->iso, iso.data (), N, "iso"))) return rc;
   if ((rc = up (*c, c->parell, parell.data (), N, "parell")) ||
       (rc = up (*c, c->nosph, nosph.data (), nosph.size (), "nosph")))

@@ -198,8 +198,10 @@ struct pmx_ctx
   long long stepped = 0;        /* steps taken by this context over all calls */
   dvec<int> parell;            /* [N] the particle's sphere, or -1 */
   dvec<int> nosph;             /* particles without a sphere */
-  dvec<signed char> iso;       /* [N] 1 when inertia is diag(j, j, j) with exact zeros */
+  dvec<signed char> iso;       /* [N] bit 0: inertia is diag(j, j, j) with exact zeros; bit 1: the
+                                  sphere's body offset is +0 */
   int niso = 0;
+  int nzc = 0;                 /* particles whose sphere offset is +0 */
   int nnosph = 0;
   bool bin_hint = false;       /* leafof holds leaves of the current tree */
   double rmax = 0.0;           /* largest sphere AABB half-width */

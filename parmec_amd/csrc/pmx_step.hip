@@ -766,7 +766,9 @@ __global__ void k_dynamics (int N, double *angular, double *linear, double *rota
   if (ctl[CTL_ABORT]) return;
   int kstep = ctl[CTL_KSTEP];
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N || (iso[i] != 0) != ISO || frozen (stop, par_inst[i], ctl)) return;
+  if (i >= N) return;
+  int flag = iso[i];
+  if ((flag & 1) != (int) ISO || frozen (stop, par_inst[i], ctl)) return;
   const double *damping = dtab + 6*((size_t) kstep*ninst + (ninst > 1 ? par_inst[i] : 0));
   double half = 0.5*step0;
   double step = 0.5*(step0+step1);
@@ -845,8 +847,8 @@ __global__ void k_dynamics (int N, double *angular, double *linear, double *rota
   xn[2] = position[2*(size_t) N + i] + step*v[2];
   /* the particle's sphere (ellipsoids_task, shapes.ispc:29-67) */
   int e = parell ? parell[i] : -1;
-  double C[3] = {0.0, 0.0, 0.0};
-  if (e >= 0)
+  double C[3] = {0.0, 0.0, 0.0};   /* kept when the offset is +0 (flag bit 1) */
+  if (e >= 0 && !(flag & 2))
   {
 #pragma unroll
     for (int k = 0; k < 3; k++) C[k] = center[(size_t) (3+k)*E + e] - position[(size_t) (3+k)*N + i];
@@ -1215,7 +1217,10 @@ int step_rows (pmx_ctx &c, double h)
 
 int step_dynamics (pmx_ctx &c, double step0, double step1)
 {
-  ProfScope ps (c, K_DYNAMICS, 384.0 * c.N + 128.0 * (c.N - c.niso) + 56.0 * c.E);
+  /* per particle: 216 B read and 168 B written (state, force, torque,
+   * masses), 5 B of flags, 128 B more for a full inertia pair; per sphere: the
+   * centre written, 48 B of reference offset read unless it is +0 */
+  ProfScope ps (c, K_DYNAMICS, 389.0 * c.N + 128.0 * (c.N - c.niso) + 24.0 * c.E + 48.0 * (c.E - c.nzc));
 #define DYN_ARGS c.N, c.angular.p, c.linear.p, c.rotation.p, \
                  c.position.p, (const double*) c.inertia.p, (const double*) c.inverse.p, (const double*) c.mass.p, \
                  (const double*) c.invm.p, (const double*) c.dtab.p, (const int*) c.ctl.p, c.force.p, c.torque.p, \
