@@ -176,14 +176,14 @@ int upload_core (pmx_ctx *c, const pmx_model *m, const std::vector<InstOff> &off
     if (zc) { iso[i] |= 2; nzc++; }
   }
   std::vector<int> pinst (std::max (m->parnum, 1)), einst (std::max (m->ellnum, 1)), tinst (std::max (m->trinum, 1));
-  std::vector<int> eoff (I + 1), poff (I + 1);
+  std::vector<int> eoff (I + 1), poff (I + 1), toff (I + 1);
   for (int i = 0; i < I; i++)
   {
     for (int q = off[i].par; q < off[i+1].par; q++) pinst[q] = i;
     for (int q = off[i].ell; q < off[i+1].ell; q++) einst[q] = i;
     for (int q = off[i].tri; q < off[i+1].tri; q++) tinst[q] = i;
   }
-  for (int i = 0; i <= I; i++) { eoff[i] = off[i].ell; poff[i] = off[i].pair; }
+  for (int i = 0; i <= I; i++) { eoff[i] = off[i].ell; poff[i] = off[i].pair; toff[i] = off[i].tri; }
   (void) hipSetDevice (c->device);
   bool same = c->uploaded && c->N == m->parnum && c->E == m->ellnum && c->T == m->trinum && c->I == I;
   for (int i = 0; same && i <= I; i++) same = c->off[i].ell == off[i].ell && c->off[i].tri == off[i].tri;
@@ -209,7 +209,8 @@ int upload_core (pmx_ctx *c, const pmx_model *m, const std::vector<InstOff> &off
   if ((rc = up (*c, c->par_inst, pinst.data (), N, "instances")) || (rc = up (*c, c->ell_inst, einst.data (), E, "instances")) ||
       (rc = up (*c, c->tri_inst, tinst.data (), T, "instances")) ||
       (rc = up (*c, c->ell_off, eoff.data (), (size_t) I + 1, "instances")) ||
-      (rc = up (*c, c->pair_off, poff.data (), (size_t) I + 1, "instances")))
+      (rc = up (*c, c->pair_off, poff.data (), (size_t) I + 1, "instances")) ||
+      (rc = up (*c, c->tri_off, toff.data (), (size_t) I + 1, "instances")))
     return rc;
   if (c->force.alloc (3*std::max (N, (size_t) 1)) || c->torque.alloc (3*std::max (N, (size_t) 1)) ||
       c->ovf.alloc (std::max (I, 1)))

@@ -125,7 +125,7 @@ struct pmx_ctx
    * contiguous range of particles, spheres and triangles and its own tree */
   int I = 1;
   std::vector<InstOff> off;    /* [I+1] prefix offsets of each item family */
-  dvec<int> par_inst, ell_inst, tri_inst, ell_off, pair_off;
+  dvec<int> par_inst, ell_inst, tri_inst, ell_off, pair_off, tri_off;
   dvec<double> bcen;           /* [3][E] centres the current tree was built from */
   dvec<int> ovf;               /* [I] leaf overflow of the instance in the aborted step */
   /* per-instance step counts of a dem call whose instances run different

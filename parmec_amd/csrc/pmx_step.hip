@@ -135,6 +135,24 @@ __global__ void k_scatter (int *ctl, int E, const double *center, const double *
   rec[1] = make_double2 (center[2*(size_t) E + e], radii[e]);
 }
 
+/* FP32 pre-test of a (sphere, record) pair on offsets from an origin O,
+ * x32 = (float) (x - O), r32 = (float) r, with |c_a - O| <= B per axis for
+ * the block sphere a.  With u = 2^-24: a pair that the FP64 test accepts has
+ * |c_a - c_b| < (r_a + r_b)(1 + 1e-15) and so |c_b - O| <= B + |c_a - c_b|;
+ * the rounding of the offsets (1.01u relative), of the difference, the
+ * squares and sums (u per operation) and of s = r32_a + r32_b then keep
+ * sqrt(d2) <= s (1 + 5.7u) + 3.6u B, below thr = s (1 + 16u) + 8u B + 1e-18
+ * even after thr and thr^2 are rounded: such a pair is never rejected.  B >
+ * 1e15 (FP32 offsets could overflow) turns the pre-test off (thr = inf);
+ * NaN and flushed denormals only keep pairs. */
+__device__ __forceinline__ bool near32 (float4 a, float4 b, float k1, float k2)
+{
+  float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+  float d2 = dx*dx + dy*dy + dz*dz;
+  float thr = (a.w + b.w) * k1 + k2;
+  return !(d2 > thr * thr);
+}
+
 /* Workgroups are dealt to the 8 XCDs round robin (block b to XCD b % 8),
  * each XCD with its own L2.  Block b takes item xcd_block (b, n) so that
  * each XCD walks one contiguous stretch of the items: neighbouring leaves
@@ -192,8 +210,10 @@ void k_detect (int E, const double *center, const double *radii, const int *part
   if (frozen (stop, leafinst[Q], ctl)) return;   /* uniform over the block */
   __shared__ double ssph[4*PMX_LSIZE];   /* block spheres: x, y, z, rx */
   __shared__ int sidx[PMX_LSIZE], spart[PMX_LSIZE], shc[PMX_LSIZE];
-  __shared__ double wrec[NW][4*PMX_LSIZE];   /* per wave: the candidate's records */
+  __shared__ float4 ssph32[PMX_LSIZE];       /* block spheres: FP32 offsets from O, radius */
+  __shared__ float4 wrec32[NW][PMX_LSIZE];   /* per wave: the candidate's kept records, the same */
   __shared__ int wmem[NW][PMX_LSIZE], wlist[NW][PMX_LSIZE];
+  __shared__ unsigned long long wmask[NW][128];   /* per wave: reach masks of 64 candidates */
   int t = threadIdx.x, lane = t & 63, w = t >> 6;
   int qoff = loff[Q], qcnt = lcount[Q];
   bool valid = t < qcnt;
@@ -246,8 +266,23 @@ void k_detect (int E, const double *center, const double *radii, const int *part
       bhi[d] += tol;
     }
   }
+  /* origin O of the FP32 offsets: the box centre; B bounds |c - O| per axis
+   * for every block sphere */
+  double O[3], B = 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+  {
+    O[d] = 0.5 * (blo[d] + bhi[d]);
+    B = fmax (B, fmax (bhi[d] - O[d], O[d] - blo[d]));
+  }
+  const float k1 = 1.0f + 0x1p-20f;   /* 1 + 16u, u = 2^-24 */
+  const float k2 = B > 1e15 ? INFINITY : (float) (0x1p-21 * B * (1.0 + 1e-6)) + 1e-18f;   /* 8u B + 1e-18 */
+  if (valid)
+    ssph32[t] = make_float4 ((float) (ssph[4*t] - O[0]), (float) (ssph[4*t+1] - O[1]), (float) (ssph[4*t+2] - O[2]),
+                             (float) ssph[4*t+3]);
+  __syncthreads ();
 
-  double *rec = wrec[w];
+  float4 *rec32 = wrec32[w];
   int *mem = wmem[w], *lst = wlist[w];
   unsigned long long lt = lane ? ((1ull << lane) - 1ull) : 0ull;
   int nc = nbrcnt[Q], c0 = nbroff[Q];
@@ -258,31 +293,47 @@ void k_detect (int E, const double *center, const double *radii, const int *part
      * which block spheres' own AABB reaches its cell (condet.ispc:104-113) */
     int np = ncw - p0 < 64 ? ncw - p0 : 64;
     int bL = 0, bcnt = 0, boff = 0;
-    unsigned long long bm0 = 0, bm1 = 0;
+    double Ll[3] = {0.0, 0.0, 0.0}, Lu[3] = {0.0, 0.0, 0.0};
     if (lane < np)
     {
       bL = nbrlist[c0 + w + (p0 + lane) * NW];
       bcnt = lcount[bL];
       boff = loff[bL];
-      double Ll[3], Lu[3];
 #pragma unroll
       for (int d = 0; d < 3; d++)
       {
         Ll[d] = cell[(size_t) d*nleaves + bL];
         Lu[d] = cell[(size_t) (3+d)*nleaves + bL];
       }
-      if (bcnt > 0)
-        for (int sI = 0; sI < qcnt; sI++)
-        {
-          double x = ssph[4*sI], y = ssph[4*sI+1], z = ssph[4*sI+2], r = ssph[4*sI+3];
-          bool in = x-r <= Lu[0] && y-r <= Lu[1] && z-r <= Lu[2] && x+r >= Ll[0] && y+r >= Ll[1] && z+r >= Ll[2];
-          if (in)
-          {
-            if (sI < 64) bm0 |= 1ull << sI;
-            else bm1 |= 1ull << (sI - 64);
-          }
-        }
     }
+    /* the (candidate, block sphere) pairs spread over the lanes, the
+     * candidate's cell fetched from its lane; reach bits gathered in LDS */
+    unsigned long long *mk = wmask[w];
+    mk[2*lane] = 0ull;
+    mk[2*lane+1] = 0ull;
+    __builtin_amdgcn_wave_barrier ();
+    int work = np * qcnt;
+    /* cand = q / qcnt exactly: q * qcnt < 64 * 96 * 96 < 2^20 */
+    unsigned qinv = qcnt > 0 ? ((1u << 20) + (unsigned) qcnt - 1u) / (unsigned) qcnt : 0u;
+    for (int base = 0; base < work; base += 64)
+    {
+      int q = base + lane;
+      bool ok = q < work;
+      int cand = ok ? (int) (((unsigned) q * qinv) >> 20) : 0;
+      int sI = q - cand * qcnt;
+      double lo[3], hi[3];
+#pragma unroll
+      for (int d = 0; d < 3; d++) { lo[d] = __shfl (Ll[d], cand, 64); hi[d] = __shfl (Lu[d], cand, 64); }
+      if (ok)
+      {
+        double x = ssph[4*sI], y = ssph[4*sI+1], z = ssph[4*sI+2], r = ssph[4*sI+3];
+        bool in = x-r <= hi[0] && y-r <= hi[1] && z-r <= hi[2] && x+r >= lo[0] && y+r >= lo[1] && z+r >= lo[2];
+        if (in) atomicOr (&mk[2*cand + (sI >> 6)], 1ull << (sI & 63));
+      }
+    }
+    __builtin_amdgcn_wave_barrier ();
+    unsigned long long bm0 = 0, bm1 = 0;
+    if (lane < np && bcnt > 0) { bm0 = mk[2*lane]; bm1 = mk[2*lane+1]; }
     /* the candidates that some block sphere reaches, in turn */
     unsigned long long todo = __ballot ((bm0 | bm1) != 0);
     while (todo)
@@ -327,7 +378,7 @@ void k_detect (int E, const double *center, const double *radii, const int *part
           if (ok)
           {
             int p = kept + __popcll (mk & lt);
-            rec[4*p] = x.x; rec[4*p+1] = x.y; rec[4*p+2] = y.x; rec[4*p+3] = y.y;
+            rec32[p] = make_float4 ((float) (x.x - O[0]), (float) (x.y - O[1]), (float) (y.x - O[2]), (float) y.y);
             mem[p] = mj;
           }
           kept += __popcll (mk);
@@ -350,6 +401,7 @@ void k_detect (int E, const double *center, const double *radii, const int *part
           int bb = a + kk + 1;
           bb = bb >= n ? bb - n : bb;
           /* rx = max (r0, r1, r2) is r0 for a sphere (r1, r2 < 0) */
+          if (!near32 (ssph32[a], ssph32[bb], k1, k2)) continue;
           if (!overlap (ssph[4*a], ssph[4*a+1], ssph[4*a+2], ssph[4*a+3],
                         ssph[4*bb], ssph[4*bb+1], ssph[4*bb+2], ssph[4*bb+3]))
             continue;
@@ -369,9 +421,11 @@ void k_detect (int E, const double *center, const double *radii, const int *part
         {
           int a = (int) (((unsigned) q * inv) >> 20), m = q - a * cnt;
           int ti = lst[a];
-          double cx = rec[4*m], cy = rec[4*m+1], cz = rec[4*m+2], rj = rec[4*m+3];
-          if (!overlap (ssph[4*ti], ssph[4*ti+1], ssph[4*ti+2], ssph[4*ti+3], cx, cy, cz, rj)) continue;
+          if (!near32 (ssph32[ti], rec32[m], k1, k2)) continue;
+          /* the record's FP64 values (the ones k_scatter copied into it) */
           int j = mem[m];
+          double cx = center[j], cy = center[(size_t) E + j], cz = center[2*(size_t) E + j], rj = radii[j];
+          if (!overlap (ssph[4*ti], ssph[4*ti+1], ssph[4*ti+2], ssph[4*ti+3], cx, cy, cz, rj)) continue;
           if (spart[ti] == part[j]) continue;
           int e = sidx[ti];
           int slot = atomicAdd (&shc[ti], 1);

@@ -498,14 +498,24 @@ __device__ __forceinline__ int reach_leaves_count (int root, const int *left, co
   return visited;
 }
 
+/* Static triangles of instances with at most TRI_LEAF_MAX triangles are
+ * listed leaf by leaf (k_tri_leaf), the others triangle by triangle through
+ * the tree (k_tri_reach).  Both give the same leaves: the descent (left when
+ * lo <= coord, right when hi >= coord) reaches leaf L exactly when, on every
+ * axis, lo <= the least coord of L's left-turn ancestors and hi >= the
+ * greatest coord of its right-turn ancestors, which are L's cell bounds. */
+#define TRI_LEAF_MAX 64
+
 __global__ void k_tri_reach (int T, const double *tri, const int *tmov, const int *roots, const int *tri_inst,
-                             const int *left, const int *right, const int *dim, const double *coord, int *tricnt,
-                             const int *trioff, int *trilist, int *fill, int *flag)
+                             const int *tri_off, const int *left, const int *right, const int *dim,
+                             const double *coord, int *tricnt, const int *trioff, int *trilist, int *fill, int *flag)
 {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= T) return;
   if (tmov[t]) return;                 /* moving obstacle triangles: reach lists per step */
-  int root = roots[tri_inst[t]];
+  int inst = tri_inst[t];
+  if (tri_off[inst+1] - tri_off[inst] <= TRI_LEAF_MAX) return;   /* listed by k_tri_leaf */
+  int root = roots[inst];
   if (root == PMX_NO_ROOT) return;     /* the triangle's instance has no spheres */
   double v[3][3];
   for (int a = 0; a < 3; a++) for (int b = 0; b < 3; b++) v[a][b] = tri[(size_t) (3*a+b)*T + t];
@@ -519,6 +529,39 @@ __global__ void k_tri_reach (int T, const double *tri, const int *tmov, const in
     lo[b] = mn; hi[b] = mx;
   }
   reach_leaves_count (root, left, right, dim, coord, lo, hi, tricnt, t, trioff, trilist, fill, flag);
+}
+
+/* the static triangles of leaf L's instance whose box reaches L's cell, in
+ * triangle order: counted into tricnt[L] (trilist null) or listed from
+ * trioff[L]; only for instances with at most TRI_LEAF_MAX triangles */
+__global__ void k_tri_leaf (int nleaves, const int *leafinst, const int *tri_off, const int *tmov, const double *tri,
+                            int T, const double *cell, int *tricnt, const int *trioff, int *trilist)
+{
+  int L = blockIdx.x * blockDim.x + threadIdx.x;
+  if (L >= nleaves) return;
+  int inst = leafinst[L];
+  int t0 = tri_off[inst], t1 = tri_off[inst+1];
+  if (t1 - t0 > TRI_LEAF_MAX) return;
+  double clo[3], chi[3];
+  for (int d = 0; d < 3; d++) { clo[d] = cell[(size_t) d*nleaves + L]; chi[d] = cell[(size_t) (3+d)*nleaves + L]; }
+  int n = 0, o = trilist ? trioff[L] : 0;
+  for (int t = t0; t < t1; t++)
+  {
+    if (tmov[t]) continue;
+    bool in = true;
+    for (int d = 0; d < 3; d++)
+    {
+      double x0 = tri[(size_t) d*T + t], x1 = tri[(size_t) (3+d)*T + t], x2 = tri[(size_t) (6+d)*T + t];
+      double mn = x0 < x1 ? x0 : x1, mx = x0 > x1 ? x0 : x1;
+      mn = mn < x2 ? mn : x2;
+      mx = mx > x2 ? mx : x2;
+      in = in && mn <= chi[d] && mx >= clo[d];
+    }
+    if (!in) continue;
+    if (trilist) trilist[o + n] = t;
+    n++;
+  }
+  if (!trilist) tricnt[L] = n;
 }
 
 /* leaves reached by each moving obstacle triangle's current box (the
@@ -801,8 +844,12 @@ int tree_trilists (pmx_ctx &c)
     return PMX_OK;
   }
   int *flag = c.dflags.p + FLAG_TRILIST_OVERFLOW;
+  hipLaunchKernelGGL (k_tri_leaf, grid (t.nleaves), dim3 (PMX_TPB), 0, c.stream, t.nleaves, (const int*) t.leafinst.p,
+                      (const int*) c.tri_off.p, (const int*) c.tmov.p, (const double*) c.tri.p, c.T,
+                      (const double*) t.cell.p, t.tricnt.p, (const int*) nullptr, (int*) nullptr);
   hipLaunchKernelGGL (k_tri_reach, grid (c.T), dim3 (PMX_TPB), 0, c.stream, c.T, (const double*) c.tri.p,
-                      (const int*) c.tmov.p, (const int*) t.roots.p, (const int*) c.tri_inst.p, (const int*) t.left.p, (const int*) t.right.p,
+                      (const int*) c.tmov.p, (const int*) t.roots.p, (const int*) c.tri_inst.p, (const int*) c.tri_off.p,
+                      (const int*) t.left.p, (const int*) t.right.p,
                       (const int*) t.dim.p, (const double*) t.coord.p, t.tricnt.p, (const int*) nullptr, (int*) nullptr,
                       (int*) nullptr, flag);
   PMX_HIP (hipGetLastError ());
@@ -815,8 +862,12 @@ int tree_trilists (pmx_ctx &c)
   dvec<int> fill;
   if (fill.alloc (L)) { pmx_set_error ("out of device memory (trilists)"); return PMX_ENOMEM; }
   hipLaunchKernelGGL (k_fill_int, grid (L), dim3 (PMX_TPB), 0, c.stream, fill.p, L, 0);
+  hipLaunchKernelGGL (k_tri_leaf, grid (t.nleaves), dim3 (PMX_TPB), 0, c.stream, t.nleaves, (const int*) t.leafinst.p,
+                      (const int*) c.tri_off.p, (const int*) c.tmov.p, (const double*) c.tri.p, c.T,
+                      (const double*) t.cell.p, (int*) nullptr, (const int*) t.trioff.p, t.trilist.p);
   hipLaunchKernelGGL (k_tri_reach, grid (c.T), dim3 (PMX_TPB), 0, c.stream, c.T, (const double*) c.tri.p,
-                      (const int*) c.tmov.p, (const int*) t.roots.p, (const int*) c.tri_inst.p, (const int*) t.left.p, (const int*) t.right.p,
+                      (const int*) c.tmov.p, (const int*) t.roots.p, (const int*) c.tri_inst.p, (const int*) c.tri_off.p,
+                      (const int*) t.left.p, (const int*) t.right.p,
                       (const int*) t.dim.p, (const double*) t.coord.p, (int*) nullptr, (const int*) t.trioff.p,
                       t.trilist.p, fill.p, flag);
   PMX_HIP (hipGetLastError ());
