@@ -275,6 +275,7 @@ int step_mtri (pmx_ctx &c);                  /* leaf -> moving obstacle triangle
 
 /* ---- scans (pmx_tree.hip) ---- */
 int scan_exclusive (pmx_ctx &c, int *data, int n, int *total_host, const int *ctl = nullptr);
+int scan_exclusive_to (pmx_ctx &c, const int *src, int *dst, int n, const int *ctl);
 
 /* ---- step kernels (pmx_step.hip); step_* only enqueue work ---- */
 int step_prepare (pmx_ctx &c);               /* per-tree buffers, after a (re)build */

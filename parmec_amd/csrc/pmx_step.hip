@@ -43,6 +43,12 @@ __global__ void k_fill_i (const int *ctl, int *a, int n, int v)
   if (i < n) a[i] = v;
 }
 
+__global__ void k_zero_i (int *a, int n)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
+}
+
 __global__ void k_copy_i (const int *ctl, int *dst, const int *src, int n)
 {
   if (ctl[CTL_ABORT]) return;
@@ -122,12 +128,13 @@ __global__ void k_bin (int E, const double *center, const int *roots, const int 
 }
 
 __global__ void k_scatter (int *ctl, int E, const double *center, const double *radii, const int *leafof, const int *slot,
-                           const int *loff, int *lmem, double *lpos, const int *stop, const int *ell_inst)
+                           const int *loff, int *lmem, double *lpos, const int *stop, const int *ell_inst, int *xcnt)
 {
   if (ctl[CTL_ABORT]) return;
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e == 0) ctl[CTL_HINT] = 1;   /* the bin of this step completed: leafof is valid */
   if (e >= E || frozen (stop, ell_inst[e], ctl)) return;
+  xcnt[e] = 0;   /* reverse hits of this step start at zero */
   int pos = loff[leafof[e]] + slot[e];
   lmem[pos] = e;
   double2 *rec = (double2*) (lpos + 4*(size_t) pos);
@@ -523,12 +530,16 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
              uint32_t *key0, R *st0, int *cnt0, uint32_t *key1, R *st1, int *cnt1, int cur0, int cap,
              uint32_t *hits, const int *hitcnt, int hcap, const int *xbuf, const int *xcnt, int xcap,
              const double *gtab, const int *ctl, double h, double *force, double *torque, int *flags,
-             const double *obspnt, const double *obsang, const double *obslin, const int *stop)
+             const double *obspnt, const double *obsang, const double *obslin, const int *stop,
+             int *lcount, int nleaves)
 {
   __shared__ uint32_t sh[ROWS_SORT * TPB];
   __shared__ int lst[TPB];         /* the block's busy spheres, in order */
   __shared__ int wbase[TPB / 64];  /* busy spheres of the waves before */
   if (ctl[CTL_ABORT]) return;
+  /* the leaf counts were last read by k_detect: cleared for the next bin
+   * (nleaves <= E: every leaf held a sphere when the tree was built) */
+  if (blockIdx.x * TPB + threadIdx.x < nleaves) lcount[blockIdx.x * TPB + threadIdx.x] = 0;
   int kstep = ctl[CTL_KSTEP];
   bool odd = ((cur0 + kstep) & 1) != 0;   /* rows of the previous step */
   constexpr bool mixed = sizeof (R) == 4;
@@ -1119,10 +1130,22 @@ __global__ void k_invert (int N, const double *inertia, double *inverse, const d
 
 } /* namespace */
 
+/* zero leaf counts, whatever the control block says (after a build, or
+ * after runs outside a step) */
+int clear_leaf_counts (pmx_ctx &c)
+{
+  int L = std::max (c.tree.nleaves, 1);
+  hipLaunchKernelGGL (k_zero_i, grid (L), dim3 (PMX_TPB), 0, c.stream, c.lcount.p, L);
+  PMX_HIP (hipGetLastError ());
+  return PMX_OK;
+}
+
 int step_prepare (pmx_ctx &c)
 {
   int L = std::max (c.tree.nleaves, 1);
   if (c.lcount.alloc (L) || c.loff.alloc (L)) { pmx_set_error ("out of device memory (bins)"); return PMX_ENOMEM; }
+  int rc = clear_leaf_counts (c);
+  if (rc) return rc;
   /* size the scan scratch for the per-step leaf scan now: nothing may
    * allocate while a step is being captured */
   size_t need = (size_t) L / (PMX_TPB*4) + 64;
@@ -1135,8 +1158,8 @@ int step_prepare (pmx_ctx &c)
 int step_bin (pmx_ctx &c)
 {
   TreeDev &t = c.tree;
-  int L = std::max (t.nleaves, 1);
-  hipLaunchKernelGGL (k_fill_i, grid (L), dim3 (PMX_TPB), 0, c.stream, (const int*) c.ctl.p, c.lcount.p, L, 0);
+  /* the leaf counts are zero here: cleared by step_prepare after a build,
+   * by k_rows in every step and by the autotuner after its runs */
   {
     ProfScope ps (c, K_BIN, 32.0 * c.E);
     hipLaunchKernelGGL (k_bin, grid (c.E), dim3 (PMX_TPB), 0, c.stream, c.E, (const double*) c.center.p,
@@ -1154,18 +1177,15 @@ int step_scatter (pmx_ctx &c)
 {
   TreeDev &t = c.tree;
   int L = std::max (t.nleaves, 1);
-  hipLaunchKernelGGL (k_copy_i, grid (L), dim3 (PMX_TPB), 0, c.stream, (const int*) c.ctl.p, c.loff.p,
-                      (const int*) c.lcount.p, L);
-  int rc = scan_exclusive (c, c.loff.p, L, nullptr, c.ctl.p);
+  int rc = scan_exclusive_to (c, (const int*) c.lcount.p, c.loff.p, L, c.ctl.p);
   if (rc) return rc;
   {
     ProfScope ps (c, K_SCATTER, 76.0 * c.E);
     hipLaunchKernelGGL (k_scatter, grid (c.E), dim3 (PMX_TPB), 0, c.stream, c.ctl.p, c.E, (const double*) c.center.p,
                         (const double*) c.radii.p, (const int*) c.leafof.p, (const int*) c.slot.p, (const int*) c.loff.p,
                         c.lmem.p, c.lpos.p, c.staggered ? (const int*) c.istop.p : (const int*) nullptr,
-                        (const int*) c.ell_inst.p);
+                        (const int*) c.ell_inst.p, c.xcnt.p);
   }
-  hipLaunchKernelGGL (k_fill_i, grid (c.E), dim3 (PMX_TPB), 0, c.stream, (const int*) c.ctl.p, c.xcnt.p, c.E, 0);
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
 }
@@ -1230,7 +1250,7 @@ void rows_launch (pmx_ctx &c, R *st0, R *st1, double h)
                       (const int*) c.xbuf.p, (const int*) c.xcnt.p, c.xcap, (const double*) c.gtab.p,
                       (const int*) c.ctl.p, h, c.force.p, c.torque.p, c.dflags.p,
                       (const double*) c.obspnt.p, (const double*) c.obsang.p, (const double*) c.obslin.p,
-                      c.staggered ? (const int*) c.istop.p : (const int*) nullptr);
+                      c.staggered ? (const int*) c.istop.p : (const int*) nullptr, c.lcount.p, c.tree.nleaves);
 }
 
 template <class R>
@@ -1398,7 +1418,7 @@ int autotune (pmx_ctx &c, double h)
   int ctl[CTL_COUNT];
   PMX_HIP (hipMemcpyAsync (ctl, c.ctl.p, sizeof (ctl), hipMemcpyDeviceToHost, c.stream));
   PMX_HIP (hipStreamSynchronize (c.stream));
-  if (ctl[CTL_ABORT]) return PMX_OK;   /* a leaf overflows: tune at a later call */
+  if (ctl[CTL_ABORT]) return clear_leaf_counts (c);   /* a leaf overflows: tune at a later call */
   hipEvent_t e0, e1;
   PMX_HIP (hipEventCreate (&e0));
   PMX_HIP (hipEventCreate (&e1));
@@ -1430,5 +1450,6 @@ int autotune (pmx_ctx &c, double h)
   if (nw < 0) { pmx_set_error ("autotuning failed"); return PMX_EDEVICE; }
   c.det_nw = nw;
   c.tuned++;
+  if ((rc = clear_leaf_counts (c))) return rc;   /* the next bin starts from zero */
   return PMX_OK;
 }

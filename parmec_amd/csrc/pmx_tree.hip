@@ -47,14 +47,14 @@ __device__ __forceinline__ int delta (int i, unsigned codei, int j, int lo, int 
 
 /* ---------------------------------------------------------------- scan */
 
-__global__ void k_scan_block (int *data, int n, int *bsums, const int *ctl)
+__global__ void k_scan_block (const int *src, int *data, int n, int *bsums, const int *ctl)
 {
   if (ctl && ctl[CTL_ABORT]) return;
   __shared__ int wsum[PMX_TPB/64];
   int base = blockIdx.x * (PMX_TPB*4) + threadIdx.x * 4;
   int v[4], s = 0;
 #pragma unroll
-  for (int k = 0; k < 4; k++) { v[k] = base+k < n ? data[base+k] : 0; s += v[k]; }
+  for (int k = 0; k < 4; k++) { v[k] = base+k < n ? src[base+k] : 0; s += v[k]; }
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int x = s;
 #pragma unroll
@@ -80,20 +80,21 @@ __global__ void k_scan_add (int *data, int n, const int *boffs, const int *ctl)
   if (i < n) data[i] += boffs[i / (PMX_TPB*4)];
 }
 
-int scan_rec (pmx_ctx &c, int *data, int n, int *scratch, size_t scratch_n, const int *ctl)
+/* exclusive scan of src into data (src may be data) */
+int scan_rec (pmx_ctx &c, const int *src, int *data, int n, int *scratch, size_t scratch_n, const int *ctl)
 {
   const int per = PMX_TPB*4;
   int nb = (n + per - 1) / per;
   if (nb <= 1)
   {
-    hipLaunchKernelGGL (k_scan_block, dim3 (1), dim3 (PMX_TPB), 0, c.stream, data, n, (int*) nullptr, ctl);
+    hipLaunchKernelGGL (k_scan_block, dim3 (1), dim3 (PMX_TPB), 0, c.stream, src, data, n, (int*) nullptr, ctl);
     PMX_HIP (hipGetLastError ());
     return PMX_OK;
   }
   if ((size_t) nb > scratch_n) { pmx_set_error ("scan scratch too small"); return PMX_ENOMEM; }
-  hipLaunchKernelGGL (k_scan_block, dim3 (nb), dim3 (PMX_TPB), 0, c.stream, data, n, scratch, ctl);
+  hipLaunchKernelGGL (k_scan_block, dim3 (nb), dim3 (PMX_TPB), 0, c.stream, src, data, n, scratch, ctl);
   PMX_HIP (hipGetLastError ());
-  int rc = scan_rec (c, scratch, nb, scratch + nb, scratch_n - nb, ctl);
+  int rc = scan_rec (c, scratch, scratch, nb, scratch + nb, scratch_n - nb, ctl);
   if (rc) return rc;
   hipLaunchKernelGGL (k_scan_add, dim3 ((n + PMX_TPB - 1)/PMX_TPB), dim3 (PMX_TPB), 0, c.stream, data, n, (const int*) scratch, ctl);
   PMX_HIP (hipGetLastError ());
@@ -695,7 +696,7 @@ int scan_exclusive (pmx_ctx &c, int *data, int n, int *total_host, const int *ct
   int last = 0;
   if (total_host)
     PMX_HIP (hipMemcpyAsync (&last, data + n - 1, sizeof (int), hipMemcpyDeviceToHost, c.stream));
-  int rc = scan_rec (c, data, n, c.tree.scanbuf.p, c.tree.scanbuf.n, ctl);
+  int rc = scan_rec (c, data, data, n, c.tree.scanbuf.p, c.tree.scanbuf.n, ctl);
   if (rc) return rc;
   if (total_host)
   {
@@ -705,6 +706,16 @@ int scan_exclusive (pmx_ctx &c, int *data, int n, int *total_host, const int *ct
     *total_host = lastex + last;
   }
   return PMX_OK;
+}
+
+/* exclusive scan of src into dst inside a step (scratch sized by
+ * step_prepare; no allocation, so it can be captured) */
+int scan_exclusive_to (pmx_ctx &c, const int *src, int *dst, int n, const int *ctl)
+{
+  if (n <= 0) return PMX_OK;
+  size_t need = (size_t) n / (PMX_TPB*4) + 64;
+  if (need > c.tree.scanbuf.n) { pmx_set_error ("scan scratch not prepared"); return PMX_ENOMEM; }
+  return scan_rec (c, src, dst, n, c.tree.scanbuf.p, c.tree.scanbuf.n, ctl);
 }
 
 int tree_build (pmx_ctx &c, bool all)
