@@ -170,7 +170,8 @@ typedef struct pmx_options
                           spheres or more: at the first dem call and again after 256
                           steps); 1: the fixed det_waves below */
   int det_waves;       /* tune = 1: waves per detection block, 2, 4 or 8 (0: 4) */
-  int rows_block;      /* threads per contact-row block, 64, 128, 256 or 512 (0: 512) */
+  int rows_block;      /* spheres per contact-row block (per list of spheres in contact), 64, 128,
+                          256, 512 or 1024 (0: 1024) */
 } pmx_options;
 
 /* one contact of the canonical pair set */

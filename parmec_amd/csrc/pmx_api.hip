@@ -50,7 +50,8 @@ int alloc_rows (pmx_ctx &c)
         (mixed ? c.rowst32[b].alloc (3 * (size_t) c.cap * E) : c.rowst[b].alloc (3 * (size_t) c.cap * E)))
       return fail (PMX_ENOMEM, "out of device memory (contact rows)");
   if (c.hits.alloc ((size_t) c.hcap * E) || c.hitcnt.alloc (E) || c.xbuf.alloc ((size_t) c.xcap * E) ||
-      c.xcnt.alloc (E) || c.leafof.alloc (E) || c.slot.alloc (E) || c.lmem.alloc (E) || c.lpos.alloc (4 * E))
+      c.xcnt.alloc (E) || c.rbusy.alloc (E + 1024) || c.rnbusy.alloc (E / 64 + 2) || c.leafof.alloc (E) ||
+      c.slot.alloc (E) || c.lmem.alloc (E) || c.lpos.alloc (4 * E))
     return fail (PMX_ENOMEM, "out of device memory (broad phase)");
   PMX_HIP (hipMemsetAsync (c.rowcnt[0].p, 0, E * sizeof (int), c.stream));
   PMX_HIP (hipMemsetAsync (c.rowcnt[1].p, 0, E * sizeof (int), c.stream));
@@ -338,8 +339,8 @@ int pmx_create (const pmx_options *opt, pmx_ctx **out)
   if (c->opt.det_waves != 0 && c->opt.det_waves != 2 && c->opt.det_waves != 4 && c->opt.det_waves != 8)
   { delete c; return fail (PMX_EINVAL, "det_waves must be 2, 4 or 8"); }
   if (c->opt.rows_block != 0 && c->opt.rows_block != 64 && c->opt.rows_block != 128 && c->opt.rows_block != 256 &&
-      c->opt.rows_block != 512)
-  { delete c; return fail (PMX_EINVAL, "rows_block must be 64, 128, 256 or 512"); }
+      c->opt.rows_block != 512 && c->opt.rows_block != 1024)
+  { delete c; return fail (PMX_EINVAL, "rows_block must be 64, 128, 256, 512 or 1024"); }
   if (c->opt.det_waves) c->det_nw = c->opt.det_waves;
   if (c->opt.rows_block) c->rows_tpb = c->opt.rows_block;
   int ndev = 0;

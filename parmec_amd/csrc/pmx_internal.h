@@ -187,13 +187,14 @@ struct pmx_ctx
   dvec<int> hitcnt;            /* [E] */
   dvec<int> xbuf;              /* [xcap][E] reverse hits */
   dvec<int> xcnt;              /* [E] */
+  dvec<int> rbusy, rnbusy;     /* per block of k_rows_idle: its busy spheres, their count */
   dvec<int> dflags;            /* [FLAG_COUNT] */
   dvec<int> ctl;               /* [CTL_COUNT] */
   int cur0 = 0;                /* row buffer holding the rows at step 0 of the call */
   hipGraphExec_t gexec = nullptr;
   bool graph_valid = false;
   int det_nw = 4;              /* waves per k_detect block (autotuned) */
-  int rows_tpb = 512;          /* threads per k_rows block */
+  int rows_tpb = 1024;         /* threads per k_rows_idle block (spheres per busy list) */
   int tuned = 0;                /* autotuner passes done: at the first call, then once settled */
   long long stepped = 0;        /* steps taken by this context over all calls */
   dvec<int> parell;            /* [N] the particle's sphere, or -1 */

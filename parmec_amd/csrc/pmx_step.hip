@@ -514,14 +514,62 @@ __global__ void k_force_init (const int *ctl, int N, const int *list, int nlist,
 
 #define ROWS_SORT 8   /* hit keys a thread sorts in LDS (more: sorted in the hit array) */
 
-/* row merge + contact forces + per-sphere accumulation.  R is the arithmetic
- * of the contact geometry and force law: double (the FP64 path) or float
- * (mixed precision: differences of FP64 positions are formed in FP64 and then
- * narrowed, the depth that decides whether a contact persists stays FP64, the
- * tangential state is stored in FP32 and the per-sphere sums are
- * Kahan-compensated FP32) */
-template <class R, int TPB>
+/* The rows of a step in two launches.  k_rows_idle (TPB threads per block,
+ * few registers) takes every sphere with neither a contact in the previous
+ * step's row nor a hit in this step: its row is empty and its sums are
+ * zero, so the merge would give force 0 + m g and torque 0, written here
+ * directly.  The block's other (busy) spheres are listed, in order, in the
+ * block's segment of the busy list (no atomics).  k_rows then merges the
+ * busy spheres' rows with one 64-thread block per list slice, so the long
+ * per-sphere merges do not hold the idle spheres' blocks resident. */
+template <int TPB>
 __global__ __launch_bounds__(TPB)
+void k_rows_idle (int E, int N, const int *part, const double *mass, int ninst, const int *ell_inst, int *cnt0,
+                  int *cnt1, int cur0, const int *hitcnt, const int *xcnt, const double *gtab, const int *ctl,
+                  double *force, double *torque, const int *stop, int *lcount, int nleaves, int *busy, int *nbusy)
+{
+  __shared__ int wbase[TPB / 64];  /* busy spheres of the waves before */
+  if (ctl[CTL_ABORT]) return;
+  /* the leaf counts were last read by k_detect: cleared for the next bin
+   * (nleaves <= E: every leaf held a sphere when the tree was built) */
+  int e = blockIdx.x * TPB + threadIdx.x;
+  if (e < nleaves) lcount[e] = 0;
+  int kstep = ctl[CTL_KSTEP];
+  bool odd = ((cur0 + kstep) & 1) != 0;   /* rows of the previous step */
+  const int *ocnt = odd ? cnt1 : cnt0;
+  int *ncnt = odd ? cnt0 : cnt1;
+  bool live = e < E && !frozen (stop, ell_inst[e], ctl);
+  bool bs = live && (hitcnt[e] | xcnt[e] | ocnt[e]) != 0;
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  unsigned long long m = __ballot (bs);
+  if (lane == 0) wbase[w] = __popcll (m);
+  __syncthreads ();
+  int base = 0, tot = 0;
+  for (int k = 0; k < TPB / 64; k++) { base += k < w ? wbase[k] : 0; tot += wbase[k]; }
+  if (bs) busy[(size_t) blockIdx.x * TPB + base + __popcll (m & ((1ull << lane) - 1ull))] = e;
+  if (threadIdx.x == 0) nbusy[blockIdx.x] = tot;
+  if (!live || bs) return;
+  ncnt[e] = 0;
+  int pe = part[e];
+  const double *g = gtab + 3*((size_t) kstep*ninst + (ninst > 1 ? ell_inst[e] : 0));
+  double ma = mass[pe];
+  force[pe] = 0.0 + ma * g[0];
+  force[(size_t) N + pe] = 0.0 + ma * g[1];
+  force[2*(size_t) N + pe] = 0.0 + ma * g[2];
+  torque[pe] = 0.0;
+  torque[(size_t) N + pe] = 0.0;
+  torque[2*(size_t) N + pe] = 0.0;
+}
+
+/* row merge + contact forces + per-sphere accumulation of the busy spheres.
+ * R is the arithmetic of the contact geometry and force law: double (the
+ * FP64 path) or float (mixed precision: differences of FP64 positions are
+ * formed in FP64 and then narrowed, the depth that decides whether a contact
+ * persists stays FP64, the tangential state is stored in FP32 and the
+ * per-sphere sums are Kahan-compensated FP32).  Block j takes slice
+ * j % slices of list j / slices (lists of LIST entries at most). */
+template <class R>
+__global__ __launch_bounds__(64)
 void k_rows (int E, int N, const double *center, const double *radii, const int *part, const int *ellcol,
              const double *tri, int T, const int *tricol, const int *triobs,
              const double *position, const double *linear, const double *angular, const double *mass,
@@ -530,16 +578,12 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
              uint32_t *key0, R *st0, int *cnt0, uint32_t *key1, R *st1, int *cnt1, int cur0, int cap,
              uint32_t *hits, const int *hitcnt, int hcap, const int *xbuf, const int *xcnt, int xcap,
              const double *gtab, const int *ctl, double h, double *force, double *torque, int *flags,
-             const double *obspnt, const double *obsang, const double *obslin, const int *stop,
-             int *lcount, int nleaves)
+             const double *obspnt, const double *obsang, const double *obslin,
+             const int *busy, const int *nbusy, int list, int slices)
 {
+  constexpr int TPB = 64;
   __shared__ uint32_t sh[ROWS_SORT * TPB];
-  __shared__ int lst[TPB];         /* the block's busy spheres, in order */
-  __shared__ int wbase[TPB / 64];  /* busy spheres of the waves before */
   if (ctl[CTL_ABORT]) return;
-  /* the leaf counts were last read by k_detect: cleared for the next bin
-   * (nleaves <= E: every leaf held a sphere when the tree was built) */
-  if (blockIdx.x * TPB + threadIdx.x < nleaves) lcount[blockIdx.x * TPB + threadIdx.x] = 0;
   int kstep = ctl[CTL_KSTEP];
   bool odd = ((cur0 + kstep) & 1) != 0;   /* rows of the previous step */
   constexpr bool mixed = sizeof (R) == 4;
@@ -549,44 +593,13 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
   uint32_t *nkey = odd ? key0 : key1;
   R *nst = odd ? st0 : st1;
   int *ncnt = odd ? cnt0 : cnt1;
-
-  /* A sphere with neither a contact in the previous step's row nor a hit in
-   * this step has an empty row and zero sums: the merge below would give
-   * force 0 + m g and torque 0, written here directly.  The block's other
-   * (busy) spheres are packed in order onto its first threads, so that the
-   * merge runs in as few waves as there are busy spheres / 64 */
+  int lb = blockIdx.x / slices, sl = blockIdx.x - lb * slices;
+  int nb = nbusy[lb];
+  int mx = 0;   /* longest row of the thread's spheres */
+  for (int q = sl * TPB + threadIdx.x; q < nb; q += TPB * slices)
   {
-    int e = blockIdx.x * TPB + threadIdx.x;
-    bool live = e < E && !frozen (stop, ell_inst[e], ctl);
-    bool busy = live && (hitcnt[e] | xcnt[e] | ocnt[e]) != 0;
-    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    unsigned long long m = __ballot (busy);
-    if (lane == 0) wbase[w] = __popcll (m);
-    __syncthreads ();
-    int base = 0;
-    for (int k = 0; k < w; k++) base += wbase[k];
-    if (busy) lst[base + __popcll (m & ((1ull << lane) - 1ull))] = e;
-    else if (live)
-    {
-      ncnt[e] = 0;
-      int pe = part[e];
-      const double *g = gtab + 3*((size_t) kstep*ninst + (ninst > 1 ? ell_inst[e] : 0));
-      double ma = mass[pe];
-      force[pe] = 0.0 + ma * g[0];
-      force[(size_t) N + pe] = 0.0 + ma * g[1];
-      force[2*(size_t) N + pe] = 0.0 + ma * g[2];
-      torque[pe] = 0.0;
-      torque[(size_t) N + pe] = 0.0;
-      torque[2*(size_t) N + pe] = 0.0;
-    }
-    __syncthreads ();
-  }
-  int nbusy = 0;
-  for (int k = 0; k < TPB / 64; k++) nbusy += wbase[k];
   int cnt = 0;
-  if (threadIdx.x < nbusy)
-  {
-  int e = lst[threadIdx.x];
+  int e = busy[(size_t) lb * list + q];
   /* the step's hit keys (own hits, then reverse hits), sorted: in a small
    * LDS buffer, or in place in the sphere's column of the hit array when
    * there are more (both orders are the same unique sorted set) */
@@ -805,8 +818,8 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
   torque[pe] = (double) ts[0];
   torque[(size_t) N + pe] = (double) ts[1];
   torque[2*(size_t) N + pe] = (double) ts[2];
+  mx = cnt > mx ? cnt : mx;
   }
-  int mx = cnt;
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1)
   {
@@ -1235,10 +1248,14 @@ int step_detect (pmx_ctx &c)
   return launch_detect (c, c.det_nw);
 }
 
-template <class R, int TPB>
-void rows_launch (pmx_ctx &c, R *st0, R *st1, double h)
+/* k_rows over the busy lists of nb idle blocks of tpb threads: one block
+ * per list, or 8 per list (slices) when there are few lists, so that small
+ * models spread their busy spheres over more waves */
+template <class R>
+void rows_launch (pmx_ctx &c, R *st0, R *st1, double h, int tpb, int nb)
 {
-  hipLaunchKernelGGL ((k_rows<R, TPB>), dim3 ((c.E + TPB - 1) / TPB), dim3 (TPB), 0, c.stream, c.E, c.N,
+  int slices = nb < 4096 ? 8 : 1;
+  hipLaunchKernelGGL ((k_rows<R>), dim3 (nb * slices), dim3 (64), 0, c.stream, c.E, c.N,
                       (const double*) c.center.p, (const double*) c.radii.p, (const int*) c.part.p,
                       (const int*) c.ellcol.p, (const double*) c.tri.p, c.T, (const int*) c.tricol.p,
                       (const int*) c.triobs.p, (const double*) c.position.p, (const double*) c.linear.p,
@@ -1250,25 +1267,34 @@ void rows_launch (pmx_ctx &c, R *st0, R *st1, double h)
                       (const int*) c.xbuf.p, (const int*) c.xcnt.p, c.xcap, (const double*) c.gtab.p,
                       (const int*) c.ctl.p, h, c.force.p, c.torque.p, c.dflags.p,
                       (const double*) c.obspnt.p, (const double*) c.obsang.p, (const double*) c.obslin.p,
-                      c.staggered ? (const int*) c.istop.p : (const int*) nullptr, c.lcount.p, c.tree.nleaves);
+                      (const int*) c.rbusy.p, (const int*) c.rnbusy.p, tpb, slices);
 }
 
-template <class R>
-void rows_launch_tpb (pmx_ctx &c, int tpb, R *st0, R *st1, double h)
+template <int TPB>
+void rows_idle_launch (pmx_ctx &c)
 {
-  if (tpb == 64) rows_launch<R, 64> (c, st0, st1, h);
-  else if (tpb == 256) rows_launch<R, 256> (c, st0, st1, h);
-  else if (tpb == 512) rows_launch<R, 512> (c, st0, st1, h);
-  else rows_launch<R, 128> (c, st0, st1, h);
+  hipLaunchKernelGGL ((k_rows_idle<TPB>), dim3 ((c.E + TPB - 1) / TPB), dim3 (TPB), 0, c.stream, c.E, c.N,
+                      (const int*) c.part.p, (const double*) c.mass.p, c.I, (const int*) c.ell_inst.p,
+                      c.rowcnt[0].p, c.rowcnt[1].p, c.cur0, (const int*) c.hitcnt.p, (const int*) c.xcnt.p,
+                      (const double*) c.gtab.p, (const int*) c.ctl.p, c.force.p, c.torque.p,
+                      c.staggered ? (const int*) c.istop.p : (const int*) nullptr, c.lcount.p, c.tree.nleaves,
+                      c.rbusy.p, c.rnbusy.p);
 }
 
-/* k_rows with tpb threads per block (64, 128, 256 or 512) */
+/* the rows of a step: k_rows_idle with tpb threads per block (64 to 1024),
+ * then k_rows over the busy lists */
 int launch_rows (pmx_ctx &c, int tpb, double h)
 {
   if (c.E == 0) return PMX_OK;
   ProfScope ps (c, K_ROWS, 136.0 * c.E);
-  if (c.opt.precision == PMX_MIXED) rows_launch_tpb<float> (c, tpb, c.rowst32[0].p, c.rowst32[1].p, h);
-  else rows_launch_tpb<double> (c, tpb, c.rowst[0].p, c.rowst[1].p, h);
+  int nb = (c.E + tpb - 1) / tpb;
+  if (tpb == 64) rows_idle_launch<64> (c);
+  else if (tpb == 128) rows_idle_launch<128> (c);
+  else if (tpb == 256) rows_idle_launch<256> (c);
+  else if (tpb == 512) rows_idle_launch<512> (c);
+  else rows_idle_launch<1024> (c);
+  if (c.opt.precision == PMX_MIXED) rows_launch<float> (c, c.rowst32[0].p, c.rowst32[1].p, h, tpb, nb);
+  else rows_launch<double> (c, c.rowst[0].p, c.rowst[1].p, h, tpb, nb);
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
 }
