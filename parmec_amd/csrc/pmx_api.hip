@@ -157,6 +157,15 @@ int upload_core (pmx_ctx *c, const pmx_model *m, const std::vector<InstOff> &off
   for (int i = 0; i < m->parnum; i++) if (parell[i] < 0) nosph.push_back (i);
   std::vector<signed char> iso (std::max (m->parnum, 1), 0);
   int niso = 0, nzc = 0;
+  /* particles whose force another stage adds to or sets (springs, prescribed
+   * motion): their force always goes through the force array */
+  std::vector<char> engaged (std::max (m->parnum, 1), 0);
+  for (int q = 0; q < m->sprnum; q++)
+  {
+    engaged[m->sprpart[q]] = 1;
+    if (m->sprpart[m->sprnum + q] >= 0) engaged[m->sprpart[m->sprnum + q]] = 1;
+  }
+  for (int q = 0; q < m->prsnum; q++) engaged[m->prspart[q]] = 1;
   for (int i = 0; i < m->parnum; i++)
   {
     const double *J = m->inertia;
@@ -175,6 +184,10 @@ int upload_core (pmx_ctx *c, const pmx_model *m, const std::vector<InstOff> &off
       zc = C == 0.0 && !std::signbit (C);
     }
     if (zc) { iso[i] |= 2; nzc++; }
+    /* bit 2: a sphere particle that nothing but its contacts and gravity
+     * loads: when its row ends up empty its force is 0 + m g and its torque
+     * 0, formed where they are used instead of stored */
+    if (e >= 0 && !engaged[i]) iso[i] |= 4;
   }
   std::vector<int> pinst (std::max (m->parnum, 1)), einst (std::max (m->ellnum, 1)), tinst (std::max (m->trinum, 1));
   std::vector<int> eoff (I + 1), poff (I + 1), toff (I + 1);
@@ -646,9 +659,14 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
     bool any = false;
     for (int i = 0; i < I; i++) any = any || (!reported[i] && nst[i] <= upto);
     if (!any) return PMX_OK;
+    /* forces and torques that the step formed where they were used */
+    std::vector<int> fin (I, -1);
+    for (int i = 0; i < I; i++) if (!reported[i] && nst[i] <= upto && nst[i] > 0) fin[i] = nst[i] - 1;
+    int r = step_materialize (*c, fin);
+    if (r) return r;
     /* mesh geometry of the current particle state (finished instances are
      * frozen at their final state) */
-    int r = step_mesh_shapes (*c, false);
+    r = step_mesh_shapes (*c, false);
     if (r) return r;
     PMX_HIP (hipStreamSynchronize (c->stream));
     for (int i = 0; i < I; i++)

@@ -188,6 +188,7 @@ struct pmx_ctx
   dvec<int> xbuf;              /* [xcap][E] reverse hits */
   dvec<int> xcnt;              /* [E] */
   dvec<int> rbusy, rnbusy;     /* per block of k_rows_idle: its busy spheres, their count */
+  dvec<int> fin;               /* [I] last step of each instance finishing at a report, or -1 */
   dvec<int> dflags;            /* [FLAG_COUNT] */
   dvec<int> ctl;               /* [CTL_COUNT] */
   int cur0 = 0;                /* row buffer holding the rows at step 0 of the call */
@@ -200,7 +201,8 @@ struct pmx_ctx
   dvec<int> parell;            /* [N] the particle's sphere, or -1 */
   dvec<int> nosph;             /* particles without a sphere */
   dvec<signed char> iso;       /* [N] bit 0: inertia is diag(j, j, j) with exact zeros; bit 1: the
-                                  sphere's body offset is +0 */
+                                  sphere's body offset is +0; bit 2: loaded by contacts and gravity
+                                  only (no spring, no prescribed motion) */
   int niso = 0;
   int nzc = 0;                 /* particles whose sphere offset is +0 */
   int nnosph = 0;
@@ -285,7 +287,7 @@ int step_detect (pmx_ctx &c);
 int step_scatter (pmx_ctx &c);
 int launch_detect (pmx_ctx &c, int nw);
 int launch_rows (pmx_ctx &c, int tpb, double h);
-int autotune (pmx_ctx &c, double h);          /* block shapes of detect and rows */
+int autotune (pmx_ctx &c, double h);          /* block shape of detection */
 int step_rows (pmx_ctx &c, double h);
 int step_dynamics (pmx_ctx &c, double step0, double step1);
 int step_end (pmx_ctx &c);
@@ -293,6 +295,7 @@ int enqueue_step (pmx_ctx &c, double h);     /* one whole time step */
 int step_obstacles (pmx_ctx &c, const double *tab, int row, double h);
 int step_prescribe_acc (pmx_ctx &c);           /* before dynamics */
 int step_prescribe_vel (pmx_ctx &c);           /* after dynamics */
+int step_materialize (pmx_ctx &c, const std::vector<int> &fin);   /* forces of finishing instances */
 int step_shapes (pmx_ctx &c);
 int step_euler (pmx_ctx &c, double h);
 int step_invert (pmx_ctx &c);

@@ -526,7 +526,8 @@ template <int TPB>
 __global__ __launch_bounds__(TPB)
 void k_rows_idle (int E, int N, const int *part, const double *mass, int ninst, const int *ell_inst, int *cnt0,
                   int *cnt1, int cur0, const int *hitcnt, const int *xcnt, const double *gtab, const int *ctl,
-                  double *force, double *torque, const int *stop, int *lcount, int nleaves, int *busy, int *nbusy)
+                  double *force, double *torque, const int *stop, int *lcount, int nleaves, int *busy, int *nbusy,
+                  const signed char *flags)
 {
   __shared__ int wbase[TPB / 64];  /* busy spheres of the waves before */
   if (ctl[CTL_ABORT]) return;
@@ -551,6 +552,7 @@ void k_rows_idle (int E, int N, const int *part, const double *mass, int ninst, 
   if (!live || bs) return;
   ncnt[e] = 0;
   int pe = part[e];
+  if (flags[pe] & 4) return;   /* k_dynamics forms 0 + m g and 0 itself */
   const double *g = gtab + 3*((size_t) kstep*ninst + (ninst > 1 ? ell_inst[e] : 0));
   double ma = mass[pe];
   force[pe] = 0.0 + ma * g[0];
@@ -839,7 +841,8 @@ __global__ void k_dynamics (int N, double *angular, double *linear, double *rota
                             const double *inertia, const double *inverse, const double *mass, const double *invm,
                             const double *dtab, const int *ctl, double *force, double *torque, double step0, double step1,
                             const int *parell, int E, double *center, const signed char *iso, int ninst,
-                            const int *par_inst, const int *stop)
+                            const int *par_inst, const int *stop, const int *cnt0, const int *cnt1, int cur0,
+                            const double *gtab)
 {
   if (ctl[CTL_ABORT]) return;
   int kstep = ctl[CTL_KSTEP];
@@ -875,8 +878,22 @@ __global__ void k_dynamics (int N, double *angular, double *linear, double *rota
   auto Ix = [&] (const double x[3], double y[3]) {
     if constexpr (ISO) { y[0] = I[0]*x[0]; y[1] = I[0]*x[1]; y[2] = I[0]*x[2]; } else nvmul (I, x, y); };
   im = invm[i];
-  f[0] = force[i]; f[1] = force[(size_t) N + i]; f[2] = force[2*(size_t) N + i];
-  t[0] = torque[i]; t[1] = torque[(size_t) N + i]; t[2] = torque[2*(size_t) N + i];
+  int e = parell ? parell[i] : -1;
+  /* a particle loaded by contacts and gravity only whose row of this step
+   * is empty: k_rows would have stored force 0 + m g and torque 0 */
+  const int *ncnt = ((cur0 + kstep) & 1) ? cnt0 : cnt1;
+  if ((flag & 4) && e >= 0 && ncnt[e] == 0)
+  {
+    double ma = mass[i];
+    const double *g = gtab + 3*((size_t) kstep*ninst + (ninst > 1 ? par_inst[i] : 0));
+    f[0] = 0.0 + ma * g[0]; f[1] = 0.0 + ma * g[1]; f[2] = 0.0 + ma * g[2];
+    t[0] = 0.0; t[1] = 0.0; t[2] = 0.0;
+  }
+  else
+  {
+    f[0] = force[i]; f[1] = force[(size_t) N + i]; f[2] = force[2*(size_t) N + i];
+    t[0] = torque[i]; t[1] = torque[(size_t) N + i]; t[2] = torque[2*(size_t) N + i];
+  }
   if (damping[0] != 0.0 || damping[1] != 0.0 || damping[2] != 0.0)
   {
     double ma = mass[i];
@@ -924,7 +941,6 @@ __global__ void k_dynamics (int N, double *angular, double *linear, double *rota
   xn[1] = position[(size_t) N + i] + step*v[1];
   xn[2] = position[2*(size_t) N + i] + step*v[2];
   /* the particle's sphere (ellipsoids_task, shapes.ispc:29-67) */
-  int e = parell ? parell[i] : -1;
   double C[3] = {0.0, 0.0, 0.0};   /* kept when the offset is +0 (flag bit 1) */
   if (e >= 0 && !(flag & 2))
   {
@@ -962,6 +978,38 @@ __global__ void k_dynamics (int N, double *angular, double *linear, double *rota
     center[e] = xn[0] + cc[0];
     center[(size_t) E + e] = xn[1] + cc[1];
     center[2*(size_t) E + e] = xn[2] + cc[2];
+  }
+}
+
+/* force and torque of the flag-4 particles (loaded by contacts and gravity
+ * only) of the instances finishing now, as k_rows would have stored them at
+ * their last step k = fin[instance] when their row came out empty; where
+ * damping was on at that step, k_dynamics stored them itself */
+__global__ void k_materialize (int N, const signed char *flags, const int *parell, const int *cnt0, const int *cnt1,
+                               int cur0, int ninst, const int *par_inst, const int *fin, const double *gtab,
+                               const double *dtab, const double *mass, double *force, double *torque)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N || !(flags[i] & 4)) return;
+  int inst = ninst > 1 ? par_inst[i] : 0;
+  int k = fin[inst];
+  if (k < 0) return;
+  const int *ncnt = ((cur0 + k) & 1) ? cnt0 : cnt1;
+  if (ncnt[parell[i]] != 0) return;
+  const double *d = dtab + 6*((size_t) k*ninst + inst);
+  if (d[0] == 0.0 && d[1] == 0.0 && d[2] == 0.0)
+  {
+    double ma = mass[i];
+    const double *g = gtab + 3*((size_t) k*ninst + inst);
+    force[i] = 0.0 + ma * g[0];
+    force[(size_t) N + i] = 0.0 + ma * g[1];
+    force[2*(size_t) N + i] = 0.0 + ma * g[2];
+  }
+  if (d[3] == 0.0 && d[4] == 0.0 && d[5] == 0.0)
+  {
+    torque[i] = 0.0;
+    torque[(size_t) N + i] = 0.0;
+    torque[2*(size_t) N + i] = 0.0;
   }
 }
 
@@ -1278,7 +1326,7 @@ void rows_idle_launch (pmx_ctx &c)
                       c.rowcnt[0].p, c.rowcnt[1].p, c.cur0, (const int*) c.hitcnt.p, (const int*) c.xcnt.p,
                       (const double*) c.gtab.p, (const int*) c.ctl.p, c.force.p, c.torque.p,
                       c.staggered ? (const int*) c.istop.p : (const int*) nullptr, c.lcount.p, c.tree.nleaves,
-                      c.rbusy.p, c.rnbusy.p);
+                      c.rbusy.p, c.rnbusy.p, (const signed char*) c.iso.p);
 }
 
 /* the rows of a step: k_rows_idle with tpb threads per block (64 to 1024),
@@ -1325,11 +1373,28 @@ int step_dynamics (pmx_ctx &c, double step0, double step1)
                  c.position.p, (const double*) c.inertia.p, (const double*) c.inverse.p, (const double*) c.mass.p, \
                  (const double*) c.invm.p, (const double*) c.dtab.p, (const int*) c.ctl.p, c.force.p, c.torque.p, \
                  step0, step1, (const int*) c.parell.p, c.E, c.center.p, (const signed char*) c.iso.p, c.I, \
-                 (const int*) c.par_inst.p, c.staggered ? (const int*) c.istop.p : (const int*) nullptr
+                 (const int*) c.par_inst.p, c.staggered ? (const int*) c.istop.p : (const int*) nullptr, \
+                 (const int*) c.rowcnt[0].p, (const int*) c.rowcnt[1].p, c.cur0, (const double*) c.gtab.p
   if (c.niso > 0) hipLaunchKernelGGL (k_dynamics<true>, grid (c.N), dim3 (PMX_TPB), 0, c.stream, DYN_ARGS);
   if (c.niso < c.N) hipLaunchKernelGGL (k_dynamics<false>, grid (c.N), dim3 (PMX_TPB), 0, c.stream, DYN_ARGS);
 #undef DYN_ARGS
   PMX_HIP (hipGetLastError ());
+  return PMX_OK;
+}
+
+/* fin[i]: the last step (in this call) of each instance finishing now, or -1 */
+int step_materialize (pmx_ctx &c, const std::vector<int> &fin)
+{
+  if (c.N == 0 || c.E == 0) return PMX_OK;
+  if (c.fin.alloc (std::max (c.I, 1))) { pmx_set_error ("out of device memory"); return PMX_ENOMEM; }
+  PMX_HIP (hipMemcpyAsync (c.fin.p, fin.data (), sizeof (int) * c.I, hipMemcpyHostToDevice, c.stream));
+  hipLaunchKernelGGL (k_materialize, grid (c.N), dim3 (PMX_TPB), 0, c.stream, c.N, (const signed char*) c.iso.p,
+                      (const int*) c.parell.p, (const int*) c.rowcnt[0].p, (const int*) c.rowcnt[1].p, c.cur0, c.I,
+                      (const int*) c.par_inst.p, (const int*) c.fin.p, (const double*) c.gtab.p,
+                      (const double*) c.dtab.p, (const double*) c.mass.p, c.force.p, c.torque.p);
+  PMX_HIP (hipGetLastError ());
+  /* fin lives on the caller's stack: the copy must be done before it returns */
+  PMX_HIP (hipStreamSynchronize (c.stream));
   return PMX_OK;
 }
 

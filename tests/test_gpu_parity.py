@@ -51,6 +51,23 @@ def test_box_lattice_gravity(engines, n, steps):
     compare(mg, cg, mc, cc)
 
 
+@pytest.mark.parametrize("damping", [(0.5, 0.5, 0.5, 0.2, 0.2, 0.2), (0.0, 0.0, 0.0, 0.3, 0.3, 0.3)])
+def test_box_lattice_damped(engines, damping):
+    """Linear and angular damping (dynamics.ispc:64-91) on spheres in and out
+    of contact: the damped force and torque are the stored ones, the
+    undamped part of the pair is formed from gravity where a row is empty."""
+    print(f"case box_lattice damped {damping}", flush=True)
+    m = workloads.box_lattice(n=2000, seed=4, spacing_factor=2.05, jitter=0.02)
+    rng = np.random.default_rng(8)
+    m.linear[:] = rng.uniform(-1.0, 1.0, size=m.linear.shape)
+    m.angular[:3] = rng.uniform(-20.0, 20.0, size=(3, m.parnum))
+    f = pmx.forcing(gravity=(0.0, 0.0, -9.81), damping=damping)
+    (mg, cg, sg), (mc, cc, sc) = _both(engines, m, 250 * 1e-5, 1e-5, f, calls=2)
+    assert sg["steps"] == sc["steps"]
+    assert len(cc) > 0
+    compare(mg, cg, mc, cc)
+
+
 def test_hopper_small(engines):
     print("case hopper n=20000 steps=200", flush=True)
     m = workloads.hopper(n=20000, seed=5, ntri=800, jitter_speed=5.0, gap=0.00002)
