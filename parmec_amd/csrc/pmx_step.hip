@@ -958,26 +958,26 @@ __global__ void k_dynamics (int N, double *angular, double *linear, double *rota
     for (int r = 0; r < 3; r++)
     {
       double l = L1[r]*DL[3*c] + L1[3+r]*DL[3*c+1] + L1[6+r]*DL[3*c+2];
-      rotation[(size_t) (3*c+r)*N + i] = l;
+      __builtin_nontemporal_store (l, &rotation[(size_t) (3*c+r)*N + i]);
       o[r] = c ? o[r] + l*O[c] : l*O[c];
       cc[r] = c ? cc[r] + l*C[c] : l*C[c];
     }
   }
-  position[i] = xn[0];
-  position[(size_t) N + i] = xn[1];
-  position[2*(size_t) N + i] = xn[2];
+  __builtin_nontemporal_store (xn[0], &position[i]);
+  __builtin_nontemporal_store (xn[1], &position[(size_t) N + i]);
+  __builtin_nontemporal_store (xn[2], &position[2*(size_t) N + i]);
 #pragma unroll
   for (int k = 0; k < 3; k++)
   {
-    angular[(size_t) k*N + i] = O[k];
-    angular[(size_t) (3+k)*N + i] = o[k];
-    linear[(size_t) k*N + i] = v[k];
+    __builtin_nontemporal_store (O[k], &angular[(size_t) k*N + i]);
+    __builtin_nontemporal_store (o[k], &angular[(size_t) (3+k)*N + i]);
+    __builtin_nontemporal_store (v[k], &linear[(size_t) k*N + i]);
   }
   if (e >= 0)
   {
-    center[e] = xn[0] + cc[0];
-    center[(size_t) E + e] = xn[1] + cc[1];
-    center[2*(size_t) E + e] = xn[2] + cc[2];
+    __builtin_nontemporal_store (xn[0] + cc[0], &center[e]);
+    __builtin_nontemporal_store (xn[1] + cc[1], &center[(size_t) E + e]);
+    __builtin_nontemporal_store (xn[2] + cc[2], &center[2*(size_t) E + e]);
   }
 }
 
