@@ -169,7 +169,7 @@ typedef struct pmx_options
   int tune;            /* 0: pick the detection block shape by timing it (models of 50,000
                           spheres or more: at the first dem call and again after 256
                           steps); 1: the fixed det_waves below */
-  int det_waves;       /* tune = 1: waves per detection block, 2, 4 or 8 (0: 4) */
+  int det_waves;       /* tune = 1: waves per detection block, 1, 2, 4 or 8 (0: 4) */
   int rows_block;      /* spheres per contact-row block (per list of spheres in contact), 64, 128,
                           256, 512 or 1024 (0: 1024) */
 } pmx_options;

@@ -51,7 +51,7 @@ int alloc_rows (pmx_ctx &c)
       return fail (PMX_ENOMEM, "out of device memory (contact rows)");
   if (c.hits.alloc ((size_t) c.hcap * E) || c.hitcnt.alloc (E) || c.xbuf.alloc ((size_t) c.xcap * E) ||
       c.xcnt.alloc (E) || c.rbusy.alloc (E + 1024) || c.rnbusy.alloc (E / 64 + 2) || c.leafof.alloc (E) ||
-      c.slot.alloc (E) || c.lmem.alloc (E) || c.lpos.alloc (4 * E))
+      c.slot.alloc (E) || c.lmem.alloc (E) || c.lpart.alloc (E) || c.lpos.alloc (4 * E))
     return fail (PMX_ENOMEM, "out of device memory (broad phase)");
   PMX_HIP (hipMemsetAsync (c.rowcnt[0].p, 0, E * sizeof (int), c.stream));
   PMX_HIP (hipMemsetAsync (c.rowcnt[1].p, 0, E * sizeof (int), c.stream));
@@ -101,7 +101,8 @@ int validate_model (const pmx_model *m)
   }
   for (int e = 0; e < m->ellnum; e++)
   {
-    if (!(m->radii[(size_t) m->ellnum + e] < 0.0)) return fail (PMX_EUNSUPPORTED, "only spherical ellipsoids are supported");
+    if (!(m->radii[(size_t) m->ellnum + e] < 0.0) || !(m->radii[2*(size_t) m->ellnum + e] < 0.0))
+      return fail (PMX_EUNSUPPORTED, "only spherical ellipsoids are supported");
     if (m->part[e] < 0 || m->part[e] >= m->parnum) return fail (PMX_EINVAL, "ellipsoid particle index out of range");
   }
   for (int t = 0; t < m->trinum; t++)
@@ -349,8 +350,9 @@ int pmx_create (const pmx_options *opt, pmx_ctx **out)
     delete c;
     return fail (PMX_EINVAL, "precision must be PMX_FP64 or PMX_MIXED");
   }
-  if (c->opt.det_waves != 0 && c->opt.det_waves != 2 && c->opt.det_waves != 4 && c->opt.det_waves != 8)
-  { delete c; return fail (PMX_EINVAL, "det_waves must be 2, 4 or 8"); }
+  if (c->opt.det_waves != 0 && c->opt.det_waves != 1 && c->opt.det_waves != 2 && c->opt.det_waves != 4 &&
+      c->opt.det_waves != 8)
+  { delete c; return fail (PMX_EINVAL, "det_waves must be 1, 2, 4 or 8"); }
   if (c->opt.rows_block != 0 && c->opt.rows_block != 64 && c->opt.rows_block != 128 && c->opt.rows_block != 256 &&
       c->opt.rows_block != 512 && c->opt.rows_block != 1024)
   { delete c; return fail (PMX_EINVAL, "rows_block must be 64, 128, 256, 512 or 1024"); }

@@ -174,7 +174,7 @@ struct pmx_ctx
 
   /* broad phase per step */
   TreeDev tree;
-  dvec<int> leafof, slot, lcount, loff, lmem;
+  dvec<int> leafof, slot, lcount, loff, lmem, lpart;   /* lmem, lpart: leaf-ordered sphere, its particle */
   dvec<double> lpos;           /* [4][E] leaf-ordered x, y, z, r */
   /* contact rows (ping-pong) */
   int cap = 32, hcap = 32, xcap = 16;

@@ -128,7 +128,8 @@ __global__ void k_bin (int E, const double *center, const int *roots, const int 
 }
 
 __global__ void k_scatter (int *ctl, int E, const double *center, const double *radii, const int *leafof, const int *slot,
-                           const int *loff, int *lmem, double *lpos, const int *stop, const int *ell_inst, int *xcnt)
+                           const int *loff, int *lmem, double *lpos, const int *stop, const int *ell_inst, int *xcnt,
+                           const int *part, int *lpart)
 {
   if (ctl[CTL_ABORT]) return;
   int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -137,6 +138,7 @@ __global__ void k_scatter (int *ctl, int E, const double *center, const double *
   xcnt[e] = 0;   /* reverse hits of this step start at zero */
   int pos = loff[leafof[e]] + slot[e];
   lmem[pos] = e;
+  lpart[pos] = part[e];
   double2 *rec = (double2*) (lpos + 4*(size_t) pos);
   rec[0] = make_double2 (center[e], center[(size_t) E + e]);
   rec[1] = make_double2 (center[2*(size_t) E + e], radii[e]);
@@ -177,8 +179,8 @@ __device__ __forceinline__ unsigned long long readlane_u64 (unsigned long long x
   return ((unsigned long long) hi << 32) | lo;
 }
 
-/* k_detect runs NW waves per block (NW * 64 >= PMX_LSIZE: one sphere of the
- * leaf per thread slot); NW is picked per workload by the autotuner */
+/* k_detect runs NW waves per block (1, 2, 4 or 8; the block's spheres are
+ * spread over its threads); NW is picked per workload by the autotuner */
 
 /* one sphere-sphere test of the reference leaf test (condet.ispc:128-144):
  * depth = r_a + r_b - |c_a - c_b| > 0.  d2 >= sr^2 (1 + 1e-10) implies the
@@ -206,7 +208,7 @@ __device__ __forceinline__ bool overlap (double ax, double ay, double az, double
 template <int NW>
 __global__ __launch_bounds__(NW*64)
 void k_detect (int E, const double *center, const double *radii, const int *part, const double *cell, int nleaves,
-               const int *lcount, const int *loff, const int *lmem, const double *lpos,
+               const int *lcount, const int *loff, const int *lmem, const double *lpos, const int *lpart,
                const double *tri, int T, const int *tricnt, const int *trioff, const int *trilist,
                const int *mtcnt, const int *mtlist, int mcap, const int *nbrcnt, const int *nbroff, const int *nbrlist,
                uint32_t *hits, int *hitcnt, int hcap, int *xbuf, int *xcnt, int xcap, int *flags, const int *ctl,
@@ -223,18 +225,16 @@ void k_detect (int E, const double *center, const double *radii, const int *part
   __shared__ unsigned long long wmask[NW][128];   /* per wave: reach masks of 64 candidates */
   int t = threadIdx.x, lane = t & 63, w = t >> 6;
   int qoff = loff[Q], qcnt = lcount[Q];
-  bool valid = t < qcnt;
-  if (valid)
+  /* the block's spheres from the leaf-ordered records (x, y, z, r0: a
+   * sphere's r1, r2 are negative, so r0 is its AABB half-width) */
+  for (int s = t; s < qcnt; s += NW*64)
   {
-    int e = lmem[qoff + t];
-    double r0 = radii[e], r1 = radii[(size_t) E + e], r2 = radii[2*(size_t) E + e];
-    double rx = r0 > r1 ? r0 : r1;
-    rx = rx > r2 ? rx : r2;
-    ssph[4*t] = center[e]; ssph[4*t+1] = center[(size_t) E + e]; ssph[4*t+2] = center[2*(size_t) E + e];
-    ssph[4*t+3] = rx;
-    sidx[t] = e;
-    spart[t] = part[e];
-    shc[t] = 0;
+    const double2 *rp = (const double2*) (lpos + 4*(size_t) (qoff + s));
+    double2 a = rp[0], b = rp[1];
+    ssph[4*s] = a.x; ssph[4*s+1] = a.y; ssph[4*s+2] = b.x; ssph[4*s+3] = b.y;
+    sidx[s] = lmem[qoff + s];
+    spart[s] = lpart[qoff + s];
+    shc[s] = 0;
   }
   double cl[3] = {cell[Q], cell[(size_t) nleaves + Q], cell[2*(size_t) nleaves + Q]};
   double cu[3] = {cell[3*(size_t) nleaves + Q], cell[4*(size_t) nleaves + Q], cell[5*(size_t) nleaves + Q]};
@@ -284,9 +284,9 @@ void k_detect (int E, const double *center, const double *radii, const int *part
   }
   const float k1 = 1.0f + 0x1p-20f;   /* 1 + 16u, u = 2^-24 */
   const float k2 = B > 1e15 ? INFINITY : (float) (0x1p-21 * B * (1.0 + 1e-6)) + 1e-18f;   /* 8u B + 1e-18 */
-  if (valid)
-    ssph32[t] = make_float4 ((float) (ssph[4*t] - O[0]), (float) (ssph[4*t+1] - O[1]), (float) (ssph[4*t+2] - O[2]),
-                             (float) ssph[4*t+3]);
+  for (int s = t; s < qcnt; s += NW*64)
+    ssph32[s] = make_float4 ((float) (ssph[4*s] - O[0]), (float) (ssph[4*s+1] - O[1]), (float) (ssph[4*s+2] - O[2]),
+                             (float) ssph[4*s+3]);
   __syncthreads ();
 
   float4 *rec32 = wrec32[w];
@@ -489,7 +489,7 @@ void k_detect (int E, const double *center, const double *radii, const int *part
     }
   }
   __syncthreads ();
-  if (valid) hitcnt[sidx[t]] = shc[t] < hcap ? shc[t] : hcap;
+  for (int s = t; s < qcnt; s += NW*64) hitcnt[sidx[s]] = shc[s] < hcap ? shc[s] : hcap;
 }
 
 __global__ void k_force_init (const int *ctl, int N, const int *list, int nlist, const double *mass, const double *gtab,
@@ -1241,50 +1241,39 @@ int step_scatter (pmx_ctx &c)
   int rc = scan_exclusive_to (c, (const int*) c.lcount.p, c.loff.p, L, c.ctl.p);
   if (rc) return rc;
   {
-    ProfScope ps (c, K_SCATTER, 76.0 * c.E);
+    ProfScope ps (c, K_SCATTER, 84.0 * c.E);
     hipLaunchKernelGGL (k_scatter, grid (c.E), dim3 (PMX_TPB), 0, c.stream, c.ctl.p, c.E, (const double*) c.center.p,
                         (const double*) c.radii.p, (const int*) c.leafof.p, (const int*) c.slot.p, (const int*) c.loff.p,
                         c.lmem.p, c.lpos.p, c.staggered ? (const int*) c.istop.p : (const int*) nullptr,
-                        (const int*) c.ell_inst.p, c.xcnt.p);
+                        (const int*) c.ell_inst.p, c.xcnt.p, (const int*) c.part.p, c.lpart.p);
   }
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
 }
 
-int launch_detect (pmx_ctx &c, int nw)
+template <int NW>
+void detect_launch (pmx_ctx &c)
 {
   TreeDev &t = c.tree;
+  hipLaunchKernelGGL (k_detect<NW>, dim3 (t.nleaves), dim3 (NW*64), 0, c.stream, c.E, (const double*) c.center.p,
+                      (const double*) c.radii.p, (const int*) c.part.p, (const double*) t.cell.p,
+                      t.nleaves, (const int*) c.lcount.p, (const int*) c.loff.p,
+                      (const int*) c.lmem.p, (const double*) c.lpos.p, (const int*) c.lpart.p, (const double*) c.tri.p, c.T,
+                      (const int*) t.tricnt.p, (const int*) t.trioff.p, (const int*) t.trilist.p,
+                      c.nmtri > 0 ? (const int*) c.mtcnt.p : (const int*) nullptr, (const int*) c.mtlist.p, c.mcap,
+                      (const int*) t.nbrcnt.p, (const int*) t.nbroff.p, (const int*) t.nbrlist.p,
+                      c.hits.p, c.hitcnt.p, c.hcap, c.xbuf.p, c.xcnt.p, c.xcap, c.dflags.p, (const int*) c.ctl.p,
+                      c.staggered ? (const int*) c.istop.p : (const int*) nullptr, (const int*) t.leafinst.p);
+}
+
+/* k_detect with nw waves per block (1, 2, 4 or 8) */
+int launch_detect (pmx_ctx &c, int nw)
+{
   ProfScope ps (c, K_DETECT, 68.0 * c.E);
-  if (nw == 2)
-    hipLaunchKernelGGL (k_detect<2>, dim3 (t.nleaves), dim3 (128), 0, c.stream, c.E, (const double*) c.center.p,
-                        (const double*) c.radii.p, (const int*) c.part.p, (const double*) t.cell.p,
-                        t.nleaves, (const int*) c.lcount.p, (const int*) c.loff.p,
-                        (const int*) c.lmem.p, (const double*) c.lpos.p, (const double*) c.tri.p, c.T,
-                        (const int*) t.tricnt.p, (const int*) t.trioff.p, (const int*) t.trilist.p,
-                        c.nmtri > 0 ? (const int*) c.mtcnt.p : (const int*) nullptr, (const int*) c.mtlist.p, c.mcap,
-                        (const int*) t.nbrcnt.p, (const int*) t.nbroff.p, (const int*) t.nbrlist.p,
-                        c.hits.p, c.hitcnt.p, c.hcap, c.xbuf.p, c.xcnt.p, c.xcap, c.dflags.p, (const int*) c.ctl.p,
-                        c.staggered ? (const int*) c.istop.p : (const int*) nullptr, (const int*) t.leafinst.p);
-  else if (nw == 8)
-    hipLaunchKernelGGL (k_detect<8>, dim3 (t.nleaves), dim3 (512), 0, c.stream, c.E, (const double*) c.center.p,
-                        (const double*) c.radii.p, (const int*) c.part.p, (const double*) t.cell.p,
-                        t.nleaves, (const int*) c.lcount.p, (const int*) c.loff.p,
-                        (const int*) c.lmem.p, (const double*) c.lpos.p, (const double*) c.tri.p, c.T,
-                        (const int*) t.tricnt.p, (const int*) t.trioff.p, (const int*) t.trilist.p,
-                        c.nmtri > 0 ? (const int*) c.mtcnt.p : (const int*) nullptr, (const int*) c.mtlist.p, c.mcap,
-                        (const int*) t.nbrcnt.p, (const int*) t.nbroff.p, (const int*) t.nbrlist.p,
-                        c.hits.p, c.hitcnt.p, c.hcap, c.xbuf.p, c.xcnt.p, c.xcap, c.dflags.p, (const int*) c.ctl.p,
-                        c.staggered ? (const int*) c.istop.p : (const int*) nullptr, (const int*) t.leafinst.p);
-  else
-    hipLaunchKernelGGL (k_detect<4>, dim3 (t.nleaves), dim3 (256), 0, c.stream, c.E, (const double*) c.center.p,
-                        (const double*) c.radii.p, (const int*) c.part.p, (const double*) t.cell.p,
-                        t.nleaves, (const int*) c.lcount.p, (const int*) c.loff.p,
-                        (const int*) c.lmem.p, (const double*) c.lpos.p, (const double*) c.tri.p, c.T,
-                        (const int*) t.tricnt.p, (const int*) t.trioff.p, (const int*) t.trilist.p,
-                        c.nmtri > 0 ? (const int*) c.mtcnt.p : (const int*) nullptr, (const int*) c.mtlist.p, c.mcap,
-                        (const int*) t.nbrcnt.p, (const int*) t.nbroff.p, (const int*) t.nbrlist.p,
-                        c.hits.p, c.hitcnt.p, c.hcap, c.xbuf.p, c.xcnt.p, c.xcap, c.dflags.p, (const int*) c.ctl.p,
-                        c.staggered ? (const int*) c.istop.p : (const int*) nullptr, (const int*) t.leafinst.p);
+  if (nw == 1) detect_launch<1> (c);
+  else if (nw == 2) detect_launch<2> (c);
+  else if (nw == 8) detect_launch<8> (c);
+  else detect_launch<4> (c);
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
 }
@@ -1530,12 +1519,12 @@ int autotune (pmx_ctx &c, double h)
       }
     return best;
   };
-  const int nws[3] = {2, 4, 8};
+  const int nws[4] = {1, 2, 4, 8};
   /* the reverse-hit counts start at zero for each detection */
   auto clear = [&] () {
     hipLaunchKernelGGL (k_fill_i, grid (c.E), dim3 (PMX_TPB), 0, c.stream, (const int*) c.ctl.p, c.xcnt.p, c.E, 0);
     return hipGetLastError () == hipSuccess ? 0 : -1; };
-  int nw = best_of (nws, 3, clear, [&] (int v) { return launch_detect (c, v); });
+  int nw = best_of (nws, 4, clear, [&] (int v) { return launch_detect (c, v); });
   (void) hipEventDestroy (e0);
   (void) hipEventDestroy (e1);
   if (nw < 0) { pmx_set_error ("autotuning failed"); return PMX_EDEVICE; }
