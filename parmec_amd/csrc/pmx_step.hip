@@ -225,6 +225,35 @@ void k_detect (int E, const double *center, const double *radii, const int *part
   __shared__ unsigned long long wmask[NW][128];   /* per wave: reach masks of 64 candidates */
   int t = threadIdx.x, lane = t & 63, w = t >> 6;
   int qoff = loff[Q], qcnt = lcount[Q];
+  int nc = nbrcnt[Q], c0 = nbroff[Q];
+  int ncw = nc > w ? (nc - w + NW - 1) / NW : 0;   /* candidates of this wave: w, w + NW, ... */
+  /* the leaf's triangle lists (static from the build, moving from this step) */
+  int nsq = T > 0 ? tricnt[Q] : 0, toff = T > 0 ? trioff[Q] : 0;
+  int nmq = mtcnt ? mtcnt[Q] : 0;
+  nmq = nmq < mcap ? nmq : mcap;
+  int ntq = nsq + nmq;
+  /* the wave's first (usually only) batch of candidate leaves, one per
+   * lane: leaf, count, offset and cell, loaded before the block's spheres
+   * so that both dependent loads are in flight together */
+  int bL = 0, bcnt = 0, boff = 0;
+  double Ll[3] = {0.0, 0.0, 0.0}, Lu[3] = {0.0, 0.0, 0.0};
+  auto batch = [&] (int p0) {
+    int np = ncw - p0 < 64 ? ncw - p0 : 64;
+    bL = 0; bcnt = 0; boff = 0;
+    if (lane < np)
+    {
+      bL = nbrlist[c0 + w + (p0 + lane) * NW];
+      bcnt = lcount[bL];
+      boff = loff[bL];
+#pragma unroll
+      for (int d = 0; d < 3; d++)
+      {
+        Ll[d] = cell[(size_t) d*nleaves + bL];
+        Lu[d] = cell[(size_t) (3+d)*nleaves + bL];
+      }
+    }
+  };
+  batch (0);
   /* the block's spheres from the leaf-ordered records (x, y, z, r0: a
    * sphere's r1, r2 are negative, so r0 is its AABB half-width) */
   for (int s = t; s < qcnt; s += NW*64)
@@ -292,27 +321,12 @@ void k_detect (int E, const double *center, const double *radii, const int *part
   float4 *rec32 = wrec32[w];
   int *mem = wmem[w], *lst = wlist[w];
   unsigned long long lt = lane ? ((1ull << lane) - 1ull) : 0ull;
-  int nc = nbrcnt[Q], c0 = nbroff[Q];
-  int ncw = nc > w ? (nc - w + NW - 1) / NW : 0;   /* candidates of this wave: w, w + NW, ... */
   for (int p0 = 0; p0 < ncw; p0 += 64)
   {
-    /* up to 64 candidates at once, one per lane: leaf, count, offset and
-     * which block spheres' own AABB reaches its cell (condet.ispc:104-113) */
+    /* up to 64 candidates at once, one per lane, and which block spheres'
+     * own AABB reaches each one's cell (condet.ispc:104-113) */
     int np = ncw - p0 < 64 ? ncw - p0 : 64;
-    int bL = 0, bcnt = 0, boff = 0;
-    double Ll[3] = {0.0, 0.0, 0.0}, Lu[3] = {0.0, 0.0, 0.0};
-    if (lane < np)
-    {
-      bL = nbrlist[c0 + w + (p0 + lane) * NW];
-      bcnt = lcount[bL];
-      boff = loff[bL];
-#pragma unroll
-      for (int d = 0; d < 3; d++)
-      {
-        Ll[d] = cell[(size_t) d*nleaves + bL];
-        Lu[d] = cell[(size_t) (3+d)*nleaves + bL];
-      }
-    }
+    if (p0 > 0) batch (p0);
     /* the (candidate, block sphere) pairs spread over the lanes, the
      * candidate's cell fetched from its lane; reach bits gathered in LDS */
     unsigned long long *mk = wmask[w];
@@ -455,13 +469,8 @@ void k_detect (int E, const double *center, const double *radii, const int *part
 
   /* triangles whose AABB reaches Q (test_triangles, condet.ispc:412-516):
    * static ones from the rebuild-time lists, moving ones from this step's */
-  int nsq = T > 0 ? tricnt[Q] : 0;
-  int nmq = mtcnt ? mtcnt[Q] : 0;
-  nmq = nmq < mcap ? nmq : mcap;
-  int ntq = nsq + nmq;
   if (ntq > 0 && qcnt > 0)
   {
-    int toff = trioff[Q];
     int work = qcnt * ntq;
     for (int q = t; q < work; q += NW*64)
     {
