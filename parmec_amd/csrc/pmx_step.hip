@@ -1278,7 +1278,8 @@ void detect_launch (pmx_ctx &c)
 /* k_detect with nw waves per block (1, 2, 4 or 8) */
 int launch_detect (pmx_ctx &c, int nw)
 {
-  ProfScope ps (c, K_DETECT, 68.0 * c.E);
+  /* per sphere: its leaf record 32, member 4, particle 4, hit count 4 */
+  ProfScope ps (c, K_DETECT, 44.0 * c.E);
   if (nw == 1) detect_launch<1> (c);
   else if (nw == 2) detect_launch<2> (c);
   else if (nw == 8) detect_launch<8> (c);
