@@ -11,6 +11,7 @@
  * curstep = step1 and stepnum++ (2521-2523).
  */
 #include "pmx_internal.h"
+#include <cstdlib>
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -357,6 +358,10 @@ int pmx_create (const pmx_options *opt, pmx_ctx **out)
       c->opt.rows_block != 512 && c->opt.rows_block != 1024)
   { delete c; return fail (PMX_EINVAL, "rows_block must be 64, 128, 256, 512 or 1024"); }
   if (c->opt.det_waves) c->det_nw = c->opt.det_waves;
+  {
+    const char *nl = std::getenv ("PMX_NEIGHBOUR_LISTS");
+    c->nbr_csr = nl && std::string (nl) == "csr";
+  }
   if (c->opt.rows_block) c->rows_tpb = c->opt.rows_block;
   int ndev = 0;
   if (hipGetDeviceCount (&ndev) != hipSuccess || ndev == 0)

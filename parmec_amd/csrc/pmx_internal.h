@@ -83,6 +83,7 @@ struct TreeDev
   dvec<int> tricnt, trioff, trilist;  /* leaf -> triangles whose AABB reaches it */
   int ntrilist = 0;
   dvec<int> nbrcnt, nbroff, nbrlist;  /* leaf -> leaves overlapping its cell inflated by rmax */
+  int nbrstride = 0;                  /* list Q at Q * nbrstride, or CSR by nbroff when 0 */
   int nnbr = 0;
 };
 
@@ -205,6 +206,7 @@ struct pmx_ctx
                                   only (no spring, no prescribed motion) */
   int niso = 0;
   int nzc = 0;                 /* particles whose sphere offset is +0 */
+  bool nbr_csr = false;        /* neighbour lists always in CSR layout (PMX_NEIGHBOUR_LISTS=csr) */
   int nnosph = 0;
   bool bin_hint = false;       /* leafof holds leaves of the current tree */
   double rmax = 0.0;           /* largest sphere AABB half-width */

@@ -211,6 +211,7 @@ void k_detect (int E, const double *center, const double *radii, const int *part
                const int *lcount, const int *loff, const int *lmem, const double *lpos, const int *lpart,
                const double *tri, int T, const int *tricnt, const int *trioff, const int *trilist,
                const int *mtcnt, const int *mtlist, int mcap, const int *nbrcnt, const int *nbroff, const int *nbrlist,
+               int nbrstride,
                uint32_t *hits, int *hitcnt, int hcap, int *xbuf, int *xcnt, int xcap, int *flags, const int *ctl,
                const int *stop, const int *leafinst)
 {
@@ -225,7 +226,8 @@ void k_detect (int E, const double *center, const double *radii, const int *part
   __shared__ unsigned long long wmask[NW][128];   /* per wave: reach masks of 64 candidates */
   int t = threadIdx.x, lane = t & 63, w = t >> 6;
   int qoff = loff[Q], qcnt = lcount[Q];
-  int nc = nbrcnt[Q], c0 = nbroff[Q];
+  int nc = nbrcnt[Q];
+  size_t c0 = nbrstride ? (size_t) Q * nbrstride : (size_t) nbroff[Q];
   int ncw = nc > w ? (nc - w + NW - 1) / NW : 0;   /* candidates of this wave: w, w + NW, ... */
   /* the leaf's triangle lists (static from the build, moving from this step) */
   int nsq = T > 0 ? tricnt[Q] : 0, toff = T > 0 ? trioff[Q] : 0;
@@ -1270,7 +1272,7 @@ void detect_launch (pmx_ctx &c)
                       (const int*) c.lmem.p, (const double*) c.lpos.p, (const int*) c.lpart.p, (const double*) c.tri.p, c.T,
                       (const int*) t.tricnt.p, (const int*) t.trioff.p, (const int*) t.trilist.p,
                       c.nmtri > 0 ? (const int*) c.mtcnt.p : (const int*) nullptr, (const int*) c.mtlist.p, c.mcap,
-                      (const int*) t.nbrcnt.p, (const int*) t.nbroff.p, (const int*) t.nbrlist.p,
+                      (const int*) t.nbrcnt.p, (const int*) t.nbroff.p, (const int*) t.nbrlist.p, t.nbrstride,
                       c.hits.p, c.hitcnt.p, c.hcap, c.xbuf.p, c.xcnt.p, c.xcap, c.dflags.p, (const int*) c.ctl.p,
                       c.staggered ? (const int*) c.istop.p : (const int*) nullptr, (const int*) t.leafinst.p);
 }

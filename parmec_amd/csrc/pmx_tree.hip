@@ -621,7 +621,7 @@ __global__ void k_mtri_reach (int nm, const int *mtri, int T, const double *tri,
  * half-width is at most rmax; rounding is monotone) */
 __global__ void k_leaf_nbr (int nleaves, const double *cell, double rmax, const int *roots, const int *leafinst,
                             const int *left, const int *right, const int *dim, const double *coord, int *cnt,
-                            const int *off, int *list, int *flag)
+                            const int *off, int *list, int stride, int *flag)
 {
   int Q = blockIdx.x * blockDim.x + threadIdx.x;
   if (Q >= nleaves) return;
@@ -652,7 +652,7 @@ __global__ void k_leaf_nbr (int nleaves, const double *cell, double rmax, const 
     }
     else
     {
-      if (list) list[off[Q] + n] = -ref-1;
+      if (list) list[stride ? (size_t) Q * stride + n : (size_t) off[Q] + n] = -ref-1;
       n++;
     }
     if (sp == 0) break;
@@ -824,18 +824,36 @@ int tree_nbrlists (pmx_ctx &c)
   hipLaunchKernelGGL (k_leaf_nbr, grid (t.nleaves), dim3 (PMX_TPB), 0, c.stream, t.nleaves, (const double*) t.cell.p,
                       c.rmax, (const int*) t.roots.p, (const int*) t.leafinst.p, (const int*) t.left.p,
                       (const int*) t.right.p, (const int*) t.dim.p, (const double*) t.coord.p, t.nbrcnt.p,
-                      (const int*) nullptr, (int*) nullptr, flag);
+                      (const int*) nullptr, (int*) nullptr, 0, flag);
   PMX_HIP (hipGetLastError ());
-  PMX_HIP (hipMemcpyAsync (t.nbroff.p, t.nbrcnt.p, sizeof (int) * t.nleaves, hipMemcpyDeviceToDevice, c.stream));
-  int total = 0;
-  int rc = scan_exclusive (c, t.nbroff.p, t.nleaves, &total);
-  if (rc) return rc;
-  t.nnbr = total;
-  if (t.nbrlist.alloc (std::max (total, 1))) { pmx_set_error ("out of device memory (neighbour lists)"); return PMX_ENOMEM; }
+  /* leaf Q's list at Q * stride (stride = the longest list), so that a
+   * detection block finds its list without first reading an offset; CSR
+   * when that layout would take more than 2 GiB (or when asked for: the
+   * environment variable PMX_NEIGHBOUR_LISTS=csr) */
+  std::vector<int> cnt (t.nleaves);
+  PMX_HIP (hipMemcpyAsync (cnt.data (), t.nbrcnt.p, sizeof (int) * t.nleaves, hipMemcpyDeviceToHost, c.stream));
+  PMX_HIP (hipStreamSynchronize (c.stream));
+  int mx = 1;
+  for (int q = 0; q < t.nleaves; q++) mx = std::max (mx, cnt[q]);
+  t.nbrstride = !c.nbr_csr && (size_t) t.nleaves * mx * sizeof (int) <= ((size_t) 2 << 30) ? mx : 0;
+  int rc;
+  if (t.nbrstride)
+  {
+    t.nnbr = t.nleaves * mx;
+    hipLaunchKernelGGL (k_fill_int, grid (L + 1), dim3 (PMX_TPB), 0, c.stream, t.nbroff.p, L + 1, 0);
+  }
+  else
+  {
+    PMX_HIP (hipMemcpyAsync (t.nbroff.p, t.nbrcnt.p, sizeof (int) * t.nleaves, hipMemcpyDeviceToDevice, c.stream));
+    int total = 0;
+    if ((rc = scan_exclusive (c, t.nbroff.p, t.nleaves, &total))) return rc;
+    t.nnbr = total;
+  }
+  if (t.nbrlist.alloc (std::max (t.nnbr, 1))) { pmx_set_error ("out of device memory (neighbour lists)"); return PMX_ENOMEM; }
   hipLaunchKernelGGL (k_leaf_nbr, grid (t.nleaves), dim3 (PMX_TPB), 0, c.stream, t.nleaves, (const double*) t.cell.p,
                       c.rmax, (const int*) t.roots.p, (const int*) t.leafinst.p, (const int*) t.left.p,
                       (const int*) t.right.p, (const int*) t.dim.p, (const double*) t.coord.p, t.nbrcnt.p,
-                      (const int*) t.nbroff.p, t.nbrlist.p, flag);
+                      (const int*) t.nbroff.p, t.nbrlist.p, t.nbrstride, flag);
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
 }

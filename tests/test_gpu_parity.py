@@ -132,3 +132,15 @@ def test_leaf_overflow_repartition(engines):
     assert sc["rebuilds"] > 5
     assert sg["rebuilds"] == sc["rebuilds"]
     compare(mg, cg, mc, cc)
+
+
+def test_csr_neighbour_lists(engines, monkeypatch):
+    """Neighbour lists in the CSR layout (used when the padded layout would
+    not fit) give the oracle's result as the padded layout does."""
+    print("case hopper n=20000 steps=100 with CSR neighbour lists", flush=True)
+    monkeypatch.setenv("PMX_NEIGHBOUR_LISTS", "csr")
+    m = workloads.hopper(n=20000, seed=6, ntri=800, jitter_speed=5.0, gap=0.00002)
+    f = pmx.forcing(gravity=(0.0, 0.0, -9.81))
+    (mg, cg, sg), (mc, cc, sc) = _both(engines, m, 100 * 2e-6, 2e-6, f)
+    assert len(cc) > 50
+    compare(mg, cg, mc, cc)
