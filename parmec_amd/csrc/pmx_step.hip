@@ -49,13 +49,6 @@ __global__ void k_zero_i (int *a, int n)
   if (i < n) a[i] = 0;
 }
 
-__global__ void k_copy_i (const int *ctl, int *dst, const int *src, int n)
-{
-  if (ctl[CTL_ABORT]) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = src[i];
-}
-
 /* the step completed: advance the step counter */
 __global__ void k_step_end (int *ctl)
 {
@@ -1298,12 +1291,12 @@ int step_detect (pmx_ctx &c)
 }
 
 /* k_rows over the busy lists of nb idle blocks of tpb threads: one block
- * per list, or 8 per list (slices) when there are few lists, so that small
- * models spread their busy spheres over more waves */
+ * per list, or up to 8 per list (slices) when there are fewer than 4096
+ * lists, so that small models spread their busy spheres over more waves */
 template <class R>
 void rows_launch (pmx_ctx &c, R *st0, R *st1, double h, int tpb, int nb)
 {
-  int slices = nb < 4096 ? 8 : 1;
+  int slices = std::max (1, std::min (8, 4096 / std::max (nb, 1)));
   hipLaunchKernelGGL ((k_rows<R>), dim3 (nb * slices), dim3 (64), 0, c.stream, c.E, c.N,
                       (const double*) c.center.p, (const double*) c.radii.p, (const int*) c.part.p,
                       (const int*) c.ellcol.p, (const double*) c.tri.p, c.T, (const int*) c.tricol.p,
