@@ -192,6 +192,7 @@ typedef struct pmx_stats
   double seconds;          /* wall time of the last pmx_dem step loop */
   int det_waves;           /* block shapes in use: waves per detection block, */
   int rows_block;          /* threads per contact-row block */
+  int leaf_slots;          /* leaf-record layout: PMX_LSIZE slots per leaf, or 0 (packed) */
 } pmx_stats;
 
 /* per-kernel timing collected with HIP events on the context's stream */

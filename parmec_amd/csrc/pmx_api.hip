@@ -361,6 +361,8 @@ int pmx_create (const pmx_options *opt, pmx_ctx **out)
   {
     const char *nl = std::getenv ("PMX_NEIGHBOUR_LISTS");
     c->nbr_csr = nl && std::string (nl) == "csr";
+    const char *lr = std::getenv ("PMX_LEAF_RECORDS");
+    c->rec_packed = lr && std::string (lr) == "packed";
   }
   if (c->opt.rows_block) c->rows_tpb = c->opt.rows_block;
   int ndev = 0;
@@ -731,6 +733,7 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
   c->stats.steps = nsteps;
   c->stats.rebuilds = rebuilds;
   c->stats.leaves = c->tree.nleaves;
+  c->stats.leaf_slots = c->lstride;
   c->stats.seconds = std::chrono::duration<double> (std::chrono::steady_clock::now () - tl).count ();
   {
     long long total = 0;

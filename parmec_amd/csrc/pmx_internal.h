@@ -176,7 +176,9 @@ struct pmx_ctx
   /* broad phase per step */
   TreeDev tree;
   dvec<int> leafof, slot, lcount, loff, lmem, lpart;   /* lmem, lpart: leaf-ordered sphere, its particle */
-  dvec<double> lpos;           /* [4][E] leaf-ordered x, y, z, r */
+  dvec<double> lpos;           /* [4][records] leaf-ordered x, y, z, r */
+  int lstride = 0;             /* leaf L's records at L * lstride (PMX_LSIZE slots per leaf, written by
+                                  k_bin), or 0: packed at loff[L] (scan of the counts, then k_scatter) */
   /* contact rows (ping-pong) */
   int cap = 32, hcap = 32, xcap = 16;
   int cur = 0;
@@ -207,6 +209,7 @@ struct pmx_ctx
   int niso = 0;
   int nzc = 0;                 /* particles whose sphere offset is +0 */
   bool nbr_csr = false;        /* neighbour lists always in CSR layout (PMX_NEIGHBOUR_LISTS=csr) */
+  bool rec_packed = false;     /* leaf records always packed (PMX_LEAF_RECORDS=packed) */
   int nnosph = 0;
   bool bin_hint = false;       /* leafof holds leaves of the current tree */
   double rmax = 0.0;           /* largest sphere AABB half-width */

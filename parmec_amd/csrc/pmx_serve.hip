@@ -69,7 +69,8 @@ size_t estimate_bytes (const pmx_model &m, int precision, int cap)
   size_t E = m.ellnum, N = m.parnum, T = m.trinum, S = m.sprnum;
   size_t st = precision == PMX_MIXED ? 4 : 8;
   size_t per_sphere = 2 * (size_t) cap * (4 + 3 * st) + 4 * (size_t) cap + 2 * (size_t) cap   /* rows, hits, reverse hits */
-                      + 48 + 24 + 8 + 32 + 24 + 24 + 200;                                      /* state, records, tree */
+                      + 48 + 24 + 8 + 32 + 24 + 24 + 200                                       /* state, records, tree */
+                      + 120;                                                                   /* slotted records */
   return E * per_sphere + N * 420 + T * 120 + S * 460 + (size_t) m.nodnum * 56 + (size_t) m.facnum * 16 +
          ((size_t) 1 << 20);
 }

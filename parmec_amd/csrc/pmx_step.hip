@@ -56,18 +56,26 @@ __global__ void k_step_end (int *ctl)
 }
 
 /* partitioning_store: descend to the leaf, take a slot (partition.ispc:600-646) */
+/* the leaf of each sphere and its slot there.  With lstride (the slotted
+ * layout: PMX_LSIZE slots per leaf) the sphere's record, member and
+ * particle go straight to slot s of its leaf and its reverse-hit count is
+ * cleared, the work k_scatter does for the packed layout */
 __global__ void k_bin (int E, const double *center, const int *roots, const int *ell_inst, const int *left,
                        const int *right, const int *dim, const double *coord, const double *cell, int nleaves, int *ctl,
-                       int *leafof, int *slot, int *lcount, int *flags, int *ovf, const int *stop)
+                       int *leafof, int *slot, int *lcount, int *flags, int *ovf, const int *stop, int lstride,
+                       const double *radii, const int *part, int *lmem, double *lpos, int *lpart, int *xcnt)
 {
   if (ctl[CTL_ABORT]) return;
   int hint = ctl[CTL_HINT];
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   bool valid = e < E && !frozen (stop, ell_inst[e < E ? e : 0], ctl);
   int L = -1;
+  double c[3] = {0.0, 0.0, 0.0}, rad = 0.0;
+  int pe = 0;
   if (valid)
   {
-    double c[3] = {center[e], center[(size_t) E + e], center[2*(size_t) E + e]};
+    c[0] = center[e]; c[1] = center[(size_t) E + e]; c[2] = center[2*(size_t) E + e];
+    if (lstride) { rad = radii[e]; pe = part[e]; }   /* in flight with the centre */
     if (hint)
     {
       /* the descent "c[d] < coord -> left" ends in leaf h iff lower <= c < upper
@@ -111,13 +119,21 @@ __global__ void k_bin (int E, const double *center, const int *roots, const int 
   int s = base + __popcll (grp & lt);
   if (!valid) return;
   leafof[e] = L;
-  slot[e] = s;
   if (s >= PMX_LSIZE)
   {
     atomicAdd (&flags[FLAG_LEAF_OVERFLOW], 1);
     atomicExch (&ovf[ell_inst[e]], 1);   /* the instance whose tree is rebuilt */
     atomicExch (&ctl[CTL_ABORT], 1);     /* re-partition before this step goes on */
+    return;
   }
+  if (!lstride) { slot[e] = s; return; }
+  size_t pos = (size_t) L * lstride + s;
+  xcnt[e] = 0;   /* reverse hits of this step start at zero */
+  lmem[pos] = e;
+  lpart[pos] = pe;
+  double2 *rec = (double2*) (lpos + 4*pos);
+  rec[0] = make_double2 (c[0], c[1]);
+  rec[1] = make_double2 (c[2], rad);
 }
 
 __global__ void k_scatter (int *ctl, int E, const double *center, const double *radii, const int *leafof, const int *slot,
@@ -126,7 +142,6 @@ __global__ void k_scatter (int *ctl, int E, const double *center, const double *
 {
   if (ctl[CTL_ABORT]) return;
   int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e == 0) ctl[CTL_HINT] = 1;   /* the bin of this step completed: leafof is valid */
   if (e >= E || frozen (stop, ell_inst[e], ctl)) return;
   xcnt[e] = 0;   /* reverse hits of this step start at zero */
   int pos = loff[leafof[e]] + slot[e];
@@ -205,10 +220,11 @@ void k_detect (int E, const double *center, const double *radii, const int *part
                const double *tri, int T, const int *tricnt, const int *trioff, const int *trilist,
                const int *mtcnt, const int *mtlist, int mcap, const int *nbrcnt, const int *nbroff, const int *nbrlist,
                int nbrstride,
-               uint32_t *hits, int *hitcnt, int hcap, int *xbuf, int *xcnt, int xcap, int *flags, const int *ctl,
-               const int *stop, const int *leafinst)
+               uint32_t *hits, int *hitcnt, int hcap, int *xbuf, int *xcnt, int xcap, int *flags, int *ctl,
+               const int *stop, const int *leafinst, int lstride)
 {
   if (ctl[CTL_ABORT]) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl[CTL_HINT] = 1;   /* this step's bin completed: leafof is valid */
   int Q = xcd_block (blockIdx.x, gridDim.x);
   if (frozen (stop, leafinst[Q], ctl)) return;   /* uniform over the block */
   __shared__ double ssph[4*PMX_LSIZE];   /* block spheres: x, y, z, rx */
@@ -218,7 +234,7 @@ void k_detect (int E, const double *center, const double *radii, const int *part
   __shared__ int wmem[NW][PMX_LSIZE], wlist[NW][PMX_LSIZE];
   __shared__ unsigned long long wmask[NW][128];   /* per wave: reach masks of 64 candidates */
   int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  int qoff = loff[Q], qcnt = lcount[Q];
+  int qoff = lstride ? Q * lstride : loff[Q], qcnt = lcount[Q];
   int nc = nbrcnt[Q];
   size_t c0 = nbrstride ? (size_t) Q * nbrstride : (size_t) nbroff[Q];
   int ncw = nc > w ? (nc - w + NW - 1) / NW : 0;   /* candidates of this wave: w, w + NW, ... */
@@ -239,7 +255,7 @@ void k_detect (int E, const double *center, const double *radii, const int *part
     {
       bL = nbrlist[c0 + w + (p0 + lane) * NW];
       bcnt = lcount[bL];
-      boff = loff[bL];
+      boff = lstride ? bL * lstride : loff[bL];
 #pragma unroll
       for (int d = 0; d < 3; d++)
       {
@@ -1211,6 +1227,15 @@ int step_prepare (pmx_ctx &c)
   if (c.lcount.alloc (L) || c.loff.alloc (L)) { pmx_set_error ("out of device memory (bins)"); return PMX_ENOMEM; }
   int rc = clear_leaf_counts (c);
   if (rc) return rc;
+  /* leaf records: PMX_LSIZE slots per leaf (k_bin writes each record into
+   * its slot; no scan, no k_scatter) while that takes at most 4 slots per
+   * sphere and 2^31 slots in all, else packed at the scanned counts (also
+   * when asked for: the environment variable PMX_LEAF_RECORDS=packed) */
+  size_t slots = (size_t) L * PMX_LSIZE, E = std::max (c.E, 1);
+  c.lstride = !c.rec_packed && slots <= 4 * E && slots < ((size_t) 1 << 31) ? PMX_LSIZE : 0;
+  size_t R = c.lstride ? slots : E;
+  if (c.lmem.alloc (R) || c.lpart.alloc (R) || c.lpos.alloc (4 * R))
+  { pmx_set_error ("out of device memory (leaf records)"); return PMX_ENOMEM; }
   /* size the scan scratch for the per-step leaf scan now: nothing may
    * allocate while a step is being captured */
   size_t need = (size_t) L / (PMX_TPB*4) + 64;
@@ -1226,20 +1251,25 @@ int step_bin (pmx_ctx &c)
   /* the leaf counts are zero here: cleared by step_prepare after a build,
    * by k_rows in every step and by the autotuner after its runs */
   {
-    ProfScope ps (c, K_BIN, 32.0 * c.E);
+    /* per sphere: centre 24, leaf 4, slot 4; slotted: no slot, but radius 8,
+     * particle 4, reverse-hit count 4 and the record 32, member 4, particle 4 */
+    ProfScope ps (c, K_BIN, (c.lstride ? 84.0 : 32.0) * c.E);
     hipLaunchKernelGGL (k_bin, grid (c.E), dim3 (PMX_TPB), 0, c.stream, c.E, (const double*) c.center.p,
                         (const int*) t.roots.p, (const int*) c.ell_inst.p, (const int*) t.left.p, (const int*) t.right.p, (const int*) t.dim.p, (const double*) t.coord.p,
                         (const double*) t.cell.p, t.nleaves, c.ctl.p, c.leafof.p, c.slot.p, c.lcount.p, c.dflags.p,
-                        c.ovf.p, c.staggered ? (const int*) c.istop.p : (const int*) nullptr);
+                        c.ovf.p, c.staggered ? (const int*) c.istop.p : (const int*) nullptr, c.lstride,
+                        (const double*) c.radii.p, (const int*) c.part.p, c.lmem.p, c.lpos.p, c.lpart.p, c.xcnt.p);
   }
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
 }
 
-/* leaf offsets, leaf-ordered records and cleared reverse-hit counts: the
- * inputs of k_detect */
+/* packed layout: leaf offsets, leaf-ordered records and cleared
+ * reverse-hit counts, the inputs of k_detect (k_bin has written them in
+ * the slotted layout) */
 int step_scatter (pmx_ctx &c)
 {
+  if (c.lstride) return PMX_OK;   /* slotted layout: k_bin wrote the records */
   TreeDev &t = c.tree;
   int L = std::max (t.nleaves, 1);
   int rc = scan_exclusive_to (c, (const int*) c.lcount.p, c.loff.p, L, c.ctl.p);
@@ -1266,8 +1296,9 @@ void detect_launch (pmx_ctx &c)
                       (const int*) t.tricnt.p, (const int*) t.trioff.p, (const int*) t.trilist.p,
                       c.nmtri > 0 ? (const int*) c.mtcnt.p : (const int*) nullptr, (const int*) c.mtlist.p, c.mcap,
                       (const int*) t.nbrcnt.p, (const int*) t.nbroff.p, (const int*) t.nbrlist.p, t.nbrstride,
-                      c.hits.p, c.hitcnt.p, c.hcap, c.xbuf.p, c.xcnt.p, c.xcap, c.dflags.p, (const int*) c.ctl.p,
-                      c.staggered ? (const int*) c.istop.p : (const int*) nullptr, (const int*) t.leafinst.p);
+                      c.hits.p, c.hitcnt.p, c.hcap, c.xbuf.p, c.xcnt.p, c.xcap, c.dflags.p, c.ctl.p,
+                      c.staggered ? (const int*) c.istop.p : (const int*) nullptr, (const int*) t.leafinst.p,
+                      c.lstride);
 }
 
 /* k_detect with nw waves per block (1, 2, 4 or 8) */

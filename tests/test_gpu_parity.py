@@ -74,6 +74,7 @@ def test_hopper_small(engines):
     f = pmx.forcing(gravity=(0.0, 0.0, -9.81))
     (mg, cg, sg), (mc, cc, sc) = _both(engines, m, 200 * 2e-6, 2e-6, f)
     assert len(cc) > 50
+    assert sg["leaf_slots"] == 96   # the slotted leaf-record layout
     compare(mg, cg, mc, cc)
 
 
@@ -143,4 +144,18 @@ def test_csr_neighbour_lists(engines, monkeypatch):
     f = pmx.forcing(gravity=(0.0, 0.0, -9.81))
     (mg, cg, sg), (mc, cc, sc) = _both(engines, m, 100 * 2e-6, 2e-6, f)
     assert len(cc) > 50
+    compare(mg, cg, mc, cc)
+
+
+def test_packed_leaf_records(engines, monkeypatch):
+    """Leaf records packed at the scanned leaf counts (used when the slotted
+    layout would take more than 4 slots per sphere) give the oracle's result
+    as the slotted layout does."""
+    print("case hopper n=20000 steps=100 with packed leaf records", flush=True)
+    monkeypatch.setenv("PMX_LEAF_RECORDS", "packed")
+    m = workloads.hopper(n=20000, seed=7, ntri=800, jitter_speed=5.0, gap=0.00002)
+    f = pmx.forcing(gravity=(0.0, 0.0, -9.81))
+    (mg, cg, sg), (mc, cc, sc) = _both(engines, m, 100 * 2e-6, 2e-6, f)
+    assert len(cc) > 50
+    assert sg["leaf_slots"] == 0
     compare(mg, cg, mc, cc)
