@@ -193,6 +193,8 @@ typedef struct pmx_stats
   int det_waves;           /* block shapes in use: waves per detection block, */
   int rows_block;          /* threads per contact-row block */
   int leaf_slots;          /* leaf-record layout: PMX_LSIZE slots per leaf, or 0 (packed) */
+  long long partial_rebuilds;  /* rebuilds of the last call that re-sorted only the overflowing
+                                  instances of a batch (counted in rebuilds too) */
 } pmx_stats;
 
 /* per-kernel timing collected with HIP events on the context's stream */

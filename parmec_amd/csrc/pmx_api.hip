@@ -630,7 +630,7 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
   c->graph_valid = false;
   c->stats.max_row = 0;
 
-  long long rebuilds = 0;
+  long long rebuilds = 0, partial = 0;
   if (c->E > 0)
   {
     if ((rc = rebuild (c, true))) return rc;
@@ -719,6 +719,7 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
       failed_at = done;
       if ((rc = rebuild (c, false))) return rc;
       rebuilds++;
+      if (c->tree.last_partial) partial++;
     }
     if ((rc = report (done))) return rc;
     if (tuning && c->tuned == 1 && !ctl[CTL_ABORT] && done < nsteps && c->stepped + done >= PMX_RETUNE_STEPS &&
@@ -732,6 +733,7 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
   double dt = std::chrono::duration<double> (std::chrono::steady_clock::now () - t0).count ();
   c->stats.steps = nsteps;
   c->stats.rebuilds = rebuilds;
+  c->stats.partial_rebuilds = partial;
   c->stats.leaves = c->tree.nleaves;
   c->stats.leaf_slots = c->lstride;
   c->stats.seconds = std::chrono::duration<double> (std::chrono::steady_clock::now () - tl).count ();

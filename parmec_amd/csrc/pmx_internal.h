@@ -74,7 +74,13 @@ struct TreeDev
   dvec<int> first, size, split, dim, parent;
   dvec<double> coord;
   dvec<int> left, right;     /* child refs: >= 0 branch node, < 0 leaf -(L+1) */
-  dvec<int> nleafkids;       /* per internal node: leaf children (scan -> leaf ids) */
+  dvec<int> nleafkids;       /* per internal node: leaf children */
+  dvec<int> lfirst;          /* per internal node: its first leaf id (scan of nleafkids) */
+  dvec<int> sel;             /* spheres (and nodes) of the instances a partial build redoes */
+  dvec<uint64_t> scode;      /* sort scratch of a partial build */
+  dvec<int> sorder;
+  bool ready = false;        /* a full build has been made: partial builds may follow */
+  bool last_partial = false; /* the last build redid only the overflowing instances */
   dvec<double> cmin;         /* [3][n] minima of a node's terminal right child */
   dvec<unsigned long long> cmkey;  /* [3][n] ordered keys of the minima of each node's range */
   /* leaves */

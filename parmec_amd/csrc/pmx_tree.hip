@@ -167,11 +167,14 @@ __global__ void k_extents_final (int nb, double *ext /* ext[0..5] out, partials 
   if (threadIdx.x == 0) for (int k = 0; k < 6; k++) ext[k] = e[k];
 }
 
-/* per-instance extents, one block per instance */
-__global__ void k_extents_seg (const int *off, const double *cx, const double *cy, const double *cz, double *ext)
+/* per-instance extents, one block per instance (only the instances with
+ * a nonzero flag when flag is not null) */
+__global__ void k_extents_seg (const int *off, const double *cx, const double *cy, const double *cz, double *ext,
+                               const int *flag)
 {
   __shared__ double sh[6*(PMX_TPB/64)];
   int inst = blockIdx.x;
+  if (flag && !flag[inst]) return;   /* uniform over the block */
   double e[6] = {DBL_MAX, DBL_MAX, DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX};
   for (int i = off[inst] + threadIdx.x; i < off[inst+1]; i += blockDim.x)
   {
@@ -214,11 +217,14 @@ __global__ void k_snap (int n, const double *center, double *bcen, const int *in
 
 /* -------------------------------------------------------------- morton */
 
+/* code and sphere of the p-th sphere of the build: sphere sel[p], or p
+ * when sel is null */
 __global__ void k_morton (int n, const double *cx, const double *cy, const double *cz, const double *ext_all,
-                          const int *inst, uint64_t *code, int *order)
+                          const int *inst, uint64_t *code, int *order, const int *sel)
 {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= n) return;
+  int i = sel ? sel[w] : w;
   int ins = inst[i];
   const double *ext = ext_all + 6*ins;
   double p[3] = {cx[i], cy[i], cz[i]};
@@ -233,8 +239,29 @@ __global__ void k_morton (int n, const double *cx, const double *cy, const doubl
     s = s < 1023.0 ? s : 1023.0;
     q[k] = expandbits ((unsigned) (int) s);
   }
-  code[i] = ((uint64_t) ins << 30) | (uint64_t) (q[0]*4 + q[1]*2 + q[2]);
-  order[i] = i;
+  code[w] = ((uint64_t) ins << 30) | (uint64_t) (q[0]*4 + q[1]*2 + q[2]);
+  order[w] = i;
+}
+
+/* a partial build sorts the spheres of whole instances, listed in sphere
+ * order: the instance segments of the sorted list are those of sel, so the
+ * p-th sorted entry goes to position sel[p] of the instance-ordered arrays */
+__global__ void k_place (int n, const int *sel, const uint64_t *key, const int *val, uint64_t *key2, int *val2)
+{
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  int i = sel[p];
+  key2[i] = key[p];
+  val2[i] = val[p];
+}
+
+/* cm keys of the selected nodes to the maximum (the identity of min) */
+__global__ void k_cm_init (int n, int stride, const int *sel, unsigned long long *cm)
+{
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  int i = sel[p];
+  for (int d = 0; d < 3; d++) cm[(size_t) d*stride + i] = ~0ull;
 }
 
 /* ---------------------------------------------------------- radix sort */
@@ -289,10 +316,13 @@ __global__ void k_radix_scatter (int n, const uint64_t *key, const int *val, int
 
 /* ----------------------------------------------------------- karras */
 
-__global__ void k_karras (int n, const uint64_t *key, const int *off, int *first, int *size, int *split, int *dim)
+/* node sel[p] (node p when sel is null) of the instance's Karras tree */
+__global__ void k_karras (int n, const uint64_t *key, const int *off, int *first, int *size, int *split, int *dim,
+                          const int *sel)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (sel) i = sel[i];
   int ins = (int) (key[i] >> 30);
   int lo = off[ins], hi = off[ins+1] - 1;
   if (i >= hi) { dim[i] = -1; size[i] = 0; first[i] = i; split[i] = 0; return; }   /* not an internal node */
@@ -330,10 +360,11 @@ __global__ void k_karras (int n, const uint64_t *key, const int *off, int *first
 /* leaf children per branch node (plus the root itself when an instance's
  * whole range is one terminal node); branch flags for the coordinate pass */
 __global__ void k_children (int n, const uint64_t *key, const int *off, const int *first, const int *size,
-                            const int *split, const int *dim, int *nleafkids, int *parent)
+                            const int *split, const int *dim, int *nleafkids, int *parent, const int *sel)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (sel) i = sel[i];
   int ins = (int) (key[i] >> 30);
   int lo = off[ins];
   int k = 0;
@@ -405,13 +436,13 @@ __device__ __forceinline__ void range_min (int a, int b, const int *order, const
 }
 
 /* one thread per (branch node, side) whose child is a terminal range */
-__global__ void k_minfold (int n, const int *first, const int *size, const int *split, const int *dim,
+__global__ void k_minfold (int cnt, int n, const int *first, const int *size, const int *split, const int *dim,
                            const int *parent, const int *order, const double *cx, const double *cy, const double *cz,
-                           unsigned long long *cm, double *rmin)
+                           unsigned long long *cm, double *rmin, const int *sel)
 {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= 2*n) return;
-  int i = q >> 1, side = q & 1;
+  if (q >= 2*cnt) return;
+  int i = sel ? sel[q >> 1] : q >> 1, side = q & 1;
   if (dim[i] < 0) return;
   int j = split[i] + side;
   int a, b;
@@ -431,11 +462,13 @@ __global__ void k_minfold (int n, const int *first, const int *size, const int *
   }
 }
 
-__global__ void k_coord_set (int n, const int *first, const int *size, const int *split, const int *dim,
-                             const unsigned long long *cm, const double *rmin, double *coord)
+__global__ void k_coord_set (int m, int n, const int *first, const int *size, const int *split, const int *dim,
+                             const unsigned long long *cm, const double *rmin, double *coord, const int *sel)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || dim[i] < 0) return;
+  if (i >= m) return;
+  if (sel) i = sel[i];
+  if (dim[i] < 0) return;
   int d = dim[i], j = split[i] + 1;
   bool term = first[i] + size[i] - 1 == j || dim[j] < 0;
   coord[i] = term ? rmin[(size_t) d*n + i] : dval (cm[(size_t) d*n + j]);
@@ -737,79 +770,133 @@ int tree_build (pmx_ctx &c, bool all)
   if (t.ext.alloc (std::max (6*(size_t) I, 6 + 6*(size_t) nbext)) || t.code.alloc (n) || t.code2.alloc (n) ||
       t.order.alloc (n) || t.order2.alloc (n) || t.hist.alloc (std::max ((size_t) RADIX * nbsort + 1, 2*(size_t) n + 2)) ||
       t.first.alloc (n) || t.size.alloc (n) || t.split.alloc (n) || t.dim.alloc (n) || t.parent.alloc (n) ||
-      t.coord.alloc (n) || t.left.alloc (n) || t.right.alloc (n) || t.nleafkids.alloc (n + 1) || t.roots.alloc (I) ||
-      t.cmin.alloc (3*(size_t) n) || t.cmkey.alloc (3*(size_t) n))
+      t.coord.alloc (n) || t.left.alloc (n) || t.right.alloc (n) || t.nleafkids.alloc (n + 1) ||
+      t.lfirst.alloc (n + 1) || t.roots.alloc (I) || t.cmin.alloc (3*(size_t) n) || t.cmkey.alloc (3*(size_t) n))
   { pmx_set_error ("out of device memory (tree)"); return PMX_ENOMEM; }
 
-  /* extents per instance (partition.ispc:29-54, 669-684) */
-  if (I == 1)
+  /* A rebuild forced by some instances (ovf) re-sorts only their spheres
+   * and rebuilds only their nodes: every other instance's sorted codes,
+   * nodes, leaf-children counts and split coordinates are functions of its
+   * unchanged snapshot and stay as the last build left them (node and
+   * sorted-sphere ranges are the instance's sphere range).  The leaf
+   * numbering, cells and per-leaf lists below are redone for all.  A full
+   * build when most of the spheres are flagged. */
+  int m = n;
+  const int *sel = nullptr;
+  std::vector<int> list;
+  t.last_partial = false;
+  if (!all && t.ready && I > 1)
   {
-    hipLaunchKernelGGL (k_extents_partial, dim3 (nbext), dim3 (PMX_TPB), 0, c.stream, n, cx, cy, cz, t.ext.p + 6);
-    hipLaunchKernelGGL (k_extents_final, dim3 (1), dim3 (PMX_TPB), 0, c.stream, nbext, t.ext.p);
+    std::vector<int> flag (I);
+    PMX_HIP (hipMemcpyAsync (flag.data (), c.ovf.p, sizeof (int) * I, hipMemcpyDeviceToHost, c.stream));
+    PMX_HIP (hipStreamSynchronize (c.stream));
+    size_t ns = 0;
+    for (int i = 0; i < I; i++) if (flag[i]) ns += (size_t) (c.off[i+1].ell - c.off[i].ell);
+    if (4 * ns <= 3 * (size_t) n)
+    {
+      list.reserve (ns);
+      for (int i = 0; i < I; i++)
+        if (flag[i]) for (int e = c.off[i].ell; e < c.off[i+1].ell; e++) list.push_back (e);
+      m = (int) ns;
+      if (t.sel.alloc (std::max (m, 1)) || t.scode.alloc (std::max (m, 1)) || t.sorder.alloc (std::max (m, 1)))
+      { pmx_set_error ("out of device memory (tree)"); return PMX_ENOMEM; }
+      if (m > 0) PMX_HIP (hipMemcpyAsync (t.sel.p, list.data (), sizeof (int) * m, hipMemcpyHostToDevice, c.stream));
+      sel = t.sel.p;
+      t.last_partial = true;
+    }
   }
-  else
-    hipLaunchKernelGGL (k_extents_seg, dim3 (I), dim3 (PMX_TPB), 0, c.stream, (const int*) c.ell_off.p, cx, cy, cz, t.ext.p);
-  hipLaunchKernelGGL (k_morton, grid (n), dim3 (PMX_TPB), 0, c.stream, n, cx, cy, cz, (const double*) t.ext.p,
-                      (const int*) c.ell_inst.p, t.code.p, t.order.p);
-  PMX_HIP (hipGetLastError ());
 
-  /* stable LSD radix sort of (instance, code) */
-  int bits = 30;
-  while ((1ll << (bits - 30)) < I) bits++;
-  uint64_t *k0 = t.code.p, *k1 = t.code2.p;
-  int *v0 = t.order.p, *v1 = t.order2.p;
-  for (int shift = 0; shift < bits; shift += RADIX_BITS)
+  if (m > 0)
   {
-    hipLaunchKernelGGL (k_radix_hist, dim3 (nbsort), dim3 (PMX_TPB), 0, c.stream, n, (const uint64_t*) k0, shift,
-                        t.hist.p, nbsort);
+    /* extents per instance (partition.ispc:29-54, 669-684) */
+    if (I == 1)
+    {
+      hipLaunchKernelGGL (k_extents_partial, dim3 (nbext), dim3 (PMX_TPB), 0, c.stream, n, cx, cy, cz, t.ext.p + 6);
+      hipLaunchKernelGGL (k_extents_final, dim3 (1), dim3 (PMX_TPB), 0, c.stream, nbext, t.ext.p);
+    }
+    else
+      hipLaunchKernelGGL (k_extents_seg, dim3 (I), dim3 (PMX_TPB), 0, c.stream, (const int*) c.ell_off.p, cx, cy, cz,
+                          t.ext.p, sel ? (const int*) c.ovf.p : (const int*) nullptr);
+    /* codes of the build's spheres: in place (full build) or packed in
+     * sel's order (partial build), sorted, then placed at sel */
+    uint64_t *k0 = sel ? t.code2.p : t.code.p, *k1 = sel ? t.scode.p : t.code2.p;
+    int *v0 = sel ? t.order2.p : t.order.p, *v1 = sel ? t.sorder.p : t.order2.p;
+    hipLaunchKernelGGL (k_morton, grid (m), dim3 (PMX_TPB), 0, c.stream, m, cx, cy, cz, (const double*) t.ext.p,
+                        (const int*) c.ell_inst.p, k0, v0, sel);
     PMX_HIP (hipGetLastError ());
-    int rc = scan_exclusive (c, t.hist.p, RADIX * nbsort, nullptr);
-    if (rc) return rc;
-    hipLaunchKernelGGL (k_radix_scatter, dim3 (nbsort), dim3 (PMX_TPB), 0, c.stream, n, (const uint64_t*) k0,
-                        (const int*) v0, shift, (const int*) t.hist.p, nbsort, k1, v1);
-    PMX_HIP (hipGetLastError ());
-    std::swap (k0, k1);
-    std::swap (v0, v1);
-  }
-  const uint64_t *key = k0;
-  const int *order = v0;
 
-  /* Karras nodes per instance range (partition.ispc:311-394); an instance
-   * without spheres keeps PMX_NO_ROOT */
+    /* stable LSD radix sort of (instance, code) */
+    int bits = 30;
+    while ((1ll << (bits - 30)) < I) bits++;
+    int nbs = (m + PMX_TPB - 1) / PMX_TPB;
+    for (int shift = 0; shift < bits; shift += RADIX_BITS)
+    {
+      hipLaunchKernelGGL (k_radix_hist, dim3 (nbs), dim3 (PMX_TPB), 0, c.stream, m, (const uint64_t*) k0, shift,
+                          t.hist.p, nbs);
+      PMX_HIP (hipGetLastError ());
+      int rc = scan_exclusive (c, t.hist.p, RADIX * nbs, nullptr);
+      if (rc) return rc;
+      hipLaunchKernelGGL (k_radix_scatter, dim3 (nbs), dim3 (PMX_TPB), 0, c.stream, m, (const uint64_t*) k0,
+                          (const int*) v0, shift, (const int*) t.hist.p, nbs, k1, v1);
+      PMX_HIP (hipGetLastError ());
+      std::swap (k0, k1);
+      std::swap (v0, v1);
+    }
+    if (sel)
+      hipLaunchKernelGGL (k_place, grid (m), dim3 (PMX_TPB), 0, c.stream, m, sel, (const uint64_t*) k0,
+                          (const int*) v0, t.code.p, t.order.p);
+    else if (k0 != t.code.p)
+    {
+      PMX_HIP (hipMemcpyAsync (t.code.p, k0, sizeof (uint64_t) * n, hipMemcpyDeviceToDevice, c.stream));
+      PMX_HIP (hipMemcpyAsync (t.order.p, v0, sizeof (int) * n, hipMemcpyDeviceToDevice, c.stream));
+    }
+    PMX_HIP (hipGetLastError ());
+    const uint64_t *key = t.code.p;
+    const int *order = t.order.p;
+
+    /* Karras nodes per instance range (partition.ispc:311-394) */
+    hipLaunchKernelGGL (k_karras, grid (m), dim3 (PMX_TPB), 0, c.stream, m, key, (const int*) c.ell_off.p, t.first.p,
+                        t.size.p, t.split.p, t.dim.p, sel);
+    PMX_HIP (hipGetLastError ());
+    hipLaunchKernelGGL (k_children, grid (m), dim3 (PMX_TPB), 0, c.stream, m, key, (const int*) c.ell_off.p,
+                        (const int*) t.first.p, (const int*) t.size.p, (const int*) t.split.p, (const int*) t.dim.p,
+                        t.nleafkids.p, t.parent.p, sel);
+    PMX_HIP (hipGetLastError ());
+
+    /* split coordinates of the branch nodes (mincoord, partition.ispc:297-308) */
+    if (sel)
+      hipLaunchKernelGGL (k_cm_init, grid (m), dim3 (PMX_TPB), 0, c.stream, m, n, sel, t.cmkey.p);
+    else
+      PMX_HIP (hipMemsetAsync (t.cmkey.p, 0xFF, sizeof (unsigned long long) * 3 * (size_t) n, c.stream));
+    hipLaunchKernelGGL (k_minfold, grid (2*m), dim3 (PMX_TPB), 0, c.stream, m, n, (const int*) t.first.p,
+                        (const int*) t.size.p, (const int*) t.split.p, (const int*) t.dim.p, (const int*) t.parent.p,
+                        order, cx, cy, cz, t.cmkey.p, t.cmin.p, sel);
+    hipLaunchKernelGGL (k_coord_set, grid (m), dim3 (PMX_TPB), 0, c.stream, m, n, (const int*) t.first.p,
+                        (const int*) t.size.p, (const int*) t.split.p, (const int*) t.dim.p,
+                        (const unsigned long long*) t.cmkey.p, (const double*) t.cmin.p, t.coord.p, sel);
+    PMX_HIP (hipGetLastError ());
+  }
+  const uint64_t *key = t.code.p;
+  /* an instance without spheres keeps PMX_NO_ROOT */
   hipLaunchKernelGGL (k_set_i, grid (I), dim3 (PMX_TPB), 0, c.stream, t.roots.p, I, PMX_NO_ROOT);
-  hipLaunchKernelGGL (k_karras, grid (n), dim3 (PMX_TPB), 0, c.stream, n, key, (const int*) c.ell_off.p, t.first.p,
-                      t.size.p, t.split.p, t.dim.p);
-  PMX_HIP (hipGetLastError ());
-  hipLaunchKernelGGL (k_children, grid (n), dim3 (PMX_TPB), 0, c.stream, n, key, (const int*) c.ell_off.p,
-                      (const int*) t.first.p, (const int*) t.size.p, (const int*) t.split.p, (const int*) t.dim.p,
-                      t.nleafkids.p, t.parent.p);
-  PMX_HIP (hipGetLastError ());
-
-  /* split coordinates of the branch nodes (mincoord, partition.ispc:297-308) */
-  PMX_HIP (hipMemsetAsync (t.cmkey.p, 0xFF, sizeof (unsigned long long) * 3 * (size_t) n, c.stream));
-  hipLaunchKernelGGL (k_minfold, grid (2*n), dim3 (PMX_TPB), 0, c.stream, n, (const int*) t.first.p,
-                      (const int*) t.size.p, (const int*) t.split.p, (const int*) t.dim.p, (const int*) t.parent.p,
-                      order, cx, cy, cz, t.cmkey.p, t.cmin.p);
-  hipLaunchKernelGGL (k_coord_set, grid (n), dim3 (PMX_TPB), 0, c.stream, n, (const int*) t.first.p,
-                      (const int*) t.size.p, (const int*) t.split.p, (const int*) t.dim.p,
-                      (const unsigned long long*) t.cmkey.p, (const double*) t.cmin.p, t.coord.p);
-  PMX_HIP (hipGetLastError ());
 
   /* leaf numbering and reach cells */
   int nleaves = 0;
-  int rc = scan_exclusive (c, t.nleafkids.p, n, &nleaves);
+  PMX_HIP (hipMemcpyAsync (t.lfirst.p, t.nleafkids.p, sizeof (int) * n, hipMemcpyDeviceToDevice, c.stream));
+  int rc = scan_exclusive (c, t.lfirst.p, n, &nleaves);
   if (rc) return rc;
   t.nleaves = nleaves;
   if (t.leafpar.alloc (nleaves) || t.leafinst.alloc (nleaves) || t.cell.alloc (6*(size_t) nleaves))
   { pmx_set_error ("out of device memory (tree)"); return PMX_ENOMEM; }
   hipLaunchKernelGGL (k_assign, grid (n), dim3 (PMX_TPB), 0, c.stream, n, key, (const int*) c.ell_off.p,
                       (const int*) t.first.p, (const int*) t.size.p, (const int*) t.split.p, (const int*) t.dim.p,
-                      (const int*) t.nleafkids.p, t.left.p, t.right.p, t.leafpar.p, t.leafinst.p, t.roots.p);
+                      (const int*) t.lfirst.p, t.left.p, t.right.p, t.leafpar.p, t.leafinst.p, t.roots.p);
   hipLaunchKernelGGL (k_cells, grid (nleaves), dim3 (PMX_TPB), 0, c.stream, nleaves, (const int*) t.leafpar.p,
                       (const int*) t.parent.p, (const int*) t.dim.p, (const double*) t.coord.p, t.cell.p);
   PMX_HIP (hipGetLastError ());
   PMX_HIP (hipMemcpyAsync (&t.root, t.roots.p, sizeof (int), hipMemcpyDeviceToHost, c.stream));
   PMX_HIP (hipStreamSynchronize (c.stream));
+  t.ready = true;
   return PMX_OK;
 }
 

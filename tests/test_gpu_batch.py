@@ -104,3 +104,31 @@ def test_batch_order_invariance():
     ms2, cs2, _ = _batch([models[p] for p in perm], duration, step, [forcings[p] for p in perm])
     for k, p in enumerate(perm):
         compare(ms2[k], cs2[k], ms1[p], cs1[p])
+
+
+def test_partial_rebuild_batch():
+    """One instance of a batch overflows its leaves again and again: each
+    rebuild re-sorts and re-partitions only that instance (the others keep
+    their trees), and every instance still gives bitwise its solo result."""
+    step, nsteps = 1e-5, 150
+    models, forcings = _sweep_models(5, 500, 3000, nsteps, step)
+    c = workloads.box_lattice(n=4000, seed=3)
+    xc = c.center[:3].mean(axis=1, keepdims=True)
+    c.linear[:] = -75.0 * (c.center[:3] - xc)       # converging flow: leaf overflow
+    models.insert(2, c)
+    forcings.insert(2, pmx.forcing(nsteps))
+    duration = (nsteps - 0.5) * step
+    print("case partial rebuilds in a batch of 6", flush=True)
+    ms, cs, st = _batch(models, duration, step, forcings, calls=2)
+    assert st["partial_rebuilds"] > 2
+    assert st["partial_rebuilds"] == st["rebuilds"] - 1   # all but the build at the call's start
+    for i, m in enumerate(models):
+        eng = pmx.Engine()
+        mm = m.copy()
+        eng.upload(mm)
+        for _ in range(2):
+            eng.dem(duration, step, forcings[i])
+        eng.download(mm)
+        ca = eng.contacts()
+        eng.close()
+        compare(ms[i], cs[i], mm, ca)
