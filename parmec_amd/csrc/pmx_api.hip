@@ -309,12 +309,12 @@ int upload_core (pmx_ctx *c, const pmx_model *m, const std::vector<InstOff> &off
   return PMX_OK;
 }
 
-void prof_resolve (pmx_ctx &c)
+void prof_resolve (pmx_ctx &c, int complete)
 {
   for (auto &p : c.prof.pend)
   {
     float ms = 0.0f;
-    if (hipEventElapsedTime (&ms, p.a, p.b) == hipSuccess)
+    if (p.step < complete && hipEventElapsedTime (&ms, p.a, p.b) == hipSuccess)
     {
       c.prof.ms[p.kid] += ms;
       c.prof.n[p.kid] += 1;
@@ -410,6 +410,8 @@ void pmx_destroy (pmx_ctx *c)
   c->tree.code.release (); c->tree.code2.release ();
   c->ctl.release ();
   if (c->gexec) (void) hipGraphExecDestroy (c->gexec);
+  if (c->hring) (void) hipHostFree (c->hring);
+  for (hipEvent_t e : c->rev) if (e) (void) hipEventDestroy (e);
   prof_resolve (*c);
   for (hipEvent_t e : c->prof.pool) (void) hipEventDestroy (e);
   if (c->stream) (void) hipStreamDestroy (c->stream);
@@ -467,14 +469,9 @@ static int rebuild (pmx_ctx *c, bool all)
   return PMX_OK;
 }
 
-/* read the control block and the error flags (synchronises the stream) */
-static int read_flags (pmx_ctx *c, int ctl[CTL_COUNT])
+/* the error flags of a group of steps */
+static int check_flags (pmx_ctx *c, const int *fl)
 {
-  int fl[FLAG_COUNT];
-  PMX_HIP (hipMemcpyAsync (ctl, c->ctl.p, sizeof (int) * CTL_COUNT, hipMemcpyDeviceToHost, c->stream));
-  PMX_HIP (hipMemcpyAsync (fl, c->dflags.p, sizeof (fl), hipMemcpyDeviceToHost, c->stream));
-  PMX_HIP (hipStreamSynchronize (c->stream));
-  prof_resolve (*c);
   if (fl[FLAG_ROW_OVERFLOW] || fl[FLAG_HIT_OVERFLOW] || fl[FLAG_EXTRA_OVERFLOW] || fl[FLAG_TRILIST_OVERFLOW])
     return fail (PMX_ECAPACITY, "contact capacity exceeded (row " + std::to_string (fl[FLAG_ROW_OVERFLOW]) + ", hits " +
                  std::to_string (fl[FLAG_HIT_OVERFLOW]) + ", reverse " + std::to_string (fl[FLAG_EXTRA_OVERFLOW]) +
@@ -687,30 +684,72 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
       }
     return PMX_OK;
   };
-  /* with profiling on, every step runs eagerly and synchronises (events per
-   * launch); otherwise one step is captured as a graph and replayed in
-   * chunks, with the stream synchronised once per chunk (and where an
-   * instance finishes) */
-  const int chunk = c->prof.on ? 1 : 64;
+  /* Steps go to the device in groups: one captured step replayed as a
+   * graph per step, or, with profiling on, each step's kernels enqueued
+   * eagerly with an event pair per launch.  After each group the control
+   * block and the flags are copied to pinned host memory, and up to
+   * PMX_LOOKAHEAD groups are enqueued ahead of the one whose copy the host
+   * waits for, so the device does not wait for the host between groups.  A
+   * leaf overflow aborts the step it occurs in; the groups enqueued behind
+   * it return at once.  Large models take groups of one step (an abort then
+   * wastes little), small ones longer groups (per-step copies would cost
+   * more than their steps).  The host stops at the steps where instances
+   * finish. */
+  const int gsize = c->E >= 1000000 ? 1 : c->E >= 100000 ? 8 : 64;
+  const int slotn = CTL_COUNT + FLAG_COUNT;
+  if (!c->hring)
+  {
+    if (hipHostMalloc ((void**) &c->hring, sizeof (int) * slotn * PMX_RING) != hipSuccess)
+    { c->hring = nullptr; return fail (PMX_ENOMEM, "out of pinned host memory"); }
+    for (int k = 0; k < PMX_RING; k++) PMX_HIP (hipEventCreateWithFlags (&c->rev[k], hipEventDisableTiming));
+  }
   int done = 0, failed_at = -1;
   while (done < nsteps)
   {
     int next = nsteps;
     for (int i = 0; i < I; i++) if (!reported[i] && nst[i] > done) next = std::min (next, nst[i]);
-    int n = std::min (chunk, next - done);
-    if (c->prof.on)
+    if (!c->prof.on && !c->graph_valid && (rc = capture (c, step0))) return rc;
+    int ctl[CTL_COUNT] = {0}, fl[FLAG_COUNT] = {0};
+    int enq = done, gq = 0, gs = 0;   /* first step not enqueued; groups enqueued, seen */
+    int err = PMX_OK;
+    bool abort = false;
+    for (;;)
     {
-      if ((rc = enqueue_step (*c, step0))) return rc;
+      while (enq < next && gq - gs < PMX_LOOKAHEAD)
+      {
+        int n = std::min (gsize, next - enq);
+        for (int q = 0; q < n; q++)
+          if (c->prof.on)
+          {
+            c->prof.step = enq + q;
+            rc = enqueue_step (*c, step0);
+            c->prof.step = -1;
+            if (rc) return rc;
+          }
+          else
+            PMX_HIP (hipGraphLaunch (c->gexec, c->stream));
+        int *slot = c->hring + (gq % PMX_RING) * slotn;
+        PMX_HIP (hipMemcpyAsync (slot, c->ctl.p, sizeof (int) * CTL_COUNT, hipMemcpyDeviceToHost, c->stream));
+        PMX_HIP (hipMemcpyAsync (slot + CTL_COUNT, c->dflags.p, sizeof (int) * FLAG_COUNT, hipMemcpyDeviceToHost,
+                                 c->stream));
+        PMX_HIP (hipEventRecord (c->rev[gq % PMX_RING], c->stream));
+        enq += n;
+        gq++;
+      }
+      if (gs == gq) break;
+      PMX_HIP (hipEventSynchronize (c->rev[gs % PMX_RING]));
+      const int *slot = c->hring + (gs % PMX_RING) * slotn;
+      std::memcpy (ctl, slot, sizeof (ctl));
+      std::memcpy (fl, slot + CTL_COUNT, sizeof (fl));
+      gs++;
+      if ((err = check_flags (c, fl))) break;
+      if ((abort = ctl[CTL_ABORT] != 0)) break;
     }
-    else
-    {
-      if (!c->graph_valid && (rc = capture (c, step0))) return rc;
-      for (int q = 0; q < n; q++) PMX_HIP (hipGraphLaunch (c->gexec, c->stream));
-    }
-    int ctl[CTL_COUNT];
-    if ((rc = read_flags (c, ctl))) return rc;
+    PMX_HIP (hipStreamSynchronize (c->stream));   /* the groups behind an abort */
+    prof_resolve (*c, ctl[CTL_KSTEP]);
+    if (err) return err;
     done = ctl[CTL_KSTEP];
-    if (ctl[CTL_ABORT])
+    if (abort)
     {
       /* a leaf overflowed in step `done`: re-partition the overflowing
        * instances from the current centres and repeat that step
@@ -722,7 +761,7 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
       if (c->tree.last_partial) partial++;
     }
     if ((rc = report (done))) return rc;
-    if (tuning && c->tuned == 1 && !ctl[CTL_ABORT] && done < nsteps && c->stepped + done >= PMX_RETUNE_STEPS &&
+    if (tuning && c->tuned == 1 && !abort && done < nsteps && c->stepped + done >= PMX_RETUNE_STEPS &&
         (rc = tune (done)))
       return rc;
   }

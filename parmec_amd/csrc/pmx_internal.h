@@ -15,7 +15,9 @@
 #include "../../include/pmx.h"
 
 #define PMX_TPB 256            /* threads per block of the elementwise kernels */
-#define PMX_RETUNE_STEPS 256   /* steps of a context before the second autotuner pass */
+#define PMX_RETUNE_STEPS 256
+#define PMX_LOOKAHEAD 2      /* groups of steps enqueued ahead of the one the host waits for */
+#define PMX_RING 4           /* slots of the pinned flag ring (> PMX_LOOKAHEAD) */   /* steps of a context before the second autotuner pass */
 #define PMX_TRI_FLAG 0x80000000u  /* row key flag of a triangle partner */
 #define PMX_NO_ROOT INT_MIN       /* tree root of an instance without spheres */
 
@@ -108,7 +110,8 @@ enum { K_BIN = 0, K_SCATTER, K_DETECT, K_FORCE_INIT, K_ROWS, K_DYNAMICS, K_SHAPE
 struct Prof
 {
   bool on = false;             /* every step runs eagerly with an event pair per kernel */
-  struct Pend { int kid; hipEvent_t a, b; double bytes; };
+  int step = -1;               /* step being enqueued (-1: launches outside the step loop) */
+  struct Pend { int kid; hipEvent_t a, b; double bytes; int step; };
   std::vector<hipEvent_t> pool;
   std::vector<Pend> pend;
   double ms[K_COUNT] = {0}, bytes[K_COUNT] = {0};
@@ -203,6 +206,8 @@ struct pmx_ctx
   int cur0 = 0;                /* row buffer holding the rows at step 0 of the call */
   hipGraphExec_t gexec = nullptr;
   bool graph_valid = false;
+  int *hring = nullptr;        /* pinned: per group of steps in flight, the control block and the flags */
+  hipEvent_t rev[PMX_RING] = {};   /* recorded after each group's copy into hring */
   int det_nw = 4;              /* waves per k_detect block (autotuned) */
   int rows_tpb = 1024;         /* threads per k_rows_idle block (spheres per busy list) */
   int tuned = 0;                /* autotuner passes done: at the first call, then once settled */
@@ -249,12 +254,14 @@ struct ProfScope
     if (!c.prof.on || !a) return;
     hipEvent_t b = c.prof.get ();
     (void) hipEventRecord (b, c.stream);
-    c.prof.pend.push_back ({kid, a, b, bytes});
+    c.prof.pend.push_back ({kid, a, b, bytes, c.prof.step});
   }
 };
 
-/* resolve pending event pairs (call after the stream is synchronised) */
-void prof_resolve (pmx_ctx &c);
+/* resolve pending event pairs (call after the stream is synchronised);
+ * launches of steps at or after `complete` (an aborted step and the steps
+ * enqueued behind it, whose kernels returned at once) are not counted */
+void prof_resolve (pmx_ctx &c, int complete = 0x7FFFFFFF);
 
 /* ---- upload (pmx_api.hip) ---- */
 int validate_model (const pmx_model *m);
