@@ -104,9 +104,12 @@ def measured_traffic(kernel, instances, spheres, precision):
     return b, src
 
 
-def roofline(kernels, instances, spheres, precision):
+def roofline(kernels, instances, spheres, precision, graph_timed):
     """The step kernel with the largest total time (kernels with algorithmic
-    bytes; the tree rebuild is reported in kernels_ms_per_launch)."""
+    bytes; the tree rebuild is reported in kernels_ms_per_launch).  Launch
+    durations come from HIP event pairs on the context's stream around every
+    launch of the timed steps: event record nodes of the replayed step graph
+    (graph_timed), else eager steps."""
     dom = max((k for k in kernels if kernels[k]["bytes"] > 0), key=lambda k: kernels[k]["ms"])
     d = kernels[dom]
     avg_s = d["ms"] / d["launches"] * 1e-3
@@ -116,7 +119,8 @@ def roofline(kernels, instances, spheres, precision):
     return {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": src,
             "avg_launch_us": avg_s * 1e6, "algorithmic_bytes_per_launch": per_launch,
-            "sampled_launches": d["launches"]}
+            "sampled_launches": d["launches"],
+            "timing": "event record nodes in the replayed step graph" if graph_timed else "eager steps with event pairs"}
 
 
 def sweep_forcings(pmx, workloads, insts, nsteps, dt, t0):
@@ -362,7 +366,7 @@ def main():
         "speedup_vs_plain_fp64": value / plain if plain else None,
         "hopper": hopper,
         "serving": serving,
-        "roofline": roofline(kernels, len(models), ntot, args.precision) if kernels else None,
+        "roofline": roofline(kernels, len(models), ntot, args.precision, st["graph_timed"]) if kernels else None,
         "kernels_ms_per_launch": {k: v["ms"] / v["launches"] for k, v in kernels.items()},
         "kernels_ms_per_step": {k: v["ms"] / K for k, v in kernels.items()},
         "cpu_baseline": cpu,

@@ -195,6 +195,8 @@ typedef struct pmx_stats
   int leaf_slots;          /* leaf-record layout: PMX_LSIZE slots per leaf, or 0 (packed) */
   long long partial_rebuilds;  /* rebuilds of the last call that re-sorted only the overflowing
                                   instances of a batch (counted in rebuilds too) */
+  int graph_timed;         /* profiling mode 1: the last call's kernel timings came from event
+                              record nodes of the replayed step graph (1) or from eager steps (0) */
 } pmx_stats;
 
 /* per-kernel timing collected with HIP events on the context's stream */
@@ -267,10 +269,13 @@ int pmx_contacts (pmx_ctx *ctx, pmx_pair *out, size_t cap, size_t *count);
 
 int pmx_get_stats (pmx_ctx *ctx, pmx_stats *stats);
 
-/* Enable (1) or disable (0) per-kernel event timing; any call clears the
- * totals.  While enabled every step runs eagerly (not as a graph replay) with
- * a HIP event pair around each kernel on the context's stream.  Timing stands
- * in for the reference's dem() wall clock (timer.h:36-49). */
+/* Per-kernel event timing: 0 off; 1 every 8th step replays a copy of the
+ * step graph with a HIP event pair around each kernel as event record nodes
+ * (fresh events per launch; the other steps replay the plain graph, since
+ * timing events slow the kernels around them); 2 every step enqueued
+ * eagerly with an event pair per kernel.  Any call clears the totals;
+ * launches of a step aborted by a leaf overflow are not counted.  Timing
+ * stands in for the reference's dem() wall clock (timer.h:36-49). */
 int pmx_set_profiling (pmx_ctx *ctx, int on);
 
 /* Per-kernel totals since profiling was enabled; count receives the number

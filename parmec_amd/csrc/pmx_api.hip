@@ -309,10 +309,13 @@ int upload_core (pmx_ctx *c, const pmx_model *m, const std::vector<InstOff> &off
   return PMX_OK;
 }
 
-void prof_resolve (pmx_ctx &c, int complete)
+void prof_resolve (pmx_ctx &c, int complete, int upto)
 {
-  for (auto &p : c.prof.pend)
+  size_t keep = 0;
+  for (size_t k = 0; k < c.prof.pend.size (); k++)
   {
+    Prof::Pend &p = c.prof.pend[k];
+    if (upto != 0x7FFFFFFF && (p.step < 0 || p.step >= upto)) { c.prof.pend[keep++] = p; continue; }
     float ms = 0.0f;
     if (p.step < complete && hipEventElapsedTime (&ms, p.a, p.b) == hipSuccess)
     {
@@ -323,7 +326,19 @@ void prof_resolve (pmx_ctx &c, int complete)
     c.prof.pool.push_back (p.a);
     c.prof.pool.push_back (p.b);
   }
-  c.prof.pend.clear ();
+  c.prof.pend.resize (keep);
+}
+
+/* the captured step graph and the events its record nodes were captured with */
+static void drop_graph (pmx_ctx *c)
+{
+  if (c->gexec) { (void) hipGraphExecDestroy (c->gexec); c->gexec = nullptr; }
+  if (c->graph) { (void) hipGraphDestroy (c->graph); c->graph = nullptr; }
+  if (c->pgexec) { (void) hipGraphExecDestroy (c->pgexec); c->pgexec = nullptr; }
+  if (c->pgraph) { (void) hipGraphDestroy (c->pgraph); c->pgraph = nullptr; }
+  for (auto &q : c->prof.cap) { if (q.a) (void) hipEventDestroy (q.a); if (q.b) (void) hipEventDestroy (q.b); }
+  c->prof.cap.clear ();
+  c->graph_valid = false;
 }
 
 extern "C" {
@@ -409,7 +424,7 @@ void pmx_destroy (pmx_ctx *c)
   c->rowkey[0].release (); c->rowkey[1].release (); c->hits.release (); c->iso.release ();
   c->tree.code.release (); c->tree.code2.release ();
   c->ctl.release ();
-  if (c->gexec) (void) hipGraphExecDestroy (c->gexec);
+  drop_graph (c);
   if (c->hring) (void) hipHostFree (c->hring);
   for (hipEvent_t e : c->rev) if (e) (void) hipEventDestroy (e);
   prof_resolve (*c);
@@ -481,19 +496,95 @@ static int check_flags (pmx_ctx *c, const int *fl)
   return PMX_OK;
 }
 
+/* event record nodes around the profiled scopes of the captured chain:
+ * A after the node before the scope and ahead of the scope's first node, B
+ * after the scope's last node and ahead of the node that follows it */
+static bool place_prof_nodes (pmx_ctx *c)
+{
+  auto next = [&] (hipGraphNode_t n, hipGraphNode_t *out) -> bool {
+    size_t k = 0;
+    if (hipGraphNodeGetDependentNodes (n, nullptr, &k) != hipSuccess || k > 1) return false;
+    *out = nullptr;
+    return k == 0 || hipGraphNodeGetDependentNodes (n, out, &k) == hipSuccess;
+  };
+  size_t nn = 0;
+  if (hipGraphGetRootNodes (c->pgraph, nullptr, &nn) != hipSuccess || nn != 1) return false;
+  hipGraphNode_t root = nullptr;
+  if (hipGraphGetRootNodes (c->pgraph, &root, &nn) != hipSuccess) return false;
+  /* first and following nodes of every scope, from the chain as captured */
+  std::vector<hipGraphNode_t> first (c->prof.cap.size ()), follow (c->prof.cap.size ());
+  for (size_t k = 0; k < c->prof.cap.size (); k++)
+  {
+    Prof::Cap &q = c->prof.cap[k];
+    if (q.before) { if (!next (q.before, &first[k]) || !first[k]) return false; }
+    else first[k] = root;
+    if (!next (q.after, &follow[k])) return false;
+  }
+  for (size_t k = 0; k < c->prof.cap.size (); k++)
+  {
+    Prof::Cap &q = c->prof.cap[k];
+    if (hipEventCreate (&q.a) != hipSuccess || hipEventCreate (&q.b) != hipSuccess) return false;
+    if (hipGraphAddEventRecordNode (&q.na, c->pgraph, q.before ? &q.before : nullptr, q.before ? 1 : 0, q.a) != hipSuccess ||
+        hipGraphAddDependencies (c->pgraph, &q.na, &first[k], 1) != hipSuccess ||
+        hipGraphAddEventRecordNode (&q.nb, c->pgraph, &q.after, 1, q.b) != hipSuccess ||
+        (follow[k] && hipGraphAddDependencies (c->pgraph, &q.nb, &follow[k], 1) != hipSuccess))
+      return false;
+  }
+  return true;
+}
+
+/* capture one step as a graph; with profiling mode 1, event record nodes
+ * are put around each profiled kernel so that fresh events can be set on
+ * them before every launch (if they cannot be placed, steps are enqueued
+ * eagerly instead) */
 static int capture (pmx_ctx *c, double h)
 {
-  if (c->gexec) { (void) hipGraphExecDestroy (c->gexec); c->gexec = nullptr; }
-  hipGraph_t g = nullptr;
-  PMX_HIP (hipStreamBeginCapture (c->stream, hipStreamCaptureModeThreadLocal));
-  int rc = enqueue_step (*c, h);
-  hipError_t e = hipStreamEndCapture (c->stream, &g);
-  if (rc) { if (g) (void) hipGraphDestroy (g); return rc; }
-  PMX_HIP (e);
-  e = hipGraphInstantiate (&c->gexec, g, nullptr, nullptr, 0);
-  (void) hipGraphDestroy (g);
-  PMX_HIP (e);
+  drop_graph (c);
+  for (int pass = 0; pass < (c->prof.mode == 1 ? 2 : 1); pass++)
+  {
+    hipGraph_t &g = pass ? c->pgraph : c->graph;
+    int mode = c->prof.mode;
+    if (pass == 0) c->prof.mode = 0;   /* the plain step: no scopes */
+    c->prof.capturing = pass == 1;
+    c->prof.err = hipSuccess;
+    PMX_HIP (hipStreamBeginCapture (c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_step (*c, h);
+    hipError_t e = hipStreamEndCapture (c->stream, &g);
+    c->prof.capturing = false;
+    c->prof.mode = mode;
+    if (rc) { drop_graph (c); return rc; }
+    PMX_HIP (e);
+  }
+  if (c->prof.mode == 1)
+  {
+    c->prof.eager = c->prof.err != hipSuccess || !place_prof_nodes (c);
+    (void) hipGetLastError ();
+    if (c->prof.eager) { drop_graph (c); return PMX_OK; }
+    PMX_HIP (hipGraphInstantiate (&c->pgexec, c->pgraph, nullptr, nullptr, 0));
+  }
+  PMX_HIP (hipGraphInstantiate (&c->gexec, c->graph, nullptr, nullptr, 0));
   c->graph_valid = true;
+  return PMX_OK;
+}
+
+/* one replay of the captured step; with profiling mode 1 every
+ * PMX_PROF_EVERY-th step replays the copy with event record nodes, which
+ * get fresh events first (a launch keeps the events it was given) */
+static int launch_step (pmx_ctx *c, int step)
+{
+  if (!c->pgexec || step % PMX_PROF_EVERY)
+  {
+    PMX_HIP (hipGraphLaunch (c->gexec, c->stream));
+    return PMX_OK;
+  }
+  for (auto &q : c->prof.cap)
+  {
+    hipEvent_t a = c->prof.get (), b = c->prof.get ();
+    PMX_HIP (hipGraphExecEventRecordNodeSetEvent (c->pgexec, q.na, a));
+    PMX_HIP (hipGraphExecEventRecordNodeSetEvent (c->pgexec, q.nb, b));
+    c->prof.pend.push_back ({q.kid, a, b, q.bytes, step});
+  }
+  PMX_HIP (hipGraphLaunch (c->pgexec, c->stream));
   return PMX_OK;
 }
 
@@ -708,9 +799,11 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
   {
     int next = nsteps;
     for (int i = 0; i < I; i++) if (!reported[i] && nst[i] > done) next = std::min (next, nst[i]);
-    if (!c->prof.on && !c->graph_valid && (rc = capture (c, step0))) return rc;
+    if (c->prof.mode != 2 && !c->prof.eager && !c->graph_valid && (rc = capture (c, step0))) return rc;
+    bool eager = c->prof.mode == 2 || (c->prof.mode == 1 && c->prof.eager);
     int ctl[CTL_COUNT] = {0}, fl[FLAG_COUNT] = {0};
     int enq = done, gq = 0, gs = 0;   /* first step not enqueued; groups enqueued, seen */
+    int gend[PMX_RING] = {0};         /* first step after each group in flight */
     int err = PMX_OK;
     bool abort = false;
     for (;;)
@@ -719,21 +812,24 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
       {
         int n = std::min (gsize, next - enq);
         for (int q = 0; q < n; q++)
-          if (c->prof.on)
+        {
+          if (eager)
           {
             c->prof.step = enq + q;
             rc = enqueue_step (*c, step0);
             c->prof.step = -1;
-            if (rc) return rc;
           }
           else
-            PMX_HIP (hipGraphLaunch (c->gexec, c->stream));
+            rc = launch_step (c, enq + q);
+          if (rc) return rc;
+        }
         int *slot = c->hring + (gq % PMX_RING) * slotn;
         PMX_HIP (hipMemcpyAsync (slot, c->ctl.p, sizeof (int) * CTL_COUNT, hipMemcpyDeviceToHost, c->stream));
         PMX_HIP (hipMemcpyAsync (slot + CTL_COUNT, c->dflags.p, sizeof (int) * FLAG_COUNT, hipMemcpyDeviceToHost,
                                  c->stream));
         PMX_HIP (hipEventRecord (c->rev[gq % PMX_RING], c->stream));
         enq += n;
+        gend[gq % PMX_RING] = enq;
         gq++;
       }
       if (gs == gq) break;
@@ -741,6 +837,7 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
       const int *slot = c->hring + (gs % PMX_RING) * slotn;
       std::memcpy (ctl, slot, sizeof (ctl));
       std::memcpy (fl, slot + CTL_COUNT, sizeof (fl));
+      if (!ctl[CTL_ABORT]) prof_resolve (*c, 0x7FFFFFFF, gend[gs % PMX_RING]);   /* the group's timings */
       gs++;
       if ((err = check_flags (c, fl))) break;
       if ((abort = ctl[CTL_ABORT] != 0)) break;
@@ -773,6 +870,7 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
   c->stats.steps = nsteps;
   c->stats.rebuilds = rebuilds;
   c->stats.partial_rebuilds = partial;
+  c->stats.graph_timed = c->prof.mode == 1 && !c->prof.eager;
   c->stats.leaves = c->tree.nleaves;
   c->stats.leaf_slots = c->lstride;
   c->stats.seconds = std::chrono::duration<double> (std::chrono::steady_clock::now () - tl).count ();
@@ -826,9 +924,12 @@ int pmx_set_profiling (pmx_ctx *c, int on)
 {
   if (!c) return fail (PMX_EINVAL, "null ctx");
   (void) hipSetDevice (c->device);
+  if (on < 0 || on > 2) return fail (PMX_EINVAL, "profiling mode must be 0, 1 or 2");
   PMX_HIP (hipStreamSynchronize (c->stream));
   prof_resolve (*c);
-  c->prof.on = on != 0;
+  if (on != c->prof.mode) drop_graph (c);   /* the graph holds event record nodes in mode 1 only */
+  c->prof.mode = on;
+  c->prof.eager = false;
   for (int k = 0; k < K_COUNT; k++) { c->prof.ms[k] = 0.0; c->prof.bytes[k] = 0.0; c->prof.n[k] = 0; }
   return PMX_OK;
 }

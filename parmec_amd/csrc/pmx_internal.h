@@ -17,6 +17,7 @@
 #define PMX_TPB 256            /* threads per block of the elementwise kernels */
 #define PMX_RETUNE_STEPS 256
 #define PMX_LOOKAHEAD 2      /* groups of steps enqueued ahead of the one the host waits for */
+#define PMX_PROF_EVERY 8     /* profiling mode 1 times every 8th step (timing events slow the kernels) */
 #define PMX_RING 4           /* slots of the pinned flag ring (> PMX_LOOKAHEAD) */   /* steps of a context before the second autotuner pass */
 #define PMX_TRI_FLAG 0x80000000u  /* row key flag of a triangle partner */
 #define PMX_NO_ROOT INT_MIN       /* tree root of an instance without spheres */
@@ -109,11 +110,21 @@ enum { K_BIN = 0, K_SCATTER, K_DETECT, K_FORCE_INIT, K_ROWS, K_DYNAMICS, K_SHAPE
  * and resolved at the per-step synchronisation */
 struct Prof
 {
-  bool on = false;             /* every step runs eagerly with an event pair per kernel */
+  int mode = 0;                /* 0 off; 1 every PMX_PROF_EVERY-th step replays a copy of the step graph
+                                  with an event pair per kernel as record nodes (fresh events swapped in
+                                  before each launch), the others the plain graph; 2 every step enqueued
+                                  eagerly with an event pair per kernel */
+  bool capturing = false;      /* a step is being captured: pairs become event record nodes */
+  hipError_t err = hipSuccess; /* a capture-time query failed */
   int step = -1;               /* step being enqueued (-1: launches outside the step loop) */
   struct Pend { int kid; hipEvent_t a, b; double bytes; int step; };
+  /* a profiled kernel scope of the captured step: the chain's tail node
+   * before and after it, and the event record nodes put around it */
+  struct Cap { int kid; double bytes; hipGraphNode_t before, after, na, nb; hipEvent_t a, b; };
+  bool eager = false;          /* mode 1 could not place its nodes: steps run as in mode 2 */
   std::vector<hipEvent_t> pool;
   std::vector<Pend> pend;
+  std::vector<Cap> cap;
   double ms[K_COUNT] = {0}, bytes[K_COUNT] = {0};
   long long n[K_COUNT] = {0};
   hipEvent_t get ()
@@ -204,7 +215,11 @@ struct pmx_ctx
   dvec<int> dflags;            /* [FLAG_COUNT] */
   dvec<int> ctl;               /* [CTL_COUNT] */
   int cur0 = 0;                /* row buffer holding the rows at step 0 of the call */
+  hipGraph_t graph = nullptr;  /* the captured step */
   hipGraphExec_t gexec = nullptr;
+  hipGraph_t pgraph = nullptr; /* profiling mode 1: the step with event record nodes (kept: they name
+                                  the nodes of pgexec), replayed every PMX_PROF_EVERY steps */
+  hipGraphExec_t pgexec = nullptr;
   bool graph_valid = false;
   int *hring = nullptr;        /* pinned: per group of steps in flight, the control block and the flags */
   hipEvent_t rev[PMX_RING] = {};   /* recorded after each group's copy into hring */
@@ -245,23 +260,44 @@ struct ProfScope
   int kid;
   double bytes;
   hipEvent_t a = nullptr;
+  hipGraphNode_t before = nullptr;
   ProfScope (pmx_ctx &c_, int kid_, double bytes_) : c (c_), kid (kid_), bytes (bytes_)
   {
-    if (c.prof.on) { a = c.prof.get (); (void) hipEventRecord (a, c.stream); }
+    if (!c.prof.mode) return;
+    if (c.prof.capturing) before = tail ();
+    else { a = c.prof.get (); (void) hipEventRecord (a, c.stream); }
   }
   ~ProfScope ()
   {
-    if (!c.prof.on || !a) return;
+    if (!c.prof.mode) return;
+    if (c.prof.capturing)
+    {
+      hipGraphNode_t after = tail ();
+      if (after && after != before) c.prof.cap.push_back ({kid, bytes, before, after, nullptr, nullptr, nullptr, nullptr});
+      return;
+    }
+    if (!a) return;
     hipEvent_t b = c.prof.get ();
     (void) hipEventRecord (b, c.stream);
     c.prof.pend.push_back ({kid, a, b, bytes, c.prof.step});
   }
+  /* the last node captured on the stream (a single-stream capture is a chain) */
+  hipGraphNode_t tail ()
+  {
+    hipStreamCaptureStatus st;
+    const hipGraphNode_t *deps = nullptr;
+    size_t nd = 0;
+    if (hipStreamGetCaptureInfo_v2 (c.stream, &st, nullptr, nullptr, &deps, &nd) != hipSuccess || nd > 1)
+    { c.prof.err = hipErrorUnknown; return nullptr; }
+    return nd ? deps[0] : nullptr;
+  }
 };
 
-/* resolve pending event pairs (call after the stream is synchronised);
- * launches of steps at or after `complete` (an aborted step and the steps
- * enqueued behind it, whose kernels returned at once) are not counted */
-void prof_resolve (pmx_ctx &c, int complete = 0x7FFFFFFF);
+/* resolve the pending event pairs of steps before `upto` (all pairs when
+ * upto is INT_MAX; their events must have completed); launches of steps at
+ * or after `complete` (an aborted step and the steps enqueued behind it,
+ * whose kernels returned at once) are not counted */
+void prof_resolve (pmx_ctx &c, int complete = 0x7FFFFFFF, int upto = 0x7FFFFFFF);
 
 /* ---- upload (pmx_api.hip) ---- */
 int validate_model (const pmx_model *m);

@@ -1525,9 +1525,9 @@ int autotune (pmx_ctx &c, double h)
 {
   struct NoProf   /* the timings are not profiled steps */
   {
-    Prof &p; bool was;
-    NoProf (Prof &p_) : p (p_), was (p_.on) { p.on = false; }
-    ~NoProf () { p.on = was; }
+    Prof &p; int was;
+    NoProf (Prof &p_) : p (p_), was (p_.mode) { p.mode = 0; }
+    ~NoProf () { p.mode = was; }
   } np (c.prof);
   int rc;
   if ((rc = step_bin (c)) || (rc = step_mtri (c)) || (rc = step_scatter (c))) return rc;

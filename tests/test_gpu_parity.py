@@ -102,19 +102,33 @@ def test_hopper_2m_short(engines):
     compare(mg, cg, mc, cc)
 
 
-def test_eager_profiled_path_matches_graph_path(engines):
-    """With profiling on every step runs eagerly (events per launch); without,
-    one captured step is replayed as a graph.  Both must give the oracle's
-    state."""
+@pytest.mark.parametrize("mode", [1, 2])
+def test_profiled_paths_match_oracle(engines, mode):
+    """Profiling mode 1 replays, every 8th step, a copy of the captured step
+    graph with event record nodes around each kernel (fresh events per
+    launch), the plain graph otherwise; mode 2 enqueues every step eagerly
+    with an event pair per launch.  Both must give the oracle's state and
+    time the detection launches of the steps they sample."""
+    import ctypes as C
     from oracle import orc
     m = workloads.box_lattice(n=4000, seed=3, spacing_factor=2.05, jitter=0.02)
     rng = np.random.default_rng(7)
     m.linear[:] = rng.uniform(-1.0, 1.0, size=m.linear.shape)
     f = pmx.forcing(gravity=(0.0, 0.0, -9.81))
     g = pmx.Engine()
-    g.lib.pmx_set_profiling(g.h, 1)
+    assert g.lib.pmx_set_profiling(g.h, mode) == 0
     mg, cg, sg = run(g, m, 150 * 1e-5 - 0.5e-5, 1e-5, f, calls=2)
+    arr = (pmx.pmx_kernel_time * 32)()
+    cnt = C.c_int(0)
+    g.lib.pmx_get_profile.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int)]
+    assert g.lib.pmx_get_profile(g.h, C.cast(arr, C.c_void_p), 32, C.byref(cnt)) == 0
+    prof = {arr[k].name.decode(): arr[k] for k in range(cnt.value)}
     g.close()
+    # both calls' sampled steps, less any step aborted by a leaf overflow (not counted)
+    sampled = 2 * 150 if mode == 2 else 2 * len(range(0, 150, 8))
+    assert sampled - 2 * sg["rebuilds"] <= prof["detect"].launches <= sampled
+    assert prof["detect"].ms > 0.0
+    assert sg["graph_timed"] == (1 if mode == 1 else 0)
     c = orc.Oracle(0)
     mc, cc, sc = run(c, m, 150 * 1e-5 - 0.5e-5, 1e-5, f, calls=2)
     c.close()
