@@ -182,7 +182,7 @@ def serve_leg(pmx, workloads, count, rate, seed):
     out = {"requests": count, "rate_per_s": rate, "seed": seed,
            "spheres": int(sum(r[1].ellnum for r in stream)), "bodies": int(sum(r[1].parnum for r in stream)),
            "particle_steps": float(sum(r[1].parnum * int(round(r[2] / r[3] + 0.5)) for r in stream))}
-    forms = (("batched_mixed", pmx.PMX_MIXED, dict(gather_s=0.05)),
+    forms = (("batched_mixed", pmx.PMX_MIXED, dict(gather_s=0.2)),
              ("plain_fp64", pmx.PMX_FP64, dict(max_batch=1, max_active=1)))
     for name, prec, kw in forms:
         srv = pmx.Server(precision=prec, **kw)
