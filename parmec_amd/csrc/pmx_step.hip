@@ -213,8 +213,12 @@ __device__ __forceinline__ bool overlap (double ax, double ay, double az, double
  * tests over its 64 lanes.  Waves never wait for each other inside the
  * candidate loop.  Hit slots per sphere come from LDS counters; the hit
  * order is irrelevant (rows sort their keys). */
+/* 5 waves per SIMD: the register budget (96 VGPRs, a 16-byte spill) and
+ * the LDS per one-wave block (7.3 KB) keep 20 blocks resident per CU, up
+ * from 16 at the unbounded register count; detection is bound by the
+ * latency of its dependent loads, so residency pays */
 template <int NW>
-__global__ __launch_bounds__(NW*64)
+__global__ __launch_bounds__(NW*64, 5)
 void k_detect (int E, const double *center, const double *radii, const int *part, const double *cell, int nleaves,
                const int *lcount, const int *loff, const int *lmem, const double *lpos, const int *lpart,
                const double *tri, int T, const int *tricnt, const int *trioff, const int *trilist,
@@ -227,12 +231,14 @@ void k_detect (int E, const double *center, const double *radii, const int *part
   if (blockIdx.x == 0 && threadIdx.x == 0) ctl[CTL_HINT] = 1;   /* this step's bin completed: leafof is valid */
   int Q = xcd_block (blockIdx.x, gridDim.x);
   if (frozen (stop, leafinst[Q], ctl)) return;   /* uniform over the block */
+  /* LDS holds what the loops read on every pair; the block spheres'
+   * members and particles are read from the leaf's slots (cache-resident)
+   * on the rare paths that need them, so that more blocks stay resident */
   __shared__ double ssph[4*PMX_LSIZE];   /* block spheres: x, y, z, rx */
-  __shared__ int sidx[PMX_LSIZE], spart[PMX_LSIZE], shc[PMX_LSIZE];
+  __shared__ int shc[PMX_LSIZE];
   __shared__ float4 ssph32[PMX_LSIZE];       /* block spheres: FP32 offsets from O, radius */
   __shared__ float4 wrec32[NW][PMX_LSIZE];   /* per wave: the candidate's kept records, the same */
   __shared__ int wmem[NW][PMX_LSIZE], wlist[NW][PMX_LSIZE];
-  __shared__ unsigned long long wmask[NW][128];   /* per wave: reach masks of 64 candidates */
   int t = threadIdx.x, lane = t & 63, w = t >> 6;
   int qoff = lstride ? Q * lstride : loff[Q], qcnt = lcount[Q];
   int nc = nbrcnt[Q];
@@ -272,8 +278,6 @@ void k_detect (int E, const double *center, const double *radii, const int *part
     const double2 *rp = (const double2*) (lpos + 4*(size_t) (qoff + s));
     double2 a = rp[0], b = rp[1];
     ssph[4*s] = a.x; ssph[4*s+1] = a.y; ssph[4*s+2] = b.x; ssph[4*s+3] = b.y;
-    sidx[s] = lmem[qoff + s];
-    spart[s] = lpart[qoff + s];
     shc[s] = 0;
   }
   double cl[3] = {cell[Q], cell[(size_t) nleaves + Q], cell[2*(size_t) nleaves + Q]};
@@ -338,16 +342,16 @@ void k_detect (int E, const double *center, const double *radii, const int *part
      * own AABB reaches each one's cell (condet.ispc:104-113) */
     int np = ncw - p0 < 64 ? ncw - p0 : 64;
     if (p0 > 0) batch (p0);
-    /* the (candidate, block sphere) pairs spread over the lanes, the
-     * candidate's cell fetched from its lane; reach bits gathered in LDS */
-    unsigned long long *mk = wmask[w];
-    mk[2*lane] = 0ull;
-    mk[2*lane+1] = 0ull;
-    __builtin_amdgcn_wave_barrier ();
+    /* the (candidate, block sphere) pairs spread over the lanes, candidate
+     * by candidate (pair q: candidate q / qcnt, sphere q mod qcnt), the
+     * candidate's cell fetched from its lane; the ballot of iteration i is
+     * kept by lane i mod 64 (bal0 for i < 64, bal1 above), so candidate c's
+     * reach bits are bits [c qcnt, c qcnt + qcnt) of the ballots in order */
     int work = np * qcnt;
     /* cand = q / qcnt exactly: q * qcnt < 64 * 96 * 96 < 2^20 */
     unsigned qinv = qcnt > 0 ? ((1u << 20) + (unsigned) qcnt - 1u) / (unsigned) qcnt : 0u;
-    for (int base = 0; base < work; base += 64)
+    unsigned long long bal0 = 0ull, bal1 = 0ull;
+    for (int base = 0, it = 0; base < work; base += 64, it++)
     {
       int q = base + lane;
       bool ok = q < work;
@@ -356,16 +360,32 @@ void k_detect (int E, const double *center, const double *radii, const int *part
       double lo[3], hi[3];
 #pragma unroll
       for (int d = 0; d < 3; d++) { lo[d] = __shfl (Ll[d], cand, 64); hi[d] = __shfl (Lu[d], cand, 64); }
+      bool in = false;
       if (ok)
       {
         double x = ssph[4*sI], y = ssph[4*sI+1], z = ssph[4*sI+2], r = ssph[4*sI+3];
-        bool in = x-r <= hi[0] && y-r <= hi[1] && z-r <= hi[2] && x+r >= lo[0] && y+r >= lo[1] && z+r >= lo[2];
-        if (in) atomicOr (&mk[2*cand + (sI >> 6)], 1ull << (sI & 63));
+        in = x-r <= hi[0] && y-r <= hi[1] && z-r <= hi[2] && x+r >= lo[0] && y+r >= lo[1] && z+r >= lo[2];
       }
+      unsigned long long bm = __ballot (in);
+      if (lane == (it & 63)) { if (it < 64) bal0 = bm; else bal1 = bm; }
     }
-    __builtin_amdgcn_wave_barrier ();
     unsigned long long bm0 = 0, bm1 = 0;
-    if (lane < np && bcnt > 0) { bm0 = mk[2*lane]; bm1 = mk[2*lane+1]; }
+    {
+      int st = lane * qcnt, w0 = st >> 6, off = st & 63;   /* lane < np: its candidate's first bit */
+      unsigned long long a[3];
+#pragma unroll
+      for (int k = 0; k < 3; k++)
+      {
+        int wi = w0 + k;
+        unsigned long long v0 = __shfl (bal0, wi & 63, 64), v1 = __shfl (bal1, wi & 63, 64);
+        a[k] = wi < 64 ? v0 : (wi < 128 ? v1 : 0ull);
+      }
+      unsigned long long m0 = off ? (a[0] >> off) | (a[1] << (64 - off)) : a[0];
+      unsigned long long m1 = off ? (a[1] >> off) | (a[2] << (64 - off)) : a[1];
+      if (qcnt < 64) { m0 &= (1ull << qcnt) - 1ull; m1 = 0ull; }
+      else m1 &= qcnt == 64 ? 0ull : (1ull << (qcnt - 64)) - 1ull;
+      if (lane < np && bcnt > 0) { bm0 = m0; bm1 = m1; }
+    }
     /* the candidates that some block sphere reaches, in turn */
     unsigned long long todo = __ballot ((bm0 | bm1) != 0);
     while (todo)
@@ -437,8 +457,8 @@ void k_detect (int E, const double *center, const double *radii, const int *part
           if (!overlap (ssph[4*a], ssph[4*a+1], ssph[4*a+2], ssph[4*a+3],
                         ssph[4*bb], ssph[4*bb+1], ssph[4*bb+2], ssph[4*bb+3]))
             continue;
-          if (spart[a] == spart[bb]) continue;
-          int ea = sidx[a], eb = sidx[bb];
+          if (lpart[qoff + a] == lpart[qoff + bb]) continue;
+          int ea = lmem[qoff + a], eb = lmem[qoff + bb];
           int sa = atomicAdd (&shc[a], 1), sb = atomicAdd (&shc[bb], 1);
           if (sa < hcap) hits[(size_t) sa * E + ea] = (uint32_t) eb; else atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1);
           if (sb < hcap) hits[(size_t) sb * E + eb] = (uint32_t) ea; else atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1);
@@ -454,12 +474,12 @@ void k_detect (int E, const double *center, const double *radii, const int *part
           int a = (int) (((unsigned) q * inv) >> 20), m = q - a * cnt;
           int ti = lst[a];
           if (!near32 (ssph32[ti], rec32[m], k1, k2)) continue;
-          /* the record's FP64 values (the ones k_scatter copied into it) */
+          /* the partner's FP64 values (the ones its record holds) */
           int j = mem[m];
           double cx = center[j], cy = center[(size_t) E + j], cz = center[2*(size_t) E + j], rj = radii[j];
           if (!overlap (ssph[4*ti], ssph[4*ti+1], ssph[4*ti+2], ssph[4*ti+3], cx, cy, cz, rj)) continue;
-          if (spart[ti] == part[j]) continue;
-          int e = sidx[ti];
+          if (lpart[qoff + ti] == part[j]) continue;
+          int e = lmem[qoff + ti];
           int slot = atomicAdd (&shc[ti], 1);
           if (slot < hcap) hits[(size_t) slot * E + e] = (uint32_t) j;
           else atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1);
@@ -490,7 +510,7 @@ void k_detect (int E, const double *center, const double *radii, const int *part
       double v[9];
 #pragma unroll
       for (int k = 0; k < 9; k++) v[k] = tri[(size_t) k*T + tt];
-      double px = ssph[4*ti], py = ssph[4*ti+1], pz = ssph[4*ti+2], rr = radii[sidx[ti]];
+      double px = ssph[4*ti], py = ssph[4*ti+1], pz = ssph[4*ti+2], rr = ssph[4*ti+3];   /* the record's r: radii[e] */
       /* the returned point lies on the triangle, so its distance is at least
        * the distance to the triangle's box; skip when that exceeds r by a
        * margin far above the rounding of these few operations */
@@ -503,13 +523,13 @@ void k_detect (int E, const double *center, const double *radii, const int *part
       if (depth > 0.0)
       {
         int slot = atomicAdd (&shc[ti], 1);
-        if (slot < hcap) hits[(size_t) slot * E + sidx[ti]] = PMX_TRI_FLAG | (uint32_t) tt;
+        if (slot < hcap) hits[(size_t) slot * E + lmem[qoff + ti]] = PMX_TRI_FLAG | (uint32_t) tt;
         else atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1);
       }
     }
   }
   __syncthreads ();
-  for (int s = t; s < qcnt; s += NW*64) hitcnt[sidx[s]] = shc[s] < hcap ? shc[s] : hcap;
+  for (int s = t; s < qcnt; s += NW*64) hitcnt[lmem[qoff + s]] = shc[s] < hcap ? shc[s] : hcap;
 }
 
 __global__ void k_force_init (const int *ctl, int N, const int *list, int nlist, const double *mass, const double *gtab,
