@@ -93,6 +93,10 @@ struct TreeDev
   int ntrilist = 0;
   dvec<int> nbrcnt, nbroff, nbrlist;  /* leaf -> leaves overlapping its cell inflated by rmax */
   int nbrstride = 0;                  /* list Q at Q * nbrstride, or CSR by nbroff when 0 */
+  dvec<int> nbrcnt2, nbrlist2;        /* the previous build's lists while a partial build shifts them */
+  dvec<int> ibase, ibase2;            /* [I+1] first leaf of each instance: this build, the previous one */
+  dvec<int> dmax;                     /* device maximum of the list lengths */
+  bool nbr_ready = false;             /* nbrcnt/nbrlist hold the lists of the current leaves */
   int nnbr = 0;
 };
 

@@ -652,13 +652,10 @@ __global__ void k_mtri_reach (int nm, const int *mtri, int T, const double *tri,
 /* leaves whose closed cell overlaps leaf Q's cell inflated by rmax: every leaf
  * a sphere binned in Q can reach (its centre lies in Q's cell and its AABB
  * half-width is at most rmax; rounding is monotone) */
-__global__ void k_leaf_nbr (int nleaves, const double *cell, double rmax, const int *roots, const int *leafinst,
-                            const int *left, const int *right, const int *dim, const double *coord, int *cnt,
-                            const int *off, int *list, int stride, int *flag)
+__device__ int leaf_nbr_one (int Q, int nleaves, const double *cell, double rmax, const int *roots,
+                             const int *leafinst, const int *left, const int *right, const int *dim,
+                             const double *coord, const int *off, int *list, int stride, int *flag)
 {
-  int Q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (Q >= nleaves) return;
-  int root = roots[leafinst[Q]];
   double lo[3], hi[3];
   for (int k = 0; k < 3; k++)
   {
@@ -666,7 +663,7 @@ __global__ void k_leaf_nbr (int nleaves, const double *cell, double rmax, const 
     hi[k] = cell[(size_t) (3+k)*nleaves + Q] + rmax;
   }
   int stack[64];
-  int sp = 0, ref = root, n = 0;
+  int sp = 0, ref = roots[leafinst[Q]], n = 0;
   for (;;)
   {
     if (ref >= 0)
@@ -691,8 +688,58 @@ __global__ void k_leaf_nbr (int nleaves, const double *cell, double rmax, const 
     if (sp == 0) break;
     ref = stack[--sp];
   }
-  if (!list) cnt[Q] = n;
+  return n;
 }
+
+/* leaves whose cell meets leaf Q's cell inflated by rmax: counted (list
+ * null; the longest count folded into cmax) or listed; only the leaves of
+ * instances with a nonzero iflag when iflag is not null */
+__global__ void k_leaf_nbr (int nleaves, const double *cell, double rmax, const int *roots, const int *leafinst,
+                            const int *left, const int *right, const int *dim, const double *coord, int *cnt,
+                            const int *off, int *list, int stride, int *flag, const int *iflag, int *cmax)
+{
+  int Q = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = 0;
+  if (Q < nleaves && !(iflag && !iflag[leafinst[Q]]))
+    n = leaf_nbr_one (Q, nleaves, cell, rmax, roots, leafinst, left, right, dim, coord, off, list, stride, flag);
+  if (Q < nleaves && !list && !(iflag && !iflag[leafinst[Q]])) cnt[Q] = n;
+  if (!list && cmax)
+  {
+    for (int o = 32; o >= 1; o >>= 1) n = max (n, __shfl_xor (n, o, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax (cmax, n);
+  }
+}
+
+/* the previous build's lists of the instances a partial build left
+ * unchanged: leaf Q' of instance i was leaf Q' - (nbase[i] - obase[i]),
+ * and so was each leaf of its list (same stride); the flagged instances'
+ * leaves are listed afresh */
+__global__ void k_nbr_shift (int nleaves, const int *leafinst, const int *iflag, const int *obase, const int *nbase,
+                             int stride, const int *cnt0, const int *list0, int *cnt, int *list)
+{
+  int Q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (Q >= nleaves) return;
+  int i = leafinst[Q];
+  if (iflag[i]) return;
+  int delta = nbase[i] - obase[i];
+  int Qo = Q - delta;
+  int c = cnt0[Qo];
+  cnt[Q] = c;
+  const int *src = list0 + (size_t) Qo * stride;
+  int *dst = list + (size_t) Q * stride;
+  for (int k = 0; k < c; k++) dst[k] = src[k] + delta;
+}
+
+/* first leaf of each instance (base[I] = nleaves): leaves are numbered in
+ * node order and an instance's nodes are its sphere range */
+__global__ void k_inst_base (int I, int n, const int *off, const int *lfirst, int nleaves, int *base)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > I) return;
+  int o = i < I ? off[i] : n;
+  base[i] = o < n ? lfirst[o] : nleaves;
+}
+
 
 __global__ void k_fill_int (int *a, int n, int v)
 {
@@ -882,9 +929,15 @@ int tree_build (pmx_ctx &c, bool all)
 
   /* leaf numbering and reach cells */
   int nleaves = 0;
+  if (t.ibase.alloc (I + 1) || t.ibase2.alloc (I + 1)) { pmx_set_error ("out of device memory (tree)"); return PMX_ENOMEM; }
+  if (sel)   /* each instance's first leaf before this build (its lists are shifted by the change) */
+    hipLaunchKernelGGL (k_inst_base, grid (I + 1), dim3 (PMX_TPB), 0, c.stream, I, n, (const int*) c.ell_off.p,
+                        (const int*) t.lfirst.p, t.nleaves, t.ibase2.p);
   PMX_HIP (hipMemcpyAsync (t.lfirst.p, t.nleafkids.p, sizeof (int) * n, hipMemcpyDeviceToDevice, c.stream));
   int rc = scan_exclusive (c, t.lfirst.p, n, &nleaves);
   if (rc) return rc;
+  hipLaunchKernelGGL (k_inst_base, grid (I + 1), dim3 (PMX_TPB), 0, c.stream, I, n, (const int*) c.ell_off.p,
+                      (const int*) t.lfirst.p, nleaves, t.ibase.p);
   t.nleaves = nleaves;
   if (t.leafpar.alloc (nleaves) || t.leafinst.alloc (nleaves) || t.cell.alloc (6*(size_t) nleaves))
   { pmx_set_error ("out of device memory (tree)"); return PMX_ENOMEM; }
@@ -904,26 +957,64 @@ int tree_nbrlists (pmx_ctx &c)
 {
   TreeDev &t = c.tree;
   int L = std::max (t.nleaves, 1);
-  if (t.nbrcnt.alloc (L) || t.nbroff.alloc (L + 1)) { pmx_set_error ("out of device memory (neighbour lists)"); return PMX_ENOMEM; }
+  bool shift = t.last_partial && t.nbr_ready && t.nbrstride > 0 && !c.nbr_csr;
+  t.nbr_ready = false;
+  if (shift)
+  {
+    std::swap (t.nbrcnt.p, t.nbrcnt2.p); std::swap (t.nbrcnt.n, t.nbrcnt2.n);
+    std::swap (t.nbrlist.p, t.nbrlist2.p); std::swap (t.nbrlist.n, t.nbrlist2.n);
+  }
+  if (t.nbrcnt.alloc (L) || t.nbroff.alloc (L + 1) || t.dmax.alloc (1))
+  { pmx_set_error ("out of device memory (neighbour lists)"); return PMX_ENOMEM; }
   t.nnbr = 0;
   if (c.E == 0 || t.nleaves == 0) return PMX_OK;
   int *flag = c.dflags.p + FLAG_TRILIST_OVERFLOW;
-  hipLaunchKernelGGL (k_leaf_nbr, grid (t.nleaves), dim3 (PMX_TPB), 0, c.stream, t.nleaves, (const double*) t.cell.p,
-                      c.rmax, (const int*) t.roots.p, (const int*) t.leafinst.p, (const int*) t.left.p,
-                      (const int*) t.right.p, (const int*) t.dim.p, (const double*) t.coord.p, t.nbrcnt.p,
-                      (const int*) nullptr, (int*) nullptr, 0, flag);
-  PMX_HIP (hipGetLastError ());
-  /* leaf Q's list at Q * stride (stride = the longest list), so that a
-   * detection block finds its list without first reading an offset; CSR
-   * when that layout would take more than 2 GiB (or when asked for: the
+  auto count = [&] (const int *iflag, int *mx) -> int {
+    hipLaunchKernelGGL (k_fill_int, dim3 (1), dim3 (64), 0, c.stream, t.dmax.p, 1, 0);
+    hipLaunchKernelGGL (k_leaf_nbr, grid (t.nleaves), dim3 (PMX_TPB), 0, c.stream, t.nleaves, (const double*) t.cell.p,
+                        c.rmax, (const int*) t.roots.p, (const int*) t.leafinst.p, (const int*) t.left.p,
+                        (const int*) t.right.p, (const int*) t.dim.p, (const double*) t.coord.p, t.nbrcnt.p,
+                        (const int*) nullptr, (int*) nullptr, 0, flag, iflag, t.dmax.p);
+    PMX_HIP (hipGetLastError ());
+    PMX_HIP (hipMemcpyAsync (mx, t.dmax.p, sizeof (int), hipMemcpyDeviceToHost, c.stream));
+    PMX_HIP (hipStreamSynchronize (c.stream));
+    return PMX_OK;
+  };
+  auto fill = [&] (const int *iflag) -> int {
+    hipLaunchKernelGGL (k_leaf_nbr, grid (t.nleaves), dim3 (PMX_TPB), 0, c.stream, t.nleaves, (const double*) t.cell.p,
+                        c.rmax, (const int*) t.roots.p, (const int*) t.leafinst.p, (const int*) t.left.p,
+                        (const int*) t.right.p, (const int*) t.dim.p, (const double*) t.coord.p, t.nbrcnt.p,
+                        (const int*) t.nbroff.p, t.nbrlist.p, t.nbrstride, flag, iflag, (int*) nullptr);
+    PMX_HIP (hipGetLastError ());
+    return PMX_OK;
+  };
+  int rc, mx = 0;
+  if (shift)
+  {
+    /* a partial build: the unchanged instances' lists move with their
+     * leaves; the flagged instances' leaves are listed afresh, in place
+     * while their longest list fits the stride */
+    if ((rc = count ((const int*) c.ovf.p, &mx))) return rc;
+    if (mx <= t.nbrstride && (size_t) t.nleaves * t.nbrstride * sizeof (int) <= ((size_t) 2 << 30))
+    {
+      t.nnbr = t.nleaves * t.nbrstride;
+      if (t.nbrlist.alloc (std::max (t.nnbr, 1))) { pmx_set_error ("out of device memory (neighbour lists)"); return PMX_ENOMEM; }
+      hipLaunchKernelGGL (k_nbr_shift, grid (t.nleaves), dim3 (PMX_TPB), 0, c.stream, t.nleaves, (const int*) t.leafinst.p,
+                          (const int*) c.ovf.p, (const int*) t.ibase2.p, (const int*) t.ibase.p, t.nbrstride,
+                          (const int*) t.nbrcnt2.p, (const int*) t.nbrlist2.p, t.nbrcnt.p, t.nbrlist.p);
+      PMX_HIP (hipGetLastError ());
+      if ((rc = fill ((const int*) c.ovf.p))) return rc;
+      t.nbr_ready = true;
+      return PMX_OK;
+    }
+  }
+  /* all leaves: leaf Q's list at Q * stride (stride = the longest list), so
+   * that a detection block finds its list without first reading an offset;
+   * CSR when that layout would take more than 2 GiB (or when asked for: the
    * environment variable PMX_NEIGHBOUR_LISTS=csr) */
-  std::vector<int> cnt (t.nleaves);
-  PMX_HIP (hipMemcpyAsync (cnt.data (), t.nbrcnt.p, sizeof (int) * t.nleaves, hipMemcpyDeviceToHost, c.stream));
-  PMX_HIP (hipStreamSynchronize (c.stream));
-  int mx = 1;
-  for (int q = 0; q < t.nleaves; q++) mx = std::max (mx, cnt[q]);
+  if ((rc = count (nullptr, &mx))) return rc;
+  mx = std::max (mx, 1);
   t.nbrstride = !c.nbr_csr && (size_t) t.nleaves * mx * sizeof (int) <= ((size_t) 2 << 30) ? mx : 0;
-  int rc;
   if (t.nbrstride)
   {
     t.nnbr = t.nleaves * mx;
@@ -937,11 +1028,8 @@ int tree_nbrlists (pmx_ctx &c)
     t.nnbr = total;
   }
   if (t.nbrlist.alloc (std::max (t.nnbr, 1))) { pmx_set_error ("out of device memory (neighbour lists)"); return PMX_ENOMEM; }
-  hipLaunchKernelGGL (k_leaf_nbr, grid (t.nleaves), dim3 (PMX_TPB), 0, c.stream, t.nleaves, (const double*) t.cell.p,
-                      c.rmax, (const int*) t.roots.p, (const int*) t.leafinst.p, (const int*) t.left.p,
-                      (const int*) t.right.p, (const int*) t.dim.p, (const double*) t.coord.p, t.nbrcnt.p,
-                      (const int*) t.nbroff.p, t.nbrlist.p, t.nbrstride, flag);
-  PMX_HIP (hipGetLastError ());
+  if ((rc = fill (nullptr))) return rc;
+  t.nbr_ready = true;
   return PMX_OK;
 }
 
