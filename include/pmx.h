@@ -158,7 +158,33 @@ typedef struct pmx_forcing
   const double *prstab;      /* [nsteps][prscols][6] prescribed linear 0-2 and angular 3-5 values of each
                                 history column at each step's curtime (input.cpp:4247-4421), or NULL (zeros) */
   int prscols;               /* columns per step of prstab (0: the model's, max prscol + 1 or prsnum) */
+  const int *hisrow;         /* [nsteps] row of the history output that samples the state at the end of
+                                each step, or -1 (parmec.cpp:2509-2514); NULL: no sampling */
+  int hisrows;               /* rows of hisout */
+  double *hisout;            /* [hisrows][histories] values written by pmx_dem (output_history,
+                                output.cpp:5143-5520); TIME entries are left to the caller */
 } pmx_forcing;
+
+/* time-history entities (HISTORY, input.cpp:3157-3555); particle entities
+ * average over their list (or take one particle's body point), spring
+ * entities over a list of spring numbers */
+enum
+{
+  PMX_HIS_TIME = 0, PMX_HIS_PX, PMX_HIS_PY, PMX_HIS_PZ, PMX_HIS_PL, PMX_HIS_DX, PMX_HIS_DY, PMX_HIS_DZ, PMX_HIS_DL,
+  PMX_HIS_VX, PMX_HIS_VY, PMX_HIS_VZ, PMX_HIS_VL, PMX_HIS_OX, PMX_HIS_OY, PMX_HIS_OZ, PMX_HIS_OL,
+  PMX_HIS_FX, PMX_HIS_FY, PMX_HIS_FZ, PMX_HIS_FL, PMX_HIS_TX, PMX_HIS_TY, PMX_HIS_TZ, PMX_HIS_TL,
+  PMX_HIS_LENGTH, PMX_HIS_STROKE, PMX_HIS_F, PMX_HIS_SF, PMX_HIS_FF, PMX_HIS_COUNT
+};
+
+typedef struct pmx_history
+{
+  int count;                 /* histories */
+  const int *entity;         /* [count] PMX_HIS_* */
+  const int *off;            /* [count+1] offsets into list */
+  const int *list;           /* particle numbers (spring numbers for LENGTH .. FF) */
+  const int *haspoint;       /* [count] 1: one particle's body point `point` (HIS_LIST|HIS_POINT), or NULL */
+  const double *point;       /* [count][3] referential point, or NULL */
+} pmx_history;
 
 typedef struct pmx_options
 {
@@ -281,6 +307,17 @@ int pmx_set_profiling (pmx_ctx *ctx, int on);
 /* Per-kernel totals since profiling was enabled; count receives the number
  * of entries (at most cap are written). */
 int pmx_get_profile (pmx_ctx *ctx, pmx_kernel_time *out, int cap, int *count);
+
+/* Time histories of the resident model (single model contexts): the specs
+ * are copied; pmx_upload clears them.  Particles of force and torque
+ * histories keep their force and torque stored every step.  NULL or count
+ * 0 removes them.  Replaces the registration half of HISTORY
+ * (input.cpp:3157-3555); the sampling is pmx_forcing.hisrow. */
+int pmx_set_history (pmx_ctx *ctx, const pmx_history *h);
+
+/* The histories' values of the current state into out[count] (the sample
+ * dem() takes before its first step when curtime is 0, parmec.cpp:2386-2394). */
+int pmx_history_sample (pmx_ctx *ctx, double *out);
 
 void pmx_destroy (pmx_ctx *ctx);
 

@@ -306,6 +306,8 @@ int upload_core (pmx_ctx *c, const pmx_model *m, const std::vector<InstOff> &off
   }
   PMX_HIP (hipStreamSynchronize (c->stream));
   c->uploaded = true;
+  c->nhis = 0;   /* histories name particles of the previous model */
+  c->his_active = false;
   return PMX_OK;
 }
 
@@ -714,6 +716,21 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
       if ((rc = up (*c, c->obstab, ot.data (), ot.size (), "obstacle table"))) return rc;
     }
   }
+  /* history samples (one model): the row of each step's end state */
+  c->his_active = I == 1 && c->nhis > 0 && fc && fc->hisrow && fc->hisrows > 0 && fc->hisout;
+  if (c->his_active)
+  {
+    std::vector<int> hr (std::max (nsteps, 1), -1);
+    for (int k = 0; k < nsteps; k++)
+    {
+      int r = fc->hisrow[k];
+      if (r >= fc->hisrows) return fail (PMX_EINVAL, "history row beyond hisrows");
+      hr[k] = r < 0 ? -1 : r;
+    }
+    if ((rc = up (*c, c->hisrow, hr.data (), hr.size (), "history rows"))) return rc;
+    if (c->hisbuf.alloc ((size_t) fc->hisrows * c->nhis) != hipSuccess) return fail (PMX_ENOMEM, "out of device memory (histories)");
+    PMX_HIP (hipStreamSynchronize (c->stream));
+  }
   c->cur0 = c->cur;
   c->graph_valid = false;
   c->stats.max_row = 0;
@@ -863,6 +880,13 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
       return rc;
   }
   if ((rc = report (nsteps))) return rc;
+  if (c->his_active)
+  {
+    PMX_HIP (hipMemcpyAsync (fc->hisout, c->hisbuf.p, sizeof (double) * (size_t) fc->hisrows * c->nhis,
+                             hipMemcpyDeviceToHost, c->stream));
+    PMX_HIP (hipStreamSynchronize (c->stream));
+    c->his_active = false;
+  }
   c->stepped += nsteps;
   c->cur = (c->cur0 + nsteps) & 1;
   c->curtime = c->icurtime[0]; c->curstep = c->icurstep[0]; c->stepnum = c->istepnum[0];
@@ -918,6 +942,21 @@ int pmx_contacts (pmx_ctx *c, pmx_pair *out, size_t cap, size_t *count)
   if (count) *count = all.size ();
   if (out) std::memcpy (out, all.data (), std::min (cap, all.size ()) * sizeof (pmx_pair));
   return PMX_OK;
+}
+
+int pmx_set_history (pmx_ctx *c, const pmx_history *h)
+{
+  if (!c) return fail (PMX_EINVAL, "null ctx");
+  (void) hipSetDevice (c->device);
+  if (!c->uploaded) return fail (PMX_EINVAL, "no model uploaded");
+  return history_set (*c, h);
+}
+
+int pmx_history_sample (pmx_ctx *c, double *out)
+{
+  if (!c || !out) return fail (PMX_EINVAL, "null argument");
+  (void) hipSetDevice (c->device);
+  return history_now (*c, out);
 }
 
 int pmx_set_profiling (pmx_ctx *c, int on)

@@ -225,6 +225,13 @@ struct pmx_ctx
                                   the nodes of pgexec), replayed every PMX_PROF_EVERY steps */
   hipGraphExec_t pgexec = nullptr;
   bool graph_valid = false;
+  /* time histories (single model): specs, the sampled row of each step of
+   * the current call, the sampled values [row][history] */
+  int nhis = 0;
+  bool his_active = false;     /* the current call samples (k_history is in the step) */
+  dvec<int> his_ent, his_off, his_list, hisrow;
+  dvec<signed char> his_pt;
+  dvec<double> his_pts, hisbuf;
   int *hring = nullptr;        /* pinned: per group of steps in flight, the control block and the flags */
   hipEvent_t rev[PMX_RING] = {};   /* recorded after each group's copy into hring */
   int det_nw = 4;              /* waves per k_detect block (autotuned) */
@@ -350,6 +357,10 @@ int step_rows (pmx_ctx &c, double h);
 int step_dynamics (pmx_ctx &c, double step0, double step1);
 int step_end (pmx_ctx &c);
 int enqueue_step (pmx_ctx &c, double h);     /* one whole time step */
+/* ---- time histories (pmx_history.hip) ---- */
+int history_set (pmx_ctx &c, const pmx_history *h);
+int history_now (pmx_ctx &c, double *out);
+int step_history (pmx_ctx &c);
 int step_obstacles (pmx_ctx &c, const double *tab, int row, double h);
 int step_prescribe_acc (pmx_ctx &c);           /* before dynamics */
 int step_prescribe_vel (pmx_ctx &c);           /* after dynamics */

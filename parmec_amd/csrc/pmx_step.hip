@@ -1500,7 +1500,7 @@ int enqueue_step (pmx_ctx &c, double h)
    * finishes, with the same arithmetic and the same result */
   if ((rc = step_rows (c, h)) || (rc = step_springs (c, h)) || (rc = step_prescribe_acc (c)) ||
       (rc = step_dynamics (c, h, h)) || (rc = step_prescribe_vel (c)) ||
-      (rc = step_obstacles (c, c.obstab.p, -1, h)) || (rc = step_end (c)))
+      (rc = step_obstacles (c, c.obstab.p, -1, h)) || (rc = step_history (c)) || (rc = step_end (c)))
     return rc;
   return PMX_OK;
 }

@@ -23,7 +23,8 @@ SPRING (1440-1788) is supported without stroke offsets (load-curve
 TSERIES); OBSTACLE with point/linear/angular (moving obstacles) and PRESCRIBE
 (2647-2795) tabulate their callables or TSERIES at the step times.  Not on
 this path (raise NotImplementedError): MESH, ANALYTICAL, torsion springs,
-unsprings, joints, HISTORY, OUTPUT, CRITICAL and adaptive time stepping.
+unsprings, joints, OUTPUT, CRITICAL, adaptive time stepping, and HISTORY's
+sphere/box sources and .h5 reading.
 """
 import bisect
 import math
@@ -84,6 +85,8 @@ class _State:
         self.dirty = True
         self.options = {"row_capacity": 0, "device": 0}
         self.springs = []            # SPRING records in creation order
+        self.histories = []          # (entity, source list, point or None, the returned Python list)
+        self.curtime_history = 0.0   # parmec.cpp:73, 2261
 
 
 _S = _State()
@@ -438,7 +441,6 @@ def _not_here(name):
 
 MESH = _not_here("MESH")
 ANALYTICAL = _not_here("ANALYTICAL")
-HISTORY = _not_here("HISTORY")
 OUTPUT = _not_here("OUTPUT")
 CRITICAL = _not_here("CRITICAL")
 
@@ -551,10 +553,78 @@ def PRESCRIBE(parnum, linear=None, angular=None, kind="vv"):
     _S.dirty = True
 
 
+_SPRING_ENTITIES = pmx.HIS_ENTITIES[pmx.HIS_SPRING_FIRST:]
+
+
+def HISTORY(entity, source=None, point=None, h5file=None, h5last=False):
+    """Time history of an entity; returns a list that DEM appends to at each
+    history interval (input.cpp:3157-3555, output.cpp:5143-5520).  Entities:
+    TIME; PX PY PZ |P| DX DY DZ |D| VX VY VZ |V| OX OY OZ |O| FX FY FZ |F| TX
+    TY TZ |T| over a particle number or list (averaged), or one particle's
+    body point (point = (x, y, z), referential); LENGTH STROKE F SF FF over
+    spring numbers.  Torsion-spring and joint entities, sphere and box
+    sources and .h5 post-processing are not on this path."""
+    if not isinstance(entity, str):
+        raise TypeError("'entity' must be a string")
+    if point is not None and not (isinstance(point, tuple) and len(point) == 3):
+        raise ValueError("'point' must be a tuple of size 3")
+    if h5file is not None:
+        raise NotImplementedError("HISTORY from .h5 files is not on the MI355X step path")
+    if entity not in pmx.HIS_ENTITIES:
+        raise ValueError("Invalid entity")
+    spring = entity in _SPRING_ENTITIES
+    nmax = len(_S.springs) if spring else len(_S.sph)
+    if source is None:                       # useful when entity is 'TIME'
+        if not _S.sph:
+            raise ValueError("No particle has been defined")
+        lst = [0]
+    elif isinstance(source, int) and not isinstance(source, bool):
+        lst = [source]
+    elif isinstance(source, list):
+        lst = [int(k) for k in source]
+    elif isinstance(source, tuple):
+        if spring:
+            raise ValueError("Invalid source kind for a spring/joint entity")
+        if len(source) not in (4, 6):
+            raise ValueError("Invalid source tuple size")
+        raise NotImplementedError(("Sphere" if len(source) == 4 else "Box") +
+                                  " based time history is not yet implemented")
+    else:
+        raise TypeError("'source' must be a number, a list or a tuple")
+    for k in lst:
+        if k < 0 or k >= nmax:
+            raise ValueError("Spring index out of range" if spring else "Particle index out of range")
+    pt = tuple(float(x) for x in point) if point is not None and len(lst) == 1 and not spring else None
+    out = []
+    _S.histories.append((entity, lst, pt, out))
+    return out
+
+
+def _interval(x, step):
+    """One DEM output interval: a number, a callable of time or a TSERIES
+    number (input.cpp:3977-4080); None: the step."""
+    if x is None:
+        return lambda t: step
+    if callable(x):
+        return lambda t: float(x(t))
+    if isinstance(x, int) and not isinstance(x, bool):
+        if x < 0 or x >= len(_S.tms):
+            raise ValueError("Invalid output interval TSERIES number")
+        ts = _S.tms[x]
+        return lambda t: float(ts(t))
+    if _is_number(x):
+        if x < 0.0:
+            raise ValueError("Invalid, negative, output interval")
+        return lambda t: float(x)
+    raise ValueError("Invalid output interval")
+
+
 def DEM(duration, step, interval=None, prefix=None, adaptive=0.0):
     """Run the DEM time stepping; returns the wall time in seconds
-    (input.cpp:3958-4091, parmec.cpp:2315-2530).  Output files are not written
-    (interval and prefix are accepted and ignored)."""
+    (input.cpp:3958-4091, parmec.cpp:2315-2530).  interval = (output, history)
+    or one value for both (number, callable or TSERIES number; default the
+    step): HISTORY lists are appended at each history interval.  Output
+    files are not written (the output interval and prefix are accepted)."""
     if not duration > 0:
         raise ValueError("'duration' must be positive")
     if not step > 0:
@@ -621,10 +691,43 @@ def DEM(duration, step, interval=None, prefix=None, adaptive=0.0):
                     prstab[i, c, :3] = [float(v) for v in lin(t)]
                 if ang is not None:
                     prstab[i, c, 3:] = [float(v) for v in ang(t)]
+    hisrow = hisout = None
+    if _S.histories:
+        # which steps' end states are sampled (parmec.cpp:2509-2514): the
+        # history interval at each step's curtime, crossed by curtime
+        if isinstance(interval, tuple):
+            if len(interval) != 2:
+                raise ValueError("Invalid output interval")
+            dt_his = _interval(interval[1], step)
+        else:
+            dt_his = _interval(interval, step)
+        ents = [e for e, _, _, _ in _S.histories]
+        _S.engine.set_history(ents, [l for _, l, _, _ in _S.histories], [p for _, _, p, _ in _S.histories])
+        if m.curtime == 0.0:               # the sample before the first step (parmec.cpp:2386-2394)
+            v = _S.engine.history_sample()
+            for i, (e, _, _, out) in enumerate(_S.histories):
+                out.append(0.0 if e == "TIME" else float(v[i]))
+        hisrow = np.full(max(len(times), 1), -1, dtype=np.int32)
+        tsamp = []
+        ch = _S.curtime_history
+        for k, t in enumerate(times):
+            d = dt_his(t)
+            if t >= ch + d:
+                hisrow[k] = len(tsamp)
+                tsamp.append(t)
+                ch += d
+        _S.curtime_history = ch
+        hisout = np.zeros((max(len(tsamp), 1), len(ents)))
+        if not tsamp:
+            hisrow = hisout = None
     f = pmx.forcing(len(times), gravity=tuple(_S.gravity), gtab=gtab, dtab=dtab, obstab=obstab, obstab0=obstab0,
-                    prstab=prstab)
+                    prstab=prstab, hisrow=hisrow, hisout=hisout)
     elapsed = _S.engine.dem(float(duration), float(step), f)
     _S.engine.download(m)
+    if hisrow is not None:
+        for r, t in enumerate(tsamp):
+            for i, (e, _, _, out) in enumerate(_S.histories):
+                out.append(float(t) if e == "TIME" else float(hisout[r, i]))
     return elapsed
 
 

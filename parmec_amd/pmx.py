@@ -47,7 +47,18 @@ class pmx_model(C.Structure):
 
 class pmx_forcing(C.Structure):
     _fields_ = [("gravity", C.c_double * 3), ("gtab", _dp), ("damping", C.c_double * 6), ("dtab", _dp),
-                ("obstab", _dp), ("obstab0", _dp), ("prstab", _dp), ("prscols", C.c_int)]
+                ("obstab", _dp), ("obstab0", _dp), ("prstab", _dp), ("prscols", C.c_int),
+                ("hisrow", _ip), ("hisrows", C.c_int), ("hisout", _dp)]
+
+
+# time-history entities of HISTORY (input.cpp:3177-3281), as PMX_HIS_* in pmx.h
+HIS_ENTITIES = ["TIME", "PX", "PY", "PZ", "|P|", "DX", "DY", "DZ", "|D|", "VX", "VY", "VZ", "|V|", "OX", "OY", "OZ",
+                "|O|", "FX", "FY", "FZ", "|F|", "TX", "TY", "TZ", "|T|", "LENGTH", "STROKE", "F", "SF", "FF"]
+HIS_SPRING_FIRST = HIS_ENTITIES.index("LENGTH")
+
+
+class pmx_history(C.Structure):
+    _fields_ = [("count", C.c_int), ("entity", _ip), ("off", _ip), ("list", _ip), ("haspoint", _ip), ("point", _dp)]
 
 
 class pmx_options(C.Structure):
@@ -71,7 +82,8 @@ class pmx_kernel_time(C.Structure):
 
 
 EXPORTS = ["pmx_step_times", "pmx_create", "pmx_upload", "pmx_dem", "pmx_download", "pmx_contacts",
-           "pmx_get_stats", "pmx_destroy", "pmx_last_error", "pmx_set_profiling", "pmx_get_profile"]
+           "pmx_get_stats", "pmx_destroy", "pmx_last_error", "pmx_set_profiling", "pmx_get_profile",
+           "pmx_set_history", "pmx_history_sample"]
 
 
 class pmx_server_options(C.Structure):
@@ -279,7 +291,7 @@ class Model:
 
 
 def forcing(nsteps=0, gravity=(0.0, 0.0, 0.0), gtab=None, damping=(0.0,) * 6, dtab=None, obstab=None, obstab0=None,
-            prstab=None):
+            prstab=None, hisrow=None, hisout=None):
     """Per-step inputs of dem(): gravity and damping constants or tables
     ([nsteps][3], [nsteps][6]); moving obstacle velocities (angular 0-2,
     linear 3-5) per step at curtime+step ([nsteps][obsnum][6]) and at step/2
@@ -315,6 +327,16 @@ def forcing(nsteps=0, gravity=(0.0, 0.0, 0.0), gtab=None, damping=(0.0,) * 6, dt
         f.prstab = prstab.ctypes.data_as(_dp)
         f.prscols = prstab.shape[1]
         keep.append(prstab)
+    if hisrow is not None:
+        # the history row sampling each step's end state (-1: none) and the
+        # output array [rows][histories] that dem() fills
+        hisrow = np.ascontiguousarray(hisrow, dtype=np.int32)
+        assert hisrow.shape[0] >= nsteps and hisout is not None and hisout.dtype == np.float64
+        assert hisout.flags["C_CONTIGUOUS"]
+        f.hisrow = hisrow.ctypes.data_as(_ip)
+        f.hisrows = hisout.shape[0]
+        f.hisout = hisout.ctypes.data_as(_dp)
+        keep.extend([hisrow, hisout])
     f._keep = keep
     return f
 
@@ -336,6 +358,33 @@ class Engine:
         if rc != PMX_OK:
             raise PmxError(rc, getattr(self.lib, self.prefix + "last_error")().decode())
         return rc
+
+    def set_history(self, entities, lists, points=None):
+        """Time histories of the uploaded model (pmx_set_history): entity
+        names of HIS_ENTITIES, a list of particle (spring) numbers each, and
+        per history a body point (3-tuple) or None."""
+        n = len(entities)
+        ent = np.array([HIS_ENTITIES.index(e) for e in entities], dtype=np.int32)
+        off = np.zeros(n + 1, dtype=np.int32)
+        for i, l in enumerate(lists):
+            off[i + 1] = off[i] + len(l)
+        lst = np.array([k for l in lists for k in l] or [0], dtype=np.int32)
+        hp = np.zeros(max(n, 1), dtype=np.int32)
+        pt = np.zeros((max(n, 1), 3))
+        for i, p in enumerate(points or [None] * n):
+            if p is not None:
+                hp[i] = 1
+                pt[i] = [float(x) for x in p]
+        h = pmx_history(n, ent.ctypes.data_as(_ip), off.ctypes.data_as(_ip), lst.ctypes.data_as(_ip),
+                        hp.ctypes.data_as(_ip), pt.ctypes.data_as(_dp))
+        self._call("set_history", self.h, C.byref(h))
+        self._nhis = n
+
+    def history_sample(self):
+        """The histories' values of the current state (pmx_history_sample)."""
+        out = np.zeros(max(getattr(self, "_nhis", 0), 1))
+        self._call("history_sample", self.h, out.ctypes.data_as(_dp))
+        return out[:getattr(self, "_nhis", 0)]
 
     def upload(self, model):
         self._call("upload", self.h, C.byref(model.struct()))

@@ -139,3 +139,27 @@ def test_engine_rejects_unknown_block_shapes(shape):
     with pytest.raises(pmx.PmxError) as err:
         pmx.Engine(tune=1, **shape)
     assert "must be" in str(err.value)
+
+
+def test_history_registration_errors():
+    """HISTORY validates like input.cpp:3157-3555 (no device needed)."""
+    from parmec_amd import parmec as P
+    P.RESET()
+    with pytest.raises(ValueError):
+        P.HISTORY("TIME")                      # no particle defined yet
+    mat = P.MATERIAL(1e3, 1e9, 0.25)
+    p = P.SPHERE((0.0, 0.0, 0.0), 0.5, mat, 1)
+    assert P.HISTORY("TIME") == []
+    with pytest.raises(ValueError):
+        P.HISTORY("NOPE", p)
+    with pytest.raises(ValueError):
+        P.HISTORY("PX", p + 1)
+    with pytest.raises(ValueError):
+        P.HISTORY("LENGTH", 0)                 # no spring
+    with pytest.raises(ValueError):
+        P.HISTORY("PX", p, (1.0, 2.0))
+    with pytest.raises(NotImplementedError):
+        P.HISTORY("PX", (0.0, 0.0, 0.0, 1.0))
+    with pytest.raises(NotImplementedError):
+        P.HISTORY("PX", p, h5file="x.h5")
+    P.RESET()
