@@ -180,6 +180,13 @@ __device__ __forceinline__ int xcd_block (int b, int n)
   return x * q + (x < r ? x : r) + k;
 }
 
+/* bits of x set in the lanes below this one (mbcnt: no lane mask held in
+ * registers) */
+__device__ __forceinline__ int below (unsigned long long x)
+{
+  return (int) __builtin_amdgcn_mbcnt_hi ((unsigned) (x >> 32), __builtin_amdgcn_mbcnt_lo ((unsigned) x, 0u));
+}
+
 __device__ __forceinline__ unsigned long long readlane_u64 (unsigned long long x, int k)
 {
   unsigned lo = (unsigned) __builtin_amdgcn_readlane ((int) (unsigned) x, k);
@@ -259,7 +266,11 @@ void k_detect (int E, const double *center, const double *radii, const int *part
     bL = 0; bcnt = 0; boff = 0;
     if (lane < np)
     {
-      bL = nbrlist[c0 + w + (p0 + lane) * NW];
+      /* the lane index taken afresh (mbcnt): fewer registers held through
+       * the candidate loop than with the first batch's address kept */
+      int ln = (int) __builtin_amdgcn_mbcnt_hi (~0u, __builtin_amdgcn_mbcnt_lo (~0u, 0u));
+      const int *nb = nbrlist + c0;
+      bL = nb[w + (p0 + ln) * NW];
       bcnt = lcount[bL];
       boff = lstride ? bL * lstride : loff[bL];
 #pragma unroll
@@ -335,7 +346,6 @@ void k_detect (int E, const double *center, const double *radii, const int *part
 
   float4 *rec32 = wrec32[w];
   int *mem = wmem[w], *lst = wlist[w];
-  unsigned long long lt = lane ? ((1ull << lane) - 1ull) : 0ull;
   for (int p0 = 0; p0 < ncw; p0 += 64)
   {
     /* up to 64 candidates at once, one per lane, and which block spheres'
@@ -400,8 +410,8 @@ void k_detect (int E, const double *center, const double *radii, const int *part
       bool own = L == Q;
       /* stage: the reaching spheres' slots and the candidate's records */
       __builtin_amdgcn_wave_barrier ();
-      if ((m0 >> lane) & 1ull) lst[__popcll (m0 & lt)] = lane;
-      if ((m1 >> lane) & 1ull) lst[n0 + __popcll (m1 & lt)] = lane + 64;
+      if ((m0 >> lane) & 1ull) lst[below (m0)] = lane;
+      if ((m1 >> lane) & 1ull) lst[n0 + below (m1)] = lane + 64;
       if (!own)
       {
         /* only the records whose AABB meets the block spheres' box, packed */
@@ -429,7 +439,7 @@ void k_detect (int E, const double *center, const double *radii, const int *part
           unsigned long long mk = __ballot (ok);
           if (ok)
           {
-            int p = kept + __popcll (mk & lt);
+            int p = kept + below (mk);
             rec32[p] = make_float4 ((float) (x.x - O[0]), (float) (x.y - O[1]), (float) (y.x - O[2]), (float) y.y);
             mem[p] = mj;
           }
