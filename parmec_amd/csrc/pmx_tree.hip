@@ -717,17 +717,17 @@ __global__ void k_leaf_nbr (int nleaves, const double *cell, double rmax, const 
 __global__ void k_nbr_shift (int nleaves, const int *leafinst, const int *iflag, const int *obase, const int *nbase,
                              int stride, const int *cnt0, const int *list0, int *cnt, int *list)
 {
-  int Q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (Q >= nleaves) return;
+  /* one thread per list slot (leaf Q, entry k), so a wave moves whole rows */
+  size_t q = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (size_t) nleaves * stride) return;
+  int Q = (int) (q / (unsigned) stride), k = (int) (q - (size_t) Q * stride);
   int i = leafinst[Q];
   if (iflag[i]) return;
   int delta = nbase[i] - obase[i];
   int Qo = Q - delta;
   int c = cnt0[Qo];
-  cnt[Q] = c;
-  const int *src = list0 + (size_t) Qo * stride;
-  int *dst = list + (size_t) Q * stride;
-  for (int k = 0; k < c; k++) dst[k] = src[k] + delta;
+  if (k == 0) cnt[Q] = c;
+  if (k < c) list[(size_t) Q * stride + k] = list0[(size_t) Qo * stride + k] + delta;
 }
 
 /* first leaf of each instance (base[I] = nleaves): leaves are numbered in
@@ -999,7 +999,7 @@ int tree_nbrlists (pmx_ctx &c)
     {
       t.nnbr = t.nleaves * t.nbrstride;
       if (t.nbrlist.alloc (std::max (t.nnbr, 1))) { pmx_set_error ("out of device memory (neighbour lists)"); return PMX_ENOMEM; }
-      hipLaunchKernelGGL (k_nbr_shift, grid (t.nleaves), dim3 (PMX_TPB), 0, c.stream, t.nleaves, (const int*) t.leafinst.p,
+      hipLaunchKernelGGL (k_nbr_shift, grid (t.nnbr), dim3 (PMX_TPB), 0, c.stream, t.nleaves, (const int*) t.leafinst.p,
                           (const int*) c.ovf.p, (const int*) t.ibase2.p, (const int*) t.ibase.p, t.nbrstride,
                           (const int*) t.nbrcnt2.p, (const int*) t.nbrlist2.p, t.nbrcnt.p, t.nbrlist.p);
       PMX_HIP (hipGetLastError ());
