@@ -69,7 +69,7 @@ __global__ void k_bin (int E, const double *center, const int *roots, const int 
   int hint = ctl[CTL_HINT];
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   bool valid = e < E && !frozen (stop, ell_inst[e < E ? e : 0], ctl);
-  int L = -1;
+  int L = -1, hprev = -1;
   double c[3] = {0.0, 0.0, 0.0}, rad = 0.0;
   int pe = 0;
   if (valid)
@@ -81,6 +81,7 @@ __global__ void k_bin (int E, const double *center, const int *roots, const int 
       /* the descent "c[d] < coord -> left" ends in leaf h iff lower <= c < upper
        * along every axis, with lower/upper the leaf's reach cell */
       int h = leafof[e];
+      hprev = h;
       bool in = true;
 #pragma unroll
       for (int k = 0; k < 3; k++)
@@ -118,7 +119,7 @@ __global__ void k_bin (int E, const double *center, const int *roots, const int 
   base = __shfl (base, leader, 64);
   int s = base + __popcll (grp & lt);
   if (!valid) return;
-  leafof[e] = L;
+  if (L != hprev) leafof[e] = L;   /* unchanged where the hint held */
   if (s >= PMX_LSIZE)
   {
     atomicAdd (&flags[FLAG_LEAF_OVERFLOW], 1);
@@ -128,7 +129,6 @@ __global__ void k_bin (int E, const double *center, const int *roots, const int 
   }
   if (!lstride) { slot[e] = s; return; }
   size_t pos = (size_t) L * lstride + s;
-  xcnt[e] = 0;   /* reverse hits of this step start at zero */
   lmem[pos] = e;
   lpart[pos] = pe;
   double2 *rec = (double2*) (lpos + 4*pos);
@@ -143,7 +143,6 @@ __global__ void k_scatter (int *ctl, int E, const double *center, const double *
   if (ctl[CTL_ABORT]) return;
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= E || frozen (stop, ell_inst[e], ctl)) return;
-  xcnt[e] = 0;   /* reverse hits of this step start at zero */
   int pos = loff[leafof[e]] + slot[e];
   lmem[pos] = e;
   lpart[pos] = part[e];
@@ -628,7 +627,7 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
              const double *invm, int pairnum, const int *pairs, const int *ikind, const double *iparam,
              int ninst, const int *ell_inst, const int *pair_off,
              uint32_t *key0, R *st0, int *cnt0, uint32_t *key1, R *st1, int *cnt1, int cur0, int cap,
-             uint32_t *hits, const int *hitcnt, int hcap, const int *xbuf, const int *xcnt, int xcap,
+             uint32_t *hits, const int *hitcnt, int hcap, const int *xbuf, int *xcnt, int xcap,
              const double *gtab, const int *ctl, double h, double *force, double *torque, int *flags,
              const double *obspnt, const double *obsang, const double *obslin,
              const int *busy, const int *nbusy, int list, int slices)
@@ -657,6 +656,7 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
    * there are more (both orders are the same unique sorted set) */
   int nh = hitcnt[e];
   int nx = xcnt[e];
+  if (nx) xcnt[e] = 0;   /* the next step's reverse hits start at zero (only busy spheres have any) */
   nx = nx < xcap ? nx : xcap;
   bool small = nh + nx <= ROWS_SORT;
   uint32_t *H = small ? sh + threadIdx.x : hits + e;
@@ -1367,7 +1367,7 @@ void rows_launch (pmx_ctx &c, R *st0, R *st1, double h, int tpb, int nb)
                       c.I, (const int*) c.ell_inst.p, (const int*) c.pair_off.p,
                       c.rowkey[0].p, st0, c.rowcnt[0].p, c.rowkey[1].p, st1, c.rowcnt[1].p,
                       c.cur0, c.cap, c.hits.p, (const int*) c.hitcnt.p, c.hcap,
-                      (const int*) c.xbuf.p, (const int*) c.xcnt.p, c.xcap, (const double*) c.gtab.p,
+                      (const int*) c.xbuf.p, c.xcnt.p, c.xcap, (const double*) c.gtab.p,
                       (const int*) c.ctl.p, h, c.force.p, c.torque.p, c.dflags.p,
                       (const double*) c.obspnt.p, (const double*) c.obsang.p, (const double*) c.obslin.p,
                       (const int*) c.rbusy.p, (const int*) c.rnbusy.p, tpb, slices);
@@ -1596,6 +1596,7 @@ int autotune (pmx_ctx &c, double h)
   if (nw < 0) { pmx_set_error ("autotuning failed"); return PMX_EDEVICE; }
   c.det_nw = nw;
   c.tuned++;
+  if (clear ()) { pmx_set_error ("autotuning failed"); return PMX_EDEVICE; }   /* reverse-hit counts of the runs */
   if ((rc = clear_leaf_counts (c))) return rc;   /* the next bin starts from zero */
   return PMX_OK;
 }
