@@ -1281,9 +1281,10 @@ int step_bin (pmx_ctx &c)
   /* the leaf counts are zero here: cleared by step_prepare after a build,
    * by k_rows in every step and by the autotuner after its runs */
   {
-    /* per sphere: centre 24, leaf 4, slot 4; slotted: no slot, but radius 8,
-     * particle 4, reverse-hit count 4 and the record 32, member 4, particle 4 */
-    ProfScope ps (c, K_BIN, (c.lstride ? 84.0 : 32.0) * c.E);
+    /* per sphere: centre 24, leaf 4 (read for the hint; written where it
+     * changes), slot 4; slotted: no slot, but radius 8, particle 4 and the
+     * record 32, member 4, particle 4 */
+    ProfScope ps (c, K_BIN, (c.lstride ? 80.0 : 32.0) * c.E);
     hipLaunchKernelGGL (k_bin, grid (c.E), dim3 (PMX_TPB), 0, c.stream, c.E, (const double*) c.center.p,
                         (const int*) t.roots.p, (const int*) c.ell_inst.p, (const int*) t.left.p, (const int*) t.right.p, (const int*) t.dim.p, (const double*) t.coord.p,
                         (const double*) t.cell.p, t.nleaves, c.ctl.p, c.leafof.p, c.slot.p, c.lcount.p, c.dflags.p,
