@@ -56,8 +56,9 @@ int alloc_rows (pmx_ctx &c)
     return fail (PMX_ENOMEM, "out of device memory (broad phase)");
   PMX_HIP (hipMemsetAsync (c.rowcnt[0].p, 0, E * sizeof (int), c.stream));
   PMX_HIP (hipMemsetAsync (c.rowcnt[1].p, 0, E * sizeof (int), c.stream));
-  /* reverse-hit counts: zero between steps (k_rows clears the ones it consumes) */
+  /* hit and reverse-hit counts: zero between steps (k_rows clears the ones it consumes) */
   PMX_HIP (hipMemsetAsync (c.xcnt.p, 0, E * sizeof (int), c.stream));
+  PMX_HIP (hipMemsetAsync (c.hitcnt.p, 0, E * sizeof (int), c.stream));
   c.cur = 0;
   return PMX_OK;
 }

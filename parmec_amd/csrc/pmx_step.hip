@@ -538,7 +538,10 @@ void k_detect (int E, const double *center, const double *radii, const int *part
     }
   }
   __syncthreads ();
-  for (int s = t; s < qcnt; s += NW*64) hitcnt[lmem[qoff + s]] = shc[s] < hcap ? shc[s] : hcap;
+  /* hit counts of the spheres with hits (every other sphere's count is
+   * zero already: k_rows clears the counts it consumes) */
+  for (int s = t; s < qcnt; s += NW*64)
+    if (shc[s] > 0) hitcnt[lmem[qoff + s]] = shc[s] < hcap ? shc[s] : hcap;
 }
 
 __global__ void k_force_init (const int *ctl, int N, const int *list, int nlist, const double *mass, const double *gtab,
@@ -627,7 +630,7 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
              const double *invm, int pairnum, const int *pairs, const int *ikind, const double *iparam,
              int ninst, const int *ell_inst, const int *pair_off,
              uint32_t *key0, R *st0, int *cnt0, uint32_t *key1, R *st1, int *cnt1, int cur0, int cap,
-             uint32_t *hits, const int *hitcnt, int hcap, const int *xbuf, int *xcnt, int xcap,
+             uint32_t *hits, int *hitcnt, int hcap, const int *xbuf, int *xcnt, int xcap,
              const double *gtab, const int *ctl, double h, double *force, double *torque, int *flags,
              const double *obspnt, const double *obsang, const double *obslin,
              const int *busy, const int *nbusy, int list, int slices)
@@ -656,7 +659,9 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
    * there are more (both orders are the same unique sorted set) */
   int nh = hitcnt[e];
   int nx = xcnt[e];
-  if (nx) xcnt[e] = 0;   /* the next step's reverse hits start at zero (only busy spheres have any) */
+  /* the next step's hit counts start at zero (only busy spheres have any) */
+  if (nh) hitcnt[e] = 0;
+  if (nx) xcnt[e] = 0;
   nx = nx < xcap ? nx : xcap;
   bool small = nh + nx <= ROWS_SORT;
   uint32_t *H = small ? sh + threadIdx.x : hits + e;
@@ -1367,7 +1372,7 @@ void rows_launch (pmx_ctx &c, R *st0, R *st1, double h, int tpb, int nb)
                       (const int*) c.pairs.p, (const int*) c.ikind.p, (const double*) c.iparam.p,
                       c.I, (const int*) c.ell_inst.p, (const int*) c.pair_off.p,
                       c.rowkey[0].p, st0, c.rowcnt[0].p, c.rowkey[1].p, st1, c.rowcnt[1].p,
-                      c.cur0, c.cap, c.hits.p, (const int*) c.hitcnt.p, c.hcap,
+                      c.cur0, c.cap, c.hits.p, c.hitcnt.p, c.hcap,
                       (const int*) c.xbuf.p, c.xcnt.p, c.xcap, (const double*) c.gtab.p,
                       (const int*) c.ctl.p, h, c.force.p, c.torque.p, c.dflags.p,
                       (const double*) c.obspnt.p, (const double*) c.obsang.p, (const double*) c.obslin.p,
@@ -1597,7 +1602,9 @@ int autotune (pmx_ctx &c, double h)
   if (nw < 0) { pmx_set_error ("autotuning failed"); return PMX_EDEVICE; }
   c.det_nw = nw;
   c.tuned++;
-  if (clear ()) { pmx_set_error ("autotuning failed"); return PMX_EDEVICE; }   /* reverse-hit counts of the runs */
+  /* hit and reverse-hit counts of the runs back to zero */
+  hipLaunchKernelGGL (k_fill_i, grid (c.E), dim3 (PMX_TPB), 0, c.stream, (const int*) c.ctl.p, c.hitcnt.p, c.E, 0);
+  if (clear ()) { pmx_set_error ("autotuning failed"); return PMX_EDEVICE; }
   if ((rc = clear_leaf_counts (c))) return rc;   /* the next bin starts from zero */
   return PMX_OK;
 }
