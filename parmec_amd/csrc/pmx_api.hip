@@ -340,6 +340,7 @@ static void drop_graph (pmx_ctx *c)
   if (c->gexec) { (void) hipGraphExecDestroy (c->gexec); c->gexec = nullptr; }
   if (c->graph) { (void) hipGraphDestroy (c->graph); c->graph = nullptr; }
   if (c->pgexec) { (void) hipGraphExecDestroy (c->pgexec); c->pgexec = nullptr; }
+  if (c->gexec_multi) { (void) hipGraphExecDestroy (c->gexec_multi); c->gexec_multi = nullptr; }
   if (c->pgraph) { (void) hipGraphDestroy (c->pgraph); c->pgraph = nullptr; }
   for (auto &q : c->prof.cap) { if (q.a) (void) hipEventDestroy (q.a); if (q.b) (void) hipEventDestroy (q.b); }
   c->prof.cap.clear ();
@@ -542,9 +543,24 @@ static bool place_prof_nodes (pmx_ctx *c)
  * are put around each profiled kernel so that fresh events can be set on
  * them before every launch (if they cannot be placed, steps are enqueued
  * eagerly instead) */
-static int capture (pmx_ctx *c, double h)
+static int capture (pmx_ctx *c, double h, bool multi)
 {
   drop_graph (c);
+  if (multi && c->prof.mode == 0)
+  {
+    /* PMX_GRAPH_STEPS steps in one graph: consecutive launches of a short
+     * step each cost the runtime's launch gap, one per graph instead */
+    hipGraph_t g = nullptr;
+    PMX_HIP (hipStreamBeginCapture (c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = PMX_OK;
+    for (int k = 0; k < PMX_GRAPH_STEPS && rc == PMX_OK; k++) rc = enqueue_step (*c, h);
+    hipError_t e = hipStreamEndCapture (c->stream, &g);
+    if (rc) { if (g) (void) hipGraphDestroy (g); return rc; }
+    PMX_HIP (e);
+    e = hipGraphInstantiate (&c->gexec_multi, g, nullptr, nullptr, 0);
+    (void) hipGraphDestroy (g);
+    PMX_HIP (e);
+  }
   for (int pass = 0; pass < (c->prof.mode == 1 ? 2 : 1); pass++)
   {
     hipGraph_t &g = pass ? c->pgraph : c->graph;
@@ -819,7 +835,8 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
   {
     int next = nsteps;
     for (int i = 0; i < I; i++) if (!reported[i] && nst[i] > done) next = std::min (next, nst[i]);
-    if (c->prof.mode != 2 && !c->prof.eager && !c->graph_valid && (rc = capture (c, step0))) return rc;
+    if (c->prof.mode != 2 && !c->prof.eager && !c->graph_valid && (rc = capture (c, step0, gsize >= PMX_GRAPH_STEPS)))
+      return rc;
     bool eager = c->prof.mode == 2 || (c->prof.mode == 1 && c->prof.eager);
     int ctl[CTL_COUNT] = {0}, fl[FLAG_COUNT] = {0};
     int enq = done, gq = 0, gs = 0;   /* first step not enqueued; groups enqueued, seen */
@@ -831,7 +848,10 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
       while (enq < next && gq - gs < PMX_LOOKAHEAD)
       {
         int n = std::min (gsize, next - enq);
-        for (int q = 0; q < n; q++)
+        int q = 0;
+        if (!eager && c->gexec_multi)
+          for (; q + PMX_GRAPH_STEPS <= n; q += PMX_GRAPH_STEPS) PMX_HIP (hipGraphLaunch (c->gexec_multi, c->stream));
+        for (; q < n; q++)
         {
           if (eager)
           {

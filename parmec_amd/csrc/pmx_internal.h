@@ -18,6 +18,7 @@
 #define PMX_RETUNE_STEPS 256
 #define PMX_LOOKAHEAD 2      /* groups of steps enqueued ahead of the one the host waits for */
 #define PMX_PROF_EVERY 8     /* profiling mode 1 times every 8th step (timing events slow the kernels) */
+#define PMX_GRAPH_STEPS 8    /* steps per graph launch where steps are short (a launch costs ~20 us) */
 #define PMX_RING 4           /* slots of the pinned flag ring (> PMX_LOOKAHEAD) */   /* steps of a context before the second autotuner pass */
 #define PMX_TRI_FLAG 0x80000000u  /* row key flag of a triangle partner */
 #define PMX_NO_ROOT INT_MIN       /* tree root of an instance without spheres */
@@ -224,6 +225,8 @@ struct pmx_ctx
   hipGraph_t pgraph = nullptr; /* profiling mode 1: the step with event record nodes (kept: they name
                                   the nodes of pgexec), replayed every PMX_PROF_EVERY steps */
   hipGraphExec_t pgexec = nullptr;
+  hipGraphExec_t gexec_multi = nullptr;   /* PMX_GRAPH_STEPS steps in one graph (groups of that many
+                                             steps or more, profiling off): one launch per 8 steps */
   bool graph_valid = false;
   /* time histories (single model): specs, the sampled row of each step of
    * the current call, the sampled values [row][history] */
