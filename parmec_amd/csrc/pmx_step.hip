@@ -1311,7 +1311,7 @@ int step_scatter (pmx_ctx &c)
   int rc = scan_exclusive_to (c, (const int*) c.lcount.p, c.loff.p, L, c.ctl.p);
   if (rc) return rc;
   {
-    ProfScope ps (c, K_SCATTER, 84.0 * c.E);
+    ProfScope ps (c, K_SCATTER, 80.0 * c.E);   /* centre 24, radius 8, leaf + slot 8, particle 4; member 4, record 32, particle 4 */
     hipLaunchKernelGGL (k_scatter, grid (c.E), dim3 (PMX_TPB), 0, c.stream, c.ctl.p, c.E, (const double*) c.center.p,
                         (const double*) c.radii.p, (const int*) c.leafof.p, (const int*) c.slot.p, (const int*) c.loff.p,
                         c.lmem.p, c.lpos.p, c.staggered ? (const int*) c.istop.p : (const int*) nullptr,
