@@ -619,6 +619,24 @@ def _interval(x, step):
     raise ValueError("Invalid output interval")
 
 
+def _history_rows(times, curtime_history, dt_his):
+    """The steps whose end state dem() samples (parmec.cpp:2509-2514): at
+    step k, with curtime = times[k], a sample is taken when curtime >=
+    curtime_history + interval(curtime), and curtime_history then advances by
+    that interval.  Returns (row of each step or -1, sampled curtimes, the new
+    curtime_history)."""
+    rows = np.full(max(len(times), 1), -1, dtype=np.int32)
+    tsamp = []
+    ch = curtime_history
+    for k, t in enumerate(times):
+        d = dt_his(t)
+        if t >= ch + d:
+            rows[k] = len(tsamp)
+            tsamp.append(t)
+            ch += d
+    return rows, tsamp, ch
+
+
 def DEM(duration, step, interval=None, prefix=None, adaptive=0.0):
     """Run the DEM time stepping; returns the wall time in seconds
     (input.cpp:3958-4091, parmec.cpp:2315-2530).  interval = (output, history)
@@ -707,16 +725,7 @@ def DEM(duration, step, interval=None, prefix=None, adaptive=0.0):
             v = _S.engine.history_sample()
             for i, (e, _, _, out) in enumerate(_S.histories):
                 out.append(0.0 if e == "TIME" else float(v[i]))
-        hisrow = np.full(max(len(times), 1), -1, dtype=np.int32)
-        tsamp = []
-        ch = _S.curtime_history
-        for k, t in enumerate(times):
-            d = dt_his(t)
-            if t >= ch + d:
-                hisrow[k] = len(tsamp)
-                tsamp.append(t)
-                ch += d
-        _S.curtime_history = ch
+        hisrow, tsamp, _S.curtime_history = _history_rows(times, _S.curtime_history, dt_his)
         hisout = np.zeros((max(len(tsamp), 1), len(ents)))
         if not tsamp:
             hisrow = hisout = None

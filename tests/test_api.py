@@ -163,3 +163,29 @@ def test_history_registration_errors():
     with pytest.raises(NotImplementedError):
         P.HISTORY("PX", p, h5file="x.h5")
     P.RESET()
+
+
+def test_history_sampling_rule():
+    """parmec.cpp:2509-2514: a step is sampled when its curtime reaches
+    curtime_history + interval, and the history time then advances by the
+    interval; both sums accumulate in floating point as the reference's do,
+    which decides the borderline steps."""
+    from parmec_amd import parmec as P, pmx
+    h = 0.1
+    times = pmx.step_times(1.0, h, 0.0)            # 11 steps: 0.1 added ten times stays below 1.0
+    assert len(times) == 11
+    rows, ts, ch = P._history_rows(times, 0.0, lambda t: h)
+    assert list(rows) == [-1] + list(range(10))    # every step from the second (same sums on both sides)
+    assert ts == list(times[1:]) and ch == ts[-1]
+    # an interval of two steps: at step 6 curtime is 0.6 but the history time
+    # plus 0.2 is 0.6000000000000001, so the sample moves to step 7 (a count
+    # of steps would give 2, 4, 6, 8, 10)
+    rows, ts, ch = P._history_rows(times, 0.0, lambda t: 2 * h)
+    assert [k for k in range(11) if rows[k] >= 0] == [2, 4, 7, 9]
+    # an interval that changes with time is evaluated at each step's curtime
+    rows, ts, ch = P._history_rows(times, 0.0, lambda t: 0.3 if t < 0.5 else 0.1)
+    assert [k for k in range(11) if rows[k] >= 0] == [3, 5, 6, 7, 8, 9, 10]
+    # a second call continues from the first call's history time
+    _, _, ch0 = P._history_rows(times, 0.0, lambda t: h)
+    rows2, _, _ = P._history_rows([1.0, 1.1], ch0, lambda t: h)
+    assert list(rows2) == [-1, 0]                  # 1.0 < 0.9999999999999999 + 0.1
