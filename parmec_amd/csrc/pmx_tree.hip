@@ -682,7 +682,7 @@ __device__ int leaf_nbr_one (int Q, int nleaves, const double *cell, double rmax
     }
     else
     {
-      if (list) list[stride ? (size_t) Q * stride + n : (size_t) off[Q] + n] = -ref-1;
+      if (list && (!stride || n < stride)) list[stride ? (size_t) Q * stride + n : (size_t) off[Q] + n] = -ref-1;
       n++;
     }
     if (sp == 0) break;
@@ -691,9 +691,10 @@ __device__ int leaf_nbr_one (int Q, int nleaves, const double *cell, double rmax
   return n;
 }
 
-/* leaves whose cell meets leaf Q's cell inflated by rmax: counted (list
- * null; the longest count folded into cmax) or listed; only the leaves of
- * instances with a nonzero iflag when iflag is not null */
+/* leaves whose cell meets leaf Q's cell inflated by rmax: counted (cnt,
+ * the longest count folded into cmax) and/or listed (at Q * stride, entries
+ * past the stride dropped, or at off[Q]); only the leaves of instances with
+ * a nonzero iflag when iflag is not null */
 __global__ void k_leaf_nbr (int nleaves, const double *cell, double rmax, const int *roots, const int *leafinst,
                             const int *left, const int *right, const int *dim, const double *coord, int *cnt,
                             const int *off, int *list, int stride, int *flag, const int *iflag, int *cmax)
@@ -702,8 +703,8 @@ __global__ void k_leaf_nbr (int nleaves, const double *cell, double rmax, const 
   int n = 0;
   if (Q < nleaves && !(iflag && !iflag[leafinst[Q]]))
     n = leaf_nbr_one (Q, nleaves, cell, rmax, roots, leafinst, left, right, dim, coord, off, list, stride, flag);
-  if (Q < nleaves && !list && !(iflag && !iflag[leafinst[Q]])) cnt[Q] = n;
-  if (!list && cmax)
+  if (Q < nleaves && cmax && !(iflag && !iflag[leafinst[Q]])) cnt[Q] = n;
+  if (cmax)
   {
     for (int o = 32; o >= 1; o >>= 1) n = max (n, __shfl_xor (n, o, 64));
     if ((threadIdx.x & 63) == 0) atomicMax (cmax, n);
@@ -989,21 +990,28 @@ int tree_nbrlists (pmx_ctx &c)
     return PMX_OK;
   };
   int rc, mx = 0;
-  if (shift)
+  if (shift && (size_t) t.nleaves * t.nbrstride * sizeof (int) <= ((size_t) 2 << 30))
   {
-    /* a partial build: the unchanged instances' lists move with their
-     * leaves; the flagged instances' leaves are listed afresh, in place
-     * while their longest list fits the stride */
-    if ((rc = count ((const int*) c.ovf.p, &mx))) return rc;
-    if (mx <= t.nbrstride && (size_t) t.nleaves * t.nbrstride * sizeof (int) <= ((size_t) 2 << 30))
+    /* a partial build: the flagged instances' leaves are counted and listed
+     * at the previous stride in one traversal, and the unchanged instances'
+     * lists move with their leaves; a list longer than the stride sends the
+     * build to the full path below */
+    t.nnbr = t.nleaves * t.nbrstride;
+    if (t.nbrlist.alloc (std::max (t.nnbr, 1))) { pmx_set_error ("out of device memory (neighbour lists)"); return PMX_ENOMEM; }
+    hipLaunchKernelGGL (k_fill_int, dim3 (1), dim3 (64), 0, c.stream, t.dmax.p, 1, 0);
+    hipLaunchKernelGGL (k_leaf_nbr, grid (t.nleaves), dim3 (PMX_TPB), 0, c.stream, t.nleaves, (const double*) t.cell.p,
+                        c.rmax, (const int*) t.roots.p, (const int*) t.leafinst.p, (const int*) t.left.p,
+                        (const int*) t.right.p, (const int*) t.dim.p, (const double*) t.coord.p, t.nbrcnt.p,
+                        (const int*) t.nbroff.p, t.nbrlist.p, t.nbrstride, flag, (const int*) c.ovf.p, t.dmax.p);
+    PMX_HIP (hipGetLastError ());
+    PMX_HIP (hipMemcpyAsync (&mx, t.dmax.p, sizeof (int), hipMemcpyDeviceToHost, c.stream));
+    PMX_HIP (hipStreamSynchronize (c.stream));
+    if (mx <= t.nbrstride)
     {
-      t.nnbr = t.nleaves * t.nbrstride;
-      if (t.nbrlist.alloc (std::max (t.nnbr, 1))) { pmx_set_error ("out of device memory (neighbour lists)"); return PMX_ENOMEM; }
       hipLaunchKernelGGL (k_nbr_shift, grid (t.nnbr), dim3 (PMX_TPB), 0, c.stream, t.nleaves, (const int*) t.leafinst.p,
                           (const int*) c.ovf.p, (const int*) t.ibase2.p, (const int*) t.ibase.p, t.nbrstride,
                           (const int*) t.nbrcnt2.p, (const int*) t.nbrlist2.p, t.nbrcnt.p, t.nbrlist.p);
       PMX_HIP (hipGetLastError ());
-      if ((rc = fill ((const int*) c.ovf.p))) return rc;
       t.nbr_ready = true;
       return PMX_OK;
     }
