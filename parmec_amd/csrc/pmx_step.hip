@@ -511,14 +511,39 @@ void k_detect (int E, const double *center, const double *radii, const int *part
    * static ones from the rebuild-time lists, moving ones from this step's */
   if (ntq > 0 && qcnt > 0)
   {
+    /* the leaf's triangles staged once in the (now free) candidate staging
+     * area when they fit (21 per wave of the block), so the pairs read
+     * their vertices from LDS instead of each gathering them from the
+     * triangle arrays */
+    double *stri = (double*) &wrec32[0][0];
+    bool staged = ntq * 9 * (int) sizeof (double) <= (int) sizeof (wrec32);
+    if (staged)
+    {
+      for (int q = t; q < 9 * ntq; q += NW*64)
+      {
+        int m = q / 9, k = q - 9*m;
+        int tt = m < nsq ? trilist[toff + m] : mtlist[(size_t) Q*mcap + (m - nsq)];
+        stri[q] = tri[(size_t) k*T + tt];
+      }
+      __syncthreads ();
+    }
     int work = qcnt * ntq;
     for (int q = t; q < work; q += NW*64)
     {
       int ti = q / ntq, m = q - ti * ntq;
-      int tt = m < nsq ? trilist[toff + m] : mtlist[(size_t) Q*mcap + (m - nsq)];
+      int tt = -1;
       double v[9];
+      if (staged)
+      {
 #pragma unroll
-      for (int k = 0; k < 9; k++) v[k] = tri[(size_t) k*T + tt];
+        for (int k = 0; k < 9; k++) v[k] = stri[9*m + k];
+      }
+      else
+      {
+        tt = m < nsq ? trilist[toff + m] : mtlist[(size_t) Q*mcap + (m - nsq)];
+#pragma unroll
+        for (int k = 0; k < 9; k++) v[k] = tri[(size_t) k*T + tt];
+      }
       double px = ssph[4*ti], py = ssph[4*ti+1], pz = ssph[4*ti+2], rr = ssph[4*ti+3];   /* the record's r: radii[e] */
       /* the returned point lies on the triangle, so its distance is at least
        * the distance to the triangle's box; skip when that exceeds r by a
@@ -531,6 +556,7 @@ void k_detect (int E, const double *center, const double *radii, const int *part
       double depth = triangle_sphere (v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8], px, py, pz, rr, pt, nr);
       if (depth > 0.0)
       {
+        if (tt < 0) tt = m < nsq ? trilist[toff + m] : mtlist[(size_t) Q*mcap + (m - nsq)];
         int slot = atomicAdd (&shc[ti], 1);
         if (slot < hcap) hits[(size_t) slot * E + lmem[qoff + ti]] = PMX_TRI_FLAG | (uint32_t) tt;
         else atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1);
