@@ -193,19 +193,51 @@ __device__ __forceinline__ unsigned long long readlane_u64 (unsigned long long x
   return ((unsigned long long) hi << 32) | lo;
 }
 
+/* FP32 bounds below / above any double v.  (float) v rounds to nearest,
+ * within half an ulp of the result (a quarter at a binade's bottom);
+ * stepping out by |f| 2^-23 (rounded to nearest: at least one ulp) and by
+ * 2^-120 (above every subnormal rounding) stays on the safe side, and the
+ * overflowed cases (inf - inf = NaN, or v beyond the float range) end at
+ * the largest finite float, which is still on the safe side of v */
+__device__ __forceinline__ float f32_below (double v)
+{
+  float f = (float) v;
+  return fminf (f - fabsf (f) * 0x1p-23f - 0x1p-120f, FLT_MAX);
+}
+
+__device__ __forceinline__ float f32_above (double v)
+{
+  float f = (float) v;
+  return fmaxf (f + fabsf (f) * 0x1p-23f + 0x1p-120f, -FLT_MAX);
+}
+
+/* a sphere's outward-rounded AABB (lo, hi) against a cell packed as
+ * (lo x, lo y), (lo z, hi x), (hi y, hi z) */
+__device__ __forceinline__ bool reach32 (const float *lo, const float *hi, float2 a, float2 b, float2 e)
+{
+  /* bitwise: one straight line of compares, no branches */
+  return (lo[0] <= b.y) & (lo[1] <= e.x) & (lo[2] <= e.y) & (hi[0] >= a.x) & (hi[1] >= a.y) & (hi[2] >= b.x);
+}
+
 /* k_detect runs NW waves per block (1, 2, 4 or 8; the block's spheres are
  * spread over its threads); NW is picked per workload by the autotuner */
 
 /* one sphere-sphere test of the reference leaf test (condet.ispc:128-144):
- * depth = r_a + r_b - |c_a - c_b| > 0.  d2 >= sr^2 (1 + 1e-10) implies the
- * rounded sqrt(d2) >= sr, hence depth <= 0: such pairs skip the sqrt */
+ * depth = r_a + r_b - |c_a - c_b| > 0, with the correctly rounded sqrt.
+ * d2 >= sr^2 (1 + 1e-10) implies sqrt(d2) rounds to >= sr, hence depth <= 0;
+ * d2 < sr^2 (1 - 1e-10) (sr^2 finite) implies sqrt(d2) < sr (1 - 4.9e-11),
+ * which rounds below sr, hence depth > 0: only pairs in between take the
+ * sqrt (the roundings of sr^2 and of the bounds are below 3e-16) */
 __device__ __forceinline__ bool overlap (double ax, double ay, double az, double ar, double bx, double by, double bz,
                                          double br)
 {
   double qx = ax-bx, qy = ay-by, qz = az-bz;
   double d2 = qx*qx + qy*qy + qz*qz;
   double sr = ar + br;
-  if (d2 >= sr * sr * 1.0000000001) return false;
+  double s2 = sr * sr;
+  if (d2 >= s2 * 1.0000000001) return false;
+  if (d2 < s2 * 0.9999999999 && s2 <= DBL_MAX) return true;
+  asm volatile ("");   /* a real branch: the sqrt is not computed for every lane */
   double len = sqrt (d2);
   return sr - len > 0.0;
 }
@@ -351,50 +383,73 @@ void k_detect (int E, const double *center, const double *radii, const int *part
      * own AABB reaches each one's cell (condet.ispc:104-113) */
     int np = ncw - p0 < 64 ? ncw - p0 : 64;
     if (p0 > 0) batch (p0);
-    /* the (candidate, block sphere) pairs spread over the lanes, candidate
-     * by candidate (pair q: candidate q / qcnt, sphere q mod qcnt), the
-     * candidate's cell fetched from its lane; the ballot of iteration i is
-     * kept by lane i mod 64 (bal0 for i < 64, bal1 above), so candidate c's
-     * reach bits are bits [c qcnt, c qcnt + qcnt) of the ballots in order */
-    int work = np * qcnt;
-    /* cand = q / qcnt exactly: q * qcnt < 64 * 96 * 96 < 2^20 */
-    unsigned qinv = qcnt > 0 ? ((1u << 20) + (unsigned) qcnt - 1u) / (unsigned) qcnt : 0u;
-    unsigned long long bal0 = 0ull, bal1 = 0ull;
-    for (int base = 0, it = 0; base < work; base += 64, it++)
+    /* which block spheres' own AABB reaches each candidate's cell
+     * (condet.ispc:104-113), tested in FP32 with every bound rounded
+     * outward: a mask holds every sphere that the FP64 test accepts and, at
+     * a bound within FP32 rounding, rarely one more (its hits are confirmed
+     * against the FP64 cell below).  The candidates' cells pass through the
+     * wave's staging area (free until the candidates are processed) and are
+     * read as LDS broadcasts; lane s tests block spheres s and s + 64 */
+    float *cf = (float*) rec32;   /* 64 cells of 6 floats: lo x, y, z, hi x, y, z */
+    __builtin_amdgcn_wave_barrier ();
+    if (lane < np)
     {
-      int q = base + lane;
-      bool ok = q < work;
-      int cand = ok ? (int) (((unsigned) q * qinv) >> 20) : 0;
-      int sI = q - cand * qcnt;
-      double lo[3], hi[3];
 #pragma unroll
-      for (int d = 0; d < 3; d++) { lo[d] = __shfl (Ll[d], cand, 64); hi[d] = __shfl (Lu[d], cand, 64); }
-      bool in = false;
-      if (ok)
+      for (int d = 0; d < 3; d++)
       {
-        double x = ssph[4*sI], y = ssph[4*sI+1], z = ssph[4*sI+2], r = ssph[4*sI+3];
-        in = x-r <= hi[0] && y-r <= hi[1] && z-r <= hi[2] && x+r >= lo[0] && y+r >= lo[1] && z+r >= lo[2];
+        cf[6*lane+d] = f32_below (Ll[d]);
+        cf[6*lane+3+d] = f32_above (Lu[d]);
       }
-      unsigned long long bm = __ballot (in);
-      if (lane == (it & 63)) { if (it < 64) bal0 = bm; else bal1 = bm; }
     }
-    unsigned long long bm0 = 0, bm1 = 0;
+    __builtin_amdgcn_wave_barrier ();
+    /* sphere AABB bounds as the FP64 test forms them (c -/+ r), rounded
+     * outward; NaN (no comparison holds) past the block's spheres */
+    float slo0[3], shi0[3], slo1[3], shi1[3];
+#pragma unroll
+    for (int k = 0; k < 2; k++)
     {
-      int st = lane * qcnt, w0 = st >> 6, off = st & 63;   /* lane < np: its candidate's first bit */
-      unsigned long long a[3];
+      int s = lane + 64*k;
+      float *lo = k ? slo1 : slo0, *hi = k ? shi1 : shi0;
 #pragma unroll
-      for (int k = 0; k < 3; k++)
+      for (int d = 0; d < 3; d++)
       {
-        int wi = w0 + k;
-        unsigned long long v0 = __shfl (bal0, wi & 63, 64), v1 = __shfl (bal1, wi & 63, 64);
-        a[k] = wi < 64 ? v0 : (wi < 128 ? v1 : 0ull);
+        lo[d] = __builtin_nanf ("");
+        hi[d] = __builtin_nanf ("");
       }
-      unsigned long long m0 = off ? (a[0] >> off) | (a[1] << (64 - off)) : a[0];
-      unsigned long long m1 = off ? (a[1] >> off) | (a[2] << (64 - off)) : a[1];
-      if (qcnt < 64) { m0 &= (1ull << qcnt) - 1ull; m1 = 0ull; }
-      else m1 &= qcnt == 64 ? 0ull : (1ull << (qcnt - 64)) - 1ull;
-      if (lane < np && bcnt > 0) { bm0 = m0; bm1 = m1; }
+      if (s < qcnt)
+      {
+        double r = ssph[4*s+3];
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+        {
+          lo[d] = f32_below (ssph[4*s+d] - r);
+          hi[d] = f32_above (ssph[4*s+d] + r);
+        }
+      }
     }
+    /* the candidate count of the wave, uniform: the loops below run with
+     * the full wave and keep their counters scalar */
+    int npu = __builtin_amdgcn_readfirstlane (np);
+    unsigned long long bm0 = 0ull, bm1 = 0ull;
+    if (qcnt <= 64)
+      for (int c = 0; c < npu; c++)
+      {
+        const float2 *cp = (const float2*) (cf + 6*c);
+        unsigned long long r0 = __ballot (reach32 (slo0, shi0, cp[0], cp[1], cp[2]));
+        bm0 = lane == c ? r0 : bm0;
+      }
+    else
+      for (int c = 0; c < npu; c++)
+      {
+        const float2 *cp = (const float2*) (cf + 6*c);
+        float2 a = cp[0], b = cp[1], e = cp[2];
+        unsigned long long r0 = __ballot (reach32 (slo0, shi0, a, b, e));
+        unsigned long long r1 = __ballot (reach32 (slo1, shi1, a, b, e));
+        bm0 = lane == c ? r0 : bm0;
+        bm1 = lane == c ? r1 : bm1;
+      }
+    if (!(lane < np && bcnt > 0)) { bm0 = 0ull; bm1 = 0ull; }
+    __builtin_amdgcn_wave_barrier ();
     /* the candidates that some block sphere reaches, in turn */
     unsigned long long todo = __ballot ((bm0 | bm1) != 0);
     while (todo)
@@ -455,12 +510,17 @@ void k_detect (int E, const double *center, const double *radii, const int *part
          * once: pairs (a, (a+k) mod n), k = 1..(n-1)/2, plus k = n/2, a < n/2 */
         int n = cnt, halfn = (n - 1) / 2;
         int work = n * halfn + ((n & 1) ? 0 : n / 2);
-        unsigned inv = ((1u << 20) + (unsigned) n - 1u) / (unsigned) n;
+        /* pair q = kk n + a, carried from pass to pass: q advances by
+         * 64 = D n + R, so a by R and kk by D, with one wrap at most */
+        int D = 64 / n, R = 64 - D * n;
+        int kk = lane / n, an = lane - kk * n;
         for (int q = lane; q < work; q += 64)
         {
-          int kk = (int) (((unsigned) q * inv) >> 20), a = q - kk * n;
-          int bb = a + kk + 1;
+          int a = an, bb = a + kk + 1;
           bb = bb >= n ? bb - n : bb;
+          an += R;
+          kk += D;
+          if (an >= n) { an -= n; kk++; }
           /* rx = max (r0, r1, r2) is r0 for a sphere (r1, r2 < 0) */
           if (!near32 (ssph32[a], ssph32[bb], k1, k2)) continue;
           if (!overlap (ssph[4*a], ssph[4*a+1], ssph[4*a+2], ssph[4*a+3],
@@ -476,17 +536,31 @@ void k_detect (int E, const double *center, const double *radii, const int *part
       else
       {
         int work = nsl * cnt;
-        /* a = q / cnt exactly for q * cnt < 2^20 (q < 96*96, cnt <= 96) */
-        unsigned inv = ((1u << 20) + (unsigned) cnt - 1u) / (unsigned) cnt;
+        /* pair q = a cnt + m (sphere a of the list, record m), carried from
+         * pass to pass like the own leaf's */
+        int D = 64 / cnt, R = 64 - D * cnt;
+        int an = lane / cnt, mn = lane - an * cnt;
         for (int q = lane; q < work; q += 64)
         {
-          int a = (int) (((unsigned) q * inv) >> 20), m = q - a * cnt;
+          int a = an, m = mn;
+          mn += R;
+          an += D;
+          if (mn >= cnt) { mn -= cnt; an++; }
           int ti = lst[a];
           if (!near32 (ssph32[ti], rec32[m], k1, k2)) continue;
           /* the partner's FP64 values (the ones its record holds) */
           int j = mem[m];
           double cx = center[j], cy = center[(size_t) E + j], cz = center[2*(size_t) E + j], rj = radii[j];
           if (!overlap (ssph[4*ti], ssph[4*ti+1], ssph[4*ti+2], ssph[4*ti+3], cx, cy, cz, rj)) continue;
+          /* the reach of sphere ti to L, confirmed in FP64 (the masks are
+           * a superset of it) */
+          {
+            double x = ssph[4*ti], y = ssph[4*ti+1], z = ssph[4*ti+2], r = ssph[4*ti+3];
+            if (!(x-r <= cell[3*(size_t) nleaves + L] && y-r <= cell[4*(size_t) nleaves + L] &&
+                  z-r <= cell[5*(size_t) nleaves + L] && x+r >= cell[L] && y+r >= cell[(size_t) nleaves + L] &&
+                  z+r >= cell[2*(size_t) nleaves + L]))
+              continue;
+          }
           if (lpart[qoff + ti] == part[j]) continue;
           int e = lmem[qoff + ti];
           int slot = atomicAdd (&shc[ti], 1);
