@@ -219,6 +219,26 @@ __device__ __forceinline__ bool reach32 (const float *lo, const float *hi, float
   return (lo[0] <= b.y) & (lo[1] <= e.x) & (lo[2] <= e.y) & (hi[0] >= a.x) & (hi[1] >= a.y) & (hi[2] >= b.x);
 }
 
+/* inv20[n] = ceil(2^20 / n), n = 1..PMX_LSIZE: for q n < 2^20 (here q <= 64,
+ * n <= 96), q / n = (q inv20[n]) >> 20 exactly; a scalar load and 24-bit
+ * multiplies in place of an integer division */
+struct Inv20
+{
+  unsigned v[PMX_LSIZE + 1];
+  constexpr Inv20 () : v ()
+  {
+    for (int n = 1; n <= PMX_LSIZE; n++) v[n] = ((1u << 20) + (unsigned) n - 1u) / (unsigned) n;
+  }
+};
+__constant__ Inv20 inv20 = Inv20 ();
+
+/* (q / n, q mod n) for 0 <= q <= 64, 1 <= n <= PMX_LSIZE */
+__device__ __forceinline__ int2 divmod_small (int q, int n)
+{
+  int d = (int) (__umul24 ((unsigned) q, inv20.v[n]) >> 20);
+  return make_int2 (d, q - (int) __umul24 ((unsigned) d, (unsigned) n));
+}
+
 /* k_detect runs NW waves per block (1, 2, 4 or 8; the block's spheres are
  * spread over its threads); NW is picked per workload by the autotuner */
 
@@ -512,8 +532,9 @@ void k_detect (int E, const double *center, const double *radii, const int *part
         int work = n * halfn + ((n & 1) ? 0 : n / 2);
         /* pair q = kk n + a, carried from pass to pass: q advances by
          * 64 = D n + R, so a by R and kk by D, with one wrap at most */
-        int D = 64 / n, R = 64 - D * n;
-        int kk = lane / n, an = lane - kk * n;
+        int2 st = divmod_small (64, n), l0 = divmod_small (lane, n);
+        int D = st.x, R = st.y;
+        int kk = l0.x, an = l0.y;
         for (int q = lane; q < work; q += 64)
         {
           int a = an, bb = a + kk + 1;
@@ -538,8 +559,9 @@ void k_detect (int E, const double *center, const double *radii, const int *part
         int work = nsl * cnt;
         /* pair q = a cnt + m (sphere a of the list, record m), carried from
          * pass to pass like the own leaf's */
-        int D = 64 / cnt, R = 64 - D * cnt;
-        int an = lane / cnt, mn = lane - an * cnt;
+        int2 st = divmod_small (64, cnt), l0 = divmod_small (lane, cnt);
+        int D = st.x, R = st.y;
+        int an = l0.x, mn = l0.y;
         for (int q = lane; q < work; q += 64)
         {
           int a = an, m = mn;
