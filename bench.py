@@ -259,10 +259,17 @@ def main():
         n = args.copy_probe << 20
         a = torch.ones(n, dtype=torch.uint8, device="cuda")
         b = torch.empty_like(a)
-        for _ in range(10):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.bitwise_not(a, out=b)
+        e0.record()
+        for _ in range(9):
             torch.bitwise_not(a, out=b)
+        e1.record()
         torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes_read": n, "copy_probe_bytes_written": n, "dispatches": 10}))
+        ms = e0.elapsed_time(e1) / 9
+        # the practical streaming rate of one elementwise kernel (read n, write n)
+        print(json.dumps({"copy_probe_bytes_read": n, "copy_probe_bytes_written": n, "dispatches": 10,
+                          "ms_per_dispatch": ms, "gb_per_s": 2.0 * n / ms / 1e6}))
         return
 
     from parmec_amd import pmx, workloads
