@@ -529,29 +529,59 @@ void k_detect (int E, const double *center, const double *radii, const int *part
          * (depth r_a + r_b - |c_a - c_b|), so each unordered pair is tested
          * once: pairs (a, (a+k) mod n), k = 1..(n-1)/2, plus k = n/2, a < n/2 */
         int n = cnt, halfn = (n - 1) / 2;
-        int work = n * halfn + ((n & 1) ? 0 : n / 2);
-        /* pair q = kk n + a, carried from pass to pass: q advances by
-         * 64 = D n + R, so a by R and kk by D, with one wrap at most */
-        int2 st = divmod_small (64, n), l0 = divmod_small (lane, n);
-        int D = st.x, R = st.y;
-        int kk = l0.x, an = l0.y;
-        for (int q = lane; q < work; q += 64)
-        {
-          int a = an, bb = a + kk + 1;
-          bb = bb >= n ? bb - n : bb;
-          an += R;
-          kk += D;
-          if (an >= n) { an -= n; kk++; }
-          /* rx = max (r0, r1, r2) is r0 for a sphere (r1, r2 < 0) */
-          if (!near32 (ssph32[a], ssph32[bb], k1, k2)) continue;
+        /* the FP64 test and the hits of a pair that passed the FP32 one */
+        auto pair = [&] (int a, int bb) {
           if (!overlap (ssph[4*a], ssph[4*a+1], ssph[4*a+2], ssph[4*a+3],
                         ssph[4*bb], ssph[4*bb+1], ssph[4*bb+2], ssph[4*bb+3]))
-            continue;
-          if (lpart[qoff + a] == lpart[qoff + bb]) continue;
+            return;
+          if (lpart[qoff + a] == lpart[qoff + bb]) return;
           int ea = lmem[qoff + a], eb = lmem[qoff + bb];
           int sa = atomicAdd (&shc[a], 1), sb = atomicAdd (&shc[bb], 1);
           if (sa < hcap) hits[(size_t) sa * E + ea] = (uint32_t) eb; else atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1);
           if (sb < hcap) hits[(size_t) sb * E + eb] = (uint32_t) ea; else atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1);
+        };
+        /* rx = max (r0, r1, r2) is r0 for a sphere (r1, r2 < 0) */
+        if (n <= 64)
+        {
+          /* lane a keeps sphere a and takes its partners (a + k) mod n two
+           * at a time (two independent LDS reads in flight), k = 1..(n-1)/2,
+           * then k = n/2 on the lanes a < n/2; idle lanes read slot 0 */
+          int kmax = halfn + ((n & 1) ? 0 : 1);
+          bool mine = lane < n;
+          float4 A = ssph32[mine ? lane : 0];
+          int bb = mine ? lane : 0;
+          for (int k = 1; k <= kmax; k += 2)
+          {
+            int b1 = bb + 1 == n ? 0 : bb + 1;
+            int b2 = b1 + 1 == n ? 0 : b1 + 1;
+            bb = b2;
+            bool t1 = lane < (k <= halfn ? n : n / 2);
+            bool t2 = k < kmax && lane < (k + 1 <= halfn ? n : n / 2);
+            float4 B1 = ssph32[b1], B2 = ssph32[b2];
+            bool p1 = t1 & near32 (A, B1, k1, k2);
+            bool p2 = t2 & near32 (A, B2, k1, k2);
+            if (p1) pair (lane, b1);
+            if (p2) pair (lane, b2);
+          }
+        }
+        else
+        {
+          int work = n * halfn + ((n & 1) ? 0 : n / 2);
+          /* pair q = kk n + a, carried from pass to pass: q advances by
+           * 64 = D n + R, so a by R and kk by D, with one wrap at most */
+          int2 st = divmod_small (64, n), l0 = divmod_small (lane, n);
+          int D = st.x, R = st.y;
+          int kk = l0.x, an = l0.y;
+          for (int q = lane; q < work; q += 64)
+          {
+            int a = an, bb = a + kk + 1;
+            bb = bb >= n ? bb - n : bb;
+            an += R;
+            kk += D;
+            if (an >= n) { an -= n; kk++; }
+            if (!near32 (ssph32[a], ssph32[bb], k1, k2)) continue;
+            pair (a, bb);
+          }
         }
       }
       else
