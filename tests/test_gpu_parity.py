@@ -149,6 +149,24 @@ def test_leaf_overflow_repartition(engines):
     compare(mg, cg, mc, cc)
 
 
+def test_full_leaves_with_contacts(engines):
+    """Leaves between 65 and 96 spheres while contacts form: a lattice at
+    2.2 radii spacing contracts by 15 % over 100 steps (density x 1.6), so
+    built leaves of up to 64 fill past 64 before they overflow, and
+    neighbours touch from step ~61.  Pins the detection paths taken by
+    leaves of more than 64 spheres (second reach-mask slot, the generic
+    own-leaf pair loop) against the oracle."""
+    print("case converging flow with contacts (leaves over 64)", flush=True)
+    m = workloads.box_lattice(n=4000, seed=5)
+    xc = m.center[:3].mean(axis=1, keepdims=True)
+    m.linear[:] = -150.0 * (m.center[:3] - xc)
+    (mg, cg, sg), (mc, cc, sc) = _both(engines, m, 100 * 1e-5 - 0.5e-5, 1e-5, pmx.forcing())
+    assert len(cc) > 1000
+    assert sc["rebuilds"] >= 2   # the first build, then at least one leaf past 96
+    assert sg["rebuilds"] == sc["rebuilds"]
+    compare(mg, cg, mc, cc)
+
+
 def test_csr_neighbour_lists(engines, monkeypatch):
     """Neighbour lists in the CSR layout (used when the padded layout would
     not fit) give the oracle's result as the padded layout does."""
