@@ -223,6 +223,8 @@ typedef struct pmx_stats
                                   instances of a batch (counted in rebuilds too) */
   int graph_timed;         /* profiling mode 1: the last call's kernel timings came from event
                               record nodes of the replayed step graph (1) or from eager steps (0) */
+  long long grown;         /* steps of the last call repeated after a contact capacity grew */
+  int row_capacity;        /* contacts per sphere row now (grows as contacts need) */
 } pmx_stats;
 
 /* per-kernel timing collected with HIP events on the context's stream */

@@ -490,15 +490,107 @@ static int rebuild (pmx_ctx *c, bool all)
   return PMX_OK;
 }
 
-/* the error flags of a group of steps */
+/* the error flags of a group of steps (capacity overflows are not errors:
+ * they abort their step, which is repeated with larger buffers) */
 static int check_flags (pmx_ctx *c, const int *fl)
 {
-  if (fl[FLAG_ROW_OVERFLOW] || fl[FLAG_HIT_OVERFLOW] || fl[FLAG_EXTRA_OVERFLOW] || fl[FLAG_TRILIST_OVERFLOW])
-    return fail (PMX_ECAPACITY, "contact capacity exceeded (row " + std::to_string (fl[FLAG_ROW_OVERFLOW]) + ", hits " +
-                 std::to_string (fl[FLAG_HIT_OVERFLOW]) + ", reverse " + std::to_string (fl[FLAG_EXTRA_OVERFLOW]) +
-                 "); raise pmx_options.row_capacity");
+  if (fl[FLAG_TRILIST_OVERFLOW])
+    return fail (PMX_ECAPACITY, "partitioning tree deeper than the traversal stack (64 pending nodes)");
   if (fl[FLAG_BAD_KIND]) return fail (PMX_EUNSUPPORTED, "unsupported interaction kind or inconsistent contact rows");
   c->stats.max_row = std::max (c->stats.max_row, fl[FLAG_MAXROW]);
+  return PMX_OK;
+}
+
+/* largest per-sphere row, hit, reverse-hit and per-leaf moving triangle
+ * capacities: far above any packing of spheres (a sphere can touch at most
+ * a few hundred others of a quarter of its radius) */
+#define PMX_CAP_LIMIT 4096
+
+/* a step aborted by a contact capacity overflow: the buffers that
+ * overflowed grow (doubled, or to the longest row the step formed) and the
+ * step is repeated.  The reference's contact lists grow without bound
+ * (CONBUF chunks appended by newcon, condet.ispc:52-81); here the rows are
+ * dense, so they are re-laid out at the new capacity.  The rows of the
+ * previous step (and those of instances frozen at other parities) are in
+ * both ping-pong buffers and are kept; hits, reverse hits and per-step
+ * lists are recomputed by the repeated step, their counts reset here to the
+ * zeros every step starts from. */
+static int grow_capacity (pmx_ctx *c, const int *fl)
+{
+  auto pow2 = [] (int v) { int p = 1; while (p < v) p <<= 1; return p; };
+  size_t E = std::max (c->E, 1);
+  int cap2 = c->cap, hcap2 = c->hcap, xcap2 = c->xcap, mcap2 = c->mcap;
+  if (fl[FLAG_ROW_OVERFLOW]) cap2 = std::max (2 * c->cap, pow2 (fl[FLAG_MAXROW]));
+  if (fl[FLAG_HIT_OVERFLOW]) hcap2 = 2 * c->hcap;
+  if (fl[FLAG_EXTRA_OVERFLOW]) xcap2 = 2 * c->xcap;
+  if (fl[FLAG_MTRI_OVERFLOW]) mcap2 = 2 * c->mcap;
+  if (cap2 > PMX_CAP_LIMIT || hcap2 > PMX_CAP_LIMIT || xcap2 > PMX_CAP_LIMIT || mcap2 > PMX_CAP_LIMIT)
+    return fail (PMX_ECAPACITY, "contact capacity beyond " + std::to_string (PMX_CAP_LIMIT) + " per sphere (row " +
+                 std::to_string (fl[FLAG_MAXROW]) + ")");
+  if (c->opt.verbose)
+    fprintf (stderr, "pmx: contact capacity grows: rows %d -> %d, hits %d -> %d, reverse %d -> %d, obstacle lists %d -> %d\n",
+             c->cap, cap2, c->hcap, hcap2, c->xcap, xcap2, c->mcap, mcap2);
+  PMX_HIP (hipStreamSynchronize (c->stream));
+  if (cap2 != c->cap)
+  {
+    bool mixed = c->opt.precision == PMX_MIXED;
+    size_t esz = mixed ? sizeof (float) : sizeof (double);
+    for (int b = 0; b < 2; b++)
+    {
+      dvec<uint32_t> key;
+      if (key.alloc ((size_t) cap2 * E)) return fail (PMX_ENOMEM, "out of device memory (contact rows)");
+      PMX_HIP (hipMemcpyAsync (key.p, c->rowkey[b].p, sizeof (uint32_t) * c->cap * E, hipMemcpyDeviceToDevice, c->stream));
+      void *old = mixed ? (void*) c->rowst32[b].p : (void*) c->rowst[b].p;
+      void *nst = nullptr;
+      if (hipMalloc (&nst, esz * 3 * cap2 * E) != hipSuccess) return fail (PMX_ENOMEM, "out of device memory (contact rows)");
+      for (int k = 0; k < 3; k++)
+        PMX_HIP (hipMemcpyAsync ((char*) nst + esz * k * cap2 * E, (const char*) old + esz * k * c->cap * E,
+                                 esz * c->cap * E, hipMemcpyDeviceToDevice, c->stream));
+      PMX_HIP (hipStreamSynchronize (c->stream));
+      std::swap (c->rowkey[b].p, key.p);
+      std::swap (c->rowkey[b].n, key.n);
+      if (mixed) { c->rowst32[b].release (); c->rowst32[b].p = (float*) nst; c->rowst32[b].n = 3 * cap2 * E; }
+      else { c->rowst[b].release (); c->rowst[b].p = (double*) nst; c->rowst[b].n = 3 * cap2 * E; }
+    }
+    c->cap = cap2;
+  }
+  if (hcap2 != c->hcap)
+  {
+    c->hits.release ();
+    if (c->hits.alloc ((size_t) hcap2 * E)) return fail (PMX_ENOMEM, "out of device memory (hits)");
+    c->hcap = hcap2;
+  }
+  if (xcap2 != c->xcap)
+  {
+    c->xbuf.release ();
+    if (c->xbuf.alloc ((size_t) xcap2 * E)) return fail (PMX_ENOMEM, "out of device memory (reverse hits)");
+    c->xcap = xcap2;
+  }
+  if (mcap2 != c->mcap)
+  {
+    c->mcap = mcap2;
+    c->mtlist.release ();
+    if (c->nmtri > 0 && c->mtlist.alloc ((size_t) std::max (c->tree.nleaves, 1) * mcap2))
+      return fail (PMX_ENOMEM, "out of device memory (obstacle lists)");
+  }
+  /* every step starts from zero hit, reverse-hit and leaf counts */
+  PMX_HIP (hipMemsetAsync (c->hitcnt.p, 0, sizeof (int) * E, c->stream));
+  PMX_HIP (hipMemsetAsync (c->xcnt.p, 0, sizeof (int) * E, c->stream));
+  int rc = clear_leaf_counts (*c);
+  if (rc) return rc;
+  int zero = 0;
+  PMX_HIP (hipMemcpyAsync (c->ctl.p + CTL_ABORT, &zero, sizeof (int), hipMemcpyHostToDevice, c->stream));
+  PMX_HIP (hipStreamSynchronize (c->stream));
+  c->graph_valid = false;
+  return PMX_OK;
+}
+
+/* the overflow flags of an aborted step, handled: back to zero */
+static int clear_overflow_flags (pmx_ctx *c)
+{
+  const int which[] = {FLAG_LEAF_OVERFLOW, FLAG_ROW_OVERFLOW, FLAG_HIT_OVERFLOW, FLAG_EXTRA_OVERFLOW, FLAG_MTRI_OVERFLOW};
+  for (int f : which) PMX_HIP (hipMemsetAsync (c->dflags.p + f, 0, sizeof (int), c->stream));
+  PMX_HIP (hipStreamSynchronize (c->stream));
   return PMX_OK;
 }
 
@@ -772,6 +864,7 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
     z[CTL_KSTEP] = kstep;
     PMX_HIP (hipMemcpyAsync (c->ctl.p, z, sizeof (z), hipMemcpyHostToDevice, c->stream));
     PMX_HIP (hipStreamSynchronize (c->stream));
+    if ((r = clear_overflow_flags (c))) return r;   /* the timing runs' detections are discarded */
     c->graph_valid = false;
     c->stats.det_waves = c->det_nw;
     c->stats.rows_block = c->rows_tpb;
@@ -830,7 +923,9 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
     { c->hring = nullptr; return fail (PMX_ENOMEM, "out of pinned host memory"); }
     for (int k = 0; k < PMX_RING; k++) PMX_HIP (hipEventCreateWithFlags (&c->rev[k], hipEventDisableTiming));
   }
-  int done = 0, failed_at = -1;
+  int done = 0;
+  long long grown = 0;
+  std::vector<int> rebuilt_at (I, -1);   /* the last step each instance was re-partitioned for */
   while (done < nsteps)
   {
     int next = nsteps;
@@ -888,14 +983,33 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
     done = ctl[CTL_KSTEP];
     if (abort)
     {
-      /* a leaf overflowed in step `done`: re-partition the overflowing
-       * instances from the current centres and repeat that step
-       * (parmec.cpp:2420-2427) */
-      if (failed_at == done) return fail (PMX_EREPART, "Repartitioning failed");
-      failed_at = done;
-      if ((rc = rebuild (c, false))) return rc;
-      rebuilds++;
-      if (c->tree.last_partial) partial++;
+      /* step `done` was aborted, before it changed any state.  A contact
+       * capacity overflow: larger buffers, then the step again.  A leaf
+       * overflow: re-partition the overflowing instances from the current
+       * centres and repeat the step (parmec.cpp:2420-2427); "Repartitioning
+       * failed" when an instance already re-partitioned for this step
+       * overflows again, as it would alone (every instance that overflows
+       * in a step is flagged by that step's binning) */
+      if (fl[FLAG_ROW_OVERFLOW] || fl[FLAG_HIT_OVERFLOW] || fl[FLAG_EXTRA_OVERFLOW] || fl[FLAG_MTRI_OVERFLOW])
+      {
+        if ((rc = grow_capacity (c, fl))) return rc;
+        grown++;
+      }
+      if (fl[FLAG_LEAF_OVERFLOW])
+      {
+        std::vector<int> ov (I, 0);
+        PMX_HIP (hipMemcpyAsync (ov.data (), c->ovf.p, sizeof (int) * I, hipMemcpyDeviceToHost, c->stream));
+        PMX_HIP (hipStreamSynchronize (c->stream));
+        for (int i = 0; i < I; i++)
+          if (ov[i] && rebuilt_at[i] == done)
+            return fail (PMX_EREPART, "Repartitioning failed (instance " + std::to_string (i) + ", step " +
+                         std::to_string (done) + ")");
+        for (int i = 0; i < I; i++) if (ov[i]) rebuilt_at[i] = done;
+        if ((rc = rebuild (c, false))) return rc;
+        rebuilds++;
+        if (c->tree.last_partial) partial++;
+      }
+      if ((rc = clear_overflow_flags (c))) return rc;
     }
     if ((rc = report (done))) return rc;
     if (tuning && c->tuned == 1 && !abort && done < nsteps && c->stepped + done >= PMX_RETUNE_STEPS &&
@@ -917,6 +1031,8 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
   c->stats.steps = nsteps;
   c->stats.rebuilds = rebuilds;
   c->stats.partial_rebuilds = partial;
+  c->stats.grown = grown;
+  c->stats.row_capacity = c->cap;
   c->stats.graph_timed = c->prof.mode == 1 && !c->prof.eager;
   c->stats.leaves = c->tree.nleaves;
   c->stats.leaf_slots = c->lstride;

@@ -23,15 +23,30 @@
 #define PMX_TRI_FLAG 0x80000000u  /* row key flag of a triangle partner */
 #define PMX_NO_ROOT INT_MIN       /* tree root of an instance without spheres */
 
-/* device control block of a dem call: abort flag (leaf overflow in the
- * current step; every later kernel of the step is a no-op), completed steps
- * (indexes the per-step tables and the contact-row ping-pong), bin hint
- * validity (leafof holds leaves of the current tree) */
-enum { CTL_ABORT = 0, CTL_KSTEP, CTL_HINT, CTL_COUNT = 8 };
+/* device control block of a dem call: abort word (a leaf or contact
+ * capacity overflow in the current step; every later kernel of the step and
+ * of the steps enqueued behind it is a no-op; it holds the aborted step's
+ * sequence number + 1), completed steps (indexes the per-step tables and the
+ * contact-row ping-pong), bin hint validity (leafof holds leaves of the
+ * current tree), step sequence number (counts every enqueued step, aborted
+ * or not: k_bin tells an abort raised by its own launch, where it keeps
+ * binning so that every overflowing instance is flagged, from one raised by
+ * an earlier step) */
+enum { CTL_ABORT = 0, CTL_KSTEP, CTL_HINT, CTL_SEQ, CTL_COUNT = 8 };
 
-/* device error/overflow flags, one int each */
+/* device error/overflow flags, one int each.  Leaf, row, hit, reverse-hit
+ * and moving-triangle list overflows abort the step (the host re-partitions
+ * or grows the capacity and repeats it); a traversal stack overflow in a
+ * tree build (FLAG_TRILIST_OVERFLOW) is an error */
 enum { FLAG_LEAF_OVERFLOW = 0, FLAG_ROW_OVERFLOW, FLAG_HIT_OVERFLOW, FLAG_EXTRA_OVERFLOW,
-       FLAG_TRILIST_OVERFLOW, FLAG_MAXROW, FLAG_BAD_KIND, FLAG_COUNT };
+       FLAG_TRILIST_OVERFLOW, FLAG_MAXROW, FLAG_BAD_KIND, FLAG_MTRI_OVERFLOW, FLAG_COUNT };
+
+/* abort the current step: the step is repeated after the host has grown the
+ * capacity that overflowed (or re-partitioned), from unchanged state */
+__device__ __forceinline__ void abort_step (int *ctl)
+{
+  atomicExch (&ctl[CTL_ABORT], ctl[CTL_SEQ] + 1);
+}
 
 void pmx_set_error (const std::string &msg);
 
@@ -350,6 +365,7 @@ int scan_exclusive_to (pmx_ctx &c, const int *src, int *dst, int n, const int *c
 
 /* ---- step kernels (pmx_step.hip); step_* only enqueue work ---- */
 int step_prepare (pmx_ctx &c);               /* per-tree buffers, after a (re)build */
+int clear_leaf_counts (pmx_ctx &c);          /* zero the per-leaf counts of the binning */
 int step_bin (pmx_ctx &c);
 int step_detect (pmx_ctx &c);
 int step_scatter (pmx_ctx &c);

@@ -36,23 +36,19 @@ namespace {
 
 inline dim3 grid (int n) { return dim3 ((n + PMX_TPB - 1) / PMX_TPB); }
 
-__global__ void k_fill_i (const int *ctl, int *a, int n, int v)
-{
-  if (ctl[CTL_ABORT]) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) a[i] = v;
-}
-
 __global__ void k_zero_i (int *a, int n)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = 0;
 }
 
-/* the step completed: advance the step counter */
+/* the step completed: advance the step counter (unless it was aborted) and
+ * the sequence number of enqueued steps */
 __global__ void k_step_end (int *ctl)
 {
-  if (threadIdx.x == 0 && !ctl[CTL_ABORT]) ctl[CTL_KSTEP] += 1;
+  if (threadIdx.x != 0) return;
+  if (!ctl[CTL_ABORT]) ctl[CTL_KSTEP] += 1;
+  ctl[CTL_SEQ] += 1;
 }
 
 /* partitioning_store: descend to the leaf, take a slot (partition.ispc:600-646) */
@@ -65,7 +61,12 @@ __global__ void k_bin (int E, const double *center, const int *roots, const int 
                        int *leafof, int *slot, int *lcount, int *flags, int *ovf, const int *stop, int lstride,
                        const double *radii, const int *part, int *lmem, double *lpos, int *lpart, int *xcnt)
 {
-  if (ctl[CTL_ABORT]) return;
+  /* an abort of an earlier step: nothing to do.  An abort raised by this
+   * launch (another sphere's leaf overflowed) does not stop the binning:
+   * every instance with an overflowing leaf must be flagged, so that one
+   * re-partition serves them all, as each would re-partition alone */
+  int seq = ctl[CTL_SEQ], ab = ctl[CTL_ABORT];
+  if (ab && ab != seq + 1) return;
   int hint = ctl[CTL_HINT];
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   bool valid = e < E && !frozen (stop, ell_inst[e < E ? e : 0], ctl);
@@ -124,7 +125,7 @@ __global__ void k_bin (int E, const double *center, const int *roots, const int 
   {
     atomicAdd (&flags[FLAG_LEAF_OVERFLOW], 1);
     atomicExch (&ovf[ell_inst[e]], 1);   /* the instance whose tree is rebuilt */
-    atomicExch (&ctl[CTL_ABORT], 1);     /* re-partition before this step goes on */
+    atomicExch (&ctl[CTL_ABORT], seq + 1);   /* re-partition before this step goes on */
     return;
   }
   if (!lstride) { slot[e] = s; return; }
@@ -537,8 +538,8 @@ void k_detect (int E, const double *center, const double *radii, const int *part
           if (lpart[qoff + a] == lpart[qoff + bb]) return;
           int ea = lmem[qoff + a], eb = lmem[qoff + bb];
           int sa = atomicAdd (&shc[a], 1), sb = atomicAdd (&shc[bb], 1);
-          if (sa < hcap) hits[(size_t) sa * E + ea] = (uint32_t) eb; else atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1);
-          if (sb < hcap) hits[(size_t) sb * E + eb] = (uint32_t) ea; else atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1);
+          if (sa < hcap) hits[(size_t) sa * E + ea] = (uint32_t) eb; else { atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1); abort_step (ctl); }
+          if (sb < hcap) hits[(size_t) sb * E + eb] = (uint32_t) ea; else { atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1); abort_step (ctl); }
         };
         /* rx = max (r0, r1, r2) is r0 for a sphere (r1, r2 < 0) */
         if (n <= 64)
@@ -617,7 +618,7 @@ void k_detect (int E, const double *center, const double *radii, const int *part
           int e = lmem[qoff + ti];
           int slot = atomicAdd (&shc[ti], 1);
           if (slot < hcap) hits[(size_t) slot * E + e] = (uint32_t) j;
-          else atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1);
+          else { atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1); abort_step (ctl); }
           /* does the partner's AABB reach this sphere's leaf Q? */
           bool back = cx-rj <= cu[0] && cy-rj <= cu[1] && cz-rj <= cu[2] &&
                       cx+rj >= cl[0] && cy+rj >= cl[1] && cz+rj >= cl[2];
@@ -625,7 +626,7 @@ void k_detect (int E, const double *center, const double *radii, const int *part
           {
             int sx = atomicAdd (&xcnt[j], 1);
             if (sx < xcap) xbuf[(size_t) sx * E + j] = e;
-            else atomicAdd (&flags[FLAG_EXTRA_OVERFLOW], 1);
+            else { atomicAdd (&flags[FLAG_EXTRA_OVERFLOW], 1); abort_step (ctl); }
           }
         }
       }
@@ -685,7 +686,7 @@ void k_detect (int E, const double *center, const double *radii, const int *part
         if (tt < 0) tt = m < nsq ? trilist[toff + m] : mtlist[(size_t) Q*mcap + (m - nsq)];
         int slot = atomicAdd (&shc[ti], 1);
         if (slot < hcap) hits[(size_t) slot * E + lmem[qoff + ti]] = PMX_TRI_FLAG | (uint32_t) tt;
-        else atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1);
+        else { atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1); abort_step (ctl); }
       }
     }
   }
@@ -783,7 +784,7 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
              int ninst, const int *ell_inst, const int *pair_off,
              uint32_t *key0, R *st0, int *cnt0, uint32_t *key1, R *st1, int *cnt1, int cur0, int cap,
              uint32_t *hits, int *hitcnt, int hcap, const int *xbuf, int *xcnt, int xcap,
-             const double *gtab, const int *ctl, double h, double *force, double *torque, int *flags,
+             const double *gtab, int *ctl, double h, double *force, double *torque, int *flags,
              const double *obspnt, const double *obsang, const double *obslin,
              const int *busy, const int *nbusy, int list, int slices)
 {
@@ -819,7 +820,7 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
   uint32_t *H = small ? sh + threadIdx.x : hits + e;
   size_t hs = small ? (size_t) TPB : (size_t) E;   /* stride of H */
   if (small) for (int q = 0; q < nh; q++) H[q*hs] = hits[(size_t) q * E + e];
-  else if (nh + nx > hcap) { atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1); nx = hcap - nh; }
+  else if (nh + nx > hcap) { atomicAdd (&flags[FLAG_HIT_OVERFLOW], 1); abort_step (ctl); nx = hcap - nh; }
   for (int q = 0; q < nx; q++) H[(nh + q)*hs] = (uint32_t) xbuf[(size_t) q * E + e];
   nh += nx;
   for (int a = 1; a < nh; a++)   /* insertion sort of the hit keys */
@@ -996,7 +997,7 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
         nst[(size_t) (2*cap + cnt) * E + e] = st[2];
       }
     }
-    else atomicAdd (&flags[FLAG_ROW_OVERFLOW], 1);
+    else { atomicAdd (&flags[FLAG_ROW_OVERFLOW], 1); abort_step (ctl); }
     cnt++;
 
     /* contacts_acc_task, forces.ispc:492-533: slaves receive -f */
@@ -1526,7 +1527,7 @@ void rows_launch (pmx_ctx &c, R *st0, R *st1, double h, int tpb, int nb)
                       c.rowkey[0].p, st0, c.rowcnt[0].p, c.rowkey[1].p, st1, c.rowcnt[1].p,
                       c.cur0, c.cap, c.hits.p, c.hitcnt.p, c.hcap,
                       (const int*) c.xbuf.p, c.xcnt.p, c.xcap, (const double*) c.gtab.p,
-                      (const int*) c.ctl.p, h, c.force.p, c.torque.p, c.dflags.p,
+                      c.ctl.p, h, c.force.p, c.torque.p, c.dflags.p,
                       (const double*) c.obspnt.p, (const double*) c.obsang.p, (const double*) c.obslin.p,
                       (const int*) c.rbusy.p, (const int*) c.rnbusy.p, tpb, slices);
 }
@@ -1722,7 +1723,13 @@ int autotune (pmx_ctx &c, double h)
   int ctl[CTL_COUNT];
   PMX_HIP (hipMemcpyAsync (ctl, c.ctl.p, sizeof (ctl), hipMemcpyDeviceToHost, c.stream));
   PMX_HIP (hipStreamSynchronize (c.stream));
-  if (ctl[CTL_ABORT]) return clear_leaf_counts (c);   /* a leaf overflows: tune at a later call */
+  if (ctl[CTL_ABORT])
+  {
+    /* a leaf (or moving-triangle list) overflows: tune at a later call; the
+     * instances this binning flagged are not re-partitioned for it */
+    PMX_HIP (hipMemsetAsync (c.ovf.p, 0, sizeof (int) * std::max (c.I, 1), c.stream));
+    return clear_leaf_counts (c);
+  }
   hipEvent_t e0, e1;
   PMX_HIP (hipEventCreate (&e0));
   PMX_HIP (hipEventCreate (&e1));
@@ -1744,9 +1751,12 @@ int autotune (pmx_ctx &c, double h)
     return best;
   };
   const int nws[4] = {1, 2, 4, 8};
-  /* the reverse-hit counts start at zero for each detection */
+  /* the reverse-hit counts start at zero for each detection, and so does the
+   * abort word (a run whose hits overflow their capacity raises it; the
+   * counts stay bounded by the capacities, the hits are discarded) */
   auto clear = [&] () {
-    hipLaunchKernelGGL (k_fill_i, grid (c.E), dim3 (PMX_TPB), 0, c.stream, (const int*) c.ctl.p, c.xcnt.p, c.E, 0);
+    hipLaunchKernelGGL (k_zero_i, grid (c.E), dim3 (PMX_TPB), 0, c.stream, c.xcnt.p, c.E);
+    hipLaunchKernelGGL (k_zero_i, dim3 (1), dim3 (64), 0, c.stream, c.ctl.p + CTL_ABORT, 1);
     return hipGetLastError () == hipSuccess ? 0 : -1; };
   int nw = best_of (nws, 4, clear, [&] (int v) { return launch_detect (c, v); });
   (void) hipEventDestroy (e0);
@@ -1755,7 +1765,7 @@ int autotune (pmx_ctx &c, double h)
   c.det_nw = nw;
   c.tuned++;
   /* hit and reverse-hit counts of the runs back to zero */
-  hipLaunchKernelGGL (k_fill_i, grid (c.E), dim3 (PMX_TPB), 0, c.stream, (const int*) c.ctl.p, c.hitcnt.p, c.E, 0);
+  hipLaunchKernelGGL (k_zero_i, grid (c.E), dim3 (PMX_TPB), 0, c.stream, c.hitcnt.p, c.E);
   if (clear ()) { pmx_set_error ("autotuning failed"); return PMX_EDEVICE; }
   if ((rc = clear_leaf_counts (c))) return rc;   /* the next bin starts from zero */
   return PMX_OK;

@@ -603,7 +603,7 @@ __global__ void k_tri_leaf (int nleaves, const int *leafinst, const int *tri_off
  * lists of capacity mcap; run every step since the triangles move */
 __global__ void k_mtri_reach (int nm, const int *mtri, int T, const double *tri, const int *roots, const int *tri_inst,
                               const int *left, const int *right, const int *dim, const double *coord, int *mtcnt,
-                              int *mtlist, int mcap, int *flag, const int *ctl, const int *stop)
+                              int *mtlist, int mcap, int *flag, int *ctl, const int *stop)
 {
   if (ctl[CTL_ABORT]) return;
   int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -632,7 +632,7 @@ __global__ void k_mtri_reach (int nm, const int *mtri, int T, const double *tri,
       if (gl && gr)
       {
         if (sp < 64) stack[sp++] = right[ref];
-        else { atomicAdd (flag, 1); return; }
+        else { atomicAdd (flag + FLAG_TRILIST_OVERFLOW, 1); abort_step (ctl); return; }
         ref = left[ref];
       }
       else if (gl) ref = left[ref];
@@ -643,7 +643,7 @@ __global__ void k_mtri_reach (int nm, const int *mtri, int T, const double *tri,
     int L = -ref-1;
     int s = atomicAdd (&mtcnt[L], 1);
     if (s < mcap) mtlist[(size_t) L*mcap + s] = t;
-    else atomicAdd (flag, 1);
+    else { atomicAdd (flag + FLAG_MTRI_OVERFLOW, 1); abort_step (ctl); }
     if (sp == 0) break;
     ref = stack[--sp];
   }
@@ -761,7 +761,7 @@ int step_mtri (pmx_ctx &c)
   hipLaunchKernelGGL (k_mtri_reach, grid (c.nmtri), dim3 (PMX_TPB), 0, c.stream, c.nmtri, (const int*) c.mtri.p, c.T,
                       (const double*) c.tri.p, (const int*) t.roots.p, (const int*) c.tri_inst.p, (const int*) t.left.p,
                       (const int*) t.right.p, (const int*) t.dim.p, (const double*) t.coord.p, c.mtcnt.p, c.mtlist.p,
-                      c.mcap, c.dflags.p + FLAG_TRILIST_OVERFLOW, (const int*) c.ctl.p,
+                      c.mcap, c.dflags.p, c.ctl.p,
                       c.staggered ? (const int*) c.istop.p : (const int*) nullptr);
   PMX_HIP (hipGetLastError ());
   return PMX_OK;

@@ -132,3 +132,27 @@ def test_partial_rebuild_batch():
         ca = eng.contacts()
         eng.close()
         compare(ms[i], cs[i], mm, ca)
+
+
+def test_coincident_overflow_batch():
+    """Two identical converging instances of 20,000 spheres (many blocks of
+    the binning launch each) overflow their leaves in the same steps, beside
+    a neutral one.  The binning that overflows flags every overflowing
+    instance, so one re-partition per such step serves both, as each would
+    re-partition alone; every instance is bitwise its solo run."""
+    step, nsteps = 1e-5, 150
+    c = workloads.box_lattice(n=20000, seed=3)
+    xc = c.center[:3].mean(axis=1, keepdims=True)
+    c.linear[:] = -75.0 * (c.center[:3] - xc)       # converging flow: leaf overflows
+    neutral = workloads.box_lattice(n=3000, seed=4)
+    models = [c, neutral, c.copy()]
+    fs = [pmx.forcing(), pmx.forcing(gravity=(0.0, 0.0, -9.81)), pmx.forcing()]
+    duration = (nsteps - 0.5) * step
+    print("case coincident leaf overflows: 2 x 20000 converging + 3000 at rest", flush=True)
+    ms, cs, st = _batch(models, duration, step, fs)
+    alone = [_alone(m, duration, step, fs[i]) for i, m in enumerate(models)]
+    print("rebuilds batch", st["rebuilds"], "alone", [a[2]["rebuilds"] for a in alone], flush=True)
+    assert alone[0][2]["rebuilds"] > 2 and alone[1][2]["rebuilds"] == 1
+    assert st["rebuilds"] == alone[0][2]["rebuilds"]
+    for i in range(3):
+        compare(ms[i], cs[i], alone[i][0], alone[i][1])
