@@ -5,14 +5,19 @@ independent instances of the 10,000-sphere box set-up (sphere count uniform in
 [1k, 64k], base excitation g(t) = (A sin 2 pi f t, 0, -9.81) with A in
 [0.1, 2] g and f in [1, 20] Hz, seed = instance number), dt = 1e-5 s, stepped
 together in one batch (one fused launch per stage, the time step replayed as a
-HIP graph), FP32 contact law / FP64 integration.  The timed window is steps
-[warmup, warmup + steps) of the sweep's 10,000-step run, with the state and
-all inputs resident in HBM.
+HIP graph), FP32 contact law / FP64 integration.  The synthetic input is the
+sweep's state after --settle steps (default 3000, run on the GPU before
+timing: the spheres have fallen into contact); then --warmup untimed steps,
+then --steps timed steps, with the state and all inputs resident in HBM.
 
 Beside it, in the same JSON line:
   plain_fp64  -- the drop-in FP64 path stepping sweep instances one at a time
                  (a sample of instances over the same window), the
                  comparison the batched mixed form is measured against;
+  dense       -- the same 512 instance sizes as confined dense packs (every
+                 sphere pressed against its six neighbours: ~3 contact pairs
+                 per sphere, the regime of a settled pack), so that the
+                 contact stage is timed where it matters;
   hopper      -- 2,000,000 polydisperse spheres in a 5,000-triangle hopper,
                  FP64 and mixed, dt = 2e-6 s;
   serving     -- a Poisson stream (20 requests/s) of independent simulations
@@ -22,11 +27,15 @@ Beside it, in the same JSON line:
                  particle-steps/s, batched mixed beside the plain FP64 path
                  serving one request at a time;
   roofline    -- the dominant kernel of the timed window: algorithmic bytes
-                 per launch over its mean duration from HIP events around
-                 every launch on the library's stream (the window runs its
-                 steps eagerly for this, about 1% slower than graph replay);
+                 per launch (the bytes its branches read and write) over its
+                 mean duration from HIP event record nodes around every 8th
+                 step's kernels on the library's stream;
+  step_roofline -- the whole step: particle-steps/s x the compulsory bytes of
+                 a fused step per particle (384 + 40 P mixed, 384 + 64 P FP64,
+                 P = contact pairs per particle) over 8 TB/s;
   cpu_baseline -- the C++/OpenMP restatement of the reference step (the
-                 oracle) on a bounded sample of the same sweep state.
+                 oracle) on a bounded sample of the same sweep state, on all
+                 of the job's host cores and on one core.
 
 One step = one DEM time step (bin, detect, contact forces, integration,
 geometry update) over every instance.
@@ -84,10 +93,10 @@ def kernel_table(pmx, eng):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_sweep512_mixed.json")
 
 
-def measured_traffic(kernel, instances, spheres, precision):
+def measured_traffic(kernel, instances, spheres, precision, window):
     """HBM bytes per launch of a step kernel from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE passes (scaled by the copy-probe factors), when
-    they were taken on this workload; None otherwise."""
+    they were taken on this workload and this timed window; None otherwise."""
     try:
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
@@ -95,6 +104,9 @@ def measured_traffic(kernel, instances, spheres, precision):
         return None, None
     if (t.get("instances"), t.get("spheres"), t.get("precision")) != (instances, spheres, precision):
         return None, None
+    if list(t.get("window", [])) != list(window):
+        return None, (f"{os.path.relpath(TRAFFIC_FILE, ROOT)} holds steps {t.get('window')}, not the timed "
+                      f"window {list(window)}")
     k = t["kernels"].get(kernel)
     if not k:
         return None, None
@@ -104,18 +116,20 @@ def measured_traffic(kernel, instances, spheres, precision):
     return b, src
 
 
-def roofline(kernels, instances, spheres, precision, graph_timed):
+def roofline(kernels, extra_bytes, instances, spheres, precision, window, graph_timed):
     """The step kernel with the largest total time (kernels with algorithmic
     bytes; the tree rebuild is reported in kernels_ms_per_launch).  Launch
     durations come from HIP event pairs on the context's stream around every
-    launch of the timed steps: event record nodes of the replayed step graph
-    (graph_timed), else eager steps."""
+    launch of the profiled steps: event record nodes of the replayed step
+    graph (graph_timed), else eager steps.  extra_bytes: bytes per launch the
+    library's count leaves out (the force and torque reads of particles
+    whose row is not empty, by kernel)."""
     dom = max((k for k in kernels if kernels[k]["bytes"] > 0), key=lambda k: kernels[k]["ms"])
     d = kernels[dom]
     avg_s = d["ms"] / d["launches"] * 1e-3
-    per_launch = d["bytes"] / d["launches"]
+    per_launch = d["bytes"] / d["launches"] + extra_bytes.get(dom, 0.0)
     achieved = per_launch / avg_s / 1e9
-    traffic, src = measured_traffic(dom, instances, spheres, precision)
+    traffic, src = measured_traffic(dom, instances, spheres, precision, window)
     return {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": src,
             "avg_launch_us": avg_s * 1e6, "algorithmic_bytes_per_launch": per_launch,
@@ -123,21 +137,42 @@ def roofline(kernels, instances, spheres, precision, graph_timed):
             "timing": "event record nodes in the replayed step graph" if graph_timed else "eager steps with event pairs"}
 
 
+def step_roofline(value, pairs_per_particle, mixed):
+    """The whole step against the HBM roof: the compulsory bytes of a fused
+    step per particle, 384 + 40 P (mixed: FP32 pair state) or 384 + 64 P."""
+    per = 384.0 + (40.0 if mixed else 64.0) * pairs_per_particle
+    gbs = value * per / 1e9
+    return {"bytes_per_particle_step": per, "pairs_per_particle": pairs_per_particle, "achieved": gbs,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
+
+
+def per_step(kernels, nsteps):
+    """Kernel time per step: every per-step kernel has one profiled scope per
+    profiled step (the library times every 8th step); rebuilds are timed
+    whenever they happen, so their total is spread over all nsteps steps."""
+    return {k: (v["ms"] / v["launches"] if k != "tree_build" else v["ms"] / nsteps) for k, v in kernels.items()}
+
+
 def sweep_forcings(pmx, workloads, insts, nsteps, dt, t0):
     return [pmx.forcing(nsteps, gtab=workloads.sweep_forcing(a, f, nsteps, dt, t0)) for (_, a, f) in insts]
 
 
-def plain_fp64_sample(pmx, workloads, insts, sample, warmup, steps, dt):
-    """The drop-in FP64 path, one instance at a time, over the same window."""
+def plain_fp64_sample(pmx, workloads, insts, sample, start, steps, dt):
+    """The drop-in FP64 path, one instance at a time, over the same window
+    (each sampled instance stepped alone from rest to the window's start)."""
+    import torch
     tot_ps, tot_s, ids = 0, 0.0, []
     for i in sample:
         m, a, f = insts[i]
         m = m.copy()
         e = pmx.Engine()
         e.upload(m)
-        e.dem(exact_duration(warmup, dt), dt, pmx.forcing(warmup, gtab=workloads.sweep_forcing(a, f, warmup, dt)))
-        fs = pmx.forcing(steps, gtab=workloads.sweep_forcing(a, f, steps, dt, warmup * dt))
-        import torch
+        done = 0
+        while done < start:
+            n = min(1000, start - done)
+            e.dem(exact_duration(n, dt), dt, pmx.forcing(n, gtab=workloads.sweep_forcing(a, f, n, dt, done * dt)))
+            done += n
+        fs = pmx.forcing(steps, gtab=workloads.sweep_forcing(a, f, steps, dt, start * dt))
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         e.dem(exact_duration(steps, dt), dt, fs)
@@ -149,6 +184,51 @@ def plain_fp64_sample(pmx, workloads, insts, sample, warmup, steps, dt):
         tot_s += sec
         ids.append(i)
     return tot_ps / tot_s, ids
+
+
+def dense_leg(pmx, workloads, args, dt):
+    """The sweep's 512 instance sizes as confined dense packs (about 3
+    contact pairs per sphere from the first step), batched, mixed precision:
+    the contact stage timed in the regime of a settled pack."""
+    import torch
+    insts = [workloads.dense_instance(i, args.nmin, args.nmax) for i in range(args.instances)]
+    models = [x[0] for x in insts]
+    ntot = sum(m.parnum for m in models)
+    eng = pmx.Engine(precision=pmx.PMX_MIXED)
+    eng.upload_batch(models)
+    W, K = args.dense_warmup, args.dense_steps
+    eng.dem_batch(exact_duration(W, dt), dt, sweep_forcings(pmx, workloads, insts, W, dt, 0.0))
+    busy0 = eng.stats()["busy"]
+    eng.lib.pmx_set_profiling(eng.h, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.dem_batch(exact_duration(K, dt), dt, sweep_forcings(pmx, workloads, insts, K, dt, W * dt))
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    st = eng.stats()
+    kernels = kernel_table(pmx, eng)
+    eng.close()
+    value = ntot * K / sec
+    busy = 0.5 * (busy0 + st["busy"])
+    pairs = pair_count(st, ntot)
+    log(f"dense packs: {value:.3e} particle-steps/s, {pairs:.2f} contact pairs per particle, rebuilds {st['rebuilds']}")
+    return {"value": value, "unit": "particle-steps/s", "instances": len(models), "spheres": ntot,
+            "window": [W, W + K], "ms_per_step": 1e3 * sec / K, "contact_pairs_per_particle": pairs,
+            "precision": "mixed",
+            "workload": "the sweep's instance sizes, amplitudes and frequencies as confined dense packs "
+                        "(lattice spacing 1.98 r, jitter 0.004 r, walls overlapping the outer spheres by 0.02 r)",
+            "step_roofline": step_roofline(value, pairs, True),
+            "roofline": roofline(kernels, {"dynamics": 40.0 * busy}, len(models), ntot, "dense", [W, W + K],
+                                 st["graph_timed"]),
+            "kernels_ms_per_step": per_step(kernels, K)}
+
+
+def pair_count(st, ntot):
+    """Contact pairs per particle from the row entries (sphere-sphere pairs
+    are held in both spheres' rows; triangle contacts once).  The split is
+    not known from the totals, so pairs are counted as half the entries: a
+    lower bound of P (exact without triangle contacts)."""
+    return 0.5 * st["pairs"] / ntot
 
 
 def hopper_leg(pmx, workloads, n, steps, dt):
@@ -206,7 +286,7 @@ def serve_leg(pmx, workloads, count, rate, seed):
     return out
 
 
-def cpu_leg(pmx, models, forcings_fn, ids, steps, threads):
+def cpu_leg(pmx, models, forcings_fn, ids, steps, threads, dt):
     """Oracle on a bounded sample: the sample instances' current states,
     stepped on the host's cores."""
     from oracle import orc
@@ -216,32 +296,57 @@ def cpu_leg(pmx, models, forcings_fn, ids, steps, threads):
         m = models[i].copy()
         o.upload(m)
         t0 = time.perf_counter()
-        o.dem(exact_duration(steps, 1e-5), 1e-5, forcings_fn(i, steps))
+        o.dem(exact_duration(steps, dt), dt, forcings_fn(i, steps))
         tot_s += time.perf_counter() - t0
         tot_ps += m.parnum * o.stats()["steps"]
         o.close()
     return tot_ps / tot_s, tot_ps, tot_s
 
 
+def host_cpu():
+    """nproc, the CPUs this process may run on, model name and SMT state of
+    the host."""
+    model, ncpu = host_info()
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = ncpu
+    smt = None
+    try:
+        with open("/sys/devices/system/cpu/smt/active") as f:
+            smt = f.read().strip() == "1"
+    except OSError:
+        pass
+    return {"model": model, "nproc": ncpu, "allowed": allowed, "smt_active": smt}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000, help="timed steps of the sweep window")
-    ap.add_argument("--warmup", type=int, default=3000, help="sweep steps before the window (contacts form)")
+    ap.add_argument("--warmup", type=int, default=0, help="untimed steps after the settled state")
+    ap.add_argument("--settle", type=int, default=3000,
+                    help="steps run before the warmup to form the input state (the spheres fall into contact)")
     ap.add_argument("--instances", type=int, default=512)
     ap.add_argument("--nmin", type=int, default=1000)
     ap.add_argument("--nmax", type=int, default=64000)
     ap.add_argument("--dt", type=float, default=1e-5)
     ap.add_argument("--precision", choices=("mixed", "fp64"), default="mixed")
     ap.add_argument("--plain-sample", type=int, default=8, help="instances timed on the plain FP64 path")
+    ap.add_argument("--dense-steps", type=int, default=200, help="timed steps of the dense-pack leg (0: skip)")
+    ap.add_argument("--dense-warmup", type=int, default=20)
     ap.add_argument("--hopper", type=int, default=2000000, help="hopper spheres (0: skip)")
     ap.add_argument("--hopper-steps", type=int, default=200)
     ap.add_argument("--serve", type=int, default=24, help="requests of the serving stream (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--cpu-instances", type=int, default=16)
     ap.add_argument("--cpu-steps", type=int, default=200)
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="all-core leg threads (0: the CPUs this job may use, "
+                                                             "at most 16, the host share of one GPU)")
+    ap.add_argument("--cpu-single-instances", type=int, default=4, help="instances of the one-core leg")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--full-run", action="store_true",
+                    help="instead of a window: the whole 10,000-step sweep from rest, end to end")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run an elementwise kernel reading and writing MIB MiB 10 times "
                          "(calibrates the HBM byte counters) and exit")
@@ -274,7 +379,12 @@ def main():
 
     from parmec_amd import pmx, workloads
 
-    dt, W, K = args.dt, args.warmup, args.steps
+    dt = args.dt
+    if args.full_run:
+        S0, W, K = 0, 0, 10000
+    else:
+        S0, W, K = args.settle, args.warmup, args.steps
+    start = S0 + W
     t_setup = time.perf_counter()
     insts = [workloads.sweep_instance(i, args.nmin, args.nmax) for i in range(args.instances)]
     models = [x[0] for x in insts]
@@ -284,26 +394,37 @@ def main():
     prec = pmx.PMX_MIXED if args.precision == "mixed" else pmx.PMX_FP64
     eng = pmx.Engine(precision=prec)
     eng.upload_batch(models)
-    if W > 0:
-        done = 0
-        while done < W:
-            n = min(1000, W - done)
-            eng.dem_batch(exact_duration(n, dt), dt, sweep_forcings(pmx, workloads, insts, n, dt, done * dt))
-            done += n
-            log(f"warmup {done}/{W} steps")
-    fs = sweep_forcings(pmx, workloads, insts, K, dt, W * dt)
-    eng.lib.pmx_set_profiling(eng.h, 1)
+    done = 0
+    while done < start:
+        n = min(1000, start - done)
+        eng.dem_batch(exact_duration(n, dt), dt, sweep_forcings(pmx, workloads, insts, n, dt, done * dt))
+        done += n
+        log(f"{'settle' if done <= S0 else 'warmup'} {done}/{start} steps")
+    busy0 = eng.stats()["busy"] if start > 0 else 0
+    if not args.full_run:
+        eng.lib.pmx_set_profiling(eng.h, 1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    eng.dem_batch(exact_duration(K, dt), dt, fs)
+    done, rebuilds, grown = 0, 0, 0
+    while done < K:
+        # the full run in calls of 1000 steps (forcing tables of that length)
+        n = K if not args.full_run else min(1000, K - done)
+        eng.dem_batch(exact_duration(n, dt), dt, sweep_forcings(pmx, workloads, insts, n, dt, (start + done) * dt))
+        st = eng.stats()
+        assert st["steps"] == n, (st, n)
+        rebuilds += st["rebuilds"]
+        grown += st["grown"]
+        done += n
+        if args.full_run:
+            log(f"full run {done}/{K} steps, {time.perf_counter() - t0:.1f} s")
     torch.cuda.synchronize()
     sec = time.perf_counter() - t0
-    st = eng.stats()
-    assert st["steps"] == K, (st, K)
-    kernels = kernel_table(pmx, eng)
+    kernels = kernel_table(pmx, eng) if not args.full_run else {}
     eng.lib.pmx_set_profiling(eng.h, 0)
     value = ntot * K / sec
-    log(f"timed {K} steps in {sec:.3f} s: {value:.3e} particle-steps/s, pairs {st['pairs']}, rebuilds {st['rebuilds']}")
+    busy = 0.5 * (busy0 + st["busy"])
+    pairs = pair_count(st, ntot)
+    log(f"timed {K} steps in {sec:.3f} s: {value:.3e} particle-steps/s, pairs {st['pairs']}, rebuilds {rebuilds}")
 
     state = None
     if not args.no_cpu:
@@ -314,31 +435,38 @@ def main():
     stride = max(1, args.instances // max(1, args.plain_sample))
     sample = list(range(0, args.instances, stride))[: args.plain_sample]
     plain, plain_ids = None, []
-    if sample:
-        plain, plain_ids = plain_fp64_sample(pmx, workloads, insts, sample, W, min(K, 1000), dt)
+    if sample and not args.full_run:
+        plain, plain_ids = plain_fp64_sample(pmx, workloads, insts, sample, start, min(K, 1000), dt)
         log(f"plain fp64 (one instance at a time): {plain:.3e} particle-steps/s")
 
+    dense = dense_leg(pmx, workloads, args, dt) if args.dense_steps > 0 and not args.full_run else None
     hopper = hopper_leg(pmx, workloads, args.hopper, args.hopper_steps, 2e-6) if args.hopper > 0 else None
     serving = serve_leg(pmx, workloads, args.serve, args.serve_rate, 0) if args.serve > 0 else None
 
     cpu = None
     if not args.no_cpu:
-        model_name, ncpu = host_info()
-        threads = args.cpu_threads or min(16, ncpu or 1)
+        host = host_cpu()
+        threads = args.cpu_threads or max(1, min(16, host["allowed"] or 1))
         ids = list(range(0, args.instances, max(1, args.instances // args.cpu_instances)))[: args.cpu_instances]
-        t_at = (W + K) * dt
+        t_at = (start + K) * dt
 
         def ffn(i, n):
             _, a, f = insts[i]
             return pmx.forcing(n, gtab=workloads.sweep_forcing(a, f, n, dt, t_at))
-        rate, ps, s = cpu_leg(pmx, state, ffn, ids, args.cpu_steps, threads)
+        rate, ps, s = cpu_leg(pmx, state, ffn, ids, args.cpu_steps, threads, dt)
+        ids1 = ids[: args.cpu_single_instances]
+        rate1, ps1, s1 = cpu_leg(pmx, state, ffn, ids1, args.cpu_steps, 1, dt)
         cpu = {"value": rate, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+               "single_core": rate1,
+               "host": host,
+               "build": "g++ -O3 -march=x86-64-v3 -ffp-contract=off, OpenMP (oracle/Makefile)",
                "sample": f"{len(ids)} sweep instances ({ps // args.cpu_steps} spheres) at their state after step "
-                         f"{W + K}, {args.cpu_steps} steps each, C++/OpenMP restatement of the reference step "
-                         f"({s:.1f} s of CPU work) on {model_name}"}
-        log(f"cpu baseline: {rate:.3e} particle-steps/s on {threads} threads")
+                         f"{start + K}, {args.cpu_steps} steps each on {threads} threads ({s:.1f} s), and "
+                         f"{len(ids1)} of them on one thread ({s1:.1f} s); the C++/OpenMP restatement of the "
+                         f"reference step (the oracle), not the ISPC build"}
+        log(f"cpu baseline: {rate:.3e} particle-steps/s on {threads} threads, {rate1:.3e} on one")
 
-    pairs_per_particle = 2.0 * st["pairs"] / 2.0 / ntot   # entries hold sphere pairs twice
+    window = [start, start + K]
     out = {
         "metric": METRIC,
         "value": value,
@@ -356,26 +484,33 @@ def main():
             "workload": f"sweep: {len(models)} instances of the box set-up (r = 5 mm spheres on a jittered "
                         f"lattice in a 12-triangle box, kn = 1e6, zeta = 0.1, mu = 0.3), {args.nmin}-{args.nmax} "
                         f"spheres each (seeded), base excitation 0.1-2 g at 1-20 Hz, dt = {dt}, batched, "
-                        f"{args.precision}; timed steps {W}-{W + K} of the 10,000-step run",
+                        f"{args.precision}; " + ("the whole 10,000-step run from rest" if args.full_run else
+                                                 f"input = the state after {S0} steps, timed steps "
+                                                 f"{start}-{start + K} of the 10,000-step run"),
             "instances": len(models),
             "spheres": ntot,
             "dt": dt,
             "precision": args.precision,
-            "window": [W, W + K],
-            "contact_pairs_per_particle": pairs_per_particle,
-            "tree_rebuilds": st["rebuilds"],
+            "window": window,
+            "contact_pairs_per_particle": pairs,
+            "spheres_in_contact": st["busy"],
+            "tree_rebuilds": rebuilds,
+            "capacity_regrowths": grown,
             "leaves": st["leaves"],
             "block_shapes": {"detect_waves_per_block": st["det_waves"], "detect_autotuned": True,
                              "rows_threads_per_block": st["rows_block"]},
         },
         "plain_fp64": {"value": plain, "unit": "particle-steps/s", "instances_sampled": plain_ids,
-                       "how": "drop-in FP64 path, each sampled instance alone (upload, warmup, timed window)"},
+                       "how": "drop-in FP64 path, each sampled instance alone from rest, timed over the same window"},
         "speedup_vs_plain_fp64": value / plain if plain else None,
+        "step_roofline": step_roofline(value, pairs, prec == pmx.PMX_MIXED),
+        "dense": dense,
         "hopper": hopper,
         "serving": serving,
-        "roofline": roofline(kernels, len(models), ntot, args.precision, st["graph_timed"]) if kernels else None,
+        "roofline": roofline(kernels, {"dynamics": 40.0 * busy}, len(models), ntot, args.precision, window,
+                             st["graph_timed"]) if kernels else None,
         "kernels_ms_per_launch": {k: v["ms"] / v["launches"] for k, v in kernels.items()},
-        "kernels_ms_per_step": {k: v["ms"] / K for k, v in kernels.items()},
+        "kernels_ms_per_step": per_step(kernels, K) if kernels else None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -225,6 +225,7 @@ typedef struct pmx_stats
                               record nodes of the replayed step graph (1) or from eager steps (0) */
   long long grown;         /* steps of the last call repeated after a contact capacity grew */
   int row_capacity;        /* contacts per sphere row now (grows as contacts need) */
+  long long busy;          /* spheres with a contact after the last step */
 } pmx_stats;
 
 /* per-kernel timing collected with HIP events on the context's stream */

@@ -36,6 +36,11 @@ def load():
         lib.orc_tree_pairs.restype = C.c_longlong
         lib.orc_morton.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double)]
         lib.orc_morton.restype = C.c_uint
+        lib.orc_tree_leaves.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        lib.orc_tree_leaves.restype = C.c_int
+        lib.orc_tree_reach.argtypes = [C.c_int, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double),
+                                       C.POINTER(C.c_longlong), C.POINTER(C.c_int), C.c_longlong]
+        lib.orc_tree_reach.restype = C.c_longlong
         _lib = lib
     return _lib
 

@@ -1180,7 +1180,7 @@ int springs (orc_ctx &s, double step)
   double *force = s.force.data (), *torque = s.torque.data ();
   double fj[3] = {0, 0, 0}, tj[3] = {0, 0, 0}, fk[3] = {0, 0, 0}, tk[3] = {0, 0, 0};
   int j0 = -1, k0 = -1;
-  double oj[3], vj[3], xj[3], Xj[3], Lj[9], ok[3], vk[3], xk[3], Xk[3], Lk[9];
+  double oj[3] = {}, vj[3] = {}, xj[3] = {}, Xj[3] = {}, Lj[9] = {}, ok[3] = {}, vk[3] = {}, xk[3] = {}, Xk[3] = {}, Lk[9] = {};
   const double *sx = s.spring.data (), *sy = s.spring.data () + s.sprcrv;
   const double *dx = s.dashpot.data (), *dy = s.dashpot.data () + s.dashcrv;
   const double *ux = s.unload.data (), *uy = s.unload.data () + s.uncrv;
@@ -1705,6 +1705,54 @@ long long orc_tree_pairs (int n, const double *center, const double *radius, int
   long long m = (long long) pr.size ();
   for (long long k = 0; k < m && k < cap; k++) { out[2*k] = pr[k].first; out[2*k+1] = pr[k].second; }
   return m;
+}
+
+/* the leaf each centre descends to in a fresh tree (partitioning_create +
+ * drop_ellipsoid, partition.ispc:600-660, 663-746), leaves numbered in the
+ * left-first depth-first order of partitioning_tree_create (525-569),
+ * whether or not the leaf overflows; returns the number of leaves and, in
+ * *overflow, the count tree_store reports */
+int orc_tree_leaves (int n, const double *center, int *leafof, int *overflow)
+{
+  Tree T;
+  const double *cx = center, *cy = center + n, *cz = center + 2*(size_t) n;
+  tree_create (n, cx, cy, cz, T);
+  *overflow = tree_store (T, n, cx, cy, cz);
+  const double *c[3] = {cx, cy, cz};
+  for (int i = 0; i < n; i++)
+  {
+    int node = 0;
+    while (T.node[node].dimension >= 0)
+    {
+      int d = T.node[node].dimension;
+      node = c[d][i] < T.node[node].coord ? T.node[node].left : T.node[node].right;
+    }
+    leafof[i] = T.node[node].leaf;
+  }
+  return (int) T.leaf.size ();
+}
+
+/* the leaves of a fresh tree over the centres that each box [lo, hi]
+ * (box[6*b .. 6*b+5] = lo xyz, hi xyz) reaches by the traversal rule of
+ * condet.ispc:104-113 (left when lo[d] <= coord, right when hi[d] >= coord),
+ * in visiting order; off[b] .. off[b+1] index `leaves` (at most cap written);
+ * returns the total count */
+long long orc_tree_reach (int n, const double *center, int nbox, const double *box, long long *off, int *leaves,
+                          long long cap)
+{
+  Tree T;
+  const double *cx = center, *cy = center + n, *cz = center + 2*(size_t) n;
+  tree_create (n, cx, cy, cz, T);
+  long long k = 0;
+  for (int b = 0; b < nbox; b++)
+  {
+    off[b] = k;
+    std::vector<int> out;
+    tree_reach (T, 0, box + 6*(size_t) b, box + 6*(size_t) b + 3, out);
+    for (int L : out) { if (k < cap) leaves[k] = L; k++; }
+  }
+  off[nbox] = k;
+  return k;
 }
 
 /* broad-phase statistics of a fresh tree: [0] leaves, [1] mean leaves reached

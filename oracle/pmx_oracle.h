@@ -44,6 +44,13 @@ double orc_triangle_sphere (const double a[3], const double b[3], const double c
  * indices assuming ellipsoid i belongs to particle i); returns count, writes
  * at most cap pairs */
 long long orc_tree_pairs (int n, const double *center, const double *radius, int *out, long long cap);
+/* leaf of each centre in a fresh tree (depth-first leaf numbering);
+ * returns the leaf count; *overflow receives the leaf overflow count */
+int orc_tree_leaves (int n, const double *center, int *leafof, int *overflow);
+/* leaves of a fresh tree reached by each box (lo xyz, hi xyz), in visiting
+ * order, indexed by off[nbox+1]; returns the total */
+long long orc_tree_reach (int n, const double *center, int nbox, const double *box, long long *off, int *leaves,
+                          long long cap);
 /* Morton code of one point given the extents (partition.ispc:69-94) */
 unsigned int orc_morton (const double p[3], const double extents[6]);
 

@@ -161,7 +161,7 @@ int upload_core (pmx_ctx *c, const pmx_model *m, const std::vector<InstOff> &off
   for (int e = 0; e < m->ellnum; e++) parell[m->part[e]] = e;
   for (int i = 0; i < m->parnum; i++) if (parell[i] < 0) nosph.push_back (i);
   std::vector<signed char> iso (std::max (m->parnum, 1), 0);
-  int niso = 0, nzc = 0;
+  int niso = 0, nzc = 0, nfree = 0;
   /* particles whose force another stage adds to or sets (springs, prescribed
    * motion): their force always goes through the force array */
   std::vector<char> engaged (std::max (m->parnum, 1), 0);
@@ -192,7 +192,7 @@ int upload_core (pmx_ctx *c, const pmx_model *m, const std::vector<InstOff> &off
     /* bit 2: a sphere particle that nothing but its contacts and gravity
      * loads: when its row ends up empty its force is 0 + m g and its torque
      * 0, formed where they are used instead of stored */
-    if (e >= 0 && !engaged[i]) iso[i] |= 4;
+    if (e >= 0 && !engaged[i]) { iso[i] |= 4; nfree++; }
   }
   std::vector<int> pinst (std::max (m->parnum, 1)), einst (std::max (m->ellnum, 1)), tinst (std::max (m->trinum, 1));
   std::vector<int> eoff (I + 1), poff (I + 1), toff (I + 1);
@@ -238,6 +238,7 @@ int upload_core (pmx_ctx *c, const pmx_model *m, const std::vector<InstOff> &off
   c->nnosph = (int) nosph.size ();
   c->niso = niso;
   c->nzc = nzc;
+  c->nfree = nfree;
   if ((rc = up (*c, c->iso, iso.data (), N, "iso"))) return rc;
   if ((rc = up (*c, c->parell, parell.data (), N, "parell")) ||
       (rc = up (*c, c->nosph, nosph.data (), nosph.size (), "nosph")))
@@ -1038,7 +1039,7 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
   c->stats.leaf_slots = c->lstride;
   c->stats.seconds = std::chrono::duration<double> (std::chrono::steady_clock::now () - tl).count ();
   {
-    long long total = 0;
+    long long total = 0, busy = 0;
     for (int i = 0; i < I; i++)
     {
       int e0 = c->off[i].ell, ne = c->off[i+1].ell - e0;
@@ -1047,9 +1048,10 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
       PMX_HIP (hipMemcpyAsync (cnt.data (), c->rowcnt[c->icur[i]].p + e0, sizeof (int) * ne, hipMemcpyDeviceToHost,
                                c->stream));
       PMX_HIP (hipStreamSynchronize (c->stream));
-      for (int e = 0; e < ne; e++) total += cnt[e];
+      for (int e = 0; e < ne; e++) { total += cnt[e]; busy += cnt[e] > 0; }
     }
     c->stats.pairs = total;  /* entries; sphere pairs are held twice */
+    c->stats.busy = busy;
   }
   if (elapsed) *elapsed = dt;
   return PMX_OK;

@@ -263,6 +263,7 @@ struct pmx_ctx
                                   only (no spring, no prescribed motion) */
   int niso = 0;
   int nzc = 0;                 /* particles whose sphere offset is +0 */
+  int nfree = 0;               /* particles with flag bit 2 (loaded by contacts and gravity only) */
   bool nbr_csr = false;        /* neighbour lists always in CSR layout (PMX_NEIGHBOUR_LISTS=csr) */
   bool rec_packed = false;     /* leaf records always packed (PMX_LEAF_RECORDS=packed) */
   int nnosph = 0;

@@ -1579,10 +1579,18 @@ int step_rows (pmx_ctx &c, double h)
 
 int step_dynamics (pmx_ctx &c, double step0, double step1)
 {
-  /* per particle: 216 B read and 168 B written (state, force, torque,
-   * masses), 5 B of flags, 128 B more for a full inertia pair; per sphere: the
-   * centre written, 48 B of reference offset read unless it is +0 */
-  ProfScope ps (c, K_DYNAMICS, 389.0 * c.N + 128.0 * (c.N - c.niso) + 24.0 * c.E + 48.0 * (c.E - c.nzc));
+  /* algorithmic bytes as each particle's branch reads and writes them, with
+   * every contact row empty and no damping: per particle the flag 1, state
+   * (Omega, v, L, x) 144 read and 168 written, 1/m 8, inertia and inverse 16
+   * (isotropic) or 144, its sphere's index 4, its instance 4 (batches); per
+   * sphere the centre written 24, the reference offset 48 unless it is +0;
+   * force and torque 48 read, except for a particle loaded by contacts and
+   * gravity only, which reads its row count 4 and mass 8 instead (each such
+   * particle whose row is not empty reads 40 B more than counted here) */
+  double dyn = 321.0 * c.N + 16.0 * c.niso + 144.0 * (c.N - c.niso) + (c.E > 0 ? 4.0 * c.N : 0.0) +
+               (c.I > 1 ? 4.0 * c.N : 0.0) + 24.0 * c.E + 48.0 * (c.E - c.nzc) + 12.0 * c.nfree +
+               48.0 * (c.N - c.nfree);
+  ProfScope ps (c, K_DYNAMICS, dyn);
 #define DYN_ARGS c.N, c.angular.p, c.linear.p, c.rotation.p, \
                  c.position.p, (const double*) c.inertia.p, (const double*) c.inverse.p, (const double*) c.mass.p, \
                  (const double*) c.invm.p, (const double*) c.dtab.p, (const int*) c.ctl.p, c.force.p, c.torque.p, \

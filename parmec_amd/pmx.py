@@ -74,7 +74,8 @@ class pmx_stats(C.Structure):
     _fields_ = [("steps", C.c_longlong), ("rebuilds", C.c_longlong), ("pairs", C.c_longlong), ("leaves", C.c_int),
                 ("max_row", C.c_int), ("seconds", C.c_double), ("det_waves", C.c_int), ("rows_block", C.c_int),
                 ("leaf_slots", C.c_int), ("partial_rebuilds", C.c_longlong),
-                ("graph_timed", C.c_int), ("grown", C.c_longlong), ("row_capacity", C.c_int)]
+                ("graph_timed", C.c_int), ("grown", C.c_longlong), ("row_capacity", C.c_int),
+                ("busy", C.c_longlong)]
 
 
 class pmx_kernel_time(C.Structure):

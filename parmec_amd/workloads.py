@@ -301,6 +301,29 @@ def sweep_instance(i, nmin=1000, nmax=64000, radius=0.005):
     return box_lattice(n=n, seed=i, radius=radius), amp, freq
 
 
+def dense_instance(i, nmin=1000, nmax=64000, radius=0.005, spacing_factor=1.98, jitter=0.004):
+    """Instance i of the sweep (same seeded count, amplitude and frequency)
+    as a confined dense pack: the lattice spacing 1.98 r makes every sphere
+    overlap its six neighbours by 0.02 r, and the six walls are placed where
+    each outer sphere overlaps them by the same 0.02 r, so the pack stays
+    compressed (its contact forces, ~100 N, dwarf gravity and the base
+    excitation) and every sphere keeps its contacts: about 3 contact pairs
+    per sphere, the regime of a settled pack.  The jitter (0.004 r) stays
+    below the overlap.  Returns (model, amplitude m/s^2, frequency Hz)."""
+    rng = np.random.default_rng(1000003 + i)
+    n = int(rng.integers(nmin, nmax + 1))
+    amp = float(rng.uniform(0.1, 2.0)) * 9.81
+    freq = float(rng.uniform(1.0, 20.0))
+    a = int(math.ceil(n ** (1.0 / 3.0)))
+    dims = (a, a, int(math.ceil(n / (a * a))))
+    m = box_lattice(n=n, dims=dims, seed=i, radius=radius, spacing_factor=spacing_factor, jitter=jitter)
+    s = spacing_factor * radius
+    inset = 0.5 * s - (radius - (2.0 - spacing_factor) * radius)   # wall overlap = neighbour overlap
+    tris = box_triangles((inset, inset, inset), (a * s - inset, a * s - inset, dims[2] * s - inset))
+    m.tri[:] = tris.T
+    return m, amp, freq
+
+
 def sweep(instances=512, nmin=1000, nmax=64000):
     """The batched sweep: independent instances of sweep_instance."""
     return [sweep_instance(i, nmin, nmax) for i in range(instances)]

@@ -191,3 +191,22 @@ def test_packed_leaf_records(engines, monkeypatch):
     assert len(cc) > 50
     assert sg["leaf_slots"] == 0
     compare(mg, cg, mc, cc)
+
+
+@pytest.mark.parametrize("b_x,tri_y,c_y,expect", [
+    (5.05, None, 4.5, set()),          # overlapping pair whose boxes never reach each other's leaves
+    (5.0, None, 4.5, {(0, 1)}),        # lower x bound exactly on the root's split coordinate: found
+    (5.05, 4.05, 4.5, set()),          # triangle box not widened by the radius: contact missed
+    (5.05, 4.1, 4.55, {(2, -1)}),      # triangle box on the split coordinate: found
+])
+def test_q10_hand_built(b_x, tri_y, c_y, expect):
+    """The reference's frozen-tree rule on cases whose pair sets are derived
+    by hand (tests/test_oracle_tree.py::q10_model): the device finds exactly
+    the pairs the rule finds, including the overlapping ones it misses."""
+    from tests.test_oracle_tree import q10_model, q10_contacts
+    print(f"case q10 b_x={b_x} tri_y={tri_y}", flush=True)
+    m = q10_model(b_x=b_x, tri_y=tri_y, c_y=c_y)
+    g = pmx.Engine()
+    got = q10_contacts(g, m)
+    g.close()
+    assert got == expect
