@@ -322,6 +322,15 @@ int pmx_set_history (pmx_ctx *ctx, const pmx_history *h);
  * dem() takes before its first step when curtime is 0, parmec.cpp:2386-2394). */
 int pmx_history_sample (pmx_ctx *ctx, double *out);
 
+/* pmx_history_sample without device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops).  The sample passes through one
+ * static device buffer per GPU (PMX_HIS_NOALLOC_MAX doubles, held until the
+ * process ends) shared by every context on that GPU, so calls that overlap
+ * on one GPU race; more than PMX_HIS_NOALLOC_MAX histories is PMX_ECAPACITY. */
+#define PMX_HIS_NOALLOC_MAX 4096
+int pmx_history_sample_noalloc (pmx_ctx *ctx, double *out);
+
 void pmx_destroy (pmx_ctx *ctx);
 
 const char *pmx_last_error (void);

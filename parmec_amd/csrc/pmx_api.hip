@@ -399,6 +399,7 @@ int pmx_create (const pmx_options *opt, pmx_ctx **out)
     delete c;
     return fail (PMX_EDEVICE, "cannot open HIP device/stream");
   }
+  if (int rc = history_scratch_init (c->device)) { delete c; return rc; }
   if (c->opt.row_capacity > 0) { c->cap = c->opt.row_capacity; c->hcap = c->opt.row_capacity; c->xcap = std::max (8, c->opt.row_capacity / 2); }
   if (c->dflags.alloc (FLAG_COUNT) != hipSuccess || c->ctl.alloc (CTL_COUNT) != hipSuccess)
   { delete c; return fail (PMX_ENOMEM, "out of device memory"); }

@@ -15,8 +15,20 @@
  * particle's rotation (NVADDMUL, macros.h:973-978).
  */
 #include "pmx_internal.h"
+#include <mutex>
+
+/* Scratch of pmx_history_sample_noalloc: one sample of at most
+ * PMX_HIS_NOALLOC_MAX histories (32 KiB), held by every GPU the library runs
+ * on until the process ends.  Host code never names it: each device's
+ * address of it is taken once, at context creation, and kept in
+ * scratch_dev. */
+static __device__ __align__(256) double scratch[PMX_HIS_NOALLOC_MAX];
 
 namespace {
+
+constexpr int MAX_DEVICES = 64;
+double *scratch_dev[MAX_DEVICES] = {};
+std::mutex scratch_mu;
 
 inline dim3 grid (int n) { return dim3 ((n + PMX_TPB - 1) / PMX_TPB); }
 
@@ -243,11 +255,43 @@ int history_now (pmx_ctx &c, double *out)
   return PMX_OK;
 }
 
+/* the device address of scratch on the current device (once per device);
+ * called by pmx_create with the context's device current */
+int history_scratch_init (int device)
+{
+  if (device < 0 || device >= MAX_DEVICES) return fail (PMX_EINVAL, "device ordinal beyond the scratch table");
+  std::lock_guard<std::mutex> lock (scratch_mu);
+  if (scratch_dev[device]) return PMX_OK;
+  void *p = nullptr;
+  PMX_HIP (hipGetSymbolAddress (&p, HIP_SYMBOL (scratch)));
+  if (!p) return fail (PMX_EDEVICE, "no device address for the history scratch");
+  scratch_dev[device] = (double*) p;
+  return PMX_OK;
+}
+
 /* the step's sample, the last kernel of the step before its counter moves */
 int step_history (pmx_ctx &c)
 {
   if (!c.his_active || c.nhis == 0) return PMX_OK;
   launch_history (c, (const int*) c.hisrow.p, -1, c.hisbuf.p);
   PMX_HIP (hipGetLastError ());
+  return PMX_OK;
+}
+
+extern "C" int pmx_history_sample_noalloc (pmx_ctx *c, double *out)
+{
+  if (!c || !out) return fail (PMX_EINVAL, "null argument");
+  if (c->nhis > PMX_HIS_NOALLOC_MAX)
+    return fail (PMX_ECAPACITY, "pmx_history_sample_noalloc: " + std::to_string (c->nhis) + " histories, at most " +
+                 std::to_string (PMX_HIS_NOALLOC_MAX));
+  if (c->nhis == 0) return PMX_OK;
+  if (c->device < 0 || c->device >= MAX_DEVICES || !scratch_dev[c->device])
+    return fail (PMX_EDEVICE, "history scratch not set up for this device");
+  (void) hipSetDevice (c->device);
+  double *buf = scratch_dev[c->device];
+  launch_history (*c, nullptr, 0, buf);
+  PMX_HIP (hipGetLastError ());
+  PMX_HIP (hipMemcpyAsync (out, buf, sizeof (double) * c->nhis, hipMemcpyDeviceToHost, c->stream));
+  PMX_HIP (hipStreamSynchronize (c->stream));
   return PMX_OK;
 }

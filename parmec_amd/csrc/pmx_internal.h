@@ -380,6 +380,7 @@ int enqueue_step (pmx_ctx &c, double h);     /* one whole time step */
 /* ---- time histories (pmx_history.hip) ---- */
 int history_set (pmx_ctx &c, const pmx_history *h);
 int history_now (pmx_ctx &c, double *out);
+int history_scratch_init (int device);        /* pmx_history_sample_noalloc's scratch address on a device */
 int step_history (pmx_ctx &c);
 int step_obstacles (pmx_ctx &c, const double *tab, int row, double h);
 int step_prescribe_acc (pmx_ctx &c);           /* before dynamics */
