@@ -19,12 +19,15 @@ Gravity and damping callbacks and TSERIES are evaluated on the host at each
 step's curtime (input.cpp:4424-4503) and handed to the device as per-step
 tables.  The step loop itself is libpmx's pmx_dem (C++ host + HIP kernels).
 
-SPRING (1440-1788) is supported without stroke offsets (load-curve
-TSERIES); OBSTACLE with point/linear/angular (moving obstacles) and PRESCRIBE
-(2647-2795) tabulate their callables or TSERIES at the step times.  Not on
-this path (raise NotImplementedError): MESH, ANALYTICAL, torsion springs,
-unsprings, joints, OUTPUT, CRITICAL, adaptive time stepping, and HISTORY's
-sphere/box sources and .h5 reading.
+MESH (840-1156) builds a meshed particle from volume elements (its
+surface triangles and mass characteristics as mesh.cpp:315-521 forms them;
+particles of SPHERE and MESH share one numbering).  SPRING (1440-1788) is
+supported without stroke offsets (load-curve TSERIES); OBSTACLE with
+point/linear/angular (moving obstacles) and PRESCRIBE (2647-2795) tabulate
+their callables or TSERIES at the step times.  Not on this path (raise
+NotImplementedError): ANALYTICAL, torsion springs, unsprings, joints, OUTPUT,
+CRITICAL, adaptive time stepping, and HISTORY's sphere/box sources and .h5
+reading.
 """
 import bisect
 import math
@@ -67,7 +70,9 @@ class _State:
 
     def reset(self):
         self.materials = []          # (density, young, poisson)
-        self.sph = []                # (cx, cy, cz, r, material, color)
+        self.sph = []                # (cx, cy, cz, r, material, color, particle)
+        self.parts = []              # particles in creation order: ("s", sphere) or ("m", mesh)
+        self.meshes = []             # MESH particles (see _mesh_create)
         self.vel = {}                # particle -> (lin or None, ang or None)
         self.tris = []               # 9-tuples
         self.tricol = []
@@ -153,9 +158,10 @@ def SPHERE(center, radius, material, color):
         raise ValueError("'material' must be >= 0 and < number of materials")
     if not color > 0:
         raise ValueError("'color' must be positive")
-    _S.sph.append((c[0], c[1], c[2], float(radius), int(material), int(color)))
+    _S.sph.append((c[0], c[1], c[2], float(radius), int(material), int(color), len(_S.parts)))
+    _S.parts.append(("s", len(_S.sph) - 1))
     _S.dirty = True
-    return len(_S.sph) - 1
+    return len(_S.parts) - 1
 
 
 def OBSTACLE(triangles, color, point=None, linear=None, angular=None):
@@ -188,6 +194,203 @@ def OBSTACLE(triangles, color, point=None, linear=None, angular=None):
     _S.dirty = True
 
 
+# faces of the volume elements, in local node numbers 1..k (mesh.cpp:39-60);
+# each face lists its nodes so that its normal points out of the element
+_TET = ((1, 3, 2), (1, 2, 4), (2, 3, 4), (3, 1, 4))
+_PYR = ((1, 4, 3, 2), (1, 2, 5), (2, 3, 5), (3, 4, 5), (4, 1, 5))
+_WED = ((1, 3, 2), (4, 5, 6), (1, 2, 5, 4), (2, 3, 6, 5), (3, 1, 4, 6))
+_HEX = ((1, 4, 3, 2), (1, 2, 6, 5), (2, 3, 7, 6), (3, 4, 8, 7), (1, 5, 8, 4), (5, 6, 7, 8))
+_ELEMENT_FACES = {4: _TET, 5: _PYR, 6: _WED, 8: _HEX}
+
+
+def _simplex(J, a, b, c, d):
+    """Integrals over a tetrahedron of 1, x, y, z and the products of the
+    coordinates, scaled by J (the density times the Jacobian): simplex.h:33-120,
+    (J/6, J/24 sum x, J/120 (sum x_i y_i + sum_i sum_j x_i y_j))."""
+    p = (a, b, c, d)
+
+    def second(u, v):
+        t = 0.0
+        for i in range(4):
+            for j in range(4):
+                t = t + (2.0 * p[i][u] * p[j][v] if i == j else p[i][u] * p[j][v])
+        return (J / 120.0) * t
+    first = [(J / 24.0) * (a[k] + b[k] + c[k] + d[k]) for k in range(3)]
+    return J / 6.0, first, (second(0, 0), second(0, 1), second(1, 1), second(0, 2), second(1, 2), second(2, 2))
+
+
+def _jacobian(a, b, c, d):
+    q = [b[0] - a[0], c[0] - a[0], d[0] - a[0], b[1] - a[1], c[1] - a[1], d[1] - a[1], b[2] - a[2], c[2] - a[2],
+         d[2] - a[2]]
+    return (q[0] * q[4] * q[8] + q[3] * q[7] * q[2] + q[6] * q[1] * q[5] - q[6] * q[4] * q[2] - q[0] * q[7] * q[5] -
+            q[3] * q[1] * q[8])
+
+
+def _mesh_create(xyz, elements, colors):
+    """Surface triangles and mass characteristics of a volume mesh
+    (MESH_Create and MESH_Char, mesh.cpp:315-521).
+
+    elements: [(node list, material)], colors: (global colour, {sorted face
+    nodes: colour}).  A face shared by two elements is inner; the rest are
+    the surface.  The surface faces are listed in the order the reference
+    leaves them: the surface elements in input order, each element's faces
+    in its face-table order, the whole list reversed; a quadrilateral gives
+    the triangles (0, 1, 2) and (0, 2, 3).  Node numbers become local to the
+    range of nodes the elements use.  Mass, centre and inertia integrate
+    each element as tetrahedra fanned from the range's first node over
+    fan-triangulated faces (bulk elements first, then surface elements, each
+    in input order): sums of density-weighted monomials, shifted to the
+    centre, inertia = trace(E) I - E of the Euler tensor E."""
+    gcolor, fcolor = colors
+    lo = min(min(nodes) for nodes, _ in elements)
+    hi = max(max(nodes) for nodes, _ in elements)
+    # faces: the first element to list a face owns it; a later one makes it inner
+    owner = {}
+    shared = [0] * len(elements)
+    order = []
+    for e, (nodes, _) in enumerate(elements):
+        for n, f in enumerate(_ELEMENT_FACES[len(nodes)]):
+            key = tuple(sorted(nodes[t - 1] for t in f))
+            if key in owner:
+                shared[e] += 1
+                o = owner[key]
+                if o is not None:
+                    shared[o[0]] += 1
+                owner[key] = None
+            else:
+                owner[key] = (e, n)
+                order.append(key)
+    surface = [e for e in range(len(elements)) if shared[e] < len(_ELEMENT_FACES[len(elements[e][0])])]
+    bulk = [e for e in range(len(elements)) if e not in set(surface)]
+    faces_of = {}
+    for key in order:
+        o = owner[key]
+        if o is not None:
+            faces_of.setdefault(o[0], []).append((o[1], key))
+    faces = []
+    for e in surface:
+        nodes = elements[e][0]
+        for n, key in sorted(faces_of.get(e, [])):
+            loc = [nodes[t - 1] - lo for t in _ELEMENT_FACES[len(nodes)][n]]
+            faces.append((loc, fcolor.get(key, gcolor)))
+    faces.reverse()
+    tris = []
+    for loc, col in faces:
+        tris.append(((loc[0], loc[1], loc[2]), col))
+        if len(loc) == 4:
+            tris.append(((loc[0], loc[2], loc[3]), col))
+    local = xyz[lo:hi + 1]
+    zero = tuple(local[0])
+    me = 0.0
+    sxyz = [0.0, 0.0, 0.0]
+    E = [0.0] * 9
+    for e in bulk + surface:
+        nodes, rho = elements[e]
+        for f in _ELEMENT_FACES[len(nodes)]:
+            a = tuple(local[nodes[f[0] - 1] - lo])
+            for j in range(1, len(f) - 1):
+                b = tuple(local[nodes[f[j] - 1] - lo])
+                c = tuple(local[nodes[f[j + 1] - 1] - lo])
+                J = rho * _jacobian(zero, a, b, c)
+                v, first, sec = _simplex(J, zero, a, b, c)
+                me += v
+                sxyz = [sxyz[k] + first[k] for k in range(3)]
+                E[0] += sec[0]
+                E[3] += sec[1]
+                E[4] += sec[2]
+                E[6] += sec[3]
+                E[7] += sec[4]
+                E[8] += sec[5]
+    sx, sy, sz = sxyz
+    cen = (sx / me, sy / me, sz / me)
+    E[0] -= (2 * sx - cen[0] * me) * cen[0]
+    E[4] -= (2 * sy - cen[1] * me) * cen[1]
+    E[8] -= (2 * sz - cen[2] * me) * cen[2]
+    E[3] -= cen[0] * sy + cen[1] * sx - cen[0] * cen[1] * me
+    E[6] -= cen[0] * sz + cen[2] * sx - cen[0] * cen[2] * me
+    E[7] -= cen[1] * sz + cen[2] * sy - cen[1] * cen[2] * me
+    E[1], E[2], E[5] = E[3], E[6], E[7]
+    tr = E[0] + E[4] + E[8]
+    inertia = [(tr if k in (0, 4, 8) else 0.0) - E[k] for k in range(9)]
+    return {"nodes": [tuple(x) for x in local], "triangles": tris, "mass": me, "center": cen, "inertia": inertia,
+            "elements": [([n - lo for n in nodes], vol) for nodes, vol in elements]}
+
+
+def MESH(nodes, elements, material, colors):
+    """Create a meshed particle; returns the particle number (input.cpp:840-1156).
+
+    nodes: [x0, y0, z0, x1, ...]; elements: [k, n_1 .. n_k, volid, ...] with
+    k = 4, 5, 6 or 8 (tetrahedron, pyramid, wedge, hexahedron) and volid the
+    material whose density the element has; colors: the surface colour, or
+    [colour, k, n_1 .. n_k, colour, ...] with k = 3 or 4 face colours over
+    the global one.  The surface faces become triangles owned by the
+    particle (triobs = its number)."""
+    if not isinstance(nodes, list) or not isinstance(elements, list):
+        raise TypeError("'nodes' and 'elements' must be lists")
+    if not (isinstance(colors, list) or _is_number(colors)):
+        raise TypeError("'colors' must be a list or a number")
+    if not (0 <= material < len(_S.materials)):
+        raise ValueError("'material' must be >= 0 and < number of materials")
+    els, i, l = [], 0, len(elements)
+    while i < l:
+        k = int(elements[i])
+        if k not in (4, 5, 6, 8):
+            raise ValueError("An element must have 4, 5, 6, or 8 nodes")
+        if i + k + 1 >= l:
+            raise ValueError("The last element definition is incomplete")
+        els.append(([int(x) for x in elements[i + 1:i + 1 + k]], int(elements[i + 1 + k])))
+        i += k + 2
+    if len(nodes) % 3:
+        raise ValueError("Nodes list length must be a multiple of 3")
+    nn = len(nodes) // 3
+    for m_, (ns, vol) in enumerate(els):
+        for j, n in enumerate(ns):
+            if n < 0 or n >= nn:
+                raise ValueError(f"Node {j} in element {m_} is outside of range [0, {nn - 1}]")
+        if len(set(ns)) != len(ns):
+            raise ValueError(f"Element {m_} repeats a node")
+        if not (0 <= vol < len(_S.materials)):
+            raise ValueError(f"Element {m_} material (volid) {vol} is not a material number")
+    if not els:
+        raise ValueError("A mesh needs at least one element")
+    fcolor = {}
+    if isinstance(colors, list):
+        gcolor = int(colors[0])
+        i, l = 1, len(colors)
+        while i < l:
+            k = int(colors[i])
+            if k not in (3, 4):
+                raise ValueError("A face must have 3 or 4 nodes")
+            if i + k + 1 >= l:
+                raise ValueError("The last face definition is incomplete")
+            fn = [int(x) for x in colors[i + 1:i + 1 + k]]
+            for j, n in enumerate(fn):
+                if n < 0 or n >= nn:
+                    raise ValueError(f"Node {j} in face {len(fcolor)} is outside of range [0, {nn - 1}]")
+            if len(set(fn)) != len(fn):
+                raise ValueError(f"Face {len(fcolor)} repeats a node")
+            fcolor[tuple(sorted(fn))] = int(colors[i + 1 + k])
+            i += k + 2
+    else:
+        gcolor = int(colors)
+    xyz = [tuple(float(nodes[3 * q + c]) for c in range(3)) for q in range(nn)]
+    dens = [_S.materials[vol][_DENSITY] for _, vol in els]
+    msh = _mesh_create(xyz, [(ns, dens[e]) for e, (ns, _) in enumerate(els)], (gcolor, fcolor))
+    part = len(_S.parts)
+    msh["part"] = part
+    msh["material"] = int(material)
+    msh["first_triangle"] = len(_S.tris)
+    for (a, b, c), col in msh["triangles"]:
+        x = msh["nodes"]
+        _S.tris.append(tuple(x[a]) + tuple(x[b]) + tuple(x[c]))
+        _S.tricol.append(col)
+        _S.triobs.append(part)
+    _S.meshes.append(msh)
+    _S.parts.append(("m", len(_S.meshes) - 1))
+    _S.dirty = True
+    return part
+
+
 def GRANULAR(color1, color2, spring, damper=1.0, friction=0.0, rolling=0.0, drilling=0.0, kskn=0.5):
     """Granular interaction parameters of a colour pair (input.cpp:2384-2455)."""
     if not spring > 0:
@@ -216,7 +419,7 @@ def GRANULAR(color1, color2, spring, damper=1.0, friction=0.0, rolling=0.0, dril
 
 def VELOCITY(parnum, linear=None, angular=None):
     """Set initial velocities of a particle (input.cpp:2798-2837)."""
-    if not (0 <= parnum < len(_S.sph)):
+    if not (0 <= parnum < len(_S.parts)):
         raise ValueError("'parnum' must be >= 0 and < number of particles")
     lin = _tuple(linear, "linear", 3) if linear is not None else None
     ang = _tuple(angular, "angular", 3) if angular is not None else None
@@ -439,54 +642,83 @@ def _not_here(name):
     return f
 
 
-MESH = _not_here("MESH")
 ANALYTICAL = _not_here("ANALYTICAL")
 OUTPUT = _not_here("OUTPUT")
 CRITICAL = _not_here("CRITICAL")
 
 
 def _build_model():
-    """SoA model of everything created so far (the buffers of parmec.h)."""
-    n, t = len(_S.sph), len(_S.tris)
+    """SoA model of everything created so far (the buffers of parmec.h):
+    particles in creation order (SPHERE and MESH share the numbering),
+    spheres as ellipsoids, mesh nodes and faces, triangles of obstacles and
+    meshes in creation order."""
+    n, e, t = len(_S.parts), len(_S.sph), len(_S.tris)
     old = _S.model
     npairs = len(_S.pairs)
-    m = pmx.Model(parnum=n, ellnum=n, trinum=t, pairnum=npairs)
-    if n:
-        a = np.array([s[:4] for s in _S.sph], dtype=np.float64)
-        mat = np.array([s[4] for s in _S.sph])
-        col = np.array([s[5] for s in _S.sph], dtype=np.int32)
+    m = pmx.Model(parnum=n, ellnum=e, trinum=t, pairnum=npairs)
+    m.flags[:] = 2   # OUTREST (constants.h:39), as SPHERE and MESH set it
+    if e:
+        a = np.array([s_[:4] for s_ in _S.sph], dtype=np.float64)
+        mat = np.array([s_[4] for s_ in _S.sph])
+        col = np.array([s_[5] for s_ in _S.sph], dtype=np.int32)
+        part = np.array([s_[6] for s_ in _S.sph], dtype=np.int32)
         dens = np.array([_S.materials[k][_DENSITY] for k in mat])
         r = a[:, 3]
-        m.position[:3] = a[:, :3].T
-        m.position[3:] = a[:, :3].T
+        m.position[:3, part] = a[:, :3].T
+        m.position[3:, part] = a[:, :3].T
         m.center[:3] = a[:, :3].T
         m.center[3:] = a[:, :3].T
         m.radii[0] = r
         m.radii[1:] = -1.0
-        m.part[:] = np.arange(n, dtype=np.int32)
+        m.part[:] = part
         m.ellcol[:] = col
         volume = (4.0 / 3.0) * math.pi * r * r * r
-        m.mass[:] = volume * dens
-        m.invm[:] = 1.0 / m.mass
-        j = 0.4 * m.mass * r * r
+        m.mass[part] = volume * dens
+        m.invm[part] = 1.0 / m.mass[part]
+        j = 0.4 * m.mass[part] * r * r
         for k in (0, 4, 8):
-            m.inertia[k] = j
-        if old is not None:   # particles created earlier keep their evolved state
-            k = old.parnum
-            for name in ("angular", "linear", "rotation", "position", "force", "torque", "center"):
-                getattr(m, name)[:, :k] = getattr(old, name)[:, :k]
-            m.curtime, m.curstep, m.stepnum = old.curtime, old.curstep, old.stepnum
-        for p, (lin, ang) in _S.vel.items():
-            if lin:
-                m.linear[:, p] = lin
-            if ang:
-                m.angular[3:, p] = ang
-                L = m.rotation[:, p]
-                o = ang
-                m.angular[0, p] = L[0] * o[0] + L[1] * o[1] + L[2] * o[2]
-                m.angular[1, p] = L[3] * o[0] + L[4] * o[1] + L[5] * o[2]
-                m.angular[2, p] = L[6] * o[0] + L[7] * o[1] + L[8] * o[2]
-        _S.vel.clear()
+            m.inertia[k, part] = j
+    nodnum = sum(len(ms["nodes"]) for ms in _S.meshes)
+    facnum = sum(len(ms["triangles"]) for ms in _S.meshes)
+    m.set_mesh(nodnum, facnum)
+    no = fo = 0
+    for ms in _S.meshes:
+        p = ms["part"]
+        m.position[:3, p] = ms["center"]
+        m.position[3:, p] = ms["center"]
+        m.mass[p] = ms["mass"]
+        m.invm[p] = 1.0 / ms["mass"]
+        m.inertia[:, p] = ms["inertia"]
+        m.inverse[:, p] = np.linalg.inv(np.array(ms["inertia"]).reshape(3, 3)).reshape(-1)
+        x = np.array(ms["nodes"], dtype=np.float64)
+        k = len(x)
+        m.nodes[:3, no:no + k] = x.T
+        m.nodes[3:, no:no + k] = x.T
+        m.nodpart[no:no + k] = p
+        tri = np.array([abc for abc, _ in ms["triangles"]], dtype=np.int32).reshape(-1, 3)
+        f = len(tri)
+        m.facnod[:, fo:fo + f] = (tri + no).T
+        m.factri[fo:fo + f] = ms["first_triangle"] + np.arange(f, dtype=np.int32)
+        no += k
+        fo += f
+    if old is not None:   # particles created earlier keep their evolved state
+        k, ke, kn = old.parnum, old.ellnum, old.nodnum
+        for name in ("angular", "linear", "rotation", "position", "force", "torque"):
+            getattr(m, name)[:, :k] = getattr(old, name)[:, :k]
+        m.center[:, :ke] = old.center[:, :ke]
+        m.nodes[:, :kn] = old.nodes[:, :kn]
+        m.curtime, m.curstep, m.stepnum = old.curtime, old.curstep, old.stepnum
+    for p, (lin, ang) in _S.vel.items():
+        if lin:
+            m.linear[:, p] = lin
+        if ang:
+            m.angular[3:, p] = ang
+            L = m.rotation[:, p]
+            o = ang
+            m.angular[0, p] = L[0] * o[0] + L[1] * o[1] + L[2] * o[2]
+            m.angular[1, p] = L[3] * o[0] + L[4] * o[1] + L[5] * o[2]
+            m.angular[2, p] = L[6] * o[0] + L[7] * o[1] + L[8] * o[2]
+    _S.vel.clear()
     if t:
         m.tri[:] = np.array(_S.tris, dtype=np.float64).T
         m.tricol[:] = np.array(_S.tricol, dtype=np.int32)
@@ -539,7 +771,7 @@ def PRESCRIBE(parnum, linear=None, angular=None, kind="vv"):
     of a particle (input.cpp:2647-2795); kind is 'vv', 'va', 'av' or 'aa'
     (linear, angular).  A history is a callable of time returning a 3-tuple,
     or a tuple of 3 TSERIES numbers."""
-    n = len(_S.sph)
+    n = len(_S.parts)
     if not isinstance(parnum, int) or parnum < 0 or parnum >= n:
         raise ValueError("'parnum' must be >= 0 and < the number of particles")
     if not isinstance(kind, str):
@@ -573,9 +805,9 @@ def HISTORY(entity, source=None, point=None, h5file=None, h5last=False):
     if entity not in pmx.HIS_ENTITIES:
         raise ValueError("Invalid entity")
     spring = entity in _SPRING_ENTITIES
-    nmax = len(_S.springs) if spring else len(_S.sph)
+    nmax = len(_S.springs) if spring else len(_S.parts)
     if source is None:                       # useful when entity is 'TIME'
-        if not _S.sph:
+        if not _S.parts:
             raise ValueError("No particle has been defined")
         lst = [0]
     elif isinstance(source, int) and not isinstance(source, bool):

@@ -60,8 +60,8 @@ def test_argument_errors_follow_the_reference():
     P.GRAVITY(0.0, 0.0, ts)
     with pytest.raises(ValueError):
         P.DEM(-1.0, 1e-3)
-    with pytest.raises(NotImplementedError):
-        P.MESH([], [], mat, 1)
+    with pytest.raises(ValueError):
+        P.MESH([], [], mat, 1)           # no element (the reference would allocate a negative node count)
 
 
 def test_tseries_interpolation_and_clamping():

@@ -76,3 +76,29 @@ def test_spring_lattice_prescribed_base(dims, steps, calls):
     bot = mc.prspart
     assert np.abs(mc.linear[0, bot]).max() > 0.0      # the base moved
     np.testing.assert_array_equal(mc.angular[3:, bot], 0.0)
+
+
+def test_deck_mesh_lattice():
+    """A spring lattice built entirely through the deck API (MESH bodies,
+    SPRING curves, PRESCRIBE base drive, GRAVITY) runs on the device
+    bitwise as on the oracle, nodes and surface triangles included."""
+    from oracle import orc
+    from parmec_amd import parmec as P
+    from tests.test_deck_mesh import deck_lattice
+    print("case deck-built lattice 6x5x4 (MESH + SPRING + PRESCRIBE), 2 calls", flush=True)
+    out = []
+    for make in (lambda: pmx.Engine(), lambda: orc.Oracle(0)):
+        P.RESET()
+        deck_lattice(6, 5, 4)
+        P.GRAVITY(0.0, 0.0, -9.81)
+        P._use_engine(make())
+        P.DEM(200 * 1e-5 - 0.5e-5, 1e-5)
+        P.DEM(150 * 1e-5 - 0.5e-5, 1e-5)
+        out.append(P.state().copy())
+        P.RESET()
+    g, c = out
+    for name in ("position", "linear", "angular", "rotation", "force", "torque", "stroke", "sprfrc", "sprpnt",
+                 "sprdir", "nodes", "tri"):
+        a, b = getattr(g, name), getattr(c, name)
+        assert np.array_equal(a, b), (name, float(np.abs(a - b).max()))
+    assert np.abs(g.sprfrc).max() > 0.0
