@@ -163,6 +163,14 @@ typedef struct pmx_forcing
   int hisrows;               /* rows of hisout */
   double *hisout;            /* [hisrows][histories] values written by pmx_dem (output_history,
                                 output.cpp:5143-5520); TIME entries are left to the caller */
+  const int *outstep;        /* [noutstep] ascending steps of this call (0-based) after which pmx_dem pauses
+                                and calls out_fn(out_user, step) with the state complete on the device
+                                (forces, mesh nodes and triangles), so that pmx_download gives the state
+                                dem() writes at its output points (output_files, parmec.cpp:2502-2507);
+                                single-model contexts; NULL: no pause */
+  int noutstep;
+  void (*out_fn) (void *user, int step);
+  void *out_user;
 } pmx_forcing;
 
 /* time-history entities (HISTORY, input.cpp:3157-3555); particle entities

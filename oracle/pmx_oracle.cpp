@@ -1618,6 +1618,10 @@ int orc_dem (orc_ctx *s, double duration, double step, const pmx_forcing *fc, do
     shapes (*s);
     obstaclev (*s, fc && fc->obstab ? fc->obstab + 6*(size_t) k * s->obsnum : nullptr);   /* at curtime+step0 */
     obstacles (*s, 0.5*(step0+step1));
+    /* output points (parmec.cpp:2502-2507): the caller reads the state */
+    if (fc && fc->out_fn && fc->outstep)
+      for (int q = 0; q < fc->noutstep; q++)
+        if (fc->outstep[q] == k) fc->out_fn (fc->out_user, (int) k);
   }
   s->curstep = step1;
   s->stepnum++;

@@ -925,6 +925,9 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
     { c->hring = nullptr; return fail (PMX_ENOMEM, "out of pinned host memory"); }
     for (int k = 0; k < PMX_RING; k++) PMX_HIP (hipEventCreateWithFlags (&c->rev[k], hipEventDisableTiming));
   }
+  /* output points of a single model: the steps after which the call pauses */
+  const bool outs = I == 1 && fc && fc->noutstep > 0 && fc->outstep && fc->out_fn;
+  int oi = 0;
   int done = 0;
   long long grown = 0;
   std::vector<int> rebuilt_at (I, -1);   /* the last step each instance was re-partitioned for */
@@ -932,6 +935,11 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
   {
     int next = nsteps;
     for (int i = 0; i < I; i++) if (!reported[i] && nst[i] > done) next = std::min (next, nst[i]);
+    if (outs)
+    {
+      while (oi < fc->noutstep && fc->outstep[oi] + 1 <= done) oi++;
+      if (oi < fc->noutstep) next = std::min (next, fc->outstep[oi] + 1);
+    }
     if (c->prof.mode != 2 && !c->prof.eager && !c->graph_valid && (rc = capture (c, step0, gsize >= PMX_GRAPH_STEPS)))
       return rc;
     bool eager = c->prof.mode == 2 || (c->prof.mode == 1 && c->prof.eager);
@@ -1014,11 +1022,30 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
       if ((rc = clear_overflow_flags (c))) return rc;
     }
     if ((rc = report (done))) return rc;
+    if (outs && !abort && oi < fc->noutstep && fc->outstep[oi] + 1 == done && done < nsteps)
+    {
+      /* an output point (the last step's output happens after the loop):
+       * forces formed where they were used, mesh geometry, then the caller */
+      if ((rc = step_materialize (*c, std::vector<int> (1, done - 1))) || (rc = step_mesh_shapes (*c, false))) return rc;
+      PMX_HIP (hipStreamSynchronize (c->stream));
+      fc->out_fn (fc->out_user, fc->outstep[oi]);
+      (void) hipSetDevice (c->device);
+      oi++;
+    }
     if (tuning && c->tuned == 1 && !abort && done < nsteps && c->stepped + done >= PMX_RETUNE_STEPS &&
         (rc = tune (done)))
       return rc;
   }
   if ((rc = report (nsteps))) return rc;
+  if (outs)
+    for (; oi < fc->noutstep; oi++)
+      if (fc->outstep[oi] == nsteps - 1)   /* the last step: the finished state */
+      {
+        c->curtime = c->icurtime[0]; c->curstep = c->icurstep[0]; c->stepnum = c->istepnum[0];
+        c->cur = (c->cur0 + nsteps) & 1;
+        fc->out_fn (fc->out_user, fc->outstep[oi]);
+        (void) hipSetDevice (c->device);
+      }
   if (c->his_active)
   {
     PMX_HIP (hipMemcpyAsync (fc->hisout, c->hisbuf.p, sizeof (double) * (size_t) fc->hisrows * c->nhis,

@@ -25,9 +25,9 @@ particles of SPHERE and MESH share one numbering).  SPRING (1440-1788) is
 supported without stroke offsets (load-curve TSERIES); OBSTACLE with
 point/linear/angular (moving obstacles) and PRESCRIBE (2647-2795) tabulate
 their callables or TSERIES at the step times.  Not on this path (raise
-NotImplementedError): ANALYTICAL, torsion springs, unsprings, joints, OUTPUT,
-CRITICAL, adaptive time stepping, and HISTORY's sphere/box sources and .h5
-reading.
+NotImplementedError): ANALYTICAL, torsion springs, unsprings, joints, XDMF
+and MED output (OUTPUT writes DUMP and VTK files), CRITICAL, adaptive time
+stepping, and HISTORY's sphere/box sources and .h5 reading.
 """
 import bisect
 import math
@@ -41,6 +41,18 @@ __all__ = ["RESET", "TSERIES", "MATERIAL", "SPHERE", "OBSTACLE", "GRANULAR", "SP
            "engine_options"]
 
 _DENSITY, _YOUNG, _POISSON = 0, 1, 2
+
+# output formats, modes and entities (constants.h:52-60)
+OUT_FORMAT_DUMP, OUT_FORMAT_VTK, OUT_FORMAT_XDMF, OUT_FORMAT_MED = 1, 2, 4, 8
+OUT_MODES = {"SPH": 1, "MESH": 2, "RB": 4, "CD": 8, "SL": 16, "ST": 32, "JT": 64}
+OUT_ALL_MODES = 127
+OUT_ENTITIES = {"NUMBER": 1, "COLOR": 2, "DISPL": 4, "ORIENT": 8, "ORIENT1": 16, "ORIENT2": 32, "ORIENT3": 64,
+                "LINVEL": 128, "ANGVEL": 256, "FORCE": 512, "TORQUE": 1024, "F": 2048, "FN": 4096, "FT": 8192,
+                "SF": 16384, "AREA": 32768, "PAIR": 65536, "LENGTH": 131072, "SS": 262144, "FF": 524288,
+                "XDIR": 1048576, "YDIR": 2097152, "ZDIR": 4194304, "TRQROT": 8388608, "TRQTOT": 16777216,
+                "TRQSPR": 33554432, "JREAC": 67108864}
+OUT_DEFAULT_ENTITIES = sum(OUT_ENTITIES.values())   # every entity (parmec.cpp:1371-1373)
+_OUTREST = 2   # particle flag: not listed by any OUTPUT subset (constants.h:39)
 
 
 class _Series:
@@ -92,6 +104,12 @@ class _State:
         self.springs = []            # SPRING records in creation order
         self.histories = []          # (entity, source list, point or None, the returned Python list)
         self.curtime_history = 0.0   # parmec.cpp:73, 2261
+        self.outputs = []            # OUTPUT records: (particles, entities, modes)
+        self.outrest = [OUT_DEFAULT_ENTITIES, OUT_ALL_MODES]   # parmec.cpp:1371-1375
+        self.outformat = OUT_FORMAT_XDMF
+        self.output_frame = 0
+        self.curtime_output = 0.0    # parmec.cpp:72, 2260
+        self.output_path = None
 
 
 _S = _State()
@@ -643,7 +661,108 @@ def _not_here(name):
 
 
 ANALYTICAL = _not_here("ANALYTICAL")
-OUTPUT = _not_here("OUTPUT")
+
+
+def OUTPUT(entities=None, subset=None, mode=None, format=None):
+    """Output entities, modes and formats for a subset of particles, or for
+    the particles no subset lists (input.cpp:3560-3956).  Every call adds a
+    record (with an empty subset when none is given), as the reference does.
+    DUMP and VTK files are written at DEM's output points; XDMF and MED need
+    libraries this build has not got (NotImplementedError)."""
+    if entities is not None and not isinstance(entities, list):
+        raise TypeError("'entities' must be a list")
+    if subset is not None and not (isinstance(subset, list) or _is_number(subset)):
+        raise TypeError("'subset' must be a list or a number")
+    if mode is not None and not isinstance(mode, (str, list)):
+        raise TypeError("'mode' must be a string or a list")
+    if format is not None and not isinstance(format, (str, list)):
+        raise TypeError("'format' must be a string or a list")
+    parts = []
+    if subset is not None:
+        for k in (subset if isinstance(subset, list) else [subset]):
+            k = int(k)
+            if k < 0 or k >= len(_S.parts):
+                raise ValueError("Invalid particle number")
+            parts.append(k)
+    rec_ent, rec_mode = 0, 0
+    if mode is not None:
+        items = [mode] if isinstance(mode, str) else mode
+        if isinstance(mode, list):
+            if subset is None:
+                _S.outrest[1] = 0
+        for it in items:
+            if not isinstance(it, str):
+                raise TypeError("mode list item is not a string")
+            if it not in OUT_MODES:
+                raise ValueError("Invalid mode")
+            b = OUT_MODES[it]
+            if subset is not None or it in ("SL", "ST", "JT"):
+                # SL, ST and JT set the record's own mode even without a subset
+                rec_mode = rec_mode | b if isinstance(mode, list) else b
+            elif it in ("SPH", "MESH"):
+                _S.outrest[1] = (_S.outrest[1] | b) if isinstance(mode, list) else b
+            # RB and CD without a subset write outmode[1], not the default
+            # modes (input.cpp:3652-3661, 3698-3707): no effect here
+    else:
+        if subset is not None:
+            rec_mode = OUT_ALL_MODES
+        else:
+            _S.outrest[1] = OUT_ALL_MODES
+    if entities is not None:
+        if subset is None:
+            _S.outrest[0] = 0
+        for it in entities:
+            if not isinstance(it, str) or it not in OUT_ENTITIES:
+                raise ValueError("Invalid entity")
+            if subset is not None:
+                rec_ent |= OUT_ENTITIES[it]
+            else:
+                _S.outrest[0] |= OUT_ENTITIES[it]
+    else:
+        if subset is not None:
+            rec_ent = 0xffffffff
+        else:
+            _S.outrest[0] = 0xffffffff
+    if format is not None:
+        names = {"DUMP": OUT_FORMAT_DUMP, "VTK": OUT_FORMAT_VTK, "XDMF": OUT_FORMAT_XDMF, "MED": OUT_FORMAT_MED}
+        fmt = 0
+        for it in ([format] if isinstance(format, str) else format):
+            if it not in names:
+                raise ValueError("Invalid format")
+            fmt |= names[it]
+        if fmt & (OUT_FORMAT_XDMF | OUT_FORMAT_MED):
+            raise NotImplementedError("XDMF and MED output need HDF5 / MED libraries, not in this build; "
+                                      "use format=['DUMP', 'VTK']")
+        _S.outformat = fmt
+    _S.outputs.append((parts, rec_ent, rec_mode))
+
+
+def _output_path(prefix):
+    """The reference's output path: the input deck's path without '.py'; a
+    prefix replaces its file name part (input.cpp:4140-4153,
+    parmec.cpp:2328-2351)."""
+    import os
+    import sys
+    if _S.output_path is None:
+        deck = sys.argv[0] if sys.argv and sys.argv[0].endswith(".py") else "parmec.py"
+        _S.output_path = deck[:-3]
+    if prefix is not None:
+        head = os.path.dirname(_S.output_path)
+        _S.output_path = os.path.join(head, prefix) if head else prefix
+    return _S.output_path
+
+
+def _write_output(m, t, prefix_path):
+    """output_files at curtime t for the state m (downloaded)."""
+    from . import output
+    flags = np.full(m.parnum, _OUTREST, dtype=np.int32)
+    for parts, _, _ in _S.outputs:
+        for k in parts:
+            flags[k] &= ~_OUTREST
+    recs = _S.springs
+    order = sorted(range(len(recs)), key=lambda i: (recs[i]["part"][0], recs[i]["part"][1], i))
+    output.write(m, t, _S.output_frame, prefix_path, _S.outformat, _S.outputs, _S.outrest, flags, order)
+    _S.output_frame += 1
 CRITICAL = _not_here("CRITICAL")
 
 
@@ -873,8 +992,9 @@ def DEM(duration, step, interval=None, prefix=None, adaptive=0.0):
     """Run the DEM time stepping; returns the wall time in seconds
     (input.cpp:3958-4091, parmec.cpp:2315-2530).  interval = (output, history)
     or one value for both (number, callable or TSERIES number; default the
-    step): HISTORY lists are appended at each history interval.  Output
-    files are not written (the output interval and prefix are accepted)."""
+    step): HISTORY lists are appended at each history interval, and the
+    DUMP/VTK files OUTPUT asked for are written at each output interval
+    (under the deck's path without '.py', or the prefix).""" 
     if not duration > 0:
         raise ValueError("'duration' must be positive")
     if not step > 0:
@@ -961,8 +1081,29 @@ def DEM(duration, step, interval=None, prefix=None, adaptive=0.0):
         hisout = np.zeros((max(len(tsamp), 1), len(ents)))
         if not tsamp:
             hisrow = hisout = None
+    outsteps, on_output = None, None
+    if _S.outformat & (OUT_FORMAT_DUMP | OUT_FORMAT_VTK):
+        # output_files at the output points (parmec.cpp:2386-2392, 2502-2507):
+        # the state before the first step of the first call, then the end
+        # state of every step whose curtime reaches curtime_output + interval
+        path = _output_path(prefix)
+        if isinstance(interval, tuple):
+            if len(interval) != 2:
+                raise ValueError("Invalid output interval")
+            dt_out = _interval(interval[0], step)
+        else:
+            dt_out = _interval(interval, step)
+        if m.curtime == 0.0:
+            _write_output(m, 0.0, path)
+        orow, tout, _S.curtime_output = _history_rows(times, _S.curtime_output, dt_out)
+        outsteps = [k for k in range(len(times)) if orow[k] >= 0]
+
+        def on_output(k):
+            snap = m.copy()
+            _S.engine.download(snap)
+            _write_output(snap, float(times[k]), path)
     f = pmx.forcing(len(times), gravity=tuple(_S.gravity), gtab=gtab, dtab=dtab, obstab=obstab, obstab0=obstab0,
-                    prstab=prstab, hisrow=hisrow, hisout=hisout)
+                    prstab=prstab, hisrow=hisrow, hisout=hisout, outsteps=outsteps, on_output=on_output)
     elapsed = _S.engine.dem(float(duration), float(step), f)
     _S.engine.download(m)
     if hisrow is not None:

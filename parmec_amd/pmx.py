@@ -48,7 +48,12 @@ class pmx_model(C.Structure):
 class pmx_forcing(C.Structure):
     _fields_ = [("gravity", C.c_double * 3), ("gtab", _dp), ("damping", C.c_double * 6), ("dtab", _dp),
                 ("obstab", _dp), ("obstab0", _dp), ("prstab", _dp), ("prscols", C.c_int),
-                ("hisrow", _ip), ("hisrows", C.c_int), ("hisout", _dp)]
+                ("hisrow", _ip), ("hisrows", C.c_int), ("hisout", _dp),
+                ("outstep", _ip), ("noutstep", C.c_int), ("out_fn", C.c_void_p), ("out_user", C.c_void_p)]
+
+
+# pmx_forcing.out_fn: void (*)(void *user, int step)
+OUT_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int)
 
 
 # time-history entities of HISTORY (input.cpp:3177-3281), as PMX_HIS_* in pmx.h
@@ -292,7 +297,7 @@ class Model:
 
 
 def forcing(nsteps=0, gravity=(0.0, 0.0, 0.0), gtab=None, damping=(0.0,) * 6, dtab=None, obstab=None, obstab0=None,
-            prstab=None, hisrow=None, hisout=None):
+            prstab=None, hisrow=None, hisout=None, outsteps=None, on_output=None):
     """Per-step inputs of dem(): gravity and damping constants or tables
     ([nsteps][3], [nsteps][6]); moving obstacle velocities (angular 0-2,
     linear 3-5) per step at curtime+step ([nsteps][obsnum][6]) and at step/2
@@ -338,6 +343,14 @@ def forcing(nsteps=0, gravity=(0.0, 0.0, 0.0), gtab=None, damping=(0.0,) * 6, dt
         f.hisrows = hisout.shape[0]
         f.hisout = hisout.ctypes.data_as(_dp)
         keep.extend([hisrow, hisout])
+    if outsteps is not None and len(outsteps) and on_output is not None:
+        # pause after these steps of the call (0-based) and call on_output(step)
+        outsteps = np.ascontiguousarray(outsteps, dtype=np.int32)
+        cb = OUT_FN(lambda user, step: on_output(int(step)))
+        f.outstep = outsteps.ctypes.data_as(_ip)
+        f.noutstep = len(outsteps)
+        f.out_fn = C.cast(cb, C.c_void_p)
+        keep.extend([outsteps, cb])
     f._keep = keep
     return f
 
