@@ -330,6 +330,15 @@ int pmx_set_history (pmx_ctx *ctx, const pmx_history *h);
  * dem() takes before its first step when curtime is 0, parmec.cpp:2386-2394). */
 int pmx_history_sample (pmx_ctx *ctx, double *out);
 
+/* Energy and momentum of one instance's particles (instance -1: all of a
+ * single model or of a batch at one time), out[9]: translational and
+ * rotational kinetic energy, gravitational potential energy -sum m g.x for
+ * the gravity g (NULL: zero), linear momentum (3) and angular momentum about
+ * the origin (3), sum of x cross m v + L J Omega.  A diagnostic for energy
+ * and momentum histories; no reference counterpart (the reference's
+ * HISTORY has no energy entity, output.cpp:5143-5520). */
+int pmx_energy (pmx_ctx *ctx, int instance, const double *g, double *out);
+
 /* pmx_history_sample without device allocation.  No counterpart in the
  * reference: for callers that cannot allocate device memory (code under HIP
  * graph capture, tight serving loops).  The sample passes through one

@@ -255,6 +255,59 @@ int history_now (pmx_ctx &c, double *out)
   return PMX_OK;
 }
 
+/* Energy and momentum of particles [p0, p1): translational and rotational
+ * kinetic energy, gravitational potential energy -m g.x, linear momentum m v
+ * and angular momentum about the origin x cross m v + L J Omega.  Each block
+ * sums a fixed stride of particles per thread and reduces its threads in a
+ * fixed tree; k_energy_sum adds the blocks in order: the result is a
+ * function of the state alone. */
+constexpr int EN_TPB = 256, EN_BLOCKS = 1024, EN_Q = 9;
+
+__global__ void k_energy_part (int N, int p0, int p1, const double *position, const double *linear,
+                               const double *angular, const double *rotation, const double *inertia,
+                               const double *mass, double gx, double gy, double gz, double *part)
+{
+  __shared__ double sh[EN_Q][EN_TPB];
+  double q[EN_Q] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int i = p0 + blockIdx.x * EN_TPB + threadIdx.x; i < p1; i += EN_TPB * gridDim.x)
+  {
+    double m = mass[i];
+    double v[3] = {linear[i], linear[(size_t) N + i], linear[2*(size_t) N + i]};
+    double x[3] = {position[i], position[(size_t) N + i], position[2*(size_t) N + i]};
+    double O[3] = {angular[i], angular[(size_t) N + i], angular[2*(size_t) N + i]};
+    double J[9], L[9], JO[3], h[3];
+    for (int k = 0; k < 9; k++) { J[k] = inertia[(size_t) k*N + i]; L[k] = rotation[(size_t) k*N + i]; }
+    for (int r = 0; r < 3; r++) JO[r] = J[r]*O[0] + J[3+r]*O[1] + J[6+r]*O[2];
+    for (int r = 0; r < 3; r++) h[r] = L[r]*JO[0] + L[3+r]*JO[1] + L[6+r]*JO[2];
+    double p[3] = {m*v[0], m*v[1], m*v[2]};
+    q[0] += 0.5*m*(v[0]*v[0] + v[1]*v[1] + v[2]*v[2]);
+    q[1] += 0.5*(O[0]*JO[0] + O[1]*JO[1] + O[2]*JO[2]);
+    q[2] += -m*(gx*x[0] + gy*x[1] + gz*x[2]);
+    q[3] += p[0]; q[4] += p[1]; q[5] += p[2];
+    q[6] += x[1]*p[2] - x[2]*p[1] + h[0];
+    q[7] += x[2]*p[0] - x[0]*p[2] + h[1];
+    q[8] += x[0]*p[1] - x[1]*p[0] + h[2];
+  }
+  for (int k = 0; k < EN_Q; k++) sh[k][threadIdx.x] = q[k];
+  __syncthreads ();
+  for (int w = EN_TPB / 2; w > 0; w >>= 1)
+  {
+    if ((int) threadIdx.x < w)
+      for (int k = 0; k < EN_Q; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+    __syncthreads ();
+  }
+  if (threadIdx.x < EN_Q) part[(size_t) blockIdx.x * EN_Q + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ void k_energy_sum (int nb, const double *part, double *out)
+{
+  int k = threadIdx.x;
+  if (k >= EN_Q) return;
+  double s = 0.0;
+  for (int b = 0; b < nb; b++) s += part[(size_t) b * EN_Q + k];
+  out[k] = s;
+}
+
 /* the device address of scratch on the current device (once per device);
  * called by pmx_create with the context's device current */
 int history_scratch_init (int device)
@@ -292,6 +345,31 @@ extern "C" int pmx_history_sample_noalloc (pmx_ctx *c, double *out)
   launch_history (*c, nullptr, 0, buf);
   PMX_HIP (hipGetLastError ());
   PMX_HIP (hipMemcpyAsync (out, buf, sizeof (double) * c->nhis, hipMemcpyDeviceToHost, c->stream));
+  PMX_HIP (hipStreamSynchronize (c->stream));
+  return PMX_OK;
+}
+
+extern "C" int pmx_energy (pmx_ctx *c, int instance, const double *g, double *out)
+{
+  if (!c || !out) return fail (PMX_EINVAL, "null argument");
+  if (!c->uploaded) return fail (PMX_EINVAL, "no model uploaded");
+  if (instance < -1 || instance >= c->I) return fail (PMX_EINVAL, "instance out of range");
+  if (c->staggered && instance < 0)
+    return fail (PMX_EINVAL, "instances are at different times: ask for one instance");
+  (void) hipSetDevice (c->device);
+  int p0 = instance < 0 ? 0 : c->off[instance].par, p1 = instance < 0 ? c->N : c->off[instance + 1].par;
+  int nb = std::max (1, std::min (EN_BLOCKS, (p1 - p0 + EN_TPB - 1) / EN_TPB));
+  if (c->energy.alloc ((size_t) EN_BLOCKS * EN_Q + EN_Q) != hipSuccess) return fail (PMX_ENOMEM, "out of device memory");
+  double gg[3] = {g ? g[0] : 0.0, g ? g[1] : 0.0, g ? g[2] : 0.0};
+  hipLaunchKernelGGL (k_energy_part, dim3 (nb), dim3 (EN_TPB), 0, c->stream, c->N, p0, p1,
+                      (const double*) c->position.p, (const double*) c->linear.p, (const double*) c->angular.p,
+                      (const double*) c->rotation.p, (const double*) c->inertia.p, (const double*) c->mass.p,
+                      gg[0], gg[1], gg[2], c->energy.p);
+  hipLaunchKernelGGL (k_energy_sum, dim3 (1), dim3 (64), 0, c->stream, nb, (const double*) c->energy.p,
+                      c->energy.p + (size_t) EN_BLOCKS * EN_Q);
+  PMX_HIP (hipGetLastError ());
+  PMX_HIP (hipMemcpyAsync (out, c->energy.p + (size_t) EN_BLOCKS * EN_Q, sizeof (double) * EN_Q, hipMemcpyDeviceToHost,
+                           c->stream));
   PMX_HIP (hipStreamSynchronize (c->stream));
   return PMX_OK;
 }

@@ -250,6 +250,7 @@ struct pmx_ctx
   dvec<int> his_ent, his_off, his_list, hisrow;
   dvec<signed char> his_pt;
   dvec<double> his_pts, hisbuf;
+  dvec<double> energy;         /* pmx_energy: per-block partial sums and the result */
   int *hring = nullptr;        /* pinned: per group of steps in flight, the control block and the flags */
   hipEvent_t rev[PMX_RING] = {};   /* recorded after each group's copy into hring */
   int det_nw = 4;              /* waves per k_detect block (autotuned) */

@@ -946,25 +946,28 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
       col1 = ellcol[es];
     }
 
-    /* contacts_task velocities, forces.ispc:219-287 */
+    /* contacts_task velocities, forces.ispc:219-287: v = omega x z + v at
+     * the contact point for both bodies, then v_j - v_i */
     double xi[3] = {position[i], position[(size_t) N + i], position[2*(size_t) N + i]};
     R oi[3] = {(R) angular[3*(size_t) N + i], (R) angular[4*(size_t) N + i], (R) angular[5*(size_t) N + i]};
-    R vi[3], vj[3], z[3];
+    R ri[3], rj[3] = {0.0, 0.0, 0.0}, z[3];   /* omega x z of body i and of its partner */
+    double li[3] = {linear[i], linear[(size_t) N + i], linear[2*(size_t) N + i]}, lj[3] = {0.0, 0.0, 0.0};
     z[0] = (R) (point[0]-xi[0]);
     z[1] = (R) (point[1]-xi[1]);
     z[2] = (R) (point[2]-xi[2]);
-    vi[0] = oi[1]*z[2]-oi[2]*z[1] + (R) linear[i];
-    vi[1] = oi[2]*z[0]-oi[0]*z[2] + (R) linear[(size_t) N + i];
-    vi[2] = oi[0]*z[1]-oi[1]*z[0] + (R) linear[2*(size_t) N + i];
+    ri[0] = oi[1]*z[2]-oi[2]*z[1];
+    ri[1] = oi[2]*z[0]-oi[0]*z[2];
+    ri[2] = oi[0]*z[1]-oi[1]*z[0];
     if (j >= 0)
     {
       z[0] = (R) (point[0]-position[j]);
       z[1] = (R) (point[1]-position[(size_t) N + j]);
       z[2] = (R) (point[2]-position[2*(size_t) N + j]);
       R oj[3] = {(R) angular[3*(size_t) N + j], (R) angular[4*(size_t) N + j], (R) angular[5*(size_t) N + j]};
-      vj[0] = oj[1]*z[2]-oj[2]*z[1] + (R) linear[j];
-      vj[1] = oj[2]*z[0]-oj[0]*z[2] + (R) linear[(size_t) N + j];
-      vj[2] = oj[0]*z[1]-oj[1]*z[0] + (R) linear[2*(size_t) N + j];
+      rj[0] = oj[1]*z[2]-oj[2]*z[1];
+      rj[1] = oj[2]*z[0]-oj[0]*z[2];
+      rj[2] = oj[0]*z[1]-oj[1]*z[0];
+      lj[0] = linear[j]; lj[1] = linear[(size_t) N + j]; lj[2] = linear[2*(size_t) N + j];
     }
     else if (j <= -2)   /* moving obstacle, forces.ispc:265-280 */
     {
@@ -973,12 +976,30 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
       z[1] = (R) (point[1]-obspnt[3*l+1]);
       z[2] = (R) (point[2]-obspnt[3*l+2]);
       R oj[3] = {(R) obsang[3*l], (R) obsang[3*l+1], (R) obsang[3*l+2]};
-      vj[0] = oj[1]*z[2]-oj[2]*z[1] + (R) obslin[3*l];
-      vj[1] = oj[2]*z[0]-oj[0]*z[2] + (R) obslin[3*l+1];
-      vj[2] = oj[0]*z[1]-oj[1]*z[0] + (R) obslin[3*l+2];
+      rj[0] = oj[1]*z[2]-oj[2]*z[1];
+      rj[1] = oj[2]*z[0]-oj[0]*z[2];
+      rj[2] = oj[0]*z[1]-oj[1]*z[0];
+      lj[0] = obslin[3*l]; lj[1] = obslin[3*l+1]; lj[2] = obslin[3*l+2];
     }
-    else { vj[0] = vj[1] = vj[2] = 0.0; }
-    R vij[3] = {vj[0]-vi[0], vj[1]-vi[1], vj[2]-vi[2]};
+    R vij[3];
+    if constexpr (mixed)
+    {
+      /* the translational parts differenced in FP64 before narrowing: two
+       * bodies moving together (a pack falling or shaken as a whole) would
+       * otherwise lose the digits of their small relative velocity to
+       * cancellation in FP32 */
+      for (int k = 0; k < 3; k++) vij[k] = (R) (lj[k] - li[k]) + (rj[k] - ri[k]);
+    }
+    else
+    {
+      /* the reference's operation order: (omega x z + v) per body, then the
+       * difference (a static partner's velocity is +0) */
+      R vi[3], vj[3];
+      for (int k = 0; k < 3; k++) vi[k] = ri[k] + (R) li[k];
+      if (j == -1) { vj[0] = vj[1] = vj[2] = 0.0; }
+      else for (int k = 0; k < 3; k++) vj[k] = rj[k] + (R) lj[k];
+      for (int k = 0; k < 3; k++) vij[k] = vj[k] - vi[k];
+    }
     int ij = po + pairing (pn, pairs + 2*(size_t) po, col0, col1);
     if (ikind[ij] != PMX_GRANULAR_FORCE) atomicAdd (&flags[FLAG_BAD_KIND], 1);
     double ma = j >= 0 ? 1.0 / (invm[i] + invm[j]) : mass[i];

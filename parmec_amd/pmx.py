@@ -394,6 +394,16 @@ class Engine:
         self._call("set_history", self.h, C.byref(h))
         self._nhis = n
 
+    def energy(self, instance=-1, gravity=(0.0, 0.0, 0.0)):
+        """pmx_energy: (kinetic translational, kinetic rotational, potential,
+        momentum x y z, angular momentum x y z) of the resident state."""
+        out = np.zeros(9)
+        g = np.ascontiguousarray(gravity, dtype=np.float64)
+        f = getattr(self.lib, self.prefix + "energy")
+        f.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        self._call("energy", self.h, int(instance), g.ctypes.data_as(_dp), out.ctypes.data_as(_dp))
+        return out
+
     def history_sample(self):
         """The histories' values of the current state (pmx_history_sample)."""
         out = np.zeros(max(getattr(self, "_nhis", 0), 1))

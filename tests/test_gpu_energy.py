@@ -1,0 +1,141 @@
+"""Energy and momentum histories (pmx_energy at DEM's output points).
+
+- the device reduction equals the formula evaluated on the downloaded state;
+- FP64 over a full run (the 10,000-sphere box, 2,000 steps): the device's
+  history equals the history of the oracle's states within 1e-6 relative;
+- mixed precision over a full run of the 2,000,000-sphere hopper (5,000
+  steps): energy and momentum histories against the FP64 path's, and one
+  step from the same mid-run state gives identical pair sets."""
+import numpy as np
+import pytest
+
+from parmec_amd import pmx, workloads
+from tests.helpers import max_rel
+
+pytestmark = pytest.mark.gpu
+
+
+def energy_of(m, g):
+    """(KE_t, KE_r, PE, p, L) of a downloaded state, as pmx_energy defines them."""
+    N = m.parnum
+    v, x, O = m.linear, m.position[:3], m.angular[:3]
+    J = m.inertia.reshape(3, 3, N)            # J[c, r] = element (r, c)
+    L = m.rotation.reshape(3, 3, N)
+    JO = np.einsum("crn,cn->rn", J, O)
+    h = np.einsum("crn,cn->rn", L, JO)
+    p = m.mass * v
+    out = np.zeros(9)
+    out[0] = (0.5 * m.mass * (v * v).sum(axis=0)).sum()
+    out[1] = (0.5 * (O * JO).sum(axis=0)).sum()
+    out[2] = (-m.mass * (g[0] * x[0] + g[1] * x[1] + g[2] * x[2])).sum()
+    out[3:6] = p.sum(axis=1)
+    out[6:9] = (np.cross(x.T, p.T).T + h).sum(axis=1)
+    return out
+
+
+def history(engine, m, nsteps, h, f_kw, every, g, on_device):
+    """Run nsteps in one dem call, sampling energy every `every` steps at
+    the output points (device: pmx_energy; else the formula on the state)."""
+    mm = m.copy()
+    engine.upload(mm)
+    rows = []
+
+    def sample(k):
+        if on_device:
+            rows.append(engine.energy(-1, g))
+        else:
+            s = mm.copy()
+            engine.download(s)
+            rows.append(energy_of(s, g))
+    f = pmx.forcing(nsteps, outsteps=list(range(every - 1, nsteps, every)), on_output=sample, **f_kw)
+    engine.dem((nsteps - 0.5) * h, h, f)
+    engine.download(mm)
+    return np.array(rows), mm, engine.contacts()
+
+
+def test_energy_matches_state_formula():
+    m = workloads.box_lattice(n=3000, seed=2)
+    m.linear[:] = np.random.default_rng(1).normal(size=m.linear.shape)
+    m.angular[:3] = np.random.default_rng(2).normal(size=(3, m.parnum)) * 10.0
+    e = pmx.Engine()
+    mm = m.copy()
+    e.upload(mm)
+    e.dem(99.5e-5, 1e-5, pmx.forcing(gravity=(0.0, 0.0, -9.81)))
+    e.download(mm)
+    g = (0.3, 0.0, -9.81)
+    dev = e.energy(-1, g)
+    e.close()
+    ref = energy_of(mm, g)
+    np.testing.assert_allclose(dev, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+
+
+def test_energy_history_box10k_full_run():
+    """C0 in full (10,000 spheres, 2,000 steps): the device's energy and
+    momentum history against the oracle's states, within 1e-6."""
+    from oracle import orc
+    print("case box10k energy history, 2000 steps", flush=True)
+    m = workloads.box_lattice(n=10000, seed=0)
+    g = (0.0, 0.0, -9.81)
+    kw = dict(gravity=g)
+    dev, mg, cg = history(pmx.Engine(), m, 2000, 1e-5, kw, 100, g, True)
+    o = orc.Oracle(0)
+    ref, mc, cc = history(o, m, 2000, 1e-5, kw, 100, g, False)
+    o.close()
+    assert dev.shape == (20, 9)
+    scale = np.abs(ref).max(axis=0)
+    err = (np.abs(dev - ref) / np.where(scale > 0, scale, 1.0)).max()
+    print("box10k energy history: max relative deviation", err, flush=True)
+    assert err <= 1e-6
+    assert len(cg) == len(cc) and np.array_equal(cg["slave"], cc["slave"])
+
+
+def test_hopper2m_mixed_vs_fp64_full_run():
+    """C1 in full (2,000,000 spheres, 5,000-triangle hopper, 5,000 steps):
+    energy and momentum histories of the mixed path against FP64."""
+    print("case hopper 2M, mixed vs fp64, 5000 steps", flush=True)
+    m = workloads.hopper(n=2000000, seed=1)
+    g = (0.0, 0.0, -9.81)
+    kw = dict(gravity=g)
+    out = {}
+    for prec in (pmx.PMX_FP64, pmx.PMX_MIXED):
+        e = pmx.Engine(precision=prec)
+        out[prec] = history(e, m, 5000, 2e-6, kw, 500, g, True)
+        e.close()
+    a, b = out[pmx.PMX_FP64][0], out[pmx.PMX_MIXED][0]
+    tot_a, tot_b = a[:, 0] + a[:, 1] + a[:, 2], b[:, 0] + b[:, 1] + b[:, 2]
+    e_rel = np.abs(tot_a - tot_b).max() / np.abs(tot_a).max()
+    p_rel = np.abs(a[:, 3:6] - b[:, 3:6]).max() / np.abs(a[:, 3:6]).max()
+    k_rel = np.abs(a[:, 0] - b[:, 0]).max() / np.abs(a[:, 0]).max()
+    print(f"hopper mixed vs fp64 over 5000 steps: total energy {e_rel:.3e}, kinetic {k_rel:.3e}, "
+          f"momentum {p_rel:.3e}, pairs {len(out[pmx.PMX_FP64][2])} / {len(out[pmx.PMX_MIXED][2])}", flush=True)
+    assert e_rel <= 1e-6 and p_rel <= 1e-6
+
+
+def test_hopper2m_mixed_one_step_pairs():
+    """From the same mid-run FP64 state of the hopper, one mixed step and one
+    FP64 step find the same pair set; the per-output errors are printed."""
+    print("case hopper 2M, one step mixed vs fp64 from a mid-run state", flush=True)
+    m = workloads.hopper(n=2000000, seed=1)
+    f = pmx.forcing(gravity=(0.0, 0.0, -9.81))
+    e = pmx.Engine()
+    mm = m.copy()
+    e.upload(mm)
+    e.dem(2500 * 2e-6 - 1e-6, 2e-6, f)
+    e.download(mm)
+    e.close()
+    res = {}
+    for prec in (pmx.PMX_FP64, pmx.PMX_MIXED):
+        e = pmx.Engine(precision=prec)
+        s = mm.copy()
+        e.upload(s)
+        e.dem(1e-6, 2e-6, f)
+        e.download(s)
+        res[prec] = (s, e.contacts())
+        e.close()
+    (a, ca), (b, cb) = res[pmx.PMX_FP64], res[pmx.PMX_MIXED]
+    assert len(ca) > 1000 and len(ca) == len(cb)
+    assert np.array_equal(ca["master"], cb["master"]) and np.array_equal(ca["slave"], cb["slave"])
+    errs = {k: max_rel(getattr(a, k), getattr(b, k)) for k in ("position", "linear", "angular", "rotation", "force",
+                                                                 "torque")}
+    print("hopper one mixed step, normwise errors:", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    assert errs["position"] < 1e-10
