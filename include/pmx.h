@@ -147,6 +147,8 @@ typedef struct pmx_model
  * them as tables evaluated at each step's curtime (pmx_step_times gives the
  * times), or as constants when the table pointer is NULL.
  */
+struct pmx_step_inputs;
+
 typedef struct pmx_forcing
 {
   double gravity[3];         /* constant gravity when gtab == NULL */
@@ -171,7 +173,26 @@ typedef struct pmx_forcing
   int noutstep;
   void (*out_fn) (void *user, int step);
   void *out_user;
+  /* pmx_dem_adaptive only (the steps and their times are known one at a
+   * time): called before step k with that step's curtime and length step0,
+   * it fills the step's host inputs; NULL: gravity and damping above, still
+   * obstacles, no prescribed values, no history sample, no output pause */
+  int (*step_fn) (void *user, int k, double curtime, double step0, struct pmx_step_inputs *in);
+  void *step_user;
 } pmx_forcing;
+
+/* the host inputs of one step of an adaptive dem call (read_gravity_and_damping,
+ * obstaclev, prescribe_*, the history and output rules at that step's curtime;
+ * parmec.cpp:2432, 2449-2509) */
+typedef struct pmx_step_inputs
+{
+  double gravity[3];
+  double damping[6];
+  double *obsvel;            /* [obsnum][6] obstacle (angular, linear) velocity at curtime + step0 */
+  double *prs;               /* [prscols][6] prescribed values of each history column at curtime */
+  int hisrow;                /* row of hisout that samples this step's end state, or -1 */
+  int output;                /* 1: pause after this step and call out_fn (as at an output point) */
+} pmx_step_inputs;
 
 /* time-history entities (HISTORY, input.cpp:3157-3555); particle entities
  * average over their list (or take one particle's body point), spring
@@ -206,6 +227,10 @@ typedef struct pmx_options
   int det_waves;       /* tune = 1: waves per detection block, 1, 2, 4 or 8 (0: 4) */
   int rows_block;      /* spheres per contact-row block (per list of spheres in contact), 64, 128,
                           256, 512 or 1024 (0: 1024) */
+  int critical;        /* 1: every step records its contacts (point, direction, stiffness and
+                          damping coefficient per row entry, 64 B each), for pmx_critical after
+                          a dem call; 0: pmx_critical's per-particle estimate is refused once
+                          steps were taken (it needs the last step's contacts) */
 } pmx_options;
 
 /* one contact of the canonical pair set */
@@ -349,6 +374,27 @@ int pmx_energy (pmx_ctx *ctx, int instance, const double *g, double *out);
 int pmx_history_sample_noalloc (pmx_ctx *ctx, double *out);
 
 void pmx_destroy (pmx_ctx *ctx);
+
+/* critical time step estimates, CRITICAL (input.cpp:3034-3154): per particle
+ * (critical_perparticle, parmec.ispc:537-888; the contacts of the last step
+ * taken and the springs) or per spring (critical_perspring,
+ * parmec.ispc:485-535); h, o, r receive N or sprnum values: the step
+ * estimate, the circular frequency and the damping ratio (o, r may be
+ * NULL).  A particle that nothing loads has h = DBL_MAX.  The caller sorts
+ * them and takes the smallest, as CRITICAL does.  Single-model contexts. */
+enum { PMX_CRIT_PARTICLE = 0, PMX_CRIT_SPRING = 1 };
+int pmx_critical (pmx_ctx *ctx, int which, double *h, double *o, double *r);
+
+/* dem() with adaptive in (0, 1] (parmec.cpp:2315-2530, 2444-2447): every
+ * step estimates each particle's critical step from what loads it in that
+ * step (contacts_acc_task and the springs, forces.ispc:453-596, 1203-1411;
+ * time_step_task, parmec.ispc:275-420) and takes step1 = adaptive times the
+ * smallest, or step0 when no particle is loaded; the step's dynamics and
+ * obstacles use step0 and step1 and time advances by (step0+step1)/2.  The
+ * step count is known only as the call runs, so the host inputs come from
+ * forcing->step_fn step by step.  Single-model contexts. */
+int pmx_dem_adaptive (pmx_ctx *ctx, double duration, double step, double adaptive, const pmx_forcing *forcing,
+                      double *elapsed);
 
 const char *pmx_last_error (void);
 

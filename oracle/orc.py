@@ -41,6 +41,14 @@ def load():
         lib.orc_tree_reach.argtypes = [C.c_int, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double),
                                        C.POINTER(C.c_longlong), C.POINTER(C.c_int), C.c_longlong]
         lib.orc_tree_reach.restype = C.c_longlong
+        lib.orc_trig.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]
+        lib.orc_trig.restype = None
+        lib.orc_time_step.argtypes = [C.POINTER(C.c_double), C.c_double, C.c_double, C.POINTER(C.c_double),
+                                      C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double)]
+        lib.orc_time_step.restype = C.c_double
+        lib.orc_extremum_eigenvalue.argtypes = [C.POINTER(C.c_double), C.c_int]
+        lib.orc_extremum_eigenvalue.restype = C.c_double
         _lib = lib
     return _lib
 
@@ -55,3 +63,33 @@ class Oracle(pmx.Engine):
         h = C.c_void_p()
         self._call("create", nthreads, C.byref(h))
         self.h = h
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def trig(fn, x):
+    """pmx_trig.h's sin (fn 0), cos (1) or acos (2) of an array."""
+    import numpy as np
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(x)
+    load().orc_trig(int(fn), _dp(x), _dp(y), int(x.size))
+    return y
+
+
+def time_step(kact, kmax, emax, krot, mass, inertia):
+    """time_step_task for one particle: (h, omega, ratio)."""
+    import numpy as np
+    ka = np.ascontiguousarray(kact, dtype=np.float64)
+    kr = np.ascontiguousarray(krot, dtype=np.float64)
+    J = np.ascontiguousarray(inertia, dtype=np.float64).reshape(9)
+    o, r = C.c_double(0.0), C.c_double(0.0)
+    h = load().orc_time_step(_dp(ka), float(kmax), float(emax), _dp(kr), float(mass), _dp(J), C.byref(o), C.byref(r))
+    return h, o.value, r.value
+
+
+def extremum_eigenvalue(A, minimum):
+    import numpy as np
+    A = np.ascontiguousarray(A, dtype=np.float64).reshape(9)
+    return load().orc_extremum_eigenvalue(_dp(A), int(minimum))

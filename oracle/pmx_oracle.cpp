@@ -18,6 +18,10 @@
  * every result is independent of the thread count.
  */
 #include "pmx_oracle.h"
+/* sin, cos and acos from +, -, *, / and sqrt, shared with the device path so
+ * that the two agree bit for bit where the reference calls them (pinned
+ * against libm in tests/test_oracle_trig.py) */
+#include "../parmec_amd/csrc/pmx_trig.h"
 
 #include <algorithm>
 #include <chrono>
@@ -66,7 +70,7 @@ void expmap (double O1, double O2, double O3, double L[9])
   {
     double t = angsq;
     angsq = std::sqrt (angsq);
-    double s = std::sin (angsq), c = std::cos (angsq);
+    double s = pmxt::sin (angsq), c = pmxt::cos (angsq);
     sx = s / angsq;
     cx = (1.0 - c) / t;
   }
@@ -394,6 +398,7 @@ struct Contact
 {
   int master, slave0, slave1, color0, color1;
   double point[3], normal[3], depth, force[3], state[3];
+  double kcur = 0.0, ecur = 0.0;   /* stiffness and damping coefficient of the last force (forces.ispc:318-319) */
 };
 
 struct Slave
@@ -463,6 +468,10 @@ struct orc_ctx
   int stepnum = 0;
   std::vector<std::vector<Contact>> master;
   std::vector<std::vector<Slave>> slave;
+  /* adaptive step: forces and springs form per particle kact[3], kmax,
+   * emax, krot[6] (forces.ispc:453-596, 1203-1411) as adp [11][parnum] */
+  bool adaptive = false;
+  std::vector<double> adp;
   pmx_stats stats {};
 
   double *ang (int k) { return angular.data () + (size_t) k * parnum; }
@@ -832,7 +841,7 @@ int test_triangles (orc_ctx &s, const Tree &T)
 
 /* granural_force, forces.ispc:125-192 */
 int granular_force (const double n[3], const double vij[3], double depth, int i, int j,
-                    const orc_ctx &s, int ij, double state[3], double step, double f[3])
+                    const orc_ctx &s, int ij, double state[3], double step, double f[5])
 {
   double ma;
   if (j >= 0) ma = 1.0 / (s.invm[i] + s.invm[j]);
@@ -846,6 +855,8 @@ int granular_force (const double n[3], const double vij[3], double depth, int i,
   f[0] = fn*n[0];
   f[1] = fn*n[1];
   f[2] = fn*n[2];
+  f[3] = kn;
+  f[4] = en;
   double kt = kn * ip[PMX_KSKN*np + ij];
   double et = en * ip[PMX_KSKN*np + ij];
   double mus = ip[PMX_FRISTAT*np + ij];
@@ -929,9 +940,10 @@ int forces (orc_ctx &s, double step, const double g[3])
       vij[2] = vj[2] - vi[2];
       int ij = pairing (s.pairnum, s.pairs.data (), con.color0, con.color1);
       if (s.ikind[ij] != PMX_GRANULAR_FORCE) { bad++; continue; }
-      double f[3];
+      double f[5];
       int gone = granular_force (con.normal, vij, con.depth, i, j, s, ij, con.state, step, f);
       con.force[0] = f[0]; con.force[1] = f[1]; con.force[2] = f[2];
+      con.kcur = f[3]; con.ecur = f[4];
       if (!gone) kept.push_back (con);   /* canonical: state moves with its pair */
     }
     ml.swap (kept);
@@ -954,15 +966,19 @@ int forces (orc_ctx &s, double step, const double g[3])
   #pragma omp parallel for num_threads(s.nthreads) schedule(dynamic, 256)
   for (int i = 0; i < np; i++)
   {
-    struct E { unsigned long long k0, k1; const double *p; double f[3]; };
+    struct E { unsigned long long k0, k1; const double *p; double f[3]; const double *n; double kc, ec; };
     std::vector<E> es;
     for (const Contact &con : s.master[i])
-      es.push_back ({side_key (con.slave1), side_key (con.master), con.point, {con.force[0], con.force[1], con.force[2]}});
-    for (const Slave &sv : s.slave[i])
-      es.push_back ({side_key (sv.master1), side_key (sv.slave1), sv.point, {sv.force[0], sv.force[1], sv.force[2]}});
+      es.push_back ({side_key (con.slave1), side_key (con.master), con.point, {con.force[0], con.force[1], con.force[2]},
+                     con.normal, con.kcur, con.ecur});
+    for (const Slave &sv : s.slave[i])   /* slave kcur, ecur: never written by the reference, 0 here */
+      es.push_back ({side_key (sv.master1), side_key (sv.slave1), sv.point, {sv.force[0], sv.force[1], sv.force[2]},
+                     nullptr, 0.0, 0.0});
     std::sort (es.begin (), es.end (), [] (const E &a, const E &b) { return a.k0 < b.k0 || (a.k0 == b.k0 && a.k1 < b.k1); });
     double po[3] = {s.pos(0)[i], s.pos(1)[i], s.pos(2)[i]};
     double fs[3] = {0.0, 0.0, 0.0}, ts[3] = {0.0, 0.0, 0.0};
+    double kact0[3] = {0.0, 0.0, 0.0}, kmax0 = 0.0, emax0 = 0.0, krot0[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, L[9];
+    if (s.adaptive) for (int c = 0; c < 9; c++) L[c] = s.rot(c)[i];
     for (const E &e : es)
     {
       double a[3] = {e.p[0]-po[0], e.p[1]-po[1], e.p[2]-po[2]};
@@ -970,6 +986,42 @@ int forces (orc_ctx &s, double step, const double g[3])
       ts[0] += a[1]*e.f[2] - a[2]*e.f[1];
       ts[1] += a[2]*e.f[0] - a[0]*e.f[2];
       ts[2] += a[0]*e.f[1] - a[1]*e.f[0];
+      if (s.adaptive)   /* forces.ispc:505-533 (master), 542-577 (slave) */
+      {
+        double d[3];
+        if (e.n) for (int c = 0; c < 3; c++) d[c] = e.n[c];
+        else
+        {
+          double f[3] = {e.f[0], e.f[1], e.f[2]};
+          double len = std::sqrt (f[0]*f[0] + f[1]*f[1] + f[2]*f[2]);
+          if (len != 0.0)
+          {
+            len = 1.0/len;
+            f[0] *= len; f[1] *= len; f[2] *= len;
+          }
+          else f[0] = f[1] = f[2] = 0.0;
+          for (int c = 0; c < 3; c++) d[c] = f[c];
+        }
+        kact0[0] += std::fabs (d[0]);
+        kact0[1] += std::fabs (d[1]);
+        kact0[2] += std::fabs (d[2]);
+        kmax0 = e.kc > kmax0 ? e.kc : kmax0;
+        emax0 = e.ec > emax0 ? e.ec : emax0;
+        double A[3];
+        nvmul (L, a, A);
+        double dot = A[0]*A[0] + A[1]*A[1] + A[2]*A[2];
+        krot0[0] += dot - A[0]*A[0];
+        krot0[1] += dot - A[1]*A[1];
+        krot0[2] += dot - A[2]*A[2];
+        krot0[3] += -A[0]*A[1];
+        krot0[4] += -A[0]*A[2];
+        krot0[5] += -A[1]*A[2];
+      }
+    }
+    if (s.adaptive)
+    {
+      double v[11] = {kact0[0], kact0[1], kact0[2], kmax0, emax0, krot0[0], krot0[1], krot0[2], krot0[3], krot0[4], krot0[5]};
+      for (int c = 0; c < 11; c++) s.adp[(size_t) c*np + i] = v[c];
     }
     double ma = s.mass[i];
     s.force[(size_t) 0*np + i] = fs[0] + ma * g[0];
@@ -1108,14 +1160,16 @@ inline void tnmul (const double A[9], const double B[9], double C[9])
       C[3*c+r] = A[3*r]*B[3*c] + A[3*r+1]*B[3*c+1] + A[3*r+2]*B[3*c+2];
 }
 
-/* smallest eigenvalue of a symmetric 3x3 matrix, macros.h:1538-1593 */
-double min_eigenvalue (const double A[9])
+/* extremum_eigenvalue, macros.h:1538-1593: the largest eigenvalue of a
+ * symmetric 3x3 matrix, or with `minimum` the smallest; a diagonal matrix
+ * returns its largest diagonal entry either way, as the reference does */
+double extremum_eigenvalue (const double A[9], bool minimum)
 {
   double p1 = A[3]*A[3] + A[6]*A[6] + A[7]*A[7];
   if (p1 == 0.0)
   {
     double r = A[0] > A[4] ? A[0] : A[4];
-    return r > A[8] ? r : A[8];   /* the reference returns the maximum here */
+    return r > A[8] ? r : A[8];
   }
   double q = (A[0]+A[4]+A[8])/3.0;
   double p2 = (A[0]-q)*(A[0]-q) + (A[4]-q)*(A[4]-q) + (A[8]-q)*(A[8]-q) + 2*p1;
@@ -1128,8 +1182,9 @@ double min_eigenvalue (const double A[9])
   double phi;
   if (r <= -1.0) phi = 3.14159265358979323846 / 3.0;
   else if (r >= 1.0) phi = 0.0;
-  else phi = std::acos (r) / 3.0;
-  return q + 2.0 * p * std::cos (phi + (2*3.14159265358979323846/3));
+  else phi = pmxt::acos (r) / 3.0;
+  if (minimum) return q + 2.0 * p * pmxt::cos (phi + (2*3.14159265358979323846/3));
+  return q + 2.0 * p * pmxt::cos (phi);
 }
 
 /* equivalent point mass along a direction, parmec.ispc:423-482 (point1 and
@@ -1166,7 +1221,7 @@ double eqm (const orc_ctx &s, int part1, const double point1[3], int part2, cons
     nvmul (Miv, dir, tmp);
     return 1.0/(tmp[0]*dir[0] + tmp[1]*dir[1] + tmp[2]*dir[2]);
   }
-  return 1.0/min_eigenvalue (Miv);
+  return 1.0/extremum_eigenvalue (Miv, true);
 }
 
 /* springs_task, forces.ispc:641-1412, run as one task: per spring force and
@@ -1180,6 +1235,38 @@ int springs (orc_ctx &s, double step)
   double *force = s.force.data (), *torque = s.torque.data ();
   double fj[3] = {0, 0, 0}, tj[3] = {0, 0, 0}, fk[3] = {0, 0, 0}, tk[3] = {0, 0, 0};
   int j0 = -1, k0 = -1;
+  /* adaptive step: the run sums of forces.ispc:725-730, flushed with the
+   * forces (1216-1241, 1288-1321, 1377-1411) */
+  const bool adaptive = s.adaptive;
+  double *adp = adaptive ? s.adp.data () : nullptr;
+  double kact_j[3] = {0, 0, 0}, kact_k[3] = {0, 0, 0}, kmax_j = 0.0, kmax_k = 0.0, emax_j = 0.0, emax_k = 0.0;
+  double krot_j[6] = {0, 0, 0, 0, 0, 0}, krot_k[6] = {0, 0, 0, 0, 0, 0};
+  auto flush = [&] (int b, double ka[3], double &km, double &em, double kr[6]) {
+    for (int c = 0; c < 3; c++) adp[c*N + b] += ka[c];
+    adp[3*N + b] = km > adp[3*N + b] ? km : adp[3*N + b];
+    adp[4*N + b] = em > adp[4*N + b] ? em : adp[4*N + b];
+    for (int c = 0; c < 6; c++) adp[(5+c)*N + b] += kr[c];
+  };
+  auto reset = [] (double ka[3], double &km, double &em, double kr[6]) {
+    ka[0] = ka[1] = ka[2] = 0.0;
+    km = em = 0.0;
+    for (int c = 0; c < 6; c++) kr[c] = 0.0;
+  };
+  auto addterms = [] (const double L[9], const double arm[3], const double z[3], double stf, double dmp,
+                      double ka[3], double &km, double &em, double kr[6]) {
+    for (int c = 0; c < 3; c++) ka[c] += std::fabs (z[c]);
+    km = stf > km ? stf : km;
+    em = dmp > em ? dmp : em;
+    double A[3];
+    tvmul (L, arm, A);
+    double dot = A[0]*A[0] + A[1]*A[1] + A[2]*A[2];
+    kr[0] += dot-A[0]*A[0];
+    kr[1] += dot-A[1]*A[1];
+    kr[2] += dot-A[2]*A[2];
+    kr[3] += -A[0]*A[1];
+    kr[4] += -A[0]*A[2];
+    kr[5] += -A[1]*A[2];
+  };
   double oj[3] = {}, vj[3] = {}, xj[3] = {}, Xj[3] = {}, Lj[9] = {}, ok[3] = {}, vk[3] = {}, xk[3] = {}, Xk[3] = {}, Lk[9] = {};
   const double *sx = s.spring.data (), *sy = s.spring.data () + s.sprcrv;
   const double *dx = s.dashpot.data (), *dy = s.dashpot.data () + s.dashcrv;
@@ -1408,9 +1495,16 @@ int springs (orc_ctx &s, double step)
     if (j0 != j && j0 >= 0)
     {
       for (int c = 0; c < 3; c++) { force[c*N + j0] += fj[c]; torque[c*N + j0] += tj[c]; fj[c] = frc[c]; tj[c] = trq[c]; }
+      if (adaptive) { flush (j0, kact_j, kmax_j, emax_j, krot_j); reset (kact_j, kmax_j, emax_j, krot_j); }
     }
     else
       for (int c = 0; c < 3; c++) { fj[c] += frc[c]; tj[c] += trq[c]; }
+    if (adaptive && total_force != 0.0)
+    {
+      stiffness = std::fabs (stiffness);
+      damping = std::fabs (damping);
+      addterms (Lj, a, z, stiffness, damping, kact_j, kmax_j, emax_j, krot_j);
+    }
     if (k >= 0)
     {
       trq[0] = b[1]*frc[2] - b[2]*frc[1];
@@ -1419,16 +1513,275 @@ int springs (orc_ctx &s, double step)
       if (k0 != k && k0 >= 0)
       {
         for (int c = 0; c < 3; c++) { force[c*N + k0] -= fk[c]; torque[c*N + k0] -= tk[c]; fk[c] = frc[c]; tk[c] = trq[c]; }
+        if (adaptive) { flush (k0, kact_k, kmax_k, emax_k, krot_k); reset (kact_k, kmax_k, emax_k, krot_k); }
       }
       else
         for (int c = 0; c < 3; c++) { fk[c] += frc[c]; tk[c] += trq[c]; }
+      /* part2's arm rotated by part1's L, as forces.ispc:1343 does */
+      if (adaptive && total_force != 0.0) addterms (Lj, b, z, stiffness, damping, kact_k, kmax_k, emax_k, krot_k);
       k0 = k;
     }
     j0 = j;
   }
-  if (j0 >= 0) for (int c = 0; c < 3; c++) { force[c*N + j0] += fj[c]; torque[c*N + j0] += tj[c]; }
-  if (k0 >= 0) for (int c = 0; c < 3; c++) { force[c*N + k0] -= fk[c]; torque[c*N + k0] -= tk[c]; }
+  if (j0 >= 0)
+  {
+    for (int c = 0; c < 3; c++) { force[c*N + j0] += fj[c]; torque[c*N + j0] += tj[c]; }
+    if (adaptive) flush (j0, kact_j, kmax_j, emax_j, krot_j);
+  }
+  if (k0 >= 0)
+  {
+    for (int c = 0; c < 3; c++) { force[c*N + k0] -= fk[c]; torque[c*N + k0] -= tk[c]; }
+    if (adaptive) flush (k0, kact_k, kmax_k, emax_k, krot_k);
+  }
   return 0;
+}
+
+/* ------------------------------------------------- critical step estimates */
+
+/* time_step_task, parmec.ispc:275-420, for one particle; min and max as
+ * ISPC's (a < b ? a : b, a > b ? a : b); INVERT leaves the inverse as it
+ * was where the determinant is zero (zeros here) */
+double time_step (const double kact[3], double kmax, double emax, const double krot[6], double mxyz, const double J[9],
+                  double *ocri, double *rcri)
+{
+  double nx = kact[0], ny = kact[1], nz = kact[2];
+  double nxy = nx > ny ? nx : ny;
+  double nxyz = nxy > nz ? nxy : nz;
+  if (!(nxyz > 0.0)) return 1.7976931348623157E+308;
+  double k = kmax;
+  double omegaxyz = std::sqrt (nxyz*k/mxyz);
+  double K[9], L[9], inv, invL[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, temp[9], A[9];
+  K[0] = k*krot[0]; K[1] = k*krot[3]; K[2] = k*krot[4];
+  K[3] = K[1]; K[4] = k*krot[1]; K[5] = k*krot[5];
+  K[6] = K[2]; K[7] = K[5]; K[8] = k*krot[2];
+  double mrot = extremum_eigenvalue (J, true);
+  L[0] = std::sqrt (J[0]);
+  inv = 1.0/L[0];
+  L[1] = inv * J[1];
+  L[2] = inv * J[2];
+  L[3] = 0.0;
+  L[4] = std::sqrt (J[4] - L[1]*L[1]);
+  inv = 1.0/L[4];
+  L[5] = inv * (J[5] - L[2]*L[1]);
+  L[6] = 0.0;
+  L[7] = 0.0;
+  L[8] = std::sqrt (J[8] - (L[2]*L[2] + L[5]*L[5]));
+  double det = L[0]*(L[4]*L[8]-L[5]*L[7]) + L[3]*(L[2]*L[7]-L[1]*L[8]) + (L[1]*L[5]-L[2]*L[4])*L[6];   /* INVERT */
+  if (det != 0.0)
+  {
+    det = 1.0 / det;
+    invL[0] = (L[4]*L[8]-L[5]*L[7])*det;
+    invL[1] = (L[2]*L[7]-L[1]*L[8])*det;
+    invL[2] = (L[1]*L[5]-L[2]*L[4])*det;
+    invL[3] = (L[5]*L[6]-L[3]*L[8])*det;
+    invL[4] = (L[0]*L[8]-L[2]*L[6])*det;
+    invL[5] = (L[2]*L[3]-L[0]*L[5])*det;
+    invL[6] = (L[3]*L[7]-L[4]*L[6])*det;
+    invL[7] = (L[1]*L[6]-L[0]*L[7])*det;
+    invL[8] = (L[0]*L[4]-L[1]*L[3])*det;
+  }
+  /* NTMUL (K, invL, temp): temp = K invL^T */
+  for (int c = 0; c < 3; c++)
+    for (int r = 0; r < 3; r++)
+      temp[3*c+r] = K[r]*invL[c] + K[3+r]*invL[3+c] + K[6+r]*invL[6+c];
+  nnmul (invL, temp, A);
+  double lambda = extremum_eigenvalue (A, false);
+  double omegarot = std::sqrt (lambda);
+  double damping = emax;
+  double ratioxyz = damping / (2.0*mxyz*omegaxyz);
+  double ratiorot = damping / (2.0*mrot*omegarot);
+  double hxyz = (2.0/omegaxyz)*(std::sqrt (1.0+ratioxyz*ratioxyz) - ratioxyz);
+  double hrot = (2.0/omegarot)*(std::sqrt (1.0+ratiorot*ratiorot) - ratiorot);
+  if (ocri) *ocri = omegaxyz > omegarot ? omegaxyz : omegarot;
+  if (rcri) *rcri = ratioxyz > ratiorot ? ratioxyz : ratiorot;
+  return hxyz < hrot ? hxyz : hrot;
+}
+
+/* adaptive_timestep, parmec.ispc:391-420 */
+double adaptive_timestep (const orc_ctx &s, double curstep, double adaptive)
+{
+  size_t N = s.parnum;
+  double hmin = 1.7976931348623157E+308;
+  for (size_t i = 0; i < N; i++)
+  {
+    double ka[3], kr[6], J[9];
+    for (int c = 0; c < 3; c++) ka[c] = s.adp[c*N + i];
+    for (int c = 0; c < 6; c++) kr[c] = s.adp[(5+c)*N + i];
+    for (int c = 0; c < 9; c++) J[c] = s.inertia[c*N + i];
+    double h = time_step (ka, s.adp[3*N + i], s.adp[4*N + i], kr, s.mass[i], J, nullptr, nullptr);
+    hmin = h < hmin ? h : hmin;
+  }
+  if (hmin < 1.7976931348623157E+308) return adaptive*hmin;
+  return curstep;
+}
+
+/* the largest |slope| of a curve, parmec.ispc:510-515 */
+double max_slope (const double *x, const double *y, int lo, int hi)
+{
+  double m = 0.0;
+  for (int q = lo; q < hi-1; q++)
+  {
+    double k = (y[q+1]-y[q]) / (x[q+1]-x[q]);
+    k = std::fabs (k);
+    m = k > m ? k : m;
+  }
+  return m;
+}
+
+/* critical_perparticle, parmec.ispc:537-888: the contacts of the last step
+ * (in the canonical order of the force sums), then the springs in spring
+ * order, then time_step_task */
+void critical_perparticle (orc_ctx &s, double *hcri, double *ocri, double *rcri)
+{
+  size_t N = s.parnum, S = s.sprnum;
+  std::vector<double> adp (11 * N, 0.0);
+  for (size_t i = 0; i < N; i++)   /* critical_perparticle_contacts, parmec.ispc:537-655 */
+  {
+    struct E { unsigned long long k0, k1; const double *p; double f[3]; const double *n; double kc, ec; };
+    std::vector<E> es;
+    for (const Contact &con : s.master[i])
+      es.push_back ({side_key (con.slave1), side_key (con.master), con.point, {con.force[0], con.force[1], con.force[2]},
+                     con.normal, con.kcur, con.ecur});
+    for (const Slave &sv : s.slave[i])
+      es.push_back ({side_key (sv.master1), side_key (sv.slave1), sv.point, {sv.force[0], sv.force[1], sv.force[2]},
+                     nullptr, 0.0, 0.0});
+    std::sort (es.begin (), es.end (), [] (const E &a, const E &b) { return a.k0 < b.k0 || (a.k0 == b.k0 && a.k1 < b.k1); });
+    double L[9], po[3];
+    for (int c = 0; c < 9; c++) L[c] = s.rotation[c*N + i];
+    for (int c = 0; c < 3; c++) po[c] = s.position[c*N + i];
+    double kact0[3] = {0.0, 0.0, 0.0}, kmax0 = 0.0, emax0 = 0.0, krot0[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (const E &e : es)
+    {
+      double a[3] = {e.p[0]-po[0], e.p[1]-po[1], e.p[2]-po[2]}, f[3];
+      if (e.n) for (int c = 0; c < 3; c++) f[c] = e.n[c];
+      else
+      {
+        for (int c = 0; c < 3; c++) f[c] = e.f[c];
+        double len = std::sqrt (f[0]*f[0] + f[1]*f[1] + f[2]*f[2]);
+        if (len != 0.0)
+        {
+          len = 1.0/len;
+          f[0] *= len; f[1] *= len; f[2] *= len;
+        }
+        else f[0] = f[1] = f[2] = 0.0;
+      }
+      for (int c = 0; c < 3; c++) kact0[c] += std::fabs (f[c]);
+      kmax0 = e.kc > kmax0 ? e.kc : kmax0;
+      emax0 = e.ec > emax0 ? e.ec : emax0;
+      double A[3];
+      nvmul (L, a, A);
+      double dot = A[0]*A[0] + A[1]*A[1] + A[2]*A[2];
+      krot0[0] += dot - A[0]*A[0];
+      krot0[1] += dot - A[1]*A[1];
+      krot0[2] += dot - A[2]*A[2];
+      krot0[3] += -A[0]*A[1];
+      krot0[4] += -A[0]*A[2];
+      krot0[5] += -A[1]*A[2];
+    }
+    double v[11] = {kact0[0], kact0[1], kact0[2], kmax0, emax0, krot0[0], krot0[1], krot0[2], krot0[3], krot0[4], krot0[5]};
+    for (int c = 0; c < 11; c++) adp[c*N + i] = v[c];
+  }
+  /* critical_perparticle_springs, parmec.ispc:658-843, one spring after
+   * another.  dashidx and sprflg are read at particle indices there (i, k);
+   * where such an index lies beyond the spring arrays the spring's own
+   * entry (dashidx) or no flag (sprflg) is taken */
+  const double *sx = s.spring.data (), *sy = s.spring.data () + s.sprcrv;
+  const double *dx = s.dashpot.data (), *dy = s.dashpot.data () + s.dashcrv;
+  for (size_t j = 0; j < S; j++)
+  {
+    int i = s.sprpart[j], k = s.sprpart[S + j];
+    double d[3] = {s.sprdir[j], s.sprdir[S + j], s.sprdir[2*S + j]};
+    double kspr = max_slope (sx, sy, s.spridx[j], s.spridx[j+1]);
+    double espr = 0.0;
+    size_t di = (size_t) i < S ? (size_t) i : j;
+    if (s.dashidx[di]+1 == s.dashidx[di+1])
+    {
+      double Pi[3], Pk[3];
+      for (int c = 0; c < 3; c++) { Pi[c] = s.sprpnt[(3+c)*S + j]; Pk[c] = s.sprpnt[(9+c)*S + j]; }
+      double mass = eqm (s, i, Pi, k, Pk, d);
+      double ratio = dx[s.dashidx[di]];
+      espr = ratio * 2.0 * std::sqrt (kspr * mass);
+    }
+    else espr = max_slope (dx, dy, s.dashidx[j], s.dashidx[j+1]);
+    for (int side = 0; side < 2; side++)
+    {
+      int b = side ? k : i;
+      if (b < 0) continue;
+      int flg = (size_t) b < S ? s.sprflg[b] : 0;
+      if (d[0]*d[0] + d[1]*d[1] + d[2]*d[2] > 0.0)
+      {
+        if (flg & PMX_SPRDIR_PLANAR)
+        {
+          double t1[3] = {d[0], d[1], d[2]}, t2[3];
+          t1[0] += 0.123;
+          t1[1] += 0.346;
+          t1[2] += 0.789;
+          t2[0] = d[1]*t1[2] - d[2]*t1[1];
+          t2[1] = d[2]*t1[0] - d[0]*t1[2];
+          t2[2] = d[0]*t1[1] - d[1]*t1[0];
+          double l2 = std::sqrt (t2[0]*t2[0] + t2[1]*t2[1] + t2[2]*t2[2]);
+          double i2 = 1.0/l2;
+          t2[0] *= i2; t2[1] *= i2; t2[2] *= i2;
+          t1[0] = t2[1]*d[2] - t2[2]*d[1];
+          t1[1] = t2[2]*d[0] - t2[0]*d[2];
+          t1[2] = t2[0]*d[1] - t2[1]*d[0];
+          for (int c = 0; c < 3; c++) adp[c*N + b] += std::fabs (t1[c]) + std::fabs (t2[c]);
+        }
+        else for (int c = 0; c < 3; c++) adp[c*N + b] += std::fabs (d[c]);
+      }
+      else for (int c = 0; c < 3; c++) adp[c*N + b] += 1.0;
+      double pnt[3], Lb[9], a[3], A[3];
+      for (int c = 0; c < 3; c++) pnt[c] = s.sprpnt[(6*side+c)*S + j];
+      for (int c = 0; c < 9; c++) Lb[c] = s.rotation[c*N + b];
+      for (int c = 0; c < 3; c++) a[c] = pnt[c] - s.position[c*N + b];
+      tvmul (Lb, a, A);
+      double dot = A[0]*A[0] + A[1]*A[1] + A[2]*A[2];
+      adp[5*N + b] += dot-A[0]*A[0];
+      adp[6*N + b] += dot-A[1]*A[1];
+      adp[7*N + b] += dot-A[2]*A[2];
+      adp[8*N + b] += -A[0]*A[1];
+      adp[9*N + b] += -A[0]*A[2];
+      adp[10*N + b] += -A[1]*A[2];
+      adp[3*N + b] = adp[3*N + b] > kspr ? adp[3*N + b] : kspr;
+      adp[4*N + b] = adp[4*N + b] > espr ? adp[4*N + b] : espr;
+    }
+  }
+  for (size_t i = 0; i < N; i++)   /* time_step_task */
+  {
+    double ka[3], kr[6], J[9], o = 0.0, r = 0.0;
+    for (int c = 0; c < 3; c++) ka[c] = adp[c*N + i];
+    for (int c = 0; c < 6; c++) kr[c] = adp[(5+c)*N + i];
+    for (int c = 0; c < 9; c++) J[c] = s.inertia[c*N + i];
+    hcri[i] = time_step (ka, adp[3*N + i], adp[4*N + i], kr, s.mass[i], J, &o, &r);
+    if (ocri) ocri[i] = o;
+    if (rcri) rcri[i] = r;
+  }
+}
+
+/* critical_perspring, parmec.ispc:485-535 */
+void critical_perspring (orc_ctx &s, double *hcri, double *ocri, double *rcri)
+{
+  size_t S = s.sprnum;
+  const double *sx = s.spring.data (), *sy = s.spring.data () + s.sprcrv;
+  const double *dx = s.dashpot.data (), *dy = s.dashpot.data () + s.dashcrv;
+  for (size_t j = 0; j < S; j++)
+  {
+    int i = s.sprpart[j], k = s.sprpart[S + j];
+    double pi[3], pk[3], d[3];
+    for (int c = 0; c < 3; c++) { pi[c] = s.sprpnt[(3+c)*S + j]; pk[c] = s.sprpnt[(9+c)*S + j]; d[c] = s.sprdir[c*S + j]; }
+    double kspr = max_slope (sx, sy, s.spridx[j], s.spridx[j+1]);
+    double mspr = eqm (s, i, pi, k, pk, d);
+    double omega = std::sqrt (kspr/mspr), ksi = 0.0;
+    if (s.dashidx[j]+1 == s.dashidx[j+1]) ksi = dx[s.dashidx[j]];
+    else
+    {
+      double espr = max_slope (dx, dy, s.dashidx[j], s.dashidx[j+1]);
+      if (omega > 0.0) ksi = espr / (2*mspr*omega);
+    }
+    hcri[j] = omega > 0.0 ? (2.0/omega)*(std::sqrt (1.0+ksi*ksi) - ksi) : 1.0;
+    if (ocri) ocri[j] = omega;
+    if (rcri) rcri[j] = ksi;
+  }
 }
 
 } /* namespace */
@@ -1569,11 +1922,18 @@ int orc_download (orc_ctx *s, pmx_model *m)
 
 int orc_step_times (double duration, double step, double curtime, double *times, int cap, int *nsteps);
 
-int orc_dem (orc_ctx *s, double duration, double step, const pmx_forcing *fc, double *elapsed)
+/* dem(), parmec.cpp:2315-2530; adaptive in (0, 1] takes each step's
+ * length from adaptive_timestep (2444-2447) and the host inputs from
+ * fc->step_fn step by step */
+static int dem_run (orc_ctx *s, double duration, double step, double adaptive, const pmx_forcing *fc, double *elapsed)
 {
   if (!s) return fail (PMX_EINVAL, "null ctx");
   if (!(duration > 0.0) || !(step > 0.0)) return fail (PMX_EINVAL, "duration and step must be positive");
   auto t0 = std::chrono::steady_clock::now ();
+  const bool adapt = adaptive > 0.0 && adaptive <= 1.0;
+  s->adaptive = adapt;
+  if (adapt) s->adp.assign (11 * (size_t) s->parnum, 0.0);
+  std::vector<double> obsv (6 * (size_t) std::max (s->obsnum, 1)), prsv (6 * (size_t) std::max (s->prscols, 1));
   double step0, step1;
   if (s->curtime == 0.0)
   {
@@ -1606,23 +1966,54 @@ int orc_dem (orc_ctx *s, double duration, double step, const pmx_forcing *fc, do
     }
     if (bad) return fail (PMX_EUNSUPPORTED, "ellipsoid (non-sphere) shapes are not supported");
     double g[3], damp[6];
-    for (int d = 0; d < 3; d++) g[d] = fc && fc->gtab ? fc->gtab[3*k+d] : (fc ? fc->gravity[d] : 0.0);
-    for (int d = 0; d < 6; d++) damp[d] = fc && fc->dtab ? fc->dtab[6*k+d] : (fc ? fc->damping[d] : 0.0);
+    const double *prow, *orow;
+    pmx_step_inputs in;
+    in.output = 0;
+    if (adapt)
+    {
+      /* the step's inputs at its curtime (read_gravity_and_damping, prescribe,
+       * obstaclev at curtime+step0) */
+      for (int d = 0; d < 3; d++) in.gravity[d] = fc ? fc->gravity[d] : 0.0;
+      for (int d = 0; d < 6; d++) in.damping[d] = fc ? fc->damping[d] : 0.0;
+      std::fill (obsv.begin (), obsv.end (), 0.0);
+      std::fill (prsv.begin (), prsv.end (), 0.0);
+      in.obsvel = obsv.data ();
+      in.prs = prsv.data ();
+      in.hisrow = -1;
+      if (fc && fc->step_fn && fc->step_fn (fc->step_user, (int) k, s->curtime, step0, &in))
+        return fail (PMX_EINVAL, "step_fn failed");
+      for (int d = 0; d < 3; d++) g[d] = in.gravity[d];
+      for (int d = 0; d < 6; d++) damp[d] = in.damping[d];
+      prow = prsv.data ();
+      orow = obsv.data ();
+    }
+    else
+    {
+      for (int d = 0; d < 3; d++) g[d] = fc && fc->gtab ? fc->gtab[3*k+d] : (fc ? fc->gravity[d] : 0.0);
+      for (int d = 0; d < 6; d++) damp[d] = fc && fc->dtab ? fc->dtab[6*k+d] : (fc ? fc->damping[d] : 0.0);
+      prow = fc && fc->prstab ? fc->prstab + 6*(size_t) k * (fc->prscols > 0 ? fc->prscols : s->prscols) : nullptr;
+      orow = fc && fc->obstab ? fc->obstab + 6*(size_t) k * s->obsnum : nullptr;
+    }
     if (forces (*s, step0, g)) return fail (PMX_EUNSUPPORTED, "unsupported contact kind");
     if (s->sprnum && springs (*s, step0)) return fail (PMX_EUNSUPPORTED, "unsupported spring kind");
-    step1 = step0;
-    const double *prow = fc && fc->prstab ? fc->prstab + 6*(size_t) k * (fc->prscols > 0 ? fc->prscols : s->prscols) : nullptr;
+    step1 = adapt ? adaptive_timestep (*s, step0, adaptive) : step0;
+    if (!(step1 > 0.0)) return fail (PMX_EINVAL, "adaptive step is not positive");
     prescribe_acceleration (*s, prow);
     dynamics (*s, damp, step0, step1);
     prescribe_velocity (*s, prow);
     shapes (*s);
-    obstaclev (*s, fc && fc->obstab ? fc->obstab + 6*(size_t) k * s->obsnum : nullptr);   /* at curtime+step0 */
+    obstaclev (*s, orow);   /* at curtime+step0 */
     obstacles (*s, 0.5*(step0+step1));
     /* output points (parmec.cpp:2502-2507): the caller reads the state */
-    if (fc && fc->out_fn && fc->outstep)
+    if (adapt ? in.output != 0 : false)
+    {
+      if (fc && fc->out_fn) fc->out_fn (fc->out_user, (int) k);
+    }
+    else if (fc && fc->out_fn && fc->outstep)
       for (int q = 0; q < fc->noutstep; q++)
         if (fc->outstep[q] == k) fc->out_fn (fc->out_user, (int) k);
   }
+  s->adaptive = false;
   s->curstep = step1;
   s->stepnum++;
   size_t pairs = 0;
@@ -1636,6 +2027,40 @@ int orc_dem (orc_ctx *s, double duration, double step, const pmx_forcing *fc, do
   if (elapsed) *elapsed = dt;
   return PMX_OK;
 }
+
+int orc_dem (orc_ctx *s, double duration, double step, const pmx_forcing *fc, double *elapsed)
+{
+  return dem_run (s, duration, step, 0.0, fc, elapsed);
+}
+
+int orc_dem_adaptive (orc_ctx *s, double duration, double step, double adaptive, const pmx_forcing *fc, double *elapsed)
+{
+  if (!(adaptive > 0.0 && adaptive <= 1.0)) return fail (PMX_EINVAL, "adaptive must lie in (0, 1]");
+  return dem_run (s, duration, step, adaptive, fc, elapsed);
+}
+
+int orc_critical (orc_ctx *s, int which, double *h, double *o, double *r)
+{
+  if (!s || !h) return fail (PMX_EINVAL, "null argument");
+  if (which == PMX_CRIT_SPRING) critical_perspring (*s, h, o, r);
+  else if (which == PMX_CRIT_PARTICLE) critical_perparticle (*s, h, o, r);
+  else return fail (PMX_EINVAL, "unknown estimate");
+  return PMX_OK;
+}
+
+void orc_trig (int fn, const double *x, double *y, int n)
+{
+  for (int i = 0; i < n; i++)
+    y[i] = fn == 0 ? pmxt::sin (x[i]) : fn == 1 ? pmxt::cos (x[i]) : pmxt::acos (x[i]);
+}
+
+double orc_time_step (const double kact[3], double kmax, double emax, const double krot[6], double mass,
+                      const double inertia[9], double *o, double *r)
+{
+  return time_step (kact, kmax, emax, krot, mass, inertia, o, r);
+}
+
+double orc_extremum_eigenvalue (const double A[9], int minimum) { return extremum_eigenvalue (A, minimum != 0); }
 
 int orc_step_times (double duration, double step, double curtime, double *times, int cap, int *nsteps)
 {

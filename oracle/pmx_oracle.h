@@ -29,6 +29,13 @@ typedef struct orc_ctx orc_ctx;
 int orc_create (int nthreads, orc_ctx **ctx);
 int orc_upload (orc_ctx *ctx, const pmx_model *model);
 int orc_dem (orc_ctx *ctx, double duration, double step, const pmx_forcing *forcing, double *elapsed);
+int orc_dem_adaptive (orc_ctx *ctx, double duration, double step, double adaptive, const pmx_forcing *forcing,
+                      double *elapsed);
+int orc_critical (orc_ctx *ctx, int which, double *h, double *o, double *r);
+void orc_trig (int fn, const double *x, double *y, int n);   /* 0 sin, 1 cos, 2 acos (pmx_trig.h) */
+double orc_time_step (const double kact[3], double kmax, double emax, const double krot[6], double mass,
+                      const double inertia[9], double *o, double *r);
+double orc_extremum_eigenvalue (const double A[9], int minimum);
 int orc_download (orc_ctx *ctx, pmx_model *model);
 int orc_contacts (orc_ctx *ctx, pmx_pair *out, size_t cap, size_t *count);
 int orc_get_stats (orc_ctx *ctx, pmx_stats *stats);

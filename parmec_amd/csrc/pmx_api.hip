@@ -54,6 +54,8 @@ int alloc_rows (pmx_ctx &c)
       c.xcnt.alloc (E) || c.rbusy.alloc (E + 1024) || c.rnbusy.alloc (E / 64 + 2) || c.leafof.alloc (E) ||
       c.slot.alloc (E) || c.lmem.alloc (E) || c.lpart.alloc (E) || c.lpos.alloc (4 * E))
     return fail (PMX_ENOMEM, "out of device memory (broad phase)");
+  if (c.opt.critical && c.crec.alloc (8 * (size_t) c.cap * E))
+    return fail (PMX_ENOMEM, "out of device memory (contact records)");
   PMX_HIP (hipMemsetAsync (c.rowcnt[0].p, 0, E * sizeof (int), c.stream));
   PMX_HIP (hipMemsetAsync (c.rowcnt[1].p, 0, E * sizeof (int), c.stream));
   /* hit and reverse-hit counts: zero between steps (k_rows clears the ones it consumes) */
@@ -310,6 +312,7 @@ int upload_core (pmx_ctx *c, const pmx_model *m, const std::vector<InstOff> &off
   }
   PMX_HIP (hipStreamSynchronize (c->stream));
   c->uploaded = true;
+  c->crec_valid = c->crec_stale = false;
   c->nhis = 0;   /* histories name particles of the previous model */
   c->his_active = false;
   return PMX_OK;
@@ -434,6 +437,7 @@ void pmx_destroy (pmx_ctx *c)
   c->ctl.release ();
   drop_graph (c);
   if (c->hring) (void) hipHostFree (c->hring);
+  if (c->hkey) (void) hipHostFree (c->hkey);
   for (hipEvent_t e : c->rev) if (e) (void) hipEventDestroy (e);
   prof_resolve (*c);
   for (hipEvent_t e : c->prof.pool) (void) hipEventDestroy (e);
@@ -555,6 +559,8 @@ static int grow_capacity (pmx_ctx *c, const int *fl)
       else { c->rowst[b].release (); c->rowst[b].p = (double*) nst; c->rowst[b].n = 3 * cap2 * E; }
     }
     c->cap = cap2;
+    if (c->opt.critical && c->crec.alloc (8 * (size_t) cap2 * E))   /* rewritten by the repeated step */
+      return fail (PMX_ENOMEM, "out of device memory (contact records)");
   }
   if (hcap2 != c->hcap)
   {
@@ -1054,6 +1060,7 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
     c->his_active = false;
   }
   c->stepped += nsteps;
+  if (nsteps > 0) { c->crec_valid = c->opt.critical != 0; c->crec_stale = c->opt.critical == 0; }
   c->cur = (c->cur0 + nsteps) & 1;
   c->curtime = c->icurtime[0]; c->curstep = c->icurstep[0]; c->stepnum = c->istepnum[0];
   double dt = std::chrono::duration<double> (std::chrono::steady_clock::now () - t0).count ();
@@ -1085,7 +1092,234 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
   return PMX_OK;
 }
 
+/* dem() with the adaptive step (parmec.cpp:2315-2530, adaptive in (0, 1]).
+ * Each step's length step1 comes from that step's forces, and the step's
+ * dynamics needs it, so every step stops once on the host: the first part
+ * of the step (binning to the forces, the per-particle estimates and their
+ * minimum) is enqueued, the control block, the flags and the minimum are
+ * read back, then the rest of the step is enqueued with step0 and step1.
+ * The host inputs of step k come from forcing->step_fn at that step's
+ * curtime; the tables the kernels read are grown as the steps go. */
+static int adaptive_core (pmx_ctx *c, double duration, double step, double adaptive, const pmx_forcing *fc,
+                          double *elapsed)
+{
+  if (!c) return fail (PMX_EINVAL, "null ctx");
+  if (!c->uploaded) return fail (PMX_EINVAL, "pmx_upload must precede pmx_dem_adaptive");
+  if (!(step > 0.0)) return fail (PMX_EINVAL, "step must be positive");
+  if (!(duration > 0.0)) return fail (PMX_EINVAL, "duration must be positive");
+  if (!(adaptive > 0.0 && adaptive <= 1.0)) return fail (PMX_EINVAL, "adaptive must lie in (0, 1]");
+  if (c->I != 1) return fail (PMX_EUNSUPPORTED, "pmx_dem_adaptive takes single-model contexts");
+  (void) hipSetDevice (c->device);
+  auto t0 = std::chrono::steady_clock::now ();
+  struct Restore
+  {
+    pmx_ctx *c; int prof;
+    ~Restore () { c->adp_on = false; c->prof.mode = prof; }
+  } restore {c, c->prof.mode};
+  c->prof.mode = 0;   /* launches are eager here; the graph timing does not apply */
+  int rc = 0;
+  int ctl0[CTL_COUNT] = {0};
+  PMX_HIP (hipMemcpyAsync (c->ctl.p, ctl0, sizeof (ctl0), hipMemcpyHostToDevice, c->stream));
+  PMX_HIP (hipMemsetAsync (c->dflags.p, 0, FLAG_COUNT * sizeof (int), c->stream));
+  c->staggered = false;
+  double curtime = c->icurtime[0];
+  double step0, step1;
+  if (curtime == 0.0)
+  {
+    step0 = step;
+    if ((rc = step_euler (*c, 0.5*step0)) || (rc = step_shapes (*c)) || (rc = step_mesh_shapes (*c, false))) return rc;
+    if (c->O > 0)
+    {
+      /* obstaclev (step0/2) and obstacles (step0), parmec.cpp:2403-2405 */
+      std::vector<double> v0 (6 * (size_t) c->O, 0.0);
+      if (fc && fc->obstab0) std::memcpy (v0.data (), fc->obstab0, v0.size () * sizeof (double));
+      if ((rc = up (*c, c->obstab, v0.data (), v0.size (), "obstacle table")) ||
+          (rc = step_obstacles (*c, c->obstab.p, 0, step0)))
+        return rc;
+    }
+  }
+  else step0 = c->icurstep[0];
+  step1 = step0;
+  if ((rc = step_invert (*c))) return rc;
+  if (c->adp.alloc (11 * (size_t) std::max (c->N, 1)) || (c->S > 0 && c->sp_adp.alloc (17 * (size_t) c->S)) ||
+      c->hmin.alloc (1))
+    return fail (PMX_ENOMEM, "out of device memory (adaptive step)");
+  const int slotn = CTL_COUNT + FLAG_COUNT;
+  if (!c->hring)
+  {
+    if (hipHostMalloc ((void**) &c->hring, sizeof (int) * slotn * PMX_RING) != hipSuccess)
+    { c->hring = nullptr; return fail (PMX_ENOMEM, "out of pinned host memory"); }
+    for (int k = 0; k < PMX_RING; k++) PMX_HIP (hipEventCreateWithFlags (&c->rev[k], hipEventDisableTiming));
+  }
+  if (!c->hkey && hipHostMalloc ((void**) &c->hkey, sizeof (unsigned long long)) != hipSuccess)
+  { c->hkey = nullptr; return fail (PMX_ENOMEM, "out of pinned host memory"); }
+  c->his_active = c->nhis > 0 && fc && fc->hisout && fc->hisrows > 0;
+  if (c->his_active && c->hisbuf.alloc ((size_t) fc->hisrows * c->nhis) != hipSuccess)
+    return fail (PMX_ENOMEM, "out of device memory (histories)");
+
+  /* per-step tables, grown as steps are added; the host keeps every row */
+  const size_t O = c->O, P = c->PRSC;
+  std::vector<double> gh, dh, oh, ph;
+  std::vector<int> hh;
+  int tcap = 0;
+  auto ensure = [&] (int k) -> int {
+    if (k < tcap) return PMX_OK;
+    int ncap = std::max (64, 2 * tcap);
+    while (ncap <= k) ncap *= 2;
+    gh.resize (3 * (size_t) ncap, 0.0); dh.resize (6 * (size_t) ncap, 0.0);
+    oh.resize (6 * O * ncap, 0.0); ph.resize (6 * P * ncap, 0.0); hh.resize (ncap, -1);
+    int r;
+    if ((r = up (*c, c->gtab, gh.data (), gh.size (), "gravity table")) ||
+        (r = up (*c, c->dtab, dh.data (), dh.size (), "damping table")) ||
+        (O > 0 && (r = up (*c, c->obstab, oh.data (), oh.size (), "obstacle table"))) ||
+        (P > 0 && (r = up (*c, c->prstab, ph.data (), ph.size (), "prescribed table"))) ||
+        (c->his_active && (r = up (*c, c->hisrow, hh.data (), hh.size (), "history rows"))))
+      return r;
+    tcap = ncap;
+    return PMX_OK;
+  };
+  c->cur0 = c->cur;
+  c->graph_valid = false;
+  c->stats.max_row = 0;
+  c->adp_on = true;
+  long long rebuilds = 0, partial = 0, grown = 0;
+  if (c->E > 0)
+  {
+    if ((rc = rebuild (c, true))) return rc;
+    rebuilds++;
+  }
+  std::vector<double> obsv (6 * std::max (O, (size_t) 1)), prsv (6 * std::max (P, (size_t) 1));
+  double time = 0.0;
+  int k = 0, rebuilt_at = -1;
+  for (; time < duration; time += 0.5*(step0+step1), curtime += 0.5*(step0+step1), step0 = step1, k++)
+  {
+    pmx_step_inputs in;
+    for (int q = 0; q < 3; q++) in.gravity[q] = fc ? fc->gravity[q] : 0.0;
+    for (int q = 0; q < 6; q++) in.damping[q] = fc ? fc->damping[q] : 0.0;
+    std::fill (obsv.begin (), obsv.end (), 0.0);
+    std::fill (prsv.begin (), prsv.end (), 0.0);
+    in.obsvel = obsv.data ();
+    in.prs = prsv.data ();
+    in.hisrow = -1;
+    in.output = 0;
+    if (fc && fc->step_fn)
+    {
+      int r = fc->step_fn (fc->step_user, k, curtime, step0, &in);
+      (void) hipSetDevice (c->device);
+      if (r) return fail (PMX_EINVAL, "step_fn failed at step " + std::to_string (k));
+    }
+    if (in.hisrow >= 0 && (!c->his_active || in.hisrow >= fc->hisrows))
+      return fail (PMX_EINVAL, "history row beyond hisrows");
+    if ((rc = ensure (k))) return rc;
+    std::memcpy (&gh[3*(size_t) k], in.gravity, 3 * sizeof (double));
+    std::memcpy (&dh[6*(size_t) k], in.damping, 6 * sizeof (double));
+    PMX_HIP (hipMemcpyAsync (c->gtab.p + 3*(size_t) k, &gh[3*(size_t) k], 3 * sizeof (double), hipMemcpyHostToDevice, c->stream));
+    PMX_HIP (hipMemcpyAsync (c->dtab.p + 6*(size_t) k, &dh[6*(size_t) k], 6 * sizeof (double), hipMemcpyHostToDevice, c->stream));
+    if (O > 0)
+    {
+      std::memcpy (&oh[6*O*k], obsv.data (), 6 * O * sizeof (double));
+      PMX_HIP (hipMemcpyAsync (c->obstab.p + 6*O*k, &oh[6*O*k], 6 * O * sizeof (double), hipMemcpyHostToDevice, c->stream));
+    }
+    if (P > 0)
+    {
+      std::memcpy (&ph[6*P*k], prsv.data (), 6 * P * sizeof (double));
+      PMX_HIP (hipMemcpyAsync (c->prstab.p + 6*P*k, &ph[6*P*k], 6 * P * sizeof (double), hipMemcpyHostToDevice, c->stream));
+    }
+    if (c->his_active)
+    {
+      hh[k] = in.hisrow;
+      PMX_HIP (hipMemcpyAsync (c->hisrow.p + k, &hh[k], sizeof (int), hipMemcpyHostToDevice, c->stream));
+    }
+    int ctl[CTL_COUNT], fl[FLAG_COUNT];
+    for (;;)
+    {
+      if ((rc = enqueue_step_forces (*c, step0))) return rc;
+      int *slot = c->hring;
+      PMX_HIP (hipMemcpyAsync (slot, c->ctl.p, sizeof (int) * CTL_COUNT, hipMemcpyDeviceToHost, c->stream));
+      PMX_HIP (hipMemcpyAsync (slot + CTL_COUNT, c->dflags.p, sizeof (int) * FLAG_COUNT, hipMemcpyDeviceToHost, c->stream));
+      PMX_HIP (hipMemcpyAsync (c->hkey, c->hmin.p, sizeof (unsigned long long), hipMemcpyDeviceToHost, c->stream));
+      PMX_HIP (hipStreamSynchronize (c->stream));
+      std::memcpy (ctl, slot, sizeof (ctl));
+      std::memcpy (fl, slot + CTL_COUNT, sizeof (fl));
+      if ((rc = check_flags (c, fl))) return rc;
+      if (!ctl[CTL_ABORT]) break;
+      /* the step was aborted before it changed any state (as in dem_core):
+       * its sequence number moves on, then larger buffers or a new tree,
+       * and the step again */
+      if ((rc = step_end (*c))) return rc;
+      if (fl[FLAG_ROW_OVERFLOW] || fl[FLAG_HIT_OVERFLOW] || fl[FLAG_EXTRA_OVERFLOW] || fl[FLAG_MTRI_OVERFLOW])
+      {
+        if ((rc = grow_capacity (c, fl))) return rc;
+        grown++;
+      }
+      if (fl[FLAG_LEAF_OVERFLOW])
+      {
+        if (rebuilt_at == k) return fail (PMX_EREPART, "Repartitioning failed (step " + std::to_string (k) + ")");
+        rebuilt_at = k;
+        if ((rc = rebuild (c, false))) return rc;
+        rebuilds++;
+        if (c->tree.last_partial) partial++;
+      }
+      if ((rc = clear_overflow_flags (c))) return rc;
+    }
+    /* adaptive_timestep (parmec.ispc:391-420): adaptive times the smallest
+     * estimate, or the current step when no particle is loaded */
+    double hmin;
+    std::memcpy (&hmin, c->hkey, sizeof (double));
+    step1 = hmin < 1.7976931348623157e308 ? adaptive*hmin : step0;
+    if (!(step1 > 0.0) || step1 != step1)
+      return fail (PMX_EINVAL, "adaptive step " + std::to_string (step1) + " at step " + std::to_string (k));
+    if ((rc = enqueue_step_motion (*c, step0, step1))) return rc;
+    if (in.output && fc && fc->out_fn)
+    {
+      /* an output point: forces formed where they were used, mesh geometry,
+       * then the caller */
+      if ((rc = step_materialize (*c, std::vector<int> (1, k))) || (rc = step_mesh_shapes (*c, false))) return rc;
+      PMX_HIP (hipStreamSynchronize (c->stream));
+      fc->out_fn (fc->out_user, k);
+      (void) hipSetDevice (c->device);
+    }
+  }
+  const int nsteps = k;
+  if (nsteps > 0 && (rc = step_materialize (*c, std::vector<int> (1, nsteps - 1)))) return rc;
+  if ((rc = step_mesh_shapes (*c, false))) return rc;
+  if (c->his_active)
+  {
+    PMX_HIP (hipMemcpyAsync (fc->hisout, c->hisbuf.p, sizeof (double) * (size_t) fc->hisrows * c->nhis,
+                             hipMemcpyDeviceToHost, c->stream));
+    c->his_active = false;
+  }
+  PMX_HIP (hipStreamSynchronize (c->stream));
+  /* curstep = step1, stepnum++ (parmec.cpp:2521-2523) */
+  c->icurtime[0] = curtime;
+  c->icurstep[0] = step1;
+  c->istepnum[0]++;
+  c->stepped += nsteps;
+  if (nsteps > 0) { c->crec_valid = c->opt.critical != 0; c->crec_stale = c->opt.critical == 0; }
+  c->cur = (c->cur0 + nsteps) & 1;
+  c->icur[0] = c->cur;
+  c->curtime = c->icurtime[0]; c->curstep = c->icurstep[0]; c->stepnum = c->istepnum[0];
+  double dt = std::chrono::duration<double> (std::chrono::steady_clock::now () - t0).count ();
+  c->stats.steps = nsteps;
+  c->stats.rebuilds = rebuilds;
+  c->stats.partial_rebuilds = partial;
+  c->stats.grown = grown;
+  c->stats.row_capacity = c->cap;
+  c->stats.graph_timed = 0;
+  c->stats.leaves = c->tree.nleaves;
+  c->stats.leaf_slots = c->lstride;
+  c->stats.seconds = dt;
+  if (elapsed) *elapsed = dt;
+  return PMX_OK;
+}
+
 extern "C" {
+
+int pmx_dem_adaptive (pmx_ctx *c, double duration, double step, double adaptive, const pmx_forcing *fc,
+                      double *elapsed)
+{
+  return adaptive_core (c, duration, step, adaptive, fc, elapsed);
+}
 
 int pmx_dem (pmx_ctx *c, double duration, double step, const pmx_forcing *fc, double *elapsed)
 {

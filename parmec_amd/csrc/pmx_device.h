@@ -7,6 +7,7 @@
 #define PMX_DEVICE_H
 
 #include <hip/hip_runtime.h>
+#include "pmx_trig.h"
 
 namespace pmxd {
 
@@ -32,8 +33,7 @@ __device__ __forceinline__ void expmap (double O1, double O2, double O3, double 
   {
     double t = angsq;
     angsq = sqrt (angsq);
-    double s, c;
-    sincos (angsq, &s, &c);
+    double s = pmxt::sin (angsq), c = pmxt::cos (angsq);
     sx = s / angsq;
     cx = (1.0 - c) / t;
   }
@@ -229,6 +229,123 @@ __device__ __forceinline__ void granular (const R n[3], const R vij[3], R depth,
   f[0] += ft0;
   f[1] += ft1;
   f[2] += ft2;
+}
+
+/* NTMUL, macros.h:932-943: C = A B^T */
+__device__ __forceinline__ void ntmul (const double A[9], const double B[9], double C[9])
+{
+  C[0] = A[0]*B[0]+A[3]*B[3]+A[6]*B[6];
+  C[1] = A[1]*B[0]+A[4]*B[3]+A[7]*B[6];
+  C[2] = A[2]*B[0]+A[5]*B[3]+A[8]*B[6];
+  C[3] = A[0]*B[1]+A[3]*B[4]+A[6]*B[7];
+  C[4] = A[1]*B[1]+A[4]*B[4]+A[7]*B[7];
+  C[5] = A[2]*B[1]+A[5]*B[4]+A[8]*B[7];
+  C[6] = A[0]*B[2]+A[3]*B[5]+A[6]*B[8];
+  C[7] = A[1]*B[2]+A[4]*B[5]+A[7]*B[8];
+  C[8] = A[2]*B[2]+A[5]*B[5]+A[8]*B[8];
+}
+
+/* INVERT, macros.h:1033-1051: the inverse where the determinant is not
+ * zero; otherwise INV keeps what it held */
+__device__ __forceinline__ void invert3 (const double F[9], double INV[9])
+{
+  double det = F[0]*(F[4]*F[8]-F[5]*F[7]) + F[3]*(F[2]*F[7]-F[1]*F[8]) + (F[1]*F[5]-F[2]*F[4])*F[6];
+  if (det != 0.0)
+  {
+    det = 1.0 / det;
+    INV[0] = (F[4]*F[8]-F[5]*F[7])*det;
+    INV[1] = (F[2]*F[7]-F[1]*F[8])*det;
+    INV[2] = (F[1]*F[5]-F[2]*F[4])*det;
+    INV[3] = (F[5]*F[6]-F[3]*F[8])*det;
+    INV[4] = (F[0]*F[8]-F[2]*F[6])*det;
+    INV[5] = (F[2]*F[3]-F[0]*F[5])*det;
+    INV[6] = (F[3]*F[7]-F[4]*F[6])*det;
+    INV[7] = (F[1]*F[6]-F[0]*F[7])*det;
+    INV[8] = (F[0]*F[4]-F[1]*F[3])*det;
+  }
+}
+
+/* extremum_eigenvalue, macros.h:1538-1593: the largest eigenvalue of a
+ * symmetric 3x3 matrix, or the smallest with `minimum`, by the
+ * trigonometric solution of the characteristic cubic.  A diagonal matrix
+ * returns its largest diagonal entry whether or not `minimum` is set, as
+ * the reference does. */
+__device__ __forceinline__ double extremum_eigenvalue (const double A[9], bool minimum)
+{
+  const double pi = 3.14159265358979323846;
+  double p1 = A[3]*A[3] + A[6]*A[6] + A[7]*A[7];
+  if (p1 == 0.0)
+  {
+    double r = A[0] > A[4] ? A[0] : A[4];
+    return r > A[8] ? r : A[8];
+  }
+  double q = (A[0]+A[4]+A[8])/3.0;
+  double p2 = (A[0]-q)*(A[0]-q) + (A[4]-q)*(A[4]-q) + (A[8]-q)*(A[8]-q) + 2*p1;
+  double p = sqrt (p2 / 6.0);
+  double invp = 1.0 / p;
+  double B[9] = {invp*(A[0]-q), invp*A[1], invp*A[2], invp*A[3], invp*(A[4]-q), invp*A[5],
+                 invp*A[6], invp*A[7], invp*(A[8]-q)};
+  double r = 0.5 * (B[0]*(B[4]*B[8]-B[5]*B[7]) + B[3]*(B[2]*B[7]-B[1]*B[8]) + B[6]*(B[1]*B[5]-B[2]*B[4]));
+  double phi;
+  if (r <= -1.0) phi = pi / 3.0;
+  else if (r >= 1.0) phi = 0.0;
+  else phi = pmxt::acos (r) / 3.0;
+  if (minimum) return q + 2.0 * p * pmxt::cos (phi + (2*pi/3));
+  return q + 2.0 * p * pmxt::cos (phi);
+}
+
+/* one particle's critical step, time_step_task (parmec.ispc:275-420): the
+ * translational estimate from the largest summed |direction| component,
+ * the rotational one from the largest eigenvalue of L^-1 K L^-T (J = L L^T
+ * by Cholesky), each shortened for damping (h = 2/w (sqrt(1+x^2) - x));
+ * returns the smaller, with the larger frequency in *o and damping ratio
+ * in *r.  A particle whose summed directions are all zero returns DBL_MAX
+ * and leaves *o and *r alone.  min and max are a < b ? a : b and
+ * a > b ? a : b throughout (so a NaN second operand is taken). */
+__device__ __forceinline__ double critical_step (const double kact[3], double kmax, double emax, const double krot[6],
+                                                 double m, const double J[9], double *o, double *r)
+{
+  double nxy = kact[0] > kact[1] ? kact[0] : kact[1];
+  double nxyz = nxy > kact[2] ? nxy : kact[2];
+  if (!(nxyz > 0.0)) return 1.7976931348623157e308;
+  double k = kmax;
+  double omegaxyz = sqrt (nxyz*k/m);
+  double K[9];
+  K[0] = k*krot[0];
+  K[1] = k*krot[3];
+  K[2] = k*krot[4];
+  K[3] = K[1];
+  K[4] = k*krot[1];
+  K[5] = k*krot[5];
+  K[6] = K[2];
+  K[7] = K[5];
+  K[8] = k*krot[2];
+  double mrot = extremum_eigenvalue (J, true);
+  double L[9], inv, invL[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, temp[9], A[9];
+  L[0] = sqrt (J[0]);
+  inv = 1.0/L[0];
+  L[1] = inv * J[1];
+  L[2] = inv * J[2];
+  L[3] = 0.0;
+  L[4] = sqrt (J[4] - L[1]*L[1]);
+  inv = 1.0/L[4];
+  L[5] = inv * (J[5] - L[2]*L[1]);
+  L[6] = 0.0;
+  L[7] = 0.0;
+  L[8] = sqrt (J[8] - (L[2]*L[2] + L[5]*L[5]));
+  invert3 (L, invL);
+  ntmul (K, invL, temp);
+  nnmul (invL, temp, A);
+  double lambda = extremum_eigenvalue (A, false);
+  double omegarot = sqrt (lambda);
+  double damping = emax;
+  double ratioxyz = damping / (2.0*m*omegaxyz);
+  double ratiorot = damping / (2.0*mrot*omegarot);
+  double hxyz = (2.0/omegaxyz)*(sqrt (1.0+ratioxyz*ratioxyz) - ratioxyz);
+  double hrot = (2.0/omegarot)*(sqrt (1.0+ratiorot*ratiorot) - ratiorot);
+  *o = omegaxyz > omegarot ? omegaxyz : omegarot;
+  *r = ratioxyz > ratiorot ? ratioxyz : ratiorot;
+  return hxyz < hrot ? hxyz : hrot;
 }
 
 /* compensated (Kahan) float accumulation */

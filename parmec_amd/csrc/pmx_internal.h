@@ -251,7 +251,19 @@ struct pmx_ctx
   dvec<signed char> his_pt;
   dvec<double> his_pts, hisbuf;
   dvec<double> energy;         /* pmx_energy: per-block partial sums and the result */
+  /* adaptive step and critical estimates (single model): per particle
+   * kact[3], kmax, emax, krot[6] as [11][N]; per spring the adaptive terms
+   * [17][S]; with opt.critical, the contact records of the last step taken
+   * (point[3], |direction|[3], kcur, ecur as [8][cap][E]); the estimates
+   * h, omega, ratio [3][max(N, S)]; the adaptive minimum as an ordered key */
+  bool adp_on = false;         /* the step forms the adaptive sums (k_rows<R, true>) */
+  bool crec_valid = false;     /* crec holds the contacts of the last step taken */
+  bool crec_stale = false;     /* steps were taken since the upload without recording their contacts */
+  dvec<double> adp, sp_adp, crec, crit;
+  dvec<unsigned long long> hmin;
+  dvec<int> sp_pofs, sp_plist; /* particle -> its springs (spring << 1 | side) in spring order */
   int *hring = nullptr;        /* pinned: per group of steps in flight, the control block and the flags */
+  unsigned long long *hkey = nullptr;   /* pinned: the adaptive minimum of the step read back */
   hipEvent_t rev[PMX_RING] = {};   /* recorded after each group's copy into hring */
   int det_nw = 4;              /* waves per k_detect block (autotuned) */
   int rows_tpb = 1024;         /* threads per k_rows_idle block (spheres per busy list) */
@@ -378,6 +390,8 @@ int step_rows (pmx_ctx &c, double h);
 int step_dynamics (pmx_ctx &c, double step0, double step1);
 int step_end (pmx_ctx &c);
 int enqueue_step (pmx_ctx &c, double h);     /* one whole time step */
+int enqueue_step_forces (pmx_ctx &c, double h);                  /* up to the forces (and the adaptive minimum) */
+int enqueue_step_motion (pmx_ctx &c, double step0, double step1); /* dynamics to the end of the step */
 /* ---- time histories (pmx_history.hip) ---- */
 int history_set (pmx_ctx &c, const pmx_history *h);
 int history_now (pmx_ctx &c, double *out);
@@ -395,6 +409,11 @@ int step_invert (pmx_ctx &c);
 int springs_setup (pmx_ctx &c, const pmx_model *m);   /* per-instance ranges from c.off */
 int springs_download (pmx_ctx &c, pmx_model *m);
 int step_springs (pmx_ctx &c, double h);
+int crit_springs (pmx_ctx &c);                 /* CRITICAL's spring part of the per-particle sums */
+int crit_perspring (pmx_ctx &c, double *h, double *o, double *r);
+int crit_contacts (pmx_ctx &c);                /* CRITICAL's contact part (from the recorded contacts) */
+int adp_begin (pmx_ctx &c);                    /* zero the per-particle sums; reset the adaptive minimum */
+int adp_time_step (pmx_ctx &c, double *h, double *o, double *r);   /* per particle; minimum when h is null */
 int step_mesh_shapes (pmx_ctx &c, bool gated = true);
 
 #endif

@@ -71,30 +71,6 @@ __device__ __forceinline__ void tnmul (const double A[9], const double B[9], dou
       C[3*c+r] = A[3*r]*B[3*c] + A[3*r+1]*B[3*c+1] + A[3*r+2]*B[3*c+2];
 }
 
-/* extremum_eigenvalue (macros.h:1538-1593) as called by eqm with minimum = 1 */
-__device__ double min_eigenvalue (const double A[9])
-{
-  double p1 = A[3]*A[3] + A[6]*A[6] + A[7]*A[7];
-  if (p1 == 0.0)
-  {
-    double r = A[0] > A[4] ? A[0] : A[4];
-    return r > A[8] ? r : A[8];
-  }
-  double q = (A[0]+A[4]+A[8])/3.0;
-  double p2 = (A[0]-q)*(A[0]-q) + (A[4]-q)*(A[4]-q) + (A[8]-q)*(A[8]-q) + 2*p1;
-  double p = sqrt (p2 / 6.0);
-  double invp = 1.0 / p;
-  double B[9] = {invp*(A[0]-q), invp*A[1], invp*A[2], invp*A[3], invp*(A[4]-q), invp*A[5],
-                 invp*A[6], invp*A[7], invp*(A[8]-q)};
-  double det = B[0]*(B[4]*B[8]-B[5]*B[7]) + B[3]*(B[2]*B[7]-B[1]*B[8]) + B[6]*(B[1]*B[5]-B[2]*B[4]);
-  double r = 0.5 * det;
-  double phi;
-  if (r <= -1.0) phi = 3.14159265358979323846 / 3.0;
-  else if (r >= 1.0) phi = 0.0;
-  else phi = acos (r) / 3.0;
-  return q + 2.0 * p * cos (phi + (2*3.14159265358979323846/3));
-}
-
 /* eqm, parmec.ispc:423-482 */
 __device__ double eqm (int N, const double *inverse, const double *invm, const double *position, int part1,
                        const double point1[3], int part2, const double point2[3], const double dir[3])
@@ -128,7 +104,7 @@ __device__ double eqm (int N, const double *inverse, const double *invm, const d
     nvmul (Miv, dir, tmp);
     return 1.0/(tmp[0]*dir[0] + tmp[1]*dir[1] + tmp[2]*dir[2]);
   }
-  return 1.0/min_eigenvalue (Miv);
+  return 1.0/extremum_eigenvalue (Miv, true);
 }
 
 struct SprArgs
@@ -141,6 +117,7 @@ struct SprArgs
   const double *sx, *sy, *dx, *dy, *ux, *uy;
   const double *angular, *linear, *rotation, *position, *inverse, *invm;
   double *out;   /* [9][S]: force, torque on part1, torque on part2 */
+  double *adp;   /* [17][S] terms of the adaptive step, or null (see k_springs) */
   int *flags;
   const int *ctl;
   const int *stop, *par_inst;   /* frozen instances (staggered dem calls) */
@@ -363,6 +340,35 @@ __global__ void k_springs (SprArgs A, double step)
     A.frc[2*S + i] = sqrt (fs[0]*fs[0] + fs[1]*fs[1] + fs[2]*fs[2]);
     for (int c = 0; c < 3; c++) frc[c] += fs[c];
   }
+  if (A.adp)
+  {
+    /* the spring's part of the adaptive step's per-particle sums
+     * (forces.ispc:1249-1270, 1329-1350), zero where the total force is:
+     * |z|, |stiffness|, |damping|, then the rotational stiffness terms of
+     * part1's arm a and part2's arm b, both rotated by part1's L^T as the
+     * reference does */
+    double *d = A.adp;
+    double t[17] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (total_force != 0.0)
+    {
+      t[0] = fabs (z[0]); t[1] = fabs (z[1]); t[2] = fabs (z[2]);
+      t[3] = fabs (stiffness); t[4] = fabs (damping);
+      double R[3], dot;
+      tvmul (Lj, a, R);
+      dot = R[0]*R[0] + R[1]*R[1] + R[2]*R[2];
+      t[5] = dot-R[0]*R[0]; t[6] = dot-R[1]*R[1]; t[7] = dot-R[2]*R[2];
+      t[8] = -R[0]*R[1]; t[9] = -R[0]*R[2]; t[10] = -R[1]*R[2];
+      if (k >= 0)
+      {
+        tvmul (Lj, b, R);
+        dot = R[0]*R[0] + R[1]*R[1] + R[2]*R[2];
+        t[11] = dot-R[0]*R[0]; t[12] = dot-R[1]*R[1]; t[13] = dot-R[2]*R[2];
+        t[14] = -R[0]*R[1]; t[15] = -R[0]*R[2]; t[16] = -R[1]*R[2];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 17; c++) d[c*S + i] = t[c];
+  }
   double *o = A.out;
   o[i] = frc[0]; o[S + i] = frc[1]; o[2*S + i] = frc[2];
   o[3*S + i] = a[1]*frc[2] - a[2]*frc[1];
@@ -413,6 +419,191 @@ __global__ void k_spring_acc (int N, int S, const int *evoff, const int *evrun, 
   }
   force[b] = f[0]; force[(size_t) N + b] = f[1]; force[2*(size_t) N + b] = f[2];
   torque[b] = t[0]; torque[(size_t) N + b] = t[1]; torque[2*(size_t) N + b] = t[2];
+}
+
+/* per particle: the adaptive step's spring terms added run by run in the
+ * flush order of k_spring_acc (forces.ispc:1203-1411): each run sums its
+ * springs' terms from 0.0 (inactive springs hold +0.0, which changes no
+ * such sum), then adds them to the particle's contact sums; kmax and emax
+ * take the larger of the run's maximum and the particle's */
+__global__ void k_spring_adp (int N, int S, const int *evoff, const int *evrun, const int *runoff, const int *runspr,
+                              const double *t, double *adp, const int *ctl)
+{
+  if (ctl[CTL_ABORT]) return;
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= N) return;
+  int e0 = evoff[b], e1 = evoff[b+1];
+  if (e0 == e1) return;
+  double v[11];
+#pragma unroll
+  for (int c = 0; c < 11; c++) v[c] = adp[(size_t) c*N + b];
+  for (int ev = e0; ev < e1; ev++)
+  {
+    int code = evrun[ev];
+    int run = code >> 2;
+    int rc = (code & 1) ? 11 : 5;   /* part2 runs take the arm-b terms */
+    double ka[3] = {0.0, 0.0, 0.0}, km = 0.0, em = 0.0, kr[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int r = runoff[run]; r < runoff[run+1]; r++)
+    {
+      size_t s = runspr[r];
+      for (int c = 0; c < 3; c++) ka[c] += t[c*(size_t) S + s];
+      double st = t[3*(size_t) S + s], dm = t[4*(size_t) S + s];
+      km = st > km ? st : km;
+      em = dm > em ? dm : em;
+      for (int c = 0; c < 6; c++) kr[c] += t[(rc+c)*(size_t) S + s];
+    }
+    for (int c = 0; c < 3; c++) v[c] += ka[c];
+    v[3] = km > v[3] ? km : v[3];
+    v[4] = em > v[4] ? em : v[4];
+    for (int c = 0; c < 6; c++) v[5+c] += kr[c];
+  }
+#pragma unroll
+  for (int c = 0; c < 11; c++) adp[(size_t) c*N + b] = v[c];
+}
+
+/* ---------------------------------------------- critical step estimates */
+
+/* the largest |slope| of curve points [lo, hi) (parmec.ispc:510-515) */
+__device__ __forceinline__ double max_slope (const double *x, const double *y, int lo, int hi)
+{
+  double m = 0.0;
+  for (int q = lo; q < hi-1; q++)
+  {
+    double s = fabs ((y[q+1]-y[q]) / (x[q+1]-x[q]));
+    m = s > m ? s : m;
+  }
+  return m;
+}
+
+/* per spring, the bounds CRITICAL's per-particle estimate adds to both of
+ * its particles (critical_perparticle_springs, parmec.ispc:741-775): kspr,
+ * the largest |slope| of the spring curve, and espr, 2 ratio sqrt(kspr m)
+ * when the dashpot entry is a damping ratio, else the largest |slope| of
+ * the spring's dashpot curve.  The reference tests for the ratio form, and
+ * takes the ratio, at dashidx[i] with i the spring's first PARTICLE index
+ * (parmec.ispc:758, 770); that is reproduced where i < S, and the spring's
+ * own entry is used where dashidx[i+1] would lie beyond the array */
+__global__ void k_crit_spring_terms (SprArgs A, double *out)
+{
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= A.S) return;
+  size_t S = A.S;
+  int i = A.part[j], k = A.part[S + j];
+  double kspr = max_slope (A.sx, A.sy, A.spridx[j], A.spridx[j+1]);
+  double espr = 0.0;
+  int di = i < A.S ? i : j;
+  if (A.dashidx[di]+1 == A.dashidx[di+1])
+  {
+    double Pi[3], Pk[3], d[3];
+    for (int c = 0; c < 3; c++)
+    {
+      Pi[c] = A.pnt[(size_t) (3+c) * S + j];
+      Pk[c] = A.pnt[(size_t) (9+c) * S + j];
+      d[c] = A.dir[(size_t) c * S + j];
+    }
+    double mass = eqm (A.N, A.inverse, A.invm, A.position, i, Pi, k, Pk, d);
+    double ratio = A.dx[A.dashidx[di]];
+    espr = ratio * 2.0 * sqrt (kspr * mass);
+  }
+  else espr = max_slope (A.dx, A.dy, A.dashidx[j], A.dashidx[j+1]);
+  out[j] = kspr;
+  out[S + j] = espr;
+}
+
+/* per particle, its springs in spring order (the reference updates each
+ * particle under a lock from parallel tasks; here in ascending spring
+ * number, side 1 before side 2 of one spring): the summed |direction| (a
+ * zero direction adds 1 to each component; a planar one adds |t1| + |t2|
+ * of two unit vectors normal to it), the rotational stiffness terms of the
+ * arm L^T (point - x), and the larger of the current and the spring's
+ * kspr, espr (parmec.ispc:777-888).  The planar test reads sprflg at the
+ * PARTICLE index, as the reference does; a particle index >= S reads no
+ * flag */
+__global__ void k_crit_springs (SprArgs A, const int *pofs, const int *plist, const double *terms, double *adp)
+{
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= A.N) return;
+  int q0 = pofs[b], q1 = pofs[b+1];
+  if (q0 == q1) return;
+  size_t S = A.S, N = A.N;
+  double v[11], L[9], x[3];
+#pragma unroll
+  for (int c = 0; c < 11; c++) v[c] = adp[c*N + b];
+#pragma unroll
+  for (int c = 0; c < 9; c++) L[c] = A.rotation[c*N + b];
+#pragma unroll
+  for (int c = 0; c < 3; c++) x[c] = A.position[c*N + b];
+  int flg = b < A.S ? A.flg[b] : 0;
+  for (int q = q0; q < q1; q++)
+  {
+    int s = plist[q] >> 1, side = plist[q] & 1;
+    double d[3] = {A.dir[s], A.dir[S + s], A.dir[2*S + s]};
+    if (d[0]*d[0] + d[1]*d[1] + d[2]*d[2] > 0.0)
+    {
+      if (flg & PMX_SPRDIR_PLANAR)
+      {
+        double t1[3] = {d[0] + 0.123, d[1] + 0.346, d[2] + 0.789}, t2[3];
+        t2[0] = d[1]*t1[2] - d[2]*t1[1];
+        t2[1] = d[2]*t1[0] - d[0]*t1[2];
+        t2[2] = d[0]*t1[1] - d[1]*t1[0];
+        double l2 = sqrt (t2[0]*t2[0] + t2[1]*t2[1] + t2[2]*t2[2]);
+        double i2 = 1.0/l2;
+        t2[0] *= i2; t2[1] *= i2; t2[2] *= i2;
+        t1[0] = t2[1]*d[2] - t2[2]*d[1];
+        t1[1] = t2[2]*d[0] - t2[0]*d[2];
+        t1[2] = t2[0]*d[1] - t2[1]*d[0];
+        for (int c = 0; c < 3; c++) v[c] += fabs (t1[c]) + fabs (t2[c]);
+      }
+      else for (int c = 0; c < 3; c++) v[c] += fabs (d[c]);
+    }
+    else for (int c = 0; c < 3; c++) v[c] += 1.0;
+    double a[3], R[3];
+    for (int c = 0; c < 3; c++) a[c] = A.pnt[(size_t) (6*side+c) * S + s] - x[c];
+    tvmul (L, a, R);
+    double dot = R[0]*R[0] + R[1]*R[1] + R[2]*R[2];
+    v[5] += dot-R[0]*R[0];
+    v[6] += dot-R[1]*R[1];
+    v[7] += dot-R[2]*R[2];
+    v[8] += -R[0]*R[1];
+    v[9] += -R[0]*R[2];
+    v[10] += -R[1]*R[2];
+    double ks = terms[s], es = terms[S + s];
+    v[3] = v[3] > ks ? v[3] : ks;
+    v[4] = v[4] > es ? v[4] : es;
+  }
+#pragma unroll
+  for (int c = 0; c < 11; c++) adp[c*N + b] = v[c];
+}
+
+/* critical_perspring (parmec.ispc:485-535): each spring as a single degree
+ * of freedom of its equivalent mass along its direction, stiffness the
+ * largest |slope| of its curve; h = 2/w (sqrt(1+x^2) - x), or 1 where w is
+ * not positive */
+__global__ void k_crit_perspring (SprArgs A, double *h, double *o, double *r)
+{
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= A.S) return;
+  size_t S = A.S;
+  int i = A.part[j], k = A.part[S + j];
+  double Pi[3], Pk[3], d[3];
+  for (int c = 0; c < 3; c++)
+  {
+    Pi[c] = A.pnt[(size_t) (3+c) * S + j];
+    Pk[c] = A.pnt[(size_t) (9+c) * S + j];
+    d[c] = A.dir[(size_t) c * S + j];
+  }
+  double kspr = max_slope (A.sx, A.sy, A.spridx[j], A.spridx[j+1]);
+  double mspr = eqm (A.N, A.inverse, A.invm, A.position, i, Pi, k, Pk, d);
+  double omega = sqrt (kspr/mspr), ksi = 0.0;
+  if (A.dashidx[j]+1 == A.dashidx[j+1]) ksi = A.dx[A.dashidx[j]];
+  else
+  {
+    double espr = max_slope (A.dx, A.dy, A.dashidx[j], A.dashidx[j+1]);
+    if (omega > 0.0) ksi = espr / (2*mspr*omega);
+  }
+  h[j] = omega > 0.0 ? (2.0/omega)*(sqrt (1.0+ksi*ksi) - ksi) : 1.0;
+  o[j] = omega;
+  r[j] = ksi;
 }
 
 /* nodes_task, shapes.ispc:101-186 */
@@ -502,6 +693,20 @@ int springs_setup (pmx_ctx &c, const pmx_model *m)
   for (auto &x : ev) evoff[x.body + 1]++;
   for (int b = 0; b < c.N; b++) evoff[b+1] += evoff[b];
   for (size_t q = 0; q < ev.size (); q++) evrun[q] = ev[q].code;
+  /* each particle's springs in ascending spring number, (spring << 1) | side
+   * (CRITICAL's per-particle spring part) */
+  std::vector<int> pofs (c.N + 1, 0), plist;
+  for (int q = 0; q < S; q++) { pofs[p1[q] + 1]++; if (p2[q] >= 0) pofs[p2[q] + 1]++; }
+  for (int b = 0; b < c.N; b++) pofs[b+1] += pofs[b];
+  plist.resize (pofs[c.N]);
+  {
+    std::vector<int> at (pofs.begin (), pofs.end () - 1);
+    for (int q = 0; q < S; q++)
+    {
+      plist[at[p1[q]]++] = q << 1;
+      if (p2[q] >= 0) plist[at[p2[q]]++] = (q << 1) | 1;
+    }
+  }
   auto upi = [&] (dvec<int> &d, const int *src, size_t n) -> int {
     if (d.alloc (std::max (n, (size_t) 1)) != hipSuccess) { pmx_set_error ("out of device memory (springs)"); return PMX_ENOMEM; }
     if (n) PMX_HIP (hipMemcpyAsync (d.p, src, n * sizeof (int), hipMemcpyHostToDevice, c.stream));
@@ -516,6 +721,7 @@ int springs_setup (pmx_ctx &c, const pmx_model *m)
   size_t SS = S;
   if ((rc = upi (c.sp_evoff, evoff.data (), evoff.size ())) || (rc = upi (c.sp_evrun, evrun.data (), evrun.size ())) ||
       (rc = upi (c.sp_runoff, runoff.data (), runoff.size ())) || (rc = upi (c.sp_runspr, runspr.data (), runspr.size ())) ||
+      (rc = upi (c.sp_pofs, pofs.data (), pofs.size ())) || (rc = upi (c.sp_plist, plist.data (), plist.size ())) ||
       (rc = upi (c.sp_type, m->sprtype, SS)) || (rc = upi (c.sp_part, m->sprpart, 2*SS)) ||
       (rc = upi (c.sp_spridx, m->spridx, SS+1)) || (rc = upi (c.sp_dashidx, m->dashidx, SS+1)) ||
       (rc = upi (c.sp_unidx, m->unidx, SS+1)) || (rc = upi (c.sp_unspring, m->unspring, SS)) ||
@@ -548,9 +754,8 @@ int springs_download (pmx_ctx &c, pmx_model *m)
   return PMX_OK;
 }
 
-int step_springs (pmx_ctx &c, double h)
+static SprArgs spr_args (pmx_ctx &c)
 {
-  if (c.S == 0) return PMX_OK;
   SprArgs A;
   A.S = c.S; A.N = c.N;
   A.type = c.sp_type.p; A.part = c.sp_part.p; A.spridx = c.sp_spridx.p; A.dashidx = c.sp_dashidx.p;
@@ -563,6 +768,14 @@ int step_springs (pmx_ctx &c, double h)
   A.angular = c.angular.p; A.linear = c.linear.p; A.rotation = c.rotation.p; A.position = c.position.p;
   A.inverse = c.inverse.p; A.invm = c.invm.p; A.out = c.sp_out.p; A.flags = c.dflags.p; A.ctl = c.ctl.p;
   A.stop = c.staggered ? c.istop.p : nullptr; A.par_inst = c.par_inst.p;
+  A.adp = c.adp_on ? c.sp_adp.p : nullptr;
+  return A;
+}
+
+int step_springs (pmx_ctx &c, double h)
+{
+  if (c.S == 0) return PMX_OK;
+  SprArgs A = spr_args (c);
   {
     ProfScope ps (c, K_SPRINGS, 420.0 * c.S);
     hipLaunchKernelGGL (k_springs, grid (c.S), dim3 (PMX_TPB), 0, c.stream, A, h);
@@ -575,6 +788,34 @@ int step_springs (pmx_ctx &c, double h)
                         (const double*) c.sp_out.p, c.force.p, c.torque.p, (const int*) c.ctl.p,
                         c.staggered ? (const int*) c.istop.p : (const int*) nullptr, (const int*) c.par_inst.p);
   }
+  if (c.adp_on)
+    hipLaunchKernelGGL (k_spring_adp, grid (c.N), dim3 (PMX_TPB), 0, c.stream, c.N, c.S, (const int*) c.sp_evoff.p,
+                        (const int*) c.sp_evrun.p, (const int*) c.sp_runoff.p, (const int*) c.sp_runspr.p,
+                        (const double*) c.sp_adp.p, c.adp.p, (const int*) c.ctl.p);
+  PMX_HIP (hipGetLastError ());
+  return PMX_OK;
+}
+
+/* CRITICAL's spring part of the per-particle sums in c.adp (the contact
+ * part already there); c.sp_adp holds kspr, espr per spring meanwhile */
+int crit_springs (pmx_ctx &c)
+{
+  if (c.S == 0) return PMX_OK;
+  if (c.sp_adp.alloc (17 * (size_t) c.S)) { pmx_set_error ("out of device memory"); return PMX_ENOMEM; }
+  SprArgs A = spr_args (c);
+  hipLaunchKernelGGL (k_crit_spring_terms, grid (c.S), dim3 (PMX_TPB), 0, c.stream, A, c.sp_adp.p);
+  hipLaunchKernelGGL (k_crit_springs, grid (c.N), dim3 (PMX_TPB), 0, c.stream, A, (const int*) c.sp_pofs.p,
+                      (const int*) c.sp_plist.p, (const double*) c.sp_adp.p, c.adp.p);
+  PMX_HIP (hipGetLastError ());
+  return PMX_OK;
+}
+
+/* critical_perspring into device arrays h, o, r of S entries */
+int crit_perspring (pmx_ctx &c, double *h, double *o, double *r)
+{
+  if (c.S == 0) return PMX_OK;
+  SprArgs A = spr_args (c);
+  hipLaunchKernelGGL (k_crit_perspring, grid (c.S), dim3 (PMX_TPB), 0, c.stream, A, h, o, r);
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
 }

@@ -775,7 +775,17 @@ void k_rows_idle (int E, int N, const int *part, const double *mass, int ninst, 
  * persists stays FP64, the tangential state is stored in FP32 and the
  * per-sphere sums are Kahan-compensated FP32).  Block j takes slice
  * j % slices of list j / slices (lists of LIST entries at most). */
-template <class R>
+/* ADP: the instantiation of adaptive and critical-step runs, which also
+ * forms per particle the sums the reference's contacts_acc_task forms when
+ * adaptive stepping is on (forces.ispc:505-533, 542-577): the summed
+ * |direction| (the normal on the master side, the unit force on the slave
+ * side), the largest contact stiffness and damping coefficient (kn and
+ * en = 2 damper sqrt(kn m) of the master side; the reference never writes
+ * them on the slave side, taken as 0 here) and the rotational stiffness
+ * sums of the arms L (point - x), into adp [11][N] (when not null), and
+ * stores each row entry's point, |direction|, kn and en into crec
+ * [8][cap][E] (when not null) for the critical step estimate */
+template <class R, bool ADP = false>
 __global__ __launch_bounds__(64)
 void k_rows (int E, int N, const double *center, const double *radii, const int *part, const int *ellcol,
              const double *tri, int T, const int *tricol, const int *triobs,
@@ -786,7 +796,8 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
              uint32_t *hits, int *hitcnt, int hcap, const int *xbuf, int *xcnt, int xcap,
              const double *gtab, int *ctl, double h, double *force, double *torque, int *flags,
              const double *obspnt, const double *obsang, const double *obslin,
-             const int *busy, const int *nbusy, int list, int slices)
+             const int *busy, const int *nbusy, int list, int slices,
+             const double *rotation, double *adp, double *crec)
 {
   constexpr int TPB = 64;
   __shared__ uint32_t sh[ROWS_SORT * TPB];
@@ -842,6 +853,10 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
   int io = 0, ih = 0;
   R fs[3] = {0.0, 0.0, 0.0}, ts[3] = {0.0, 0.0, 0.0};
   R cf[3] = {0.0, 0.0, 0.0}, ct[3] = {0.0, 0.0, 0.0};   /* Kahan compensation (mixed) */
+  double ka[3] = {0.0, 0.0, 0.0}, km = 0.0, em = 0.0, kr[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, Le[9];
+  if constexpr (ADP)
+    if (adp)
+      for (int c = 0; c < 9; c++) Le[c] = rotation[(size_t) c*N + pe];
 
   while (io < oc || ih < nh)
   {
@@ -1023,6 +1038,46 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
 
     /* contacts_acc_task, forces.ispc:492-533: slaves receive -f */
     if (!master) { f[0] = -f[0]; f[1] = -f[1]; f[2] = -f[2]; }
+    if constexpr (ADP)
+    {
+      double dir[3] = {0.0, 0.0, 0.0}, kc = 0.0, ec = 0.0;
+      if (master)
+      {
+        for (int c = 0; c < 3; c++) dir[c] = fabs ((double) normal[c]);
+        kc = iparam[PMX_SPRING*pairnum + ij];
+        ec = iparam[PMX_DAMPER*pairnum + ij] * 2.0 * sqrt (kc*ma);
+      }
+      else
+      {
+        double g[3] = {(double) f[0], (double) f[1], (double) f[2]};
+        double fl = sqrt (g[0]*g[0] + g[1]*g[1] + g[2]*g[2]);
+        if (fl != 0.0)
+        {
+          fl = 1.0/fl;
+          for (int c = 0; c < 3; c++) dir[c] = fabs (g[c]*fl);
+        }
+      }
+      if (crec && cnt - 1 < cap)
+      {
+        double rec[8] = {point[0], point[1], point[2], dir[0], dir[1], dir[2], kc, ec};
+        for (int c = 0; c < 8; c++) crec[(size_t) (c*cap + cnt - 1) * E + e] = rec[c];
+      }
+      if (adp)
+      {
+        for (int c = 0; c < 3; c++) ka[c] += dir[c];
+        km = kc > km ? kc : km;
+        em = ec > em ? ec : em;
+        double av[3] = {point[0]-xe[0], point[1]-xe[1], point[2]-xe[2]}, A[3];
+        nvmul (Le, av, A);
+        double dot = A[0]*A[0] + A[1]*A[1] + A[2]*A[2];
+        kr[0] += dot - A[0]*A[0];
+        kr[1] += dot - A[1]*A[1];
+        kr[2] += dot - A[2]*A[2];
+        kr[3] += -A[0]*A[1];
+        kr[4] += -A[0]*A[2];
+        kr[5] += -A[1]*A[2];
+      }
+    }
     R a[3] = {(R) (point[0]-xe[0]), (R) (point[1]-xe[1]), (R) (point[2]-xe[2])};
     if constexpr (mixed)
     {
@@ -1049,6 +1104,12 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
   torque[pe] = (double) ts[0];
   torque[(size_t) N + pe] = (double) ts[1];
   torque[2*(size_t) N + pe] = (double) ts[2];
+  if constexpr (ADP)
+    if (adp)
+    {
+      double v[11] = {ka[0], ka[1], ka[2], km, em, kr[0], kr[1], kr[2], kr[3], kr[4], kr[5]};
+      for (int c = 0; c < 11; c++) adp[(size_t) c*N + pe] = v[c];
+    }
   mx = cnt > mx ? cnt : mx;
   }
 #pragma unroll
@@ -1534,11 +1595,11 @@ int step_detect (pmx_ctx &c)
 /* k_rows over the busy lists of nb idle blocks of tpb threads: one block
  * per list, or up to 8 per list (slices) when there are fewer than 4096
  * lists, so that small models spread their busy spheres over more waves */
-template <class R>
-void rows_launch (pmx_ctx &c, R *st0, R *st1, double h, int tpb, int nb)
+template <class R, bool ADP>
+void rows_launch_t (pmx_ctx &c, R *st0, R *st1, double h, int tpb, int nb)
 {
   int slices = std::max (1, std::min (8, 4096 / std::max (nb, 1)));
-  hipLaunchKernelGGL ((k_rows<R>), dim3 (nb * slices), dim3 (64), 0, c.stream, c.E, c.N,
+  hipLaunchKernelGGL ((k_rows<R, ADP>), dim3 (nb * slices), dim3 (64), 0, c.stream, c.E, c.N,
                       (const double*) c.center.p, (const double*) c.radii.p, (const int*) c.part.p,
                       (const int*) c.ellcol.p, (const double*) c.tri.p, c.T, (const int*) c.tricol.p,
                       (const int*) c.triobs.p, (const double*) c.position.p, (const double*) c.linear.p,
@@ -1550,7 +1611,16 @@ void rows_launch (pmx_ctx &c, R *st0, R *st1, double h, int tpb, int nb)
                       (const int*) c.xbuf.p, c.xcnt.p, c.xcap, (const double*) c.gtab.p,
                       c.ctl.p, h, c.force.p, c.torque.p, c.dflags.p,
                       (const double*) c.obspnt.p, (const double*) c.obsang.p, (const double*) c.obslin.p,
-                      (const int*) c.rbusy.p, (const int*) c.rnbusy.p, tpb, slices);
+                      (const int*) c.rbusy.p, (const int*) c.rnbusy.p, tpb, slices,
+                      (const double*) c.rotation.p, ADP && c.adp_on ? c.adp.p : (double*) nullptr,
+                      ADP && c.opt.critical ? c.crec.p : (double*) nullptr);
+}
+
+template <class R>
+void rows_launch (pmx_ctx &c, R *st0, R *st1, double h, int tpb, int nb)
+{
+  if (c.adp_on || c.opt.critical) rows_launch_t<R, true> (c, st0, st1, h, tpb, nb);
+  else rows_launch_t<R, false> (c, st0, st1, h, tpb, nb);
 }
 
 template <int TPB>
@@ -1687,20 +1757,41 @@ int step_prescribe_vel (pmx_ctx &c)
   return PMX_OK;
 }
 
-int enqueue_step (pmx_ctx &c, double h)
+/* a step up to its forces (partitioning_store .. forces, prescribed
+ * accelerations; with the adaptive step on, the per-particle estimates and
+ * their minimum): everything that runs before the step's length step1 is
+ * known.  h is step0, the length of the tangential contact update */
+int enqueue_step_forces (pmx_ctx &c, double h)
 {
   int rc;
   if (c.E > 0 && ((rc = step_bin (c)) || (rc = step_mtri (c)) || (rc = step_detect (c)))) return rc;
+  if (c.adp_on && (rc = adp_begin (c))) return rc;
   /* mesh nodes and faces (shapes.ispc:101-211) are not updated per step:
    * no contact on this path reads mesh triangles (spheres and meshed
    * particles never share an instance) and springs carry their own points,
    * so they are formed from the final particle state when an instance
    * finishes, with the same arithmetic and the same result */
-  if ((rc = step_rows (c, h)) || (rc = step_springs (c, h)) || (rc = step_prescribe_acc (c)) ||
-      (rc = step_dynamics (c, h, h)) || (rc = step_prescribe_vel (c)) ||
-      (rc = step_obstacles (c, c.obstab.p, -1, h)) || (rc = step_history (c)) || (rc = step_end (c)))
+  if ((rc = step_rows (c, h)) || (rc = step_springs (c, h)) || (rc = step_prescribe_acc (c))) return rc;
+  if (c.adp_on && (rc = adp_time_step (c, nullptr, nullptr, nullptr))) return rc;
+  return PMX_OK;
+}
+
+/* the rest of a step: dynamics with step0 and step1, prescribed
+ * velocities, obstacles moved by (step0+step1)/2, the history sample and
+ * the step counter */
+int enqueue_step_motion (pmx_ctx &c, double step0, double step1)
+{
+  int rc;
+  if ((rc = step_dynamics (c, step0, step1)) || (rc = step_prescribe_vel (c)) ||
+      (rc = step_obstacles (c, c.obstab.p, -1, 0.5*(step0+step1))) || (rc = step_history (c)) || (rc = step_end (c)))
     return rc;
   return PMX_OK;
+}
+
+int enqueue_step (pmx_ctx &c, double h)
+{
+  int rc = enqueue_step_forces (c, h);
+  return rc ? rc : enqueue_step_motion (c, h, h);
 }
 
 int step_shapes (pmx_ctx &c)

@@ -14,10 +14,12 @@ supported subset:
 Names, argument order, defaults and error types follow the reference:
 MATERIAL (input.cpp:728-746), SPHERE (749-837), OBSTACLE (1367-1437),
 GRANULAR (2384-2455), VELOCITY (2798-2837), GRAVITY (2840-2933),
-DAMPING (2936-3013), TSERIES (626-725), DEM (3958-4091), RESET (610-623).
-Gravity and damping callbacks and TSERIES are evaluated on the host at each
-step's curtime (input.cpp:4424-4503) and handed to the device as per-step
-tables.  The step loop itself is libpmx's pmx_dem (C++ host + HIP kernels).
+DAMPING (2936-3013), TSERIES (626-725), DEM (3958-4091), RESET (610-623),
+CRITICAL (3034-3154).  Gravity and damping callbacks and TSERIES are
+evaluated on the host at each step's curtime (input.cpp:4424-4503) and
+handed to the device as per-step tables, or step by step when DEM takes the
+adaptive step.  The step loop itself is libpmx's pmx_dem / pmx_dem_adaptive
+(C++ host + HIP kernels); CRITICAL is pmx_critical.
 
 MESH (840-1156) builds a meshed particle from volume elements (its
 surface triangles and mass characteristics as mesh.cpp:315-521 forms them;
@@ -26,8 +28,8 @@ supported without stroke offsets (load-curve TSERIES); OBSTACLE with
 point/linear/angular (moving obstacles) and PRESCRIBE (2647-2795) tabulate
 their callables or TSERIES at the step times.  Not on this path (raise
 NotImplementedError): ANALYTICAL, torsion springs, unsprings, joints, XDMF
-and MED output (OUTPUT writes DUMP and VTK files), CRITICAL, adaptive time
-stepping, and HISTORY's sphere/box sources and .h5 reading.
+and MED output (OUTPUT writes DUMP and VTK files), and HISTORY's sphere/box
+sources and .h5 reading.
 """
 import bisect
 import math
@@ -763,7 +765,6 @@ def _write_output(m, t, prefix_path):
     order = sorted(range(len(recs)), key=lambda i: (recs[i]["part"][0], recs[i]["part"][1], i))
     output.write(m, t, _S.output_frame, prefix_path, _S.outformat, _S.outputs, _S.outrest, flags, order)
     _S.output_frame += 1
-CRITICAL = _not_here("CRITICAL")
 
 
 def _build_model():
@@ -988,51 +989,40 @@ def _history_rows(times, curtime_history, dt_his):
     return rows, tsamp, ch
 
 
-def DEM(duration, step, interval=None, prefix=None, adaptive=0.0):
-    """Run the DEM time stepping; returns the wall time in seconds
-    (input.cpp:3958-4091, parmec.cpp:2315-2530).  interval = (output, history)
-    or one value for both (number, callable or TSERIES number; default the
-    step): HISTORY lists are appended at each history interval, and the
-    DUMP/VTK files OUTPUT asked for are written at each output interval
-    (under the deck's path without '.py', or the prefix).""" 
-    if not duration > 0:
-        raise ValueError("'duration' must be positive")
-    if not step > 0:
-        raise ValueError("'step' must be positive")
-    if prefix is not None and not isinstance(prefix, str):
-        raise TypeError("'prefix' must be a string")
-    if not (0.0 <= adaptive <= 1.0):
-        raise ValueError("'adaptive' must be >= 0 and <= 1")
-    if adaptive > 0.0:
-        raise NotImplementedError("adaptive time stepping is not on the MI355X step path")
+def _engine_model():
+    """The engine holding the deck's model, uploading the model when the
+    deck changed since the last upload."""
     if _S.engine is None:
-        _S.engine = pmx.Engine(device=_S.options["device"], row_capacity=_S.options["row_capacity"])
+        _S.engine = pmx.Engine(device=_S.options["device"], row_capacity=_S.options["row_capacity"], critical=1)
         _S.dirty = True
     if _S.dirty or _S.model is None:
         _S.model = _build_model()
         _S.engine.upload(_S.model)
         _S.dirty = False
+    return _S.engine, _S.model
+
+
+def _inputs_at():
+    """Evaluators of dem()'s host inputs at a time (input.cpp:4199-4503):
+    gravity, damping, obstacle velocities (angular, linear) and the
+    prescribed values of each history column; None where the deck has
+    nothing time dependent of that kind."""
     m = _S.model
-    h = step if m.curtime == 0.0 else m.curstep
-    times = pmx.step_times(duration, h, m.curtime)
-    gtab = None
+    grav = None
     if any(f is not None for f in _S.gravfunc):
-        gtab = np.empty((len(times), 3))
-        for k in range(3):
-            f = _S.gravfunc[k]
-            gtab[:, k] = [float(f(t)) for t in times] if f is not None else _S.gravity[k]
-    dtab = None
+        def grav(t):
+            return [float(_S.gravfunc[k](t)) if _S.gravfunc[k] is not None else _S.gravity[k] for k in range(3)]
+    damp = None
     if _S.lindamp is not None or _S.angdamp is not None:
-        dtab = np.zeros((len(times), 6))
-        for i, t in enumerate(times):
+        def damp(t):
+            d = np.zeros(6)
             if _S.lindamp is not None:
-                dtab[i, :3] = _S.lindamp(t)
+                d[:3] = _S.lindamp(t)
             if _S.angdamp is not None:
-                dtab[i, 3:] = _S.angdamp(t)
-    obstab = obstab0 = None
+                d[3:] = _S.angdamp(t)
+            return d
+    vel = None
     if _S.obstacles:
-        # obstaclev (input.cpp:4199-4244) at curtime+h of each step, and at
-        # h/2 before the first step of the first call
         def vel(t):
             row = np.zeros((len(_S.obstacles), 6))
             for k, (_, _, _, lin, ang) in enumerate(_S.obstacles):
@@ -1041,76 +1031,189 @@ def DEM(duration, step, interval=None, prefix=None, adaptive=0.0):
                 if lin is not None:
                     row[k, 3:] = [float(x) for x in lin(t)]
             return row
-        obstab = np.array([vel(t + h) for t in times]).reshape(len(times), len(_S.obstacles), 6)
-        if m.curtime == 0.0:
-            obstab0 = vel(0.5 * h)
-    prstab = None
+    prs = None
     if _S.prescribed:
-        # prescribe_acceleration / prescribe_velocity at each step's curtime,
-        # one column per distinct pair of histories
         ncol = int(m.prscol.max()) + 1
-        prstab = np.zeros((len(times), ncol, 6))
-        done = set()
+        cols = {}
         for k, (_, lin, ang, _, _) in enumerate(_S.prescribed):
-            c = int(m.prscol[k])
-            if c in done:
-                continue
-            done.add(c)
-            for i, t in enumerate(times):
+            cols.setdefault(int(m.prscol[k]), (lin, ang))
+
+        def prs(t):
+            row = np.zeros((ncol, 6))
+            for c, (lin, ang) in cols.items():
                 if lin is not None:
-                    prstab[i, c, :3] = [float(v) for v in lin(t)]
+                    row[c, :3] = [float(v) for v in lin(t)]
                 if ang is not None:
-                    prstab[i, c, 3:] = [float(v) for v in ang(t)]
+                    row[c, 3:] = [float(v) for v in ang(t)]
+            return row
+    return grav, damp, vel, prs
+
+
+def _intervals(interval, step):
+    """(output, history) interval evaluators of DEM's interval argument."""
+    if isinstance(interval, tuple):
+        if len(interval) != 2:
+            raise ValueError("Invalid output interval")
+        return _interval(interval[0], step), _interval(interval[1], step)
+    return _interval(interval, step), _interval(interval, step)
+
+
+def DEM(duration, step, interval=None, prefix=None, adaptive=0.0):
+    """Run the DEM time stepping; returns the wall time in seconds
+    (input.cpp:3958-4091, parmec.cpp:2315-2530).  interval = (output, history)
+    or one value for both (number, callable or TSERIES number; default the
+    step): HISTORY lists are appended at each history interval, and the
+    DUMP/VTK files OUTPUT asked for are written at each output interval
+    (under the deck's path without '.py', or the prefix).  adaptive in
+    (0, 1] takes each step as that fraction of the step's critical estimate
+    (parmec.cpp:2444-2447).""" 
+    if not duration > 0:
+        raise ValueError("'duration' must be positive")
+    if not step > 0:
+        raise ValueError("'step' must be positive")
+    if prefix is not None and not isinstance(prefix, str):
+        raise TypeError("'prefix' must be a string")
+    if not (0.0 <= adaptive <= 1.0):
+        raise ValueError("'adaptive' must be >= 0 and <= 1")
+    engine, m = _engine_model()
+    h = step if m.curtime == 0.0 else m.curstep
+    grav, damp, vel, prs = _inputs_at()
+    dt_out, dt_his = _intervals(interval, step)
+    obstab0 = vel(0.5 * h) if vel is not None and m.curtime == 0.0 else None
+    if _S.histories:
+        ents = [e for e, _, _, _ in _S.histories]
+        engine.set_history(ents, [l for _, l, _, _ in _S.histories], [p for _, _, p, _ in _S.histories])
+        if m.curtime == 0.0:               # the sample before the first step (parmec.cpp:2386-2394)
+            v = engine.history_sample()
+            for i, (e, _, _, out) in enumerate(_S.histories):
+                out.append(0.0 if e == "TIME" else float(v[i]))
+    files = bool(_S.outformat & (OUT_FORMAT_DUMP | OUT_FORMAT_VTK))
+    path = _output_path(prefix) if files else None
+    if files and m.curtime == 0.0:         # output_files before the first step (parmec.cpp:2386-2392)
+        _write_output(m, 0.0, path)
+    if adaptive > 0.0:
+        return _dem_adaptive(engine, m, duration, step, adaptive, grav, damp, vel, prs, obstab0, dt_out, dt_his,
+                             path)
+    times = pmx.step_times(duration, h, m.curtime)
+    n = len(times)
+    gtab = np.array([grav(t) for t in times]).reshape(n, 3) if grav is not None else None
+    dtab = np.array([damp(t) for t in times]).reshape(n, 6) if damp is not None else None
+    # obstaclev at curtime+h of each step, prescribed values at curtime
+    obstab = np.array([vel(t + h) for t in times]).reshape(n, len(_S.obstacles), 6) if vel is not None else None
+    prstab = np.array([prs(t) for t in times]).reshape(n, -1, 6) if prs is not None else None
     hisrow = hisout = None
     if _S.histories:
         # which steps' end states are sampled (parmec.cpp:2509-2514): the
         # history interval at each step's curtime, crossed by curtime
-        if isinstance(interval, tuple):
-            if len(interval) != 2:
-                raise ValueError("Invalid output interval")
-            dt_his = _interval(interval[1], step)
-        else:
-            dt_his = _interval(interval, step)
-        ents = [e for e, _, _, _ in _S.histories]
-        _S.engine.set_history(ents, [l for _, l, _, _ in _S.histories], [p for _, _, p, _ in _S.histories])
-        if m.curtime == 0.0:               # the sample before the first step (parmec.cpp:2386-2394)
-            v = _S.engine.history_sample()
-            for i, (e, _, _, out) in enumerate(_S.histories):
-                out.append(0.0 if e == "TIME" else float(v[i]))
         hisrow, tsamp, _S.curtime_history = _history_rows(times, _S.curtime_history, dt_his)
-        hisout = np.zeros((max(len(tsamp), 1), len(ents)))
+        hisout = np.zeros((max(len(tsamp), 1), len(_S.histories)))
         if not tsamp:
             hisrow = hisout = None
     outsteps, on_output = None, None
-    if _S.outformat & (OUT_FORMAT_DUMP | OUT_FORMAT_VTK):
-        # output_files at the output points (parmec.cpp:2386-2392, 2502-2507):
-        # the state before the first step of the first call, then the end
-        # state of every step whose curtime reaches curtime_output + interval
-        path = _output_path(prefix)
-        if isinstance(interval, tuple):
-            if len(interval) != 2:
-                raise ValueError("Invalid output interval")
-            dt_out = _interval(interval[0], step)
-        else:
-            dt_out = _interval(interval, step)
-        if m.curtime == 0.0:
-            _write_output(m, 0.0, path)
+    if files:
+        # output_files at the end state of every step whose curtime reaches
+        # curtime_output + interval (parmec.cpp:2502-2507)
         orow, tout, _S.curtime_output = _history_rows(times, _S.curtime_output, dt_out)
-        outsteps = [k for k in range(len(times)) if orow[k] >= 0]
+        outsteps = [k for k in range(n) if orow[k] >= 0]
 
         def on_output(k):
             snap = m.copy()
-            _S.engine.download(snap)
+            engine.download(snap)
             _write_output(snap, float(times[k]), path)
-    f = pmx.forcing(len(times), gravity=tuple(_S.gravity), gtab=gtab, dtab=dtab, obstab=obstab, obstab0=obstab0,
+    f = pmx.forcing(n, gravity=tuple(_S.gravity), gtab=gtab, dtab=dtab, obstab=obstab, obstab0=obstab0,
                     prstab=prstab, hisrow=hisrow, hisout=hisout, outsteps=outsteps, on_output=on_output)
-    elapsed = _S.engine.dem(float(duration), float(step), f)
-    _S.engine.download(m)
+    elapsed = engine.dem(float(duration), float(step), f)
+    engine.download(m)
     if hisrow is not None:
         for r, t in enumerate(tsamp):
             for i, (e, _, _, out) in enumerate(_S.histories):
                 out.append(float(t) if e == "TIME" else float(hisout[r, i]))
     return elapsed
+
+
+def _dem_adaptive(engine, m, duration, step, adaptive, grav, damp, vel, prs, obstab0, dt_out, dt_his, path):
+    """DEM with the adaptive step: the inputs of each step are evaluated at
+    its curtime as the step is reached, and the history samples and output
+    files are taken at pauses after the steps that reach an interval (the
+    same states the tabulated path samples inside the step loop)."""
+    due = {}
+
+    def on_step(k, t, h0, inp):
+        if grav is not None:
+            inp.gravity[:] = grav(t)
+        if damp is not None:
+            inp.damping[:] = damp(t)
+        if vel is not None:
+            inp.obsvel[:] = vel(t + h0)
+        if prs is not None:
+            inp.prs[:] = prs(t)
+        files = his = False
+        if path is not None:
+            d = dt_out(t)
+            if t >= _S.curtime_output + d:
+                _S.curtime_output += d
+                files = True
+        if _S.histories:
+            d = dt_his(t)
+            if t >= _S.curtime_history + d:
+                _S.curtime_history += d
+                his = True
+        if files or his:
+            due[k] = (t, files, his)
+            inp.output = 1
+
+    def on_output(k):
+        t, files, his = due.pop(k)
+        if files:
+            snap = m.copy()
+            engine.download(snap)
+            _write_output(snap, t, path)
+        if his:
+            v = engine.history_sample()
+            for i, (e, _, _, out) in enumerate(_S.histories):
+                out.append(t if e == "TIME" else float(v[i]))
+
+    obsnum = len(_S.obstacles)
+    prscols = int(m.prscol.max()) + 1 if _S.prescribed else 0
+    f = pmx.forcing(0, gravity=tuple(_S.gravity), obstab0=obstab0, on_step=on_step, on_output=on_output,
+                    obsnum=obsnum, prscols=prscols)
+    elapsed = engine.dem_adaptive(float(duration), float(step), float(adaptive), f)
+    engine.download(m)
+    return elapsed
+
+
+def CRITICAL(perspring=-1, perparticle=-1):
+    """Critical time step estimates (input.cpp:3034-3154): with no argument
+    the smallest per-particle estimate; perspring = n and/or perparticle = n
+    give the n smallest as lists of (step, index, omega, ratio), sorted by
+    step (a pair of lists when both are asked for).  The per-particle
+    estimate takes the contacts of the last step taken, so before the first
+    DEM only springs load a particle, and a particle nothing loads estimates
+    DBL_MAX.  NaN estimates (a loaded particle with zero stiffness) sort
+    last, and equal ones by index."""
+    if not isinstance(perspring, int) or not isinstance(perparticle, int):
+        raise TypeError("'perspring' and 'perparticle' must be integers")
+    engine, m = _engine_model()
+    if perspring < -1 or perspring > m.sprnum:
+        raise ValueError("Parameter 'perspring' is out of range")
+    if perparticle < -1 or perparticle > m.parnum:
+        raise ValueError("Parameter 'perparticle' is out of range")
+
+    def ranked(which, count):
+        h, o, r = engine.critical(which)
+        order = sorted(range(len(h)), key=lambda i: (h[i] != h[i], h[i] if h[i] == h[i] else 0.0, i))
+        return [(float(h[i]), int(i), float(o[i]), float(r[i])) for i in order[:count]]
+
+    sp = ranked(pmx.CRIT_SPRING, perspring) if perspring >= 0 else None
+    pp = ranked(pmx.CRIT_PARTICLE, perparticle) if perparticle >= 0 else None
+    if sp is not None and pp is not None:
+        return (sp, pp)
+    if sp is not None:
+        return sp
+    if pp is not None:
+        return pp
+    v = ranked(pmx.CRIT_PARTICLE, 1)
+    return v[0][0] if v else 1.7976931348623157e308
 
 
 def _use_engine(engine):

@@ -45,15 +45,43 @@ class pmx_model(C.Structure):
     ]
 
 
+class pmx_step_inputs(C.Structure):
+    _fields_ = [("gravity", C.c_double * 3), ("damping", C.c_double * 6), ("obsvel", _dp), ("prs", _dp),
+                ("hisrow", C.c_int), ("output", C.c_int)]
+
+
 class pmx_forcing(C.Structure):
     _fields_ = [("gravity", C.c_double * 3), ("gtab", _dp), ("damping", C.c_double * 6), ("dtab", _dp),
                 ("obstab", _dp), ("obstab0", _dp), ("prstab", _dp), ("prscols", C.c_int),
                 ("hisrow", _ip), ("hisrows", C.c_int), ("hisout", _dp),
-                ("outstep", _ip), ("noutstep", C.c_int), ("out_fn", C.c_void_p), ("out_user", C.c_void_p)]
+                ("outstep", _ip), ("noutstep", C.c_int), ("out_fn", C.c_void_p), ("out_user", C.c_void_p),
+                ("step_fn", C.c_void_p), ("step_user", C.c_void_p)]
 
 
 # pmx_forcing.out_fn: void (*)(void *user, int step)
 OUT_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int)
+# pmx_forcing.step_fn: int (*)(void *user, int k, double curtime, double step0, pmx_step_inputs *in)
+STEP_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_double, C.c_double, C.POINTER(pmx_step_inputs))
+
+# pmx_critical estimates
+CRIT_PARTICLE, CRIT_SPRING = 0, 1
+
+
+class StepInputs:
+    """The host inputs of one adaptive step, as on_step(k, curtime, step0,
+    inputs) fills them: gravity (3), damping (6), obsvel [obsnum][6]
+    (angular, linear velocity at curtime + step0), prs [prscols][6]
+    (prescribed linear, angular values at curtime), hisrow (the history row
+    sampling the step's end state, or -1) and output (1: output point)."""
+
+    def __init__(self, raw, obsnum, prscols):
+        self._raw = raw
+        self.gravity = np.ctypeslib.as_array(raw.gravity)
+        self.damping = np.ctypeslib.as_array(raw.damping)
+        self.obsvel = np.ctypeslib.as_array(raw.obsvel, shape=(max(obsnum, 1) * 6,)).reshape(-1, 6)[:obsnum]
+        self.prs = np.ctypeslib.as_array(raw.prs, shape=(max(prscols, 1) * 6,)).reshape(-1, 6)[:prscols]
+        self.hisrow = -1
+        self.output = 0
 
 
 # time-history entities of HISTORY (input.cpp:3177-3281), as PMX_HIS_* in pmx.h
@@ -68,7 +96,7 @@ class pmx_history(C.Structure):
 
 class pmx_options(C.Structure):
     _fields_ = [("precision", C.c_int), ("device", C.c_int), ("row_capacity", C.c_int), ("verbose", C.c_int),
-                ("tune", C.c_int), ("det_waves", C.c_int), ("rows_block", C.c_int)]
+                ("tune", C.c_int), ("det_waves", C.c_int), ("rows_block", C.c_int), ("critical", C.c_int)]
 
 
 class pmx_pair(C.Structure):
@@ -130,6 +158,8 @@ def _declare(lib, prefix):
     f("get_stats").argtypes = [C.c_void_p, C.POINTER(pmx_stats)]
     f("destroy").argtypes = [C.c_void_p]
     f("destroy").restype = None
+    f("critical").argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp]
+    f("dem_adaptive").argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.POINTER(pmx_forcing), _dp]
     return lib
 
 
@@ -297,11 +327,14 @@ class Model:
 
 
 def forcing(nsteps=0, gravity=(0.0, 0.0, 0.0), gtab=None, damping=(0.0,) * 6, dtab=None, obstab=None, obstab0=None,
-            prstab=None, hisrow=None, hisout=None, outsteps=None, on_output=None):
+            prstab=None, hisrow=None, hisout=None, outsteps=None, on_output=None, on_step=None, obsnum=0,
+            prscols=0):
     """Per-step inputs of dem(): gravity and damping constants or tables
     ([nsteps][3], [nsteps][6]); moving obstacle velocities (angular 0-2,
     linear 3-5) per step at curtime+step ([nsteps][obsnum][6]) and at step/2
-    for the first call ([obsnum][6])."""
+    for the first call ([obsnum][6]).  For dem_adaptive, on_step(k, curtime,
+    step0, inputs) fills each step's inputs (StepInputs; obsnum and prscols
+    size its arrays); an exception there stops the call."""
     f = pmx_forcing()
     for k in range(3):
         f.gravity[k] = gravity[k]
@@ -351,6 +384,27 @@ def forcing(nsteps=0, gravity=(0.0, 0.0, 0.0), gtab=None, damping=(0.0,) * 6, dt
         f.noutstep = len(outsteps)
         f.out_fn = C.cast(cb, C.c_void_p)
         keep.extend([outsteps, cb])
+    if on_step is not None:
+        errors = []
+
+        def step_cb(user, k, curtime, step0, raw):
+            try:
+                inp = StepInputs(raw.contents, obsnum, prscols)
+                on_step(int(k), float(curtime), float(step0), inp)
+                raw.contents.hisrow = int(inp.hisrow)
+                raw.contents.output = int(inp.output)
+                return 0
+            except BaseException as e:   # reported after the call returns
+                errors.append(e)
+                return 1
+        cb = STEP_FN(step_cb)
+        f.step_fn = C.cast(cb, C.c_void_p)
+        f._step_errors = errors
+        keep.append(cb)
+        if on_output is not None and not (outsteps is not None and len(outsteps)):
+            ocb = OUT_FN(lambda user, step: on_output(int(step)))
+            f.out_fn = C.cast(ocb, C.c_void_p)
+            keep.append(ocb)
     f._keep = keep
     return f
 
@@ -360,9 +414,10 @@ class Engine:
 
     prefix = "pmx_"
 
-    def __init__(self, precision=PMX_FP64, device=0, row_capacity=0, lib=None, tune=0, det_waves=0, rows_block=0):
+    def __init__(self, precision=PMX_FP64, device=0, row_capacity=0, lib=None, tune=0, det_waves=0, rows_block=0,
+                 critical=0):
         self.lib = lib or load()
-        opt = pmx_options(precision, device, row_capacity, 0, tune, det_waves, rows_block)
+        opt = pmx_options(precision, device, row_capacity, 0, tune, det_waves, rows_block, critical)
         h = C.c_void_p()
         self._call("create", C.byref(opt), C.byref(h))
         self.h = h
@@ -412,6 +467,30 @@ class Engine:
 
     def upload(self, model):
         self._call("upload", self.h, C.byref(model.struct()))
+        self._counts = (model.parnum, model.sprnum)
+
+    def critical(self, which=CRIT_PARTICLE):
+        """pmx_critical: per particle (CRIT_PARTICLE) or per spring
+        (CRIT_SPRING) the critical step estimate, circular frequency and
+        damping ratio, as three arrays."""
+        n = self._counts[0] if which == CRIT_PARTICLE else self._counts[1]
+        h, o, r = np.zeros(max(n, 1)), np.zeros(max(n, 1)), np.zeros(max(n, 1))
+        self._call("critical", self.h, int(which), h.ctypes.data_as(_dp), o.ctypes.data_as(_dp),
+                   r.ctypes.data_as(_dp))
+        return h[:n], o[:n], r[:n]
+
+    def dem_adaptive(self, duration, step, adaptive, forcing_=None):
+        """pmx_dem_adaptive; forcing_ from forcing(on_step=...)."""
+        el = C.c_double(0.0)
+        try:
+            self._call("dem_adaptive", self.h, duration, step, adaptive,
+                       C.byref(forcing_) if forcing_ is not None else None, C.byref(el))
+        except PmxError:
+            errs = getattr(forcing_, "_step_errors", None)
+            if errs:
+                raise errs[0]
+            raise
+        return el.value
 
     def download(self, model):
         s = model.struct()
