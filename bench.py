@@ -7,8 +7,15 @@ independent instances of the 10,000-sphere box set-up (sphere count uniform in
 together in one batch (one fused launch per stage, the time step replayed as a
 HIP graph), FP32 contact law / FP64 integration.  The synthetic input is the
 sweep's state after --settle steps (default 3000, run on the GPU before
-timing: the spheres have fallen into contact); then --warmup untimed steps,
-then --steps timed steps, with the state and all inputs resident in HBM.
+timing: the spheres have fallen into contact), then --warmup steps, then
+--steps timed steps, all in one dem call as a long run takes them (the
+call's own set-up, the partitioning tree of every instance and the step
+graph, falls at its start).  The timed steps are measured on the device
+timeline, from an event recorded on the library's stream before the first
+of them to one after the last (pmx_set_window; a step repeated after a leaf
+overflow re-records its event, so a rebuild inside the window is counted),
+with the state and all inputs resident in HBM; the host clock of the whole
+call is reported beside it.
 
 Beside it, in the same JSON line:
   plain_fp64  -- the drop-in FP64 path stepping sweep instances one at a time
@@ -157,9 +164,10 @@ def sweep_forcings(pmx, workloads, insts, nsteps, dt, t0):
     return [pmx.forcing(nsteps, gtab=workloads.sweep_forcing(a, f, nsteps, dt, t0)) for (_, a, f) in insts]
 
 
-def plain_fp64_sample(pmx, workloads, insts, sample, start, steps, dt):
-    """The drop-in FP64 path, one instance at a time, over the same window
-    (each sampled instance stepped alone from rest to the window's start)."""
+def plain_fp64_sample(pmx, workloads, insts, sample, settle, warm, steps, dt):
+    """The drop-in FP64 path, one instance at a time, over the same window,
+    timed the same way (each sampled instance stepped alone from rest in one
+    call, the window's steps timed on the device)."""
     import torch
     tot_ps, tot_s, ids = 0, 0.0, []
     for i in sample:
@@ -167,20 +175,17 @@ def plain_fp64_sample(pmx, workloads, insts, sample, start, steps, dt):
         m = m.copy()
         e = pmx.Engine()
         e.upload(m)
-        done = 0
-        while done < start:
-            n = min(1000, start - done)
-            e.dem(exact_duration(n, dt), dt, pmx.forcing(n, gtab=workloads.sweep_forcing(a, f, n, dt, done * dt)))
-            done += n
-        fs = pmx.forcing(steps, gtab=workloads.sweep_forcing(a, f, steps, dt, start * dt))
+        n = settle + warm + steps
+        e.lib.pmx_set_window.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        e.lib.pmx_set_window(e.h, settle + warm, n - 1)
+        fs = pmx.forcing(n, gtab=workloads.sweep_forcing(a, f, n, dt, 0.0))
+        e.dem(exact_duration(n, dt), dt, fs)
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        e.dem(exact_duration(steps, dt), dt, fs)
-        torch.cuda.synchronize()
-        sec = time.perf_counter() - t0
         st = e.stats()
         e.close()
-        tot_ps += m.parnum * st["steps"]
+        assert st["steps"] == n and st["window_ms"] > 0
+        sec = st["window_ms"] / 1e3
+        tot_ps += m.parnum * steps
         tot_s += sec
         ids.append(i)
     return tot_ps / tot_s, ids
@@ -197,19 +202,19 @@ def dense_leg(pmx, workloads, args, dt):
     eng = pmx.Engine(precision=pmx.PMX_MIXED)
     eng.upload_batch(models)
     W, K = args.dense_warmup, args.dense_steps
-    eng.dem_batch(exact_duration(W, dt), dt, sweep_forcings(pmx, workloads, insts, W, dt, 0.0))
-    busy0 = eng.stats()["busy"]
+    # warm-up and timed steps in one call, the timed steps measured on the
+    # device and profiled (as the sweep's window)
     eng.lib.pmx_set_profiling(eng.h, 1)
+    eng.lib.pmx_set_window.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    eng.lib.pmx_set_window(eng.h, W, W + K - 1)
+    eng.dem_batch(exact_duration(W + K, dt), dt, sweep_forcings(pmx, workloads, insts, W + K, dt, 0.0))
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    eng.dem_batch(exact_duration(K, dt), dt, sweep_forcings(pmx, workloads, insts, K, dt, W * dt))
-    torch.cuda.synchronize()
-    sec = time.perf_counter() - t0
     st = eng.stats()
+    sec = st["window_ms"] / 1e3
     kernels = kernel_table(pmx, eng)
     eng.close()
     value = ntot * K / sec
-    busy = 0.5 * (busy0 + st["busy"])
+    busy = st["busy"]
     pairs = pair_count(st, ntot)
     log(f"dense packs: {value:.3e} particle-steps/s, {pairs:.2f} contact pairs per particle, rebuilds {st['rebuilds']}")
     return {"value": value, "unit": "particle-steps/s", "instances": len(models), "spheres": ntot,
@@ -324,7 +329,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000, help="timed steps of the sweep window")
-    ap.add_argument("--warmup", type=int, default=0, help="untimed steps after the settled state")
+    ap.add_argument("--warmup", type=int, default=5, help="untimed steps after the settled state, in the timed "
+                    "steps' dem call (the clock starts at a pause after them)")
     ap.add_argument("--settle", type=int, default=3000,
                     help="steps run before the warmup to form the input state (the spheres fall into contact)")
     ap.add_argument("--instances", type=int, default=512)
@@ -394,35 +400,47 @@ def main():
     prec = pmx.PMX_MIXED if args.precision == "mixed" else pmx.PMX_FP64
     eng = pmx.Engine(precision=prec)
     eng.upload_batch(models)
-    done = 0
-    while done < start:
-        n = min(1000, start - done)
-        eng.dem_batch(exact_duration(n, dt), dt, sweep_forcings(pmx, workloads, insts, n, dt, done * dt))
-        done += n
-        log(f"{'settle' if done <= S0 else 'warmup'} {done}/{start} steps")
-    busy0 = eng.stats()["busy"] if start > 0 else 0
-    if not args.full_run:
-        eng.lib.pmx_set_profiling(eng.h, 1)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
     done, rebuilds, grown = 0, 0, 0
+    call_s = None
+    eng.lib.pmx_set_window.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    if not args.full_run:
+        # settling, warm-up and timed steps in one dem call, as a long run
+        # takes them (the device busy throughout, the call's own set-up at
+        # its start); the timed steps are measured on the device timeline
+        # (events before step `start` and after the last step) and only
+        # their kernels are profiled
+        eng.lib.pmx_set_profiling(eng.h, 1)
+        n = start + K
+        eng.lib.pmx_set_window(eng.h, start, start + K - 1)
+        fcs = sweep_forcings(pmx, workloads, insts, n, dt, 0.0)
+        log(f"one dem call of {n} steps, steps {start}-{start + K - 1} timed on the device")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        eng.dem_batch(exact_duration(n, dt), dt, fcs)
+        torch.cuda.synchronize()
+        call_s = time.perf_counter() - t0
+        st = eng.stats()
+        assert st["steps"] == n and st["window_ms"] > 0, (st, n)
+        rebuilds, grown, done = st["rebuilds"], st["grown"], K
+    else:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
     while done < K:
         # the full run in calls of 1000 steps (forcing tables of that length)
-        n = K if not args.full_run else min(1000, K - done)
+        n = min(1000, K - done)
         eng.dem_batch(exact_duration(n, dt), dt, sweep_forcings(pmx, workloads, insts, n, dt, (start + done) * dt))
         st = eng.stats()
         assert st["steps"] == n, (st, n)
         rebuilds += st["rebuilds"]
         grown += st["grown"]
         done += n
-        if args.full_run:
-            log(f"full run {done}/{K} steps, {time.perf_counter() - t0:.1f} s")
+        log(f"full run {done}/{K} steps, {time.perf_counter() - t0:.1f} s")
     torch.cuda.synchronize()
-    sec = time.perf_counter() - t0
+    sec = time.perf_counter() - t0 if args.full_run else st["window_ms"] / 1e3
     kernels = kernel_table(pmx, eng) if not args.full_run else {}
     eng.lib.pmx_set_profiling(eng.h, 0)
     value = ntot * K / sec
-    busy = 0.5 * (busy0 + st["busy"])
+    busy = st["busy"]
     pairs = pair_count(st, ntot)
     log(f"timed {K} steps in {sec:.3f} s: {value:.3e} particle-steps/s, pairs {st['pairs']}, rebuilds {rebuilds}")
 
@@ -436,7 +454,7 @@ def main():
     sample = list(range(0, args.instances, stride))[: args.plain_sample]
     plain, plain_ids = None, []
     if sample and not args.full_run:
-        plain, plain_ids = plain_fp64_sample(pmx, workloads, insts, sample, start, min(K, 1000), dt)
+        plain, plain_ids = plain_fp64_sample(pmx, workloads, insts, sample, S0, W, min(K, 1000), dt)
         log(f"plain fp64 (one instance at a time): {plain:.3e} particle-steps/s")
 
     dense = dense_leg(pmx, workloads, args, dt) if args.dense_steps > 0 and not args.full_run else None
@@ -475,6 +493,11 @@ def main():
         "steps": K,
         "warmup": W,
         "ms_per_step": 1e3 * sec / K,
+        "timing": ({"how": "device events around the timed steps inside one dem call (pmx_set_window)",
+                    "call_seconds": call_s, "call_steps": start + K,
+                    "call_includes": "settling, warm-up and timed steps, the call's tree builds and graph "
+                                     "capture, its host step loop"}
+                   if call_s is not None else {"how": "host clock around the calls of the full run"}),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -486,7 +509,8 @@ def main():
                         f"spheres each (seeded), base excitation 0.1-2 g at 1-20 Hz, dt = {dt}, batched, "
                         f"{args.precision}; " + ("the whole 10,000-step run from rest" if args.full_run else
                                                  f"input = the state after {S0} steps, timed steps "
-                                                 f"{start}-{start + K} of the 10,000-step run"),
+                                                 f"{start}-{start + K} of the 10,000-step run (one dem call of "
+                                                 f"steps 0-{start + K}; the timed steps on the device timeline)"),
             "instances": len(models),
             "spheres": ntot,
             "dt": dt,
@@ -501,7 +525,8 @@ def main():
                              "rows_threads_per_block": st["rows_block"]},
         },
         "plain_fp64": {"value": plain, "unit": "particle-steps/s", "instances_sampled": plain_ids,
-                       "how": "drop-in FP64 path, each sampled instance alone from rest, timed over the same window"},
+                       "how": "drop-in FP64 path, each sampled instance alone from rest, timed over the same window "
+                              "the same way (clock started after the warm-up steps of the timed call)"},
         "speedup_vs_plain_fp64": value / plain if plain else None,
         "step_roofline": step_roofline(value, pairs, prec == pmx.PMX_MIXED),
         "dense": dense,

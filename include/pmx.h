@@ -259,6 +259,7 @@ typedef struct pmx_stats
   long long grown;         /* steps of the last call repeated after a contact capacity grew */
   int row_capacity;        /* contacts per sphere row now (grows as contacts need) */
   long long busy;          /* spheres with a contact after the last step */
+  double window_ms;        /* device time of the steps pmx_set_window named in the last call, or -1 */
 } pmx_stats;
 
 /* per-kernel timing collected with HIP events on the context's stream */
@@ -343,6 +344,15 @@ int pmx_set_profiling (pmx_ctx *ctx, int on);
 /* Per-kernel totals since profiling was enabled; count receives the number
  * of entries (at most cap are written). */
 int pmx_get_profile (pmx_ctx *ctx, pmx_kernel_time *out, int cap, int *count);
+
+/* Measurement hook (no counterpart in the reference): time steps first..last
+ * (0-based) of the next dem call on the device timeline, from an event
+ * recorded on the context's stream before step `first` to one after step
+ * `last` (a step repeated after an abort records its event again, so the
+ * window holds the repeat and the rebuild before it), and, with profiling
+ * on, time only those steps' kernels.  The result is pmx_stats.window_ms;
+ * the window applies to that one call.  first < 0 clears it. */
+int pmx_set_window (pmx_ctx *ctx, int first, int last);
 
 /* Time histories of the resident model (single model contexts): the specs
  * are copied; pmx_upload clears them.  Particles of force and torque

@@ -439,6 +439,7 @@ void pmx_destroy (pmx_ctx *c)
   if (c->hring) (void) hipHostFree (c->hring);
   if (c->hkey) (void) hipHostFree (c->hkey);
   for (hipEvent_t e : c->rev) if (e) (void) hipEventDestroy (e);
+  for (hipEvent_t e : c->wev) if (e) (void) hipEventDestroy (e);
   prof_resolve (*c);
   for (hipEvent_t e : c->prof.pool) (void) hipEventDestroy (e);
   if (c->stream) (void) hipStreamDestroy (c->stream);
@@ -693,7 +694,7 @@ static int capture (pmx_ctx *c, double h, bool multi)
  * get fresh events first (a launch keeps the events it was given) */
 static int launch_step (pmx_ctx *c, int step)
 {
-  if (!c->pgexec || step % PMX_PROF_EVERY)
+  if (!c->pgexec || step % PMX_PROF_EVERY || !c->prof.in_window (step))
   {
     PMX_HIP (hipGraphLaunch (c->gexec, c->stream));
     return PMX_OK;
@@ -931,8 +932,9 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
     { c->hring = nullptr; return fail (PMX_ENOMEM, "out of pinned host memory"); }
     for (int k = 0; k < PMX_RING; k++) PMX_HIP (hipEventCreateWithFlags (&c->rev[k], hipEventDisableTiming));
   }
-  /* output points of a single model: the steps after which the call pauses */
-  const bool outs = I == 1 && fc && fc->noutstep > 0 && fc->outstep && fc->out_fn;
+  /* output points: the steps after which the call pauses (the first
+   * forcing's list; a batch pauses every instance at the same step) */
+  const bool outs = fc && fc->noutstep > 0 && fc->outstep && fc->out_fn;
   int oi = 0;
   int done = 0;
   long long grown = 0;
@@ -960,10 +962,14 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
       {
         int n = std::min (gsize, next - enq);
         int q = 0;
-        if (!eager && c->gexec_multi)
+        if (!eager && c->gexec_multi && c->win[0] < 0)
           for (; q + PMX_GRAPH_STEPS <= n; q += PMX_GRAPH_STEPS) PMX_HIP (hipGraphLaunch (c->gexec_multi, c->stream));
         for (; q < n; q++)
         {
+          /* the device-timed window (pmx_set_window): an event before its
+           * first step and after its last, recorded again when a step is
+           * repeated */
+          if (enq + q == c->win[0]) PMX_HIP (hipEventRecord (c->wev[0], c->stream));
           if (eager)
           {
             c->prof.step = enq + q;
@@ -973,6 +979,7 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
           else
             rc = launch_step (c, enq + q);
           if (rc) return rc;
+          if (enq + q == c->win[1]) PMX_HIP (hipEventRecord (c->wev[1], c->stream));
         }
         int *slot = c->hring + (gq % PMX_RING) * slotn;
         PMX_HIP (hipMemcpyAsync (slot, c->ctl.p, sizeof (int) * CTL_COUNT, hipMemcpyDeviceToHost, c->stream));
@@ -1021,7 +1028,10 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
             return fail (PMX_EREPART, "Repartitioning failed (instance " + std::to_string (i) + ", step " +
                          std::to_string (done) + ")");
         for (int i = 0; i < I; i++) if (ov[i]) rebuilt_at[i] = done;
-        if ((rc = rebuild (c, false))) return rc;
+        c->prof.step = done;   /* the rebuild is timed with the step it repeats */
+        rc = rebuild (c, false);
+        c->prof.step = -1;
+        if (rc) return rc;
         rebuilds++;
         if (c->tree.last_partial) partial++;
       }
@@ -1032,7 +1042,9 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
     {
       /* an output point (the last step's output happens after the loop):
        * forces formed where they were used, mesh geometry, then the caller */
-      if ((rc = step_materialize (*c, std::vector<int> (1, done - 1))) || (rc = step_mesh_shapes (*c, false))) return rc;
+      std::vector<int> fin (I, -1);
+      for (int i = 0; i < I; i++) if (!reported[i] && nst[i] >= done) fin[i] = done - 1;
+      if ((rc = step_materialize (*c, fin)) || (rc = step_mesh_shapes (*c, false))) return rc;
       PMX_HIP (hipStreamSynchronize (c->stream));
       fc->out_fn (fc->out_user, fc->outstep[oi]);
       (void) hipSetDevice (c->device);
@@ -1043,7 +1055,7 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
       return rc;
   }
   if ((rc = report (nsteps))) return rc;
-  if (outs)
+  if (outs && I == 1)
     for (; oi < fc->noutstep; oi++)
       if (fc->outstep[oi] == nsteps - 1)   /* the last step: the finished state */
       {
@@ -1070,23 +1082,22 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
   c->stats.grown = grown;
   c->stats.row_capacity = c->cap;
   c->stats.graph_timed = c->prof.mode == 1 && !c->prof.eager;
+  c->stats.window_ms = -1.0;
+  if (c->win[0] >= 0 && c->win[1] < nsteps)
+  {
+    float ms = 0.0f;
+    if (hipEventElapsedTime (&ms, c->wev[0], c->wev[1]) == hipSuccess) c->stats.window_ms = ms;
+  }
+  c->win[0] = c->win[1] = -1;
+  c->prof.win0 = c->prof.win1 = -1;
   c->stats.leaves = c->tree.nleaves;
   c->stats.leaf_slots = c->lstride;
   c->stats.seconds = std::chrono::duration<double> (std::chrono::steady_clock::now () - tl).count ();
   {
-    long long total = 0, busy = 0;
-    for (int i = 0; i < I; i++)
-    {
-      int e0 = c->off[i].ell, ne = c->off[i+1].ell - e0;
-      if (ne == 0) continue;
-      std::vector<int> cnt (ne);
-      PMX_HIP (hipMemcpyAsync (cnt.data (), c->rowcnt[c->icur[i]].p + e0, sizeof (int) * ne, hipMemcpyDeviceToHost,
-                               c->stream));
-      PMX_HIP (hipStreamSynchronize (c->stream));
-      for (int e = 0; e < ne; e++) { total += cnt[e]; busy += cnt[e] > 0; }
-    }
-    c->stats.pairs = total;  /* entries; sphere pairs are held twice */
-    c->stats.busy = busy;
+    long long tb[2] = {0, 0};
+    if ((rc = row_totals (*c, tb))) return rc;
+    c->stats.pairs = tb[0];  /* entries; sphere pairs are held twice */
+    c->stats.busy = tb[1];
   }
   if (elapsed) *elapsed = dt;
   return PMX_OK;
@@ -1306,9 +1317,18 @@ static int adaptive_core (pmx_ctx *c, double duration, double step, double adapt
   c->stats.grown = grown;
   c->stats.row_capacity = c->cap;
   c->stats.graph_timed = 0;
+  c->stats.window_ms = -1.0;
+  c->win[0] = c->win[1] = -1;
+  c->prof.win0 = c->prof.win1 = -1;
   c->stats.leaves = c->tree.nleaves;
   c->stats.leaf_slots = c->lstride;
   c->stats.seconds = dt;
+  {
+    long long tb[2] = {0, 0};
+    if ((rc = row_totals (*c, tb))) return rc;
+    c->stats.pairs = tb[0];
+    c->stats.busy = tb[1];
+  }
   if (elapsed) *elapsed = dt;
   return PMX_OK;
 }
@@ -1393,6 +1413,19 @@ int pmx_get_profile (pmx_ctx *c, pmx_kernel_time *out, int cap, int *count)
     out[k].ms = c->prof.ms[k];
     out[k].bytes = c->prof.bytes[k];
   }
+  return PMX_OK;
+}
+
+int pmx_set_window (pmx_ctx *c, int first, int last)
+{
+  if (!c) return fail (PMX_EINVAL, "null ctx");
+  if (first < 0) { c->win[0] = c->win[1] = -1; c->prof.win0 = c->prof.win1 = -1; return PMX_OK; }
+  if (last < first) return fail (PMX_EINVAL, "window: last < first");
+  (void) hipSetDevice (c->device);
+  for (hipEvent_t &e : c->wev)
+    if (!e) PMX_HIP (hipEventCreate (&e));
+  c->win[0] = c->prof.win0 = first;
+  c->win[1] = c->prof.win1 = last;
   return PMX_OK;
 }
 

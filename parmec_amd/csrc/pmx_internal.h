@@ -137,6 +137,8 @@ struct Prof
   bool capturing = false;      /* a step is being captured: pairs become event record nodes */
   hipError_t err = hipSuccess; /* a capture-time query failed */
   int step = -1;               /* step being enqueued (-1: launches outside the step loop) */
+  int win0 = -1, win1 = -1;    /* pmx_set_window: only these steps of the call are timed (-1: all) */
+  bool in_window (int s) const { return win0 < 0 || (s >= win0 && s <= win1); }
   struct Pend { int kid; hipEvent_t a, b; double bytes; int step; };
   /* a profiled kernel scope of the captured step: the chain's tail node
    * before and after it, and the event record nodes put around it */
@@ -264,6 +266,8 @@ struct pmx_ctx
   dvec<int> sp_pofs, sp_plist; /* particle -> its springs (spring << 1 | side) in spring order */
   int *hring = nullptr;        /* pinned: per group of steps in flight, the control block and the flags */
   unsigned long long *hkey = nullptr;   /* pinned: the adaptive minimum of the step read back */
+  int win[2] = {-1, -1};       /* pmx_set_window: steps timed on the device in the next dem call */
+  hipEvent_t wev[2] = {nullptr, nullptr};
   hipEvent_t rev[PMX_RING] = {};   /* recorded after each group's copy into hring */
   int det_nw = 4;              /* waves per k_detect block (autotuned) */
   int rows_tpb = 1024;         /* threads per k_rows_idle block (spheres per busy list) */
@@ -304,15 +308,17 @@ struct ProfScope
   double bytes;
   hipEvent_t a = nullptr;
   hipGraphNode_t before = nullptr;
+  bool on;
   ProfScope (pmx_ctx &c_, int kid_, double bytes_) : c (c_), kid (kid_), bytes (bytes_)
   {
-    if (!c.prof.mode) return;
+    on = c.prof.mode && (c.prof.capturing || c.prof.in_window (c.prof.step));
+    if (!on) return;
     if (c.prof.capturing) before = tail ();
     else { a = c.prof.get (); (void) hipEventRecord (a, c.stream); }
   }
   ~ProfScope ()
   {
-    if (!c.prof.mode) return;
+    if (!on) return;
     if (c.prof.capturing)
     {
       hipGraphNode_t after = tail ();
@@ -401,6 +407,7 @@ int step_obstacles (pmx_ctx &c, const double *tab, int row, double h);
 int step_prescribe_acc (pmx_ctx &c);           /* before dynamics */
 int step_prescribe_vel (pmx_ctx &c);           /* after dynamics */
 int step_materialize (pmx_ctx &c, const std::vector<int> &fin);   /* forces of finishing instances */
+int row_totals (pmx_ctx &c, long long out[2]);   /* row entries and busy spheres of the current rows */
 int step_shapes (pmx_ctx &c);
 int step_euler (pmx_ctx &c, double h);
 int step_invert (pmx_ctx &c);

@@ -44,6 +44,12 @@ __global__ void k_zero_i (int *a, int n)
 
 /* the step completed: advance the step counter (unless it was aborted) and
  * the sequence number of enqueued steps */
+__global__ void k_zero_u64 (unsigned long long *a, int n)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
+}
+
 __global__ void k_step_end (int *ctl)
 {
   if (threadIdx.x != 0) return;
@@ -1692,6 +1698,52 @@ int step_dynamics (pmx_ctx &c, double step0, double step1)
   if (c.niso < c.N) hipLaunchKernelGGL (k_dynamics<false>, grid (c.N), dim3 (PMX_TPB), 0, c.stream, DYN_ARGS);
 #undef DYN_ARGS
   PMX_HIP (hipGetLastError ());
+  return PMX_OK;
+}
+
+/* entries of every sphere's current row and spheres with a non-empty row,
+ * each instance read from the row buffer holding its rows (icur) */
+__global__ void k_row_totals (int E, const int *ell_inst, const int *icur, const int *cnt0, const int *cnt1,
+                              unsigned long long *out)
+{
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long n = 0, b = 0;
+  if (e < E)
+  {
+    int buf = icur[ell_inst ? ell_inst[e] : 0];
+    int k = buf ? cnt1[e] : cnt0[e];
+    n = (unsigned long long) k;
+    b = k > 0;
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1)
+  {
+    n += __shfl_xor (n, off, 64);
+    b += __shfl_xor (b, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0 && (n || b))
+  {
+    atomicAdd (&out[0], n);
+    atomicAdd (&out[1], b);
+  }
+}
+
+int row_totals (pmx_ctx &c, long long out[2])
+{
+  out[0] = out[1] = 0;
+  if (c.E == 0) return PMX_OK;
+  if (c.fin.alloc (std::max (c.I, 4)) || c.hmin.alloc (2)) { pmx_set_error ("out of device memory"); return PMX_ENOMEM; }
+  PMX_HIP (hipMemcpyAsync (c.fin.p, c.icur.data (), sizeof (int) * c.I, hipMemcpyHostToDevice, c.stream));
+  hipLaunchKernelGGL (k_zero_u64, dim3 (1), dim3 (64), 0, c.stream, c.hmin.p, 2);
+  hipLaunchKernelGGL (k_row_totals, grid (c.E), dim3 (PMX_TPB), 0, c.stream, c.E,
+                      c.I > 1 ? (const int*) c.ell_inst.p : (const int*) nullptr, (const int*) c.fin.p,
+                      (const int*) c.rowcnt[0].p, (const int*) c.rowcnt[1].p, c.hmin.p);
+  PMX_HIP (hipGetLastError ());
+  unsigned long long h[2];
+  PMX_HIP (hipMemcpyAsync (h, c.hmin.p, sizeof (h), hipMemcpyDeviceToHost, c.stream));
+  PMX_HIP (hipStreamSynchronize (c.stream));
+  out[0] = (long long) h[0];
+  out[1] = (long long) h[1];
   return PMX_OK;
 }
 

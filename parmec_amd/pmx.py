@@ -108,7 +108,7 @@ class pmx_stats(C.Structure):
                 ("max_row", C.c_int), ("seconds", C.c_double), ("det_waves", C.c_int), ("rows_block", C.c_int),
                 ("leaf_slots", C.c_int), ("partial_rebuilds", C.c_longlong),
                 ("graph_timed", C.c_int), ("grown", C.c_longlong), ("row_capacity", C.c_int),
-                ("busy", C.c_longlong)]
+                ("busy", C.c_longlong), ("window_ms", C.c_double)]
 
 
 class pmx_kernel_time(C.Structure):

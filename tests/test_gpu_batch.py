@@ -156,3 +156,53 @@ def test_coincident_overflow_batch():
     assert st["rebuilds"] == alone[0][2]["rebuilds"]
     for i in range(3):
         compare(ms[i], cs[i], alone[i][0], alone[i][1])
+
+
+def test_batch_pause_changes_nothing():
+    """A batch paused after a step (the first forcing's output point, where a
+    caller may read the state or start a clock) continues bitwise as without
+    the pause; the callback sees the step once."""
+    step, nsteps = 1e-5, 120
+    print(f"case batch pause, 4 instances x {nsteps} steps", flush=True)
+    models, forcings = _sweep_models(4, 300, 2000, nsteps, step)
+    duration = (nsteps - 0.5) * step
+    ms, cs, _ = _batch(models, duration, step, forcings)
+    seen = []
+    m0, a0, f0 = workloads.sweep_instance(0, 300, 2000)
+    paused = list(forcings)
+    paused[0] = pmx.forcing(nsteps, gtab=workloads.sweep_forcing(a0, f0, nsteps, step), outsteps=[59],
+                            on_output=seen.append)
+    mp, cp, _ = _batch(models, duration, step, paused)
+    assert seen == [59]
+    for i in range(len(models)):
+        compare(mp[i], cp[i], ms[i], cs[i])
+
+
+def test_device_window_changes_nothing():
+    """pmx_set_window only records events: the batch steps bitwise as
+    without it, and the window's device time is positive and below the
+    call's wall time."""
+    import ctypes as C
+    import time
+    step, nsteps = 1e-5, 80
+    print(f"case device window, 4 instances x {nsteps} steps", flush=True)
+    models, forcings = _sweep_models(4, 300, 2000, nsteps, step)
+    duration = (nsteps - 0.5) * step
+    ms, cs, _ = _batch(models, duration, step, forcings)
+    eng = pmx.Engine()
+    mw = [m.copy() for m in models]
+    eng.upload_batch(mw)
+    eng.lib.pmx_set_window.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    eng.lib.pmx_set_window(eng.h, 30, 69)
+    t0 = time.perf_counter()
+    eng.dem_batch(duration, step, forcings)
+    wall = time.perf_counter() - t0
+    st = eng.stats()
+    eng.download_batch(mw)
+    cw = [eng.contacts(i) for i in range(len(mw))]
+    eng.dem_batch(duration, step, forcings)
+    assert eng.stats()["window_ms"] == -1.0      # one call only
+    eng.close()
+    assert 0.0 < st["window_ms"] < 1e3 * wall
+    for i in range(len(models)):
+        compare(mw[i], cw[i], ms[i], cs[i])
