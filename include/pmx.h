@@ -386,7 +386,7 @@ int pmx_history_sample_noalloc (pmx_ctx *ctx, double *out);
 void pmx_destroy (pmx_ctx *ctx);
 
 /* critical time step estimates, CRITICAL (input.cpp:3034-3154): per particle
- * (critical_perparticle, parmec.ispc:537-888; the contacts of the last step
+ * (critical_perparticle, parmec.ispc:547-888; the contacts of the last step
  * taken and the springs) or per spring (critical_perspring,
  * parmec.ispc:485-535); h, o, r receive N or sprnum values: the step
  * estimate, the circular frequency and the damping ratio (o, r may be

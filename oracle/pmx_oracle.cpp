@@ -1628,14 +1628,14 @@ double max_slope (const double *x, const double *y, int lo, int hi)
   return m;
 }
 
-/* critical_perparticle, parmec.ispc:537-888: the contacts of the last step
+/* critical_perparticle, parmec.ispc:547-888: the contacts of the last step
  * (in the canonical order of the force sums), then the springs in spring
  * order, then time_step_task */
 void critical_perparticle (orc_ctx &s, double *hcri, double *ocri, double *rcri)
 {
   size_t N = s.parnum, S = s.sprnum;
   std::vector<double> adp (11 * N, 0.0);
-  for (size_t i = 0; i < N; i++)   /* critical_perparticle_contacts, parmec.ispc:537-655 */
+  for (size_t i = 0; i < N; i++)   /* critical_perparticle_contacts, parmec.ispc:547-656 */
   {
     struct E { unsigned long long k0, k1; const double *p; double f[3]; const double *n; double kc, ec; };
     std::vector<E> es;
@@ -1681,7 +1681,7 @@ void critical_perparticle (orc_ctx &s, double *hcri, double *ocri, double *rcri)
     double v[11] = {kact0[0], kact0[1], kact0[2], kmax0, emax0, krot0[0], krot0[1], krot0[2], krot0[3], krot0[4], krot0[5]};
     for (int c = 0; c < 11; c++) adp[c*N + i] = v[c];
   }
-  /* critical_perparticle_springs, parmec.ispc:658-843, one spring after
+  /* critical_perparticle_springs, parmec.ispc:659-857, one spring after
    * another.  dashidx and sprflg are read at particle indices there (i, k);
    * where such an index lies beyond the spring arrays the spring's own
    * entry (dashidx) or no flag (sprflg) is taken */

@@ -74,7 +74,7 @@ __global__ void k_time_step (int N, const double *adp, const double *mass, const
     atomicMin (hmin, (unsigned long long) __double_as_longlong (v));
 }
 
-/* critical_perparticle_contacts (parmec.ispc:537-655) from the contacts the
+/* critical_perparticle_contacts (parmec.ispc:547-656) from the contacts the
  * last step recorded: per sphere its row entries in row order, arms taken
  * from the particle's current position and rotated by its current L */
 __global__ void k_crit_contacts (int E, int N, const int *part, const int *cnt, int cap, const double *crec,

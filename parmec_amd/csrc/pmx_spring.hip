@@ -476,12 +476,12 @@ __device__ __forceinline__ double max_slope (const double *x, const double *y, i
 }
 
 /* per spring, the bounds CRITICAL's per-particle estimate adds to both of
- * its particles (critical_perparticle_springs, parmec.ispc:741-775): kspr,
+ * its particles (critical_perparticle_springs, parmec.ispc:717-750): kspr,
  * the largest |slope| of the spring curve, and espr, 2 ratio sqrt(kspr m)
  * when the dashpot entry is a damping ratio, else the largest |slope| of
  * the spring's dashpot curve.  The reference tests for the ratio form, and
  * takes the ratio, at dashidx[i] with i the spring's first PARTICLE index
- * (parmec.ispc:758, 770); that is reproduced where i < S, and the spring's
+ * (parmec.ispc:728, 742); that is reproduced where i < S, and the spring's
  * own entry is used where dashidx[i+1] would lie beyond the array */
 __global__ void k_crit_spring_terms (SprArgs A, double *out)
 {
@@ -516,7 +516,7 @@ __global__ void k_crit_spring_terms (SprArgs A, double *out)
  * zero direction adds 1 to each component; a planar one adds |t1| + |t2|
  * of two unit vectors normal to it), the rotational stiffness terms of the
  * arm L^T (point - x), and the larger of the current and the spring's
- * kspr, espr (parmec.ispc:777-888).  The planar test reads sprflg at the
+ * kspr, espr (parmec.ispc:753-857).  The planar test reads sprflg at the
  * PARTICLE index, as the reference does; a particle index >= S reads no
  * flag */
 __global__ void k_crit_springs (SprArgs A, const int *pofs, const int *plist, const double *terms, double *adp)
