@@ -971,9 +971,12 @@ int forces (orc_ctx &s, double step, const double g[3])
     for (const Contact &con : s.master[i])
       es.push_back ({side_key (con.slave1), side_key (con.master), con.point, {con.force[0], con.force[1], con.force[2]},
                      con.normal, con.kcur, con.ecur});
-    for (const Slave &sv : s.slave[i])   /* slave kcur, ecur: never written by the reference, 0 here */
-      es.push_back ({side_key (sv.master1), side_key (sv.slave1), sv.point, {sv.force[0], sv.force[1], sv.force[2]},
-                     nullptr, 0.0, 0.0});
+    /* slave kcur, ecur: never written by the reference, 0 here; the reactions
+     * on a meshed particle's triangles go triangle by triangle, spheres in
+     * index order within a triangle */
+    for (const Slave &sv : s.slave[i])
+      es.push_back ({side_key (sv.slave1 < 0 ? sv.slave1 : sv.master1), side_key (sv.slave1 < 0 ? sv.master1 : sv.slave1),
+                     sv.point, {sv.force[0], sv.force[1], sv.force[2]}, nullptr, 0.0, 0.0});
     std::sort (es.begin (), es.end (), [] (const E &a, const E &b) { return a.k0 < b.k0 || (a.k0 == b.k0 && a.k1 < b.k1); });
     double po[3] = {s.pos(0)[i], s.pos(1)[i], s.pos(2)[i]};
     double fs[3] = {0.0, 0.0, 0.0}, ts[3] = {0.0, 0.0, 0.0};
@@ -1643,8 +1646,8 @@ void critical_perparticle (orc_ctx &s, double *hcri, double *ocri, double *rcri)
       es.push_back ({side_key (con.slave1), side_key (con.master), con.point, {con.force[0], con.force[1], con.force[2]},
                      con.normal, con.kcur, con.ecur});
     for (const Slave &sv : s.slave[i])
-      es.push_back ({side_key (sv.master1), side_key (sv.slave1), sv.point, {sv.force[0], sv.force[1], sv.force[2]},
-                     nullptr, 0.0, 0.0});
+      es.push_back ({side_key (sv.slave1 < 0 ? sv.slave1 : sv.master1), side_key (sv.slave1 < 0 ? sv.master1 : sv.slave1),
+                     sv.point, {sv.force[0], sv.force[1], sv.force[2]}, nullptr, 0.0, 0.0});
     std::sort (es.begin (), es.end (), [] (const E &a, const E &b) { return a.k0 < b.k0 || (a.k0 == b.k0 && a.k1 < b.k1); });
     double L[9], po[3];
     for (int c = 0; c < 9; c++) L[c] = s.rotation[c*N + i];

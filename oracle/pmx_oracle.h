@@ -13,7 +13,8 @@
  *     move it on compaction);
  *   - per-particle force/torque sums run sequentially over the particle's
  *     contacts ordered by the partner's identity (forces.ispc:492-577 sums in
- *     lane/task-dependent order).
+ *     lane/task-dependent order); a meshed particle sums the reactions on its
+ *     triangles triangle by triangle, spheres in index order within one.
  */
 #ifndef PMX_ORACLE_H
 #define PMX_ORACLE_H

@@ -115,8 +115,6 @@ int validate_model (const pmx_model *m)
   {
     if (m->triobs[t] <= -2 && -m->triobs[t]-2 >= m->obsnum) return fail (PMX_EINVAL, "triangle obstacle index out of range");
     if (m->triobs[t] >= m->parnum) return fail (PMX_EINVAL, "triangle particle index out of range");
-    if (m->triobs[t] >= 0 && m->ellnum > 0)
-      return fail (PMX_EUNSUPPORTED, "contact between spheres and meshed particles is not supported yet");
   }
   for (int q = 0; q < m->nodnum; q++)
     if (m->nodpart[q] < 0 || m->nodpart[q] >= m->parnum) return fail (PMX_EINVAL, "node particle index out of range");
@@ -279,19 +277,48 @@ int upload_core (pmx_ctx *c, const pmx_model *m, const std::vector<InstOff> &off
         (rc = up (*c, c->prs_av, av.data (), n, "prescribed")))
       return rc;
   }
-  /* moving obstacles */
+  /* moving obstacles; and, where spheres can touch them, the triangles of
+   * meshed particles, which move every step as well: both reach the leaves
+   * through per-step lists (obstacle index -1 for a meshed particle's
+   * triangle, moved by its particle's faces instead) */
   {
     c->O = m->obsnum;
     std::vector<int> mtri, mtobs, tmov (std::max (m->trinum, 1), 0);
     for (int i = 0; i < m->obsnum; i++)
       for (int t = m->trirng[2*i]; t < m->trirng[2*i+1]; t++) { mtri.push_back (t); mtobs.push_back (i); tmov[t] = 1; }
+    std::vector<int> mp_part, mp_toff (1, 0), mp_tlist;
+    if (m->ellnum > 0)
+    {
+      std::vector<std::vector<int>> bypart (std::max (m->parnum, 1));
+      for (int t = 0; t < m->trinum; t++)
+        if (m->triobs[t] >= 0) { mtri.push_back (t); mtobs.push_back (-1); tmov[t] = 1; bypart[m->triobs[t]].push_back (t); }
+      for (int p = 0; p < m->parnum; p++)
+        if (!bypart[p].empty ())
+        {
+          mp_part.push_back (p);
+          mp_tlist.insert (mp_tlist.end (), bypart[p].begin (), bypart[p].end ());
+          mp_toff.push_back ((int) mp_tlist.size ());
+        }
+    }
     c->nmtri = (int) mtri.size ();
+    c->nmp = (int) mp_part.size ();
+    c->mesh_contact = c->nmp > 0;
     size_t O = c->O;
     if ((rc = up (*c, c->obspnt, m->obspnt, 3*O, "obspnt")) || (rc = up (*c, c->obsang, m->obsang, 3*O, "obsang")) ||
         (rc = up (*c, c->obslin, m->obslin, 3*O, "obslin")) || (rc = up (*c, c->mtri, mtri.data (), mtri.size (), "obstacles")) ||
         (rc = up (*c, c->mtobs, mtobs.data (), mtobs.size (), "obstacles")) ||
-        (rc = up (*c, c->tmov, tmov.data (), T, "obstacles")))
+        (rc = up (*c, c->tmov, tmov.data (), T, "obstacles")) ||
+        (rc = up (*c, c->mp_part, mp_part.data (), mp_part.size (), "meshed particles")) ||
+        (rc = up (*c, c->mp_toff, mp_toff.data (), mp_toff.size (), "meshed particles")) ||
+        (rc = up (*c, c->mp_tlist, mp_tlist.data (), mp_tlist.size (), "meshed particles")))
       return rc;
+    if (c->mesh_contact)
+    {
+      if (c->tcnt.alloc (std::max (T, (size_t) 1)) || c->trec_e.alloc ((size_t) c->tcap * T) ||
+          c->trec.alloc (6 * (size_t) c->tcap * T))
+        return fail (PMX_ENOMEM, "out of device memory (mesh reactions)");
+      PMX_HIP (hipMemsetAsync (c->tcnt.p, 0, sizeof (int) * T, c->stream));
+    }
   }
   if ((rc = up (*c, c->nodes, m->nodes, 6*(size_t) c->NOD, "nodes")) ||
       (rc = up (*c, c->nodpart, m->nodpart, (size_t) c->NOD, "nodpart")) ||
@@ -531,7 +558,9 @@ static int grow_capacity (pmx_ctx *c, const int *fl)
   if (fl[FLAG_HIT_OVERFLOW]) hcap2 = 2 * c->hcap;
   if (fl[FLAG_EXTRA_OVERFLOW]) xcap2 = 2 * c->xcap;
   if (fl[FLAG_MTRI_OVERFLOW]) mcap2 = 2 * c->mcap;
-  if (cap2 > PMX_CAP_LIMIT || hcap2 > PMX_CAP_LIMIT || xcap2 > PMX_CAP_LIMIT || mcap2 > PMX_CAP_LIMIT)
+  int tcap2 = fl[FLAG_TREV_OVERFLOW] ? 2 * c->tcap : c->tcap;
+  if (cap2 > PMX_CAP_LIMIT || hcap2 > PMX_CAP_LIMIT || xcap2 > PMX_CAP_LIMIT || mcap2 > PMX_CAP_LIMIT ||
+      tcap2 > PMX_CAP_LIMIT)
     return fail (PMX_ECAPACITY, "contact capacity beyond " + std::to_string (PMX_CAP_LIMIT) + " per sphere (row " +
                  std::to_string (fl[FLAG_MAXROW]) + ")");
   if (c->opt.verbose)
@@ -562,6 +591,14 @@ static int grow_capacity (pmx_ctx *c, const int *fl)
     c->cap = cap2;
     if (c->opt.critical && c->crec.alloc (8 * (size_t) cap2 * E))   /* rewritten by the repeated step */
       return fail (PMX_ENOMEM, "out of device memory (contact records)");
+  }
+  if (tcap2 != c->tcap)   /* a meshed particle's triangle with more contacts than slots: slots of the
+                             repeated step */
+  {
+    size_t T = std::max (c->T, 1);
+    if (c->trec_e.alloc ((size_t) tcap2 * T) || c->trec.alloc (6 * (size_t) tcap2 * T))
+      return fail (PMX_ENOMEM, "out of device memory (mesh reactions)");
+    c->tcap = tcap2;
   }
   if (hcap2 != c->hcap)
   {
@@ -774,7 +811,7 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
   step1 = step0;
   if ((rc = step_invert (*c))) return rc;
 
-  /* steps of each instance (the reference's accumulation, Q13) */
+  /* steps of each instance (the reference's accumulation of time by (step0+step1)/2, parmec.cpp:2418) */
   std::vector<int> nst (I, 0);
   int nsteps = 0;
   for (int i = 0; i < I; i++)
@@ -1013,7 +1050,8 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
        * failed" when an instance already re-partitioned for this step
        * overflows again, as it would alone (every instance that overflows
        * in a step is flagged by that step's binning) */
-      if (fl[FLAG_ROW_OVERFLOW] || fl[FLAG_HIT_OVERFLOW] || fl[FLAG_EXTRA_OVERFLOW] || fl[FLAG_MTRI_OVERFLOW])
+      if (fl[FLAG_ROW_OVERFLOW] || fl[FLAG_HIT_OVERFLOW] || fl[FLAG_EXTRA_OVERFLOW] || fl[FLAG_MTRI_OVERFLOW] ||
+          fl[FLAG_TREV_OVERFLOW])
       {
         if ((rc = grow_capacity (c, fl))) return rc;
         grown++;
@@ -1258,7 +1296,8 @@ static int adaptive_core (pmx_ctx *c, double duration, double step, double adapt
        * its sequence number moves on, then larger buffers or a new tree,
        * and the step again */
       if ((rc = step_end (*c))) return rc;
-      if (fl[FLAG_ROW_OVERFLOW] || fl[FLAG_HIT_OVERFLOW] || fl[FLAG_EXTRA_OVERFLOW] || fl[FLAG_MTRI_OVERFLOW])
+      if (fl[FLAG_ROW_OVERFLOW] || fl[FLAG_HIT_OVERFLOW] || fl[FLAG_EXTRA_OVERFLOW] || fl[FLAG_MTRI_OVERFLOW] ||
+          fl[FLAG_TREV_OVERFLOW])
       {
         if ((rc = grow_capacity (c, fl))) return rc;
         grown++;

@@ -170,7 +170,7 @@ int pmx_critical (pmx_ctx *c, int which, double *h, double *o, double *r)
     size_t m = 11 * (size_t) c->N;
     hipLaunchKernelGGL (k_adp_zero, dim3 ((m + PMX_TPB - 1) / PMX_TPB), dim3 (PMX_TPB), 0, c->stream, m, c->adp.p,
                         (unsigned long long*) nullptr, (const int*) nullptr);
-    if (c->crec_valid && (rc = crit_contacts (*c))) return rc;
+    if (c->crec_valid && ((rc = crit_contacts (*c)) || (rc = step_react (*c, true)))) return rc;
     if ((rc = crit_springs (*c))) return rc;
     hipLaunchKernelGGL (k_time_step, grid (c->N), dim3 (PMX_TPB), 0, c->stream, c->N, (const double*) c->adp.p,
                         (const double*) c->mass.p, (const double*) c->inertia.p, dh, dov, dr,

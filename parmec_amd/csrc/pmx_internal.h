@@ -39,7 +39,7 @@ enum { CTL_ABORT = 0, CTL_KSTEP, CTL_HINT, CTL_SEQ, CTL_COUNT = 8 };
  * or grows the capacity and repeats it); a traversal stack overflow in a
  * tree build (FLAG_TRILIST_OVERFLOW) is an error */
 enum { FLAG_LEAF_OVERFLOW = 0, FLAG_ROW_OVERFLOW, FLAG_HIT_OVERFLOW, FLAG_EXTRA_OVERFLOW,
-       FLAG_TRILIST_OVERFLOW, FLAG_MAXROW, FLAG_BAD_KIND, FLAG_MTRI_OVERFLOW, FLAG_COUNT };
+       FLAG_TRILIST_OVERFLOW, FLAG_MAXROW, FLAG_BAD_KIND, FLAG_MTRI_OVERFLOW, FLAG_TREV_OVERFLOW, FLAG_COUNT };
 
 /* abort the current step: the step is repeated after the host has grown the
  * capacity that overflowed (or re-partitioned), from unchanged state */
@@ -258,6 +258,14 @@ struct pmx_ctx
    * [17][S]; with opt.critical, the contact records of the last step taken
    * (point[3], |direction|[3], kcur, ecur as [8][cap][E]); the estimates
    * h, omega, ratio [3][max(N, S)]; the adaptive minimum as an ordered key */
+  /* spheres in contact with triangles of meshed particles: each contact's
+   * reaction (the sphere, -f, the point) in its triangle's slots
+   * (trec_e [tcap][T], trec [6][tcap][T], tcnt [T]), summed per meshed
+   * particle over its triangles (mp_toff/mp_tlist, ascending) by k_react */
+  bool mesh_contact = false;
+  int tcap = 8, nmp = 0;
+  dvec<int> tcnt, trec_e, mp_part, mp_toff, mp_tlist;
+  dvec<double> trec;
   bool adp_on = false;         /* the step forms the adaptive sums (k_rows<R, true>) */
   bool crec_valid = false;     /* crec holds the contacts of the last step taken */
   bool crec_stale = false;     /* steps were taken since the upload without recording their contacts */
@@ -419,6 +427,7 @@ int step_springs (pmx_ctx &c, double h);
 int crit_springs (pmx_ctx &c);                 /* CRITICAL's spring part of the per-particle sums */
 int crit_perspring (pmx_ctx &c, double *h, double *o, double *r);
 int crit_contacts (pmx_ctx &c);                /* CRITICAL's contact part (from the recorded contacts) */
+int step_react (pmx_ctx &c, bool critical);    /* meshed particles' contact reactions (forces, or CRITICAL's sums) */
 int adp_begin (pmx_ctx &c);                    /* zero the per-particle sums; reset the adaptive minimum */
 int adp_time_step (pmx_ctx &c, double *h, double *o, double *r);   /* per particle; minimum when h is null */
 int step_mesh_shapes (pmx_ctx &c, bool gated = true);

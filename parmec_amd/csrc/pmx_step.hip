@@ -723,6 +723,98 @@ __global__ void k_force_init (const int *ctl, int N, const int *list, int nlist,
 }
 
 
+/* the reactions on meshed particles: the slave copies of the contacts of
+ * spheres with their triangles (forces.ispc:383-405), summed as
+ * contacts_acc_task sums slave entries (453-596).  Per particle, its
+ * triangles in ascending number and each triangle's contacts by ascending
+ * sphere (the reference's order follows its task interleaving); force =
+ * sum + m g, torque = sum of (point - x) x f.  With adp, the adaptive sums of
+ * the same entries (unit force directions, kcur = ecur = 0 on the slave
+ * side, arms L (point - x)); crit: CRITICAL's contact part only (sums from
+ * the contacts the last step left in the slots, current x and L) */
+__global__ void k_react (int nmp, const int *mp_part, const int *mp_toff, const int *mp_tlist, int T,
+                         const int *tcnt, const int *trec_e, const double *trec, int tcap, int N,
+                         const double *position, const double *rotation, const double *mass, const double *gtab,
+                         int ninst, const int *par_inst, double *force, double *torque, double *adp,
+                         const int *ctl, const int *stop, int crit)
+{
+  if (ctl && ctl[CTL_ABORT]) return;
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nmp) return;
+  int p = mp_part[q];
+  if (!crit && frozen (stop, par_inst[p], ctl)) return;
+  size_t NN = N, TT = T;
+  double x[3] = {position[p], position[NN + p], position[2*NN + p]}, L[9];
+  if (adp)
+    for (int c = 0; c < 9; c++) L[c] = rotation[c*NN + p];
+  double fs[3] = {0.0, 0.0, 0.0}, ts[3] = {0.0, 0.0, 0.0}, ka[3] = {0.0, 0.0, 0.0};
+  double kr[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int k = mp_toff[q]; k < mp_toff[q+1]; k++)
+  {
+    int t = mp_tlist[k];
+    int n = tcnt[t];
+    n = n < tcap ? n : tcap;
+    int last = -1;
+    for (int r = 0; r < n; r++)
+    {
+      int best = 0, be = 0x7FFFFFFF;   /* the next sphere in ascending order */
+      for (int sl = 0; sl < n; sl++)
+      {
+        int es = trec_e[(size_t) sl * TT + t];
+        if (es > last && es < be) { be = es; best = sl; }
+      }
+      last = be;
+      double f[3], a[3];
+      for (int c = 0; c < 3; c++)
+      {
+        f[c] = trec[(size_t) (c*tcap + best) * TT + t];
+        a[c] = trec[(size_t) ((3+c)*tcap + best) * TT + t] - x[c];
+      }
+      fs[0] += f[0]; fs[1] += f[1]; fs[2] += f[2];
+      ts[0] += a[1]*f[2] - a[2]*f[1];
+      ts[1] += a[2]*f[0] - a[0]*f[2];
+      ts[2] += a[0]*f[1] - a[1]*f[0];
+      if (adp)
+      {
+        double d[3] = {f[0], f[1], f[2]};
+        double len = sqrt (d[0]*d[0] + d[1]*d[1] + d[2]*d[2]);
+        if (len != 0.0)
+        {
+          len = 1.0/len;
+          d[0] *= len; d[1] *= len; d[2] *= len;
+        }
+        else d[0] = d[1] = d[2] = 0.0;
+        for (int c = 0; c < 3; c++) ka[c] += fabs (d[c]);
+        double A[3];
+        nvmul (L, a, A);
+        double dot = A[0]*A[0] + A[1]*A[1] + A[2]*A[2];
+        kr[0] += dot - A[0]*A[0];
+        kr[1] += dot - A[1]*A[1];
+        kr[2] += dot - A[2]*A[2];
+        kr[3] += -A[0]*A[1];
+        kr[4] += -A[0]*A[2];
+        kr[5] += -A[1]*A[2];
+      }
+    }
+  }
+  if (!crit)
+  {
+    const double *g = gtab + 3*((size_t) ctl[CTL_KSTEP]*ninst + (ninst > 1 ? par_inst[p] : 0));
+    double ma = mass[p];
+    force[p] = fs[0] + ma * g[0];
+    force[NN + p] = fs[1] + ma * g[1];
+    force[2*NN + p] = fs[2] + ma * g[2];
+    torque[p] = ts[0];
+    torque[NN + p] = ts[1];
+    torque[2*NN + p] = ts[2];
+  }
+  if (adp)
+  {
+    double v[11] = {ka[0], ka[1], ka[2], 0.0, 0.0, kr[0], kr[1], kr[2], kr[3], kr[4], kr[5]};
+    for (int c = 0; c < 11; c++) adp[c*NN + p] = v[c];
+  }
+}
+
 #define ROWS_SORT 8   /* hit keys a thread sorts in LDS (more: sorted in the hit array) */
 
 /* The rows of a step in two launches.  k_rows_idle (TPB threads per block,
@@ -781,16 +873,19 @@ void k_rows_idle (int E, int N, const int *part, const double *mass, int ninst, 
  * persists stays FP64, the tangential state is stored in FP32 and the
  * per-sphere sums are Kahan-compensated FP32).  Block j takes slice
  * j % slices of list j / slices (lists of LIST entries at most). */
-/* ADP: the instantiation of adaptive and critical-step runs, which also
- * forms per particle the sums the reference's contacts_acc_task forms when
- * adaptive stepping is on (forces.ispc:505-533, 542-577): the summed
- * |direction| (the normal on the master side, the unit force on the slave
- * side), the largest contact stiffness and damping coefficient (kn and
- * en = 2 damper sqrt(kn m) of the master side; the reference never writes
- * them on the slave side, taken as 0 here) and the rotational stiffness
- * sums of the arms L (point - x), into adp [11][N] (when not null), and
- * stores each row entry's point, |direction|, kn and en into crec
- * [8][cap][E] (when not null) for the critical step estimate */
+/* ADP: the instantiation of adaptive, critical-step and sphere-on-mesh
+ * runs, which also forms per particle the sums the reference's
+ * contacts_acc_task forms when adaptive stepping is on (forces.ispc:505-533,
+ * 542-577): the summed |direction| (the normal on the master side, the unit
+ * force on the slave side), the largest contact stiffness and damping
+ * coefficient (kn and en = 2 damper sqrt(kn m) of the master side; the
+ * reference never writes them on the slave side, taken as 0 here) and the
+ * rotational stiffness sums of the arms L (point - x), into adp [11][N]
+ * (when not null); stores each row entry's point, |direction|, kn and en
+ * into crec [8][cap][E] (when not null) for the critical step estimate; and
+ * puts the reaction of each contact with a meshed particle's triangle (the
+ * slave copy of forces.ispc:383-405) into that triangle's slots (when tcnt
+ * is not null) */
 template <class R, bool ADP = false>
 __global__ __launch_bounds__(64)
 void k_rows (int E, int N, const double *center, const double *radii, const int *part, const int *ellcol,
@@ -803,7 +898,7 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
              const double *gtab, int *ctl, double h, double *force, double *torque, int *flags,
              const double *obspnt, const double *obsang, const double *obslin,
              const int *busy, const int *nbusy, int list, int slices,
-             const double *rotation, double *adp, double *crec)
+             const double *rotation, double *adp, double *crec, int *tcnt, int *trec_e, double *trec, int tcap)
 {
   constexpr int TPB = 64;
   __shared__ uint32_t sh[ROWS_SORT * TPB];
@@ -881,10 +976,11 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
     double point[3], depth;
     R normal[3], st[3] = {0.0, 0.0, 0.0};
     bool master;
-    int i, j, col0, col1;
+    int i, j, col0, col1, tk = -1;
     if (key & PMX_TRI_FLAG)
     {
       int t = (int) (key & ~PMX_TRI_FLAG);
+      tk = t;
       double nd[3];   /* closest-point regions and depth in FP64: they decide the pair */
       depth = triangle_sphere (tri[t], tri[(size_t) T + t], tri[2*(size_t) T + t],
                                tri[3*(size_t) T + t], tri[4*(size_t) T + t], tri[5*(size_t) T + t],
@@ -1042,6 +1138,19 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
     else { atomicAdd (&flags[FLAG_ROW_OVERFLOW], 1); abort_step (ctl); }
     cnt++;
 
+    if constexpr (ADP)
+      if (tcnt && tk >= 0 && j >= 0)
+      {
+        /* a meshed particle's triangle: its reaction -f at the point */
+        int sl = atomicAdd (&tcnt[tk], 1);
+        if (sl < tcap)
+        {
+          trec_e[(size_t) sl * T + tk] = e;
+          double rv[6] = {-(double) f[0], -(double) f[1], -(double) f[2], point[0], point[1], point[2]};
+          for (int c = 0; c < 6; c++) trec[(size_t) (c*tcap + sl) * T + tk] = rv[c];
+        }
+        else { atomicAdd (&flags[FLAG_TREV_OVERFLOW], 1); abort_step (ctl); }
+      }
     /* contacts_acc_task, forces.ispc:492-533: slaves receive -f */
     if (!master) { f[0] = -f[0]; f[1] = -f[1]; f[2] = -f[2]; }
     if constexpr (ADP)
@@ -1382,7 +1491,7 @@ __global__ void k_obstacles (int nm, const int *mtri, const int *mtobs, int T, d
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nm) return;
   int t = mtri[q], i = mtobs[q];
-  if (frozen (stop, tri_inst[t], ctl)) return;
+  if (i < 0 || frozen (stop, tri_inst[t], ctl)) return;   /* i < 0: a meshed particle's triangle */
   const double *r = tab + 6*((size_t) (row >= 0 ? row : ctl[CTL_KSTEP]) * O + i);
   double x[3] = {obspnt[3*i], obspnt[3*i+1], obspnt[3*i+2]};
   double y[3], DL[9];
@@ -1619,13 +1728,14 @@ void rows_launch_t (pmx_ctx &c, R *st0, R *st1, double h, int tpb, int nb)
                       (const double*) c.obspnt.p, (const double*) c.obsang.p, (const double*) c.obslin.p,
                       (const int*) c.rbusy.p, (const int*) c.rnbusy.p, tpb, slices,
                       (const double*) c.rotation.p, ADP && c.adp_on ? c.adp.p : (double*) nullptr,
-                      ADP && c.opt.critical ? c.crec.p : (double*) nullptr);
+                      ADP && c.opt.critical ? c.crec.p : (double*) nullptr,
+                      ADP && c.mesh_contact ? c.tcnt.p : (int*) nullptr, c.trec_e.p, c.trec.p, c.tcap);
 }
 
 template <class R>
 void rows_launch (pmx_ctx &c, R *st0, R *st1, double h, int tpb, int nb)
 {
-  if (c.adp_on || c.opt.critical) rows_launch_t<R, true> (c, st0, st1, h, tpb, nb);
+  if (c.adp_on || c.opt.critical || c.mesh_contact) rows_launch_t<R, true> (c, st0, st1, h, tpb, nb);
   else rows_launch_t<R, false> (c, st0, st1, h, tpb, nb);
 }
 
@@ -1668,8 +1778,26 @@ int step_rows (pmx_ctx &c, double h)
                         c.I, (const int*) c.par_inst.p, c.force.p, c.torque.p,
                         c.staggered ? (const int*) c.istop.p : (const int*) nullptr);
   }
+  if (c.mesh_contact)   /* this step's reactions on meshed particles start from empty slots */
+    hipLaunchKernelGGL (k_zero_i, grid (c.T), dim3 (PMX_TPB), 0, c.stream, c.tcnt.p, c.T);
   int rc = launch_rows (c, c.rows_tpb, h);
   if (rc) return rc;
+  if (c.mesh_contact && (rc = step_react (c, false))) return rc;
+  PMX_HIP (hipGetLastError ());
+  return PMX_OK;
+}
+
+int step_react (pmx_ctx &c, bool critical)
+{
+  if (!c.mesh_contact || c.nmp == 0) return PMX_OK;
+  double *adp = critical || c.adp_on ? c.adp.p : nullptr;
+  hipLaunchKernelGGL (k_react, grid (c.nmp), dim3 (PMX_TPB), 0, c.stream, c.nmp, (const int*) c.mp_part.p,
+                      (const int*) c.mp_toff.p, (const int*) c.mp_tlist.p, c.T, (const int*) c.tcnt.p,
+                      (const int*) c.trec_e.p, (const double*) c.trec.p, c.tcap, c.N, (const double*) c.position.p,
+                      (const double*) c.rotation.p, (const double*) c.mass.p, (const double*) c.gtab.p, c.I,
+                      (const int*) c.par_inst.p, c.force.p, c.torque.p, adp,
+                      critical ? (const int*) nullptr : (const int*) c.ctl.p,
+                      c.staggered ? (const int*) c.istop.p : (const int*) nullptr, critical ? 1 : 0);
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
 }
@@ -1818,11 +1946,11 @@ int enqueue_step_forces (pmx_ctx &c, double h)
   int rc;
   if (c.E > 0 && ((rc = step_bin (c)) || (rc = step_mtri (c)) || (rc = step_detect (c)))) return rc;
   if (c.adp_on && (rc = adp_begin (c))) return rc;
-  /* mesh nodes and faces (shapes.ispc:101-211) are not updated per step:
-   * no contact on this path reads mesh triangles (spheres and meshed
-   * particles never share an instance) and springs carry their own points,
-   * so they are formed from the final particle state when an instance
-   * finishes, with the same arithmetic and the same result */
+  /* mesh nodes and faces (shapes.ispc:101-211) are updated per step only
+   * where spheres can touch them (enqueue_step_motion): elsewhere nothing on
+   * this path reads them (springs carry their own points), so they are formed
+   * from the final particle state when an instance finishes, with the same
+   * arithmetic and the same result */
   if ((rc = step_rows (c, h)) || (rc = step_springs (c, h)) || (rc = step_prescribe_acc (c))) return rc;
   if (c.adp_on && (rc = adp_time_step (c, nullptr, nullptr, nullptr))) return rc;
   return PMX_OK;
@@ -1834,8 +1962,11 @@ int enqueue_step_forces (pmx_ctx &c, double h)
 int enqueue_step_motion (pmx_ctx &c, double step0, double step1)
 {
   int rc;
-  if ((rc = step_dynamics (c, step0, step1)) || (rc = step_prescribe_vel (c)) ||
-      (rc = step_obstacles (c, c.obstab.p, -1, 0.5*(step0+step1))) || (rc = step_history (c)) || (rc = step_end (c)))
+  if ((rc = step_dynamics (c, step0, step1)) || (rc = step_prescribe_vel (c))) return rc;
+  /* spheres touch meshed particles: their nodes and triangles follow every
+   * step (shapes.ispc:101-211), as the next step's detection reads them */
+  if (c.mesh_contact && (rc = step_mesh_shapes (c, true))) return rc;
+  if ((rc = step_obstacles (c, c.obstab.p, -1, 0.5*(step0+step1))) || (rc = step_history (c)) || (rc = step_end (c)))
     return rc;
   return PMX_OK;
 }

@@ -19,7 +19,7 @@ the left child, strict traversal compares, another dimension rule) must
 differ from the oracle on them.
 
 Then two hand-built cases derive, on paper, where the reference rule misses
-an overlapping pair (Q10): a sphere pair whose boxes do not reach each
+an overlapping pair: a sphere pair whose boxes do not reach each
 other's leaves across a two-level split, and a triangle whose box is not
 inflated by the sphere radius (condet.ispc:412-426, 510-511).
 """
@@ -225,7 +225,7 @@ def test_misreadings_differ_from_oracle(mutation):
     assert differs, f"{mutation} is not distinguished by any input"
 
 
-# ---------------------------------------------------------------- Q10 cases
+# ------------------------------------------- pairs the frozen tree misses
 
 def q10_model(b_x=5.05, tri_y=None, c_y=4.5):
     """256 spheres in x, y in [0, 8] (z in [0, 8]) that the tree splits into
