@@ -70,7 +70,7 @@ def test_energy_matches_state_formula():
 
 
 def test_energy_history_box10k_full_run():
-    """C0 in full (10,000 spheres, 2,000 steps): the device's energy and
+    """The 10,000-sphere box in full (2,000 steps): the device's energy and
     momentum history against the oracle's states, within 1e-6."""
     from oracle import orc
     print("case box10k energy history, 2000 steps", flush=True)
@@ -90,7 +90,7 @@ def test_energy_history_box10k_full_run():
 
 
 def test_hopper2m_mixed_vs_fp64_full_run():
-    """C1 in full (2,000,000 spheres, 5,000-triangle hopper, 5,000 steps):
+    """The 2,000,000-sphere hopper (5,000 triangles) over 5,000 steps:
     energy and momentum histories of the mixed path against FP64."""
     print("case hopper 2M, mixed vs fp64, 5000 steps", flush=True)
     m = workloads.hopper(n=2000000, seed=1)
