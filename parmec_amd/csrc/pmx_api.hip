@@ -450,7 +450,7 @@ void pmx_destroy (pmx_ctx *c)
   if (c->stream) (void) hipStreamSynchronize (c->stream);
   dvec<double> *d[] = {&c->angular, &c->linear, &c->rotation, &c->position, &c->inertia, &c->inverse, &c->mass,
                        &c->invm, &c->force, &c->torque, &c->center, &c->radii, &c->tri, &c->iparam, &c->gtab, &c->dtab,
-                       &c->lpos, &c->rowst[0], &c->rowst[1], &c->tree.ext, &c->tree.coord, &c->tree.cell};
+                       &c->lpos, &c->lbox, &c->rowst[0], &c->rowst[1], &c->tree.ext, &c->tree.coord, &c->tree.cell};
   for (auto *p : d) p->release ();
   dvec<int> *ii[] = {&c->flags, &c->part, &c->ellcol, &c->tricol, &c->triobs, &c->pairs, &c->ikind, &c->leafof,
                      &c->slot, &c->lcount, &c->loff, &c->lmem, &c->rowcnt[0], &c->rowcnt[1], &c->hitcnt, &c->xbuf,

@@ -219,6 +219,7 @@ struct pmx_ctx
   TreeDev tree;
   dvec<int> leafof, slot, lcount, loff, lmem, lpart;   /* lmem, lpart: leaf-ordered sphere, its particle */
   dvec<double> lpos;           /* [4][records] leaf-ordered x, y, z, r */
+  dvec<double> lbox;           /* [6][nleaves] this step's box of each leaf's records (k_leafbox) */
   int lstride = 0;             /* leaf L's records at L * lstride (PMX_LSIZE slots per leaf, written by
                                   k_bin), or 0: packed at loff[L] (scan of the counts, then k_scatter) */
   /* contact rows (ping-pong) */

@@ -278,6 +278,66 @@ __device__ __forceinline__ bool overlap (double ax, double ay, double az, double
  * tests over its 64 lanes.  Waves never wait for each other inside the
  * candidate loop.  Hit slots per sphere come from LDS counters; the hit
  * order is irrelevant (rows sort their keys). */
+/* the box of each leaf's records in this step, one wave per leaf: the AABB
+ * of its spheres (c -/+ r per axis) widened by a relative margin, a pair
+ * that the FP64 test accepts having |c_a - c_b| < (r_a + r_b)(1 + 2e-10)
+ * along every axis, lbox [6][nleaves] (lo x y z, hi x y z).  k_detect takes
+ * its own leaf's box from here, and skips a reached neighbour whose box
+ * misses it: each of that neighbour's records then misses the own box by
+ * more than the neighbour's margin, which bounds the margin of the record
+ * filter (1e-9 (|c| + r) <= 1e-9 (|lo| + |hi| + (hi - lo))), so the filter
+ * would have kept none of them.  An empty leaf gets zeros (no sphere reads
+ * it: an empty candidate is skipped by its count). */
+__global__ __launch_bounds__(256)
+void k_leafbox (int nleaves, const int *lcount, const int *loff, const double *lpos, int lstride, double *lbox,
+                const int *ctl)
+{
+  if (ctl[CTL_ABORT]) return;
+  int L = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (L >= nleaves) return;
+  int cnt = lcount[L], off = lstride ? L * lstride : loff[L];
+  double blo[3] = {0.0, 0.0, 0.0}, bhi[3] = {0.0, 0.0, 0.0};
+  if (cnt > 0)
+  {
+#pragma unroll
+    for (int d = 0; d < 3; d++) { blo[d] = DBL_MAX; bhi[d] = -DBL_MAX; }
+    for (int k = lane; k < cnt; k += 64)
+    {
+      const double2 *rp = (const double2*) (lpos + 4*(size_t) (off + k));
+      double2 a = rp[0], b = rp[1];
+      double c[3] = {a.x, a.y, b.x}, r = b.y;
+#pragma unroll
+      for (int d = 0; d < 3; d++)
+      {
+        blo[d] = fmin (blo[d], c[d] - r);
+        bhi[d] = fmax (bhi[d], c[d] + r);
+      }
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1)
+#pragma unroll
+      for (int d = 0; d < 3; d++)
+      {
+        blo[d] = fmin (blo[d], __shfl_xor (blo[d], o, 64));
+        bhi[d] = fmax (bhi[d], __shfl_xor (bhi[d], o, 64));
+      }
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+    {
+      double tol = 1e-9 * (fabs (blo[d]) + fabs (bhi[d]) + (bhi[d] - blo[d]));
+      blo[d] -= tol;
+      bhi[d] += tol;
+    }
+  }
+  if (lane == 0)
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+    {
+      lbox[(size_t) d * nleaves + L] = blo[d];
+      lbox[(size_t) (3+d) * nleaves + L] = bhi[d];
+    }
+}
+
 /* 5 waves per SIMD: the register budget (96 VGPRs, a 16-byte spill) and
  * the LDS per one-wave block (7.3 KB) keep 20 blocks resident per CU, up
  * from 16 at the unbounded register count; detection is bound by the
@@ -290,7 +350,7 @@ void k_detect (int E, const double *center, const double *radii, const int *part
                const int *mtcnt, const int *mtlist, int mcap, const int *nbrcnt, const int *nbroff, const int *nbrlist,
                int nbrstride,
                uint32_t *hits, int *hitcnt, int hcap, int *xbuf, int *xcnt, int xcap, int *flags, int *ctl,
-               const int *stop, const int *leafinst, int lstride)
+               const int *stop, const int *leafinst, int lstride, const double *lbox)
 {
   if (ctl[CTL_ABORT]) return;
   if (blockIdx.x == 0 && threadIdx.x == 0) ctl[CTL_HINT] = 1;   /* this step's bin completed: leafof is valid */
@@ -353,39 +413,16 @@ void k_detect (int E, const double *center, const double *radii, const int *part
   double cu[3] = {cell[3*(size_t) nleaves + Q], cell[4*(size_t) nleaves + Q], cell[5*(size_t) nleaves + Q]};
   __syncthreads ();
 
-  /* the AABB of the block's spheres, widened by a relative margin: a pair
-   * that the FP64 test below accepts has |c_a - c_b| < (r_a + r_b)(1 + 2e-10)
-   * along every axis, so a candidate record whose AABB misses this box by
-   * more than 1e-9 (|c_b| + r_b) overlaps no block sphere (the margins stay
-   * far above the rounding of every comparison) */
+  /* the AABB of the block's spheres, widened by a relative margin
+   * (k_leafbox, from the same records) */
   double blo[3] = {0.0, 0.0, 0.0}, bhi[3] = {0.0, 0.0, 0.0};
   if (qcnt > 0)
-  {
-#pragma unroll
-    for (int d = 0; d < 3; d++) { blo[d] = DBL_MAX; bhi[d] = -DBL_MAX; }
-    for (int k = lane; k < qcnt; k += 64)
-#pragma unroll
-      for (int d = 0; d < 3; d++)
-      {
-        blo[d] = fmin (blo[d], ssph[4*k+d] - ssph[4*k+3]);
-        bhi[d] = fmax (bhi[d], ssph[4*k+d] + ssph[4*k+3]);
-      }
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1)
-#pragma unroll
-      for (int d = 0; d < 3; d++)
-      {
-        blo[d] = fmin (blo[d], __shfl_xor (blo[d], o, 64));
-        bhi[d] = fmax (bhi[d], __shfl_xor (bhi[d], o, 64));
-      }
 #pragma unroll
     for (int d = 0; d < 3; d++)
     {
-      double tol = 1e-9 * (fabs (blo[d]) + fabs (bhi[d]) + (bhi[d] - blo[d]));
-      blo[d] -= tol;
-      bhi[d] += tol;
+      blo[d] = lbox[(size_t) d * nleaves + Q];
+      bhi[d] = lbox[(size_t) (3+d) * nleaves + Q];
     }
-  }
   /* origin O of the FP32 offsets: the box centre; B bounds |c - O| per axis
    * for every block sphere */
   double O[3], B = 0.0;
@@ -475,7 +512,13 @@ void k_detect (int E, const double *center, const double *radii, const int *part
         bm0 = lane == c ? r0 : bm0;
         bm1 = lane == c ? r1 : bm1;
       }
-    if (!(lane < np && bcnt > 0)) { bm0 = 0ull; bm1 = 0ull; }
+    /* a neighbour whose records' box misses the block's: nothing to stage */
+    bool far = false;
+    if (lane < np && bcnt > 0 && bL != Q)
+#pragma unroll
+      for (int d = 0; d < 3; d++)
+        far = far || lbox[(size_t) d * nleaves + bL] > bhi[d] || lbox[(size_t) (3+d) * nleaves + bL] < blo[d];
+    if (!(lane < np && bcnt > 0) || far) { bm0 = 0ull; bm1 = 0ull; }
     __builtin_amdgcn_wave_barrier ();
     /* the candidates that some block sphere reaches, in turn */
     unsigned long long todo = __ballot ((bm0 | bm1) != 0);
@@ -1609,7 +1652,8 @@ int clear_leaf_counts (pmx_ctx &c)
 int step_prepare (pmx_ctx &c)
 {
   int L = std::max (c.tree.nleaves, 1);
-  if (c.lcount.alloc (L) || c.loff.alloc (L)) { pmx_set_error ("out of device memory (bins)"); return PMX_ENOMEM; }
+  if (c.lcount.alloc (L) || c.loff.alloc (L) || c.lbox.alloc (6 * (size_t) L))
+  { pmx_set_error ("out of device memory (bins)"); return PMX_ENOMEM; }
   int rc = clear_leaf_counts (c);
   if (rc) return rc;
   /* leaf records: PMX_LSIZE slots per leaf (k_bin writes each record into
@@ -1684,7 +1728,7 @@ void detect_launch (pmx_ctx &c)
                       (const int*) t.nbrcnt.p, (const int*) t.nbroff.p, (const int*) t.nbrlist.p, t.nbrstride,
                       c.hits.p, c.hitcnt.p, c.hcap, c.xbuf.p, c.xcnt.p, c.xcap, c.dflags.p, c.ctl.p,
                       c.staggered ? (const int*) c.istop.p : (const int*) nullptr, (const int*) t.leafinst.p,
-                      c.lstride);
+                      c.lstride, (const double*) c.lbox.p);
 }
 
 /* k_detect with nw waves per block (1, 2, 4 or 8) */
@@ -1692,6 +1736,9 @@ int launch_detect (pmx_ctx &c, int nw)
 {
   /* per sphere: its leaf record 32, member 4, particle 4, hit count 4 */
   ProfScope ps (c, K_DETECT, 44.0 * c.E);
+  hipLaunchKernelGGL (k_leafbox, dim3 ((c.tree.nleaves + 3) / 4), dim3 (256), 0, c.stream, c.tree.nleaves,
+                      (const int*) c.lcount.p, (const int*) c.loff.p, (const double*) c.lpos.p, c.lstride, c.lbox.p,
+                      (const int*) c.ctl.p);
   if (nw == 1) detect_launch<1> (c);
   else if (nw == 2) detect_launch<2> (c);
   else if (nw == 8) detect_launch<8> (c);
