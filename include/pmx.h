@@ -138,8 +138,68 @@ typedef struct pmx_model
   /* time bookkeeping, parmec.h:44-49 */
   double curtime;
   double curstep;
-  int stepnum;
+  int stepnum;       /* dem() calls so far (parmec.cpp:2523); UNSPRING tests run when stepnum % nsteps == 0 */
+
+  /* load curves: time series as look-up tables (lcurve_from_time_series,
+   * parmec.cpp:1239-1269), read by spring stroke offsets (sproffset) and by
+   * UNSPRING's unloading curves (unaction) */
+  int lcnum;
+  int *lcidx;        /* [lcnum+1] into lcurve */
+  double *lcurve;    /* [2][lccrv]: time (or stroke), value */
+  int lccrv;
+
+  /* torsion springs, parmec.cpp:180-198 (TORSION_SPRING, input.cpp:1791-2080),
+   * sorted by (part1, part2); the roll, pitch and yaw curves of all springs
+   * share one table per kind */
+  int trqsprnum;
+  int *trqsprpart;   /* [2][trqsprnum] (part2 = -1: a spring to the ground) */
+  double *trqzdir0;  /* [3][trqsprnum] reference z (roll) direction in part1's body frame */
+  double *trqxdir0;  /* [3][trqsprnum] reference x direction */
+  int *krpyidx;      /* [3][trqsprnum+1] roll, pitch, yaw stiffness curves in krpy */
+  double *krpy;      /* [2][krpycrv]: angle, torque */
+  int krpycrv;
+  int *drpyidx;      /* [3][trqsprnum+1] damper curves in drpy; one point: a critical damping ratio */
+  double *drpy;      /* [2][drpycrv]: angular velocity, torque (or ratio, ratio) */
+  int drpycrv;
+  int *trqcone;      /* [trqsprnum] cone flags: 1 roll, 2 pitch, 4 yaw (constants.h:66-68) */
+  double *trqzdir1;  /* [3][trqsprnum] (output) current z direction */
+  double *trqxdir1;  /* [3][trqsprnum] (output) current x direction */
+  double *trqrpy;    /* [3][trqsprnum] (state) roll, pitch, yaw angles */
+  double *trqrpytot; /* [3][trqsprnum] (output) total torques conjugate with the angles */
+  double *trqrpyspr; /* [3][trqsprnum] (output) spring torques without the damper parts */
+
+  /* UNSPRING conditions (input.cpp:2083-2295; unspring_task, forces.ispc:2058-2127);
+   * spring numbers here are indices of the sorted springs */
+  int unsprnum;
+  int *tspridx;      /* [unsprnum+1] into tsprings: the springs each condition tests */
+  int *tsprings;
+  int *mspridx;      /* [unsprnum+1] into msprings: the springs it switches off */
+  int *msprings;
+  int *actidx;       /* [unsprnum+1] into activate: the inactive springs it switches on */
+  int *activate;
+  double *unlim;     /* [2][unsprnum] lower, upper limit */
+  int *unent;        /* [unsprnum] PMX_HIS_STROKE, _F, _SF or _FF */
+  int *unop;         /* [unsprnum] PMX_OP_* */
+  int *unabs;        /* [unsprnum] test |value| */
+  int *nsteps;       /* [unsprnum] test when stepnum % nsteps == 0 */
+  int *nfreq;        /* [unsprnum] (state) failures left before the condition triggers; 0: triggered */
+  int *unaction;     /* [unsprnum] -1: zero force, >= 0: unloading load curve */
+
+  /* RESTRAIN (input.cpp:2458-2644; restrain.ispc:30-154) */
+  int rstnum;
+  int *rstpart;      /* [rstnum] */
+  double *rstlin;    /* [9][rstnum] up to three orthonormal restrained linear directions (zeros: none) */
+  double *rstang;    /* [9][rstnum] restrained angular directions */
+
+  /* BALL_JOINT (input.cpp:2298-2333; joints.cpp:83-567) */
+  int jnum;
+  int *jpart;        /* [2][jnum] (part2 = -1: a joint to the ground) */
+  double *jpoint;    /* [3][jnum] joint point in the reference configuration */
+  double *jreac;     /* [3][jnum] (output) joint reactions */
 } pmx_model;
+
+/* UNSPRING operators, constants.h:70 */
+enum { PMX_OP_SUM = 0, PMX_OP_MAX = 1, PMX_OP_MIN = 2 };
 
 /*
  * Per-step external inputs.  Parmec evaluates gravity and global damping
@@ -202,7 +262,11 @@ enum
   PMX_HIS_TIME = 0, PMX_HIS_PX, PMX_HIS_PY, PMX_HIS_PZ, PMX_HIS_PL, PMX_HIS_DX, PMX_HIS_DY, PMX_HIS_DZ, PMX_HIS_DL,
   PMX_HIS_VX, PMX_HIS_VY, PMX_HIS_VZ, PMX_HIS_VL, PMX_HIS_OX, PMX_HIS_OY, PMX_HIS_OZ, PMX_HIS_OL,
   PMX_HIS_FX, PMX_HIS_FY, PMX_HIS_FZ, PMX_HIS_FL, PMX_HIS_TX, PMX_HIS_TY, PMX_HIS_TZ, PMX_HIS_TL,
-  PMX_HIS_LENGTH, PMX_HIS_STROKE, PMX_HIS_F, PMX_HIS_SF, PMX_HIS_FF, PMX_HIS_COUNT
+  PMX_HIS_LENGTH, PMX_HIS_STROKE, PMX_HIS_F, PMX_HIS_SF, PMX_HIS_FF, PMX_HIS_SS,
+  PMX_HIS_ZDIR_X, PMX_HIS_ZDIR_Y, PMX_HIS_ZDIR_Z, PMX_HIS_XDIR_X, PMX_HIS_XDIR_Y, PMX_HIS_XDIR_Z,
+  PMX_HIS_ROLL, PMX_HIS_PITCH, PMX_HIS_YAW, PMX_HIS_TRQTOT_R, PMX_HIS_TRQTOT_P, PMX_HIS_TRQTOT_Y,
+  PMX_HIS_TRQSPR_R, PMX_HIS_TRQSPR_P, PMX_HIS_TRQSPR_Y, PMX_HIS_JREAC_X, PMX_HIS_JREAC_Y, PMX_HIS_JREAC_Z,
+  PMX_HIS_JREAC_L, PMX_HIS_COUNT
 };
 
 typedef struct pmx_history
@@ -210,7 +274,8 @@ typedef struct pmx_history
   int count;                 /* histories */
   const int *entity;         /* [count] PMX_HIS_* */
   const int *off;            /* [count+1] offsets into list */
-  const int *list;           /* particle numbers (spring numbers for LENGTH .. FF) */
+  const int *list;           /* particle numbers; spring numbers for LENGTH .. SS; torsion spring
+                                numbers for ZDIR_X .. TRQSPR_Y; joint numbers for JREAC_* */
   const int *haspoint;       /* [count] 1: one particle's body point `point` (HIS_LIST|HIS_POINT), or NULL */
   const double *point;       /* [count][3] referential point, or NULL */
 } pmx_history;

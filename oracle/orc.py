@@ -93,3 +93,26 @@ def extremum_eigenvalue(A, minimum):
     import numpy as np
     A = np.ascontiguousarray(A, dtype=np.float64).reshape(9)
     return load().orc_extremum_eigenvalue(_dp(A), int(minimum))
+
+
+def skyline_lu(A, b):
+    """amgcl's skyline_lu solve of a dense matrix (None on a zero pivot)."""
+    import numpy as np
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(len(b))
+    lib = load()
+    lib.orc_skyline_lu.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    return None if lib.orc_skyline_lu(len(b), _dp(A), _dp(b), _dp(x)) else x
+
+
+def cuthill_mckee(ptr, col):
+    """amgcl's Cuthill-McKee order of a CSR pattern."""
+    import numpy as np
+    ptr = np.ascontiguousarray(ptr, dtype=np.int32)
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    n = len(ptr) - 1
+    perm = np.zeros(n, dtype=np.int32)
+    ip = C.POINTER(C.c_int)
+    load().orc_cuthill_mckee(n, ptr.ctypes.data_as(ip), col.ctypes.data_as(ip), perm.ctypes.data_as(ip))
+    return perm

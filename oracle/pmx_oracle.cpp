@@ -28,6 +28,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <set>
 #include <string>
 #include <vector>
 #ifdef _OPENMP
@@ -118,6 +120,31 @@ inline void nnmul (const double A[9], const double B[9], double C[9])
   for (int c = 0; c < 3; c++)
     for (int r = 0; r < 3; r++)
       C[3*c+r] = A[r]*B[3*c] + A[3+r]*B[3*c+1] + A[6+r]*B[3*c+2];
+}
+/* NTMUL: C = A B^T */
+inline void ntmul (const double A[9], const double B[9], double C[9])
+{
+  for (int c = 0; c < 3; c++)
+    for (int r = 0; r < 3; r++)
+      C[3*c+r] = A[r]*B[c] + A[3+r]*B[3+c] + A[6+r]*B[6+c];
+}
+/* INVERT, macros.h:1033-1051: INV is left as it was where det = 0 */
+inline void invert (const double F[9], double INV[9])
+{
+  double det = F[0]*(F[4]*F[8]-F[5]*F[7]) + F[3]*(F[2]*F[7]-F[1]*F[8]) + (F[1]*F[5]-F[2]*F[4])*F[6];
+  if (det != 0.0)
+  {
+    det = 1.0 / det;
+    INV[0] = (F[4]*F[8]-F[5]*F[7])*det;
+    INV[1] = (F[2]*F[7]-F[1]*F[8])*det;
+    INV[2] = (F[1]*F[5]-F[2]*F[4])*det;
+    INV[3] = (F[5]*F[6]-F[3]*F[8])*det;
+    INV[4] = (F[0]*F[8]-F[2]*F[6])*det;
+    INV[5] = (F[2]*F[3]-F[0]*F[5])*det;
+    INV[6] = (F[3]*F[7]-F[4]*F[6])*det;
+    INV[7] = (F[1]*F[6]-F[0]*F[7])*det;
+    INV[8] = (F[0]*F[4]-F[1]*F[3])*det;
+  }
 }
 
 /* Eberly closest point on a triangle, condet.ispc:294-400 (detection form) and
@@ -464,6 +491,27 @@ struct orc_ctx
   std::vector<int> sprtype, sprpart, spridx, dashidx, unidx, sprflg, unspring, sproffset;
   std::vector<double> sprpnt, spring, dashpot, unload, yield, sprdir, sprfric, sprkskn, sprsdsp, stroke0, stroke, sprfrc;
   int sprcrv = 0, dashcrv = 0, uncrv = 0;
+  /* load curves (lcurve_from_time_series, parmec.cpp:1239-1269) */
+  int lcnum = 0, lccrv = 0;
+  std::vector<int> lcidx;
+  std::vector<double> lcurve;
+  /* torsion springs, sorted by (part1, part2); curve tables shared by the
+   * roll, pitch and yaw curves */
+  int trqsprnum = 0, krpycrv = 0, drpycrv = 0;
+  std::vector<int> trqsprpart, krpyidx, drpyidx, trqcone;
+  std::vector<double> trqzdir0, trqxdir0, krpy, drpy, trqzdir1, trqxdir1, trqrpy, trqrpytot, trqrpyspr;
+  /* UNSPRING conditions (spring numbers of the sorted springs) */
+  int unsprnum = 0;
+  std::vector<int> tspridx, tsprings, mspridx, msprings, actidx, activate, unent, unop, unabs, nsteps, nfreq, unaction;
+  std::vector<double> unlim;
+  /* RESTRAIN */
+  int rstnum = 0;
+  std::vector<int> rstpart;
+  std::vector<double> rstlin, rstang;
+  /* BALL_JOINT */
+  int jnum = 0;
+  std::vector<int> jpart;
+  std::vector<double> jpoint, jreac;
   double curtime = 0.0, curstep = 0.0;
   int stepnum = 0;
   std::vector<std::vector<Contact>> master;
@@ -1361,7 +1409,14 @@ int springs (orc_ctx &s, double step)
     /* the reference evaluates the relative normal velocity only on the
      * force-law branch; evaluated here for every spring (its friction uses it) */
     velocity = z[0]*u[0] + z[1]*u[1] + z[2]*u[2];
+    /* forces.ispc:915-919: the stroke offset, a load curve of time */
     double offset = 0.0;
+    int ilc = s.sproffset.empty () ? -1 : s.sproffset[i];
+    if (ilc >= 0)
+    {
+      double slope;
+      offset = look_up (s.lcurve.data (), s.lcurve.data () + s.lccrv, s.lcidx[ilc], s.lcidx[ilc+1], s.curtime, &slope);
+    }
     int unspr = s.unspring[i];
     double *stroke = s.stroke.data (), *yld = s.yield.data ();
     if (unspr <= -2)
@@ -1449,7 +1504,28 @@ int springs (orc_ctx &s, double step)
       damping = 0.;
       spring_force = dashpot_force = total_force = 0.;
     }
-    else return 1;   /* load-curve unloading (UNSPRING) is not on this path */
+    else
+    {
+      /* forces.ispc:1080-1115: unloading along load curve unspr */
+      double s0 = stroke[i];
+      double s1 = len - s.stroke0[i];
+      double ds = s1 - s0;
+      double f0 = s.sprfrc[SS + i];
+      double sgn = f0 > 0. ? 1. : -1.;
+      stroke[i] = s1;
+      double at;
+      double dfds = look_up_slope (1, s.lcurve.data (), s.lcurve.data () + s.lccrv, s.lcidx[unspr], s.lcidx[unspr+1], f0, &at);
+      stiffness = dfds;
+      spring_force = f0 - sgn*std::fabs (dfds*ds);
+      if (spring_force * f0 < 0.0)
+      {
+        s.unspring[i] = -1;
+        spring_force = 0.;
+      }
+      damping = 0.;
+      dashpot_force = 0.;
+      total_force = spring_force;
+    }
 
     for (int c = 0; c < 3; c++) frc[c] = z[c] * total_force;
     s.sprfrc[i] = total_force;
@@ -1535,6 +1611,638 @@ int springs (orc_ctx &s, double step)
   {
     for (int c = 0; c < 3; c++) { force[c*N + k0] -= fk[c]; torque[c*N + k0] -= tk[c]; }
     if (adaptive) flush (k0, kact_k, kmax_k, emax_k, krot_k);
+  }
+  return 0;
+}
+
+/* --------------------------------------------------------- torsion springs */
+
+/* trqspr_task, forces.ispc:1415-2055, as one task over all torsion springs
+ * (sorted by part1, part2): curve torques and dampers per axis or on a cone,
+ * local sums flushed where part1 or part2 changes */
+void torsion_springs (orc_ctx &s, double step)
+{
+  const int T = s.trqsprnum;
+  const size_t N = s.parnum, TT = T;
+  double *torque = s.torque.data ();
+  const double *kx = s.krpy.data (), *ky = kx + s.krpycrv, *dx = s.drpy.data (), *dy = dx + s.drpycrv;
+  auto kidx = [&] (int o, int i) { return s.krpyidx[o*(TT+1) + i]; };
+  auto didx = [&] (int o, int i) { return s.drpyidx[o*(TT+1) + i]; };
+  double oj[3], Lj[9], ok[3];
+  int j0 = -1, k0 = -1;
+  double tj[3] = {0, 0, 0}, tk[3] = {0, 0, 0};
+  double krot_j[6] = {0, 0, 0, 0, 0, 0}, krot_k[6] = {0, 0, 0, 0, 0, 0};
+  double *adp = s.adaptive ? s.adp.data () : nullptr;
+  for (int i = 0; i < T; i++)
+  {
+    double zdir0[3], xdir0[3], ydir0[3], rot1j[9];
+    int j = s.trqsprpart[i], k = s.trqsprpart[TT + i];
+    if (j0 != j)
+    {
+      for (int c = 0; c < 3; c++) oj[c] = s.angular[(3+c)*N + j];
+      for (int c = 0; c < 9; c++) Lj[c] = s.rotation[c*N + j];
+    }
+    for (int c = 0; c < 3; c++) { zdir0[c] = s.trqzdir0[c*TT + i]; xdir0[c] = s.trqxdir0[c*TT + i]; }
+    ydir0[0] = zdir0[1]*xdir0[2] - zdir0[2]*xdir0[1];
+    ydir0[1] = zdir0[2]*xdir0[0] - zdir0[0]*xdir0[2];
+    ydir0[2] = zdir0[0]*xdir0[1] - zdir0[1]*xdir0[0];
+    nvmul (Lj, xdir0, rot1j);
+    nvmul (Lj, ydir0, rot1j+3);
+    nvmul (Lj, zdir0, rot1j+6);
+    for (int c = 0; c < 3; c++) { s.trqzdir1[c*TT + i] = rot1j[6+c]; s.trqxdir1[c*TT + i] = rot1j[c]; }
+    if (k >= 0) { if (k0 != k) for (int c = 0; c < 3; c++) ok[c] = s.angular[(3+c)*N + k]; }
+    else ok[0] = ok[1] = ok[2] = 0.0;
+    double ojk[3] = {oj[0]-ok[0], oj[1]-ok[1], oj[2]-ok[2]};
+    double rolldot = rot1j[0]*ojk[0] + rot1j[1]*ojk[1] + rot1j[2]*ojk[2];
+    double pitchdot = rot1j[3]*ojk[0] + rot1j[4]*ojk[1] + rot1j[5]*ojk[2];
+    double yawdot = rot1j[6]*ojk[0] + rot1j[7]*ojk[1] + rot1j[8]*ojk[2];
+    double roll = s.trqrpy[i], pitch = s.trqrpy[TT + i], yaw = s.trqrpy[2*TT + i];
+    roll += rolldot*step;
+    pitch += pitchdot*step;
+    yaw += yawdot*step;
+    s.trqrpy[i] = roll; s.trqrpy[TT + i] = pitch; s.trqrpy[2*TT + i] = yaw;
+    double I[9], tmp[9], Ixx = 0.0, Iyy = 0.0, Izz = 0.0;
+    if (didx (0, i)+1 == didx (0, i+1) || didx (1, i)+1 == didx (1, i+1) || didx (2, i)+1 == didx (2, i+1))
+    {
+      for (int c = 0; c < 9; c++) I[c] = s.inverse[c*N + j];
+      if (k >= 0) for (int c = 0; c < 9; c++) I[c] += s.inverse[c*N + k];
+      ntmul (I, rot1j, tmp);
+      nnmul (rot1j, tmp, I);
+      invert (I, tmp);
+      Ixx = tmp[0]; Iyy = tmp[4]; Izz = tmp[8];
+    }
+    double kroll = 0.0, kpitch = 0.0, kyaw = 0.0, droll, dpitch, dyaw;
+    double trqroll = 0.0, trqpitch = 0.0, trqyaw = 0.0, trqrolldot = 0.0, trqpitchdot = 0.0, trqyawdot = 0.0;
+    double trqrolltot = 0.0, trqpitchtot = 0.0, trqyawtot = 0.0;
+    /* curve o: forces.ispc:1625-1637 */
+    auto curve = [&] (int o, double arg, double *kk) -> double {
+      if (kidx (o, i+1) > kidx (o, i)) return look_up (kx, ky, kidx (o, i), kidx (o, i+1), arg, kk);
+      *kk = 0.0;
+      return 0.0;
+    };
+    /* damper o, its ratio read from table r (forces.ispc:1641-1675) */
+    auto damper = [&] (int o, int r, double kk, double Iq, double vel, double *d) -> double {
+      if (didx (o, i)+1 == didx (o, i+1))
+      {
+        double ratio = dx[didx (r, i)];
+        *d = - ratio * 2.0 * std::sqrt (std::fabs (kk) * Iq);
+        return vel * *d;
+      }
+      return look_up (dx, dy, didx (o, i), didx (o, i+1), vel, d);
+    };
+    switch (s.trqcone[i])
+    {
+    case 0:
+      trqroll = curve (0, roll, &kroll);
+      trqpitch = curve (1, pitch, &kpitch);
+      trqyaw = curve (2, yaw, &kyaw);
+      trqrolldot = damper (0, 0, kroll, Ixx, rolldot, &droll);
+      trqpitchdot = damper (1, 1, kpitch, Iyy, pitchdot, &dpitch);
+      trqyawdot = damper (2, 2, kyaw, Izz, yawdot, &dyaw);
+      trqrolltot = trqroll + (trqroll == 0.0 ? 0.0 : trqrolldot);
+      trqpitchtot = trqpitch + (trqpitch == 0.0 ? 0.0 : trqpitchdot);
+      trqyawtot = trqyaw + (trqyaw == 0.0 ? 0.0 : trqyawdot);
+      break;
+    case 3:
+    {
+      double angle = std::sqrt (roll*roll + pitch*pitch);
+      double angledot = angle > 0.0 ? (rolldot*roll+pitchdot*pitch)/angle : 0.0;
+      trqroll = curve (0, angle, &kroll);
+      kpitch = kroll;
+      trqrolldot = damper (0, 0, kroll, std::sqrt (Ixx*Ixx + Iyy*Iyy), angledot, &droll);
+      trqyaw = curve (2, yaw, &kyaw);
+      trqyawdot = damper (2, 2, kyaw, Izz, yawdot, &dyaw);
+      trqyawtot = trqyaw + (trqyaw == 0.0 ? 0.0 : trqyawdot);
+      if (angle > 0.)
+      {
+        double xcoef = roll/angle, ycoef = pitch/angle;
+        trqrolltot = xcoef*(trqroll + (trqroll == 0.0 ? 0.0 : trqrolldot));
+        trqpitchtot = ycoef*(trqroll + (trqroll == 0.0 ? 0.0 : trqrolldot));
+        trqpitch = trqroll*ycoef;
+        trqroll *= xcoef;
+      }
+      else { trqrolltot = trqpitchtot = 0.0; trqroll = trqpitch = 0.0; }
+      break;
+    }
+    case 5:
+    {
+      double angle = std::sqrt (roll*roll + yaw*yaw);
+      double angledot = angle > 0.0 ? (rolldot*roll+yawdot*yaw)/angle : 0.0;
+      trqroll = curve (0, angle, &kroll);
+      kyaw = kroll;
+      trqrolldot = damper (0, 0, kroll, std::sqrt (Ixx*Ixx + Izz*Izz), angledot, &droll);
+      trqpitch = curve (1, pitch, &kpitch);
+      trqpitchdot = damper (1, 1, kpitch, Iyy, pitchdot, &dpitch);
+      trqpitchtot = trqpitch + (trqpitch == 0.0 ? 0.0 : trqpitchdot);
+      if (angle > 0.)
+      {
+        double xcoef = roll/angle, zcoef = yaw/angle;
+        trqrolltot = xcoef*(trqroll + (trqroll == 0.0 ? 0.0 : trqrolldot));
+        trqyawtot = zcoef*(trqroll + (trqroll == 0.0 ? 0.0 : trqrolldot));
+        trqyaw = trqroll*zcoef;
+        trqroll *= xcoef;
+      }
+      else { trqrolltot = trqyawtot = 0.0; trqroll = trqyaw = 0.0; }
+      break;
+    }
+    case 6:
+    {
+      trqroll = curve (0, roll, &kroll);
+      trqrolldot = damper (0, 0, kroll, Ixx, rolldot, &droll);
+      double angle = std::sqrt (pitch*pitch + yaw*yaw);
+      double angledot = angle > 0.0 ? (pitchdot*pitch+yawdot*yaw)/angle : 0.0;
+      trqpitch = curve (1, angle, &kpitch);
+      kyaw = kpitch;
+      /* the ratio read from roll's table (forces.ispc:1840) */
+      trqpitchdot = damper (1, 0, kpitch, std::sqrt (Iyy*Iyy + Izz*Izz), angledot, &dpitch);
+      trqrolltot = trqroll + (trqroll == 0.0 ? 0.0 : trqrolldot);
+      if (angle > 0.)
+      {
+        double ycoef = pitch/angle, zcoef = yaw/angle;
+        trqpitchtot = ycoef*(trqpitch + (trqpitch == 0.0 ? 0.0 : trqpitchdot));
+        trqyawtot = zcoef*(trqpitch + (trqpitch == 0.0 ? 0.0 : trqpitchdot));
+        trqyaw = zcoef*trqpitch;
+        trqpitch *= ycoef;
+      }
+      else { trqpitchtot = trqyawtot = 0.0; trqpitch = trqyaw = 0.0; }
+      break;
+    }
+    case 7:
+    {
+      double angle = std::sqrt (roll*roll + pitch*pitch + yaw*yaw);
+      double angledot = angle > 0.0 ? (rolldot*roll+pitchdot*pitch+yawdot*yaw)/angle : 0.0;
+      trqroll = curve (0, angle, &kroll);
+      kpitch = kyaw = kroll;
+      trqrolldot = damper (0, 0, kroll, std::sqrt (Ixx*Ixx + Iyy*Iyy + Izz*Izz), angledot, &droll);
+      if (angle > 0.)
+      {
+        double xcoef = roll/angle, ycoef = pitch/angle, zcoef = yaw/angle;
+        trqrolltot = xcoef*(trqroll + (trqroll == 0.0 ? 0.0 : trqrolldot));
+        trqpitchtot = ycoef*(trqroll + (trqroll == 0.0 ? 0.0 : trqrolldot));
+        trqyawtot = zcoef*(trqroll + (trqroll == 0.0 ? 0.0 : trqrolldot));
+        trqpitch = ycoef*trqroll;
+        trqyaw = zcoef*trqroll;
+        trqroll *= xcoef;
+      }
+      else { trqrolltot = trqpitchtot = trqyawtot = 0.0; trqroll = trqpitch = trqyaw = 0.0; }
+      break;
+    }
+    }
+    double trq[3];
+    trq[0] = trqrolltot*rot1j[0] + trqpitchtot*rot1j[3] + trqyawtot*rot1j[6];
+    trq[1] = trqrolltot*rot1j[1] + trqpitchtot*rot1j[4] + trqyawtot*rot1j[7];
+    trq[2] = trqrolltot*rot1j[2] + trqpitchtot*rot1j[5] + trqyawtot*rot1j[8];
+    s.trqrpytot[i] = trqrolltot; s.trqrpytot[TT + i] = trqpitchtot; s.trqrpytot[2*TT + i] = trqyawtot;
+    s.trqrpyspr[i] = trqroll; s.trqrpyspr[TT + i] = trqpitch; s.trqrpyspr[2*TT + i] = trqyaw;
+    if (j0 != j && j0 >= 0)
+    {
+      for (int c = 0; c < 3; c++) { torque[c*N + j0] += tj[c]; tj[c] = trq[c]; }
+      if (adp) for (int c = 0; c < 6; c++) { adp[(5+c)*N + j0] += krot_j[c]; krot_j[c] = 0.0; }
+    }
+    else for (int c = 0; c < 3; c++) tj[c] += trq[c];
+    if (adp)
+    {
+      krot_j[0] += std::fabs (kroll)*rot1j[0];
+      krot_j[1] += std::fabs (kpitch)*rot1j[4];
+      krot_j[2] += std::fabs (kyaw)*rot1j[8];
+    }
+    if (k >= 0)
+    {
+      if (k0 != k && k0 >= 0)
+      {
+        for (int c = 0; c < 3; c++) torque[c*N + k0] -= tk[c];
+        if (adp) for (int c = 0; c < 6; c++) adp[(5+c)*N + k0] += krot_k[c];
+        for (int c = 0; c < 3; c++) tk[c] = trq[c];
+        if (adp) for (int c = 0; c < 6; c++) krot_k[c] = 0.0;
+      }
+      else for (int c = 0; c < 3; c++) tk[c] += trq[c];
+      if (adp)
+      {
+        krot_k[0] += std::fabs (kroll)*rot1j[0];
+        krot_k[1] += std::fabs (kpitch)*rot1j[4];
+        krot_k[2] += std::fabs (kyaw)*rot1j[8];
+      }
+      k0 = k;
+    }
+    j0 = j;
+  }
+  if (j0 >= 0)
+  {
+    for (int c = 0; c < 3; c++) torque[c*N + j0] += tj[c];
+    if (adp) for (int c = 0; c < 6; c++) adp[(5+c)*N + j0] += krot_j[c];
+  }
+  if (k0 >= 0)
+  {
+    for (int c = 0; c < 3; c++) torque[c*N + k0] -= tk[c];
+    if (adp) for (int c = 0; c < 6; c++) adp[(5+c)*N + k0] += krot_k[c];
+  }
+}
+
+/* unspring_task, forces.ispc:2058-2127 */
+void unspring_test (orc_ctx &s)
+{
+  const size_t SS = s.sprnum;
+  for (int i = 0; i < s.unsprnum; i++)
+  {
+    if (s.stepnum % s.nsteps[i]) continue;
+    if (s.nfreq[i] == 0) continue;
+    int op = s.unop[i];
+    double result = op == PMX_OP_MIN ? 1.7976931348623157e308 : op == PMX_OP_MAX ? -1.7976931348623157e308 : 0.;
+    for (int q = s.tspridx[i]; q < s.tspridx[i+1]; q++)
+    {
+      int j = s.tsprings[q];
+      double value;
+      switch (s.unent[i])
+      {
+      case PMX_HIS_STROKE: value = s.stroke[j]; break;
+      case PMX_HIS_F: value = s.sprfrc[j]; break;
+      case PMX_HIS_SF: value = s.sprfrc[SS + j]; break;
+      default: value = s.sprfrc[2*SS + j]; break;
+      }
+      if (s.unabs[i]) value = std::fabs (value);
+      switch (op)
+      {
+      case PMX_OP_MIN: result = value < result ? value : result; break;
+      case PMX_OP_MAX: result = value > result ? value : result; break;
+      default: result += value; break;
+      }
+    }
+    if (result < s.unlim[i] || result > s.unlim[s.unsprnum + i])
+    {
+      s.nfreq[i] --;
+      if (s.nfreq[i] == 0)
+      {
+        for (int q = s.mspridx[i]; q < s.mspridx[i+1]; q++) s.unspring[s.msprings[q]] = s.unaction[i];
+        for (int q = s.actidx[i]; q < s.actidx[i+1]; q++) s.unspring[s.activate[q]] = -3;
+      }
+    }
+  }
+}
+
+/* -------------------------------------------------------------- restraints */
+
+/* SUBMUL (v, DOT (v, d), d, v) */
+void unproject (double v[3], const double d[3])
+{
+  double dot = v[0]*d[0] + v[1]*d[1] + v[2]*d[2];
+  for (int c = 0; c < 3; c++) v[c] = v[c] - dot*d[c];
+}
+
+/* restrain_velocities_task, restrain.ispc:30-89, entries in order */
+void restrain_velocities (orc_ctx &s)
+{
+  const size_t N = s.parnum, R = s.rstnum;
+  for (size_t i = 0; i < R; i++)
+  {
+    int j = s.rstpart[i];
+    double v[3], O[3], o[3], L[9], l[9], a[9], A[9];
+    for (int c = 0; c < 3; c++) { v[c] = s.linear[c*N + j]; O[c] = s.angular[c*N + j]; o[c] = s.angular[(3+c)*N + j]; }
+    for (int c = 0; c < 9; c++) { L[c] = s.rotation[c*N + j]; l[c] = s.rstlin[c*R + i]; a[c] = s.rstang[c*R + i]; }
+    unproject (v, l); unproject (v, l+3); unproject (v, l+6);
+    unproject (o, a); unproject (o, a+3); unproject (o, a+6);
+    tvmul (L, a, A); tvmul (L, a+3, A+3); tvmul (L, a+6, A+6);
+    unproject (O, A); unproject (O, A+3); unproject (O, A+6);
+    for (int c = 0; c < 3; c++) { s.linear[c*N + j] = v[c]; s.angular[c*N + j] = O[c]; s.angular[(3+c)*N + j] = o[c]; }
+  }
+}
+
+/* restrain_forces_task, restrain.ispc:92-133 */
+void restrain_forces (orc_ctx &s)
+{
+  const size_t N = s.parnum, R = s.rstnum;
+  for (size_t i = 0; i < R; i++)
+  {
+    int j = s.rstpart[i];
+    double f[3], t[3], l[9], a[9];
+    for (int c = 0; c < 3; c++) { f[c] = s.force[c*N + j]; t[c] = s.torque[c*N + j]; }
+    for (int c = 0; c < 9; c++) { l[c] = s.rstlin[c*R + i]; a[c] = s.rstang[c*R + i]; }
+    unproject (f, l); unproject (f, l+3); unproject (f, l+6);
+    unproject (t, a); unproject (t, a+3); unproject (t, a+6);
+    for (int c = 0; c < 3; c++) { s.force[c*N + j] = f[c]; s.torque[c*N + j] = t[c]; }
+  }
+}
+
+/* ------------------------------------------------------------ ball joints */
+
+/* amgcl::reorder::cuthill_mckee<false>::get, cuthill_mckee.hpp:49-170, on a
+ * CSR pattern (its level-set walk visits nodes while `node > 0`) */
+void cuthill_mckee (int n, const std::vector<int> &ptr, const std::vector<int> &col, std::vector<int> &perm)
+{
+  perm.assign (n, 0);
+  if (n == 0) return;
+  std::vector<int> degree (n), levelSet (n, 0), nextSameDegree (n, -1);
+  int maxDegree = 0;
+  for (int i = 0; i < n; i++) { degree[i] = ptr[i+1] - ptr[i]; maxDegree = std::max (maxDegree, degree[i]); }
+  std::vector<int> firstWithDegree (maxDegree + 1, -1), nFirstWithDegree (maxDegree + 1);
+  int initialNode = 0;
+  perm[0] = initialNode;
+  int currentLevelSet = 1;
+  levelSet[initialNode] = currentLevelSet;
+  int maxDegreeInCurrentLevelSet = degree[initialNode];
+  firstWithDegree[maxDegreeInCurrentLevelSet] = initialNode;
+  for (int next = 1; next < n; )
+  {
+    int nMDICLS = 0;
+    std::fill (nFirstWithDegree.begin (), nFirstWithDegree.end (), -1);
+    bool empty = true;
+    for (int soughtDegree = 0; soughtDegree != maxDegreeInCurrentLevelSet + 1; soughtDegree++)
+    {
+      int node = firstWithDegree[soughtDegree];
+      while (node > 0)
+      {
+        for (int q = ptr[node]; q < ptr[node+1]; q++)
+        {
+          int c = col[q];
+          if (levelSet[c] == 0)
+          {
+            levelSet[c] = currentLevelSet + 1;
+            perm[next] = c;
+            ++next;
+            empty = false;
+            nextSameDegree[c] = nFirstWithDegree[degree[c]];
+            nFirstWithDegree[degree[c]] = c;
+            nMDICLS = std::max (nMDICLS, degree[c]);
+          }
+        }
+        node = nextSameDegree[node];
+      }
+    }
+    ++currentLevelSet;
+    maxDegreeInCurrentLevelSet = nMDICLS;
+    for (int i = 0; i <= nMDICLS; ++i) firstWithDegree[i] = nFirstWithDegree[i];
+    if (empty)
+    {
+      for (int i = 0; i < n; ++i)
+        if (levelSet[i] == 0)
+        {
+          perm[next] = i;
+          ++next;
+          levelSet[i] = currentLevelSet;
+          maxDegreeInCurrentLevelSet = degree[i];
+          firstWithDegree[maxDegreeInCurrentLevelSet] = i;
+          break;
+        }
+    }
+  }
+}
+
+/* amgcl::solver::skyline_lu (skyline_lu.hpp:74-303): the matrix reordered,
+ * stored on its skyline of non-zero entries, factorised by Crout's method
+ * with the diagonal held as inverses, then solved; returns 1 on a zero
+ * pivot (the reference's precondition fails there) */
+int skyline_lu (int n, const std::vector<int> &aptr, const std::vector<int> &col, const std::vector<double> &val,
+                const std::vector<double> &rhs, std::vector<double> &x)
+{
+  std::vector<int> perm, invperm (n), ptr (n + 1, 0);
+  cuthill_mckee (n, aptr, col, perm);
+  for (int i = 0; i < n; ++i) invperm[perm[i]] = i;
+  for (int i = 0; i < n; ++i)
+    for (int q = aptr[i]; q < aptr[i+1]; q++)
+    {
+      int j = col[q], newi = invperm[i], newj = invperm[j];
+      if (val[q] != 0.0)
+      {
+        if (newi > newj) { if (ptr[newi] < newi - newj) ptr[newi] = newi - newj; }
+        else if (newi < newj) { if (ptr[newj] < newj - newi) ptr[newj] = newj - newi; }
+      }
+    }
+  {
+    int last = 0;
+    for (int i = 1; i <= n; ++i) { int tmp = ptr[i]; ptr[i] = ptr[i-1] + last; last = tmp; }
+  }
+  std::vector<double> L (ptr.back (), 0.0), U (ptr.back (), 0.0), D (n, 0.0), y (n);
+  for (int i = 0; i < n; ++i)
+    for (int q = aptr[i]; q < aptr[i+1]; q++)
+    {
+      int j = col[q], newi = invperm[i], newj = invperm[j];
+      double v = val[q];
+      if (v != 0.0)
+      {
+        if (newi < newj) U[ptr[newj + 1] + newi - newj] = v;
+        else if (newi == newj) D[newi] = v;
+        else L[ptr[newi + 1] + newj - newi] = v;
+      }
+    }
+  if (D[0] == 0.0) return 1;
+  D[0] = 1.0 / D[0];
+  for (int k = 0; k < n - 1; ++k)
+  {
+    if (ptr[k + 1] + k + 1 == ptr[k + 2]) U[ptr[k+1]] = D[0] * U[ptr[k+1]];
+    int indexEntry = ptr[k + 1];
+    int iBeginCol = k + 1 - ptr[k + 2] + ptr[k + 1];
+    for (int i = iBeginCol; i <= k; ++indexEntry, ++i)
+    {
+      if (i == 0) continue;
+      double sum = U[indexEntry];
+      int jBeginRow = i - ptr[i + 1] + ptr[i];
+      int jBeginMult = std::max (iBeginCol, jBeginRow);
+      int indexL = ptr[i] + jBeginMult - jBeginRow;
+      int indexU = ptr[k+1] + jBeginMult - iBeginCol;
+      for (int j = jBeginMult; j < i; ++j, ++indexL, ++indexU) sum -= L[indexL] * U[indexU];
+      U[indexEntry] = D[i] * sum;
+    }
+    indexEntry = ptr[k+1];
+    int jBeginRow = k + 1 - ptr[k + 2] + ptr[k + 1];
+    for (int i = iBeginCol; i <= k; ++indexEntry, ++i)
+    {
+      if (i == 0) continue;
+      double sum = L[indexEntry];
+      int jBeginCol = i - ptr[i+1] + ptr[i];
+      int jBeginMult = std::max (jBeginCol, jBeginRow);
+      int indexL = ptr[k+1] + jBeginMult - jBeginRow;
+      int indexU = ptr[i] + jBeginMult - jBeginCol;
+      for (int j = jBeginMult; j < i; ++j, ++indexL, ++indexU) sum -= L[indexL] * U[indexU];
+      L[indexEntry] = sum;
+    }
+    double sum = D[k+1];
+    for (int j = ptr[k+1]; j < ptr[k+2]; ++j) sum -= L[j] * U[j];
+    if (sum == 0.0) return 1;
+    D[k+1] = 1.0 / sum;
+  }
+  for (int i = 0; i < n; ++i)
+  {
+    double sum = rhs[perm[i]];
+    for (int k = ptr[i], j = i - ptr[i+1] + k; k < ptr[i+1]; ++k, ++j) sum -= L[k] * y[j];
+    y[i] = D[i] * sum;
+  }
+  for (int j = n - 1; j >= 0; --j)
+    for (int k = ptr[j], i = j - ptr[j+1] + k; k < ptr[j+1]; ++k, ++i) y[i] -= U[k] * y[j];
+  x.assign (n, 0.0);
+  for (int i = 0; i < n; ++i) x[perm[i]] = y[i];
+  return 0;
+}
+
+/* the recursive jmark of joints.cpp:67-80 */
+void jmark (std::vector<std::set<int>> &jadj, std::set<int> &jall, int joint, std::vector<int> &iset)
+{
+  if (jall.find (joint) != jall.end ())
+  {
+    jall.erase (joint);
+    iset.push_back (joint);
+    for (int j : jadj[joint]) jmark (jadj, jall, j, iset);
+  }
+}
+
+/* Rot * skew (X - P) */
+void joint_h (const orc_ctx &s, int part, const double Rot[9], const double P[3], double H[9])
+{
+  size_t N = s.parnum;
+  double A[3], Ask[9];
+  for (int c = 0; c < 3; c++) A[c] = s.position[(3+c)*N + part] - P[c];
+  Ask[0] = 0; Ask[1] = A[2]; Ask[2] = -A[1];
+  Ask[3] = -A[2]; Ask[4] = 0; Ask[5] = A[0];
+  Ask[6] = A[1]; Ask[7] = -A[0]; Ask[8] = 0;
+  nnmul (Rot, Ask, H);
+}
+
+/* reset_joints_symbolic and solve_joints, joints.cpp:83-567; returns 1 on a
+ * zero pivot */
+int solve_joints (orc_ctx &s, const double damping[6], double step0, double step1)
+{
+  const int J = s.jnum;
+  if (J == 0) return 0;
+  const size_t N = s.parnum, JJ = J;
+  const int *jp0 = s.jpart.data (), *jp1 = s.jpart.data () + J;
+  auto JP = [&] (int k, int i) { return k ? jp1[i] : jp0[i]; };
+  std::map<int, std::set<int>> partadj;
+  for (int i = 0; i < J; i++) { partadj[jp0[i]].insert (i); if (jp1[i] >= 0) partadj[jp1[i]].insert (i); }
+  std::vector<std::set<int>> jadj (J);
+  for (auto &pa : partadj)
+    for (int a : pa.second) for (int b : pa.second) if (a != b) jadj[a].insert (b);
+  std::vector<int> jmap (J);
+  std::vector<std::vector<int>> isets;
+  std::set<int> jall;
+  for (int i = 0; i < J; i++) jall.insert (i);
+  while (!jall.empty ())
+  {
+    std::vector<int> iset;
+    jmark (jadj, jall, *jall.begin (), iset);
+    for (size_t i = 0; i < iset.size (); i++) jmap[iset[i]] = (int) i;
+    isets.push_back (iset);
+  }
+  const double half = 0.5*step0, step = 0.5*(step0+step1);
+  for (auto &iset : isets)
+  {
+    int n = 3 * (int) iset.size ();
+    /* CSR of the dense pattern, columns ascending */
+    std::vector<int> ptr (n + 1), col ((size_t) n * n);
+    for (int i = 0; i <= n; i++) ptr[i] = i * n;
+    for (int i = 0; i < n; i++) for (int j = 0; j < n; j++) col[(size_t) i*n + j] = j;
+    std::vector<double> val ((size_t) n * n, 0.0);
+    auto V = [&] (int r, int c) -> double & { return val[(size_t) r * n + c]; };
+    for (size_t i0 = 0; i0 < iset.size (); i0++)
+    {
+      int i = iset[i0];
+      double Wii[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Pi[3] = {s.jpoint[i], s.jpoint[JJ + i], s.jpoint[2*JJ + i]};
+      for (int k = 0; k < 2; k++)
+      {
+        int part = JP (k, i);
+        if (part < 0) continue;
+        double Rot[9], Jiv[9], Hi[9], Hj[9], C[9], D[9], Wij[9];
+        for (int c = 0; c < 9; c++) { Rot[c] = s.rotation[c*N + part]; Jiv[c] = s.inverse[c*N + part]; }
+        double im = s.invm[part];
+        joint_h (s, part, Rot, Pi, Hi);
+        ntmul (Jiv, Hi, C);
+        nnmul (Hi, C, D);
+        for (int c = 0; c < 9; c++) Wii[c] = Wii[c] + D[c];
+        Wii[0] += im; Wii[4] += im; Wii[8] += im;
+        for (int j : partadj[part])
+        {
+          if (i == j) continue;
+          int j0 = jmap[j];
+          double Pj[3] = {s.jpoint[j], s.jpoint[JJ + j], s.jpoint[2*JJ + j]};
+          joint_h (s, part, Rot, Pj, Hj);
+          ntmul (Jiv, Hj, C);
+          nnmul (Hi, C, Wij);
+          Wij[0] += im; Wij[4] += im; Wij[8] += im;
+          if ((part == jp0[i] && part == jp1[j]) || (part == jp1[i] && part == jp0[j]))
+            for (int c = 0; c < 9; c++) Wij[c] *= -1.0;
+          for (int i1 = 0; i1 < 3; i1++)
+            for (int j1 = 0; j1 < 3; j1++) V (3*(int) i0 + i1, 3*j0 + j1) += Wij[i1*3+j1];
+        }
+      }
+      for (int i1 = 0; i1 < 3; i1++)
+        for (int j1 = 0; j1 < 3; j1++) V (3*(int) i0 + i1, 3*(int) i0 + j1) += Wii[i1*3+j1];
+    }
+    std::vector<double> rhs (n, 0.0), x;
+    for (size_t i0 = 0; i0 < iset.size (); i0++)
+    {
+      int i = iset[i0];
+      double *B = &rhs[3*i0], Pi[3] = {s.jpoint[i], s.jpoint[JJ + i], s.jpoint[2*JJ + i]};
+      for (int k = 0; k < 2; k++)
+      {
+        int part = JP (k, i);
+        if (part < 0) continue;
+        double Rot[9], Jiv[9], Jm[9], O[3], o[3], v[3], f[3], t[3], A[3], Vv[3], C[3], T[3], dRot[9], Hi[9];
+        for (int c = 0; c < 9; c++)
+        {
+          Rot[c] = s.rotation[c*N + part]; Jiv[c] = s.inverse[c*N + part]; Jm[c] = s.inertia[c*N + part];
+        }
+        for (int c = 0; c < 3; c++)
+        {
+          O[c] = s.angular[c*N + part]; f[c] = s.force[c*N + part]; t[c] = s.torque[c*N + part]; v[c] = s.linear[c*N + part];
+        }
+        double im = s.invm[part];
+        joint_h (s, part, Rot, Pi, Hi);
+        nvmul (Hi, O, Vv);
+        for (int c = 0; c < 3; c++) Vv[c] += v[c];
+        if (damping[0] != 0.0 || damping[1] != 0.0 || damping[2] != 0.0)
+        {
+          double ma = s.mass[i];   /* joints.cpp:399 reads mass[joint] */
+          for (int c = 0; c < 3; c++) f[c] -= ma * damping[c] * v[c];
+        }
+        if (damping[3] != 0.0 || damping[4] != 0.0 || damping[5] != 0.0)
+        {
+          for (int c = 0; c < 3; c++) o[c] = damping[3+c]*s.angular[(3+c)*N + part];
+          tvmul (Rot, o, C);
+          nvmul (Jm, C, T);
+          nvmul (Rot, T, C);
+          for (int c = 0; c < 3; c++) t[c] -= C[c];
+        }
+        expmap (-half*O[0], -half*O[1], -half*O[2], dRot);
+        tvmul (Rot, t, T);
+        nvmul (Jm, O, A);
+        nvmul (dRot, A, C);
+        for (int c = 0; c < 3; c++) C[c] = C[c] + half*T[c];
+        nvmul (Jiv, C, A);
+        nvmul (Jm, A, C);
+        T[0] -= A[1]*C[2] - A[2]*C[1];
+        T[1] -= A[2]*C[0] - A[0]*C[2];
+        T[2] -= A[0]*C[1] - A[1]*C[0];
+        for (int c = 0; c < 3; c++) T[c] *= step;
+        double Ot[3];
+        Ot[0] = O[0] + (Jiv[0]*T[0]+Jiv[3]*T[1]+Jiv[6]*T[2]);
+        Ot[1] = O[1] + (Jiv[1]*T[0]+Jiv[4]*T[1]+Jiv[7]*T[2]);
+        Ot[2] = O[2] + (Jiv[2]*T[0]+Jiv[5]*T[1]+Jiv[8]*T[2]);
+        nvmul (Hi, Ot, C);
+        for (int c = 0; c < 3; c++) C[c] += v[c];
+        for (int c = 0; c < 3; c++) C[c] = C[c] + (step*im)*f[c];
+        if (part == jp0[i]) for (int c = 0; c < 3; c++) B[c] -= Vv[c]+C[c];
+        else for (int c = 0; c < 3; c++) B[c] += Vv[c]+C[c];
+      }
+    }
+    if (skyline_lu (n, ptr, col, val, rhs, x)) return 1;
+    double invstep = 1./step;
+    for (size_t i0 = 0; i0 < iset.size (); i0++)
+    {
+      int i = iset[i0];
+      double R[3] = {x[3*i0] * invstep, x[3*i0+1] * invstep, x[3*i0+2] * invstep};
+      for (int c = 0; c < 3; c++) s.jreac[c*JJ + i] = R[c];
+      for (int k = 0; k < 2; k++)
+      {
+        int part = JP (k, i);
+        if (part < 0) continue;
+        double Rot[9], A[3], a[3], t[3];
+        for (int c = 0; c < 9; c++) Rot[c] = s.rotation[c*N + part];
+        for (int c = 0; c < 3; c++) A[c] = s.position[(3+c)*N + part] - s.jpoint[c*JJ + i];
+        nvmul (Rot, A, a);
+        t[0] = R[1]*a[2] - R[2]*a[1];
+        t[1] = R[2]*a[0] - R[0]*a[2];
+        t[2] = R[0]*a[1] - R[1]*a[0];
+        if (part == jp0[i]) for (int c = 0; c < 3; c++) { s.torque[c*N + part] += t[c]; s.force[c*N + part] += R[c]; }
+        else for (int c = 0; c < 3; c++) { s.torque[c*N + part] -= t[c]; s.force[c*N + part] -= R[c]; }
+      }
+    }
   }
   return 0;
 }
@@ -1823,8 +2531,6 @@ int orc_upload (orc_ctx *s, const pmx_model *m)
       return fail (PMX_EINVAL, "obstacle triangle range out of bounds");
   for (int i = 0; i < m->sprnum; i++)
   {
-    if (m->sproffset && m->sproffset[i] != -1) return fail (PMX_EUNSUPPORTED, "spring stroke offsets are not supported");
-    if (m->unspring && m->unspring[i] >= 0) return fail (PMX_EUNSUPPORTED, "UNSPRING load curves are not supported");
     if (i && (m->sprpart[i-1] > m->sprpart[i] ||
               (m->sprpart[i-1] == m->sprpart[i] && m->sprpart[m->sprnum + i-1] > m->sprpart[m->sprnum + i])))
       return fail (PMX_EINVAL, "springs must be sorted by (part1, part2)");
@@ -1896,6 +2602,50 @@ int orc_upload (orc_ctx *s, const pmx_model *m)
   take (s->sprfrc, m->sprfrc, 3*S);
   take (s->unspring, m->unspring, S);
   take (s->sproffset, m->sproffset, S);
+  if (!m->sproffset) s->sproffset.assign (S, -1);
+  s->lcnum = m->lcnum; s->lccrv = m->lccrv;
+  take (s->lcidx, m->lcidx, m->lcnum > 0 ? (size_t) m->lcnum + 1 : 0);
+  take (s->lcurve, m->lcurve, 2*(size_t) m->lccrv);
+  size_t T = m->trqsprnum;
+  s->trqsprnum = m->trqsprnum; s->krpycrv = m->krpycrv; s->drpycrv = m->drpycrv;
+  take (s->trqsprpart, m->trqsprpart, 2*T);
+  take (s->trqzdir0, m->trqzdir0, 3*T);
+  take (s->trqxdir0, m->trqxdir0, 3*T);
+  take (s->krpyidx, m->krpyidx, T ? 3*(T+1) : 0);
+  take (s->krpy, m->krpy, 2*(size_t) m->krpycrv);
+  take (s->drpyidx, m->drpyidx, T ? 3*(T+1) : 0);
+  take (s->drpy, m->drpy, 2*(size_t) m->drpycrv);
+  take (s->trqcone, m->trqcone, T);
+  take (s->trqzdir1, m->trqzdir1, 3*T);
+  take (s->trqxdir1, m->trqxdir1, 3*T);
+  take (s->trqrpy, m->trqrpy, 3*T);
+  take (s->trqrpytot, m->trqrpytot, 3*T);
+  take (s->trqrpyspr, m->trqrpyspr, 3*T);
+  size_t U = m->unsprnum;
+  s->unsprnum = m->unsprnum;
+  take (s->tspridx, m->tspridx, U ? U+1 : 0);
+  take (s->tsprings, m->tsprings, U ? (size_t) m->tspridx[U] : 0);
+  take (s->mspridx, m->mspridx, U ? U+1 : 0);
+  take (s->msprings, m->msprings, U ? (size_t) m->mspridx[U] : 0);
+  take (s->actidx, m->actidx, U ? U+1 : 0);
+  take (s->activate, m->activate, U ? (size_t) m->actidx[U] : 0);
+  take (s->unlim, m->unlim, 2*U);
+  take (s->unent, m->unent, U);
+  take (s->unop, m->unop, U);
+  take (s->unabs, m->unabs, U);
+  take (s->nsteps, m->nsteps, U);
+  take (s->nfreq, m->nfreq, U);
+  take (s->unaction, m->unaction, U);
+  size_t R = m->rstnum;
+  s->rstnum = m->rstnum;
+  take (s->rstpart, m->rstpart, R);
+  take (s->rstlin, m->rstlin, 9*R);
+  take (s->rstang, m->rstang, 9*R);
+  size_t JN = m->jnum;
+  s->jnum = m->jnum;
+  take (s->jpart, m->jpart, 2*JN);
+  take (s->jpoint, m->jpoint, 3*JN);
+  take (s->jreac, m->jreac, 3*JN);
   s->curtime = m->curtime; s->curstep = m->curstep; s->stepnum = m->stepnum;
   if (!same)
   {
@@ -1919,6 +2669,10 @@ int orc_download (orc_ctx *s, pmx_model *m)
   put (m->sprpnt, s->sprpnt); put (m->sprdir, s->sprdir); put (m->sprsdsp, s->sprsdsp); put (m->stroke, s->stroke);
   put (m->sprfrc, s->sprfrc); put (m->yield, s->yield);
   if (m->sprflg) std::memcpy (m->sprflg, s->sprflg.data (), s->sprflg.size () * sizeof (int));
+  if (m->unspring) std::memcpy (m->unspring, s->unspring.data (), s->unspring.size () * sizeof (int));
+  put (m->trqzdir1, s->trqzdir1); put (m->trqxdir1, s->trqxdir1); put (m->trqrpy, s->trqrpy);
+  put (m->trqrpytot, s->trqrpytot); put (m->trqrpyspr, s->trqrpyspr); put (m->jreac, s->jreac);
+  if (m->nfreq && s->unsprnum) std::memcpy (m->nfreq, s->nfreq.data (), s->nfreq.size () * sizeof (int));
   m->curtime = s->curtime; m->curstep = s->curstep; m->stepnum = s->stepnum;
   return PMX_OK;
 }
@@ -1949,6 +2703,7 @@ static int dem_run (orc_ctx *s, double duration, double step, double adaptive, c
   else step0 = s->curstep;
   step1 = step0;
   invert_inertia (*s);
+  restrain_velocities (*s);
   Tree T;
   tree_create (s->ellnum, s->cen(0), s->cen(1), s->cen(2), T);
   long long k = 0, rebuilds = 1;
@@ -1999,8 +2754,13 @@ static int dem_run (orc_ctx *s, double duration, double step, double adaptive, c
     }
     if (forces (*s, step0, g)) return fail (PMX_EUNSUPPORTED, "unsupported contact kind");
     if (s->sprnum && springs (*s, step0)) return fail (PMX_EUNSUPPORTED, "unsupported spring kind");
+    if (s->trqsprnum) torsion_springs (*s, step0);
+    if (s->unsprnum) unspring_test (*s);
     step1 = adapt ? adaptive_timestep (*s, step0, adaptive) : step0;
     if (!(step1 > 0.0)) return fail (PMX_EINVAL, "adaptive step is not positive");
+    if (solve_joints (*s, damp, step0, step1))
+      return fail (PMX_EINVAL, "zero pivot in the ball joint system (skyline LU, as amgcl's precondition)");
+    restrain_forces (*s);
     prescribe_acceleration (*s, prow);
     dynamics (*s, damp, step0, step1);
     prescribe_velocity (*s, prow);
@@ -2102,6 +2862,24 @@ int orc_get_stats (orc_ctx *s, pmx_stats *st)
 }
 
 void orc_expmap (const double omega[3], double lambda[9]) { expmap (omega[0], omega[1], omega[2], lambda); }
+
+int orc_skyline_lu (int n, const double *A, const double *b, double *x)
+{
+  std::vector<int> ptr (n + 1), col ((size_t) n * n);
+  for (int i = 0; i <= n; i++) ptr[i] = i * n;
+  for (int i = 0; i < n; i++) for (int j = 0; j < n; j++) col[(size_t) i*n + j] = j;
+  std::vector<double> val (A, A + (size_t) n * n), rhs (b, b + n), y;
+  if (skyline_lu (n, ptr, col, val, rhs, y)) return 1;
+  std::copy (y.begin (), y.end (), x);
+  return 0;
+}
+
+void orc_cuthill_mckee (int n, const int *ptr, const int *col, int *perm)
+{
+  std::vector<int> p (ptr, ptr + n + 1), c (col, col + ptr[n]), out;
+  cuthill_mckee (n, p, c, out);
+  std::copy (out.begin (), out.end (), perm);
+}
 
 double orc_triangle_sphere (const double a[3], const double b[3], const double c[3],
                             const double p[3], double r, double point[3], double normal[3])

@@ -61,6 +61,11 @@ long long orc_tree_reach (int n, const double *center, int nbox, const double *b
                           long long cap);
 /* Morton code of one point given the extents (partition.ispc:69-94) */
 unsigned int orc_morton (const double p[3], const double extents[6]);
+/* amgcl's skyline_lu (Cuthill-McKee order, Crout on the non-zero profile)
+ * of a dense n x n row-major matrix: x = A^-1 b; returns 1 on a zero pivot */
+int orc_skyline_lu (int n, const double *A, const double *b, double *x);
+/* amgcl's Cuthill-McKee order of a CSR pattern */
+void orc_cuthill_mckee (int n, const int *ptr, const int *col, int *perm);
 
 #ifdef __cplusplus
 }

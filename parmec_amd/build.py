@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpmx.so")
 SOURCES = ["pmx_api.hip", "pmx_batch.hip", "pmx_serve.hip", "pmx_tree.hip", "pmx_step.hip", "pmx_spring.hip",
-           "pmx_history.hip", "pmx_critical.hip"]
+           "pmx_history.hip", "pmx_critical.hip", "pmx_joint.hip"]
 HEADERS = ["pmx_internal.h", "pmx_device.h", os.path.join("..", "..", "include", "pmx.h")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

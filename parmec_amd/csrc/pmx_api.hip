@@ -74,6 +74,7 @@ int validate_model (const pmx_model *m)
     return fail (PMX_EINVAL, "invalid counts (pairnum must include the default pairing)");
   if (m->obsnum < 0) return fail (PMX_EINVAL, "invalid obstacle count");
   if (m->prsnum < 0) return fail (PMX_EINVAL, "invalid prescribed-motion count");
+  if (int rc = jt_validate (m)) return rc;
   for (int i = 0; i < m->prsnum; i++)
   {
     if (m->prspart[i] < 0 || m->prspart[i] >= m->parnum) return fail (PMX_EINVAL, "prescribed particle out of range");
@@ -97,8 +98,6 @@ int validate_model (const pmx_model *m)
   }
   for (int i = 0; i < m->sprnum; i++)
   {
-    if (m->sproffset && m->sproffset[i] != -1) return fail (PMX_EUNSUPPORTED, "spring stroke offsets are not supported");
-    if (m->unspring[i] >= 0) return fail (PMX_EUNSUPPORTED, "UNSPRING load curves are not supported");
     if (m->sprpart[i] < 0 || m->sprpart[i] >= m->parnum || m->sprpart[m->sprnum + i] < -1 ||
         m->sprpart[m->sprnum + i] >= m->parnum) return fail (PMX_EINVAL, "spring particle index out of range");
     if (i && (m->sprpart[i-1] > m->sprpart[i] ||
@@ -162,8 +161,9 @@ int upload_core (pmx_ctx *c, const pmx_model *m, const std::vector<InstOff> &off
   for (int i = 0; i < m->parnum; i++) if (parell[i] < 0) nosph.push_back (i);
   std::vector<signed char> iso (std::max (m->parnum, 1), 0);
   int niso = 0, nzc = 0, nfree = 0;
-  /* particles whose force another stage adds to or sets (springs, prescribed
-   * motion): their force always goes through the force array */
+  /* particles whose force another stage reads, adds to or sets (springs,
+   * torsion springs, prescribed motion, restraints, ball joints): their force
+   * always goes through the force array */
   std::vector<char> engaged (std::max (m->parnum, 1), 0);
   for (int q = 0; q < m->sprnum; q++)
   {
@@ -171,6 +171,17 @@ int upload_core (pmx_ctx *c, const pmx_model *m, const std::vector<InstOff> &off
     if (m->sprpart[m->sprnum + q] >= 0) engaged[m->sprpart[m->sprnum + q]] = 1;
   }
   for (int q = 0; q < m->prsnum; q++) engaged[m->prspart[q]] = 1;
+  for (int q = 0; q < m->trqsprnum; q++)
+  {
+    engaged[m->trqsprpart[q]] = 1;
+    if (m->trqsprpart[m->trqsprnum + q] >= 0) engaged[m->trqsprpart[m->trqsprnum + q]] = 1;
+  }
+  for (int q = 0; q < m->rstnum; q++) engaged[m->rstpart[q]] = 1;
+  for (int q = 0; q < m->jnum; q++)
+  {
+    engaged[m->jpart[q]] = 1;
+    if (m->jpart[m->jnum + q] >= 0) engaged[m->jpart[m->jnum + q]] = 1;
+  }
   for (int i = 0; i < m->parnum; i++)
   {
     const double *J = m->inertia;
@@ -324,7 +335,7 @@ int upload_core (pmx_ctx *c, const pmx_model *m, const std::vector<InstOff> &off
       (rc = up (*c, c->nodpart, m->nodpart, (size_t) c->NOD, "nodpart")) ||
       (rc = up (*c, c->facnod, m->facnod, 3*(size_t) c->FAC, "facnod")) ||
       (rc = up (*c, c->factri, m->factri, (size_t) c->FAC, "factri")) ||
-      (rc = springs_setup (*c, m)))
+      (rc = springs_setup (*c, m)) || (rc = jt_setup (*c, m)))
     return rc;
   c->curtime = m->curtime; c->curstep = m->curstep; c->stepnum = m->stepnum;
   c->icurtime.assign (I, m->curtime);
@@ -496,7 +507,8 @@ int pmx_download (pmx_ctx *c, pmx_model *m)
       (rc = down (*c, m->inverse, c->inverse, 9*N)) || (rc = down (*c, m->invm, c->invm, N)) ||
       (rc = down (*c, m->force, c->force, 3*N)) || (rc = down (*c, m->torque, c->torque, 3*N)) ||
       (rc = down (*c, m->center, c->center, 6*E)) || (rc = down (*c, m->tri, c->tri, 9*T)) ||
-      (rc = down (*c, m->nodes, c->nodes, 6*(size_t) c->NOD)) || (rc = springs_download (*c, m)))
+      (rc = down (*c, m->nodes, c->nodes, 6*(size_t) c->NOD)) || (rc = springs_download (*c, m)) ||
+      (rc = jt_download (*c, m)))
     return rc;
   if (m->obsnum == c->O && c->O > 0 &&
       ((rc = down (*c, m->obspnt, c->obspnt, 3*(size_t) c->O)) || (rc = down (*c, m->obsang, c->obsang, 3*(size_t) c->O)) ||
@@ -531,6 +543,8 @@ static int check_flags (pmx_ctx *c, const int *fl)
   if (fl[FLAG_TRILIST_OVERFLOW])
     return fail (PMX_ECAPACITY, "partitioning tree deeper than the traversal stack (64 pending nodes)");
   if (fl[FLAG_BAD_KIND]) return fail (PMX_EUNSUPPORTED, "unsupported interaction kind or inconsistent contact rows");
+  if (fl[FLAG_JOINT_PIVOT])
+    return fail (PMX_EINVAL, "zero pivot in the ball joint system (skyline LU, as amgcl's precondition)");
   c->stats.max_row = std::max (c->stats.max_row, fl[FLAG_MAXROW]);
   return PMX_OK;
 }
@@ -809,7 +823,8 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
   }
   else step0 = c->icurstep[0];
   step1 = step0;
-  if ((rc = step_invert (*c))) return rc;
+  /* invert_inertia, then restrain_velocities (parmec.cpp:2409-2411) */
+  if ((rc = step_invert (*c)) || (rc = step_restrain_vel (*c))) return rc;
 
   /* steps of each instance (the reference's accumulation of time by (step0+step1)/2, parmec.cpp:2418) */
   std::vector<int> nst (I, 0);
@@ -821,6 +836,15 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
   }
   for (int i = 0; i < I; i++) c->staggered = c->staggered || nst[i] != nsteps;
   if (c->staggered && (rc = up (*c, c->istop, nst.data (), I, "instance steps"))) return rc;
+  if (c->sp_offsets)
+  {
+    /* the curtime of each step, where spring stroke offsets read their load
+     * curves (one model: offsets are refused in batches) */
+    std::vector<double> tt (std::max (nsteps, 1), 0.0);
+    int n = 0;
+    pmx_step_times (durations ? durations[0] : duration, step0, curtime, tt.data (), nsteps, &n);
+    if ((rc = step_times_table (*c, tt.data (), nsteps))) return rc;
+  }
 
   /* per-step tables of the host-evaluated inputs (input.cpp:4424-4503),
    * laid out [step][instance][component]; rows past an instance's own steps
@@ -1189,7 +1213,7 @@ static int adaptive_core (pmx_ctx *c, double duration, double step, double adapt
   }
   else step0 = c->icurstep[0];
   step1 = step0;
-  if ((rc = step_invert (*c))) return rc;
+  if ((rc = step_invert (*c)) || (rc = step_restrain_vel (*c))) return rc;
   if (c->adp.alloc (11 * (size_t) std::max (c->N, 1)) || (c->S > 0 && c->sp_adp.alloc (17 * (size_t) c->S)) ||
       c->hmin.alloc (1))
     return fail (PMX_ENOMEM, "out of device memory (adaptive step)");
@@ -1208,7 +1232,7 @@ static int adaptive_core (pmx_ctx *c, double duration, double step, double adapt
 
   /* per-step tables, grown as steps are added; the host keeps every row */
   const size_t O = c->O, P = c->PRSC;
-  std::vector<double> gh, dh, oh, ph;
+  std::vector<double> gh, dh, oh, ph, th;
   std::vector<int> hh;
   int tcap = 0;
   auto ensure = [&] (int k) -> int {
@@ -1216,13 +1240,14 @@ static int adaptive_core (pmx_ctx *c, double duration, double step, double adapt
     int ncap = std::max (64, 2 * tcap);
     while (ncap <= k) ncap *= 2;
     gh.resize (3 * (size_t) ncap, 0.0); dh.resize (6 * (size_t) ncap, 0.0);
-    oh.resize (6 * O * ncap, 0.0); ph.resize (6 * P * ncap, 0.0); hh.resize (ncap, -1);
+    oh.resize (6 * O * ncap, 0.0); ph.resize (6 * P * ncap, 0.0); hh.resize (ncap, -1); th.resize (ncap, 0.0);
     int r;
     if ((r = up (*c, c->gtab, gh.data (), gh.size (), "gravity table")) ||
         (r = up (*c, c->dtab, dh.data (), dh.size (), "damping table")) ||
         (O > 0 && (r = up (*c, c->obstab, oh.data (), oh.size (), "obstacle table"))) ||
         (P > 0 && (r = up (*c, c->prstab, ph.data (), ph.size (), "prescribed table"))) ||
-        (c->his_active && (r = up (*c, c->hisrow, hh.data (), hh.size (), "history rows"))))
+        (c->his_active && (r = up (*c, c->hisrow, hh.data (), hh.size (), "history rows"))) ||
+        (c->sp_offsets && (r = step_times_table (*c, th.data (), (int) th.size ()))))
       return r;
     tcap = ncap;
     return PMX_OK;
@@ -1279,6 +1304,11 @@ static int adaptive_core (pmx_ctx *c, double duration, double step, double adapt
       hh[k] = in.hisrow;
       PMX_HIP (hipMemcpyAsync (c->hisrow.p + k, &hh[k], sizeof (int), hipMemcpyHostToDevice, c->stream));
     }
+    if (c->sp_offsets)
+    {
+      th[k] = curtime;
+      PMX_HIP (hipMemcpyAsync (c->ttab.p + k, &th[k], sizeof (double), hipMemcpyHostToDevice, c->stream));
+    }
     int ctl[CTL_COUNT], fl[FLAG_COUNT];
     for (;;)
     {
@@ -1331,6 +1361,14 @@ static int adaptive_core (pmx_ctx *c, double duration, double step, double adapt
     }
   }
   const int nsteps = k;
+  {
+    /* a zero joint pivot in the last step's motion */
+    int fl[FLAG_COUNT];
+    PMX_HIP (hipMemcpyAsync (c->hring, c->dflags.p, sizeof (int) * FLAG_COUNT, hipMemcpyDeviceToHost, c->stream));
+    PMX_HIP (hipStreamSynchronize (c->stream));
+    std::memcpy (fl, c->hring, sizeof (fl));
+    if ((rc = check_flags (c, fl))) return rc;
+  }
   if (nsteps > 0 && (rc = step_materialize (*c, std::vector<int> (1, nsteps - 1)))) return rc;
   if ((rc = step_mesh_shapes (*c, false))) return rc;
   if (c->his_active)

@@ -116,6 +116,11 @@ int merge_models (const pmx_model *ms, int count, Merged &g, std::vector<InstOff
     if (rc) return fail (rc, "instance " + std::to_string (i) + ": " + pmx_last_error ());
     if (ms[i].curtime != ms[0].curtime || ms[i].curstep != ms[0].curstep || ms[i].stepnum != ms[0].stepnum)
       return fail (PMX_EINVAL, "instances of a batch must share curtime, curstep and stepnum");
+    bool offsets = false;
+    for (int q = 0; ms[i].sproffset && q < ms[i].sprnum; q++) offsets = offsets || ms[i].sproffset[q] >= 0;
+    if (ms[i].trqsprnum || ms[i].unsprnum || ms[i].rstnum || ms[i].jnum || ms[i].lcnum || offsets)
+      return fail (PMX_EUNSUPPORTED, "torsion springs, UNSPRING, RESTRAIN, BALL_JOINT and load curves run in "
+                                     "single-model contexts (pmx_upload)");
   }
   off.assign (count + 1, InstOff ());
   for (int i = 0; i < count; i++)

@@ -34,11 +34,19 @@ inline dim3 grid (int n) { return dim3 ((n + PMX_TPB - 1) / PMX_TPB); }
 
 __device__ __forceinline__ double len3 (double a, double b, double c) { return sqrt (a*a + b*b + c*c); }
 
+/* the state of the torsion springs, UNSPRING and joints a history reads */
+struct HisMore
+{
+  int T, J;
+  const int *unspring;
+  const double *tz1, *tx1, *rpy, *tot, *spr, *jreac;
+};
+
 /* one history's value (entity e over list[o0..o1)) from the state arrays */
 __device__ double his_value (int e, const int *list, int o0, int o1, bool point, const double *pt, int N, int S,
                              const double *position, const double *linear, const double *angular,
                              const double *rotation, const double *force, const double *torque, const double *spnt,
-                             const double *stroke, const double *sfrc)
+                             const double *stroke, const double *sfrc, const HisMore &M)
 {
   if (point)
   {
@@ -129,6 +137,28 @@ __device__ double his_value (int e, const int *list, int o0, int o1, bool point,
       case PMX_HIS_F: value += sfrc[k]; break;
       case PMX_HIS_SF: value += sfrc[(size_t) S + k]; break;
       case PMX_HIS_FF: value += sfrc[2*(size_t) S + k]; break;
+      case PMX_HIS_SS: value += (double) M.unspring[k]; break;
+      /* torsion springs, component c of spring k at c T + k */
+      case PMX_HIS_ZDIR_X: value += M.tz1[k]; break;
+      case PMX_HIS_ZDIR_Y: value += M.tz1[(size_t) M.T + k]; break;
+      case PMX_HIS_ZDIR_Z: value += M.tz1[2*(size_t) M.T + k]; break;
+      case PMX_HIS_XDIR_X: value += M.tx1[k]; break;
+      case PMX_HIS_XDIR_Y: value += M.tx1[(size_t) M.T + k]; break;
+      case PMX_HIS_XDIR_Z: value += M.tx1[2*(size_t) M.T + k]; break;
+      case PMX_HIS_ROLL: value += M.rpy[k]; break;
+      case PMX_HIS_PITCH: value += M.rpy[(size_t) M.T + k]; break;
+      case PMX_HIS_YAW: value += M.rpy[2*(size_t) M.T + k]; break;
+      case PMX_HIS_TRQTOT_R: value += M.tot[k]; break;
+      case PMX_HIS_TRQTOT_P: value += M.tot[(size_t) M.T + k]; break;
+      case PMX_HIS_TRQTOT_Y: value += M.tot[2*(size_t) M.T + k]; break;
+      case PMX_HIS_TRQSPR_R: value += M.spr[k]; break;
+      case PMX_HIS_TRQSPR_P: value += M.spr[(size_t) M.T + k]; break;
+      case PMX_HIS_TRQSPR_Y: value += M.spr[2*(size_t) M.T + k]; break;
+      /* joint reactions, joint k in creation order */
+      case PMX_HIS_JREAC_X: value += M.jreac[k]; break;
+      case PMX_HIS_JREAC_Y: value += M.jreac[(size_t) M.J + k]; break;
+      case PMX_HIS_JREAC_Z: value += M.jreac[2*(size_t) M.J + k]; break;
+      case PMX_HIS_JREAC_L: value += len3 (M.jreac[k], M.jreac[(size_t) M.J + k], M.jreac[2*(size_t) M.J + k]); break;
       default: break;
     }
   }
@@ -141,7 +171,7 @@ __global__ void k_history (const int *ctl, const int *hisrow, int row0, int nhis
                            const int *list, const signed char *haspt, const double *pts, int N, int S,
                            const double *position, const double *linear, const double *angular,
                            const double *rotation, const double *force, const double *torque, const double *spnt,
-                           const double *stroke, const double *sfrc, double *out)
+                           const double *stroke, const double *sfrc, HisMore M, double *out)
 {
   int row = row0;
   if (hisrow)
@@ -154,7 +184,7 @@ __global__ void k_history (const int *ctl, const int *hisrow, int row0, int nhis
   int e = ent[h];
   double v = e == PMX_HIS_TIME ? 0.0
            : his_value (e, list, off[h], off[h+1], haspt[h] != 0, pts + 3*(size_t) h, N, S, position, linear,
-                        angular, rotation, force, torque, spnt, stroke, sfrc);
+                        angular, rotation, force, torque, spnt, stroke, sfrc, M);
   out[(size_t) row * nhis + h] = v;
 }
 
@@ -170,12 +200,16 @@ int fail (int code, const std::string &msg) { pmx_set_error (msg); return code; 
 
 void launch_history (pmx_ctx &c, const int *hisrow, int row0, double *out)
 {
+  HisMore M;
+  M.T = c.TS; M.J = c.J;
+  M.unspring = c.sp_unspring.p; M.tz1 = c.ts_z1.p; M.tx1 = c.ts_x1.p; M.rpy = c.ts_rpy.p; M.tot = c.ts_tot.p;
+  M.spr = c.ts_spr.p; M.jreac = c.jt_reac.p;
   hipLaunchKernelGGL (k_history, grid (c.nhis), dim3 (PMX_TPB), 0, c.stream, (const int*) c.ctl.p, hisrow, row0,
                       c.nhis, (const int*) c.his_ent.p, (const int*) c.his_off.p, (const int*) c.his_list.p,
                       (const signed char*) c.his_pt.p, (const double*) c.his_pts.p, c.N, c.S,
                       (const double*) c.position.p, (const double*) c.linear.p, (const double*) c.angular.p,
                       (const double*) c.rotation.p, (const double*) c.force.p, (const double*) c.torque.p,
-                      (const double*) c.sp_pnt.p, (const double*) c.sp_stroke.p, (const double*) c.sp_frc.p, out);
+                      (const double*) c.sp_pnt.p, (const double*) c.sp_stroke.p, (const double*) c.sp_frc.p, M, out);
 }
 
 } /* namespace */
@@ -199,12 +233,16 @@ int history_set (pmx_ctx &c, const pmx_history *h)
     if (e < 0 || e >= PMX_HIS_COUNT) return fail (PMX_EINVAL, "invalid history entity");
     if (o0 < 0 || o1 < o0 || o1 > nl) return fail (PMX_EINVAL, "invalid history offsets");
     bool spring = e >= PMX_HIS_LENGTH;
+    /* the count a source number must stay below: particles, springs,
+     * torsion springs or joints */
+    int lim = e >= PMX_HIS_JREAC_X ? c.J : e >= PMX_HIS_ZDIR_X ? c.TS : spring ? c.S : c.N;
+    const char *what = e >= PMX_HIS_JREAC_X ? "Joint index out of range" : e >= PMX_HIS_ZDIR_X ?
+                       "Torsion spring index out of range" : spring ? "Spring index out of range" : "Particle index out of range";
     if (e != PMX_HIS_TIME && o1 == o0) return fail (PMX_EINVAL, "empty history source");
     for (int j = o0; j < o1; j++)
     {
       int k = h->list[j];
-      if (spring ? (k < 0 || k >= c.S) : (k < 0 || k >= c.N))
-        return fail (PMX_EINVAL, spring ? "Spring index out of range" : "Particle index out of range");
+      if (k < 0 || k >= lim) return fail (PMX_EINVAL, what);
       if (e >= PMX_HIS_FX && e <= PMX_HIS_TL) keep.push_back (k);
     }
     if (h->haspoint && h->haspoint[i])

@@ -39,7 +39,8 @@ enum { CTL_ABORT = 0, CTL_KSTEP, CTL_HINT, CTL_SEQ, CTL_COUNT = 8 };
  * or grows the capacity and repeats it); a traversal stack overflow in a
  * tree build (FLAG_TRILIST_OVERFLOW) is an error */
 enum { FLAG_LEAF_OVERFLOW = 0, FLAG_ROW_OVERFLOW, FLAG_HIT_OVERFLOW, FLAG_EXTRA_OVERFLOW,
-       FLAG_TRILIST_OVERFLOW, FLAG_MAXROW, FLAG_BAD_KIND, FLAG_MTRI_OVERFLOW, FLAG_TREV_OVERFLOW, FLAG_COUNT };
+       FLAG_TRILIST_OVERFLOW, FLAG_MAXROW, FLAG_BAD_KIND, FLAG_MTRI_OVERFLOW, FLAG_TREV_OVERFLOW,
+       FLAG_JOINT_PIVOT, FLAG_COUNT };
 
 /* abort the current step: the step is repeated after the host has grown the
  * capacity that overflowed (or re-partitioned), from unchanged state */
@@ -214,6 +215,32 @@ struct pmx_ctx
   dvec<double> sp_pnt, sp_dir, sp_yield, sp_sdsp, sp_stroke, sp_frc, sp_stroke0, sp_fric, sp_kskn;
   dvec<double> sp_curve, sp_dash, sp_unload, sp_out;
   dvec<int> sp_evoff, sp_evrun, sp_runoff, sp_runspr;
+  dvec<int> sp_offset;         /* [S] load curve of each spring's stroke offset, or -1 */
+  bool sp_offsets = false;     /* some spring has a stroke offset: k_springs reads ttab */
+  /* load curves (spring stroke offsets, UNSPRING unloading) */
+  int LC = 0, lccrv = 0;
+  dvec<int> lcidx;
+  dvec<double> lcurve;         /* [2][lccrv] */
+  dvec<double> ttab;           /* [step] curtime of each step of the call (stroke offsets) */
+  /* torsion springs (sorted by part1, part2) */
+  int TS = 0, krpycrv = 0, drpycrv = 0;
+  dvec<int> ts_part, ts_kidx, ts_didx, ts_cone;
+  dvec<double> ts_z0, ts_x0, ts_krpy, ts_drpy, ts_z1, ts_x1, ts_rpy, ts_tot, ts_spr;
+  dvec<double> ts_out;         /* [6][TS] torque, adaptive terms |k| R(d,d) of each spring */
+  dvec<int> ts_evoff, ts_evrun, ts_runoff, ts_runspr;   /* flush order, as the springs' */
+  /* UNSPRING conditions */
+  int US = 0;
+  dvec<int> un_tidx, un_t, un_midx, un_m, un_aidx, un_a, un_ent, un_op, un_abs, un_nsteps, un_nfreq, un_action;
+  dvec<double> un_lim;
+  /* restraints: runs of one particle's entries in entry order */
+  int RS = 0, RR = 0;
+  dvec<int> rs_part, rs_runoff;
+  dvec<double> rs_lin, rs_ang;
+  /* ball joints: independent sets solved per step (joints.cpp:83-567) */
+  int J = 0, JS = 0, JP = 0;   /* joints, independent sets, particles with joints */
+  dvec<int> jt_part, jt_set, jt_pos, jt_adjoff, jt_adj, jt_soff, jt_sk, jt_ppart, jt_poff, jt_plist, jt_iw;
+  dvec<double> jt_point, jt_reac, jt_mat, jt_rhs, jt_work;
+  dvec<long long> jt_moff, jt_woff;   /* per set: its dense matrix, its skyline work space */
 
   /* broad phase per step */
   TreeDev tree;
@@ -432,5 +459,17 @@ int step_react (pmx_ctx &c, bool critical);    /* meshed particles' contact reac
 int adp_begin (pmx_ctx &c);                    /* zero the per-particle sums; reset the adaptive minimum */
 int adp_time_step (pmx_ctx &c, double *h, double *o, double *r);   /* per particle; minimum when h is null */
 int step_mesh_shapes (pmx_ctx &c, bool gated = true);
+int torsion_setup (pmx_ctx &c, const pmx_model *m);   /* torsion springs and UNSPRING conditions */
+int torsion_download (pmx_ctx &c, pmx_model *m);
+int step_torsion (pmx_ctx &c, double h);        /* torsion springs (after the springs) */
+int step_unspring (pmx_ctx &c);                 /* UNSPRING tests (after the torsion springs) */
+
+/* ---- restraints and ball joints (pmx_joint.hip) ---- */
+int jt_validate (const pmx_model *m);          /* torsion springs, UNSPRING, load curves, restraints, joints */
+int jt_setup (pmx_ctx &c, const pmx_model *m);
+int jt_download (pmx_ctx &c, pmx_model *m);
+int step_restrain_vel (pmx_ctx &c);            /* once per dem call, before the steps */
+int step_joints (pmx_ctx &c, double step0, double step1);   /* ball joints, then restrained forces */
+int step_times_table (pmx_ctx &c, const double *times, int n);   /* curtime of each step (stroke offsets) */
 
 #endif

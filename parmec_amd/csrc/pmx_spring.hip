@@ -110,8 +110,10 @@ __device__ double eqm (int N, const double *inverse, const double *invm, const d
 struct SprArgs
 {
   int S, N;
-  const int *type, *part, *spridx, *dashidx, *unidx, *unspring;
-  int *flg;
+  const int *type, *part, *spridx, *dashidx, *unidx;
+  int *unspring, *flg;
+  const int *offset, *lcidx;          /* stroke offset curves (-1: none), load curve ranges */
+  const double *lcx, *lcy, *ttab;     /* load curves; curtime of each step of the call */
   double *pnt, *dir, *yield, *sdsp, *stroke, *frc;
   const double *stroke0, *fric, *kskn;
   const double *sx, *sy, *dx, *dy, *ux, *uy;
@@ -220,6 +222,15 @@ __global__ void k_springs (SprArgs A, double step)
     break;
   }
   velocity = z[0]*u[0] + z[1]*u[1] + z[2]*u[2];
+  /* the stroke offset: a load curve of time at the step's curtime
+   * (forces.ispc:915-919) */
+  double offset = 0.0;
+  int ilc = A.offset ? A.offset[i] : -1;
+  if (ilc >= 0)
+  {
+    double slope;
+    offset = look_up (A.lcx, A.lcy, A.lcidx[ilc], A.lcidx[ilc+1], A.ttab[A.ctl[CTL_KSTEP]], &slope);
+  }
   int unspr = A.unspring[i];
   double *stroke = A.stroke, *yld = A.yield;
   if (unspr <= -2)
@@ -227,12 +238,12 @@ __global__ void k_springs (SprArgs A, double step)
     if (A.type[i] == PMX_SPRING_NONLINEAR_ELASTIC)
     {
       stroke[i] = len - A.stroke0[i];
-      spring_force = look_up (A.sx, A.sy, A.spridx[i], A.spridx[i+1], stroke[i] - 0.0, &stiffness);
+      spring_force = look_up (A.sx, A.sy, A.spridx[i], A.spridx[i+1], stroke[i] - offset, &stiffness);
     }
     else if ((flg & PMX_SPRING_YIELDED) == 0)
     {
       stroke[i] = len - A.stroke0[i];
-      spring_force = look_up (A.sx, A.sy, A.spridx[i], A.spridx[i+1], stroke[i] - 0.0, &stiffness);
+      spring_force = look_up (A.sx, A.sy, A.spridx[i], A.spridx[i+1], stroke[i] - offset, &stiffness);
       if (spring_force < yld[i] || spring_force > yld[S + i])
       {
         flg |= PMX_SPRING_YIELDED;
@@ -289,13 +300,36 @@ __global__ void k_springs (SprArgs A, double step)
     else dashpot_force = look_up (A.dx, A.dy, A.dashidx[i], A.dashidx[i+1], velocity, &damping);
     total_force = spring_force + (spring_force == 0.0 ? 0.0 : dashpot_force);
   }
-  else
+  else if (unspr == -1)
   {
-    /* unspr == -1: inactive spring (load-curve unloading is rejected at upload) */
+    /* an inactive spring (zero force) */
     stroke[i] = len - A.stroke0[i];
     stiffness = 0.;
     damping = 0.;
     spring_force = dashpot_force = total_force = 0.;
+  }
+  else
+  {
+    /* switched off by UNSPRING onto load curve unspr: incremental unloading
+     * to zero, then zero for ever (forces.ispc:1080-1115) */
+    double s0 = stroke[i];
+    double s1 = len - A.stroke0[i];
+    double ds = s1 - s0;
+    double f0 = A.frc[S + i];
+    double sgn = f0 > 0. ? 1. : -1.;
+    stroke[i] = s1;
+    double at;
+    double dfds = look_up_slope_y (A.lcx, A.lcy, A.lcidx[unspr], A.lcidx[unspr+1], f0, &at);
+    stiffness = dfds;
+    spring_force = f0 - sgn*fabs (dfds*ds);
+    if (spring_force * f0 < 0.0)
+    {
+      A.unspring[i] = -1;
+      spring_force = 0.;
+    }
+    damping = 0.;
+    dashpot_force = 0.;
+    total_force = spring_force;
   }
   A.flg[i] = flg;
 #pragma unroll
@@ -643,6 +677,328 @@ __global__ void k_faces (int FAC, int NOD, int T, const int *facnod, const int *
   }
 }
 
+/* ------------------------------------------------------- torsion springs */
+
+struct TsArgs
+{
+  int T, N;
+  const int *part, *kidx, *didx, *cone;   /* kidx, didx: [3][T+1] into krpy, drpy */
+  const double *z0, *x0, *kx, *ky, *dx, *dy;
+  double *z1, *x1, *rpy, *tot, *spr;
+  double *out;                            /* [6][T]: torque, adaptive terms */
+  const double *angular, *rotation, *inverse;
+  const int *ctl;
+};
+
+/* the torque of a spring curve of a torsion spring, or zero torque and
+ * stiffness when the curve is empty (forces.ispc:1625-1637) */
+__device__ __forceinline__ double trq_curve (const TsArgs &A, int o, int i, double arg, double *k)
+{
+  int lo = A.kidx[o*(A.T+1) + i], hi = A.kidx[o*(A.T+1) + i + 1];
+  if (hi > lo) return look_up (A.kx, A.ky, lo, hi, arg, k);
+  *k = 0.0;
+  return 0.0;
+}
+
+/* the damper of axis o: a critical damping ratio (one table point) against
+ * the local inertia `I`, or a curve of the angular velocity
+ * (forces.ispc:1641-1675); `ratio_axis` is the table whose first point is
+ * read as the ratio (the pitch of the pitch-yaw cone reads roll's, as the
+ * reference does) */
+__device__ __forceinline__ double trq_damper (const TsArgs &A, int o, int ratio_axis, int i, double k, double I,
+                                              double vel, double *d)
+{
+  int lo = A.didx[o*(A.T+1) + i], hi = A.didx[o*(A.T+1) + i + 1];
+  if (lo + 1 == hi)
+  {
+    double ratio = A.dx[A.didx[ratio_axis*(A.T+1) + i]];
+    *d = - ratio * 2.0 * sqrt (fabs (k) * I);
+    return vel * *d;
+  }
+  return look_up (A.dx, A.dy, lo, hi, vel, d);
+}
+
+/* trqspr_task, forces.ispc:1415-2055, one thread per torsion spring: the
+ * current axes (part1's rotation of the reference axes), the roll, pitch
+ * and yaw rates of the relative angular velocity and their integrals, the
+ * curve torques with dampers (per axis, or on the cone of two or three axes
+ * when `cone` joins them), and the global torque; the flush into the
+ * particles' torques follows in k_torsion_acc */
+__global__ void k_torsion (TsArgs A, double step)
+{
+  if (A.ctl[CTL_ABORT]) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.T) return;
+  size_t T = A.T, N = A.N;
+  int j = A.part[i], k = A.part[T + i];
+  double oj[3], Lj[9], ok[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int c = 0; c < 3; c++) oj[c] = A.angular[(3+c)*N + j];
+#pragma unroll
+  for (int c = 0; c < 9; c++) Lj[c] = A.rotation[c*N + j];
+  double zdir0[3], xdir0[3], ydir0[3], rot1j[9];
+#pragma unroll
+  for (int c = 0; c < 3; c++) { zdir0[c] = A.z0[c*T + i]; xdir0[c] = A.x0[c*T + i]; }
+  ydir0[0] = zdir0[1]*xdir0[2] - zdir0[2]*xdir0[1];
+  ydir0[1] = zdir0[2]*xdir0[0] - zdir0[0]*xdir0[2];
+  ydir0[2] = zdir0[0]*xdir0[1] - zdir0[1]*xdir0[0];
+  nvmul (Lj, xdir0, rot1j);
+  nvmul (Lj, ydir0, rot1j+3);
+  nvmul (Lj, zdir0, rot1j+6);
+#pragma unroll
+  for (int c = 0; c < 3; c++) { A.z1[c*T + i] = rot1j[6+c]; A.x1[c*T + i] = rot1j[c]; }
+  /* part2's angular velocity; zero for a spring to the ground, and also
+   * where the reference's loop would still hold the zeros of such a spring
+   * (it reloads part2's velocity only when part2 changes; bit 8 of cone) */
+  if (k >= 0 && !(A.cone[i] & 256))
+  {
+#pragma unroll
+    for (int c = 0; c < 3; c++) ok[c] = A.angular[(3+c)*N + k];
+  }
+  double ojk[3] = {oj[0]-ok[0], oj[1]-ok[1], oj[2]-ok[2]};
+  double rolldot = rot1j[0]*ojk[0] + rot1j[1]*ojk[1] + rot1j[2]*ojk[2];
+  double pitchdot = rot1j[3]*ojk[0] + rot1j[4]*ojk[1] + rot1j[5]*ojk[2];
+  double yawdot = rot1j[6]*ojk[0] + rot1j[7]*ojk[1] + rot1j[8]*ojk[2];
+  double roll = A.rpy[i], pitch = A.rpy[T + i], yaw = A.rpy[2*T + i];
+  roll += rolldot*step;
+  pitch += pitchdot*step;
+  yaw += yawdot*step;
+  A.rpy[i] = roll; A.rpy[T + i] = pitch; A.rpy[2*T + i] = yaw;
+  /* local equivalent inertia inv (R (inverse_j + inverse_k) R') where a
+   * damper is a critical damping ratio */
+  double Ixx = 0.0, Iyy = 0.0, Izz = 0.0;
+  const int *D = A.didx;
+  if (D[i]+1 == D[i+1] || D[(T+1) + i]+1 == D[(T+1) + i+1] || D[2*(T+1) + i]+1 == D[2*(T+1) + i+1])
+  {
+    double I[9], tmp[9];
+#pragma unroll
+    for (int c = 0; c < 9; c++) I[c] = A.inverse[c*N + j];
+    if (k >= 0)
+#pragma unroll
+      for (int c = 0; c < 9; c++) I[c] += A.inverse[c*N + k];
+    ntmul (I, rot1j, tmp);
+    nnmul (rot1j, tmp, I);
+    invert3 (I, tmp);
+    Ixx = tmp[0]; Iyy = tmp[4]; Izz = tmp[8];
+  }
+  double kroll = 0.0, kpitch = 0.0, kyaw = 0.0, droll, dpitch, dyaw;
+  double trqroll, trqpitch, trqyaw, trqrolldot, trqpitchdot, trqyawdot;
+  double trqrolltot = 0.0, trqpitchtot = 0.0, trqyawtot = 0.0;
+  switch (A.cone[i] & 255)
+  {
+  case 0:
+    trqroll = trq_curve (A, 0, i, roll, &kroll);
+    trqpitch = trq_curve (A, 1, i, pitch, &kpitch);
+    trqyaw = trq_curve (A, 2, i, yaw, &kyaw);
+    trqrolldot = trq_damper (A, 0, 0, i, kroll, Ixx, rolldot, &droll);
+    trqpitchdot = trq_damper (A, 1, 1, i, kpitch, Iyy, pitchdot, &dpitch);
+    trqyawdot = trq_damper (A, 2, 2, i, kyaw, Izz, yawdot, &dyaw);
+    trqrolltot = trqroll + (trqroll == 0.0 ? 0.0 : trqrolldot);
+    trqpitchtot = trqpitch + (trqpitch == 0.0 ? 0.0 : trqpitchdot);
+    trqyawtot = trqyaw + (trqyaw == 0.0 ? 0.0 : trqyawdot);
+    break;
+  case 3:   /* roll and pitch on one cone */
+  {
+    double angle = sqrt (roll*roll + pitch*pitch);
+    double angledot = angle > 0.0 ? (rolldot*roll+pitchdot*pitch)/angle : 0.0;
+    trqroll = trq_curve (A, 0, i, angle, &kroll);
+    kpitch = kroll;
+    trqrolldot = trq_damper (A, 0, 0, i, kroll, sqrt (Ixx*Ixx + Iyy*Iyy), angledot, &droll);
+    trqyaw = trq_curve (A, 2, i, yaw, &kyaw);
+    trqyawdot = trq_damper (A, 2, 2, i, kyaw, Izz, yawdot, &dyaw);
+    trqyawtot = trqyaw + (trqyaw == 0.0 ? 0.0 : trqyawdot);
+    if (angle > 0.)
+    {
+      double xcoef = roll/angle, ycoef = pitch/angle;
+      trqrolltot = xcoef*(trqroll + (trqroll == 0.0 ? 0.0 : trqrolldot));
+      trqpitchtot = ycoef*(trqroll + (trqroll == 0.0 ? 0.0 : trqrolldot));
+      trqpitch = trqroll*ycoef;
+      trqroll *= xcoef;
+    }
+    else { trqrolltot = trqpitchtot = 0.0; trqroll = trqpitch = 0.0; }
+    break;
+  }
+  case 5:   /* roll and yaw */
+  {
+    double angle = sqrt (roll*roll + yaw*yaw);
+    double angledot = angle > 0.0 ? (rolldot*roll+yawdot*yaw)/angle : 0.0;
+    trqroll = trq_curve (A, 0, i, angle, &kroll);
+    kyaw = kroll;
+    trqrolldot = trq_damper (A, 0, 0, i, kroll, sqrt (Ixx*Ixx + Izz*Izz), angledot, &droll);
+    trqpitch = trq_curve (A, 1, i, pitch, &kpitch);
+    trqpitchdot = trq_damper (A, 1, 1, i, kpitch, Iyy, pitchdot, &dpitch);
+    trqpitchtot = trqpitch + (trqpitch == 0.0 ? 0.0 : trqpitchdot);
+    if (angle > 0.)
+    {
+      double xcoef = roll/angle, zcoef = yaw/angle;
+      trqrolltot = xcoef*(trqroll + (trqroll == 0.0 ? 0.0 : trqrolldot));
+      trqyawtot = zcoef*(trqroll + (trqroll == 0.0 ? 0.0 : trqrolldot));
+      trqyaw = trqroll*zcoef;
+      trqroll *= xcoef;
+    }
+    else { trqrolltot = trqyawtot = 0.0; trqroll = trqyaw = 0.0; }
+    break;
+  }
+  case 6:   /* pitch and yaw */
+  {
+    trqroll = trq_curve (A, 0, i, roll, &kroll);
+    trqrolldot = trq_damper (A, 0, 0, i, kroll, Ixx, rolldot, &droll);
+    double angle = sqrt (pitch*pitch + yaw*yaw);
+    double angledot = angle > 0.0 ? (pitchdot*pitch+yawdot*yaw)/angle : 0.0;
+    trqpitch = trq_curve (A, 1, i, angle, &kpitch);
+    kyaw = kpitch;
+    trqpitchdot = trq_damper (A, 1, 0, i, kpitch, sqrt (Iyy*Iyy + Izz*Izz), angledot, &dpitch);
+    trqrolltot = trqroll + (trqroll == 0.0 ? 0.0 : trqrolldot);
+    if (angle > 0.)
+    {
+      double ycoef = pitch/angle, zcoef = yaw/angle;
+      trqpitchtot = ycoef*(trqpitch + (trqpitch == 0.0 ? 0.0 : trqpitchdot));
+      trqyawtot = zcoef*(trqpitch + (trqpitch == 0.0 ? 0.0 : trqpitchdot));
+      trqyaw = zcoef*trqpitch;
+      trqpitch *= ycoef;
+    }
+    else { trqpitchtot = trqyawtot = 0.0; trqpitch = trqyaw = 0.0; }
+    break;
+  }
+  default:  /* 7: roll, pitch and yaw */
+  {
+    double angle = sqrt (roll*roll + pitch*pitch + yaw*yaw);
+    double angledot = angle > 0.0 ? (rolldot*roll+pitchdot*pitch+yawdot*yaw)/angle : 0.0;
+    trqroll = trq_curve (A, 0, i, angle, &kroll);
+    kpitch = kyaw = kroll;
+    trqrolldot = trq_damper (A, 0, 0, i, kroll, sqrt (Ixx*Ixx + Iyy*Iyy + Izz*Izz), angledot, &droll);
+    if (angle > 0.)
+    {
+      double xcoef = roll/angle, ycoef = pitch/angle, zcoef = yaw/angle;
+      trqrolltot = xcoef*(trqroll + (trqroll == 0.0 ? 0.0 : trqrolldot));
+      trqpitchtot = ycoef*(trqroll + (trqroll == 0.0 ? 0.0 : trqrolldot));
+      trqyawtot = zcoef*(trqroll + (trqroll == 0.0 ? 0.0 : trqrolldot));
+      trqpitch = ycoef*trqroll;
+      trqyaw = zcoef*trqroll;
+      trqroll *= xcoef;
+    }
+    else { trqrolltot = trqpitchtot = trqyawtot = 0.0; trqroll = trqpitch = trqyaw = 0.0; }
+    break;
+  }
+  }
+  A.tot[i] = trqrolltot; A.tot[T + i] = trqpitchtot; A.tot[2*T + i] = trqyawtot;
+  A.spr[i] = trqroll; A.spr[T + i] = trqpitch; A.spr[2*T + i] = trqyaw;
+  double *o = A.out;
+  o[i] = trqrolltot*rot1j[0] + trqpitchtot*rot1j[3] + trqyawtot*rot1j[6];
+  o[T + i] = trqrolltot*rot1j[1] + trqpitchtot*rot1j[4] + trqyawtot*rot1j[7];
+  o[2*T + i] = trqrolltot*rot1j[2] + trqpitchtot*rot1j[5] + trqyawtot*rot1j[8];
+  /* the adaptive step's rotational stiffness terms K(1,1), K(2,2), K(3,3)
+   * (forces.ispc:1968-1977) */
+  o[3*T + i] = fabs (kroll)*rot1j[0];
+  o[4*T + i] = fabs (kpitch)*rot1j[4];
+  o[5*T + i] = fabs (kyaw)*rot1j[8];
+}
+
+/* per particle: the torsion springs' torques in the flush order of the
+ * single-task loop (forces.ispc:1942-2054), as k_spring_acc for forces */
+__global__ void k_torsion_acc (int N, int T, const int *evoff, const int *evrun, const int *runoff, const int *runspr,
+                               const double *out, double *torque, const int *ctl)
+{
+  if (ctl[CTL_ABORT]) return;
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= N) return;
+  int e0 = evoff[b], e1 = evoff[b+1];
+  if (e0 == e1) return;
+  double t[3] = {torque[b], torque[(size_t) N + b], torque[2*(size_t) N + b]};
+  for (int ev = e0; ev < e1; ev++)
+  {
+    int code = evrun[ev];
+    int run = code >> 2;
+    int r0 = runoff[run], r1 = runoff[run+1];
+    double at[3];
+    int s = runspr[r0];
+    if (code & 2) for (int c = 0; c < 3; c++) at[c] = 0.0 + out[(size_t) c*T + s];
+    else for (int c = 0; c < 3; c++) at[c] = out[(size_t) c*T + s];
+    for (int r = r0 + 1; r < r1; r++)
+    {
+      s = runspr[r];
+      for (int c = 0; c < 3; c++) at[c] += out[(size_t) c*T + s];
+    }
+    if (code & 1) for (int c = 0; c < 3; c++) t[c] -= at[c];
+    else for (int c = 0; c < 3; c++) t[c] += at[c];
+  }
+  torque[b] = t[0]; torque[(size_t) N + b] = t[1]; torque[2*(size_t) N + b] = t[2];
+}
+
+/* per particle: the torsion springs' rotational stiffness terms added run
+ * by run (each run summed from 0.0; the off-diagonal terms add 0.0, as the
+ * reference's zero buffers do) */
+__global__ void k_torsion_adp (int N, int T, const int *evoff, const int *evrun, const int *runoff, const int *runspr,
+                               const double *out, double *adp, const int *ctl)
+{
+  if (ctl[CTL_ABORT]) return;
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= N) return;
+  int e0 = evoff[b], e1 = evoff[b+1];
+  if (e0 == e1) return;
+  double v[6];
+#pragma unroll
+  for (int c = 0; c < 6; c++) v[c] = adp[(size_t) (5+c)*N + b];
+  for (int ev = e0; ev < e1; ev++)
+  {
+    int run = evrun[ev] >> 2;
+    double kr[3] = {0.0, 0.0, 0.0};
+    for (int r = runoff[run]; r < runoff[run+1]; r++)
+    {
+      size_t s = runspr[r];
+      for (int c = 0; c < 3; c++) kr[c] += out[(3+c)*(size_t) T + s];
+    }
+    for (int c = 0; c < 3; c++) v[c] += kr[c];
+    for (int c = 3; c < 6; c++) v[c] += 0.0;
+  }
+#pragma unroll
+  for (int c = 0; c < 6; c++) adp[(size_t) (5+c)*N + b] = v[c];
+}
+
+/* unspring_task, forces.ispc:2058-2127, one thread per condition: on the
+ * dem calls whose number is a multiple of nsteps, the sum, minimum or
+ * maximum of an entity over the tested springs; outside the limits the
+ * failure count falls, and at zero the condition switches its springs off
+ * (to zero force or onto an unloading curve) and activates its inactive
+ * ones.  min and max as ISPC's (a < b ? a : b, a > b ? a : b) */
+__global__ void k_unspring (int U, int S, int stepnum, const int *tidx, const int *tspr, const int *midx, const int *mspr,
+                            const int *aidx, const int *aspr, const int *ent, const int *op, const int *uabs,
+                            const int *nsteps, int *nfreq, const int *action, const double *lim, const double *stroke,
+                            const double *frc, int *unspring, const int *ctl)
+{
+  if (ctl[CTL_ABORT]) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= U) return;
+  if (stepnum % nsteps[i]) return;
+  if (nfreq[i] == 0) return;
+  double result = op[i] == PMX_OP_MIN ? 1.7976931348623157e308 : op[i] == PMX_OP_MAX ? -1.7976931348623157e308 : 0.;
+  for (int q = tidx[i]; q < tidx[i+1]; q++)
+  {
+    int s = tspr[q];
+    double value;
+    switch (ent[i])
+    {
+    case PMX_HIS_STROKE: value = stroke[s]; break;
+    case PMX_HIS_F: value = frc[s]; break;
+    case PMX_HIS_FF: value = frc[2*(size_t) S + s]; break;
+    default: value = frc[(size_t) S + s]; break;
+    }
+    if (uabs[i]) value = fabs (value);
+    if (op[i] == PMX_OP_MIN) result = value < result ? value : result;
+    else if (op[i] == PMX_OP_MAX) result = value > result ? value : result;
+    else result += value;
+  }
+  if (result < lim[i] || result > lim[U + i])
+  {
+    int f = --nfreq[i];
+    if (f == 0)
+    {
+      for (int q = midx[i]; q < midx[i+1]; q++) unspring[mspr[q]] = action[i];
+      for (int q = aidx[i]; q < aidx[i+1]; q++) unspring[aspr[q]] = -3;
+    }
+  }
+}
+
 } /* namespace */
 
 int springs_setup (pmx_ctx &c, const pmx_model *m)
@@ -733,6 +1089,12 @@ int springs_setup (pmx_ctx &c, const pmx_model *m)
       (rc = upd (c.sp_curve, m->spring, 2*(size_t) m->sprcrv)) || (rc = upd (c.sp_dash, m->dashpot, 2*(size_t) m->dashcrv)) ||
       (rc = upd (c.sp_unload, m->unload, 2*(size_t) m->uncrv)))
     return rc;
+  c.sp_offsets = false;
+  if (m->sproffset)
+  {
+    for (int q = 0; q < S; q++) c.sp_offsets = c.sp_offsets || m->sproffset[q] >= 0;
+    if (c.sp_offsets && (rc = upi (c.sp_offset, m->sproffset, SS))) return rc;
+  }
   c.sprcrv = m->sprcrv; c.dashcrv = m->dashcrv; c.uncrv = m->uncrv;
   if (c.sp_out.alloc (9*SS) != hipSuccess) { pmx_set_error ("out of device memory (springs)"); return PMX_ENOMEM; }
   return PMX_OK;
@@ -751,6 +1113,141 @@ int springs_download (pmx_ctx &c, pmx_model *m)
       (rc = dn (m->sprsdsp, c.sp_sdsp, 3*S)) || (rc = dn (m->stroke, c.sp_stroke, 3*S)) || (rc = dn (m->sprfrc, c.sp_frc, 3*S)))
     return rc;
   if (m->sprflg) PMX_HIP (hipMemcpyAsync (m->sprflg, c.sp_flg.p, S * sizeof (int), hipMemcpyDeviceToHost, c.stream));
+  /* UNSPRING and load-curve unloading change a spring's state */
+  if (m->unspring) PMX_HIP (hipMemcpyAsync (m->unspring, c.sp_unspring.p, S * sizeof (int), hipMemcpyDeviceToHost, c.stream));
+  return PMX_OK;
+}
+
+/* the flush order of the single-task loop over springs sorted by
+ * (part1, part2) (forces.ispc:1203-1411, 1942-2054): runs of consecutive
+ * springs sharing part1 (added) and, among those with a part2, sharing
+ * part2 (subtracted), each flushed where it ends; per particle its flushes
+ * in loop order.  evrun: (run << 2) | (first run of its side ? 2 : 0) |
+ * (part2 run ? 1 : 0) */
+static void flush_runs (int N, int n, const int *p1, const int *p2, std::vector<int> &evoff, std::vector<int> &evrun,
+                        std::vector<int> &runoff, std::vector<int> &runspr)
+{
+  struct Ev { int pos, sub, body, code; };
+  std::vector<Ev> ev;
+  runoff.assign (1, 0);
+  runspr.clear ();
+  int nrun = 0;
+  bool firstj = true, firstk = true;
+  for (int i = 0; i < n; )
+  {
+    int e = i;
+    while (e < n && p1[e] == p1[i]) e++;
+    for (int q = i; q < e; q++) runspr.push_back (q);
+    runoff.push_back ((int) runspr.size ());
+    ev.push_back ({e, 0, p1[i], (nrun << 2) | (firstj ? 2 : 0)});
+    firstj = false;
+    nrun++;
+    i = e;
+  }
+  int cur = -1;
+  for (int i = 0; i <= n; i++)
+  {
+    if (i < n && p2[i] < 0) continue;
+    int k = i < n ? p2[i] : -2;
+    if (cur >= 0 && k != cur)
+    {
+      runoff.push_back ((int) runspr.size ());
+      ev.push_back ({i, 1, cur, (nrun << 2) | 1 | (firstk ? 2 : 0)});
+      firstk = false;
+      nrun++;
+    }
+    if (i < n) { runspr.push_back (i); cur = k; }
+  }
+  std::stable_sort (ev.begin (), ev.end (), [] (const Ev &a, const Ev &b) {
+    return a.body < b.body || (a.body == b.body && (a.pos < b.pos || (a.pos == b.pos && a.sub < b.sub))); });
+  evoff.assign (N + 1, 0);
+  evrun.resize (ev.size ());
+  for (auto &x : ev) evoff[x.body + 1]++;
+  for (int b = 0; b < N; b++) evoff[b+1] += evoff[b];
+  for (size_t q = 0; q < ev.size (); q++) evrun[q] = ev[q].code;
+}
+
+int torsion_setup (pmx_ctx &c, const pmx_model *m)
+{
+  auto upi = [&] (dvec<int> &d, const int *src, size_t n) -> int {
+    if (d.alloc (std::max (n, (size_t) 1)) != hipSuccess) { pmx_set_error ("out of device memory (torsion springs)"); return PMX_ENOMEM; }
+    if (n) PMX_HIP (hipMemcpyAsync (d.p, src, n * sizeof (int), hipMemcpyHostToDevice, c.stream));
+    return PMX_OK;
+  };
+  auto upd = [&] (dvec<double> &d, const double *src, size_t n) -> int {
+    if (d.alloc (std::max (n, (size_t) 1)) != hipSuccess) { pmx_set_error ("out of device memory (torsion springs)"); return PMX_ENOMEM; }
+    if (n) PMX_HIP (hipMemcpyAsync (d.p, src, n * sizeof (double), hipMemcpyHostToDevice, c.stream));
+    return PMX_OK;
+  };
+  int rc;
+  int T = c.TS = m->trqsprnum;
+  if (T > 0)
+  {
+    size_t TT = T;
+    std::vector<int> evoff, evrun, runoff, runspr;
+    flush_runs (c.N, T, m->trqsprpart, m->trqsprpart + T, evoff, evrun, runoff, runspr);
+    /* trqspr_task reloads part2's angular velocity only where part2 differs
+     * from the last spring's with a part2, and zeroes it for a spring to
+     * the ground (forces.ispc:1500-1526): a spring whose part2 repeats after
+     * such a spring reads zeros; bit 8 of its cone word marks it */
+    std::vector<int> cone (m->trqcone, m->trqcone + T);
+    {
+      int k0 = -1;
+      bool zero = false;
+      for (int i = 0; i < T; i++)
+      {
+        int k = m->trqsprpart[T + i];
+        if (k >= 0)
+        {
+          if (k != k0) zero = false;
+          if (zero) cone[i] |= 256;
+          k0 = k;
+        }
+        else zero = true;
+      }
+    }
+    c.krpycrv = m->krpycrv; c.drpycrv = m->drpycrv;
+    if ((rc = upi (c.ts_part, m->trqsprpart, 2*TT)) || (rc = upi (c.ts_kidx, m->krpyidx, 3*(TT+1))) ||
+        (rc = upi (c.ts_didx, m->drpyidx, 3*(TT+1))) || (rc = upi (c.ts_cone, cone.data (), TT)) ||
+        (rc = upd (c.ts_z0, m->trqzdir0, 3*TT)) || (rc = upd (c.ts_x0, m->trqxdir0, 3*TT)) ||
+        (rc = upd (c.ts_krpy, m->krpy, 2*(size_t) m->krpycrv)) || (rc = upd (c.ts_drpy, m->drpy, 2*(size_t) m->drpycrv)) ||
+        (rc = upd (c.ts_z1, m->trqzdir1, 3*TT)) || (rc = upd (c.ts_x1, m->trqxdir1, 3*TT)) ||
+        (rc = upd (c.ts_rpy, m->trqrpy, 3*TT)) || (rc = upd (c.ts_tot, m->trqrpytot, 3*TT)) ||
+        (rc = upd (c.ts_spr, m->trqrpyspr, 3*TT)) ||
+        (rc = upi (c.ts_evoff, evoff.data (), evoff.size ())) || (rc = upi (c.ts_evrun, evrun.data (), evrun.size ())) ||
+        (rc = upi (c.ts_runoff, runoff.data (), runoff.size ())) || (rc = upi (c.ts_runspr, runspr.data (), runspr.size ())))
+      return rc;
+    if (c.ts_out.alloc (6*TT) != hipSuccess) { pmx_set_error ("out of device memory (torsion springs)"); return PMX_ENOMEM; }
+  }
+  int U = c.US = m->unsprnum;
+  if (U > 0)
+  {
+    size_t UU = U;
+    if ((rc = upi (c.un_tidx, m->tspridx, UU+1)) || (rc = upi (c.un_t, m->tsprings, m->tspridx[U])) ||
+        (rc = upi (c.un_midx, m->mspridx, UU+1)) || (rc = upi (c.un_m, m->msprings, m->mspridx[U])) ||
+        (rc = upi (c.un_aidx, m->actidx, UU+1)) || (rc = upi (c.un_a, m->activate, m->actidx[U])) ||
+        (rc = upi (c.un_ent, m->unent, UU)) || (rc = upi (c.un_op, m->unop, UU)) || (rc = upi (c.un_abs, m->unabs, UU)) ||
+        (rc = upi (c.un_nsteps, m->nsteps, UU)) || (rc = upi (c.un_nfreq, m->nfreq, UU)) ||
+        (rc = upi (c.un_action, m->unaction, UU)) || (rc = upd (c.un_lim, m->unlim, 2*UU)))
+      return rc;
+  }
+  return PMX_OK;
+}
+
+int torsion_download (pmx_ctx &c, pmx_model *m)
+{
+  size_t T = c.TS;
+  auto dn = [&] (double *dst, const dvec<double> &d, size_t n) -> int {
+    if (dst && n) PMX_HIP (hipMemcpyAsync (dst, d.p, n * sizeof (double), hipMemcpyDeviceToHost, c.stream));
+    return PMX_OK;
+  };
+  int rc;
+  if (T && ((rc = dn (m->trqzdir1, c.ts_z1, 3*T)) || (rc = dn (m->trqxdir1, c.ts_x1, 3*T)) ||
+            (rc = dn (m->trqrpy, c.ts_rpy, 3*T)) || (rc = dn (m->trqrpytot, c.ts_tot, 3*T)) ||
+            (rc = dn (m->trqrpyspr, c.ts_spr, 3*T))))
+    return rc;
+  if (c.US && m->nfreq)
+    PMX_HIP (hipMemcpyAsync (m->nfreq, c.un_nfreq.p, c.US * sizeof (int), hipMemcpyDeviceToHost, c.stream));
   return PMX_OK;
 }
 
@@ -769,6 +1266,8 @@ static SprArgs spr_args (pmx_ctx &c)
   A.inverse = c.inverse.p; A.invm = c.invm.p; A.out = c.sp_out.p; A.flags = c.dflags.p; A.ctl = c.ctl.p;
   A.stop = c.staggered ? c.istop.p : nullptr; A.par_inst = c.par_inst.p;
   A.adp = c.adp_on ? c.sp_adp.p : nullptr;
+  A.offset = c.sp_offsets ? (const int*) c.sp_offset.p : nullptr;
+  A.lcidx = c.lcidx.p; A.lcx = c.lcurve.p; A.lcy = c.lcurve.p + c.lccrv; A.ttab = c.ttab.p;
   return A;
 }
 
@@ -792,6 +1291,41 @@ int step_springs (pmx_ctx &c, double h)
     hipLaunchKernelGGL (k_spring_adp, grid (c.N), dim3 (PMX_TPB), 0, c.stream, c.N, c.S, (const int*) c.sp_evoff.p,
                         (const int*) c.sp_evrun.p, (const int*) c.sp_runoff.p, (const int*) c.sp_runspr.p,
                         (const double*) c.sp_adp.p, c.adp.p, (const int*) c.ctl.p);
+  PMX_HIP (hipGetLastError ());
+  return PMX_OK;
+}
+
+int step_torsion (pmx_ctx &c, double h)
+{
+  if (c.TS == 0) return PMX_OK;
+  TsArgs A;
+  A.T = c.TS; A.N = c.N;
+  A.part = c.ts_part.p; A.kidx = c.ts_kidx.p; A.didx = c.ts_didx.p; A.cone = c.ts_cone.p;
+  A.z0 = c.ts_z0.p; A.x0 = c.ts_x0.p;
+  A.kx = c.ts_krpy.p; A.ky = c.ts_krpy.p + c.krpycrv; A.dx = c.ts_drpy.p; A.dy = c.ts_drpy.p + c.drpycrv;
+  A.z1 = c.ts_z1.p; A.x1 = c.ts_x1.p; A.rpy = c.ts_rpy.p; A.tot = c.ts_tot.p; A.spr = c.ts_spr.p; A.out = c.ts_out.p;
+  A.angular = c.angular.p; A.rotation = c.rotation.p; A.inverse = c.inverse.p; A.ctl = c.ctl.p;
+  hipLaunchKernelGGL (k_torsion, grid (c.TS), dim3 (PMX_TPB), 0, c.stream, A, h);
+  hipLaunchKernelGGL (k_torsion_acc, grid (c.N), dim3 (PMX_TPB), 0, c.stream, c.N, c.TS, (const int*) c.ts_evoff.p,
+                      (const int*) c.ts_evrun.p, (const int*) c.ts_runoff.p, (const int*) c.ts_runspr.p,
+                      (const double*) c.ts_out.p, c.torque.p, (const int*) c.ctl.p);
+  if (c.adp_on)
+    hipLaunchKernelGGL (k_torsion_adp, grid (c.N), dim3 (PMX_TPB), 0, c.stream, c.N, c.TS, (const int*) c.ts_evoff.p,
+                        (const int*) c.ts_evrun.p, (const int*) c.ts_runoff.p, (const int*) c.ts_runspr.p,
+                        (const double*) c.ts_out.p, c.adp.p, (const int*) c.ctl.p);
+  PMX_HIP (hipGetLastError ());
+  return PMX_OK;
+}
+
+int step_unspring (pmx_ctx &c)
+{
+  if (c.US == 0) return PMX_OK;
+  hipLaunchKernelGGL (k_unspring, grid (c.US), dim3 (PMX_TPB), 0, c.stream, c.US, c.S, c.stepnum,
+                      (const int*) c.un_tidx.p, (const int*) c.un_t.p, (const int*) c.un_midx.p, (const int*) c.un_m.p,
+                      (const int*) c.un_aidx.p, (const int*) c.un_a.p, (const int*) c.un_ent.p, (const int*) c.un_op.p,
+                      (const int*) c.un_abs.p, (const int*) c.un_nsteps.p, c.un_nfreq.p, (const int*) c.un_action.p,
+                      (const double*) c.un_lim.p, (const double*) c.sp_stroke.p, (const double*) c.sp_frc.p,
+                      c.sp_unspring.p, (const int*) c.ctl.p);
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
 }

@@ -1998,18 +1998,26 @@ int enqueue_step_forces (pmx_ctx &c, double h)
    * this path reads them (springs carry their own points), so they are formed
    * from the final particle state when an instance finishes, with the same
    * arithmetic and the same result */
-  if ((rc = step_rows (c, h)) || (rc = step_springs (c, h)) || (rc = step_prescribe_acc (c))) return rc;
+  /* forces(): contacts, springs, torsion springs, UNSPRING tests
+   * (forces.ispc:2153-2217) */
+  if ((rc = step_rows (c, h)) || (rc = step_springs (c, h)) || (rc = step_torsion (c, h)) || (rc = step_unspring (c)))
+    return rc;
   if (c.adp_on && (rc = adp_time_step (c, nullptr, nullptr, nullptr))) return rc;
   return PMX_OK;
 }
 
-/* the rest of a step: dynamics with step0 and step1, prescribed
- * velocities, obstacles moved by (step0+step1)/2, the history sample and
- * the step counter */
+/* the rest of a step: ball joints and restraints, prescribed
+ * accelerations, dynamics with step0 and step1, prescribed velocities,
+ * obstacles moved by (step0+step1)/2, the history sample and the step
+ * counter */
 int enqueue_step_motion (pmx_ctx &c, double step0, double step1)
 {
   int rc;
-  if ((rc = step_dynamics (c, step0, step1)) || (rc = step_prescribe_vel (c))) return rc;
+  /* solve_joints, restrain_forces, prescribe_acceleration, dynamics,
+   * prescribe_velocity (parmec.cpp:2453-2466) */
+  if ((rc = step_joints (c, step0, step1)) || (rc = step_prescribe_acc (c)) || (rc = step_dynamics (c, step0, step1)) ||
+      (rc = step_prescribe_vel (c)))
+    return rc;
   /* spheres touch meshed particles: their nodes and triangles follow every
    * step (shapes.ispc:101-211), as the next step's detection reads them */
   if (c.mesh_contact && (rc = step_mesh_shapes (c, true))) return rc;

@@ -24,12 +24,13 @@ adaptive step.  The step loop itself is libpmx's pmx_dem / pmx_dem_adaptive
 MESH (840-1156) builds a meshed particle from volume elements (its
 surface triangles and mass characteristics as mesh.cpp:315-521 forms them;
 particles of SPHERE and MESH share one numbering).  SPRING (1440-1788) is
-supported without stroke offsets (load-curve TSERIES); OBSTACLE with
-point/linear/angular (moving obstacles) and PRESCRIBE (2647-2795) tabulate
-their callables or TSERIES at the step times.  Not on this path (raise
-NotImplementedError): ANALYTICAL, torsion springs, unsprings, joints, XDMF
-and MED output (OUTPUT writes DUMP and VTK files), and HISTORY's sphere/box
-sources and .h5 reading.
+supported with stroke offsets (load-curve TSERIES); TORSION_SPRING
+(1791-2080), UNSPRING (2083-2295), BALL_JOINT (2298-2333) and RESTRAIN
+(2458-2644) run on the device; OBSTACLE with point/linear/angular (moving
+obstacles) and PRESCRIBE (2647-2795) tabulate their callables or TSERIES at
+the step times.  Not on this path (raise NotImplementedError): ANALYTICAL,
+XDMF and MED output (OUTPUT writes DUMP and VTK files), and HISTORY's
+sphere/box sources and .h5 reading.
 """
 import bisect
 import math
@@ -39,8 +40,8 @@ import numpy as np
 from . import pmx
 
 __all__ = ["RESET", "TSERIES", "MATERIAL", "SPHERE", "OBSTACLE", "GRANULAR", "SPRING", "VELOCITY", "GRAVITY", "DAMPING",
-           "DEM", "MESH", "ANALYTICAL", "SPRING", "PRESCRIBE", "HISTORY", "OUTPUT", "CRITICAL", "state", "contacts",
-           "engine_options"]
+           "DEM", "MESH", "ANALYTICAL", "PRESCRIBE", "HISTORY", "OUTPUT", "CRITICAL", "TORSION_SPRING", "UNSPRING",
+           "RESTRAIN", "BALL_JOINT", "state", "contacts", "engine_options"]
 
 _DENSITY, _YOUNG, _POISSON = 0, 1, 2
 
@@ -104,6 +105,14 @@ class _State:
         self.dirty = True
         self.options = {"row_capacity": 0, "device": 0}
         self.springs = []            # SPRING records in creation order
+        self.torsion = []            # TORSION_SPRING records in creation order
+        self.unsprings = []          # UNSPRING records
+        self.restraints = []         # RESTRAIN entries: (particle, linear [9], angular [9])
+        self.joints = []             # BALL_JOINT entries: (part1, part2, point)
+        self.lcurves = []            # load curves (x, y) in order of first use
+        self.lcmap = {}              # TSERIES number -> load curve number (lcurve_from_time_series)
+        self.sprmap = []             # spring number -> index of the sorted springs
+        self.trqsprmap = []          # torsion spring number -> sorted index
         self.histories = []          # (entity, source list, point or None, the returned Python list)
         self.curtime_history = 0.0   # parmec.cpp:73, 2261
         self.outputs = []            # OUTPUT records: (particles, entities, modes)
@@ -521,8 +530,6 @@ def SPRING(part1, point1, part2, geom2, spring, dashpot=None, direction=None, pl
     p1 = _tuple(point1, "point1", 3)
     if offset < -1 or offset >= len(_S.tms):
         raise ValueError("offset TSERIES index is out of range")
-    if offset != -1:
-        raise NotImplementedError("spring stroke offsets are not on the MI355X step path")
     if part2 < -1:
         raise ValueError("Particle two index is out of range [-1, 0, ...]")
     if not isinstance(spring, list) or len(spring) < 4 or len(spring) % 2:
@@ -534,7 +541,8 @@ def SPRING(part1, point1, part2, geom2, spring, dashpot=None, direction=None, pl
     if friction < 0 or kskn < 0:
         raise ValueError("'friction' and 'kskn' must be non-negative")
     r = {"part": (part1, part2), "pnt": [list(p1), None], "dir": [0.0] * 6, "flg": 0, "fric": float(friction),
-         "kskn": float(kskn), "unspring": -1 if inactive else -3, "yield": (0.0, 0.0), "unload": None}
+         "kskn": float(kskn), "unspring": -1 if inactive else -3, "yield": (0.0, 0.0), "unload": None,
+         "offset": -1 if offset < 0 else _lcurve(offset)}
     if isinstance(geom2, tuple):
         r["pnt"][1] = list(_tuple(geom2, "geom2", 3))
         project = False
@@ -613,12 +621,303 @@ def SPRING(part1, point1, part2, geom2, spring, dashpot=None, direction=None, pl
     return len(_S.springs) - 1
 
 
+def _lcurve(ts):
+    """The load curve of a TSERIES (lcurve_from_time_series,
+    parmec.cpp:1239-1269): its points as a look-up table, made once per
+    series."""
+    if ts not in _S.lcmap:
+        series = _S.tms[ts]
+        if series.constant is not None or len(series.t) < 2:
+            raise ValueError("a load curve TSERIES needs at least two points")
+        _S.lcmap[ts] = len(_S.lcurves)
+        _S.lcurves.append((list(series.t), list(series.v)))
+    return _S.lcmap[ts]
+
+
+def _curve_table(x, name, what):
+    """A look-up table list [x0, y0, x1, y1, ...] with increasing x."""
+    if len(x) < 4 or len(x) % 2:
+        raise ValueError(f"Invalid {name} lookup table list length")
+    xs, ys = [float(v) for v in x[0::2]], [float(v) for v in x[1::2]]
+    if any(xs[k] <= xs[k - 1] for k in range(1, len(xs))):
+        raise ValueError(what)
+    return xs, ys
+
+
+def TORSION_SPRING(part1, part2, zdir, xdir, kroll=None, kpitch=None, kyaw=None, droll=None, dpitch=None, dyaw=None,
+                   cone=None, refpnt=None):
+    """Torsion spring between two particles, or a particle and the ground
+    (part2 = -1), about the roll (zdir), pitch and yaw axes of part1's frame
+    (input.cpp:1791-2080); returns the torsion spring number.  kroll, kpitch,
+    kyaw: [angle, torque, ...] curves (default zero torque); droll, dpitch,
+    dyaw: [angular velocity, torque, ...] curves or a critical damping ratio;
+    cone: two or three of 'roll', 'pitch', 'yaw' whose angles act as one.
+    As the reference does, zdir is scaled by 1/|zdir|^2 and xdir is kept as
+    given (input.cpp:1882-1888)."""
+    n = len(_S.parts)
+    if not (isinstance(part1, int) and 0 <= part1 < n):
+        raise ValueError("'part1' must be >= 0 and < the number of particles")
+    if not isinstance(part2, int) or part2 < -1:
+        raise ValueError("Particle two index is out of range [-1, 0, ...]")
+    if part2 >= n:
+        raise ValueError("Particle two index is out of range")
+    z = list(_tuple(zdir, "zdir", 3))
+    x = list(_tuple(xdir, "xdir", 3))
+    curves = []
+    for name, c in (("kroll", kroll), ("kpitch", kpitch), ("kyaw", kyaw)):
+        if c is not None and not isinstance(c, list):
+            raise TypeError(f"'{name}' must be a list")
+        curves.append(_curve_table(c, name, "Torsion spring angle values must increase") if c is not None
+                      else ([-_REAL_MAX, _REAL_MAX], [0.0, 0.0]))
+    dampers = []
+    for name, d in (("droll", droll), ("dpitch", dpitch), ("dyaw", dyaw)):
+        if isinstance(d, list):
+            dampers.append(_curve_table(d, name, "Torsion damper angular velocity values must increase"))
+        elif d is not None:
+            if not _is_number(d):
+                raise TypeError(f"'{name}' must be a list or a number")
+            if d < 0.0:
+                raise ValueError("Critical damping ratio not in [0.0, +Inf) interval")
+            dampers.append(([float(d)], [float(d)]))
+        else:
+            dampers.append(([-_REAL_MAX, _REAL_MAX], [0.0, 0.0]))
+    zlen0 = _norm3(z)
+    xlen0 = _norm3(z)   # the reference measures zdir twice (input.cpp:1878)
+    if zlen0 == 0.0:
+        raise ValueError("Zero zdir length")
+    if xlen0 == 0.0:
+        raise ValueError("Zero xdir length")
+    izl0, ixl0 = 1.0 / zlen0, 1.0 / xlen0
+    z = [v * izl0 for v in z]
+    z = [v * ixl0 for v in z]
+    if z[0] * x[0] + z[1] * x[1] + z[2] * x[2] > 1e-8 * 1e-8:
+        raise ValueError("Directions zdir and xdir do not seem orthogonal")
+    flags = 0
+    if cone is not None:
+        if not isinstance(cone, tuple):
+            raise TypeError("'cone' must be a tuple")
+        if not 2 <= len(cone) <= 3:
+            raise ValueError("Invalid cone tuple size: neither 2 nor 3")
+        for a in cone:
+            if not isinstance(a, str):
+                raise ValueError("Invalid cone tuple component: not a string")
+            if a not in ("roll", "pitch", "yaw"):
+                raise ValueError("Invalid cone tuple component name")
+            flags |= {"roll": pmx.TRQCONE_ROLL, "pitch": pmx.TRQCONE_PITCH, "yaw": pmx.TRQCONE_YAW}[a]
+        if flags in (3, 5, 7) and kroll is None and droll is None:
+            raise ValueError("A roll curve not defined for cone = " + str(cone))
+        if flags == 6 and kpitch is None and dpitch is None:
+            raise ValueError("A pitch curve not defined for cone = ('pitch', 'yaw')")
+        if flags not in (3, 5, 6, 7):
+            raise ValueError("Invalid cone tuple: components repeat")
+    ref = _tuple(refpnt, "refpnt", 3) if refpnt is not None else (_REAL_MAX,) * 3
+    _S.torsion.append({"part": (part1, part2), "z": z, "x": x, "k": curves, "d": dampers, "cone": flags,
+                       "refpnt": ref})
+    _S.dirty = True
+    return len(_S.torsion) - 1
+
+
+def UNSPRING(tsprings, msprings, limits, entity=None, operator=None, abs=False, nsteps=1, nfreq=1, unload=-1,
+             activate=None):
+    """Switch springs off when an entity of other springs leaves its limits
+    (input.cpp:2083-2295): on every nsteps-th DEM call, the SUM (default),
+    MIN or MAX of STROKE, F, SF (default) or FF over tsprings (|values| with
+    abs) is tested against limits = (lower or None, upper or None); after
+    nfreq failures the msprings drop to zero force, or unload along TSERIES
+    `unload`, and the inactive springs in `activate` switch on.  Returns the
+    condition's number."""
+    if not isinstance(tsprings, list) or not isinstance(msprings, list):
+        raise TypeError("'tsprings' and 'msprings' must be lists")
+    if not isinstance(limits, tuple) or len(limits) != 2:
+        raise TypeError("'limits' must be a tuple of 2 elements")
+    if activate is not None and not isinstance(activate, list):
+        raise TypeError("'activate' must be a list")
+    if not isinstance(abs, bool):
+        raise TypeError("'abs' must be a boolean")
+    if not (isinstance(nsteps, int) and nsteps > 0) or not (isinstance(nfreq, int) and nfreq > 0):
+        raise ValueError("'nsteps' and 'nfreq' must be positive")
+    if unload < -1 or unload >= len(_S.tms):
+        raise ValueError("unload TSERIES index is out of range")
+    if unload >= 0:
+        ts = _S.tms[unload]
+        for j in range(1, len(ts.t)):
+            if (ts.v[j] - ts.v[j - 1]) / (ts.t[j] - ts.t[j - 1]) <= 0.0:
+                raise ValueError("Unloading curve must be monotonically increasing")
+    S = len(_S.springs)
+    for j in tsprings:
+        if j < 0 or j >= S:
+            raise ValueError("tsprings SPRING index is out of range")
+    for j in msprings:
+        if j < 0 or j >= S:
+            raise ValueError("msprings SPRING index is out of range")
+        if _S.springs[j]["unspring"] == -2:
+            raise ValueError(f"in msprings, spring index {j} already in use by an earlier UNSPRING call")
+        if _S.springs[j]["unspring"] == -1:
+            raise ValueError(f"in msprings, trying to use an inactive SPRING with index {j}")
+    lo = -_REAL_MAX if limits[0] is None else limits[0]
+    hi = _REAL_MAX if limits[1] is None else limits[1]
+    if not _is_number(lo):
+        raise ValueError("Invalid lower limit")
+    if not _is_number(hi):
+        raise ValueError("Invalid upper limit")
+    if lo >= hi:
+        raise ValueError("Lower limit >= upper limit")
+    ents = {"STROKE": pmx.HIS_ENTITIES.index("STROKE"), "F": pmx.HIS_ENTITIES.index("F"),
+            "SF": pmx.HIS_ENTITIES.index("SF"), "FF": pmx.HIS_ENTITIES.index("FF")}
+    if entity is not None and entity not in ents:
+        raise ValueError("Invalid entity")
+    ops = {"SUM": pmx.OP_SUM, "MIN": pmx.OP_MIN, "MAX": pmx.OP_MAX}
+    if operator is not None and operator not in ops:
+        raise ValueError("Invalid operator")
+    for j in activate or []:
+        if j < 0 or j >= S or _S.springs[j]["unspring"] != -1:
+            raise ValueError(f"in activate list: SPRING {j} is already active")
+    for j in msprings:
+        _S.springs[j]["unspring"] = -2   # used by UNSPRING (input.cpp:2184)
+    _S.unsprings.append({"t": [int(j) for j in tsprings], "m": [int(j) for j in msprings],
+                         "a": [int(j) for j in activate or []], "lim": (float(lo), float(hi)),
+                         "ent": ents[entity or "SF"], "op": ops[operator or "SUM"], "abs": 1 if abs else 0,
+                         "nsteps": nsteps, "nfreq": nfreq, "action": -1 if unload < 0 else _lcurve(unload)})
+    _S.dirty = True
+    return len(_S.unsprings) - 1
+
+
+def _directions(d, name):
+    """Up to three orthonormal directions from 3, 6 or 9 numbers
+    (input.cpp:2486-2559: normalised, then Gram-Schmidt)."""
+    def normalize(v):
+        ln = _norm3(v)
+        return [v[0] / ln, v[1] / ln, v[2] / ln]
+
+    def submul(a, eps, b):
+        return [a[0] - eps * b[0], a[1] - eps * b[1], a[2] - eps * b[2]]
+
+    def dot(a, b):
+        return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]
+    v = [float(x) for x in d]
+    z = [0.0, 0.0, 0.0]
+    if len(v) == 3:
+        return normalize(v[0:3]) + z + z
+    if len(v) == 6:
+        d0 = normalize(v[0:3])
+        d1 = normalize(submul(v[3:6], dot(d0, v[3:6]), d0))
+        return d0 + d1 + z
+    if len(v) == 9:
+        d0 = normalize(v[0:3])
+        d1 = normalize(submul(v[3:6], dot(d0, v[3:6]), d0))
+        d2 = submul(v[6:9], dot(d0, v[6:9]), d0)
+        d2 = normalize(submul(d2, dot(d1, d2), d1))
+        return d0 + d1 + d2
+    raise ValueError(f"Invalid {name} constrained directions list")
+
+
+def RESTRAIN(parnum, linear=None, angular=None):
+    """Restrain a particle's motion along up to three linear and three
+    angular directions (input.cpp:2458-2644): [x, y, z], [x0, y0, z0, x1,
+    y1, z1] or nine numbers, orthonormalised."""
+    if not (isinstance(parnum, int) and 0 <= parnum < len(_S.parts)):
+        raise ValueError("'parnum' must be >= 0 and < the number of particles")
+    for name, x in (("linear", linear), ("angular", angular)):
+        if x is not None and not isinstance(x, list):
+            raise TypeError(f"'{name}' must be a list")
+    if linear is None and angular is None:
+        return None
+    lin = _directions(linear, "linear") if linear is not None else [0.0] * 9
+    ang = _directions(angular, "angular") if angular is not None else [0.0] * 9
+    _S.restraints.append((parnum, lin, ang))
+    _S.dirty = True
+    return None
+
+
+def BALL_JOINT(part1, point, part2=-1):
+    """Algebraic ball joint of part1 at `point` to part2 or to the ground
+    (part2 = -1) (input.cpp:2298-2333); returns the joint number."""
+    n = len(_S.parts)
+    if not (isinstance(part1, int) and 0 <= part1 < n):
+        raise ValueError("'part1' must be >= 0 and < the number of particles")
+    pt = _tuple(point, "point", 3)
+    if part2 >= 0 and part2 >= n:
+        raise ValueError("Particle number 'part2' is out of range")
+    _S.joints.append((part1, int(part2) if part2 >= 0 else -1, pt))
+    _S.dirty = True
+    return len(_S.joints) - 1
+
+
+def _f3_arrays(m):
+    """Load curves, torsion springs (sorted by part1, part2; ties keep
+    creation order), UNSPRING conditions (spring numbers mapped to the
+    sorted springs), restraints and joints into the model."""
+    m.set_lcurves(_S.lcurves)
+    recs = _S.torsion
+    order = sorted(range(len(recs)), key=lambda i: (recs[i]["part"][0], recs[i]["part"][1], i))
+    T = len(recs)
+    _S.trqsprmap = [0] * T
+    nk = sum(len(recs[i]["k"][o][0]) for i in order for o in range(3))
+    nd = sum(len(recs[i]["d"][o][0]) for i in order for o in range(3))
+    old = _S.model
+    m.set_torsion(T, nk, nd)
+    ck = cd = 0
+    for o in range(3):
+        for k, i in enumerate(order):
+            r = recs[i]
+            _S.trqsprmap[i] = k
+            x, y = r["k"][o]
+            m.krpyidx[o, k] = ck
+            m.krpy[0, ck:ck + len(x)], m.krpy[1, ck:ck + len(x)] = x, y
+            ck += len(x)
+            x, y = r["d"][o]
+            m.drpyidx[o, k] = cd
+            m.drpy[0, cd:cd + len(x)], m.drpy[1, cd:cd + len(x)] = x, y
+            cd += len(x)
+        m.krpyidx[o, T], m.drpyidx[o, T] = ck, cd
+    for k, i in enumerate(order):
+        r = recs[i]
+        m.trqsprpart[:, k] = r["part"]
+        m.trqzdir0[:, k], m.trqxdir0[:, k] = r["z"], r["x"]
+        m.trqcone[k] = r["cone"]
+    if old is not None and old.trqsprnum == T and T:
+        for name in ("trqzdir1", "trqxdir1", "trqrpy", "trqrpytot", "trqrpyspr"):
+            getattr(m, name)[:] = getattr(old, name)
+    U = _S.unsprings
+    m.set_unsprings(len(U), sum(len(u["t"]) for u in U), sum(len(u["m"]) for u in U), sum(len(u["a"]) for u in U))
+    at = am = aa = 0
+    for q, u in enumerate(U):
+        m.tspridx[q], m.mspridx[q], m.actidx[q] = at, am, aa
+        for j in u["t"]:
+            m.tsprings[at] = _S.sprmap[j]
+            at += 1
+        for j in u["m"]:
+            m.msprings[am] = _S.sprmap[j]
+            am += 1
+        for j in u["a"]:
+            m.activate[aa] = _S.sprmap[j]
+            aa += 1
+        m.unlim[:, q] = u["lim"]
+        m.unent[q], m.unop[q], m.unabs[q] = u["ent"], u["op"], u["abs"]
+        m.nsteps[q], m.nfreq[q], m.unaction[q] = u["nsteps"], u["nfreq"], u["action"]
+    n = len(U)
+    m.tspridx[n], m.mspridx[n], m.actidx[n] = at, am, aa
+    if old is not None and old.unsprnum == n and n:
+        m.nfreq[:] = old.nfreq
+    m.set_restraints(len(_S.restraints))
+    for q, (p, lin, ang) in enumerate(_S.restraints):
+        m.rstpart[q] = p
+        m.rstlin[:, q], m.rstang[:, q] = lin, ang
+    m.set_joints(len(_S.joints))
+    for q, (p1, p2, pt) in enumerate(_S.joints):
+        m.jpart[:, q] = (p1, p2)
+        m.jpoint[:, q] = pt
+
+
 def _spring_arrays(m):
     """Spring records -> SoA arrays, sorted by (part1, part2) as sort_springs
     leaves them (parmec.cpp:1683-1966; ties keep creation order)."""
     recs = _S.springs
     order = sorted(range(len(recs)), key=lambda i: (recs[i]["part"][0], recs[i]["part"][1], i))
     S = len(recs)
+    _S.sprmap = [0] * S
+    for k, i in enumerate(order):
+        _S.sprmap[i] = k
     ncrv = sum(len(recs[i]["spring"][0]) for i in order)
     ndash = sum(len(recs[i]["dashpot"][0]) for i in order)
     nun = sum(len(recs[i]["unload"][0]) for i in order if recs[i]["unload"])
@@ -634,6 +933,7 @@ def _spring_arrays(m):
         m.sprflg[k] = r["flg"]
         m.sprfric[k], m.sprkskn[k] = r["fric"], r["kskn"]
         m.unspring[k] = r["unspring"]
+        m.sproffset[k] = r["offset"]
         m.yield_[:, k] = r["yield"]
         m.stroke0[k] = r["stroke0"]
         x, y = r["spring"]
@@ -860,6 +1160,11 @@ def _build_model():
         m.obspnt[:k], m.obslin[:k], m.obsang[:k] = old.obspnt, old.obslin, old.obsang
     if _S.springs:
         _spring_arrays(m)
+        if old is not None and old.sprnum == m.sprnum:
+            # springs keep the state UNSPRING gave them; a spring an UNSPRING
+            # made since the last run is controlled by it (-2)
+            m.unspring[:] = np.where((m.unspring == -2) & (old.unspring == -3), -2, old.unspring)
+    _f3_arrays(m)
     # sort_materials, parmec.cpp:1636-1663: pairs ordered by colours
     pairs = sorted(_S.pairs, key=lambda p: (p[0], p[1]))
     for i, (c1, c2, par) in enumerate(pairs):
@@ -905,7 +1210,9 @@ def PRESCRIBE(parnum, linear=None, angular=None, kind="vv"):
     _S.dirty = True
 
 
-_SPRING_ENTITIES = pmx.HIS_ENTITIES[pmx.HIS_SPRING_FIRST:]
+_SPRING_ENTITIES = pmx.HIS_ENTITIES[pmx.HIS_SPRING_FIRST:pmx.HIS_TORSION_FIRST]
+_TORSION_ENTITIES = pmx.HIS_ENTITIES[pmx.HIS_TORSION_FIRST:pmx.HIS_JOINT_FIRST]
+_JOINT_ENTITIES = pmx.HIS_ENTITIES[pmx.HIS_JOINT_FIRST:]
 
 
 def HISTORY(entity, source=None, point=None, h5file=None, h5last=False):
@@ -913,9 +1220,13 @@ def HISTORY(entity, source=None, point=None, h5file=None, h5last=False):
     history interval (input.cpp:3157-3555, output.cpp:5143-5520).  Entities:
     TIME; PX PY PZ |P| DX DY DZ |D| VX VY VZ |V| OX OY OZ |O| FX FY FZ |F| TX
     TY TZ |T| over a particle number or list (averaged), or one particle's
-    body point (point = (x, y, z), referential); LENGTH STROKE F SF FF over
-    spring numbers.  Torsion-spring and joint entities, sphere and box
-    sources and .h5 post-processing are not on this path."""
+    body point (point = (x, y, z), referential); LENGTH STROKE F SF FF SS
+    over spring numbers; ZDIR_X .. XDIR_Z, ROLL PITCH YAW, TRQTOT_R/P/Y and
+    TRQSPR_R/P/Y over torsion spring numbers; JREAC_X/Y/Z and |JREAC| over
+    joint numbers.  SS reads the spring's UNSPRING state and ZDIR/XDIR the
+    torsion spring's current axes (the reference indexes both through the
+    torsion spring map, output.cpp:5400-5440).  Sphere and box sources and
+    .h5 post-processing are not on this path."""
     if not isinstance(entity, str):
         raise TypeError("'entity' must be a string")
     if point is not None and not (isinstance(point, tuple) and len(point) == 3):
@@ -924,8 +1235,9 @@ def HISTORY(entity, source=None, point=None, h5file=None, h5last=False):
         raise NotImplementedError("HISTORY from .h5 files is not on the MI355X step path")
     if entity not in pmx.HIS_ENTITIES:
         raise ValueError("Invalid entity")
-    spring = entity in _SPRING_ENTITIES
-    nmax = len(_S.springs) if spring else len(_S.parts)
+    torsion, joint = entity in _TORSION_ENTITIES, entity in _JOINT_ENTITIES
+    spring = entity in _SPRING_ENTITIES or torsion or joint
+    nmax = (len(_S.torsion) if torsion else len(_S.joints) if joint else len(_S.springs)) if spring else len(_S.parts)
     if source is None:                       # useful when entity is 'TIME'
         if not _S.parts:
             raise ValueError("No particle has been defined")
@@ -945,11 +1257,23 @@ def HISTORY(entity, source=None, point=None, h5file=None, h5last=False):
         raise TypeError("'source' must be a number, a list or a tuple")
     for k in lst:
         if k < 0 or k >= nmax:
-            raise ValueError("Spring index out of range" if spring else "Particle index out of range")
+            raise ValueError(("Torsion spring" if torsion else "Joint" if joint else "Spring") + " index out of range"
+                             if spring else "Particle index out of range")
     pt = tuple(float(x) for x in point) if point is not None and len(lst) == 1 and not spring else None
     out = []
     _S.histories.append((entity, lst, pt, out))
     return out
+
+
+def _history_list(entity, lst):
+    """A history's source numbers as the model arrays index them: springs
+    and torsion springs by their place in the sorted arrays (sprmap,
+    trqsprmap), particles and joints by number."""
+    if entity in _SPRING_ENTITIES:
+        return [_S.sprmap[k] for k in lst]
+    if entity in _TORSION_ENTITIES:
+        return [_S.trqsprmap[k] for k in lst]
+    return lst
 
 
 def _interval(x, step):
@@ -1082,7 +1406,8 @@ def DEM(duration, step, interval=None, prefix=None, adaptive=0.0):
     obstab0 = vel(0.5 * h) if vel is not None and m.curtime == 0.0 else None
     if _S.histories:
         ents = [e for e, _, _, _ in _S.histories]
-        engine.set_history(ents, [l for _, l, _, _ in _S.histories], [p for _, _, p, _ in _S.histories])
+        engine.set_history(ents, [_history_list(e, l) for e, l, _, _ in _S.histories],
+                           [p for _, _, p, _ in _S.histories])
         if m.curtime == 0.0:               # the sample before the first step (parmec.cpp:2386-2394)
             v = engine.history_sample()
             for i, (e, _, _, out) in enumerate(_S.histories):

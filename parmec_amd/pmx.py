@@ -42,7 +42,20 @@ class pmx_model(C.Structure):
         ("sproffset", _ip),
         ("prsnum", C.c_int), ("prspart", _ip), ("prskind", _ip), ("prscol", _ip),
         ("curtime", C.c_double), ("curstep", C.c_double), ("stepnum", C.c_int),
+        ("lcnum", C.c_int), ("lcidx", _ip), ("lcurve", _dp), ("lccrv", C.c_int),
+        ("trqsprnum", C.c_int), ("trqsprpart", _ip), ("trqzdir0", _dp), ("trqxdir0", _dp), ("krpyidx", _ip),
+        ("krpy", _dp), ("krpycrv", C.c_int), ("drpyidx", _ip), ("drpy", _dp), ("drpycrv", C.c_int), ("trqcone", _ip),
+        ("trqzdir1", _dp), ("trqxdir1", _dp), ("trqrpy", _dp), ("trqrpytot", _dp), ("trqrpyspr", _dp),
+        ("unsprnum", C.c_int), ("tspridx", _ip), ("tsprings", _ip), ("mspridx", _ip), ("msprings", _ip),
+        ("actidx", _ip), ("activate", _ip), ("unlim", _dp), ("unent", _ip), ("unop", _ip), ("unabs", _ip),
+        ("nsteps", _ip), ("nfreq", _ip), ("unaction", _ip),
+        ("rstnum", C.c_int), ("rstpart", _ip), ("rstlin", _dp), ("rstang", _dp),
+        ("jnum", C.c_int), ("jpart", _ip), ("jpoint", _dp), ("jreac", _dp),
     ]
+
+# UNSPRING operators (constants.h:70) and torsion spring cone flags (66-68)
+OP_SUM, OP_MAX, OP_MIN = 0, 1, 2
+TRQCONE_ROLL, TRQCONE_PITCH, TRQCONE_YAW = 1, 2, 4
 
 
 class pmx_step_inputs(C.Structure):
@@ -86,8 +99,12 @@ class StepInputs:
 
 # time-history entities of HISTORY (input.cpp:3177-3281), as PMX_HIS_* in pmx.h
 HIS_ENTITIES = ["TIME", "PX", "PY", "PZ", "|P|", "DX", "DY", "DZ", "|D|", "VX", "VY", "VZ", "|V|", "OX", "OY", "OZ",
-                "|O|", "FX", "FY", "FZ", "|F|", "TX", "TY", "TZ", "|T|", "LENGTH", "STROKE", "F", "SF", "FF"]
+                "|O|", "FX", "FY", "FZ", "|F|", "TX", "TY", "TZ", "|T|", "LENGTH", "STROKE", "F", "SF", "FF", "SS",
+                "ZDIR_X", "ZDIR_Y", "ZDIR_Z", "XDIR_X", "XDIR_Y", "XDIR_Z", "ROLL", "PITCH", "YAW", "TRQTOT_R",
+                "TRQTOT_P", "TRQTOT_Y", "TRQSPR_R", "TRQSPR_P", "TRQSPR_Y", "JREAC_X", "JREAC_Y", "JREAC_Z", "|JREAC|"]
 HIS_SPRING_FIRST = HIS_ENTITIES.index("LENGTH")
+HIS_TORSION_FIRST = HIS_ENTITIES.index("ZDIR_X")
+HIS_JOINT_FIRST = HIS_ENTITIES.index("JREAC_X")
 
 
 class pmx_history(C.Structure):
@@ -231,6 +248,66 @@ class Model:
         self.set_springs(0, 0, 0, 0)
         self.set_obstacles(0)
         self.set_prescribed(0)
+        self.set_lcurves([])
+        self.set_torsion(0, 0, 0)
+        self.set_unsprings(0, 0, 0, 0)
+        self.set_restraints(0)
+        self.set_joints(0)
+
+    def set_lcurves(self, curves):
+        """Load curves (lcurve_from_time_series, parmec.cpp:1239-1269): a list
+        of (x list, y list); lcidx [lcnum+1], lcurve [2][lccrv]."""
+        self.lcnum = len(curves)
+        self.lccrv = sum(len(x) for x, _ in curves)
+        self.lcidx = np.zeros(self.lcnum + 1, dtype=np.int32)
+        self.lcurve = np.zeros((2, self.lccrv))
+        k = 0
+        for i, (x, y) in enumerate(curves):
+            self.lcidx[i] = k
+            self.lcurve[0, k:k + len(x)], self.lcurve[1, k:k + len(x)] = x, y
+            k += len(x)
+        self.lcidx[self.lcnum] = k
+
+    def set_torsion(self, trqsprnum, krpycrv, drpycrv):
+        """Torsion springs (parmec.cpp:180-198): parts [2][n], reference axes
+        [3][n], curve index ranges [3][n+1] into the shared krpy, drpy tables
+        [2][crv], cone flags, and the state and outputs [3][n]."""
+        T = trqsprnum
+        self.trqsprnum, self.krpycrv, self.drpycrv = T, krpycrv, drpycrv
+        self.trqsprpart = np.zeros((2, T), dtype=np.int32)
+        self.trqzdir0, self.trqxdir0 = np.zeros((3, T)), np.zeros((3, T))
+        self.krpyidx, self.drpyidx = np.zeros((3, T + 1), dtype=np.int32), np.zeros((3, T + 1), dtype=np.int32)
+        self.krpy, self.drpy = np.zeros((2, krpycrv)), np.zeros((2, drpycrv))
+        self.trqcone = np.zeros(T, dtype=np.int32)
+        self.trqzdir1, self.trqxdir1, self.trqrpy = np.zeros((3, T)), np.zeros((3, T)), np.zeros((3, T))
+        self.trqrpytot, self.trqrpyspr = np.zeros((3, T)), np.zeros((3, T))
+
+    def set_unsprings(self, unsprnum, ntest, nswitch, nact):
+        """UNSPRING conditions (input.cpp:2083-2295): tested, switched-off and
+        activated spring lists (index ranges [n+1]), limits [2][n], entity,
+        operator, abs, nsteps, nfreq (state) and action per condition."""
+        U = unsprnum
+        self.unsprnum = U
+        self.tspridx, self.tsprings = np.zeros(U + 1, dtype=np.int32), np.zeros(ntest, dtype=np.int32)
+        self.mspridx, self.msprings = np.zeros(U + 1, dtype=np.int32), np.zeros(nswitch, dtype=np.int32)
+        self.actidx, self.activate = np.zeros(U + 1, dtype=np.int32), np.zeros(nact, dtype=np.int32)
+        self.unlim = np.zeros((2, U))
+        self.unent, self.unop, self.unabs = (np.zeros(U, dtype=np.int32) for _ in range(3))
+        self.nsteps, self.nfreq = np.ones(U, dtype=np.int32), np.ones(U, dtype=np.int32)
+        self.unaction = np.full(U, -1, dtype=np.int32)
+
+    def set_restraints(self, rstnum):
+        """RESTRAIN entries: particle, linear and angular directions [9][n]."""
+        self.rstnum = rstnum
+        self.rstpart = np.zeros(rstnum, dtype=np.int32)
+        self.rstlin, self.rstang = np.zeros((9, rstnum)), np.zeros((9, rstnum))
+
+    def set_joints(self, jnum):
+        """BALL_JOINT entries: parts [2][n] (part2 -1: ground), point [3][n],
+        reactions [3][n] (output)."""
+        self.jnum = jnum
+        self.jpart = np.full((2, jnum), -1, dtype=np.int32)
+        self.jpoint, self.jreac = np.zeros((3, jnum)), np.zeros((3, jnum))
 
     def set_prescribed(self, prsnum):
         """Prescribed motion entries (parmec.h:244-251): particle, and the kind
@@ -292,9 +369,13 @@ class Model:
 
     _DBL = ("angular", "linear", "rotation", "position", "inertia", "inverse", "mass", "invm", "force", "torque",
             "center", "radii", "tri", "iparam", "nodes", "sprpnt", "spring", "dashpot", "unload", "yield_", "sprdir",
-            "sprfric", "sprkskn", "sprsdsp", "stroke0", "stroke", "sprfrc", "obspnt", "obslin", "obsang")
+            "sprfric", "sprkskn", "sprsdsp", "stroke0", "stroke", "sprfrc", "obspnt", "obslin", "obsang",
+            "lcurve", "trqzdir0", "trqxdir0", "krpy", "drpy", "trqzdir1", "trqxdir1", "trqrpy", "trqrpytot", "trqrpyspr",
+            "unlim", "rstlin", "rstang", "jpoint", "jreac")
     _INT = ("flags", "part", "ellcol", "tricol", "triobs", "pairs", "ikind", "nodpart", "facnod", "factri", "sprtype",
-            "sprpart", "spridx", "dashidx", "unidx", "sprflg", "unspring", "sproffset", "trirng", "prspart", "prskind", "prscol")
+            "sprpart", "spridx", "dashidx", "unidx", "sprflg", "unspring", "sproffset", "trirng", "prspart", "prskind", "prscol",
+            "lcidx", "trqsprpart", "krpyidx", "drpyidx", "trqcone", "tspridx", "tsprings", "mspridx", "msprings",
+            "actidx", "activate", "unent", "unop", "unabs", "nsteps", "nfreq", "unaction", "rstpart", "jpart")
 
     def check(self):
         for name in self._DBL:
@@ -314,6 +395,9 @@ class Model:
         s.obsnum = getattr(self, "obsnum", 0)
         s.prsnum = getattr(self, "prsnum", 0)
         s.sprcrv, s.dashcrv, s.uncrv = self.sprcrv, self.dashcrv, self.uncrv
+        s.lcnum, s.lccrv = self.lcnum, self.lccrv
+        s.trqsprnum, s.krpycrv, s.drpycrv = self.trqsprnum, self.krpycrv, self.drpycrv
+        s.unsprnum, s.rstnum, s.jnum = self.unsprnum, self.rstnum, self.jnum
         for name in self._DBL:
             setattr(s, name, _ptr(getattr(self, name), "d"))
         for name in self._INT:
