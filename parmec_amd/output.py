@@ -8,6 +8,8 @@ The file names and the contents follow the reference writers:
   <path><k>.vtk.<frame>     triangles (MESH mode)
   <path><k>rb.vtk.<frame>   rigid bodies: particle positions (RB mode)
   <path><k>sl.vtk.<frame>   linear springs (SL mode)
+  <path><k>st.vtk.<frame>   torsion springs (ST mode)
+  <path><k>jt.vtk.<frame>   ball joints (JT mode)
 where k = 0 is the set of particles no OUTPUT subset lists and k = j + 1 the
 j-th OUTPUT record.  Numbers are written as C++ streams write doubles
 (six significant digits, like %g).  XDMF/HDF5 and MED are not written (no
@@ -19,8 +21,10 @@ from . import pmx
 
 E = {"NUMBER": 1, "COLOR": 2, "DISPL": 4, "ORIENT": 8, "ORIENT1": 16, "ORIENT2": 32, "ORIENT3": 64, "LINVEL": 128,
      "ANGVEL": 256, "FORCE": 512, "TORQUE": 1024, "F": 2048, "SF": 16384, "LENGTH": 131072, "SS": 262144,
-     "FF": 524288}
-M_SPH, M_MESH, M_RB, M_SL = 1, 2, 4, 16
+     "FF": 524288, "XDIR": 1048576, "YDIR": 2097152, "ZDIR": 4194304, "TRQROT": 8388608, "TRQTOT": 16777216,
+     "TRQSPR": 33554432, "JREAC": 67108864}
+M_SPH, M_MESH, M_RB, M_SL, M_ST, M_JT = 1, 2, 4, 16, 32, 64
+_REAL_MAX = 1.7976931348623157e308
 OUTREST = 2
 
 
@@ -192,11 +196,89 @@ def _spring_dataset(m, sset, sprid, ent, out):
         scal("SS", lambda j: float(np.float32(m.unspring[j])))
 
 
-def write(m, t, frame, path, outformat, records, outrest, flags, sprid):
+def _body_point(m, k, P):
+    """NVADDMUL (x, L, P - X): a referential point of particle k now."""
+    d = [P[0] - m.position[3, k], P[1] - m.position[4, k], P[2] - m.position[5, k]]
+    L = m.rotation[:, k]
+    return [m.position[c, k] + (L[c] * d[0] + L[3 + c] * d[1] + L[6 + c] * d[2]) for c in range(3)]
+
+
+def _torsion_dataset(m, tset, trqsprid, refpnt, ent, out):
+    """vtk_torsional_spring_dataset, output.cpp:2969-3142 (torsion spring
+    numbers in the sorted arrays; trqsprid their creation numbers, refpnt
+    their reference points or REAL_MAX)."""
+    n = len(tset)
+    out.write(f"DATASET UNSTRUCTURED_GRID\nPOINTS {n} float\n")
+    for j in tset:
+        k, l = m.trqsprpart[0, j], m.trqsprpart[1, j]
+        r = refpnt[j]
+        if r[0] != _REAL_MAX:
+            p = _body_point(m, k, r)
+        elif l == -1:
+            p = list(m.position[:3, k])
+        else:
+            p = [0.5 * (m.position[c, k] + m.position[c, l]) for c in range(3)]
+        out.write(vec(*p) + "\n")
+    if ent & (E["NUMBER"] | E["ZDIR"] | E["XDIR"] | E["YDIR"] | E["TRQROT"] | E["TRQTOT"] | E["TRQSPR"]):
+        out.write(f"POINT_DATA {n}\n")
+    if ent & E["NUMBER"]:
+        out.write("SCALARS number int\nLOOKUP_TABLE default\n" + "".join(f"{trqsprid[j]}\n" for j in tset))
+
+    def axes(j):
+        z, x = m.trqzdir1[:, j], m.trqxdir1[:, j]
+        y = [z[1] * x[2] - z[2] * x[1], z[2] * x[0] - z[0] * x[2], z[0] * x[1] - z[1] * x[0]]
+        return x, y, z
+
+    def frame(j, a):
+        x, y, z = axes(j)
+        return [x[c] * a[0, j] + y[c] * a[1, j] + z[c] * a[2, j] for c in range(3)]
+    if ent & E["ZDIR"]:
+        out.write("VECTORS ZDIR float\n" + "".join(vec(*m.trqzdir1[:, j]) + "\n" for j in tset))
+    if ent & E["XDIR"]:
+        out.write("VECTORS XDIR float\n" + "".join(vec(*m.trqxdir1[:, j]) + "\n" for j in tset))
+    if ent & E["YDIR"]:
+        out.write("VECTORS YDIR float\n" + "".join(vec(*axes(j)[1]) + "\n" for j in tset))
+    if ent & E["TRQROT"]:
+        out.write("VECTORS TRQROT float\n" + "".join(vec(*frame(j, m.trqrpy)) + "\n" for j in tset))
+    if ent & E["TRQTOT"]:
+        out.write("VECTORS TRQTOT float\n" + "".join(vec(*frame(j, m.trqrpytot)) + "\n" for j in tset))
+    if ent & E["TRQSPR"]:
+        out.write("VECTORS TRQSPR float\n" + "".join(vec(*frame(j, m.trqrpyspr)) + "\n" for j in tset))
+
+
+def _joint_dataset(m, jset, ent, out):
+    """vtk_joints_dataset, output.cpp:3145-3197: the joint point on part1
+    now (the reference indexes part1 through the torsion spring array,
+    output.cpp:3157; here the joint's own part1), its number and reaction."""
+    n = len(jset)
+    out.write(f"DATASET UNSTRUCTURED_GRID\nPOINTS {n} float\n")
+    for j in jset:
+        out.write(vec(*_body_point(m, m.jpart[0, j], m.jpoint[:, j])) + "\n")
+    if ent & (E["NUMBER"] | E["JREAC"]):
+        out.write(f"POINT_DATA {n}\n")
+    if ent & E["NUMBER"]:
+        out.write("SCALARS number int\nLOOKUP_TABLE default\n" + "".join(f"{j}\n" for j in jset))
+    if ent & E["JREAC"]:
+        out.write("VECTORS JREAC float\n" + "".join(vec(*m.jreac[:, j]) + "\n" for j in jset))
+
+
+def _attached(parts_of, count, parts, flags):
+    """The items (springs, torsion springs, joints) output for a record:
+    those on its particles in item order per particle, or, for the rest,
+    those whose part1 has any flag or whose part2 is unlisted
+    (output.cpp:4551-4552, 4623-4624, 4695-4696)."""
+    p1, p2 = parts_of
+    if parts is None:
+        return [k for k in range(count) if flags[p1[k]] != 0 or (p2[k] >= 0 and flags[p2[k]] & OUTREST)]
+    return [k for p in parts for k in range(count) if p1[k] == p or p2[k] == p]
+
+
+def write(m, t, frame, path, outformat, records, outrest, flags, sprid, trq=None):
     """output_files (output.cpp:5127-5141) for the state m at curtime t:
     records = [(particles, entities, modes)] of the OUTPUT calls; outrest =
     (entities, modes) of the particles no record lists; flags: particle
-    flags with OUTREST cleared for listed particles."""
+    flags with OUTREST cleared for listed particles; trq = (creation numbers,
+    reference points) of the sorted torsion springs."""
     written = []
     sets = [(-1, None, outrest[0], outrest[1])] + [(j, parts, ent, mode) for j, (parts, ent, mode) in
                                                     enumerate(records)]
@@ -254,5 +336,28 @@ def write(m, t, frame, path, outformat, records, outrest, flags, sprid):
                     out.write(f"# vtk DataFile Version 2.0\nPARMEC linear springs output at time {g(t)}\nASCII\n")
                     if parts is None or sset:
                         _spring_dataset(m, sset, sprid, ent, out)
+                written.append(name)
+        if getattr(m, "trqsprnum", 0):
+            ids, refs = trq if trq is not None else (list(range(m.trqsprnum)), [(_REAL_MAX,) * 3] * m.trqsprnum)
+            for j, parts, ent, mode in sets:
+                if not (mode & M_ST):
+                    continue
+                tset = _attached((m.trqsprpart[0], m.trqsprpart[1]), m.trqsprnum, parts, flags)
+                name = f"{path}{j + 1}st.vtk.{frame}"
+                with open(name, "w") as out:
+                    out.write(f"# vtk DataFile Version 2.0\nPARMEC torsional springs output at time {g(t)}\nASCII\n")
+                    if parts is None or tset:
+                        _torsion_dataset(m, tset, ids, refs, ent, out)
+                written.append(name)
+        if getattr(m, "jnum", 0):
+            for j, parts, ent, mode in sets:
+                if not (mode & M_JT):
+                    continue
+                jset = _attached((m.jpart[0], m.jpart[1]), m.jnum, parts, flags)
+                name = f"{path}{j + 1}jt.vtk.{frame}"
+                with open(name, "w") as out:
+                    out.write(f"# vtk DataFile Version 2.0\nPARMEC joints output at time {g(t)}\nASCII\n")
+                    if parts is None or jset:
+                        _joint_dataset(m, jset, ent, out)
                 written.append(name)
     return written

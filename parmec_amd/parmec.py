@@ -1063,7 +1063,10 @@ def _write_output(m, t, prefix_path):
             flags[k] &= ~_OUTREST
     recs = _S.springs
     order = sorted(range(len(recs)), key=lambda i: (recs[i]["part"][0], recs[i]["part"][1], i))
-    output.write(m, t, _S.output_frame, prefix_path, _S.outformat, _S.outputs, _S.outrest, flags, order)
+    trs = _S.torsion
+    torder = sorted(range(len(trs)), key=lambda i: (trs[i]["part"][0], trs[i]["part"][1], i))
+    output.write(m, t, _S.output_frame, prefix_path, _S.outformat, _S.outputs, _S.outrest, flags, order,
+                 (torder, [trs[i]["refpnt"] for i in torder]))
     _S.output_frame += 1
 
 

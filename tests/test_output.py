@@ -110,3 +110,45 @@ def test_output_argument_errors():
         P.OUTPUT(format="XDMF")
     with pytest.raises(TypeError):
         P.OUTPUT(entities="NUMBER")
+
+
+def test_torsion_spring_and_joint_files(tmp_path):
+    """ST and JT modes (output.cpp:2969-3197, 4590-4730): a torsion spring
+    between two cubes and a ball joint holding one of them, through DEM on
+    the oracle; the files carry each item's point, number and vectors as
+    the state gives them."""
+    from oracle import orc
+    from tests import f3_decks as D
+    mat = P.MATERIAL(1E3, 1E9, 0.25)
+    a = D.cube(mat, 0, 0, 0)
+    b = D.cube(mat, 0, 0, 1)
+    P.TORSION_SPRING(a, b, (0, 0, 1), (1, 0, 0), kroll=D.K, kpitch=D.K, kyaw=D.K, refpnt=(0.5, 0.5, 1.0))
+    P.BALL_JOINT(b, (0.5, 0.5, 2.0))
+    P.VELOCITY(a, angular=(0.5, -0.2, 0.1))
+    P.GRAVITY(0.0, 0.0, -9.81)
+    P.OUTPUT(["NUMBER", "ZDIR", "XDIR", "YDIR", "TRQROT", "TRQTOT", "TRQSPR", "JREAC"], [a, b], mode=["ST", "JT"],
+             format=["VTK"])
+    P._use_engine(orc.Oracle(1))
+    prefix = os.path.join(str(tmp_path), "run")
+    # output points after the steps starting at 0.005 and 0.010; the last is
+    # the final state
+    P.DEM(0.0105, 1e-3, 0.005, prefix=prefix)
+    m = P.state()
+    st = sorted(glob.glob(prefix + "1st.vtk.*"))
+    jt = sorted(glob.glob(prefix + "1jt.vtk.*"))
+    assert st and jt
+    last = max(st, key=lambda p: int(p.rsplit(".", 1)[1]))
+    text = open(last).read()
+    # the spring is listed once for each of its particles, as the
+    # reference's per-particle maps list it
+    assert "PARMEC torsional springs output" in text and "POINTS 2 float" in text
+    z = m.trqzdir1[:, 0]
+    assert "VECTORS ZDIR float\n%g %g %g\n%g %g %g\n" % (tuple(z) * 2) in text
+    L = m.rotation[:, a]
+    d = np.array([0.5, 0.5, 1.0]) - m.position[3:, a]
+    p = [m.position[c, a] + (L[c] * d[0] + L[3 + c] * d[1] + L[6 + c] * d[2]) for c in range(3)]
+    assert text.split("POINTS 2 float\n")[1].split("\n")[0] == "%g %g %g" % tuple(p)
+    last = max(jt, key=lambda p: int(p.rsplit(".", 1)[1]))
+    text = open(last).read()
+    assert "VECTORS JREAC float\n%g %g %g\n" % tuple(m.jreac[:, 0]) in text
+    assert "SCALARS number int\nLOOKUP_TABLE default\n0\n" in text
