@@ -1628,7 +1628,7 @@ void torsion_springs (orc_ctx &s, double step)
   const double *kx = s.krpy.data (), *ky = kx + s.krpycrv, *dx = s.drpy.data (), *dy = dx + s.drpycrv;
   auto kidx = [&] (int o, int i) { return s.krpyidx[o*(TT+1) + i]; };
   auto didx = [&] (int o, int i) { return s.drpyidx[o*(TT+1) + i]; };
-  double oj[3], Lj[9], ok[3];
+  double oj[3] = {0, 0, 0}, Lj[9], ok[3] = {0, 0, 0};   /* loaded at the first spring (j0, k0 start at -1) */
   int j0 = -1, k0 = -1;
   double tj[3] = {0, 0, 0}, tk[3] = {0, 0, 0};
   double krot_j[6] = {0, 0, 0, 0, 0, 0}, krot_k[6] = {0, 0, 0, 0, 0, 0};

@@ -31,7 +31,10 @@ template <class T> int up (pmx_ctx &c, dvec<T> &d, const T *src, size_t n, const
   if (d.alloc (std::max (n, (size_t) 1)) != hipSuccess) return fail (PMX_ENOMEM, std::string ("out of device memory: ") + what);
   if (n == 0) return PMX_OK;
   if (!src) return fail (PMX_EINVAL, std::string ("missing array: ") + what);
+  /* the source is pageable host memory that may be freed when this returns:
+   * the copy completes first */
   PMX_HIP (hipMemcpyAsync (d.p, src, n * sizeof (T), hipMemcpyHostToDevice, c.stream));
+  PMX_HIP (hipStreamSynchronize (c.stream));
   return PMX_OK;
 }
 

@@ -480,11 +480,13 @@ int jt_setup (pmx_ctx &c, const pmx_model *m)
   auto upi = [&] (dvec<int> &d, const int *src, size_t n) -> int {
     if (d.alloc (std::max (n, (size_t) 1)) != hipSuccess) return fail (PMX_ENOMEM, "out of device memory (joints)");
     if (n) PMX_HIP (hipMemcpyAsync (d.p, src, n * sizeof (int), hipMemcpyHostToDevice, c.stream));
+    PMX_HIP (hipStreamSynchronize (c.stream));   /* the source may be freed once this returns */
     return PMX_OK;
   };
   auto upd = [&] (dvec<double> &d, const double *src, size_t n) -> int {
     if (d.alloc (std::max (n, (size_t) 1)) != hipSuccess) return fail (PMX_ENOMEM, "out of device memory (joints)");
     if (n) PMX_HIP (hipMemcpyAsync (d.p, src, n * sizeof (double), hipMemcpyHostToDevice, c.stream));
+    PMX_HIP (hipStreamSynchronize (c.stream));
     return PMX_OK;
   };
   /* restraints: entries grouped by particle, entry order kept */
@@ -609,6 +611,7 @@ int jt_setup (pmx_ctx &c, const pmx_model *m)
       return fail (PMX_ENOMEM, "out of device memory (joints)");
     PMX_HIP (hipMemcpyAsync (c.jt_moff.p, moff.data (), JS * sizeof (long long), hipMemcpyHostToDevice, c.stream));
     PMX_HIP (hipMemcpyAsync (c.jt_woff.p, woff.data (), JS * sizeof (long long), hipMemcpyHostToDevice, c.stream));
+    PMX_HIP (hipStreamSynchronize (c.stream));
   }
   PMX_HIP (hipStreamSynchronize (c.stream));
   return PMX_OK;
@@ -669,5 +672,6 @@ int step_times_table (pmx_ctx &c, const double *times, int n)
 {
   if (c.ttab.alloc (std::max (n, 1)) != hipSuccess) return fail (PMX_ENOMEM, "out of device memory (step times)");
   if (n > 0) PMX_HIP (hipMemcpyAsync (c.ttab.p, times, n * sizeof (double), hipMemcpyHostToDevice, c.stream));
+  PMX_HIP (hipStreamSynchronize (c.stream));
   return PMX_OK;
 }

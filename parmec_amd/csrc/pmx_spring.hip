@@ -1066,11 +1066,13 @@ int springs_setup (pmx_ctx &c, const pmx_model *m)
   auto upi = [&] (dvec<int> &d, const int *src, size_t n) -> int {
     if (d.alloc (std::max (n, (size_t) 1)) != hipSuccess) { pmx_set_error ("out of device memory (springs)"); return PMX_ENOMEM; }
     if (n) PMX_HIP (hipMemcpyAsync (d.p, src, n * sizeof (int), hipMemcpyHostToDevice, c.stream));
+    PMX_HIP (hipStreamSynchronize (c.stream));   /* the source may be freed once this returns */
     return PMX_OK;
   };
   auto upd = [&] (dvec<double> &d, const double *src, size_t n) -> int {
     if (d.alloc (std::max (n, (size_t) 1)) != hipSuccess) { pmx_set_error ("out of device memory (springs)"); return PMX_ENOMEM; }
     if (n) PMX_HIP (hipMemcpyAsync (d.p, src, n * sizeof (double), hipMemcpyHostToDevice, c.stream));
+    PMX_HIP (hipStreamSynchronize (c.stream));
     return PMX_OK;
   };
   int rc;
@@ -1172,11 +1174,13 @@ int torsion_setup (pmx_ctx &c, const pmx_model *m)
   auto upi = [&] (dvec<int> &d, const int *src, size_t n) -> int {
     if (d.alloc (std::max (n, (size_t) 1)) != hipSuccess) { pmx_set_error ("out of device memory (torsion springs)"); return PMX_ENOMEM; }
     if (n) PMX_HIP (hipMemcpyAsync (d.p, src, n * sizeof (int), hipMemcpyHostToDevice, c.stream));
+    PMX_HIP (hipStreamSynchronize (c.stream));   /* the source may be freed once this returns */
     return PMX_OK;
   };
   auto upd = [&] (dvec<double> &d, const double *src, size_t n) -> int {
     if (d.alloc (std::max (n, (size_t) 1)) != hipSuccess) { pmx_set_error ("out of device memory (torsion springs)"); return PMX_ENOMEM; }
     if (n) PMX_HIP (hipMemcpyAsync (d.p, src, n * sizeof (double), hipMemcpyHostToDevice, c.stream));
+    PMX_HIP (hipStreamSynchronize (c.stream));
     return PMX_OK;
   };
   int rc;

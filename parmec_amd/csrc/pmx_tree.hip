@@ -849,6 +849,7 @@ int tree_build (pmx_ctx &c, bool all)
       if (t.sel.alloc (std::max (m, 1)) || t.scode.alloc (std::max (m, 1)) || t.sorder.alloc (std::max (m, 1)))
       { pmx_set_error ("out of device memory (tree)"); return PMX_ENOMEM; }
       if (m > 0) PMX_HIP (hipMemcpyAsync (t.sel.p, list.data (), sizeof (int) * m, hipMemcpyHostToDevice, c.stream));
+      PMX_HIP (hipStreamSynchronize (c.stream));   /* list is freed when this function returns */
       sel = t.sel.p;
       t.last_partial = true;
     }
