@@ -254,6 +254,16 @@ int upload_core (pmx_ctx *c, const pmx_model *m, const std::vector<InstOff> &off
   c->nzc = nzc;
   c->nfree = nfree;
   if ((rc = up (*c, c->iso, iso.data (), N, "iso"))) return rc;
+  if (niso > 0 && niso < (int) N)
+  {
+    /* both inertia forms: k_dynamics runs once per form over its own
+     * particles, the scalar ones first (ascending within each) */
+    std::vector<int> dl;
+    dl.reserve (N);
+    for (int i = 0; i < (int) N; i++) if (iso[i] & 1) dl.push_back (i);
+    for (int i = 0; i < (int) N; i++) if (!(iso[i] & 1)) dl.push_back (i);
+    if ((rc = up (*c, c->dynlist, dl.data (), N, "dynamics lists"))) return rc;
+  }
   if ((rc = up (*c, c->parell, parell.data (), N, "parell")) ||
       (rc = up (*c, c->nosph, nosph.data (), nosph.size (), "nosph")))
     return rc;
@@ -301,22 +311,38 @@ int upload_core (pmx_ctx *c, const pmx_model *m, const std::vector<InstOff> &off
     for (int i = 0; i < m->obsnum; i++)
       for (int t = m->trirng[2*i]; t < m->trirng[2*i+1]; t++) { mtri.push_back (t); mtobs.push_back (i); tmov[t] = 1; }
     std::vector<int> mp_part, mp_toff (1, 0), mp_tlist;
-    if (m->ellnum > 0)
-    {
-      std::vector<std::vector<int>> bypart (std::max (m->parnum, 1));
-      for (int t = 0; t < m->trinum; t++)
-        if (m->triobs[t] >= 0) { mtri.push_back (t); mtobs.push_back (-1); tmov[t] = 1; bypart[m->triobs[t]].push_back (t); }
-      for (int p = 0; p < m->parnum; p++)
-        if (!bypart[p].empty ())
-        {
-          mp_part.push_back (p);
-          mp_tlist.insert (mp_tlist.end (), bypart[p].begin (), bypart[p].end ());
-          mp_toff.push_back ((int) mp_tlist.size ());
-        }
-    }
+    /* a meshed particle's triangles can meet spheres only in an instance
+     * that has spheres: only those move every step (the others' nodes and
+     * faces are formed where the state is read) */
+    std::vector<std::vector<int>> bypart (std::max (m->parnum, 1));
+    for (int i = 0; i < I; i++)
+      if (off[i+1].ell > off[i].ell)
+        for (int t = off[i].tri; t < off[i+1].tri; t++)
+          if (m->triobs[t] >= 0) { mtri.push_back (t); mtobs.push_back (-1); tmov[t] = 1; bypart[m->triobs[t]].push_back (t); }
+    std::vector<char> contact (std::max (m->parnum, 1), 0);
+    for (int p = 0; p < m->parnum; p++)
+      if (!bypart[p].empty ())
+      {
+        mp_part.push_back (p);
+        contact[p] = 1;
+        mp_tlist.insert (mp_tlist.end (), bypart[p].begin (), bypart[p].end ());
+        mp_toff.push_back ((int) mp_tlist.size ());
+      }
     c->nmtri = (int) mtri.size ();
     c->nmp = (int) mp_part.size ();
     c->mesh_contact = c->nmp > 0;
+    /* the nodes and faces moved every step, where they are not all of them */
+    std::vector<int> mnod, mfac;
+    if (c->mesh_contact)
+    {
+      for (int q = 0; q < m->nodnum; q++) if (contact[m->nodpart[q]]) mnod.push_back (q);
+      for (int q = 0; q < m->facnum; q++) if (tmov[m->factri[q]]) mfac.push_back (q);
+    }
+    c->nmnod = (int) mnod.size () < m->nodnum ? (int) mnod.size () : -1;
+    c->nmfac = (int) mfac.size () < m->facnum ? (int) mfac.size () : -1;
+    if ((c->nmnod >= 0 && (rc = up (*c, c->mnod, mnod.data (), mnod.size (), "mesh lists"))) ||
+        (c->nmfac >= 0 && (rc = up (*c, c->mfac, mfac.data (), mfac.size (), "mesh lists"))))
+      return rc;
     size_t O = c->O;
     if ((rc = up (*c, c->obspnt, m->obspnt, 3*O, "obspnt")) || (rc = up (*c, c->obsang, m->obsang, 3*O, "obsang")) ||
         (rc = up (*c, c->obslin, m->obslin, 3*O, "obslin")) || (rc = up (*c, c->mtri, mtri.data (), mtri.size (), "obstacles")) ||

@@ -292,6 +292,9 @@ struct pmx_ctx
    * particle over its triangles (mp_toff/mp_tlist, ascending) by k_react */
   bool mesh_contact = false;
   int tcap = 8, nmp = 0;
+  int nmnod = -1, nmfac = -1;  /* nodes and faces of the meshed particles in mp_part (mnod, mfac), or -1:
+                                  all of them */
+  dvec<int> mnod, mfac;
   dvec<int> tcnt, trec_e, mp_part, mp_toff, mp_tlist;
   dvec<double> trec;
   bool adp_on = false;         /* the step forms the adaptive sums (k_rows<R, true>) */
@@ -315,6 +318,7 @@ struct pmx_ctx
                                   sphere's body offset is +0; bit 2: loaded by contacts and gravity
                                   only (no spring, no prescribed motion) */
   int niso = 0;
+  dvec<int> dynlist;           /* with both inertia forms: the niso scalar-inertia particles, then the rest */
   int nzc = 0;                 /* particles whose sphere offset is +0 */
   int nfree = 0;               /* particles with flag bit 2 (loaded by contacts and gravity only) */
   bool nbr_csr = false;        /* neighbour lists always in CSR layout (PMX_NEIGHBOUR_LISTS=csr) */

@@ -642,11 +642,12 @@ __global__ void k_crit_perspring (SprArgs A, double *h, double *o, double *r)
 
 /* nodes_task, shapes.ispc:101-186 */
 __global__ void k_nodes (int NOD, int N, const int *nodpart, double *nodes, const double *rotation, const double *position,
-                         const int *ctl, const int *stop, const int *par_inst)
+                         const int *ctl, const int *stop, const int *par_inst, const int *list, int nlist)
 {
   if (ctl && ctl[CTL_ABORT]) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= NOD) return;
+  if (i >= nlist) return;
+  if (list) i = list[i];
   int j = nodpart[i];
   if (ctl && frozen (stop, par_inst[j], ctl)) return;
   double L[9], C[3];
@@ -661,11 +662,12 @@ __global__ void k_nodes (int NOD, int N, const int *nodpart, double *nodes, cons
 
 /* faces_task, shapes.ispc:189-211 */
 __global__ void k_faces (int FAC, int NOD, int T, const int *facnod, const int *factri, const double *nodes, double *tri,
-                         const int *ctl, const int *stop, const int *tri_inst)
+                         const int *ctl, const int *stop, const int *tri_inst, const int *list, int nlist)
 {
   if (ctl && ctl[CTL_ABORT]) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= FAC) return;
+  if (i >= nlist) return;
+  if (list) i = list[i];
   int t = factri[i];
   if (ctl && frozen (stop, tri_inst[t], ctl)) return;
 #pragma unroll
@@ -1358,17 +1360,26 @@ int crit_perspring (pmx_ctx &c, double *h, double *o, double *r)
   return PMX_OK;
 }
 
+/* the nodes and faces of meshed particles from their particles' current
+ * pose; gated (inside the step): only those that spheres can touch
+ * (mnod, mfac where not all), the others formed ungated where the state is
+ * read */
 int step_mesh_shapes (pmx_ctx &c, bool gated)
 {
   if (c.NOD == 0) return PMX_OK;
   const int *ctl = gated ? (const int*) c.ctl.p : nullptr;
-  hipLaunchKernelGGL (k_nodes, grid (c.NOD), dim3 (PMX_TPB), 0, c.stream, c.NOD, c.N, (const int*) c.nodpart.p,
-                      c.nodes.p, (const double*) c.rotation.p, (const double*) c.position.p, ctl,
-                      c.staggered ? (const int*) c.istop.p : (const int*) nullptr, (const int*) c.par_inst.p);
-  if (c.FAC > 0)
-    hipLaunchKernelGGL (k_faces, grid (c.FAC), dim3 (PMX_TPB), 0, c.stream, c.FAC, c.NOD, c.T, (const int*) c.facnod.p,
+  const bool ln = gated && c.nmnod >= 0, lf = gated && c.nmfac >= 0;
+  const int nn = ln ? c.nmnod : c.NOD, nf = lf ? c.nmfac : c.FAC;
+  if (nn > 0)
+    hipLaunchKernelGGL (k_nodes, grid (nn), dim3 (PMX_TPB), 0, c.stream, c.NOD, c.N, (const int*) c.nodpart.p,
+                        c.nodes.p, (const double*) c.rotation.p, (const double*) c.position.p, ctl,
+                        c.staggered ? (const int*) c.istop.p : (const int*) nullptr, (const int*) c.par_inst.p,
+                        ln ? (const int*) c.mnod.p : (const int*) nullptr, nn);
+  if (nf > 0)
+    hipLaunchKernelGGL (k_faces, grid (nf), dim3 (PMX_TPB), 0, c.stream, c.FAC, c.NOD, c.T, (const int*) c.facnod.p,
                         (const int*) c.factri.p, (const double*) c.nodes.p, c.tri.p, ctl,
-                        c.staggered ? (const int*) c.istop.p : (const int*) nullptr, (const int*) c.tri_inst.p);
+                        c.staggered ? (const int*) c.istop.p : (const int*) nullptr, (const int*) c.tri_inst.p,
+                        lf ? (const int*) c.mfac.p : (const int*) nullptr, nf);
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
 }

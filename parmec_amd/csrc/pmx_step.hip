@@ -1290,12 +1290,13 @@ __global__ void k_dynamics (int N, double *angular, double *linear, double *rota
                             const double *dtab, const int *ctl, double *force, double *torque, double step0, double step1,
                             const int *parell, int E, double *center, const signed char *iso, int ninst,
                             const int *par_inst, const int *stop, const int *cnt0, const int *cnt1, int cur0,
-                            const double *gtab)
+                            const double *gtab, const int *list, int nlist)
 {
   if (ctl[CTL_ABORT]) return;
   int kstep = ctl[CTL_KSTEP];
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
+  if (i >= nlist) return;
+  if (list) i = list[i];
   int flag = iso[i];
   if ((flag & 1) != (int) ISO || frozen (stop, par_inst[i], ctl)) return;
   const double *damping = dtab + 6*((size_t) kstep*ninst + (ninst > 1 ? par_inst[i] : 0));
@@ -1869,8 +1870,13 @@ int step_dynamics (pmx_ctx &c, double step0, double step1)
                  step0, step1, (const int*) c.parell.p, c.E, c.center.p, (const signed char*) c.iso.p, c.I, \
                  (const int*) c.par_inst.p, c.staggered ? (const int*) c.istop.p : (const int*) nullptr, \
                  (const int*) c.rowcnt[0].p, (const int*) c.rowcnt[1].p, c.cur0, (const double*) c.gtab.p
-  if (c.niso > 0) hipLaunchKernelGGL (k_dynamics<true>, grid (c.N), dim3 (PMX_TPB), 0, c.stream, DYN_ARGS);
-  if (c.niso < c.N) hipLaunchKernelGGL (k_dynamics<false>, grid (c.N), dim3 (PMX_TPB), 0, c.stream, DYN_ARGS);
+  /* one launch per inertia form, each over its own particles (dynlist) */
+  const int *dl = c.niso > 0 && c.niso < c.N ? (const int*) c.dynlist.p : (const int*) nullptr;
+  if (c.niso > 0)
+    hipLaunchKernelGGL (k_dynamics<true>, grid (c.niso), dim3 (PMX_TPB), 0, c.stream, DYN_ARGS, dl, c.niso);
+  if (c.niso < c.N)
+    hipLaunchKernelGGL (k_dynamics<false>, grid (c.N - c.niso), dim3 (PMX_TPB), 0, c.stream, DYN_ARGS,
+                        dl ? dl + c.niso : dl, c.N - c.niso);
 #undef DYN_ARGS
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
