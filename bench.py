@@ -31,8 +31,8 @@ Beside it, in the same JSON line:
                  of uneven size (70 % sphere boxes, 30 % spring lattices,
                  lognormal size, median 20k, 500-20,000 steps) submitted in
                  real time to the resident request loop: p50/p99 latency and
-                 particle-steps/s, batched mixed beside the plain FP64 path
-                 serving one request at a time;
+                 particle-steps/s, batched mixed and batched FP64 beside the
+                 plain FP64 path serving one request at a time;
   roofline    -- the dominant kernel of the timed window: algorithmic bytes
                  per launch (the bytes its branches read and write) over its
                  mean duration from HIP event record nodes around every 8th
@@ -260,14 +260,18 @@ def hopper_leg(pmx, workloads, n, steps, dt):
 def serve_leg(pmx, workloads, count, rate, seed):
     """A Poisson stream of independent requests (boxes of spheres and spring
     lattices of uneven size and length) submitted in real time to the
-    resident request loop: batched mixed precision, and the plain FP64 path
-    serving them one at a time.  Latency = result on the host - submission."""
+    resident request loop: batched in mixed precision and in FP64, and the
+    plain FP64 path serving them one at a time.  Latency = result on the host
+    - submission."""
     import numpy as np
     stream = workloads.serve_stream(count, rate=rate, seed=seed)
     out = {"requests": count, "rate_per_s": rate, "seed": seed,
            "spheres": int(sum(r[1].ellnum for r in stream)), "bodies": int(sum(r[1].parnum for r in stream)),
            "particle_steps": float(sum(r[1].parnum * int(round(r[2] / r[3] + 0.5)) for r in stream))}
-    forms = (("batched_mixed", pmx.PMX_MIXED, dict(gather_s=0.2)),
+    # batched in both precisions (batching and precision separated), and the
+    # plain path: one request at a time, FP64
+    forms = (("batched_mixed", pmx.PMX_MIXED, dict(gather_s=0.5, max_active=4)),
+             ("batched_fp64", pmx.PMX_FP64, dict(gather_s=0.5, max_active=4)),
              ("plain_fp64", pmx.PMX_FP64, dict(max_batch=1, max_active=1)))
     for name, prec, kw in forms:
         srv = pmx.Server(precision=prec, **kw)
@@ -288,6 +292,9 @@ def serve_leg(pmx, workloads, count, rate, seed):
                      "value": ps / span, "unit": "particle-steps/s", "span_s": span, "batches": st["batches"]}
         log(f"serve {name}: p50 {out[name]['p50_s']:.2f} s, p99 {out[name]['p99_s']:.2f} s, "
             f"{out[name]['value']:.3e} particle-steps/s over {span:.1f} s, {st['batches']} batches")
+    for name in ("batched_mixed", "batched_fp64"):
+        out[name]["speedup_vs_plain_fp64"] = out[name]["value"] / out["plain_fp64"]["value"]
+    out["policy"] = "batches gathered over 0.5 s windows, at most 4 running at once, in arrival order"
     return out
 
 
