@@ -562,6 +562,7 @@ static int rebuild (pmx_ctx *c, bool all)
   PMX_HIP (hipMemcpyAsync (c->ctl.p + CTL_ABORT, zero, sizeof (int), hipMemcpyHostToDevice, c->stream));
   PMX_HIP (hipStreamSynchronize (c->stream));
   c->graph_valid = false;
+  c->need_bin = true;   /* new leaves: the records are binned again */
   return PMX_OK;
 }
 
@@ -671,6 +672,7 @@ static int grow_capacity (pmx_ctx *c, const int *fl)
   PMX_HIP (hipMemcpyAsync (c->ctl.p + CTL_ABORT, &zero, sizeof (int), hipMemcpyHostToDevice, c->stream));
   PMX_HIP (hipStreamSynchronize (c->stream));
   c->graph_valid = false;
+  c->need_bin = true;   /* the leaf counts were cleared */
   return PMX_OK;
 }
 
@@ -946,10 +948,16 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
   c->stats.max_row = 0;
 
   long long rebuilds = 0, partial = 0;
+  c->fuse_bin = false;
   if (c->E > 0)
   {
     if ((rc = rebuild (c, true))) return rc;
     rebuilds++;
+    /* each step's dynamics bins the centres for the next (slotted records;
+     * PMX_FUSED_BIN=0 keeps a separate k_bin at every step's start) */
+    const char *fb = getenv ("PMX_FUSED_BIN");
+    c->fuse_bin = c->lstride > 0 && !(fb && fb[0] == '0');
+    c->bin_until = nsteps;
   }
   /* block shapes for this workload (large models only: small ones are
    * bound by launch latency, not by the shapes): a first pass on the state
@@ -965,6 +973,7 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
     PMX_HIP (hipStreamSynchronize (c->stream));
     if ((r = clear_overflow_flags (c))) return r;   /* the timing runs' detections are discarded */
     c->graph_valid = false;
+    c->need_bin = true;   /* the tuning runs cleared the leaf counts */
     c->stats.det_waves = c->det_nw;
     c->stats.rows_block = c->rows_tpb;
     return PMX_OK;
@@ -1052,6 +1061,13 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
       {
         int n = std::min (gsize, next - enq);
         int q = 0;
+        if (c->fuse_bin && c->need_bin)
+        {
+          /* the records of step `enq`, where no dynamics has binned them
+           * (an overflow here aborts that step, as in an unfused step) */
+          if ((rc = step_bin (*c))) return rc;
+          c->need_bin = false;
+        }
         if (!eager && c->gexec_multi && c->win[0] < 0)
           for (; q + PMX_GRAPH_STEPS <= n; q += PMX_GRAPH_STEPS) PMX_HIP (hipGraphLaunch (c->gexec_multi, c->stream));
         for (; q < n; q++)
@@ -1285,6 +1301,7 @@ static int adaptive_core (pmx_ctx *c, double duration, double step, double adapt
   c->graph_valid = false;
   c->stats.max_row = 0;
   c->adp_on = true;
+  c->fuse_bin = false;   /* each step is enqueued in two parts: k_bin at its start */
   long long rebuilds = 0, partial = 0, grown = 0;
   if (c->E > 0)
   {

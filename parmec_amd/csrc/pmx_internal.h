@@ -32,7 +32,11 @@
  * or not: k_bin tells an abort raised by its own launch, where it keeps
  * binning so that every overflowing instance is flagged, from one raised by
  * an earlier step) */
-enum { CTL_ABORT = 0, CTL_KSTEP, CTL_HINT, CTL_SEQ, CTL_COUNT = 8 };
+enum { CTL_ABORT = 0, CTL_KSTEP, CTL_HINT, CTL_SEQ, CTL_REBIN, CTL_COUNT = 8 };
+/* CTL_REBIN: the binning for the next step, done at the end of this step's
+ * dynamics (k_dynamics with fused binning), overflowed a leaf: this step
+ * completes, and k_step_end aborts the next one, which the host repeats
+ * after re-partitioning */
 
 /* device error/overflow flags, one int each.  Leaf, row, hit, reverse-hit
  * and moving-triangle list overflows abort the step (the host re-partitions
@@ -325,6 +329,13 @@ struct pmx_ctx
   bool rec_packed = false;     /* leaf records always packed (PMX_LEAF_RECORDS=packed) */
   int nnosph = 0;
   bool bin_hint = false;       /* leafof holds leaves of the current tree */
+  /* fused binning: k_dynamics bins the new centres for the next step (the
+   * slotted leaf records of dem_core's steps); a separate k_bin runs only
+   * where the records are not in place (need_bin: after a build, an aborted
+   * step or a tuning run).  bin_until: the call's step count (the last
+   * step bins nothing) */
+  bool fuse_bin = false, need_bin = false;
+  int bin_until = 0;
   double rmax = 0.0;           /* largest sphere AABB half-width */
 
   double curtime = 0.0, curstep = 0.0;

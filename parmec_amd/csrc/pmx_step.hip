@@ -55,6 +55,74 @@ __global__ void k_step_end (int *ctl)
   if (threadIdx.x != 0) return;
   if (!ctl[CTL_ABORT]) ctl[CTL_KSTEP] += 1;
   ctl[CTL_SEQ] += 1;
+  /* the fused binning for the next step overflowed a leaf: this step is
+   * complete; the next is aborted, to be repeated after the re-partition */
+  if (ctl[CTL_REBIN])
+  {
+    ctl[CTL_REBIN] = 0;
+    if (!ctl[CTL_ABORT]) ctl[CTL_ABORT] = ctl[CTL_SEQ] + 1;
+  }
+}
+
+/* the leaf a centre descends to ("c[d] < coord -> left", partition.ispc:
+ * 600-646) from the instance's root; or the hint leaf h (the sphere's leaf
+ * in the previous step, h < 0: none) when the centre lies in h's reach
+ * cell, lower <= c < upper along every axis, which is where the descent
+ * ends */
+__device__ __forceinline__ int bin_leaf (const double c[3], int h, int root, const int *left, const int *right,
+                                         const int *dim, const double *coord, const double *cell, int nleaves)
+{
+  if (h >= 0)
+  {
+    bool in = true;
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+      in = in && cell[(size_t) k*nleaves + h] <= c[k] && c[k] < cell[(size_t) (3+k)*nleaves + h];
+    if (in) return h;
+  }
+  int ref = root;
+  while (ref >= 0)
+  {
+    int d = dim[ref];
+    ref = c[d] < coord[ref] ? left[ref] : right[ref];
+  }
+  return -ref-1;
+}
+
+/* slot = leaf count before this sphere: the wave's spheres grouped by
+ * leaf, then one atomic per group, all groups' atomics in one instruction.
+ * Called by every active lane of the wave; valid: the lane bins a sphere
+ * into leaf L */
+__device__ __forceinline__ int bin_slot (bool valid, int L, int *lcount)
+{
+  int lane = threadIdx.x & 63;
+  unsigned long long todo = __ballot (valid);
+  unsigned long long lt = lane ? ((1ull << lane) - 1ull) : 0ull;
+  int leader = lane;
+  unsigned long long grp = 0;
+  while (todo)
+  {
+    int ld = __ffsll ((long long) todo) - 1;
+    int Lw = __shfl (L, ld, 64);
+    unsigned long long g = __ballot (valid && L == Lw) & todo;
+    if ((g >> lane) & 1ull) { leader = ld; grp = g; }
+    todo &= ~g;
+  }
+  int base = 0;
+  if (valid && leader == lane) base = atomicAdd (&lcount[L], __popcll (grp));
+  base = __shfl (base, leader, 64);
+  return base + __popcll (grp & lt);
+}
+
+/* a sphere's record in slot s of leaf L (the slotted layout) */
+__device__ __forceinline__ void bin_record (size_t pos, int e, int pe, const double c[3], double rad, int *lmem,
+                                            int *lpart, double *lpos)
+{
+  lmem[pos] = e;
+  lpart[pos] = pe;
+  double2 *rec = (double2*) (lpos + 4*pos);
+  rec[0] = make_double2 (c[0], c[1]);
+  rec[1] = make_double2 (c[2], rad);
 }
 
 /* partitioning_store: descend to the leaf, take a slot (partition.ispc:600-646) */
@@ -83,48 +151,10 @@ __global__ void k_bin (int E, const double *center, const int *roots, const int 
   {
     c[0] = center[e]; c[1] = center[(size_t) E + e]; c[2] = center[2*(size_t) E + e];
     if (lstride) { rad = radii[e]; pe = part[e]; }   /* in flight with the centre */
-    if (hint)
-    {
-      /* the descent "c[d] < coord -> left" ends in leaf h iff lower <= c < upper
-       * along every axis, with lower/upper the leaf's reach cell */
-      int h = leafof[e];
-      hprev = h;
-      bool in = true;
-#pragma unroll
-      for (int k = 0; k < 3; k++)
-        in = in && cell[(size_t) k*nleaves + h] <= c[k] && c[k] < cell[(size_t) (3+k)*nleaves + h];
-      if (in) L = h;
-    }
-    if (L < 0)
-    {
-      int ref = roots[ell_inst[e]];
-      while (ref >= 0)
-      {
-        int d = dim[ref];
-        ref = c[d] < coord[ref] ? left[ref] : right[ref];
-      }
-      L = -ref-1;
-    }
+    if (hint) hprev = leafof[e];
+    L = bin_leaf (c, hprev, roots[ell_inst[e]], left, right, dim, coord, cell, nleaves);
   }
-  /* slot = leaf count before this sphere: the wave's spheres grouped by
-   * leaf, then one atomic per group, all groups' atomics in one instruction */
-  int lane = threadIdx.x & 63;
-  unsigned long long todo = __ballot (valid);
-  unsigned long long lt = lane ? ((1ull << lane) - 1ull) : 0ull;
-  int leader = lane;
-  unsigned long long grp = 0;
-  while (todo)
-  {
-    int ld = __ffsll ((long long) todo) - 1;
-    int Lw = __shfl (L, ld, 64);
-    unsigned long long g = __ballot (valid && L == Lw) & todo;
-    if ((g >> lane) & 1ull) { leader = ld; grp = g; }
-    todo &= ~g;
-  }
-  int base = 0;
-  if (valid && leader == lane) base = atomicAdd (&lcount[L], __popcll (grp));
-  base = __shfl (base, leader, 64);
-  int s = base + __popcll (grp & lt);
+  int s = bin_slot (valid, L, lcount);
   if (!valid) return;
   if (L != hprev) leafof[e] = L;   /* unchanged where the hint held */
   if (s >= PMX_LSIZE)
@@ -135,12 +165,7 @@ __global__ void k_bin (int E, const double *center, const int *roots, const int 
     return;
   }
   if (!lstride) { slot[e] = s; return; }
-  size_t pos = (size_t) L * lstride + s;
-  lmem[pos] = e;
-  lpart[pos] = pe;
-  double2 *rec = (double2*) (lpos + 4*pos);
-  rec[0] = make_double2 (c[0], c[1]);
-  rec[1] = make_double2 (c[2], rad);
+  bin_record ((size_t) L * lstride + s, e, pe, c, rad, lmem, lpart, lpos);
 }
 
 __global__ void k_scatter (int *ctl, int E, const double *center, const double *radii, const int *leafof, const int *slot,
@@ -1279,18 +1304,33 @@ void k_rows (int E, int N, const double *center, const double *radii, const int 
   if ((threadIdx.x & 63) == 0 && mx > 0) atomicMax (&flags[FLAG_MAXROW], mx);
 }
 
+/* the binning for the next step fused into k_dynamics (fused: lstride >
+ * 0, the slotted leaf records): partitioning_store of the next step
+ * (partition.ispc:600-646) from the centre k_dynamics has just formed,
+ * which is the centre the next step's store would read.  until: the
+ * call's step count (the last step bins nothing) */
+struct BinArgs
+{
+  const int *roots, *left, *right, *dim;
+  const double *coord, *cell, *radii;
+  int nleaves, lstride, until;
+  int *leafof, *lcount, *flags, *ovf, *lmem, *lpart;
+  double *lpos;
+};
+
 /* dynamics_task, dynamics.ispc:30-169 */
 /* ISO: the launch over particles whose inertia is diag(j, j, j) with exact
  * zeros elsewhere (inverse diag(v, v, v), INVERT macros.h:1033-1051): J x
  * and I x are the scalar products j x and v x, equal in value to the
  * matrix products; the other launch takes the remaining particles */
-template <bool ISO>
-__global__ void k_dynamics (int N, double *angular, double *linear, double *rotation, double *position,
+template <bool ISO, bool FUSE>
+__global__ __launch_bounds__(PMX_TPB)   /* the launch size: no register cap below what the kernel takes (no spill) */
+void k_dynamics (int N, double *angular, double *linear, double *rotation, double *position,
                             const double *inertia, const double *inverse, const double *mass, const double *invm,
                             const double *dtab, const int *ctl, double *force, double *torque, double step0, double step1,
                             const int *parell, int E, double *center, const signed char *iso, int ninst,
                             const int *par_inst, const int *stop, const int *cnt0, const int *cnt1, int cur0,
-                            const double *gtab, const int *list, int nlist)
+                            const double *gtab, const int *list, int nlist, BinArgs bn, int *ctlw)
 {
   if (ctl[CTL_ABORT]) return;
   int kstep = ctl[CTL_KSTEP];
@@ -1328,6 +1368,21 @@ __global__ void k_dynamics (int N, double *angular, double *linear, double *rota
     if constexpr (ISO) { y[0] = I[0]*x[0]; y[1] = I[0]*x[1]; y[2] = I[0]*x[2]; } else nvmul (I, x, y); };
   im = invm[i];
   int e = parell ? parell[i] : -1;
+  /* fused binning: the sphere's radius, its previous leaf (the hint) and
+   * that leaf's reach cell are fetched now, in flight while the motion is
+   * integrated (indices clamped: every load is in bounds) */
+  int hprev = -1;
+  double rad = 0.0, hc[6];
+  if constexpr (FUSE)
+  {
+    int es = e < 0 ? 0 : e;
+    rad = bn.radii[es];
+    int h = bn.leafof[es];
+    int hs = (unsigned) h < (unsigned) bn.nleaves ? h : 0;
+#pragma unroll
+    for (int k = 0; k < 6; k++) hc[k] = bn.cell[(size_t) k*bn.nleaves + hs];
+    hprev = ctl[CTL_HINT] && e >= 0 ? h : -1;
+  }
   /* a particle loaded by contacts and gravity only whose row of this step
    * is empty: k_rows would have stored force 0 + m g and torque 0 */
   const int *ncnt = ((cur0 + kstep) & 1) ? cnt0 : cnt1;
@@ -1422,12 +1477,42 @@ __global__ void k_dynamics (int N, double *angular, double *linear, double *rota
     __builtin_nontemporal_store (o[k], &angular[(size_t) (3+k)*N + i]);
     __builtin_nontemporal_store (v[k], &linear[(size_t) k*N + i]);
   }
+  double nc[3] = {xn[0] + cc[0], xn[1] + cc[1], xn[2] + cc[2]};
   if (e >= 0)
   {
-    __builtin_nontemporal_store (xn[0] + cc[0], &center[e]);
-    __builtin_nontemporal_store (xn[1] + cc[1], &center[(size_t) E + e]);
-    __builtin_nontemporal_store (xn[2] + cc[2], &center[2*(size_t) E + e]);
+    __builtin_nontemporal_store (nc[0], &center[e]);
+    __builtin_nontemporal_store (nc[1], &center[(size_t) E + e]);
+    __builtin_nontemporal_store (nc[2], &center[2*(size_t) E + e]);
   }
+  if constexpr (FUSE)
+  {
+    /* the next step's binning (every active lane takes part in the slot
+     * grouping); an instance's last step bins nothing */
+    int inst = ninst > 1 ? par_inst[i] : 0;
+    bool bv = e >= 0 && kstep + 1 < bn.until && !(stop && kstep + 1 >= stop[inst]);
+    int L = -1;
+    if (bv)
+    {
+      bool in = hprev >= 0;
+#pragma unroll
+      for (int k = 0; k < 3; k++) in = in && hc[k] <= nc[k] && nc[k] < hc[3+k];
+      L = in ? hprev : bin_leaf (nc, -1, bn.roots[inst], bn.left, bn.right, bn.dim, bn.coord, bn.cell, bn.nleaves);
+    }
+    int s = bin_slot (bv, L, bn.lcount);
+    if (!bv) return;
+    if (L != hprev) bn.leafof[e] = L;
+    if (s >= PMX_LSIZE)
+    {
+      /* this step stands; the next one is aborted (k_step_end) and
+       * repeated once the instance is re-partitioned */
+      atomicAdd (&bn.flags[FLAG_LEAF_OVERFLOW], 1);
+      atomicExch (&bn.ovf[inst], 1);
+      atomicExch (&ctlw[CTL_REBIN], 1);
+      return;
+    }
+    bin_record ((size_t) L * bn.lstride + s, e, i, nc, rad, bn.lmem, bn.lpart, bn.lpos);
+  }
+
 }
 
 /* force and torque of the flag-4 particles (loaded by contacts and gravity
@@ -1863,6 +1948,19 @@ int step_dynamics (pmx_ctx &c, double step0, double step1)
   double dyn = 321.0 * c.N + 16.0 * c.niso + 144.0 * (c.N - c.niso) + (c.E > 0 ? 4.0 * c.N : 0.0) +
                (c.I > 1 ? 4.0 * c.N : 0.0) + 24.0 * c.E + 48.0 * (c.E - c.nzc) + 12.0 * c.nfree +
                48.0 * (c.N - c.nfree);
+  /* fused binning (slotted records): per sphere its radius 8 and leaf 4
+   * read, the record 32, member 4 and particle 4 written (the centre is in
+   * registers) */
+  BinArgs B {};
+  if (c.fuse_bin)
+  {
+    TreeDev &t = c.tree;
+    dyn += 52.0 * c.E;
+    B = BinArgs {(const int*) t.roots.p, (const int*) t.left.p, (const int*) t.right.p, (const int*) t.dim.p,
+                 (const double*) t.coord.p, (const double*) t.cell.p, (const double*) c.radii.p, t.nleaves,
+                 c.lstride, c.bin_until, c.leafof.p, c.lcount.p, c.dflags.p, c.ovf.p, c.lmem.p, c.lpart.p,
+                 c.lpos.p};
+  }
   ProfScope ps (c, K_DYNAMICS, dyn);
 #define DYN_ARGS c.N, c.angular.p, c.linear.p, c.rotation.p, \
                  c.position.p, (const double*) c.inertia.p, (const double*) c.inverse.p, (const double*) c.mass.p, \
@@ -1873,10 +1971,23 @@ int step_dynamics (pmx_ctx &c, double step0, double step1)
   /* one launch per inertia form, each over its own particles (dynlist) */
   const int *dl = c.niso > 0 && c.niso < c.N ? (const int*) c.dynlist.p : (const int*) nullptr;
   if (c.niso > 0)
-    hipLaunchKernelGGL (k_dynamics<true>, grid (c.niso), dim3 (PMX_TPB), 0, c.stream, DYN_ARGS, dl, c.niso);
+  {
+    if (c.fuse_bin)
+      hipLaunchKernelGGL ((k_dynamics<true, true>), grid (c.niso), dim3 (PMX_TPB), 0, c.stream, DYN_ARGS, dl, c.niso, B,
+                          c.ctl.p);
+    else
+      hipLaunchKernelGGL ((k_dynamics<true, false>), grid (c.niso), dim3 (PMX_TPB), 0, c.stream, DYN_ARGS, dl, c.niso,
+                          B, c.ctl.p);
+  }
   if (c.niso < c.N)
-    hipLaunchKernelGGL (k_dynamics<false>, grid (c.N - c.niso), dim3 (PMX_TPB), 0, c.stream, DYN_ARGS,
-                        dl ? dl + c.niso : dl, c.N - c.niso);
+  {
+    if (c.fuse_bin)
+      hipLaunchKernelGGL ((k_dynamics<false, true>), grid (c.N - c.niso), dim3 (PMX_TPB), 0, c.stream, DYN_ARGS,
+                          dl ? dl + c.niso : dl, c.N - c.niso, B, c.ctl.p);
+    else
+      hipLaunchKernelGGL ((k_dynamics<false, false>), grid (c.N - c.niso), dim3 (PMX_TPB), 0, c.stream, DYN_ARGS,
+                          dl ? dl + c.niso : dl, c.N - c.niso, B, c.ctl.p);
+  }
 #undef DYN_ARGS
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
@@ -1997,7 +2108,9 @@ int step_prescribe_vel (pmx_ctx &c)
 int enqueue_step_forces (pmx_ctx &c, double h)
 {
   int rc;
-  if (c.E > 0 && ((rc = step_bin (c)) || (rc = step_mtri (c)) || (rc = step_detect (c)))) return rc;
+  /* with fused binning the previous step's dynamics (or the call's own
+   * k_bin) has binned this step's centres */
+  if (c.E > 0 && ((!c.fuse_bin && (rc = step_bin (c))) || (rc = step_mtri (c)) || (rc = step_detect (c)))) return rc;
   if (c.adp_on && (rc = adp_begin (c))) return rc;
   /* mesh nodes and faces (shapes.ispc:101-211) are updated per step only
    * where spheres can touch them (enqueue_step_motion): elsewhere nothing on

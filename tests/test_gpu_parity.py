@@ -167,6 +167,22 @@ def test_full_leaves_with_contacts(engines):
     compare(mg, cg, mc, cc)
 
 
+def test_separate_binning(engines, monkeypatch):
+    """Binning at every step's start (k_bin; PMX_FUSED_BIN=0) instead of at
+    the end of the previous step's dynamics gives the oracle's result, with
+    the same re-partitions, through leaf overflows (the fused form's
+    overflow completes its step and aborts the next one)."""
+    print("case converging flow, separate binning", flush=True)
+    monkeypatch.setenv("PMX_FUSED_BIN", "0")
+    m = workloads.box_lattice(n=4000, seed=3)
+    xc = m.center[:3].mean(axis=1, keepdims=True)
+    m.linear[:] = -75.0 * (m.center[:3] - xc)
+    (mg, cg, sg), (mc, cc, sc) = _both(engines, m, 200 * 1e-5 - 0.5e-5, 1e-5, pmx.forcing())
+    assert sc["rebuilds"] > 5
+    assert sg["rebuilds"] == sc["rebuilds"]
+    compare(mg, cg, mc, cc)
+
+
 def test_csr_neighbour_lists(engines, monkeypatch):
     """Neighbour lists in the CSR layout (used when the padded layout would
     not fit) give the oracle's result as the padded layout does."""
