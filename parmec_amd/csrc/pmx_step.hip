@@ -402,11 +402,22 @@ void k_detect (int E, const double *center, const double *radii, const int *part
   /* the wave's first (usually only) batch of candidate leaves, one per
    * lane: leaf, count, offset and cell, loaded before the block's spheres
    * so that both dependent loads are in flight together */
+  /* the AABB of the block's spheres, widened by a relative margin
+   * (k_leafbox, from the same records; zeros for an empty leaf): scalar
+   * loads, ahead of the candidates that are tested against it */
+  double blo[3], bhi[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+  {
+    blo[d] = lbox[(size_t) d * nleaves + Q];
+    bhi[d] = lbox[(size_t) (3+d) * nleaves + Q];
+  }
   int bL = 0, bcnt = 0, boff = 0;
+  bool bfar = false;   /* a neighbour whose records' box misses the block's: nothing to stage */
   double Ll[3] = {0.0, 0.0, 0.0}, Lu[3] = {0.0, 0.0, 0.0};
   auto batch = [&] (int p0) {
     int np = ncw - p0 < 64 ? ncw - p0 : 64;
-    bL = 0; bcnt = 0; boff = 0;
+    bL = 0; bcnt = 0; boff = 0; bfar = false;
     if (lane < np)
     {
       /* the lane index taken afresh (mbcnt): fewer registers held through
@@ -422,6 +433,10 @@ void k_detect (int E, const double *center, const double *radii, const int *part
         Ll[d] = cell[(size_t) d*nleaves + bL];
         Lu[d] = cell[(size_t) (3+d)*nleaves + bL];
       }
+      if (bL != Q)
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+          bfar = bfar || lbox[(size_t) d * nleaves + bL] > bhi[d] || lbox[(size_t) (3+d) * nleaves + bL] < blo[d];
     }
   };
   batch (0);
@@ -438,16 +453,6 @@ void k_detect (int E, const double *center, const double *radii, const int *part
   double cu[3] = {cell[3*(size_t) nleaves + Q], cell[4*(size_t) nleaves + Q], cell[5*(size_t) nleaves + Q]};
   __syncthreads ();
 
-  /* the AABB of the block's spheres, widened by a relative margin
-   * (k_leafbox, from the same records) */
-  double blo[3] = {0.0, 0.0, 0.0}, bhi[3] = {0.0, 0.0, 0.0};
-  if (qcnt > 0)
-#pragma unroll
-    for (int d = 0; d < 3; d++)
-    {
-      blo[d] = lbox[(size_t) d * nleaves + Q];
-      bhi[d] = lbox[(size_t) (3+d) * nleaves + Q];
-    }
   /* origin O of the FP32 offsets: the box centre; B bounds |c - O| per axis
    * for every block sphere */
   double O[3], B = 0.0;
@@ -518,18 +523,22 @@ void k_detect (int E, const double *center, const double *radii, const int *part
     }
     /* the candidate count of the wave, uniform: the loops below run with
      * the full wave and keep their counters scalar */
-    int npu = __builtin_amdgcn_readfirstlane (np);
+    /* the masks of the candidates with records whose box meets the
+     * block's (one bit per lane, uniform), the others left empty */
+    unsigned long long km = __ballot (lane < np && bcnt > 0 && !bfar);
     unsigned long long bm0 = 0ull, bm1 = 0ull;
     if (qcnt <= 64)
-      for (int c = 0; c < npu; c++)
+      for (unsigned long long k = km; k; k &= k - 1ull)
       {
+        int c = __ffsll ((long long) k) - 1;
         const float2 *cp = (const float2*) (cf + 6*c);
         unsigned long long r0 = __ballot (reach32 (slo0, shi0, cp[0], cp[1], cp[2]));
         bm0 = lane == c ? r0 : bm0;
       }
     else
-      for (int c = 0; c < npu; c++)
+      for (unsigned long long k = km; k; k &= k - 1ull)
       {
+        int c = __ffsll ((long long) k) - 1;
         const float2 *cp = (const float2*) (cf + 6*c);
         float2 a = cp[0], b = cp[1], e = cp[2];
         unsigned long long r0 = __ballot (reach32 (slo0, shi0, a, b, e));
@@ -537,13 +546,6 @@ void k_detect (int E, const double *center, const double *radii, const int *part
         bm0 = lane == c ? r0 : bm0;
         bm1 = lane == c ? r1 : bm1;
       }
-    /* a neighbour whose records' box misses the block's: nothing to stage */
-    bool far = false;
-    if (lane < np && bcnt > 0 && bL != Q)
-#pragma unroll
-      for (int d = 0; d < 3; d++)
-        far = far || lbox[(size_t) d * nleaves + bL] > bhi[d] || lbox[(size_t) (3+d) * nleaves + bL] < blo[d];
-    if (!(lane < np && bcnt > 0) || far) { bm0 = 0ull; bm1 = 0ull; }
     __builtin_amdgcn_wave_barrier ();
     /* the candidates that some block sphere reaches, in turn */
     unsigned long long todo = __ballot ((bm0 | bm1) != 0);
