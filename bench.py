@@ -268,10 +268,13 @@ def serve_leg(pmx, workloads, count, rate, seed):
     out = {"requests": count, "rate_per_s": rate, "seed": seed,
            "spheres": int(sum(r[1].ellnum for r in stream)), "bodies": int(sum(r[1].parnum for r in stream)),
            "particle_steps": float(sum(r[1].parnum * int(round(r[2] / r[3] + 0.5)) for r in stream))}
-    # batched in both precisions (batching and precision separated), and the
+    # batched in both precisions (batching and precision separated), the
+    # continuous form (requests join running batches of their kind), and the
     # plain path: one request at a time, FP64
-    forms = (("batched_mixed", pmx.PMX_MIXED, dict(gather_s=0.5, max_active=4)),
-             ("batched_fp64", pmx.PMX_FP64, dict(gather_s=0.5, max_active=4)),
+    batched = dict(gather_s=0.25, max_active=6, max_batch=12)
+    forms = (("batched_mixed", pmx.PMX_MIXED, batched),
+             ("batched_fp64", pmx.PMX_FP64, batched),
+             ("continuous_mixed", pmx.PMX_MIXED, dict(gather_s=0.05, max_active=4, max_batch=16, repack_s=0.25)),
              ("plain_fp64", pmx.PMX_FP64, dict(max_batch=1, max_active=1)))
     for name, prec, kw in forms:
         srv = pmx.Server(precision=prec, **kw)
@@ -292,9 +295,13 @@ def serve_leg(pmx, workloads, count, rate, seed):
                      "value": ps / span, "unit": "particle-steps/s", "span_s": span, "batches": st["batches"]}
         log(f"serve {name}: p50 {out[name]['p50_s']:.2f} s, p99 {out[name]['p99_s']:.2f} s, "
             f"{out[name]['value']:.3e} particle-steps/s over {span:.1f} s, {st['batches']} batches")
-    for name in ("batched_mixed", "batched_fp64"):
+    for name in ("batched_mixed", "batched_fp64", "continuous_mixed"):
         out[name]["speedup_vs_plain_fp64"] = out[name]["value"] / out["plain_fp64"]["value"]
-    out["policy"] = "batches gathered over 0.5 s windows, at most 4 running at once, in arrival order"
+    out["policy"] = ("batched: one kind of request per batch (spheres, or bodies joined by springs), gathered over "
+                     "0.25 s windows, at most 12 requests per batch and 6 batches at once, in arrival order; "
+                     "continuous: a running batch of a kind (at most 16 requests, 4 batches at once) pauses at "
+                     "most every 0.25 s to take in queued requests of its kind, carrying its unfinished requests "
+                     "over with their contact rows and tree snapshots")
     return out
 
 
@@ -350,7 +357,7 @@ def main():
     ap.add_argument("--dense-warmup", type=int, default=20)
     ap.add_argument("--hopper", type=int, default=2000000, help="hopper spheres (0: skip)")
     ap.add_argument("--hopper-steps", type=int, default=200)
-    ap.add_argument("--serve", type=int, default=24, help="requests of the serving stream (0: skip)")
+    ap.add_argument("--serve", type=int, default=200, help="requests of the serving stream (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--cpu-instances", type=int, default=16)
     ap.add_argument("--cpu-steps", type=int, default=200)

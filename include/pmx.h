@@ -496,6 +496,9 @@ typedef struct pmx_server_options
   int max_active;       /* batch contexts stepping at once (0: 16) */
   double gather_s;      /* how long the oldest queued request waits for others to batch with */
   size_t hbm_budget;    /* bytes of device memory the admitted requests may use (0: 200 GiB) */
+  double repack_s;      /* continuous batching: > 0, a running batch takes in queued requests of its kind
+                         * (pausing at a step, carrying its unfinished requests over with their contact rows
+                         * and tree snapshots) at most this often; 0: a batch runs until its last request ends */
 } pmx_server_options;
 
 typedef struct pmx_request_info

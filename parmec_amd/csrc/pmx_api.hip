@@ -45,6 +45,8 @@ template <class T> int down (pmx_ctx &c, T *dst, const dvec<T> &d, size_t n)
   return PMX_OK;
 }
 
+} /* namespace */
+
 int alloc_rows (pmx_ctx &c)
 {
   size_t E = std::max (c.E, 1);
@@ -67,8 +69,6 @@ int alloc_rows (pmx_ctx &c)
   c.cur = 0;
   return PMX_OK;
 }
-
-} /* namespace */
 
 /* checks a single model (one instance) for what the device path supports */
 int validate_model (const pmx_model *m)
@@ -810,33 +810,63 @@ static void finish_instance (pmx_ctx *c, int i, int nst, double step0, double st
  * entries: one forcing for all instances, or one per instance.  done(user,
  * i) is called from inside the call as soon as instance i has finished. */
 int dem_core (pmx_ctx *c, const double *durations, double duration, double step, const pmx_forcing *fc, int nfc,
-              void (*done_fn) (void*, int), void *user, double *elapsed)
+              void (*done_fn) (void*, int), void *user, double *elapsed, DemExtra *x)
 {
   if (!c) return fail (PMX_EINVAL, "null ctx");
   if (!c->uploaded) return fail (PMX_EINVAL, "pmx_upload must precede pmx_dem");
   if (!(step > 0.0)) return fail (PMX_EINVAL, "step must be positive");
   int I = c->I;
+  const int *steps_in = x ? x->steps : nullptr;
+  if (x) x->paused_at = -1;
+  /* instances at time 0 and resumed ones (given step counts only): the
+   * resumed ones must have taken this step */
+  int nfresh = 0;
   for (int i = 0; i < I; i++)
   {
+    if (steps_in)
+    {
+      if (steps_in[i] < 1) return fail (PMX_EINVAL, "step counts must be positive");
+      if (c->icurtime[i] != 0.0 && c->icurstep[i] != step)
+        return fail (PMX_EINVAL, "a resumed instance steps with the step it has taken so far");
+      nfresh += c->icurtime[i] == 0.0;
+      continue;
+    }
     double d = durations ? durations[i] : duration;
     if (!(d > 0.0)) return fail (PMX_EINVAL, "durations must be positive");
     if (c->icurtime[i] != c->icurtime[0] || c->icurstep[i] != c->icurstep[0])
       return fail (PMX_EINVAL, "instances are at different times after a call with different durations; "
                                "download them and upload a new batch");
   }
+  const bool mixed_start = steps_in && nfresh > 0 && nfresh < I;
+  if (steps_in && c->sp_offsets) return fail (PMX_EUNSUPPORTED, "step counts with spring stroke offsets");
+  if (mixed_start && (c->O > 0 || c->RS > 0))
+    return fail (PMX_EUNSUPPORTED, "resumed instances beside new ones: moving obstacles and restraints are not supported");
   (void) hipSetDevice (c->device);
   auto t0 = std::chrono::steady_clock::now ();
   int rc = 0;
   double step0, step1;
-  const double curtime = c->icurtime[0];
+  const double curtime = steps_in ? (nfresh > 0 ? 0.0 : c->icurtime[0]) : c->icurtime[0];
   int ctl0[CTL_COUNT] = {0};
   PMX_HIP (hipMemcpyAsync (c->ctl.p, ctl0, sizeof (ctl0), hipMemcpyHostToDevice, c->stream));
   PMX_HIP (hipMemsetAsync (c->dflags.p, 0, FLAG_COUNT * sizeof (int), c->stream));
   c->staggered = false;
+  c->first_sel = false;
+  if (mixed_start)
+  {
+    /* euler and shapes of the first call only for the instances at time 0
+     * (the resumed ones took theirs in their first context) */
+    std::vector<int> fresh (I);
+    for (int i = 0; i < I; i++) fresh[i] = c->icurtime[i] == 0.0;
+    if ((rc = up (*c, c->isel, fresh.data (), I, "instance selection"))) return rc;
+    c->first_sel = true;
+  }
   if (curtime == 0.0)
   {
     step0 = step;
-    if ((rc = step_euler (*c, 0.5*step0)) || (rc = step_shapes (*c)) || (rc = step_mesh_shapes (*c, false))) return rc;
+    rc = step_euler (*c, 0.5*step0);
+    if (!rc) rc = step_shapes (*c);
+    c->first_sel = false;
+    if (rc || (rc = step_mesh_shapes (*c, false))) return rc;
     if (c->O > 0)
     {
       /* obstaclev (step0/2) and obstacles (step0), parmec.cpp:2403-2405 */
@@ -862,7 +892,8 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
   int nsteps = 0;
   for (int i = 0; i < I; i++)
   {
-    pmx_step_times (durations ? durations[i] : duration, step0, curtime, nullptr, 0, &nst[i]);
+    if (steps_in) nst[i] = steps_in[i];
+    else pmx_step_times (durations ? durations[i] : duration, step0, curtime, nullptr, 0, &nst[i]);
     nsteps = std::max (nsteps, nst[i]);
   }
   for (int i = 0; i < I; i++) c->staggered = c->staggered || nst[i] != nsteps;
@@ -951,7 +982,17 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
   c->fuse_bin = false;
   if (c->E > 0)
   {
-    if ((rc = rebuild (c, true))) return rc;
+    if (c->snap_sel)
+    {
+      /* resumed instances brought the centres their trees were built from
+       * (import_rows): the build takes new snapshots of the others only */
+      std::vector<int> fresh (I);
+      for (int i = 0; i < I; i++) fresh[i] = c->icurtime[i] == 0.0;
+      if ((rc = up (*c, c->isel, fresh.data (), I, "instance selection"))) return rc;
+    }
+    rc = rebuild (c, true);
+    c->snap_sel = false;
+    if (rc) return rc;
     rebuilds++;
     /* each step's dynamics bins the centres for the next (slotted records;
      * PMX_FUSED_BIN=0 keeps a separate k_bin at every step's start) */
@@ -1038,6 +1079,7 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
   int done = 0;
   long long grown = 0;
   std::vector<int> rebuilt_at (I, -1);   /* the last step each instance was re-partitioned for */
+  bool pause_req = false;                /* the caller's stop_fn asked for a pause: no more groups */
   while (done < nsteps)
   {
     int next = nsteps;
@@ -1057,7 +1099,7 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
     bool abort = false;
     for (;;)
     {
-      while (enq < next && gq - gs < PMX_LOOKAHEAD)
+      while (!pause_req && enq < next && gq - gs < PMX_LOOKAHEAD)
       {
         int n = std::min (gsize, next - enq);
         int q = 0;
@@ -1105,6 +1147,7 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
       gs++;
       if ((err = check_flags (c, fl))) break;
       if ((abort = ctl[CTL_ABORT] != 0)) break;
+      if (x && x->stop_fn && !pause_req && x->stop_fn (x->stop_user, ctl[CTL_KSTEP])) pause_req = true;
     }
     PMX_HIP (hipStreamSynchronize (c->stream));   /* the groups behind an abort */
     prof_resolve (*c, ctl[CTL_KSTEP]);
@@ -1157,12 +1200,29 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
       (void) hipSetDevice (c->device);
       oi++;
     }
+    if (pause_req && done < nsteps) break;
     if (tuning && c->tuned == 1 && !abort && done < nsteps && c->stepped + done >= PMX_RETUNE_STEPS &&
         (rc = tune (done)))
       return rc;
   }
-  if ((rc = report (nsteps))) return rc;
-  if (outs && I == 1)
+  const bool paused = done < nsteps;
+  if (paused)
+  {
+    /* paused at step `done`: the unfinished instances keep the time of the
+     * steps they took (the same additions, parmec.cpp:2418) and their rows,
+     * and take no stepnum (their dem() has not returned) */
+    for (int i = 0; i < I; i++)
+      if (!reported[i])
+      {
+        for (int k = 0; k < done; k++) c->icurtime[i] += 0.5*(step0+step1);
+        c->icurstep[i] = step1;
+        c->icur[i] = (c->cur0 + done) & 1;
+      }
+    nsteps = done;
+    if (x) x->paused_at = done;
+  }
+  else if ((rc = report (nsteps))) return rc;
+  if (outs && I == 1 && !paused)
     for (; oi < fc->noutstep; oi++)
       if (fc->outstep[oi] == nsteps - 1)   /* the last step: the finished state */
       {

@@ -107,6 +107,108 @@ int contacts_range (pmx_ctx &c, int inst, bool local, std::vector<pmx_pair> &out
 
 
 
+/* an instance's rows (the buffer holding them, every slot of its row
+ * capacity: a slot past the count is never read) and its tree snapshot,
+ * partners renumbered to the instance's own spheres and triangles */
+int export_rows (pmx_ctx &c, int inst, ResumeRows &r)
+{
+  int e0 = c.off[inst].ell, ne = c.off[inst+1].ell - e0, t0 = c.off[inst].tri;
+  r.cap = c.cap; r.ne = ne; r.nt = c.off[inst+1].tri - t0;
+  size_t cap = c.cap, E = c.E;
+  size_t esz = c.opt.precision == PMX_MIXED ? sizeof (float) : sizeof (double);
+  r.key.assign (cap * ne, 0u);
+  r.st.assign (3 * cap * ne * esz, 0);
+  r.cnt.assign (ne, 0);
+  r.bcen.assign (3 * (size_t) ne, 0.0);
+  if (ne == 0) return PMX_OK;
+  int par = c.icur[inst];
+  const void *st = c.opt.precision == PMX_MIXED ? (const void*) c.rowst32[par].p : (const void*) c.rowst[par].p;
+  PMX_HIP (hipMemcpy2DAsync (r.key.data (), ne * sizeof (uint32_t), c.rowkey[par].p + e0, E * sizeof (uint32_t),
+                             ne * sizeof (uint32_t), cap, hipMemcpyDeviceToHost, c.stream));
+  PMX_HIP (hipMemcpy2DAsync (r.st.data (), ne * esz, (const char*) st + e0 * esz, E * esz, ne * esz, 3 * cap,
+                             hipMemcpyDeviceToHost, c.stream));
+  PMX_HIP (hipMemcpyAsync (r.cnt.data (), c.rowcnt[par].p + e0, ne * sizeof (int), hipMemcpyDeviceToHost, c.stream));
+  PMX_HIP (hipMemcpy2DAsync (r.bcen.data (), ne * sizeof (double), c.bcen.p + e0, E * sizeof (double),
+                             ne * sizeof (double), 3, hipMemcpyDeviceToHost, c.stream));
+  PMX_HIP (hipStreamSynchronize (c.stream));
+  for (int e = 0; e < ne; e++)
+    for (int k = 0; k < r.cnt[e]; k++)
+    {
+      uint32_t &kk = r.key[(size_t) k * ne + e];
+      kk = (kk & PMX_TRI_FLAG) ? (PMX_TRI_FLAG | ((kk & ~PMX_TRI_FLAG) - (uint32_t) t0)) : kk - (uint32_t) e0;
+    }
+  return PMX_OK;
+}
+
+int import_rows (pmx_ctx &c, const std::vector<const ResumeRows*> &rs)
+{
+  if ((int) rs.size () != c.I) { pmx_set_error ("resume states: one entry per instance"); return PMX_EINVAL; }
+  int cap = c.cap;
+  bool any = false;
+  for (int i = 0; i < c.I; i++)
+    if (rs[i])
+    {
+      if (rs[i]->ne != c.off[i+1].ell - c.off[i].ell || rs[i]->nt != c.off[i+1].tri - c.off[i].tri)
+      { pmx_set_error ("resume state of instance " + std::to_string (i) + ": counts differ"); return PMX_EINVAL; }
+      if (!(rs[i]->curtime > 0.0)) { pmx_set_error ("a resumed instance must carry its time"); return PMX_EINVAL; }
+      cap = std::max (cap, rs[i]->cap);
+      any = true;
+    }
+  (void) hipSetDevice (c.device);
+  if (cap > c.cap)
+  {
+    /* rows as wide as the widest resumed row (grown in its earlier context) */
+    c.cap = cap;
+    int rc = alloc_rows (c);
+    if (rc) return rc;
+  }
+  for (int i = 0; i < c.I; i++)
+  {
+    /* the fresh instances: no contact history, whatever rows an upload of
+     * the same layout kept */
+    c.icur[i] = c.cur;
+    if (rs[i]) continue;
+    int e0 = c.off[i].ell, ne = c.off[i+1].ell - e0;
+    if (ne > 0) PMX_HIP (hipMemsetAsync (c.rowcnt[c.cur].p + e0, 0, ne * sizeof (int), c.stream));
+  }
+  if (!any) return PMX_OK;
+  size_t E = c.E, esz = c.opt.precision == PMX_MIXED ? sizeof (float) : sizeof (double);
+  if (c.bcen.alloc (3 * std::max (E, (size_t) 1)) != hipSuccess)
+  { pmx_set_error ("out of device memory (tree)"); return PMX_ENOMEM; }
+  int par = c.cur;
+  void *st = c.opt.precision == PMX_MIXED ? (void*) c.rowst32[par].p : (void*) c.rowst[par].p;
+  for (int i = 0; i < c.I; i++)
+  {
+    const ResumeRows *r = rs[i];
+    if (!r) continue;
+    c.icurtime[i] = r->curtime;
+    c.icurstep[i] = r->curstep;
+    c.istepnum[i] = r->stepnum;
+    if (r->ne == 0) continue;
+    int e0 = c.off[i].ell, ne = r->ne, t0 = c.off[i].tri;
+    std::vector<uint32_t> key (r->key);
+    for (int e = 0; e < ne; e++)
+      for (int k = 0; k < r->cnt[e]; k++)
+      {
+        uint32_t &kk = key[(size_t) k * ne + e];
+        kk = (kk & PMX_TRI_FLAG) ? (PMX_TRI_FLAG | ((kk & ~PMX_TRI_FLAG) + (uint32_t) t0)) : kk + (uint32_t) e0;
+      }
+    PMX_HIP (hipMemcpy2DAsync (c.rowkey[par].p + e0, E * sizeof (uint32_t), key.data (), ne * sizeof (uint32_t),
+                               ne * sizeof (uint32_t), r->cap, hipMemcpyHostToDevice, c.stream));
+    for (int d = 0; d < 3; d++)
+      PMX_HIP (hipMemcpy2DAsync ((char*) st + ((size_t) d * c.cap * E + e0) * esz, E * esz,
+                                 r->st.data () + (size_t) d * r->cap * ne * esz, ne * esz, ne * esz, r->cap,
+                                 hipMemcpyHostToDevice, c.stream));
+    PMX_HIP (hipMemcpyAsync (c.rowcnt[par].p + e0, r->cnt.data (), ne * sizeof (int), hipMemcpyHostToDevice, c.stream));
+    PMX_HIP (hipMemcpy2DAsync (c.bcen.p + e0, E * sizeof (double), r->bcen.data (), ne * sizeof (double),
+                               ne * sizeof (double), 3, hipMemcpyHostToDevice, c.stream));
+    /* the host copies are the caller's: done before it may free them */
+    PMX_HIP (hipStreamSynchronize (c.stream));
+  }
+  c.snap_sel = true;
+  return PMX_OK;
+}
+
 int merge_models (const pmx_model *ms, int count, Merged &g, std::vector<InstOff> &off)
 {
   if (!ms || count < 1) return fail (PMX_EINVAL, "null argument or empty batch");

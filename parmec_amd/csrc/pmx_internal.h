@@ -336,6 +336,13 @@ struct pmx_ctx
    * step bins nothing) */
   bool fuse_bin = false, need_bin = false;
   int bin_until = 0;
+  /* a call whose instances did not all start at time 0 (instances resumed
+   * from an earlier context beside new ones): isel [I] flags the instances
+   * at time 0, the only ones the first-call stages (euler, shapes) update
+   * when first_sel is set, and the only ones whose tree snapshot the call's
+   * first build takes when snap_sel is set (the others bring theirs) */
+  dvec<int> isel;
+  bool first_sel = false, snap_sel = false;
   double rmax = 0.0;           /* largest sphere AABB half-width */
 
   double curtime = 0.0, curstep = 0.0;
@@ -404,8 +411,40 @@ int validate_model (const pmx_model *m);
 InstOff model_counts (const pmx_model &m);
 int upload_core (pmx_ctx *c, const pmx_model *m, const std::vector<InstOff> &off);
 
+/* what a call may take beyond dem(): steps, each instance's step count
+ * (in place of durations; instances may then be at different times, each
+ * at time 0 or resumed at the call's step); stop_fn, asked after each group
+ * of steps whether the call should pause there (paused_at: the steps taken,
+ * the unfinished instances keep their time and take no stepnum) */
+struct DemExtra
+{
+  const int *steps = nullptr;
+  int (*stop_fn) (void*, int) = nullptr;
+  void *stop_user = nullptr;
+  int paused_at = -1;
+};
+
 int dem_core (pmx_ctx *c, const double *durations, double duration, double step, const pmx_forcing *fc, int nfc,
-              void (*done_fn) (void*, int), void *user, double *elapsed);
+              void (*done_fn) (void*, int), void *user, double *elapsed, DemExtra *x = nullptr);
+int alloc_rows (pmx_ctx &c);
+
+/* an instance's contact rows and tree snapshot, carried from one context to
+ * another (the rows in the instance's own numbering) */
+struct ResumeRows
+{
+  double curtime = 0.0, curstep = 0.0;   /* the instance's time, step and dem() count */
+  int stepnum = 0;
+  int cap = 0, ne = 0, nt = 0;
+  std::vector<uint32_t> key;        /* [cap][ne] */
+  std::vector<unsigned char> st;    /* [3][cap][ne] of the context's state type */
+  std::vector<int> cnt;             /* [ne] */
+  std::vector<double> bcen;         /* [3][ne] */
+};
+int export_rows (pmx_ctx &c, int inst, ResumeRows &r);
+/* into the row buffer of the context's next call, after pmx_upload_batch of
+ * models at time 0: rs[i] sets instance i's time and rows; rs[i] null:
+ * instance i starts at time 0 with no contact */
+int import_rows (pmx_ctx &c, const std::vector<const ResumeRows*> &rs);
 
 /* ---- instances (pmx_batch.hip) ---- */
 /* host copy of a (possibly concatenated) model: owns the arrays the view points to */

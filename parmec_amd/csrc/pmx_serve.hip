@@ -7,7 +7,10 @@
  * requests queued together (within the gather window, up to max_batch) are
  * concatenated into one batch context, and each context is stepped by its
  * own worker thread on its own HIP stream, so batches admitted at different
- * times overlap on the device.  Inside a batch every request runs its own
+ * times overlap on the device.  A batch holds one kind of request (spheres,
+ * or bodies joined by springs), so that each of its launches serves one
+ * kind of work and batches of the two kinds overlap on the device.  Inside
+ * a batch every request runs its own
  * step count (pmx_dem_batch_durations): it is frozen when done and its state
  * is copied back at once, so a short request does not wait for a long one.
  * A batch's device memory is released when its last request has finished.
@@ -26,6 +29,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <thread>
 
@@ -52,6 +56,10 @@ struct Req
   int batch = 0;                  /* requests in its batch */
   long long pairs = 0;
   std::string err;
+  /* continuous batching: steps taken in earlier contexts, and the contact
+   * rows and tree snapshot carried from the last one */
+  int done_steps = 0;
+  std::unique_ptr<ResumeRows> rr;
 };
 
 struct Group
@@ -61,6 +69,9 @@ struct Group
   size_t bytes = 0;
   std::thread th;
   bool finished = false;
+  int kind = 0;
+  bool rolling = false;           /* takes in queued requests of its kind while it runs */
+  int nlive = 0;                  /* requests it runs now (rolling; under the server's lock) */
 };
 
 /* device bytes of one request in a context (rows, broad phase, state) */
@@ -73,6 +84,21 @@ size_t estimate_bytes (const pmx_model &m, int precision, int cap)
                       + 120;                                                                   /* slotted records */
   return E * per_sphere + N * 420 + T * 120 + S * 460 + (size_t) m.nodnum * 56 + (size_t) m.facnum * 16 +
          ((size_t) 1 << 20);
+}
+
+/* the kind of work a request brings: spheres (contact detection and rows),
+ * bodies with springs, torsion springs or mesh faces */
+int request_kind (const pmx_model &m)
+{
+  return (m.ellnum > 0 ? 1 : 0) | (m.sprnum > 0 || m.trqsprnum > 0 || m.facnum > 0 ? 2 : 0);
+}
+
+/* a request a running batch can take in and carry over (continuous
+ * batching): it starts at time 0, and nothing in it depends on the call's
+ * first step beyond euler and shapes (moving obstacles, restraints) */
+bool rollable (const pmx_model &m)
+{
+  return m.curtime == 0.0 && m.obsnum == 0 && m.rstnum == 0;
 }
 
 void copy_outputs (const pmx_model &s, pmx_model *d)
@@ -177,6 +203,167 @@ void worker (pmx_server *s, Group *g)
   s->cv.notify_all ();
 }
 
+/* continuous batching: the batch runs its requests in one context; at most
+ * every repack_s seconds, when requests of its kind are queued, the call
+ * pauses after a group of steps, the unfinished requests are carried over
+ * (state, contact rows, tree snapshot, time), the queued ones join, and the
+ * call resumes over all of them.  Each request's result stays bitwise its
+ * solo result: its tree is rebuilt from the centres it was built from, its
+ * rows and time go on from the step where it paused. */
+struct RollCtx
+{
+  pmx_server *s;
+  Group *g;
+  pmx_ctx *c;
+  std::vector<Req*> *live;
+  std::chrono::steady_clock::time_point t_round;
+};
+
+void on_done_roll (void *user, int i)
+{
+  RollCtx *d = (RollCtx*) user;
+  Req *r = (*d->live)[i];
+  int rc = pmx_download_instance (d->c, i, &r->g.m);
+  size_t npairs = 0;
+  if (!rc) rc = pmx_contacts_instance (d->c, i, nullptr, 0, &npairs);
+  std::string err = rc ? pmx_last_error () : "";
+  {
+    std::lock_guard<std::mutex> lk (d->s->mu);
+    r->t_done = now_s (d->s->t0);
+    r->pairs = (long long) npairs;
+    r->status = rc ? PMX_REQ_FAILED : PMX_REQ_DONE;
+    r->err = err;
+    r->rr.reset ();
+    if (rc) d->s->failed++;
+    else { d->s->done++; d->s->particle_steps += (double) r->g.m.parnum * r->nsteps; }
+    d->s->used -= r->bytes;   /* its memory is back in the budget */
+    d->g->bytes -= r->bytes;
+    d->g->nlive--;
+  }
+  d->s->cv.notify_all ();
+}
+
+/* take the queued requests of the batch's kind that fit the budget (the
+ * lock held) */
+void take_queued (pmx_server *s, Group *g, std::vector<Req*> &into, size_t room)
+{
+  double t = now_s (s->t0);
+  for (auto it = s->queue.begin (); it != s->queue.end () && into.size () < room; )
+  {
+    Req *r = *it;
+    if (!rollable (r->g.m) || request_kind (r->g.m) != g->kind || r->step != into[0]->step)
+    {
+      ++it;
+      continue;
+    }
+    if (s->used + r->bytes > s->opt.hbm_budget) break;
+    s->used += r->bytes;
+    g->bytes += r->bytes;
+    r->status = PMX_REQ_RUNNING;
+    r->t_admit = t;
+    into.push_back (r);
+    it = s->queue.erase (it);
+  }
+}
+
+int roll_pause (void *user, int)
+{
+  RollCtx *d = (RollCtx*) user;
+  double age = std::chrono::duration<double> (std::chrono::steady_clock::now () - d->t_round).count ();
+  if (age < d->s->opt.repack_s) return 0;
+  std::lock_guard<std::mutex> lk (d->s->mu);
+  if ((int) d->live->size () >= d->s->opt.max_batch) return 0;
+  for (Req *r : d->s->queue)
+    if (rollable (r->g.m) && request_kind (r->g.m) == d->g->kind && r->step == (*d->live)[0]->step &&
+        d->s->used + r->bytes <= d->s->opt.hbm_budget)
+      return 1;
+  return 0;
+}
+
+void rolling_worker (pmx_server *s, Group *g)
+{
+  pmx_options o {};
+  o.precision = s->opt.precision;
+  o.device = s->opt.device;
+  o.row_capacity = s->opt.row_capacity;
+  pmx_ctx *c = nullptr;
+  int rc = pmx_create (&o, &c);
+  std::vector<Req*> live = g->reqs;
+  std::string err;
+  while (!rc && !live.empty ())
+  {
+    int n = (int) live.size ();
+    double step = live[0]->step;
+    /* models at time 0 for the upload (the resumed ones get their time from
+     * their carried state); forcing tables from each request's own step */
+    std::vector<pmx_model> ms (n);
+    std::vector<pmx_forcing> fs (n);
+    std::vector<int> steps (n);
+    std::vector<const ResumeRows*> rs (n, nullptr);
+    for (int i = 0; i < n; i++)
+    {
+      Req *r = live[i];
+      ms[i] = r->g.m;
+      ms[i].curtime = 0.0; ms[i].curstep = 0.0; ms[i].stepnum = 0;
+      size_t k = r->done_steps;
+      fs[i] = r->f;
+      if (fs[i].gtab) fs[i].gtab += 3*k;
+      if (fs[i].dtab) fs[i].dtab += 6*k;
+      if (fs[i].prstab) fs[i].prstab += 6*k*(size_t) (fs[i].prscols > 0 ? fs[i].prscols : 0);
+      steps[i] = r->nsteps - r->done_steps;
+      rs[i] = r->rr.get ();
+      r->batch = n;
+    }
+    if ((rc = pmx_upload_batch (c, ms.data (), n)) || (rc = import_rows (*c, rs))) break;
+    RollCtx d {s, g, c, &live, std::chrono::steady_clock::now ()};
+    DemExtra x;
+    x.steps = steps.data ();
+    x.stop_fn = roll_pause;
+    x.stop_user = &d;
+    double el = 0.0;
+    rc = dem_core (c, nullptr, 0.0, step, fs.data (), n, on_done_roll, &d, &el, &x);
+    if (rc) break;
+    if (x.paused_at < 0) break;   /* every request finished */
+    /* carry the unfinished requests over; the queued ones of this kind join */
+    std::vector<Req*> next;
+    for (int i = 0; i < n && !rc; i++)
+    {
+      Req *r = live[i];
+      if (r->status != PMX_REQ_RUNNING) continue;
+      std::unique_ptr<ResumeRows> rr (new ResumeRows);
+      if ((rc = pmx_download_instance (c, i, &r->g.m)) || (rc = export_rows (*c, i, *rr))) break;
+      rr->curtime = r->g.m.curtime; rr->curstep = r->g.m.curstep; rr->stepnum = r->g.m.stepnum;
+      r->rr = std::move (rr);
+      r->done_steps += x.paused_at;
+      next.push_back (r);
+    }
+    if (rc) break;
+    {
+      std::lock_guard<std::mutex> lk (s->mu);
+      if (!next.empty ()) take_queued (s, g, next, (size_t) s->opt.max_batch);
+      g->nlive = (int) next.size ();
+    }
+    live.swap (next);
+  }
+  if (rc) err = pmx_last_error ();
+  if (c) pmx_destroy (c);
+  {
+    std::lock_guard<std::mutex> lk (s->mu);
+    for (Req *r : live)
+      if (r->status == PMX_REQ_RUNNING)
+      {
+        r->status = PMX_REQ_FAILED;
+        r->err = err.empty () ? "batch ended before the request finished" : err;
+        r->t_done = now_s (s->t0);
+        r->rr.reset ();
+        s->failed++;
+      }
+    s->used -= g->bytes;
+    g->finished = true;
+  }
+  s->cv.notify_all ();
+}
+
 void scheduler (pmx_server *s)
 {
   std::unique_lock<std::mutex> lk (s->mu);
@@ -209,18 +396,44 @@ void scheduler (pmx_server *s)
         s->cv.wait_for (lk, std::chrono::duration<double> (wait));
         continue;
       }
+      /* the batch: the oldest request and, in arrival order, the queued
+       * requests of its kind (same step and time state, same mix of
+       * spheres and bodies with springs or meshes: one batch's launches then
+       * serve one kind of work, and batches of the other kind run beside it
+       * on their own streams) while they fit the budget */
       std::vector<Req*> take;
       size_t bytes = 0;
-      while (!s->queue.empty () && (int) take.size () < s->opt.max_batch)
+      const bool roll = s->opt.repack_s > 0.0;
+      auto joins_running = [&] (const Req *r) {
+        /* a running continuous batch of its kind will take it in */
+        if (!roll || !rollable (r->g.m)) return false;
+        for (Group *g : s->groups)
+          if (g->rolling && !g->finished && g->kind == request_kind (r->g.m) && g->nlive < s->opt.max_batch)
+            return true;
+        return false;
+      };
+      for (auto it = s->queue.begin (); it != s->queue.end () && (int) take.size () < s->opt.max_batch; )
       {
-        Req *r = s->queue.front ();
-        if (!take.empty () && (r->step != take[0]->step || r->g.m.curtime != take[0]->g.m.curtime ||
-                               r->g.m.curstep != take[0]->g.m.curstep || r->g.m.stepnum != take[0]->g.m.stepnum))
-          break;
+        Req *r = *it;
+        if (joins_running (r) ||
+            (!take.empty () && (r->step != take[0]->step || r->g.m.curtime != take[0]->g.m.curtime ||
+                                r->g.m.curstep != take[0]->g.m.curstep || r->g.m.stepnum != take[0]->g.m.stepnum ||
+                                request_kind (r->g.m) != request_kind (take[0]->g.m) ||
+                                (roll && rollable (r->g.m) != rollable (take[0]->g.m)))))
+        {
+          ++it;
+          continue;
+        }
         if (s->used + bytes + r->bytes > s->opt.hbm_budget) break;   /* arrival order: wait for memory */
         take.push_back (r);
         bytes += r->bytes;
-        s->queue.pop_front ();
+        it = s->queue.erase (it);
+      }
+      if (take.empty () && !s->queue.empty () && joins_running (s->queue.front ()))
+      {
+        /* everything queued waits for a running batch of its kind */
+        s->cv.wait (lk);
+        continue;
       }
       if (take.empty ())
       {
@@ -243,12 +456,15 @@ void scheduler (pmx_server *s)
       g->reqs = take;
       g->reported.assign (take.size (), 0);
       g->bytes = bytes;
+      g->kind = request_kind (take[0]->g.m);
+      g->rolling = roll && rollable (take[0]->g.m);
+      g->nlive = (int) take.size ();
       s->used += bytes;
       double t = now_s (s->t0);
       for (Req *r : take) { r->status = PMX_REQ_RUNNING; r->t_admit = t; r->batch = (int) take.size (); }
       s->batches++;
       s->groups.push_back (g);
-      g->th = std::thread (worker, s, g);
+      g->th = std::thread (g->rolling ? rolling_worker : worker, s, g);
       continue;
     }
     s->cv.wait (lk);

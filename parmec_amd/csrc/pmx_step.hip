@@ -1658,10 +1658,12 @@ __global__ void k_obspnt (int O, double *obspnt, double *obsang, double *obslin,
   }
 }
 
-__global__ void k_shapes (int E, int N, const int *part, double *center, const double *rotation, const double *position)
+/* sel (per instance, or null for all): the instances the launch updates */
+__global__ void k_shapes (int E, int N, const int *part, double *center, const double *rotation, const double *position,
+                          const int *sel, const int *ell_inst)
 {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= E) return;
+  if (e >= E || (sel && !sel[ell_inst[e]])) return;
   int j = part[e];
   double L[9], C[3], c[3];
 #pragma unroll
@@ -1677,10 +1679,11 @@ __global__ void k_shapes (int E, int N, const int *part, double *center, const d
 }
 
 /* euler_task, parmec.ispc:224-266 */
-__global__ void k_euler (int N, double *angular, const double *linear, double *rotation, double *position, double step)
+__global__ void k_euler (int N, double *angular, const double *linear, double *rotation, double *position, double step,
+                         const int *sel, const int *par_inst)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
+  if (i >= N || (sel && !sel[par_inst[i]])) return;
   double O[3], L1[9], DL[9], L2[9], o[3];
   O[0] = step*angular[i];
   O[1] = step*angular[(size_t) N + i];
@@ -2158,7 +2161,8 @@ int step_shapes (pmx_ctx &c)
   if (c.E == 0) return PMX_OK;
   ProfScope ps (c, K_SHAPES, 168.0 * c.E);
   hipLaunchKernelGGL (k_shapes, grid (c.E), dim3 (PMX_TPB), 0, c.stream, c.E, c.N, (const int*) c.part.p, c.center.p,
-                      (const double*) c.rotation.p, (const double*) c.position.p);
+                      (const double*) c.rotation.p, (const double*) c.position.p,
+                      c.first_sel ? (const int*) c.isel.p : (const int*) nullptr, (const int*) c.ell_inst.p);
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
 }
@@ -2166,7 +2170,8 @@ int step_shapes (pmx_ctx &c)
 int step_euler (pmx_ctx &c, double h)
 {
   hipLaunchKernelGGL (k_euler, grid (c.N), dim3 (PMX_TPB), 0, c.stream, c.N, c.angular.p, (const double*) c.linear.p,
-                      c.rotation.p, c.position.p, h);
+                      c.rotation.p, c.position.p, h, c.first_sel ? (const int*) c.isel.p : (const int*) nullptr,
+                      (const int*) c.par_inst.p);
   PMX_HIP (hipGetLastError ());
   return PMX_OK;
 }

@@ -808,9 +808,12 @@ int tree_build (pmx_ctx &c, bool all)
   if (n == 0) { t.nleaves = 0; t.root = -1; return PMX_OK; }
   /* the tree of an instance is a function of the centres it was last built
    * from, so a rebuild forced by one instance leaves the others unchanged */
-  if (c.bcen.alloc (3*(size_t) n) != hipSuccess) { pmx_set_error ("out of device memory (tree)"); return PMX_ENOMEM; }
+  /* all: every instance, except resumed ones, which brought their snapshots
+   * (snap_sel: isel flags the instances to take) */
+  if (!c.snap_sel && c.bcen.alloc (3*(size_t) n) != hipSuccess) { pmx_set_error ("out of device memory (tree)"); return PMX_ENOMEM; }
   hipLaunchKernelGGL (k_snap, grid (n), dim3 (PMX_TPB), 0, c.stream, n, (const double*) c.center.p, c.bcen.p,
-                      (const int*) c.ell_inst.p, all ? (const int*) nullptr : (const int*) c.ovf.p);
+                      (const int*) c.ell_inst.p,
+                      all ? (c.snap_sel ? (const int*) c.isel.p : (const int*) nullptr) : (const int*) c.ovf.p);
   PMX_HIP (hipGetLastError ());
   const double *cx = c.bcen.p, *cy = c.bcen.p + n, *cz = c.bcen.p + 2*(size_t) n;
   int nbext = std::min (1024, (n + PMX_TPB - 1) / PMX_TPB);

@@ -74,15 +74,42 @@ def _same(a, b):
 def test_served_equals_alone(prec):
     reqs = _requests()
     print(f"case serve precision {prec}: {len(reqs)} requests, batched and staggered", flush=True)
-    together, info1 = _serve(prec, reqs, gather_s=0.5)           # one batch of all
+    together, info1 = _serve(prec, reqs, gather_s=0.5)           # one batch per kind of request
     staggered, info2 = _serve(prec, reqs, gap=0.03, gather_s=0.0)  # separate, overlapping batches
-    assert max(i["batch"] for i in info1) == len(reqs)
+    # a batch holds one kind: spheres, or bodies joined by springs or meshes
+    kinds = [int(m.ellnum > 0) + 2 * int(m.sprnum > 0 or m.facnum > 0) for (m, d, h, f) in reqs]
+    assert [i["batch"] for i in info1] == [kinds.count(k) for k in kinds]
     assert all(i["status"] == pmx.REQ_DONE for i in info1 + info2)
     for k, (m, d, h, f) in enumerate(reqs):
         alone = _alone(prec, m, d, h, f)
         _same(together[k], alone)
         _same(staggered[k], alone)
         assert together[k].curtime == alone.curtime
+
+
+@pytest.mark.parametrize("prec", [pmx.PMX_FP64, pmx.PMX_MIXED])
+def test_continuous_batching_equals_alone(prec):
+    """Requests submitted one by one join the running batch of their kind
+    (continuous batching: the batch pauses after a group of steps, carries
+    its unfinished requests over with their contact rows, tree snapshots and
+    time, and resumes with the new ones): each result is still bitwise its
+    solo run, and the requests ran in batches of changing size."""
+    reqs = _requests()
+    print(f"case continuous batching precision {prec}: {len(reqs)} requests, staggered", flush=True)
+    outs, info = _serve(prec, reqs, gap=0.004, gather_s=0.0, repack_s=1e-4)
+    assert all(i["status"] == pmx.REQ_DONE for i in info)
+    print("batch sizes", [i["batch"] for i in info], flush=True)
+    assert max(i["batch"] for i in info) > 1
+    bad = []
+    for k, (m, d, h, f) in enumerate(reqs):
+        alone = _alone(prec, m, d, h, f)
+        for name in OUT:
+            x, y = getattr(outs[k], name, None), getattr(alone, name, None)
+            if x is not None and not np.array_equal(x, y):
+                bad.append((k, name, float(np.abs(np.asarray(x) - np.asarray(y)).max())))
+        if outs[k].curtime != alone.curtime or outs[k].stepnum != alone.stepnum:
+            bad.append((k, "time", outs[k].curtime - alone.curtime, outs[k].stepnum, alone.stepnum))
+    assert not bad, bad
 
 
 def test_budget_serialises_admission():
