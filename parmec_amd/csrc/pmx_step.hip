@@ -313,6 +313,33 @@ __device__ __forceinline__ bool overlap (double ax, double ay, double az, double
  * filter (1e-9 (|c| + r) <= 1e-9 (|lo| + |hi| + (hi - lo))), so the filter
  * would have kept none of them.  An empty leaf gets zeros (no sphere reads
  * it: an empty candidate is skipped by its count). */
+/* a DPP lane move of a double: the value of the lane the control selects,
+ * or the lane's own where there is none */
+template <int CTRL, int ROWS, int BANKS>
+__device__ __forceinline__ double dpp_f64 (double v)
+{
+  int lo = __double2loint (v), hi = __double2hiint (v);
+  lo = __builtin_amdgcn_update_dpp (lo, lo, CTRL, ROWS, BANKS, false);
+  hi = __builtin_amdgcn_update_dpp (hi, hi, CTRL, ROWS, BANKS, false);
+  return __hiloint2double (hi, lo);
+}
+
+/* the minimum over the wave's 64 lanes, in lane 63 (the other lanes hold
+ * partial minima): row_shr 1, 2, 3, 4, 8 within rows of 16, then
+ * row_bcast 15 and 31 across rows.  min is exact, so the order of the
+ * folds does not change the result */
+__device__ __forceinline__ double wave_min63 (double v)
+{
+  v = fmin (v, dpp_f64<0x111, 0xf, 0xf> (v));   /* row_shr:1 */
+  v = fmin (v, dpp_f64<0x112, 0xf, 0xf> (v));   /* row_shr:2 */
+  v = fmin (v, dpp_f64<0x113, 0xf, 0xf> (v));   /* row_shr:3 */
+  v = fmin (v, dpp_f64<0x114, 0xf, 0xe> (v));   /* row_shr:4 */
+  v = fmin (v, dpp_f64<0x118, 0xf, 0xc> (v));   /* row_shr:8 */
+  v = fmin (v, dpp_f64<0x142, 0xa, 0xf> (v));   /* row_bcast:15 */
+  v = fmin (v, dpp_f64<0x143, 0xc, 0xf> (v));   /* row_bcast:31 */
+  return v;
+}
+
 __global__ __launch_bounds__(256)
 void k_leafbox (int nleaves, const int *lcount, const int *loff, const double *lpos, int lstride, double *lbox,
                 const int *ctl)
@@ -320,6 +347,16 @@ void k_leafbox (int nleaves, const int *lcount, const int *loff, const double *l
   if (ctl[CTL_ABORT]) return;
   int L = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (L >= nleaves) return;
+  /* slotted records: the first 64 slots are read together with the count
+   * (every slot of a leaf is in bounds; the ones past the count are not
+   * used), one memory round trip instead of two */
+  double2 pa = make_double2 (0.0, 0.0), pb = pa;
+  if (lstride)
+  {
+    const double2 *rp = (const double2*) (lpos + 4*((size_t) L * lstride + lane));
+    pa = rp[0];
+    pb = rp[1];
+  }
   int cnt = lcount[L], off = lstride ? L * lstride : loff[L];
   double blo[3] = {0.0, 0.0, 0.0}, bhi[3] = {0.0, 0.0, 0.0};
   if (cnt > 0)
@@ -328,8 +365,13 @@ void k_leafbox (int nleaves, const int *lcount, const int *loff, const double *l
     for (int d = 0; d < 3; d++) { blo[d] = DBL_MAX; bhi[d] = -DBL_MAX; }
     for (int k = lane; k < cnt; k += 64)
     {
-      const double2 *rp = (const double2*) (lpos + 4*(size_t) (off + k));
-      double2 a = rp[0], b = rp[1];
+      double2 a = pa, b = pb;
+      if (!lstride || k != lane)
+      {
+        const double2 *rp = (const double2*) (lpos + 4*(size_t) (off + k));
+        a = rp[0];
+        b = rp[1];
+      }
       double c[3] = {a.x, a.y, b.x}, r = b.y;
 #pragma unroll
       for (int d = 0; d < 3; d++)
@@ -338,14 +380,14 @@ void k_leafbox (int nleaves, const int *lcount, const int *loff, const double *l
         bhi[d] = fmax (bhi[d], c[d] + r);
       }
     }
+    /* the wave's min and max into lane 63 by DPP row shifts and
+     * broadcasts (plain lane moves, no LDS crossbar) */
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1)
-#pragma unroll
-      for (int d = 0; d < 3; d++)
-      {
-        blo[d] = fmin (blo[d], __shfl_xor (blo[d], o, 64));
-        bhi[d] = fmax (bhi[d], __shfl_xor (bhi[d], o, 64));
-      }
+    for (int d = 0; d < 3; d++)
+    {
+      blo[d] = wave_min63 (blo[d]);
+      bhi[d] = -wave_min63 (-bhi[d]);
+    }
 #pragma unroll
     for (int d = 0; d < 3; d++)
     {
@@ -354,7 +396,7 @@ void k_leafbox (int nleaves, const int *lcount, const int *loff, const double *l
       bhi[d] += tol;
     }
   }
-  if (lane == 0)
+  if (lane == 63)
 #pragma unroll
     for (int d = 0; d < 3; d++)
     {
