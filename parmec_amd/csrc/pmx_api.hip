@@ -925,6 +925,7 @@ int dem_core (pmx_ctx *c, const double *durations, double duration, double step,
     }
     if ((rc = up (*c, c->gtab, g.data (), g.size (), "gravity table")) || (rc = up (*c, c->dtab, d.data (), d.size (), "damping table")))
       return rc;
+    c->damped = std::any_of (d.begin (), d.end (), [] (double v) { return v != 0.0; });
     if (c->PRSC > 0)
     {
       /* prescribed values of each history column at each step's curtime,
@@ -1361,7 +1362,8 @@ static int adaptive_core (pmx_ctx *c, double duration, double step, double adapt
   c->graph_valid = false;
   c->stats.max_row = 0;
   c->adp_on = true;
-  c->fuse_bin = false;   /* each step is enqueued in two parts: k_bin at its start */
+  c->fuse_bin = false;
+  c->damped = true;      /* damping rows come step by step */   /* each step is enqueued in two parts: k_bin at its start */
   long long rebuilds = 0, partial = 0, grown = 0;
   if (c->E > 0)
   {

@@ -334,6 +334,7 @@ struct pmx_ctx
    * where the records are not in place (need_bin: after a build, an aborted
    * step or a tuning run).  bin_until: the call's step count (the last
    * step bins nothing) */
+  bool damped = true;          /* the call's damping table holds a nonzero entry (else k_dynamics skips it) */
   bool fuse_bin = false, need_bin = false;
   int bin_until = 0;
   /* a call whose instances did not all start at time 0 (instances resumed

@@ -1383,7 +1383,8 @@ void k_dynamics (int N, double *angular, double *linear, double *rotation, doubl
   if (list) i = list[i];
   int flag = iso[i];
   if ((flag & 1) != (int) ISO || frozen (stop, par_inst[i], ctl)) return;
-  const double *damping = dtab + 6*((size_t) kstep*ninst + (ninst > 1 ? par_inst[i] : 0));
+  /* dtab null: no damping anywhere in the call (no loads, no terms) */
+  const double *damping = dtab ? dtab + 6*((size_t) kstep*ninst + (ninst > 1 ? par_inst[i] : 0)) : nullptr;
   double half = 0.5*step0;
   double step = 0.5*(step0+step1);
   double O[3], o[3], v[3], L1[9], im, f[3], t[3], T[3], DL[9], A[3], B[3];
@@ -1442,7 +1443,7 @@ void k_dynamics (int N, double *angular, double *linear, double *rotation, doubl
     f[0] = force[i]; f[1] = force[(size_t) N + i]; f[2] = force[2*(size_t) N + i];
     t[0] = torque[i]; t[1] = torque[(size_t) N + i]; t[2] = torque[2*(size_t) N + i];
   }
-  if (damping[0] != 0.0 || damping[1] != 0.0 || damping[2] != 0.0)
+  if (damping && (damping[0] != 0.0 || damping[1] != 0.0 || damping[2] != 0.0))
   {
     double ma = mass[i];
     f[0] -= ma * damping[0] * v[0];
@@ -1450,7 +1451,7 @@ void k_dynamics (int N, double *angular, double *linear, double *rotation, doubl
     f[2] -= ma * damping[2] * v[2];
     force[i] = f[0]; force[(size_t) N + i] = f[1]; force[2*(size_t) N + i] = f[2];
   }
-  if (damping[3] != 0.0 || damping[4] != 0.0 || damping[5] != 0.0)
+  if (damping && (damping[3] != 0.0 || damping[4] != 0.0 || damping[5] != 0.0))
   {
     double w[3], tt[3], TT[3];
     w[0] = damping[3]*angular[3*(size_t) N + i];
@@ -2011,7 +2012,8 @@ int step_dynamics (pmx_ctx &c, double step0, double step1)
   ProfScope ps (c, K_DYNAMICS, dyn);
 #define DYN_ARGS c.N, c.angular.p, c.linear.p, c.rotation.p, \
                  c.position.p, (const double*) c.inertia.p, (const double*) c.inverse.p, (const double*) c.mass.p, \
-                 (const double*) c.invm.p, (const double*) c.dtab.p, (const int*) c.ctl.p, c.force.p, c.torque.p, \
+                 (const double*) c.invm.p, c.damped ? (const double*) c.dtab.p : nullptr, (const int*) c.ctl.p, \
+                 c.force.p, c.torque.p, \
                  step0, step1, (const int*) c.parell.p, c.E, c.center.p, (const signed char*) c.iso.p, c.I, \
                  (const int*) c.par_inst.p, c.staggered ? (const int*) c.istop.p : (const int*) nullptr, \
                  (const int*) c.rowcnt[0].p, (const int*) c.rowcnt[1].p, c.cur0, (const double*) c.gtab.p
