@@ -712,7 +712,9 @@ static bool place_prof_nodes (pmx_ctx *c)
   for (size_t k = 0; k < c->prof.cap.size (); k++)
   {
     Prof::Cap &q = c->prof.cap[k];
-    if (hipEventCreate (&q.a) != hipSuccess || hipEventCreate (&q.b) != hipSuccess) return false;
+    if (hipEventCreateWithFlags (&q.a, hipEventDisableSystemFence) != hipSuccess ||
+        hipEventCreateWithFlags (&q.b, hipEventDisableSystemFence) != hipSuccess)
+      return false;
     if (hipGraphAddEventRecordNode (&q.na, c->pgraph, q.before ? &q.before : nullptr, q.before ? 1 : 0, q.a) != hipSuccess ||
         hipGraphAddDependencies (c->pgraph, &q.na, &first[k], 1) != hipSuccess ||
         hipGraphAddEventRecordNode (&q.nb, c->pgraph, &q.after, 1, q.b) != hipSuccess ||

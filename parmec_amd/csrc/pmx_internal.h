@@ -157,8 +157,11 @@ struct Prof
   hipEvent_t get ()
   {
     if (!pool.empty ()) { hipEvent_t e = pool.back (); pool.pop_back (); return e; }
+    /* no system-scope fence: the event marks the kernel's completion on the
+     * device timeline without writing the caches back for the host (a
+     * writeback after each timed kernel would be counted in its time) */
     hipEvent_t e = nullptr;
-    (void) hipEventCreate (&e);
+    (void) hipEventCreateWithFlags (&e, hipEventDisableSystemFence);
     return e;
   }
 };
