@@ -216,6 +216,8 @@ struct RollCtx
   Group *g;
   pmx_ctx *c;
   std::vector<Req*> *live;
+  std::vector<char> *reported;    /* a reported request is the waiter's: never touched again */
+  double step;
   std::chrono::steady_clock::time_point t_round;
 };
 
@@ -239,6 +241,7 @@ void on_done_roll (void *user, int i)
     d->s->used -= r->bytes;   /* its memory is back in the budget */
     d->g->bytes -= r->bytes;
     d->g->nlive--;
+    (*d->reported)[i] = 1;
   }
   d->s->cv.notify_all ();
 }
@@ -274,7 +277,7 @@ int roll_pause (void *user, int)
   std::lock_guard<std::mutex> lk (d->s->mu);
   if ((int) d->live->size () >= d->s->opt.max_batch) return 0;
   for (Req *r : d->s->queue)
-    if (rollable (r->g.m) && request_kind (r->g.m) == d->g->kind && r->step == (*d->live)[0]->step &&
+    if (rollable (r->g.m) && request_kind (r->g.m) == d->g->kind && r->step == d->step &&
         d->s->used + r->bytes <= d->s->opt.hbm_budget)
       return 1;
   return 0;
@@ -289,11 +292,13 @@ void rolling_worker (pmx_server *s, Group *g)
   pmx_ctx *c = nullptr;
   int rc = pmx_create (&o, &c);
   std::vector<Req*> live = g->reqs;
+  std::vector<char> reported (live.size (), 0);
   std::string err;
   while (!rc && !live.empty ())
   {
     int n = (int) live.size ();
     double step = live[0]->step;
+    reported.assign (n, 0);
     /* models at time 0 for the upload (the resumed ones get their time from
      * their carried state); forcing tables from each request's own step */
     std::vector<pmx_model> ms (n);
@@ -315,7 +320,7 @@ void rolling_worker (pmx_server *s, Group *g)
       r->batch = n;
     }
     if ((rc = pmx_upload_batch (c, ms.data (), n)) || (rc = import_rows (*c, rs))) break;
-    RollCtx d {s, g, c, &live, std::chrono::steady_clock::now ()};
+    RollCtx d {s, g, c, &live, &reported, step, std::chrono::steady_clock::now ()};
     DemExtra x;
     x.steps = steps.data ();
     x.stop_fn = roll_pause;
@@ -328,8 +333,13 @@ void rolling_worker (pmx_server *s, Group *g)
     std::vector<Req*> next;
     for (int i = 0; i < n && !rc; i++)
     {
+      bool was;
+      {
+        std::lock_guard<std::mutex> lk (s->mu);
+        was = reported[i] != 0;
+      }
+      if (was) continue;   /* finished in this round (and maybe freed by its waiter) */
       Req *r = live[i];
-      if (r->status != PMX_REQ_RUNNING) continue;
       std::unique_ptr<ResumeRows> rr (new ResumeRows);
       if ((rc = pmx_download_instance (c, i, &r->g.m)) || (rc = export_rows (*c, i, *rr))) break;
       rr->curtime = r->g.m.curtime; rr->curstep = r->g.m.curstep; rr->stepnum = r->g.m.stepnum;
@@ -344,14 +354,16 @@ void rolling_worker (pmx_server *s, Group *g)
       g->nlive = (int) next.size ();
     }
     live.swap (next);
+    reported.assign (live.size (), 0);
   }
   if (rc) err = pmx_last_error ();
   if (c) pmx_destroy (c);
   {
     std::lock_guard<std::mutex> lk (s->mu);
-    for (Req *r : live)
-      if (r->status == PMX_REQ_RUNNING)
+    for (size_t i = 0; i < live.size (); i++)
+      if (!reported[i])
       {
+        Req *r = live[i];
         r->status = PMX_REQ_FAILED;
         r->err = err.empty () ? "batch ended before the request finished" : err;
         r->t_done = now_s (s->t0);
