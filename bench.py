@@ -271,10 +271,11 @@ def serve_leg(pmx, workloads, count, rate, seed):
     # batched in both precisions (batching and precision separated), the
     # continuous form (requests join running batches of their kind), and the
     # plain path: one request at a time, FP64
-    batched = dict(gather_s=0.25, max_active=6, max_batch=12)
+    batched = dict(gather_s=0.25, max_active=4, max_batch=16, longest_first=True)
     forms = (("batched_mixed", pmx.PMX_MIXED, batched),
              ("batched_fp64", pmx.PMX_FP64, batched),
-             ("continuous_mixed", pmx.PMX_MIXED, dict(gather_s=0.05, max_active=4, max_batch=16, repack_s=0.25)),
+             ("continuous_mixed", pmx.PMX_MIXED, dict(gather_s=0.05, max_active=4, max_batch=16, repack_s=0.25,
+                                                      longest_first=True)),
              ("plain_fp64", pmx.PMX_FP64, dict(max_batch=1, max_active=1)))
     for name, prec, kw in forms:
         srv = pmx.Server(precision=prec, **kw)
@@ -298,7 +299,8 @@ def serve_leg(pmx, workloads, count, rate, seed):
     for name in ("batched_mixed", "batched_fp64", "continuous_mixed"):
         out[name]["speedup_vs_plain_fp64"] = out[name]["value"] / out["plain_fp64"]["value"]
     out["policy"] = ("batched: one kind of request per batch (spheres, or bodies joined by springs), gathered over "
-                     "0.25 s windows, at most 12 requests per batch and 6 batches at once, in arrival order; "
+                     "0.25 s windows, at most 16 requests per batch and 4 batches at once, the queued requests with "
+                     "the most steps first; "
                      "continuous: a running batch of a kind (at most 16 requests, 4 batches at once) pauses at "
                      "most every 0.25 s to take in queued requests of its kind, carrying its unfinished requests "
                      "over with their contact rows and tree snapshots")

@@ -499,6 +499,8 @@ typedef struct pmx_server_options
   double repack_s;      /* continuous batching: > 0, a running batch takes in queued requests of its kind
                          * (pausing at a step, carrying its unfinished requests over with their contact rows
                          * and tree snapshots) at most this often; 0: a batch runs until its last request ends */
+  int longest_first;    /* nonzero: a batch takes the queued requests with the most steps first (they set how long
+                         * a stream takes); 0: arrival order */
 } pmx_server_options;
 
 typedef struct pmx_request_info

@@ -24,6 +24,7 @@
  * (python/acc_sweep.py).
  */
 #include "pmx_internal.h"
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -133,6 +134,7 @@ struct pmx_server
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now ();
   long long done = 0, failed = 0, batches = 0;
   double particle_steps = 0.0;
+  bool longest_first = false;   /* batches take the queued requests with the most steps first */
 };
 
 namespace {
@@ -251,22 +253,22 @@ void on_done_roll (void *user, int i)
 void take_queued (pmx_server *s, Group *g, std::vector<Req*> &into, size_t room)
 {
   double t = now_s (s->t0);
-  for (auto it = s->queue.begin (); it != s->queue.end () && into.size () < room; )
+  std::vector<Req*> order (s->queue.begin (), s->queue.end ());
+  if (s->longest_first)
+    std::stable_sort (order.begin (), order.end (), [] (const Req *a, const Req *b) { return a->nsteps > b->nsteps; });
+  size_t n0 = into.size ();
+  for (Req *r : order)
   {
-    Req *r = *it;
-    if (!rollable (r->g.m) || request_kind (r->g.m) != g->kind || r->step != into[0]->step)
-    {
-      ++it;
-      continue;
-    }
+    if (into.size () >= room) break;
+    if (!rollable (r->g.m) || request_kind (r->g.m) != g->kind || r->step != into[0]->step) continue;
     if (s->used + r->bytes > s->opt.hbm_budget) break;
     s->used += r->bytes;
     g->bytes += r->bytes;
     r->status = PMX_REQ_RUNNING;
     r->t_admit = t;
     into.push_back (r);
-    it = s->queue.erase (it);
   }
+  for (size_t k = n0; k < into.size (); k++) s->queue.erase (std::find (s->queue.begin (), s->queue.end (), into[k]));
 }
 
 int roll_pause (void *user, int)
@@ -424,23 +426,25 @@ void scheduler (pmx_server *s)
             return true;
         return false;
       };
-      for (auto it = s->queue.begin (); it != s->queue.end () && (int) take.size () < s->opt.max_batch; )
+      /* arrival order, or (longest_first) the requests with the most
+       * steps first: they set how long the stream takes */
+      std::vector<Req*> order (s->queue.begin (), s->queue.end ());
+      if (s->longest_first)
+        std::stable_sort (order.begin (), order.end (), [] (const Req *a, const Req *b) { return a->nsteps > b->nsteps; });
+      for (Req *r : order)
       {
-        Req *r = *it;
+        if ((int) take.size () >= s->opt.max_batch) break;
         if (joins_running (r) ||
             (!take.empty () && (r->step != take[0]->step || r->g.m.curtime != take[0]->g.m.curtime ||
                                 r->g.m.curstep != take[0]->g.m.curstep || r->g.m.stepnum != take[0]->g.m.stepnum ||
                                 request_kind (r->g.m) != request_kind (take[0]->g.m) ||
                                 (roll && rollable (r->g.m) != rollable (take[0]->g.m)))))
-        {
-          ++it;
           continue;
-        }
-        if (s->used + bytes + r->bytes > s->opt.hbm_budget) break;   /* arrival order: wait for memory */
+        if (s->used + bytes + r->bytes > s->opt.hbm_budget) break;   /* in order: wait for memory */
         take.push_back (r);
         bytes += r->bytes;
-        it = s->queue.erase (it);
       }
+      for (Req *r : take) s->queue.erase (std::find (s->queue.begin (), s->queue.end (), r));
       if (take.empty () && !s->queue.empty () && joins_running (s->queue.front ()))
       {
         /* everything queued waits for a running batch of its kind */
@@ -501,6 +505,7 @@ int pmx_server_create (const pmx_server_options *opt, pmx_server **out)
   if (s->opt.max_batch <= 0) s->opt.max_batch = 256;
   if (s->opt.max_active <= 0) s->opt.max_active = 16;
   if (s->opt.gather_s < 0.0) s->opt.gather_s = 0.0;
+  s->longest_first = s->opt.longest_first != 0;
   s->sched = std::thread (scheduler, s);
   *out = s;
   return PMX_OK;

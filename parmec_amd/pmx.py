@@ -140,7 +140,7 @@ EXPORTS = ["pmx_step_times", "pmx_create", "pmx_upload", "pmx_dem", "pmx_downloa
 class pmx_server_options(C.Structure):
     _fields_ = [("precision", C.c_int), ("device", C.c_int), ("row_capacity", C.c_int), ("max_batch", C.c_int),
                 ("max_active", C.c_int), ("gather_s", C.c_double), ("hbm_budget", C.c_size_t),
-                ("repack_s", C.c_double)]
+                ("repack_s", C.c_double), ("longest_first", C.c_int)]
 
 
 class pmx_request_info(C.Structure):
@@ -648,10 +648,10 @@ class Server:
     duration, step and forcing; wait for each result."""
 
     def __init__(self, precision=PMX_FP64, device=0, row_capacity=0, max_batch=0, max_active=0, gather_s=0.0,
-                 hbm_budget=0, repack_s=0.0, lib=None):
+                 hbm_budget=0, repack_s=0.0, longest_first=False, lib=None):
         self.lib = lib or load()
         opt = pmx_server_options(precision, device, row_capacity, max_batch, max_active, gather_s, int(hbm_budget),
-                                 float(repack_s))
+                                 float(repack_s), int(bool(longest_first)))
         h = C.c_void_p()
         self._check(self.lib.pmx_server_create(C.byref(opt), C.byref(h)))
         self.h = h

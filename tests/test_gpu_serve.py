@@ -76,14 +76,16 @@ def test_served_equals_alone(prec):
     print(f"case serve precision {prec}: {len(reqs)} requests, batched and staggered", flush=True)
     together, info1 = _serve(prec, reqs, gather_s=0.5)           # one batch per kind of request
     staggered, info2 = _serve(prec, reqs, gap=0.03, gather_s=0.0)  # separate, overlapping batches
+    longest, info3 = _serve(prec, reqs, gap=0.01, gather_s=0.1, max_batch=3, longest_first=True)
     # a batch holds one kind: spheres, or bodies joined by springs or meshes
     kinds = [int(m.ellnum > 0) + 2 * int(m.sprnum > 0 or m.facnum > 0) for (m, d, h, f) in reqs]
     assert [i["batch"] for i in info1] == [kinds.count(k) for k in kinds]
-    assert all(i["status"] == pmx.REQ_DONE for i in info1 + info2)
+    assert all(i["status"] == pmx.REQ_DONE for i in info1 + info2 + info3)
     for k, (m, d, h, f) in enumerate(reqs):
         alone = _alone(prec, m, d, h, f)
         _same(together[k], alone)
         _same(staggered[k], alone)
+        _same(longest[k], alone)
         assert together[k].curtime == alone.curtime
 
 
